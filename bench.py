@@ -1,0 +1,151 @@
+"""Benchmark: one frame of the compute-raster pipeline on one MI355X.
+
+Workload: 4096x4096 RGBA8 canvas, 200,000 paths (60% fill, 40% stroke, 4-16
+cubics each, 10% linear gradients, mixed fill rules) under 32 nested rounded-rect
+clip layers, synthetic and seeded.  A step renders the whole frame (scan, backdrop
+prefix, coarse, fine for every DrawBatch) from scene buffers already resident in
+HBM.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import tileink_amd as tk  # noqa: E402
+from tileink_amd import scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def fine_algorithmic_bytes(stats, width, height):
+    """Minimum HBM traffic of the fine stage: every PTCL record and tile segment read
+    once, the (range, kind) word pair of every batch x tile read once, the image
+    written once."""
+    K = int(stats["ptcl_records"])
+    S = int(stats["segments_emitted"])
+    QT = int(stats["batches"]) * int(stats["tiles"])
+    N = width * height
+    return 24 * K + 20 * S + 12 * QT + 4 * N
+
+
+def cpu_baseline(args):
+    """The oracle (single-threaded C restatement) on a bounded sample of the same
+    generator: a 1024x1024 canvas with 1/16 of the paths and the same clip schedule."""
+    import oracle
+    side = 1024
+    paths = args.paths // 16
+    canvas = scenes.config2(side, side, paths, clips=args.clips, workload=2)
+    arrays = canvas.arrays()
+    t0 = time.perf_counter()
+    oracle.render_frame(arrays)
+    dt = time.perf_counter() - t0
+    return {"value": side * side / dt / 1e9, "unit": "Gpix/s", "cores": 1, "kind": "port",
+            "sample": f"oracle frame of the same generator at {side}x{side} with {paths} paths and {args.clips} "
+                      f"nested clips ({dt:.1f} s, 1 thread, {platform.processor() or platform.machine()})",
+            "ms_per_frame": dt * 1e3}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--paths", type=int, default=200_000)
+    ap.add_argument("--clips", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the scene would shard by tile "
+              f"stripes, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    canvas = scenes.config2(args.size, args.size, args.paths, clips=args.clips, workload=2)
+    scene = canvas.encode(0)
+    log(f"scene: {scene.path_count} paths, {scene.line_count} lines, {scene.batch_count} batches, "
+        f"{scene.tile_draw_index_count} tile refs, generated in {time.perf_counter() - t0:.1f} s")
+
+    ctx = tk.NativeContext(0)
+    frame = tk.Frame(ctx, scene)
+    for _ in range(args.warmup):
+        frame.render()
+    ctx.synchronize()
+    stats = frame.stats()
+    if stats["overflow"]:
+        raise SystemExit(f"capacity overflow flags {stats['overflow']}")
+    log(f"warmup done: K={stats['ptcl_records']} PTCL records, S={stats['segments_emitted']} segments")
+
+    # Per-stage HIP events on the pipeline's stream, over the timed steps.
+    frame.enable_timing(True)
+    stage_sum = {}
+    try:
+        import torch
+        torch_sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only; the pipeline does not need it
+        torch_sync = lambda: None  # noqa: E731
+    torch_sync()
+    ctx.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        frame.render()
+        times = frame.stage_times()  # waits for this frame's last event
+        for k, v in times.items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    ctx.synchronize()
+    torch_sync()
+    wall = time.perf_counter() - t_start
+    stage_avg = {k: v / args.steps for k, v in stage_sum.items()}
+    ms_per_step = stage_avg["total"]
+    log("stages (ms): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_avg.items()))
+
+    W = H = args.size
+    N = W * H
+    gpix = N / (ms_per_step * 1e-3) / 1e9
+    fine_bytes = fine_algorithmic_bytes(stats, W, H)
+    fine_gbs = fine_bytes / (stage_avg["fine"] * 1e-3) / 1e9
+    result = {
+        "metric": "ms/frame & Gpix/s at 4096², 200k paths; batched fwd+bwd scenes/s vs plain",
+        "value": round(gpix, 4),
+        "unit": "Gpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain forward render, {W}x{H} RGBA8, {scene.path_count} paths, {args.clips} nested "
+                        f"rounded-rect clips, 16x16 tiles",
+            "paths_per_s": round(scene.path_count / (ms_per_step * 1e-3), 1),
+            "lines": scene.line_count, "batches": scene.batch_count, "tiles": (W // 16) * (H // 16),
+            "ptcl_records": int(stats["ptcl_records"]), "segments": int(stats["segments_emitted"]),
+            "wall_ms_per_step": round(wall / args.steps * 1e3, 4),
+            "stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
+        },
+        "roofline": {"bound": "hbm", "kernel": "tk_k_fine", "achieved": round(fine_gbs, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(fine_gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                     "algorithmic_bytes": fine_bytes},
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline: oracle on a bounded sample ...")
+        result["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

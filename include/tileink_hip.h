@@ -1,0 +1,460 @@
+/*
+ * tileink_hip.h — C ABI of the MI355X (gfx950) compute-raster backend.
+ *
+ * This library replaces the GPU half of Tileink's native renderer: the scan
+ * (path_count / segment emit), backdrop prefix, coarse per-tile command list
+ * (PTCL) and fine analytic-coverage stages, plus the host-side scene recording
+ * that feeds them.  Every entry point takes plain pointers and sizes; device
+ * pointers are HIP device addresses owned by the caller unless stated.
+ *
+ * Record layouts are byte-for-byte the reference's GPU ABI so that a Rust
+ * `Device::Hip` adapter can hand its existing upload buffers straight through:
+ *   Line                 src/shared/line.rs:4-9,           scene_records.hlsli:10-12
+ *   PathRecord           src/shared/path.rs:6-23,          scene_records.hlsli:5-15
+ *   DrawRecord           src/shared/draw_record.rs:54-84,  draw_records.hlsli:5-21
+ *   LineSegment          src/shared/line_seg.rs:3-9
+ *   GpuScanChunk(+Range) src/shared/gpu_plan/path_plans.rs:15-29
+ *   LayerStackRecord     src/shared/gpu_coarse.rs:49-53
+ *   PtclRecord           src/shared/gpu_coarse.rs:57-64, coarse_records.hlsli:14
+ *   TileDrawRecord       src/shared/gpu_coarse.rs:42-46
+ *   ScanConfig           src/shaders/hlsl/scan/config.hlsli:4-8
+ *   CumsumConfig         src/shaders/hlsl/cumsum.hlsl:5
+ *
+ * Errors: every function returns tk_status; on failure tk_last_error() returns
+ * a thread-local message.  No C++ exception crosses this boundary.
+ */
+#ifndef TILEINK_HIP_H
+#define TILEINK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Status                                                                     */
+/* ------------------------------------------------------------------------- */
+
+typedef enum tk_status {
+    TK_OK = 0,
+    TK_ERR_INVALID = 1,     /* bad argument / inconsistent scene metadata        */
+    TK_ERR_HIP = 2,         /* a HIP runtime call failed                         */
+    TK_ERR_OOM = 3,         /* device or host allocation failed                  */
+    TK_ERR_UNAVAILABLE = 4, /* no gfx950 device at the requested ordinal         */
+    TK_ERR_OVERFLOW = 5,    /* a capacity bound was exceeded; writes were dropped */
+} tk_status;
+
+/* Thread-local description of the last failure on this thread.
+ * Replaces NativeError's Display (src/native/context.rs:16-25). */
+const char *tk_last_error(void);
+
+/* Library version word: (major << 16) | minor. */
+uint32_t tk_version(void);
+
+/* ------------------------------------------------------------------------- */
+/* Constants (src/shaders/hlsl/constants.hlsli and shared/ tag headers)        */
+/* ------------------------------------------------------------------------- */
+
+#define TK_TILE_SIZE 16u
+#define TK_SCAN_CHUNK_SIZE 256u
+#define TK_CUMSUM_CHUNK_SIZE 256u
+#define TK_SDF_RECORD_WORDS 17u
+#define TK_INVALID_INDEX 0xFFFFFFFFu
+
+#define TK_DRAW_BRUSH 0u
+#define TK_DRAW_CLIP 1u
+#define TK_DRAW_PATH_GLYPH 5u
+#define TK_LAYER_CLIP 0u
+#define TK_FILL_NONZERO 0u
+#define TK_FILL_EVEN_ODD 1u
+
+#define TK_PTCL_END 0u
+#define TK_PTCL_FILL 1u
+#define TK_PTCL_COLOR 2u
+#define TK_PTCL_BEGIN_CLIP 3u
+#define TK_PTCL_END_CLIP 4u
+#define TK_PTCL_SDF 9u
+#define TK_PTCL_BEGIN_SDF_CLIP 12u
+
+#define TK_TILE_KIND_INTERPRETER 0u
+#define TK_TILE_KIND_EMPTY 1u
+#define TK_TILE_KIND_COLOR 2u
+#define TK_TILE_KIND_SDF 3u
+#define TK_TILE_KIND_MIXED 4u
+#define TK_TILE_KIND_MASK 7u
+#define TK_TILE_PREFIX_SHIFT 3u
+
+#define TK_BRUSH_SOLID 1u
+#define TK_BRUSH_LINEAR 2u
+#define TK_BRUSH_HEADER_WORDS 9u
+#define TK_BRUSH_PARAM_WORDS 12u
+#define TK_SDF_KIND_RECT 1u
+
+/* ------------------------------------------------------------------------- */
+/* GPU record ABI                                                              */
+/* ------------------------------------------------------------------------- */
+
+typedef struct tk_affine { float a, b, c, d, e, f; } tk_affine; /* (x,y)->(a x + c y + e, b x + d y + f) */
+
+typedef struct tk_line {           /* 24 B */
+    uint32_t path_id;
+    float pad;
+    float p0[2];
+    float p1[2];
+} tk_line;
+
+typedef struct tk_path_record {    /* 76 B */
+    uint32_t path_id, line_count, line_start, flags;
+    uint32_t data_offset, data_len;
+    uint32_t tile_x0, tile_y0, tile_x1, tile_y1;
+    uint32_t segment_start, segment_capacity, segment_count;
+    tk_affine transform;
+} tk_path_record;
+
+typedef struct tk_pixel_bounds { int32_t x0, y0, x1, y1; } tk_pixel_bounds;
+
+typedef struct tk_draw_record {    /* 124 B */
+    uint32_t path_id, glyph_run_id, sdf_offset, sdf_len;
+    uint32_t sdf_shadow_offset, sdf_shadow_len, brush_offset, brush_len;
+    uint32_t tag, fill_rule;
+    tk_pixel_bounds pixel_bounds, local_pixel_bounds;
+    uint32_t solid_rect;
+    tk_affine transform, inverse_transform;
+} tk_draw_record;
+
+typedef struct tk_line_segment { float p0x, p0y, p1x, p1y, y_edge; } tk_line_segment; /* 20 B */
+typedef struct tk_segment_range { uint32_t start, end; } tk_segment_range;          /* 8 B  */
+typedef struct tk_scan_chunk { uint32_t path_id, backdrop_offset, segment_start, len; } tk_scan_chunk;
+typedef struct tk_scan_chunk_range { uint32_t start, end; } tk_scan_chunk_range;
+typedef struct tk_layer_record { uint32_t tag, draw, payload; } tk_layer_record;   /* 12 B */
+typedef struct tk_ptcl_record {                                                     /* 24 B */
+    uint32_t tag;
+    int32_t backdrop;
+    uint32_t fill_rule, segment_start, segment_end, color;
+} tk_ptcl_record;
+typedef struct tk_tile_draw_record { uint32_t start, len; } tk_tile_draw_record;   /* 8 B  */
+
+/* One fused DrawBatch: a painter-ordered run of draws sharing one layer stack
+ * snapshot (src/shared/execution.rs:106-113).  The draws of batch b are those
+ * whose draw_batch_ids entry equals b. */
+typedef struct tk_batch_record { uint32_t layer_start, layer_end; } tk_batch_record;
+
+typedef struct tk_scan_config {    /* scan/config.hlsli:4-8 */
+    uint32_t clear_len, backdrop_len, path_count, scan_chunk_count;
+    uint32_t line_count, segment_capacity, incremental, line_base;
+    uint32_t path_base, chunk_base, backdrop_base;
+} tk_scan_config;
+
+typedef struct tk_cumsum_config {  /* cumsum.hlsl:5 */
+    uint32_t row_count, chunk_count, pad0, pad1;
+} tk_cumsum_config;
+
+/* ------------------------------------------------------------------------- */
+/* Scene: everything the GPU stages read, as produced by the recording layer. */
+/* The host view is owned by whoever filled it (a tk_canvas, or the caller).  */
+/* ------------------------------------------------------------------------- */
+
+typedef struct tk_scene {
+    uint32_t width, height;          /* physical pixels */
+    uint32_t tiles_width, tiles_height;
+    uint32_t clear_color;            /* premultiplied RGBA8, little-endian u32 */
+
+    const tk_line *lines;            uint32_t line_count;
+    const tk_path_record *paths;     uint32_t path_count;
+    const tk_draw_record *draws;     uint32_t draw_count;
+    const uint32_t *paint;           uint32_t paint_words;   /* sdf || sdf_shadow || brush */
+    uint32_t paint_sdf_shadow_base, paint_brush_base;
+
+    const uint32_t *draw_batch_ids;  /* draw_count entries; TK_INVALID_INDEX outside batches */
+    const tk_layer_record *layers;   uint32_t layer_count;
+    const tk_batch_record *batches;  uint32_t batch_count;
+
+    const tk_tile_draw_record *tile_draws;   /* tiles_width*tiles_height */
+    const uint32_t *tile_draw_indices;       uint32_t tile_draw_index_count;
+
+    const tk_scan_chunk *scan_chunks;        uint32_t scan_chunk_count;
+    const tk_scan_chunk_range *scan_chunk_ranges; /* path_count */
+    const uint32_t *cumsum_chunk_offsets;    /* backdrop offset of each cumsum chunk */
+    const uint32_t *cumsum_chunk_lens;
+    uint32_t cumsum_chunk_count;
+    const uint32_t *cumsum_row_starts;       /* first chunk of each backdrop row */
+    const uint32_t *cumsum_row_ends;
+    uint32_t cumsum_row_count;
+
+    uint32_t backdrop_len;           /* B = sum of path data_len */
+    uint32_t segment_capacity;       /* S bound = sum of path segment_capacity */
+    uint32_t max_clip_depth;         /* deepest fused clip stack of any batch */
+    uint64_t ptcl_capacity;          /* PTCL records bound over all batches */
+} tk_scene;
+
+/* ------------------------------------------------------------------------- */
+/* Recording (host).  Mirrors the Canvas API (src/canvas/primitives/ and       */
+/* src/canvas/layers.rs) for the primitives on the compute-raster path.        */
+/* ------------------------------------------------------------------------- */
+
+typedef struct tk_canvas tk_canvas;
+
+/* Path element verbs, kurbo::PathEl order.  Points per verb: 1,1,2,3,0. */
+#define TK_PATH_MOVE 0u
+#define TK_PATH_LINE 1u
+#define TK_PATH_QUAD 2u
+#define TK_PATH_CUBIC 3u
+#define TK_PATH_CLOSE 4u
+
+typedef struct tk_gradient_stop { float offset; float rgba[4]; } tk_gradient_stop; /* straight sRGB */
+
+typedef struct tk_brush {
+    uint32_t kind;                 /* TK_BRUSH_SOLID or TK_BRUSH_LINEAR */
+    float rgba[4];                 /* solid: straight-alpha sRGB in [0,1] */
+    float start[2], end[2];        /* linear: gradient line in user space */
+    uint32_t extend;               /* 0 pad, 1 repeat, 2 reflect */
+    const tk_gradient_stop *stops; uint32_t stop_count;
+} tk_brush;
+
+tk_status tk_canvas_create(uint32_t width, uint32_t height, double scale, tk_canvas **out);
+void tk_canvas_destroy(tk_canvas *canvas);
+/* Canvas::push_path (src/canvas/primitives/paths.rs:50-60). transform may be NULL (identity). */
+tk_status tk_canvas_push_path(tk_canvas *canvas, const uint8_t *verbs, size_t verb_count,
+                              const double *points, size_t point_count, const tk_brush *brush,
+                              const double *transform, uint32_t fill_rule, double tolerance,
+                              uint32_t *draw_id);
+/* Canvas::push_stroke (src/canvas/primitives/paths.rs:24-48): round joins and caps. */
+tk_status tk_canvas_push_stroke(tk_canvas *canvas, const uint8_t *verbs, size_t verb_count,
+                                const double *points, size_t point_count, double width,
+                                const tk_brush *brush, const double *transform, uint32_t fill_rule,
+                                double tolerance, uint32_t *draw_id);
+/* Canvas::push_rect (src/canvas/primitives/rectangles.rs:33-43); radii = tl,tr,bl,br. */
+tk_status tk_canvas_push_rect(tk_canvas *canvas, double x0, double y0, double x1, double y1,
+                              const float radii[4], const tk_brush *brush, uint32_t *draw_id);
+/* Canvas::push_clip_sdf_rect_layer (src/canvas/layers.rs:34-40). */
+tk_status tk_canvas_push_clip_sdf_rect_layer(tk_canvas *canvas, double x0, double y0, double x1,
+                                             double y1, const float radii[4]);
+/* Canvas::push_clip_layer (src/canvas/layers.rs:7-27): pixel-aligned axis rects become SDF clips. */
+tk_status tk_canvas_push_clip_layer(tk_canvas *canvas, const uint8_t *verbs, size_t verb_count,
+                                    const double *points, size_t point_count,
+                                    const double *transform, uint32_t fill_rule, double tolerance);
+/* Canvas::pop_layer (src/canvas/layers.rs:280-284). TK_ERR_INVALID when no layer is open. */
+tk_status tk_canvas_pop_layer(tk_canvas *canvas);
+/* Compile + plan: ExecPlan batches (src/canvas/compile.rs:39-70), tile bins
+ * (src/shared/gpu_plan/tile_bins/build.rs:4-83), scan/cumsum chunk plans
+ * (src/shared/gpu_plan/path_plans.rs:463-504).  The view stays valid until the
+ * next push or destroy. */
+tk_status tk_canvas_encode(tk_canvas *canvas, uint32_t clear_color, tk_scene *out);
+/* Flattened line count of the canvas so far (diagnostic). */
+uint32_t tk_canvas_line_count(const tk_canvas *canvas);
+
+/* CPU restatement of line_scanned_tile_count (src/shared/scan_line.rs:17-31),
+ * used for segment capacities. */
+uint32_t tk_line_scanned_tile_count(const tk_line *line, uint32_t tile_x0, uint32_t tile_y0,
+                                    uint32_t tile_x1, uint32_t tile_y1);
+
+/* ------------------------------------------------------------------------- */
+/* Device context                                                              */
+/* ------------------------------------------------------------------------- */
+
+typedef struct tk_context tk_context;
+
+/* Explicit device ordinal; never falls back (NativeContextOptions::physical_adapter,
+ * src/native/context.rs:4-13).  Loads the embedded gfx950 code object. */
+tk_status tk_context_create(int device, tk_context **out);
+void tk_context_destroy(tk_context *ctx);
+/* Stream every entry point of this context is ordered on (a hipStream_t). */
+void *tk_context_stream(tk_context *ctx);
+tk_status tk_context_synchronize(tk_context *ctx);
+
+/* Limits (src/native/runtime/renderer/recording.rs:17-23). */
+typedef struct tk_limits {
+    uint32_t image_dimension;
+    uint32_t max_tiles;
+    uint64_t device_memory_bytes;
+    uint32_t compute_units;
+} tk_limits;
+tk_status tk_context_limits(tk_context *ctx, tk_limits *out);
+
+/* Device buffers (Adapter::allocate_buffer / readback, src/native/runtime/adapter.rs:159-251).
+ * Copies are ordered on the context stream and complete before returning. */
+tk_status tk_device_alloc(tk_context *ctx, size_t bytes, void **out);
+tk_status tk_device_free(tk_context *ctx, void *ptr);
+tk_status tk_copy_to_device(tk_context *ctx, void *dst, const void *src, size_t bytes);
+tk_status tk_copy_to_host(tk_context *ctx, void *dst, const void *src, size_t bytes);
+
+/* ------------------------------------------------------------------------- */
+/* Whole-frame path (NativeRenderer::render_to_image, src/native/renderer.rs:177-188) */
+/* ------------------------------------------------------------------------- */
+
+typedef struct tk_frame tk_frame;
+
+/* Allocates every device buffer for the scene and uploads its inputs. */
+tk_status tk_frame_create(tk_context *ctx, const tk_scene *scene, tk_frame **out);
+/* Re-uploads inputs of a scene with identical sizes (animation frames). */
+tk_status tk_frame_update(tk_frame *frame, const tk_scene *scene);
+void tk_frame_destroy(tk_frame *frame);
+/* Enqueues scan -> backdrop prefix -> coarse -> fine on the context stream.
+ * Inputs and outputs stay in device memory. */
+tk_status tk_frame_render(tk_frame *frame);
+/* Synchronises and copies the RGBA8 image (premultiplied, row-major, tightly
+ * packed little-endian u32, src/native/submission.rs:125-146). */
+tk_status tk_frame_read_image(tk_frame *frame, uint32_t *rgba_out, size_t pixel_count);
+/* Device address of the RGBA8 image. */
+void *tk_frame_image_device(tk_frame *frame);
+
+/* Intermediate buffers, for parity checks. */
+typedef enum tk_frame_buffer {
+    TK_BUF_BACKDROPS = 0,      /* i32 x B: after the row prefix                     */
+    TK_BUF_SEGMENT_COUNTS = 1, /* u32 x B: scan_count per backdrop tile            */
+    TK_BUF_SEGMENT_RANGES = 2, /* {start,end} x B                                   */
+    TK_BUF_SEGMENT_CURSORS = 3,/* u32 x B: after emit = range.end                   */
+    TK_BUF_SEGMENT_BUMPS = 4,  /* u32 x P                                           */
+    TK_BUF_SEGMENTS = 5,       /* tk_line_segment x S                               */
+    TK_BUF_TILE_PTCL_COUNTS = 6,   /* u32 x Q*T (batch-major)                       */
+    TK_BUF_TILE_PTCL_RANGES = 7,   /* {start,end} x Q*T, per-batch offsets          */
+    TK_BUF_TILE_KINDS = 8,         /* u32 x Q*T: kind | skip << 3                   */
+    TK_BUF_BATCH_PTCL_BASES = 9,   /* u32 x Q: first record of each batch           */
+    TK_BUF_PTCL = 10,              /* tk_ptcl_record x K                            */
+    TK_BUF_RAW_BACKDROPS = 11,     /* i32 x B: backdrop deltas before the row prefix */
+} tk_frame_buffer;
+tk_status tk_frame_read_buffer(tk_frame *frame, tk_frame_buffer which, void *out, size_t bytes);
+
+typedef struct tk_frame_stats {
+    uint32_t paths, lines, draws, batches, tiles;   /* P, L, D, Q, T */
+    uint32_t backdrop_len;                          /* B */
+    uint32_t segment_capacity;                      /* S bound */
+    uint32_t tile_draw_refs;                        /* R */
+    uint64_t ptcl_capacity;                         /* K bound */
+    uint64_t ptcl_records;                          /* K after the last render */
+    uint32_t segments_emitted;                      /* S after the last render */
+    uint32_t overflow;                              /* nonzero when a bound was exceeded */
+    uint64_t device_bytes;                          /* device memory held by the frame */
+} tk_frame_stats;
+/* Synchronises; ptcl_records / segments_emitted / overflow come from the device. */
+tk_status tk_frame_stats_read(tk_frame *frame, tk_frame_stats *out);
+
+/* When enabled, later renders keep a copy of the backdrop deltas before the
+ * row prefix (TK_BUF_RAW_BACKDROPS). */
+tk_status tk_frame_set_capture(tk_frame *frame, int enable);
+
+/* Per-stage event timings of the last tk_frame_render, milliseconds. */
+typedef struct tk_stage_times {
+    float scan_clear, scan_count, scan_prefix, scan_emit, segment_order, cumsum;
+    float coarse_count, coarse_prefix, coarse_emit, fine, total;
+} tk_stage_times;
+/* Enables hipEvent timestamps around each stage of later renders. */
+tk_status tk_frame_enable_timing(tk_frame *frame, int enable);
+tk_status tk_frame_stage_times(tk_frame *frame, tk_stage_times *out);
+
+/* One-call convenience: upload, render, read back, release. */
+tk_status tk_render_to_image(tk_context *ctx, const tk_scene *scene, uint32_t *rgba_out,
+                             size_t pixel_count);
+
+/* ------------------------------------------------------------------------- */
+/* Stage entry points (device pointers, ordered on the context stream).       */
+/* Each restates one reference dispatch contract so the reference's raw        */
+/* ComputeBatch known-answer cases can be replayed unchanged.                   */
+/* ------------------------------------------------------------------------- */
+
+/* scan_clear (scan/clear.hlsl:16-32), non-incremental form. Any pointer may be NULL. */
+tk_status tk_scan_clear(tk_context *ctx, const tk_scan_config *config, void *backdrops,
+                        void *segment_ranges, void *segment_counts, void *segment_cursors,
+                        void *segment_bumps, void *chunk_totals, void *chunk_offsets);
+/* scan_count (scan/count.hlsl:13-40): per-line DDA, atomic tile counts and
+ * backdrop deltas. path_record_count bounds path ids (GetDimensions guard). */
+tk_status tk_scan_count(tk_context *ctx, const tk_scan_config *config, const tk_line *lines,
+                        const tk_path_record *paths, uint32_t path_record_count,
+                        int32_t *backdrops, uint32_t *segment_counts);
+/* scan_prefix_chunks + scan_chunk_offsets + scan_apply_chunk_offsets
+ * (scan/prefix_chunks.hlsl:16-38, chunk_offsets.hlsl:14-26,
+ * apply_chunk_offsets.hlsl:15-27) as one decoupled-lookback pass.  Writes the
+ * same five outputs the three reference passes leave behind. */
+tk_status tk_scan_prefix(tk_context *ctx, const tk_scan_config *config,
+                         const tk_scan_chunk *chunks, const tk_scan_chunk_range *chunk_ranges,
+                         const tk_path_record *paths, const uint32_t *segment_counts,
+                         tk_segment_range *segment_ranges, uint32_t *segment_cursors,
+                         uint32_t *segment_bumps, uint32_t *chunk_totals, uint32_t *chunk_offsets);
+/* scan_emit (scan/emit.hlsl:14-28): writes tile-clipped segments at atomically
+ * claimed cursor slots. */
+tk_status tk_scan_emit(tk_context *ctx, const tk_scan_config *config, const tk_line *lines,
+                       const tk_path_record *paths, uint32_t path_record_count,
+                       uint32_t *segment_cursors, tk_line_segment *segments);
+/* cumsum_prefix_chunks / chunk_offsets / apply_chunk_offsets (cumsum.hlsl:16-71):
+ * in-place inclusive row prefix of backdrop deltas. */
+tk_status tk_cumsum(tk_context *ctx, const tk_cumsum_config *config,
+                    const uint32_t *chunk_backdrop_offsets, const uint32_t *chunk_lens,
+                    const uint32_t *row_chunk_starts, const uint32_t *row_chunk_ends,
+                    int32_t *backdrops, uint32_t *chunk_totals, uint32_t *chunk_offsets);
+
+/* Coarse, all batches at once (replaces Q runs of coarse_count_bins ->
+ * coarse_prefix_* -> coarse_emit_bins, coarse/count.hlsl:48-73,
+ * coarse/prefix.hlsl:11-63, coarse/emit.hlsl:82-142). */
+typedef struct tk_coarse_config {
+    uint32_t tile_count, tiles_width, tiles_height, batch_count;
+    uint32_t draw_count, path_count, paint_words, paint_brush_base;
+    uint32_t layer_count, tile_draw_index_count, pad0, pad1;
+    uint64_t ptcl_capacity;
+} tk_coarse_config;
+typedef struct tk_coarse_inputs {
+    const tk_draw_record *draws;
+    const tk_path_record *paths;
+    const int32_t *backdrops;             /* after the row prefix */
+    const tk_segment_range *segment_ranges;
+    const tk_layer_record *layers;
+    const tk_batch_record *batches;
+    const uint32_t *paint;
+    const uint32_t *draw_batch_ids;
+    const tk_tile_draw_record *tile_draws;
+    const uint32_t *tile_draw_indices;
+} tk_coarse_inputs;
+/* tile_ptcl_ranges are global record ranges: the stream of (batch b, tile t)
+ * is ptcl[start, end).  The reference's per-batch tile offsets
+ * (ptcl_start/end of TileCoarseRecord) are start - batch_ptcl_bases[b]. */
+typedef struct tk_coarse_outputs {
+    uint32_t *tile_ptcl_counts;           /* Q*T, batch-major (b*T + t) */
+    tk_segment_range *tile_ptcl_ranges;   /* Q*T, global wrapping exclusive prefix */
+    uint32_t *tile_kinds;                 /* Q*T: kind | opaque-prefix skip << 3 */
+    uint32_t *batch_ptcl_bases;           /* Q+1: first record of each batch, total K last */
+    tk_ptcl_record *ptcl;                 /* ptcl_capacity records */
+    uint32_t *status;                     /* 4 words, zeroed by the caller: [0] overflow,
+                                             [1] K, [3] tile list not in batch order */
+} tk_coarse_outputs;
+tk_status tk_coarse(tk_context *ctx, const tk_coarse_config *config, const tk_coarse_inputs *in,
+                    const tk_coarse_outputs *out);
+
+/* Fine, all batches at once: one 256-lane workgroup per 16x16 tile walks the
+ * batches in order, re-quantising to RGBA8 at each batch boundary exactly where
+ * the reference stores and re-loads its RGBA8 target (fine/main.hlsl:23-40,
+ * fine/interpreter.hlsli:21-219, fine/specialized.hlsli:19-111). */
+typedef struct tk_fine_config {
+    uint32_t width, height, clear_color, tile_count;
+    uint32_t tiles_width, tiles_height, batch_count, clip_spill_depth;
+    uint32_t paint_words, paint_sdf_shadow_base, paint_brush_base, draw_count;
+    uint32_t ptcl_capacity, segment_capacity;   /* records beyond these are never read */
+} tk_fine_config;
+typedef struct tk_fine_inputs {
+    const tk_draw_record *draws;
+    const uint32_t *paint;
+    const tk_line_segment *segments;
+    const tk_segment_range *tile_ptcl_ranges;
+    const uint32_t *tile_kinds;
+    const uint32_t *batch_ptcl_bases;
+    const tk_ptcl_record *ptcl;
+} tk_fine_inputs;
+tk_status tk_fine(tk_context *ctx, const tk_fine_config *config, const tk_fine_inputs *in,
+                  uint32_t *rgba_out);
+
+/* Validation helpers (validation/geometry.hlsl:22-29, validation/pixel.hlsl:9-34).
+ * requests: 8 words each {segment_start, segment_end, backdrop, x, y, rule, 0, 0}.
+ * alpha_out: u8 alpha per request; coverage_out (optional): FP32 winding area
+ * backdrop + row sweep before the fill rule. */
+tk_status tk_fill_coverage_words(tk_context *ctx, uint32_t count, const uint32_t *requests,
+                                 const tk_line_segment *segments, uint32_t *alpha_out,
+                                 float *coverage_out);
+/* requests: 8 words {a, b, src, dst, c_bits, t_bits, rule, 0}; 14 output words each. */
+tk_status tk_pixel_math_words(tk_context *ctx, uint32_t count, const uint32_t *requests,
+                              uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TILEINK_HIP_H */

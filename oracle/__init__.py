@@ -1,0 +1,191 @@
+"""Parity oracle: ctypes bindings of oracle/liboracle.so.
+
+TEST INFRASTRUCTURE ONLY.  Used by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py as the checker; the product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _lib.or_render_frame.restype = C.c_int
+        _lib.or_line_scanned_tile_count.restype = C.c_uint32
+        _lib.or_polygon_pixel_area_f64.restype = C.c_double
+        _lib.or_polygon_pixel_area_f64.argtypes = [C.c_void_p, C.c_uint32, C.c_double, C.c_double]
+    return _lib
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def u32(a):
+    return np.ascontiguousarray(np.asarray(a).view(np.uint32) if np.asarray(a).dtype.itemsize == 4 and
+                                np.asarray(a).dtype != np.uint32 else np.asarray(a, dtype=np.uint32))
+
+
+class ScanConfig(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("clear_len", "backdrop_len", "path_count", "scan_chunk_count", "line_count",
+                                          "segment_capacity", "incremental", "line_base", "path_base", "chunk_base",
+                                          "backdrop_base")]
+
+
+class CoarseConfig(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("tile_count", "tiles_width", "tiles_height", "batch_id",
+                                          "layer_stack_start", "layer_stack_end", "ptcl_capacity", "paint_brush_base",
+                                          "draw_count", "path_count", "paint_words", "layer_count",
+                                          "tile_draw_index_count")]
+
+
+class CoarseInputs(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("draws", "paths", "backdrops", "segment_ranges", "layers", "paint",
+                                          "draw_batch_ids", "tile_draws", "tile_draw_indices")]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32), ("tiles_width", C.c_uint32), ("tiles_height", C.c_uint32),
+        ("clear_color", C.c_uint32),
+        ("lines", C.c_void_p), ("line_count", C.c_uint32),
+        ("paths", C.c_void_p), ("path_count", C.c_uint32),
+        ("draws", C.c_void_p), ("draw_count", C.c_uint32),
+        ("paint", C.c_void_p), ("paint_words", C.c_uint32), ("paint_sdf_shadow_base", C.c_uint32),
+        ("paint_brush_base", C.c_uint32),
+        ("draw_batch_ids", C.c_void_p),
+        ("layers", C.c_void_p), ("layer_count", C.c_uint32),
+        ("batches", C.c_void_p), ("batch_count", C.c_uint32),
+        ("tile_draws", C.c_void_p), ("tile_draw_indices", C.c_void_p), ("tile_draw_index_count", C.c_uint32),
+        ("scan_chunks", C.c_void_p), ("scan_chunk_count", C.c_uint32),
+        ("scan_chunk_ranges", C.c_void_p),
+        ("cumsum_chunk_offsets", C.c_void_p), ("cumsum_chunk_lens", C.c_void_p), ("cumsum_chunk_count", C.c_uint32),
+        ("cumsum_row_starts", C.c_void_p), ("cumsum_row_ends", C.c_void_p), ("cumsum_row_count", C.c_uint32),
+        ("backdrop_len", C.c_uint32), ("segment_capacity", C.c_uint32), ("max_clip_depth", C.c_uint32),
+    ]
+
+
+class FrameOutputs(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("raw_backdrops", "backdrops", "segment_counts", "segment_ranges",
+                                          "segment_cursors", "segment_bumps", "segments", "tile_ptcl_counts",
+                                          "tile_ptcl_starts", "tile_ptcl_ends", "tile_kinds", "batch_bases",
+                                          "ptcl")] + [("ptcl_capacity_words", C.c_uint64)]
+
+
+ARRAY_FIELDS = ("lines", "paths", "draws", "paint", "draw_batch_ids", "layers", "batches", "tile_draws",
+                "tile_draw_indices", "scan_chunks", "scan_chunk_ranges", "cumsum_chunk_offsets", "cumsum_chunk_lens",
+                "cumsum_row_starts", "cumsum_row_ends")
+
+
+def render_frame(scene, ptcl_capacity=None):
+    """Runs the whole frame on the CPU.  `scene` is a tileink_amd.SceneArrays.
+
+    Returns (image[h, w] u32, dict of intermediate arrays)."""
+    s = Scene()
+    keep = []
+    for name, _ in Scene._fields_:
+        if name in ARRAY_FIELDS:
+            arr = np.ascontiguousarray(getattr(scene, name)).view(np.uint8).reshape(-1)
+            arr = np.ascontiguousarray(np.asarray(getattr(scene, name)))
+            raw = np.ascontiguousarray(arr.view(np.uint32).reshape(-1)) if arr.size else np.zeros(1, np.uint32)
+            keep.append(raw)
+            setattr(s, name, raw.ctypes.data)
+        else:
+            setattr(s, name, int(scene.scalars[name]))
+    B, P = s.backdrop_len, s.path_count
+    T = s.tiles_width * s.tiles_height
+    Q = s.batch_count
+    if ptcl_capacity is None:
+        ptcl_capacity = int(scene.scalars["ptcl_capacity"])
+    out = {
+        "raw_backdrops": np.zeros(B, np.int32), "backdrops": np.zeros(B, np.int32),
+        "segment_counts": np.zeros(B, np.uint32), "segment_ranges": np.zeros((B, 2), np.uint32),
+        "segment_cursors": np.zeros(B, np.uint32), "segment_bumps": np.zeros(P, np.uint32),
+        "segments": np.zeros((s.segment_capacity, 5), np.uint32),
+        "tile_ptcl_counts": np.zeros(Q * T, np.uint32), "tile_ptcl_starts": np.zeros(Q * T, np.uint32),
+        "tile_ptcl_ends": np.zeros(Q * T, np.uint32), "tile_kinds": np.zeros(Q * T, np.uint32),
+        "batch_bases": np.zeros(Q + 1, np.uint32), "ptcl": np.zeros((max(ptcl_capacity, 1), 6), np.uint32),
+    }
+    fo = FrameOutputs()
+    for name, arr in out.items():
+        setattr(fo, name, arr.ctypes.data)
+    fo.ptcl_capacity_words = out["ptcl"].size
+    image = np.zeros((s.height, s.width), np.uint32)
+    status = lib().or_render_frame(C.byref(s), _p(image), C.byref(fo))
+    if status != 0:
+        raise RuntimeError(f"oracle frame exceeded a bound (status {status})")
+    out["ptcl"] = out["ptcl"][: int(out["batch_bases"][Q])]
+    return image, out
+
+
+def scan_count(cfg, lines, paths, path_record_count, backdrops, counts):
+    lib().or_scan_count(C.byref(cfg), _p(lines), _p(paths), C.c_uint32(path_record_count), _p(backdrops), _p(counts))
+
+
+def scan_prefix(cfg, chunks, chunk_ranges, paths, counts, ranges, cursors, bumps, totals, offsets):
+    """The reference's three prefix passes in order."""
+    lib().or_scan_prefix_chunks(C.byref(cfg), _p(chunks), _p(counts), _p(ranges), _p(totals))
+    lib().or_scan_chunk_offsets(C.byref(cfg), _p(paths), _p(chunk_ranges), _p(bumps), _p(totals), _p(offsets))
+    lib().or_scan_apply_chunk_offsets(C.byref(cfg), _p(chunks), _p(ranges), _p(cursors), _p(offsets))
+
+
+def coarse_prefix(tile_count, work, chunk_records):
+    lib().or_coarse_prefix(C.c_uint32(tile_count), _p(work), _p(chunk_records))
+
+
+def coarse_count_bins(cfg, inputs, work, kinds):
+    lib().or_coarse_count_bins(C.byref(cfg), C.byref(inputs), _p(work), _p(kinds))
+
+
+def coarse_emit_bins(cfg, inputs, work, kinds):
+    lib().or_coarse_emit_bins(C.byref(cfg), C.byref(inputs), _p(work), _p(kinds))
+
+
+def line_scanned_tile_count(p0, p1, bbox):
+    a = np.asarray(p0, np.float32)
+    b = np.asarray(p1, np.float32)
+    return lib().or_line_scanned_tile_count(_p(a), _p(b), *[C.c_uint32(int(v)) for v in bbox])
+
+
+def fill_coverage_words(requests, segments):
+    requests = np.ascontiguousarray(requests, dtype=np.uint32).reshape(-1, 8)
+    segments = np.ascontiguousarray(segments).view(np.uint32).reshape(-1)
+    n = requests.shape[0]
+    alpha = np.zeros(n, np.uint32)
+    cov = np.zeros(n, np.float32)
+    lib().or_fill_coverage_words(C.c_uint32(n), _p(requests), _p(segments), _p(alpha), _p(cov))
+    return alpha, cov
+
+
+def fill_coverage_f64(requests, segments):
+    requests = np.ascontiguousarray(requests, dtype=np.uint32).reshape(-1, 8)
+    segments = np.ascontiguousarray(segments).view(np.uint32).reshape(-1)
+    out = np.zeros(requests.shape[0], np.float64)
+    lib().or_fill_coverage_f64(C.c_uint32(requests.shape[0]), _p(requests), _p(segments), _p(out))
+    return out
+
+
+def polygon_pixel_area_f64(lines, px, py):
+    lines = np.ascontiguousarray(lines).view(np.uint32).reshape(-1)
+    return lib().or_polygon_pixel_area_f64(_p(lines), C.c_uint32(lines.size // 6), float(px), float(py))
+
+
+def pixel_math_words(requests):
+    requests = np.ascontiguousarray(requests, dtype=np.uint32).reshape(-1, 8)
+    out = np.zeros((requests.shape[0], 14), np.uint32)
+    lib().or_pixel_math_words(C.c_uint32(requests.shape[0]), _p(requests), _p(out))
+    return out

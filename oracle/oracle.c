@@ -1,0 +1,1434 @@
+/*
+ * oracle.c — CPU restatement of Tileink's compute-raster stages (test
+ * infrastructure; see oracle.h).  Build: gcc -O2 -std=c11 -ffp-contract=off.
+ *
+ * Float semantics follow the reference shaders: HLSL `mad` is fmaf, every other
+ * product and sum is rounded separately (no contraction), HLSL round() is
+ * round-half-to-even (rintf), and float->int conversions saturate with NaN -> 0.
+ * The CPU segment-capacity routine (scan_line.rs) uses Rust semantics instead:
+ * round() is half-away-from-zero (roundf).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NONE 0xFFFFFFFFu
+#define TILE 16u
+
+/* ---------- scalar helpers --------------------------------------------------- */
+
+static uint32_t f2u(float v) {
+    if (!(v > 0.0f)) return 0u;
+    if (v >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)v;
+}
+static int32_t f2i(float v) {
+    if (v != v) return 0;
+    if (v <= -2147483648.0f) return INT32_MIN;
+    if (v >= 2147483648.0f) return INT32_MAX;
+    return (int32_t)v;
+}
+static float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+static int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+static float bits_f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static uint32_t f_bits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+static uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+
+/* ---------- scan geometry (src/shaders/hlsl/scan/geometry.hlsli:8-114) ---------- */
+
+static const float SCAN_EPSILON = 1.0e-6f;
+static const float TOP_TOUCH_EPSILON = 1.0e-12f;
+static const float TILE_CLIP_NUDGE = 1.0e-3f;
+
+typedef struct {
+    uint32_t bbox[4]; /* x0 y0 x1 y1 */
+    uint32_t data_offset;
+    float pts[4];     /* oriented: pts[3] >= pts[1] */
+    int down;
+    uint32_t count;
+    float a, b, sign, x0, y0;
+    uint32_t imin, imax;
+    int32_t ymin, ymax;
+} Traversal;
+
+static int is_tile_boundary_y(float v) { return fabsf(v - floorf(v)) <= TOP_TOUCH_EPSILON; }
+static int32_t ceil_tile_boundary_y(float v) { return is_tile_boundary_y(v) ? f2i(floorf(v)) : f2i(ceilf(v)); }
+static uint32_t span(float a, float b) {
+    float hi = ceilf(a);
+    if (b > a) hi = ceilf(b);
+    float lo = floorf(a);
+    if (b < a) lo = floorf(b);
+    float value = hi - lo;
+    if (value < 1.0f) value = 1.0f;
+    return f2u(value);
+}
+
+/* scan_geometry (scan/geometry.hlsli:26-104). */
+static int scan_geometry(const uint32_t *lines, const uint32_t *paths, uint32_t path_records, uint32_t line_index,
+                         Traversal *s) {
+    memset(s, 0, sizeof(*s));
+    const uint32_t *line = lines + (size_t)line_index * 6u;
+    uint32_t path_id = line[0];
+    if (path_id >= path_records) return 0;
+    const uint32_t *path = paths + (size_t)path_id * 19u;
+    for (int i = 0; i < 4; ++i) s->bbox[i] = path[6 + i];
+    s->data_offset = path[4];
+    if (s->bbox[2] - s->bbox[0] == 0u || s->bbox[1] >= s->bbox[3]) return 0;
+    float ta = bits_f(path[13]), tb = bits_f(path[14]), tc = bits_f(path[15]);
+    float td = bits_f(path[16]), te = bits_f(path[17]), tf = bits_f(path[18]);
+    float l0x = bits_f(line[2]), l0y = bits_f(line[3]), l1x = bits_f(line[4]), l1y = bits_f(line[5]);
+    float p0x = ta * l0x + tc * l0y + te, p0y = tb * l0x + td * l0y + tf;
+    float p1x = ta * l1x + tc * l1y + te, p1y = tb * l1x + td * l1y + tf;
+    s->down = p1y >= p0y;
+    if (s->down) {
+        s->pts[0] = p0x; s->pts[1] = p0y; s->pts[2] = p1x; s->pts[3] = p1y;
+    } else {
+        s->pts[0] = p1x; s->pts[1] = p1y; s->pts[2] = p0x; s->pts[3] = p0y;
+    }
+    const float scale = 1.0f / (float)TILE;
+    float sx = s->pts[0] * scale, sy = s->pts[1] * scale, sz = s->pts[2] * scale, sw = s->pts[3] * scale;
+    uint32_t count_x = span(sx, sz) - 1u;
+    s->count = count_x + span(sy, sw);
+    float dx = fabsf(sz - sx), dy = sw - sy;
+    if (dx + dy == 0.0f || (dy == 0.0f && floorf(sy) == sy)) return 0;
+    float reciprocal = 1.0f / (dx + dy);
+    s->a = dx * reciprocal;
+    int positive = sz >= sx;
+    s->sign = positive ? 1.0f : -1.0f;
+    float xt0 = floorf(sx * s->sign);
+    float c = sx * s->sign - xt0;
+    s->y0 = floorf(sy);
+    float ytop = s->y0 + 1.0f;
+    if (sy == sw) ytop = ceilf(sy);
+    s->b = fminf((dy * c + dx * (ytop - sy)) * reciprocal, 0.99999994f);
+    float error = floorf(s->a * ((float)s->count - 1.0f) + s->b) - (float)count_x;
+    if (error != 0.0f) {
+        if (error > 0.0f) s->a -= 0.0000002f;
+        else s->a += 0.0000002f;
+    }
+    s->x0 = xt0 * s->sign - 1.0f;
+    if (positive) s->x0 = xt0 * s->sign;
+    float xmin = fminf(sx, sz);
+    float bx = (float)s->bbox[0], by = (float)s->bbox[1], bz = (float)s->bbox[2], bw = (float)s->bbox[3];
+    if (sy >= bw || sw <= by + TOP_TOUCH_EPSILON || xmin >= bz) return 0;
+    s->imin = 0u;
+    if (sy < by) {
+        float first = rintf((by - s->y0 + s->b - s->a) / (1.0f - s->a)) - 1.0f;
+        if (s->y0 + first - floorf(s->a * first + s->b) < by) first += 1.0f;
+        s->imin = f2u(first);
+    }
+    s->imax = s->count;
+    if (sw > bw) {
+        float last = rintf((bw - s->y0 + s->b - s->a) / (1.0f - s->a)) - 1.0f;
+        if (s->y0 + last - floorf(s->a * last + s->b) < bw) last += 1.0f;
+        s->imax = f2u(last);
+    }
+    s->ymin = 0;
+    s->ymax = 0;
+    if (fmaxf(sx, sz) < bx) {
+        s->ymin = ceil_tile_boundary_y(sy);
+        s->ymax = ceil_tile_boundary_y(sw);
+        s->imax = s->imin;
+    } else {
+        float fudge = positive ? 0.0f : 1.0f;
+        if (xmin < bx) {
+            float cut = rintf((s->sign * (bx - s->x0) - s->b + fudge) / s->a);
+            if ((s->x0 + s->sign * floorf(s->a * cut + s->b) < bx) == positive) cut += 1.0f;
+            int32_t ynext = f2i(s->y0 + cut - floorf(s->a * cut + s->b) + 1.0f);
+            if (positive) {
+                if (f2u(cut) > s->imin) {
+                    float ystart = s->y0 + 1.0f;
+                    if (is_tile_boundary_y(sy)) ystart = s->y0;
+                    s->ymin = f2i(ystart);
+                    s->ymax = ynext;
+                    s->imin = f2u(cut);
+                }
+            } else if (f2u(cut) < s->imax) {
+                s->ymin = ynext;
+                s->ymax = ceil_tile_boundary_y(sw);
+                s->imax = f2u(cut);
+            }
+        }
+        if (fmaxf(sx, sz) > bz) {
+            float cut = rintf((s->sign * (bz - s->x0) - s->b + fudge) / s->a);
+            if ((s->x0 + s->sign * floorf(s->a * cut + s->b) < bz) == positive) cut += 1.0f;
+            if (positive) s->imax = umin(s->imax, f2u(cut));
+            else s->imin = umax(s->imin, f2u(cut));
+        }
+    }
+    s->imax = umax(s->imin, s->imax);
+    if (s->ymin < (int32_t)s->bbox[1]) s->ymin = (int32_t)s->bbox[1];
+    if (s->ymax > (int32_t)s->bbox[3]) s->ymax = (int32_t)s->bbox[3];
+    return 1;
+}
+
+static void scan_tile(const Traversal *s, uint32_t index, float z, int32_t *tx, int32_t *ty) {
+    *tx = f2i(s->x0 + s->sign * z);
+    *ty = f2i(s->y0 + (float)index - z);
+}
+static int scan_tile_inside(const Traversal *s, int32_t tx, int32_t ty) {
+    return ty >= (int32_t)s->bbox[1] && ty < (int32_t)s->bbox[3] && tx >= (int32_t)s->bbox[0] &&
+           tx < (int32_t)s->bbox[2];
+}
+static uint32_t scan_tile_offset(const Traversal *s, int32_t tx, int32_t ty) {
+    return s->data_offset + ((uint32_t)ty - s->bbox[1]) * (s->bbox[2] - s->bbox[0]) + (uint32_t)tx - s->bbox[0];
+}
+
+/* scan_count (scan/count.hlsl:13-40). */
+void or_scan_count(const or_scan_config *cfg, const uint32_t *lines, const uint32_t *paths,
+                   uint32_t path_record_count, int32_t *backdrops, uint32_t *counts) {
+    for (uint32_t id = 0; id < cfg->line_count; ++id) {
+        Traversal s;
+        if (!scan_geometry(lines, paths, path_record_count, id, &s)) continue;
+        int32_t delta = s.down ? -1 : 1;
+        uint32_t stride = s.bbox[2] - s.bbox[0];
+        for (int32_t y = s.ymin; y < s.ymax; y++) {
+            uint32_t local = (uint32_t)(y - (int32_t)s.bbox[1]) * stride;
+            backdrops[s.data_offset + local] = (int32_t)((uint32_t)backdrops[s.data_offset + local] + (uint32_t)delta);
+        }
+        float last_z = floorf(s.a * ((float)s.imin - 1.0f) + s.b);
+        for (uint32_t i = s.imin; i < s.imax; i++) {
+            float z = floorf(s.a * (float)i + s.b);
+            int32_t tx, ty;
+            scan_tile(&s, i, z, &tx, &ty);
+            if (scan_tile_inside(&s, tx, ty)) {
+                int top = last_z == z;
+                if (i == 0u) top = fabsf(s.y0 - s.pts[1] * (1.0f / (float)TILE)) <= SCAN_EPSILON;
+                if (top && tx + 1 < (int32_t)s.bbox[2]) {
+                    int32_t x = tx + 1 > (int32_t)s.bbox[0] ? tx + 1 : (int32_t)s.bbox[0];
+                    uint32_t local = (uint32_t)(ty - (int32_t)s.bbox[1]) * stride + (uint32_t)(x - (int32_t)s.bbox[0]);
+                    backdrops[s.data_offset + local] =
+                        (int32_t)((uint32_t)backdrops[s.data_offset + local] + (uint32_t)delta);
+                }
+                counts[scan_tile_offset(&s, tx, ty)] += 1u;
+            }
+            last_z = z;
+        }
+    }
+}
+
+/* scan_prefix_chunks (scan/prefix_chunks.hlsl:16-38). chunks: 4 words {path, backdrop_offset, segment_start, len}. */
+void or_scan_prefix_chunks(const or_scan_config *cfg, const uint32_t *chunks, const uint32_t *counts,
+                           uint32_t *ranges, uint32_t *chunk_totals) {
+    for (uint32_t k = 0; k < cfg->scan_chunk_count; ++k) {
+        const uint32_t *chunk = chunks + (size_t)k * 4u;
+        uint32_t inclusive = 0u;
+        for (uint32_t lane = 0; lane < 256u; ++lane) {
+            uint32_t count = lane < chunk[3] ? counts[chunk[1] + lane] : 0u;
+            inclusive += count;
+            if (lane < chunk[3]) {
+                ranges[(size_t)(chunk[1] + lane) * 2u] = inclusive - count;
+                ranges[(size_t)(chunk[1] + lane) * 2u + 1u] = inclusive;
+            }
+            if (lane + 1u == chunk[3]) chunk_totals[k] = inclusive;
+        }
+    }
+}
+
+/* scan_chunk_offsets (scan/chunk_offsets.hlsl:14-26). */
+void or_scan_chunk_offsets(const or_scan_config *cfg, const uint32_t *paths, const uint32_t *chunk_ranges,
+                           uint32_t *bumps, const uint32_t *chunk_totals, uint32_t *chunk_offsets) {
+    for (uint32_t p = 0; p < cfg->path_count; ++p) {
+        uint32_t start = paths[(size_t)p * 19u + 10u];
+        uint32_t next = start;
+        for (uint32_t c = chunk_ranges[2 * p]; c < chunk_ranges[2 * p + 1]; ++c) {
+            chunk_offsets[c] = next;
+            next += chunk_totals[c];
+        }
+        bumps[p] = next - start;
+    }
+}
+
+/* scan_apply_chunk_offsets (scan/apply_chunk_offsets.hlsl:15-27). */
+void or_scan_apply_chunk_offsets(const or_scan_config *cfg, const uint32_t *chunks, uint32_t *ranges,
+                                 uint32_t *cursors, const uint32_t *chunk_offsets) {
+    for (uint32_t k = 0; k < cfg->scan_chunk_count; ++k) {
+        const uint32_t *chunk = chunks + (size_t)k * 4u;
+        uint32_t base = chunk_offsets[k];
+        for (uint32_t lane = 0; lane < 256u && lane < chunk[3]; ++lane) {
+            size_t index = chunk[1] + lane;
+            ranges[index * 2u] += base;
+            ranges[index * 2u + 1u] += base;
+            cursors[index] = ranges[index * 2u];
+        }
+    }
+}
+
+/* x_at_y / y_at_x / clip_y_at_x / write_clipped_segment (scan/clip.hlsli:10-75). */
+static float x_at_y(const float *p, float y) { return fmaf(p[2] - p[0], (y - p[1]) / (p[3] - p[1]), p[0]); }
+static float y_at_x(const float *p, float x) { return fmaf(p[3] - p[1], (x - p[0]) / (p[2] - p[0]), p[1]); }
+static float clip_y_at_x(const float *p, float x, float lo, float hi) {
+    float y = y_at_x(p, x);
+    if (y <= lo + SCAN_EPSILON) {
+        float top_x = x_at_y(p, lo);
+        if (fabsf(top_x - x) > TILE_CLIP_NUDGE) return lo;
+        return lo + TILE_CLIP_NUDGE;
+    }
+    return clampf(y, lo + TILE_CLIP_NUDGE, hi);
+}
+
+static void write_clipped_segment(uint32_t *out, const Traversal *s, uint32_t sub, float z, int32_t tx, int32_t ty) {
+    const float size = (float)TILE;
+    float minx = (float)tx * size, miny = (float)ty * size;
+    float maxx = minx + size, maxy = miny + size;
+    float q[4] = {s->pts[0], s->pts[1], s->pts[2], s->pts[3]};
+    if (sub > 0u) {
+        float previous = floorf(s->a * ((float)sub - 1.0f) + s->b);
+        if (z == previous) {
+            float x = clampf(x_at_y(s->pts, miny), minx + TILE_CLIP_NUDGE, maxx);
+            q[0] = x;
+            q[1] = miny;
+        } else {
+            float xc = maxx;
+            if (s->sign > 0.0f) xc = minx;
+            q[0] = xc;
+            q[1] = clip_y_at_x(s->pts, xc, miny, maxy);
+        }
+    }
+    if (sub < s->count - 1u) {
+        float next = floorf(s->a * ((float)sub + 1.0f) + s->b);
+        if (z == next) {
+            float x = clampf(x_at_y(s->pts, maxy), minx + TILE_CLIP_NUDGE, maxx);
+            q[2] = x;
+            q[3] = maxy;
+        } else {
+            float xc = minx;
+            if (s->sign > 0.0f) xc = maxx;
+            q[2] = xc;
+            q[3] = clip_y_at_x(s->pts, xc, miny, maxy);
+        }
+    }
+    float edge = 1000000000.0f;
+    float p0x = clampf(q[0] - minx, 0.0f, size), p0y = clampf(q[1] - miny, 0.0f, size);
+    float p1x = clampf(q[2] - minx, 0.0f, size), p1y = clampf(q[3] - miny, 0.0f, size);
+    if (p0x <= SCAN_EPSILON) p0x = 0.0f; else if (size - p0x <= SCAN_EPSILON) p0x = size;
+    if (p0y <= SCAN_EPSILON) p0y = 0.0f; else if (size - p0y <= SCAN_EPSILON) p0y = size;
+    if (p1x <= SCAN_EPSILON) p1x = 0.0f; else if (size - p1x <= SCAN_EPSILON) p1x = size;
+    if (p1y <= SCAN_EPSILON) p1y = 0.0f; else if (size - p1y <= SCAN_EPSILON) p1y = size;
+    if (p0x == 0.0f) {
+        if (p1x == 0.0f) {
+            p0x = SCAN_EPSILON;
+            if (p0y == 0.0f) { p1x = SCAN_EPSILON; p1y = size; }
+            else { p1x = 2.0f * SCAN_EPSILON; p1y = p0y; }
+        } else if (p0y == 0.0f) {
+            p0x = SCAN_EPSILON;
+        } else {
+            edge = p0y;
+        }
+    } else if (p1x == 0.0f) {
+        if (p1y == 0.0f) p1x = SCAN_EPSILON;
+        else edge = p1y;
+    }
+    if (floorf(p0x) == p0x && p0x != 0.0f) p0x -= SCAN_EPSILON;
+    if (floorf(p1x) == p1x && p1x != 0.0f) p1x -= SCAN_EPSILON;
+    if (!s->down) {
+        float x = p0x, y = p0y;
+        p0x = p1x; p0y = p1y; p1x = x; p1y = y;
+    }
+    out[0] = f_bits(p0x);
+    out[1] = f_bits(p0y);
+    out[2] = f_bits(p1x);
+    out[3] = f_bits(p1y);
+    out[4] = f_bits(edge);
+}
+
+/* scan_emit (scan/emit.hlsl:14-28), lines in index order. */
+void or_scan_emit(const or_scan_config *cfg, const uint32_t *lines, const uint32_t *paths,
+                  uint32_t path_record_count, uint32_t *cursors, uint32_t *segments) {
+    for (uint32_t id = 0; id < cfg->line_count; ++id) {
+        Traversal s;
+        if (!scan_geometry(lines, paths, path_record_count, id, &s)) continue;
+        for (uint32_t i = s.imin; i < s.imax; i++) {
+            float z = floorf(s.a * (float)i + s.b);
+            int32_t tx, ty;
+            scan_tile(&s, i, z, &tx, &ty);
+            if (scan_tile_inside(&s, tx, ty)) {
+                uint32_t dst = cursors[scan_tile_offset(&s, tx, ty)]++;
+                if (dst < cfg->segment_capacity) write_clipped_segment(segments + (size_t)dst * 5u, &s, i, z, tx, ty);
+            }
+        }
+    }
+}
+
+static int segment_cmp(const void *a, const void *b) {
+    const uint32_t *x = (const uint32_t *)a, *y = (const uint32_t *)b;
+    for (int i = 0; i < 5; ++i)
+        if (x[i] != y[i]) return x[i] < y[i] ? -1 : 1;
+    return 0;
+}
+
+void or_segment_order(const uint32_t *ranges, uint32_t backdrop_len, uint32_t cap, uint32_t *segments) {
+    for (uint32_t b = 0; b < backdrop_len; ++b) {
+        uint32_t start = ranges[2 * (size_t)b], end = umin(ranges[2 * (size_t)b + 1], cap);
+        if (end > start + 1u) qsort(segments + (size_t)start * 5u, end - start, 20, segment_cmp);
+    }
+}
+
+/* cumsum_prefix_chunks, cumsum_chunk_offsets, cumsum_apply_chunk_offsets (cumsum.hlsl:16-71). */
+void or_cumsum(uint32_t row_count, uint32_t chunk_count, const uint32_t *offs, const uint32_t *lens,
+               const uint32_t *row_starts, const uint32_t *row_ends, int32_t *backdrops) {
+    uint32_t *totals = (uint32_t *)calloc(chunk_count ? chunk_count : 1, 4);
+    uint32_t *carries = (uint32_t *)calloc(chunk_count ? chunk_count : 1, 4);
+    for (uint32_t c = 0; c < chunk_count; ++c) {
+        uint32_t sum = 0u;
+        for (uint32_t lane = 0; lane < 256u; ++lane) {
+            uint32_t v = lane < lens[c] ? (uint32_t)backdrops[offs[c] + lane] : 0u;
+            sum += v;
+            if (lane < lens[c]) backdrops[offs[c] + lane] = (int32_t)sum;
+        }
+        totals[c] = sum;
+    }
+    for (uint32_t r = 0; r < row_count; ++r) {
+        uint32_t carry = 0u;
+        for (uint32_t c = row_starts[r]; c < row_ends[r]; ++c) {
+            carries[c] = carry;
+            carry += totals[c];
+        }
+    }
+    for (uint32_t c = 0; c < chunk_count; ++c)
+        for (uint32_t lane = 0; lane < lens[c] && lane < 256u; ++lane)
+            backdrops[offs[c] + lane] = (int32_t)((uint32_t)backdrops[offs[c] + lane] + carries[c]);
+    free(totals);
+    free(carries);
+}
+
+/* line_scanned_tile_count / plan_scan_line (src/shared/scan_line.rs:17-151). */
+static uint32_t rs_u32(float v) {
+    if (!(v > 0.0f)) return 0u;
+    if (v >= 4294967295.0f) return 0xFFFFFFFFu;
+    return (uint32_t)v;
+}
+static int32_t rs_i32(float v) {
+    if (v != v) return 0;
+    if (v <= -2147483648.0f) return INT32_MIN;
+    if (v >= 2147483647.0f) return INT32_MAX;
+    return (int32_t)v;
+}
+static uint32_t rs_span(float a, float b) {
+    return rs_u32(fmaxf(ceilf(fmaxf(a, b)) - floorf(fminf(a, b)), 1.0f));
+}
+uint32_t or_line_scanned_tile_count(const float p0[2], const float p1[2], uint32_t bx0, uint32_t by0, uint32_t bx1,
+                                    uint32_t by1) {
+    int is_down = p1[1] >= p0[1];
+    const float *xy0 = is_down ? p0 : p1, *xy1 = is_down ? p1 : p0;
+    const float scale = 1.0f / 16.0f;
+    float s0x = xy0[0] * scale, s0y = xy0[1] * scale, s1x = xy1[0] * scale, s1y = xy1[1] * scale;
+    uint32_t count_x = rs_span(s0x, s1x) - 1u;
+    uint32_t count = count_x + rs_span(s0y, s1y);
+    float dx = fabsf(s1x - s0x), dy = s1y - s0y;
+    if (dx + dy == 0.0f) return 0;
+    if (dy == 0.0f && floorf(s0y) == s0y) return 0;
+    float idxdy = 1.0f / (dx + dy);
+    float a = dx * idxdy;
+    int pos = s1x >= s0x;
+    float sign = pos ? 1.0f : -1.0f;
+    float xt0 = floorf(s0x * sign);
+    float c = s0x * sign - xt0;
+    float y0 = floorf(s0y);
+    float ytop = s0y == s1y ? ceilf(s0y) : y0 + 1.0f;
+    float b = fminf((dy * c + dx * (ytop - s0y)) * idxdy, 0.99999994f);
+    float err = floorf(a * ((float)count - 1.0f) + b) - (float)count_x;
+    if (err != 0.0f) a -= copysignf(2e-7f, err);
+    float x0 = xt0 * sign + (pos ? 0.0f : -1.0f);
+    float xmin = fminf(s0x, s1x);
+    if (s0y >= (float)by1 || s1y <= (float)by0 + 1.0e-12f || xmin >= (float)bx1) return 0;
+    uint32_t imin = 0, imax = count;
+    if (s0y < (float)by0) {
+        float f = roundf(((float)by0 - y0 + b - a) / (1.0f - a)) - 1.0f;
+        if (y0 + f - floorf(a * f + b) < (float)by0) f += 1.0f;
+        imin = rs_u32(f);
+    }
+    if (s1y > (float)by1) {
+        float f = roundf(((float)by1 - y0 + b - a) / (1.0f - a)) - 1.0f;
+        if (y0 + f - floorf(a * f + b) < (float)by1) f += 1.0f;
+        imax = rs_u32(f);
+    }
+    if (fmaxf(s0x, s1x) < (float)bx0) {
+        imax = imin;
+    } else {
+        float fudge = pos ? 0.0f : 1.0f;
+        if (xmin < (float)bx0) {
+            float f = roundf((sign * ((float)bx0 - x0) - b + fudge) / a);
+            if ((x0 + sign * floorf(a * f + b) < (float)bx0) == pos) f += 1.0f;
+            if (pos) {
+                if (rs_u32(f) > imin) imin = rs_u32(f);
+            } else if (rs_u32(f) < imax) {
+                imax = rs_u32(f);
+            }
+        }
+        if (fmaxf(s0x, s1x) > (float)bx1) {
+            float f = roundf((sign * ((float)bx1 - x0) - b + fudge) / a);
+            if ((x0 + sign * floorf(a * f + b) < (float)bx1) == pos) f += 1.0f;
+            if (pos) imax = umin(imax, rs_u32(f));
+            else imin = umax(imin, rs_u32(f));
+        }
+    }
+    imax = umax(imin, imax);
+    uint32_t total = 0;
+    for (uint32_t i = imin; i < imax; ++i) {
+        float z = floorf(a * (float)i + b);
+        int32_t y = rs_i32(y0 + (float)i - z), x = rs_i32(x0 + sign * z);
+        total += (y >= (int32_t)by0 && y < (int32_t)by1 && x >= (int32_t)bx0 && x < (int32_t)bx1);
+    }
+    return total;
+}
+
+/* ---------- pixel math (shared/pixel.hlsli:7-66) ----------------------------- */
+
+static const float CHANNEL_SCALE = 1.0f / 255.0f;
+static uint32_t coverage_to_u8(float c) { return f2u(clampf(c, 0.0f, 1.0f) * 255.0f + 0.5f); }
+static uint32_t coverage_to_alpha(float v, uint32_t rule) {
+    float alpha = fminf(fabsf(v), 1.0f);
+    if (rule == 1u) alpha = fabsf(v - 2.0f * rintf(0.5f * v));
+    return coverage_to_u8(alpha);
+}
+static float signum_f32(float v) { return v < 0.0f ? -1.0f : 1.0f; }
+static uint32_t pack4(uint32_t r, uint32_t g, uint32_t b, uint32_t a) { return r | (g << 8) | (b << 16) | (a << 24); }
+typedef struct { float c[4]; } Px;
+static uint32_t unorm_to_rgba8(Px p) {
+    uint32_t ch[4];
+    for (int i = 0; i < 4; ++i) ch[i] = f2u(clampf(p.c[i] * 255.0f + 0.5f, 0.0f, 255.0f));
+    return pack4(ch[0], ch[1], ch[2], ch[3]);
+}
+static Px rgba8_to_unorm(uint32_t p) {
+    Px r;
+    for (int i = 0; i < 4; ++i) r.c[i] = (float)((p >> (8 * i)) & 255u) * CHANNEL_SCALE;
+    return r;
+}
+static uint32_t mul_div255(uint32_t a, uint32_t b) {
+    uint32_t product = a * b + 128u;
+    return (product + (product >> 8)) >> 8;
+}
+static uint32_t combine_alpha(uint32_t a, uint32_t b) { return mul_div255(a, b); }
+static float straight_channel(uint32_t premul, uint32_t alpha) { return alpha == 0u ? 0.0f : (float)premul / (float)alpha; }
+static uint32_t scale_premul_u8(uint32_t s, uint32_t f) {
+    if (f == 0u) return 0u;
+    if (f == 255u) return s;
+    return pack4(mul_div255(s & 255u, f), mul_div255((s >> 8) & 255u, f), mul_div255((s >> 16) & 255u, f),
+                 mul_div255(s >> 24, f));
+}
+static Px scale_premul_u8_to_unorm(uint32_t s, uint32_t f) {
+    Px r = {{0, 0, 0, 0}};
+    if (f == 0u || (s >> 24) == 0u) return r;
+    float scale = (float)f * (1.0f / 65025.0f);
+    for (int i = 0; i < 4; ++i) r.c[i] = (float)((s >> (8 * i)) & 255u) * scale;
+    return r;
+}
+static uint32_t src_over_premul_u8(uint32_t d, uint32_t s) {
+    uint32_t alpha = s >> 24;
+    if (alpha == 0u) return d;
+    if (alpha == 255u) return s;
+    uint32_t inv = 255u - alpha;
+    return pack4((s & 255u) + mul_div255(d & 255u, inv), ((s >> 8) & 255u) + mul_div255((d >> 8) & 255u, inv),
+                 ((s >> 16) & 255u) + mul_div255((d >> 16) & 255u, inv), alpha + mul_div255(d >> 24, inv));
+}
+static Px src_over_premul_unorm(Px d, Px s) {
+    if (s.c[3] <= 0.0f) return d;
+    if (s.c[3] >= 1.0f) return s;
+    Px r;
+    for (int i = 0; i < 4; ++i) r.c[i] = s.c[i] + d.c[i] * (1.0f - s.c[3]);
+    return r;
+}
+static uint32_t pack_premul_rgba8(float r, float g, float b, float a) {
+    return pack4(f2u(clampf(r, 0, 1) * 255.0f + 0.5f), f2u(clampf(g, 0, 1) * 255.0f + 0.5f),
+                 f2u(clampf(b, 0, 1) * 255.0f + 0.5f), f2u(clampf(a, 0, 1) * 255.0f + 0.5f));
+}
+static uint32_t lerp_premul_u8(uint32_t a, uint32_t b, float t) {
+    uint32_t v[4];
+    for (int i = 0; i < 4; ++i) {
+        float l = (float)((a >> (8 * i)) & 255u), r = (float)((b >> (8 * i)) & 255u);
+        v[i] = f2u(clampf(fmaf(r - l, t, l) + 0.5f, 0.0f, 255.0f));
+    }
+    return pack4(v[0], v[1], v[2], v[3]);
+}
+static float rem_euclid_f32(float v, float m) { return fmaf(-floorf(v / m), m, v); }
+static int premul_opaque(uint32_t c) { return (c >> 24) == 255u; }
+
+/* ---------- row-sweep coverage (shared/coverage.hlsli:7-51) --------------------- */
+
+typedef struct { float edge, dy, xmin, xmax; } RowParts;
+static RowParts segment_row_parts(float p0x, float p0y, float p1x, float p1y, float y_edge, uint32_t y) {
+    RowParts r;
+    float delta_x = p1x - p0x, delta_y = p1y - p0y;
+    float row_y = (float)y, local_y = p0y - row_y;
+    float y0 = clampf(local_y, 0.0f, 1.0f), y1 = clampf(p1y - row_y, 0.0f, 1.0f);
+    float dy = y0 - y1;
+    r.edge = signum_f32(delta_x) * clampf(row_y - y_edge + 1.0f, 0.0f, 1.0f);
+    r.dy = dy;
+    r.xmin = r.xmax = 0.0f;
+    if (dy == 0.0f) return r;
+    float slope = delta_x / delta_y;
+    int nearer_p0 = fabsf(local_y - 0.5f) <= fabsf(p1y - row_y - 0.5f);
+    float ax = nearer_p0 ? p0x : p1x;
+    float ay = nearer_p0 ? local_y : p1y - row_y;
+    float sx0 = fmaf(y0 - ay, slope, ax), sx1 = fmaf(y1 - ay, slope, ax);
+    r.xmin = fminf(sx0, sx1);
+    r.xmax = fmaxf(sx0, sx1);
+    return r;
+}
+static float segment_area_at(float xmin_abs, float xmax_abs, uint32_t x) {
+    float xmin = xmin_abs - (float)x, xmax = xmax_abs - (float)x;
+    float width = xmax - xmin;
+    if (width == 0.0f) return clampf(1.0f - xmin, 0.0f, 1.0f);
+    if (xmin >= 0.0f && xmax <= 1.0f) return fmaf(-0.5f, xmin + xmax, 1.0f);
+    float left = clampf(xmin, 0.0f, 1.0f), right = clampf(xmax, 0.0f, 1.0f);
+    float full = clampf(-xmin, 0.0f, width);
+    return fmaf(right - left, fmaf(-0.5f, left + right, 1.0f), full) / width;
+}
+static float fill_value_at(const uint32_t *segs, int32_t backdrop, uint32_t start, uint32_t end, uint32_t x,
+                           uint32_t y) {
+    float base = (float)backdrop, running = 0.0f, partial = 0.0f;
+    for (uint32_t i = start; i < end; ++i) {
+        const uint32_t *s = segs + (size_t)i * 5u;
+        RowParts p = segment_row_parts(bits_f(s[0]), bits_f(s[1]), bits_f(s[2]), bits_f(s[3]), bits_f(s[4]), y);
+        base += p.edge;
+        if (p.dy != 0.0f) {
+            int full_start = clampi(f2i(ceilf(p.xmax)), 0, (int)TILE);
+            if (full_start < (int)TILE && (int)x >= full_start) running += p.dy;
+            int ps = clampi(f2i(floorf(p.xmin)), 0, (int)TILE), pe = clampi(f2i(ceilf(p.xmax)), 0, (int)TILE);
+            if ((int)x >= ps && (int)x < pe) partial += segment_area_at(p.xmin, p.xmax, x) * p.dy;
+        }
+    }
+    return base + running + partial;
+}
+
+/* ---------- SDF + brush (shared/sdf/ and shared/brush/ headers) -------------------- */
+
+static uint32_t blob_word(const uint32_t *paint, uint32_t count, uint32_t base, uint32_t index) {
+    if (base < count && index < count - base) return paint[base + index];
+    return 0u;
+}
+static float blob_float(const uint32_t *paint, uint32_t count, uint32_t base, uint32_t index) {
+    return bits_f(blob_word(paint, count, base, index));
+}
+typedef struct { float a, b, c, d, e, f; } Aff;
+typedef struct { float distance, nx, ny; } Sdf;
+static float vlen(float x, float y) { return sqrtf(x * x + y * y); }
+static float sdf_length(float x, float y) {
+    float m = vlen(x, y);
+    if (m > 0.0f && isfinite(m)) return m;
+    float axis = fmaxf(fabsf(x), fabsf(y));
+    return axis == 0.0f ? 0.0f : axis * vlen(x / axis, y / axis);
+}
+static Sdf rounded_box(float px, float py, float hx, float hy, float radius) {
+    float qx = fabsf(px) - hx + radius, qy = fabsf(py) - hy + radius;
+    float ox = fmaxf(qx, 0.0f), oy = fmaxf(qy, 0.0f);
+    float ol = sdf_length(ox, oy);
+    Sdf s;
+    s.distance = fminf(fmaxf(qx, qy), 0.0f) + ol - radius;
+    float sx = px >= 0.0f ? 1.0f : -1.0f, sy = py >= 0.0f ? 1.0f : -1.0f;
+    if (ol > 0.0f) { s.nx = ox / ol * sx; s.ny = oy / ol * sy; }
+    else if (qx > qy) { s.nx = sx; s.ny = 0.0f; }
+    else { s.nx = 0.0f; s.ny = sy; }
+    return s;
+}
+static Sdf rect_sdf(float x, float y, float x0r, float y0r, float x1r, float y1r, float tl, float tr, float bl, float br) {
+    float x0 = fminf(x0r, x1r), y0 = fminf(y0r, y1r), x1 = fmaxf(x0r, x1r), y1 = fmaxf(y0r, y1r);
+    float cx = (x0 + x1) * 0.5f, cy = (y0 + y1) * 0.5f, hx = (x1 - x0) * 0.5f, hy = (y1 - y0) * 0.5f;
+    float px = x - cx, py = y - cy;
+    float radius = tl;
+    if (px >= 0.0f) radius = py <= 0.0f ? tr : br;
+    else if (py > 0.0f) radius = bl;
+    float r = fmaxf(fminf(fminf(radius, hx), hy), 0.0f);
+    return rounded_box(px, py, hx, hy, r);
+}
+static float sdf_cov(Sdf s, const Aff *inv) {
+    float gx = fmaf(inv->a, s.nx, inv->b * s.ny), gy = fmaf(inv->c, s.nx, inv->d * s.ny);
+    float fp = sdf_length(gx, gy);
+    if (fp == 0.0f) return s.distance < 0.0f ? 1.0f : (s.distance > 0.0f ? 0.0f : 0.5f);
+    return clampf(0.5f - s.distance / fp, 0.0f, 1.0f);
+}
+static float sdf_coverage_from_blob(const uint32_t *paint, uint32_t n, uint32_t base, float x, float y, const Aff *inv) {
+    uint32_t kind = blob_word(paint, n, base, 0);
+    float v[17];
+    for (uint32_t i = 1; i < 17; ++i) v[i] = blob_float(paint, n, base, i);
+    if (kind == 1u) return sdf_cov(rect_sdf(x, y, v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]), inv);
+    if (kind == 3u) {
+        float ht = fmaxf(v[9], 0.0f), hr = fmaxf(v[10], 0.0f), hb = fmaxf(v[11], 0.0f), hl = fmaxf(v[12], 0.0f);
+        float rx0 = fminf(v[1], v[3]), ry0 = fminf(v[2], v[4]), rx1 = fmaxf(v[1], v[3]), ry1 = fmaxf(v[2], v[4]);
+        float outer = sdf_cov(rect_sdf(x, y, rx0 - hl, ry0 - ht, rx1 + hr, ry1 + hb, v[5] + fmaxf(ht, hl),
+                                       v[6] + fmaxf(ht, hr), v[7] + fmaxf(hb, hl), v[8] + fmaxf(hb, hr)), inv);
+        float ix0 = rx0 + hl, iy0 = ry0 + ht, ix1 = rx1 - hr, iy1 = ry1 - hb, inner = 0.0f;
+        if (ix0 < ix1 && iy0 < iy1)
+            inner = sdf_cov(rect_sdf(x, y, ix0, iy0, ix1, iy1, fmaxf(v[5] - fmaxf(ht, hl), 0.0f),
+                                     fmaxf(v[6] - fmaxf(ht, hr), 0.0f), fmaxf(v[7] - fmaxf(hb, hl), 0.0f),
+                                     fmaxf(v[8] - fmaxf(hb, hr), 0.0f)), inv);
+        return clampf(outer - inner, 0.0f, 1.0f);
+    }
+    return 0.0f;
+}
+static float apply_extend(float t, uint32_t extend) {
+    if (extend == 1u) return rem_euclid_f32(t, 1.0f);
+    if (extend == 2u) {
+        float v = rem_euclid_f32(t, 2.0f);
+        return v <= 1.0f ? v : 2.0f - v;
+    }
+    return clampf(t, 0.0f, 1.0f);
+}
+static uint32_t sample_ramp(const uint32_t *paint, uint32_t n, uint32_t bb, uint32_t payload, uint32_t len, float t,
+                            uint32_t extend) {
+    if (len == 0u) return 0u;
+    uint32_t last = len - 1u;
+    float e = apply_extend(t, extend);
+    float position = e * (float)last;
+    uint32_t li = f2u(floorf(position)), ri = umin(li + 1u, last);
+    float fraction = fmaf(e, (float)last, -(float)li);
+    uint32_t left = blob_word(paint, n, bb, payload + li), right = blob_word(paint, n, bb, payload + ri);
+    if (fraction <= 0.00000011920929f || li == ri) return left;
+    return lerp_premul_u8(left, right, fraction);
+}
+static uint32_t sample_brush(const uint32_t *paint, uint32_t n, uint32_t bb, uint32_t db, float x, float y) {
+    uint32_t kind = blob_word(paint, n, bb, db);
+    uint32_t extend = blob_word(paint, n, bb, db + 1u);
+    uint32_t payload = db + blob_word(paint, n, bb, db + 2u);
+    uint32_t len = blob_word(paint, n, bb, db + 3u);
+    uint32_t base = db + 9u;
+    uint32_t color = blob_word(paint, n, bb, db + 4u);
+    if (kind == 2u) {
+        float p[10];
+        for (uint32_t i = 0; i < 10; ++i) p[i] = blob_float(paint, n, bb, base + i);
+        float tx = fmaf(p[4], x, fmaf(p[6], y, p[8])), ty = fmaf(p[5], x, fmaf(p[7], y, p[9]));
+        float dx = p[2] - p[0], dy = p[3] - p[1];
+        float den = dx * dx + dy * dy, t = 0.0f;
+        if (den > 0.00000011920929f) t = fmaf(tx - p[0], dx, (ty - p[1]) * dy) / den;
+        return sample_ramp(paint, n, bb, payload, len, t, extend);
+    }
+    if (kind == 6u) return 0u;
+    return color;
+}
+
+/* ---------- coarse (src/shaders/hlsl/coarse/, shared/draw.hlsli) ------------------ */
+
+typedef struct {
+    uint32_t path_id, glyph_run_id, sdf_offset, sdf_len, shadow_offset, tag, fill_rule, brush_offset, solid_rect;
+    int32_t pb[4];
+    float lin[4], tr[2];
+} Draw;
+
+static Draw load_draw(const uint32_t *draws, uint32_t count, uint32_t index) {
+    Draw d;
+    memset(&d, 0, sizeof(d));
+    if (index >= count) return d;
+    const uint32_t *r = draws + (size_t)index * 31u;
+    d.path_id = r[0]; d.glyph_run_id = r[1]; d.sdf_offset = r[2]; d.sdf_len = r[3]; d.shadow_offset = r[4];
+    d.brush_offset = r[6]; d.tag = r[8]; d.fill_rule = r[9]; d.solid_rect = r[18];
+    for (int i = 0; i < 4; ++i) d.pb[i] = (int32_t)r[10 + i];
+    for (int i = 0; i < 4; ++i) d.lin[i] = bits_f(r[19 + i]);
+    d.tr[0] = bits_f(r[23]);
+    d.tr[1] = bits_f(r[24]);
+    return d;
+}
+static int has_sdf(const Draw *d) { return d->sdf_offset != NONE || d->shadow_offset != NONE; }
+static uint32_t tmin_of(int32_t p, uint32_t lim) { return p > 0 ? umin((uint32_t)p / TILE, lim) : 0u; }
+static uint32_t tmax_of(int32_t p, uint32_t lim) { return p > 0 ? umin(((uint32_t)p + TILE - 1u) / TILE, lim) : 0u; }
+static int draw_hits_tile(const Draw *d, uint32_t tx, uint32_t ty, uint32_t tw, uint32_t th) {
+    return tx >= tmin_of(d->pb[0], tw) && ty >= tmin_of(d->pb[1], th) && tx < tmax_of(d->pb[2], tw) &&
+           ty < tmax_of(d->pb[3], th);
+}
+static uint32_t draw_backdrop(const or_coarse_config *cfg, const uint32_t *paths, const Draw *d, uint32_t tx,
+                              uint32_t ty) {
+    if (d->path_id == NONE || d->tag > 5u || d->path_id >= cfg->path_count ||
+        !draw_hits_tile(d, tx, ty, cfg->tiles_width, cfg->tiles_height))
+        return NONE;
+    const uint32_t *p = paths + (size_t)d->path_id * 19u;
+    if (p[8] <= p[6] || tx < p[6] || ty < p[7] || tx >= p[8] || ty >= p[9]) return NONE;
+    return p[4] + (ty - p[7]) * (p[8] - p[6]) + tx - p[6];
+}
+static uint32_t pw(const or_coarse_config *cfg, const or_coarse_inputs *in, uint32_t i) {
+    return i < cfg->paint_words ? in->paint[i] : 0u;
+}
+static int corner_covers(float px, float py, float cx, float cy, float ox, float oy, float r, float inset) {
+    int in_square = r > 0.0f && fabsf(px - cx) < r && fabsf(py - cy) < r;
+    float inner = r - inset, dx = px - ox, dy = py - oy;
+    return !in_square || (inner > 0.0f && dx * dx + dy * dy <= inner * inner);
+}
+/* rect_covers_tile (coarse/sdf_clip.hlsli:15-32). */
+static int rect_covers_tile(const float *rect, const float *radii_in, const Draw *d, uint32_t tx, uint32_t ty,
+                            float inset) {
+    float lx = fminf(rect[0], rect[2]), ly = fminf(rect[1], rect[3]);
+    float ux = fmaxf(rect[0], rect[2]), uy = fmaxf(rect[1], rect[3]);
+    float tlx = (float)(tx * TILE) + 0.5f - d->tr[0], tly = (float)(ty * TILE) + 0.5f - d->tr[1];
+    float tux = tlx + (float)(TILE - 1u), tuy = tly + (float)(TILE - 1u);
+    float sx = ux - lx, sy = uy - ly, need = (float)(TILE - 1u) + 2.0f * inset;
+    if (sx < need || sy < need || tlx < lx + inset || tly < ly + inset || tux > ux - inset || tuy > uy - inset)
+        return 0;
+    float limit = fminf(sx, sy) * 0.5f;
+    for (int i = 0; i < 4; ++i)
+        if (!(fabsf(rect[i]) <= 65536.0f) || !(fabsf(radii_in[i]) <= 65536.0f)) return 0;
+    if (!(fabsf(d->tr[0]) <= 65536.0f) || !(fabsf(d->tr[1]) <= 65536.0f)) return 0;
+    float r[4];
+    for (int i = 0; i < 4; ++i) r[i] = fminf(fmaxf(radii_in[i], 0.0f), limit);
+    return corner_covers(tlx, tly, lx, ly, lx + r[0], ly + r[0], r[0], inset) &&
+           corner_covers(tux, tly, ux, ly, ux - r[1], ly + r[1], r[1], inset) &&
+           corner_covers(tlx, tuy, lx, uy, lx + r[2], uy - r[2], r[2], inset) &&
+           corner_covers(tux, tuy, ux, uy, ux - r[3], uy - r[3], r[3], inset);
+}
+static int identity_linear(const Draw *d) {
+    return d->lin[0] == 1.0f && d->lin[1] == 0.0f && d->lin[2] == 0.0f && d->lin[3] == 1.0f;
+}
+/* sdf_clip_covers (coarse/sdf_clip.hlsli:34-39). */
+static int sdf_clip_covers(const or_coarse_config *cfg, const or_coarse_inputs *in, const Draw *d, uint32_t tx,
+                           uint32_t ty) {
+    if (d->sdf_offset == NONE || d->shadow_offset != NONE || d->sdf_len < 9u) return 0;
+    uint32_t base = d->sdf_offset;
+    if (pw(cfg, in, base) != 1u || !identity_linear(d)) return 0;
+    float rect[4], radii[4];
+    for (int i = 0; i < 4; ++i) {
+        rect[i] = bits_f(pw(cfg, in, base + 1u + (uint32_t)i));
+        radii[i] = bits_f(pw(cfg, in, base + 5u + (uint32_t)i));
+    }
+    return rect_covers_tile(rect, radii, d, tx, ty, 0.5f);
+}
+/* sdf_stroke_empty (coarse/sdf_clip.hlsli:40-59). */
+static int sdf_stroke_empty(const or_coarse_config *cfg, const or_coarse_inputs *in, const Draw *d, uint32_t tx,
+                            uint32_t ty) {
+    if (d->sdf_offset == NONE || d->shadow_offset != NONE || d->sdf_len < 13u || !identity_linear(d)) return 0;
+    uint32_t base = d->sdf_offset;
+    if (pw(cfg, in, base) != 3u) return 0;
+    float rect[4], radii[4], stroke[4];
+    for (int i = 0; i < 4; ++i) {
+        rect[i] = bits_f(pw(cfg, in, base + 1u + (uint32_t)i));
+        radii[i] = bits_f(pw(cfg, in, base + 5u + (uint32_t)i));
+        stroke[i] = bits_f(pw(cfg, in, base + 9u + (uint32_t)i));
+    }
+    for (int i = 0; i < 4; ++i)
+        if (!(fabsf(rect[i]) <= 65536.0f) || !(fabsf(radii[i]) <= 65536.0f) || !(fabsf(stroke[i]) <= 65536.0f))
+            return 0;
+    float hw[4];
+    for (int i = 0; i < 4; ++i) hw[i] = fmaxf(stroke[i], 0.0f);
+    float lx = fminf(rect[0], rect[2]), ly = fminf(rect[1], rect[3]), ux = fmaxf(rect[0], rect[2]),
+          uy = fmaxf(rect[1], rect[3]);
+    float inner[4] = {lx + hw[3], ly + hw[0], ux - hw[1], uy - hw[2]};
+    if (inner[0] >= inner[2] || inner[1] >= inner[3]) return 0;
+    float corner[4] = {fmaxf(hw[0], hw[3]), fmaxf(hw[0], hw[1]), fmaxf(hw[2], hw[3]), fmaxf(hw[2], hw[1])};
+    float r[4];
+    for (int i = 0; i < 4; ++i) r[i] = fmaxf(radii[i] - corner[i], 0.0f);
+    return rect_covers_tile(inner, r, d, tx, ty, 1.0f);
+}
+static int nontransparent_solid(const or_coarse_config *cfg, const or_coarse_inputs *in, const Draw *d) {
+    if (d->brush_offset == NONE) return 0;
+    uint32_t b = cfg->paint_brush_base + d->brush_offset;
+    return pw(cfg, in, b) == 1u && pw(cfg, in, b + 4u) != 0u;
+}
+static uint32_t solid_color(const or_coarse_config *cfg, const or_coarse_inputs *in, const Draw *d) {
+    return pw(cfg, in, cfg->paint_brush_base + d->brush_offset + 4u);
+}
+
+/* stack_wrapper_count (coarse/classify.hlsli:11-33). */
+static uint32_t stack_wrapper_count(const or_coarse_config *cfg, const or_coarse_inputs *in, uint32_t tx, uint32_t ty) {
+    uint32_t count = 0;
+    for (uint32_t i = cfg->layer_stack_start; i < cfg->layer_stack_end; ++i) {
+        const uint32_t *layer = in->layers + (size_t)i * 3u;
+        Draw d = load_draw(in->draws, cfg->draw_count, layer[1]);
+        if (has_sdf(&d)) {
+            if (sdf_clip_covers(cfg, in, &d, tx, ty)) continue;
+            if (!draw_hits_tile(&d, tx, ty, cfg->tiles_width, cfg->tiles_height)) return NONE;
+        } else {
+            uint32_t bd = draw_backdrop(cfg, in->paths, &d, tx, ty);
+            if (bd == NONE) return NONE;
+            uint32_t r0 = in->segment_ranges[2 * (size_t)bd], r1 = in->segment_ranges[2 * (size_t)bd + 1];
+            int32_t w = in->backdrops[bd];
+            int full = d.fill_rule == 1u ? (((uint32_t)w & 1u) != 0u) : w != 0;
+            if (layer[0] == 0u && r0 == r1 && full) continue;
+            if (r0 == r1 && w == 0) return NONE;
+        }
+        count++;
+    }
+    return count;
+}
+
+typedef struct { int valid; uint32_t tag, winding, fill_rule, s0, s1, color; } Particle;
+
+/* draw_particle (coarse/emit_draw.hlsli:10-52); draw_particle_count (classify.hlsli:34-49) is its validity. */
+static Particle draw_particle(const or_coarse_config *cfg, const or_coarse_inputs *in, uint32_t di, uint32_t tx,
+                              uint32_t ty) {
+    Particle p = {0, 1u, 0, 0, 0, 0, 0};
+    if (di >= cfg->draw_count || in->draw_batch_ids[di] != cfg->batch_id) return p;
+    Draw d = load_draw(in->draws, cfg->draw_count, di);
+    if (has_sdf(&d)) {
+        if (d.tag == 0u && !sdf_stroke_empty(cfg, in, &d, tx, ty)) {
+            p.valid = 1;
+            uint32_t color = nontransparent_solid(cfg, in, &d) && sdf_clip_covers(cfg, in, &d, tx, ty)
+                                 ? solid_color(cfg, in, &d) : 0u;
+            uint32_t bb = cfg->paint_brush_base + d.brush_offset;
+            int image = d.brush_offset != NONE && pw(cfg, in, bb) == 7u && pw(cfg, in, bb + 7u) == 255u &&
+                        sdf_clip_covers(cfg, in, &d, tx, ty);
+            if (color != 0u) { p.tag = 2u; p.color = color; }
+            else if (image) { p.tag = 13u; p.color = di; }
+            else { p.tag = 9u; p.s0 = di; p.color = di; }
+        }
+        return p;
+    }
+    uint32_t bd = draw_backdrop(cfg, in->paths, &d, tx, ty);
+    if (bd == NONE || (d.tag != 0u && d.tag != 5u && d.tag != 1u)) return p;
+    uint32_t r0 = in->segment_ranges[2 * (size_t)bd], r1 = in->segment_ranges[2 * (size_t)bd + 1];
+    uint32_t w = (uint32_t)in->backdrops[bd];
+    if (r0 == r1 && w == 0u) return p;
+    p.valid = 1;
+    p.winding = w;
+    p.fill_rule = d.fill_rule;
+    p.s0 = r0;
+    p.s1 = r1;
+    if (d.tag == 1u) p.tag = 3u;
+    else if (d.tag == 5u) p.tag = 11u;
+    else if (d.solid_rect != 0u && d.lin[1] == 0.0f && d.lin[2] == 0.0f && nontransparent_solid(cfg, in, &d) && r0 == r1)
+        p.tag = 2u;
+    if (p.tag == 2u) p.color = solid_color(cfg, in, &d);
+    else if (d.tag == 0u || d.tag == 5u) p.color = di;
+    return p;
+}
+
+/* coarse_count_bins (coarse/count.hlsl:48-73) + store_tile_counts (coarse/draw_list.hlsli:19-24). */
+void or_coarse_count_bins(const or_coarse_config *cfg, const or_coarse_inputs *in, uint32_t *work, uint32_t *kinds) {
+    for (uint32_t tile = 0; tile < cfg->tile_count; ++tile) {
+        uint32_t tx = tile % cfg->tiles_width, ty = tile / cfg->tiles_width;
+        uint32_t w = stack_wrapper_count(cfg, in, tx, ty);
+        uint32_t count = 0;
+        if (w != NONE) {
+            uint32_t start = in->tile_draws[2 * (size_t)tile], remaining = in->tile_draws[2 * (size_t)tile + 1];
+            for (uint32_t i = 0; start != NONE && i < remaining; ++i) {
+                uint32_t di = start + i < cfg->tile_draw_index_count ? in->tile_draw_indices[start + i] : 0u;
+                count += draw_particle(cfg, in, di, tx, ty).valid ? 1u : 0u;
+            }
+        }
+        if (count > 0u) count += w * 2u + 1u;
+        work[(size_t)tile * 6u] = count;
+        work[(size_t)tile * 6u + 3u] = 0u;
+        kinds[tile] = count > 0u ? 0u : 1u;
+    }
+}
+
+/* coarse_prefix_chunks / coarse_chunk_offsets / coarse_apply_chunk_offsets (coarse/prefix.hlsl:11-63). */
+void or_coarse_prefix(uint32_t tile_count, uint32_t *work, uint32_t *chunk_records) {
+    uint32_t chunks = (tile_count + 255u) / 256u;
+    for (uint32_t c = 0; c < chunks; ++c) {
+        uint32_t run[2] = {0u, 0u};
+        for (uint32_t lane = 0; lane < 256u; ++lane) {
+            uint32_t item = c * 256u + lane;
+            if (item >= tile_count) break;
+            uint32_t *rec = work + (size_t)item * 6u;
+            for (int k = 0; k < 2; ++k) {
+                uint32_t v = rec[3 * k];
+                rec[3 * k + 1] = run[k];
+                rec[3 * k + 2] = run[k] + v;
+                run[k] += v;
+            }
+        }
+        chunk_records[4 * c] = run[0];
+        chunk_records[4 * c + 2] = run[1];
+    }
+    uint32_t carry[2] = {0u, 0u};
+    for (uint32_t c = 0; c < chunks; ++c) {
+        chunk_records[4 * c + 1] = carry[0];
+        chunk_records[4 * c + 3] = carry[1];
+        carry[0] += chunk_records[4 * c];
+        carry[1] += chunk_records[4 * c + 2];
+    }
+    for (uint32_t item = 0; item < tile_count; ++item) {
+        uint32_t c = item / 256u;
+        uint32_t *rec = work + (size_t)item * 6u;
+        rec[1] += chunk_records[4 * c + 1];
+        rec[2] += chunk_records[4 * c + 1];
+        rec[4] += chunk_records[4 * c + 3];
+        rec[5] += chunk_records[4 * c + 3];
+    }
+}
+
+static void store_particle(const or_coarse_config *cfg, uint32_t *work, uint32_t dst, uint32_t tag, uint32_t w,
+                           uint32_t rule, uint32_t s0, uint32_t s1, uint32_t color) {
+    if (dst >= cfg->ptcl_capacity) return;
+    uint32_t *r = work + (size_t)cfg->tile_count * 6u + (size_t)dst * 6u;
+    r[0] = tag; r[1] = w; r[2] = rule; r[3] = s0; r[4] = s1; r[5] = color;
+}
+static int noop_stack_clip(const or_coarse_config *cfg, const or_coarse_inputs *in, const uint32_t *layer,
+                           const Draw *d, uint32_t tx, uint32_t ty) {
+    if (layer[0] != 0u) return 0;
+    if (has_sdf(d)) return sdf_clip_covers(cfg, in, d, tx, ty);
+    uint32_t bd = draw_backdrop(cfg, in->paths, d, tx, ty);
+    uint32_t r0 = in->segment_ranges[2 * (size_t)bd], r1 = in->segment_ranges[2 * (size_t)bd + 1];
+    uint32_t w = (uint32_t)in->backdrops[bd];
+    return r0 == r1 && (d->fill_rule == 1u ? (w & 1u) != 0u : w != 0u);
+}
+/* occluded_particle_prefix (coarse/occlusion.hlsli:12-34). */
+static uint32_t occluded_prefix(const or_coarse_config *cfg, const uint32_t *work, uint32_t tile, uint32_t end,
+                                uint32_t wrappers) {
+    const uint32_t *rec = work + (size_t)tile * 6u;
+    uint32_t start = rec[1];
+    if (wrappers != 0u || end <= start || end - start <= 2u || end > cfg->ptcl_capacity || end > rec[2]) return 0u;
+    const uint32_t *ptcl = work + (size_t)cfg->tile_count * 6u;
+    if (ptcl[(size_t)(end - 1u) * 6u] != 0u) return 0u;
+    uint32_t depth = 0;
+    for (uint32_t cursor = end - 1u; cursor > start;) {
+        --cursor;
+        uint32_t tag = ptcl[(size_t)cursor * 6u];
+        if (tag == 4u || tag == 6u || tag == 8u) depth++;
+        else if (tag == 3u || tag == 12u || tag == 5u || tag == 7u) {
+            if (depth == 0u) return 0u;
+            depth--;
+        } else if (tag == 2u && depth == 0u && (ptcl[(size_t)cursor * 6u + 5u] >> 24) == 255u) {
+            return cursor - start;
+        }
+    }
+    return 0u;
+}
+
+/* coarse_emit_bins (coarse/emit.hlsl:82-142) with emit_stack_begins/ends (coarse/emit_stack.hlsli:18-44). */
+void or_coarse_emit_bins(const or_coarse_config *cfg, const or_coarse_inputs *in, uint32_t *work, uint32_t *kinds) {
+    for (uint32_t tile = 0; tile < cfg->tile_count; ++tile) {
+        uint32_t tx = tile % cfg->tiles_width, ty = tile / cfg->tiles_width;
+        uint32_t *rec = work + (size_t)tile * 6u;
+        uint32_t px = rec[1], py = rec[2], raster_start = rec[1];
+        if (px >= py) { kinds[tile] = 1u; continue; }
+        uint32_t wrappers = stack_wrapper_count(cfg, in, tx, ty);
+        if (wrappers == NONE) {
+            store_particle(cfg, work, px, 0u, 0, 0, 0, 0, 0);
+            kinds[tile] = 1u;
+            continue;
+        }
+        uint32_t flags = wrappers != 0u ? 4u : 0u;
+        uint32_t dst = px;
+        for (uint32_t i = cfg->layer_stack_start; i < cfg->layer_stack_end; ++i) {
+            const uint32_t *layer = in->layers + (size_t)i * 3u;
+            Draw d = load_draw(in->draws, cfg->draw_count, layer[1]);
+            if (noop_stack_clip(cfg, in, layer, &d, tx, ty)) continue;
+            if (has_sdf(&d)) {
+                store_particle(cfg, work, dst++, 12u, 0, 0, 0, 0, layer[1]);
+            } else {
+                uint32_t bd = draw_backdrop(cfg, in->paths, &d, tx, ty);
+                uint32_t tag = layer[0] == 1u ? 5u : (layer[0] == 2u ? 7u : 3u);
+                store_particle(cfg, work, dst++, tag, (uint32_t)in->backdrops[bd], d.fill_rule,
+                               in->segment_ranges[2 * (size_t)bd], in->segment_ranges[2 * (size_t)bd + 1], layer[2]);
+            }
+        }
+        px += wrappers;
+        uint32_t start = in->tile_draws[2 * (size_t)tile], remaining = in->tile_draws[2 * (size_t)tile + 1];
+        for (uint32_t i = 0; start != NONE && i < remaining; ++i) {
+            uint32_t di = start + i < cfg->tile_draw_index_count ? in->tile_draw_indices[start + i] : 0u;
+            Particle p = draw_particle(cfg, in, di, tx, ty);
+            if (!p.valid) continue;
+            Draw d = load_draw(in->draws, cfg->draw_count, di);
+            uint32_t cls = 4u;
+            if (p.tag == 2u) cls = 1u;
+            else if (p.tag == 13u) cls = 2u;
+            else if (p.tag == 9u && nontransparent_solid(cfg, in, &d) && d.sdf_offset != NONE &&
+                     d.shadow_offset == NONE && d.sdf_len != 0u) {
+                uint32_t kind = pw(cfg, in, d.sdf_offset);
+                if (kind == 1u || kind == 5u || kind == 14u) cls = 2u;
+            }
+            flags |= cls;
+            if (wrappers == 0u && p.tag == 2u && (p.color >> 24) == 255u && px > raster_start) flags |= 8u;
+            if (px < py) store_particle(cfg, work, px++, p.tag, p.winding, p.fill_rule, p.s0, p.s1, p.color);
+        }
+        uint32_t end_dst = px;
+        for (uint32_t i = cfg->layer_stack_end; i > cfg->layer_stack_start;) {
+            --i;
+            const uint32_t *layer = in->layers + (size_t)i * 3u;
+            Draw d = load_draw(in->draws, cfg->draw_count, layer[1]);
+            if (noop_stack_clip(cfg, in, layer, &d, tx, ty)) continue;
+            uint32_t tag = layer[0] == 1u ? 6u : (layer[0] == 2u ? 8u : 4u);
+            store_particle(cfg, work, end_dst++, tag, 0, 0, 0, 0, 0);
+        }
+        store_particle(cfg, work, px + wrappers, 0u, 0, 0, 0, 0, 0);
+        uint32_t skip = (flags & 8u) ? occluded_prefix(cfg, work, tile, px + wrappers + 1u, wrappers) : 0u;
+        uint32_t kind;
+        if (flags & 4u) kind = 0u;
+        else if (flags & 2u) kind = (flags & 1u) ? 4u : 3u;
+        else kind = (flags & 1u) ? 2u : 1u;
+        kinds[tile] = kind | (skip << 3);
+    }
+}
+
+/* ---------- fine (fine/main.hlsl, interpreter.hlsli, specialized.hlsli, clip.hlsli) ---- */
+
+typedef struct { uint32_t brush_offset, sdf_offset, shadow_offset; int32_t pb[4]; Aff inv; } FDraw;
+static FDraw load_fdraw(const uint32_t *draws, uint32_t count, uint32_t index) {
+    FDraw f;
+    memset(&f, 0, sizeof(f));
+    if (index >= count) return f;
+    const uint32_t *r = draws + (size_t)index * 31u;
+    f.brush_offset = r[6]; f.sdf_offset = r[2]; f.shadow_offset = r[4];
+    for (int i = 0; i < 4; ++i) f.pb[i] = (int32_t)r[10 + i];
+    f.inv.a = bits_f(r[25]); f.inv.b = bits_f(r[26]); f.inv.c = bits_f(r[27]);
+    f.inv.d = bits_f(r[28]); f.inv.e = bits_f(r[29]); f.inv.f = bits_f(r[30]);
+    return f;
+}
+static void aff_point(const Aff *t, float x, float y, float *ox, float *oy) {
+    *ox = fmaf(t->a, x, fmaf(t->c, y, t->e));
+    *oy = fmaf(t->b, x, fmaf(t->d, y, t->f));
+}
+static uint32_t draw_brush(const or_fine_config *cfg, const uint32_t *paint, const FDraw *d, float x, float y) {
+    if (blob_word(paint, cfg->paint_words, cfg->paint_brush_base, d->brush_offset) == 1u)
+        return blob_word(paint, cfg->paint_words, cfg->paint_brush_base, d->brush_offset + 4u);
+    float lx, ly;
+    aff_point(&d->inv, x, y, &lx, &ly);
+    return sample_brush(paint, cfg->paint_words, cfg->paint_brush_base, d->brush_offset, lx, ly);
+}
+static float draw_sdf_coverage(const or_fine_config *cfg, const uint32_t *paint, const FDraw *d, float x, float y) {
+    float lx, ly;
+    aff_point(&d->inv, x, y, &lx, &ly);
+    if (d->sdf_offset != NONE) return sdf_coverage_from_blob(paint, cfg->paint_words, d->sdf_offset, lx, ly, &d->inv);
+    if (d->shadow_offset != NONE)
+        return sdf_coverage_from_blob(paint, cfg->paint_words, cfg->paint_sdf_shadow_base + d->shadow_offset, lx, ly,
+                                      &d->inv);
+    return 0.0f;
+}
+
+/* One pixel of one batch, from the loaded target value. */
+static Px fine_pixel(const or_fine_config *cfg, const uint32_t *draws, const uint32_t *paint, const uint32_t *segs,
+                     const uint32_t *work, uint32_t kind_word, uint32_t tile, uint32_t lane, Px pixel,
+                     uint8_t *spill /* clip_spill_depth bytes */) {
+    uint32_t kind = kind_word & 7u;
+    if (kind == 1u) return pixel;
+    const uint32_t *rec = work + (size_t)tile * 6u;
+    uint32_t start = rec[1] + (kind_word >> 3), end = rec[2];
+    const uint32_t *ptcl = work + cfg->ptcl_base_words;
+    uint32_t lx = lane % TILE, ly = lane / TILE;
+    uint32_t gx = (tile % cfg->tiles_width) * TILE + lx, gy = (tile / cfg->tiles_width) * TILE + ly;
+    float sx = (float)gx + 0.5f, sy = (float)gy + 0.5f;
+    if (kind == 2u) {
+        for (uint32_t i = start; i < end; ++i) {
+            const uint32_t *r = ptcl + (size_t)i * 6u;
+            if (r[0] == 0u) break;
+            pixel = premul_opaque(r[5]) ? rgba8_to_unorm(r[5]) : src_over_premul_unorm(pixel, rgba8_to_unorm(r[5]));
+        }
+        return pixel;
+    }
+    if (kind == 3u || kind == 4u) {
+        for (uint32_t i = start; i < end; ++i) {
+            const uint32_t *r = ptcl + (size_t)i * 6u;
+            if (r[0] == 0u) break;
+            if (r[0] == 2u) {
+                pixel = premul_opaque(r[5]) ? rgba8_to_unorm(r[5]) : src_over_premul_unorm(pixel, rgba8_to_unorm(r[5]));
+            } else if (r[0] == 13u) {
+                /* image brushes are not produced on this path */
+            } else {
+                FDraw d = load_fdraw(draws, cfg->draw_count, r[5]);
+                float coverage = 0.0f;
+                if ((int32_t)gx >= d.pb[0] && (int32_t)gy >= d.pb[1] && (int32_t)gx < d.pb[2] && (int32_t)gy < d.pb[3])
+                    coverage = draw_sdf_coverage(cfg, paint, &d, sx, sy);
+                uint32_t alpha = coverage_to_u8(coverage);
+                if (alpha != 0u) {
+                    uint32_t color = blob_word(paint, cfg->paint_words, cfg->paint_brush_base, d.brush_offset + 4u);
+                    pixel = alpha == 255u && premul_opaque(color) ? rgba8_to_unorm(color)
+                                                                  : src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, alpha));
+                }
+            }
+        }
+        return pixel;
+    }
+    /* tile_pixel (fine/interpreter.hlsli:21-219) */
+    uint32_t clip_mask = 255u, depth = 0u, stack[4] = {255u, 255u, 255u, 255u};
+    for (uint32_t i = start; i < end; ++i) {
+        const uint32_t *r = ptcl + (size_t)i * 6u;
+        uint32_t tag = r[0];
+        if (tag == 0u) break;
+        if (tag == 2u) {
+            if (clip_mask != 0u) {
+                if (clip_mask == 255u && premul_opaque(r[5])) pixel = rgba8_to_unorm(r[5]);
+                else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(r[5], clip_mask));
+            }
+        } else if (tag == 9u) {
+            if (clip_mask != 0u) {
+                FDraw d = load_fdraw(draws, cfg->draw_count, r[5]);
+                uint32_t alpha = combine_alpha(coverage_to_u8(draw_sdf_coverage(cfg, paint, &d, sx, sy)), clip_mask);
+                if (alpha != 0u) {
+                    uint32_t color = draw_brush(cfg, paint, &d, sx, sy);
+                    if (alpha == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
+                    else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, alpha));
+                }
+            }
+        } else if (tag == 4u) {
+            if (depth > 0u) {
+                depth -= 1u;
+                if (depth < 4u) clip_mask = stack[depth];
+                else if (depth - 4u < cfg->clip_spill_depth) clip_mask = spill[depth - 4u];
+            } else {
+                clip_mask = 255u;
+            }
+        } else if (tag == 12u || tag == 1u || tag == 3u) {
+            uint32_t parent = clip_mask;
+            uint32_t alpha;
+            if (tag == 12u) {
+                alpha = 0u;
+                if (parent != 0u) {
+                    FDraw d = load_fdraw(draws, cfg->draw_count, r[5]);
+                    alpha = coverage_to_u8(draw_sdf_coverage(cfg, paint, &d, sx, sy));
+                }
+            } else {
+                alpha = coverage_to_alpha(fill_value_at(segs, (int32_t)r[1], r[3], r[4], lx, ly), r[2]);
+            }
+            if (tag == 1u) {
+                uint32_t masked = combine_alpha(alpha, clip_mask);
+                if (masked != 0u) {
+                    FDraw d = load_fdraw(draws, cfg->draw_count, r[5]);
+                    uint32_t color = draw_brush(cfg, paint, &d, sx, sy);
+                    if (masked == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
+                    else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, masked));
+                }
+            } else {
+                /* push_clip (fine/clip.hlsli:7-38) */
+                if (depth < 4u) stack[depth++] = parent;
+                else if (depth - 4u < cfg->clip_spill_depth) { spill[depth - 4u] = (uint8_t)parent; depth++; }
+                clip_mask = combine_alpha(parent, alpha);
+            }
+        }
+    }
+    return pixel;
+}
+
+/* fine_tile_main (fine/main.hlsl:23-40) with load_target: reads and writes the RGBA8 target. */
+void or_fine(const or_fine_config *cfg, const uint32_t *draws, const uint32_t *paint, const uint32_t *segments,
+             const uint32_t *work, const uint32_t *kinds, uint32_t *target) {
+    uint8_t *spill = (uint8_t *)malloc(cfg->clip_spill_depth + 1u);
+    for (uint32_t tile = 0; tile < cfg->tile_count; ++tile) {
+        for (uint32_t lane = 0; lane < 256u; ++lane) {
+            uint32_t gx = (tile % cfg->tiles_width) * TILE + lane % TILE;
+            uint32_t gy = (tile / cfg->tiles_width) * TILE + lane / TILE;
+            if (gx >= cfg->width || gy >= cfg->height) continue;
+            uint32_t *dst = target + (size_t)gy * cfg->width + gx;
+            Px pixel = rgba8_to_unorm(*dst);
+            pixel = fine_pixel(cfg, draws, paint, segments, work, kinds[tile], tile, lane, pixel, spill);
+            *dst = unorm_to_rgba8(pixel);
+        }
+    }
+    free(spill);
+}
+
+/* ---------- whole frame ------------------------------------------------------------ */
+
+int or_render_frame(const or_scene *s, uint32_t *image, or_frame_outputs *out) {
+    int status = 0;
+    const uint32_t T = s->tiles_width * s->tiles_height;
+    const uint32_t B = s->backdrop_len;
+    or_scan_config sc;
+    memset(&sc, 0, sizeof(sc));
+    sc.backdrop_len = B;
+    sc.path_count = s->path_count;
+    sc.scan_chunk_count = s->scan_chunk_count;
+    sc.line_count = s->line_count;
+    sc.segment_capacity = s->segment_capacity;
+
+    int32_t *backdrops = (int32_t *)calloc(B + 1u, 4);
+    uint32_t *counts = (uint32_t *)calloc(B + 1u, 4);
+    uint32_t *ranges = (uint32_t *)calloc(2u * (size_t)B + 2u, 4);
+    uint32_t *cursors = (uint32_t *)calloc(B + 1u, 4);
+    uint32_t *bumps = (uint32_t *)calloc(s->path_count + 1u, 4);
+    uint32_t *totals = (uint32_t *)calloc(s->scan_chunk_count + 1u, 4);
+    uint32_t *offsets = (uint32_t *)calloc(s->scan_chunk_count + 1u, 4);
+    uint32_t *segments = (uint32_t *)calloc(5u * (size_t)s->segment_capacity + 5u, 4);
+
+    or_scan_count(&sc, s->lines, s->paths, s->path_count, backdrops, counts);
+    or_scan_prefix_chunks(&sc, s->scan_chunks, counts, ranges, totals);
+    or_scan_chunk_offsets(&sc, s->paths, s->scan_chunk_ranges, bumps, totals, offsets);
+    or_scan_apply_chunk_offsets(&sc, s->scan_chunks, ranges, cursors, offsets);
+    or_scan_emit(&sc, s->lines, s->paths, s->path_count, cursors, segments);
+    or_segment_order(ranges, B, s->segment_capacity, segments);
+    for (uint32_t p = 0; p < s->path_count; ++p)
+        if (bumps[p] > s->segment_capacity) status = 1;
+    if (out && out->raw_backdrops) memcpy(out->raw_backdrops, backdrops, (size_t)B * 4u);
+    or_cumsum(s->cumsum_row_count, s->cumsum_chunk_count, s->cumsum_chunk_offsets, s->cumsum_chunk_lens,
+              s->cumsum_row_starts, s->cumsum_row_ends, backdrops);
+
+    /* root clear (filter_clear_region), then coarse + fine per batch on the RGBA8 target */
+    for (size_t i = 0; i < (size_t)s->width * s->height; ++i) image[i] = s->clear_color;
+    uint32_t *work = NULL;
+    size_t work_words = 0;
+    uint32_t *kinds = (uint32_t *)calloc(T + 1u, 4);
+    uint32_t chunk_count = (T + 255u) / 256u;
+    uint32_t *chunk_records = (uint32_t *)calloc(4u * (size_t)chunk_count + 4u, 4);
+    uint64_t base = 0;
+    or_coarse_inputs in = {s->draws, s->paths, backdrops, ranges, s->layers, s->paint, s->draw_batch_ids,
+                           s->tile_draws, s->tile_draw_indices};
+    for (uint32_t b = 0; b < s->batch_count; ++b) {
+        or_coarse_config cc;
+        memset(&cc, 0, sizeof(cc));
+        cc.tile_count = T;
+        cc.tiles_width = s->tiles_width;
+        cc.tiles_height = s->tiles_height;
+        cc.batch_id = b;
+        cc.layer_stack_start = s->batches[2 * b];
+        cc.layer_stack_end = s->batches[2 * b + 1];
+        cc.paint_brush_base = s->paint_brush_base;
+        cc.draw_count = s->draw_count;
+        cc.path_count = s->path_count;
+        cc.paint_words = s->paint_words;
+        cc.layer_count = s->layer_count;
+        cc.tile_draw_index_count = s->tile_draw_index_count;
+        /* count with a tile-record-only work buffer, then size the PTCL exactly */
+        size_t need = (size_t)T * 6u;
+        if (need > work_words) {
+            free(work);
+            work_words = need;
+            work = (uint32_t *)calloc(work_words, 4);
+        }
+        cc.ptcl_capacity = 0;
+        or_coarse_count_bins(&cc, &in, work, kinds);
+        or_coarse_prefix(T, work, chunk_records);
+        uint32_t total = 0;
+        for (uint32_t c = 0; c < chunk_count; ++c) total += chunk_records[4 * c];
+        need = (size_t)T * 6u + (size_t)total * 6u;
+        if (need > work_words) {
+            uint32_t *grown = (uint32_t *)calloc(need, 4);
+            memcpy(grown, work, (size_t)T * 6u * 4u);
+            free(work);
+            work = grown;
+            work_words = need;
+        }
+        cc.ptcl_capacity = total;
+        or_coarse_emit_bins(&cc, &in, work, kinds);
+        if (out) {
+            for (uint32_t t = 0; t < T; ++t) {
+                size_t idx = (size_t)b * T + t;
+                if (out->tile_ptcl_counts) out->tile_ptcl_counts[idx] = work[(size_t)t * 6u];
+                if (out->tile_ptcl_starts) out->tile_ptcl_starts[idx] = work[(size_t)t * 6u + 1u];
+                if (out->tile_ptcl_ends) out->tile_ptcl_ends[idx] = work[(size_t)t * 6u + 2u];
+                if (out->tile_kinds) out->tile_kinds[idx] = kinds[t];
+            }
+            if (out->batch_bases) out->batch_bases[b] = (uint32_t)base;
+            if (out->ptcl) {
+                if ((base + total) * 6u > out->ptcl_capacity_words) status = 2;
+                else memcpy(out->ptcl + base * 6u, work + (size_t)T * 6u, (size_t)total * 24u);
+            }
+        }
+        base += total;
+        or_fine_config fc;
+        memset(&fc, 0, sizeof(fc));
+        fc.width = s->width;
+        fc.height = s->height;
+        fc.tile_count = T;
+        fc.tiles_width = s->tiles_width;
+        fc.tiles_height = s->tiles_height;
+        fc.clip_spill_depth = s->max_clip_depth > 4u ? s->max_clip_depth - 4u : 0u;
+        fc.paint_words = s->paint_words;
+        fc.paint_sdf_shadow_base = s->paint_sdf_shadow_base;
+        fc.paint_brush_base = s->paint_brush_base;
+        fc.draw_count = s->draw_count;
+        fc.ptcl_base_words = T * 6u;
+        or_fine(&fc, s->draws, s->paint, segments, work, kinds, image);
+    }
+    if (out && out->batch_bases) out->batch_bases[s->batch_count] = (uint32_t)base;
+    if (out) {
+        if (out->backdrops) memcpy(out->backdrops, backdrops, (size_t)B * 4u);
+        if (out->segment_counts) memcpy(out->segment_counts, counts, (size_t)B * 4u);
+        if (out->segment_ranges) memcpy(out->segment_ranges, ranges, (size_t)B * 8u);
+        if (out->segment_cursors) memcpy(out->segment_cursors, cursors, (size_t)B * 4u);
+        if (out->segment_bumps) memcpy(out->segment_bumps, bumps, (size_t)s->path_count * 4u);
+        if (out->segments) memcpy(out->segments, segments, (size_t)s->segment_capacity * 20u);
+    }
+    free(work); free(kinds); free(chunk_records);
+    free(backdrops); free(counts); free(ranges); free(cursors); free(bumps); free(totals); free(offsets); free(segments);
+    return status;
+}
+
+/* ---------- validation entries ---------------------------------------------------- */
+
+void or_fill_coverage_words(uint32_t count, const uint32_t *q, const uint32_t *segments, uint32_t *alpha_out,
+                            float *coverage_out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t *r = q + (size_t)i * 8u;
+        float v = fill_value_at(segments, (int32_t)r[2], r[0], r[1], r[3], r[4]);
+        alpha_out[i] = coverage_to_alpha(v, r[5]);
+        if (coverage_out) coverage_out[i] = v;
+    }
+}
+
+void or_pixel_math_words(uint32_t count, const uint32_t *q, uint32_t *out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t *r = q + (size_t)i * 8u;
+        uint32_t *o = out + (size_t)i * 14u;
+        uint32_t a = r[0], b = r[1], src = r[2], dst = r[3];
+        float c = bits_f(r[4]), t = bits_f(r[5]);
+        uint32_t rule = r[6];
+        Px pixel = rgba8_to_unorm(src);
+        o[0] = mul_div255(a, b);
+        o[1] = combine_alpha(a, b);
+        o[2] = scale_premul_u8(src, b);
+        o[3] = src_over_premul_u8(dst, src);
+        o[4] = src;
+        o[5] = coverage_to_u8(c);
+        o[6] = coverage_to_alpha(c, rule);
+        o[7] = lerp_premul_u8(src, dst, t);
+        o[8] = unorm_to_rgba8(src_over_premul_unorm(rgba8_to_unorm(dst), rgba8_to_unorm(src)));
+        o[9] = unorm_to_rgba8(scale_premul_u8_to_unorm(src, b));
+        o[10] = coverage_to_u8(straight_channel(a, b));
+        o[11] = pack_premul_rgba8(pixel.c[0], pixel.c[1], pixel.c[2], pixel.c[3]);
+        o[12] = coverage_to_u8(rem_euclid_f32(c, 3.0f));
+        o[13] = f_bits(signum_f32(c));
+    }
+}
+
+/* ---------- FP64 area oracles ------------------------------------------------------ */
+
+/* Exact integral over y in [ya, yb] of (X + 1 - clamp(x(y), X, X + 1)), x linear in y. */
+static double band_area(double x0, double y0, double x1, double y1, double X, double ya, double yb) {
+    if (!(yb > ya)) return 0.0;
+    double cuts[4] = {ya, yb, ya, ya};
+    int n = 2;
+    if (y1 != y0) {
+        double k = (x1 - x0) / (y1 - y0);
+        double edges[2] = {X, X + 1.0};
+        for (int e = 0; e < 2; ++e) {
+            if (k != 0.0) {
+                double yc = y0 + (edges[e] - x0) / k;
+                if (yc > ya && yc < yb) cuts[n++] = yc;
+            }
+        }
+    }
+    /* sort the (at most 4) cut points */
+    for (int i = 1; i < n; ++i)
+        for (int j = i; j > 0 && cuts[j] < cuts[j - 1]; --j) {
+            double t = cuts[j]; cuts[j] = cuts[j - 1]; cuts[j - 1] = t;
+        }
+    double total = 0.0;
+    for (int i = 0; i + 1 < n; ++i) {
+        double u = cuts[i], v = cuts[i + 1];
+        if (!(v > u)) continue;
+        double xu, xv;
+        if (y1 != y0) {
+            xu = x0 + (x1 - x0) * (u - y0) / (y1 - y0);
+            xv = x0 + (x1 - x0) * (v - y0) / (y1 - y0);
+        } else {
+            xu = xv = x0;
+        }
+        /* within a piece the clamp is affine, so the trapezoid rule is exact */
+        double fu = X + 1.0 - fmin(fmax(xu, X), X + 1.0);
+        double fv = X + 1.0 - fmin(fmax(xv, X), X + 1.0);
+        total += 0.5 * (fu + fv) * (v - u);
+    }
+    return total;
+}
+
+void or_fill_coverage_f64(uint32_t count, const uint32_t *q, const uint32_t *segments, double *out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t *r = q + (size_t)i * 8u;
+        double X = (double)r[3], Y = (double)r[4];
+        double total = (double)(int32_t)r[2];
+        for (uint32_t k = r[0]; k < r[1]; ++k) {
+            const uint32_t *sgm = segments + (size_t)k * 5u;
+            double p0x = bits_f(sgm[0]), p0y = bits_f(sgm[1]), p1x = bits_f(sgm[2]), p1y = bits_f(sgm[3]);
+            double y_edge = bits_f(sgm[4]);
+            double edge = (p1x - p0x < 0.0 ? -1.0 : 1.0) * fmin(fmax(Y - y_edge + 1.0, 0.0), 1.0);
+            total += edge;
+            double lo = fmax(fmin(p0y, p1y), Y), hi = fmin(fmax(p0y, p1y), Y + 1.0);
+            double sign = p0y > p1y ? 1.0 : -1.0;
+            total += sign * band_area(p0x, p0y, p1x, p1y, X, lo, hi);
+        }
+        out[i] = total;
+    }
+}
+
+double or_polygon_pixel_area_f64(const uint32_t *lines, uint32_t n, double px, double py) {
+    double total = 0.0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t *l = lines + (size_t)k * 6u;
+        double x0 = bits_f(l[2]), y0 = bits_f(l[3]), x1 = bits_f(l[4]), y1 = bits_f(l[5]);
+        if (y0 == y1) continue;
+        double lo = fmax(fmin(y0, y1), py), hi = fmin(fmax(y0, y1), py + 1.0);
+        double sign = y0 > y1 ? 1.0 : -1.0;
+        total += sign * band_area(x0, y0, x1, y1, px, lo, hi);
+    }
+    return total;
+}

@@ -1,0 +1,74 @@
+"""Shared parity checks: the gfx950 frame against the CPU oracle on one scene."""
+import numpy as np
+
+import oracle
+import tileink_amd as tk
+from tileink_amd import _abi
+
+
+def gpu_frame(ctx, canvas, clear=0):
+    scene = canvas.encode(clear)
+    frame = tk.Frame(ctx, scene)
+    frame.set_capture(True)
+    frame.render()
+    return frame, scene
+
+
+def read_frame(frame, scene):
+    stats = frame.stats()
+    B, P, S = scene.backdrop_len, scene.path_count, scene.segment_capacity
+    QT = scene.batch_count * scene.tiles_width * scene.tiles_height
+    Q = scene.batch_count
+    out = {
+        "image": frame.image(),
+        "raw_backdrops": frame.buffer(_abi.BUF_RAW_BACKDROPS, np.int32, B),
+        "backdrops": frame.buffer(_abi.BUF_BACKDROPS, np.int32, B),
+        "segment_counts": frame.buffer(_abi.BUF_SEGMENT_COUNTS, np.uint32, B),
+        "segment_ranges": frame.buffer(_abi.BUF_SEGMENT_RANGES, np.uint32, 2 * B).reshape(-1, 2),
+        "segment_cursors": frame.buffer(_abi.BUF_SEGMENT_CURSORS, np.uint32, B),
+        "segment_bumps": frame.buffer(_abi.BUF_SEGMENT_BUMPS, np.uint32, P),
+        "segments": frame.buffer(_abi.BUF_SEGMENTS, np.uint32, 5 * S).reshape(-1, 5),
+        "tile_ptcl_counts": frame.buffer(_abi.BUF_TILE_PTCL_COUNTS, np.uint32, QT),
+        "tile_ptcl_ranges": frame.buffer(_abi.BUF_TILE_PTCL_RANGES, np.uint32, 2 * QT).reshape(-1, 2),
+        "tile_kinds": frame.buffer(_abi.BUF_TILE_KINDS, np.uint32, QT),
+        "batch_bases": frame.buffer(_abi.BUF_BATCH_PTCL_BASES, np.uint32, Q + 1),
+        "stats": stats,
+    }
+    out["ptcl"] = frame.buffer(_abi.BUF_PTCL, np.uint32, 6 * int(stats["ptcl_records"])).reshape(-1, 6)
+    return out
+
+
+def compare_frame(ctx, canvas, clear=0):
+    """Returns (gpu, ref, mismatches) where mismatches lists the differing stages."""
+    frame, scene = gpu_frame(ctx, canvas, clear)
+    gpu = read_frame(frame, scene)
+    ref_image, ref = oracle.render_frame(canvas.arrays(clear))
+    ref["image"] = ref_image
+    bad = []
+    for key in ("raw_backdrops", "backdrops", "segment_counts", "segment_bumps", "segment_cursors",
+                "tile_ptcl_counts", "tile_kinds", "image"):
+        if not np.array_equal(gpu[key], ref[key]):
+            bad.append(key)
+    if not np.array_equal(gpu["segment_ranges"], ref["segment_ranges"]):
+        bad.append("segment_ranges")
+    used = int(gpu["segment_bumps"].sum(dtype=np.uint64))
+    if not np.array_equal(gpu["segments"][:used], ref["segments"][:used]):
+        bad.append("segments")
+    if not np.array_equal(gpu["batch_bases"], ref["batch_bases"]):
+        bad.append("batch_bases")
+    T = scene.tiles_width * scene.tiles_height
+    if scene.batch_count:
+        bases = np.repeat(gpu["batch_bases"][:-1], T)
+        starts = gpu["tile_ptcl_ranges"][:, 0] - bases
+        ends = gpu["tile_ptcl_ranges"][:, 1] - bases
+        if not (np.array_equal(starts, ref["tile_ptcl_starts"]) and np.array_equal(ends, ref["tile_ptcl_ends"])):
+            bad.append("tile_ptcl_ranges")
+    if not np.array_equal(gpu["ptcl"], ref["ptcl"]):
+        bad.append("ptcl")
+    if gpu["stats"]["overflow"]:
+        bad.append("overflow")
+    return gpu, ref, bad
+
+
+def channel_diff(a, b):
+    return int(np.abs(a.view(np.uint8).astype(np.int32) - b.view(np.uint8).astype(np.int32)).max(initial=0))

@@ -1,0 +1,410 @@
+"""tileink_amd — MI355X (gfx950) backend for Tileink's compute-raster path.
+
+Python mirror of the reference's public API for this path:
+
+* ``Canvas`` records paths, strokes, rounded rectangles and clip layers
+  (src/canvas/primitives/paths.rs, rectangles.rs, src/canvas/layers.rs);
+* ``NativeRenderer(NativeBackend.Hip, width, height).render_to_image(canvas)``
+  returns a submission whose ``readback()`` gives the premultiplied RGBA8 image
+  (src/native/renderer.rs:177-188, src/native/submission.rs:120-146).
+
+All work goes through libtileink_hip.so (C ABI in include/tileink_hip.h).
+"""
+import ctypes as C
+import enum
+
+import numpy as np
+
+from . import _abi
+from ._abi import (BRUSH_LINEAR, BRUSH_SOLID, FILL_EVEN_ODD, FILL_NONZERO, INVALID, LINE_DTYPE, PTCL_DTYPE,
+                   SEGMENT_DTYPE, TileinkError, check, lib, ptr)
+
+__all__ = [
+    "BezPath", "Canvas", "Color", "LinearGradient", "NativeBackend", "NativeContext", "NativeRenderer", "Frame",
+    "SceneArrays", "TileinkError", "FILL_NONZERO", "FILL_EVEN_ODD",
+]
+
+
+class FillRule(enum.IntEnum):
+    NonZero = FILL_NONZERO
+    EvenOdd = FILL_EVEN_ODD
+
+
+class NativeBackend(enum.Enum):
+    """Backend selector (src/native.rs:6-11); this package provides ``Hip``."""
+    Hip = "hip"
+
+
+# ---------------------------------------------------------------------------
+# Geometry and paint
+# ---------------------------------------------------------------------------
+
+class BezPath:
+    """Move/line/quad/cubic/close path, kurbo::BezPath element order."""
+
+    def __init__(self):
+        self.verbs = []
+        self.points = []
+
+    def move_to(self, x, y):
+        self.verbs.append(_abi.PATH_MOVE)
+        self.points.append((x, y))
+        return self
+
+    def line_to(self, x, y):
+        self.verbs.append(_abi.PATH_LINE)
+        self.points.append((x, y))
+        return self
+
+    def quad_to(self, x1, y1, x2, y2):
+        self.verbs.append(_abi.PATH_QUAD)
+        self.points += [(x1, y1), (x2, y2)]
+        return self
+
+    def curve_to(self, x1, y1, x2, y2, x3, y3):
+        self.verbs.append(_abi.PATH_CUBIC)
+        self.points += [(x1, y1), (x2, y2), (x3, y3)]
+        return self
+
+    def close_path(self):
+        self.verbs.append(_abi.PATH_CLOSE)
+        return self
+
+    def arrays(self):
+        verbs = np.asarray(self.verbs, dtype=np.uint8)
+        points = np.asarray(self.points, dtype=np.float64).reshape(-1, 2)
+        return np.ascontiguousarray(verbs), np.ascontiguousarray(points)
+
+
+class Color:
+    """Straight-alpha sRGB colour in [0, 1] (peniko::Color)."""
+
+    def __init__(self, r, g, b, a=1.0):
+        self.rgba = (float(r), float(g), float(b), float(a))
+
+    def _brush(self):
+        b = _abi.Brush()
+        b.kind = BRUSH_SOLID
+        b.rgba[:] = self.rgba
+        return b, None
+
+
+class LinearGradient:
+    """Linear gradient brush: stops are (offset, Color); extend 0 pad, 1 repeat, 2 reflect."""
+
+    def __init__(self, start, end, stops, extend=0):
+        self.start, self.end, self.stops, self.extend = start, end, list(stops), int(extend)
+
+    def _brush(self):
+        b = _abi.Brush()
+        b.kind = BRUSH_LINEAR
+        b.start[:] = self.start
+        b.end[:] = self.end
+        b.extend = self.extend
+        arr = (_abi.GradientStop * max(len(self.stops), 1))()
+        for i, (offset, color) in enumerate(self.stops):
+            arr[i].offset = float(offset)
+            arr[i].rgba[:] = color.rgba
+        b.stops = C.cast(arr, C.POINTER(_abi.GradientStop))
+        b.stop_count = len(self.stops)
+        return b, arr
+
+
+def _brush_arg(brush):
+    if isinstance(brush, tuple):
+        brush = Color(*brush)
+    b, keep = brush._brush()
+    return b, keep
+
+
+def _affine_arg(transform):
+    if transform is None:
+        return None, None
+    arr = np.ascontiguousarray(np.asarray(transform, dtype=np.float64).reshape(6))
+    return ptr(arr), arr
+
+
+# ---------------------------------------------------------------------------
+# Canvas
+# ---------------------------------------------------------------------------
+
+class SceneArrays:
+    """Host copies of every array of an encoded scene (what the GPU stages read)."""
+
+    FIELDS = ("lines", "paths", "draws", "paint", "draw_batch_ids", "layers", "batches", "tile_draws",
+              "tile_draw_indices", "scan_chunks", "scan_chunk_ranges", "cumsum_chunk_offsets", "cumsum_chunk_lens",
+              "cumsum_row_starts", "cumsum_row_ends")
+
+    def __init__(self, scene):
+        s = scene
+        T = s.tiles_width * s.tiles_height
+        self.scalars = {name: getattr(s, name) for name, kind in _abi.Scene._fields_ if kind is not C.c_void_p}
+        self.lines = _abi.view(s.lines, LINE_DTYPE, s.line_count)
+        self.paths = _abi.view(s.paths, np.uint32, s.path_count * 19).reshape(-1, 19)
+        self.draws = _abi.view(s.draws, np.uint32, s.draw_count * 31).reshape(-1, 31)
+        self.paint = _abi.view(s.paint, np.uint32, s.paint_words)
+        self.draw_batch_ids = _abi.view(s.draw_batch_ids, np.uint32, s.draw_count)
+        self.layers = _abi.view(s.layers, np.uint32, s.layer_count * 3).reshape(-1, 3)
+        self.batches = _abi.view(s.batches, np.uint32, s.batch_count * 2).reshape(-1, 2)
+        self.tile_draws = _abi.view(s.tile_draws, np.uint32, T * 2).reshape(-1, 2)
+        self.tile_draw_indices = _abi.view(s.tile_draw_indices, np.uint32, s.tile_draw_index_count)
+        self.scan_chunks = _abi.view(s.scan_chunks, np.uint32, s.scan_chunk_count * 4).reshape(-1, 4)
+        self.scan_chunk_ranges = _abi.view(s.scan_chunk_ranges, np.uint32, s.path_count * 2).reshape(-1, 2)
+        self.cumsum_chunk_offsets = _abi.view(s.cumsum_chunk_offsets, np.uint32, s.cumsum_chunk_count)
+        self.cumsum_chunk_lens = _abi.view(s.cumsum_chunk_lens, np.uint32, s.cumsum_chunk_count)
+        self.cumsum_row_starts = _abi.view(s.cumsum_row_starts, np.uint32, s.cumsum_row_count)
+        self.cumsum_row_ends = _abi.view(s.cumsum_row_ends, np.uint32, s.cumsum_row_count)
+
+    def __getattr__(self, name):
+        scalars = self.__dict__.get("scalars", {})
+        if name in scalars:
+            return scalars[name]
+        raise AttributeError(name)
+
+    def to_scene(self):
+        """A tk_scene pointing at these arrays (kept alive by self)."""
+        s = _abi.Scene()
+        for name, value in self.scalars.items():
+            setattr(s, name, value)
+        for name in self.FIELDS:
+            arr = np.ascontiguousarray(getattr(self, name))
+            setattr(self, name, arr)
+            setattr(s, name, ptr(arr) if arr.size else None)
+        return s
+
+
+class Canvas:
+    """Scene recorder (src/canvas.rs).  Coordinates are logical; scale maps to physical pixels."""
+
+    def __init__(self, width, height, scale=1.0):
+        h = C.c_void_p()
+        check(lib().tk_canvas_create(int(width), int(height), float(scale), C.byref(h)))
+        self._h = h
+        self.width, self.height, self.scale = width, height, scale
+        self._scene = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().tk_canvas_destroy(h)
+            self._h = None
+
+    @staticmethod
+    def _path_args(path):
+        verbs, points = path.arrays()
+        return verbs, points
+
+    def push_path(self, path, brush, transform=None, rule=FillRule.NonZero, tolerance=0.25):
+        verbs, points = self._path_args(path)
+        b, keep = _brush_arg(brush)
+        tp, tarr = _affine_arg(transform)
+        out = C.c_uint32()
+        check(lib().tk_canvas_push_path(self._h, ptr(verbs), len(verbs), ptr(points), len(points), C.byref(b), tp,
+                                        int(rule), float(tolerance), C.byref(out)))
+        self._scene = None
+        return out.value
+
+    def push_stroke(self, path, width, brush, transform=None, rule=FillRule.NonZero, tolerance=0.25):
+        verbs, points = self._path_args(path)
+        b, keep = _brush_arg(brush)
+        tp, tarr = _affine_arg(transform)
+        out = C.c_uint32()
+        check(lib().tk_canvas_push_stroke(self._h, ptr(verbs), len(verbs), ptr(points), len(points), float(width),
+                                          C.byref(b), tp, int(rule), float(tolerance), C.byref(out)))
+        self._scene = None
+        return out.value
+
+    def push_rect(self, x0, y0, x1, y1, radii, brush):
+        r = (C.c_float * 4)(*((radii,) * 4 if np.isscalar(radii) else radii))
+        b, keep = _brush_arg(brush)
+        out = C.c_uint32()
+        check(lib().tk_canvas_push_rect(self._h, float(x0), float(y0), float(x1), float(y1), r, C.byref(b),
+                                        C.byref(out)))
+        self._scene = None
+        return out.value
+
+    def push_clip_sdf_rect_layer(self, x0, y0, x1, y1, radii=0.0):
+        r = (C.c_float * 4)(*((radii,) * 4 if np.isscalar(radii) else radii))
+        check(lib().tk_canvas_push_clip_sdf_rect_layer(self._h, float(x0), float(y0), float(x1), float(y1), r))
+        self._scene = None
+
+    def push_clip_layer(self, path, transform=None, rule=FillRule.NonZero, tolerance=0.25):
+        verbs, points = self._path_args(path)
+        tp, tarr = _affine_arg(transform)
+        check(lib().tk_canvas_push_clip_layer(self._h, ptr(verbs), len(verbs), ptr(points), len(points), tp, int(rule),
+                                              float(tolerance)))
+        self._scene = None
+
+    def pop_layer(self):
+        check(lib().tk_canvas_pop_layer(self._h))
+        self._scene = None
+
+    def line_count(self):
+        return lib().tk_canvas_line_count(self._h)
+
+    def encode(self, clear_color=0):
+        """Compiles and plans the scene; returns the tk_scene view (owned by the canvas)."""
+        s = _abi.Scene()
+        check(lib().tk_canvas_encode(self._h, int(clear_color), C.byref(s)))
+        self._scene = s
+        return s
+
+    def arrays(self, clear_color=0):
+        return SceneArrays(self.encode(clear_color))
+
+
+# ---------------------------------------------------------------------------
+# Device
+# ---------------------------------------------------------------------------
+
+class NativeContext:
+    """One gfx950 device, chosen explicitly (NativeContextOptions::physical_adapter)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().tk_context_create(int(device), C.byref(h)))
+        self._h = h
+        self.device = device
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().tk_context_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        check(lib().tk_context_synchronize(self._h))
+
+    def to_device(self, array):
+        """Copies a numpy array into a new device buffer; returns its DeviceBuffer."""
+        array = np.ascontiguousarray(array)
+        buf = DeviceBuffer(self, max(array.nbytes, 4))
+        check(lib().tk_copy_to_device(self._h, buf.ptr, ptr(array), array.nbytes))
+        return buf
+
+    def limits(self):
+        out = _abi.Limits()
+        check(lib().tk_context_limits(self._h, C.byref(out)))
+        return {f: getattr(out, f) for f, _ in out._fields_}
+
+
+class DeviceBuffer:
+    """Device allocation owned by a NativeContext."""
+
+    def __init__(self, context, nbytes):
+        self.context = context
+        self.nbytes = int(nbytes)
+        h = C.c_void_p()
+        check(lib().tk_device_alloc(context.handle, self.nbytes, C.byref(h)))
+        self._ptr = h
+
+    @property
+    def ptr(self):
+        return self._ptr.value
+
+    def read(self, dtype, count=None):
+        dtype = np.dtype(dtype)
+        count = self.nbytes // dtype.itemsize if count is None else count
+        out = np.empty(count, dtype)
+        check(lib().tk_copy_to_host(self.context.handle, ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def write(self, array):
+        array = np.ascontiguousarray(array)
+        if array.nbytes > self.nbytes:
+            raise TileinkError(_abi.TK_ERR_INVALID, "write larger than the buffer")
+        check(lib().tk_copy_to_device(self.context.handle, self.ptr, ptr(array), array.nbytes))
+
+    def __del__(self):
+        p = getattr(self, "_ptr", None)
+        if p is not None and p.value:
+            lib().tk_device_free(self.context.handle, p)
+            self._ptr = None
+
+
+class Frame:
+    """Device-resident scene: inputs uploaded once, rendered any number of times."""
+
+    def __init__(self, context, scene):
+        self.context = context
+        self._scene = scene
+        h = C.c_void_p()
+        check(lib().tk_frame_create(context.handle, C.byref(scene), C.byref(h)))
+        self._h = h
+        self.width, self.height = scene.width, scene.height
+        self.batch_count = scene.batch_count
+        self.tiles = scene.tiles_width * scene.tiles_height
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().tk_frame_destroy(h)
+            self._h = None
+
+    def render(self):
+        check(lib().tk_frame_render(self._h))
+
+    def set_capture(self, enable=True):
+        check(lib().tk_frame_set_capture(self._h, int(bool(enable))))
+
+    def enable_timing(self, enable=True):
+        check(lib().tk_frame_enable_timing(self._h, int(bool(enable))))
+
+    def stage_times(self):
+        out = _abi.StageTimes()
+        check(lib().tk_frame_stage_times(self._h, C.byref(out)))
+        return {f: getattr(out, f) for f, _ in out._fields_}
+
+    def image(self):
+        out = np.empty((self.height, self.width), dtype=np.uint32)
+        check(lib().tk_frame_read_image(self._h, ptr(out), out.size))
+        return out
+
+    def stats(self):
+        out = _abi.FrameStats()
+        check(lib().tk_frame_stats_read(self._h, C.byref(out)))
+        return {f: getattr(out, f) for f, _ in out._fields_}
+
+    def buffer(self, which, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        check(lib().tk_frame_read_buffer(self._h, int(which), ptr(out), out.nbytes))
+        return out
+
+
+class ImageSubmission:
+    """Result of render_to_image (src/native/submission.rs:120-146)."""
+
+    def __init__(self, rgba):
+        self._rgba = rgba
+
+    def readback(self):
+        return self._rgba
+
+
+class NativeRenderer:
+    """NativeRenderer::new(backend, width, height) + render_to_image (src/native/renderer.rs:51-188)."""
+
+    def __init__(self, backend, width, height, device=0, context=None):
+        if backend is not NativeBackend.Hip:
+            raise TileinkError(_abi.TK_ERR_UNAVAILABLE, f"backend {backend} is not provided by this package")
+        self.width, self.height = width, height
+        self.context = context or NativeContext(device)
+
+    def render_to_image(self, canvas, clear_color=0):
+        scene = canvas.encode(clear_color)
+        if (scene.width, scene.height) != (self.width, self.height):
+            raise TileinkError(_abi.TK_ERR_INVALID,
+                               f"canvas is {scene.width}x{scene.height} physical pixels, renderer {self.width}x{self.height}")
+        out = np.empty((scene.height, scene.width), dtype=np.uint32)
+        check(lib().tk_render_to_image(self.context.handle, C.byref(scene), ptr(out), out.size))
+        return ImageSubmission(out)
+
+
+def line_scanned_tile_count(p0, p1, bbox):
+    """CPU tile count of a line within a tile bbox (src/shared/scan_line.rs:17-31)."""
+    line = _abi.Line(0, 0.0, (C.c_float * 2)(*p0), (C.c_float * 2)(*p1))
+    return lib().tk_line_scanned_tile_count(C.byref(line), *[int(v) for v in bbox])

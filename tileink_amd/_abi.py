@@ -1,0 +1,244 @@
+"""ctypes view of include/tileink_hip.h and the loader for libtileink_hip.so.
+
+The shared library is the product: it carries the gfx950 code object and the
+host recorder.  Nothing here falls back to another implementation; a missing or
+unloadable library raises immediately.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtileink_hip.so")
+
+TK_OK, TK_ERR_INVALID, TK_ERR_HIP, TK_ERR_OOM, TK_ERR_UNAVAILABLE, TK_ERR_OVERFLOW = range(6)
+STATUS_NAMES = {0: "ok", 1: "invalid", 2: "hip", 3: "oom", 4: "unavailable", 5: "overflow"}
+INVALID = 0xFFFFFFFF
+
+PATH_MOVE, PATH_LINE, PATH_QUAD, PATH_CUBIC, PATH_CLOSE = range(5)
+FILL_NONZERO, FILL_EVEN_ODD = 0, 1
+BRUSH_SOLID, BRUSH_LINEAR = 1, 2
+
+# Frame buffer identifiers (tk_frame_buffer).
+BUF_BACKDROPS, BUF_SEGMENT_COUNTS, BUF_SEGMENT_RANGES, BUF_SEGMENT_CURSORS, BUF_SEGMENT_BUMPS, BUF_SEGMENTS = range(6)
+BUF_TILE_PTCL_COUNTS, BUF_TILE_PTCL_RANGES, BUF_TILE_KINDS, BUF_BATCH_PTCL_BASES, BUF_PTCL, BUF_RAW_BACKDROPS = range(6, 12)
+
+
+class TileinkError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"tileink {STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+# --- record structs (byte-exact with the header) ------------------------------
+
+class Affine(C.Structure):
+    _fields_ = [(n, C.c_float) for n in "abcdef"]
+
+
+class Line(C.Structure):
+    _fields_ = [("path_id", C.c_uint32), ("pad", C.c_float), ("p0", C.c_float * 2), ("p1", C.c_float * 2)]
+
+
+class PathRecord(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("path_id", "line_count", "line_start", "flags", "data_offset", "data_len",
+                                          "tile_x0", "tile_y0", "tile_x1", "tile_y1", "segment_start",
+                                          "segment_capacity", "segment_count")] + [("transform", Affine)]
+
+
+class PixelBounds(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("x0", "y0", "x1", "y1")]
+
+
+class DrawRecord(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("path_id", "glyph_run_id", "sdf_offset", "sdf_len", "sdf_shadow_offset",
+                                          "sdf_shadow_len", "brush_offset", "brush_len", "tag", "fill_rule")] + [
+        ("pixel_bounds", PixelBounds), ("local_pixel_bounds", PixelBounds), ("solid_rect", C.c_uint32),
+        ("transform", Affine), ("inverse_transform", Affine)]
+
+
+class ScanConfig(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("clear_len", "backdrop_len", "path_count", "scan_chunk_count", "line_count",
+                                          "segment_capacity", "incremental", "line_base", "path_base", "chunk_base",
+                                          "backdrop_base")]
+
+
+class CumsumConfig(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("row_count", "chunk_count", "pad0", "pad1")]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32), ("tiles_width", C.c_uint32), ("tiles_height", C.c_uint32),
+        ("clear_color", C.c_uint32),
+        ("lines", C.c_void_p), ("line_count", C.c_uint32),
+        ("paths", C.c_void_p), ("path_count", C.c_uint32),
+        ("draws", C.c_void_p), ("draw_count", C.c_uint32),
+        ("paint", C.c_void_p), ("paint_words", C.c_uint32),
+        ("paint_sdf_shadow_base", C.c_uint32), ("paint_brush_base", C.c_uint32),
+        ("draw_batch_ids", C.c_void_p),
+        ("layers", C.c_void_p), ("layer_count", C.c_uint32),
+        ("batches", C.c_void_p), ("batch_count", C.c_uint32),
+        ("tile_draws", C.c_void_p),
+        ("tile_draw_indices", C.c_void_p), ("tile_draw_index_count", C.c_uint32),
+        ("scan_chunks", C.c_void_p), ("scan_chunk_count", C.c_uint32),
+        ("scan_chunk_ranges", C.c_void_p),
+        ("cumsum_chunk_offsets", C.c_void_p), ("cumsum_chunk_lens", C.c_void_p), ("cumsum_chunk_count", C.c_uint32),
+        ("cumsum_row_starts", C.c_void_p), ("cumsum_row_ends", C.c_void_p), ("cumsum_row_count", C.c_uint32),
+        ("backdrop_len", C.c_uint32), ("segment_capacity", C.c_uint32), ("max_clip_depth", C.c_uint32),
+        ("ptcl_capacity", C.c_uint64),
+    ]
+
+
+class GradientStop(C.Structure):
+    _fields_ = [("offset", C.c_float), ("rgba", C.c_float * 4)]
+
+
+class Brush(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("rgba", C.c_float * 4), ("start", C.c_float * 2), ("end", C.c_float * 2),
+                ("extend", C.c_uint32), ("stops", C.POINTER(GradientStop)), ("stop_count", C.c_uint32)]
+
+
+class Limits(C.Structure):
+    _fields_ = [("image_dimension", C.c_uint32), ("max_tiles", C.c_uint32), ("device_memory_bytes", C.c_uint64),
+                ("compute_units", C.c_uint32)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("paths", "lines", "draws", "batches", "tiles", "backdrop_len",
+                                          "segment_capacity", "tile_draw_refs")] + [
+        ("ptcl_capacity", C.c_uint64), ("ptcl_records", C.c_uint64), ("segments_emitted", C.c_uint32),
+        ("overflow", C.c_uint32), ("device_bytes", C.c_uint64)]
+
+
+class StageTimes(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("scan_clear", "scan_count", "scan_prefix", "scan_emit", "segment_order",
+                                         "cumsum", "coarse_count", "coarse_prefix", "coarse_emit", "fine", "total")]
+
+
+class CoarseConfig(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("tile_count", "tiles_width", "tiles_height", "batch_count", "draw_count",
+                                          "path_count", "paint_words", "paint_brush_base", "layer_count",
+                                          "tile_draw_index_count", "pad0", "pad1")] + [("ptcl_capacity", C.c_uint64)]
+
+
+class CoarseInputs(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("draws", "paths", "backdrops", "segment_ranges", "layers", "batches",
+                                          "paint", "draw_batch_ids", "tile_draws", "tile_draw_indices")]
+
+
+class CoarseOutputs(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("tile_ptcl_counts", "tile_ptcl_ranges", "tile_kinds", "batch_ptcl_bases",
+                                          "ptcl", "status")]
+
+
+class FineConfig(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("width", "height", "clear_color", "tile_count", "tiles_width",
+                                          "tiles_height", "batch_count", "clip_spill_depth", "paint_words",
+                                          "paint_sdf_shadow_base", "paint_brush_base", "draw_count",
+                                          "ptcl_capacity", "segment_capacity")]
+
+
+class FineInputs(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("draws", "paint", "segments", "tile_ptcl_ranges", "tile_kinds",
+                                          "batch_ptcl_bases", "ptcl")]
+
+
+# numpy dtypes for reading record arrays back
+LINE_DTYPE = np.dtype([("path_id", "<u4"), ("pad", "<f4"), ("p0", "<f4", 2), ("p1", "<f4", 2)])
+SEGMENT_DTYPE = np.dtype([("p0x", "<f4"), ("p0y", "<f4"), ("p1x", "<f4"), ("p1y", "<f4"), ("y_edge", "<f4")])
+PTCL_DTYPE = np.dtype([("tag", "<u4"), ("backdrop", "<i4"), ("fill_rule", "<u4"), ("segment_start", "<u4"),
+                       ("segment_end", "<u4"), ("color", "<u4")])
+assert LINE_DTYPE.itemsize == C.sizeof(Line) == 24
+assert C.sizeof(PathRecord) == 76 and C.sizeof(DrawRecord) == 124 and SEGMENT_DTYPE.itemsize == 20
+assert PTCL_DTYPE.itemsize == 24 and C.sizeof(ScanConfig) == 44
+
+# Every exported entry point, with (restype, argtypes).  The list is the C ABI.
+_P = C.c_void_p
+_U32 = C.c_uint32
+_SZ = C.c_size_t
+SIGNATURES = {
+    "tk_last_error": (C.c_char_p, []),
+    "tk_version": (_U32, []),
+    "tk_canvas_create": (C.c_int, [_U32, _U32, C.c_double, C.POINTER(_P)]),
+    "tk_canvas_destroy": (None, [_P]),
+    "tk_canvas_push_path": (C.c_int, [_P, _P, _SZ, _P, _SZ, _P, _P, _U32, C.c_double, C.POINTER(_U32)]),
+    "tk_canvas_push_stroke": (C.c_int, [_P, _P, _SZ, _P, _SZ, C.c_double, _P, _P, _U32, C.c_double, C.POINTER(_U32)]),
+    "tk_canvas_push_rect": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double, _P, _P, C.POINTER(_U32)]),
+    "tk_canvas_push_clip_sdf_rect_layer": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double, _P]),
+    "tk_canvas_push_clip_layer": (C.c_int, [_P, _P, _SZ, _P, _SZ, _P, _U32, C.c_double]),
+    "tk_canvas_pop_layer": (C.c_int, [_P]),
+    "tk_canvas_encode": (C.c_int, [_P, _U32, C.POINTER(Scene)]),
+    "tk_canvas_line_count": (_U32, [_P]),
+    "tk_line_scanned_tile_count": (_U32, [C.POINTER(Line), _U32, _U32, _U32, _U32]),
+    "tk_context_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "tk_context_destroy": (None, [_P]),
+    "tk_context_stream": (_P, [_P]),
+    "tk_context_synchronize": (C.c_int, [_P]),
+    "tk_context_limits": (C.c_int, [_P, C.POINTER(Limits)]),
+    "tk_device_alloc": (C.c_int, [_P, _SZ, C.POINTER(_P)]),
+    "tk_device_free": (C.c_int, [_P, _P]),
+    "tk_copy_to_device": (C.c_int, [_P, _P, _P, _SZ]),
+    "tk_copy_to_host": (C.c_int, [_P, _P, _P, _SZ]),
+    "tk_frame_create": (C.c_int, [_P, C.POINTER(Scene), C.POINTER(_P)]),
+    "tk_frame_update": (C.c_int, [_P, C.POINTER(Scene)]),
+    "tk_frame_destroy": (None, [_P]),
+    "tk_frame_render": (C.c_int, [_P]),
+    "tk_frame_read_image": (C.c_int, [_P, _P, _SZ]),
+    "tk_frame_image_device": (_P, [_P]),
+    "tk_frame_read_buffer": (C.c_int, [_P, C.c_int, _P, _SZ]),
+    "tk_frame_stats_read": (C.c_int, [_P, C.POINTER(FrameStats)]),
+    "tk_frame_set_capture": (C.c_int, [_P, C.c_int]),
+    "tk_frame_enable_timing": (C.c_int, [_P, C.c_int]),
+    "tk_frame_stage_times": (C.c_int, [_P, C.POINTER(StageTimes)]),
+    "tk_render_to_image": (C.c_int, [_P, C.POINTER(Scene), _P, _SZ]),
+    "tk_scan_clear": (C.c_int, [_P, C.POINTER(ScanConfig), _P, _P, _P, _P, _P, _P, _P]),
+    "tk_scan_count": (C.c_int, [_P, C.POINTER(ScanConfig), _P, _P, _U32, _P, _P]),
+    "tk_scan_prefix": (C.c_int, [_P, C.POINTER(ScanConfig), _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "tk_scan_emit": (C.c_int, [_P, C.POINTER(ScanConfig), _P, _P, _U32, _P, _P]),
+    "tk_cumsum": (C.c_int, [_P, C.POINTER(CumsumConfig), _P, _P, _P, _P, _P, _P, _P]),
+    "tk_coarse": (C.c_int, [_P, C.POINTER(CoarseConfig), C.POINTER(CoarseInputs), C.POINTER(CoarseOutputs)]),
+    "tk_fine": (C.c_int, [_P, C.POINTER(FineConfig), C.POINTER(FineInputs), _P]),
+    "tk_fill_coverage_words": (C.c_int, [_P, _U32, _P, _P, _P, _P]),
+    "tk_pixel_math_words": (C.c_int, [_P, _U32, _P, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libtileink_hip.so (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TileinkError(TK_ERR_UNAVAILABLE,
+                               f"{LIB_PATH} is missing; build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        handle = C.CDLL(LIB_PATH)
+        for name, (restype, argtypes) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        _lib = handle
+    return _lib
+
+
+def check(status):
+    if status != TK_OK:
+        raise TileinkError(status, lib().tk_last_error().decode(errors="replace"))
+
+
+def ptr(array):
+    """Address of a numpy array's data (None for None)."""
+    if array is None:
+        return None
+    return array.ctypes.data
+
+
+def view(address, dtype, count):
+    """Copies `count` elements of `dtype` from a host address into a new array."""
+    dtype = np.dtype(dtype)
+    if count == 0 or not address:
+        return np.zeros(0, dtype)
+    raw = (C.c_uint8 * (dtype.itemsize * count)).from_address(address)
+    return np.frombuffer(raw, dtype=dtype, count=count).copy()

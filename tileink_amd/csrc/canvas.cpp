@@ -1,0 +1,1338 @@
+// Host-side scene recording and planning for the compute-raster path.
+//
+// This is the recording layer the GPU stages consume: Canvas::push_* turn
+// shapes into flattened Lines plus PathRecord / DrawRecord / paint words
+// (src/canvas/paths.rs:65-156, src/canvas/primitives/rectangles.rs:33-43,
+// src/canvas/layers.rs:7-78), compile groups draws into DrawBatches with fused
+// clip stacks (src/canvas/compile.rs:134-217, src/shared/execution/batches.rs:
+// 202-273), and the planner derives tile bins and scan/cumsum chunk metadata
+// (src/shared/gpu_plan/tile_bins/build.rs:4-83, gpu_plan/path_plans.rs:463-504).
+//
+// Curve flattening and stroke outlining are this project's own geometry
+// front end: the reference delegates both to kurbo 0.13.1, which is not part of
+// the reference tree.  Flattening places chords by the parabola-integral method
+// so the chord error stays within the tolerance with few chords; strokes are the
+// exact union of per-segment rectangles, round joins and round caps, traced as
+// one outline per side (inner joints pass through the joint vertex so that the
+// nonzero winding number counts covering pieces).
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "tk_internal.h"
+
+namespace tk {
+namespace {
+
+struct Pt {
+    double x, y;
+};
+
+struct BezPath {
+    std::vector<uint8_t> verbs;
+    std::vector<Pt> pts;
+};
+
+constexpr uint32_t kNone = TK_INVALID_INDEX;
+
+inline Pt apply(const double *m, Pt p) {
+    return Pt{m[0] * p.x + m[2] * p.y + m[4], m[1] * p.x + m[3] * p.y + m[5]};
+}
+
+int points_for_verb(uint8_t verb) {
+    switch (verb) {
+    case TK_PATH_MOVE:
+    case TK_PATH_LINE: return 1;
+    case TK_PATH_QUAD: return 2;
+    case TK_PATH_CUBIC: return 3;
+    case TK_PATH_CLOSE: return 0;
+    default: return -1;
+    }
+}
+
+bool parse_path(const uint8_t *verbs, size_t verb_count, const double *points, size_t point_count,
+                BezPath *out) {
+    size_t needed = 0;
+    for (size_t i = 0; i < verb_count; ++i) {
+        int n = points_for_verb(verbs[i]);
+        if (n < 0) return false;
+        if (i == 0 && verbs[i] != TK_PATH_MOVE) return false;
+        needed += size_t(n);
+    }
+    if (needed != point_count) return false;
+    out->verbs.assign(verbs, verbs + verb_count);
+    out->pts.resize(point_count);
+    for (size_t i = 0; i < point_count; ++i) {
+        out->pts[i] = Pt{points[2 * i], points[2 * i + 1]};
+        if (!std::isfinite(out->pts[i].x) || !std::isfinite(out->pts[i].y)) return false;
+    }
+    return true;
+}
+
+BezPath transform_path(const BezPath &path, const double *m) {
+    BezPath out = path;
+    for (Pt &p : out.pts) p = apply(m, p);
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Exact curve bounds (kurbo's BezPath::bounding_box semantics: segment extrema).
+// ---------------------------------------------------------------------------
+
+struct Box {
+    double x0 = std::numeric_limits<double>::infinity(), y0 = x0;
+    double x1 = -std::numeric_limits<double>::infinity(), y1 = x1;
+    bool empty() const { return !(x0 <= x1); }
+    void add(Pt p) {
+        x0 = std::min(x0, p.x);
+        y0 = std::min(y0, p.y);
+        x1 = std::max(x1, p.x);
+        y1 = std::max(y1, p.y);
+    }
+};
+
+double cubic_at(double a, double b, double c, double d, double t) {
+    double mt = 1.0 - t;
+    return mt * mt * mt * a + 3.0 * mt * mt * t * b + 3.0 * mt * t * t * c + t * t * t * d;
+}
+
+void cubic_axis_extrema(double a, double b, double c, double d, double *t_out, int *n_out) {
+    // derivative / 3 = qa t^2 + qb t + qc
+    double qa = d - 3.0 * c + 3.0 * b - a;
+    double qb = 2.0 * (c - 2.0 * b + a);
+    double qc = b - a;
+    int n = 0;
+    if (std::fabs(qa) < 1e-12) {
+        if (std::fabs(qb) > 1e-12) {
+            double t = -qc / qb;
+            if (t > 0.0 && t < 1.0) t_out[n++] = t;
+        }
+    } else {
+        double disc = qb * qb - 4.0 * qa * qc;
+        if (disc >= 0.0) {
+            double s = std::sqrt(disc);
+            double t1 = (-qb + s) / (2.0 * qa), t2 = (-qb - s) / (2.0 * qa);
+            if (t1 > 0.0 && t1 < 1.0) t_out[n++] = t1;
+            if (t2 > 0.0 && t2 < 1.0) t_out[n++] = t2;
+        }
+    }
+    *n_out = n;
+}
+
+void add_cubic_bounds(Box &box, Pt p0, Pt p1, Pt p2, Pt p3) {
+    box.add(p0);
+    box.add(p3);
+    double ts[2];
+    int n;
+    cubic_axis_extrema(p0.x, p1.x, p2.x, p3.x, ts, &n);
+    for (int i = 0; i < n; ++i) {
+        double t = ts[i];
+        box.add(Pt{cubic_at(p0.x, p1.x, p2.x, p3.x, t), cubic_at(p0.y, p1.y, p2.y, p3.y, t)});
+    }
+    cubic_axis_extrema(p0.y, p1.y, p2.y, p3.y, ts, &n);
+    for (int i = 0; i < n; ++i) {
+        double t = ts[i];
+        box.add(Pt{cubic_at(p0.x, p1.x, p2.x, p3.x, t), cubic_at(p0.y, p1.y, p2.y, p3.y, t)});
+    }
+}
+
+Box path_bounds(const BezPath &path) {
+    Box box;
+    size_t pi = 0;
+    Pt last{0, 0};
+    for (uint8_t verb : path.verbs) {
+        switch (verb) {
+        case TK_PATH_MOVE: last = path.pts[pi++]; break;
+        case TK_PATH_LINE:
+            box.add(last);
+            box.add(path.pts[pi]);
+            last = path.pts[pi++];
+            break;
+        case TK_PATH_QUAD: {
+            Pt q0 = last, q1 = path.pts[pi], q2 = path.pts[pi + 1];
+            Pt c1{q0.x + 2.0 / 3.0 * (q1.x - q0.x), q0.y + 2.0 / 3.0 * (q1.y - q0.y)};
+            Pt c2{q2.x + 2.0 / 3.0 * (q1.x - q2.x), q2.y + 2.0 / 3.0 * (q1.y - q2.y)};
+            add_cubic_bounds(box, q0, c1, c2, q2);
+            last = q2;
+            pi += 2;
+            break;
+        }
+        case TK_PATH_CUBIC:
+            add_cubic_bounds(box, last, path.pts[pi], path.pts[pi + 1], path.pts[pi + 2]);
+            last = path.pts[pi + 2];
+            pi += 3;
+            break;
+        default: break;
+        }
+    }
+    return box;
+}
+
+// ---------------------------------------------------------------------------
+// Flattening with the reference's contour rules
+// (src/shared/path_flatten.rs:20-82): implicit close when the gap exceeds 1e-6,
+// degenerate edges with |d|^2 <= 1e-24 dropped.
+// ---------------------------------------------------------------------------
+
+struct Polyline {
+    std::vector<Pt> pts;
+    bool closed = false;
+};
+
+// Quadratic flattening by the parabola-integral method (R. Levien, "Flattening
+// quadratic Beziers", 2019): the quadratic is mapped onto a segment of the
+// parabola y = x^2, where the number of chords needed for a given chord error
+// is proportional to an integral that has a closed-form approximation; chords
+// are then placed at equal steps of that integral.  Cubics are first split
+// into quadratics whose approximation error is a tenth of the tolerance.
+double parabola_integral(double x) {
+    const double d = 0.67;
+    return x / (1.0 - d + std::sqrt(std::sqrt(d * d * d * d + 0.25 * x * x)));
+}
+double parabola_inv_integral(double x) {
+    const double b = 0.39;
+    return x * (1.0 - b + std::sqrt(b * b + 0.25 * x * x));
+}
+
+struct QuadParams {
+    Pt p0, p1, p2;
+    double a0 = 0, a1 = 0, u0 = 0, uscale = 0, val = 0;
+};
+
+QuadParams quad_params(Pt p0, Pt p1, Pt p2, double sqrt_tol) {
+    QuadParams q;
+    q.p0 = p0;
+    q.p1 = p1;
+    q.p2 = p2;
+    double ddx = 2.0 * p1.x - p0.x - p2.x, ddy = 2.0 * p1.y - p0.y - p2.y;
+    double cross = (p2.x - p0.x) * ddy - (p2.y - p0.y) * ddx;
+    double x0 = ((p1.x - p0.x) * ddx + (p1.y - p0.y) * ddy) / cross;
+    double x1 = ((p2.x - p1.x) * ddx + (p2.y - p1.y) * ddy) / cross;
+    double scale = std::fabs(cross / (std::hypot(ddx, ddy) * (x1 - x0)));
+    q.a0 = parabola_integral(x0);
+    q.a1 = parabola_integral(x1);
+    if (std::isfinite(scale) && std::isfinite(q.a0) && std::isfinite(q.a1)) {
+        double da = std::fabs(q.a1 - q.a0);
+        double sqrt_scale = std::sqrt(scale);
+        if ((x0 < 0.0) == (x1 < 0.0)) {
+            q.val = da * sqrt_scale;
+        } else {
+            // The vertex of the parabola lies inside the segment: bound the
+            // integrand near the cusp by the tolerance.
+            double xmin = sqrt_tol / sqrt_scale;
+            q.val = sqrt_tol * da / parabola_integral(xmin);
+        }
+        double u0 = parabola_inv_integral(q.a0), u2 = parabola_inv_integral(q.a1);
+        q.u0 = u0;
+        q.uscale = 1.0 / (u2 - u0);
+        if (!std::isfinite(q.val) || !std::isfinite(q.uscale)) q.val = -1.0;
+    } else {
+        q.val = -1.0;  // (nearly) a straight line: one chord
+    }
+    return q;
+}
+
+Pt quad_eval(const QuadParams &q, double t) {
+    double mt = 1.0 - t;
+    return Pt{mt * mt * q.p0.x + 2.0 * mt * t * q.p1.x + t * t * q.p2.x,
+              mt * mt * q.p0.y + 2.0 * mt * t * q.p1.y + t * t * q.p2.y};
+}
+
+// Parameter on the quadratic at a fraction u in [0, 1] of its integral.
+double quad_param_at(const QuadParams &q, double u) {
+    double a = q.a0 + (q.a1 - q.a0) * u;
+    double t = (parabola_inv_integral(a) - q.u0) * q.uscale;
+    return std::fmin(std::fmax(t, 0.0), 1.0);
+}
+
+// Emits the chord end points (excluding the start) of a run of quadratics
+// approximating one curve, sharing one chord budget across them.
+void flatten_quads(const std::vector<QuadParams> &quads, double sqrt_tol, std::vector<Pt> &out) {
+    double sum = 0.0;
+    for (const QuadParams &q : quads) sum += std::max(q.val, 0.0);
+    int n = int(std::ceil(0.5 * sum / sqrt_tol));
+    n = std::max(n, 1);
+    n = std::min(n, 1 << 16);
+    double step = sum / n;
+    double acc = 0.0;  // integral consumed before the current quadratic
+    int emitted = 0;
+    for (size_t k = 0; k < quads.size(); ++k) {
+        const QuadParams &q = quads[k];
+        double val = std::max(q.val, 0.0);
+        if (q.val < 0.0) {
+            // Degenerate parabola mapping: the quadratic is a straight chord.
+            if (k + 1 < quads.size()) out.push_back(q.p2);
+            continue;
+        }
+        while (emitted + 1 < n) {
+            double target = step * (emitted + 1);
+            if (target > acc + val) break;
+            double u = val > 0.0 ? (target - acc) / val : 1.0;
+            out.push_back(quad_eval(q, quad_param_at(q, u)));
+            ++emitted;
+        }
+        acc += val;
+        if (k + 1 < quads.size()) out.push_back(q.p2);  // keep sub-curve joints exact
+    }
+}
+
+void flatten_quad_to(Pt p0, Pt p1, Pt p2, double tol, std::vector<Pt> &out) {
+    double sqrt_tol = std::sqrt(tol);
+    std::vector<QuadParams> quads{quad_params(p0, p1, p2, sqrt_tol)};
+    flatten_quads(quads, sqrt_tol, out);
+    out.push_back(p2);
+}
+
+void flatten_cubic_to(Pt p0, Pt p1, Pt p2, Pt p3, double tol, std::vector<Pt> &out) {
+    const double quad_tol = 0.1 * tol;
+    const double sqrt_tol = std::sqrt(0.9 * tol);
+    // Midpoint-quadratic error of a cubic is sqrt(3)/36 |p3 - 3p2 + 3p1 - p0| h^3 on a span h.
+    double d3x = p3.x - 3.0 * p2.x + 3.0 * p1.x - p0.x, d3y = p3.y - 3.0 * p2.y + 3.0 * p1.y - p0.y;
+    double err = std::sqrt(3.0) / 36.0 * std::hypot(d3x, d3y);
+    int nq = int(std::ceil(std::cbrt(err / quad_tol)));
+    nq = std::max(1, std::min(nq, 1 << 12));
+    std::vector<QuadParams> quads;
+    quads.reserve(size_t(nq));
+    auto eval = [&](double t) { return Pt{cubic_at(p0.x, p1.x, p2.x, p3.x, t), cubic_at(p0.y, p1.y, p2.y, p3.y, t)}; };
+    auto deriv = [&](double t, double a, double b, double c, double d) {
+        double mt = 1.0 - t;
+        return 3.0 * (mt * mt * (b - a) + 2.0 * mt * t * (c - b) + t * t * (d - c));
+    };
+    for (int i = 0; i < nq; ++i) {
+        double t0 = double(i) / nq, t1 = double(i + 1) / nq, h = t1 - t0;
+        Pt q0 = i == 0 ? p0 : eval(t0), q2 = i + 1 == nq ? p3 : eval(t1);
+        Pt c1{q0.x + deriv(t0, p0.x, p1.x, p2.x, p3.x) * h / 3.0, q0.y + deriv(t0, p0.y, p1.y, p2.y, p3.y) * h / 3.0};
+        Pt c2{q2.x - deriv(t1, p0.x, p1.x, p2.x, p3.x) * h / 3.0, q2.y - deriv(t1, p0.y, p1.y, p2.y, p3.y) * h / 3.0};
+        Pt qc{(3.0 * (c1.x + c2.x) - q0.x - q2.x) * 0.25, (3.0 * (c1.y + c2.y) - q0.y - q2.y) * 0.25};
+        quads.push_back(quad_params(q0, qc, q2, sqrt_tol));
+    }
+    flatten_quads(quads, sqrt_tol, out);
+    out.push_back(p3);
+}
+
+// Flattens every contour into point lists; closed marks an explicit ClosePath.
+std::vector<Polyline> flatten_contours(const BezPath &path, double tol) {
+    std::vector<Polyline> out;
+    size_t pi = 0;
+    Pt last{0, 0};
+    for (uint8_t verb : path.verbs) {
+        switch (verb) {
+        case TK_PATH_MOVE:
+            out.emplace_back();
+            last = path.pts[pi++];
+            out.back().pts.push_back(last);
+            break;
+        case TK_PATH_LINE:
+            last = path.pts[pi++];
+            out.back().pts.push_back(last);
+            break;
+        case TK_PATH_QUAD:
+            flatten_quad_to(last, path.pts[pi], path.pts[pi + 1], tol, out.back().pts);
+            last = path.pts[pi + 1];
+            pi += 2;
+            break;
+        case TK_PATH_CUBIC:
+            flatten_cubic_to(last, path.pts[pi], path.pts[pi + 1], path.pts[pi + 2], tol, out.back().pts);
+            last = path.pts[pi + 2];
+            pi += 3;
+            break;
+        case TK_PATH_CLOSE:
+            if (!out.empty()) {
+                out.back().closed = true;
+                last = out.back().pts.front();
+            }
+            break;
+        default: break;
+        }
+    }
+    return out;
+}
+
+bool is_non_degenerate(Pt a, Pt b) {
+    double dx = a.x - b.x, dy = a.y - b.y;
+    return dx * dx + dy * dy > 1.0e-12 * 1.0e-12;
+}
+
+bool is_open_contour_close(Pt start, Pt end) {
+    double dx = start.x - end.x, dy = start.y - end.y;
+    return dx * dx + dy * dy > 1.0e-6 * 1.0e-6;
+}
+
+void push_line(std::vector<tk_line> &out, uint32_t path_id, Pt a, Pt b) {
+    if (!is_non_degenerate(a, b)) return;
+    tk_line line;
+    line.path_id = path_id;
+    line.pad = 0.0f;
+    line.p0[0] = float(a.x);
+    line.p0[1] = float(a.y);
+    line.p1[0] = float(b.x);
+    line.p1[1] = float(b.y);
+    out.push_back(line);
+}
+
+void flatten_to_lines(const BezPath &path, double tol, uint32_t path_id, std::vector<tk_line> &out) {
+    for (const Polyline &contour : flatten_contours(path, tol)) {
+        const auto &p = contour.pts;
+        for (size_t i = 1; i < p.size(); ++i) push_line(out, path_id, p[i - 1], p[i]);
+        if (p.empty()) continue;
+        // Explicit ClosePath always emits the closing edge; an open contour is
+        // closed only when its end is not already at its start.
+        if (contour.closed || is_open_contour_close(p.front(), p.back()))
+            push_line(out, path_id, p.back(), p.front());
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stroke outline (round joins, round caps).
+// ---------------------------------------------------------------------------
+
+struct OutlineWriter {
+    BezPath path;
+    void move(Pt p) {
+        path.verbs.push_back(TK_PATH_MOVE);
+        path.pts.push_back(p);
+    }
+    void line(Pt p) {
+        path.verbs.push_back(TK_PATH_LINE);
+        path.pts.push_back(p);
+    }
+    void close() { path.verbs.push_back(TK_PATH_CLOSE); }
+};
+
+// Appends the interior points of an arc around c of radius h from angle a0
+// turning by delta (signed), with chord error at most tol.
+void arc_points(std::vector<Pt> &out, Pt c, double h, double a0, double delta, double tol) {
+    double step = M_PI;
+    if (h > tol) step = 2.0 * std::acos(1.0 - tol / h);
+    int n = std::max(1, int(std::ceil(std::fabs(delta) / step)));
+    for (int i = 1; i < n; ++i) {
+        double a = a0 + delta * double(i) / n;
+        out.push_back(Pt{c.x + h * std::cos(a), c.y + h * std::sin(a)});
+    }
+}
+
+// Intersection parameter of segments a0->a1 and b0->b1; false when parallel
+// or when the crossing lies outside either segment.
+bool segment_cross(Pt a0, Pt a1, Pt b0, Pt b1, Pt *x) {
+    double rx = a1.x - a0.x, ry = a1.y - a0.y, sx = b1.x - b0.x, sy = b1.y - b0.y;
+    double den = rx * sy - ry * sx;
+    if (den == 0.0) return false;
+    double qx = b0.x - a0.x, qy = b0.y - a0.y;
+    double t = (qx * sy - qy * sx) / den, u = (qx * ry - qy * rx) / den;
+    if (!(t >= 0.0 && t <= 1.0 && u >= 0.0 && u <= 1.0)) return false;
+    *x = Pt{a0.x + t * rx, a0.y + t * ry};
+    return true;
+}
+
+// Stroke outline of a polyline with round joins and caps.
+//
+// The stroked region is the union of one rectangle per segment, a disc sector
+// at every outer joint and a half disc at each open end.  Tracing the sum of
+// those pieces' boundaries (all with one orientation) and cancelling shared
+// half-edges leaves one chain per side: on the outer side of a joint the arc,
+// on the inner side a detour through the joint vertex.  The nonzero winding of
+// that outline counts covering pieces, so nonzero filling gives the union.
+// Two reductions keep the line count down: an inner detour is replaced by the
+// crossing of the two inner edges when that crossing lies on both (this only
+// removes a loop lying inside both rectangles), and an outer arc whose miter
+// is within the join tolerance of it is replaced by the miter point.
+void stroke_side(std::vector<Pt> &out, const std::vector<Pt> &p, const std::vector<Pt> &n, const std::vector<Pt> &d,
+                 bool closed, double side, double h, double join_tol) {
+    size_t m = n.size();
+    auto vert = [&](size_t i) { return p[i % p.size()]; };
+    auto off = [&](size_t i, size_t seg) {
+        Pt q = vert(i);
+        return Pt{q.x + side * h * n[seg].x, q.y + side * h * n[seg].y};
+    };
+    size_t joints = closed ? m : m - 1;
+    std::vector<Pt> pts;
+    Pt start = off(0, 0);
+    Pt cur = start;  // where the chain entered the current edge
+    for (size_t i = 0; i < m; ++i) {
+        Pt e = off(i + 1, i);
+        if (i >= joints) {
+            pts.push_back(e);
+            break;
+        }
+        size_t j = (i + 1) % m;
+        Pt c = vert(i + 1);
+        double cr = d[i].x * d[j].y - d[i].y * d[j].x, dt = d[i].x * d[j].x + d[i].y * d[j].y;
+        double ang = std::atan2(cr, dt);
+        if (cr == 0.0 && dt < 0.0) ang = -M_PI;  // U-turn: the +n side is outer
+        Pt sj = off(i + 1, j);
+        bool outer = side > 0.0 ? ang < 0.0 : ang > 0.0;
+        bool closing = closed && i + 1 == m;  // the joint at vertex 0 of a loop keeps the edge start exact
+        if (ang == 0.0) {
+            pts.push_back(e);
+            cur = e;
+        } else if (outer) {
+            double sweep = std::fabs(ang);
+            double mx = side * (n[i].x + n[j].x), my = side * (n[i].y + n[j].y);
+            double den = 1.0 + (n[i].x * n[j].x + n[i].y * n[j].y);
+            if (!closing && sweep < 0.5 * M_PI && h * (1.0 / std::cos(0.5 * sweep) - 1.0) <= join_tol && den > 0.0) {
+                Pt mp{c.x + h * mx / den, c.y + h * my / den};
+                pts.push_back(mp);
+                cur = mp;
+            } else {
+                pts.push_back(e);
+                double a0 = std::atan2(side * n[i].y, side * n[i].x);
+                arc_points(pts, c, h, a0, side > 0.0 ? -sweep : sweep, join_tol);
+                pts.push_back(sj);
+                cur = sj;
+            }
+        } else {
+            Pt x;
+            if (!closing && segment_cross(cur, e, sj, off(i + 2, j), &x)) {
+                pts.push_back(x);
+                cur = x;
+            } else {
+                pts.push_back(e);
+                pts.push_back(c);
+                pts.push_back(sj);
+                cur = sj;
+            }
+        }
+    }
+    if (closed && !pts.empty()) pts.pop_back();  // the closing joint ends on the loop's first point
+    out.push_back(start);
+    out.insert(out.end(), pts.begin(), pts.end());
+}
+
+void stroke_polyline(OutlineWriter &w, std::vector<Pt> pts, bool closed, double h, double tol) {
+    std::vector<Pt> p;
+    for (const Pt &q : pts)
+        if (p.empty() || std::hypot(q.x - p.back().x, q.y - p.back().y) > 1e-9) p.push_back(q);
+    if (closed && p.size() > 1 && std::hypot(p.front().x - p.back().x, p.front().y - p.back().y) <= 1e-9)
+        p.pop_back();
+    if (p.size() < 2) {
+        if (p.size() == 1 && !closed) {  // a dot: round caps make a disc
+            Pt c = p[0];
+            std::vector<Pt> ring{Pt{c.x + h, c.y}};
+            arc_points(ring, c, h, 0.0, -2.0 * M_PI, tol);
+            w.move(ring[0]);
+            for (size_t i = 1; i < ring.size(); ++i) w.line(ring[i]);
+            w.close();
+        }
+        return;
+    }
+    size_t m = closed ? p.size() : p.size() - 1;
+    std::vector<Pt> d(m), n(m);
+    for (size_t i = 0; i < m; ++i) {
+        Pt a = p[i], b = p[(i + 1) % p.size()];
+        double len = std::hypot(b.x - a.x, b.y - a.y);
+        d[i] = Pt{(b.x - a.x) / len, (b.y - a.y) / len};
+        n[i] = Pt{-d[i].y, d[i].x};
+    }
+    std::vector<Pt> left, right;
+    stroke_side(left, p, n, d, closed, 1.0, h, tol);
+    stroke_side(right, p, n, d, closed, -1.0, h, tol);
+    w.move(left[0]);
+    for (size_t i = 1; i < left.size(); ++i) w.line(left[i]);
+    if (closed) {
+        w.close();
+        w.move(right.back());
+        for (size_t i = right.size() - 1; i-- > 0;) w.line(right[i]);
+        w.close();
+        return;
+    }
+    // End cap: from +n to -n through the direction of travel.
+    Pt pe = p.back();
+    std::vector<Pt> cap;
+    arc_points(cap, pe, h, std::atan2(n[m - 1].y, n[m - 1].x), -M_PI, tol);
+    for (const Pt &q : cap) w.line(q);
+    for (size_t i = right.size(); i-- > 0;) w.line(right[i]);
+    // Start cap: from -n back to +n through the reverse direction.
+    Pt ps = p.front();
+    cap.clear();
+    arc_points(cap, ps, h, std::atan2(-n[0].y, -n[0].x), -M_PI, tol);
+    for (const Pt &q : cap) w.line(q);
+    w.close();
+}
+
+BezPath stroke_outline(const BezPath &path, double width, double tol) {
+    OutlineWriter w;
+    double h = 0.5 * width;
+    if (!(h > 0.0)) return w.path;
+    // The centreline gets three quarters of the tolerance, joins and caps the rest.
+    for (const Polyline &c : flatten_contours(path, 0.75 * tol)) stroke_polyline(w, c.pts, c.closed, h, 0.25 * tol);
+    return w.path;
+}
+
+// ---------------------------------------------------------------------------
+// line_scanned_tile_count: CPU restatement with Rust f32 semantics
+// (src/shared/scan_line.rs:17-151): round() is half-away-from-zero and float
+// to int casts saturate.
+// ---------------------------------------------------------------------------
+
+inline uint32_t rs_u32(float v) {
+    if (!(v > 0.0f)) return 0u;
+    if (v >= 4294967295.0f) return 0xFFFFFFFFu;
+    return uint32_t(v);
+}
+inline int32_t rs_i32(float v) {
+    if (v != v) return 0;
+    if (v <= -2147483648.0f) return INT32_MIN;
+    if (v >= 2147483647.0f) return INT32_MAX;
+    return int32_t(v);
+}
+inline uint32_t rs_span(float a, float b) {
+    float hi = std::ceil(std::fmax(a, b));
+    float lo = std::floor(std::fmin(a, b));
+    return rs_u32(std::fmax(hi - lo, 1.0f));
+}
+
+}  // namespace
+
+uint32_t line_scanned_tile_count(const tk_line &line, uint32_t bx0, uint32_t by0, uint32_t bx1,
+                                 uint32_t by1) {
+    const float scale = 1.0f / 16.0f;
+    const float top_touch_epsilon = 1.0e-12f;
+    float p0x = line.p0[0], p0y = line.p0[1], p1x = line.p1[0], p1y = line.p1[1];
+    bool is_down = p1y >= p0y;
+    float xy0x = is_down ? p0x : p1x, xy0y = is_down ? p0y : p1y;
+    float xy1x = is_down ? p1x : p0x, xy1y = is_down ? p1y : p0y;
+    float s0x = xy0x * scale, s0y = xy0y * scale, s1x = xy1x * scale, s1y = xy1y * scale;
+    uint32_t count_x = rs_span(s0x, s1x) - 1u;
+    uint32_t count = count_x + rs_span(s0y, s1y);
+    float dx = std::fabs(s1x - s0x);
+    float dy = s1y - s0y;
+    if (dx + dy == 0.0f) return 0;
+    if (dy == 0.0f && std::floor(s0y) == s0y) return 0;
+    float idxdy = 1.0f / (dx + dy);
+    float a = dx * idxdy;
+    bool is_positive_slope = s1x >= s0x;
+    float sign = is_positive_slope ? 1.0f : -1.0f;
+    float xt0 = std::floor(s0x * sign);
+    float c = s0x * sign - xt0;
+    float y0 = std::floor(s0y);
+    float ytop = s0y == s1y ? std::ceil(s0y) : y0 + 1.0f;
+    float b = std::fmin((dy * c + dx * (ytop - s0y)) * idxdy, 0.99999994f);
+    float robust_err = std::floor(a * (float(count) - 1.0f) + b) - float(count_x);
+    if (robust_err != 0.0f) a -= std::copysign(2e-7f, robust_err);
+    float x0 = xt0 * sign + (is_positive_slope ? 0.0f : -1.0f);
+    float xmin = std::fmin(s0x, s1x);
+    if (s0y >= float(by1) || s1y <= float(by0) + top_touch_epsilon || xmin >= float(bx1)) return 0;
+    uint32_t imin = 0;
+    if (s0y < float(by0)) {
+        float iminf = std::round((float(by0) - y0 + b - a) / (1.0f - a)) - 1.0f;
+        if (y0 + iminf - std::floor(a * iminf + b) < float(by0)) iminf += 1.0f;
+        imin = rs_u32(iminf);
+    }
+    uint32_t imax = count;
+    if (s1y > float(by1)) {
+        float imaxf = std::round((float(by1) - y0 + b - a) / (1.0f - a)) - 1.0f;
+        if (y0 + imaxf - std::floor(a * imaxf + b) < float(by1)) imaxf += 1.0f;
+        imax = rs_u32(imaxf);
+    }
+    if (std::fmax(s0x, s1x) < float(bx0)) {
+        imax = imin;
+    } else {
+        float fudge = is_positive_slope ? 0.0f : 1.0f;
+        if (xmin < float(bx0)) {
+            float f = std::round((sign * (float(bx0) - x0) - b + fudge) / a);
+            if ((x0 + sign * std::floor(a * f + b) < float(bx0)) == is_positive_slope) f += 1.0f;
+            if (is_positive_slope) {
+                if (rs_u32(f) > imin) imin = rs_u32(f);
+            } else if (rs_u32(f) < imax) {
+                imax = rs_u32(f);
+            }
+        }
+        if (std::fmax(s0x, s1x) > float(bx1)) {
+            float f = std::round((sign * (float(bx1) - x0) - b + fudge) / a);
+            if ((x0 + sign * std::floor(a * f + b) < float(bx1)) == is_positive_slope) f += 1.0f;
+            if (is_positive_slope)
+                imax = std::min(imax, rs_u32(f));
+            else
+                imin = std::max(imin, rs_u32(f));
+        }
+    }
+    imax = std::max(imin, imax);
+    uint32_t total = 0;
+    for (uint32_t i = imin; i < imax; ++i) {
+        float z = std::floor(a * float(i) + b);
+        int32_t y = rs_i32(y0 + float(i) - z);
+        int32_t x = rs_i32(x0 + sign * z);
+        total += (y >= int32_t(by0) && y < int32_t(by1) && x >= int32_t(bx0) && x < int32_t(bx1)) ? 1u : 0u;
+    }
+    return total;
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Brush and SDF encoding (src/shared/brush.rs:249-274,638-767, gpu_sdf.rs:48-63)
+// ---------------------------------------------------------------------------
+
+uint32_t f2bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+uint32_t premul_f32_to_u32(const float c[4]) {
+    // premul_f32_to_u32 (src/shared/pixel.rs:14-21): (c * 255 + 0.5) as u8, saturating.
+    uint32_t out = 0;
+    for (int i = 0; i < 4; ++i) {
+        float v = c[i] * 255.0f + 0.5f;
+        uint32_t b = v <= 0.0f ? 0u : (v >= 255.0f ? 255u : uint32_t(v));
+        out |= b << (8 * i);
+    }
+    return out;
+}
+
+uint32_t premultiply_pack(const float rgba[4]) {
+    float p[4] = {rgba[0] * rgba[3], rgba[1] * rgba[3], rgba[2] * rgba[3], rgba[3]};
+    return premul_f32_to_u32(p);
+}
+
+size_t quantize_ramp_size(float span, size_t stop_count) {
+    size_t span_samples = size_t(std::ceil(std::fmax(span, 1.0f) * 2.0f));
+    size_t stop_samples = (std::max<size_t>(stop_count, 2) - 1) * 16;
+    size_t target = std::max(span_samples, stop_samples);
+    target = std::min<size_t>(std::max<size_t>(target, 64), 4096);
+    size_t p = 1;
+    while (p < target) p <<= 1;
+    return std::min<size_t>(p, 4096);
+}
+
+// Premultiplied-sRGB interpolation between sorted stops (peniko's default
+// interpolation space); peniko's exact colour arithmetic is not restated.
+std::vector<uint32_t> build_ramp(std::vector<tk_gradient_stop> stops, size_t ramp_size) {
+    ramp_size = std::max<size_t>(ramp_size, 2);
+    std::vector<uint32_t> ramp(ramp_size, 0u);
+    if (stops.empty()) return ramp;
+    std::stable_sort(stops.begin(), stops.end(),
+                     [](const tk_gradient_stop &a, const tk_gradient_stop &b) { return a.offset < b.offset; });
+    for (size_t ix = 0; ix < ramp_size; ++ix) {
+        float t = float(ix) / float(ramp_size - 1);
+        size_t upper = 0;
+        while (upper < stops.size() && stops[upper].offset < t) ++upper;
+        tk_gradient_stop left, right;
+        if (upper == 0) left = right = stops[0];
+        else if (upper >= stops.size()) left = right = stops.back();
+        else { left = stops[upper - 1]; right = stops[upper]; }
+        float span = right.offset - left.offset;
+        float lt = std::fabs(span) <= 1.1920929e-7f ? 0.0f : std::fmin(std::fmax((t - left.offset) / span, 0.0f), 1.0f);
+        float lp[4] = {left.rgba[0] * left.rgba[3], left.rgba[1] * left.rgba[3], left.rgba[2] * left.rgba[3], left.rgba[3]};
+        float rp[4] = {right.rgba[0] * right.rgba[3], right.rgba[1] * right.rgba[3], right.rgba[2] * right.rgba[3], right.rgba[3]};
+        float c[4];
+        for (int k = 0; k < 4; ++k) c[k] = lp[k] + (rp[k] - lp[k]) * lt;
+        ramp[ix] = premul_f32_to_u32(c);
+    }
+    return ramp;
+}
+
+// Encodes a brush into its 9 header + 12 parameter words (+ ramp).
+bool encode_brush(const tk_brush *brush, double scale, std::vector<uint32_t> &blob, uint32_t *offset,
+                  uint32_t *len) {
+    uint32_t data[TK_BRUSH_HEADER_WORDS] = {TK_BRUSH_SOLID, 0u, 0u, 0u, 0u, 0u, 0u, 255u, 0u};
+    uint32_t params[TK_BRUSH_PARAM_WORDS] = {};
+    *offset = uint32_t(blob.size());
+    if (brush == nullptr || brush->kind == TK_BRUSH_SOLID) {
+        float transparent[4] = {0, 0, 0, 0};
+        data[4] = premultiply_pack(brush ? brush->rgba : transparent);
+        blob.insert(blob.end(), data, data + TK_BRUSH_HEADER_WORDS);
+        blob.insert(blob.end(), params, params + TK_BRUSH_PARAM_WORDS);
+    } else if (brush->kind == TK_BRUSH_LINEAR) {
+        if (brush->extend > 2u) return false;
+        std::vector<tk_gradient_stop> stops(brush->stops, brush->stops + brush->stop_count);
+        float span = std::hypot(brush->end[0] - brush->start[0], brush->end[1] - brush->start[1]);
+        std::vector<uint32_t> ramp = build_ramp(stops, quantize_ramp_size(span, stops.size()));
+        data[0] = TK_BRUSH_LINEAR;
+        data[1] = brush->extend;
+        data[2] = TK_BRUSH_HEADER_WORDS + TK_BRUSH_PARAM_WORDS;
+        data[3] = uint32_t(ramp.size());
+        float inv = float(1.0 / scale);
+        float xf[6] = {inv, 0.0f, 0.0f, inv, 0.0f, 0.0f};  // scale_brush_transform(identity, 1/scale)
+        params[0] = f2bits(brush->start[0]);
+        params[1] = f2bits(brush->start[1]);
+        params[2] = f2bits(brush->end[0]);
+        params[3] = f2bits(brush->end[1]);
+        for (int i = 0; i < 6; ++i) params[4 + i] = f2bits(xf[i]);
+        blob.insert(blob.end(), data, data + TK_BRUSH_HEADER_WORDS);
+        blob.insert(blob.end(), params, params + TK_BRUSH_PARAM_WORDS);
+        blob.insert(blob.end(), ramp.begin(), ramp.end());
+    } else {
+        return false;
+    }
+    *len = uint32_t(blob.size()) - *offset;
+    return true;
+}
+
+tk_affine identity_affine() { return tk_affine{1.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f}; }
+
+struct TileBox {
+    uint32_t x0, y0, x1, y1;
+    uint64_t area() const { return uint64_t(x1 > x0 ? x1 - x0 : 0) * uint64_t(y1 > y0 ? y1 - y0 : 0); }
+};
+
+// PixelBounds::tile_bbox (src/shared/bounds.rs:113-128).
+TileBox tile_bbox(const tk_pixel_bounds &b, uint32_t tw, uint32_t th) {
+    auto clamp0 = [](int32_t v) { return uint32_t(std::max(v, 0)); };
+    TileBox t;
+    t.x0 = std::min(clamp0(b.x0) / TK_TILE_SIZE, tw);
+    t.y0 = std::min(clamp0(b.y0) / TK_TILE_SIZE, th);
+    t.x1 = std::min(uint32_t((uint64_t(clamp0(b.x1)) + TK_TILE_SIZE - 1) / TK_TILE_SIZE), tw);
+    t.y1 = std::min(uint32_t((uint64_t(clamp0(b.y1)) + TK_TILE_SIZE - 1) / TK_TILE_SIZE), th);
+    return t;
+}
+
+int32_t sat_i32(double v) {
+    if (!(v == v)) return 0;
+    if (v <= -2147483648.0) return INT32_MIN;
+    if (v >= 2147483647.0) return INT32_MAX;
+    return int32_t(v);
+}
+
+}  // namespace
+}  // namespace tk
+
+// ---------------------------------------------------------------------------
+// Canvas
+// ---------------------------------------------------------------------------
+
+struct tk_canvas {
+    uint32_t width = 0, height = 0;  // physical
+    double scale = 1.0;
+    uint32_t tiles_w = 0, tiles_h = 0;
+
+    std::vector<tk_line> lines;
+    std::vector<tk_path_record> paths;
+    std::vector<tk_draw_record> draws;
+    std::vector<uint32_t> sdf_blob, shadow_blob, brush_blob;
+    uint32_t backdrop_pool = 0;
+    uint64_t segment_total = 0;
+
+    struct Cmd {
+        uint8_t kind;  // 0 draw, 1 clip layer
+        uint32_t draw;
+        uint32_t children;
+    };
+    std::vector<std::vector<Cmd>> lists{1};
+    std::vector<uint32_t> open{0};
+
+    // Encoded plan (valid until the next push).
+    std::vector<uint32_t> paint;
+    std::vector<uint32_t> draw_batch_ids;
+    std::vector<tk_layer_record> layer_records;
+    std::vector<tk_batch_record> batch_records;
+    std::vector<tk_tile_draw_record> tile_draws;
+    std::vector<uint32_t> tile_draw_indices;
+    std::vector<tk_scan_chunk> scan_chunks;
+    std::vector<tk_scan_chunk_range> scan_chunk_ranges;
+    std::vector<uint32_t> cumsum_offsets, cumsum_lens, cumsum_row_starts, cumsum_row_ends;
+};
+
+namespace tk {
+namespace {
+
+void device_matrix(const tk_canvas *c, const double *transform, double out[6]) {
+    double t[6] = {1, 0, 0, 1, 0, 0};
+    if (transform) std::memcpy(t, transform, sizeof(t));
+    for (int i = 0; i < 6; ++i) out[i] = t[i] * c->scale;
+}
+
+// push_path_inner_with_tag (src/canvas/paths.rs:65-156) for an already
+// transformed (physical) path.
+uint32_t push_physical_path(tk_canvas *c, const BezPath &path, double device_tol, uint32_t rule,
+                            uint32_t tag, const tk_brush *brush, bool emit_command, tk_status *status) {
+    uint32_t line_start = uint32_t(c->lines.size());
+    uint32_t path_id = uint32_t(c->paths.size());
+    flatten_to_lines(path, device_tol, path_id, c->lines);
+    uint32_t line_count = uint32_t(c->lines.size()) - line_start;
+    Box box = path_bounds(path);
+    tk_pixel_bounds pb{0, 0, 0, 0};
+    if (!box.empty()) {
+        pb.x0 = sat_i32(std::floor(box.x0));
+        pb.y0 = sat_i32(std::floor(box.y0));
+        pb.x1 = sat_i32(std::ceil(box.x1));
+        pb.y1 = sat_i32(std::ceil(box.y1));
+    }
+    TileBox tb = tile_bbox(pb, c->tiles_w, c->tiles_h);
+    uint32_t stride = tb.x1 > tb.x0 ? tb.x1 - tb.x0 : 0;
+    uint32_t height = tb.y1 > tb.y0 ? tb.y1 - tb.y0 : 0;
+    uint32_t capacity = 0;
+    for (uint32_t i = line_start; i < line_start + line_count; ++i) {
+        uint64_t next = uint64_t(capacity) + line_scanned_tile_count(c->lines[i], tb.x0, tb.y0, tb.x1, tb.y1);
+        capacity = next > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(next);
+    }
+    tk_path_record rec{};
+    rec.path_id = path_id;
+    rec.line_count = line_count;
+    rec.line_start = line_start;
+    rec.flags = 0;
+    rec.data_offset = c->backdrop_pool;
+    rec.data_len = stride * height;
+    rec.tile_x0 = tb.x0;
+    rec.tile_y0 = tb.y0;
+    rec.tile_x1 = tb.x1;
+    rec.tile_y1 = tb.y1;
+    rec.segment_start = uint32_t(c->segment_total);
+    rec.segment_capacity = capacity;
+    rec.segment_count = 0;
+    rec.transform = identity_affine();
+    c->backdrop_pool += stride * height;
+    c->segment_total += capacity;
+    c->paths.push_back(rec);
+
+    uint32_t brush_offset = 0, brush_len = 0;
+    if (!encode_brush(brush, c->scale, c->brush_blob, &brush_offset, &brush_len)) {
+        *status = TK_ERR_INVALID;
+        return kNone;
+    }
+    tk_draw_record d{};
+    d.path_id = path_id;
+    d.glyph_run_id = kNone;
+    d.sdf_offset = kNone;
+    d.sdf_len = 0;
+    d.sdf_shadow_offset = kNone;
+    d.sdf_shadow_len = 0;
+    d.brush_offset = brush_offset;
+    d.brush_len = brush_len;
+    d.tag = tag;
+    d.fill_rule = rule;
+    d.pixel_bounds = pb;
+    d.local_pixel_bounds = pb;
+    d.solid_rect = 0;
+    d.transform = identity_affine();
+    d.inverse_transform = identity_affine();
+    uint32_t draw_ix = uint32_t(c->draws.size());
+    c->draws.push_back(d);
+    if (emit_command) c->lists[c->open.back()].push_back(tk_canvas::Cmd{0, draw_ix, 0});
+    *status = TK_OK;
+    return draw_ix;
+}
+
+// push_physical_sdf_record (src/canvas/paths.rs:207-250) for an SdfRect.
+uint32_t push_sdf_rect(tk_canvas *c, double x0, double y0, double x1, double y1, const float radii[4],
+                       const tk_brush *brush, uint32_t tag, bool emit_command, tk_status *status) {
+    double s = c->scale;
+    double ax0 = std::min(x0, x1) * s, ay0 = std::min(y0, y1) * s;
+    double ax1 = std::max(x0, x1) * s, ay1 = std::max(y0, y1) * s;
+    uint32_t sdf_offset = uint32_t(c->sdf_blob.size());
+    float r[4] = {0, 0, 0, 0};
+    if (radii) for (int i = 0; i < 4; ++i) r[i] = float(radii[i] * s);
+    c->sdf_blob.push_back(TK_SDF_KIND_RECT);
+    float coords[4] = {float(ax0), float(ay0), float(ax1), float(ay1)};
+    for (float v : coords) c->sdf_blob.push_back(f2bits(v));
+    for (float v : r) c->sdf_blob.push_back(f2bits(v));
+    for (int i = 0; i < 8; ++i) c->sdf_blob.push_back(0u);  // stroke[4], shadow[4]
+    uint32_t brush_offset = 0, brush_len = 0;
+    if (!encode_brush(brush, s, c->brush_blob, &brush_offset, &brush_len)) {
+        *status = TK_ERR_INVALID;
+        return kNone;
+    }
+    tk_pixel_bounds pb{sat_i32(std::floor(ax0)), sat_i32(std::floor(ay0)), sat_i32(std::ceil(ax1)),
+                       sat_i32(std::ceil(ay1))};
+    tk_draw_record d{};
+    d.path_id = kNone;
+    d.glyph_run_id = kNone;
+    d.sdf_offset = sdf_offset;
+    d.sdf_len = TK_SDF_RECORD_WORDS;
+    d.sdf_shadow_offset = kNone;
+    d.sdf_shadow_len = 0;
+    d.brush_offset = brush_offset;
+    d.brush_len = brush_len;
+    d.tag = tag;
+    d.fill_rule = TK_FILL_NONZERO;
+    d.pixel_bounds = pb;
+    d.local_pixel_bounds = pb;
+    d.solid_rect = 0;
+    d.transform = identity_affine();
+    d.inverse_transform = identity_affine();
+    uint32_t draw_ix = uint32_t(c->draws.size());
+    c->draws.push_back(d);
+    if (emit_command) c->lists[c->open.back()].push_back(tk_canvas::Cmd{0, draw_ix, 0});
+    *status = TK_OK;
+    return draw_ix;
+}
+
+void open_clip_layer(tk_canvas *c, uint32_t draw) {
+    uint32_t child = uint32_t(c->lists.size());
+    c->lists.emplace_back();
+    c->lists[c->open.back()].push_back(tk_canvas::Cmd{1, draw, child});
+    c->open.push_back(child);
+}
+
+// axis_aligned_rect_path (src/canvas/layers.rs:286+): a closed 4/5-point line
+// loop whose edges are axis-aligned.
+bool axis_aligned_rect(const BezPath &path, double *r) {
+    std::vector<Pt> p;
+    bool closed = false;
+    size_t pi = 0;
+    for (uint8_t v : path.verbs) {
+        if (v == TK_PATH_MOVE && p.empty()) p.push_back(path.pts[pi++]);
+        else if (v == TK_PATH_LINE && !closed) p.push_back(path.pts[pi++]);
+        else if (v == TK_PATH_CLOSE && !closed) closed = true;
+        else return false;
+    }
+    if (!closed || p.size() < 4 || p.size() > 5) return false;
+    if (p.size() == 5) {
+        if (p[4].x != p[0].x || p[4].y != p[0].y) return false;
+        p.pop_back();
+    }
+    for (size_t i = 0; i < 4; ++i) {
+        Pt a = p[i], b = p[(i + 1) % 4];
+        if (a.x != b.x && a.y != b.y) return false;
+        if (a.x == b.x && a.y == b.y) return false;
+    }
+    double x0 = p[0].x, x1 = p[0].x, y0 = p[0].y, y1 = p[0].y;
+    for (const Pt &q : p) {
+        x0 = std::min(x0, q.x);
+        x1 = std::max(x1, q.x);
+        y0 = std::min(y0, q.y);
+        y1 = std::max(y1, q.y);
+    }
+    for (const Pt &q : p)
+        if ((q.x != x0 && q.x != x1) || (q.y != y0 && q.y != y1)) return false;
+    r[0] = x0;
+    r[1] = y0;
+    r[2] = x1;
+    r[3] = y1;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Compile + plan
+// ---------------------------------------------------------------------------
+
+struct PlannedBatch {
+    std::vector<uint32_t> draws;
+    std::vector<uint32_t> stack;  // clip draw ids, outermost first
+};
+
+void compile_into(const tk_canvas *c, uint32_t list, std::vector<uint32_t> &stack,
+                  std::vector<PlannedBatch> &ops) {
+    std::vector<uint32_t> pending;
+    auto flush = [&]() {
+        if (pending.empty()) return;
+        ops.push_back(PlannedBatch{pending, stack});
+        pending.clear();
+    };
+    for (const tk_canvas::Cmd &cmd : c->lists[list]) {
+        if (cmd.kind == 0) {
+            pending.push_back(cmd.draw);
+        } else {
+            flush();
+            stack.push_back(cmd.draw);
+            compile_into(c, cmd.children, stack, ops);
+            stack.pop_back();
+        }
+    }
+    flush();
+}
+
+tk_status encode(tk_canvas *c, uint32_t clear_color, tk_scene *out) {
+    const uint32_t tw = c->tiles_w, th = c->tiles_h;
+    const uint64_t tiles = uint64_t(tw) * th;
+
+    // ExecPlan: batches in painter order, adjacent equal stacks coalesced.
+    std::vector<PlannedBatch> ops;
+    std::vector<uint32_t> stack;
+    compile_into(c, 0, stack, ops);
+    std::vector<PlannedBatch> batches;
+    for (PlannedBatch &op : ops) {
+        if (!batches.empty() && batches.back().stack == op.stack) {
+            batches.back().draws.insert(batches.back().draws.end(), op.draws.begin(), op.draws.end());
+        } else {
+            batches.push_back(std::move(op));
+        }
+    }
+    c->draw_batch_ids.assign(c->draws.size(), kNone);
+    c->layer_records.clear();
+    c->batch_records.clear();
+    std::vector<uint32_t> draw_order;
+    uint32_t max_depth = 0;
+    for (size_t b = 0; b < batches.size(); ++b) {
+        tk_batch_record rec;
+        rec.layer_start = uint32_t(c->layer_records.size());
+        for (uint32_t clip : batches[b].stack) c->layer_records.push_back(tk_layer_record{TK_LAYER_CLIP, clip, 0u});
+        rec.layer_end = uint32_t(c->layer_records.size());
+        max_depth = std::max<uint32_t>(max_depth, uint32_t(batches[b].stack.size()));
+        c->batch_records.push_back(rec);
+        for (uint32_t d : batches[b].draws) {
+            c->draw_batch_ids[d] = uint32_t(b);
+            draw_order.push_back(d);
+        }
+    }
+
+    // Tile bins: painter-ordered candidate lists per tile.
+    std::vector<uint32_t> counts(tiles, 0u);
+    std::vector<TileBox> boxes(c->draws.size());
+    for (uint32_t d : draw_order) {
+        boxes[d] = tile_bbox(c->draws[d].pixel_bounds, tw, th);
+        const TileBox &bb = boxes[d];
+        for (uint32_t y = bb.y0; y < bb.y1; ++y)
+            for (uint32_t x = bb.x0; x < bb.x1; ++x) counts[size_t(y) * tw + x]++;
+    }
+    c->tile_draws.assign(tiles, tk_tile_draw_record{kNone, 0u});
+    uint64_t total = 0;
+    for (size_t t = 0; t < tiles; ++t) {
+        if (counts[t] != 0) c->tile_draws[t].start = uint32_t(total);
+        c->tile_draws[t].len = counts[t];
+        total += counts[t];
+    }
+    if (total > 0xFFFFFFFFull) return fail(TK_ERR_OVERFLOW, "tile draw references exceed u32");
+    c->tile_draw_indices.assign(total, 0u);
+    std::vector<uint32_t> cursor(tiles, 0u);
+    for (uint32_t d : draw_order) {
+        const TileBox &bb = boxes[d];
+        for (uint32_t y = bb.y0; y < bb.y1; ++y)
+            for (uint32_t x = bb.x0; x < bb.x1; ++x) {
+                size_t t = size_t(y) * tw + x;
+                c->tile_draw_indices[c->tile_draws[t].start + cursor[t]++] = d;
+            }
+    }
+
+    // PTCL bound over all batches: per batch, particles plus (2 depth + 1) per
+    // tile that can hold a particle; tiles outside any clip's bbox hold none.
+    uint64_t ptcl_capacity = 0;
+    for (size_t b = 0; b < batches.size(); ++b) {
+        TileBox clip{0, 0, tw, th};
+        for (uint32_t cd : batches[b].stack) {
+            TileBox cb = tile_bbox(c->draws[cd].pixel_bounds, tw, th);
+            clip.x0 = std::max(clip.x0, cb.x0);
+            clip.y0 = std::max(clip.y0, cb.y0);
+            clip.x1 = std::min(clip.x1, cb.x1);
+            clip.y1 = std::min(clip.y1, cb.y1);
+        }
+        uint64_t area = 0;
+        for (uint32_t d : batches[b].draws) {
+            TileBox bb = boxes[d];
+            bb.x0 = std::max(bb.x0, clip.x0);
+            bb.y0 = std::max(bb.y0, clip.y0);
+            bb.x1 = std::min(bb.x1, clip.x1);
+            bb.y1 = std::min(bb.y1, clip.y1);
+            area += bb.area();
+        }
+        uint64_t depth = batches[b].stack.size();
+        ptcl_capacity += area + (2 * depth + 1) * std::min<uint64_t>(clip.area(), area);
+    }
+
+    // Scan and cumsum chunk plans.
+    c->scan_chunks.clear();
+    c->scan_chunk_ranges.assign(c->paths.size(), tk_scan_chunk_range{0, 0});
+    c->cumsum_offsets.clear();
+    c->cumsum_lens.clear();
+    c->cumsum_row_starts.clear();
+    c->cumsum_row_ends.clear();
+    for (size_t p = 0; p < c->paths.size(); ++p) {
+        const tk_path_record &rec = c->paths[p];
+        c->scan_chunk_ranges[p].start = uint32_t(c->scan_chunks.size());
+        for (uint32_t local = 0; local < rec.data_len;) {
+            uint32_t len = std::min(rec.data_len - local, TK_SCAN_CHUNK_SIZE);
+            c->scan_chunks.push_back(tk_scan_chunk{rec.path_id, rec.data_offset + local, rec.segment_start, len});
+            local += len;
+        }
+        c->scan_chunk_ranges[p].end = uint32_t(c->scan_chunks.size());
+        uint32_t stride = rec.tile_x1 > rec.tile_x0 ? rec.tile_x1 - rec.tile_x0 : 0;
+        uint32_t height = rec.tile_y1 > rec.tile_y0 ? rec.tile_y1 - rec.tile_y0 : 0;
+        if (stride == 0 || height == 0) continue;
+        for (uint32_t row = 0; row < height; ++row) {
+            c->cumsum_row_starts.push_back(uint32_t(c->cumsum_offsets.size()));
+            uint32_t row_offset = rec.data_offset + row * stride;
+            for (uint32_t lx = 0; lx < stride;) {
+                uint32_t len = std::min(stride - lx, TK_CUMSUM_CHUNK_SIZE);
+                c->cumsum_offsets.push_back(row_offset + lx);
+                c->cumsum_lens.push_back(len);
+                lx += len;
+            }
+            c->cumsum_row_ends.push_back(uint32_t(c->cumsum_offsets.size()));
+        }
+    }
+    if (c->segment_total > 0xFFFFFFFFull) return fail(TK_ERR_OVERFLOW, "segment capacity exceeds u32");
+
+    // Paint blob = sdf || sdf_shadow || brush (src/render/upload/paint.rs:112-113).
+    c->paint.clear();
+    c->paint.insert(c->paint.end(), c->sdf_blob.begin(), c->sdf_blob.end());
+    c->paint.insert(c->paint.end(), c->shadow_blob.begin(), c->shadow_blob.end());
+    c->paint.insert(c->paint.end(), c->brush_blob.begin(), c->brush_blob.end());
+
+    std::memset(out, 0, sizeof(*out));
+    out->width = c->width;
+    out->height = c->height;
+    out->tiles_width = tw;
+    out->tiles_height = th;
+    out->clear_color = clear_color;
+    out->lines = c->lines.data();
+    out->line_count = uint32_t(c->lines.size());
+    out->paths = c->paths.data();
+    out->path_count = uint32_t(c->paths.size());
+    out->draws = c->draws.data();
+    out->draw_count = uint32_t(c->draws.size());
+    out->paint = c->paint.data();
+    out->paint_words = uint32_t(c->paint.size());
+    out->paint_sdf_shadow_base = uint32_t(c->sdf_blob.size());
+    out->paint_brush_base = uint32_t(c->sdf_blob.size() + c->shadow_blob.size());
+    out->draw_batch_ids = c->draw_batch_ids.data();
+    out->layers = c->layer_records.data();
+    out->layer_count = uint32_t(c->layer_records.size());
+    out->batches = c->batch_records.data();
+    out->batch_count = uint32_t(c->batch_records.size());
+    out->tile_draws = c->tile_draws.data();
+    out->tile_draw_indices = c->tile_draw_indices.data();
+    out->tile_draw_index_count = uint32_t(c->tile_draw_indices.size());
+    out->scan_chunks = c->scan_chunks.data();
+    out->scan_chunk_count = uint32_t(c->scan_chunks.size());
+    out->scan_chunk_ranges = c->scan_chunk_ranges.data();
+    out->cumsum_chunk_offsets = c->cumsum_offsets.data();
+    out->cumsum_chunk_lens = c->cumsum_lens.data();
+    out->cumsum_chunk_count = uint32_t(c->cumsum_offsets.size());
+    out->cumsum_row_starts = c->cumsum_row_starts.data();
+    out->cumsum_row_ends = c->cumsum_row_ends.data();
+    out->cumsum_row_count = uint32_t(c->cumsum_row_starts.size());
+    out->backdrop_len = c->backdrop_pool;
+    out->segment_capacity = uint32_t(c->segment_total);
+    out->max_clip_depth = max_depth;
+    out->ptcl_capacity = ptcl_capacity;
+    return TK_OK;
+}
+
+}  // namespace
+}  // namespace tk
+
+using namespace tk;
+
+extern "C" {
+
+tk_status tk_canvas_create(uint32_t width, uint32_t height, double scale, tk_canvas **out) {
+    if (!out) return fail(TK_ERR_INVALID, "null output");
+    if (width == 0 || height == 0 || !(scale > 0.0) || !std::isfinite(scale))
+        return fail(TK_ERR_INVALID, "canvas size and scale must be positive");
+    double pw = std::ceil(double(width) * scale), ph = std::ceil(double(height) * scale);
+    if (pw > 65536.0 || ph > 65536.0) return fail(TK_ERR_INVALID, "canvas exceeds 65536 physical pixels");
+    tk_canvas *c = new (std::nothrow) tk_canvas();
+    if (!c) return fail(TK_ERR_OOM, "canvas allocation failed");
+    c->width = uint32_t(pw);
+    c->height = uint32_t(ph);
+    c->scale = scale;
+    c->tiles_w = (c->width + TK_TILE_SIZE - 1) / TK_TILE_SIZE;
+    c->tiles_h = (c->height + TK_TILE_SIZE - 1) / TK_TILE_SIZE;
+    *out = c;
+    return TK_OK;
+}
+
+void tk_canvas_destroy(tk_canvas *canvas) { delete canvas; }
+
+uint32_t tk_canvas_line_count(const tk_canvas *canvas) { return canvas ? uint32_t(canvas->lines.size()) : 0u; }
+
+static tk_status push_path_common(tk_canvas *c, const uint8_t *verbs, size_t verb_count, const double *points,
+                                  size_t point_count, double stroke_width, bool stroke, const tk_brush *brush,
+                                  const double *transform, uint32_t fill_rule, double tolerance, uint32_t *draw_id) {
+    if (!c) return fail(TK_ERR_INVALID, "null canvas");
+    if (fill_rule > 1u) return fail(TK_ERR_INVALID, "fill rule must be 0 (nonzero) or 1 (even-odd)");
+    if (!(tolerance > 0.0)) return fail(TK_ERR_INVALID, "tolerance must be positive");
+    BezPath path;
+    if (verb_count > 0 && !parse_path(verbs, verb_count, points, point_count, &path))
+        return fail(TK_ERR_INVALID, "malformed path: verbs and points disagree or non-finite");
+    if (stroke) {
+        if (!(stroke_width >= 0.0) || !std::isfinite(stroke_width)) return fail(TK_ERR_INVALID, "invalid stroke width");
+        path = stroke_outline(path, stroke_width, tolerance);
+    }
+    double m[6];
+    device_matrix(c, transform, m);
+    BezPath physical = transform_path(path, m);
+    tk_status status;
+    uint32_t d = push_physical_path(c, physical, tolerance * c->scale, fill_rule, TK_DRAW_BRUSH, brush, true, &status);
+    if (status != TK_OK) return fail(status, "unsupported brush");
+    if (draw_id) *draw_id = d;
+    return TK_OK;
+}
+
+tk_status tk_canvas_push_path(tk_canvas *canvas, const uint8_t *verbs, size_t verb_count, const double *points,
+                              size_t point_count, const tk_brush *brush, const double *transform, uint32_t fill_rule,
+                              double tolerance, uint32_t *draw_id) {
+    return push_path_common(canvas, verbs, verb_count, points, point_count, 0.0, false, brush, transform, fill_rule,
+                            tolerance, draw_id);
+}
+
+tk_status tk_canvas_push_stroke(tk_canvas *canvas, const uint8_t *verbs, size_t verb_count, const double *points,
+                                size_t point_count, double width, const tk_brush *brush, const double *transform,
+                                uint32_t fill_rule, double tolerance, uint32_t *draw_id) {
+    return push_path_common(canvas, verbs, verb_count, points, point_count, width, true, brush, transform, fill_rule,
+                            tolerance, draw_id);
+}
+
+tk_status tk_canvas_push_rect(tk_canvas *canvas, double x0, double y0, double x1, double y1, const float radii[4],
+                              const tk_brush *brush, uint32_t *draw_id) {
+    if (!canvas) return fail(TK_ERR_INVALID, "null canvas");
+    if (!std::isfinite(x0) || !std::isfinite(y0) || !std::isfinite(x1) || !std::isfinite(y1))
+        return fail(TK_ERR_INVALID, "non-finite rectangle");
+    tk_status status;
+    uint32_t d = push_sdf_rect(canvas, x0, y0, x1, y1, radii, brush, TK_DRAW_BRUSH, true, &status);
+    if (status != TK_OK) return fail(status, "unsupported brush");
+    if (draw_id) *draw_id = d;
+    return TK_OK;
+}
+
+tk_status tk_canvas_push_clip_sdf_rect_layer(tk_canvas *canvas, double x0, double y0, double x1, double y1,
+                                             const float radii[4]) {
+    if (!canvas) return fail(TK_ERR_INVALID, "null canvas");
+    if (!std::isfinite(x0) || !std::isfinite(y0) || !std::isfinite(x1) || !std::isfinite(y1))
+        return fail(TK_ERR_INVALID, "non-finite rectangle");
+    tk_status status;
+    uint32_t d = push_sdf_rect(canvas, x0, y0, x1, y1, radii, nullptr, TK_DRAW_CLIP, false, &status);
+    if (status != TK_OK) return fail(status, "clip encoding failed");
+    open_clip_layer(canvas, d);
+    return TK_OK;
+}
+
+tk_status tk_canvas_push_clip_layer(tk_canvas *canvas, const uint8_t *verbs, size_t verb_count, const double *points,
+                                    size_t point_count, const double *transform, uint32_t fill_rule, double tolerance) {
+    if (!canvas) return fail(TK_ERR_INVALID, "null canvas");
+    if (fill_rule > 1u) return fail(TK_ERR_INVALID, "fill rule must be 0 or 1");
+    if (!(tolerance > 0.0)) return fail(TK_ERR_INVALID, "tolerance must be positive");
+    BezPath path;
+    if (verb_count > 0 && !parse_path(verbs, verb_count, points, point_count, &path))
+        return fail(TK_ERR_INVALID, "malformed path");
+    double m[6];
+    device_matrix(canvas, transform, m);
+    BezPath physical = transform_path(path, m);
+    double r[4];
+    if (axis_aligned_rect(physical, r)) {
+        bool aligned = true;
+        for (double v : r) aligned = aligned && std::fabs(v - std::round(v)) <= 1.0e-9;
+        if (aligned) {
+            tk_status status;
+            // physical coordinates: undo the scale push_sdf_rect applies
+            uint32_t d = push_sdf_rect(canvas, r[0] / canvas->scale, r[1] / canvas->scale, r[2] / canvas->scale,
+                                       r[3] / canvas->scale, nullptr, nullptr, TK_DRAW_CLIP, false, &status);
+            if (status != TK_OK) return fail(status, "clip encoding failed");
+            open_clip_layer(canvas, d);
+            return TK_OK;
+        }
+    }
+    tk_status status;
+    tk_brush transparent{};
+    transparent.kind = TK_BRUSH_SOLID;
+    uint32_t d = push_physical_path(canvas, physical, tolerance * canvas->scale, fill_rule, TK_DRAW_CLIP, &transparent,
+                                    false, &status);
+    if (status != TK_OK) return fail(status, "clip encoding failed");
+    open_clip_layer(canvas, d);
+    return TK_OK;
+}
+
+tk_status tk_canvas_pop_layer(tk_canvas *canvas) {
+    if (!canvas) return fail(TK_ERR_INVALID, "null canvas");
+    if (canvas->open.size() <= 1) return fail(TK_ERR_INVALID, "no open layer");
+    canvas->open.pop_back();
+    return TK_OK;
+}
+
+tk_status tk_canvas_encode(tk_canvas *canvas, uint32_t clear_color, tk_scene *out) {
+    if (!canvas || !out) return fail(TK_ERR_INVALID, "null argument");
+    return encode(canvas, clear_color, out);
+}
+
+uint32_t tk_line_scanned_tile_count(const tk_line *line, uint32_t tile_x0, uint32_t tile_y0, uint32_t tile_x1,
+                                    uint32_t tile_y1) {
+    if (!line) return 0;
+    return line_scanned_tile_count(*line, tile_x0, tile_y0, tile_x1, tile_y1);
+}
+
+}  // extern "C"

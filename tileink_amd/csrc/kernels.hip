@@ -1,0 +1,1033 @@
+// gfx950 kernels of the compute-raster pipeline.
+//
+// Built with: hipcc --genco --offload-arch=gfx950 -O3 -ffp-contract=off
+//             -Xclang -target-feature -Xclang -packed-fp32-ops
+// and loaded by the host through hipModuleLoadData.  Each kernel takes one
+// argument block (tk_kernel_args.h) by value.
+//
+// Stage map (reference entry -> kernel):
+//   scan_clear                       -> tk_k_scan_clear
+//   scan_count                       -> tk_k_scan_count
+//   scan_prefix_chunks + scan_chunk_offsets + scan_apply_chunk_offsets
+//                                    -> tk_k_scan_prefix (single pass, decoupled lookback)
+//   scan_emit                        -> tk_k_scan_emit (+ tk_k_segment_order: canonical slot order)
+//   cumsum_*                         -> tk_k_cumsum
+//   coarse_count_bins   (x batches)  -> tk_k_coarse_count  (all batches, one pass over each tile list)
+//   coarse_prefix_*     (x batches)  -> tk_k_coarse_scan   (one decoupled-lookback pass over batch x tile)
+//   coarse_emit_bins    (x batches)  -> tk_k_coarse_emit
+//   fine_tile_main      (x batches)  -> tk_k_fine          (all batches per 16x16 tile workgroup)
+
+#include <hip/hip_runtime.h>
+
+#include "raster_device.h"
+#include "tk_kernel_args.h"
+
+using namespace tkd;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Block-wide scan and decoupled lookback
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kFlagAggregate = 1u;
+constexpr uint32_t kFlagPrefix = 2u;
+
+// Exclusive wrapping-u32 scan over a 256-thread block (four 64-lane waves).
+__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t *s_wave, uint32_t &total) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63u) s_wave[wave] = x;
+    __syncthreads();
+    uint32_t before = 0u, sum = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < 4u; ++w) {
+        uint32_t s = s_wave[w];
+        if (w < wave) before += s;
+        sum += s;
+    }
+    total = sum;
+    __syncthreads();
+    return before + x - v;
+}
+
+__device__ __forceinline__ unsigned long long pack_status(uint32_t value, uint32_t flag, uint32_t epoch) {
+    return (unsigned long long)value | ((unsigned long long)flag << 32) | ((unsigned long long)(epoch & 0x3FFFFFFFu) << 34);
+}
+
+__device__ __forceinline__ void publish(unsigned long long *status, uint32_t index, uint32_t value, uint32_t flag,
+                                        uint32_t epoch) {
+    __hip_atomic_store(&status[index], pack_status(value, flag, epoch), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sums the published values of blocks [lo, k) walking backward until an
+// inclusive prefix is found.  Returns true when one was found.
+__device__ bool lookback(unsigned long long *status, uint32_t k, uint32_t lo, uint32_t epoch, uint32_t &acc) {
+    acc = 0u;
+    int64_t j = (int64_t)k - 1;
+    while (j >= (int64_t)lo) {
+        unsigned long long s = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t ep = (uint32_t)(s >> 34) & 0x3FFFFFFFu;
+        uint32_t flag = (uint32_t)(s >> 32) & 3u;
+        if (ep != (epoch & 0x3FFFFFFFu) || flag == 0u) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        acc += (uint32_t)s;
+        if (flag == kFlagPrefix) return true;
+        --j;
+    }
+    return false;
+}
+
+// Dynamic block order: ticket k is only handed out after tickets < k, so a
+// block only ever waits on blocks that are already resident.
+__device__ __forceinline__ uint32_t draw_ticket(uint32_t *ticket, uint32_t block_count, uint32_t *s_ticket) {
+    if (threadIdx.x == 0) {
+        uint32_t t = atomicAdd(ticket, 1u);
+        if (t == block_count - 1u) atomicExch(ticket, 0u);
+        *s_ticket = t;
+    }
+    __syncthreads();
+    return *s_ticket;
+}
+
+// ---------------------------------------------------------------------------
+// Coarse helpers (shared/draw.hlsli, coarse/{classify,sdf_clip,paint,emit_draw,emit_stack}.hlsli)
+// ---------------------------------------------------------------------------
+
+struct DrawData {
+    uint32_t path_id, glyph_run_id, sdf_offset, sdf_len, shadow_offset, tag, fill_rule, brush_offset, solid_rect;
+    int32_t pb[4];
+    float la, lb, lc, ld, te, tf;
+};
+
+__device__ __forceinline__ DrawData load_draw(const tk_draw_record *draws, uint32_t count, uint32_t index) {
+    DrawData d{};
+    if (index >= count) return d;  // robust read of an absent record: zeros
+    const tk_draw_record &r = draws[index];
+    d.path_id = r.path_id;
+    d.glyph_run_id = r.glyph_run_id;
+    d.sdf_offset = r.sdf_offset;
+    d.sdf_len = r.sdf_len;
+    d.shadow_offset = r.sdf_shadow_offset;
+    d.brush_offset = r.brush_offset;
+    d.solid_rect = r.solid_rect;
+    d.tag = r.tag;
+    d.fill_rule = r.fill_rule;
+    d.pb[0] = r.pixel_bounds.x0;
+    d.pb[1] = r.pixel_bounds.y0;
+    d.pb[2] = r.pixel_bounds.x1;
+    d.pb[3] = r.pixel_bounds.y1;
+    d.la = r.transform.a;
+    d.lb = r.transform.b;
+    d.lc = r.transform.c;
+    d.ld = r.transform.d;
+    d.te = r.transform.e;
+    d.tf = r.transform.f;
+    return d;
+}
+
+__device__ __forceinline__ bool draw_has_sdf(const DrawData &d) { return d.sdf_offset != kNone || d.shadow_offset != kNone; }
+__device__ __forceinline__ uint32_t tile_min_of(int32_t pixel, uint32_t limit) {
+    return pixel > 0 ? min((uint32_t)pixel / 16u, limit) : 0u;
+}
+__device__ __forceinline__ uint32_t tile_max_of(int32_t pixel, uint32_t limit) {
+    return pixel > 0 ? min(((uint32_t)pixel + 15u) / 16u, limit) : 0u;
+}
+__device__ __forceinline__ bool draw_hits_tile(const DrawData &d, uint32_t tx, uint32_t ty, uint32_t tw, uint32_t th) {
+    uint32_t lx = tile_min_of(d.pb[0], tw), ly = tile_min_of(d.pb[1], th);
+    uint32_t ux = tile_max_of(d.pb[2], tw), uy = tile_max_of(d.pb[3], th);
+    return tx >= lx && ty >= ly && tx < ux && ty < uy;
+}
+__device__ __forceinline__ uint32_t draw_backdrop(const tk_path_record *paths, uint32_t path_count, const DrawData &d,
+                                                  uint32_t tx, uint32_t ty, uint32_t tw, uint32_t th) {
+    if (d.path_id == kNone || d.tag > TK_DRAW_PATH_GLYPH || d.path_id >= path_count || !draw_hits_tile(d, tx, ty, tw, th))
+        return kNone;
+    const tk_path_record &p = paths[d.path_id];
+    if (p.tile_x1 <= p.tile_x0 || tx < p.tile_x0 || ty < p.tile_y0 || tx >= p.tile_x1 || ty >= p.tile_y1) return kNone;
+    return p.data_offset + (ty - p.tile_y0) * (p.tile_x1 - p.tile_x0) + tx - p.tile_x0;
+}
+
+__device__ __forceinline__ bool rounded_corner_covers(float px, float py, float cx, float cy, float ox, float oy,
+                                                      float radius, float inset) {
+    bool in_square = radius > 0.0f && fabsf(px - cx) < radius && fabsf(py - cy) < radius;
+    float inner = radius - inset;
+    float dx = px - ox, dy = py - oy;
+    return !in_square || (inner > 0.0f && dx * dx + dy * dy <= inner * inner);
+}
+
+__device__ bool rect_covers_tile(const float rect[4], const float radii_in[4], const DrawData &d, uint32_t tx,
+                                 uint32_t ty, float inset) {
+    float lx = fminf(rect[0], rect[2]), ly = fminf(rect[1], rect[3]);
+    float ux = fmaxf(rect[0], rect[2]), uy = fmaxf(rect[1], rect[3]);
+    float tlx = (float)(tx * 16u) + 0.5f - d.te, tly = (float)(ty * 16u) + 0.5f - d.tf;
+    float tux = tlx + 15.0f, tuy = tly + 15.0f;
+    float sx = ux - lx, sy = uy - ly;
+    float need = 15.0f + 2.0f * inset;
+    if (sx < need || sy < need || tlx < lx + inset || tly < ly + inset || tux > ux - inset || tuy > uy - inset)
+        return false;
+    float limit = fminf(sx, sy) * 0.5f;
+    for (int i = 0; i < 4; ++i)
+        if (!(fabsf(rect[i]) <= 65536.0f) || !(fabsf(radii_in[i]) <= 65536.0f)) return false;
+    if (!(fabsf(d.te) <= 65536.0f) || !(fabsf(d.tf) <= 65536.0f)) return false;
+    float r[4];
+    for (int i = 0; i < 4; ++i) r[i] = fminf(fmaxf(radii_in[i], 0.0f), limit);
+    return rounded_corner_covers(tlx, tly, lx, ly, lx + r[0], ly + r[0], r[0], inset) &&
+           rounded_corner_covers(tux, tly, ux, ly, ux - r[1], ly + r[1], r[1], inset) &&
+           rounded_corner_covers(tlx, tuy, lx, uy, lx + r[2], uy - r[2], r[2], inset) &&
+           rounded_corner_covers(tux, tuy, ux, uy, ux - r[3], uy - r[3], r[3], inset);
+}
+
+__device__ __forceinline__ bool identity_linear(const DrawData &d) {
+    return d.la == 1.0f && d.lb == 0.0f && d.lc == 0.0f && d.ld == 1.0f;
+}
+
+__device__ bool sdf_clip_covers(const uint32_t *paint, uint32_t pc, const DrawData &d, uint32_t tx, uint32_t ty) {
+    if (d.sdf_offset == kNone || d.shadow_offset != kNone || d.sdf_len < 9u) return false;
+    uint32_t base = d.sdf_offset;
+    if (paint_word(paint, pc, base) != TK_SDF_KIND_RECT || !identity_linear(d)) return false;
+    float rect[4], radii[4];
+    for (int i = 0; i < 4; ++i) {
+        rect[i] = asfloat(paint_word(paint, pc, base + 1u + (uint32_t)i));
+        radii[i] = asfloat(paint_word(paint, pc, base + 5u + (uint32_t)i));
+    }
+    return rect_covers_tile(rect, radii, d, tx, ty, 0.5f);
+}
+
+__device__ bool sdf_stroke_empty(const uint32_t *paint, uint32_t pc, const DrawData &d, uint32_t tx, uint32_t ty) {
+    if (d.sdf_offset == kNone || d.shadow_offset != kNone || d.sdf_len < 13u || !identity_linear(d)) return false;
+    uint32_t base = d.sdf_offset;
+    if (paint_word(paint, pc, base) != 3u) return false;  // SDF_KIND_RECT_STROKE
+    float rect[4], radii[4], stroke[4];
+    for (int i = 0; i < 4; ++i) {
+        rect[i] = asfloat(paint_word(paint, pc, base + 1u + (uint32_t)i));
+        radii[i] = asfloat(paint_word(paint, pc, base + 5u + (uint32_t)i));
+        stroke[i] = asfloat(paint_word(paint, pc, base + 9u + (uint32_t)i));
+    }
+    for (int i = 0; i < 4; ++i)
+        if (!(fabsf(rect[i]) <= 65536.0f) || !(fabsf(radii[i]) <= 65536.0f) || !(fabsf(stroke[i]) <= 65536.0f))
+            return false;
+    float hw[4];
+    for (int i = 0; i < 4; ++i) hw[i] = fmaxf(stroke[i], 0.0f);
+    float lx = fminf(rect[0], rect[2]), ly = fminf(rect[1], rect[3]);
+    float ux = fmaxf(rect[0], rect[2]), uy = fmaxf(rect[1], rect[3]);
+    float inner[4] = {lx + hw[3], ly + hw[0], ux - hw[1], uy - hw[2]};
+    if (inner[0] >= inner[2] || inner[1] >= inner[3]) return false;
+    float corner[4] = {fmaxf(hw[0], hw[3]), fmaxf(hw[0], hw[1]), fmaxf(hw[2], hw[3]), fmaxf(hw[2], hw[1])};
+    float r[4];
+    for (int i = 0; i < 4; ++i) r[i] = fmaxf(radii[i] - corner[i], 0.0f);
+    return rect_covers_tile(inner, r, d, tx, ty, 1.0f);
+}
+
+__device__ __forceinline__ uint32_t brush_base_of(uint32_t paint_brush_base, const DrawData &d) {
+    return paint_brush_base + d.brush_offset;
+}
+__device__ __forceinline__ bool nontransparent_solid(const uint32_t *paint, uint32_t pc, uint32_t pbb, const DrawData &d) {
+    if (d.brush_offset == kNone) return false;
+    uint32_t base = brush_base_of(pbb, d);
+    return paint_word(paint, pc, base) == TK_BRUSH_SOLID && paint_word(paint, pc, base + 4u) != 0u;
+}
+__device__ __forceinline__ uint32_t solid_color(const uint32_t *paint, uint32_t pc, uint32_t pbb, const DrawData &d) {
+    return paint_word(paint, pc, brush_base_of(pbb, d) + 4u);
+}
+__device__ bool solid_supported_sdf(const uint32_t *paint, uint32_t pc, uint32_t pbb, const DrawData &d) {
+    if (!nontransparent_solid(paint, pc, pbb, d) || d.sdf_offset == kNone || d.shadow_offset != kNone || d.sdf_len == 0u)
+        return false;
+    uint32_t kind = paint_word(paint, pc, d.sdf_offset);
+    return kind == TK_SDF_KIND_RECT || kind == 5u || kind == 14u;
+}
+__device__ __forceinline__ uint32_t full_tile_solid(const uint32_t *paint, uint32_t pc, uint32_t pbb, const DrawData &d,
+                                                    uint32_t tx, uint32_t ty) {
+    return nontransparent_solid(paint, pc, pbb, d) && sdf_clip_covers(paint, pc, d, tx, ty) ? solid_color(paint, pc, pbb, d)
+                                                                                            : 0u;
+}
+__device__ __forceinline__ bool full_tile_image(const uint32_t *paint, uint32_t pc, uint32_t pbb, const DrawData &d,
+                                                uint32_t tx, uint32_t ty) {
+    if (d.brush_offset == kNone) return false;
+    uint32_t base = brush_base_of(pbb, d);
+    return paint_word(paint, pc, base) == 7u && paint_word(paint, pc, base + 7u) == 255u &&
+           sdf_clip_covers(paint, pc, d, tx, ty);
+}
+
+constexpr uint32_t kClassColor = 1u, kClassSdf = 2u, kClassOther = 4u;
+constexpr uint32_t kPtclImage = 13u, kPtclPathGlyph = 11u;
+constexpr uint32_t kPtclBeginOpacity = 5u, kPtclEndOpacity = 6u, kPtclBeginBlend = 7u, kPtclEndBlend = 8u;
+
+__device__ __forceinline__ uint32_t classify_tile_flags(uint32_t flags) {
+    if (flags & kClassOther) return TK_TILE_KIND_INTERPRETER;
+    if (flags & kClassSdf) return (flags & kClassColor) ? TK_TILE_KIND_MIXED : TK_TILE_KIND_SDF;
+    return (flags & kClassColor) ? TK_TILE_KIND_COLOR : TK_TILE_KIND_EMPTY;
+}
+
+struct CoarseCtx {
+    const tk_coarse_config *cfg;
+    const tk_coarse_inputs *in;
+    uint32_t tx, ty;
+};
+
+// stack_wrapper_count (coarse/classify.hlsli:11-33).
+__device__ uint32_t stack_wrapper_count(const CoarseCtx &c, uint32_t batch) {
+    const tk_coarse_config &cfg = *c.cfg;
+    const tk_coarse_inputs &in = *c.in;
+    tk_batch_record br = in.batches[batch];
+    uint32_t count = 0u;
+    for (uint32_t i = br.layer_start; i < br.layer_end && i < cfg.layer_count; ++i) {
+        tk_layer_record layer = in.layers[i];
+        DrawData d = load_draw(in.draws, cfg.draw_count, layer.draw);
+        if (draw_has_sdf(d)) {
+            if (sdf_clip_covers(in.paint, cfg.paint_words, d, c.tx, c.ty)) continue;
+            if (!draw_hits_tile(d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height)) return kNone;
+        } else {
+            uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
+            if (bd == kNone) return kNone;
+            tk_segment_range range = in.segment_ranges[bd];
+            int32_t winding = in.backdrops[bd];
+            bool full = d.fill_rule == TK_FILL_EVEN_ODD ? (((uint32_t)winding & 1u) != 0u) : winding != 0;
+            if (layer.tag == TK_LAYER_CLIP && range.start == range.end && full) continue;
+            if (range.start == range.end && winding == 0) return kNone;
+        }
+        count++;
+    }
+    return count;
+}
+
+struct Particle {
+    bool valid;
+    uint32_t tag, winding, fill_rule, seg_start, seg_end, color;
+};
+
+// draw_particle (coarse/emit_draw.hlsli:10-52); draw_particle_count is its validity.
+__device__ Particle draw_particle(const CoarseCtx &c, uint32_t batch, uint32_t draw_index) {
+    const tk_coarse_config &cfg = *c.cfg;
+    const tk_coarse_inputs &in = *c.in;
+    Particle p{false, TK_PTCL_FILL, 0u, 0u, 0u, 0u, 0u};
+    if (draw_index >= cfg.draw_count || in.draw_batch_ids[draw_index] != batch) return p;
+    DrawData d = load_draw(in.draws, cfg.draw_count, draw_index);
+    if (draw_has_sdf(d)) {
+        if (d.tag == TK_DRAW_BRUSH && !sdf_stroke_empty(in.paint, cfg.paint_words, d, c.tx, c.ty)) {
+            p.valid = true;
+            uint32_t color = full_tile_solid(in.paint, cfg.paint_words, cfg.paint_brush_base, d, c.tx, c.ty);
+            if (color != 0u) {
+                p.tag = TK_PTCL_COLOR;
+                p.color = color;
+            } else if (full_tile_image(in.paint, cfg.paint_words, cfg.paint_brush_base, d, c.tx, c.ty)) {
+                p.tag = kPtclImage;
+                p.color = draw_index;
+            } else {
+                p.tag = TK_PTCL_SDF;
+                p.seg_start = draw_index;
+                p.color = draw_index;
+            }
+        }
+        return p;
+    }
+    uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
+    if (bd == kNone || (d.tag != TK_DRAW_BRUSH && d.tag != TK_DRAW_PATH_GLYPH && d.tag != TK_DRAW_CLIP)) return p;
+    tk_segment_range range = in.segment_ranges[bd];
+    uint32_t winding = (uint32_t)in.backdrops[bd];
+    if (range.start == range.end && winding == 0u) return p;
+    p.valid = true;
+    p.winding = winding;
+    p.fill_rule = d.fill_rule;
+    p.seg_start = range.start;
+    p.seg_end = range.end;
+    if (d.tag == TK_DRAW_CLIP) p.tag = TK_PTCL_BEGIN_CLIP;
+    else if (d.tag == TK_DRAW_PATH_GLYPH) p.tag = kPtclPathGlyph;
+    else if (d.solid_rect != 0u && d.lb == 0.0f && d.lc == 0.0f &&
+             nontransparent_solid(in.paint, cfg.paint_words, cfg.paint_brush_base, d) && range.start == range.end)
+        p.tag = TK_PTCL_COLOR;
+    if (p.tag == TK_PTCL_COLOR) p.color = solid_color(in.paint, cfg.paint_words, cfg.paint_brush_base, d);
+    else if (d.tag == TK_DRAW_BRUSH || d.tag == TK_DRAW_PATH_GLYPH) p.color = draw_index;
+    return p;
+}
+
+__device__ __forceinline__ uint32_t particle_class_flags(const CoarseCtx &c, uint32_t draw_index, uint32_t tag) {
+    if (tag == TK_PTCL_COLOR) return kClassColor;
+    if (tag == kPtclImage) return kClassSdf;
+    if (tag == TK_PTCL_SDF) {
+        DrawData d = load_draw(c.in->draws, c.cfg->draw_count, draw_index);
+        if (solid_supported_sdf(c.in->paint, c.cfg->paint_words, c.cfg->paint_brush_base, d)) return kClassSdf;
+    }
+    return kClassOther;
+}
+
+// noop_stack_clip (coarse/emit_stack.hlsli:9-17).
+__device__ bool noop_stack_clip(const CoarseCtx &c, const tk_layer_record &layer, const DrawData &d) {
+    if (layer.tag != TK_LAYER_CLIP) return false;
+    if (draw_has_sdf(d)) return sdf_clip_covers(c.in->paint, c.cfg->paint_words, d, c.tx, c.ty);
+    uint32_t bd = draw_backdrop(c.in->paths, c.cfg->path_count, d, c.tx, c.ty, c.cfg->tiles_width, c.cfg->tiles_height);
+    if (bd == kNone) return false;  // excluded by stack_wrapper_count before any emission
+    tk_segment_range range = c.in->segment_ranges[bd];
+    uint32_t winding = (uint32_t)c.in->backdrops[bd];
+    return range.start == range.end && (d.fill_rule == TK_FILL_EVEN_ODD ? (winding & 1u) != 0u : winding != 0u);
+}
+
+struct Emitter {
+    tk_ptcl_record *ptcl;
+    uint64_t capacity;
+    uint32_t *status;
+    __device__ void store(uint32_t dst, uint32_t tag, uint32_t winding, uint32_t fill_rule, uint32_t s0, uint32_t s1,
+                          uint32_t color) {
+        if ((uint64_t)dst >= capacity) {
+            atomicOr(&status[0], 1u);
+            return;
+        }
+        tk_ptcl_record r;
+        r.tag = tag;
+        r.backdrop = (int32_t)winding;
+        r.fill_rule = fill_rule;
+        r.segment_start = s0;
+        r.segment_end = s1;
+        r.color = color;
+        ptcl[dst] = r;
+    }
+};
+
+__device__ uint32_t emit_stack_begins(const CoarseCtx &c, uint32_t batch, Emitter &e, uint32_t dst) {
+    tk_batch_record br = c.in->batches[batch];
+    for (uint32_t i = br.layer_start; i < br.layer_end && i < c.cfg->layer_count; ++i) {
+        tk_layer_record layer = c.in->layers[i];
+        DrawData d = load_draw(c.in->draws, c.cfg->draw_count, layer.draw);
+        if (noop_stack_clip(c, layer, d)) continue;
+        if (draw_has_sdf(d)) {
+            e.store(dst++, TK_PTCL_BEGIN_SDF_CLIP, 0u, 0u, 0u, 0u, layer.draw);
+        } else {
+            uint32_t bd = draw_backdrop(c.in->paths, c.cfg->path_count, d, c.tx, c.ty, c.cfg->tiles_width,
+                                        c.cfg->tiles_height);
+            uint32_t tag = layer.tag == 1u ? kPtclBeginOpacity : (layer.tag == 2u ? kPtclBeginBlend : TK_PTCL_BEGIN_CLIP);
+            tk_segment_range range{0u, 0u};
+            uint32_t winding = 0u;
+            if (bd != kNone) {
+                range = c.in->segment_ranges[bd];
+                winding = (uint32_t)c.in->backdrops[bd];
+            }
+            e.store(dst++, tag, winding, d.fill_rule, range.start, range.end, layer.payload);
+        }
+    }
+    return dst;
+}
+
+__device__ uint32_t emit_stack_ends(const CoarseCtx &c, uint32_t batch, Emitter &e, uint32_t dst) {
+    tk_batch_record br = c.in->batches[batch];
+    uint32_t end = min(br.layer_end, c.cfg->layer_count);
+    for (uint32_t i = end; i > br.layer_start;) {
+        --i;
+        tk_layer_record layer = c.in->layers[i];
+        DrawData d = load_draw(c.in->draws, c.cfg->draw_count, layer.draw);
+        if (noop_stack_clip(c, layer, d)) continue;
+        uint32_t tag = layer.tag == 1u ? kPtclEndOpacity : (layer.tag == 2u ? kPtclEndBlend : TK_PTCL_END_CLIP);
+        e.store(dst++, tag, 0u, 0u, 0u, 0u, 0u);
+    }
+    return dst;
+}
+
+__device__ __forceinline__ bool is_begin_tag(uint32_t tag) {
+    return tag == TK_PTCL_BEGIN_CLIP || tag == TK_PTCL_BEGIN_SDF_CLIP || tag == kPtclBeginOpacity || tag == kPtclBeginBlend;
+}
+
+// ---------------------------------------------------------------------------
+// Fine helpers (fine/{interpreter,specialized,clip,draw,records}.hlsli)
+// ---------------------------------------------------------------------------
+
+struct FineDraw {
+    uint32_t brush_offset, sdf_offset, shadow_offset;
+    int32_t pb[4];
+    Affine inv;
+};
+
+__device__ __forceinline__ FineDraw load_fine_draw(const tk_draw_record *draws, uint32_t count, uint32_t index) {
+    FineDraw f{};
+    if (index >= count) return f;
+    const tk_draw_record &r = draws[index];
+    f.brush_offset = r.brush_offset;
+    f.sdf_offset = r.sdf_offset;
+    f.shadow_offset = r.sdf_shadow_offset;
+    f.pb[0] = r.pixel_bounds.x0;
+    f.pb[1] = r.pixel_bounds.y0;
+    f.pb[2] = r.pixel_bounds.x1;
+    f.pb[3] = r.pixel_bounds.y1;
+    f.inv = Affine{r.inverse_transform.a, r.inverse_transform.b, r.inverse_transform.c,
+                   r.inverse_transform.d, r.inverse_transform.e, r.inverse_transform.f};
+    return f;
+}
+
+__device__ __forceinline__ uint32_t sample_draw_brush(const tk_fine_config &cfg, const uint32_t *paint,
+                                                      const FineDraw &d, float x, float y) {
+    if (blob_word(paint, cfg.paint_words, cfg.paint_brush_base, d.brush_offset) == TK_BRUSH_SOLID)
+        return blob_word(paint, cfg.paint_words, cfg.paint_brush_base, d.brush_offset + 4u);
+    float lx, ly;
+    affine_point(d.inv, x, y, lx, ly);
+    return sample_brush(paint, cfg.paint_words, cfg.paint_brush_base, d.brush_offset, lx, ly);
+}
+
+__device__ __forceinline__ float sdf_coverage_from_draw(const tk_fine_config &cfg, const uint32_t *paint,
+                                                        const FineDraw &d, float x, float y) {
+    float lx, ly;
+    affine_point(d.inv, x, y, lx, ly);
+    if (d.sdf_offset != kNone) return sdf_coverage_from_blob(paint, cfg.paint_words, d.sdf_offset, lx, ly, d.inv);
+    if (d.shadow_offset != kNone)
+        return sdf_coverage_from_blob(paint, cfg.paint_words, cfg.paint_sdf_shadow_base + d.shadow_offset, lx, ly, d.inv);
+    return 0.0f;
+}
+
+__device__ __forceinline__ bool premul_opaque(uint32_t c) { return (c >> 24) == 255u; }
+
+__device__ __forceinline__ float fill_coverage(const tk_line_segment *segments, int32_t backdrop, uint32_t start,
+                                               uint32_t end, uint32_t x, uint32_t y) {
+    float base = (float)backdrop, running = 0.0f, partial = 0.0f;
+    for (uint32_t i = start; i < end; ++i) accumulate_segment(segments[i], x, y, base, running, partial);
+    return base + running + partial;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Kernels
+// ===========================================================================
+
+extern "C" __global__ __launch_bounds__(256) void tk_k_fill(tk_args_fill a) {
+    uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * 256u;
+    for (; i < a.count; i += stride) a.dst[i] = a.value;
+}
+
+// scan_clear (scan/clear.hlsl:16-32), non-incremental.
+extern "C" __global__ __launch_bounds__(256) void tk_k_scan_clear(tk_args_scan_clear a) {
+    uint32_t index = blockIdx.x * 256u + threadIdx.x;
+    if (index >= a.cfg.clear_len) return;
+    if (index < a.cfg.backdrop_len) {
+        if (a.backdrops) a.backdrops[index] = 0;
+        if (a.ranges) a.ranges[index] = tk_segment_range{0u, 0u};
+        if (a.counts) a.counts[index] = 0u;
+        if (a.cursors) a.cursors[index] = 0u;
+    }
+    if (index < a.cfg.path_count && a.bumps) a.bumps[index] = 0u;
+    if (index < a.cfg.scan_chunk_count) {
+        if (a.chunk_totals) a.chunk_totals[index] = 0u;
+        if (a.chunk_offsets) a.chunk_offsets[index] = 0u;
+    }
+}
+
+// scan_count (scan/count.hlsl:13-40).
+extern "C" __global__ __launch_bounds__(256) void tk_k_scan_count(tk_args_scan_lines a) {
+    uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= a.cfg.line_count) return;
+    ScanTraversal s;
+    if (!scan_geometry(a.lines, a.paths, a.path_record_count, id, s)) return;
+    int32_t delta = s.down ? -1 : 1;
+    uint32_t stride = s.bz - s.bx;
+    for (int32_t y = s.ymin; y < s.ymax; y++) {
+        uint32_t local = (uint32_t)(y - (int32_t)s.by) * stride;
+        atomicAdd(&a.backdrops[s.data_offset + local], delta);
+    }
+    float last_z = floorf(s.a * ((float)s.imin - 1.0f) + s.b);
+    for (uint32_t i = s.imin; i < s.imax; i++) {
+        float z = floorf(s.a * (float)i + s.b);
+        int32_t tx, ty;
+        scan_tile(s, i, z, tx, ty);
+        if (scan_tile_inside(s, tx, ty)) {
+            bool top = last_z == z;
+            if (i == 0u) top = fabsf(s.y0 - s.py0 * (1.0f / 16.0f)) <= kScanEpsilon;
+            if (top && tx + 1 < (int32_t)s.bz) {
+                int32_t x = max(tx + 1, (int32_t)s.bx);
+                uint32_t local = (uint32_t)(ty - (int32_t)s.by) * stride + (uint32_t)(x - (int32_t)s.bx);
+                atomicAdd(&a.backdrops[s.data_offset + local], delta);
+            }
+            atomicAdd(&a.counts_or_cursors[scan_tile_offset(s, tx, ty)], 1u);
+        }
+        last_z = z;
+    }
+}
+
+// scan_prefix_chunks + scan_chunk_offsets + scan_apply_chunk_offsets as one
+// pass: each workgroup scans one 256-tile chunk, then finds its path-relative
+// base by decoupled lookback over the path's earlier chunks.
+extern "C" __global__ __launch_bounds__(256) void tk_k_scan_prefix(tk_args_scan_prefix a) {
+    __shared__ uint32_t s_ticket, s_base;
+    __shared__ uint32_t s_wave[4];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t k = draw_ticket(a.ticket, a.block_count, &s_ticket);
+
+    // Paths without chunks still receive a zero bump (scan_chunk_offsets runs per path).
+    uint32_t p = k * 256u + lane;
+    if (p < a.cfg.path_count) {
+        tk_scan_chunk_range r = a.chunk_ranges[p];
+        if (r.start >= r.end) a.bumps[p] = 0u;
+    }
+    if (k >= a.cfg.scan_chunk_count) return;
+
+    tk_scan_chunk chunk = a.chunks[k];
+    uint32_t len = chunk.len;
+    uint32_t count = lane < len ? a.counts[chunk.backdrop_offset + lane] : 0u;
+    uint32_t total;
+    uint32_t local_excl = block_exclusive_scan_256(count, s_wave, total);
+
+    if (lane == 0u) {
+        uint32_t path = chunk.path_id;
+        bool in_range = false;
+        tk_scan_chunk_range r{0u, 0u};
+        uint32_t seg_start = 0u;
+        if (path < a.cfg.path_count) {
+            r = a.chunk_ranges[path];
+            in_range = k >= r.start && k < r.end;
+            seg_start = a.paths[path].segment_start;
+        }
+        uint32_t base;
+        if (in_range) {
+            if (k == r.start) {
+                base = seg_start;
+            } else {
+                publish(a.status, k, total, kFlagAggregate, a.epoch);
+                uint32_t acc;
+                bool found = lookback(a.status, k, r.start, a.epoch, acc);
+                base = found ? acc : acc + seg_start;
+            }
+            publish(a.status, k, base + total, kFlagPrefix, a.epoch);
+            a.chunk_offsets[k] = base;
+            if (k + 1u == r.end) a.bumps[path] = base + total - seg_start;
+        } else {
+            // Not reached by any path's offset pass: the apply pass reads the current offset word.
+            base = a.chunk_offsets[k];
+            publish(a.status, k, base + total, kFlagPrefix, a.epoch);
+        }
+        s_base = base;
+    }
+    __syncthreads();
+    if (lane < len) {
+        uint32_t start = s_base + local_excl;
+        a.ranges[chunk.backdrop_offset + lane] = tk_segment_range{start, start + count};
+        a.cursors[chunk.backdrop_offset + lane] = start;
+    }
+    if (lane + 1u == len) a.chunk_totals[k] = total;
+}
+
+// scan_emit (scan/emit.hlsl:14-28).
+extern "C" __global__ __launch_bounds__(256) void tk_k_scan_emit(tk_args_scan_lines a) {
+    uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= a.cfg.line_count) return;
+    ScanTraversal s;
+    if (!scan_geometry(a.lines, a.paths, a.path_record_count, id, s)) return;
+    for (uint32_t i = s.imin; i < s.imax; i++) {
+        float z = floorf(s.a * (float)i + s.b);
+        int32_t tx, ty;
+        scan_tile(s, i, z, tx, ty);
+        if (scan_tile_inside(s, tx, ty)) {
+            uint32_t dst = atomicAdd(&a.counts_or_cursors[scan_tile_offset(s, tx, ty)], 1u);
+            if (dst < a.cfg.segment_capacity) a.segments[dst] = clipped_segment(s, i, z, tx, ty);
+        }
+    }
+}
+
+namespace {
+__device__ __forceinline__ bool segment_less(const tk_line_segment &x, const tk_line_segment &y) {
+    uint32_t a[5] = {asuint(x.p0x), asuint(x.p0y), asuint(x.p1x), asuint(x.p1y), asuint(x.y_edge)};
+    uint32_t b[5] = {asuint(y.p0x), asuint(y.p0y), asuint(y.p1x), asuint(y.p1y), asuint(y.y_edge)};
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+        if (a[i] != b[i]) return a[i] < b[i];
+    return false;
+}
+}  // namespace
+
+// Canonical order of each backdrop tile's segments: emit claims slots with
+// atomics, so the stored order varies run to run; sorting every tile's slots by
+// the segments' bit patterns makes the stored order (and hence the fine stage's
+// summation order) a function of the geometry alone.
+extern "C" __global__ __launch_bounds__(256) void tk_k_segment_order(tk_args_segment_sort a) {
+    uint32_t b = blockIdx.x * 256u + threadIdx.x;
+    if (b >= a.backdrop_len) return;
+    tk_segment_range r = a.ranges[b];
+    uint32_t start = r.start, end = min(r.end, a.segment_capacity);
+    if (end <= start || end - start < 2u) return;
+    tk_line_segment *seg = a.segments;
+    for (uint32_t i = start + 1u; i < end; ++i) {
+        tk_line_segment cur = seg[i];
+        uint32_t j = i;
+        while (j > start) {
+            tk_line_segment prev = seg[j - 1u];
+            if (!segment_less(cur, prev)) break;
+            seg[j] = prev;
+            --j;
+        }
+        seg[j] = cur;
+    }
+}
+
+// cumsum_prefix_chunks + cumsum_chunk_offsets + cumsum_apply_chunk_offsets
+// (cumsum.hlsl:16-71): one thread carries one backdrop row across its chunks.
+extern "C" __global__ __launch_bounds__(256) void tk_k_cumsum(tk_args_cumsum a) {
+    uint32_t row = blockIdx.x * 256u + threadIdx.x;
+    if (row >= a.cfg.row_count) return;
+    uint32_t carry = 0u;
+    uint32_t end = a.row_ends[row];
+    for (uint32_t c = a.row_starts[row]; c < end && c < a.cfg.chunk_count; ++c) {
+        uint32_t off = a.chunk_offsets_in[c];
+        uint32_t len = min(a.chunk_lens[c], (uint32_t)TK_CUMSUM_CHUNK_SIZE);
+        if (a.chunk_carries) a.chunk_carries[c] = carry;
+        uint32_t sum = 0u;
+        for (uint32_t i = 0; i < len; ++i) {
+            sum += (uint32_t)a.backdrops[off + i];
+            a.backdrops[off + i] = (int32_t)(carry + sum);
+        }
+        if (a.chunk_totals) a.chunk_totals[c] = sum;
+        carry += sum;
+    }
+}
+
+// coarse_count_bins for every batch: one lane per tile walks the tile's
+// painter-ordered draw list once; batches appear as contiguous runs.
+extern "C" __global__ __launch_bounds__(64) void tk_k_coarse_count(tk_args_coarse a) {
+    const tk_coarse_config &cfg = a.cfg;
+    uint32_t tile = blockIdx.x * 64u + threadIdx.x;
+    if (tile >= cfg.tile_count) return;
+    CoarseCtx c{&a.cfg, &a.in, tile % cfg.tiles_width, tile / cfg.tiles_width};
+    const uint32_t T = cfg.tile_count, Q = cfg.batch_count;
+    uint32_t *counts = a.out.tile_ptcl_counts;
+    tk_tile_draw_record list = a.in.tile_draws[tile];
+    uint32_t cur = kNone, particles = 0u, next = 0u;
+    auto flush = [&]() {
+        if (cur == kNone) return;
+        uint32_t count = 0u;
+        if (particles > 0u) {
+            uint32_t w = stack_wrapper_count(c, cur);
+            if (w != kNone) count = particles + 2u * w + 1u;
+        }
+        counts[(size_t)cur * T + tile] = count;
+        next = cur + 1u;
+    };
+    if (list.start != kNone) {
+        for (uint32_t i = 0; i < list.len; ++i) {
+            uint32_t slot = list.start + i;
+            if (slot >= cfg.tile_draw_index_count) break;
+            uint32_t d = a.in.tile_draw_indices[slot];
+            uint32_t b = d < cfg.draw_count ? a.in.draw_batch_ids[d] : kNone;
+            if (b >= Q) continue;
+            if (b != cur) {
+                if (cur != kNone && b < cur) {
+                    atomicOr(&a.out.status[3], 1u);  // list not in batch order
+                    continue;
+                }
+                flush();
+                for (uint32_t bb = next; bb < b; ++bb) counts[(size_t)bb * T + tile] = 0u;
+                cur = b;
+                particles = 0u;
+            }
+            if (draw_particle(c, b, d).valid) particles++;
+        }
+    }
+    flush();
+    for (uint32_t bb = next; bb < Q; ++bb) counts[(size_t)bb * T + tile] = 0u;
+}
+
+// coarse_prefix_chunks + coarse_chunk_offsets + coarse_apply_chunk_offsets for
+// every batch: one wrapping exclusive scan over the batch-major count table.
+// A batch's reference (per-batch) offsets are start - batch_ptcl_bases[batch].
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coarse a) {
+    __shared__ uint32_t s_ticket, s_base;
+    __shared__ uint32_t s_wave[4];
+    constexpr uint32_t kItems = 4u;
+    const uint32_t k = draw_ticket(a.ticket, a.scan_blocks, &s_ticket);
+    const uint64_t n = (uint64_t)a.cfg.batch_count * a.cfg.tile_count;
+    const uint64_t first = ((uint64_t)k * 256u + threadIdx.x) * kItems;
+    uint32_t v[kItems];
+    uint32_t sum = 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < kItems; ++i) {
+        v[i] = first + i < n ? a.out.tile_ptcl_counts[first + i] : 0u;
+        sum += v[i];
+    }
+    uint32_t total;
+    uint32_t excl = block_exclusive_scan_256(sum, s_wave, total);
+    if (threadIdx.x == 0u) {
+        uint32_t base = 0u;
+        if (k > 0u) {
+            publish(a.scan_status, k, total, kFlagAggregate, a.epoch);
+            uint32_t acc;
+            lookback(a.scan_status, k, 0u, a.epoch, acc);
+            base = acc;
+        }
+        publish(a.scan_status, k, base + total, kFlagPrefix, a.epoch);
+        s_base = base;
+        if (k + 1u == a.scan_blocks) {
+            a.out.batch_ptcl_bases[a.cfg.batch_count] = base + total;
+            a.out.status[1] = base + total;
+        }
+    }
+    __syncthreads();
+    uint32_t running = s_base + excl;
+#pragma unroll
+    for (uint32_t i = 0; i < kItems; ++i) {
+        uint64_t idx = first + i;
+        if (idx < n) {
+            a.out.tile_ptcl_ranges[idx] = tk_segment_range{running, running + v[i]};
+            if (idx % a.cfg.tile_count == 0u) a.out.batch_ptcl_bases[idx / a.cfg.tile_count] = running;
+        }
+        running += v[i];
+    }
+}
+
+// coarse_emit_bins for every batch (coarse/emit.hlsl:82-142): writes
+// [stack begins][particles][stack ends][END] for each (batch, tile) with a
+// nonzero count and the tile kind word, including the opaque-prefix skip that
+// occluded_particle_prefix (coarse/occlusion.hlsli:12-34) derives.
+extern "C" __global__ __launch_bounds__(64) void tk_k_coarse_emit(tk_args_coarse a) {
+    const tk_coarse_config &cfg = a.cfg;
+    uint32_t tile = blockIdx.x * 64u + threadIdx.x;
+    if (tile >= cfg.tile_count) return;
+    CoarseCtx c{&a.cfg, &a.in, tile % cfg.tiles_width, tile / cfg.tiles_width};
+    const uint32_t T = cfg.tile_count, Q = cfg.batch_count;
+    Emitter e{a.out.ptcl, cfg.ptcl_capacity, a.out.status};
+    uint32_t *kinds = a.out.tile_kinds;
+    tk_tile_draw_record list = a.in.tile_draws[tile];
+
+    uint32_t cur = kNone, next = 0u;
+    bool active = false;  // current batch has a nonzero count
+    uint32_t start = 0u, end = 0u, wrappers = 0u, dst = 0u, flags = 0u, last_opaque = 0u;
+    bool begin_after = false;
+    auto finish = [&]() {
+        if (cur == kNone) return;
+        size_t idx = (size_t)cur * T + tile;
+        if (!active) {
+            kinds[idx] = TK_TILE_KIND_EMPTY;
+        } else {
+            dst = emit_stack_ends(c, cur, e, dst);
+            e.store(dst, TK_PTCL_END, 0u, 0u, 0u, 0u, 0u);
+            uint32_t count = end - start;
+            uint32_t skip = 0u;
+            if (wrappers == 0u && count > 2u && !begin_after) skip = last_opaque;
+            if ((flags & 8u) == 0u) skip = 0u;
+            kinds[idx] = classify_tile_flags(flags) | (skip << TK_TILE_PREFIX_SHIFT);
+        }
+        next = cur + 1u;
+    };
+    auto begin = [&](uint32_t b) {
+        cur = b;
+        tk_segment_range r = a.out.tile_ptcl_ranges[(size_t)b * T + tile];
+        start = r.start;
+        end = r.end;
+        active = end > start;
+        flags = 0u;
+        last_opaque = 0u;
+        begin_after = false;
+        if (active) {
+            wrappers = stack_wrapper_count(c, b);
+            if (wrappers == kNone) {  // cannot happen with a consistent count; keep the stream terminated
+                e.store(start, TK_PTCL_END, 0u, 0u, 0u, 0u, 0u);
+                active = false;
+                return;
+            }
+            flags = wrappers != 0u ? kClassOther : 0u;
+            dst = emit_stack_begins(c, b, e, start);
+        }
+    };
+    if (list.start != kNone) {
+        for (uint32_t i = 0; i < list.len; ++i) {
+            uint32_t slot = list.start + i;
+            if (slot >= cfg.tile_draw_index_count) break;
+            uint32_t d = a.in.tile_draw_indices[slot];
+            uint32_t b = d < cfg.draw_count ? a.in.draw_batch_ids[d] : kNone;
+            if (b >= Q) continue;
+            if (b != cur) {
+                if (cur != kNone && b < cur) continue;
+                finish();
+                for (uint32_t bb = next; bb < b; ++bb) kinds[(size_t)bb * T + tile] = TK_TILE_KIND_EMPTY;
+                begin(b);
+            }
+            if (!active) continue;
+            Particle p = draw_particle(c, b, d);
+            if (!p.valid) continue;
+            uint32_t pos = dst - start;  // relative record index
+            flags |= particle_class_flags(c, d, p.tag);
+            if (wrappers == 0u && p.tag == TK_PTCL_COLOR && (p.color >> 24) == 255u && dst > start) flags |= 8u;
+            if (is_begin_tag(p.tag)) begin_after = true;
+            if (p.tag == TK_PTCL_COLOR && (p.color >> 24) == 255u) {
+                last_opaque = pos;
+                begin_after = false;
+            }
+            if (dst < end) e.store(dst++, p.tag, p.winding, p.fill_rule, p.seg_start, p.seg_end, p.color);
+        }
+    }
+    finish();
+    for (uint32_t bb = next; bb < Q; ++bb) kinds[(size_t)bb * T + tile] = TK_TILE_KIND_EMPTY;
+}
+
+// fine_tile_main for every batch: one 256-lane workgroup per tile, one lane per
+// pixel.  Between batches the pixel goes through the RGBA8 store/load round trip
+// the reference performs on its target.
+extern "C" __global__ __launch_bounds__(256) void tk_k_fine(tk_args_fine a) {
+    __shared__ uint8_t clip_spill[TK_FINE_MAX_CLIP_SPILL * 256u];
+    const tk_fine_config &cfg = a.cfg;
+    const uint32_t tile_ix = blockIdx.x + blockIdx.y * gridDim.x;
+    if (tile_ix >= cfg.tile_count) return;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t lx = lane & 15u, ly = lane >> 4;
+    const uint32_t gx = (tile_ix % cfg.tiles_width) * 16u + lx;
+    const uint32_t gy = (tile_ix / cfg.tiles_width) * 16u + ly;
+    const float sx = (float)gx + 0.5f, sy = (float)gy + 0.5f;
+    const uint32_t *paint = a.in.paint;
+    const tk_ptcl_record *ptcl = a.in.ptcl;
+    const tk_line_segment *segs = a.in.segments;
+
+    Px pixel = rgba8_to_unorm(cfg.clear_color);
+    for (uint32_t b = 0; b < cfg.batch_count; ++b) {
+        const size_t idx = (size_t)b * cfg.tile_count + tile_ix;
+        const uint32_t kind_word = a.in.tile_kinds[idx];
+        const uint32_t kind = kind_word & TK_TILE_KIND_MASK;
+        if (kind == TK_TILE_KIND_EMPTY) continue;
+        const tk_segment_range r = a.in.tile_ptcl_ranges[idx];
+        const uint32_t pstart = r.start + (kind_word >> TK_TILE_PREFIX_SHIFT);
+        const uint32_t pend = min(r.end, cfg.ptcl_capacity);
+        if (kind == TK_TILE_KIND_INTERPRETER) {
+            uint32_t clip_mask = 255u, clip_depth = 0u;
+            uint32_t cs0 = 255u, cs1 = 255u, cs2 = 255u, cs3 = 255u;
+            auto push_clip = [&](uint32_t mask) {
+                uint32_t dd = clip_depth;
+                if (dd == 0u) { cs0 = mask; clip_depth = 1u; }
+                else if (dd == 1u) { cs1 = mask; clip_depth = 2u; }
+                else if (dd == 2u) { cs2 = mask; clip_depth = 3u; }
+                else if (dd == 3u) { cs3 = mask; clip_depth = 4u; }
+                else {
+                    uint32_t spill = dd - 4u;
+                    if (spill < cfg.clip_spill_depth) {
+                        clip_spill[spill * 256u + lane] = (uint8_t)mask;
+                        clip_depth = dd + 1u;
+                    }
+                }
+            };
+            for (uint32_t pi = pstart; pi < pend; ++pi) {
+                const tk_ptcl_record rec = ptcl[pi];
+                const uint32_t tag = rec.tag;
+                if (tag == TK_PTCL_END) break;
+                if (tag == TK_PTCL_COLOR) {
+                    if (clip_mask != 0u) {
+                        if (clip_mask == 255u && premul_opaque(rec.color)) pixel = rgba8_to_unorm(rec.color);
+                        else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(rec.color, clip_mask));
+                    }
+                } else if (tag == TK_PTCL_SDF) {
+                    if (clip_mask != 0u) {
+                        FineDraw d = load_fine_draw(a.in.draws, cfg.draw_count, rec.color);
+                        float coverage = sdf_coverage_from_draw(cfg, paint, d, sx, sy);
+                        uint32_t alpha = combine_alpha(coverage_to_u8(coverage), clip_mask);
+                        if (alpha != 0u) {
+                            uint32_t color = sample_draw_brush(cfg, paint, d, sx, sy);
+                            if (alpha == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
+                            else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, alpha));
+                        }
+                    }
+                } else if (tag == TK_PTCL_END_CLIP) {
+                    if (clip_depth > 0u) {
+                        clip_depth -= 1u;
+                        if (clip_depth == 0u) clip_mask = cs0;
+                        else if (clip_depth == 1u) clip_mask = cs1;
+                        else if (clip_depth == 2u) clip_mask = cs2;
+                        else if (clip_depth == 3u) clip_mask = cs3;
+                        else {
+                            uint32_t spill = clip_depth - 4u;
+                            if (spill < cfg.clip_spill_depth) clip_mask = clip_spill[spill * 256u + lane];
+                        }
+                    } else {
+                        clip_mask = 255u;
+                    }
+                } else if (tag == TK_PTCL_BEGIN_SDF_CLIP) {
+                    uint32_t parent = clip_mask, alpha = 0u;
+                    if (parent != 0u) {
+                        FineDraw d = load_fine_draw(a.in.draws, cfg.draw_count, rec.color);
+                        alpha = coverage_to_u8(sdf_coverage_from_draw(cfg, paint, d, sx, sy));
+                    }
+                    push_clip(parent);
+                    clip_mask = combine_alpha(parent, alpha);
+                } else if (tag == TK_PTCL_FILL || tag == TK_PTCL_BEGIN_CLIP) {
+                    float v = fill_coverage(segs, rec.backdrop, rec.segment_start, min(rec.segment_end, cfg.segment_capacity),
+                                            lx, ly);
+                    uint32_t alpha = coverage_to_alpha(v, rec.fill_rule);
+                    if (tag == TK_PTCL_BEGIN_CLIP) {
+                        push_clip(clip_mask);
+                        clip_mask = combine_alpha(clip_mask, alpha);
+                    } else {
+                        uint32_t masked = combine_alpha(alpha, clip_mask);
+                        if (masked != 0u) {
+                            FineDraw d = load_fine_draw(a.in.draws, cfg.draw_count, rec.color);
+                            uint32_t color = sample_draw_brush(cfg, paint, d, sx, sy);
+                            if (masked == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
+                            else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, masked));
+                        }
+                    }
+                }
+                // Image, glyph, opacity and blend records are not produced on this path.
+            }
+        } else if (kind == TK_TILE_KIND_COLOR) {
+            for (uint32_t pi = pstart; pi < pend; ++pi) {
+                const tk_ptcl_record rec = ptcl[pi];
+                if (rec.tag == TK_PTCL_END) break;
+                if (premul_opaque(rec.color)) pixel = rgba8_to_unorm(rec.color);
+                else pixel = src_over_premul_unorm(pixel, rgba8_to_unorm(rec.color));
+            }
+        } else {  // TILE_KIND_SDF / TILE_KIND_MIXED
+            for (uint32_t pi = pstart; pi < pend; ++pi) {
+                const tk_ptcl_record rec = ptcl[pi];
+                if (rec.tag == TK_PTCL_END) break;
+                if (rec.tag == TK_PTCL_COLOR) {
+                    if (premul_opaque(rec.color)) pixel = rgba8_to_unorm(rec.color);
+                    else pixel = src_over_premul_unorm(pixel, rgba8_to_unorm(rec.color));
+                } else if (rec.tag == kPtclImage) {
+                    // image brushes are not produced on this path
+                } else {
+                    FineDraw d = load_fine_draw(a.in.draws, cfg.draw_count, rec.color);
+                    float coverage = 0.0f;
+                    if ((int32_t)gx >= d.pb[0] && (int32_t)gy >= d.pb[1] && (int32_t)gx < d.pb[2] && (int32_t)gy < d.pb[3])
+                        coverage = sdf_coverage_from_draw(cfg, paint, d, sx, sy);
+                    uint32_t alpha = coverage_to_u8(coverage);
+                    if (alpha != 0u) {
+                        uint32_t color = blob_word(paint, cfg.paint_words, cfg.paint_brush_base, d.brush_offset + 4u);
+                        if (alpha == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
+                        else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, alpha));
+                    }
+                }
+            }
+        }
+        pixel = rgba8_to_unorm(unorm_to_rgba8(pixel));
+    }
+    if (gx < cfg.width && gy < cfg.height) a.rgba[(size_t)gy * cfg.width + gx] = unorm_to_rgba8(pixel);
+}
+
+// fill_coverage_words (validation/geometry.hlsl:22-29) plus the raw FP32 area.
+extern "C" __global__ __launch_bounds__(256) void tk_k_fill_coverage(tk_args_fill_coverage a) {
+    uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= a.count) return;
+    const uint32_t *q = a.requests + (size_t)id * 8u;
+    float v = fill_coverage(a.segments, (int32_t)q[2], q[0], q[1], q[3], q[4]);
+    a.alpha_out[id] = coverage_to_alpha(v, q[5]);
+    if (a.coverage_out) a.coverage_out[id] = v;
+}
+
+// pixel_math_words (validation/pixel.hlsl:9-34).
+extern "C" __global__ __launch_bounds__(256) void tk_k_pixel_math(tk_args_pixel_math a) {
+    uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= a.count) return;
+    const uint32_t *q = a.requests + (size_t)id * 8u;
+    uint32_t *o = a.out + (size_t)id * 14u;
+    uint32_t av = q[0], bv = q[1], src = q[2], dst = q[3];
+    float c = asfloat(q[4]), t = asfloat(q[5]);
+    uint32_t rule = q[6];
+    Px pixel = rgba8_to_unorm(src);
+    o[0] = mul_div255(av, bv);
+    o[1] = combine_alpha(av, bv);
+    o[2] = scale_premul_u8(src, bv);
+    o[3] = src_over_premul_u8(dst, src);
+    o[4] = src;
+    o[5] = coverage_to_u8(c);
+    o[6] = coverage_to_alpha(c, rule);
+    o[7] = lerp_premul_u8(src, dst, t);
+    o[8] = unorm_to_rgba8(src_over_premul_unorm(rgba8_to_unorm(dst), rgba8_to_unorm(src)));
+    o[9] = unorm_to_rgba8(scale_premul_u8_to_unorm(src, bv));
+    o[10] = coverage_to_u8(straight_channel(av, bv));
+    o[11] = pack_premul_rgba8(pixel.r, pixel.g, pixel.b, pixel.a);
+    o[12] = coverage_to_u8(rem_euclid_f32(c, 3.0f));
+    o[13] = asuint(signum_f32(c));
+}

@@ -1,0 +1,850 @@
+// Device runtime of libtileink_hip: context (explicit gfx950 device, embedded
+// code object loaded with hipModuleLoadData), whole-frame pipeline, and the
+// stage entry points of the C ABI.
+//
+// The frame pipeline replaces the reference's per-frame dispatch list
+// (src/native/runtime/program/scene_scan.rs:73-325, program/cumsum.rs:171-269,
+// program/coarse.rs:29-129, program/fine.rs:26-67, one coarse+fine pair per
+// DrawBatch) with a fixed sequence of gfx950 kernels on one stream:
+//   clear -> scan_count -> scan_prefix -> scan_emit -> segment_order -> cumsum
+//   -> coarse_count -> coarse_scan -> coarse_emit -> fine
+// where coarse and fine process every DrawBatch in a single launch.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "tk_internal.h"
+#include "tk_kernel_args.h"
+
+extern "C" const unsigned char tk_code_object_begin[];
+extern "C" const unsigned char tk_code_object_end[];
+
+namespace tk {
+
+static thread_local std::string g_error;
+
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+}
+
+namespace {
+
+#define TK_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                      __LINE__);                                                       \
+            return TK_ERR_HIP;                                                         \
+        }                                                                              \
+    } while (0)
+
+#define TK_TRY(expr)                  \
+    do {                              \
+        tk_status s_ = (expr);        \
+        if (s_ != TK_OK) return s_;   \
+    } while (0)
+
+}  // namespace
+}  // namespace tk
+
+using namespace tk;
+
+struct tk_context {
+    int device = -1;
+    hipModule_t module = nullptr;
+    hipStream_t stream = nullptr;
+    hipFunction_t f_fill = nullptr, f_scan_clear = nullptr, f_scan_count = nullptr, f_scan_prefix = nullptr,
+                  f_scan_emit = nullptr, f_segment_order = nullptr, f_cumsum = nullptr, f_coarse_count = nullptr,
+                  f_coarse_scan = nullptr, f_coarse_emit = nullptr, f_fine = nullptr, f_fill_coverage = nullptr,
+                  f_pixel_math = nullptr;
+    // Scratch for the stage entry points (frames own their own).
+    uint32_t *tickets = nullptr;  // [0] scan prefix, [1] coarse scan
+    unsigned long long *status = nullptr;
+    size_t status_words = 0;
+    uint32_t epoch = 0;
+    uint32_t next_epoch() {
+        epoch = (epoch + 1u) & 0x3FFFFFFFu;
+        if (epoch == 0u) epoch = 1u;
+        return epoch;
+    }
+};
+
+namespace tk {
+namespace {
+
+uint32_t div_up(uint64_t n, uint32_t d) { return uint32_t((n + d - 1) / d); }
+
+template <class Args>
+tk_status launch(tk_context *ctx, hipFunction_t f, uint32_t gx, uint32_t gy, uint32_t block, const Args &args) {
+    if (gx == 0 || gy == 0) return TK_OK;
+    Args copy = args;
+    void *params[] = {&copy};
+    hipError_t e = hipModuleLaunchKernel(f, gx, gy, 1, block, 1, 1, 0, ctx->stream, params, nullptr);
+    if (e != hipSuccess) {
+        set_error("kernel launch failed: %s", hipGetErrorString(e));
+        return TK_ERR_HIP;
+    }
+    return TK_OK;
+}
+
+tk_status ensure_status(tk_context *ctx, size_t words) {
+    if (words <= ctx->status_words) return TK_OK;
+    if (ctx->status) {
+        TK_HIP(hipStreamSynchronize(ctx->stream));
+        TK_HIP(hipFree(ctx->status));
+        ctx->status = nullptr;
+        ctx->status_words = 0;
+    }
+    size_t n = std::max<size_t>(words, 1024);
+    TK_HIP(hipMalloc(&ctx->status, n * sizeof(unsigned long long)));
+    TK_HIP(hipMemsetAsync(ctx->status, 0, n * sizeof(unsigned long long), ctx->stream));
+    ctx->status_words = n;
+    return TK_OK;
+}
+
+tk_status fill_u32(tk_context *ctx, void *dst, uint64_t count, uint32_t value) {
+    if (!dst || count == 0) return TK_OK;
+    tk_args_fill a{static_cast<uint32_t *>(dst), count, value};
+    uint32_t blocks = uint32_t(std::min<uint64_t>(div_up(count, 256), 65536));
+    return launch(ctx, ctx->f_fill, blocks, 1, 256, a);
+}
+
+// --- stage launchers shared by the C ABI and the frame pipeline -------------
+
+tk_status run_scan_count(tk_context *ctx, const tk_scan_config &cfg, const tk_line *lines, const tk_path_record *paths,
+                         uint32_t path_record_count, int32_t *backdrops, uint32_t *counts) {
+    tk_args_scan_lines a{cfg, lines, paths, path_record_count, backdrops, counts, nullptr};
+    return launch(ctx, ctx->f_scan_count, div_up(cfg.line_count, 256), 1, 256, a);
+}
+
+tk_status run_scan_emit(tk_context *ctx, const tk_scan_config &cfg, const tk_line *lines, const tk_path_record *paths,
+                        uint32_t path_record_count, uint32_t *cursors, tk_line_segment *segments) {
+    tk_args_scan_lines a{cfg, lines, paths, path_record_count, nullptr, cursors, segments};
+    return launch(ctx, ctx->f_scan_emit, div_up(cfg.line_count, 256), 1, 256, a);
+}
+
+tk_status run_scan_prefix(tk_context *ctx, const tk_scan_config &cfg, const tk_scan_chunk *chunks,
+                          const tk_scan_chunk_range *chunk_ranges, const tk_path_record *paths, const uint32_t *counts,
+                          tk_segment_range *ranges, uint32_t *cursors, uint32_t *bumps, uint32_t *chunk_totals,
+                          uint32_t *chunk_offsets, unsigned long long *status, uint32_t *ticket, uint32_t epoch) {
+    uint32_t blocks = std::max<uint32_t>(cfg.scan_chunk_count, div_up(cfg.path_count, 256));
+    if (blocks == 0) return TK_OK;
+    tk_args_scan_prefix a{cfg,           chunks, chunk_ranges, paths,  counts, ranges, cursors,
+                          bumps,         chunk_totals, chunk_offsets, status, ticket, epoch,  blocks};
+    return launch(ctx, ctx->f_scan_prefix, blocks, 1, 256, a);
+}
+
+tk_status run_cumsum(tk_context *ctx, const tk_cumsum_config &cfg, const uint32_t *offs, const uint32_t *lens,
+                     const uint32_t *row_starts, const uint32_t *row_ends, int32_t *backdrops, uint32_t *totals,
+                     uint32_t *carries) {
+    tk_args_cumsum a{cfg, offs, lens, row_starts, row_ends, backdrops, totals, carries};
+    return launch(ctx, ctx->f_cumsum, div_up(cfg.row_count, 256), 1, 256, a);
+}
+
+uint32_t coarse_scan_blocks(const tk_coarse_config &cfg) {
+    uint64_t n = uint64_t(cfg.batch_count) * cfg.tile_count;
+    return uint32_t(std::max<uint64_t>((n + 1023) / 1024, 1));
+}
+
+tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coarse_inputs &in,
+                     const tk_coarse_outputs &out, unsigned long long *status, uint32_t *ticket, uint32_t epoch,
+                     hipEvent_t *marks) {
+    if (cfg.batch_count == 0 || cfg.tile_count == 0) return TK_OK;
+    tk_args_coarse a{cfg, in, out, status, ticket, epoch, coarse_scan_blocks(cfg)};
+    TK_TRY(launch(ctx, ctx->f_coarse_count, div_up(cfg.tile_count, 64), 1, 64, a));
+    if (marks) TK_HIP(hipEventRecord(marks[0], ctx->stream));
+    TK_TRY(launch(ctx, ctx->f_coarse_scan, a.scan_blocks, 1, 256, a));
+    if (marks) TK_HIP(hipEventRecord(marks[1], ctx->stream));
+    TK_TRY(launch(ctx, ctx->f_coarse_emit, div_up(cfg.tile_count, 64), 1, 64, a));
+    return TK_OK;
+}
+
+tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inputs &in, uint32_t *rgba) {
+    if (cfg.clip_spill_depth > TK_FINE_MAX_CLIP_SPILL) {
+        set_error("clip nesting deeper than %u is not supported", 4u + TK_FINE_MAX_CLIP_SPILL);
+        return TK_ERR_INVALID;
+    }
+    tk_args_fine a{cfg, in, rgba};
+    uint32_t gx = std::min<uint32_t>(cfg.tile_count, 65535u);
+    uint32_t gy = div_up(cfg.tile_count, gx);
+    return launch(ctx, ctx->f_fine, gx, gy, 256, a);
+}
+
+}  // namespace
+}  // namespace tk
+
+// ---------------------------------------------------------------------------
+// Frame
+// ---------------------------------------------------------------------------
+
+struct tk_frame {
+    tk_context *ctx = nullptr;
+    tk_scene meta{};  // sizes and scalars only; pointers are cleared
+    uint64_t device_bytes = 0;
+    std::vector<void *> allocations;
+
+    // inputs
+    tk_line *lines = nullptr;
+    tk_path_record *paths = nullptr;
+    tk_draw_record *draws = nullptr;
+    uint32_t *paint = nullptr, *batch_ids = nullptr;
+    tk_layer_record *layers = nullptr;
+    tk_batch_record *batches = nullptr;
+    tk_tile_draw_record *tile_draws = nullptr;
+    uint32_t *tile_draw_indices = nullptr;
+    tk_scan_chunk *scan_chunks = nullptr;
+    tk_scan_chunk_range *scan_chunk_ranges = nullptr;
+    uint32_t *cs_offsets = nullptr, *cs_lens = nullptr, *cs_row_starts = nullptr, *cs_row_ends = nullptr;
+    // scan outputs
+    int32_t *backdrops = nullptr, *raw_backdrops = nullptr;
+    uint32_t *seg_counts = nullptr, *cursors = nullptr, *bumps = nullptr, *chunk_totals = nullptr,
+             *chunk_offsets = nullptr, *cs_totals = nullptr, *cs_carries = nullptr;
+    tk_segment_range *seg_ranges = nullptr;
+    tk_line_segment *segments = nullptr;
+    // coarse outputs
+    uint32_t *tile_counts = nullptr, *tile_kinds = nullptr, *batch_bases = nullptr, *status = nullptr;
+    tk_segment_range *tile_ranges = nullptr;
+    tk_ptcl_record *ptcl = nullptr;
+    uint64_t ptcl_capacity = 0;
+    // output
+    uint32_t *image = nullptr;
+    // scratch
+    uint32_t *tickets = nullptr;
+    unsigned long long *prefix_status = nullptr, *coarse_status = nullptr;
+    uint32_t epoch = 0;
+    bool capture_raw = false;
+    bool timing = false;
+    hipEvent_t ev[12] = {};
+    bool have_times = false;
+};
+
+namespace tk {
+namespace {
+
+template <class T>
+tk_status dalloc(tk_frame *f, T **ptr, uint64_t count) {
+    uint64_t bytes = std::max<uint64_t>(count * sizeof(T), 16);
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%llu) failed: %s", (unsigned long long)bytes, hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? TK_ERR_OOM : TK_ERR_HIP;
+    }
+    f->allocations.push_back(p);
+    f->device_bytes += bytes;
+    *ptr = static_cast<T *>(p);
+    return TK_OK;
+}
+
+template <class T>
+tk_status upload(tk_frame *f, T *dst, const T *src, uint64_t count) {
+    if (count == 0) return TK_OK;
+    if (!src) return fail(TK_ERR_INVALID, "scene array missing");
+    TK_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, f->ctx->stream));
+    return TK_OK;
+}
+
+tk_status validate_scene(const tk_scene *s) {
+    if (!s) return fail(TK_ERR_INVALID, "null scene");
+    if (s->width == 0 || s->height == 0) return fail(TK_ERR_INVALID, "empty canvas");
+    if (s->tiles_width != (s->width + 15) / 16 || s->tiles_height != (s->height + 15) / 16)
+        return fail(TK_ERR_INVALID, "tile grid does not match canvas size");
+    if (uint64_t(s->tiles_width) * s->tiles_height > 0xFFFFFFFFull) return fail(TK_ERR_INVALID, "too many tiles");
+    if (s->max_clip_depth > 4u + TK_FINE_MAX_CLIP_SPILL) return fail(TK_ERR_INVALID, "clip nesting too deep");
+    if (s->ptcl_capacity >= 0xFFFFFFFFull) return fail(TK_ERR_OVERFLOW, "PTCL capacity exceeds u32 record indices");
+    return TK_OK;
+}
+
+tk_status frame_upload(tk_frame *f, const tk_scene *s) {
+    TK_TRY(upload(f, f->lines, s->lines, s->line_count));
+    TK_TRY(upload(f, f->paths, s->paths, s->path_count));
+    TK_TRY(upload(f, f->draws, s->draws, s->draw_count));
+    TK_TRY(upload(f, f->paint, s->paint, s->paint_words));
+    TK_TRY(upload(f, f->batch_ids, s->draw_batch_ids, s->draw_count));
+    TK_TRY(upload(f, f->layers, s->layers, s->layer_count));
+    TK_TRY(upload(f, f->batches, s->batches, s->batch_count));
+    TK_TRY(upload(f, f->tile_draws, s->tile_draws, uint64_t(s->tiles_width) * s->tiles_height));
+    TK_TRY(upload(f, f->tile_draw_indices, s->tile_draw_indices, s->tile_draw_index_count));
+    TK_TRY(upload(f, f->scan_chunks, s->scan_chunks, s->scan_chunk_count));
+    TK_TRY(upload(f, f->scan_chunk_ranges, s->scan_chunk_ranges, s->path_count));
+    TK_TRY(upload(f, f->cs_offsets, s->cumsum_chunk_offsets, s->cumsum_chunk_count));
+    TK_TRY(upload(f, f->cs_lens, s->cumsum_chunk_lens, s->cumsum_chunk_count));
+    TK_TRY(upload(f, f->cs_row_starts, s->cumsum_row_starts, s->cumsum_row_count));
+    TK_TRY(upload(f, f->cs_row_ends, s->cumsum_row_ends, s->cumsum_row_count));
+    TK_HIP(hipStreamSynchronize(f->ctx->stream));
+    return TK_OK;
+}
+
+bool same_sizes(const tk_scene &a, const tk_scene &b) {
+    return a.width == b.width && a.height == b.height && a.line_count == b.line_count && a.path_count == b.path_count &&
+           a.draw_count == b.draw_count && a.paint_words == b.paint_words && a.layer_count == b.layer_count &&
+           a.batch_count == b.batch_count && a.tile_draw_index_count == b.tile_draw_index_count &&
+           a.scan_chunk_count == b.scan_chunk_count && a.cumsum_chunk_count == b.cumsum_chunk_count &&
+           a.cumsum_row_count == b.cumsum_row_count && a.backdrop_len == b.backdrop_len &&
+           a.segment_capacity == b.segment_capacity && a.ptcl_capacity <= b.ptcl_capacity &&
+           a.max_clip_depth <= b.max_clip_depth;
+}
+
+tk_scan_config frame_scan_config(const tk_frame *f) {
+    const tk_scene &s = f->meta;
+    tk_scan_config c{};
+    c.clear_len = std::max({s.backdrop_len, s.path_count, s.scan_chunk_count});
+    c.backdrop_len = s.backdrop_len;
+    c.path_count = s.path_count;
+    c.scan_chunk_count = s.scan_chunk_count;
+    c.line_count = s.line_count;
+    c.segment_capacity = s.segment_capacity;
+    return c;
+}
+
+tk_coarse_config frame_coarse_config(const tk_frame *f) {
+    const tk_scene &s = f->meta;
+    tk_coarse_config c{};
+    c.tile_count = s.tiles_width * s.tiles_height;
+    c.tiles_width = s.tiles_width;
+    c.tiles_height = s.tiles_height;
+    c.batch_count = s.batch_count;
+    c.draw_count = s.draw_count;
+    c.path_count = s.path_count;
+    c.paint_words = s.paint_words;
+    c.paint_brush_base = s.paint_brush_base;
+    c.layer_count = s.layer_count;
+    c.tile_draw_index_count = s.tile_draw_index_count;
+    c.ptcl_capacity = f->ptcl_capacity;
+    return c;
+}
+
+tk_fine_config frame_fine_config(const tk_frame *f) {
+    const tk_scene &s = f->meta;
+    tk_fine_config c{};
+    c.width = s.width;
+    c.height = s.height;
+    c.clear_color = s.clear_color;
+    c.tile_count = s.tiles_width * s.tiles_height;
+    c.tiles_width = s.tiles_width;
+    c.tiles_height = s.tiles_height;
+    c.batch_count = s.batch_count;
+    c.clip_spill_depth = s.max_clip_depth > 4u ? s.max_clip_depth - 4u : 0u;
+    c.paint_words = s.paint_words;
+    c.paint_sdf_shadow_base = s.paint_sdf_shadow_base;
+    c.paint_brush_base = s.paint_brush_base;
+    c.draw_count = s.draw_count;
+    c.ptcl_capacity = uint32_t(f->ptcl_capacity);
+    c.segment_capacity = s.segment_capacity;
+    return c;
+}
+
+tk_status record(tk_frame *f, int mark) {
+    if (f->timing) TK_HIP(hipEventRecord(f->ev[mark], f->ctx->stream));
+    return TK_OK;
+}
+
+tk_status frame_render(tk_frame *f) {
+    tk_context *ctx = f->ctx;
+    const tk_scene &s = f->meta;
+    tk_scan_config scfg = frame_scan_config(f);
+    const uint32_t T = s.tiles_width * s.tiles_height;
+    f->epoch = (f->epoch + 1u) & 0x3FFFFFFFu;
+    if (f->epoch == 0u) f->epoch = 1u;
+
+    TK_TRY(record(f, 0));
+    // scan_clear: the deltas and counts accumulate; every other scan output is fully rewritten.
+    TK_TRY(fill_u32(ctx, f->backdrops, s.backdrop_len, 0u));
+    TK_TRY(fill_u32(ctx, f->seg_counts, s.backdrop_len, 0u));
+    TK_TRY(fill_u32(ctx, f->status, 4, 0u));
+    TK_TRY(record(f, 1));
+    TK_TRY(run_scan_count(ctx, scfg, f->lines, f->paths, s.path_count, f->backdrops, f->seg_counts));
+    TK_TRY(record(f, 2));
+    TK_TRY(run_scan_prefix(ctx, scfg, f->scan_chunks, f->scan_chunk_ranges, f->paths, f->seg_counts, f->seg_ranges,
+                           f->cursors, f->bumps, f->chunk_totals, f->chunk_offsets, f->prefix_status, &f->tickets[0],
+                           f->epoch));
+    TK_TRY(record(f, 3));
+    TK_TRY(run_scan_emit(ctx, scfg, f->lines, f->paths, s.path_count, f->cursors, f->segments));
+    TK_TRY(record(f, 4));
+    {
+        tk_args_segment_sort a{f->seg_ranges, s.backdrop_len, s.segment_capacity, f->segments};
+        TK_TRY(launch(ctx, ctx->f_segment_order, div_up(s.backdrop_len, 256), 1, 256, a));
+    }
+    TK_TRY(record(f, 5));
+    if (f->capture_raw && s.backdrop_len)
+        TK_HIP(hipMemcpyAsync(f->raw_backdrops, f->backdrops, uint64_t(s.backdrop_len) * 4, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+    {
+        tk_cumsum_config cc{s.cumsum_row_count, s.cumsum_chunk_count, 0u, 0u};
+        TK_TRY(run_cumsum(ctx, cc, f->cs_offsets, f->cs_lens, f->cs_row_starts, f->cs_row_ends, f->backdrops,
+                          f->cs_totals, f->cs_carries));
+    }
+    TK_TRY(record(f, 6));
+    {
+        tk_coarse_config cc = frame_coarse_config(f);
+        tk_coarse_inputs in{f->draws,  f->paths,    f->backdrops, f->seg_ranges, f->layers,
+                            f->batches, f->paint, f->batch_ids, f->tile_draws, f->tile_draw_indices};
+        tk_coarse_outputs out{f->tile_counts, f->tile_ranges, f->tile_kinds, f->batch_bases, f->ptcl, f->status};
+        TK_TRY(run_coarse(ctx, cc, in, out, f->coarse_status, &f->tickets[1], f->epoch, f->timing ? &f->ev[7] : nullptr));
+        if (cc.batch_count == 0 && f->timing) {
+            TK_TRY(record(f, 7));
+            TK_TRY(record(f, 8));
+        }
+    }
+    TK_TRY(record(f, 9));
+    {
+        tk_fine_config fc = frame_fine_config(f);
+        tk_fine_inputs in{f->draws, f->paint, f->segments, f->tile_ranges, f->tile_kinds, f->batch_bases, f->ptcl};
+        (void)T;
+        TK_TRY(run_fine(ctx, fc, in, f->image));
+    }
+    TK_TRY(record(f, 10));
+    f->have_times = f->timing;
+    return TK_OK;
+}
+
+}  // namespace
+}  // namespace tk
+
+extern "C" {
+
+const char *tk_last_error(void) { return tk::g_error.c_str(); }
+uint32_t tk_version(void) { return (0u << 16) | 1u; }
+
+tk_status tk_context_create(int device, tk_context **out) {
+    if (!out) return fail(TK_ERR_INVALID, "null output");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(TK_ERR_UNAVAILABLE, "no HIP device");
+    if (device < 0 || device >= count) return fail(TK_ERR_UNAVAILABLE, "device ordinal out of range");
+    hipDeviceProp_t prop;
+    TK_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_error("device %d is %s, this library is built for gfx950", device, prop.gcnArchName);
+        return TK_ERR_UNAVAILABLE;
+    }
+    TK_HIP(hipSetDevice(device));
+    tk_context *ctx = new (std::nothrow) tk_context();
+    if (!ctx) return fail(TK_ERR_OOM, "context allocation failed");
+    ctx->device = device;
+    auto cleanup = [&](tk_status st) {
+        tk_context_destroy(ctx);
+        return st;
+    };
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(TK_ERR_HIP, "stream creation failed"));
+    if (hipModuleLoadData(&ctx->module, tk_code_object_begin) != hipSuccess)
+        return cleanup(fail(TK_ERR_HIP, "loading the gfx950 code object failed"));
+    struct {
+        hipFunction_t *fn;
+        const char *name;
+    } table[] = {
+        {&ctx->f_fill, "tk_k_fill"},
+        {&ctx->f_scan_clear, "tk_k_scan_clear"},
+        {&ctx->f_scan_count, "tk_k_scan_count"},
+        {&ctx->f_scan_prefix, "tk_k_scan_prefix"},
+        {&ctx->f_scan_emit, "tk_k_scan_emit"},
+        {&ctx->f_segment_order, "tk_k_segment_order"},
+        {&ctx->f_cumsum, "tk_k_cumsum"},
+        {&ctx->f_coarse_count, "tk_k_coarse_count"},
+        {&ctx->f_coarse_scan, "tk_k_coarse_scan"},
+        {&ctx->f_coarse_emit, "tk_k_coarse_emit"},
+        {&ctx->f_fine, "tk_k_fine"},
+        {&ctx->f_fill_coverage, "tk_k_fill_coverage"},
+        {&ctx->f_pixel_math, "tk_k_pixel_math"},
+    };
+    for (auto &entry : table) {
+        if (hipModuleGetFunction(entry.fn, ctx->module, entry.name) != hipSuccess) {
+            set_error("kernel %s missing from the code object", entry.name);
+            return cleanup(TK_ERR_HIP);
+        }
+    }
+    if (hipMalloc(&ctx->tickets, 16) != hipSuccess) return cleanup(fail(TK_ERR_OOM, "ticket allocation failed"));
+    if (hipMemset(ctx->tickets, 0, 16) != hipSuccess) return cleanup(fail(TK_ERR_HIP, "ticket clear failed"));
+    *out = ctx;
+    return TK_OK;
+}
+
+void tk_context_destroy(tk_context *ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->status) (void)hipFree(ctx->status);
+    if (ctx->tickets) (void)hipFree(ctx->tickets);
+    if (ctx->module) (void)hipModuleUnload(ctx->module);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+void *tk_context_stream(tk_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+tk_status tk_context_synchronize(tk_context *ctx) {
+    if (!ctx) return fail(TK_ERR_INVALID, "null context");
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    return TK_OK;
+}
+
+tk_status tk_context_limits(tk_context *ctx, tk_limits *out) {
+    if (!ctx || !out) return fail(TK_ERR_INVALID, "null argument");
+    hipDeviceProp_t prop;
+    TK_HIP(hipGetDeviceProperties(&prop, ctx->device));
+    out->image_dimension = 65536u;
+    out->max_tiles = 4096u * 4096u;
+    out->device_memory_bytes = prop.totalGlobalMem;
+    out->compute_units = uint32_t(prop.multiProcessorCount);
+    return TK_OK;
+}
+
+// --- device buffers ---------------------------------------------------------
+
+tk_status tk_device_alloc(tk_context *ctx, size_t bytes, void **out) {
+    if (!ctx || !out) return fail(TK_ERR_INVALID, "null argument");
+    *out = nullptr;
+    hipError_t e = hipMalloc(out, std::max<size_t>(bytes, 16));
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? TK_ERR_OOM : TK_ERR_HIP;
+    }
+    return TK_OK;
+}
+
+tk_status tk_device_free(tk_context *ctx, void *ptr) {
+    if (!ctx) return fail(TK_ERR_INVALID, "null context");
+    if (!ptr) return TK_OK;
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    TK_HIP(hipFree(ptr));
+    return TK_OK;
+}
+
+tk_status tk_copy_to_device(tk_context *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(TK_ERR_INVALID, "null argument");
+    if (!bytes) return TK_OK;
+    TK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    return TK_OK;
+}
+
+tk_status tk_copy_to_host(tk_context *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(TK_ERR_INVALID, "null argument");
+    if (!bytes) return TK_OK;
+    TK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    return TK_OK;
+}
+
+// --- frames -----------------------------------------------------------------
+
+tk_status tk_frame_create(tk_context *ctx, const tk_scene *s, tk_frame **out) {
+    if (!ctx || !out) return fail(TK_ERR_INVALID, "null argument");
+    *out = nullptr;
+    TK_TRY(validate_scene(s));
+    tk_frame *f = new (std::nothrow) tk_frame();
+    if (!f) return fail(TK_ERR_OOM, "frame allocation failed");
+    f->ctx = ctx;
+    f->meta = *s;
+    f->ptcl_capacity = std::max<uint64_t>(s->ptcl_capacity, 1);
+    auto bail = [&](tk_status st) {
+        tk_frame_destroy(f);
+        return st;
+    };
+    const uint64_t T = uint64_t(s->tiles_width) * s->tiles_height;
+    const uint64_t QT = uint64_t(s->batch_count) * T;
+    tk_status st = TK_OK;
+#define ALLOC(ptr, n) \
+    if ((st = dalloc(f, &f->ptr, (n))) != TK_OK) return bail(st)
+    ALLOC(lines, s->line_count);
+    ALLOC(paths, s->path_count);
+    ALLOC(draws, s->draw_count);
+    ALLOC(paint, s->paint_words);
+    ALLOC(batch_ids, s->draw_count);
+    ALLOC(layers, s->layer_count);
+    ALLOC(batches, s->batch_count);
+    ALLOC(tile_draws, T);
+    ALLOC(tile_draw_indices, s->tile_draw_index_count);
+    ALLOC(scan_chunks, s->scan_chunk_count);
+    ALLOC(scan_chunk_ranges, s->path_count);
+    ALLOC(cs_offsets, s->cumsum_chunk_count);
+    ALLOC(cs_lens, s->cumsum_chunk_count);
+    ALLOC(cs_row_starts, s->cumsum_row_count);
+    ALLOC(cs_row_ends, s->cumsum_row_count);
+    ALLOC(backdrops, s->backdrop_len);
+    ALLOC(raw_backdrops, s->backdrop_len);
+    ALLOC(seg_counts, s->backdrop_len);
+    ALLOC(seg_ranges, s->backdrop_len);
+    ALLOC(cursors, s->backdrop_len);
+    ALLOC(bumps, s->path_count);
+    ALLOC(chunk_totals, s->scan_chunk_count);
+    ALLOC(chunk_offsets, s->scan_chunk_count);
+    ALLOC(cs_totals, s->cumsum_chunk_count);
+    ALLOC(cs_carries, s->cumsum_chunk_count);
+    ALLOC(segments, s->segment_capacity);
+    ALLOC(tile_counts, QT);
+    ALLOC(tile_ranges, QT);
+    ALLOC(tile_kinds, QT);
+    ALLOC(batch_bases, uint64_t(s->batch_count) + 1);
+    ALLOC(status, 4);
+    ALLOC(ptcl, f->ptcl_capacity);
+    ALLOC(image, uint64_t(s->width) * s->height);
+    ALLOC(tickets, 4);
+    ALLOC(prefix_status, std::max<uint64_t>(s->scan_chunk_count, 1));
+    {
+        tk_coarse_config cc = frame_coarse_config(f);
+        ALLOC(coarse_status, coarse_scan_blocks(cc));
+    }
+#undef ALLOC
+    if (hipMemsetAsync(f->tickets, 0, 16, ctx->stream) != hipSuccess ||
+        hipMemsetAsync(f->prefix_status, 0, std::max<uint64_t>(s->scan_chunk_count, 1) * 8, ctx->stream) != hipSuccess ||
+        hipMemsetAsync(f->coarse_status, 0, uint64_t(coarse_scan_blocks(frame_coarse_config(f))) * 8, ctx->stream) !=
+            hipSuccess ||
+        hipMemsetAsync(f->cs_carries, 0, std::max<uint64_t>(s->cumsum_chunk_count, 1) * 4, ctx->stream) != hipSuccess)
+        return bail(fail(TK_ERR_HIP, "scratch clear failed"));
+    for (auto &e : f->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(TK_ERR_HIP, "event creation failed"));
+    if ((st = frame_upload(f, s)) != TK_OK) return bail(st);
+    // Only sizes survive in the frame; host pointers belong to the caller.
+    f->meta.lines = nullptr;
+    f->meta.paths = nullptr;
+    f->meta.draws = nullptr;
+    f->meta.paint = nullptr;
+    f->meta.draw_batch_ids = nullptr;
+    f->meta.layers = nullptr;
+    f->meta.batches = nullptr;
+    f->meta.tile_draws = nullptr;
+    f->meta.tile_draw_indices = nullptr;
+    f->meta.scan_chunks = nullptr;
+    f->meta.scan_chunk_ranges = nullptr;
+    f->meta.cumsum_chunk_offsets = nullptr;
+    f->meta.cumsum_chunk_lens = nullptr;
+    f->meta.cumsum_row_starts = nullptr;
+    f->meta.cumsum_row_ends = nullptr;
+    *out = f;
+    return TK_OK;
+}
+
+tk_status tk_frame_update(tk_frame *f, const tk_scene *s) {
+    if (!f) return fail(TK_ERR_INVALID, "null frame");
+    TK_TRY(validate_scene(s));
+    if (!same_sizes(*s, f->meta)) return fail(TK_ERR_INVALID, "scene sizes differ from the frame's");
+    uint32_t clear = s->clear_color;
+    TK_TRY(frame_upload(f, s));
+    f->meta.clear_color = clear;
+    return TK_OK;
+}
+
+void tk_frame_destroy(tk_frame *f) {
+    if (!f) return;
+    if (f->ctx && f->ctx->stream) (void)hipStreamSynchronize(f->ctx->stream);
+    for (void *p : f->allocations) (void)hipFree(p);
+    for (auto &e : f->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete f;
+}
+
+tk_status tk_frame_render(tk_frame *f) {
+    if (!f) return fail(TK_ERR_INVALID, "null frame");
+    return frame_render(f);
+}
+
+tk_status tk_frame_set_capture(tk_frame *f, int enable) {
+    if (!f) return fail(TK_ERR_INVALID, "null frame");
+    f->capture_raw = enable != 0;
+    return TK_OK;
+}
+
+tk_status tk_frame_enable_timing(tk_frame *f, int enable) {
+    if (!f) return fail(TK_ERR_INVALID, "null frame");
+    f->timing = enable != 0;
+    f->have_times = false;
+    return TK_OK;
+}
+
+tk_status tk_frame_stage_times(tk_frame *f, tk_stage_times *out) {
+    if (!f || !out) return fail(TK_ERR_INVALID, "null argument");
+    if (!f->have_times) return fail(TK_ERR_INVALID, "no timed render recorded");
+    TK_HIP(hipEventSynchronize(f->ev[10]));
+    float t[11] = {};
+    for (int i = 1; i <= 10; ++i) TK_HIP(hipEventElapsedTime(&t[i], f->ev[i - 1], f->ev[i]));
+    out->scan_clear = t[1];
+    out->scan_count = t[2];
+    out->scan_prefix = t[3];
+    out->scan_emit = t[4];
+    out->segment_order = t[5];
+    out->cumsum = t[6];
+    out->coarse_count = t[7];
+    out->coarse_prefix = t[8];
+    out->coarse_emit = t[9];
+    out->fine = t[10];
+    float total = 0;
+    TK_HIP(hipEventElapsedTime(&total, f->ev[0], f->ev[10]));
+    out->total = total;
+    return TK_OK;
+}
+
+tk_status tk_frame_read_image(tk_frame *f, uint32_t *rgba, size_t pixel_count) {
+    if (!f || !rgba) return fail(TK_ERR_INVALID, "null argument");
+    uint64_t n = uint64_t(f->meta.width) * f->meta.height;
+    if (pixel_count < n) return fail(TK_ERR_INVALID, "output buffer too small");
+    TK_HIP(hipMemcpyAsync(rgba, f->image, n * 4, hipMemcpyDeviceToHost, f->ctx->stream));
+    TK_HIP(hipStreamSynchronize(f->ctx->stream));
+    return TK_OK;
+}
+
+void *tk_frame_image_device(tk_frame *f) { return f ? f->image : nullptr; }
+
+tk_status tk_frame_read_buffer(tk_frame *f, tk_frame_buffer which, void *out, size_t bytes) {
+    if (!f || !out) return fail(TK_ERR_INVALID, "null argument");
+    const tk_scene &s = f->meta;
+    const uint64_t QT = uint64_t(s.batch_count) * s.tiles_width * s.tiles_height;
+    const void *src = nullptr;
+    uint64_t n = 0;
+    switch (which) {
+    case TK_BUF_BACKDROPS: src = f->backdrops; n = uint64_t(s.backdrop_len) * 4; break;
+    case TK_BUF_RAW_BACKDROPS: src = f->raw_backdrops; n = uint64_t(s.backdrop_len) * 4; break;
+    case TK_BUF_SEGMENT_COUNTS: src = f->seg_counts; n = uint64_t(s.backdrop_len) * 4; break;
+    case TK_BUF_SEGMENT_RANGES: src = f->seg_ranges; n = uint64_t(s.backdrop_len) * 8; break;
+    case TK_BUF_SEGMENT_CURSORS: src = f->cursors; n = uint64_t(s.backdrop_len) * 4; break;
+    case TK_BUF_SEGMENT_BUMPS: src = f->bumps; n = uint64_t(s.path_count) * 4; break;
+    case TK_BUF_SEGMENTS: src = f->segments; n = uint64_t(s.segment_capacity) * 20; break;
+    case TK_BUF_TILE_PTCL_COUNTS: src = f->tile_counts; n = QT * 4; break;
+    case TK_BUF_TILE_PTCL_RANGES: src = f->tile_ranges; n = QT * 8; break;
+    case TK_BUF_TILE_KINDS: src = f->tile_kinds; n = QT * 4; break;
+    case TK_BUF_BATCH_PTCL_BASES: src = f->batch_bases; n = (uint64_t(s.batch_count) + 1) * 4; break;
+    case TK_BUF_PTCL: src = f->ptcl; n = f->ptcl_capacity * 24; break;
+    default: return fail(TK_ERR_INVALID, "unknown buffer");
+    }
+    if (bytes > n) return fail(TK_ERR_INVALID, "read larger than the buffer");
+    if (bytes == 0) return TK_OK;
+    TK_HIP(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, f->ctx->stream));
+    TK_HIP(hipStreamSynchronize(f->ctx->stream));
+    return TK_OK;
+}
+
+tk_status tk_frame_stats_read(tk_frame *f, tk_frame_stats *out) {
+    if (!f || !out) return fail(TK_ERR_INVALID, "null argument");
+    const tk_scene &s = f->meta;
+    uint32_t status[4] = {};
+    TK_HIP(hipMemcpyAsync(status, f->status, 16, hipMemcpyDeviceToHost, f->ctx->stream));
+    TK_HIP(hipStreamSynchronize(f->ctx->stream));
+    std::memset(out, 0, sizeof(*out));
+    out->paths = s.path_count;
+    out->lines = s.line_count;
+    out->draws = s.draw_count;
+    out->batches = s.batch_count;
+    out->tiles = s.tiles_width * s.tiles_height;
+    out->backdrop_len = s.backdrop_len;
+    out->segment_capacity = s.segment_capacity;
+    out->tile_draw_refs = s.tile_draw_index_count;
+    out->ptcl_capacity = f->ptcl_capacity;
+    out->ptcl_records = status[1];
+    out->overflow = status[0] | (status[3] << 1) | (status[1] > f->ptcl_capacity ? 4u : 0u);
+    uint32_t emitted = 0;
+    if (s.path_count) {
+        // Segments emitted = sum of per-path bumps.
+        std::vector<uint32_t> bumps(s.path_count);
+        TK_HIP(hipMemcpy(bumps.data(), f->bumps, uint64_t(s.path_count) * 4, hipMemcpyDeviceToHost));
+        for (uint32_t b : bumps) emitted += b;
+    }
+    out->segments_emitted = emitted;
+    if (emitted > s.segment_capacity) out->overflow |= 8u;
+    out->device_bytes = f->device_bytes;
+    return TK_OK;
+}
+
+tk_status tk_render_to_image(tk_context *ctx, const tk_scene *scene, uint32_t *rgba, size_t pixel_count) {
+    tk_frame *f = nullptr;
+    TK_TRY(tk_frame_create(ctx, scene, &f));
+    tk_status st = tk_frame_render(f);
+    if (st == TK_OK) {
+        tk_frame_stats stats;
+        st = tk_frame_stats_read(f, &stats);
+        if (st == TK_OK && stats.overflow) st = fail(TK_ERR_OVERFLOW, "a pipeline capacity was exceeded");
+    }
+    if (st == TK_OK) st = tk_frame_read_image(f, rgba, pixel_count);
+    tk_frame_destroy(f);
+    return st;
+}
+
+// --- stage entry points -----------------------------------------------------
+
+tk_status tk_scan_clear(tk_context *ctx, const tk_scan_config *cfg, void *backdrops, void *ranges, void *counts,
+                        void *cursors, void *bumps, void *chunk_totals, void *chunk_offsets) {
+    if (!ctx || !cfg) return fail(TK_ERR_INVALID, "null argument");
+    if (cfg->incremental) return fail(TK_ERR_INVALID, "incremental scan is not supported");
+    tk_args_scan_clear a{*cfg,
+                         static_cast<int32_t *>(backdrops),
+                         static_cast<tk_segment_range *>(ranges),
+                         static_cast<uint32_t *>(counts),
+                         static_cast<uint32_t *>(cursors),
+                         static_cast<uint32_t *>(bumps),
+                         static_cast<uint32_t *>(chunk_totals),
+                         static_cast<uint32_t *>(chunk_offsets)};
+    return launch(ctx, ctx->f_scan_clear, div_up(cfg->clear_len, 256), 1, 256, a);
+}
+
+tk_status tk_scan_count(tk_context *ctx, const tk_scan_config *cfg, const tk_line *lines, const tk_path_record *paths,
+                        uint32_t path_record_count, int32_t *backdrops, uint32_t *counts) {
+    if (!ctx || !cfg) return fail(TK_ERR_INVALID, "null argument");
+    if (cfg->incremental) return fail(TK_ERR_INVALID, "incremental scan is not supported");
+    if (cfg->line_count && (!lines || !backdrops || !counts)) return fail(TK_ERR_INVALID, "null buffer");
+    return run_scan_count(ctx, *cfg, lines, paths, path_record_count, backdrops, counts);
+}
+
+tk_status tk_scan_prefix(tk_context *ctx, const tk_scan_config *cfg, const tk_scan_chunk *chunks,
+                         const tk_scan_chunk_range *chunk_ranges, const tk_path_record *paths,
+                         const uint32_t *counts, tk_segment_range *ranges, uint32_t *cursors, uint32_t *bumps,
+                         uint32_t *chunk_totals, uint32_t *chunk_offsets) {
+    if (!ctx || !cfg) return fail(TK_ERR_INVALID, "null argument");
+    if (cfg->incremental) return fail(TK_ERR_INVALID, "incremental scan is not supported");
+    TK_TRY(ensure_status(ctx, std::max<uint32_t>(cfg->scan_chunk_count, 1)));
+    return run_scan_prefix(ctx, *cfg, chunks, chunk_ranges, paths, counts, ranges, cursors, bumps, chunk_totals,
+                           chunk_offsets, ctx->status, &ctx->tickets[0], ctx->next_epoch());
+}
+
+tk_status tk_scan_emit(tk_context *ctx, const tk_scan_config *cfg, const tk_line *lines, const tk_path_record *paths,
+                       uint32_t path_record_count, uint32_t *cursors, tk_line_segment *segments) {
+    if (!ctx || !cfg) return fail(TK_ERR_INVALID, "null argument");
+    if (cfg->incremental) return fail(TK_ERR_INVALID, "incremental scan is not supported");
+    return run_scan_emit(ctx, *cfg, lines, paths, path_record_count, cursors, segments);
+}
+
+tk_status tk_cumsum(tk_context *ctx, const tk_cumsum_config *cfg, const uint32_t *offs, const uint32_t *lens,
+                    const uint32_t *row_starts, const uint32_t *row_ends, int32_t *backdrops, uint32_t *totals,
+                    uint32_t *carries) {
+    if (!ctx || !cfg) return fail(TK_ERR_INVALID, "null argument");
+    return run_cumsum(ctx, *cfg, offs, lens, row_starts, row_ends, backdrops, totals, carries);
+}
+
+tk_status tk_coarse(tk_context *ctx, const tk_coarse_config *cfg, const tk_coarse_inputs *in,
+                    const tk_coarse_outputs *out) {
+    if (!ctx || !cfg || !in || !out) return fail(TK_ERR_INVALID, "null argument");
+    if (uint64_t(cfg->tiles_width) * cfg->tiles_height != cfg->tile_count)
+        return fail(TK_ERR_INVALID, "tile_count must equal tiles_width * tiles_height");
+    TK_TRY(ensure_status(ctx, coarse_scan_blocks(*cfg)));
+    return run_coarse(ctx, *cfg, *in, *out, ctx->status, &ctx->tickets[1], ctx->next_epoch(), nullptr);
+}
+
+tk_status tk_fine(tk_context *ctx, const tk_fine_config *cfg, const tk_fine_inputs *in, uint32_t *rgba) {
+    if (!ctx || !cfg || !in || !rgba) return fail(TK_ERR_INVALID, "null argument");
+    if (uint64_t(cfg->tiles_width) * cfg->tiles_height != cfg->tile_count)
+        return fail(TK_ERR_INVALID, "tile_count must equal tiles_width * tiles_height");
+    return run_fine(ctx, *cfg, *in, rgba);
+}
+
+tk_status tk_fill_coverage_words(tk_context *ctx, uint32_t count, const uint32_t *requests,
+                                 const tk_line_segment *segments, uint32_t *alpha_out, float *coverage_out) {
+    if (!ctx || (count && (!requests || !alpha_out))) return fail(TK_ERR_INVALID, "null argument");
+    tk_args_fill_coverage a{count, requests, segments, alpha_out, coverage_out};
+    return launch(ctx, ctx->f_fill_coverage, div_up(count, 256), 1, 256, a);
+}
+
+tk_status tk_pixel_math_words(tk_context *ctx, uint32_t count, const uint32_t *requests, uint32_t *out) {
+    if (!ctx || (count && (!requests || !out))) return fail(TK_ERR_INVALID, "null argument");
+    tk_args_pixel_math a{count, requests, out};
+    return launch(ctx, ctx->f_pixel_math, div_up(count, 256), 1, 256, a);
+}
+
+}  // extern "C"

@@ -1,0 +1,102 @@
+// Kernel argument blocks shared by the gfx950 code object and the host launcher.
+// Each kernel takes exactly one of these structs by value; the host passes it as
+// the single entry of hipModuleLaunchKernel's kernelParams array.
+#pragma once
+
+#include <stdint.h>
+
+#include "tileink_hip.h"
+
+#define TK_FINE_MAX_CLIP_SPILL 60u  // clip stack levels beyond the 4 register slots, held in LDS
+
+struct tk_args_fill {
+    uint32_t *dst;
+    uint64_t count;
+    uint32_t value;
+};
+
+struct tk_args_scan_clear {
+    tk_scan_config cfg;
+    int32_t *backdrops;
+    tk_segment_range *ranges;
+    uint32_t *counts;
+    uint32_t *cursors;
+    uint32_t *bumps;
+    uint32_t *chunk_totals;
+    uint32_t *chunk_offsets;
+};
+
+struct tk_args_scan_lines {
+    tk_scan_config cfg;
+    const tk_line *lines;
+    const tk_path_record *paths;
+    uint32_t path_record_count;
+    int32_t *backdrops;          // count only
+    uint32_t *counts_or_cursors; // count: segment counts, emit: cursors
+    tk_line_segment *segments;   // emit only
+};
+
+struct tk_args_scan_prefix {
+    tk_scan_config cfg;
+    const tk_scan_chunk *chunks;
+    const tk_scan_chunk_range *chunk_ranges;
+    const tk_path_record *paths;
+    const uint32_t *counts;
+    tk_segment_range *ranges;
+    uint32_t *cursors;
+    uint32_t *bumps;
+    uint32_t *chunk_totals;
+    uint32_t *chunk_offsets;
+    unsigned long long *status;  // one word per chunk: value | flag << 32 | epoch << 34
+    uint32_t *ticket;
+    uint32_t epoch;
+    uint32_t block_count;
+};
+
+struct tk_args_segment_sort {
+    const tk_segment_range *ranges;
+    uint32_t backdrop_len;
+    uint32_t segment_capacity;
+    tk_line_segment *segments;
+};
+
+struct tk_args_cumsum {
+    tk_cumsum_config cfg;
+    const uint32_t *chunk_offsets_in;  // backdrop offset per chunk
+    const uint32_t *chunk_lens;
+    const uint32_t *row_starts;
+    const uint32_t *row_ends;
+    int32_t *backdrops;
+    uint32_t *chunk_totals;
+    uint32_t *chunk_carries;
+};
+
+struct tk_args_coarse {
+    tk_coarse_config cfg;
+    tk_coarse_inputs in;
+    tk_coarse_outputs out;
+    unsigned long long *scan_status;  // per scan block
+    uint32_t *ticket;
+    uint32_t epoch;
+    uint32_t scan_blocks;
+};
+
+struct tk_args_fine {
+    tk_fine_config cfg;
+    tk_fine_inputs in;
+    uint32_t *rgba;
+};
+
+struct tk_args_fill_coverage {
+    uint32_t count;
+    const uint32_t *requests;
+    const tk_line_segment *segments;
+    uint32_t *alpha_out;
+    float *coverage_out;
+};
+
+struct tk_args_pixel_math {
+    uint32_t count;
+    const uint32_t *requests;
+    uint32_t *out;
+};

@@ -1,0 +1,115 @@
+"""Seeded synthetic scenes for the benchmark and parity workloads.
+
+Each generator records a scene through the Canvas API, the way an application
+would, from a fixed seed (0x7117E1C0 + workload number).  Geometry, paint and
+layer schedules are fixed by the seed, so the oracle and the gfx950 pipeline
+see identical inputs.
+"""
+import math
+
+import numpy as np
+
+from . import BezPath, Canvas, Color, FillRule, LinearGradient
+
+SEED_BASE = 0x7117E1C0
+
+
+def _rng(workload, salt=0):
+    return np.random.Generator(np.random.PCG64(SEED_BASE + workload + 1000003 * salt))
+
+
+def _color(rng, alpha_lo=0.3):
+    r, g, b = rng.uniform(0.0, 1.0, 3)
+    return Color(r, g, b, rng.uniform(alpha_lo, 1.0))
+
+
+def _cubic_path(rng, n_cubics, lo, hi, closed):
+    """A contour of n cubic segments with points uniform in the box [lo, hi]."""
+    pts = rng.uniform(lo, hi, size=(3 * n_cubics + 1, 2))
+    p = BezPath().move_to(*pts[0])
+    for i in range(n_cubics):
+        a, b, c = pts[3 * i + 1], pts[3 * i + 2], pts[3 * i + 3]
+        if closed and i == n_cubics - 1:
+            c = pts[0]
+        p.curve_to(a[0], a[1], b[0], b[1], c[0], c[1])
+    if closed:
+        p.close_path()
+    return p
+
+
+def _gradient(rng, lo, hi):
+    n = int(rng.integers(2, 5))
+    offsets = np.sort(rng.uniform(0.0, 1.0, n))
+    offsets[0], offsets[-1] = 0.0, 1.0
+    start = rng.uniform(lo, hi)
+    end = rng.uniform(lo, hi)
+    return LinearGradient(tuple(start), tuple(end), [(float(o), _color(rng)) for o in offsets],
+                          extend=int(rng.integers(0, 3)))
+
+
+def config1(width=640, height=360):
+    """640x360: 256 rounded rects, 256 filled closed cubic paths, 64 stroked paths, shuffled draw order."""
+    rng = _rng(1)
+    canvas = Canvas(width, height, 1.0)
+    kinds = np.array([0] * 256 + [1] * 256 + [2] * 64)
+    rng.shuffle(kinds)
+    size = np.array([width, height], dtype=np.float64)
+    for kind in kinds:
+        if kind == 0:
+            c = rng.uniform(0.0, 1.0, 2) * size
+            w, h = rng.uniform(8.0, 160.0, 2)
+            canvas.push_rect(c[0] - w / 2, c[1] - h / 2, c[0] + w / 2, c[1] + h / 2, float(rng.uniform(0.0, 24.0)),
+                             _color(rng))
+        elif kind == 1:
+            path = _cubic_path(rng, int(rng.integers(4, 13)), (0.0, 0.0), tuple(size), True)
+            rule = FillRule.EvenOdd if rng.uniform() < 0.5 else FillRule.NonZero
+            canvas.push_path(path, _color(rng), rule=rule, tolerance=0.25)
+        else:
+            path = _cubic_path(rng, int(rng.integers(4, 13)), (0.0, 0.0), tuple(size), True)
+            width_px = float(rng.lognormal(math.log(2.0), 0.5))
+            canvas.push_stroke(path, width_px, _color(rng), tolerance=0.25)
+    return canvas
+
+
+def config2(width=4096, height=4096, paths=200_000, clips=32, workload=2):
+    """4096^2, 200k paths (60% fill, 40% stroke), 4-16 cubics each, 10% linear gradients,
+    `clips` nested rounded-rect clip layers pushed then popped between equal draw groups."""
+    rng = _rng(workload)
+    canvas = Canvas(width, height, 1.0)
+    groups = 2 * clips + 1
+    per_group = np.full(groups, paths // groups)
+    per_group[: paths - per_group.sum()] += 1
+    clip_box = np.array([0.0, 0.0, float(width), float(height)])
+    drawn = 0
+    for g in range(groups):
+        for _ in range(int(per_group[g])):
+            anchor = rng.uniform(0.0, 1.0, 2) * (width, height)
+            extent = float(rng.lognormal(math.log(48.0), 1.0))
+            lo = anchor - extent / 2
+            hi = anchor + extent / 2
+            n = int(rng.integers(4, 17))
+            fill = rng.uniform() < 0.6
+            brush = _gradient(rng, lo, hi) if rng.uniform() < 0.1 else _color(rng)
+            if fill:
+                rule = FillRule.EvenOdd if rng.uniform() < 0.5 else FillRule.NonZero
+                canvas.push_path(_cubic_path(rng, n, lo, hi, True), brush, rule=rule, tolerance=0.25)
+            else:
+                w = float(rng.lognormal(math.log(1.5), 0.7))
+                canvas.push_stroke(_cubic_path(rng, n, lo, hi, False), w, brush, tolerance=0.25)
+            drawn += 1
+        if g < clips:
+            # Each clip sits inside the previous one, each side pulled in by up to 3% of the current size.
+            w, h = clip_box[2] - clip_box[0], clip_box[3] - clip_box[1]
+            m = rng.uniform(0.0, 0.03, 4) * (w, h, w, h)
+            clip_box = clip_box + (m[0], m[1], -m[2], -m[3])
+            canvas.push_clip_sdf_rect_layer(*clip_box, float(rng.uniform(0.0, 64.0)))
+        elif g < 2 * clips:
+            canvas.pop_layer()
+    return canvas
+
+
+def config2_small(scale=8, paths=None):
+    """Config-2 generator on a (4096/scale)^2 canvas with proportionally fewer paths (parity cases)."""
+    side = 4096 // scale
+    n = paths if paths is not None else max(64, 200_000 // (scale * scale))
+    return config2(side, side, n, clips=min(32, max(2, n // 64)), workload=20 + scale)
