@@ -337,7 +337,7 @@ tk_status tk_frame_set_capture(tk_frame *frame, int enable);
 
 /* Per-stage event timings of the last tk_frame_render, milliseconds. */
 typedef struct tk_stage_times {
-    float scan_clear, scan_count, scan_prefix, scan_emit, segment_order, cumsum;
+    float scan_clear, scan_count, scan_prefix, scan_emit, cumsum;
     float coarse_count, coarse_prefix, coarse_emit, fine, total;
 } tk_stage_times;
 /* Enables hipEvent timestamps around each stage of later renders. */
@@ -423,7 +423,9 @@ tk_status tk_coarse(tk_context *ctx, const tk_coarse_config *config, const tk_co
 /* Fine, all batches at once: one 256-lane workgroup per 16x16 tile walks the
  * batches in order, re-quantising to RGBA8 at each batch boundary exactly where
  * the reference stores and re-loads its RGBA8 target (fine/main.hlsl:23-40,
- * fine/interpreter.hlsli:21-219, fine/specialized.hlsli:19-111). */
+ * fine/interpreter.hlsli:21-219, fine/specialized.hlsli:19-111).  Each fill's
+ * segments are staged in LDS in canonical (bit-pattern) order before the row
+ * sweep, so the result does not depend on emit's atomic slot order. */
 typedef struct tk_fine_config {
     uint32_t width, height, clear_color, tile_count;
     uint32_t tiles_width, tiles_height, batch_count, clip_spill_depth;
@@ -438,6 +440,8 @@ typedef struct tk_fine_inputs {
     const uint32_t *tile_kinds;
     const uint32_t *batch_ptcl_bases;
     const tk_ptcl_record *ptcl;
+    uint32_t *status;  /* optional; bit 0 set when a fill's segment list exceeded the
+                          1024-entry LDS stage and was summed in stored order */
 } tk_fine_inputs;
 tk_status tk_fine(tk_context *ctx, const tk_fine_config *config, const tk_fine_inputs *in,
                   uint32_t *rgba_out);

@@ -51,8 +51,8 @@ def compare_frame(ctx, canvas, clear=0):
             bad.append(key)
     if not np.array_equal(gpu["segment_ranges"], ref["segment_ranges"]):
         bad.append("segment_ranges")
-    used = int(gpu["segment_bumps"].sum(dtype=np.uint64))
-    if not np.array_equal(gpu["segments"][:used], ref["segments"][:used]):
+    if not np.array_equal(tile_sorted_segments(gpu["segments"], gpu["segment_ranges"]),
+                          tile_sorted_segments(ref["segments"], ref["segment_ranges"])):
         bad.append("segments")
     if not np.array_equal(gpu["batch_bases"], ref["batch_bases"]):
         bad.append("batch_bases")
@@ -68,6 +68,20 @@ def compare_frame(ctx, canvas, clear=0):
     if gpu["stats"]["overflow"]:
         bad.append("overflow")
     return gpu, ref, bad
+
+
+def tile_sorted_segments(segments, ranges):
+    """Each backdrop tile's segments sorted by their five words, tiles in order: the
+    segment multiset per tile, independent of emit's atomic slot order."""
+    ranges = np.asarray(ranges, np.int64).reshape(-1, 2)
+    counts = np.clip(ranges[:, 1] - ranges[:, 0], 0, None)
+    total = int(counts.sum())
+    tile = np.repeat(np.arange(len(ranges)), counts)
+    first = np.repeat(ranges[:, 0], counts)
+    offset = np.arange(total) - np.repeat(np.cumsum(counts) - counts, counts)
+    rows = np.asarray(segments, np.uint32).reshape(-1, 5)[first + offset]
+    order = np.lexsort((rows[:, 4], rows[:, 3], rows[:, 2], rows[:, 1], rows[:, 0], tile))
+    return np.concatenate([tile[order, None].astype(np.uint32), rows[order]], axis=1)
 
 
 def channel_diff(a, b):

@@ -1,0 +1,63 @@
+"""The C ABI library (CPU): it loads, exports every entry point include/tileink_hip.h
+declares, and its record layouts match the reference's GPU ABI."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import tileink_amd as tk
+from tileink_amd import _abi
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "tileink_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tk_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_entry_point():
+    lib = C.CDLL(_abi.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_header():
+    assert set(declared_functions()) == set(_abi.SIGNATURES)
+
+
+def test_record_layouts_match_reference_abi():
+    assert C.sizeof(_abi.Line) == 24
+    assert C.sizeof(_abi.PathRecord) == 76
+    assert _abi.PathRecord.segment_start.offset == 40
+    assert _abi.PathRecord.transform.offset == 52
+    assert C.sizeof(_abi.DrawRecord) == 124
+    assert _abi.DrawRecord.tag.offset == 32 and _abi.DrawRecord.fill_rule.offset == 36
+    assert _abi.DrawRecord.pixel_bounds.offset == 40 and _abi.DrawRecord.solid_rect.offset == 72
+    assert _abi.DrawRecord.transform.offset == 76 and _abi.DrawRecord.inverse_transform.offset == 100
+    assert _abi.SEGMENT_DTYPE.itemsize == 20 and _abi.PTCL_DTYPE.itemsize == 24
+    assert C.sizeof(_abi.ScanConfig) == 44
+
+
+def test_missing_device_is_an_error_not_a_fallback():
+    with pytest.raises(tk.TileinkError) as err:
+        tk.NativeContext(-1)
+    assert err.value.status == _abi.TK_ERR_UNAVAILABLE
+
+
+def test_renderer_rejects_other_backends():
+    class Vulkan:
+        pass
+    with pytest.raises(tk.TileinkError):
+        tk.NativeRenderer(Vulkan, 16, 16)
+
+
+def test_version_and_error_string():
+    assert _abi.lib().tk_version() >= 1
+    with pytest.raises(tk.TileinkError):
+        tk.Canvas(16, 16).pop_layer()
+    assert b"layer" in _abi.lib().tk_last_error()

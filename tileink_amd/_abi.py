@@ -113,8 +113,7 @@ class FrameStats(C.Structure):
 
 
 class StageTimes(C.Structure):
-    _fields_ = [(n, C.c_float) for n in ("scan_clear", "scan_count", "scan_prefix", "scan_emit", "segment_order",
-                                         "cumsum", "coarse_count", "coarse_prefix", "coarse_emit", "fine", "total")]
+    _fields_ = [(n, C.c_float) for n in ("scan_clear", "scan_count", "scan_prefix", "scan_emit", "cumsum", "coarse_count", "coarse_prefix", "coarse_emit", "fine", "total")]
 
 
 class CoarseConfig(C.Structure):
@@ -142,7 +141,7 @@ class FineConfig(C.Structure):
 
 class FineInputs(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("draws", "paint", "segments", "tile_ptcl_ranges", "tile_kinds",
-                                          "batch_ptcl_bases", "ptcl")]
+                                          "batch_ptcl_bases", "ptcl", "status")]
 
 
 # numpy dtypes for reading record arrays back

@@ -10,7 +10,7 @@
 //   scan_count                       -> tk_k_scan_count
 //   scan_prefix_chunks + scan_chunk_offsets + scan_apply_chunk_offsets
 //                                    -> tk_k_scan_prefix (single pass, decoupled lookback)
-//   scan_emit                        -> tk_k_scan_emit (+ tk_k_segment_order: canonical slot order)
+//   scan_emit                        -> tk_k_scan_emit (slot order canonicalised when fine stages a tile)
 //   cumsum_*                         -> tk_k_cumsum
 //   coarse_count_bins   (x batches)  -> tk_k_coarse_count  (all batches, one pass over each tile list)
 //   coarse_prefix_*     (x batches)  -> tk_k_coarse_scan   (one decoupled-lookback pass over batch x tile)
@@ -485,6 +485,63 @@ __device__ __forceinline__ float fill_coverage(const tk_line_segment *segments, 
     return base + running + partial;
 }
 
+// Segments of one FILL record staged in LDS per 16x16 tile workgroup.  Emit
+// claims slots with atomics, so the stored order of a tile's segments varies
+// from run to run; staging sorts them by their bit patterns (a rank sort over
+// LDS broadcast reads), which fixes the summation order of the row sweep to a
+// function of the geometry alone.
+constexpr uint32_t kFineSegCap = 1024u;
+constexpr uint32_t kFineSegPerLane = kFineSegCap / 256u;
+
+__device__ __forceinline__ bool segment_key_less(const tk_line_segment &x, const tk_line_segment &y) {
+    const uint32_t a[5] = {asuint(x.p0x), asuint(x.p0y), asuint(x.p1x), asuint(x.p1y), asuint(x.y_edge)};
+    const uint32_t b[5] = {asuint(y.p0x), asuint(y.p0y), asuint(y.p1x), asuint(y.p1y), asuint(y.y_edge)};
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+        if (a[i] != b[i]) return a[i] < b[i];
+    return false;
+}
+
+// Loads segments [0, n) of `src` into `lds` in canonical order (n <= kFineSegCap).
+// Every lane of the workgroup must call it with the same n.
+__device__ void stage_sorted_segments(const tk_line_segment *src, uint32_t n, tk_line_segment *lds, uint32_t lane) {
+    tk_line_segment mine[kFineSegPerLane];
+    __syncthreads();  // earlier readers of lds are done
+#pragma unroll
+    for (uint32_t m = 0; m < kFineSegPerLane; ++m) {
+        uint32_t k = lane + 256u * m;
+        if (k < n) {
+            mine[m] = src[k];
+            lds[k] = mine[m];
+        }
+    }
+    __syncthreads();
+    if (n < 2u) return;
+    uint32_t rank[kFineSegPerLane] = {};
+    for (uint32_t j = 0; j < n; ++j) {
+        const tk_line_segment other = lds[j];
+#pragma unroll
+        for (uint32_t m = 0; m < kFineSegPerLane; ++m) {
+            uint32_t k = lane + 256u * m;
+            if (k < n && (segment_key_less(other, mine[m]) || (!segment_key_less(mine[m], other) && j < k))) rank[m]++;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t m = 0; m < kFineSegPerLane; ++m) {
+        uint32_t k = lane + 256u * m;
+        if (k < n) lds[rank[m]] = mine[m];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ float lds_coverage(const tk_line_segment *lds, uint32_t n, int32_t backdrop, uint32_t x,
+                                              uint32_t y) {
+    float base = (float)backdrop, running = 0.0f, partial = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) accumulate_segment(lds[i], x, y, base, running, partial);
+    return base + running + partial;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -545,31 +602,60 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_count(tk_args_scan_l
     }
 }
 
+namespace {
+// Exclusive wrapping-u32 scan across one 64-lane wave.
+__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t lane, uint32_t &total) {
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+}  // namespace
+
 // scan_prefix_chunks + scan_chunk_offsets + scan_apply_chunk_offsets as one
-// pass: each workgroup scans one 256-tile chunk, then finds its path-relative
-// base by decoupled lookback over the path's earlier chunks.
+// pass: each 64-lane wave scans one 256-tile chunk (four tiles per lane), then
+// finds its path-relative base by decoupled lookback over the path's earlier
+// chunks.  Wave w of the launch takes ticket w in dispatch order, so a wave only
+// waits on chunks whose waves are already running.
 extern "C" __global__ __launch_bounds__(256) void tk_k_scan_prefix(tk_args_scan_prefix a) {
-    __shared__ uint32_t s_ticket, s_base;
-    __shared__ uint32_t s_wave[4];
-    const uint32_t lane = threadIdx.x;
-    const uint32_t k = draw_ticket(a.ticket, a.block_count, &s_ticket);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave_id = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (wave_id >= a.block_count) return;
+    uint32_t t = 0u;
+    if (lane == 0u) {
+        t = atomicAdd(a.ticket, 1u);
+        if (t == a.block_count - 1u) atomicExch(a.ticket, 0u);
+    }
+    const uint32_t k = (uint32_t)__shfl((int)t, 0, 64);
 
     // Paths without chunks still receive a zero bump (scan_chunk_offsets runs per path).
-    uint32_t p = k * 256u + lane;
+    const uint32_t p = k * 64u + lane;
     if (p < a.cfg.path_count) {
-        tk_scan_chunk_range r = a.chunk_ranges[p];
+        const tk_scan_chunk_range r = a.chunk_ranges[p];
         if (r.start >= r.end) a.bumps[p] = 0u;
     }
     if (k >= a.cfg.scan_chunk_count) return;
 
-    tk_scan_chunk chunk = a.chunks[k];
-    uint32_t len = chunk.len;
-    uint32_t count = lane < len ? a.counts[chunk.backdrop_offset + lane] : 0u;
+    const tk_scan_chunk chunk = a.chunks[k];
+    const uint32_t len = min(chunk.len, 256u);
+    uint32_t v[4];
+    uint32_t mine = 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) {
+        const uint32_t e = lane * 4u + i;
+        v[i] = e < len ? a.counts[chunk.backdrop_offset + e] : 0u;
+        mine += v[i];
+    }
     uint32_t total;
-    uint32_t local_excl = block_exclusive_scan_256(count, s_wave, total);
+    const uint32_t excl = wave_exclusive_scan(mine, lane, total);
 
+    uint32_t base = 0u;
     if (lane == 0u) {
-        uint32_t path = chunk.path_id;
+        const uint32_t path = chunk.path_id;
         bool in_range = false;
         tk_scan_chunk_range r{0u, 0u};
         uint32_t seg_start = 0u;
@@ -578,14 +664,13 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_prefix(tk_args_scan_
             in_range = k >= r.start && k < r.end;
             seg_start = a.paths[path].segment_start;
         }
-        uint32_t base;
         if (in_range) {
             if (k == r.start) {
                 base = seg_start;
             } else {
                 publish(a.status, k, total, kFlagAggregate, a.epoch);
                 uint32_t acc;
-                bool found = lookback(a.status, k, r.start, a.epoch, acc);
+                const bool found = lookback(a.status, k, r.start, a.epoch, acc);
                 base = found ? acc : acc + seg_start;
             }
             publish(a.status, k, base + total, kFlagPrefix, a.epoch);
@@ -596,15 +681,19 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_prefix(tk_args_scan_
             base = a.chunk_offsets[k];
             publish(a.status, k, base + total, kFlagPrefix, a.epoch);
         }
-        s_base = base;
+        if (len > 0u) a.chunk_totals[k] = total;
     }
-    __syncthreads();
-    if (lane < len) {
-        uint32_t start = s_base + local_excl;
-        a.ranges[chunk.backdrop_offset + lane] = tk_segment_range{start, start + count};
-        a.cursors[chunk.backdrop_offset + lane] = start;
+    base = (uint32_t)__shfl((int)base, 0, 64);
+    uint32_t running = base + excl;
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) {
+        const uint32_t e = lane * 4u + i;
+        if (e < len) {
+            a.ranges[chunk.backdrop_offset + e] = tk_segment_range{running, running + v[i]};
+            a.cursors[chunk.backdrop_offset + e] = running;
+        }
+        running += v[i];
     }
-    if (lane + 1u == len) a.chunk_totals[k] = total;
 }
 
 // scan_emit (scan/emit.hlsl:14-28).
@@ -621,41 +710,6 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_emit(tk_args_scan_li
             uint32_t dst = atomicAdd(&a.counts_or_cursors[scan_tile_offset(s, tx, ty)], 1u);
             if (dst < a.cfg.segment_capacity) a.segments[dst] = clipped_segment(s, i, z, tx, ty);
         }
-    }
-}
-
-namespace {
-__device__ __forceinline__ bool segment_less(const tk_line_segment &x, const tk_line_segment &y) {
-    uint32_t a[5] = {asuint(x.p0x), asuint(x.p0y), asuint(x.p1x), asuint(x.p1y), asuint(x.y_edge)};
-    uint32_t b[5] = {asuint(y.p0x), asuint(y.p0y), asuint(y.p1x), asuint(y.p1y), asuint(y.y_edge)};
-#pragma unroll
-    for (int i = 0; i < 5; ++i)
-        if (a[i] != b[i]) return a[i] < b[i];
-    return false;
-}
-}  // namespace
-
-// Canonical order of each backdrop tile's segments: emit claims slots with
-// atomics, so the stored order varies run to run; sorting every tile's slots by
-// the segments' bit patterns makes the stored order (and hence the fine stage's
-// summation order) a function of the geometry alone.
-extern "C" __global__ __launch_bounds__(256) void tk_k_segment_order(tk_args_segment_sort a) {
-    uint32_t b = blockIdx.x * 256u + threadIdx.x;
-    if (b >= a.backdrop_len) return;
-    tk_segment_range r = a.ranges[b];
-    uint32_t start = r.start, end = min(r.end, a.segment_capacity);
-    if (end <= start || end - start < 2u) return;
-    tk_line_segment *seg = a.segments;
-    for (uint32_t i = start + 1u; i < end; ++i) {
-        tk_line_segment cur = seg[i];
-        uint32_t j = i;
-        while (j > start) {
-            tk_line_segment prev = seg[j - 1u];
-            if (!segment_less(cur, prev)) break;
-            seg[j] = prev;
-            --j;
-        }
-        seg[j] = cur;
     }
 }
 
@@ -680,49 +734,184 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_cumsum(tk_args_cumsum a) 
     }
 }
 
-// coarse_count_bins for every batch: one lane per tile walks the tile's
-// painter-ordered draw list once; batches appear as contiguous runs.
-extern "C" __global__ __launch_bounds__(64) void tk_k_coarse_count(tk_args_coarse a) {
+// ---------------------------------------------------------------------------
+// Coarse: one 64-lane wave per tile.  Lanes take 64 consecutive entries of the
+// tile's painter-ordered draw list at a time; a DrawBatch is a contiguous run of
+// entries, so per-batch particle counts and emission offsets come from ballots.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t lanes_below(uint32_t lane) { return lane ? (~0ull >> (64u - lane)) : 0ull; }
+
+// stack_wrapper_count with the layers of one batch evaluated across the wave:
+// the stack is invalid when any layer rejects the tile, otherwise the count of
+// layers that are not no-ops.
+__device__ uint32_t stack_wrapper_count_wave(const CoarseCtx &c, uint32_t batch, uint32_t lane) {
+    const tk_coarse_config &cfg = *c.cfg;
+    const tk_coarse_inputs &in = *c.in;
+    const tk_batch_record br = in.batches[batch];
+    const uint32_t end = min(br.layer_end, cfg.layer_count);
+    uint32_t count = 0u;
+    for (uint32_t base = br.layer_start; base < end; base += 64u) {
+        const uint32_t i = base + lane;
+        bool invalid = false, counted = false;
+        if (i < end) {
+            const tk_layer_record layer = in.layers[i];
+            const DrawData d = load_draw(in.draws, cfg.draw_count, layer.draw);
+            if (draw_has_sdf(d)) {
+                if (!sdf_clip_covers(in.paint, cfg.paint_words, d, c.tx, c.ty)) {
+                    if (!draw_hits_tile(d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height)) invalid = true;
+                    else counted = true;
+                }
+            } else {
+                const uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
+                if (bd == kNone) {
+                    invalid = true;
+                } else {
+                    const tk_segment_range range = in.segment_ranges[bd];
+                    const int32_t winding = in.backdrops[bd];
+                    const bool full = d.fill_rule == TK_FILL_EVEN_ODD ? (((uint32_t)winding & 1u) != 0u) : winding != 0;
+                    if (!(layer.tag == TK_LAYER_CLIP && range.start == range.end && full)) {
+                        if (range.start == range.end && winding == 0) invalid = true;
+                        else counted = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(invalid)) return kNone;
+        count += (uint32_t)__popcll(__ballot(counted));
+    }
+    return count;
+}
+
+// Writes the stack's BEGIN records (forward layer order) from dst; returns the next slot.
+__device__ uint32_t emit_stack_begins_wave(const CoarseCtx &c, uint32_t batch, Emitter &e, uint32_t dst, uint32_t lane) {
+    const tk_batch_record br = c.in->batches[batch];
+    const uint32_t end = min(br.layer_end, c.cfg->layer_count);
+    for (uint32_t base = br.layer_start; base < end; base += 64u) {
+        const uint32_t i = base + lane;
+        bool emit = false;
+        tk_layer_record layer{};
+        DrawData d{};
+        if (i < end) {
+            layer = c.in->layers[i];
+            d = load_draw(c.in->draws, c.cfg->draw_count, layer.draw);
+            emit = !noop_stack_clip(c, layer, d);
+        }
+        const uint64_t mask = __ballot(emit);
+        if (emit) {
+            const uint32_t slot = dst + (uint32_t)__popcll(mask & lanes_below(lane));
+            if (draw_has_sdf(d)) {
+                e.store(slot, TK_PTCL_BEGIN_SDF_CLIP, 0u, 0u, 0u, 0u, layer.draw);
+            } else {
+                const uint32_t bd = draw_backdrop(c.in->paths, c.cfg->path_count, d, c.tx, c.ty, c.cfg->tiles_width,
+                                                  c.cfg->tiles_height);
+                const uint32_t tag =
+                    layer.tag == 1u ? kPtclBeginOpacity : (layer.tag == 2u ? kPtclBeginBlend : TK_PTCL_BEGIN_CLIP);
+                tk_segment_range range{0u, 0u};
+                uint32_t winding = 0u;
+                if (bd != kNone) {
+                    range = c.in->segment_ranges[bd];
+                    winding = (uint32_t)c.in->backdrops[bd];
+                }
+                e.store(slot, tag, winding, d.fill_rule, range.start, range.end, layer.payload);
+            }
+        }
+        dst += (uint32_t)__popcll(mask);
+    }
+    return dst;
+}
+
+// Writes the stack's END records (reverse layer order) from dst; returns the next slot.
+__device__ uint32_t emit_stack_ends_wave(const CoarseCtx &c, uint32_t batch, Emitter &e, uint32_t dst, uint32_t lane) {
+    const tk_batch_record br = c.in->batches[batch];
+    const uint32_t end = min(br.layer_end, c.cfg->layer_count);
+    for (uint32_t top = end; top > br.layer_start;) {
+        const uint32_t span = min(64u, top - br.layer_start);
+        // lane k takes layer top-1-k, so lane order is emission order
+        bool emit = false;
+        uint32_t tag = TK_PTCL_END_CLIP;
+        if (lane < span) {
+            const tk_layer_record layer = c.in->layers[top - 1u - lane];
+            const DrawData d = load_draw(c.in->draws, c.cfg->draw_count, layer.draw);
+            emit = !noop_stack_clip(c, layer, d);
+            tag = layer.tag == 1u ? kPtclEndOpacity : (layer.tag == 2u ? kPtclEndBlend : TK_PTCL_END_CLIP);
+        }
+        const uint64_t mask = __ballot(emit);
+        if (emit) e.store(dst + (uint32_t)__popcll(mask & lanes_below(lane)), tag, 0u, 0u, 0u, 0u, 0u);
+        dst += (uint32_t)__popcll(mask);
+        top -= span;
+    }
+    return dst;
+}
+
+// One chunk of up to 64 list entries: lane's draw and batch (kNone when absent).
+__device__ __forceinline__ void load_list_entry(const tk_args_coarse &a, const tk_tile_draw_record &list, uint32_t base,
+                                                uint32_t lane, uint32_t &draw, uint32_t &batch) {
+    draw = kNone;
+    batch = kNone;
+    const uint32_t i = base + lane;
+    if (i < list.len) {
+        const uint32_t slot = list.start + i;
+        if (slot < a.cfg.tile_draw_index_count) {
+            draw = a.in.tile_draw_indices[slot];
+            if (draw < a.cfg.draw_count) batch = a.in.draw_batch_ids[draw];
+            if (batch >= a.cfg.batch_count) batch = kNone;
+        }
+    }
+}
+
+// coarse_count_bins for every batch (coarse/count.hlsl:48-73).
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coarse a) {
     const tk_coarse_config &cfg = a.cfg;
-    uint32_t tile = blockIdx.x * 64u + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t tile = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (tile >= cfg.tile_count) return;
-    CoarseCtx c{&a.cfg, &a.in, tile % cfg.tiles_width, tile / cfg.tiles_width};
+    const CoarseCtx c{&a.cfg, &a.in, tile % cfg.tiles_width, tile / cfg.tiles_width};
     const uint32_t T = cfg.tile_count, Q = cfg.batch_count;
     uint32_t *counts = a.out.tile_ptcl_counts;
-    tk_tile_draw_record list = a.in.tile_draws[tile];
+    const tk_tile_draw_record list = a.in.tile_draws[tile];
+    const uint32_t len = list.start == kNone ? 0u : list.len;
+
     uint32_t cur = kNone, particles = 0u, next = 0u;
+    auto zero_until = [&](uint32_t stop) {
+        for (uint32_t bb = next + lane; bb < stop; bb += 64u) counts[(size_t)bb * T + tile] = 0u;
+    };
     auto flush = [&]() {
         if (cur == kNone) return;
         uint32_t count = 0u;
         if (particles > 0u) {
-            uint32_t w = stack_wrapper_count(c, cur);
+            const uint32_t w = stack_wrapper_count_wave(c, cur, lane);
             if (w != kNone) count = particles + 2u * w + 1u;
         }
-        counts[(size_t)cur * T + tile] = count;
+        if (lane == 0u) counts[(size_t)cur * T + tile] = count;
         next = cur + 1u;
     };
-    if (list.start != kNone) {
-        for (uint32_t i = 0; i < list.len; ++i) {
-            uint32_t slot = list.start + i;
-            if (slot >= cfg.tile_draw_index_count) break;
-            uint32_t d = a.in.tile_draw_indices[slot];
-            uint32_t b = d < cfg.draw_count ? a.in.draw_batch_ids[d] : kNone;
-            if (b >= Q) continue;
-            if (b != cur) {
-                if (cur != kNone && b < cur) {
-                    atomicOr(&a.out.status[3], 1u);  // list not in batch order
+    for (uint32_t base = 0; base < len; base += 64u) {
+        uint32_t d, b;
+        load_list_entry(a, list, base, lane, d, b);
+        const bool valid = b != kNone && draw_particle(c, b, d).valid;
+        uint64_t pending = __ballot(b != kNone);
+        const uint64_t valid_mask = __ballot(valid);
+        while (pending) {
+            const uint32_t first = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+            const uint32_t bb = (uint32_t)__shfl((int)b, (int)first, 64);
+            const uint64_t run = __ballot(b == bb) & pending;
+            pending &= ~run;
+            if (bb != cur) {
+                if (cur != kNone && bb < cur) {  // list not in batch order
+                    if (lane == 0u) atomicOr(&a.out.status[3], 1u);
                     continue;
                 }
                 flush();
-                for (uint32_t bb = next; bb < b; ++bb) counts[(size_t)bb * T + tile] = 0u;
-                cur = b;
+                zero_until(bb);
+                cur = bb;
                 particles = 0u;
             }
-            if (draw_particle(c, b, d).valid) particles++;
+            particles += (uint32_t)__popcll(run & valid_mask);
         }
     }
     flush();
-    for (uint32_t bb = next; bb < Q; ++bb) counts[(size_t)bb * T + tile] = 0u;
+    zero_until(Q);
 }
 
 // coarse_prefix_chunks + coarse_chunk_offsets + coarse_apply_chunk_offsets for
@@ -775,86 +964,111 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coars
 // coarse_emit_bins for every batch (coarse/emit.hlsl:82-142): writes
 // [stack begins][particles][stack ends][END] for each (batch, tile) with a
 // nonzero count and the tile kind word, including the opaque-prefix skip that
-// occluded_particle_prefix (coarse/occlusion.hlsli:12-34) derives.
-extern "C" __global__ __launch_bounds__(64) void tk_k_coarse_emit(tk_args_coarse a) {
+// occluded_particle_prefix (coarse/occlusion.hlsli:12-34) derives: the last
+// opaque colour particle, unless a begin record follows it.
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coarse a) {
     const tk_coarse_config &cfg = a.cfg;
-    uint32_t tile = blockIdx.x * 64u + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t tile = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (tile >= cfg.tile_count) return;
-    CoarseCtx c{&a.cfg, &a.in, tile % cfg.tiles_width, tile / cfg.tiles_width};
+    const CoarseCtx c{&a.cfg, &a.in, tile % cfg.tiles_width, tile / cfg.tiles_width};
     const uint32_t T = cfg.tile_count, Q = cfg.batch_count;
     Emitter e{a.out.ptcl, cfg.ptcl_capacity, a.out.status};
     uint32_t *kinds = a.out.tile_kinds;
-    tk_tile_draw_record list = a.in.tile_draws[tile];
+    const tk_tile_draw_record list = a.in.tile_draws[tile];
+    const uint32_t len = list.start == kNone ? 0u : list.len;
 
+    // Uniform per-batch state.
     uint32_t cur = kNone, next = 0u;
-    bool active = false;  // current batch has a nonzero count
+    bool active = false;
     uint32_t start = 0u, end = 0u, wrappers = 0u, dst = 0u, flags = 0u, last_opaque = 0u;
     bool begin_after = false;
+    auto empty_until = [&](uint32_t stop) {
+        for (uint32_t bb = next + lane; bb < stop; bb += 64u) kinds[(size_t)bb * T + tile] = TK_TILE_KIND_EMPTY;
+    };
     auto finish = [&]() {
         if (cur == kNone) return;
-        size_t idx = (size_t)cur * T + tile;
-        if (!active) {
-            kinds[idx] = TK_TILE_KIND_EMPTY;
-        } else {
-            dst = emit_stack_ends(c, cur, e, dst);
-            e.store(dst, TK_PTCL_END, 0u, 0u, 0u, 0u, 0u);
-            uint32_t count = end - start;
+        uint32_t kind = TK_TILE_KIND_EMPTY;
+        if (active) {
+            dst = emit_stack_ends_wave(c, cur, e, dst, lane);
+            if (lane == 0u) e.store(dst, TK_PTCL_END, 0u, 0u, 0u, 0u, 0u);
             uint32_t skip = 0u;
-            if (wrappers == 0u && count > 2u && !begin_after) skip = last_opaque;
-            if ((flags & 8u) == 0u) skip = 0u;
-            kinds[idx] = classify_tile_flags(flags) | (skip << TK_TILE_PREFIX_SHIFT);
+            if ((flags & 8u) && wrappers == 0u && end - start > 2u && !begin_after) skip = last_opaque;
+            kind = classify_tile_flags(flags) | (skip << TK_TILE_PREFIX_SHIFT);
         }
+        if (lane == 0u) kinds[(size_t)cur * T + tile] = kind;
         next = cur + 1u;
     };
-    auto begin = [&](uint32_t b) {
-        cur = b;
-        tk_segment_range r = a.out.tile_ptcl_ranges[(size_t)b * T + tile];
+    auto begin = [&](uint32_t bb) {
+        cur = bb;
+        const tk_segment_range r = a.out.tile_ptcl_ranges[(size_t)bb * T + tile];
         start = r.start;
         end = r.end;
         active = end > start;
         flags = 0u;
         last_opaque = 0u;
         begin_after = false;
+        wrappers = 0u;
+        dst = start;
         if (active) {
-            wrappers = stack_wrapper_count(c, b);
+            wrappers = stack_wrapper_count_wave(c, bb, lane);
             if (wrappers == kNone) {  // cannot happen with a consistent count; keep the stream terminated
-                e.store(start, TK_PTCL_END, 0u, 0u, 0u, 0u, 0u);
+                if (lane == 0u) e.store(start, TK_PTCL_END, 0u, 0u, 0u, 0u, 0u);
                 active = false;
+                wrappers = 0u;
                 return;
             }
             flags = wrappers != 0u ? kClassOther : 0u;
-            dst = emit_stack_begins(c, b, e, start);
+            dst = emit_stack_begins_wave(c, bb, e, start, lane);
         }
     };
-    if (list.start != kNone) {
-        for (uint32_t i = 0; i < list.len; ++i) {
-            uint32_t slot = list.start + i;
-            if (slot >= cfg.tile_draw_index_count) break;
-            uint32_t d = a.in.tile_draw_indices[slot];
-            uint32_t b = d < cfg.draw_count ? a.in.draw_batch_ids[d] : kNone;
-            if (b >= Q) continue;
-            if (b != cur) {
-                if (cur != kNone && b < cur) continue;
+
+    for (uint32_t base = 0; base < len; base += 64u) {
+        uint32_t d, b;
+        load_list_entry(a, list, base, lane, d, b);
+        uint64_t pending = __ballot(b != kNone);
+        while (pending) {
+            const uint32_t first = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+            const uint32_t bb = (uint32_t)__shfl((int)b, (int)first, 64);
+            const uint64_t run = __ballot(b == bb) & pending;
+            pending &= ~run;
+            if (bb != cur) {
+                if (cur != kNone && bb < cur) continue;
                 finish();
-                for (uint32_t bb = next; bb < b; ++bb) kinds[(size_t)bb * T + tile] = TK_TILE_KIND_EMPTY;
-                begin(b);
+                empty_until(bb);
+                begin(bb);
             }
             if (!active) continue;
-            Particle p = draw_particle(c, b, d);
-            if (!p.valid) continue;
-            uint32_t pos = dst - start;  // relative record index
-            flags |= particle_class_flags(c, d, p.tag);
-            if (wrappers == 0u && p.tag == TK_PTCL_COLOR && (p.color >> 24) == 255u && dst > start) flags |= 8u;
-            if (is_begin_tag(p.tag)) begin_after = true;
-            if (p.tag == TK_PTCL_COLOR && (p.color >> 24) == 255u) {
-                last_opaque = pos;
-                begin_after = false;
+            const bool mine = (run >> lane) & 1ull;
+            Particle p{false, TK_PTCL_FILL, 0u, 0u, 0u, 0u, 0u};
+            if (mine) p = draw_particle(c, bb, d);
+            const uint64_t valid = __ballot(p.valid);
+            const uint32_t pos = dst + (uint32_t)__popcll(valid & lanes_below(lane));
+            uint32_t cls = 0u;
+            bool opaque = false, begins = false;
+            if (p.valid) {
+                cls = particle_class_flags(c, d, p.tag);
+                opaque = p.tag == TK_PTCL_COLOR && (p.color >> 24) == 255u;
+                begins = is_begin_tag(p.tag);
+                if (pos < end) e.store(pos, p.tag, p.winding, p.fill_rule, p.seg_start, p.seg_end, p.color);
             }
-            if (dst < end) e.store(dst++, p.tag, p.winding, p.fill_rule, p.seg_start, p.seg_end, p.color);
+            if (__ballot(cls & kClassColor)) flags |= kClassColor;
+            if (__ballot(cls & kClassSdf)) flags |= kClassSdf;
+            if (__ballot(cls & kClassOther)) flags |= kClassOther;
+            if (__ballot(opaque && wrappers == 0u && pos > start)) flags |= 8u;
+            const uint64_t opaque_mask = __ballot(opaque), begin_mask = __ballot(begins);
+            if (opaque_mask) {
+                const uint32_t h = 63u - (uint32_t)__clzll((long long)opaque_mask);
+                last_opaque = dst + (uint32_t)__popcll(valid & lanes_below(h)) - start;
+                begin_after = (begin_mask >> h) > 1ull;
+            } else if (begin_mask) {
+                begin_after = true;
+            }
+            dst += (uint32_t)__popcll(valid);
         }
     }
     finish();
-    for (uint32_t bb = next; bb < Q; ++bb) kinds[(size_t)bb * T + tile] = TK_TILE_KIND_EMPTY;
+    empty_until(Q);
 }
 
 // fine_tile_main for every batch: one 256-lane workgroup per tile, one lane per
@@ -862,6 +1076,7 @@ extern "C" __global__ __launch_bounds__(64) void tk_k_coarse_emit(tk_args_coarse
 // the reference performs on its target.
 extern "C" __global__ __launch_bounds__(256) void tk_k_fine(tk_args_fine a) {
     __shared__ uint8_t clip_spill[TK_FINE_MAX_CLIP_SPILL * 256u];
+    __shared__ tk_line_segment seg_lds[kFineSegCap];
     const tk_fine_config &cfg = a.cfg;
     const uint32_t tile_ix = blockIdx.x + blockIdx.y * gridDim.x;
     if (tile_ix >= cfg.tile_count) return;
@@ -943,8 +1158,17 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fine(tk_args_fine a) {
                     push_clip(parent);
                     clip_mask = combine_alpha(parent, alpha);
                 } else if (tag == TK_PTCL_FILL || tag == TK_PTCL_BEGIN_CLIP) {
-                    float v = fill_coverage(segs, rec.backdrop, rec.segment_start, min(rec.segment_end, cfg.segment_capacity),
-                                            lx, ly);
+                    const uint32_t s0 = rec.segment_start, s1 = min(rec.segment_end, cfg.segment_capacity);
+                    const uint32_t n = s1 > s0 ? s1 - s0 : 0u;
+                    float v;
+                    if (n <= kFineSegCap) {
+                        stage_sorted_segments(segs + s0, n, seg_lds, lane);
+                        v = lds_coverage(seg_lds, n, rec.backdrop, lx, ly);
+                    } else {
+                        // Longer lists are summed in stored order; the status word records it.
+                        v = fill_coverage(segs, rec.backdrop, s0, s1, lx, ly);
+                        if (lane == 0u && a.in.status) atomicOr(a.in.status, 1u);
+                    }
                     uint32_t alpha = coverage_to_alpha(v, rec.fill_rule);
                     if (tag == TK_PTCL_BEGIN_CLIP) {
                         push_clip(clip_mask);

@@ -6,7 +6,7 @@
 // (src/native/runtime/program/scene_scan.rs:73-325, program/cumsum.rs:171-269,
 // program/coarse.rs:29-129, program/fine.rs:26-67, one coarse+fine pair per
 // DrawBatch) with a fixed sequence of gfx950 kernels on one stream:
-//   clear -> scan_count -> scan_prefix -> scan_emit -> segment_order -> cumsum
+//   clear -> scan_count -> scan_prefix -> scan_emit -> cumsum
 //   -> coarse_count -> coarse_scan -> coarse_emit -> fine
 // where coarse and fine process every DrawBatch in a single launch.
 
@@ -64,7 +64,7 @@ struct tk_context {
     hipModule_t module = nullptr;
     hipStream_t stream = nullptr;
     hipFunction_t f_fill = nullptr, f_scan_clear = nullptr, f_scan_count = nullptr, f_scan_prefix = nullptr,
-                  f_scan_emit = nullptr, f_segment_order = nullptr, f_cumsum = nullptr, f_coarse_count = nullptr,
+                  f_scan_emit = nullptr, f_cumsum = nullptr, f_coarse_count = nullptr,
                   f_coarse_scan = nullptr, f_coarse_emit = nullptr, f_fine = nullptr, f_fill_coverage = nullptr,
                   f_pixel_math = nullptr;
     // Scratch for the stage entry points (frames own their own).
@@ -137,11 +137,12 @@ tk_status run_scan_prefix(tk_context *ctx, const tk_scan_config &cfg, const tk_s
                           const tk_scan_chunk_range *chunk_ranges, const tk_path_record *paths, const uint32_t *counts,
                           tk_segment_range *ranges, uint32_t *cursors, uint32_t *bumps, uint32_t *chunk_totals,
                           uint32_t *chunk_offsets, unsigned long long *status, uint32_t *ticket, uint32_t epoch) {
-    uint32_t blocks = std::max<uint32_t>(cfg.scan_chunk_count, div_up(cfg.path_count, 256));
-    if (blocks == 0) return TK_OK;
+    // One wave per chunk; waves beyond the chunks zero the bumps of chunk-less paths.
+    uint32_t waves = std::max<uint32_t>(cfg.scan_chunk_count, div_up(cfg.path_count, 64));
+    if (waves == 0) return TK_OK;
     tk_args_scan_prefix a{cfg,           chunks, chunk_ranges, paths,  counts, ranges, cursors,
-                          bumps,         chunk_totals, chunk_offsets, status, ticket, epoch,  blocks};
-    return launch(ctx, ctx->f_scan_prefix, blocks, 1, 256, a);
+                          bumps,         chunk_totals, chunk_offsets, status, ticket, epoch,  waves};
+    return launch(ctx, ctx->f_scan_prefix, div_up(waves, 4), 1, 256, a);
 }
 
 tk_status run_cumsum(tk_context *ctx, const tk_cumsum_config &cfg, const uint32_t *offs, const uint32_t *lens,
@@ -161,11 +162,11 @@ tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coar
                      hipEvent_t *marks) {
     if (cfg.batch_count == 0 || cfg.tile_count == 0) return TK_OK;
     tk_args_coarse a{cfg, in, out, status, ticket, epoch, coarse_scan_blocks(cfg)};
-    TK_TRY(launch(ctx, ctx->f_coarse_count, div_up(cfg.tile_count, 64), 1, 64, a));
+    TK_TRY(launch(ctx, ctx->f_coarse_count, div_up(cfg.tile_count, 4), 1, 256, a));
     if (marks) TK_HIP(hipEventRecord(marks[0], ctx->stream));
     TK_TRY(launch(ctx, ctx->f_coarse_scan, a.scan_blocks, 1, 256, a));
     if (marks) TK_HIP(hipEventRecord(marks[1], ctx->stream));
-    TK_TRY(launch(ctx, ctx->f_coarse_emit, div_up(cfg.tile_count, 64), 1, 64, a));
+    TK_TRY(launch(ctx, ctx->f_coarse_emit, div_up(cfg.tile_count, 4), 1, 256, a));
     return TK_OK;
 }
 
@@ -371,11 +372,6 @@ tk_status frame_render(tk_frame *f) {
     TK_TRY(record(f, 3));
     TK_TRY(run_scan_emit(ctx, scfg, f->lines, f->paths, s.path_count, f->cursors, f->segments));
     TK_TRY(record(f, 4));
-    {
-        tk_args_segment_sort a{f->seg_ranges, s.backdrop_len, s.segment_capacity, f->segments};
-        TK_TRY(launch(ctx, ctx->f_segment_order, div_up(s.backdrop_len, 256), 1, 256, a));
-    }
-    TK_TRY(record(f, 5));
     if (f->capture_raw && s.backdrop_len)
         TK_HIP(hipMemcpyAsync(f->raw_backdrops, f->backdrops, uint64_t(s.backdrop_len) * 4, hipMemcpyDeviceToDevice,
                               ctx->stream));
@@ -399,7 +395,8 @@ tk_status frame_render(tk_frame *f) {
     TK_TRY(record(f, 9));
     {
         tk_fine_config fc = frame_fine_config(f);
-        tk_fine_inputs in{f->draws, f->paint, f->segments, f->tile_ranges, f->tile_kinds, f->batch_bases, f->ptcl};
+        tk_fine_inputs in{f->draws,      f->paint,       f->segments, f->tile_ranges,
+                          f->tile_kinds, f->batch_bases, f->ptcl,     f->status + 2};
         (void)T;
         TK_TRY(run_fine(ctx, fc, in, f->image));
     }
@@ -449,7 +446,6 @@ tk_status tk_context_create(int device, tk_context **out) {
         {&ctx->f_scan_count, "tk_k_scan_count"},
         {&ctx->f_scan_prefix, "tk_k_scan_prefix"},
         {&ctx->f_scan_emit, "tk_k_scan_emit"},
-        {&ctx->f_segment_order, "tk_k_segment_order"},
         {&ctx->f_cumsum, "tk_k_cumsum"},
         {&ctx->f_coarse_count, "tk_k_coarse_count"},
         {&ctx->f_coarse_scan, "tk_k_coarse_scan"},
@@ -667,12 +663,14 @@ tk_status tk_frame_stage_times(tk_frame *f, tk_stage_times *out) {
     if (!f->have_times) return fail(TK_ERR_INVALID, "no timed render recorded");
     TK_HIP(hipEventSynchronize(f->ev[10]));
     float t[11] = {};
-    for (int i = 1; i <= 10; ++i) TK_HIP(hipEventElapsedTime(&t[i], f->ev[i - 1], f->ev[i]));
+    for (int i = 1; i <= 10; ++i) {
+        if (i == 5) continue;  // event 5 is not recorded
+        TK_HIP(hipEventElapsedTime(&t[i], f->ev[i == 6 ? 4 : i - 1], f->ev[i]));
+    }
     out->scan_clear = t[1];
     out->scan_count = t[2];
     out->scan_prefix = t[3];
     out->scan_emit = t[4];
-    out->segment_order = t[5];
     out->cumsum = t[6];
     out->coarse_count = t[7];
     out->coarse_prefix = t[8];
@@ -740,7 +738,7 @@ tk_status tk_frame_stats_read(tk_frame *f, tk_frame_stats *out) {
     out->tile_draw_refs = s.tile_draw_index_count;
     out->ptcl_capacity = f->ptcl_capacity;
     out->ptcl_records = status[1];
-    out->overflow = status[0] | (status[3] << 1) | (status[1] > f->ptcl_capacity ? 4u : 0u);
+    out->overflow = status[0] | (status[3] << 1) | (status[1] > f->ptcl_capacity ? 4u : 0u) | (status[2] << 4);
     uint32_t emitted = 0;
     if (s.path_count) {
         // Segments emitted = sum of per-path bumps.

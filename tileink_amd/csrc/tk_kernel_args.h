@@ -50,14 +50,7 @@ struct tk_args_scan_prefix {
     unsigned long long *status;  // one word per chunk: value | flag << 32 | epoch << 34
     uint32_t *ticket;
     uint32_t epoch;
-    uint32_t block_count;
-};
-
-struct tk_args_segment_sort {
-    const tk_segment_range *ranges;
-    uint32_t backdrop_len;
-    uint32_t segment_capacity;
-    tk_line_segment *segments;
+    uint32_t block_count;  // waves in the launch, one ticket each
 };
 
 struct tk_args_cumsum {
