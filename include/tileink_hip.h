@@ -423,9 +423,11 @@ tk_status tk_coarse(tk_context *ctx, const tk_coarse_config *config, const tk_co
 /* Fine, all batches at once: one 256-lane workgroup per 16x16 tile walks the
  * batches in order, re-quantising to RGBA8 at each batch boundary exactly where
  * the reference stores and re-loads its RGBA8 target (fine/main.hlsl:23-40,
- * fine/interpreter.hlsli:21-219, fine/specialized.hlsli:19-111).  Each fill's
- * segments are staged in LDS in canonical (bit-pattern) order before the row
- * sweep, so the result does not depend on emit's atomic slot order. */
+ * fine/interpreter.hlsli:21-219, fine/specialized.hlsli:19-111).  Coverage keeps
+ * fill_alpha_at's three sums and their FP32 combination (base + running) +
+ * partial (shared/coverage.hlsli:34-51), but each sum is exact: its FP32 terms
+ * are added as 32.32 fixed point, so the result does not depend on the atomic
+ * slot order in which emit stored a tile's segments. */
 typedef struct tk_fine_config {
     uint32_t width, height, clear_color, tile_count;
     uint32_t tiles_width, tiles_height, batch_count, clip_spill_depth;
@@ -440,8 +442,8 @@ typedef struct tk_fine_inputs {
     const uint32_t *tile_kinds;
     const uint32_t *batch_ptcl_bases;
     const tk_ptcl_record *ptcl;
-    uint32_t *status;  /* optional; bit 0 set when a fill's segment list exceeded the
-                          1024-entry LDS stage and was summed in stored order */
+    uint32_t *status;  /* optional; bit 1 set when a tile's clip stack went deeper than
+                          4 + clip_spill_depth levels (the extra levels were dropped) */
 } tk_fine_inputs;
 tk_status tk_fine(tk_context *ctx, const tk_fine_config *config, const tk_fine_inputs *in,
                   uint32_t *rgba_out);
@@ -449,7 +451,8 @@ tk_status tk_fine(tk_context *ctx, const tk_fine_config *config, const tk_fine_i
 /* Validation helpers (validation/geometry.hlsl:22-29, validation/pixel.hlsl:9-34).
  * requests: 8 words each {segment_start, segment_end, backdrop, x, y, rule, 0, 0}.
  * alpha_out: u8 alpha per request; coverage_out (optional): FP32 winding area
- * backdrop + row sweep before the fill rule. */
+ * backdrop + row sweep before the fill rule, with the three sums exact as in
+ * tk_fine, evaluated per pixel (independently of tk_fine's row accumulation). */
 tk_status tk_fill_coverage_words(tk_context *ctx, uint32_t count, const uint32_t *requests,
                                  const tk_line_segment *segments, uint32_t *alpha_out,
                                  float *coverage_out);

@@ -161,13 +161,16 @@ def line_scanned_tile_count(p0, p1, bbox):
     return lib().or_line_scanned_tile_count(_p(a), _p(b), *[C.c_uint32(int(v)) for v in bbox])
 
 
-def fill_coverage_words(requests, segments):
+def fill_coverage_words(requests, segments, sequential=False):
+    """Alpha and FP32 coverage per request.  Default: the three sums exact (the form
+    the fine stage computes); sequential=True: FP32 sums in stored segment order."""
     requests = np.ascontiguousarray(requests, dtype=np.uint32).reshape(-1, 8)
     segments = np.ascontiguousarray(segments).view(np.uint32).reshape(-1)
     n = requests.shape[0]
     alpha = np.zeros(n, np.uint32)
     cov = np.zeros(n, np.float32)
-    lib().or_fill_coverage_words(C.c_uint32(n), _p(requests), _p(segments), _p(alpha), _p(cov))
+    fn = lib().or_fill_coverage_words_sequential if sequential else lib().or_fill_coverage_words
+    fn(C.c_uint32(n), _p(requests), _p(segments), _p(alpha), _p(cov))
     return alpha, cov
 
 

@@ -353,20 +353,6 @@ void or_scan_emit(const or_scan_config *cfg, const uint32_t *lines, const uint32
     }
 }
 
-static int segment_cmp(const void *a, const void *b) {
-    const uint32_t *x = (const uint32_t *)a, *y = (const uint32_t *)b;
-    for (int i = 0; i < 5; ++i)
-        if (x[i] != y[i]) return x[i] < y[i] ? -1 : 1;
-    return 0;
-}
-
-void or_segment_order(const uint32_t *ranges, uint32_t backdrop_len, uint32_t cap, uint32_t *segments) {
-    for (uint32_t b = 0; b < backdrop_len; ++b) {
-        uint32_t start = ranges[2 * (size_t)b], end = umin(ranges[2 * (size_t)b + 1], cap);
-        if (end > start + 1u) qsort(segments + (size_t)start * 5u, end - start, 20, segment_cmp);
-    }
-}
-
 /* cumsum_prefix_chunks, cumsum_chunk_offsets, cumsum_apply_chunk_offsets (cumsum.hlsl:16-71). */
 void or_cumsum(uint32_t row_count, uint32_t chunk_count, const uint32_t *offs, const uint32_t *lens,
                const uint32_t *row_starts, const uint32_t *row_ends, int32_t *backdrops) {
@@ -578,8 +564,12 @@ static float segment_area_at(float xmin_abs, float xmax_abs, uint32_t x) {
     float full = clampf(-xmin, 0.0f, width);
     return fmaf(right - left, fmaf(-0.5f, left + right, 1.0f), full) / width;
 }
-static float fill_value_at(const uint32_t *segs, int32_t backdrop, uint32_t start, uint32_t end, uint32_t x,
-                           uint32_t y) {
+/* fill_alpha_at's FP32 sums taken literally, in stored segment order
+ * (shared/coverage.hlsli:34-51).  The reference's stored order comes from emit's
+ * atomic slots, so this value can change from run to run in its last bits; it is
+ * kept only to bound the difference of the exact form below. */
+static float fill_value_sequential(const uint32_t *segs, int32_t backdrop, uint32_t start, uint32_t end, uint32_t x,
+                                   uint32_t y) {
     float base = (float)backdrop, running = 0.0f, partial = 0.0f;
     for (uint32_t i = start; i < end; ++i) {
         const uint32_t *s = segs + (size_t)i * 5u;
@@ -593,6 +583,34 @@ static float fill_value_at(const uint32_t *segs, int32_t backdrop, uint32_t star
         }
     }
     return base + running + partial;
+}
+
+/* The same three sums, each exact: every FP32 term is truncated to 32.32 fixed
+ * point (|t| clamped to 2^30, NaN -> 0) and summed as a 64-bit integer, then the
+ * sums are converted back and combined as (base + running) + partial in FP32,
+ * as fill_alpha_at combines them.  No segment order enters the result. */
+static int64_t coverage_fixed(float t) {
+    if (!(t == t)) return 0;
+    int64_t m = (int64_t)((double)fminf(fabsf(t), 1073741824.0f) * 4294967296.0);
+    return t < 0.0f ? -m : m;
+}
+static float fixed_to_float(int64_t v) { return (float)((double)v * 2.3283064365386962890625e-10); }
+
+static float fill_value_at(const uint32_t *segs, int32_t backdrop, uint32_t start, uint32_t end, uint32_t x,
+                           uint32_t y) {
+    int64_t base = (int64_t)backdrop * 4294967296LL, running = 0, partial = 0;
+    for (uint32_t i = start; i < end; ++i) {
+        const uint32_t *s = segs + (size_t)i * 5u;
+        RowParts p = segment_row_parts(bits_f(s[0]), bits_f(s[1]), bits_f(s[2]), bits_f(s[3]), bits_f(s[4]), y);
+        base += coverage_fixed(p.edge);
+        if (p.dy != 0.0f) {
+            int full_start = clampi(f2i(ceilf(p.xmax)), 0, (int)TILE);
+            if (full_start < (int)TILE && (int)x >= full_start) running += coverage_fixed(p.dy);
+            int ps = clampi(f2i(floorf(p.xmin)), 0, (int)TILE), pe = clampi(f2i(ceilf(p.xmax)), 0, (int)TILE);
+            if ((int)x >= ps && (int)x < pe) partial += coverage_fixed(segment_area_at(p.xmin, p.xmax, x) * p.dy);
+        }
+    }
+    return (fixed_to_float(base) + fixed_to_float(running)) + fixed_to_float(partial);
 }
 
 /* ---------- SDF + brush (shared/sdf/ and shared/brush/ headers) -------------------- */
@@ -1225,7 +1243,6 @@ int or_render_frame(const or_scene *s, uint32_t *image, or_frame_outputs *out) {
     or_scan_chunk_offsets(&sc, s->paths, s->scan_chunk_ranges, bumps, totals, offsets);
     or_scan_apply_chunk_offsets(&sc, s->scan_chunks, ranges, cursors, offsets);
     or_scan_emit(&sc, s->lines, s->paths, s->path_count, cursors, segments);
-    or_segment_order(ranges, B, s->segment_capacity, segments);
     for (uint32_t p = 0; p < s->path_count; ++p)
         if (bumps[p] > s->segment_capacity) status = 1;
     if (out && out->raw_backdrops) memcpy(out->raw_backdrops, backdrops, (size_t)B * 4u);
@@ -1330,6 +1347,16 @@ void or_fill_coverage_words(uint32_t count, const uint32_t *q, const uint32_t *s
     for (uint32_t i = 0; i < count; ++i) {
         const uint32_t *r = q + (size_t)i * 8u;
         float v = fill_value_at(segments, (int32_t)r[2], r[0], r[1], r[3], r[4]);
+        alpha_out[i] = coverage_to_alpha(v, r[5]);
+        if (coverage_out) coverage_out[i] = v;
+    }
+}
+
+void or_fill_coverage_words_sequential(uint32_t count, const uint32_t *q, const uint32_t *segments,
+                                       uint32_t *alpha_out, float *coverage_out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t *r = q + (size_t)i * 8u;
+        float v = fill_value_sequential(segments, (int32_t)r[2], r[0], r[1], r[3], r[4]);
         alpha_out[i] = coverage_to_alpha(v, r[5]);
         if (coverage_out) coverage_out[i] = v;
     }

@@ -40,9 +40,6 @@ void or_scan_apply_chunk_offsets(const or_scan_config *cfg, const uint32_t *chun
                                  uint32_t *cursors, const uint32_t *chunk_offsets);
 void or_scan_emit(const or_scan_config *cfg, const uint32_t *lines, const uint32_t *paths,
                   uint32_t path_record_count, uint32_t *cursors, uint32_t *segments);
-/* Canonical slot order of each tile's segments (the reference's order is that of its atomics). */
-void or_segment_order(const uint32_t *ranges, uint32_t backdrop_len, uint32_t segment_capacity,
-                      uint32_t *segments);
 /* cumsum.hlsl: prefix_chunks, chunk_offsets, apply (all three). */
 void or_cumsum(uint32_t row_count, uint32_t chunk_count, const uint32_t *chunk_backdrop_offsets,
                const uint32_t *chunk_lens, const uint32_t *row_starts, const uint32_t *row_ends,
@@ -109,8 +106,14 @@ typedef struct or_frame_outputs {
 int or_render_frame(const or_scene *scene, uint32_t *image, or_frame_outputs *out);
 
 /* Validation entries (validation/geometry.hlsl:22-29, validation/pixel.hlsl:9-34). */
+/* or_fill_coverage_words: fill_alpha_at with each of its three sums exact (32.32
+ * fixed point), independent of segment order; the fine stage uses this form.
+ * or_fill_coverage_words_sequential: the FP32 sums in stored segment order, as
+ * the reference's shader adds them. */
 void or_fill_coverage_words(uint32_t count, const uint32_t *requests, const uint32_t *segments,
                             uint32_t *alpha_out, float *coverage_out);
+void or_fill_coverage_words_sequential(uint32_t count, const uint32_t *requests, const uint32_t *segments,
+                                       uint32_t *alpha_out, float *coverage_out);
 void or_pixel_math_words(uint32_t count, const uint32_t *requests, uint32_t *out);
 
 /* FP64 area oracles.

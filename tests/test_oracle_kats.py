@@ -398,6 +398,40 @@ def test_fp32_row_sweep_within_1e5_of_fp64_area():
     assert err.max() <= 1e-5, err.max()
 
 
+def _shuffle_within_ranges(segs, req, rng):
+    """Copy of `segs` with every requested [start, end) range permuted."""
+    out = segs.copy()
+    for s0, s1 in {(int(r[0]), int(r[1])) for r in req}:
+        out[s0:s1] = segs[s0:s1][rng.permutation(s1 - s0)]
+    return out
+
+
+@pytest.mark.parametrize("scene", ["config1", "config2_small"])
+def test_exact_sums_are_order_free_and_within_1_lsb_of_fp32_order(scene):
+    """The fine stage's coverage (three exact sums) does not depend on the slot order
+    emit's atomics give a tile's segments, and differs from the reference's literal
+    FP32 row sweep (shared/coverage.hlsli:34-51), in any order, by at most 1 LSB of
+    alpha and a few FP32 ulps of coverage."""
+    canvas = scenes.config1() if scene == "config1" else scenes.config2_small(16)
+    a = canvas.arrays()
+    _, out = oracle.render_frame(a)
+    req = _coverage_requests(a, out, limit=40000, seed=5)
+    segs = out["segments"]
+    rng = np.random.default_rng(8)
+    shuffled = _shuffle_within_ranges(segs, req, rng)
+    alpha, cov = oracle.fill_coverage_words(req, segs)
+    alpha_s, cov_s = oracle.fill_coverage_words(req, shuffled)
+    assert np.array_equal(alpha, alpha_s) and np.array_equal(cov.view(np.uint32), cov_s.view(np.uint32))
+    for order in (segs, shuffled):
+        alpha_q, cov_q = oracle.fill_coverage_words(req, order, sequential=True)
+        assert np.abs(alpha.astype(np.int64) - alpha_q).max() <= 1
+        assert np.abs(cov.astype(np.float64) - cov_q).max() <= 1e-5
+    # the sequential form really is order-dependent somewhere, or this test proves nothing new
+    _, c0 = oracle.fill_coverage_words(req, segs, sequential=True)
+    _, c1 = oracle.fill_coverage_words(req, shuffled, sequential=True)
+    assert not np.array_equal(c0.view(np.uint32), c1.view(np.uint32))
+
+
 def test_tile_segment_areas_reproduce_polygon_area():
     """Backdrop + clipped segments of a tile integrate to the path's exact winding area
     over that tile (up to the scan's 1e-3 px tile-edge nudges)."""

@@ -478,68 +478,69 @@ __device__ __forceinline__ float sdf_coverage_from_draw(const tk_fine_config &cf
 
 __device__ __forceinline__ bool premul_opaque(uint32_t c) { return (c >> 24) == 255u; }
 
-__device__ __forceinline__ float fill_coverage(const tk_line_segment *segments, int32_t backdrop, uint32_t start,
-                                               uint32_t end, uint32_t x, uint32_t y) {
-    float base = (float)backdrop, running = 0.0f, partial = 0.0f;
-    for (uint32_t i = start; i < end; ++i) accumulate_segment(segments[i], x, y, base, running, partial);
-    return base + running + partial;
-}
+// Exact coverage sums of one FILL / BEGIN_CLIP record over a 16x16 tile, built
+// segment-row by segment-row instead of pixel by pixel: one lane takes one
+// (segment, row) pair, evaluates the row parts once and adds its terms into LDS
+// with 64-bit integer atomics (exact, so the order of arrival does not matter):
+//   base[row]       += edge term,
+//   run[row][fs]    += dy, where fs = full_start (prefix-summed along the row later),
+//   part[row][x]    += area(x) * dy for the straddled pixels x in [partial_start, fs).
+struct CoverageAcc {
+    long long base[16];
+    long long run[256];
+    long long part[256];
+};
 
-// Segments of one FILL record staged in LDS per 16x16 tile workgroup.  Emit
-// claims slots with atomics, so the stored order of a tile's segments varies
-// from run to run; staging sorts them by their bit patterns (a rank sort over
-// LDS broadcast reads), which fixes the summation order of the row sweep to a
-// function of the geometry alone.
-constexpr uint32_t kFineSegCap = 1024u;
-constexpr uint32_t kFineSegPerLane = kFineSegCap / 256u;
-
-__device__ __forceinline__ bool segment_key_less(const tk_line_segment &x, const tk_line_segment &y) {
-    const uint32_t a[5] = {asuint(x.p0x), asuint(x.p0y), asuint(x.p1x), asuint(x.p1y), asuint(x.y_edge)};
-    const uint32_t b[5] = {asuint(y.p0x), asuint(y.p0y), asuint(y.p1x), asuint(y.p1y), asuint(y.y_edge)};
-#pragma unroll
-    for (int i = 0; i < 5; ++i)
-        if (a[i] != b[i]) return a[i] < b[i];
-    return false;
-}
-
-// Loads segments [0, n) of `src` into `lds` in canonical order (n <= kFineSegCap).
-// Every lane of the workgroup must call it with the same n.
-__device__ void stage_sorted_segments(const tk_line_segment *src, uint32_t n, tk_line_segment *lds, uint32_t lane) {
-    tk_line_segment mine[kFineSegPerLane];
-    __syncthreads();  // earlier readers of lds are done
-#pragma unroll
-    for (uint32_t m = 0; m < kFineSegPerLane; ++m) {
-        uint32_t k = lane + 256u * m;
-        if (k < n) {
-            mine[m] = src[k];
-            lds[k] = mine[m];
+__device__ __forceinline__ void accumulate_record(const tk_line_segment *segs, uint32_t n, CoverageAcc &acc,
+                                                  uint32_t lane) {
+    const uint32_t row = lane & 15u;
+    for (uint64_t i = lane >> 4; i < n; i += 16u) {
+        const tk_line_segment seg = segs[i];
+        const RowParts p = segment_row_parts(seg.p0x, seg.p0y, seg.p1x, seg.p1y, seg.y_edge, row);
+        if (p.edge != 0.0f) atomicAdd((unsigned long long *)&acc.base[row], (unsigned long long)coverage_fixed(p.edge));
+        if (p.dy != 0.0f) {
+            const int fs = clampi(f2i(ceilf(p.xmax)), 0, 16);
+            const int ps = clampi(f2i(floorf(p.xmin)), 0, 16);
+            if (fs < 16) atomicAdd((unsigned long long *)&acc.run[row * 16u + fs], (unsigned long long)coverage_fixed(p.dy));
+            for (int x = ps; x < fs; ++x)
+                atomicAdd((unsigned long long *)&acc.part[row * 16u + x],
+                          (unsigned long long)coverage_fixed(segment_area_at(p.xmin, p.xmax, (uint32_t)x) * p.dy));
         }
     }
-    __syncthreads();
-    if (n < 2u) return;
-    uint32_t rank[kFineSegPerLane] = {};
-    for (uint32_t j = 0; j < n; ++j) {
-        const tk_line_segment other = lds[j];
-#pragma unroll
-        for (uint32_t m = 0; m < kFineSegPerLane; ++m) {
-            uint32_t k = lane + 256u * m;
-            if (k < n && (segment_key_less(other, mine[m]) || (!segment_key_less(mine[m], other) && j < k))) rank[m]++;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (uint32_t m = 0; m < kFineSegPerLane; ++m) {
-        uint32_t k = lane + 256u * m;
-        if (k < n) lds[rank[m]] = mine[m];
-    }
-    __syncthreads();
 }
 
-__device__ __forceinline__ float lds_coverage(const tk_line_segment *lds, uint32_t n, int32_t backdrop, uint32_t x,
-                                              uint32_t y) {
-    float base = (float)backdrop, running = 0.0f, partial = 0.0f;
-    for (uint32_t i = 0; i < n; ++i) accumulate_segment(lds[i], x, y, base, running, partial);
-    return base + running + partial;
+// Inclusive prefix sum over each 16-lane DPP row (one pixel row of the tile).
+__device__ __forceinline__ long long row16_inclusive_sum(long long v) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)((unsigned long long)v >> 32);
+#define TK_ROW_SHR_STEP(ctrl)                                                                   \
+    {                                                                                           \
+        uint32_t slo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, ctrl, 0xf, 0xf, true); \
+        uint32_t shi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, ctrl, 0xf, 0xf, true); \
+        uint32_t nlo = lo + slo;                                                                \
+        hi = hi + shi + (nlo < lo ? 1u : 0u);                                                   \
+        lo = nlo;                                                                               \
+    }
+    TK_ROW_SHR_STEP(0x111)  // row_shr:1
+    TK_ROW_SHR_STEP(0x112)  // row_shr:2
+    TK_ROW_SHR_STEP(0x114)  // row_shr:4
+    TK_ROW_SHR_STEP(0x118)  // row_shr:8
+#undef TK_ROW_SHR_STEP
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// Reads this lane's pixel (x = lane & 15, y = lane >> 4) out of `acc` and clears
+// the entries it owns.  All 256 lanes call it together, after the barrier that
+// follows accumulate_record; the 16 lanes of a pixel row share one wave, so the
+// row's base word is read by all of them before lane x = 0 clears it.
+__device__ __forceinline__ float take_coverage(CoverageAcc &acc, int32_t backdrop, uint32_t lane) {
+    const uint32_t row = lane >> 4;
+    const long long base = acc.base[row] + backdrop_fixed(backdrop);
+    const long long running = row16_inclusive_sum(acc.run[lane]);
+    const long long partial = acc.part[lane];
+    acc.run[lane] = 0;
+    acc.part[lane] = 0;
+    if ((lane & 15u) == 0u) acc.base[row] = 0;
+    return (fixed_to_float(base) + fixed_to_float(running)) + fixed_to_float(partial);
 }
 
 }  // namespace
@@ -1075,8 +1076,12 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
 // pixel.  Between batches the pixel goes through the RGBA8 store/load round trip
 // the reference performs on its target.
 extern "C" __global__ __launch_bounds__(256) void tk_k_fine(tk_args_fine a) {
-    __shared__ uint8_t clip_spill[TK_FINE_MAX_CLIP_SPILL * 256u];
-    __shared__ tk_line_segment seg_lds[kFineSegCap];
+    // Clip levels beyond the four register slots: clip_spill_depth * 256 bytes of
+    // dynamic LDS, sized by the launch.
+    extern __shared__ uint8_t clip_spill[];
+    __shared__ CoverageAcc cov_acc[2];  // double-buffered: one barrier per fill
+    __shared__ uint32_t batch_kind[256];
+    __shared__ tk_segment_range batch_range[256];
     const tk_fine_config &cfg = a.cfg;
     const uint32_t tile_ix = blockIdx.x + blockIdx.y * gridDim.x;
     if (tile_ix >= cfg.tile_count) return;
@@ -1089,13 +1094,32 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fine(tk_args_fine a) {
     const tk_ptcl_record *ptcl = a.in.ptcl;
     const tk_line_segment *segs = a.in.segments;
 
+    cov_acc[0].run[lane] = 0;
+    cov_acc[0].part[lane] = 0;
+    cov_acc[1].run[lane] = 0;
+    cov_acc[1].part[lane] = 0;
+    if (lane < 16u) {
+        cov_acc[0].base[lane] = 0;
+        cov_acc[1].base[lane] = 0;
+    }
+    uint32_t acc_sel = 0u;
+
     Px pixel = rgba8_to_unorm(cfg.clear_color);
-    for (uint32_t b = 0; b < cfg.batch_count; ++b) {
-        const size_t idx = (size_t)b * cfg.tile_count + tile_ix;
-        const uint32_t kind_word = a.in.tile_kinds[idx];
+    for (uint32_t b0 = 0; b0 < cfg.batch_count; b0 += 256u) {
+      // This tile's (kind, range) word pairs of up to 256 batches, one load per lane.
+      __syncthreads();
+      if (b0 + lane < cfg.batch_count) {
+          const size_t idx = (size_t)(b0 + lane) * cfg.tile_count + tile_ix;
+          batch_kind[lane] = a.in.tile_kinds[idx];
+          batch_range[lane] = a.in.tile_ptcl_ranges[idx];
+      }
+      __syncthreads();
+      const uint32_t chunk = min(256u, cfg.batch_count - b0);
+      for (uint32_t bj = 0; bj < chunk; ++bj) {
+        const uint32_t kind_word = batch_kind[bj];
         const uint32_t kind = kind_word & TK_TILE_KIND_MASK;
         if (kind == TK_TILE_KIND_EMPTY) continue;
-        const tk_segment_range r = a.in.tile_ptcl_ranges[idx];
+        const tk_segment_range r = batch_range[bj];
         const uint32_t pstart = r.start + (kind_word >> TK_TILE_PREFIX_SHIFT);
         const uint32_t pend = min(r.end, cfg.ptcl_capacity);
         if (kind == TK_TILE_KIND_INTERPRETER) {
@@ -1112,6 +1136,8 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fine(tk_args_fine a) {
                     if (spill < cfg.clip_spill_depth) {
                         clip_spill[spill * 256u + lane] = (uint8_t)mask;
                         clip_depth = dd + 1u;
+                    } else if (a.in.status) {
+                        atomicOr(a.in.status, 2u);  // deeper than the scene's declared clip depth
                     }
                 }
             };
@@ -1160,15 +1186,11 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fine(tk_args_fine a) {
                 } else if (tag == TK_PTCL_FILL || tag == TK_PTCL_BEGIN_CLIP) {
                     const uint32_t s0 = rec.segment_start, s1 = min(rec.segment_end, cfg.segment_capacity);
                     const uint32_t n = s1 > s0 ? s1 - s0 : 0u;
-                    float v;
-                    if (n <= kFineSegCap) {
-                        stage_sorted_segments(segs + s0, n, seg_lds, lane);
-                        v = lds_coverage(seg_lds, n, rec.backdrop, lx, ly);
-                    } else {
-                        // Longer lists are summed in stored order; the status word records it.
-                        v = fill_coverage(segs, rec.backdrop, s0, s1, lx, ly);
-                        if (lane == 0u && a.in.status) atomicOr(a.in.status, 1u);
-                    }
+                    CoverageAcc &acc = cov_acc[acc_sel];
+                    acc_sel ^= 1u;
+                    accumulate_record(segs + s0, n, acc, lane);
+                    __syncthreads();
+                    const float v = take_coverage(acc, rec.backdrop, lane);
                     uint32_t alpha = coverage_to_alpha(v, rec.fill_rule);
                     if (tag == TK_PTCL_BEGIN_CLIP) {
                         push_clip(clip_mask);
@@ -1216,6 +1238,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fine(tk_args_fine a) {
             }
         }
         pixel = rgba8_to_unorm(unorm_to_rgba8(pixel));
+      }
     }
     if (gx < cfg.width && gy < cfg.height) a.rgba[(size_t)gy * cfg.width + gx] = unorm_to_rgba8(pixel);
 }
@@ -1225,7 +1248,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fill_coverage(tk_args_fil
     uint32_t id = blockIdx.x * 256u + threadIdx.x;
     if (id >= a.count) return;
     const uint32_t *q = a.requests + (size_t)id * 8u;
-    float v = fill_coverage(a.segments, (int32_t)q[2], q[0], q[1], q[3], q[4]);
+    float v = exact_fill_value(a.segments, (int32_t)q[2], q[0], q[1], q[3], q[4]);
     a.alpha_out[id] = coverage_to_alpha(v, q[5]);
     if (a.coverage_out) a.coverage_out[id] = v;
 }

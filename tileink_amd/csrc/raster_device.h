@@ -374,18 +374,36 @@ TKD float segment_area_at(float xmin_abs, float xmax_abs, uint32_t x) {
     return fmaf(right - left, fmaf(-0.5f, left + right, 1.0f), full) / width;
 }
 
-// Accumulates one segment of fill_alpha_at into the three ordered sums.
-TKD void accumulate_segment(const tk_line_segment &seg, uint32_t x, uint32_t y, float &base, float &running,
-                            float &partial) {
-    RowParts parts = segment_row_parts(seg.p0x, seg.p0y, seg.p1x, seg.p1y, seg.y_edge, y);
-    base += parts.edge;
-    if (parts.dy != 0.0f) {
-        int full_start = clampi(f2i(ceilf(parts.xmax)), 0, 16);
-        if (full_start < 16 && (int)x >= full_start) running += parts.dy;
-        int partial_start = clampi(f2i(floorf(parts.xmin)), 0, 16);
-        int partial_end = clampi(f2i(ceilf(parts.xmax)), 0, 16);
-        if ((int)x >= partial_start && (int)x < partial_end) partial += segment_area_at(parts.xmin, parts.xmax, x) * parts.dy;
+// fill_alpha_at keeps three sums per pixel (base, running, partial) and combines
+// them as (base + running) + partial.  Each sum here is exact: every FP32 term is
+// truncated to 32.32 fixed point (|t| <= 2^30, NaN -> 0) and the terms are added
+// as 64-bit integers, so a sum's value does not depend on the order in which the
+// tile's segments were emitted.  Only the conversion back to FP32 rounds.
+TKD long long coverage_fixed(float t) {
+    if (!(t == t)) return 0;
+    long long m = (long long)((double)fminf(fabsf(t), 1073741824.0f) * 4294967296.0);
+    return t < 0.0f ? -m : m;
+}
+TKD float fixed_to_float(long long v) { return (float)((double)v * 2.3283064365386962890625e-10); }
+TKD long long backdrop_fixed(int32_t backdrop) { return (long long)backdrop * 4294967296LL; }
+
+// One pixel's coverage from segments [start, end), terms summed exactly.
+TKD float exact_fill_value(const tk_line_segment *segs, int32_t backdrop, uint32_t start, uint32_t end, uint32_t x,
+                           uint32_t y) {
+    long long base = backdrop_fixed(backdrop), running = 0, partial = 0;
+    for (uint32_t i = start; i < end; ++i) {
+        const tk_line_segment seg = segs[i];
+        RowParts parts = segment_row_parts(seg.p0x, seg.p0y, seg.p1x, seg.p1y, seg.y_edge, y);
+        base += coverage_fixed(parts.edge);
+        if (parts.dy != 0.0f) {
+            int full_start = clampi(f2i(ceilf(parts.xmax)), 0, 16);
+            if (full_start < 16 && (int)x >= full_start) running += coverage_fixed(parts.dy);
+            int partial_start = clampi(f2i(floorf(parts.xmin)), 0, 16);
+            if ((int)x >= partial_start && (int)x < full_start)
+                partial += coverage_fixed(segment_area_at(parts.xmin, parts.xmax, x) * parts.dy);
+        }
     }
+    return (fixed_to_float(base) + fixed_to_float(running)) + fixed_to_float(partial);
 }
 
 // ---------------------------------------------------------------------------

@@ -85,11 +85,12 @@ namespace {
 uint32_t div_up(uint64_t n, uint32_t d) { return uint32_t((n + d - 1) / d); }
 
 template <class Args>
-tk_status launch(tk_context *ctx, hipFunction_t f, uint32_t gx, uint32_t gy, uint32_t block, const Args &args) {
+tk_status launch(tk_context *ctx, hipFunction_t f, uint32_t gx, uint32_t gy, uint32_t block, const Args &args,
+                 uint32_t dynamic_lds = 0) {
     if (gx == 0 || gy == 0) return TK_OK;
     Args copy = args;
     void *params[] = {&copy};
-    hipError_t e = hipModuleLaunchKernel(f, gx, gy, 1, block, 1, 1, 0, ctx->stream, params, nullptr);
+    hipError_t e = hipModuleLaunchKernel(f, gx, gy, 1, block, 1, 1, dynamic_lds, ctx->stream, params, nullptr);
     if (e != hipSuccess) {
         set_error("kernel launch failed: %s", hipGetErrorString(e));
         return TK_ERR_HIP;
@@ -178,7 +179,7 @@ tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inp
     tk_args_fine a{cfg, in, rgba};
     uint32_t gx = std::min<uint32_t>(cfg.tile_count, 65535u);
     uint32_t gy = div_up(cfg.tile_count, gx);
-    return launch(ctx, ctx->f_fine, gx, gy, 256, a);
+    return launch(ctx, ctx->f_fine, gx, gy, 256, a, cfg.clip_spill_depth * 256u);
 }
 
 }  // namespace
