@@ -7,6 +7,7 @@ prefix, coarse, fine for every DrawBatch) from scene buffers already resident in
 HBM.  Prints one JSON line.
 """
 import argparse
+import hashlib
 import json
 import os
 import platform
@@ -19,7 +20,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import tileink_amd as tk  # noqa: E402
-from tileink_amd import scenes  # noqa: E402
+from tileink_amd import _abi, scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -28,15 +29,56 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def fine_algorithmic_bytes(stats, width, height):
-    """Minimum HBM traffic of the fine stage: every PTCL record and tile segment read
-    once, the (range, kind) word pair of every batch x tile read once, the image
-    written once."""
+def fine_walked_work(frame, scene, stats):
+    """Records and tile segments the fine stage reads: per (batch, tile) stream the
+    records from start + opaque-prefix skip to END, and the segments of the FILL /
+    BEGIN_CLIP records among them in INTERPRETER tiles."""
+    Q, T = scene.batch_count, scene.tiles_width * scene.tiles_height
     K = int(stats["ptcl_records"])
-    S = int(stats["segments_emitted"])
-    QT = int(stats["batches"]) * int(stats["tiles"])
-    N = width * height
-    return 24 * K + 20 * S + 12 * QT + 4 * N
+    kinds = frame.buffer(_abi.BUF_TILE_KINDS, np.uint32, Q * T)
+    ranges = frame.buffer(_abi.BUF_TILE_PTCL_RANGES, np.uint32, 2 * Q * T).reshape(-1, 2).astype(np.int64)
+    ptcl = frame.buffer(_abi.BUF_PTCL, np.uint32, 6 * K).reshape(-1, 6)
+    counts = ranges[:, 1] - ranges[:, 0]
+    # The ranges are one exclusive scan in (batch, tile) order, so record k belongs
+    # to the k-th position of the streams laid end to end.
+    has = np.nonzero(counts > 0)[0]
+    stream = np.repeat(has, counts[has])[:K]
+    index = np.arange(stream.size, dtype=np.int64)
+    kind = kinds[stream] & 7
+    walked = (index >= ranges[stream, 0] + (kinds[stream] >> 3)) & (kind != _abi.TK_TILE_KIND_EMPTY)
+    rec = ptcl[:stream.size]
+    fills = walked & (kind == _abi.TK_TILE_KIND_INTERPRETER) & ((rec[:, 0] == 1) | (rec[:, 0] == 3))
+    seg = rec[fills, 4].astype(np.int64) - rec[fills, 3].astype(np.int64)
+    return int(walked.sum()), int(np.clip(seg, 0, None).sum())
+
+
+def fine_algorithmic_bytes(walked_records, used_segments, scene):
+    """Minimum HBM traffic of one fine launch: every walked PTCL record (24 B) and
+    every segment those records use (20 B) read once, the (range, kind) word pair
+    (12 B) of every batch x tile read once, the RGBA8 image (4 B/pixel) written once."""
+    QT = scene.batch_count * scene.tiles_width * scene.tiles_height
+    return 24 * walked_records + 20 * used_segments + 12 * QT + 4 * scene.width * scene.height
+
+
+def source_digest():
+    h = hashlib.sha256()
+    for rel in ("tileink_amd/csrc/kernels.hip", "tileink_amd/csrc/raster_device.h"):
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch from the committed --pmc profile of this kernel source,
+    or None when there is none for the current source."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic_4096px_200k_paths.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    if rec.get("source_digest") != source_digest():
+        return None
+    return rec.get("hbm_bytes_per_launch")
 
 
 def cpu_baseline(args):
@@ -65,6 +107,8 @@ def main():
     ap.add_argument("--paths", type=int, default=200_000)
     ap.add_argument("--clips", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device three times with tk_k_copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the scene would shard by tile "
@@ -78,6 +122,13 @@ def main():
         f"{scene.tile_draw_index_count} tile refs, generated in {time.perf_counter() - t0:.1f} s")
 
     ctx = tk.NativeContext(0)
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        src, dst = tk.DeviceBuffer(ctx, nbytes), tk.DeviceBuffer(ctx, nbytes)
+        for _ in range(3):
+            _abi.check(_abi.lib().tk_copy_device(ctx.handle, dst.ptr, src.ptr, nbytes))
+        log(f"calibration: copied {args.calibrate_copy} MiB three times with tk_k_copy")
+        del src, dst
     frame = tk.Frame(ctx, scene)
     for _ in range(args.warmup):
         frame.render()
@@ -113,7 +164,8 @@ def main():
     W = H = args.size
     N = W * H
     gpix = N / (ms_per_step * 1e-3) / 1e9
-    fine_bytes = fine_algorithmic_bytes(stats, W, H)
+    walked_records, used_segments = fine_walked_work(frame, scene, stats)
+    fine_bytes = fine_algorithmic_bytes(walked_records, used_segments, scene)
     fine_gbs = fine_bytes / (stage_avg["fine"] * 1e-3) / 1e9
     result = {
         "metric": "ms/frame & Gpix/s at 4096², 200k paths; batched fwd+bwd scenes/s vs plain",
@@ -138,8 +190,9 @@ def main():
             "stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
         },
         "roofline": {"bound": "hbm", "kernel": "tk_k_fine", "achieved": round(fine_gbs, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(fine_gbs / HBM_PEAK_GBS, 5), "traffic": None,
-                     "algorithmic_bytes": fine_bytes},
+                     "unit": "GB/s", "frac": round(fine_gbs / HBM_PEAK_GBS, 5), "traffic": measured_traffic("tk_k_fine"),
+                     "algorithmic_bytes": fine_bytes, "walked_records": walked_records,
+                     "used_segments": used_segments, "fine_ms": round(stage_avg["fine"], 4)},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline: oracle on a bounded sample ...")

@@ -279,6 +279,10 @@ tk_status tk_device_alloc(tk_context *ctx, size_t bytes, void **out);
 tk_status tk_device_free(tk_context *ctx, void *ptr);
 tk_status tk_copy_to_device(tk_context *ctx, void *dst, const void *src, size_t bytes);
 tk_status tk_copy_to_host(tk_context *ctx, void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by a plain 16-byte-per-lane copy kernel (tk_k_copy);
+ * bytes and both pointers must be multiples of 16.  Its measured HBM traffic
+ * calibrates the FETCH_SIZE / WRITE_SIZE counters against a known byte count. */
+tk_status tk_copy_device(tk_context *ctx, void *dst, const void *src, size_t bytes);
 
 /* ------------------------------------------------------------------------- */
 /* Whole-frame path (NativeRenderer::render_to_image, src/native/renderer.rs:177-188) */

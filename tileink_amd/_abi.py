@@ -22,6 +22,7 @@ BRUSH_SOLID, BRUSH_LINEAR = 1, 2
 
 # Frame buffer identifiers (tk_frame_buffer).
 BUF_BACKDROPS, BUF_SEGMENT_COUNTS, BUF_SEGMENT_RANGES, BUF_SEGMENT_CURSORS, BUF_SEGMENT_BUMPS, BUF_SEGMENTS = range(6)
+TK_TILE_KIND_INTERPRETER, TK_TILE_KIND_EMPTY, TK_TILE_KIND_COLOR, TK_TILE_KIND_SDF, TK_TILE_KIND_MIXED = range(5)
 BUF_TILE_PTCL_COUNTS, BUF_TILE_PTCL_RANGES, BUF_TILE_KINDS, BUF_BATCH_PTCL_BASES, BUF_PTCL, BUF_RAW_BACKDROPS = range(6, 12)
 
 
@@ -180,6 +181,7 @@ SIGNATURES = {
     "tk_device_free": (C.c_int, [_P, _P]),
     "tk_copy_to_device": (C.c_int, [_P, _P, _P, _SZ]),
     "tk_copy_to_host": (C.c_int, [_P, _P, _P, _SZ]),
+    "tk_copy_device": (C.c_int, [_P, _P, _P, _SZ]),
     "tk_frame_create": (C.c_int, [_P, C.POINTER(Scene), C.POINTER(_P)]),
     "tk_frame_update": (C.c_int, [_P, C.POINTER(Scene)]),
     "tk_frame_destroy": (None, [_P]),

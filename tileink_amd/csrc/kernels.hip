@@ -56,6 +56,12 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_
     return before + x - v;
 }
 
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
+    return v;
+}
+
 __device__ __forceinline__ unsigned long long pack_status(uint32_t value, uint32_t flag, uint32_t epoch) {
     return (unsigned long long)value | ((unsigned long long)flag << 32) | ((unsigned long long)(epoch & 0x3FFFFFFFu) << 34);
 }
@@ -65,24 +71,36 @@ __device__ __forceinline__ void publish(unsigned long long *status, uint32_t ind
     __hip_atomic_store(&status[index], pack_status(value, flag, epoch), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Sums the published values of blocks [lo, k) walking backward until an
-// inclusive prefix is found.  Returns true when one was found.
-__device__ bool lookback(unsigned long long *status, uint32_t k, uint32_t lo, uint32_t epoch, uint32_t &acc) {
-    acc = 0u;
-    int64_t j = (int64_t)k - 1;
-    while (j >= (int64_t)lo) {
-        unsigned long long s = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t ep = (uint32_t)(s >> 34) & 0x3FFFFFFFu;
-        uint32_t flag = (uint32_t)(s >> 32) & 3u;
-        if (ep != (epoch & 0x3FFFFFFFu) || flag == 0u) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
+// Exclusive prefix of block k by decoupled lookback, run by one full wave: lane i
+// reads the status word of block hi - i, so each step inspects 64 predecessors.
+// The lanes up to the nearest inclusive prefix are summed once all of them have
+// published; otherwise the published run nearest to k is consumed and the rest
+// is read again.  Blocks before 0 count as an inclusive prefix of 0.
+__device__ uint32_t wave_lookback(unsigned long long *status, uint32_t k, uint32_t epoch, uint32_t lane) {
+    uint32_t acc = 0u;
+    int64_t hi = (int64_t)k - 1;
+    const uint32_t want = epoch & 0x3FFFFFFFu;
+    while (hi >= 0) {
+        const int64_t j = hi - (int64_t)lane;
+        uint32_t flag = kFlagPrefix, value = 0u;
+        if (j >= 0) {
+            const unsigned long long st = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            flag = ((uint32_t)(st >> 34) & 0x3FFFFFFFu) == want ? (uint32_t)(st >> 32) & 3u : 0u;
+            value = (uint32_t)st;
         }
-        acc += (uint32_t)s;
-        if (flag == kFlagPrefix) return true;
-        --j;
+        const unsigned long long prefix_lanes = __ballot(flag == kFlagPrefix);
+        const unsigned long long missing_lanes = __ballot(flag == 0u);
+        const uint32_t stop = prefix_lanes ? (uint32_t)__builtin_ctzll(prefix_lanes) : 64u;
+        const uint32_t gap = missing_lanes ? (uint32_t)__builtin_ctzll(missing_lanes) : 64u;
+        if (stop < gap) {
+            acc += wave_sum_u32(lane <= stop ? value : 0u);
+            break;
+        }
+        acc += wave_sum_u32(lane < gap ? value : 0u);
+        hi -= (int64_t)gap;
+        if (gap < 64u) __builtin_amdgcn_s_sleep(1);
     }
-    return false;
+    return acc;
 }
 
 // Dynamic block order: ticket k is only handed out after tickets < k, so a
@@ -555,6 +573,13 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fill(tk_args_fill a) {
     for (; i < a.count; i += stride) a.dst[i] = a.value;
 }
 
+// Plain device copy, 16 bytes per lane per step (traffic calibration).
+extern "C" __global__ __launch_bounds__(256) void tk_k_copy(tk_args_copy a) {
+    uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    for (; i < a.count; i += stride) a.dst[i] = a.src[i];
+}
+
 // scan_clear (scan/clear.hlsl:16-32), non-incremental.
 extern "C" __global__ __launch_bounds__(256) void tk_k_scan_clear(tk_args_scan_clear a) {
     uint32_t index = blockIdx.x * 256u + threadIdx.x;
@@ -572,17 +597,58 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_clear(tk_args_scan_c
     }
 }
 
+// Wave-aggregated counter updates.  Consecutive lines of a path are short and
+// mostly fall in the same few tiles, so the active lanes of a wave are grouped by
+// target word and each group issues one atomic.  Must be called from code where
+// every lane that reaches it takes part (any subset of the wave may be active).
+__device__ __forceinline__ void wave_add_pm1(uint32_t *words, uint32_t index, bool plus) {
+    const uint32_t lane = __lane_id();
+    unsigned long long pending = __ballot(1);
+    for (;;) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
+        const uint32_t lead_index = (uint32_t)__shfl((int)index, (int)leader, 64);
+        const unsigned long long peers = __ballot(index == lead_index);
+        const unsigned long long ups = __ballot(index == lead_index && plus);
+        if (index == lead_index) {
+            if (lane == leader)
+                atomicAdd(&words[index], (uint32_t)(__popcll(ups) - __popcll(peers & ~ups)));
+            return;
+        }
+        pending &= ~peers;
+    }
+}
+
+// Claims one slot of cursors[index] per lane; lanes with the same index receive
+// consecutive slots from one atomic.
+__device__ __forceinline__ uint32_t wave_claim_slot(uint32_t *cursors, uint32_t index) {
+    const uint32_t lane = __lane_id();
+    unsigned long long pending = __ballot(1);
+    for (;;) {
+        const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
+        const uint32_t lead_index = (uint32_t)__shfl((int)index, (int)leader, 64);
+        const unsigned long long peers = __ballot(index == lead_index);
+        if (index == lead_index) {
+            uint32_t first = 0u;
+            if (lane == leader) first = atomicAdd(&cursors[index], (uint32_t)__popcll(peers));
+            first = (uint32_t)__shfl((int)first, (int)leader, 64);
+            return first + (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+        }
+        pending &= ~peers;
+    }
+}
+
 // scan_count (scan/count.hlsl:13-40).
 extern "C" __global__ __launch_bounds__(256) void tk_k_scan_count(tk_args_scan_lines a) {
     uint32_t id = blockIdx.x * 256u + threadIdx.x;
     if (id >= a.cfg.line_count) return;
     ScanTraversal s;
     if (!scan_geometry(a.lines, a.paths, a.path_record_count, id, s)) return;
-    int32_t delta = s.down ? -1 : 1;
+    uint32_t *backdrops = reinterpret_cast<uint32_t *>(a.backdrops);
+    const bool up = !s.down;  // delta = down ? -1 : +1
     uint32_t stride = s.bz - s.bx;
     for (int32_t y = s.ymin; y < s.ymax; y++) {
         uint32_t local = (uint32_t)(y - (int32_t)s.by) * stride;
-        atomicAdd(&a.backdrops[s.data_offset + local], delta);
+        wave_add_pm1(backdrops, s.data_offset + local, up);
     }
     float last_z = floorf(s.a * ((float)s.imin - 1.0f) + s.b);
     for (uint32_t i = s.imin; i < s.imax; i++) {
@@ -595,9 +661,9 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_count(tk_args_scan_l
             if (top && tx + 1 < (int32_t)s.bz) {
                 int32_t x = max(tx + 1, (int32_t)s.bx);
                 uint32_t local = (uint32_t)(ty - (int32_t)s.by) * stride + (uint32_t)(x - (int32_t)s.bx);
-                atomicAdd(&a.backdrops[s.data_offset + local], delta);
+                wave_add_pm1(backdrops, s.data_offset + local, up);
             }
-            atomicAdd(&a.counts_or_cursors[scan_tile_offset(s, tx, ty)], 1u);
+            wave_add_pm1(a.counts_or_cursors, scan_tile_offset(s, tx, ty), true);
         }
         last_z = z;
     }
@@ -617,21 +683,73 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t lan
 }
 }  // namespace
 
-// scan_prefix_chunks + scan_chunk_offsets + scan_apply_chunk_offsets as one
-// pass: each 64-lane wave scans one 256-tile chunk (four tiles per lane), then
-// finds its path-relative base by decoupled lookback over the path's earlier
-// chunks.  Wave w of the launch takes ticket w in dispatch order, so a wave only
-// waits on chunks whose waves are already running.
+// Shared by both scan passes: the chunk's local scan (four tiles per lane) and
+// whether the offsets pass of the chunk's path covers it.
+struct ScanChunkView {
+    tk_scan_chunk chunk;
+    uint32_t len;            // tiles this chunk's lanes touch: min(chunk.len, 256)
+    uint32_t v[4];           // counts of tiles lane*4 .. lane*4+3
+    uint32_t excl, total;    // lane-exclusive prefix and chunk total
+    uint32_t path;           // chunk.path_id
+    tk_scan_chunk_range r;   // the path's chunk range (empty when the path id is invalid)
+    bool in_range;           // r.start <= k < r.end
+    uint32_t seg_start;      // the path's segment_start
+};
+
+__device__ __forceinline__ void load_scan_chunk(const tk_args_scan_prefix &a, uint32_t k, uint32_t lane, ScanChunkView &c) {
+    c.chunk = a.chunks[k];
+    c.len = min(c.chunk.len, 256u);
+    uint32_t mine = 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) {
+        const uint32_t e = lane * 4u + i;
+        c.v[i] = e < c.len ? a.counts[c.chunk.backdrop_offset + e] : 0u;
+        mine += c.v[i];
+    }
+    c.excl = wave_exclusive_scan(mine, lane, c.total);
+    c.path = c.chunk.path_id;
+    c.r = tk_scan_chunk_range{0u, 0u};
+    c.seg_start = 0u;
+    if (c.path < a.cfg.path_count) {
+        c.r = a.chunk_ranges[c.path];
+        c.seg_start = a.paths[c.path].segment_start;
+    }
+    c.in_range = k >= c.r.start && k < c.r.end;
+}
+
+// scan_apply_chunk_offsets for one chunk whose offset is `base`.
+__device__ __forceinline__ void apply_scan_chunk(const tk_args_scan_prefix &a, const ScanChunkView &c, uint32_t lane,
+                                                 uint32_t base) {
+    uint32_t running = base + c.excl;
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) {
+        const uint32_t e = lane * 4u + i;
+        if (e < c.len) {
+            a.ranges[c.chunk.backdrop_offset + e] = tk_segment_range{running, running + c.v[i]};
+            a.cursors[c.chunk.backdrop_offset + e] = running;
+        }
+        running += c.v[i];
+    }
+}
+
+// The chunk total as scan_prefix_chunks stores it: only a chunk of 1..256 tiles
+// writes its word (prefix_chunks.hlsl:37); any other keeps the buffer's value.
+__device__ __forceinline__ bool scan_chunk_writes_total(const ScanChunkView &c) {
+    return c.chunk.len >= 1u && c.chunk.len <= 256u;
+}
+
+// Pass A of scan_prefix_chunks + scan_chunk_offsets + scan_apply_chunk_offsets
+// (scan/prefix_chunks.hlsl:16-38, chunk_offsets.hlsl:14-26,
+// apply_chunk_offsets.hlsl:15-27).  One 64-lane wave per chunk.  Each path's
+// offsets form an independent chain over its own chunks, and almost every path
+// is a single chunk, so this pass finishes every chunk whose offset needs no
+// other chunk (a path's first chunk, or a chunk outside its path's range) and
+// stores every chunk total; tk_k_scan_chunk_apply finishes the rest.  No wave
+// ever waits on another.
 extern "C" __global__ __launch_bounds__(256) void tk_k_scan_prefix(tk_args_scan_prefix a) {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave_id = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (wave_id >= a.block_count) return;
-    uint32_t t = 0u;
-    if (lane == 0u) {
-        t = atomicAdd(a.ticket, 1u);
-        if (t == a.block_count - 1u) atomicExch(a.ticket, 0u);
-    }
-    const uint32_t k = (uint32_t)__shfl((int)t, 0, 64);
+    const uint32_t k = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (k >= a.wave_count) return;
 
     // Paths without chunks still receive a zero bump (scan_chunk_offsets runs per path).
     const uint32_t p = k * 64u + lane;
@@ -641,60 +759,55 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_prefix(tk_args_scan_
     }
     if (k >= a.cfg.scan_chunk_count) return;
 
-    const tk_scan_chunk chunk = a.chunks[k];
-    const uint32_t len = min(chunk.len, 256u);
-    uint32_t v[4];
-    uint32_t mine = 0u;
-#pragma unroll
-    for (uint32_t i = 0; i < 4u; ++i) {
-        const uint32_t e = lane * 4u + i;
-        v[i] = e < len ? a.counts[chunk.backdrop_offset + e] : 0u;
-        mine += v[i];
+    ScanChunkView c;
+    load_scan_chunk(a, k, lane, c);
+    if (c.in_range && k != c.r.start) {
+        if (lane == 0u && scan_chunk_writes_total(c)) a.chunk_totals[k] = c.total;
+        return;  // offset depends on earlier chunks of the path: pass B
     }
-    uint32_t total;
-    const uint32_t excl = wave_exclusive_scan(mine, lane, total);
-
     uint32_t base = 0u;
     if (lane == 0u) {
-        const uint32_t path = chunk.path_id;
-        bool in_range = false;
-        tk_scan_chunk_range r{0u, 0u};
-        uint32_t seg_start = 0u;
-        if (path < a.cfg.path_count) {
-            r = a.chunk_ranges[path];
-            in_range = k >= r.start && k < r.end;
-            seg_start = a.paths[path].segment_start;
-        }
-        if (in_range) {
-            if (k == r.start) {
-                base = seg_start;
-            } else {
-                publish(a.status, k, total, kFlagAggregate, a.epoch);
-                uint32_t acc;
-                const bool found = lookback(a.status, k, r.start, a.epoch, acc);
-                base = found ? acc : acc + seg_start;
-            }
-            publish(a.status, k, base + total, kFlagPrefix, a.epoch);
+        uint32_t stored_total = c.total;
+        if (scan_chunk_writes_total(c)) a.chunk_totals[k] = c.total;
+        else stored_total = a.chunk_totals[k];
+        if (c.in_range) {
+            base = c.seg_start;
             a.chunk_offsets[k] = base;
-            if (k + 1u == r.end) a.bumps[path] = base + total - seg_start;
+            if (k + 1u == c.r.end) a.bumps[c.path] = stored_total;
         } else {
-            // Not reached by any path's offset pass: the apply pass reads the current offset word.
-            base = a.chunk_offsets[k];
-            publish(a.status, k, base + total, kFlagPrefix, a.epoch);
+            base = a.chunk_offsets[k];  // no path's offsets pass reaches it: keeps the buffer's offset
         }
-        if (len > 0u) a.chunk_totals[k] = total;
     }
     base = (uint32_t)__shfl((int)base, 0, 64);
-    uint32_t running = base + excl;
-#pragma unroll
-    for (uint32_t i = 0; i < 4u; ++i) {
-        const uint32_t e = lane * 4u + i;
-        if (e < len) {
-            a.ranges[chunk.backdrop_offset + e] = tk_segment_range{running, running + v[i]};
-            a.cursors[chunk.backdrop_offset + e] = running;
-        }
-        running += v[i];
+    apply_scan_chunk(a, c, lane, base);
+}
+
+
+// Pass B: the chunks after the first of their path.  The offset is the path's
+// segment_start plus the stored totals of its earlier chunks (chunk_offsets.hlsl
+// :21-24), summed 64 at a time by one wave; the last chunk also stores the bump.
+extern "C" __global__ __launch_bounds__(256) void tk_k_scan_chunk_apply(tk_args_scan_prefix a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t k = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (k >= a.cfg.scan_chunk_count) return;
+    const tk_scan_chunk head = a.chunks[k];
+    if (head.path_id >= a.cfg.path_count) return;
+    const tk_scan_chunk_range r = a.chunk_ranges[head.path_id];
+    if (!(k > r.start && k < r.end)) return;
+
+    ScanChunkView c;
+    load_scan_chunk(a, k, lane, c);
+    uint32_t before = 0u;
+    for (uint32_t j0 = r.start; j0 < k; j0 += 64u) {
+        const uint32_t j = j0 + lane;
+        before += j < k ? a.chunk_totals[j] : 0u;
     }
+    const uint32_t base = c.seg_start + wave_sum_u32(before);
+    if (lane == 0u) {
+        a.chunk_offsets[k] = base;
+        if (k + 1u == r.end) a.bumps[c.path] = base + a.chunk_totals[k] - c.seg_start;
+    }
+    apply_scan_chunk(a, c, lane, base);
 }
 
 // scan_emit (scan/emit.hlsl:14-28).
@@ -708,7 +821,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_emit(tk_args_scan_li
         int32_t tx, ty;
         scan_tile(s, i, z, tx, ty);
         if (scan_tile_inside(s, tx, ty)) {
-            uint32_t dst = atomicAdd(&a.counts_or_cursors[scan_tile_offset(s, tx, ty)], 1u);
+            uint32_t dst = wave_claim_slot(a.counts_or_cursors, scan_tile_offset(s, tx, ty));
             if (dst < a.cfg.segment_capacity) a.segments[dst] = clipped_segment(s, i, z, tx, ty);
         }
     }
@@ -921,44 +1034,62 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coar
 extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coarse a) {
     __shared__ uint32_t s_ticket, s_base;
     __shared__ uint32_t s_wave[4];
-    constexpr uint32_t kItems = 4u;
+    constexpr uint32_t kItems = TK_COARSE_SCAN_ITEMS / 256u;  // 16 consecutive counts per lane
     const uint32_t k = draw_ticket(a.ticket, a.scan_blocks, &s_ticket);
     const uint64_t n = (uint64_t)a.cfg.batch_count * a.cfg.tile_count;
     const uint64_t first = ((uint64_t)k * 256u + threadIdx.x) * kItems;
     uint32_t v[kItems];
     uint32_t sum = 0u;
+    if (first + kItems <= n && ((uintptr_t)a.out.tile_ptcl_counts & 15u) == 0u) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.out.tile_ptcl_counts + first);
 #pragma unroll
-    for (uint32_t i = 0; i < kItems; ++i) {
-        v[i] = first + i < n ? a.out.tile_ptcl_counts[first + i] : 0u;
-        sum += v[i];
+        for (uint32_t q = 0; q < kItems / 4u; ++q) {
+            const uint4 w = src[q];
+            v[4 * q] = w.x;
+            v[4 * q + 1] = w.y;
+            v[4 * q + 2] = w.z;
+            v[4 * q + 3] = w.w;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t i = 0; i < kItems; ++i) v[i] = first + i < n ? a.out.tile_ptcl_counts[first + i] : 0u;
     }
+#pragma unroll
+    for (uint32_t i = 0; i < kItems; ++i) sum += v[i];
     uint32_t total;
-    uint32_t excl = block_exclusive_scan_256(sum, s_wave, total);
-    if (threadIdx.x == 0u) {
+    const uint32_t excl = block_exclusive_scan_256(sum, s_wave, total);
+    if (threadIdx.x < 64u) {
         uint32_t base = 0u;
         if (k > 0u) {
-            publish(a.scan_status, k, total, kFlagAggregate, a.epoch);
-            uint32_t acc;
-            lookback(a.scan_status, k, 0u, a.epoch, acc);
-            base = acc;
+            if (threadIdx.x == 0u) publish(a.scan_status, k, total, kFlagAggregate, a.epoch);
+            base = wave_lookback(a.scan_status, k, a.epoch, threadIdx.x);
         }
-        publish(a.scan_status, k, base + total, kFlagPrefix, a.epoch);
-        s_base = base;
-        if (k + 1u == a.scan_blocks) {
-            a.out.batch_ptcl_bases[a.cfg.batch_count] = base + total;
-            a.out.status[1] = base + total;
+        if (threadIdx.x == 0u) {
+            publish(a.scan_status, k, base + total, kFlagPrefix, a.epoch);
+            s_base = base;
+            if (k + 1u == a.scan_blocks) {
+                a.out.batch_ptcl_bases[a.cfg.batch_count] = base + total;
+                a.out.status[1] = base + total;
+            }
         }
     }
     __syncthreads();
+    if (first >= n) return;
     uint32_t running = s_base + excl;
+    uint64_t batch = first / a.cfg.tile_count;
+    uint32_t offset = (uint32_t)(first - batch * a.cfg.tile_count);
 #pragma unroll
     for (uint32_t i = 0; i < kItems; ++i) {
-        uint64_t idx = first + i;
+        const uint64_t idx = first + i;
         if (idx < n) {
             a.out.tile_ptcl_ranges[idx] = tk_segment_range{running, running + v[i]};
-            if (idx % a.cfg.tile_count == 0u) a.out.batch_ptcl_bases[idx / a.cfg.tile_count] = running;
+            if (offset == 0u) a.out.batch_ptcl_bases[batch] = running;
         }
         running += v[i];
+        if (++offset == a.cfg.tile_count) {
+            offset = 0u;
+            ++batch;
+        }
     }
 }
 

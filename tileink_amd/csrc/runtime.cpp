@@ -63,7 +63,7 @@ struct tk_context {
     int device = -1;
     hipModule_t module = nullptr;
     hipStream_t stream = nullptr;
-    hipFunction_t f_fill = nullptr, f_scan_clear = nullptr, f_scan_count = nullptr, f_scan_prefix = nullptr,
+    hipFunction_t f_fill = nullptr, f_copy = nullptr, f_scan_clear = nullptr, f_scan_count = nullptr, f_scan_prefix = nullptr, f_scan_chunk_apply = nullptr,
                   f_scan_emit = nullptr, f_cumsum = nullptr, f_coarse_count = nullptr,
                   f_coarse_scan = nullptr, f_coarse_emit = nullptr, f_fine = nullptr, f_fill_coverage = nullptr,
                   f_pixel_math = nullptr;
@@ -137,13 +137,15 @@ tk_status run_scan_emit(tk_context *ctx, const tk_scan_config &cfg, const tk_lin
 tk_status run_scan_prefix(tk_context *ctx, const tk_scan_config &cfg, const tk_scan_chunk *chunks,
                           const tk_scan_chunk_range *chunk_ranges, const tk_path_record *paths, const uint32_t *counts,
                           tk_segment_range *ranges, uint32_t *cursors, uint32_t *bumps, uint32_t *chunk_totals,
-                          uint32_t *chunk_offsets, unsigned long long *status, uint32_t *ticket, uint32_t epoch) {
-    // One wave per chunk; waves beyond the chunks zero the bumps of chunk-less paths.
+                          uint32_t *chunk_offsets) {
+    // Pass A: one wave per chunk; waves beyond the chunks zero the bumps of chunk-less paths.
     uint32_t waves = std::max<uint32_t>(cfg.scan_chunk_count, div_up(cfg.path_count, 64));
     if (waves == 0) return TK_OK;
-    tk_args_scan_prefix a{cfg,           chunks, chunk_ranges, paths,  counts, ranges, cursors,
-                          bumps,         chunk_totals, chunk_offsets, status, ticket, epoch,  waves};
-    return launch(ctx, ctx->f_scan_prefix, div_up(waves, 4), 1, 256, a);
+    tk_args_scan_prefix a{cfg, chunks, chunk_ranges, paths, counts, ranges, cursors, bumps, chunk_totals,
+                          chunk_offsets, waves};
+    TK_TRY(launch(ctx, ctx->f_scan_prefix, div_up(waves, 4), 1, 256, a));
+    // Pass B: chunks after the first of their path.
+    return launch(ctx, ctx->f_scan_chunk_apply, div_up(cfg.scan_chunk_count, 4), 1, 256, a);
 }
 
 tk_status run_cumsum(tk_context *ctx, const tk_cumsum_config &cfg, const uint32_t *offs, const uint32_t *lens,
@@ -155,7 +157,7 @@ tk_status run_cumsum(tk_context *ctx, const tk_cumsum_config &cfg, const uint32_
 
 uint32_t coarse_scan_blocks(const tk_coarse_config &cfg) {
     uint64_t n = uint64_t(cfg.batch_count) * cfg.tile_count;
-    return uint32_t(std::max<uint64_t>((n + 1023) / 1024, 1));
+    return uint32_t(std::max<uint64_t>((n + TK_COARSE_SCAN_ITEMS - 1) / TK_COARSE_SCAN_ITEMS, 1));
 }
 
 tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coarse_inputs &in,
@@ -222,7 +224,7 @@ struct tk_frame {
     uint32_t *image = nullptr;
     // scratch
     uint32_t *tickets = nullptr;
-    unsigned long long *prefix_status = nullptr, *coarse_status = nullptr;
+    unsigned long long *coarse_status = nullptr;
     uint32_t epoch = 0;
     bool capture_raw = false;
     bool timing = false;
@@ -368,8 +370,7 @@ tk_status frame_render(tk_frame *f) {
     TK_TRY(run_scan_count(ctx, scfg, f->lines, f->paths, s.path_count, f->backdrops, f->seg_counts));
     TK_TRY(record(f, 2));
     TK_TRY(run_scan_prefix(ctx, scfg, f->scan_chunks, f->scan_chunk_ranges, f->paths, f->seg_counts, f->seg_ranges,
-                           f->cursors, f->bumps, f->chunk_totals, f->chunk_offsets, f->prefix_status, &f->tickets[0],
-                           f->epoch));
+                           f->cursors, f->bumps, f->chunk_totals, f->chunk_offsets));
     TK_TRY(record(f, 3));
     TK_TRY(run_scan_emit(ctx, scfg, f->lines, f->paths, s.path_count, f->cursors, f->segments));
     TK_TRY(record(f, 4));
@@ -443,9 +444,11 @@ tk_status tk_context_create(int device, tk_context **out) {
         const char *name;
     } table[] = {
         {&ctx->f_fill, "tk_k_fill"},
+        {&ctx->f_copy, "tk_k_copy"},
         {&ctx->f_scan_clear, "tk_k_scan_clear"},
         {&ctx->f_scan_count, "tk_k_scan_count"},
         {&ctx->f_scan_prefix, "tk_k_scan_prefix"},
+        {&ctx->f_scan_chunk_apply, "tk_k_scan_chunk_apply"},
         {&ctx->f_scan_emit, "tk_k_scan_emit"},
         {&ctx->f_cumsum, "tk_k_cumsum"},
         {&ctx->f_coarse_count, "tk_k_coarse_count"},
@@ -533,6 +536,17 @@ tk_status tk_copy_to_host(tk_context *ctx, void *dst, const void *src, size_t by
     return TK_OK;
 }
 
+tk_status tk_copy_device(tk_context *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(TK_ERR_INVALID, "null argument");
+    if ((bytes | (uintptr_t)dst | (uintptr_t)src) & 15u) return fail(TK_ERR_INVALID, "copy must be 16-byte aligned");
+    if (!bytes) return TK_OK;
+    tk_args_copy a{static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), bytes / 16u};
+    const uint32_t blocks = uint32_t(std::min<uint64_t>((a.count + 255) / 256, 8192));
+    TK_TRY(launch(ctx, ctx->f_copy, blocks, 1, 256, a));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    return TK_OK;
+}
+
 // --- frames -----------------------------------------------------------------
 
 tk_status tk_frame_create(tk_context *ctx, const tk_scene *s, tk_frame **out) {
@@ -587,14 +601,12 @@ tk_status tk_frame_create(tk_context *ctx, const tk_scene *s, tk_frame **out) {
     ALLOC(ptcl, f->ptcl_capacity);
     ALLOC(image, uint64_t(s->width) * s->height);
     ALLOC(tickets, 4);
-    ALLOC(prefix_status, std::max<uint64_t>(s->scan_chunk_count, 1));
     {
         tk_coarse_config cc = frame_coarse_config(f);
         ALLOC(coarse_status, coarse_scan_blocks(cc));
     }
 #undef ALLOC
     if (hipMemsetAsync(f->tickets, 0, 16, ctx->stream) != hipSuccess ||
-        hipMemsetAsync(f->prefix_status, 0, std::max<uint64_t>(s->scan_chunk_count, 1) * 8, ctx->stream) != hipSuccess ||
         hipMemsetAsync(f->coarse_status, 0, uint64_t(coarse_scan_blocks(frame_coarse_config(f))) * 8, ctx->stream) !=
             hipSuccess ||
         hipMemsetAsync(f->cs_carries, 0, std::max<uint64_t>(s->cumsum_chunk_count, 1) * 4, ctx->stream) != hipSuccess)
@@ -798,9 +810,8 @@ tk_status tk_scan_prefix(tk_context *ctx, const tk_scan_config *cfg, const tk_sc
                          uint32_t *chunk_totals, uint32_t *chunk_offsets) {
     if (!ctx || !cfg) return fail(TK_ERR_INVALID, "null argument");
     if (cfg->incremental) return fail(TK_ERR_INVALID, "incremental scan is not supported");
-    TK_TRY(ensure_status(ctx, std::max<uint32_t>(cfg->scan_chunk_count, 1)));
     return run_scan_prefix(ctx, *cfg, chunks, chunk_ranges, paths, counts, ranges, cursors, bumps, chunk_totals,
-                           chunk_offsets, ctx->status, &ctx->tickets[0], ctx->next_epoch());
+                           chunk_offsets);
 }
 
 tk_status tk_scan_emit(tk_context *ctx, const tk_scan_config *cfg, const tk_line *lines, const tk_path_record *paths,

@@ -7,12 +7,19 @@
 
 #include "tileink_hip.h"
 
-#define TK_FINE_MAX_CLIP_SPILL 60u  // clip stack levels beyond the 4 register slots, held in LDS
+#define TK_FINE_MAX_CLIP_SPILL 60u    // clip stack levels beyond the 4 register slots, held in LDS
+#define TK_COARSE_SCAN_ITEMS 4096u    // counts per workgroup of the coarse exclusive scan
 
 struct tk_args_fill {
     uint32_t *dst;
     uint64_t count;
     uint32_t value;
+};
+
+struct tk_args_copy {
+    const uint4 *src;
+    uint4 *dst;
+    uint64_t count;  // 16-byte words
 };
 
 struct tk_args_scan_clear {
@@ -47,10 +54,7 @@ struct tk_args_scan_prefix {
     uint32_t *bumps;
     uint32_t *chunk_totals;
     uint32_t *chunk_offsets;
-    unsigned long long *status;  // one word per chunk: value | flag << 32 | epoch << 34
-    uint32_t *ticket;
-    uint32_t epoch;
-    uint32_t block_count;  // waves in the launch, one ticket each
+    uint32_t wave_count;  // pass A: max(chunks, ceil(paths / 64)) waves
 };
 
 struct tk_args_cumsum {
