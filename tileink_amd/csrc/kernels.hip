@@ -56,10 +56,39 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_
     return before + x - v;
 }
 
+// Exclusive wrapping-u32 scan across one 64-lane wave.
+__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t lane, uint32_t &total) {
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
     return v;
+}
+
+// Workgroups are dealt to the eight XCDs round-robin in launch order.  This
+// renumbers a launch of `grid` workgroups (a multiple of 8 * chunk) so that each
+// XCD receives runs of `chunk` consecutive block indices, every eighth run: the
+// cache lines that neighbouring blocks share (tile table words, a path's
+// consecutive lines and their counters) are then fetched into one XCD's L2, and
+// the runs stay interleaved across the XCDs so that spatially uneven work is
+// still spread evenly.  chunk == 0 keeps the launch order.
+__device__ __forceinline__ uint32_t xcd_chunked_block(uint32_t linear, uint32_t chunk) {
+    if (chunk == 0u) return linear;
+    const uint32_t x = linear & 7u, k = linear >> 3;
+    return ((k / chunk) * 8u + x) * chunk + (k % chunk);
+}
+__device__ __forceinline__ uint32_t xcd_grid(uint64_t blocks, uint32_t chunk) {
+    const uint64_t unit = chunk ? 8u * (uint64_t)chunk : 1u;
+    return (uint32_t)((blocks + unit - 1u) / unit * unit);
 }
 
 __device__ __forceinline__ unsigned long long pack_status(uint32_t value, uint32_t flag, uint32_t epoch) {
@@ -497,18 +526,19 @@ __device__ __forceinline__ float sdf_coverage_from_draw(const tk_fine_config &cf
 __device__ __forceinline__ bool premul_opaque(uint32_t c) { return (c >> 24) == 255u; }
 
 // Exact coverage sums of one FILL / BEGIN_CLIP record over a 16x16 tile, built
-// segment-row by segment-row instead of pixel by pixel: one lane takes one
-// (segment, row) pair, evaluates the row parts once and adds its terms into LDS
-// with 64-bit integer atomics (exact, so the order of arrival does not matter):
-//   base[row]       += edge term,
-//   run[row][fs]    += dy, where fs = full_start (prefix-summed along the row later),
-//   part[row][x]    += area(x) * dy for the straddled pixels x in [partial_start, fs).
+// segment by segment instead of pixel by pixel, with 64-bit integer atomics in
+// LDS (exact, so the order of arrival does not matter):
+//   base[row]     += the row's edge term,
+//   run[row][fs]  += dy, where fs = full_start (prefix-summed along the row later),
+//   part[row][x]  += area(x) * dy for the straddled pixels x in [partial_start, fs).
 struct CoverageAcc {
     long long base[16];
     long long run[256];
     long long part[256];
 };
 
+// All terms of segments [0, n) of one record: lane l takes (segment l / 16,
+// row l % 16), evaluates segment_row_parts for that row once, and adds its terms.
 __device__ __forceinline__ void accumulate_record(const tk_line_segment *segs, uint32_t n, CoverageAcc &acc,
                                                   uint32_t lane) {
     const uint32_t row = lane & 15u;
@@ -639,7 +669,8 @@ __device__ __forceinline__ uint32_t wave_claim_slot(uint32_t *cursors, uint32_t 
 
 // scan_count (scan/count.hlsl:13-40).
 extern "C" __global__ __launch_bounds__(256) void tk_k_scan_count(tk_args_scan_lines a) {
-    uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t block = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_SCAN);
+    uint32_t id = block * 256u + threadIdx.x;
     if (id >= a.cfg.line_count) return;
     ScanTraversal s;
     if (!scan_geometry(a.lines, a.paths, a.path_record_count, id, s)) return;
@@ -669,19 +700,6 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_count(tk_args_scan_l
     }
 }
 
-namespace {
-// Exclusive wrapping-u32 scan across one 64-lane wave.
-__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t lane, uint32_t &total) {
-    uint32_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(x, off, 64);
-        if (lane >= (uint32_t)off) x += y;
-    }
-    total = __shfl(x, 63, 64);
-    return x - v;
-}
-}  // namespace
 
 // Shared by both scan passes: the chunk's local scan (four tiles per lane) and
 // whether the offsets pass of the chunk's path covers it.
@@ -812,7 +830,8 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_chunk_apply(tk_args_
 
 // scan_emit (scan/emit.hlsl:14-28).
 extern "C" __global__ __launch_bounds__(256) void tk_k_scan_emit(tk_args_scan_lines a) {
-    uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t block = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_SCAN);
+    uint32_t id = block * 256u + threadIdx.x;
     if (id >= a.cfg.line_count) return;
     ScanTraversal s;
     if (!scan_geometry(a.lines, a.paths, a.path_record_count, id, s)) return;
@@ -978,7 +997,8 @@ __device__ __forceinline__ void load_list_entry(const tk_args_coarse &a, const t
 extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coarse a) {
     const tk_coarse_config &cfg = a.cfg;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t tile = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t tile =
+        xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + (threadIdx.x >> 6);
     if (tile >= cfg.tile_count) return;
     const CoarseCtx c{&a.cfg, &a.in, tile % cfg.tiles_width, tile / cfg.tiles_width};
     const uint32_t T = cfg.tile_count, Q = cfg.batch_count;
@@ -1101,7 +1121,8 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coars
 extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coarse a) {
     const tk_coarse_config &cfg = a.cfg;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t tile = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t tile =
+        xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + (threadIdx.x >> 6);
     if (tile >= cfg.tile_count) return;
     const CoarseCtx c{&a.cfg, &a.in, tile % cfg.tiles_width, tile / cfg.tiles_width};
     const uint32_t T = cfg.tile_count, Q = cfg.batch_count;
@@ -1214,7 +1235,9 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fine(tk_args_fine a) {
     __shared__ uint32_t batch_kind[256];
     __shared__ tk_segment_range batch_range[256];
     const tk_fine_config &cfg = a.cfg;
-    const uint32_t tile_ix = blockIdx.x + blockIdx.y * gridDim.x;
+    const uint32_t linear = blockIdx.x + blockIdx.y * gridDim.x;
+    if (linear >= xcd_grid(cfg.tile_count, TK_XCD_CHUNK_FINE)) return;
+    const uint32_t tile_ix = xcd_chunked_block(linear, TK_XCD_CHUNK_FINE);
     if (tile_ix >= cfg.tile_count) return;
     const uint32_t lane = threadIdx.x;
     const uint32_t lx = lane & 15u, ly = lane >> 4;

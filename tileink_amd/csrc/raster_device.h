@@ -379,10 +379,19 @@ TKD float segment_area_at(float xmin_abs, float xmax_abs, uint32_t x) {
 // truncated to 32.32 fixed point (|t| <= 2^30, NaN -> 0) and the terms are added
 // as 64-bit integers, so a sum's value does not depend on the order in which the
 // tile's segments were emitted.  Only the conversion back to FP32 rounds.
+//
+// The truncation uses FP32 steps that are all exact: with a = min(|t|, 2^30),
+// trunc(a) is the integer part and (a - trunc(a)) * 2^32 the fraction scaled to
+// 32 bits (a - trunc(a) is exact, and so is the power-of-two scaling), so
+// trunc(a * 2^32) = trunc(a) * 2^32 + trunc((a - trunc(a)) * 2^32).
 TKD long long coverage_fixed(float t) {
     if (!(t == t)) return 0;
-    long long m = (long long)((double)fminf(fabsf(t), 1073741824.0f) * 4294967296.0);
-    return t < 0.0f ? -m : m;
+    const float m = fminf(fabsf(t), 1073741824.0f);
+    const float whole = truncf(m);
+    const uint32_t hi = (uint32_t)whole;
+    const uint32_t lo = (uint32_t)((m - whole) * 4294967296.0f);
+    const long long q = (long long)(((unsigned long long)hi << 32) | lo);
+    return t < 0.0f ? -q : q;
 }
 TKD float fixed_to_float(long long v) { return (float)((double)v * 2.3283064365386962890625e-10); }
 TKD long long backdrop_fixed(int32_t backdrop) { return (long long)backdrop * 4294967296LL; }

@@ -83,6 +83,11 @@ namespace tk {
 namespace {
 
 uint32_t div_up(uint64_t n, uint32_t d) { return uint32_t((n + d - 1) / d); }
+// Grid of a kernel that renumbers its workgroups per XCD: a multiple of 8 * chunk.
+uint32_t xcd_grid(uint64_t blocks, uint32_t chunk) {
+    const uint64_t unit = chunk ? 8ull * chunk : 1ull;
+    return uint32_t((blocks + unit - 1) / unit * unit);
+}
 
 template <class Args>
 tk_status launch(tk_context *ctx, hipFunction_t f, uint32_t gx, uint32_t gy, uint32_t block, const Args &args,
@@ -125,13 +130,13 @@ tk_status fill_u32(tk_context *ctx, void *dst, uint64_t count, uint32_t value) {
 tk_status run_scan_count(tk_context *ctx, const tk_scan_config &cfg, const tk_line *lines, const tk_path_record *paths,
                          uint32_t path_record_count, int32_t *backdrops, uint32_t *counts) {
     tk_args_scan_lines a{cfg, lines, paths, path_record_count, backdrops, counts, nullptr};
-    return launch(ctx, ctx->f_scan_count, div_up(cfg.line_count, 256), 1, 256, a);
+    return launch(ctx, ctx->f_scan_count, xcd_grid(div_up(cfg.line_count, 256), TK_XCD_CHUNK_SCAN), 1, 256, a);
 }
 
 tk_status run_scan_emit(tk_context *ctx, const tk_scan_config &cfg, const tk_line *lines, const tk_path_record *paths,
                         uint32_t path_record_count, uint32_t *cursors, tk_line_segment *segments) {
     tk_args_scan_lines a{cfg, lines, paths, path_record_count, nullptr, cursors, segments};
-    return launch(ctx, ctx->f_scan_emit, div_up(cfg.line_count, 256), 1, 256, a);
+    return launch(ctx, ctx->f_scan_emit, xcd_grid(div_up(cfg.line_count, 256), TK_XCD_CHUNK_SCAN), 1, 256, a);
 }
 
 tk_status run_scan_prefix(tk_context *ctx, const tk_scan_config &cfg, const tk_scan_chunk *chunks,
@@ -165,11 +170,11 @@ tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coar
                      hipEvent_t *marks) {
     if (cfg.batch_count == 0 || cfg.tile_count == 0) return TK_OK;
     tk_args_coarse a{cfg, in, out, status, ticket, epoch, coarse_scan_blocks(cfg)};
-    TK_TRY(launch(ctx, ctx->f_coarse_count, div_up(cfg.tile_count, 4), 1, 256, a));
+    TK_TRY(launch(ctx, ctx->f_coarse_count, xcd_grid(div_up(cfg.tile_count, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
     if (marks) TK_HIP(hipEventRecord(marks[0], ctx->stream));
     TK_TRY(launch(ctx, ctx->f_coarse_scan, a.scan_blocks, 1, 256, a));
     if (marks) TK_HIP(hipEventRecord(marks[1], ctx->stream));
-    TK_TRY(launch(ctx, ctx->f_coarse_emit, div_up(cfg.tile_count, 4), 1, 256, a));
+    TK_TRY(launch(ctx, ctx->f_coarse_emit, xcd_grid(div_up(cfg.tile_count, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
     return TK_OK;
 }
 
@@ -179,8 +184,9 @@ tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inp
         return TK_ERR_INVALID;
     }
     tk_args_fine a{cfg, in, rgba};
-    uint32_t gx = std::min<uint32_t>(cfg.tile_count, 65535u);
-    uint32_t gy = div_up(cfg.tile_count, gx);
+    const uint32_t blocks = xcd_grid(cfg.tile_count, TK_XCD_CHUNK_FINE);
+    uint32_t gx = std::min<uint32_t>(blocks, 65535u);
+    uint32_t gy = div_up(blocks, gx);
     return launch(ctx, ctx->f_fine, gx, gy, 256, a, cfg.clip_spill_depth * 256u);
 }
 

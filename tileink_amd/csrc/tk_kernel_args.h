@@ -10,6 +10,18 @@
 #define TK_FINE_MAX_CLIP_SPILL 60u    // clip stack levels beyond the 4 register slots, held in LDS
 #define TK_COARSE_SCAN_ITEMS 4096u    // counts per workgroup of the coarse exclusive scan
 
+// Runs of consecutive workgroups kept on one XCD (0: launch order).  Fine: tiles;
+// coarse: 4-tile workgroups; scan: 256-line workgroups.
+#ifndef TK_XCD_CHUNK_FINE
+#define TK_XCD_CHUNK_FINE 256u
+#endif
+#ifndef TK_XCD_CHUNK_COARSE
+#define TK_XCD_CHUNK_COARSE 64u
+#endif
+#ifndef TK_XCD_CHUNK_SCAN
+#define TK_XCD_CHUNK_SCAN 0u
+#endif
+
 struct tk_args_fill {
     uint32_t *dst;
     uint64_t count;
