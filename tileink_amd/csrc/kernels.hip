@@ -954,6 +954,81 @@ __device__ uint32_t emit_stack_begins_wave(const CoarseCtx &c, uint32_t batch, E
     return dst;
 }
 
+// One pass over a stack of at most 64 layers (lane k takes layer layer_start + k):
+// the validity and wrapper count of stack_wrapper_count (coarse/classify.hlsli:
+// 11-33) and the BEGIN records of emit_stack_begins (coarse/emit_stack.hlsli:18-31)
+// from the same loads.  Returns the slot after the BEGIN records; wrappers is
+// kNone (and nothing is written) when a layer rejects the tile.  emitted and
+// end_tag keep what emit_stack_ends needs, so the END records are written
+// without loading the layers again (stack_ends_from_registers).
+__device__ uint32_t stack_begins_single_pass(const CoarseCtx &c, uint32_t batch, Emitter &e, uint32_t dst,
+                                             uint32_t lane, uint32_t &wrappers, uint64_t &emitted,
+                                             uint32_t &end_tag) {
+    const tk_coarse_config &cfg = *c.cfg;
+    const tk_coarse_inputs &in = *c.in;
+    const tk_batch_record br = in.batches[batch];
+    const uint32_t end = min(br.layer_end, cfg.layer_count);
+    const uint32_t i = br.layer_start + lane;
+    bool invalid = false, counted = false, emit = false;
+    uint32_t tag = TK_PTCL_BEGIN_CLIP, winding = 0u, fill_rule = 0u, payload = 0u;
+    tk_segment_range range{0u, 0u};
+    end_tag = TK_PTCL_END_CLIP;
+    if (i < end) {
+        const tk_layer_record layer = in.layers[i];
+        const DrawData d = load_draw(in.draws, cfg.draw_count, layer.draw);
+        end_tag = layer.tag == 1u ? kPtclEndOpacity : (layer.tag == 2u ? kPtclEndBlend : TK_PTCL_END_CLIP);
+        if (draw_has_sdf(d)) {
+            const bool covers = sdf_clip_covers(in.paint, cfg.paint_words, d, c.tx, c.ty);
+            if (!covers) {
+                if (!draw_hits_tile(d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height)) invalid = true;
+                else counted = true;
+            }
+            emit = !(layer.tag == TK_LAYER_CLIP && covers);
+            tag = TK_PTCL_BEGIN_SDF_CLIP;
+            payload = layer.draw;
+        } else {
+            const uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
+            tag = layer.tag == 1u ? kPtclBeginOpacity : (layer.tag == 2u ? kPtclBeginBlend : TK_PTCL_BEGIN_CLIP);
+            fill_rule = d.fill_rule;
+            payload = layer.payload;
+            if (bd == kNone) {
+                invalid = true;
+            } else {
+                range = in.segment_ranges[bd];
+                winding = (uint32_t)in.backdrops[bd];
+                const bool empty = range.start == range.end;
+                const bool full = d.fill_rule == TK_FILL_EVEN_ODD ? (winding & 1u) != 0u : winding != 0u;
+                const bool noop = layer.tag == TK_LAYER_CLIP && empty && full;
+                if (!noop) {
+                    if (empty && winding == 0u) invalid = true;
+                    else counted = true;
+                }
+                emit = !noop;
+            }
+        }
+    }
+    emitted = 0ull;
+    if (__ballot(invalid)) {
+        wrappers = kNone;
+        return dst;
+    }
+    wrappers = (uint32_t)__popcll(__ballot(counted));
+    emitted = __ballot(emit);
+    if (emit) e.store(dst + (uint32_t)__popcll(emitted & lanes_below(lane)), tag, winding, fill_rule, range.start,
+                      range.end, payload);
+    return dst + (uint32_t)__popcll(emitted);
+}
+
+// emit_stack_ends for a stack handled by stack_begins_single_pass: the emitted
+// layers' END records in reverse layer order from dst; returns the next slot.
+__device__ __forceinline__ uint32_t stack_ends_from_registers(Emitter &e, uint32_t dst, uint32_t lane, uint64_t emitted,
+                                                              uint32_t end_tag) {
+    const uint32_t n = (uint32_t)__popcll(emitted);
+    if ((emitted >> lane) & 1ull)
+        e.store(dst + n - 1u - (uint32_t)__popcll(emitted & lanes_below(lane)), end_tag, 0u, 0u, 0u, 0u, 0u);
+    return dst + n;
+}
+
 // Writes the stack's END records (reverse layer order) from dst; returns the next slot.
 __device__ uint32_t emit_stack_ends_wave(const CoarseCtx &c, uint32_t batch, Emitter &e, uint32_t dst, uint32_t lane) {
     const tk_batch_record br = c.in->batches[batch];
@@ -1136,6 +1211,9 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
     bool active = false;
     uint32_t start = 0u, end = 0u, wrappers = 0u, dst = 0u, flags = 0u, last_opaque = 0u;
     bool begin_after = false;
+    bool single_pass = false;  // the batch's stack is at most 64 layers: ENDs come from registers
+    uint64_t stack_emitted = 0ull;
+    uint32_t stack_end_tag = TK_PTCL_END_CLIP;
     auto empty_until = [&](uint32_t stop) {
         for (uint32_t bb = next + lane; bb < stop; bb += 64u) kinds[(size_t)bb * T + tile] = TK_TILE_KIND_EMPTY;
     };
@@ -1143,7 +1221,8 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
         if (cur == kNone) return;
         uint32_t kind = TK_TILE_KIND_EMPTY;
         if (active) {
-            dst = emit_stack_ends_wave(c, cur, e, dst, lane);
+            dst = single_pass ? stack_ends_from_registers(e, dst, lane, stack_emitted, stack_end_tag)
+                              : emit_stack_ends_wave(c, cur, e, dst, lane);
             if (lane == 0u) e.store(dst, TK_PTCL_END, 0u, 0u, 0u, 0u, 0u);
             uint32_t skip = 0u;
             if ((flags & 8u) && wrappers == 0u && end - start > 2u && !begin_after) skip = last_opaque;
@@ -1164,7 +1243,14 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
         wrappers = 0u;
         dst = start;
         if (active) {
-            wrappers = stack_wrapper_count_wave(c, bb, lane);
+            const tk_batch_record br = a.in.batches[bb];
+            const uint32_t layer_end = min(br.layer_end, cfg.layer_count);
+            single_pass = layer_end <= br.layer_start || layer_end - br.layer_start <= 64u;
+            if (single_pass) {
+                dst = stack_begins_single_pass(c, bb, e, start, lane, wrappers, stack_emitted, stack_end_tag);
+            } else {
+                wrappers = stack_wrapper_count_wave(c, bb, lane);
+            }
             if (wrappers == kNone) {  // cannot happen with a consistent count; keep the stream terminated
                 if (lane == 0u) e.store(start, TK_PTCL_END, 0u, 0u, 0u, 0u, 0u);
                 active = false;
@@ -1172,7 +1258,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
                 return;
             }
             flags = wrappers != 0u ? kClassOther : 0u;
-            dst = emit_stack_begins_wave(c, bb, e, start, lane);
+            if (!single_pass) dst = emit_stack_begins_wave(c, bb, e, start, lane);
         }
     };
 
