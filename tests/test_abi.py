@@ -43,6 +43,38 @@ def test_record_layouts_match_reference_abi():
     assert C.sizeof(_abi.ScanConfig) == 44
 
 
+def header_struct_fields():
+    """Field names of every `typedef struct NAME {...} NAME;` in the header, in order."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for name, body in re.findall(r"typedef struct (tk_\w+) \{(.*?)\} \1;", text, flags=re.S):
+        fields = []
+        for decl in body.split(";"):
+            decl = re.sub(r"\[[^\]]*\]", "", decl).strip()
+            if not decl:
+                continue
+            head, *rest = decl.split(",")
+            fields.append(head.split()[-1].lstrip("*"))
+            fields += [r.strip().lstrip("*") for r in rest]
+        out[name] = fields
+    return out
+
+
+MIRRORS = {"tk_scan_config": "ScanConfig", "tk_cumsum_config": "CumsumConfig",
+           "tk_scene": "Scene", "tk_limits": "Limits", "tk_frame_stats": "FrameStats", "tk_stage_times": "StageTimes",
+           "tk_coarse_config": "CoarseConfig", "tk_coarse_inputs": "CoarseInputs",
+           "tk_coarse_outputs": "CoarseOutputs", "tk_fine_config": "FineConfig", "tk_fine_inputs": "FineInputs"}
+
+
+@pytest.mark.parametrize("struct", sorted(k for k, v in MIRRORS.items() if v))
+def test_python_mirrors_follow_header_field_order(struct):
+    """A ctypes mirror with fields in another order than the header passes wrong
+    pointers through the ABI; the names must match position by position."""
+    fields = header_struct_fields()[struct]
+    mirror = [n for n, *_ in getattr(_abi, MIRRORS[struct])._fields_]
+    assert mirror == fields, (struct, mirror, fields)
+
+
 def test_missing_device_is_an_error_not_a_fallback():
     with pytest.raises(tk.TileinkError) as err:
         tk.NativeContext(-1)

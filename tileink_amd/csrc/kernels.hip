@@ -318,32 +318,6 @@ struct CoarseCtx {
     uint32_t tx, ty;
 };
 
-// stack_wrapper_count (coarse/classify.hlsli:11-33).
-__device__ uint32_t stack_wrapper_count(const CoarseCtx &c, uint32_t batch) {
-    const tk_coarse_config &cfg = *c.cfg;
-    const tk_coarse_inputs &in = *c.in;
-    tk_batch_record br = in.batches[batch];
-    uint32_t count = 0u;
-    for (uint32_t i = br.layer_start; i < br.layer_end && i < cfg.layer_count; ++i) {
-        tk_layer_record layer = in.layers[i];
-        DrawData d = load_draw(in.draws, cfg.draw_count, layer.draw);
-        if (draw_has_sdf(d)) {
-            if (sdf_clip_covers(in.paint, cfg.paint_words, d, c.tx, c.ty)) continue;
-            if (!draw_hits_tile(d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height)) return kNone;
-        } else {
-            uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
-            if (bd == kNone) return kNone;
-            tk_segment_range range = in.segment_ranges[bd];
-            int32_t winding = in.backdrops[bd];
-            bool full = d.fill_rule == TK_FILL_EVEN_ODD ? (((uint32_t)winding & 1u) != 0u) : winding != 0;
-            if (layer.tag == TK_LAYER_CLIP && range.start == range.end && full) continue;
-            if (range.start == range.end && winding == 0) return kNone;
-        }
-        count++;
-    }
-    return count;
-}
-
 struct Particle {
     bool valid;
     uint32_t tag, winding, fill_rule, seg_start, seg_end, color;
@@ -435,44 +409,6 @@ struct Emitter {
         ptcl[dst] = r;
     }
 };
-
-__device__ uint32_t emit_stack_begins(const CoarseCtx &c, uint32_t batch, Emitter &e, uint32_t dst) {
-    tk_batch_record br = c.in->batches[batch];
-    for (uint32_t i = br.layer_start; i < br.layer_end && i < c.cfg->layer_count; ++i) {
-        tk_layer_record layer = c.in->layers[i];
-        DrawData d = load_draw(c.in->draws, c.cfg->draw_count, layer.draw);
-        if (noop_stack_clip(c, layer, d)) continue;
-        if (draw_has_sdf(d)) {
-            e.store(dst++, TK_PTCL_BEGIN_SDF_CLIP, 0u, 0u, 0u, 0u, layer.draw);
-        } else {
-            uint32_t bd = draw_backdrop(c.in->paths, c.cfg->path_count, d, c.tx, c.ty, c.cfg->tiles_width,
-                                        c.cfg->tiles_height);
-            uint32_t tag = layer.tag == 1u ? kPtclBeginOpacity : (layer.tag == 2u ? kPtclBeginBlend : TK_PTCL_BEGIN_CLIP);
-            tk_segment_range range{0u, 0u};
-            uint32_t winding = 0u;
-            if (bd != kNone) {
-                range = c.in->segment_ranges[bd];
-                winding = (uint32_t)c.in->backdrops[bd];
-            }
-            e.store(dst++, tag, winding, d.fill_rule, range.start, range.end, layer.payload);
-        }
-    }
-    return dst;
-}
-
-__device__ uint32_t emit_stack_ends(const CoarseCtx &c, uint32_t batch, Emitter &e, uint32_t dst) {
-    tk_batch_record br = c.in->batches[batch];
-    uint32_t end = min(br.layer_end, c.cfg->layer_count);
-    for (uint32_t i = end; i > br.layer_start;) {
-        --i;
-        tk_layer_record layer = c.in->layers[i];
-        DrawData d = load_draw(c.in->draws, c.cfg->draw_count, layer.draw);
-        if (noop_stack_clip(c, layer, d)) continue;
-        uint32_t tag = layer.tag == 1u ? kPtclEndOpacity : (layer.tag == 2u ? kPtclEndBlend : TK_PTCL_END_CLIP);
-        e.store(dst++, tag, 0u, 0u, 0u, 0u, 0u);
-    }
-    return dst;
-}
 
 __device__ __forceinline__ bool is_begin_tag(uint32_t tag) {
     return tag == TK_PTCL_BEGIN_CLIP || tag == TK_PTCL_BEGIN_SDF_CLIP || tag == kPtclBeginOpacity || tag == kPtclBeginBlend;
@@ -1068,12 +1004,81 @@ __device__ __forceinline__ void load_list_entry(const tk_args_coarse &a, const t
     }
 }
 
-// coarse_count_bins for every batch (coarse/count.hlsl:48-73).
+// Wrapper validity and count (stack_wrapper_count, coarse/classify.hlsli:11-33)
+// of up to 64 batches at once: lane j describes batch `batch` with `particles`
+// particles; the (batch, layer) pairs of the batches that have particles are laid
+// end to end and dealt to the lanes, so every layer's loads are in flight
+// together.  Returns, in lane j, the wrapper count of batch j or kNone.
+__device__ uint32_t stack_wrappers_of_batches(const CoarseCtx &c, uint32_t pending, uint32_t batch,
+                                              uint32_t particles, uint32_t lane, uint32_t *invalid_flags,
+                                              uint32_t *counted) {
+    const tk_coarse_config &cfg = *c.cfg;
+    const tk_coarse_inputs &in = *c.in;
+    uint32_t first_layer = 0u, depth = 0u;
+    if (lane < pending && particles > 0u) {
+        const tk_batch_record br = in.batches[batch];
+        const uint32_t end = min(br.layer_end, cfg.layer_count);
+        first_layer = br.layer_start;
+        depth = end > first_layer ? end - first_layer : 0u;
+    }
+    invalid_flags[lane] = 0u;
+    counted[lane] = 0u;
+    uint32_t total;
+    const uint32_t off = wave_exclusive_scan(depth, lane, total);
+    const uint32_t incl = off + depth;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t t0 = 0u; t0 < total; t0 += 64u) {
+        const uint32_t t = t0 + lane;
+        uint32_t j = 0u;  // owner: the number of batches whose layers all precede pair t
+#pragma unroll
+        for (uint32_t step = 32u; step >= 1u; step >>= 1) {
+            if ((uint32_t)__shfl((int)incl, (int)(j + step - 1u), 64) <= t) j += step;
+        }
+        j = min(j, 63u);
+        const uint32_t i = (uint32_t)__shfl((int)first_layer, (int)j, 64) + t - (uint32_t)__shfl((int)off, (int)j, 64);
+        if (t < total) {
+            const tk_layer_record layer = in.layers[i];
+            const DrawData d = load_draw(in.draws, cfg.draw_count, layer.draw);
+            bool invalid = false, count_it = false;
+            if (draw_has_sdf(d)) {
+                if (!sdf_clip_covers(in.paint, cfg.paint_words, d, c.tx, c.ty)) {
+                    if (!draw_hits_tile(d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height)) invalid = true;
+                    else count_it = true;
+                }
+            } else {
+                const uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
+                if (bd == kNone) {
+                    invalid = true;
+                } else {
+                    const tk_segment_range range = in.segment_ranges[bd];
+                    const int32_t winding = in.backdrops[bd];
+                    const bool full = d.fill_rule == TK_FILL_EVEN_ODD ? (((uint32_t)winding & 1u) != 0u) : winding != 0;
+                    if (!(layer.tag == TK_LAYER_CLIP && range.start == range.end && full)) {
+                        if (range.start == range.end && winding == 0) invalid = true;
+                        else count_it = true;
+                    }
+                }
+            }
+            if (invalid) atomicOr(&invalid_flags[j], 1u);
+            if (count_it) atomicAdd(&counted[j], 1u);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    return invalid_flags[lane] ? kNone : counted[lane];
+}
+
+// coarse_count_bins for every batch (coarse/count.hlsl:48-73): one wave per tile.
+// The tile's draw list is read 64 entries at a time and its particles counted per
+// batch; the batches met are kept one per lane and their stacks resolved together
+// (stack_wrappers_of_batches) once 64 are pending or the list ends.
 extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coarse a) {
+    __shared__ uint32_t invalid_flags[4][64], counted[4][64];
     const tk_coarse_config &cfg = a.cfg;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t tile =
-        xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + (threadIdx.x >> 6);
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t tile = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + wave;
     if (tile >= cfg.tile_count) return;
     const CoarseCtx c{&a.cfg, &a.in, tile % cfg.tiles_width, tile / cfg.tiles_width};
     const uint32_t T = cfg.tile_count, Q = cfg.batch_count;
@@ -1081,38 +1086,49 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coar
     const tk_tile_draw_record list = a.in.tile_draws[tile];
     const uint32_t len = list.start == kNone ? 0u : list.len;
 
-    uint32_t cur = kNone, particles = 0u, next = 0u;
-    auto zero_until = [&](uint32_t stop) {
-        for (uint32_t bb = next + lane; bb < stop; bb += 64u) counts[(size_t)bb * T + tile] = 0u;
+    uint32_t next = 0u;                     // first batch whose count is not written yet
+    uint32_t pending = 0u;                  // batches held, one per lane
+    uint32_t my_batch = 0u, my_particles = 0u;
+    auto resolve = [&]() {
+        if (pending == 0u) return;
+        const uint32_t w = stack_wrappers_of_batches(c, pending, my_batch, my_particles, lane, invalid_flags[wave],
+                                                     counted[wave]);
+        uint32_t prev = (uint32_t)__shfl_up((int)my_batch, 1u, 64);
+        if (lane == 0u) prev = next - 1u;  // wraps to ~0u when next == 0: zeroing then starts at batch 0
+        if (lane < pending) {
+            for (uint32_t bb = prev + 1u; bb < my_batch; ++bb) counts[(size_t)bb * T + tile] = 0u;
+            const uint32_t count = (my_particles > 0u && w != kNone) ? my_particles + 2u * w + 1u : 0u;
+            counts[(size_t)my_batch * T + tile] = count;
+        }
+        next = (uint32_t)__shfl((int)my_batch, (int)(pending - 1u), 64) + 1u;
+        pending = 0u;
     };
+    uint32_t cur = kNone, particles = 0u;
     auto flush = [&]() {
         if (cur == kNone) return;
-        uint32_t count = 0u;
-        if (particles > 0u) {
-            const uint32_t w = stack_wrapper_count_wave(c, cur, lane);
-            if (w != kNone) count = particles + 2u * w + 1u;
+        if (lane == pending) {
+            my_batch = cur;
+            my_particles = particles;
         }
-        if (lane == 0u) counts[(size_t)cur * T + tile] = count;
-        next = cur + 1u;
+        if (++pending == 64u) resolve();
     };
     for (uint32_t base = 0; base < len; base += 64u) {
         uint32_t d, b;
         load_list_entry(a, list, base, lane, d, b);
         const bool valid = b != kNone && draw_particle(c, b, d).valid;
-        uint64_t pending = __ballot(b != kNone);
+        uint64_t pending_runs = __ballot(b != kNone);
         const uint64_t valid_mask = __ballot(valid);
-        while (pending) {
-            const uint32_t first = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+        while (pending_runs) {
+            const uint32_t first = (uint32_t)__ffsll((unsigned long long)pending_runs) - 1u;
             const uint32_t bb = (uint32_t)__shfl((int)b, (int)first, 64);
-            const uint64_t run = __ballot(b == bb) & pending;
-            pending &= ~run;
+            const uint64_t run = __ballot(b == bb) & pending_runs;
+            pending_runs &= ~run;
             if (bb != cur) {
                 if (cur != kNone && bb < cur) {  // list not in batch order
                     if (lane == 0u) atomicOr(&a.out.status[3], 1u);
                     continue;
                 }
                 flush();
-                zero_until(bb);
                 cur = bb;
                 particles = 0u;
             }
@@ -1120,7 +1136,8 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coar
         }
     }
     flush();
-    zero_until(Q);
+    resolve();
+    for (uint32_t bb = next + lane; bb < Q; bb += 64u) counts[(size_t)bb * T + tile] = 0u;
 }
 
 // coarse_prefix_chunks + coarse_chunk_offsets + coarse_apply_chunk_offsets for

@@ -391,9 +391,25 @@ tk_status frame_render(tk_frame *f) {
     TK_TRY(record(f, 6));
     {
         tk_coarse_config cc = frame_coarse_config(f);
-        tk_coarse_inputs in{f->draws,  f->paths,    f->backdrops, f->seg_ranges, f->layers,
-                            f->batches, f->paint, f->batch_ids, f->tile_draws, f->tile_draw_indices};
-        tk_coarse_outputs out{f->tile_counts, f->tile_ranges, f->tile_kinds, f->batch_bases, f->ptcl, f->status};
+        // Fields are set by name: the structs are part of the public ABI and may grow.
+        tk_coarse_inputs in{};
+        in.draws = f->draws;
+        in.paths = f->paths;
+        in.backdrops = f->backdrops;
+        in.segment_ranges = f->seg_ranges;
+        in.layers = f->layers;
+        in.batches = f->batches;
+        in.paint = f->paint;
+        in.draw_batch_ids = f->batch_ids;
+        in.tile_draws = f->tile_draws;
+        in.tile_draw_indices = f->tile_draw_indices;
+        tk_coarse_outputs out{};
+        out.tile_ptcl_counts = f->tile_counts;
+        out.tile_ptcl_ranges = f->tile_ranges;
+        out.tile_kinds = f->tile_kinds;
+        out.batch_ptcl_bases = f->batch_bases;
+        out.ptcl = f->ptcl;
+        out.status = f->status;
         TK_TRY(run_coarse(ctx, cc, in, out, f->coarse_status, &f->tickets[1], f->epoch, f->timing ? &f->ev[7] : nullptr));
         if (cc.batch_count == 0 && f->timing) {
             TK_TRY(record(f, 7));
@@ -403,8 +419,15 @@ tk_status frame_render(tk_frame *f) {
     TK_TRY(record(f, 9));
     {
         tk_fine_config fc = frame_fine_config(f);
-        tk_fine_inputs in{f->draws,      f->paint,       f->segments, f->tile_ranges,
-                          f->tile_kinds, f->batch_bases, f->ptcl,     f->status + 2};
+        tk_fine_inputs in{};
+        in.draws = f->draws;
+        in.paint = f->paint;
+        in.segments = f->segments;
+        in.tile_ptcl_ranges = f->tile_ranges;
+        in.tile_kinds = f->tile_kinds;
+        in.batch_ptcl_bases = f->batch_bases;
+        in.ptcl = f->ptcl;
+        in.status = f->status + 2;
         (void)T;
         TK_TRY(run_fine(ctx, fc, in, f->image));
     }
