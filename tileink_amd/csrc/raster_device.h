@@ -393,7 +393,14 @@ TKD long long coverage_fixed(float t) {
     const long long q = (long long)(((unsigned long long)hi << 32) | lo);
     return t < 0.0f ? -q : q;
 }
-TKD float fixed_to_float(long long v) { return (float)((double)v * 2.3283064365386962890625e-10); }
+// Back to FP32: the sum floored to a multiple of 2^-24 (at most 6e-8 below it),
+// then one rounding.  hi and lo >> 8 convert exactly (|hi| < 2^24 for any
+// coverage sum), the scaling by 2^-24 is exact, and only the add rounds.
+TKD float fixed_to_float(long long v) {
+    const float hi = (float)(int32_t)(v >> 32);
+    const float lo = (float)((uint32_t)v >> 8) * 5.9604644775390625e-8f;
+    return hi + lo;
+}
 TKD long long backdrop_fixed(int32_t backdrop) { return (long long)backdrop * 4294967296LL; }
 
 // One pixel's coverage from segments [start, end), terms summed exactly.
