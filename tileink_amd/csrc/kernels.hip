@@ -1327,6 +1327,24 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
     empty_until(Q);
 }
 
+// The colour sample_draw_brush returns for a draw whose brush does not depend on
+// the pixel (solid, and the kinds sample_brush maps to the header colour or to
+// 0), so that a fill reads one word pair instead of the draw record and then
+// its brush header (shared/brush/sample.hlsli:15-42).
+extern "C" __global__ __launch_bounds__(256) void tk_k_draw_brush(tk_args_draw_brush a) {
+    const uint32_t d = blockIdx.x * 256u + threadIdx.x;
+    if (d >= a.draw_count) return;
+    const uint32_t offset = a.draws[d].brush_offset;
+    const uint32_t kind = blob_word(a.paint, a.paint_words, a.paint_brush_base, offset);
+    const uint32_t color = blob_word(a.paint, a.paint_words, a.paint_brush_base, offset + 4u);
+    uint32_t flag = 1u, value = color;
+    if (kind == TK_BRUSH_SOLID) value = color;
+    else if (kind == TK_BRUSH_LINEAR) flag = 0u, value = 0u;
+    else if (kind == 6u) value = 0u;  // BRUSH_PATTERN
+    a.table[2u * d] = flag;
+    a.table[2u * d + 1u] = value;
+}
+
 // fine_tile_main for every batch: one 256-lane workgroup per tile, one lane per
 // pixel.  Between batches the pixel goes through the RGBA8 store/load round trip
 // the reference performs on its target.
@@ -1443,6 +1461,12 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fine(tk_args_fine a) {
                 } else if (tag == TK_PTCL_FILL || tag == TK_PTCL_BEGIN_CLIP) {
                     const uint32_t s0 = rec.segment_start, s1 = min(rec.segment_end, cfg.segment_capacity);
                     const uint32_t n = s1 > s0 ? s1 - s0 : 0u;
+                    // Issued before the coverage work so that its latency overlaps it.
+                    uint32_t brush_flag = 0u, brush_color = 0u;
+                    if (tag == TK_PTCL_FILL && rec.color < cfg.draw_count) {
+                        brush_flag = a.draw_brush[2u * rec.color];
+                        brush_color = a.draw_brush[2u * rec.color + 1u];
+                    }
                     CoverageAcc &acc = cov_acc[acc_sel];
                     acc_sel ^= 1u;
                     accumulate_record(segs + s0, n, acc, lane);
@@ -1455,8 +1479,10 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fine(tk_args_fine a) {
                     } else {
                         uint32_t masked = combine_alpha(alpha, clip_mask);
                         if (masked != 0u) {
-                            FineDraw d = load_fine_draw(a.in.draws, cfg.draw_count, rec.color);
-                            uint32_t color = sample_draw_brush(cfg, paint, d, sx, sy);
+                            uint32_t color = brush_color;
+                            if (brush_flag == 0u)
+                                color = sample_draw_brush(cfg, paint, load_fine_draw(a.in.draws, cfg.draw_count, rec.color),
+                                                          sx, sy);
                             if (masked == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
                             else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, masked));
                         }

@@ -65,12 +65,14 @@ struct tk_context {
     hipStream_t stream = nullptr;
     hipFunction_t f_fill = nullptr, f_copy = nullptr, f_scan_clear = nullptr, f_scan_count = nullptr, f_scan_prefix = nullptr, f_scan_chunk_apply = nullptr,
                   f_scan_emit = nullptr, f_cumsum = nullptr, f_coarse_count = nullptr,
-                  f_coarse_scan = nullptr, f_coarse_emit = nullptr, f_fine = nullptr, f_fill_coverage = nullptr,
-                  f_pixel_math = nullptr;
+                  f_coarse_scan = nullptr, f_coarse_emit = nullptr, f_draw_brush = nullptr, f_fine = nullptr,
+                  f_fill_coverage = nullptr, f_pixel_math = nullptr;
     // Scratch for the stage entry points (frames own their own).
     uint32_t *tickets = nullptr;  // [0] scan prefix, [1] coarse scan
     unsigned long long *status = nullptr;
     size_t status_words = 0;
+    uint32_t *draw_brush = nullptr;  // fine's per-draw brush words (tk_fine)
+    size_t draw_brush_words = 0;
     uint32_t epoch = 0;
     uint32_t next_epoch() {
         epoch = (epoch + 1u) & 0x3FFFFFFFu;
@@ -115,6 +117,20 @@ tk_status ensure_status(tk_context *ctx, size_t words) {
     TK_HIP(hipMalloc(&ctx->status, n * sizeof(unsigned long long)));
     TK_HIP(hipMemsetAsync(ctx->status, 0, n * sizeof(unsigned long long), ctx->stream));
     ctx->status_words = n;
+    return TK_OK;
+}
+
+tk_status ensure_draw_brush(tk_context *ctx, size_t words) {
+    if (words <= ctx->draw_brush_words) return TK_OK;
+    if (ctx->draw_brush) {
+        TK_HIP(hipStreamSynchronize(ctx->stream));
+        TK_HIP(hipFree(ctx->draw_brush));
+        ctx->draw_brush = nullptr;
+        ctx->draw_brush_words = 0;
+    }
+    size_t n = std::max<size_t>(words, 1024);
+    TK_HIP(hipMalloc(&ctx->draw_brush, n * sizeof(uint32_t)));
+    ctx->draw_brush_words = n;
     return TK_OK;
 }
 
@@ -178,12 +194,19 @@ tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coar
     return TK_OK;
 }
 
-tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inputs &in, uint32_t *rgba) {
+// `draw_brush` holds 2 * cfg.draw_count words; it is rebuilt here from the draws
+// and paint before fine reads it.
+tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inputs &in, uint32_t *rgba,
+                   uint32_t *draw_brush) {
     if (cfg.clip_spill_depth > TK_FINE_MAX_CLIP_SPILL) {
         set_error("clip nesting deeper than %u is not supported", 4u + TK_FINE_MAX_CLIP_SPILL);
         return TK_ERR_INVALID;
     }
-    tk_args_fine a{cfg, in, rgba};
+    if (cfg.draw_count) {
+        tk_args_draw_brush b{in.draws, cfg.draw_count, in.paint, cfg.paint_words, cfg.paint_brush_base, draw_brush};
+        TK_TRY(launch(ctx, ctx->f_draw_brush, div_up(cfg.draw_count, 256), 1, 256, b));
+    }
+    tk_args_fine a{cfg, in, rgba, draw_brush};
     const uint32_t blocks = xcd_grid(cfg.tile_count, TK_XCD_CHUNK_FINE);
     uint32_t gx = std::min<uint32_t>(blocks, 65535u);
     uint32_t gy = div_up(blocks, gx);
@@ -226,6 +249,7 @@ struct tk_frame {
     tk_segment_range *tile_ranges = nullptr;
     tk_ptcl_record *ptcl = nullptr;
     uint64_t ptcl_capacity = 0;
+    uint32_t *draw_brush = nullptr;  // fine's per-draw brush words
     // output
     uint32_t *image = nullptr;
     // scratch
@@ -429,7 +453,7 @@ tk_status frame_render(tk_frame *f) {
         in.ptcl = f->ptcl;
         in.status = f->status + 2;
         (void)T;
-        TK_TRY(run_fine(ctx, fc, in, f->image));
+        TK_TRY(run_fine(ctx, fc, in, f->image, f->draw_brush));
     }
     TK_TRY(record(f, 10));
     f->have_times = f->timing;
@@ -483,6 +507,7 @@ tk_status tk_context_create(int device, tk_context **out) {
         {&ctx->f_coarse_count, "tk_k_coarse_count"},
         {&ctx->f_coarse_scan, "tk_k_coarse_scan"},
         {&ctx->f_coarse_emit, "tk_k_coarse_emit"},
+        {&ctx->f_draw_brush, "tk_k_draw_brush"},
         {&ctx->f_fine, "tk_k_fine"},
         {&ctx->f_fill_coverage, "tk_k_fill_coverage"},
         {&ctx->f_pixel_math, "tk_k_pixel_math"},
@@ -504,6 +529,7 @@ void tk_context_destroy(tk_context *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->status) (void)hipFree(ctx->status);
     if (ctx->tickets) (void)hipFree(ctx->tickets);
+    if (ctx->draw_brush) (void)hipFree(ctx->draw_brush);
     if (ctx->module) (void)hipModuleUnload(ctx->module);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -628,6 +654,7 @@ tk_status tk_frame_create(tk_context *ctx, const tk_scene *s, tk_frame **out) {
     ALLOC(batch_bases, uint64_t(s->batch_count) + 1);
     ALLOC(status, 4);
     ALLOC(ptcl, f->ptcl_capacity);
+    ALLOC(draw_brush, 2 * std::max<uint64_t>(s->draw_count, 1));
     ALLOC(image, uint64_t(s->width) * s->height);
     ALLOC(tickets, 4);
     {
@@ -870,7 +897,8 @@ tk_status tk_fine(tk_context *ctx, const tk_fine_config *cfg, const tk_fine_inpu
     if (!ctx || !cfg || !in || !rgba) return fail(TK_ERR_INVALID, "null argument");
     if (uint64_t(cfg->tiles_width) * cfg->tiles_height != cfg->tile_count)
         return fail(TK_ERR_INVALID, "tile_count must equal tiles_width * tiles_height");
-    return run_fine(ctx, *cfg, *in, rgba);
+    TK_TRY(ensure_draw_brush(ctx, 2 * size_t(cfg->draw_count)));
+    return run_fine(ctx, *cfg, *in, rgba, ctx->draw_brush);
 }
 
 tk_status tk_fill_coverage_words(tk_context *ctx, uint32_t count, const uint32_t *requests,

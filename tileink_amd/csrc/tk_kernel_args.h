@@ -90,10 +90,21 @@ struct tk_args_coarse {
     uint32_t scan_blocks;
 };
 
+// Per-draw brush words for fine: two words per draw, {1, colour} when the draw's
+// brush colour does not depend on the pixel, {0, 0} when it must be sampled.
+struct tk_args_draw_brush {
+    const tk_draw_record *draws;
+    uint32_t draw_count;
+    const uint32_t *paint;
+    uint32_t paint_words, paint_brush_base;
+    uint32_t *table;
+};
+
 struct tk_args_fine {
     tk_fine_config cfg;
     tk_fine_inputs in;
     uint32_t *rgba;
+    const uint32_t *draw_brush;  // tk_args_draw_brush::table, draw_count entries
 };
 
 struct tk_args_fill_coverage {
