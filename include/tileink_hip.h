@@ -88,6 +88,9 @@ uint32_t tk_version(void);
 
 #define TK_BRUSH_SOLID 1u
 #define TK_BRUSH_LINEAR 2u
+#define TK_BRUSH_RADIAL 3u
+#define TK_BRUSH_SWEEP 4u
+#define TK_BRUSH_FOUR_CORNER 5u
 #define TK_BRUSH_HEADER_WORDS 9u
 #define TK_BRUSH_PARAM_WORDS 12u
 #define TK_SDF_KIND_RECT 1u
@@ -205,12 +208,20 @@ typedef struct tk_canvas tk_canvas;
 
 typedef struct tk_gradient_stop { float offset; float rgba[4]; } tk_gradient_stop; /* straight sRGB */
 
+/* Brush (src/shared/brush.rs:33-80): solid colour, or a gradient over sorted stops
+ * (peniko::Gradient), or a four-corner gradient.  Fields a kind does not use are ignored. */
 typedef struct tk_brush {
-    uint32_t kind;                 /* TK_BRUSH_SOLID or TK_BRUSH_LINEAR */
+    uint32_t kind;                 /* TK_BRUSH_SOLID, _LINEAR, _RADIAL, _SWEEP or _FOUR_CORNER */
     float rgba[4];                 /* solid: straight-alpha sRGB in [0,1] */
-    float start[2], end[2];        /* linear: gradient line in user space */
+    float start[2], end[2];        /* linear: gradient line; radial: start and end centres;
+                                      sweep: start[] is the centre (user space) */
     uint32_t extend;               /* 0 pad, 1 repeat, 2 reflect */
     const tk_gradient_stop *stops; uint32_t stop_count;
+    float start_radius, end_radius;  /* radial */
+    float start_angle, end_angle;    /* sweep, radians (peniko SweepGradientPosition) */
+    float bounds[4];                 /* four-corner: x0, y0, x1, y1 (user space) */
+    float corners[4][4];             /* four-corner: top-left, top-right, bottom-right,
+                                        bottom-left, straight-alpha sRGB */
 } tk_brush;
 
 tk_status tk_canvas_create(uint32_t width, uint32_t height, double scale, tk_canvas **out);
@@ -463,6 +474,13 @@ tk_status tk_fill_coverage_words(tk_context *ctx, uint32_t count, const uint32_t
 /* requests: 8 words {a, b, src, dst, c_bits, t_bits, rule, 0}; 14 output words each. */
 tk_status tk_pixel_math_words(tk_context *ctx, uint32_t count, const uint32_t *requests,
                               uint32_t *out);
+/* brush_words (validation/brush.hlsl:14-18): sample_brush of the encoded brush at
+ * word data_base of the brush blob (paint + paint_brush_base) at point (x, y).
+ * requests: 4 words each {data_base, x_bits, y_bits, 0}; one premultiplied RGBA8
+ * word out per request. */
+tk_status tk_brush_words(tk_context *ctx, const uint32_t *paint, uint32_t paint_words,
+                         uint32_t paint_brush_base, uint32_t count, const uint32_t *requests,
+                         uint32_t *out);
 
 #ifdef __cplusplus
 }

@@ -192,3 +192,13 @@ def pixel_math_words(requests):
     out = np.zeros((requests.shape[0], 14), np.uint32)
     lib().or_pixel_math_words(C.c_uint32(requests.shape[0]), _p(requests), _p(out))
     return out
+
+
+def brush_words(paint, paint_brush_base, requests):
+    """sample_brush at each request {data_base, x_bits, y_bits, 0} (validation/brush.hlsl:14-18)."""
+    paint = np.ascontiguousarray(paint, dtype=np.uint32).reshape(-1)
+    requests = np.ascontiguousarray(requests, dtype=np.uint32).reshape(-1, 4)
+    out = np.zeros(requests.shape[0], np.uint32)
+    lib().or_brush_words(_p(paint), C.c_uint32(paint.size), C.c_uint32(paint_brush_base),
+                         C.c_uint32(requests.shape[0]), _p(requests), _p(out))
+    return out

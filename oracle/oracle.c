@@ -721,8 +721,67 @@ static uint32_t sample_brush(const uint32_t *paint, uint32_t n, uint32_t bb, uin
         if (den > 0.00000011920929f) t = fmaf(tx - p[0], dx, (ty - p[1]) * dy) / den;
         return sample_ramp(paint, n, bb, payload, len, t, extend);
     }
+    if (kind == 3u) { /* sample_radial (shared/brush/radial.hlsli:5-33) */
+        float p[12];
+        for (uint32_t i = 0; i < 12; ++i) p[i] = blob_float(paint, n, bb, base + i);
+        float tx = fmaf(p[6], x, fmaf(p[8], y, p[10])), ty = fmaf(p[7], x, fmaf(p[9], y, p[11]));
+        float r0 = p[4], qx = tx - p[0], qy = ty - p[1], dcx = p[2] - p[0], dcy = p[3] - p[1], dr = p[5] - r0;
+        float a = fmaf(dcx, dcx, fmaf(dcy, dcy, -(dr * dr)));
+        float b = -2.0f * fmaf(qx, dcx, fmaf(qy, dcy, r0 * dr));
+        float c = fmaf(qx, qx, fmaf(qy, qy, -(r0 * r0)));
+        int has_t = 0;
+        float t = 0.0f;
+        if (fabsf(a) <= 0.000001f) {
+            if (fabsf(b) > 0.000001f) {
+                float candidate = -c / b;
+                if (fmaf(candidate, dr, r0) >= 0.0f) { has_t = 1; t = candidate; }
+            }
+        } else {
+            float disc = fmaf(b, b, (-4.0f * a) * c);
+            if (disc >= 0.0f) {
+                float root = sqrtf(disc);
+                float t0 = (-b - root) / (2.0f * a), t1 = (-b + root) / (2.0f * a);
+                int v0 = fmaf(t0, dr, r0) >= 0.0f, v1 = fmaf(t1, dr, r0) >= 0.0f;
+                if (v0) { has_t = 1; t = v1 ? fmaxf(t0, t1) : t0; }
+                else if (v1) { has_t = 1; t = t1; }
+            }
+        }
+        return has_t ? sample_ramp(paint, n, bb, payload, len, t, extend) : 0u;
+    }
+    if (kind == 4u) { /* sample_sweep (shared/brush/sweep.hlsli:5-20) */
+        float cx = blob_float(paint, n, bb, base), cy = blob_float(paint, n, bb, base + 1u);
+        float start = blob_float(paint, n, bb, base + 2u), span = blob_float(paint, n, bb, base + 3u), t = 0.0f;
+        if (fabsf(span) > 0.00000011920929f) {
+            float angle = 0.0f;
+            if (x != cx || y != cy) angle = atan2f(y - cy, x - cx);
+            float distance = angle - start;
+            if (span > 0.0f) { if (distance < 0.0f) distance += 6.2831855f; }
+            else if (distance > 0.0f) distance -= 6.2831855f;
+            t = distance / span;
+        }
+        return sample_ramp(paint, n, bb, payload, len, t, extend);
+    }
+    if (kind == 5u) { /* sample_four_corner (shared/brush/four_corner.hlsli:5-15) */
+        float x0 = blob_float(paint, n, bb, base), y0 = blob_float(paint, n, bb, base + 1u);
+        float x1 = blob_float(paint, n, bb, base + 2u), y1 = blob_float(paint, n, bb, base + 3u);
+        float w = x1 - x0, h = y1 - y0, u = 0.0f, v = 0.0f;
+        if (fabsf(w) > 0.00000011920929f) u = clampf((x - x0) / w, 0.0f, 1.0f);
+        if (fabsf(h) > 0.00000011920929f) v = clampf((y - y0) / h, 0.0f, 1.0f);
+        uint32_t tl = blob_word(paint, n, bb, payload), tr = blob_word(paint, n, bb, payload + 1u);
+        uint32_t br = blob_word(paint, n, bb, payload + 2u), bl = blob_word(paint, n, bb, payload + 3u);
+        return lerp_premul_u8(lerp_premul_u8(tl, tr, u), lerp_premul_u8(bl, br, u), v);
+    }
     if (kind == 6u) return 0u;
     return color;
+}
+
+/* brush_words (validation/brush.hlsl:14-18). */
+void or_brush_words(const uint32_t *paint, uint32_t paint_words, uint32_t paint_brush_base, uint32_t count,
+                    const uint32_t *requests, uint32_t *out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t *q = requests + (size_t)i * 4u;
+        out[i] = sample_brush(paint, paint_words, paint_brush_base, q[0], bits_f(q[1]), bits_f(q[2]));
+    }
 }
 
 /* ---------- coarse (src/shaders/hlsl/coarse/, shared/draw.hlsli) ------------------ */

@@ -115,6 +115,10 @@ void or_fill_coverage_words(uint32_t count, const uint32_t *requests, const uint
 void or_fill_coverage_words_sequential(uint32_t count, const uint32_t *requests, const uint32_t *segments,
                                        uint32_t *alpha_out, float *coverage_out);
 void or_pixel_math_words(uint32_t count, const uint32_t *requests, uint32_t *out);
+/* brush_words: sample_brush of the brush at word q[0] of the brush blob
+ * (paint + paint_brush_base) at (bits q[1], bits q[2]); 4 request words each. */
+void or_brush_words(const uint32_t *paint, uint32_t paint_words, uint32_t paint_brush_base, uint32_t count,
+                    const uint32_t *requests, uint32_t *out);
 
 /* FP64 area oracles.
  * or_fill_coverage_f64: the row-sweep quantity of fill_coverage_words in exact

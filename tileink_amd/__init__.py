@@ -16,11 +16,12 @@ import enum
 import numpy as np
 
 from . import _abi
-from ._abi import (BRUSH_LINEAR, BRUSH_SOLID, FILL_EVEN_ODD, FILL_NONZERO, INVALID, LINE_DTYPE, PTCL_DTYPE,
+from ._abi import (BRUSH_FOUR_CORNER, BRUSH_LINEAR, BRUSH_RADIAL, BRUSH_SOLID, BRUSH_SWEEP, FILL_EVEN_ODD, FILL_NONZERO, INVALID, LINE_DTYPE, PTCL_DTYPE,
                    SEGMENT_DTYPE, TileinkError, check, lib, ptr)
 
 __all__ = [
-    "BezPath", "Canvas", "Color", "LinearGradient", "NativeBackend", "NativeContext", "NativeRenderer", "Frame",
+    "BezPath", "Canvas", "Color", "LinearGradient", "RadialGradient", "SweepGradient", "FourCornerGradient",
+    "NativeBackend", "NativeContext", "NativeRenderer", "Frame",
     "SceneArrays", "TileinkError", "FILL_NONZERO", "FILL_EVEN_ODD",
 ]
 
@@ -89,25 +90,80 @@ class Color:
         return b, None
 
 
+def _gradient_brush(kind, stops, extend):
+    b = _abi.Brush()
+    b.kind = kind
+    b.extend = int(extend)
+    arr = (_abi.GradientStop * max(len(stops), 1))()
+    for i, (offset, color) in enumerate(stops):
+        arr[i].offset = float(offset)
+        arr[i].rgba[:] = color.rgba
+    b.stops = C.cast(arr, C.POINTER(_abi.GradientStop))
+    b.stop_count = len(stops)
+    return b, arr
+
+
 class LinearGradient:
-    """Linear gradient brush: stops are (offset, Color); extend 0 pad, 1 repeat, 2 reflect."""
+    """Linear gradient brush (peniko Gradient::new_linear): stops are (offset, Color);
+    extend 0 pad, 1 repeat, 2 reflect."""
 
     def __init__(self, start, end, stops, extend=0):
         self.start, self.end, self.stops, self.extend = start, end, list(stops), int(extend)
 
     def _brush(self):
-        b = _abi.Brush()
-        b.kind = BRUSH_LINEAR
+        b, arr = _gradient_brush(BRUSH_LINEAR, self.stops, self.extend)
         b.start[:] = self.start
         b.end[:] = self.end
-        b.extend = self.extend
-        arr = (_abi.GradientStop * max(len(self.stops), 1))()
-        for i, (offset, color) in enumerate(self.stops):
-            arr[i].offset = float(offset)
-            arr[i].rgba[:] = color.rgba
-        b.stops = C.cast(arr, C.POINTER(_abi.GradientStop))
-        b.stop_count = len(self.stops)
         return b, arr
+
+
+class RadialGradient:
+    """Two-point conical gradient (peniko Gradient::new_two_point_radial): circles
+    (start_center, start_radius) to (end_center, end_radius)."""
+
+    def __init__(self, start_center, start_radius, end_center, end_radius, stops, extend=0):
+        self.start_center, self.end_center = start_center, end_center
+        self.start_radius, self.end_radius = float(start_radius), float(end_radius)
+        self.stops, self.extend = list(stops), int(extend)
+
+    def _brush(self):
+        b, arr = _gradient_brush(BRUSH_RADIAL, self.stops, self.extend)
+        b.start[:] = self.start_center
+        b.end[:] = self.end_center
+        b.start_radius, b.end_radius = self.start_radius, self.end_radius
+        return b, arr
+
+
+class SweepGradient:
+    """Sweep (conic) gradient around `center` from start_angle to end_angle, in
+    radians (peniko Gradient::new_sweep)."""
+
+    def __init__(self, center, start_angle, end_angle, stops, extend=0):
+        self.center, self.start_angle, self.end_angle = center, float(start_angle), float(end_angle)
+        self.stops, self.extend = list(stops), int(extend)
+
+    def _brush(self):
+        b, arr = _gradient_brush(BRUSH_SWEEP, self.stops, self.extend)
+        b.start[:] = self.center
+        b.start_angle, b.end_angle = self.start_angle, self.end_angle
+        return b, arr
+
+
+class FourCornerGradient:
+    """Bilinear blend of four corner colours over `bounds` (x0, y0, x1, y1)
+    (Brush::four_corner); colours in the order top-left, top-right, bottom-right,
+    bottom-left."""
+
+    def __init__(self, bounds, colors):
+        self.bounds, self.colors = tuple(float(v) for v in bounds), list(colors)
+
+    def _brush(self):
+        b = _abi.Brush()
+        b.kind = BRUSH_FOUR_CORNER
+        b.bounds[:] = self.bounds
+        for i, color in enumerate(self.colors):
+            b.corners[i][:] = color.rgba
+        return b, None
 
 
 def _brush_arg(brush):

@@ -18,7 +18,7 @@ INVALID = 0xFFFFFFFF
 
 PATH_MOVE, PATH_LINE, PATH_QUAD, PATH_CUBIC, PATH_CLOSE = range(5)
 FILL_NONZERO, FILL_EVEN_ODD = 0, 1
-BRUSH_SOLID, BRUSH_LINEAR = 1, 2
+BRUSH_SOLID, BRUSH_LINEAR, BRUSH_RADIAL, BRUSH_SWEEP, BRUSH_FOUR_CORNER = 1, 2, 3, 4, 5
 
 # Frame buffer identifiers (tk_frame_buffer).
 BUF_BACKDROPS, BUF_SEGMENT_COUNTS, BUF_SEGMENT_RANGES, BUF_SEGMENT_CURSORS, BUF_SEGMENT_BUMPS, BUF_SEGMENTS = range(6)
@@ -98,7 +98,9 @@ class GradientStop(C.Structure):
 
 class Brush(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("rgba", C.c_float * 4), ("start", C.c_float * 2), ("end", C.c_float * 2),
-                ("extend", C.c_uint32), ("stops", C.POINTER(GradientStop)), ("stop_count", C.c_uint32)]
+                ("extend", C.c_uint32), ("stops", C.POINTER(GradientStop)), ("stop_count", C.c_uint32),
+                ("start_radius", C.c_float), ("end_radius", C.c_float), ("start_angle", C.c_float),
+                ("end_angle", C.c_float), ("bounds", C.c_float * 4), ("corners", (C.c_float * 4) * 4)]
 
 
 class Limits(C.Structure):
@@ -203,6 +205,7 @@ SIGNATURES = {
     "tk_fine": (C.c_int, [_P, C.POINTER(FineConfig), C.POINTER(FineInputs), _P]),
     "tk_fill_coverage_words": (C.c_int, [_P, _U32, _P, _P, _P, _P]),
     "tk_pixel_math_words": (C.c_int, [_P, _U32, _P, _P]),
+    "tk_brush_words": (C.c_int, [_P, _P, _U32, _U32, _U32, _P, _P]),
 }
 
 _lib = None

@@ -727,39 +727,94 @@ std::vector<uint32_t> build_ramp(std::vector<tk_gradient_stop> stops, size_t ram
     return ramp;
 }
 
-// Encodes a brush into its 9 header + 12 parameter words (+ ramp).
+// Ramp sizes (src/shared/brush.rs:503-535).
+size_t radial_ramp_size(const tk_brush *b, size_t stop_count) {
+    float center_span = std::hypot(b->end[0] - b->start[0], b->end[1] - b->start[1]);
+    float radius_span = std::fabs(b->start_radius) + std::fabs(b->end_radius);
+    float span = std::fmax(center_span + radius_span, std::fabs(b->end_radius - b->start_radius)) * 1.0f;
+    return quantize_ramp_size(span, stop_count);
+}
+size_t sweep_ramp_size(float start_angle, float end_angle, size_t stop_count) {
+    float turns = std::fmax(std::fabs(end_angle - start_angle) / 6.2831855f, 0.25f);
+    return quantize_ramp_size(turns * 256.0f, stop_count);
+}
+
+// f64::rem_euclid.
+double rem_euclid_f64(double a, double b) {
+    double r = std::fmod(a, b);
+    return r < 0.0 ? r + std::fabs(b) : r;
+}
+
+// Encodes a brush into its 9 header + 12 parameter words and its payload
+// (push_encoded_brush, src/shared/brush.rs:249-345, after Surface::physical_brush,
+// src/canvas/surface.rs:395-420, has applied the canvas scale).
 bool encode_brush(const tk_brush *brush, double scale, std::vector<uint32_t> &blob, uint32_t *offset,
                   uint32_t *len) {
     uint32_t data[TK_BRUSH_HEADER_WORDS] = {TK_BRUSH_SOLID, 0u, 0u, 0u, 0u, 0u, 0u, 255u, 0u};
     uint32_t params[TK_BRUSH_PARAM_WORDS] = {};
-    *offset = uint32_t(blob.size());
-    if (brush == nullptr || brush->kind == TK_BRUSH_SOLID) {
+    std::vector<uint32_t> payload;
+    const uint32_t kind = brush ? brush->kind : TK_BRUSH_SOLID;
+    const float scale_f = float(scale), inv = float(1.0 / scale);
+    // scale_brush_transform(identity, 1 / scale) (src/canvas.rs:203-212).
+    const float xf[6] = {inv, 0.0f, 0.0f, inv, 0.0f, 0.0f};
+    auto ramp_of = [&](size_t size) {
+        std::vector<tk_gradient_stop> stops(brush->stops, brush->stops + brush->stop_count);
+        return build_ramp(stops, size);
+    };
+    if (kind == TK_BRUSH_SOLID) {
         float transparent[4] = {0, 0, 0, 0};
         data[4] = premultiply_pack(brush ? brush->rgba : transparent);
-        blob.insert(blob.end(), data, data + TK_BRUSH_HEADER_WORDS);
-        blob.insert(blob.end(), params, params + TK_BRUSH_PARAM_WORDS);
-    } else if (brush->kind == TK_BRUSH_LINEAR) {
+    } else if (kind == TK_BRUSH_LINEAR) {
         if (brush->extend > 2u) return false;
-        std::vector<tk_gradient_stop> stops(brush->stops, brush->stops + brush->stop_count);
         float span = std::hypot(brush->end[0] - brush->start[0], brush->end[1] - brush->start[1]);
-        std::vector<uint32_t> ramp = build_ramp(stops, quantize_ramp_size(span, stops.size()));
-        data[0] = TK_BRUSH_LINEAR;
+        payload = ramp_of(quantize_ramp_size(span, brush->stop_count));
         data[1] = brush->extend;
-        data[2] = TK_BRUSH_HEADER_WORDS + TK_BRUSH_PARAM_WORDS;
-        data[3] = uint32_t(ramp.size());
-        float inv = float(1.0 / scale);
-        float xf[6] = {inv, 0.0f, 0.0f, inv, 0.0f, 0.0f};  // scale_brush_transform(identity, 1/scale)
         params[0] = f2bits(brush->start[0]);
         params[1] = f2bits(brush->start[1]);
         params[2] = f2bits(brush->end[0]);
         params[3] = f2bits(brush->end[1]);
         for (int i = 0; i < 6; ++i) params[4 + i] = f2bits(xf[i]);
-        blob.insert(blob.end(), data, data + TK_BRUSH_HEADER_WORDS);
-        blob.insert(blob.end(), params, params + TK_BRUSH_PARAM_WORDS);
-        blob.insert(blob.end(), ramp.begin(), ramp.end());
+    } else if (kind == TK_BRUSH_RADIAL) {
+        if (brush->extend > 2u) return false;
+        payload = ramp_of(radial_ramp_size(brush, brush->stop_count));
+        data[1] = brush->extend;
+        params[0] = f2bits(brush->start[0]);
+        params[1] = f2bits(brush->start[1]);
+        params[2] = f2bits(brush->end[0]);
+        params[3] = f2bits(brush->end[1]);
+        params[4] = f2bits(brush->start_radius);
+        params[5] = f2bits(brush->end_radius);
+        for (int i = 0; i < 6; ++i) params[6 + i] = f2bits(xf[i]);
+    } else if (kind == TK_BRUSH_SWEEP) {
+        if (brush->extend > 2u) return false;
+        const double start = double(brush->start_angle), end = double(brush->end_angle);
+        if (!std::isfinite(start) || !std::isfinite(end)) return false;  // the reference panics
+        // One reduction in f64 to [-pi, pi]; the signed span is kept separately
+        // (Brush::from_gradient_with_ramp_size, src/shared/brush.rs:200-226).
+        const double tau = 6.283185307179586, pi = 3.141592653589793;
+        double orientation = rem_euclid_f64(start, tau);
+        if (orientation > pi) orientation -= tau;
+        payload = ramp_of(sweep_ramp_size(brush->start_angle, brush->end_angle, brush->stop_count));
+        data[1] = brush->extend;
+        params[0] = f2bits(brush->start[0] * scale_f);
+        params[1] = f2bits(brush->start[1] * scale_f);
+        params[2] = f2bits(float(orientation));
+        params[3] = f2bits(float(end - start));
+    } else if (kind == TK_BRUSH_FOUR_CORNER) {
+        for (int i = 0; i < 4; ++i) params[i] = f2bits(brush->bounds[i] * scale_f);
+        for (int i = 0; i < 4; ++i) payload.push_back(premultiply_pack(brush->corners[i]));
     } else {
         return false;
     }
+    if (kind != TK_BRUSH_SOLID) {
+        data[0] = kind;
+        data[2] = TK_BRUSH_HEADER_WORDS + TK_BRUSH_PARAM_WORDS;  // set_local_payload
+        data[3] = uint32_t(payload.size());
+    }
+    *offset = uint32_t(blob.size());
+    blob.insert(blob.end(), data, data + TK_BRUSH_HEADER_WORDS);
+    blob.insert(blob.end(), params, params + TK_BRUSH_PARAM_WORDS);
+    blob.insert(blob.end(), payload.begin(), payload.end());
     *len = uint32_t(blob.size()) - *offset;
     return true;
 }

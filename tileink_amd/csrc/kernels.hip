@@ -1338,9 +1338,10 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_draw_brush(tk_args_draw_b
     const uint32_t kind = blob_word(a.paint, a.paint_words, a.paint_brush_base, offset);
     const uint32_t color = blob_word(a.paint, a.paint_words, a.paint_brush_base, offset + 4u);
     uint32_t flag = 1u, value = color;
-    if (kind == TK_BRUSH_SOLID) value = color;
-    else if (kind == TK_BRUSH_LINEAR) flag = 0u, value = 0u;
-    else if (kind == 6u) value = 0u;  // BRUSH_PATTERN
+    if (kind == TK_BRUSH_LINEAR || kind == TK_BRUSH_RADIAL || kind == TK_BRUSH_SWEEP || kind == TK_BRUSH_FOUR_CORNER)
+        flag = 0u, value = 0u;
+    else if (kind == 6u)  // BRUSH_PATTERN
+        value = 0u;
     a.table[2u * d] = flag;
     a.table[2u * d + 1u] = value;
 }
@@ -1560,4 +1561,12 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_pixel_math(tk_args_pixel_
     o[11] = pack_premul_rgba8(pixel.r, pixel.g, pixel.b, pixel.a);
     o[12] = coverage_to_u8(rem_euclid_f32(c, 3.0f));
     o[13] = asuint(signum_f32(c));
+}
+
+// brush_words (validation/brush.hlsl:14-18).
+extern "C" __global__ __launch_bounds__(256) void tk_k_brush_words(tk_args_brush_words a) {
+    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= a.count) return;
+    const uint32_t *q = a.requests + (size_t)id * 4u;
+    a.out[id] = sample_brush(a.paint, a.paint_words, a.paint_brush_base, q[0], asfloat(q[1]), asfloat(q[2]));
 }

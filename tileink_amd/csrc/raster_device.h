@@ -567,7 +567,88 @@ TKD uint32_t sample_linear(const uint32_t *paint, uint32_t count, uint32_t brush
     if (denominator > kGradientEpsilon) t = fmaf(tx - p[0], dx, (ty - p[1]) * dy) / denominator;
     return sample_ramp(paint, count, brush_base, payload, len, t, extend);
 }
-// sample_brush (shared/brush/sample.hlsli:15-42) for solid and linear brushes.
+// sample_radial (shared/brush/radial.hlsli:5-33): the two-point conical gradient,
+// t from the quadratic |q - t dc| = r0 + t dr with r0 + t dr >= 0.
+constexpr float kRadialQuadraticEpsilon = 0.000001f;  // shared/brush/radial.hlsli:4
+TKD uint32_t sample_radial(const uint32_t *paint, uint32_t count, uint32_t brush_base, float x, float y, uint32_t base,
+                           uint32_t extend, uint32_t payload, uint32_t len) {
+    float p[12];
+    for (int i = 0; i < 12; ++i) p[i] = blob_float(paint, count, brush_base, base + (uint32_t)i);
+    const float tx = fmaf(p[6], x, fmaf(p[8], y, p[10]));
+    const float ty = fmaf(p[7], x, fmaf(p[9], y, p[11]));
+    const float r0 = p[4], r1 = p[5];
+    const float qx = tx - p[0], qy = ty - p[1], dcx = p[2] - p[0], dcy = p[3] - p[1], dr = r1 - r0;
+    const float qa = fmaf(dcx, dcx, fmaf(dcy, dcy, -(dr * dr)));
+    const float qb = -2.0f * fmaf(qx, dcx, fmaf(qy, dcy, r0 * dr));
+    const float qc = fmaf(qx, qx, fmaf(qy, qy, -(r0 * r0)));
+    bool has_t = false;
+    float t = 0.0f;
+    if (fabsf(qa) <= kRadialQuadraticEpsilon) {
+        if (fabsf(qb) > kRadialQuadraticEpsilon) {
+            const float candidate = -qc / qb;
+            if (fmaf(candidate, dr, r0) >= 0.0f) {
+                has_t = true;
+                t = candidate;
+            }
+        }
+    } else {
+        const float discriminant = fmaf(qb, qb, (-4.0f * qa) * qc);
+        if (discriminant >= 0.0f) {
+            const float root = sqrtf(discriminant);
+            const float t0 = (-qb - root) / (2.0f * qa), t1 = (-qb + root) / (2.0f * qa);
+            const bool valid0 = fmaf(t0, dr, r0) >= 0.0f, valid1 = fmaf(t1, dr, r0) >= 0.0f;
+            if (valid0) {
+                has_t = true;
+                t = valid1 ? fmaxf(t0, t1) : t0;
+            } else if (valid1) {
+                has_t = true;
+                t = t1;
+            }
+        }
+    }
+    return has_t ? sample_ramp(paint, count, brush_base, payload, len, t, extend) : 0u;
+}
+
+// sample_sweep (shared/brush/sweep.hlsli:5-20): the angle from the centre,
+// measured from the start orientation in the direction of the signed span.
+constexpr float kGradientTau = 6.2831855f;  // shared/brush/sweep.hlsli:4
+TKD uint32_t sample_sweep(const uint32_t *paint, uint32_t count, uint32_t brush_base, float x, float y, uint32_t base,
+                          uint32_t extend, uint32_t payload, uint32_t len) {
+    const float cx = blob_float(paint, count, brush_base, base), cy = blob_float(paint, count, brush_base, base + 1u);
+    const float start_angle = blob_float(paint, count, brush_base, base + 2u);
+    const float span = blob_float(paint, count, brush_base, base + 3u);
+    float t = 0.0f;
+    if (fabsf(span) > kGradientEpsilon) {
+        float angle = 0.0f;
+        if (x != cx || y != cy) angle = atan2f(y - cy, x - cx);
+        float distance = angle - start_angle;
+        if (span > 0.0f) {
+            if (distance < 0.0f) distance += kGradientTau;
+        } else if (distance > 0.0f) {
+            distance -= kGradientTau;
+        }
+        t = distance / span;
+    }
+    return sample_ramp(paint, count, brush_base, payload, len, t, extend);
+}
+
+// sample_four_corner (shared/brush/four_corner.hlsli:5-15): bilinear blend of the
+// corner colours tl, tr, br, bl over the bounds.
+TKD uint32_t sample_four_corner(const uint32_t *paint, uint32_t count, uint32_t brush_base, float x, float y,
+                                uint32_t base, uint32_t payload) {
+    const float x0 = blob_float(paint, count, brush_base, base), y0 = blob_float(paint, count, brush_base, base + 1u);
+    const float x1 = blob_float(paint, count, brush_base, base + 2u), y1 = blob_float(paint, count, brush_base, base + 3u);
+    const float width = x1 - x0, height = y1 - y0;
+    float u = 0.0f, v = 0.0f;
+    if (fabsf(width) > kGradientEpsilon) u = clampf((x - x0) / width, 0.0f, 1.0f);
+    if (fabsf(height) > kGradientEpsilon) v = clampf((y - y0) / height, 0.0f, 1.0f);
+    const uint32_t tl = blob_word(paint, count, brush_base, payload), tr = blob_word(paint, count, brush_base, payload + 1u);
+    const uint32_t br = blob_word(paint, count, brush_base, payload + 2u), bl = blob_word(paint, count, brush_base, payload + 3u);
+    return lerp_premul_u8(lerp_premul_u8(tl, tr, u), lerp_premul_u8(bl, br, u), v);
+}
+
+// sample_brush (shared/brush/sample.hlsli:15-42) for the solid, linear, radial,
+// sweep and four-corner brushes.
 TKD uint32_t sample_brush(const uint32_t *paint, uint32_t count, uint32_t brush_base, uint32_t data_base, float x,
                           float y) {
     uint32_t kind = blob_word(paint, count, brush_base, data_base);
@@ -577,8 +658,11 @@ TKD uint32_t sample_brush(const uint32_t *paint, uint32_t count, uint32_t brush_
     uint32_t base = data_base + TK_BRUSH_HEADER_WORDS;
     uint32_t color = blob_word(paint, count, brush_base, data_base + 4u);
     if (kind == TK_BRUSH_LINEAR) return sample_linear(paint, count, brush_base, x, y, base, extend, payload, len);
+    if (kind == TK_BRUSH_RADIAL) return sample_radial(paint, count, brush_base, x, y, base, extend, payload, len);
+    if (kind == TK_BRUSH_SWEEP) return sample_sweep(paint, count, brush_base, x, y, base, extend, payload, len);
+    if (kind == TK_BRUSH_FOUR_CORNER) return sample_four_corner(paint, count, brush_base, x, y, base, payload);
     if (kind == 6u) return 0u;  // BRUSH_PATTERN
-    // Radial, sweep, mesh and image brushes are not produced on this path.
+    // Mesh, program and image brushes are not produced on this path.
     return color;
 }
 

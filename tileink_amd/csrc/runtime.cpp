@@ -66,7 +66,7 @@ struct tk_context {
     hipFunction_t f_fill = nullptr, f_copy = nullptr, f_scan_clear = nullptr, f_scan_count = nullptr, f_scan_prefix = nullptr, f_scan_chunk_apply = nullptr,
                   f_scan_emit = nullptr, f_cumsum = nullptr, f_coarse_count = nullptr,
                   f_coarse_scan = nullptr, f_coarse_emit = nullptr, f_draw_brush = nullptr, f_fine = nullptr,
-                  f_fill_coverage = nullptr, f_pixel_math = nullptr;
+                  f_fill_coverage = nullptr, f_pixel_math = nullptr, f_brush_words = nullptr;
     // Scratch for the stage entry points (frames own their own).
     uint32_t *tickets = nullptr;  // [0] scan prefix, [1] coarse scan
     unsigned long long *status = nullptr;
@@ -511,6 +511,7 @@ tk_status tk_context_create(int device, tk_context **out) {
         {&ctx->f_fine, "tk_k_fine"},
         {&ctx->f_fill_coverage, "tk_k_fill_coverage"},
         {&ctx->f_pixel_math, "tk_k_pixel_math"},
+        {&ctx->f_brush_words, "tk_k_brush_words"},
     };
     for (auto &entry : table) {
         if (hipModuleGetFunction(entry.fn, ctx->module, entry.name) != hipSuccess) {
@@ -912,6 +913,14 @@ tk_status tk_pixel_math_words(tk_context *ctx, uint32_t count, const uint32_t *r
     if (!ctx || (count && (!requests || !out))) return fail(TK_ERR_INVALID, "null argument");
     tk_args_pixel_math a{count, requests, out};
     return launch(ctx, ctx->f_pixel_math, div_up(count, 256), 1, 256, a);
+}
+
+tk_status tk_brush_words(tk_context *ctx, const uint32_t *paint, uint32_t paint_words, uint32_t paint_brush_base,
+                         uint32_t count, const uint32_t *requests, uint32_t *out) {
+    if (!ctx || (count && (!requests || !out || (paint_words && !paint)))) return fail(TK_ERR_INVALID, "null argument");
+    if (!count) return TK_OK;
+    tk_args_brush_words a{count, requests, paint, paint_words, paint_brush_base, out};
+    return launch(ctx, ctx->f_brush_words, div_up(count, 256), 1, 256, a);
 }
 
 }  // extern "C"

@@ -120,3 +120,11 @@ struct tk_args_pixel_math {
     const uint32_t *requests;
     uint32_t *out;
 };
+
+struct tk_args_brush_words {
+    uint32_t count;
+    const uint32_t *requests;  // {data_base, x_bits, y_bits, 0}
+    const uint32_t *paint;
+    uint32_t paint_words, paint_brush_base;
+    uint32_t *out;
+};
