@@ -585,27 +585,25 @@ static float fill_value_sequential(const uint32_t *segs, int32_t backdrop, uint3
     return base + running + partial;
 }
 
-/* The same three sums, each exact: every FP32 term is truncated to 32.32 fixed
- * point (|t| clamped to 2^30, NaN -> 0) and summed as a 64-bit integer, then the
- * sums are converted back (floored to 2^-24, rounded once) and combined as
+/* The same three sums, each exact: every FP32 term (in (-2, 2) by construction)
+ * is truncated to a multiple of 2^-30 (trunc(t * 2^30) as a saturating int32
+ * conversion, NaN -> 0) and summed as a 64-bit integer, then the sums are
+ * converted back (floored to 2^-24, rounded once) and combined as
  * (base + running) + partial in FP32, as fill_alpha_at combines them.  No
  * segment order enters the result. */
-static int64_t coverage_fixed(float t) {
-    if (!(t == t)) return 0;
-    int64_t m = (int64_t)((double)fminf(fabsf(t), 1073741824.0f) * 4294967296.0);
-    return t < 0.0f ? -m : m;
-}
-/* Back to FP32: the sum floored to a multiple of 2^-24, then one rounding (hi and
- * lo >> 8 convert exactly, the 2^-24 scaling is exact, the add rounds). */
+static int64_t coverage_fixed(float t) { return (int64_t)f2i(t * 1073741824.0f); }
+/* Back to FP32: the sum floored to a multiple of 2^-24, then one rounding
+ * (floor(v / 2^30) and the top 24 fraction bits convert exactly, the 2^-24
+ * scaling is exact, the add rounds). */
 static float fixed_to_float(int64_t v) {
-    const float hi = (float)(int32_t)(v >> 32);
-    const float lo = (float)((uint32_t)v >> 8) * 5.9604644775390625e-8f;
+    const float hi = (float)(int32_t)(v >> 30);
+    const float lo = (float)(((uint32_t)v & 0x3FFFFFFFu) >> 6) * 5.9604644775390625e-8f;
     return hi + lo;
 }
 
 static float fill_value_at(const uint32_t *segs, int32_t backdrop, uint32_t start, uint32_t end, uint32_t x,
                            uint32_t y) {
-    int64_t base = (int64_t)backdrop * 4294967296LL, running = 0, partial = 0;
+    int64_t base = (int64_t)backdrop * 1073741824LL, running = 0, partial = 0;
     for (uint32_t i = start; i < end; ++i) {
         const uint32_t *s = segs + (size_t)i * 5u;
         RowParts p = segment_row_parts(bits_f(s[0]), bits_f(s[1]), bits_f(s[2]), bits_f(s[3]), bits_f(s[4]), y);

@@ -483,8 +483,8 @@ __device__ __forceinline__ void accumulate_record(const tk_line_segment *segs, u
         const RowParts p = segment_row_parts(seg.p0x, seg.p0y, seg.p1x, seg.p1y, seg.y_edge, row);
         if (p.edge != 0.0f) atomicAdd((unsigned long long *)&acc.base[row], (unsigned long long)coverage_fixed(p.edge));
         if (p.dy != 0.0f) {
-            const int fs = clampi(f2i(ceilf(p.xmax)), 0, 16);
-            const int ps = clampi(f2i(floorf(p.xmin)), 0, 16);
+            const int fs = clamp_cell16(ceilf(p.xmax));
+            const int ps = clamp_cell16(floorf(p.xmin));
             if (fs < 16) atomicAdd((unsigned long long *)&acc.run[row * 16u + fs], (unsigned long long)coverage_fixed(p.dy));
             for (int x = ps; x < fs; ++x)
                 atomicAdd((unsigned long long *)&acc.part[row * 16u + x],
@@ -1349,7 +1349,9 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_draw_brush(tk_args_draw_b
 // fine_tile_main for every batch: one 256-lane workgroup per tile, one lane per
 // pixel.  Between batches the pixel goes through the RGBA8 store/load round trip
 // the reference performs on its target.
-extern "C" __global__ __launch_bounds__(256) void tk_k_fine(tk_args_fine a) {
+// Seven waves per SIMD: at most 72 VGPRs (two dwords spill, in the per-batch-run
+// setup outside the record loop).
+extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void tk_k_fine(tk_args_fine a) {
     // Clip levels beyond the four register slots: clip_spill_depth * 256 bytes of
     // dynamic LDS, sized by the launch.
     extern __shared__ uint8_t clip_spill[];

@@ -38,6 +38,9 @@ TKD int32_t f2i(float v) {
 }
 TKD float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 TKD int clampi(int x, int lo, int hi) { return min(max(x, lo), hi); }
+// clampi(f2i(v), 0, 16) for every v (NaN included, as fmaxf(NaN, 0) = 0) without
+// f2i's range branches: v is clamped in FP32 first, so the conversion is exact.
+TKD int clamp_cell16(float v) { return (int)fminf(fmaxf(v, 0.0f), 16.0f); }
 TKD float asfloat(uint32_t u) { return __uint_as_float(u); }
 TKD uint32_t asuint(float f) { return __float_as_uint(f); }
 
@@ -376,32 +379,28 @@ TKD float segment_area_at(float xmin_abs, float xmax_abs, uint32_t x) {
 
 // fill_alpha_at keeps three sums per pixel (base, running, partial) and combines
 // them as (base + running) + partial.  Each sum here is exact: every FP32 term is
-// truncated to 32.32 fixed point (|t| <= 2^30, NaN -> 0) and the terms are added
-// as 64-bit integers, so a sum's value does not depend on the order in which the
-// tile's segments were emitted.  Only the conversion back to FP32 rounds.
+// truncated to a multiple of 2^-30 and the terms are added as 64-bit integers, so
+// a sum's value does not depend on the order in which the tile's segments were
+// emitted.  Only the conversion back to FP32 rounds.
 //
-// The truncation uses FP32 steps that are all exact: with a = min(|t|, 2^30),
-// trunc(a) is the integer part and (a - trunc(a)) * 2^32 the fraction scaled to
-// 32 bits (a - trunc(a) is exact, and so is the power-of-two scaling), so
-// trunc(a * 2^32) = trunc(a) * 2^32 + trunc((a - trunc(a)) * 2^32).
+// Every term lies in (-2, 2): the edge term is signum * a clamp to [0, 1], dy is
+// the difference of two values clamped to [0, 1], and the area factor is at most
+// 1 up to rounding.  So t * 2^30 (exact, a power-of-two scaling) fits an int32
+// and one truncating v_cvt_i32_f32 gives trunc(t * 2^30); a NaN term counts as 0.
 TKD long long coverage_fixed(float t) {
-    if (!(t == t)) return 0;
-    const float m = fminf(fabsf(t), 1073741824.0f);
-    const float whole = truncf(m);
-    const uint32_t hi = (uint32_t)whole;
-    const uint32_t lo = (uint32_t)((m - whole) * 4294967296.0f);
-    const long long q = (long long)(((unsigned long long)hi << 32) | lo);
-    return t < 0.0f ? -q : q;
+    const int32_t q = (int32_t)(t * 1073741824.0f);
+    return t == t ? (long long)q : 0ll;
 }
 // Back to FP32: the sum floored to a multiple of 2^-24 (at most 6e-8 below it),
-// then one rounding.  hi and lo >> 8 convert exactly (|hi| < 2^24 for any
-// coverage sum), the scaling by 2^-24 is exact, and only the add rounds.
+// then one rounding.  hi = floor(v / 2^30) and the top 24 of the 30 fraction bits
+// convert exactly (|hi| < 2^24 for any coverage sum), the scaling by 2^-24 is
+// exact, and only the add rounds.
 TKD float fixed_to_float(long long v) {
-    const float hi = (float)(int32_t)(v >> 32);
-    const float lo = (float)((uint32_t)v >> 8) * 5.9604644775390625e-8f;
+    const float hi = (float)(int32_t)(v >> 30);
+    const float lo = (float)(((uint32_t)v & 0x3FFFFFFFu) >> 6) * 5.9604644775390625e-8f;
     return hi + lo;
 }
-TKD long long backdrop_fixed(int32_t backdrop) { return (long long)backdrop * 4294967296LL; }
+TKD long long backdrop_fixed(int32_t backdrop) { return (long long)backdrop * 1073741824LL; }
 
 // One pixel's coverage from segments [start, end), terms summed exactly.
 TKD float exact_fill_value(const tk_line_segment *segs, int32_t backdrop, uint32_t start, uint32_t end, uint32_t x,
@@ -412,9 +411,9 @@ TKD float exact_fill_value(const tk_line_segment *segs, int32_t backdrop, uint32
         RowParts parts = segment_row_parts(seg.p0x, seg.p0y, seg.p1x, seg.p1y, seg.y_edge, y);
         base += coverage_fixed(parts.edge);
         if (parts.dy != 0.0f) {
-            int full_start = clampi(f2i(ceilf(parts.xmax)), 0, 16);
+            int full_start = clamp_cell16(ceilf(parts.xmax));
             if (full_start < 16 && (int)x >= full_start) running += coverage_fixed(parts.dy);
-            int partial_start = clampi(f2i(floorf(parts.xmin)), 0, 16);
+            int partial_start = clamp_cell16(floorf(parts.xmin));
             if ((int)x >= partial_start && (int)x < full_start)
                 partial += coverage_fixed(segment_area_at(parts.xmin, parts.xmax, x) * parts.dy);
         }
@@ -556,8 +555,11 @@ TKD uint32_t sample_ramp(const uint32_t *paint, uint32_t count, uint32_t brush_b
     if (fraction <= kGradientEpsilon || left_ix == right_ix) return left;
     return lerp_premul_u8(left, right, fraction);
 }
-TKD uint32_t sample_linear(const uint32_t *paint, uint32_t count, uint32_t brush_base, float x, float y, uint32_t base,
-                           uint32_t extend, uint32_t payload, uint32_t len) {
+// The ramp parameter of each gradient kind at (x, y); the ramp lookup is shared
+// (sample_brush below), so fine carries one inlined copy of it.
+//
+// linear_t: sample_linear (shared/brush/linear.hlsli:5-14).
+TKD float linear_t(const uint32_t *paint, uint32_t count, uint32_t brush_base, float x, float y, uint32_t base) {
     float p[10];
     for (int i = 0; i < 10; ++i) p[i] = blob_float(paint, count, brush_base, base + (uint32_t)i);
     float tx = fmaf(p[4], x, fmaf(p[6], y, p[8]));
@@ -565,13 +567,15 @@ TKD uint32_t sample_linear(const uint32_t *paint, uint32_t count, uint32_t brush
     float dx = p[2] - p[0], dy = p[3] - p[1];
     float denominator = dx * dx + dy * dy, t = 0.0f;
     if (denominator > kGradientEpsilon) t = fmaf(tx - p[0], dx, (ty - p[1]) * dy) / denominator;
-    return sample_ramp(paint, count, brush_base, payload, len, t, extend);
+    return t;
 }
-// sample_radial (shared/brush/radial.hlsli:5-33): the two-point conical gradient,
-// t from the quadratic |q - t dc| = r0 + t dr with r0 + t dr >= 0.
+
+// radial_t: sample_radial (shared/brush/radial.hlsli:5-33), the two-point conical
+// gradient: t from the quadratic |q - t dc| = r0 + t dr with r0 + t dr >= 0;
+// has_t is false where no such t exists (the sample is transparent).
 constexpr float kRadialQuadraticEpsilon = 0.000001f;  // shared/brush/radial.hlsli:4
-TKD uint32_t sample_radial(const uint32_t *paint, uint32_t count, uint32_t brush_base, float x, float y, uint32_t base,
-                           uint32_t extend, uint32_t payload, uint32_t len) {
+TKD float radial_t(const uint32_t *paint, uint32_t count, uint32_t brush_base, float x, float y, uint32_t base,
+                   bool &has_t) {
     float p[12];
     for (int i = 0; i < 12; ++i) p[i] = blob_float(paint, count, brush_base, base + (uint32_t)i);
     const float tx = fmaf(p[6], x, fmaf(p[8], y, p[10]));
@@ -581,7 +585,7 @@ TKD uint32_t sample_radial(const uint32_t *paint, uint32_t count, uint32_t brush
     const float qa = fmaf(dcx, dcx, fmaf(dcy, dcy, -(dr * dr)));
     const float qb = -2.0f * fmaf(qx, dcx, fmaf(qy, dcy, r0 * dr));
     const float qc = fmaf(qx, qx, fmaf(qy, qy, -(r0 * r0)));
-    bool has_t = false;
+    has_t = false;
     float t = 0.0f;
     if (fabsf(qa) <= kRadialQuadraticEpsilon) {
         if (fabsf(qb) > kRadialQuadraticEpsilon) {
@@ -606,21 +610,43 @@ TKD uint32_t sample_radial(const uint32_t *paint, uint32_t count, uint32_t brush
             }
         }
     }
-    return has_t ? sample_ramp(paint, count, brush_base, payload, len, t, extend) : 0u;
+    return t;
 }
 
-// sample_sweep (shared/brush/sweep.hlsli:5-20): the angle from the centre,
-// measured from the start orientation in the direction of the signed span.
+// atan2(y, x) for (x, y) != (0, 0): the octant reduction a = min / max of |x|, |y|
+// and the odd polynomial of Abramowitz & Stegun 4.4.49 (Hastings) for atan(a) on
+// [0, 1], absolute error 2e-8 before FP32 rounding (the ratio uses the hardware
+// reciprocal, 1 ulp); then the quadrant.  It keeps fine's register budget that
+// the device library's atan2f would raise; sweep samples stay within 1 LSB of an
+// exact angle.
+TKD float sweep_atan2(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float a = fminf(ax, ay) * __builtin_amdgcn_rcpf(fmaxf(ax, ay)), s = a * a;
+    float p = 0.0028662257f;
+    p = fmaf(p, s, -0.0161657367f);
+    p = fmaf(p, s, 0.0429096138f);
+    p = fmaf(p, s, -0.0752896400f);
+    p = fmaf(p, s, 0.1065626393f);
+    p = fmaf(p, s, -0.1420889944f);
+    p = fmaf(p, s, 0.1999355085f);
+    p = fmaf(p, s, -0.3333314528f);
+    float r = fmaf(a * s, p, a);
+    if (ay > ax) r = 1.57079637f - r;
+    if (x < 0.0f) r = 3.14159274f - r;
+    return y < 0.0f ? -r : r;
+}
+
+// sweep_t: sample_sweep (shared/brush/sweep.hlsli:5-20), the angle from the
+// centre measured from the start orientation in the direction of the signed span.
 constexpr float kGradientTau = 6.2831855f;  // shared/brush/sweep.hlsli:4
-TKD uint32_t sample_sweep(const uint32_t *paint, uint32_t count, uint32_t brush_base, float x, float y, uint32_t base,
-                          uint32_t extend, uint32_t payload, uint32_t len) {
+TKD float sweep_t(const uint32_t *paint, uint32_t count, uint32_t brush_base, float x, float y, uint32_t base) {
     const float cx = blob_float(paint, count, brush_base, base), cy = blob_float(paint, count, brush_base, base + 1u);
     const float start_angle = blob_float(paint, count, brush_base, base + 2u);
     const float span = blob_float(paint, count, brush_base, base + 3u);
     float t = 0.0f;
     if (fabsf(span) > kGradientEpsilon) {
         float angle = 0.0f;
-        if (x != cx || y != cy) angle = atan2f(y - cy, x - cx);
+        if (x != cx || y != cy) angle = sweep_atan2(y - cy, x - cx);
         float distance = angle - start_angle;
         if (span > 0.0f) {
             if (distance < 0.0f) distance += kGradientTau;
@@ -629,7 +655,7 @@ TKD uint32_t sample_sweep(const uint32_t *paint, uint32_t count, uint32_t brush_
         }
         t = distance / span;
     }
-    return sample_ramp(paint, count, brush_base, payload, len, t, extend);
+    return t;
 }
 
 // sample_four_corner (shared/brush/four_corner.hlsli:5-15): bilinear blend of the
@@ -657,9 +683,14 @@ TKD uint32_t sample_brush(const uint32_t *paint, uint32_t count, uint32_t brush_
     uint32_t len = blob_word(paint, count, brush_base, data_base + 3u);
     uint32_t base = data_base + TK_BRUSH_HEADER_WORDS;
     uint32_t color = blob_word(paint, count, brush_base, data_base + 4u);
-    if (kind == TK_BRUSH_LINEAR) return sample_linear(paint, count, brush_base, x, y, base, extend, payload, len);
-    if (kind == TK_BRUSH_RADIAL) return sample_radial(paint, count, brush_base, x, y, base, extend, payload, len);
-    if (kind == TK_BRUSH_SWEEP) return sample_sweep(paint, count, brush_base, x, y, base, extend, payload, len);
+    if (kind == TK_BRUSH_LINEAR || kind == TK_BRUSH_RADIAL || kind == TK_BRUSH_SWEEP) {
+        bool has_t = true;
+        float t;
+        if (kind == TK_BRUSH_LINEAR) t = linear_t(paint, count, brush_base, x, y, base);
+        else if (kind == TK_BRUSH_RADIAL) t = radial_t(paint, count, brush_base, x, y, base, has_t);
+        else t = sweep_t(paint, count, brush_base, x, y, base);
+        return has_t ? sample_ramp(paint, count, brush_base, payload, len, t, extend) : 0u;
+    }
     if (kind == TK_BRUSH_FOUR_CORNER) return sample_four_corner(paint, count, brush_base, x, y, base, payload);
     if (kind == 6u) return 0u;  // BRUSH_PATTERN
     // Mesh, program and image brushes are not produced on this path.
