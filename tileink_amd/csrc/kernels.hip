@@ -563,43 +563,50 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_clear(tk_args_scan_c
     }
 }
 
-// Wave-aggregated counter updates.  Consecutive lines of a path are short and
-// mostly fall in the same few tiles, so the active lanes of a wave are grouped by
-// target word and each group issues one atomic.  Must be called from code where
-// every lane that reaches it takes part (any subset of the wave may be active).
-__device__ __forceinline__ void wave_add_pm1(uint32_t *words, uint32_t index, bool plus) {
+// Run-length aggregation over the active lanes: a run is a maximal stretch of
+// consecutive active lanes with the same index (consecutive lines of a path
+// usually share their tile).  Returns the run's first lane and its length
+// (meaningful in that first lane); constant cost however many distinct indices
+// the wave holds.  Called where every lane that reaches it takes part (any subset
+// of the wave may be active); indices repeated in lanes that are not adjacent
+// form separate runs, each with its own atomic.
+struct LaneRun {
+    uint32_t head, length;
+};
+__device__ __forceinline__ LaneRun lane_run(uint32_t index) {
     const uint32_t lane = __lane_id();
-    unsigned long long pending = __ballot(1);
-    for (;;) {
-        const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
-        const uint32_t lead_index = (uint32_t)__shfl((int)index, (int)leader, 64);
-        const unsigned long long peers = __ballot(index == lead_index);
-        const unsigned long long ups = __ballot(index == lead_index && plus);
-        if (index == lead_index) {
-            if (lane == leader)
-                atomicAdd(&words[index], (uint32_t)(__popcll(ups) - __popcll(peers & ~ups)));
-            return;
-        }
-        pending &= ~peers;
-    }
+    const unsigned long long active = __ballot(1);
+    const uint32_t prev = (uint32_t)__shfl_up((int)index, 1, 64);
+    const bool head = lane == 0u || ((active >> (lane - 1u)) & 1ull) == 0ull || prev != index;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
+    LaneRun r;
+    r.head = 63u - (uint32_t)__clzll(heads & upto);
+    const unsigned long long stops = (heads | ~active) & ~upto;  // lanes after this one that end the run
+    r.length = (stops ? (uint32_t)__builtin_ctzll(stops) : 64u) - lane;
+    return r;
 }
 
-// Claims one slot of cursors[index] per lane; lanes with the same index receive
+// Claims one slot of cursors[index] per lane; the lanes of a run receive
 // consecutive slots from one atomic.
-__device__ __forceinline__ uint32_t wave_claim_slot(uint32_t *cursors, uint32_t index) {
+__device__ __forceinline__ uint32_t wave_claim_slot_runs(uint32_t *cursors, uint32_t index) {
     const uint32_t lane = __lane_id();
-    unsigned long long pending = __ballot(1);
-    for (;;) {
-        const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
-        const uint32_t lead_index = (uint32_t)__shfl((int)index, (int)leader, 64);
-        const unsigned long long peers = __ballot(index == lead_index);
-        if (index == lead_index) {
-            uint32_t first = 0u;
-            if (lane == leader) first = atomicAdd(&cursors[index], (uint32_t)__popcll(peers));
-            first = (uint32_t)__shfl((int)first, (int)leader, 64);
-            return first + (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
-        }
-        pending &= ~peers;
+    const LaneRun r = lane_run(index);
+    uint32_t first = 0u;
+    if (r.head == lane) first = atomicAdd(&cursors[index], r.length);
+    first = (uint32_t)__shfl((int)first, (int)r.head, 64);
+    return first + (lane - r.head);
+}
+
+// Adds +1 or -1 per lane to words[index], one atomic per run of equal indices.
+__device__ __forceinline__ void wave_add_pm1_runs(uint32_t *words, uint32_t index, bool plus) {
+    const uint32_t lane = __lane_id();
+    const LaneRun r = lane_run(index);
+    const unsigned long long ups = __ballot(plus);
+    if (r.head == lane) {
+        const unsigned long long span = (r.length >= 64u ? ~0ull : ((1ull << r.length) - 1ull)) << lane;
+        const uint32_t n_up = (uint32_t)__popcll(ups & span);
+        atomicAdd(&words[index], n_up - (r.length - n_up));
     }
 }
 
@@ -615,7 +622,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_count(tk_args_scan_l
     uint32_t stride = s.bz - s.bx;
     for (int32_t y = s.ymin; y < s.ymax; y++) {
         uint32_t local = (uint32_t)(y - (int32_t)s.by) * stride;
-        wave_add_pm1(backdrops, s.data_offset + local, up);
+        wave_add_pm1_runs(backdrops, s.data_offset + local, up);
     }
     float last_z = floorf(s.a * ((float)s.imin - 1.0f) + s.b);
     for (uint32_t i = s.imin; i < s.imax; i++) {
@@ -628,9 +635,9 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_count(tk_args_scan_l
             if (top && tx + 1 < (int32_t)s.bz) {
                 int32_t x = max(tx + 1, (int32_t)s.bx);
                 uint32_t local = (uint32_t)(ty - (int32_t)s.by) * stride + (uint32_t)(x - (int32_t)s.bx);
-                wave_add_pm1(backdrops, s.data_offset + local, up);
+                wave_add_pm1_runs(backdrops, s.data_offset + local, up);
             }
-            wave_add_pm1(a.counts_or_cursors, scan_tile_offset(s, tx, ty), true);
+            wave_add_pm1_runs(a.counts_or_cursors, scan_tile_offset(s, tx, ty), true);
         }
         last_z = z;
     }
@@ -776,7 +783,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_emit(tk_args_scan_li
         int32_t tx, ty;
         scan_tile(s, i, z, tx, ty);
         if (scan_tile_inside(s, tx, ty)) {
-            uint32_t dst = wave_claim_slot(a.counts_or_cursors, scan_tile_offset(s, tx, ty));
+            uint32_t dst = wave_claim_slot_runs(a.counts_or_cursors, scan_tile_offset(s, tx, ty));
             if (dst < a.cfg.segment_capacity) a.segments[dst] = clipped_segment(s, i, z, tx, ty);
         }
     }
