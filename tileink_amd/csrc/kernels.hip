@@ -1289,6 +1289,14 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
     for (uint32_t base = 0; base < len; base += 64u) {
         uint32_t d, b;
         load_list_entry(a, list, base, lane, d, b);
+        // Every lane's particle and class at once (a lane's batch is its own b), so
+        // the particle code runs once per 64 entries however many batches they span.
+        Particle p{false, TK_PTCL_FILL, 0u, 0u, 0u, 0u, 0u};
+        if (b != kNone) p = draw_particle(c, b, d);
+        uint32_t cls = 0u;
+        if (p.valid) cls = particle_class_flags(c, d, p.tag);
+        const bool p_opaque = p.valid && p.tag == TK_PTCL_COLOR && (p.color >> 24) == 255u;
+        const bool p_begins = p.valid && is_begin_tag(p.tag);
         uint64_t pending = __ballot(b != kNone);
         while (pending) {
             const uint32_t first = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
@@ -1303,21 +1311,15 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
             }
             if (!active) continue;
             const bool mine = (run >> lane) & 1ull;
-            Particle p{false, TK_PTCL_FILL, 0u, 0u, 0u, 0u, 0u};
-            if (mine) p = draw_particle(c, bb, d);
-            const uint64_t valid = __ballot(p.valid);
+            const bool emits = mine && p.valid;
+            const uint64_t valid = __ballot(emits);
             const uint32_t pos = dst + (uint32_t)__popcll(valid & lanes_below(lane));
-            uint32_t cls = 0u;
-            bool opaque = false, begins = false;
-            if (p.valid) {
-                cls = particle_class_flags(c, d, p.tag);
-                opaque = p.tag == TK_PTCL_COLOR && (p.color >> 24) == 255u;
-                begins = is_begin_tag(p.tag);
-                if (pos < end) e.store(pos, p.tag, p.winding, p.fill_rule, p.seg_start, p.seg_end, p.color);
-            }
-            if (__ballot(cls & kClassColor)) flags |= kClassColor;
-            if (__ballot(cls & kClassSdf)) flags |= kClassSdf;
-            if (__ballot(cls & kClassOther)) flags |= kClassOther;
+            const uint32_t my_cls = emits ? cls : 0u;
+            const bool opaque = emits && p_opaque, begins = emits && p_begins;
+            if (emits && pos < end) e.store(pos, p.tag, p.winding, p.fill_rule, p.seg_start, p.seg_end, p.color);
+            if (__ballot(my_cls & kClassColor)) flags |= kClassColor;
+            if (__ballot(my_cls & kClassSdf)) flags |= kClassSdf;
+            if (__ballot(my_cls & kClassOther)) flags |= kClassOther;
             if (__ballot(opaque && wrappers == 0u && pos > start)) flags |= 8u;
             const uint64_t opaque_mask = __ballot(opaque), begin_mask = __ballot(begins);
             if (opaque_mask) {
