@@ -6,7 +6,9 @@ import pytest
 import tileink_amd as tk
 from tileink_amd import scenes
 
-from helpers import channel_diff, compare_frame
+import oracle
+
+from helpers import channel_diff, compare_frame, read_frame
 
 pytestmark = pytest.mark.gpu
 
@@ -44,3 +46,33 @@ def test_deep_clip_stack_spills_to_lds(ctx):
         canvas.pop_layer()
     gpu, ref, bad = compare_frame(ctx, canvas)
     assert not bad, _report(gpu, ref, bad)
+
+
+def _frame_matches_oracle(frame, scene, arrays):
+    frame.set_capture(True)
+    frame.render()
+    gpu = read_frame(frame, scene)
+    image, ref = oracle.render_frame(arrays)
+    for key in ("tile_ptcl_counts", "tile_kinds", "ptcl"):
+        assert np.array_equal(gpu[key], ref[key]), key
+    assert np.array_equal(gpu["image"], image), channel_diff(gpu["image"], image)
+
+
+def test_frame_update_with_more_layer_pairs_than_its_class_table(ctx):
+    """A frame sizes its per-tile layer class table for the distinct (draw, tag)
+    pairs of the scene it was created with; an update that brings more pairs
+    falls back to evaluating each layer, with the same records and image."""
+    arrays = scenes.config2_small(16).arrays()
+    assert len({tuple(r) for r in arrays.layers[:, :2].tolist()}) > 1
+    collapsed = tk.SceneArrays.__new__(tk.SceneArrays)
+    collapsed.__dict__.update(arrays.__dict__)
+    collapsed.layers = arrays.layers.copy()
+    collapsed.layers[:, 1] = arrays.layers[0, 1]  # every stack level clips by the first clip draw
+    first = collapsed.to_scene()
+    frame = tk.Frame(ctx, first)
+    _frame_matches_oracle(frame, first, collapsed)
+    second = arrays.to_scene()
+    frame.update(second)
+    _frame_matches_oracle(frame, second, arrays)
+    frame.update(first)
+    _frame_matches_oracle(frame, first, collapsed)

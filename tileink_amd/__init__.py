@@ -404,6 +404,11 @@ class Frame:
     def render(self):
         check(lib().tk_frame_render(self._h))
 
+    def update(self, scene):
+        """Replaces the scene's arrays; every size must match (tk_frame_update)."""
+        check(lib().tk_frame_update(self._h, C.byref(scene)))
+        self._scene = scene
+
     def set_capture(self, enable=True):
         check(lib().tk_frame_set_capture(self._h, int(bool(enable))))
 

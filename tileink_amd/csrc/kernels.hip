@@ -316,6 +316,8 @@ struct CoarseCtx {
     const tk_coarse_config *cfg;
     const tk_coarse_inputs *in;
     uint32_t tx, ty;
+    const uint8_t *class_row;      // this tile's layer-pair classes, or null
+    const uint32_t *class_pair_of; // per layer record: its pair
 };
 
 struct Particle {
@@ -387,6 +389,48 @@ __device__ bool noop_stack_clip(const CoarseCtx &c, const tk_layer_record &layer
     tk_segment_range range = c.in->segment_ranges[bd];
     uint32_t winding = (uint32_t)c.in->backdrops[bd];
     return range.start == range.end && (d.fill_rule == TK_FILL_EVEN_ODD ? (winding & 1u) != 0u : winding != 0u);
+}
+
+// A layer's verdict on this tile, as stack_wrapper_count (coarse/classify.hlsli:
+// 11-33) and emit_stack_begins (coarse/emit_stack.hlsli:18-31) decide it:
+// kLayerInvalid rejects the tile for the whole batch, kLayerCounted counts a
+// wrapper, kLayerEmits writes BEGIN/END records, kLayerSdf marks an SDF clip.
+constexpr uint32_t kLayerInvalid = 1u, kLayerCounted = 2u, kLayerEmits = 4u, kLayerSdf = 8u;
+__device__ uint32_t layer_class_bits(const CoarseCtx &c, const tk_layer_record &layer) {
+    const tk_coarse_config &cfg = *c.cfg;
+    const tk_coarse_inputs &in = *c.in;
+    const DrawData d = load_draw(in.draws, cfg.draw_count, layer.draw);
+    uint32_t bits = 0u;
+    if (draw_has_sdf(d)) {
+        const bool covers = sdf_clip_covers(in.paint, cfg.paint_words, d, c.tx, c.ty);
+        if (!covers) bits |= draw_hits_tile(d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height) ? kLayerCounted : kLayerInvalid;
+        if (!(layer.tag == TK_LAYER_CLIP && covers)) bits |= kLayerEmits;
+        return bits | kLayerSdf;
+    }
+    const uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
+    if (bd == kNone) return kLayerInvalid;
+    const tk_segment_range range = in.segment_ranges[bd];
+    const uint32_t winding = (uint32_t)in.backdrops[bd];
+    const bool empty = range.start == range.end;
+    const bool full = d.fill_rule == TK_FILL_EVEN_ODD ? (winding & 1u) != 0u : winding != 0u;
+    if (layer.tag == TK_LAYER_CLIP && empty && full) return 0u;  // a no-op clip
+    return kLayerEmits | ((empty && winding == 0u) ? kLayerInvalid : kLayerCounted);
+}
+
+// The verdict of layer record i: from the per-tile class table when the frame
+// built one, else evaluated here.
+__device__ __forceinline__ uint32_t layer_bits_at(const CoarseCtx &c, uint32_t i, const tk_layer_record &layer) {
+    if (c.class_row) return c.class_row[c.class_pair_of[i]];
+    return layer_class_bits(c, layer);
+}
+
+__device__ __forceinline__ CoarseCtx coarse_ctx(const tk_args_coarse &a, uint32_t tile) {
+    CoarseCtx c{&a.cfg, &a.in, tile % a.cfg.tiles_width, tile / a.cfg.tiles_width, nullptr, nullptr};
+    if (a.classes.bits) {
+        c.class_row = a.classes.bits + (size_t)tile * a.classes.pair_count;
+        c.class_pair_of = a.classes.pair_of;
+    }
+    return c;
 }
 
 struct Emitter {
@@ -918,35 +962,26 @@ __device__ uint32_t stack_begins_single_pass(const CoarseCtx &c, uint32_t batch,
     end_tag = TK_PTCL_END_CLIP;
     if (i < end) {
         const tk_layer_record layer = in.layers[i];
-        const DrawData d = load_draw(in.draws, cfg.draw_count, layer.draw);
         end_tag = layer.tag == 1u ? kPtclEndOpacity : (layer.tag == 2u ? kPtclEndBlend : TK_PTCL_END_CLIP);
-        if (draw_has_sdf(d)) {
-            const bool covers = sdf_clip_covers(in.paint, cfg.paint_words, d, c.tx, c.ty);
-            if (!covers) {
-                if (!draw_hits_tile(d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height)) invalid = true;
-                else counted = true;
-            }
-            emit = !(layer.tag == TK_LAYER_CLIP && covers);
+        const uint32_t bits = layer_bits_at(c, i, layer);
+        invalid = (bits & kLayerInvalid) != 0u;
+        counted = (bits & kLayerCounted) != 0u;
+        emit = (bits & kLayerEmits) != 0u;
+        if (bits & kLayerSdf) {
             tag = TK_PTCL_BEGIN_SDF_CLIP;
             payload = layer.draw;
         } else {
-            const uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
             tag = layer.tag == 1u ? kPtclBeginOpacity : (layer.tag == 2u ? kPtclBeginBlend : TK_PTCL_BEGIN_CLIP);
-            fill_rule = d.fill_rule;
             payload = layer.payload;
-            if (bd == kNone) {
-                invalid = true;
-            } else {
-                range = in.segment_ranges[bd];
-                winding = (uint32_t)in.backdrops[bd];
-                const bool empty = range.start == range.end;
-                const bool full = d.fill_rule == TK_FILL_EVEN_ODD ? (winding & 1u) != 0u : winding != 0u;
-                const bool noop = layer.tag == TK_LAYER_CLIP && empty && full;
-                if (!noop) {
-                    if (empty && winding == 0u) invalid = true;
-                    else counted = true;
+            if (emit && !invalid) {  // a path clip's BEGIN record carries its tile segments and winding
+                const DrawData d = load_draw(in.draws, cfg.draw_count, layer.draw);
+                fill_rule = d.fill_rule;
+                const uint32_t bd =
+                    draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
+                if (bd != kNone) {
+                    range = in.segment_ranges[bd];
+                    winding = (uint32_t)in.backdrops[bd];
                 }
-                emit = !noop;
             }
         }
     }
@@ -1045,35 +1080,24 @@ __device__ uint32_t stack_wrappers_of_batches(const CoarseCtx &c, uint32_t pendi
         j = min(j, 63u);
         const uint32_t i = (uint32_t)__shfl((int)first_layer, (int)j, 64) + t - (uint32_t)__shfl((int)off, (int)j, 64);
         if (t < total) {
-            const tk_layer_record layer = in.layers[i];
-            const DrawData d = load_draw(in.draws, cfg.draw_count, layer.draw);
-            bool invalid = false, count_it = false;
-            if (draw_has_sdf(d)) {
-                if (!sdf_clip_covers(in.paint, cfg.paint_words, d, c.tx, c.ty)) {
-                    if (!draw_hits_tile(d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height)) invalid = true;
-                    else count_it = true;
-                }
-            } else {
-                const uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
-                if (bd == kNone) {
-                    invalid = true;
-                } else {
-                    const tk_segment_range range = in.segment_ranges[bd];
-                    const int32_t winding = in.backdrops[bd];
-                    const bool full = d.fill_rule == TK_FILL_EVEN_ODD ? (((uint32_t)winding & 1u) != 0u) : winding != 0;
-                    if (!(layer.tag == TK_LAYER_CLIP && range.start == range.end && full)) {
-                        if (range.start == range.end && winding == 0) invalid = true;
-                        else count_it = true;
-                    }
-                }
-            }
-            if (invalid) atomicOr(&invalid_flags[j], 1u);
-            if (count_it) atomicAdd(&counted[j], 1u);
+            const uint32_t bits = c.class_row ? c.class_row[c.class_pair_of[i]] : layer_class_bits(c, in.layers[i]);
+            if (bits & kLayerInvalid) atomicOr(&invalid_flags[j], 1u);
+            if (bits & kLayerCounted) atomicAdd(&counted[j], 1u);
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
     return invalid_flags[lane] ? kNone : counted[lane];
+}
+
+// The class table: layer_class_bits of every distinct (draw, tag) pair of the layer
+// records on every tile, one thread per (tile, pair), tile-major.
+extern "C" __global__ __launch_bounds__(256) void tk_k_layer_classes(tk_args_layer_classes a) {
+    const uint64_t id = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (id >= (uint64_t)a.cfg.tile_count * a.pair_count) return;
+    const uint32_t tile = (uint32_t)(id / a.pair_count), pair = (uint32_t)(id % a.pair_count);
+    const CoarseCtx c{&a.cfg, &a.in, tile % a.cfg.tiles_width, tile / a.cfg.tiles_width, nullptr, nullptr};
+    a.bits[id] = (uint8_t)layer_class_bits(c, a.in.layers[a.pair_layer[pair]]);
 }
 
 // coarse_count_bins for every batch (coarse/count.hlsl:48-73): one wave per tile.
@@ -1087,7 +1111,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coar
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t tile = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + wave;
     if (tile >= cfg.tile_count) return;
-    const CoarseCtx c{&a.cfg, &a.in, tile % cfg.tiles_width, tile / cfg.tiles_width};
+    const CoarseCtx c = coarse_ctx(a, tile);
     const uint32_t T = cfg.tile_count, Q = cfg.batch_count;
     uint32_t *counts = a.out.tile_ptcl_counts;
     const tk_tile_draw_record list = a.in.tile_draws[tile];
@@ -1223,7 +1247,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
     const uint32_t tile =
         xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + (threadIdx.x >> 6);
     if (tile >= cfg.tile_count) return;
-    const CoarseCtx c{&a.cfg, &a.in, tile % cfg.tiles_width, tile / cfg.tiles_width};
+    const CoarseCtx c = coarse_ctx(a, tile);
     const uint32_t T = cfg.tile_count, Q = cfg.batch_count;
     Emitter e{a.out.ptcl, cfg.ptcl_capacity, a.out.status};
     uint32_t *kinds = a.out.tile_kinds;

@@ -65,7 +65,8 @@ struct tk_context {
     hipStream_t stream = nullptr;
     hipFunction_t f_fill = nullptr, f_copy = nullptr, f_scan_clear = nullptr, f_scan_count = nullptr, f_scan_prefix = nullptr, f_scan_chunk_apply = nullptr,
                   f_scan_emit = nullptr, f_cumsum = nullptr, f_coarse_count = nullptr,
-                  f_coarse_scan = nullptr, f_coarse_emit = nullptr, f_draw_brush = nullptr, f_fine = nullptr,
+                  f_coarse_scan = nullptr, f_coarse_emit = nullptr, f_layer_classes = nullptr, f_draw_brush = nullptr,
+                  f_fine = nullptr,
                   f_fill_coverage = nullptr, f_pixel_math = nullptr, f_brush_words = nullptr;
     // Scratch for the stage entry points (frames own their own).
     uint32_t *tickets = nullptr;  // [0] scan prefix, [1] coarse scan
@@ -181,11 +182,26 @@ uint32_t coarse_scan_blocks(const tk_coarse_config &cfg) {
     return uint32_t(std::max<uint64_t>((n + TK_COARSE_SCAN_ITEMS - 1) / TK_COARSE_SCAN_ITEMS, 1));
 }
 
+// Where the frame keeps its per-tile layer classes: pair_layer[pair] is a layer
+// record of each distinct (draw, tag) pair, pair_of[layer] its pair, and bits
+// holds tile_count * pair_count bytes.  bits == nullptr: no table.
+struct LayerClassPlan {
+    const uint32_t *pair_layer = nullptr, *pair_of = nullptr;
+    uint32_t pair_count = 0;
+    uint8_t *bits = nullptr;
+};
+
 tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coarse_inputs &in,
                      const tk_coarse_outputs &out, unsigned long long *status, uint32_t *ticket, uint32_t epoch,
-                     hipEvent_t *marks) {
+                     hipEvent_t *marks, const LayerClassPlan &plan = LayerClassPlan{}) {
     if (cfg.batch_count == 0 || cfg.tile_count == 0) return TK_OK;
-    tk_args_coarse a{cfg, in, out, status, ticket, epoch, coarse_scan_blocks(cfg)};
+    tk_args_coarse a{cfg, in, out, status, ticket, epoch, coarse_scan_blocks(cfg), tk_layer_classes{nullptr, nullptr, 0u}};
+    if (plan.bits && plan.pair_count) {
+        tk_args_layer_classes b{cfg, in, plan.pair_layer, plan.pair_count, plan.bits};
+        TK_TRY(launch(ctx, ctx->f_layer_classes, uint32_t(div_up(uint64_t(cfg.tile_count) * plan.pair_count, 256)), 1,
+                      256, b));
+        a.classes = tk_layer_classes{plan.bits, plan.pair_of, plan.pair_count};
+    }
     TK_TRY(launch(ctx, ctx->f_coarse_count, xcd_grid(div_up(cfg.tile_count, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
     if (marks) TK_HIP(hipEventRecord(marks[0], ctx->stream));
     TK_TRY(launch(ctx, ctx->f_coarse_scan, a.scan_blocks, 1, 256, a));
@@ -250,6 +266,11 @@ struct tk_frame {
     tk_ptcl_record *ptcl = nullptr;
     uint64_t ptcl_capacity = 0;
     uint32_t *draw_brush = nullptr;  // fine's per-draw brush words
+    // per-tile layer classes (coarse): see LayerClassPlan
+    uint32_t *layer_pair_of = nullptr, *pair_layer = nullptr;
+    uint8_t *layer_bits = nullptr;
+    uint32_t pair_capacity = 0, pair_count = 0;
+    std::vector<uint32_t> host_pair_of, host_pair_layer;
     // output
     uint32_t *image = nullptr;
     // scratch
@@ -299,6 +320,27 @@ tk_status validate_scene(const tk_scene *s) {
     return TK_OK;
 }
 
+// The distinct (draw, tag) pairs among the layer records: pair_of[layer] and one
+// representative layer per pair.  The classes a coarse kernel derives from a layer
+// depend on nothing else of the record.
+uint32_t layer_pairs(const tk_scene *s, std::vector<uint32_t> &pair_of, std::vector<uint32_t> &pair_layer) {
+    pair_of.assign(s->layer_count, 0u);
+    pair_layer.clear();
+    std::vector<std::pair<uint64_t, uint32_t>> keyed(s->layer_count);
+    for (uint32_t i = 0; i < s->layer_count; ++i)
+        keyed[i] = {(uint64_t(s->layers[i].draw) << 32) | s->layers[i].tag, i};
+    std::sort(keyed.begin(), keyed.end());
+    for (uint32_t k = 0; k < s->layer_count; ++k) {
+        if (k == 0 || keyed[k].first != keyed[k - 1].first) pair_layer.push_back(keyed[k].second);
+        pair_of[keyed[k].second] = uint32_t(pair_layer.size() - 1);
+    }
+    return uint32_t(pair_layer.size());
+}
+
+// Largest class table a frame allocates (tiles x pairs bytes); beyond it the
+// coarse kernels evaluate each layer themselves.
+constexpr uint64_t kMaxLayerClassBytes = 64ull << 20;
+
 tk_status frame_upload(tk_frame *f, const tk_scene *s) {
     TK_TRY(upload(f, f->lines, s->lines, s->line_count));
     TK_TRY(upload(f, f->paths, s->paths, s->path_count));
@@ -315,6 +357,15 @@ tk_status frame_upload(tk_frame *f, const tk_scene *s) {
     TK_TRY(upload(f, f->cs_lens, s->cumsum_chunk_lens, s->cumsum_chunk_count));
     TK_TRY(upload(f, f->cs_row_starts, s->cumsum_row_starts, s->cumsum_row_count));
     TK_TRY(upload(f, f->cs_row_ends, s->cumsum_row_ends, s->cumsum_row_count));
+    f->pair_count = 0;
+    if (f->layer_bits && s->layer_count && s->layers) {
+        const uint32_t pairs = layer_pairs(s, f->host_pair_of, f->host_pair_layer);
+        if (pairs <= f->pair_capacity) {  // else: no table for this scene
+            TK_TRY(upload(f, f->layer_pair_of, f->host_pair_of.data(), s->layer_count));
+            TK_TRY(upload(f, f->pair_layer, f->host_pair_layer.data(), pairs));
+            f->pair_count = pairs;
+        }
+    }
     TK_HIP(hipStreamSynchronize(f->ctx->stream));
     return TK_OK;
 }
@@ -434,7 +485,15 @@ tk_status frame_render(tk_frame *f) {
         out.batch_ptcl_bases = f->batch_bases;
         out.ptcl = f->ptcl;
         out.status = f->status;
-        TK_TRY(run_coarse(ctx, cc, in, out, f->coarse_status, &f->tickets[1], f->epoch, f->timing ? &f->ev[7] : nullptr));
+        LayerClassPlan plan;
+        if (f->pair_count) {
+            plan.pair_layer = f->pair_layer;
+            plan.pair_of = f->layer_pair_of;
+            plan.pair_count = f->pair_count;
+            plan.bits = f->layer_bits;
+        }
+        TK_TRY(run_coarse(ctx, cc, in, out, f->coarse_status, &f->tickets[1], f->epoch, f->timing ? &f->ev[7] : nullptr,
+                          plan));
         if (cc.batch_count == 0 && f->timing) {
             TK_TRY(record(f, 7));
             TK_TRY(record(f, 8));
@@ -507,6 +566,7 @@ tk_status tk_context_create(int device, tk_context **out) {
         {&ctx->f_coarse_count, "tk_k_coarse_count"},
         {&ctx->f_coarse_scan, "tk_k_coarse_scan"},
         {&ctx->f_coarse_emit, "tk_k_coarse_emit"},
+        {&ctx->f_layer_classes, "tk_k_layer_classes"},
         {&ctx->f_draw_brush, "tk_k_draw_brush"},
         {&ctx->f_fine, "tk_k_fine"},
         {&ctx->f_fill_coverage, "tk_k_fill_coverage"},
@@ -656,6 +716,16 @@ tk_status tk_frame_create(tk_context *ctx, const tk_scene *s, tk_frame **out) {
     ALLOC(status, 4);
     ALLOC(ptcl, f->ptcl_capacity);
     ALLOC(draw_brush, 2 * std::max<uint64_t>(s->draw_count, 1));
+    if (s->layer_count && s->layers) {
+        std::vector<uint32_t> pair_of, pair_layer;
+        const uint64_t pairs = layer_pairs(s, pair_of, pair_layer);
+        if (pairs * T <= kMaxLayerClassBytes) {
+            ALLOC(layer_pair_of, s->layer_count);
+            ALLOC(pair_layer, pairs);
+            ALLOC(layer_bits, pairs * T);
+            f->pair_capacity = uint32_t(pairs);
+        }
+    }
     ALLOC(image, uint64_t(s->width) * s->height);
     ALLOC(tickets, 4);
     {

@@ -80,6 +80,16 @@ struct tk_args_cumsum {
     uint32_t *chunk_carries;
 };
 
+// Per-tile classes of the distinct (draw, tag) pairs among the layer records, one
+// byte per (tile, pair) in tile-major order: bit 0 the layer rejects the tile,
+// bit 1 it counts as a wrapper, bit 2 it emits BEGIN/END records, bit 3 it is an
+// SDF clip.  Null when the coarse kernels evaluate each layer themselves.
+struct tk_layer_classes {
+    const uint8_t *bits;        // tile_count * pair_count
+    const uint32_t *pair_of;    // per layer record: its pair
+    uint32_t pair_count;
+};
+
 struct tk_args_coarse {
     tk_coarse_config cfg;
     tk_coarse_inputs in;
@@ -88,6 +98,15 @@ struct tk_args_coarse {
     uint32_t *ticket;
     uint32_t epoch;
     uint32_t scan_blocks;
+    tk_layer_classes classes;
+};
+
+struct tk_args_layer_classes {
+    tk_coarse_config cfg;
+    tk_coarse_inputs in;
+    const uint32_t *pair_layer;  // per pair: a layer record with that (draw, tag)
+    uint32_t pair_count;
+    uint8_t *bits;               // tile_count * pair_count, tile-major
 };
 
 // Per-draw brush words for fine: two words per draw, {1, colour} when the draw's
