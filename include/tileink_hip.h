@@ -309,6 +309,18 @@ void tk_frame_destroy(tk_frame *frame);
 /* Enqueues scan -> backdrop prefix -> coarse -> fine on the context stream.
  * Inputs and outputs stay in device memory. */
 tk_status tk_frame_render(tk_frame *frame);
+/* Incremental render of the listed tiles only (row-major tile indices, as the
+ * reference's DamageTiles worklist, src/render/damage_tiles.rs:14-20): the
+ * active_tile_list / incremental dispatch of scan, coarse and fine
+ * (scan/index.hlsli:5-8, coarse/active_tiles.hlsli:6-9, fine/main.hlsl:26-27).
+ * Scan takes only the paths whose tile box meets a listed tile; coarse and fine
+ * take only the listed tiles.  Pixels of other tiles keep the values of earlier
+ * renders, so after tk_frame_update with a scene that differs from the rendered
+ * one only inside the listed tiles the image equals a full render of the new
+ * scene.  Duplicates are ignored; an index >= tiles_width * tiles_height is
+ * TK_ERR_INVALID.  Afterwards the per-(batch, tile) tables read back through
+ * tk_frame_read_buffer hold one column per listed tile, in list order. */
+tk_status tk_frame_render_tiles(tk_frame *frame, const uint32_t *tiles, uint32_t tile_count);
 /* Synchronises and copies the RGBA8 image (premultiplied, row-major, tightly
  * packed little-endian u32, src/native/submission.rs:125-146). */
 tk_status tk_frame_read_image(tk_frame *frame, uint32_t *rgba_out, size_t pixel_count);

@@ -404,6 +404,12 @@ class Frame:
     def render(self):
         check(lib().tk_frame_render(self._h))
 
+    def render_tiles(self, tiles):
+        """Incremental render of the listed tiles (row-major indices); other tiles keep
+        their pixels (tk_frame_render_tiles)."""
+        tiles = np.ascontiguousarray(np.asarray(tiles, dtype=np.uint32).reshape(-1))
+        check(lib().tk_frame_render_tiles(self._h, ptr(tiles) if tiles.size else None, int(tiles.size)))
+
     def update(self, scene):
         """Replaces the scene's arrays; every size must match (tk_frame_update)."""
         check(lib().tk_frame_update(self._h, C.byref(scene)))

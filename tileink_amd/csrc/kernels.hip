@@ -424,10 +424,10 @@ __device__ __forceinline__ uint32_t layer_bits_at(const CoarseCtx &c, uint32_t i
     return layer_class_bits(c, layer);
 }
 
-__device__ __forceinline__ CoarseCtx coarse_ctx(const tk_args_coarse &a, uint32_t tile) {
+__device__ __forceinline__ CoarseCtx coarse_ctx(const tk_args_coarse &a, uint32_t tile, uint32_t column) {
     CoarseCtx c{&a.cfg, &a.in, tile % a.cfg.tiles_width, tile / a.cfg.tiles_width, nullptr, nullptr};
     if (a.classes.bits) {
-        c.class_row = a.classes.bits + (size_t)tile * a.classes.pair_count;
+        c.class_row = a.classes.bits + (size_t)column * a.classes.pair_count;
         c.class_pair_of = a.classes.pair_of;
     }
     return c;
@@ -654,11 +654,32 @@ __device__ __forceinline__ void wave_add_pm1_runs(uint32_t *words, uint32_t inde
     }
 }
 
+// Incremental scan: a path takes part when its tile box meets a damaged tile,
+// by one summed-area-table query over the damage bitmap.
+extern "C" __global__ __launch_bounds__(256) void tk_k_active_paths(tk_args_active_paths a) {
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= a.path_count) return;
+    const tk_path_record &r = a.paths[p];
+    const uint32_t x0 = min(r.tile_x0, a.tiles_width), x1 = min(r.tile_x1, a.tiles_width);
+    const uint32_t y0 = min(r.tile_y0, a.tiles_height), y1 = min(r.tile_y1, a.tiles_height);
+    uint32_t active = 0u;
+    if (x0 < x1 && y0 < y1) {
+        const uint32_t w = a.tiles_width + 1u;
+        const uint32_t *t = a.damage_sat;
+        active = (t[y1 * w + x1] - t[y0 * w + x1] - t[y1 * w + x0] + t[y0 * w + x0]) != 0u ? 1u : 0u;
+    }
+    a.path_active[p] = active;
+}
+
 // scan_count (scan/count.hlsl:13-40).
 extern "C" __global__ __launch_bounds__(256) void tk_k_scan_count(tk_args_scan_lines a) {
     const uint32_t block = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_SCAN);
     uint32_t id = block * 256u + threadIdx.x;
     if (id >= a.cfg.line_count) return;
+    if (a.path_active) {  // incremental: only paths whose tile box meets the damage
+        const uint32_t path = a.lines[id].path_id;
+        if (path < a.path_record_count && a.path_active[path] == 0u) return;
+    }
     ScanTraversal s;
     if (!scan_geometry(a.lines, a.paths, a.path_record_count, id, s)) return;
     uint32_t *backdrops = reinterpret_cast<uint32_t *>(a.backdrops);
@@ -820,6 +841,10 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_emit(tk_args_scan_li
     const uint32_t block = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_SCAN);
     uint32_t id = block * 256u + threadIdx.x;
     if (id >= a.cfg.line_count) return;
+    if (a.path_active) {  // incremental: only paths whose tile box meets the damage
+        const uint32_t path = a.lines[id].path_id;
+        if (path < a.path_record_count && a.path_active[path] == 0u) return;
+    }
     ScanTraversal s;
     if (!scan_geometry(a.lines, a.paths, a.path_record_count, id, s)) return;
     for (uint32_t i = s.imin; i < s.imax; i++) {
@@ -1091,11 +1116,12 @@ __device__ uint32_t stack_wrappers_of_batches(const CoarseCtx &c, uint32_t pendi
 }
 
 // The class table: layer_class_bits of every distinct (draw, tag) pair of the layer
-// records on every tile, one thread per (tile, pair), tile-major.
+// records on every table column's tile, one thread per (column, pair).
 extern "C" __global__ __launch_bounds__(256) void tk_k_layer_classes(tk_args_layer_classes a) {
     const uint64_t id = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    if (id >= (uint64_t)a.cfg.tile_count * a.pair_count) return;
-    const uint32_t tile = (uint32_t)(id / a.pair_count), pair = (uint32_t)(id % a.pair_count);
+    if (id >= (uint64_t)a.list.columns * a.pair_count) return;
+    const uint32_t column = (uint32_t)(id / a.pair_count), pair = (uint32_t)(id % a.pair_count);
+    const uint32_t tile = a.list.tiles ? a.list.tiles[column] : column;
     const CoarseCtx c{&a.cfg, &a.in, tile % a.cfg.tiles_width, tile / a.cfg.tiles_width, nullptr, nullptr};
     a.bits[id] = (uint8_t)layer_class_bits(c, a.in.layers[a.pair_layer[pair]]);
 }
@@ -1109,10 +1135,11 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coar
     const tk_coarse_config &cfg = a.cfg;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t tile = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + wave;
-    if (tile >= cfg.tile_count) return;
-    const CoarseCtx c = coarse_ctx(a, tile);
-    const uint32_t T = cfg.tile_count, Q = cfg.batch_count;
+    const uint32_t column = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + wave;
+    const uint32_t T = a.list.columns, Q = cfg.batch_count;  // T: table columns
+    if (column >= T) return;
+    const uint32_t tile = a.list.tiles ? a.list.tiles[column] : column;
+    const CoarseCtx c = coarse_ctx(a, tile, column);
     uint32_t *counts = a.out.tile_ptcl_counts;
     const tk_tile_draw_record list = a.in.tile_draws[tile];
     const uint32_t len = list.start == kNone ? 0u : list.len;
@@ -1127,9 +1154,9 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coar
         uint32_t prev = (uint32_t)__shfl_up((int)my_batch, 1u, 64);
         if (lane == 0u) prev = next - 1u;  // wraps to ~0u when next == 0: zeroing then starts at batch 0
         if (lane < pending) {
-            for (uint32_t bb = prev + 1u; bb < my_batch; ++bb) counts[(size_t)bb * T + tile] = 0u;
+            for (uint32_t bb = prev + 1u; bb < my_batch; ++bb) counts[(size_t)bb * T + column] = 0u;
             const uint32_t count = (my_particles > 0u && w != kNone) ? my_particles + 2u * w + 1u : 0u;
-            counts[(size_t)my_batch * T + tile] = count;
+            counts[(size_t)my_batch * T + column] = count;
         }
         next = (uint32_t)__shfl((int)my_batch, (int)(pending - 1u), 64) + 1u;
         pending = 0u;
@@ -1168,7 +1195,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coar
     }
     flush();
     resolve();
-    for (uint32_t bb = next + lane; bb < Q; bb += 64u) counts[(size_t)bb * T + tile] = 0u;
+    for (uint32_t bb = next + lane; bb < Q; bb += 64u) counts[(size_t)bb * T + column] = 0u;
 }
 
 // coarse_prefix_chunks + coarse_chunk_offsets + coarse_apply_chunk_offsets for
@@ -1179,7 +1206,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coars
     __shared__ uint32_t s_wave[4];
     constexpr uint32_t kItems = TK_COARSE_SCAN_ITEMS / 256u;  // 16 consecutive counts per lane
     const uint32_t k = draw_ticket(a.ticket, a.scan_blocks, &s_ticket);
-    const uint64_t n = (uint64_t)a.cfg.batch_count * a.cfg.tile_count;
+    const uint64_t n = (uint64_t)a.cfg.batch_count * a.list.columns;
     const uint64_t first = ((uint64_t)k * 256u + threadIdx.x) * kItems;
     uint32_t v[kItems];
     uint32_t sum = 0u;
@@ -1219,8 +1246,8 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coars
     __syncthreads();
     if (first >= n) return;
     uint32_t running = s_base + excl;
-    uint64_t batch = first / a.cfg.tile_count;
-    uint32_t offset = (uint32_t)(first - batch * a.cfg.tile_count);
+    uint64_t batch = first / a.list.columns;
+    uint32_t offset = (uint32_t)(first - batch * a.list.columns);
 #pragma unroll
     for (uint32_t i = 0; i < kItems; ++i) {
         const uint64_t idx = first + i;
@@ -1229,7 +1256,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coars
             if (offset == 0u) a.out.batch_ptcl_bases[batch] = running;
         }
         running += v[i];
-        if (++offset == a.cfg.tile_count) {
+        if (++offset == a.list.columns) {
             offset = 0u;
             ++batch;
         }
@@ -1244,11 +1271,11 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coars
 extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coarse a) {
     const tk_coarse_config &cfg = a.cfg;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t tile =
-        xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + (threadIdx.x >> 6);
-    if (tile >= cfg.tile_count) return;
-    const CoarseCtx c = coarse_ctx(a, tile);
-    const uint32_t T = cfg.tile_count, Q = cfg.batch_count;
+    const uint32_t column = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + (threadIdx.x >> 6);
+    const uint32_t T = a.list.columns, Q = cfg.batch_count;  // T: table columns
+    if (column >= T) return;
+    const uint32_t tile = a.list.tiles ? a.list.tiles[column] : column;
+    const CoarseCtx c = coarse_ctx(a, tile, column);
     Emitter e{a.out.ptcl, cfg.ptcl_capacity, a.out.status};
     uint32_t *kinds = a.out.tile_kinds;
     const tk_tile_draw_record list = a.in.tile_draws[tile];
@@ -1263,7 +1290,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
     uint64_t stack_emitted = 0ull;
     uint32_t stack_end_tag = TK_PTCL_END_CLIP;
     auto empty_until = [&](uint32_t stop) {
-        for (uint32_t bb = next + lane; bb < stop; bb += 64u) kinds[(size_t)bb * T + tile] = TK_TILE_KIND_EMPTY;
+        for (uint32_t bb = next + lane; bb < stop; bb += 64u) kinds[(size_t)bb * T + column] = TK_TILE_KIND_EMPTY;
     };
     auto finish = [&]() {
         if (cur == kNone) return;
@@ -1276,12 +1303,12 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
             if ((flags & 8u) && wrappers == 0u && end - start > 2u && !begin_after) skip = last_opaque;
             kind = classify_tile_flags(flags) | (skip << TK_TILE_PREFIX_SHIFT);
         }
-        if (lane == 0u) kinds[(size_t)cur * T + tile] = kind;
+        if (lane == 0u) kinds[(size_t)cur * T + column] = kind;
         next = cur + 1u;
     };
     auto begin = [&](uint32_t bb) {
         cur = bb;
-        const tk_segment_range r = a.out.tile_ptcl_ranges[(size_t)bb * T + tile];
+        const tk_segment_range r = a.out.tile_ptcl_ranges[(size_t)bb * T + column];
         start = r.start;
         end = r.end;
         active = end > start;
@@ -1393,8 +1420,11 @@ extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     __shared__ tk_segment_range batch_range[256];
     const tk_fine_config &cfg = a.cfg;
     const uint32_t linear = blockIdx.x + blockIdx.y * gridDim.x;
-    if (linear >= xcd_grid(cfg.tile_count, TK_XCD_CHUNK_FINE)) return;
-    const uint32_t tile_ix = xcd_chunked_block(linear, TK_XCD_CHUNK_FINE);
+    const uint32_t columns = a.list.columns;  // table columns: listed tiles, or every tile
+    if (linear >= xcd_grid(columns, TK_XCD_CHUNK_FINE)) return;
+    const uint32_t column = xcd_chunked_block(linear, TK_XCD_CHUNK_FINE);
+    if (column >= columns) return;
+    const uint32_t tile_ix = a.list.tiles ? a.list.tiles[column] : column;
     if (tile_ix >= cfg.tile_count) return;
     const uint32_t lane = threadIdx.x;
     const uint32_t lx = lane & 15u, ly = lane >> 4;
@@ -1420,7 +1450,7 @@ extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
       // This tile's (kind, range) word pairs of up to 256 batches, one load per lane.
       __syncthreads();
       if (b0 + lane < cfg.batch_count) {
-          const size_t idx = (size_t)(b0 + lane) * cfg.tile_count + tile_ix;
+          const size_t idx = (size_t)(b0 + lane) * columns + column;
           batch_kind[lane] = a.in.tile_kinds[idx];
           batch_range[lane] = a.in.tile_ptcl_ranges[idx];
       }

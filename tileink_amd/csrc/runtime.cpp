@@ -65,7 +65,8 @@ struct tk_context {
     hipStream_t stream = nullptr;
     hipFunction_t f_fill = nullptr, f_copy = nullptr, f_scan_clear = nullptr, f_scan_count = nullptr, f_scan_prefix = nullptr, f_scan_chunk_apply = nullptr,
                   f_scan_emit = nullptr, f_cumsum = nullptr, f_coarse_count = nullptr,
-                  f_coarse_scan = nullptr, f_coarse_emit = nullptr, f_layer_classes = nullptr, f_draw_brush = nullptr,
+                  f_coarse_scan = nullptr, f_coarse_emit = nullptr, f_layer_classes = nullptr, f_active_paths = nullptr,
+                  f_draw_brush = nullptr,
                   f_fine = nullptr,
                   f_fill_coverage = nullptr, f_pixel_math = nullptr, f_brush_words = nullptr;
     // Scratch for the stage entry points (frames own their own).
@@ -145,14 +146,16 @@ tk_status fill_u32(tk_context *ctx, void *dst, uint64_t count, uint32_t value) {
 // --- stage launchers shared by the C ABI and the frame pipeline -------------
 
 tk_status run_scan_count(tk_context *ctx, const tk_scan_config &cfg, const tk_line *lines, const tk_path_record *paths,
-                         uint32_t path_record_count, int32_t *backdrops, uint32_t *counts) {
-    tk_args_scan_lines a{cfg, lines, paths, path_record_count, backdrops, counts, nullptr};
+                         uint32_t path_record_count, int32_t *backdrops, uint32_t *counts,
+                         const uint32_t *path_active = nullptr) {
+    tk_args_scan_lines a{cfg, lines, paths, path_record_count, backdrops, counts, nullptr, path_active};
     return launch(ctx, ctx->f_scan_count, xcd_grid(div_up(cfg.line_count, 256), TK_XCD_CHUNK_SCAN), 1, 256, a);
 }
 
 tk_status run_scan_emit(tk_context *ctx, const tk_scan_config &cfg, const tk_line *lines, const tk_path_record *paths,
-                        uint32_t path_record_count, uint32_t *cursors, tk_line_segment *segments) {
-    tk_args_scan_lines a{cfg, lines, paths, path_record_count, nullptr, cursors, segments};
+                        uint32_t path_record_count, uint32_t *cursors, tk_line_segment *segments,
+                        const uint32_t *path_active = nullptr) {
+    tk_args_scan_lines a{cfg, lines, paths, path_record_count, nullptr, cursors, segments, path_active};
     return launch(ctx, ctx->f_scan_emit, xcd_grid(div_up(cfg.line_count, 256), TK_XCD_CHUNK_SCAN), 1, 256, a);
 }
 
@@ -177,10 +180,13 @@ tk_status run_cumsum(tk_context *ctx, const tk_cumsum_config &cfg, const uint32_
     return launch(ctx, ctx->f_cumsum, div_up(cfg.row_count, 256), 1, 256, a);
 }
 
-uint32_t coarse_scan_blocks(const tk_coarse_config &cfg) {
-    uint64_t n = uint64_t(cfg.batch_count) * cfg.tile_count;
+// Scan blocks over the batch x column count table (columns: the listed tiles of an
+// incremental render, else every tile).
+uint32_t coarse_scan_blocks(const tk_coarse_config &cfg, uint32_t columns) {
+    uint64_t n = uint64_t(cfg.batch_count) * columns;
     return uint32_t(std::max<uint64_t>((n + TK_COARSE_SCAN_ITEMS - 1) / TK_COARSE_SCAN_ITEMS, 1));
 }
+uint32_t coarse_scan_blocks(const tk_coarse_config &cfg) { return coarse_scan_blocks(cfg, cfg.tile_count); }
 
 // Where the frame keeps its per-tile layer classes: pair_layer[pair] is a layer
 // record of each distinct (draw, tag) pair, pair_of[layer] its pair, and bits
@@ -193,27 +199,31 @@ struct LayerClassPlan {
 
 tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coarse_inputs &in,
                      const tk_coarse_outputs &out, unsigned long long *status, uint32_t *ticket, uint32_t epoch,
-                     hipEvent_t *marks, const LayerClassPlan &plan = LayerClassPlan{}) {
-    if (cfg.batch_count == 0 || cfg.tile_count == 0) return TK_OK;
-    tk_args_coarse a{cfg, in, out, status, ticket, epoch, coarse_scan_blocks(cfg), tk_layer_classes{nullptr, nullptr, 0u}};
+                     hipEvent_t *marks, const LayerClassPlan &plan = LayerClassPlan{},
+                     tk_tile_list list = tk_tile_list{nullptr, 0u}) {
+    if (!list.tiles) list.columns = cfg.tile_count;
+    if (cfg.batch_count == 0 || list.columns == 0) return TK_OK;
+    tk_args_coarse a{cfg,  in, out, status, ticket, epoch, coarse_scan_blocks(cfg, list.columns),
+                     tk_layer_classes{nullptr, nullptr, 0u}, list};
     if (plan.bits && plan.pair_count) {
-        tk_args_layer_classes b{cfg, in, plan.pair_layer, plan.pair_count, plan.bits};
-        TK_TRY(launch(ctx, ctx->f_layer_classes, uint32_t(div_up(uint64_t(cfg.tile_count) * plan.pair_count, 256)), 1,
+        tk_args_layer_classes b{cfg, in, plan.pair_layer, plan.pair_count, plan.bits, list};
+        TK_TRY(launch(ctx, ctx->f_layer_classes, uint32_t(div_up(uint64_t(list.columns) * plan.pair_count, 256)), 1,
                       256, b));
         a.classes = tk_layer_classes{plan.bits, plan.pair_of, plan.pair_count};
     }
-    TK_TRY(launch(ctx, ctx->f_coarse_count, xcd_grid(div_up(cfg.tile_count, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
+    TK_TRY(launch(ctx, ctx->f_coarse_count, xcd_grid(div_up(list.columns, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
     if (marks) TK_HIP(hipEventRecord(marks[0], ctx->stream));
     TK_TRY(launch(ctx, ctx->f_coarse_scan, a.scan_blocks, 1, 256, a));
     if (marks) TK_HIP(hipEventRecord(marks[1], ctx->stream));
-    TK_TRY(launch(ctx, ctx->f_coarse_emit, xcd_grid(div_up(cfg.tile_count, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
+    TK_TRY(launch(ctx, ctx->f_coarse_emit, xcd_grid(div_up(list.columns, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
     return TK_OK;
 }
 
 // `draw_brush` holds 2 * cfg.draw_count words; it is rebuilt here from the draws
 // and paint before fine reads it.
 tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inputs &in, uint32_t *rgba,
-                   uint32_t *draw_brush) {
+                   uint32_t *draw_brush, tk_tile_list list = tk_tile_list{nullptr, 0u}) {
+    if (!list.tiles) list.columns = cfg.tile_count;
     if (cfg.clip_spill_depth > TK_FINE_MAX_CLIP_SPILL) {
         set_error("clip nesting deeper than %u is not supported", 4u + TK_FINE_MAX_CLIP_SPILL);
         return TK_ERR_INVALID;
@@ -222,8 +232,9 @@ tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inp
         tk_args_draw_brush b{in.draws, cfg.draw_count, in.paint, cfg.paint_words, cfg.paint_brush_base, draw_brush};
         TK_TRY(launch(ctx, ctx->f_draw_brush, div_up(cfg.draw_count, 256), 1, 256, b));
     }
-    tk_args_fine a{cfg, in, rgba, draw_brush};
-    const uint32_t blocks = xcd_grid(cfg.tile_count, TK_XCD_CHUNK_FINE);
+    tk_args_fine a{cfg, in, rgba, draw_brush, list};
+    if (list.columns == 0) return TK_OK;
+    const uint32_t blocks = xcd_grid(list.columns, TK_XCD_CHUNK_FINE);
     uint32_t gx = std::min<uint32_t>(blocks, 65535u);
     uint32_t gy = div_up(blocks, gx);
     return launch(ctx, ctx->f_fine, gx, gy, 256, a, cfg.clip_spill_depth * 256u);
@@ -271,6 +282,9 @@ struct tk_frame {
     uint8_t *layer_bits = nullptr;
     uint32_t pair_capacity = 0, pair_count = 0;
     std::vector<uint32_t> host_pair_of, host_pair_layer;
+    // incremental renders: the listed tiles, the damage summed-area table, per-path flags
+    uint32_t *active_tiles = nullptr, *damage_sat = nullptr, *path_active = nullptr;
+    std::vector<uint32_t> host_active, host_sat;
     // output
     uint32_t *image = nullptr;
     // scratch
@@ -434,9 +448,15 @@ tk_status record(tk_frame *f, int mark) {
     return TK_OK;
 }
 
-tk_status frame_render(tk_frame *f) {
+// A full render, or (active_count != 0) an incremental one over f->active_tiles
+// with f->damage_sat uploaded: only paths whose tile box meets a listed tile are
+// scanned, and coarse and fine run on the listed tiles, whose tables take one
+// column each in list order.  Other tiles of the image keep their pixels.
+tk_status frame_render(tk_frame *f, uint32_t active_count = 0) {
     tk_context *ctx = f->ctx;
     const tk_scene &s = f->meta;
+    const tk_tile_list list{active_count ? f->active_tiles : nullptr, active_count};
+    const uint32_t *path_active = nullptr;
     tk_scan_config scfg = frame_scan_config(f);
     const uint32_t T = s.tiles_width * s.tiles_height;
     f->epoch = (f->epoch + 1u) & 0x3FFFFFFFu;
@@ -447,13 +467,18 @@ tk_status frame_render(tk_frame *f) {
     TK_TRY(fill_u32(ctx, f->backdrops, s.backdrop_len, 0u));
     TK_TRY(fill_u32(ctx, f->seg_counts, s.backdrop_len, 0u));
     TK_TRY(fill_u32(ctx, f->status, 4, 0u));
+    if (active_count && s.path_count) {
+        tk_args_active_paths ap{f->paths, s.path_count, s.tiles_width, s.tiles_height, f->damage_sat, f->path_active};
+        TK_TRY(launch(ctx, ctx->f_active_paths, div_up(s.path_count, 256), 1, 256, ap));
+        path_active = f->path_active;
+    }
     TK_TRY(record(f, 1));
-    TK_TRY(run_scan_count(ctx, scfg, f->lines, f->paths, s.path_count, f->backdrops, f->seg_counts));
+    TK_TRY(run_scan_count(ctx, scfg, f->lines, f->paths, s.path_count, f->backdrops, f->seg_counts, path_active));
     TK_TRY(record(f, 2));
     TK_TRY(run_scan_prefix(ctx, scfg, f->scan_chunks, f->scan_chunk_ranges, f->paths, f->seg_counts, f->seg_ranges,
                            f->cursors, f->bumps, f->chunk_totals, f->chunk_offsets));
     TK_TRY(record(f, 3));
-    TK_TRY(run_scan_emit(ctx, scfg, f->lines, f->paths, s.path_count, f->cursors, f->segments));
+    TK_TRY(run_scan_emit(ctx, scfg, f->lines, f->paths, s.path_count, f->cursors, f->segments, path_active));
     TK_TRY(record(f, 4));
     if (f->capture_raw && s.backdrop_len)
         TK_HIP(hipMemcpyAsync(f->raw_backdrops, f->backdrops, uint64_t(s.backdrop_len) * 4, hipMemcpyDeviceToDevice,
@@ -493,7 +518,7 @@ tk_status frame_render(tk_frame *f) {
             plan.bits = f->layer_bits;
         }
         TK_TRY(run_coarse(ctx, cc, in, out, f->coarse_status, &f->tickets[1], f->epoch, f->timing ? &f->ev[7] : nullptr,
-                          plan));
+                          plan, list));
         if (cc.batch_count == 0 && f->timing) {
             TK_TRY(record(f, 7));
             TK_TRY(record(f, 8));
@@ -512,7 +537,7 @@ tk_status frame_render(tk_frame *f) {
         in.ptcl = f->ptcl;
         in.status = f->status + 2;
         (void)T;
-        TK_TRY(run_fine(ctx, fc, in, f->image, f->draw_brush));
+        TK_TRY(run_fine(ctx, fc, in, f->image, f->draw_brush, list));
     }
     TK_TRY(record(f, 10));
     f->have_times = f->timing;
@@ -567,6 +592,7 @@ tk_status tk_context_create(int device, tk_context **out) {
         {&ctx->f_coarse_scan, "tk_k_coarse_scan"},
         {&ctx->f_coarse_emit, "tk_k_coarse_emit"},
         {&ctx->f_layer_classes, "tk_k_layer_classes"},
+        {&ctx->f_active_paths, "tk_k_active_paths"},
         {&ctx->f_draw_brush, "tk_k_draw_brush"},
         {&ctx->f_fine, "tk_k_fine"},
         {&ctx->f_fill_coverage, "tk_k_fill_coverage"},
@@ -716,6 +742,9 @@ tk_status tk_frame_create(tk_context *ctx, const tk_scene *s, tk_frame **out) {
     ALLOC(status, 4);
     ALLOC(ptcl, f->ptcl_capacity);
     ALLOC(draw_brush, 2 * std::max<uint64_t>(s->draw_count, 1));
+    ALLOC(active_tiles, T);
+    ALLOC(damage_sat, uint64_t(s->tiles_width + 1) * (s->tiles_height + 1));
+    ALLOC(path_active, std::max<uint64_t>(s->path_count, 1));
     if (s->layer_count && s->layers) {
         std::vector<uint32_t> pair_of, pair_layer;
         const uint64_t pairs = layer_pairs(s, pair_of, pair_layer);
@@ -783,6 +812,38 @@ void tk_frame_destroy(tk_frame *f) {
 tk_status tk_frame_render(tk_frame *f) {
     if (!f) return fail(TK_ERR_INVALID, "null frame");
     return frame_render(f);
+}
+
+tk_status tk_frame_render_tiles(tk_frame *f, const uint32_t *tiles, uint32_t tile_count) {
+    if (!f || (tile_count && !tiles)) return fail(TK_ERR_INVALID, "null argument");
+    const tk_scene &s = f->meta;
+    const uint32_t tw = s.tiles_width, th = s.tiles_height, T = tw * th;
+    // The host arrays are reused: wait until earlier uploads from them have completed.
+    TK_HIP(hipStreamSynchronize(f->ctx->stream));
+    std::vector<uint8_t> damaged(T, 0u);
+    f->host_active.clear();
+    for (uint32_t i = 0; i < tile_count; ++i) {
+        const uint32_t t = tiles[i];
+        if (t >= T) {
+            set_error("tile index %u out of range (%u tiles)", t, T);
+            return TK_ERR_INVALID;
+        }
+        if (!damaged[t]) {
+            damaged[t] = 1u;
+            f->host_active.push_back(t);
+        }
+    }
+    const uint32_t active = uint32_t(f->host_active.size());
+    if (active == 0) return TK_OK;
+    const uint32_t w = tw + 1;
+    f->host_sat.assign(uint64_t(w) * (th + 1), 0u);
+    for (uint32_t y = 0; y < th; ++y)
+        for (uint32_t x = 0; x < tw; ++x)
+            f->host_sat[(y + 1) * w + x + 1] =
+                damaged[y * tw + x] + f->host_sat[y * w + x + 1] + f->host_sat[(y + 1) * w + x] - f->host_sat[y * w + x];
+    TK_TRY(upload(f, f->active_tiles, f->host_active.data(), active));
+    TK_TRY(upload(f, f->damage_sat, f->host_sat.data(), f->host_sat.size()));
+    return frame_render(f, active);
 }
 
 tk_status tk_frame_set_capture(tk_frame *f, int enable) {

@@ -53,6 +53,24 @@ struct tk_args_scan_lines {
     int32_t *backdrops;          // count only
     uint32_t *counts_or_cursors; // count: segment counts, emit: cursors
     tk_line_segment *segments;   // emit only
+    const uint32_t *path_active; // incremental: lines of paths whose flag is 0 are skipped; null: all
+};
+
+// Incremental rendering over a list of damaged tiles (the reference's
+// active_tile_list): the per-(batch, tile) tables hold one column per listed
+// tile, in list order, instead of one per tile of the canvas.
+struct tk_tile_list {
+    const uint32_t *tiles;  // column -> tile index; null: column == tile
+    uint32_t columns;       // list length, or tile_count
+};
+
+// Per path: 1 when the path's tile box meets a damaged tile (damage_sat is the
+// summed-area table of the damage bitmap, (tiles_width + 1) x (tiles_height + 1)).
+struct tk_args_active_paths {
+    const tk_path_record *paths;
+    uint32_t path_count, tiles_width, tiles_height;
+    const uint32_t *damage_sat;
+    uint32_t *path_active;
 };
 
 struct tk_args_scan_prefix {
@@ -81,11 +99,11 @@ struct tk_args_cumsum {
 };
 
 // Per-tile classes of the distinct (draw, tag) pairs among the layer records, one
-// byte per (tile, pair) in tile-major order: bit 0 the layer rejects the tile,
+// byte per (table column, pair), column-major: bit 0 the layer rejects the tile,
 // bit 1 it counts as a wrapper, bit 2 it emits BEGIN/END records, bit 3 it is an
 // SDF clip.  Null when the coarse kernels evaluate each layer themselves.
 struct tk_layer_classes {
-    const uint8_t *bits;        // tile_count * pair_count
+    const uint8_t *bits;        // columns * pair_count (one row of pair_count bytes per table column)
     const uint32_t *pair_of;    // per layer record: its pair
     uint32_t pair_count;
 };
@@ -99,6 +117,7 @@ struct tk_args_coarse {
     uint32_t epoch;
     uint32_t scan_blocks;
     tk_layer_classes classes;
+    tk_tile_list list;
 };
 
 struct tk_args_layer_classes {
@@ -106,7 +125,8 @@ struct tk_args_layer_classes {
     tk_coarse_inputs in;
     const uint32_t *pair_layer;  // per pair: a layer record with that (draw, tag)
     uint32_t pair_count;
-    uint8_t *bits;               // tile_count * pair_count, tile-major
+    uint8_t *bits;               // columns * pair_count, column-major
+    tk_tile_list list;
 };
 
 // Per-draw brush words for fine: two words per draw, {1, colour} when the draw's
@@ -124,6 +144,7 @@ struct tk_args_fine {
     tk_fine_inputs in;
     uint32_t *rgba;
     const uint32_t *draw_brush;  // tk_args_draw_brush::table, draw_count entries
+    tk_tile_list list;
 };
 
 struct tk_args_fill_coverage {
