@@ -302,6 +302,7 @@ class Canvas:
         """Compiles and plans the scene; returns the tk_scene view (owned by the canvas)."""
         s = _abi.Scene()
         check(lib().tk_canvas_encode(self._h, int(clear_color), C.byref(s)))
+        s._canvas = self  # the scene points into this canvas's arrays
         self._scene = s
         return s
 
