@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <initializer_list>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -63,6 +64,12 @@ struct tk_context {
     int device = -1;
     hipModule_t module = nullptr;
     hipStream_t stream = nullptr;
+    // The backdrop prefix and coarse run on `side`, overlapping scan_emit on
+    // `stream` (they need the segment ranges and backdrops, not the segments);
+    // fork/join events order the two.  Both streams have the default priority.
+    hipStream_t side = nullptr;
+    hipStream_t cur = nullptr;  // where launches go; null: `stream`
+    hipEvent_t fork_a = nullptr, fork_b = nullptr, join = nullptr;
     hipFunction_t f_fill = nullptr, f_copy = nullptr, f_scan_clear = nullptr, f_scan_count = nullptr, f_scan_prefix = nullptr, f_scan_chunk_apply = nullptr,
                   f_scan_emit = nullptr, f_cumsum = nullptr, f_coarse_count = nullptr,
                   f_coarse_scan = nullptr, f_coarse_emit = nullptr, f_layer_classes = nullptr, f_active_paths = nullptr,
@@ -93,13 +100,22 @@ uint32_t xcd_grid(uint64_t blocks, uint32_t chunk) {
     return uint32_t((blocks + unit - 1) / unit * unit);
 }
 
+hipStream_t launch_stream(const tk_context *ctx) { return ctx->cur ? ctx->cur : ctx->stream; }
+
+// Sends the launches made while it lives to `stream`.
+struct StreamScope {
+    tk_context *ctx;
+    StreamScope(tk_context *c, hipStream_t stream) : ctx(c) { ctx->cur = stream; }
+    ~StreamScope() { ctx->cur = nullptr; }
+};
+
 template <class Args>
 tk_status launch(tk_context *ctx, hipFunction_t f, uint32_t gx, uint32_t gy, uint32_t block, const Args &args,
                  uint32_t dynamic_lds = 0) {
     if (gx == 0 || gy == 0) return TK_OK;
     Args copy = args;
     void *params[] = {&copy};
-    hipError_t e = hipModuleLaunchKernel(f, gx, gy, 1, block, 1, 1, dynamic_lds, ctx->stream, params, nullptr);
+    hipError_t e = hipModuleLaunchKernel(f, gx, gy, 1, block, 1, 1, dynamic_lds, launch_stream(ctx), params, nullptr);
     if (e != hipSuccess) {
         set_error("kernel launch failed: %s", hipGetErrorString(e));
         return TK_ERR_HIP;
@@ -212,9 +228,9 @@ tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coar
         a.classes = tk_layer_classes{plan.bits, plan.pair_of, plan.pair_count};
     }
     TK_TRY(launch(ctx, ctx->f_coarse_count, xcd_grid(div_up(list.columns, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
-    if (marks) TK_HIP(hipEventRecord(marks[0], ctx->stream));
+    if (marks) TK_HIP(hipEventRecord(marks[0], launch_stream(ctx)));
     TK_TRY(launch(ctx, ctx->f_coarse_scan, a.scan_blocks, 1, 256, a));
-    if (marks) TK_HIP(hipEventRecord(marks[1], ctx->stream));
+    if (marks) TK_HIP(hipEventRecord(marks[1], launch_stream(ctx)));
     TK_TRY(launch(ctx, ctx->f_coarse_emit, xcd_grid(div_up(list.columns, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
     return TK_OK;
 }
@@ -293,7 +309,7 @@ struct tk_frame {
     uint32_t epoch = 0;
     bool capture_raw = false;
     bool timing = false;
-    hipEvent_t ev[12] = {};
+    hipEvent_t ev[13] = {};
     bool have_times = false;
 };
 
@@ -444,7 +460,7 @@ tk_fine_config frame_fine_config(const tk_frame *f) {
 }
 
 tk_status record(tk_frame *f, int mark) {
-    if (f->timing) TK_HIP(hipEventRecord(f->ev[mark], f->ctx->stream));
+    if (f->timing) TK_HIP(hipEventRecord(f->ev[mark], launch_stream(f->ctx)));
     return TK_OK;
 }
 
@@ -475,21 +491,30 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0) {
     TK_TRY(record(f, 1));
     TK_TRY(run_scan_count(ctx, scfg, f->lines, f->paths, s.path_count, f->backdrops, f->seg_counts, path_active));
     TK_TRY(record(f, 2));
-    TK_TRY(run_scan_prefix(ctx, scfg, f->scan_chunks, f->scan_chunk_ranges, f->paths, f->seg_counts, f->seg_ranges,
-                           f->cursors, f->bumps, f->chunk_totals, f->chunk_offsets));
-    TK_TRY(record(f, 3));
-    TK_TRY(run_scan_emit(ctx, scfg, f->lines, f->paths, s.path_count, f->cursors, f->segments, path_active));
-    TK_TRY(record(f, 4));
-    if (f->capture_raw && s.backdrop_len)
-        TK_HIP(hipMemcpyAsync(f->raw_backdrops, f->backdrops, uint64_t(s.backdrop_len) * 4, hipMemcpyDeviceToDevice,
-                              ctx->stream));
+    // Fork A: the backdrop prefix needs only scan_count's deltas.
+    TK_HIP(hipEventRecord(ctx->fork_a, ctx->stream));
+    TK_HIP(hipStreamWaitEvent(ctx->side, ctx->fork_a, 0));
     {
+        StreamScope on_side(ctx, ctx->side);
+        TK_TRY(record(f, 5));
+        if (f->capture_raw && s.backdrop_len)
+            TK_HIP(hipMemcpyAsync(f->raw_backdrops, f->backdrops, uint64_t(s.backdrop_len) * 4, hipMemcpyDeviceToDevice,
+                                  ctx->side));
         tk_cumsum_config cc{s.cumsum_row_count, s.cumsum_chunk_count, 0u, 0u};
         TK_TRY(run_cumsum(ctx, cc, f->cs_offsets, f->cs_lens, f->cs_row_starts, f->cs_row_ends, f->backdrops,
                           f->cs_totals, f->cs_carries));
+        TK_TRY(record(f, 6));
     }
-    TK_TRY(record(f, 6));
+    TK_TRY(run_scan_prefix(ctx, scfg, f->scan_chunks, f->scan_chunk_ranges, f->paths, f->seg_counts, f->seg_ranges,
+                           f->cursors, f->bumps, f->chunk_totals, f->chunk_offsets));
+    TK_TRY(record(f, 3));
+    // Fork B: coarse needs the segment ranges (and the backdrops, earlier on the
+    // side stream), not the segments, so it overlaps scan_emit.
+    TK_HIP(hipEventRecord(ctx->fork_b, ctx->stream));
+    TK_HIP(hipStreamWaitEvent(ctx->side, ctx->fork_b, 0));
     {
+        StreamScope on_side(ctx, ctx->side);
+        TK_TRY(record(f, 11));
         tk_coarse_config cc = frame_coarse_config(f);
         // Fields are set by name: the structs are part of the public ABI and may grow.
         tk_coarse_inputs in{};
@@ -523,8 +548,14 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0) {
             TK_TRY(record(f, 7));
             TK_TRY(record(f, 8));
         }
+        TK_TRY(record(f, 9));
+        TK_HIP(hipEventRecord(ctx->join, ctx->side));
     }
-    TK_TRY(record(f, 9));
+    TK_TRY(run_scan_emit(ctx, scfg, f->lines, f->paths, s.path_count, f->cursors, f->segments, path_active));
+    TK_TRY(record(f, 4));
+    // Join: fine reads the segments and the PTCL.
+    TK_HIP(hipStreamWaitEvent(ctx->stream, ctx->join, 0));
+    TK_TRY(record(f, 12));
     {
         tk_fine_config fc = frame_fine_config(f);
         tk_fine_inputs in{};
@@ -572,8 +603,12 @@ tk_status tk_context_create(int device, tk_context **out) {
         tk_context_destroy(ctx);
         return st;
     };
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(TK_ERR_HIP, "stream creation failed"));
+    for (hipEvent_t *e : {&ctx->fork_a, &ctx->fork_b, &ctx->join})
+        if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(TK_ERR_HIP, "event creation failed"));
     if (hipModuleLoadData(&ctx->module, tk_code_object_begin) != hipSuccess)
         return cleanup(fail(TK_ERR_HIP, "loading the gfx950 code object failed"));
     struct {
@@ -613,11 +648,15 @@ tk_status tk_context_create(int device, tk_context **out) {
 
 void tk_context_destroy(tk_context *ctx) {
     if (!ctx) return;
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->status) (void)hipFree(ctx->status);
     if (ctx->tickets) (void)hipFree(ctx->tickets);
     if (ctx->draw_brush) (void)hipFree(ctx->draw_brush);
     if (ctx->module) (void)hipModuleUnload(ctx->module);
+    for (hipEvent_t e : {ctx->fork_a, ctx->fork_b, ctx->join})
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -626,6 +665,7 @@ void *tk_context_stream(tk_context *ctx) { return ctx ? (void *)ctx->stream : nu
 
 tk_status tk_context_synchronize(tk_context *ctx) {
     if (!ctx) return fail(TK_ERR_INVALID, "null context");
+    TK_HIP(hipStreamSynchronize(ctx->side));
     TK_HIP(hipStreamSynchronize(ctx->stream));
     return TK_OK;
 }
@@ -802,6 +842,7 @@ tk_status tk_frame_update(tk_frame *f, const tk_scene *s) {
 
 void tk_frame_destroy(tk_frame *f) {
     if (!f) return;
+    if (f->ctx && f->ctx->side) (void)hipStreamSynchronize(f->ctx->side);
     if (f->ctx && f->ctx->stream) (void)hipStreamSynchronize(f->ctx->stream);
     for (void *p : f->allocations) (void)hipFree(p);
     for (auto &e : f->ev)
@@ -863,20 +904,21 @@ tk_status tk_frame_stage_times(tk_frame *f, tk_stage_times *out) {
     if (!f || !out) return fail(TK_ERR_INVALID, "null argument");
     if (!f->have_times) return fail(TK_ERR_INVALID, "no timed render recorded");
     TK_HIP(hipEventSynchronize(f->ev[10]));
-    float t[11] = {};
-    for (int i = 1; i <= 10; ++i) {
-        if (i == 5) continue;  // event 5 is not recorded
-        TK_HIP(hipEventElapsedTime(&t[i], f->ev[i == 6 ? 4 : i - 1], f->ev[i]));
-    }
-    out->scan_clear = t[1];
-    out->scan_count = t[2];
-    out->scan_prefix = t[3];
-    out->scan_emit = t[4];
-    out->cumsum = t[6];
-    out->coarse_count = t[7];
-    out->coarse_prefix = t[8];
-    out->coarse_emit = t[9];
-    out->fine = t[10];
+    // Each stage from its own start event; the side-stream stages (cumsum, coarse)
+    // overlap scan_prefix and scan_emit, so the stages add up to more than total.
+    auto span = [&](int from, int to, float &ms) -> tk_status {
+        TK_HIP(hipEventElapsedTime(&ms, f->ev[from], f->ev[to]));
+        return TK_OK;
+    };
+    TK_TRY(span(0, 1, out->scan_clear));
+    TK_TRY(span(1, 2, out->scan_count));
+    TK_TRY(span(2, 3, out->scan_prefix));
+    TK_TRY(span(3, 4, out->scan_emit));
+    TK_TRY(span(5, 6, out->cumsum));
+    TK_TRY(span(11, 7, out->coarse_count));
+    TK_TRY(span(7, 8, out->coarse_prefix));
+    TK_TRY(span(8, 9, out->coarse_emit));
+    TK_TRY(span(12, 10, out->fine));
     float total = 0;
     TK_HIP(hipEventElapsedTime(&total, f->ev[0], f->ev[10]));
     out->total = total;
