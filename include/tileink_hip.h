@@ -321,6 +321,13 @@ tk_status tk_frame_render(tk_frame *frame);
  * TK_ERR_INVALID.  Afterwards the per-(batch, tile) tables read back through
  * tk_frame_read_buffer hold one column per listed tile, in list order. */
 tk_status tk_frame_render_tiles(tk_frame *frame, const uint32_t *tiles, uint32_t tile_count);
+/* enable != 0: tk_frame_render captures its launch sequence (both streams) into a
+ * hipGraph on first use and replays it as one launch afterwards; renders with
+ * timing or raw-backdrop capture enabled, and tk_frame_render_tiles, still launch
+ * kernel by kernel.  tk_frame_update drops the graph (it is captured again).  No
+ * counterpart in the reference: its passes are recorded into one command buffer
+ * per frame (src/native/runtime/compute.rs:33-331). */
+tk_status tk_frame_set_graph(tk_frame *frame, int enable);
 /* Synchronises and copies the RGBA8 image (premultiplied, row-major, tightly
  * packed little-endian u32, src/native/submission.rs:125-146). */
 tk_status tk_frame_read_image(tk_frame *frame, uint32_t *rgba_out, size_t pixel_count);

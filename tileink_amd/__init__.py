@@ -416,6 +416,10 @@ class Frame:
         check(lib().tk_frame_update(self._h, C.byref(scene)))
         self._scene = scene
 
+    def set_graph(self, enable=True):
+        """Replay full renders from a captured hipGraph (tk_frame_set_graph)."""
+        check(lib().tk_frame_set_graph(self._h, int(bool(enable))))
+
     def set_capture(self, enable=True):
         check(lib().tk_frame_set_capture(self._h, int(bool(enable))))
 

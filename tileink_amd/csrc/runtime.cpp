@@ -309,6 +309,10 @@ struct tk_frame {
     uint32_t epoch = 0;
     bool capture_raw = false;
     bool timing = false;
+    // hipGraph replay of a full render (tk_frame_set_graph)
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
     hipEvent_t ev[13] = {};
     bool have_times = false;
 };
@@ -483,6 +487,10 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0) {
     TK_TRY(fill_u32(ctx, f->backdrops, s.backdrop_len, 0u));
     TK_TRY(fill_u32(ctx, f->seg_counts, s.backdrop_len, 0u));
     TK_TRY(fill_u32(ctx, f->status, 4, 0u));
+    // The coarse scan's lookback words start each frame at zero, so a replayed
+    // graph (whose epoch argument is fixed) never sees the previous frame's flags.
+    TK_TRY(fill_u32(ctx, f->coarse_status,
+                    2ull * coarse_scan_blocks(frame_coarse_config(f), active_count ? active_count : T), 0u));
     if (active_count && s.path_count) {
         tk_args_active_paths ap{f->paths, s.path_count, s.tiles_width, s.tiles_height, f->damage_sat, f->path_active};
         TK_TRY(launch(ctx, ctx->f_active_paths, div_up(s.path_count, 256), 1, 256, ap));
@@ -572,6 +580,40 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0) {
     }
     TK_TRY(record(f, 10));
     f->have_times = f->timing;
+    return TK_OK;
+}
+
+void drop_graph(tk_frame *f) {
+    if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
+    if (f->graph) (void)hipGraphDestroy(f->graph);
+    f->graph_exec = nullptr;
+    f->graph = nullptr;
+}
+
+// A full render through the frame's hipGraph: captured from the two streams on
+// first use (fill kernels only, no memset nodes), then replayed as one launch.
+tk_status frame_render_graph(tk_frame *f) {
+    tk_context *ctx = f->ctx;
+    if (!f->graph_exec) {
+        TK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const tk_status st = frame_render(f);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+        if (st != TK_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        if (e != hipSuccess) {
+            set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+            return TK_ERR_HIP;
+        }
+        f->graph = graph;
+        if (hipGraphInstantiate(&f->graph_exec, f->graph, nullptr, nullptr, 0) != hipSuccess) {
+            drop_graph(f);
+            return fail(TK_ERR_HIP, "hipGraphInstantiate failed");
+        }
+    }
+    TK_HIP(hipGraphLaunch(f->graph_exec, ctx->stream));
     return TK_OK;
 }
 
@@ -835,6 +877,10 @@ tk_status tk_frame_update(tk_frame *f, const tk_scene *s) {
     TK_TRY(validate_scene(s));
     if (!same_sizes(*s, f->meta)) return fail(TK_ERR_INVALID, "scene sizes differ from the frame's");
     uint32_t clear = s->clear_color;
+    if (f->graph_exec) {  // the captured arguments (layer pair count, clear colour) may change
+        TK_HIP(hipStreamSynchronize(f->ctx->stream));
+        drop_graph(f);
+    }
     TK_TRY(frame_upload(f, s));
     f->meta.clear_color = clear;
     return TK_OK;
@@ -844,6 +890,7 @@ void tk_frame_destroy(tk_frame *f) {
     if (!f) return;
     if (f->ctx && f->ctx->side) (void)hipStreamSynchronize(f->ctx->side);
     if (f->ctx && f->ctx->stream) (void)hipStreamSynchronize(f->ctx->stream);
+    drop_graph(f);
     for (void *p : f->allocations) (void)hipFree(p);
     for (auto &e : f->ev)
         if (e) (void)hipEventDestroy(e);
@@ -852,7 +899,18 @@ void tk_frame_destroy(tk_frame *f) {
 
 tk_status tk_frame_render(tk_frame *f) {
     if (!f) return fail(TK_ERR_INVALID, "null frame");
+    if (f->use_graph && !f->timing && !f->capture_raw) return frame_render_graph(f);
     return frame_render(f);
+}
+
+tk_status tk_frame_set_graph(tk_frame *f, int enable) {
+    if (!f) return fail(TK_ERR_INVALID, "null frame");
+    f->use_graph = enable != 0;
+    if (!f->use_graph) {
+        TK_HIP(hipStreamSynchronize(f->ctx->stream));
+        drop_graph(f);
+    }
+    return TK_OK;
 }
 
 tk_status tk_frame_render_tiles(tk_frame *f, const uint32_t *tiles, uint32_t tile_count) {
