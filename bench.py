@@ -115,6 +115,35 @@ def incremental_render(frame, tiles_w, tiles_h, steps, side=32):
             "stage_ms": {k: round(v, 4) for k, v in total.items()}}
 
 
+def scene_loop(ctx, count=64, repeats=5):
+    """Many independent scenes rendered one after another (the per-scene loop):
+    sizes from the independent-scene distribution (side log-uniform 64-2048 px,
+    aspect 0.5-2) with path counts lognormal(ln 150, 0.8) clipped to 10-2000 so the
+    scenes generate in seconds.  Each scene is its own frame; the loop is timed with
+    kernel-by-kernel launches and with each frame replayed from its hipGraph."""
+    sizes = scenes.config3_sizes(count, path_mu=np.log(150.0), path_sigma=0.8, max_paths=2000)
+    canvases = [scenes.config3_scene(w, h, n, i) for i, (w, h, n) in enumerate(sizes)]
+    frames = [tk.Frame(ctx, c.encode(0)) for c in canvases]
+    out = {"scenes": count, "paths": int(sum(n for _, _, n in sizes)),
+           "pixels": int(sum(w * h for w, h, _ in sizes))}
+    for mode in ("eager", "graph"):
+        for f in frames:
+            f.set_graph(mode == "graph")
+            f.render()  # warm-up (captures the graph)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            for f in frames:
+                f.render()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) / repeats * 1e3
+        out[f"{mode}_ms"] = round(ms, 3)
+        out[f"{mode}_scenes_per_s"] = round(count / (ms * 1e-3), 1)
+    out["graph_speedup"] = round(out["eager_ms"] / out["graph_ms"], 3)
+    del frames
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +153,7 @@ def main():
     ap.add_argument("--paths", type=int, default=200_000)
     ap.add_argument("--clips", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-scene-loop", action="store_true", help="skip the many-small-scenes loop")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device three times with tk_k_copy (counter calibration)")
     args = ap.parse_args()
@@ -185,6 +215,7 @@ def main():
     fine_bytes = fine_algorithmic_bytes(walked_records, used_segments, scene)
     fine_gbs = fine_bytes / (stage_avg["fine"] * 1e-3) / 1e9
     incremental = incremental_render(frame, W // 16, H // 16, args.steps)
+    loop = scene_loop(ctx) if not args.no_scene_loop else None
     result = {
         "metric": "ms/frame & Gpix/s at 4096², 200k paths; batched fwd+bwd scenes/s vs plain",
         "value": round(gpix, 4),
@@ -207,6 +238,7 @@ def main():
             "wall_ms_per_step": round(wall / args.steps * 1e3, 4),
             "stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
             "incremental": incremental,
+            "scene_loop": loop,
         },
         "roofline": {"bound": "hbm", "kernel": "tk_k_fine", "achieved": round(fine_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(fine_gbs / HBM_PEAK_GBS, 5), "traffic": measured_traffic("tk_k_fine"),

@@ -113,3 +113,25 @@ def config2_small(scale=8, paths=None):
     side = 4096 // scale
     n = paths if paths is not None else max(64, 200_000 // (scale * scale))
     return config2(side, side, n, clips=min(32, max(2, n // 64)), workload=20 + scale)
+
+
+def config3_sizes(count, seed=3, path_mu=math.log(1500.0), path_sigma=1.2, max_paths=20_000):
+    """(width, height, paths) of `count` independent scenes: side log-uniform in
+    [64, 2048] px, aspect ratio uniform in [0.5, 2], paths lognormal(path_mu,
+    path_sigma) clipped to [10, max_paths]."""
+    rng = _rng(seed)
+    out = []
+    for _ in range(count):
+        side = math.exp(rng.uniform(math.log(64.0), math.log(2048.0)))
+        aspect = rng.uniform(0.5, 2.0)
+        w = int(min(max(round(side * math.sqrt(aspect)), 16), 4096))
+        h = int(min(max(round(side / math.sqrt(aspect)), 16), 4096))
+        n = int(min(max(round(rng.lognormal(path_mu, path_sigma)), 10), max_paths))
+        out.append((w, h, n))
+    return out
+
+
+def config3_scene(width, height, paths, index):
+    """One scene of the independent-scene workload: the config-2 generator at this
+    size, with clip nesting scaled to the path count."""
+    return config2(width, height, paths, clips=min(32, max(1, paths // 256)), workload=1000 + index)
