@@ -328,6 +328,23 @@ tk_status tk_frame_render_tiles(tk_frame *frame, const uint32_t *tiles, uint32_t
  * counterpart in the reference: its passes are recorded into one command buffer
  * per frame (src/native/runtime/compute.rs:33-331). */
 tk_status tk_frame_set_graph(tk_frame *frame, int enable);
+
+/* A batch of independent scenes rendered as one frame: the scenes (each with its
+ * own canvas size, clear colour, draws, clip layers and tile bins) are laid end
+ * to end and go through one launch sequence of scan, coarse and fine, whose
+ * coarse and fine kernels read each tile's canvas from a per-scene slot.  Each
+ * scene's image equals its own tk_frame_render's.  The batched form of the
+ * pipeline (SURVEY config 3); no counterpart in the reference, which renders one
+ * canvas per submission. */
+typedef struct tk_batch tk_batch;
+tk_status tk_batch_create(tk_context *ctx, const tk_scene *scenes, uint32_t scene_count, tk_batch **out);
+tk_status tk_batch_render(tk_batch *batch);
+/* Copies scene `scene`'s RGBA8 image (its width x height pixels, row-major). */
+tk_status tk_batch_read_image(tk_batch *batch, uint32_t scene, uint32_t *rgba_out, size_t pixel_count);
+/* The frame behind the batch: stats, stage times, timing and graph switches.
+ * tk_frame_update and tk_frame_render_tiles do not apply to it. */
+tk_frame *tk_batch_frame(tk_batch *batch);
+void tk_batch_destroy(tk_batch *batch);
 /* Synchronises and copies the RGBA8 image (premultiplied, row-major, tightly
  * packed little-endian u32, src/native/submission.rs:125-146). */
 tk_status tk_frame_read_image(tk_frame *frame, uint32_t *rgba_out, size_t pixel_count);

@@ -20,7 +20,7 @@ from ._abi import (BRUSH_FOUR_CORNER, BRUSH_LINEAR, BRUSH_RADIAL, BRUSH_SOLID, B
                    SEGMENT_DTYPE, TileinkError, check, lib, ptr)
 
 __all__ = [
-    "BezPath", "Canvas", "Color", "LinearGradient", "RadialGradient", "SweepGradient", "FourCornerGradient",
+    "Batch", "BezPath", "Canvas", "Color", "LinearGradient", "RadialGradient", "SweepGradient", "FourCornerGradient",
     "NativeBackend", "NativeContext", "NativeRenderer", "Frame",
     "SceneArrays", "TileinkError", "FILL_NONZERO", "FILL_EVEN_ODD",
 ]
@@ -398,7 +398,7 @@ class Frame:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        if h is not None and h.value and not getattr(self, "_borrowed", False):
             lib().tk_frame_destroy(h)
             self._h = None
 
@@ -444,6 +444,45 @@ class Frame:
     def buffer(self, which, dtype, count):
         out = np.empty(count, dtype=dtype)
         check(lib().tk_frame_read_buffer(self._h, int(which), ptr(out), out.nbytes))
+        return out
+
+
+class Batch:
+    """Independent scenes rendered as one frame (tk_batch_*): `scenes` are encoded
+    tk_scene views (Canvas.encode / SceneArrays.to_scene), kept alive by the batch."""
+
+    def __init__(self, context, scenes):
+        self.context = context
+        self._scenes = list(scenes)
+        arr = (_abi.Scene * len(self._scenes))(*self._scenes)
+        h = C.c_void_p()
+        check(lib().tk_batch_create(context.handle, arr, len(self._scenes), C.byref(h)))
+        self._h = h
+        self.sizes = [(sc.width, sc.height) for sc in self._scenes]
+        frame = Frame.__new__(Frame)
+        frame._borrowed = True
+        frame._h = C.c_void_p(lib().tk_batch_frame(h))
+        frame.context, frame._scene = context, None
+        self.frame = frame  # stats, stage times, timing and graph switches
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().tk_batch_destroy(h)
+            self._h = None
+
+    def __len__(self):
+        return len(self._scenes)
+
+    def render(self):
+        check(lib().tk_batch_render(self._h))
+
+    def image(self, index):
+        if not 0 <= index < len(self.sizes):
+            raise TileinkError(_abi.TK_ERR_INVALID, f"scene index {index} out of range ({len(self.sizes)} scenes)")
+        w, h = self.sizes[index]
+        out = np.empty((h, w), dtype=np.uint32)
+        check(lib().tk_batch_read_image(self._h, int(index), ptr(out), out.size))
         return out
 
 

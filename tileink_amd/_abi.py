@@ -190,6 +190,11 @@ SIGNATURES = {
     "tk_frame_render": (C.c_int, [_P]),
     "tk_frame_render_tiles": (C.c_int, [_P, _P, _U32]),
     "tk_frame_set_graph": (C.c_int, [_P, C.c_int]),
+    "tk_batch_create": (C.c_int, [_P, _P, _U32, _P]),
+    "tk_batch_render": (C.c_int, [_P]),
+    "tk_batch_read_image": (C.c_int, [_P, _U32, _P, _SZ]),
+    "tk_batch_frame": (_P, [_P]),
+    "tk_batch_destroy": (None, [_P]),
     "tk_frame_read_image": (C.c_int, [_P, _P, _SZ]),
     "tk_frame_image_device": (_P, [_P]),
     "tk_frame_read_buffer": (C.c_int, [_P, C.c_int, _P, _SZ]),

@@ -318,7 +318,39 @@ struct CoarseCtx {
     uint32_t tx, ty;
     const uint8_t *class_row;      // this tile's layer-pair classes, or null
     const uint32_t *class_pair_of; // per layer record: its pair
+    uint32_t tw, th;               // the tile grid of this tile's canvas
 };
+
+// Where a table column lives: its canvas's tile grid, its batches [q0, q1), and
+// its table entries base + (batch - q0) * cols + local.  One scene: the whole
+// table, batch-major over `columns` columns.
+struct SceneView {
+    uint32_t tw, th, width, height, clear, q0, q1, cols, local;
+    uint64_t base, image;
+};
+__device__ __forceinline__ SceneView scene_view(const tk_scene_slots &sl, uint32_t column, uint32_t columns,
+                                                uint32_t batch_count, uint32_t tw, uint32_t th, uint32_t width,
+                                                uint32_t height, uint32_t clear) {
+    SceneView v{tw, th, width, height, clear, 0u, batch_count, columns, column, 0ull, 0ull};
+    if (sl.slots) {
+        const tk_scene_slot &s = sl.slots[sl.scene_of_column[column]];
+        v.tw = s.tiles_width;
+        v.th = s.tiles_height;
+        v.width = s.width;
+        v.height = s.height;
+        v.clear = s.clear_color;
+        v.q0 = s.batch_start;
+        v.q1 = s.batch_end;
+        v.cols = s.tiles_width * s.tiles_height;
+        v.local = column - s.column_start;
+        v.base = s.table_base;
+        v.image = s.image_offset;
+    }
+    return v;
+}
+__device__ __forceinline__ size_t table_index(const SceneView &v, uint32_t batch) {
+    return (size_t)(v.base + (uint64_t)(batch - v.q0) * v.cols + v.local);
+}
 
 struct Particle {
     bool valid;
@@ -350,7 +382,7 @@ __device__ Particle draw_particle(const CoarseCtx &c, uint32_t batch, uint32_t d
         }
         return p;
     }
-    uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
+    uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, c.tw, c.th);
     if (bd == kNone || (d.tag != TK_DRAW_BRUSH && d.tag != TK_DRAW_PATH_GLYPH && d.tag != TK_DRAW_CLIP)) return p;
     tk_segment_range range = in.segment_ranges[bd];
     uint32_t winding = (uint32_t)in.backdrops[bd];
@@ -384,7 +416,7 @@ __device__ __forceinline__ uint32_t particle_class_flags(const CoarseCtx &c, uin
 __device__ bool noop_stack_clip(const CoarseCtx &c, const tk_layer_record &layer, const DrawData &d) {
     if (layer.tag != TK_LAYER_CLIP) return false;
     if (draw_has_sdf(d)) return sdf_clip_covers(c.in->paint, c.cfg->paint_words, d, c.tx, c.ty);
-    uint32_t bd = draw_backdrop(c.in->paths, c.cfg->path_count, d, c.tx, c.ty, c.cfg->tiles_width, c.cfg->tiles_height);
+    uint32_t bd = draw_backdrop(c.in->paths, c.cfg->path_count, d, c.tx, c.ty, c.tw, c.th);
     if (bd == kNone) return false;  // excluded by stack_wrapper_count before any emission
     tk_segment_range range = c.in->segment_ranges[bd];
     uint32_t winding = (uint32_t)c.in->backdrops[bd];
@@ -403,11 +435,11 @@ __device__ uint32_t layer_class_bits(const CoarseCtx &c, const tk_layer_record &
     uint32_t bits = 0u;
     if (draw_has_sdf(d)) {
         const bool covers = sdf_clip_covers(in.paint, cfg.paint_words, d, c.tx, c.ty);
-        if (!covers) bits |= draw_hits_tile(d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height) ? kLayerCounted : kLayerInvalid;
+        if (!covers) bits |= draw_hits_tile(d, c.tx, c.ty, c.tw, c.th) ? kLayerCounted : kLayerInvalid;
         if (!(layer.tag == TK_LAYER_CLIP && covers)) bits |= kLayerEmits;
         return bits | kLayerSdf;
     }
-    const uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
+    const uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, c.tw, c.th);
     if (bd == kNone) return kLayerInvalid;
     const tk_segment_range range = in.segment_ranges[bd];
     const uint32_t winding = (uint32_t)in.backdrops[bd];
@@ -424,8 +456,10 @@ __device__ __forceinline__ uint32_t layer_bits_at(const CoarseCtx &c, uint32_t i
     return layer_class_bits(c, layer);
 }
 
-__device__ __forceinline__ CoarseCtx coarse_ctx(const tk_args_coarse &a, uint32_t tile, uint32_t column) {
-    CoarseCtx c{&a.cfg, &a.in, tile % a.cfg.tiles_width, tile / a.cfg.tiles_width, nullptr, nullptr};
+// The coarse view of table column `column`, whose tile is `tile` of its canvas.
+__device__ __forceinline__ CoarseCtx coarse_ctx(const tk_args_coarse &a, const SceneView &v, uint32_t tile,
+                                                uint32_t column) {
+    CoarseCtx c{&a.cfg, &a.in, tile % v.tw, tile / v.tw, nullptr, nullptr, v.tw, v.th};
     if (a.classes.bits) {
         c.class_row = a.classes.bits + (size_t)column * a.classes.pair_count;
         c.class_pair_of = a.classes.pair_of;
@@ -904,11 +938,11 @@ __device__ uint32_t stack_wrapper_count_wave(const CoarseCtx &c, uint32_t batch,
             const DrawData d = load_draw(in.draws, cfg.draw_count, layer.draw);
             if (draw_has_sdf(d)) {
                 if (!sdf_clip_covers(in.paint, cfg.paint_words, d, c.tx, c.ty)) {
-                    if (!draw_hits_tile(d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height)) invalid = true;
+                    if (!draw_hits_tile(d, c.tx, c.ty, c.tw, c.th)) invalid = true;
                     else counted = true;
                 }
             } else {
-                const uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
+                const uint32_t bd = draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, c.tw, c.th);
                 if (bd == kNone) {
                     invalid = true;
                 } else {
@@ -948,8 +982,7 @@ __device__ uint32_t emit_stack_begins_wave(const CoarseCtx &c, uint32_t batch, E
             if (draw_has_sdf(d)) {
                 e.store(slot, TK_PTCL_BEGIN_SDF_CLIP, 0u, 0u, 0u, 0u, layer.draw);
             } else {
-                const uint32_t bd = draw_backdrop(c.in->paths, c.cfg->path_count, d, c.tx, c.ty, c.cfg->tiles_width,
-                                                  c.cfg->tiles_height);
+                const uint32_t bd = draw_backdrop(c.in->paths, c.cfg->path_count, d, c.tx, c.ty, c.tw, c.th);
                 const uint32_t tag =
                     layer.tag == 1u ? kPtclBeginOpacity : (layer.tag == 2u ? kPtclBeginBlend : TK_PTCL_BEGIN_CLIP);
                 tk_segment_range range{0u, 0u};
@@ -1002,7 +1035,7 @@ __device__ uint32_t stack_begins_single_pass(const CoarseCtx &c, uint32_t batch,
                 const DrawData d = load_draw(in.draws, cfg.draw_count, layer.draw);
                 fill_rule = d.fill_rule;
                 const uint32_t bd =
-                    draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, cfg.tiles_width, cfg.tiles_height);
+                    draw_backdrop(in.paths, cfg.path_count, d, c.tx, c.ty, c.tw, c.th);
                 if (bd != kNone) {
                     range = in.segment_ranges[bd];
                     winding = (uint32_t)in.backdrops[bd];
@@ -1057,7 +1090,7 @@ __device__ uint32_t emit_stack_ends_wave(const CoarseCtx &c, uint32_t batch, Emi
 
 // One chunk of up to 64 list entries: lane's draw and batch (kNone when absent).
 __device__ __forceinline__ void load_list_entry(const tk_args_coarse &a, const tk_tile_draw_record &list, uint32_t base,
-                                                uint32_t lane, uint32_t &draw, uint32_t &batch) {
+                                                uint32_t lane, const SceneView &v, uint32_t &draw, uint32_t &batch) {
     draw = kNone;
     batch = kNone;
     const uint32_t i = base + lane;
@@ -1066,7 +1099,7 @@ __device__ __forceinline__ void load_list_entry(const tk_args_coarse &a, const t
         if (slot < a.cfg.tile_draw_index_count) {
             draw = a.in.tile_draw_indices[slot];
             if (draw < a.cfg.draw_count) batch = a.in.draw_batch_ids[draw];
-            if (batch >= a.cfg.batch_count) batch = kNone;
+            if (batch < v.q0 || batch >= v.q1) batch = kNone;  // outside this canvas's batches
         }
     }
 }
@@ -1122,7 +1155,8 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_layer_classes(tk_args_lay
     if (id >= (uint64_t)a.list.columns * a.pair_count) return;
     const uint32_t column = (uint32_t)(id / a.pair_count), pair = (uint32_t)(id % a.pair_count);
     const uint32_t tile = a.list.tiles ? a.list.tiles[column] : column;
-    const CoarseCtx c{&a.cfg, &a.in, tile % a.cfg.tiles_width, tile / a.cfg.tiles_width, nullptr, nullptr};
+    const CoarseCtx c{&a.cfg,   &a.in,   tile % a.cfg.tiles_width, tile / a.cfg.tiles_width, nullptr, nullptr,
+                      a.cfg.tiles_width, a.cfg.tiles_height};
     a.bits[id] = (uint8_t)layer_class_bits(c, a.in.layers[a.pair_layer[pair]]);
 }
 
@@ -1136,15 +1170,17 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coar
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t column = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + wave;
-    const uint32_t T = a.list.columns, Q = cfg.batch_count;  // T: table columns
-    if (column >= T) return;
-    const uint32_t tile = a.list.tiles ? a.list.tiles[column] : column;
-    const CoarseCtx c = coarse_ctx(a, tile, column);
+    if (column >= a.list.columns) return;
+    const SceneView v = scene_view(a.scenes, column, a.list.columns, cfg.batch_count, cfg.tiles_width, cfg.tiles_height,
+                                   0u, 0u, 0u);
+    const uint32_t tile = a.list.tiles ? a.list.tiles[column] : v.local;
+    const CoarseCtx c = coarse_ctx(a, v, tile, column);
     uint32_t *counts = a.out.tile_ptcl_counts;
-    const tk_tile_draw_record list = a.in.tile_draws[tile];
+    const tk_tile_draw_record list = a.in.tile_draws[a.scenes.slots ? column : tile];
     const uint32_t len = list.start == kNone ? 0u : list.len;
+    const uint32_t Q = v.q1;
 
-    uint32_t next = 0u;                     // first batch whose count is not written yet
+    uint32_t next = v.q0;                   // first batch whose count is not written yet
     uint32_t pending = 0u;                  // batches held, one per lane
     uint32_t my_batch = 0u, my_particles = 0u;
     auto resolve = [&]() {
@@ -1154,9 +1190,9 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coar
         uint32_t prev = (uint32_t)__shfl_up((int)my_batch, 1u, 64);
         if (lane == 0u) prev = next - 1u;  // wraps to ~0u when next == 0: zeroing then starts at batch 0
         if (lane < pending) {
-            for (uint32_t bb = prev + 1u; bb < my_batch; ++bb) counts[(size_t)bb * T + column] = 0u;
+            for (uint32_t bb = prev + 1u; bb < my_batch; ++bb) counts[table_index(v, bb)] = 0u;
             const uint32_t count = (my_particles > 0u && w != kNone) ? my_particles + 2u * w + 1u : 0u;
-            counts[(size_t)my_batch * T + column] = count;
+            counts[table_index(v, my_batch)] = count;
         }
         next = (uint32_t)__shfl((int)my_batch, (int)(pending - 1u), 64) + 1u;
         pending = 0u;
@@ -1172,7 +1208,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coar
     };
     for (uint32_t base = 0; base < len; base += 64u) {
         uint32_t d, b;
-        load_list_entry(a, list, base, lane, d, b);
+        load_list_entry(a, list, base, lane, v, d, b);
         const bool valid = b != kNone && draw_particle(c, b, d).valid;
         uint64_t pending_runs = __ballot(b != kNone);
         const uint64_t valid_mask = __ballot(valid);
@@ -1195,7 +1231,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coar
     }
     flush();
     resolve();
-    for (uint32_t bb = next + lane; bb < Q; bb += 64u) counts[(size_t)bb * T + column] = 0u;
+    for (uint32_t bb = next + lane; bb < Q; bb += 64u) counts[table_index(v, bb)] = 0u;
 }
 
 // coarse_prefix_chunks + coarse_chunk_offsets + coarse_apply_chunk_offsets for
@@ -1206,7 +1242,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coars
     __shared__ uint32_t s_wave[4];
     constexpr uint32_t kItems = TK_COARSE_SCAN_ITEMS / 256u;  // 16 consecutive counts per lane
     const uint32_t k = draw_ticket(a.ticket, a.scan_blocks, &s_ticket);
-    const uint64_t n = (uint64_t)a.cfg.batch_count * a.list.columns;
+    const uint64_t n = a.scenes.slots ? a.scenes.table_entries : (uint64_t)a.cfg.batch_count * a.list.columns;
     const uint64_t first = ((uint64_t)k * 256u + threadIdx.x) * kItems;
     uint32_t v[kItems];
     uint32_t sum = 0u;
@@ -1253,7 +1289,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coars
         const uint64_t idx = first + i;
         if (idx < n) {
             a.out.tile_ptcl_ranges[idx] = tk_segment_range{running, running + v[i]};
-            if (offset == 0u) a.out.batch_ptcl_bases[batch] = running;
+            if (offset == 0u && !a.scenes.slots) a.out.batch_ptcl_bases[batch] = running;
         }
         running += v[i];
         if (++offset == a.list.columns) {
@@ -1272,17 +1308,19 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
     const tk_coarse_config &cfg = a.cfg;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t column = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + (threadIdx.x >> 6);
-    const uint32_t T = a.list.columns, Q = cfg.batch_count;  // T: table columns
-    if (column >= T) return;
-    const uint32_t tile = a.list.tiles ? a.list.tiles[column] : column;
-    const CoarseCtx c = coarse_ctx(a, tile, column);
+    if (column >= a.list.columns) return;
+    const SceneView v = scene_view(a.scenes, column, a.list.columns, cfg.batch_count, cfg.tiles_width, cfg.tiles_height,
+                                   0u, 0u, 0u);
+    const uint32_t tile = a.list.tiles ? a.list.tiles[column] : v.local;
+    const CoarseCtx c = coarse_ctx(a, v, tile, column);
+    const uint32_t Q = v.q1;
     Emitter e{a.out.ptcl, cfg.ptcl_capacity, a.out.status};
     uint32_t *kinds = a.out.tile_kinds;
-    const tk_tile_draw_record list = a.in.tile_draws[tile];
+    const tk_tile_draw_record list = a.in.tile_draws[a.scenes.slots ? column : tile];
     const uint32_t len = list.start == kNone ? 0u : list.len;
 
     // Uniform per-batch state.
-    uint32_t cur = kNone, next = 0u;
+    uint32_t cur = kNone, next = v.q0;
     bool active = false;
     uint32_t start = 0u, end = 0u, wrappers = 0u, dst = 0u, flags = 0u, last_opaque = 0u;
     bool begin_after = false;
@@ -1290,7 +1328,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
     uint64_t stack_emitted = 0ull;
     uint32_t stack_end_tag = TK_PTCL_END_CLIP;
     auto empty_until = [&](uint32_t stop) {
-        for (uint32_t bb = next + lane; bb < stop; bb += 64u) kinds[(size_t)bb * T + column] = TK_TILE_KIND_EMPTY;
+        for (uint32_t bb = next + lane; bb < stop; bb += 64u) kinds[table_index(v, bb)] = TK_TILE_KIND_EMPTY;
     };
     auto finish = [&]() {
         if (cur == kNone) return;
@@ -1303,12 +1341,12 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
             if ((flags & 8u) && wrappers == 0u && end - start > 2u && !begin_after) skip = last_opaque;
             kind = classify_tile_flags(flags) | (skip << TK_TILE_PREFIX_SHIFT);
         }
-        if (lane == 0u) kinds[(size_t)cur * T + column] = kind;
+        if (lane == 0u) kinds[table_index(v, cur)] = kind;
         next = cur + 1u;
     };
     auto begin = [&](uint32_t bb) {
         cur = bb;
-        const tk_segment_range r = a.out.tile_ptcl_ranges[(size_t)bb * T + column];
+        const tk_segment_range r = a.out.tile_ptcl_ranges[table_index(v, bb)];
         start = r.start;
         end = r.end;
         active = end > start;
@@ -1339,7 +1377,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
 
     for (uint32_t base = 0; base < len; base += 64u) {
         uint32_t d, b;
-        load_list_entry(a, list, base, lane, d, b);
+        load_list_entry(a, list, base, lane, v, d, b);
         // Every lane's particle and class at once (a lane's batch is its own b), so
         // the particle code runs once per 64 entries however many batches they span.
         Particle p{false, TK_PTCL_FILL, 0u, 0u, 0u, 0u, 0u};
@@ -1424,12 +1462,14 @@ extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     if (linear >= xcd_grid(columns, TK_XCD_CHUNK_FINE)) return;
     const uint32_t column = xcd_chunked_block(linear, TK_XCD_CHUNK_FINE);
     if (column >= columns) return;
-    const uint32_t tile_ix = a.list.tiles ? a.list.tiles[column] : column;
+    const SceneView v = scene_view(a.scenes, column, columns, cfg.batch_count, cfg.tiles_width, cfg.tiles_height,
+                                   cfg.width, cfg.height, cfg.clear_color);
+    const uint32_t tile_ix = a.list.tiles ? a.list.tiles[column] : v.local;  // the tile of its canvas
     if (tile_ix >= cfg.tile_count) return;
     const uint32_t lane = threadIdx.x;
     const uint32_t lx = lane & 15u, ly = lane >> 4;
-    const uint32_t gx = (tile_ix % cfg.tiles_width) * 16u + lx;
-    const uint32_t gy = (tile_ix / cfg.tiles_width) * 16u + ly;
+    const uint32_t gx = (tile_ix % v.tw) * 16u + lx;
+    const uint32_t gy = (tile_ix / v.tw) * 16u + ly;
     const float sx = (float)gx + 0.5f, sy = (float)gy + 0.5f;
     const uint32_t *paint = a.in.paint;
     const tk_ptcl_record *ptcl = a.in.ptcl;
@@ -1445,17 +1485,17 @@ extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     }
     uint32_t acc_sel = 0u;
 
-    Px pixel = rgba8_to_unorm(cfg.clear_color);
-    for (uint32_t b0 = 0; b0 < cfg.batch_count; b0 += 256u) {
+    Px pixel = rgba8_to_unorm(v.clear);
+    for (uint32_t b0 = v.q0; b0 < v.q1; b0 += 256u) {
       // This tile's (kind, range) word pairs of up to 256 batches, one load per lane.
       __syncthreads();
-      if (b0 + lane < cfg.batch_count) {
-          const size_t idx = (size_t)(b0 + lane) * columns + column;
+      if (b0 + lane < v.q1) {
+          const size_t idx = table_index(v, b0 + lane);
           batch_kind[lane] = a.in.tile_kinds[idx];
           batch_range[lane] = a.in.tile_ptcl_ranges[idx];
       }
       __syncthreads();
-      const uint32_t chunk = min(256u, cfg.batch_count - b0);
+      const uint32_t chunk = min(256u, v.q1 - b0);
       for (uint32_t bj = 0; bj < chunk; ++bj) {
         const uint32_t kind_word = batch_kind[bj];
         const uint32_t kind = kind_word & TK_TILE_KIND_MASK;
@@ -1589,7 +1629,7 @@ extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
         pixel = rgba8_to_unorm(unorm_to_rgba8(pixel));
       }
     }
-    if (gx < cfg.width && gy < cfg.height) a.rgba[(size_t)gy * cfg.width + gx] = unorm_to_rgba8(pixel);
+    if (gx < v.width && gy < v.height) a.rgba[v.image + (size_t)gy * v.width + gx] = unorm_to_rgba8(pixel);
 }
 
 // fill_coverage_words (validation/geometry.hlsl:22-29) plus the raw FP32 area.

@@ -203,6 +203,9 @@ uint32_t coarse_scan_blocks(const tk_coarse_config &cfg, uint32_t columns) {
     return uint32_t(std::max<uint64_t>((n + TK_COARSE_SCAN_ITEMS - 1) / TK_COARSE_SCAN_ITEMS, 1));
 }
 uint32_t coarse_scan_blocks(const tk_coarse_config &cfg) { return coarse_scan_blocks(cfg, cfg.tile_count); }
+uint32_t coarse_scan_blocks_for(uint64_t entries) {
+    return uint32_t(std::max<uint64_t>((entries + TK_COARSE_SCAN_ITEMS - 1) / TK_COARSE_SCAN_ITEMS, 1));
+}
 
 // Where the frame keeps its per-tile layer classes: pair_layer[pair] is a layer
 // record of each distinct (draw, tag) pair, pair_of[layer] its pair, and bits
@@ -216,11 +219,13 @@ struct LayerClassPlan {
 tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coarse_inputs &in,
                      const tk_coarse_outputs &out, unsigned long long *status, uint32_t *ticket, uint32_t epoch,
                      hipEvent_t *marks, const LayerClassPlan &plan = LayerClassPlan{},
-                     tk_tile_list list = tk_tile_list{nullptr, 0u}) {
+                     tk_tile_list list = tk_tile_list{nullptr, 0u},
+                     tk_scene_slots scenes = tk_scene_slots{nullptr, nullptr, 0u, 0ull}) {
     if (!list.tiles) list.columns = cfg.tile_count;
     if (cfg.batch_count == 0 || list.columns == 0) return TK_OK;
-    tk_args_coarse a{cfg,  in, out, status, ticket, epoch, coarse_scan_blocks(cfg, list.columns),
-                     tk_layer_classes{nullptr, nullptr, 0u}, list};
+    const uint32_t blocks = scenes.slots ? coarse_scan_blocks_for(scenes.table_entries)
+                                         : coarse_scan_blocks(cfg, list.columns);
+    tk_args_coarse a{cfg, in, out, status, ticket, epoch, blocks, tk_layer_classes{nullptr, nullptr, 0u}, list, scenes};
     if (plan.bits && plan.pair_count) {
         tk_args_layer_classes b{cfg, in, plan.pair_layer, plan.pair_count, plan.bits, list};
         TK_TRY(launch(ctx, ctx->f_layer_classes, uint32_t(div_up(uint64_t(list.columns) * plan.pair_count, 256)), 1,
@@ -238,7 +243,8 @@ tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coar
 // `draw_brush` holds 2 * cfg.draw_count words; it is rebuilt here from the draws
 // and paint before fine reads it.
 tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inputs &in, uint32_t *rgba,
-                   uint32_t *draw_brush, tk_tile_list list = tk_tile_list{nullptr, 0u}) {
+                   uint32_t *draw_brush, tk_tile_list list = tk_tile_list{nullptr, 0u},
+                   tk_scene_slots scenes = tk_scene_slots{nullptr, nullptr, 0u, 0ull}) {
     if (!list.tiles) list.columns = cfg.tile_count;
     if (cfg.clip_spill_depth > TK_FINE_MAX_CLIP_SPILL) {
         set_error("clip nesting deeper than %u is not supported", 4u + TK_FINE_MAX_CLIP_SPILL);
@@ -248,7 +254,7 @@ tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inp
         tk_args_draw_brush b{in.draws, cfg.draw_count, in.paint, cfg.paint_words, cfg.paint_brush_base, draw_brush};
         TK_TRY(launch(ctx, ctx->f_draw_brush, div_up(cfg.draw_count, 256), 1, 256, b));
     }
-    tk_args_fine a{cfg, in, rgba, draw_brush, list};
+    tk_args_fine a{cfg, in, rgba, draw_brush, list, scenes};
     if (list.columns == 0) return TK_OK;
     const uint32_t blocks = xcd_grid(list.columns, TK_XCD_CHUNK_FINE);
     uint32_t gx = std::min<uint32_t>(blocks, 65535u);
@@ -309,6 +315,11 @@ struct tk_frame {
     uint32_t epoch = 0;
     bool capture_raw = false;
     bool timing = false;
+    // a batch of independent scenes (tk_batch): per-scene slots, column -> scene
+    tk_scene_slot *scene_slots = nullptr;
+    uint32_t *scene_of_column = nullptr;
+    uint32_t scene_count = 0;
+    uint64_t table_entries = 0;  // batch: sum of the scenes' table blocks
     // hipGraph replay of a full render (tk_frame_set_graph)
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -463,6 +474,17 @@ tk_fine_config frame_fine_config(const tk_frame *f) {
     return c;
 }
 
+// Entries of the per-(batch, tile) tables: one block per scene for a batch of
+// scenes, else batches x columns (the listed tiles of an incremental render).
+uint64_t frame_table_entries(const tk_frame *f, uint32_t active_count = 0) {
+    if (f->scene_count) return f->table_entries;
+    const uint64_t columns = active_count ? active_count : uint64_t(f->meta.tiles_width) * f->meta.tiles_height;
+    return uint64_t(f->meta.batch_count) * columns;
+}
+uint32_t frame_scan_blocks(const tk_frame *f, uint32_t active_count = 0) {
+    return coarse_scan_blocks_for(frame_table_entries(f, active_count));
+}
+
 tk_status record(tk_frame *f, int mark) {
     if (f->timing) TK_HIP(hipEventRecord(f->ev[mark], launch_stream(f->ctx)));
     return TK_OK;
@@ -476,6 +498,7 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0) {
     tk_context *ctx = f->ctx;
     const tk_scene &s = f->meta;
     const tk_tile_list list{active_count ? f->active_tiles : nullptr, active_count};
+    const tk_scene_slots scenes{f->scene_slots, f->scene_of_column, f->scene_count, f->table_entries};
     const uint32_t *path_active = nullptr;
     tk_scan_config scfg = frame_scan_config(f);
     const uint32_t T = s.tiles_width * s.tiles_height;
@@ -489,8 +512,7 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0) {
     TK_TRY(fill_u32(ctx, f->status, 4, 0u));
     // The coarse scan's lookback words start each frame at zero, so a replayed
     // graph (whose epoch argument is fixed) never sees the previous frame's flags.
-    TK_TRY(fill_u32(ctx, f->coarse_status,
-                    2ull * coarse_scan_blocks(frame_coarse_config(f), active_count ? active_count : T), 0u));
+    TK_TRY(fill_u32(ctx, f->coarse_status, 2ull * frame_scan_blocks(f, active_count), 0u));
     if (active_count && s.path_count) {
         tk_args_active_paths ap{f->paths, s.path_count, s.tiles_width, s.tiles_height, f->damage_sat, f->path_active};
         TK_TRY(launch(ctx, ctx->f_active_paths, div_up(s.path_count, 256), 1, 256, ap));
@@ -551,7 +573,7 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0) {
             plan.bits = f->layer_bits;
         }
         TK_TRY(run_coarse(ctx, cc, in, out, f->coarse_status, &f->tickets[1], f->epoch, f->timing ? &f->ev[7] : nullptr,
-                          plan, list));
+                          plan, list, scenes));
         if (cc.batch_count == 0 && f->timing) {
             TK_TRY(record(f, 7));
             TK_TRY(record(f, 8));
@@ -576,7 +598,7 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0) {
         in.ptcl = f->ptcl;
         in.status = f->status + 2;
         (void)T;
-        TK_TRY(run_fine(ctx, fc, in, f->image, f->draw_brush, list));
+        TK_TRY(run_fine(ctx, fc, in, f->image, f->draw_brush, list, scenes));
     }
     TK_TRY(record(f, 10));
     f->have_times = f->timing;
@@ -614,6 +636,121 @@ tk_status frame_render_graph(tk_frame *f) {
         }
     }
     TK_HIP(hipGraphLaunch(f->graph_exec, ctx->stream));
+    return TK_OK;
+}
+
+// How a batch of independent scenes sits in one frame (tk_batch_create).
+struct BatchLayout {
+    std::vector<tk_scene_slot> slots;
+    std::vector<uint32_t> scene_of_column;
+    uint64_t table_entries = 0;
+};
+
+tk_status frame_create(tk_context *ctx, const tk_scene *s, const BatchLayout *batch, tk_frame **out) {
+    if (!ctx || !out) return fail(TK_ERR_INVALID, "null argument");
+    *out = nullptr;
+    TK_TRY(validate_scene(s));
+    tk_frame *f = new (std::nothrow) tk_frame();
+    if (!f) return fail(TK_ERR_OOM, "frame allocation failed");
+    f->ctx = ctx;
+    f->meta = *s;
+    f->ptcl_capacity = std::max<uint64_t>(s->ptcl_capacity, 1);
+    if (batch) {
+        f->scene_count = uint32_t(batch->slots.size());
+        f->table_entries = batch->table_entries;
+    }
+    auto bail = [&](tk_status st) {
+        tk_frame_destroy(f);
+        return st;
+    };
+    const uint64_t T = uint64_t(s->tiles_width) * s->tiles_height;
+    const uint64_t QT = frame_table_entries(f);
+    tk_status st = TK_OK;
+#define ALLOC(ptr, n) \
+    if ((st = dalloc(f, &f->ptr, (n))) != TK_OK) return bail(st)
+    ALLOC(lines, s->line_count);
+    ALLOC(paths, s->path_count);
+    ALLOC(draws, s->draw_count);
+    ALLOC(paint, s->paint_words);
+    ALLOC(batch_ids, s->draw_count);
+    ALLOC(layers, s->layer_count);
+    ALLOC(batches, s->batch_count);
+    ALLOC(tile_draws, T);
+    ALLOC(tile_draw_indices, s->tile_draw_index_count);
+    ALLOC(scan_chunks, s->scan_chunk_count);
+    ALLOC(scan_chunk_ranges, s->path_count);
+    ALLOC(cs_offsets, s->cumsum_chunk_count);
+    ALLOC(cs_lens, s->cumsum_chunk_count);
+    ALLOC(cs_row_starts, s->cumsum_row_count);
+    ALLOC(cs_row_ends, s->cumsum_row_count);
+    ALLOC(backdrops, s->backdrop_len);
+    ALLOC(raw_backdrops, s->backdrop_len);
+    ALLOC(seg_counts, s->backdrop_len);
+    ALLOC(seg_ranges, s->backdrop_len);
+    ALLOC(cursors, s->backdrop_len);
+    ALLOC(bumps, s->path_count);
+    ALLOC(chunk_totals, s->scan_chunk_count);
+    ALLOC(chunk_offsets, s->scan_chunk_count);
+    ALLOC(cs_totals, s->cumsum_chunk_count);
+    ALLOC(cs_carries, s->cumsum_chunk_count);
+    ALLOC(segments, s->segment_capacity);
+    ALLOC(tile_counts, QT);
+    ALLOC(tile_ranges, QT);
+    ALLOC(tile_kinds, QT);
+    ALLOC(batch_bases, uint64_t(s->batch_count) + 1);
+    ALLOC(status, 4);
+    ALLOC(ptcl, f->ptcl_capacity);
+    ALLOC(draw_brush, 2 * std::max<uint64_t>(s->draw_count, 1));
+    ALLOC(active_tiles, T);
+    ALLOC(damage_sat, uint64_t(s->tiles_width + 1) * (s->tiles_height + 1));
+    ALLOC(path_active, std::max<uint64_t>(s->path_count, 1));
+    if (batch) {
+        ALLOC(scene_slots, batch->slots.size());
+        ALLOC(scene_of_column, batch->scene_of_column.size());
+        if ((st = upload(f, f->scene_slots, batch->slots.data(), batch->slots.size())) != TK_OK ||
+            (st = upload(f, f->scene_of_column, batch->scene_of_column.data(), batch->scene_of_column.size())) != TK_OK)
+            return bail(st);
+    }
+    if (s->layer_count && s->layers && !batch) {  // the class table indexes one canvas's tiles
+        std::vector<uint32_t> pair_of, pair_layer;
+        const uint64_t pairs = layer_pairs(s, pair_of, pair_layer);
+        if (pairs * T <= kMaxLayerClassBytes) {
+            ALLOC(layer_pair_of, s->layer_count);
+            ALLOC(pair_layer, pairs);
+            ALLOC(layer_bits, pairs * T);
+            f->pair_capacity = uint32_t(pairs);
+        }
+    }
+    ALLOC(image, uint64_t(s->width) * s->height);
+    ALLOC(tickets, 4);
+    ALLOC(coarse_status, frame_scan_blocks(f));
+#undef ALLOC
+    // batch_ptcl_bases: a scene without batches never runs coarse, so its one entry stays 0.
+    if (hipMemsetAsync(f->tickets, 0, 16, ctx->stream) != hipSuccess ||
+        hipMemsetAsync(f->batch_bases, 0, (uint64_t(s->batch_count) + 1) * 4, ctx->stream) != hipSuccess ||
+        hipMemsetAsync(f->coarse_status, 0, uint64_t(frame_scan_blocks(f)) * 8, ctx->stream) != hipSuccess ||
+        hipMemsetAsync(f->cs_carries, 0, std::max<uint64_t>(s->cumsum_chunk_count, 1) * 4, ctx->stream) != hipSuccess)
+        return bail(fail(TK_ERR_HIP, "scratch clear failed"));
+    for (auto &e : f->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(TK_ERR_HIP, "event creation failed"));
+    if ((st = frame_upload(f, s)) != TK_OK) return bail(st);
+    // Only sizes survive in the frame; host pointers belong to the caller.
+    f->meta.lines = nullptr;
+    f->meta.paths = nullptr;
+    f->meta.draws = nullptr;
+    f->meta.paint = nullptr;
+    f->meta.draw_batch_ids = nullptr;
+    f->meta.layers = nullptr;
+    f->meta.batches = nullptr;
+    f->meta.tile_draws = nullptr;
+    f->meta.tile_draw_indices = nullptr;
+    f->meta.scan_chunks = nullptr;
+    f->meta.scan_chunk_ranges = nullptr;
+    f->meta.cumsum_chunk_offsets = nullptr;
+    f->meta.cumsum_chunk_lens = nullptr;
+    f->meta.cumsum_row_starts = nullptr;
+    f->meta.cumsum_row_ends = nullptr;
+    *out = f;
     return TK_OK;
 }
 
@@ -774,106 +911,250 @@ tk_status tk_copy_device(tk_context *ctx, void *dst, const void *src, size_t byt
 // --- frames -----------------------------------------------------------------
 
 tk_status tk_frame_create(tk_context *ctx, const tk_scene *s, tk_frame **out) {
-    if (!ctx || !out) return fail(TK_ERR_INVALID, "null argument");
-    *out = nullptr;
-    TK_TRY(validate_scene(s));
-    tk_frame *f = new (std::nothrow) tk_frame();
-    if (!f) return fail(TK_ERR_OOM, "frame allocation failed");
-    f->ctx = ctx;
-    f->meta = *s;
-    f->ptcl_capacity = std::max<uint64_t>(s->ptcl_capacity, 1);
-    auto bail = [&](tk_status st) {
-        tk_frame_destroy(f);
-        return st;
-    };
-    const uint64_t T = uint64_t(s->tiles_width) * s->tiles_height;
-    const uint64_t QT = uint64_t(s->batch_count) * T;
-    tk_status st = TK_OK;
-#define ALLOC(ptr, n) \
-    if ((st = dalloc(f, &f->ptr, (n))) != TK_OK) return bail(st)
-    ALLOC(lines, s->line_count);
-    ALLOC(paths, s->path_count);
-    ALLOC(draws, s->draw_count);
-    ALLOC(paint, s->paint_words);
-    ALLOC(batch_ids, s->draw_count);
-    ALLOC(layers, s->layer_count);
-    ALLOC(batches, s->batch_count);
-    ALLOC(tile_draws, T);
-    ALLOC(tile_draw_indices, s->tile_draw_index_count);
-    ALLOC(scan_chunks, s->scan_chunk_count);
-    ALLOC(scan_chunk_ranges, s->path_count);
-    ALLOC(cs_offsets, s->cumsum_chunk_count);
-    ALLOC(cs_lens, s->cumsum_chunk_count);
-    ALLOC(cs_row_starts, s->cumsum_row_count);
-    ALLOC(cs_row_ends, s->cumsum_row_count);
-    ALLOC(backdrops, s->backdrop_len);
-    ALLOC(raw_backdrops, s->backdrop_len);
-    ALLOC(seg_counts, s->backdrop_len);
-    ALLOC(seg_ranges, s->backdrop_len);
-    ALLOC(cursors, s->backdrop_len);
-    ALLOC(bumps, s->path_count);
-    ALLOC(chunk_totals, s->scan_chunk_count);
-    ALLOC(chunk_offsets, s->scan_chunk_count);
-    ALLOC(cs_totals, s->cumsum_chunk_count);
-    ALLOC(cs_carries, s->cumsum_chunk_count);
-    ALLOC(segments, s->segment_capacity);
-    ALLOC(tile_counts, QT);
-    ALLOC(tile_ranges, QT);
-    ALLOC(tile_kinds, QT);
-    ALLOC(batch_bases, uint64_t(s->batch_count) + 1);
-    ALLOC(status, 4);
-    ALLOC(ptcl, f->ptcl_capacity);
-    ALLOC(draw_brush, 2 * std::max<uint64_t>(s->draw_count, 1));
-    ALLOC(active_tiles, T);
-    ALLOC(damage_sat, uint64_t(s->tiles_width + 1) * (s->tiles_height + 1));
-    ALLOC(path_active, std::max<uint64_t>(s->path_count, 1));
-    if (s->layer_count && s->layers) {
-        std::vector<uint32_t> pair_of, pair_layer;
-        const uint64_t pairs = layer_pairs(s, pair_of, pair_layer);
-        if (pairs * T <= kMaxLayerClassBytes) {
-            ALLOC(layer_pair_of, s->layer_count);
-            ALLOC(pair_layer, pairs);
-            ALLOC(layer_bits, pairs * T);
-            f->pair_capacity = uint32_t(pairs);
+    return tk::frame_create(ctx, s, nullptr, out);
+}
+
+
+// --- batches of independent scenes -------------------------------------------
+
+}  // extern "C"
+
+struct tk_batch {
+    tk_frame *frame = nullptr;
+    std::vector<tk_scene_slot> slots;
+};
+
+namespace tk {
+namespace {
+
+constexpr uint32_t kNoIndex = 0xFFFFFFFFu;
+uint32_t shifted(uint32_t v, uint64_t by) { return v == kNoIndex ? v : uint32_t(v + by); }
+
+// The scenes laid end to end as one scene: every index into another array is
+// offset by the lengths of the earlier scenes' arrays, the paint blob keeps its
+// three regions (SDF, shadow, brush) each concatenated over the scenes, and the
+// tile bins of the scenes' tile grids follow one another (one table column per
+// tile).  The merged canvas is a single row of tiles; the slots carry each
+// scene's real canvas.
+struct MergedScenes {
+    std::vector<tk_line> lines;
+    std::vector<tk_path_record> paths;
+    std::vector<tk_draw_record> draws;
+    std::vector<uint32_t> paint, batch_ids, tile_draw_indices, cs_offsets, cs_lens, cs_row_starts, cs_row_ends;
+    std::vector<tk_layer_record> layers;
+    std::vector<tk_batch_record> batches;
+    std::vector<tk_tile_draw_record> tile_draws;
+    std::vector<tk_scan_chunk> chunks;
+    std::vector<tk_scan_chunk_range> chunk_ranges;
+    tk_scene scene{};
+    BatchLayout layout;
+};
+
+template <class T>
+void append(std::vector<T> &dst, const T *src, uint64_t n) {
+    if (n) dst.insert(dst.end(), src, src + n);
+}
+
+tk_status merge_scenes(const tk_scene *scenes, uint32_t count, MergedScenes &m) {
+    uint64_t sdf_total = 0, shadow_total = 0, columns = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        sdf_total += scenes[i].paint_sdf_shadow_base;
+        shadow_total += scenes[i].paint_brush_base - scenes[i].paint_sdf_shadow_base;
+    }
+    uint64_t line_off = 0, path_off = 0, draw_off = 0, sdf_off = 0, shadow_off = 0, brush_off = 0, batch_off = 0,
+             layer_off = 0, tdi_off = 0, chunk_off = 0, cs_chunk_off = 0, backdrop_off = 0, segment_off = 0,
+             entries = 0, ptcl = 0;
+    uint32_t depth = 0;
+    std::vector<uint32_t> sdf, shadow, brush;
+    for (uint32_t i = 0; i < count; ++i) {
+        const tk_scene &s = scenes[i];
+        if (s.paint_sdf_shadow_base > s.paint_brush_base || s.paint_brush_base > s.paint_words)
+            return fail(TK_ERR_INVALID, "scene paint regions out of order");
+        const uint64_t T = uint64_t(s.tiles_width) * s.tiles_height;
+        tk_scene_slot slot{};
+        slot.width = s.width;
+        slot.height = s.height;
+        slot.tiles_width = s.tiles_width;
+        slot.tiles_height = s.tiles_height;
+        slot.clear_color = s.clear_color;
+        slot.batch_start = uint32_t(batch_off);
+        slot.batch_end = uint32_t(batch_off + s.batch_count);
+        slot.column_start = uint32_t(columns);
+        slot.table_base = entries;
+        slot.image_offset = columns * 256u;  // the scene's tiles cover its pixels
+        m.layout.slots.push_back(slot);
+        m.layout.scene_of_column.insert(m.layout.scene_of_column.end(), T, i);
+
+        for (uint32_t k = 0; k < s.line_count; ++k) {
+            tk_line l = s.lines[k];
+            l.path_id = uint32_t(l.path_id + path_off);
+            m.lines.push_back(l);
         }
+        for (uint32_t k = 0; k < s.path_count; ++k) {
+            tk_path_record p = s.paths[k];
+            p.path_id = uint32_t(p.path_id + path_off);
+            p.line_start = uint32_t(p.line_start + line_off);
+            p.data_offset = uint32_t(p.data_offset + backdrop_off);
+            p.segment_start = uint32_t(p.segment_start + segment_off);
+            m.paths.push_back(p);
+            tk_scan_chunk_range r = s.scan_chunk_ranges[k];
+            r.start = uint32_t(r.start + chunk_off);
+            r.end = uint32_t(r.end + chunk_off);
+            m.chunk_ranges.push_back(r);
+        }
+        for (uint32_t k = 0; k < s.draw_count; ++k) {
+            tk_draw_record d = s.draws[k];
+            d.path_id = shifted(d.path_id, path_off);
+            d.sdf_offset = shifted(d.sdf_offset, sdf_off);
+            d.sdf_shadow_offset = shifted(d.sdf_shadow_offset, shadow_off);
+            d.brush_offset = shifted(d.brush_offset, brush_off);
+            m.draws.push_back(d);
+            m.batch_ids.push_back(shifted(s.draw_batch_ids[k], batch_off));
+        }
+        append(sdf, s.paint, s.paint_sdf_shadow_base);
+        append(shadow, s.paint + s.paint_sdf_shadow_base, s.paint_brush_base - s.paint_sdf_shadow_base);
+        append(brush, s.paint + s.paint_brush_base, s.paint_words - s.paint_brush_base);
+        for (uint32_t k = 0; k < s.layer_count; ++k) {
+            tk_layer_record l = s.layers[k];
+            l.draw = shifted(l.draw, draw_off);
+            m.layers.push_back(l);
+        }
+        for (uint32_t k = 0; k < s.batch_count; ++k)
+            m.batches.push_back(tk_batch_record{uint32_t(s.batches[k].layer_start + layer_off),
+                                                uint32_t(s.batches[k].layer_end + layer_off)});
+        for (uint64_t k = 0; k < T; ++k) {
+            tk_tile_draw_record t = s.tile_draws[k];
+            t.start = shifted(t.start, tdi_off);
+            m.tile_draws.push_back(t);
+        }
+        for (uint32_t k = 0; k < s.tile_draw_index_count; ++k)
+            m.tile_draw_indices.push_back(shifted(s.tile_draw_indices[k], draw_off));
+        for (uint32_t k = 0; k < s.scan_chunk_count; ++k) {
+            tk_scan_chunk c = s.scan_chunks[k];
+            c.path_id = uint32_t(c.path_id + path_off);
+            c.backdrop_offset = uint32_t(c.backdrop_offset + backdrop_off);
+            c.segment_start = uint32_t(c.segment_start + segment_off);
+            m.chunks.push_back(c);
+        }
+        for (uint32_t k = 0; k < s.cumsum_chunk_count; ++k) {
+            m.cs_offsets.push_back(uint32_t(s.cumsum_chunk_offsets[k] + backdrop_off));
+            m.cs_lens.push_back(s.cumsum_chunk_lens[k]);
+        }
+        for (uint32_t k = 0; k < s.cumsum_row_count; ++k) {
+            m.cs_row_starts.push_back(uint32_t(s.cumsum_row_starts[k] + cs_chunk_off));
+            m.cs_row_ends.push_back(uint32_t(s.cumsum_row_ends[k] + cs_chunk_off));
+        }
+        line_off += s.line_count;
+        path_off += s.path_count;
+        draw_off += s.draw_count;
+        sdf_off += s.paint_sdf_shadow_base;
+        shadow_off += s.paint_brush_base - s.paint_sdf_shadow_base;
+        brush_off += s.paint_words - s.paint_brush_base;
+        batch_off += s.batch_count;
+        layer_off += s.layer_count;
+        tdi_off += s.tile_draw_index_count;
+        chunk_off += s.scan_chunk_count;
+        cs_chunk_off += s.cumsum_chunk_count;
+        backdrop_off += s.backdrop_len;
+        segment_off += s.segment_capacity;
+        columns += T;
+        entries += uint64_t(s.batch_count) * T;
+        ptcl += s.ptcl_capacity;
+        depth = std::max(depth, s.max_clip_depth);
     }
-    ALLOC(image, uint64_t(s->width) * s->height);
-    ALLOC(tickets, 4);
-    {
-        tk_coarse_config cc = frame_coarse_config(f);
-        ALLOC(coarse_status, coarse_scan_blocks(cc));
-    }
-#undef ALLOC
-    if (hipMemsetAsync(f->tickets, 0, 16, ctx->stream) != hipSuccess ||
-        hipMemsetAsync(f->coarse_status, 0, uint64_t(coarse_scan_blocks(frame_coarse_config(f))) * 8, ctx->stream) !=
-            hipSuccess ||
-        hipMemsetAsync(f->cs_carries, 0, std::max<uint64_t>(s->cumsum_chunk_count, 1) * 4, ctx->stream) != hipSuccess)
-        return bail(fail(TK_ERR_HIP, "scratch clear failed"));
-    for (auto &e : f->ev)
-        if (hipEventCreate(&e) != hipSuccess) return bail(fail(TK_ERR_HIP, "event creation failed"));
-    if ((st = frame_upload(f, s)) != TK_OK) return bail(st);
-    // Only sizes survive in the frame; host pointers belong to the caller.
-    f->meta.lines = nullptr;
-    f->meta.paths = nullptr;
-    f->meta.draws = nullptr;
-    f->meta.paint = nullptr;
-    f->meta.draw_batch_ids = nullptr;
-    f->meta.layers = nullptr;
-    f->meta.batches = nullptr;
-    f->meta.tile_draws = nullptr;
-    f->meta.tile_draw_indices = nullptr;
-    f->meta.scan_chunks = nullptr;
-    f->meta.scan_chunk_ranges = nullptr;
-    f->meta.cumsum_chunk_offsets = nullptr;
-    f->meta.cumsum_chunk_lens = nullptr;
-    f->meta.cumsum_row_starts = nullptr;
-    f->meta.cumsum_row_ends = nullptr;
-    *out = f;
+    const uint64_t limit = 0xFFFFFFF0ull;
+    if (line_off > limit || path_off > limit || draw_off > limit || sdf_off + shadow_off + brush_off > limit ||
+        batch_off > limit || layer_off > limit || tdi_off > limit || chunk_off > limit || cs_chunk_off > limit ||
+        backdrop_off > limit || segment_off > limit || columns > (1ull << 27))
+        return fail(TK_ERR_OVERFLOW, "batch exceeds 32-bit record indices");
+    m.paint = std::move(sdf);
+    m.paint.insert(m.paint.end(), shadow.begin(), shadow.end());
+    m.paint.insert(m.paint.end(), brush.begin(), brush.end());
+    m.layout.table_entries = entries;
+    tk_scene &o = m.scene;
+    o.tiles_width = uint32_t(columns);
+    o.tiles_height = 1;
+    o.width = o.tiles_width * 16u;
+    o.height = 16;
+    o.clear_color = 0;
+    o.lines = m.lines.data(), o.line_count = uint32_t(line_off);
+    o.paths = m.paths.data(), o.path_count = uint32_t(path_off);
+    o.draws = m.draws.data(), o.draw_count = uint32_t(draw_off);
+    o.paint = m.paint.data(), o.paint_words = uint32_t(m.paint.size());
+    o.paint_sdf_shadow_base = uint32_t(sdf_total);
+    o.paint_brush_base = uint32_t(sdf_total + shadow_total);
+    o.draw_batch_ids = m.batch_ids.data();
+    o.layers = m.layers.data(), o.layer_count = uint32_t(layer_off);
+    o.batches = m.batches.data(), o.batch_count = uint32_t(batch_off);
+    o.tile_draws = m.tile_draws.data();
+    o.tile_draw_indices = m.tile_draw_indices.data(), o.tile_draw_index_count = uint32_t(tdi_off);
+    o.scan_chunks = m.chunks.data(), o.scan_chunk_count = uint32_t(chunk_off);
+    o.scan_chunk_ranges = m.chunk_ranges.data();
+    o.cumsum_chunk_offsets = m.cs_offsets.data();
+    o.cumsum_chunk_lens = m.cs_lens.data();
+    o.cumsum_chunk_count = uint32_t(cs_chunk_off);
+    o.cumsum_row_starts = m.cs_row_starts.data();
+    o.cumsum_row_ends = m.cs_row_ends.data();
+    o.cumsum_row_count = uint32_t(m.cs_row_starts.size());
+    o.backdrop_len = uint32_t(backdrop_off);
+    o.segment_capacity = uint32_t(segment_off);
+    o.max_clip_depth = depth;
+    o.ptcl_capacity = ptcl;
     return TK_OK;
+}
+
+}  // namespace
+}  // namespace tk
+
+extern "C" {
+
+tk_status tk_batch_create(tk_context *ctx, const tk_scene *scenes, uint32_t count, tk_batch **out) {
+    if (!ctx || !out || (count && !scenes)) return fail(TK_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (count == 0) return fail(TK_ERR_INVALID, "empty batch");
+    for (uint32_t i = 0; i < count; ++i) TK_TRY(validate_scene(&scenes[i]));
+    MergedScenes m;
+    TK_TRY(merge_scenes(scenes, count, m));
+    tk_frame *f = nullptr;
+    TK_TRY(tk::frame_create(ctx, &m.scene, &m.layout, &f));
+    tk_batch *b = new (std::nothrow) tk_batch();
+    if (!b) {
+        tk_frame_destroy(f);
+        return fail(TK_ERR_OOM, "batch allocation failed");
+    }
+    b->frame = f;
+    b->slots = m.layout.slots;
+    *out = b;
+    return TK_OK;
+}
+
+tk_frame *tk_batch_frame(tk_batch *b) { return b ? b->frame : nullptr; }
+
+tk_status tk_batch_render(tk_batch *b) {
+    if (!b) return fail(TK_ERR_INVALID, "null batch");
+    return tk_frame_render(b->frame);
+}
+
+tk_status tk_batch_read_image(tk_batch *b, uint32_t scene, uint32_t *rgba, size_t pixel_count) {
+    if (!b || !rgba) return fail(TK_ERR_INVALID, "null argument");
+    if (scene >= b->slots.size()) return fail(TK_ERR_INVALID, "scene index out of range");
+    const tk_scene_slot &s = b->slots[scene];
+    const uint64_t n = uint64_t(s.width) * s.height;
+    if (pixel_count < n) return fail(TK_ERR_INVALID, "output smaller than the scene's image");
+    TK_HIP(hipMemcpyAsync(rgba, b->frame->image + s.image_offset, n * 4, hipMemcpyDeviceToHost, b->frame->ctx->stream));
+    TK_HIP(hipStreamSynchronize(b->frame->ctx->stream));
+    return TK_OK;
+}
+
+void tk_batch_destroy(tk_batch *b) {
+    if (!b) return;
+    tk_frame_destroy(b->frame);
+    delete b;
 }
 
 tk_status tk_frame_update(tk_frame *f, const tk_scene *s) {
     if (!f) return fail(TK_ERR_INVALID, "null frame");
+    if (f->scene_count) return fail(TK_ERR_INVALID, "a batch of scenes cannot be updated in place");
     TK_TRY(validate_scene(s));
     if (!same_sizes(*s, f->meta)) return fail(TK_ERR_INVALID, "scene sizes differ from the frame's");
     uint32_t clear = s->clear_color;
@@ -915,6 +1196,7 @@ tk_status tk_frame_set_graph(tk_frame *f, int enable) {
 
 tk_status tk_frame_render_tiles(tk_frame *f, const uint32_t *tiles, uint32_t tile_count) {
     if (!f || (tile_count && !tiles)) return fail(TK_ERR_INVALID, "null argument");
+    if (f->scene_count) return fail(TK_ERR_INVALID, "incremental rendering does not apply to a batch of scenes");
     const tk_scene &s = f->meta;
     const uint32_t tw = s.tiles_width, th = s.tiles_height, T = tw * th;
     // The host arrays are reused: wait until earlier uploads from them have completed.

@@ -56,6 +56,23 @@ struct tk_args_scan_lines {
     const uint32_t *path_active; // incremental: lines of paths whose flag is 0 are skipped; null: all
 };
 
+// A batch of independent scenes rendered as one frame: each scene keeps its own
+// canvas and tile grid, its tiles take the table columns [column_start,
+// column_start + tiles_width * tiles_height), its draws use the batches
+// [batch_start, batch_end), and its (batch, tile) tables form one block of
+// (batch_end - batch_start) x columns entries at table_base, batch-major.
+struct tk_scene_slot {
+    uint32_t width, height, tiles_width, tiles_height;
+    uint32_t clear_color, batch_start, batch_end, column_start;
+    uint64_t table_base, image_offset;  // image_offset: the scene's first pixel in the RGBA8 output
+};
+struct tk_scene_slots {
+    const tk_scene_slot *slots;      // null: one scene, described by the stage configs
+    const uint32_t *scene_of_column;
+    uint32_t count;
+    uint64_t table_entries;          // sum of the scenes' table blocks
+};
+
 // Incremental rendering over a list of damaged tiles (the reference's
 // active_tile_list): the per-(batch, tile) tables hold one column per listed
 // tile, in list order, instead of one per tile of the canvas.
@@ -118,6 +135,7 @@ struct tk_args_coarse {
     uint32_t scan_blocks;
     tk_layer_classes classes;
     tk_tile_list list;
+    tk_scene_slots scenes;
 };
 
 struct tk_args_layer_classes {
@@ -145,6 +163,7 @@ struct tk_args_fine {
     uint32_t *rgba;
     const uint32_t *draw_brush;  // tk_args_draw_brush::table, draw_count entries
     tk_tile_list list;
+    tk_scene_slots scenes;
 };
 
 struct tk_args_fill_coverage {
