@@ -328,11 +328,12 @@ struct SceneView {
     uint32_t tw, th, width, height, clear, q0, q1, cols, local;
     uint64_t base, image;
 };
+template <bool kScenes>
 __device__ __forceinline__ SceneView scene_view(const tk_scene_slots &sl, uint32_t column, uint32_t columns,
                                                 uint32_t batch_count, uint32_t tw, uint32_t th, uint32_t width,
                                                 uint32_t height, uint32_t clear) {
     SceneView v{tw, th, width, height, clear, 0u, batch_count, columns, column, 0ull, 0ull};
-    if (sl.slots) {
+    if constexpr (kScenes) {
         const tk_scene_slot &s = sl.slots[sl.scene_of_column[column]];
         v.tw = s.tiles_width;
         v.th = s.tiles_height;
@@ -1164,19 +1165,21 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_layer_classes(tk_args_lay
 // The tile's draw list is read 64 entries at a time and its particles counted per
 // batch; the batches met are kept one per lane and their stacks resolved together
 // (stack_wrappers_of_batches) once 64 are pending or the list ends.
-extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coarse a) {
+// kScenes: a batch of scenes (per-tile canvases from a.scenes); else one canvas.
+template <bool kScenes>
+__device__ __forceinline__ void coarse_count_tile(const tk_args_coarse &a) {
     __shared__ uint32_t invalid_flags[4][64], counted[4][64];
     const tk_coarse_config &cfg = a.cfg;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t column = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + wave;
     if (column >= a.list.columns) return;
-    const SceneView v = scene_view(a.scenes, column, a.list.columns, cfg.batch_count, cfg.tiles_width, cfg.tiles_height,
-                                   0u, 0u, 0u);
+    const SceneView v = scene_view<kScenes>(a.scenes, column, a.list.columns, cfg.batch_count, cfg.tiles_width,
+                                            cfg.tiles_height, 0u, 0u, 0u);
     const uint32_t tile = a.list.tiles ? a.list.tiles[column] : v.local;
     const CoarseCtx c = coarse_ctx(a, v, tile, column);
     uint32_t *counts = a.out.tile_ptcl_counts;
-    const tk_tile_draw_record list = a.in.tile_draws[a.scenes.slots ? column : tile];
+    const tk_tile_draw_record list = a.in.tile_draws[kScenes ? column : tile];
     const uint32_t len = list.start == kNone ? 0u : list.len;
     const uint32_t Q = v.q1;
 
@@ -1232,6 +1235,11 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coar
     flush();
     resolve();
     for (uint32_t bb = next + lane; bb < Q; bb += 64u) counts[table_index(v, bb)] = 0u;
+}
+
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coarse a) { coarse_count_tile<false>(a); }
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count_scenes(tk_args_coarse a) {
+    coarse_count_tile<true>(a);
 }
 
 // coarse_prefix_chunks + coarse_chunk_offsets + coarse_apply_chunk_offsets for
@@ -1304,19 +1312,20 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coars
 // nonzero count and the tile kind word, including the opaque-prefix skip that
 // occluded_particle_prefix (coarse/occlusion.hlsli:12-34) derives: the last
 // opaque colour particle, unless a begin record follows it.
-extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coarse a) {
+template <bool kScenes>
+__device__ __forceinline__ void coarse_emit_tile(const tk_args_coarse &a) {
     const tk_coarse_config &cfg = a.cfg;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t column = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + (threadIdx.x >> 6);
     if (column >= a.list.columns) return;
-    const SceneView v = scene_view(a.scenes, column, a.list.columns, cfg.batch_count, cfg.tiles_width, cfg.tiles_height,
-                                   0u, 0u, 0u);
+    const SceneView v = scene_view<kScenes>(a.scenes, column, a.list.columns, cfg.batch_count, cfg.tiles_width,
+                                            cfg.tiles_height, 0u, 0u, 0u);
     const uint32_t tile = a.list.tiles ? a.list.tiles[column] : v.local;
     const CoarseCtx c = coarse_ctx(a, v, tile, column);
     const uint32_t Q = v.q1;
     Emitter e{a.out.ptcl, cfg.ptcl_capacity, a.out.status};
     uint32_t *kinds = a.out.tile_kinds;
-    const tk_tile_draw_record list = a.in.tile_draws[a.scenes.slots ? column : tile];
+    const tk_tile_draw_record list = a.in.tile_draws[kScenes ? column : tile];
     const uint32_t len = list.start == kNone ? 0u : list.len;
 
     // Uniform per-batch state.
@@ -1425,6 +1434,11 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coars
     empty_until(Q);
 }
 
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coarse a) { coarse_emit_tile<false>(a); }
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit_scenes(tk_args_coarse a) {
+    coarse_emit_tile<true>(a);
+}
+
 // The colour sample_draw_brush returns for a draw whose brush does not depend on
 // the pixel (solid, and the kinds sample_brush maps to the header colour or to
 // 0), so that a fill reads one word pair instead of the draw record and then
@@ -1446,10 +1460,9 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_draw_brush(tk_args_draw_b
 
 // fine_tile_main for every batch: one 256-lane workgroup per tile, one lane per
 // pixel.  Between batches the pixel goes through the RGBA8 store/load round trip
-// the reference performs on its target.
-// Seven waves per SIMD: at most 72 VGPRs (two dwords spill, in the per-batch-run
-// setup outside the record loop).
-extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void tk_k_fine(tk_args_fine a) {
+// the reference performs on its target.  kScenes: a batch of scenes.
+template <bool kScenes>
+__device__ __forceinline__ void fine_tile(const tk_args_fine &a) {
     // Clip levels beyond the four register slots: clip_spill_depth * 256 bytes of
     // dynamic LDS, sized by the launch.
     extern __shared__ uint8_t clip_spill[];
@@ -1462,8 +1475,8 @@ extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     if (linear >= xcd_grid(columns, TK_XCD_CHUNK_FINE)) return;
     const uint32_t column = xcd_chunked_block(linear, TK_XCD_CHUNK_FINE);
     if (column >= columns) return;
-    const SceneView v = scene_view(a.scenes, column, columns, cfg.batch_count, cfg.tiles_width, cfg.tiles_height,
-                                   cfg.width, cfg.height, cfg.clear_color);
+    const SceneView v = scene_view<kScenes>(a.scenes, column, columns, cfg.batch_count, cfg.tiles_width,
+                                            cfg.tiles_height, cfg.width, cfg.height, cfg.clear_color);
     const uint32_t tile_ix = a.list.tiles ? a.list.tiles[column] : v.local;  // the tile of its canvas
     if (tile_ix >= cfg.tile_count) return;
     const uint32_t lane = threadIdx.x;
@@ -1630,6 +1643,16 @@ extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
       }
     }
     if (gx < v.width && gy < v.height) a.rgba[v.image + (size_t)gy * v.width + gx] = unorm_to_rgba8(pixel);
+}
+
+// Seven waves per SIMD: at most 72 VGPRs (a few dwords spill, in the per-batch-run
+// setup outside the record loop).
+extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void tk_k_fine(tk_args_fine a) {
+    fine_tile<false>(a);
+}
+extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void tk_k_fine_scenes(
+    tk_args_fine a) {
+    fine_tile<true>(a);
 }
 
 // fill_coverage_words (validation/geometry.hlsl:22-29) plus the raw FP32 area.

@@ -74,8 +74,8 @@ struct tk_context {
                   f_scan_emit = nullptr, f_cumsum = nullptr, f_coarse_count = nullptr,
                   f_coarse_scan = nullptr, f_coarse_emit = nullptr, f_layer_classes = nullptr, f_active_paths = nullptr,
                   f_draw_brush = nullptr,
-                  f_fine = nullptr,
-                  f_fill_coverage = nullptr, f_pixel_math = nullptr, f_brush_words = nullptr;
+                  f_fine = nullptr, f_coarse_count_scenes = nullptr, f_coarse_emit_scenes = nullptr,
+                  f_fine_scenes = nullptr, f_fill_coverage = nullptr, f_pixel_math = nullptr, f_brush_words = nullptr;
     // Scratch for the stage entry points (frames own their own).
     uint32_t *tickets = nullptr;  // [0] scan prefix, [1] coarse scan
     unsigned long long *status = nullptr;
@@ -232,11 +232,11 @@ tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coar
                       256, b));
         a.classes = tk_layer_classes{plan.bits, plan.pair_of, plan.pair_count};
     }
-    TK_TRY(launch(ctx, ctx->f_coarse_count, xcd_grid(div_up(list.columns, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
+    TK_TRY(launch(ctx, scenes.slots ? ctx->f_coarse_count_scenes : ctx->f_coarse_count, xcd_grid(div_up(list.columns, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
     if (marks) TK_HIP(hipEventRecord(marks[0], launch_stream(ctx)));
     TK_TRY(launch(ctx, ctx->f_coarse_scan, a.scan_blocks, 1, 256, a));
     if (marks) TK_HIP(hipEventRecord(marks[1], launch_stream(ctx)));
-    TK_TRY(launch(ctx, ctx->f_coarse_emit, xcd_grid(div_up(list.columns, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
+    TK_TRY(launch(ctx, scenes.slots ? ctx->f_coarse_emit_scenes : ctx->f_coarse_emit, xcd_grid(div_up(list.columns, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
     return TK_OK;
 }
 
@@ -259,7 +259,7 @@ tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inp
     const uint32_t blocks = xcd_grid(list.columns, TK_XCD_CHUNK_FINE);
     uint32_t gx = std::min<uint32_t>(blocks, 65535u);
     uint32_t gy = div_up(blocks, gx);
-    return launch(ctx, ctx->f_fine, gx, gy, 256, a, cfg.clip_spill_depth * 256u);
+    return launch(ctx, scenes.slots ? ctx->f_fine_scenes : ctx->f_fine, gx, gy, 256, a, cfg.clip_spill_depth * 256u);
 }
 
 }  // namespace
@@ -809,6 +809,9 @@ tk_status tk_context_create(int device, tk_context **out) {
         {&ctx->f_active_paths, "tk_k_active_paths"},
         {&ctx->f_draw_brush, "tk_k_draw_brush"},
         {&ctx->f_fine, "tk_k_fine"},
+        {&ctx->f_coarse_count_scenes, "tk_k_coarse_count_scenes"},
+        {&ctx->f_coarse_emit_scenes, "tk_k_coarse_emit_scenes"},
+        {&ctx->f_fine_scenes, "tk_k_fine_scenes"},
         {&ctx->f_fill_coverage, "tk_k_fill_coverage"},
         {&ctx->f_pixel_math, "tk_k_pixel_math"},
         {&ctx->f_brush_words, "tk_k_brush_words"},
