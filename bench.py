@@ -120,27 +120,39 @@ def scene_loop(ctx, count=64, repeats=5):
     sizes from the independent-scene distribution (side log-uniform 64-2048 px,
     aspect 0.5-2) with path counts lognormal(ln 150, 0.8) clipped to 10-2000 so the
     scenes generate in seconds.  Each scene is its own frame; the loop is timed with
-    kernel-by-kernel launches and with each frame replayed from its hipGraph."""
+    kernel-by-kernel launches and with each frame replayed from its hipGraph.  The
+    same scenes are then rendered as one batch (tk_batch_*: one frame, one launch
+    per stage for all of them), eager and replayed from a hipGraph."""
     sizes = scenes.config3_sizes(count, path_mu=np.log(150.0), path_sigma=0.8, max_paths=2000)
     canvases = [scenes.config3_scene(w, h, n, i) for i, (w, h, n) in enumerate(sizes)]
-    frames = [tk.Frame(ctx, c.encode(0)) for c in canvases]
+    encoded = [c.encode(0) for c in canvases]
+    frames = [tk.Frame(ctx, sc) for sc in encoded]
     out = {"scenes": count, "paths": int(sum(n for _, _, n in sizes)),
            "pixels": int(sum(w * h for w, h, _ in sizes))}
-    for mode in ("eager", "graph"):
-        for f in frames:
-            f.set_graph(mode == "graph")
-            f.render()  # warm-up (captures the graph)
+
+    def timed(name, render_all):
+        render_all()  # warm-up (captures graphs)
         ctx.synchronize()
         t0 = time.perf_counter()
         for _ in range(repeats):
-            for f in frames:
-                f.render()
+            render_all()
         ctx.synchronize()
         ms = (time.perf_counter() - t0) / repeats * 1e3
-        out[f"{mode}_ms"] = round(ms, 3)
-        out[f"{mode}_scenes_per_s"] = round(count / (ms * 1e-3), 1)
+        out[f"{name}_ms"] = round(ms, 3)
+        out[f"{name}_scenes_per_s"] = round(count / (ms * 1e-3), 1)
+
+    for mode in ("eager", "graph"):
+        for f in frames:
+            f.set_graph(mode == "graph")
+        timed(mode, lambda: [f.render() for f in frames])
     out["graph_speedup"] = round(out["eager_ms"] / out["graph_ms"], 3)
     del frames
+    batch = tk.Batch(ctx, encoded)
+    for mode in ("eager", "graph"):
+        batch.frame.set_graph(mode == "graph")
+        timed(f"batch_{mode}", batch.render)
+    out["batch_speedup"] = round(out["graph_ms"] / min(out["batch_eager_ms"], out["batch_graph_ms"]), 3)
+    del batch
     return out
 
 
