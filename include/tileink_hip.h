@@ -321,6 +321,23 @@ tk_status tk_frame_render(tk_frame *frame);
  * TK_ERR_INVALID.  Afterwards the per-(batch, tile) tables read back through
  * tk_frame_read_buffer hold one column per listed tile, in list order. */
 tk_status tk_frame_render_tiles(tk_frame *frame, const uint32_t *tiles, uint32_t tile_count);
+/* The differentiable form (no counterpart in the reference, which only renders).
+ * Its forward pass is the fine stage (fine/main.hlsl:23-40) with the RGBA8 round
+ * trips between batches left out: an FP32 premultiplied source-over composite of
+ * the scene's FILL records.  Scenes: path draws only (fills and strokes), no
+ * layers; each draw's colour is colors[4 * draw .. 4 * draw + 3] (premultiplied
+ * RGBA, FP32), not its brush.  All float pointers are device pointers.
+ * tk_frame_render_diff: the whole pipeline; writes `image` (4 floats per pixel,
+ *   row-major; a batch's scenes at their image offsets, one after another).
+ * tk_frame_backward: after tk_frame_render_diff; from image_grad = dL/d(image)
+ *   writes line_grad = dL/d(p0.x, p0.y, p1.x, p1.y) for every line of the scene
+ *   and color_grad = dL/d(colour) for every draw (both overwritten).
+ * tk_frame_diff_status: bit 0 when a record the form does not take was met.   */
+tk_status tk_frame_render_diff(tk_frame *frame, const float *colors, float *image);
+tk_status tk_frame_backward(tk_frame *frame, const float *colors, const float *image_grad, float *line_grad,
+                            float *color_grad);
+tk_status tk_frame_diff_status(tk_frame *frame, uint32_t *flags);
+
 /* enable != 0: tk_frame_render captures its launch sequence (both streams) into a
  * hipGraph on first use and replays it as one launch afterwards; renders with
  * timing or raw-backdrop capture enabled, and tk_frame_render_tiles, still launch

@@ -202,3 +202,51 @@ def brush_words(paint, paint_brush_base, requests):
     lib().or_brush_words(_p(paint), C.c_uint32(paint.size), C.c_uint32(paint_brush_base),
                          C.c_uint32(requests.shape[0]), _p(requests), _p(out))
     return out
+
+
+# --- the differentiable form in FP64 ---------------------------------------------
+
+def _diff_args(draws, lines):
+    starts = np.ascontiguousarray([d[0] for d in draws], np.uint32)
+    counts = np.ascontiguousarray([d[1] for d in draws], np.uint32)
+    rules = np.ascontiguousarray([d[2] for d in draws], np.uint32)
+    lines = np.ascontiguousarray(lines, np.float64).reshape(-1, 4)
+    return starts, counts, rules, lines
+
+
+def diff_area(lines, start, count, width, height):
+    """(box (x0, y0, w, h), per-pixel winding integral over the box) of lines[start:start+count]."""
+    lines = np.ascontiguousarray(lines, np.float64).reshape(-1, 4)
+    box = np.zeros(4, np.int32)
+    lib().or_diff_area_f64(_p(lines), C.c_uint32(start), C.c_uint32(count), C.c_uint32(width),
+                           C.c_uint32(height), _p(box), None)
+    area = np.zeros((box[3], box[2]), np.float64)
+    lib().or_diff_area_f64(_p(lines), C.c_uint32(start), C.c_uint32(count), C.c_uint32(width),
+                           C.c_uint32(height), _p(box), _p(area))
+    return box, area
+
+
+def diff_render(width, height, clear, draws, colors, lines):
+    """FP64 forward of the differentiable form.  draws: (line_start, line_count, rule)
+    per draw in painter order; colors: premultiplied RGBA per draw; lines: (L, 4)."""
+    starts, counts, rules, lines = _diff_args(draws, lines)
+    clear = np.ascontiguousarray(clear, np.float64)
+    colors = np.ascontiguousarray(colors, np.float64).reshape(-1, 4)
+    image = np.zeros((height, width, 4), np.float64)
+    lib().or_diff_render_f64(C.c_uint32(width), C.c_uint32(height), _p(clear), C.c_uint32(len(draws)), _p(starts),
+                             _p(counts), _p(rules), _p(colors), _p(lines), _p(image))
+    return image
+
+
+def diff_backward(width, height, clear, draws, colors, lines, image_grad):
+    """FP64 analytic gradient: (dL/d lines (L, 4), dL/d colours (D, 4)) for dL/d image."""
+    starts, counts, rules, lines = _diff_args(draws, lines)
+    clear = np.ascontiguousarray(clear, np.float64)
+    colors = np.ascontiguousarray(colors, np.float64).reshape(-1, 4)
+    image_grad = np.ascontiguousarray(image_grad, np.float64).reshape(height, width, 4)
+    line_grad = np.zeros_like(lines)
+    color_grad = np.zeros((len(draws), 4), np.float64)
+    lib().or_diff_backward_f64(C.c_uint32(width), C.c_uint32(height), _p(clear), C.c_uint32(len(draws)),
+                               _p(starts), _p(counts), _p(rules), _p(colors), _p(lines), _p(image_grad),
+                               _p(line_grad), _p(color_grad))
+    return line_grad, color_grad

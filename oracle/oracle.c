@@ -1523,3 +1523,253 @@ double or_polygon_pixel_area_f64(const uint32_t *lines, uint32_t n, double px, d
     }
     return total;
 }
+
+/* ------------------------------------------------------------------------ */
+/* The differentiable form in FP64: the checker of tk_frame_render_diff and   */
+/* tk_frame_backward.  Not a restatement of a reference routine (the          */
+/* reference has no backward pass): the forward is the fine composite         */
+/* (fine/main.hlsl:23-40, shared/pixel.hlsli:7-66) with exact pixel areas in  */
+/* place of the FP32 sums and no RGBA8 steps.                                 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int x0, y0, w, h; /* pixel box, clipped to the canvas */
+} DiffBox;
+
+static DiffBox diff_box(const double *lines, uint32_t start, uint32_t count, uint32_t width, uint32_t height) {
+    double xl = INFINITY, xh = -INFINITY, yl = INFINITY, yh = -INFINITY;
+    for (uint32_t i = start; i < start + count; ++i) {
+        const double *l = lines + 4 * (size_t)i;
+        xl = fmin(xl, fmin(l[0], l[2]));
+        xh = fmax(xh, fmax(l[0], l[2]));
+        yl = fmin(yl, fmin(l[1], l[3]));
+        yh = fmax(yh, fmax(l[1], l[3]));
+    }
+    DiffBox b = {0, 0, 0, 0};
+    if (!count || !(xh >= xl) || !(yh >= yl)) return b;
+    double fx0 = fmax(floor(xl), 0.0), fy0 = fmax(floor(yl), 0.0);
+    double fx1 = fmin(ceil(xh), (double)width), fy1 = fmin(ceil(yh), (double)height);
+    if (fx1 <= fx0 || fy1 <= fy0) return b;
+    b.x0 = (int)fx0;
+    b.y0 = (int)fy0;
+    b.w = (int)(fx1 - fx0);
+    b.h = (int)(fy1 - fy0);
+    return b;
+}
+
+/* Signed area accumulation: each line adds, in every row slab it crosses, its
+ * exact signed area to the right of it inside each cell and the rest of its dy
+ * to the next cell; a prefix sum along each row then gives the integral of the
+ * winding number over every pixel of the box (closed paths: zero outside the
+ * box of their lines).  dy is taken in line order (y at the first parameter
+ * minus y at the second), the sign convention of segment_row_parts. */
+static void diff_area(const double *lines, uint32_t start, uint32_t count, DiffBox b, double *area) {
+    const int W = b.w + 1;
+    double *acc = (double *)calloc((size_t)b.h * W, sizeof(double));
+    for (uint32_t i = start; i < start + count; ++i) {
+        const double *l = lines + 4 * (size_t)i;
+        const double x0 = l[0], y0 = l[1], dx = l[2] - l[0], dy = l[3] - l[1];
+        if (dy == 0.0) continue;
+        int r0 = (int)floor(fmin(y0, l[3])), r1 = (int)ceil(fmax(y0, l[3])) - 1;
+        if (r0 < b.y0) r0 = b.y0;
+        if (r1 > b.y0 + b.h - 1) r1 = b.y0 + b.h - 1;
+        for (int r = r0; r <= r1; ++r) {
+            double ta = ((double)r - y0) / dy, tb = ((double)(r + 1) - y0) / dy;
+            double sa = fmax(fmin(ta, tb), 0.0), sb = fmin(fmax(ta, tb), 1.0);
+            if (!(sb > sa)) continue;
+            double *row = acc + (size_t)(r - b.y0) * W;
+            /* pieces of [sa, sb] between consecutive integer x */
+            double xa = x0 + sa * dx, xb = x0 + sb * dx;
+            int c0 = (int)floor(fmin(xa, xb)), c1 = dx == 0.0 ? c0 : (int)ceil(fmax(xa, xb)) - 1;
+            for (int c = c0; c <= c1; ++c) {
+                double ua = sa, ub = sb;
+                if (dx != 0.0) {
+                    double qa = ((double)c - x0) / dx, qb = ((double)(c + 1) - x0) / dx;
+                    ua = fmax(fmin(qa, qb), sa);
+                    ub = fmin(fmax(qa, qb), sb);
+                }
+                if (!(ub > ua)) continue;
+                double py0 = y0 + ua * dy, py1 = y0 + ub * dy;
+                double d = py0 - py1; /* this piece's dy */
+                if (c < b.x0) {
+                    row[0] += d;
+                } else if (c < b.x0 + b.w) {
+                    double mx = 0.5 * ((x0 + ua * dx) + (x0 + ub * dx));
+                    double inside = d * ((double)(c + 1) - mx);
+                    row[c - b.x0] += inside;
+                    row[c - b.x0 + 1] += d - inside;
+                }
+            }
+        }
+    }
+    for (int y = 0; y < b.h; ++y) {
+        double s = 0.0;
+        for (int x = 0; x < b.w; ++x) {
+            s += acc[(size_t)y * W + x];
+            area[(size_t)y * b.w + x] = s;
+        }
+    }
+    free(acc);
+}
+
+static double diff_alpha64(double v, uint32_t rule, double *slope) {
+    if (rule == 1u) {
+        double w = v - 2.0 * nearbyint(0.5 * v);
+        *slope = w < 0.0 ? -1.0 : 1.0;
+        return fabs(w);
+    }
+    double m = fabs(v);
+    *slope = m < 1.0 ? (v < 0.0 ? -1.0 : 1.0) : 0.0;
+    return fmin(m, 1.0);
+}
+
+void or_diff_area_f64(const double *lines, uint32_t start, uint32_t count, uint32_t width, uint32_t height,
+                      int32_t *box, double *area) {
+    DiffBox b = diff_box(lines, start, count, width, height);
+    box[0] = b.x0;
+    box[1] = b.y0;
+    box[2] = b.w;
+    box[3] = b.h;
+    if (area && b.w && b.h) diff_area(lines, start, count, b, area);
+}
+
+void or_diff_render_f64(uint32_t width, uint32_t height, const double *clear, uint32_t draw_count,
+                        const uint32_t *line_start, const uint32_t *line_count, const uint32_t *rule,
+                        const double *colors, const double *lines, double *image) {
+    for (size_t p = 0; p < (size_t)width * height; ++p)
+        for (int k = 0; k < 4; ++k) image[4 * p + k] = clear[k];
+    for (uint32_t d = 0; d < draw_count; ++d) {
+        DiffBox b = diff_box(lines, line_start[d], line_count[d], width, height);
+        if (!b.w || !b.h) continue;
+        double *area = (double *)malloc((size_t)b.w * b.h * sizeof(double));
+        diff_area(lines, line_start[d], line_count[d], b, area);
+        const double *c = colors + 4 * (size_t)d;
+        for (int y = 0; y < b.h; ++y)
+            for (int x = 0; x < b.w; ++x) {
+                double slope, alpha = diff_alpha64(area[(size_t)y * b.w + x], rule[d], &slope);
+                double f = 1.0 - alpha * c[3];
+                double *px = image + 4 * ((size_t)(b.y0 + y) * width + b.x0 + x);
+                for (int k = 0; k < 4; ++k) px[k] = px[k] * f + alpha * c[k];
+            }
+        free(area);
+    }
+}
+
+/* dL/d(line endpoints) and dL/d(colour) for a given dL/d(image).  Per pixel,
+ * with T_k the product of (1 - alpha_j c_j.a) over the draws above k and R the
+ * composite below k: dL/dalpha_k = T_k (g.c_k - c_k.a g.R).  The line pass
+ * integrates dL/dcoverage along each line: over the part of the line with
+ * parameter in [ua, ub] inside a pixel, the pixel's coverage moves by
+ * -n.(dp0 int(1-s) + dp1 int s), n = (p0.y - p1.y, p1.x - p0.x). */
+void or_diff_backward_f64(uint32_t width, uint32_t height, const double *clear, uint32_t draw_count,
+                          const uint32_t *line_start, const uint32_t *line_count, const uint32_t *rule,
+                          const double *colors, const double *lines, const double *image_grad, double *line_grad,
+                          double *color_grad) {
+    const size_t N = (size_t)width * height;
+    DiffBox *boxes = (DiffBox *)calloc(draw_count ? draw_count : 1, sizeof(DiffBox));
+    double **maps = (double **)calloc(draw_count ? draw_count : 1, sizeof(double *));
+    double **trans = (double **)calloc(draw_count ? draw_count : 1, sizeof(double *));
+    for (uint32_t d = 0; d < draw_count; ++d) {
+        boxes[d] = diff_box(lines, line_start[d], line_count[d], width, height);
+        size_t n = (size_t)boxes[d].w * boxes[d].h;
+        if (!n) continue;
+        maps[d] = (double *)malloc(n * sizeof(double));
+        trans[d] = (double *)malloc(n * sizeof(double));
+        diff_area(lines, line_start[d], line_count[d], boxes[d], maps[d]);
+    }
+    /* transmittance above each draw, top to bottom */
+    double *T = (double *)malloc((N ? N : 1) * sizeof(double));
+    for (size_t p = 0; p < N; ++p) T[p] = 1.0;
+    for (uint32_t d = draw_count; d-- > 0;) {
+        DiffBox b = boxes[d];
+        for (int y = 0; y < b.h; ++y)
+            for (int x = 0; x < b.w; ++x) {
+                size_t p = (size_t)(b.y0 + y) * width + b.x0 + x, q = (size_t)y * b.w + x;
+                double slope, alpha = diff_alpha64(maps[d][q], rule[d], &slope);
+                trans[d][q] = T[p];
+                T[p] *= 1.0 - alpha * colors[4 * (size_t)d + 3];
+            }
+    }
+    /* bottom to top: R, the colour gradients, and dL/dcoverage in place of the coverage */
+    double *R = (double *)malloc((N ? N : 1) * 4 * sizeof(double));
+    for (size_t p = 0; p < N; ++p)
+        for (int k = 0; k < 4; ++k) R[4 * p + k] = clear[k];
+    for (uint32_t d = 0; d < draw_count; ++d) {
+        DiffBox b = boxes[d];
+        const double *c = colors + 4 * (size_t)d;
+        double *cg = color_grad + 4 * (size_t)d;
+        cg[0] = cg[1] = cg[2] = cg[3] = 0.0;
+        for (int y = 0; y < b.h; ++y)
+            for (int x = 0; x < b.w; ++x) {
+                size_t p = (size_t)(b.y0 + y) * width + b.x0 + x, q = (size_t)y * b.w + x;
+                double slope, alpha = diff_alpha64(maps[d][q], rule[d], &slope);
+                const double *g = image_grad + 4 * p;
+                double *r = R + 4 * p;
+                double t = trans[d][q];
+                double g_r = g[0] * r[0] + g[1] * r[1] + g[2] * r[2] + g[3] * r[3];
+                double g_c = g[0] * c[0] + g[1] * c[1] + g[2] * c[2] + g[3] * c[3];
+                maps[d][q] = t * (g_c - c[3] * g_r) * slope;
+                cg[0] += t * alpha * g[0];
+                cg[1] += t * alpha * g[1];
+                cg[2] += t * alpha * g[2];
+                cg[3] += t * alpha * (g[3] - g_r);
+                double f = 1.0 - alpha * c[3];
+                for (int k = 0; k < 4; ++k) r[k] = r[k] * f + alpha * c[k];
+            }
+    }
+    /* lines: integrate dL/dcoverage over the pixels each line crosses */
+    for (uint32_t d = 0; d < draw_count; ++d) {
+        DiffBox b = boxes[d];
+        for (uint32_t i = line_start[d]; i < line_start[d] + line_count[d]; ++i) {
+            const double *l = lines + 4 * (size_t)i;
+            double *out = line_grad + 4 * (size_t)i;
+            double g0 = 0.0, g1 = 0.0;
+            const double x0 = l[0], y0 = l[1], dx = l[2] - l[0], dy = l[3] - l[1];
+            if (b.w && b.h) {
+                int r0 = (int)floor(fmin(y0, l[3])), r1 = dy == 0.0 ? r0 : (int)ceil(fmax(y0, l[3])) - 1;
+                if (r0 < b.y0) r0 = b.y0;
+                if (r1 > b.y0 + b.h - 1) r1 = b.y0 + b.h - 1;
+                for (int r = r0; r <= r1; ++r) {
+                    double sa = 0.0, sb = 1.0;
+                    if (dy != 0.0) {
+                        double ta = ((double)r - y0) / dy, tb = ((double)(r + 1) - y0) / dy;
+                        sa = fmax(fmin(ta, tb), 0.0);
+                        sb = fmin(fmax(ta, tb), 1.0);
+                    }
+                    if (!(sb > sa)) continue;
+                    double xa = x0 + sa * dx, xb = x0 + sb * dx;
+                    int c0 = (int)floor(fmin(xa, xb)), c1 = dx == 0.0 ? c0 : (int)ceil(fmax(xa, xb)) - 1;
+                    if (c0 < b.x0) c0 = b.x0;
+                    if (c1 > b.x0 + b.w - 1) c1 = b.x0 + b.w - 1;
+                    for (int c = c0; c <= c1; ++c) {
+                        double ua = sa, ub = sb;
+                        if (dx != 0.0) {
+                            double qa = ((double)c - x0) / dx, qb = ((double)(c + 1) - x0) / dx;
+                            ua = fmax(fmin(qa, qb), sa);
+                            ub = fmin(fmax(qa, qb), sb);
+                        }
+                        if (!(ub > ua)) continue;
+                        double gv = maps[d][(size_t)(r - b.y0) * b.w + (c - b.x0)];
+                        double i1 = 0.5 * (ub * ub - ua * ua);
+                        g0 += gv * ((ub - ua) - i1);
+                        g1 += gv * i1;
+                    }
+                }
+            }
+            double nx = y0 - l[3], ny = dx;
+            out[0] = -nx * g0;
+            out[1] = -ny * g0;
+            out[2] = -nx * g1;
+            out[3] = -ny * g1;
+        }
+    }
+    for (uint32_t d = 0; d < draw_count; ++d) {
+        free(maps[d]);
+        free(trans[d]);
+    }
+    free(maps);
+    free(trans);
+    free(boxes);
+    free(T);
+    free(R);
+}

@@ -130,6 +130,23 @@ void or_brush_words(const uint32_t *paint, uint32_t paint_words, uint32_t paint_
 void or_fill_coverage_f64(uint32_t count, const uint32_t *requests, const uint32_t *segments, double *out);
 double or_polygon_pixel_area_f64(const uint32_t *lines, uint32_t line_count, double px, double py);
 
+/* The differentiable form in FP64 (checker of tk_frame_render_diff and
+ * tk_frame_backward).  lines: 4 doubles each (p0.x, p0.y, p1.x, p1.y); draw d
+ * fills lines [line_start[d], line_start[d] + line_count[d]) with rule[d] and
+ * the premultiplied colour colors[4d..4d+3]; images are 4 doubles per pixel.
+ * or_diff_area_f64: the box (x0, y0, w, h) of one draw's lines on the canvas
+ * and, when area is given, the integral of the winding number over each pixel
+ * of the box (w * h doubles, row-major). */
+void or_diff_area_f64(const double *lines, uint32_t start, uint32_t count, uint32_t width, uint32_t height,
+                      int32_t *box, double *area);
+void or_diff_render_f64(uint32_t width, uint32_t height, const double *clear, uint32_t draw_count,
+                        const uint32_t *line_start, const uint32_t *line_count, const uint32_t *rule,
+                        const double *colors, const double *lines, double *image);
+void or_diff_backward_f64(uint32_t width, uint32_t height, const double *clear, uint32_t draw_count,
+                          const uint32_t *line_start, const uint32_t *line_count, const uint32_t *rule,
+                          const double *colors, const double *lines, const double *image_grad, double *line_grad,
+                          double *color_grad);
+
 #ifdef __cplusplus
 }
 #endif

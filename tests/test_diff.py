@@ -1,0 +1,185 @@
+"""The differentiable form on the CPU: the FP64 oracle's areas against the exact
+polygon area of another oracle routine, its analytic gradient against FP64
+central finite differences (through the torch geometry: control points, stroke
+widths, colours), and the geometry's recorded lines against its polygon edges."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import tileink_amd as tk
+from tileink_amd import diff
+
+
+def weights(h, w, seed=0):
+    return np.random.default_rng(seed).normal(size=(h, w, 4))
+
+
+def analytic_grads(sc, w):
+    """dL/d(parameters) for L = sum(w * image) from the FP64 oracle's line and colour
+    gradients, carried through the geometry by torch."""
+    st = sc.structure()
+    params = [p for p in (sc.fills, sc.strokes, sc.widths, sc.colors) if p is not None]
+    for p in params:
+        p.grad = None
+        p.requires_grad_(True)
+    v, k = sc.polygons(st)
+    pm = sc.premultiplied()
+    draws, lines, la, lb = diff.oracle_inputs(sc, v, k)
+    lg, cg = oracle.diff_backward(sc.width, sc.height, diff.clear_color(sc), draws, pm.detach().numpy(), lines, w)
+    gv = torch.zeros_like(v)
+    lgt = torch.as_tensor(lg)
+    gv.index_add_(0, torch.as_tensor(la), lgt[:, :2])
+    gv.index_add_(0, torch.as_tensor(lb), lgt[:, 2:])
+    torch.autograd.backward([v, pm], [gv, torch.as_tensor(cg)])
+    return st
+
+
+def fd_loss(sc, st, w):
+    with torch.no_grad():
+        v, k = sc.polygons(st)
+        draws, lines, _, _ = diff.oracle_inputs(sc, v, k)
+        image = oracle.diff_render(sc.width, sc.height, diff.clear_color(sc), draws, sc.premultiplied().numpy(), lines)
+    return float((image * w).sum())
+
+
+def check_fd(sc, w, per_param=12, h=1e-3, seed=1, rtol=2e-5):
+    st = analytic_grads(sc, w)
+    rng = np.random.default_rng(seed)
+    checked = 0
+    for p in (sc.fills, sc.strokes, sc.widths, sc.colors):
+        if p is None:
+            continue
+        flat = p.data.view(-1)
+        g = p.grad.view(-1)
+        scale = float(g.abs().max())
+        for i in rng.choice(flat.numel(), size=min(per_param, flat.numel()), replace=False):
+            o = float(flat[i])
+            flat[i] = o + h
+            lp = fd_loss(sc, st, w)
+            flat[i] = o - h
+            lm = fd_loss(sc, st, w)
+            flat[i] = o
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - float(g[i])) <= rtol * max(abs(fd), 0.01 * scale) + 1e-9, (i, fd, float(g[i]))
+            checked += 1
+    return checked
+
+
+def test_oracle_area_matches_exact_polygon_area():
+    rng = np.random.default_rng(1)
+    for n in (3, 7, 12):
+        ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+        rad = rng.uniform(4, 14, n)
+        pts = np.stack([20.3 + rad * np.cos(ang), 17.7 + rad * np.sin(ang)], 1).astype(np.float32)
+        lines = np.concatenate([pts, np.roll(pts, -1, 0)], 1).astype(np.float64)
+        box, area = oracle.diff_area(lines, 0, n, 40, 36)
+        rec = np.zeros((n, 6), np.float32)
+        rec[:, 2:6] = lines
+        words = rec.view(np.uint32).copy()
+        for y in range(box[3]):
+            for x in range(box[2]):
+                exact = oracle.lib().or_polygon_pixel_area_f64(words.ctypes.data, n, float(box[0] + x),
+                                                               float(box[1] + y))
+                assert abs(exact - area[y, x]) < 1e-9
+        # the areas of a simple polygon sum to its shoelace area
+        shoelace = 0.5 * abs(np.sum(pts[:, 0].astype(np.float64) * np.roll(pts[:, 1], -1)
+                                    - np.roll(pts[:, 0], -1).astype(np.float64) * pts[:, 1]))
+        assert abs(abs(area.sum()) - shoelace) < 1e-9
+
+
+def test_oracle_area_clips_to_the_canvas_and_counts_winding():
+    # A square traced twice has winding 2 inside; half of it lies left of the canvas.
+    sq = np.array([[-3.5, 2.25], [4.5, 2.25], [4.5, 9.75], [-3.5, 9.75]])
+    ring = np.concatenate([sq, np.roll(sq, -1, 0)], 1)
+    lines = np.concatenate([ring, ring])
+    box, area = oracle.diff_area(lines, 0, 8, 16, 16)
+    assert list(box) == [0, 2, 5, 8]
+    assert np.isclose(area[1:7, 0:4], 2.0).all() or np.isclose(area[1:7, 0:4], -2.0).all()
+    assert np.isclose(abs(area[0, 0]), 1.5) and np.isclose(abs(area[1, 4]), 1.0)
+
+
+def test_oracle_gradient_of_one_polygon_matches_finite_differences():
+    rng = np.random.default_rng(2)
+    n, W, H = 9, 40, 36
+    ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+    rad = rng.uniform(5, 14, n)
+    pts = np.stack([20.3 + rad * np.cos(ang), 17.7 + rad * np.sin(ang)], 1)
+    draws, cols, clear = [(0, n, 0)], np.array([[0.3, 0.2, 0.1, 0.6]]), [0.1, 0.0, 0.2, 0.3]
+    w = weights(H, W, 3)
+    lines = lambda p: np.concatenate([p, np.roll(p, -1, 0)], 1)
+    lg, cg = oracle.diff_backward(W, H, clear, draws, cols, lines(pts), w)
+    loss = lambda p, c=cols: float((oracle.diff_render(W, H, clear, draws, c, lines(p)) * w).sum())
+    h = 1e-4
+    for i in range(n):
+        for k in range(2):
+            pp, pm = pts.copy(), pts.copy()
+            pp[i, k] += h
+            pm[i, k] -= h
+            fd = (loss(pp) - loss(pm)) / (2 * h)
+            assert abs(fd - (lg[i, k] + lg[i - 1, 2 + k])) < 1e-6 * max(1.0, abs(fd))
+    for k in range(4):
+        cp, cm = cols.copy(), cols.copy()
+        cp[0, k] += h
+        cm[0, k] -= h
+        assert abs((loss(pts, cp) - loss(pts, cm)) / (2 * h) - cg[0, k]) < 1e-6 * max(1.0, abs(cg[0, k]))
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_gradient_through_geometry_matches_finite_differences(seed):
+    sc = diff.random_scene(seed, 96, 80, fills=5, strokes=3, cubics=4, radius_mu=math.log(14.0))
+    assert check_fd(sc, weights(80, 96, seed)) >= 30
+
+
+def test_opaque_overlaps_and_even_odd_match_finite_differences():
+    # Opaque draws make zero factors in the transmittance; even-odd folds the winding.
+    sc = diff.random_scene(7, 64, 64, fills=6, strokes=2, cubics=3, radius_mu=math.log(16.0))
+    with torch.no_grad():
+        sc.colors[:3, 3] = 1.0
+    sc.rules[:] = 1
+    sc.clear = (0.2, 0.1, 0.0, 0.4)
+    assert check_fd(sc, weights(64, 64, 8)) >= 30
+
+
+def test_uniform_flattening_stays_within_tolerance():
+    sc = diff.random_scene(5, 128, 128, fills=8, cubics=4)
+    st = sc.structure()
+    v, k = sc.polygons(st)
+    cub = diff._fill_cubics(sc.fills).detach().numpy()
+    t = np.linspace(0, 1, 2001)
+    B = np.stack([(1 - t) ** 3, 3 * t * (1 - t) ** 2, 3 * t ** 2 * (1 - t), t ** 3], 1)
+    n = st.fill_steps
+    base = 0
+    for c in range(len(cub)):
+        curve = B @ cub[c]
+        # the chords of this cubic: its n points and the next cubic's first point
+        idx = np.arange(n[c] + 1) / n[c]
+        chord_pts = np.stack([np.interp(t, idx, (B[np.round(idx * 2000).astype(int)] @ cub[c])[:, j]) for j in (0, 1)], 1)
+        assert np.max(np.linalg.norm(curve - chord_pts, axis=1)) <= diff.TOLERANCE + 1e-9
+        base += n[c]
+
+
+def test_recorded_lines_are_the_polygon_edges():
+    sc = diff.random_scene(11, 120, 90, fills=6, strokes=4)
+    v, k = sc.polygons()
+    canvas, enc, la, lb, per_draw = diff.encode(sc, v, k)  # raises when they differ
+    assert enc.line_count == len(la) == int(per_draw.sum())
+    assert enc.draw_count == len(k) and enc.layer_count == 0
+    assert per_draw.min() >= 3
+
+
+def test_stroke_outline_width_is_the_stroke_width():
+    # A straight stroke: its outline is a capsule of the given width.
+    strokes = torch.tensor([[[10.0, 20.0], [20.0, 20.0], [30.0, 20.0], [40.0, 20.0]]], dtype=torch.float64)
+    sc = diff.DiffScene(64, 48, strokes=strokes, widths=torch.tensor([6.0], dtype=torch.float64),
+                        colors=torch.tensor([[1.0, 0.0, 0.0, 1.0]], dtype=torch.float64))
+    v, k = sc.polygons()
+    p = v.detach().numpy()
+    assert np.isclose(p[:, 1].max(), 23.0) and np.isclose(p[:, 1].min(), 17.0)
+    assert np.isclose(p[:, 0].max(), 43.0) and np.isclose(p[:, 0].min(), 7.0)
+    draws, lines, _, _ = diff.oracle_inputs(sc, v, k)
+    image = oracle.diff_render(64, 48, [0, 0, 0, 0], draws, sc.premultiplied().detach().numpy(), lines)
+    # capsule area: 30 x 6 rectangle + a disc of radius 3 (its polygon is slightly smaller)
+    assert 30 * 6 + 0.97 * math.pi * 9 < image[:, :, 3].sum() <= 30 * 6 + math.pi * 9

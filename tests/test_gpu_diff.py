@@ -1,0 +1,165 @@
+"""The differentiable form on the gfx950 device (tk_frame_render_diff,
+tk_frame_backward) through the torch.autograd harness:
+
+* the FP32 image against the FP64 oracle's exact-area composite;
+* parameter gradients (control points, stroke widths, colours) against the FP64
+  analytic gradient and against FP64 central finite differences (h = 1e-3 px),
+  within 1e-3 relative;
+* a batch of scenes against each scene alone;
+* scenes outside the form (layers) are rejected."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import tileink_amd as tk
+from tileink_amd import diff
+
+from test_diff import analytic_grads, fd_loss
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def gpu_grads(ctx, sc, w):
+    params = [p for p in (sc.fills, sc.strokes, sc.widths, sc.colors) if p is not None]
+    for p in params:
+        p.grad = None
+        p.requires_grad_(True)
+    image = diff.render(ctx, sc, DEV)
+    loss = (image.double() * torch.as_tensor(w, device=DEV)).sum()
+    loss.backward()
+    return image, [p.grad.clone() for p in params]
+
+
+def oracle_image(sc):
+    """The FP64 composite of the lines as recorded (FP32 vertices)."""
+    v, k = sc.polygons()
+    draws, lines, _, _ = diff.oracle_inputs(sc, v, k, recorded=True)
+    return oracle.diff_render(sc.width, sc.height, diff.clear_color(sc), draws,
+                              sc.premultiplied().detach().numpy(), lines)
+
+
+def assert_grads_close(got, ref, rtol=1e-3, floor=0.02):
+    """|got - ref| <= rtol * max(|ref|, floor * max|ref|) per parameter."""
+    for g, r in zip(got, ref):
+        g, r = g.cpu().double().numpy().ravel(), r.cpu().double().numpy().ravel()
+        scale = np.abs(r).max()
+        bound = rtol * np.maximum(np.abs(r), floor * scale)
+        bad = np.abs(g - r) > bound + 1e-12
+        assert not bad.any(), (int(bad.sum()), g[bad][:5], r[bad][:5])
+
+
+@pytest.mark.parametrize("seed", [3, 4, 5])
+def test_forward_matches_fp64_oracle(ctx, seed):
+    sc = diff.random_scene(seed, 160, 112, fills=24, strokes=8, radius_mu=math.log(18.0))
+    image = diff.render(ctx, sc, DEV).cpu().double().numpy()
+    ref = oracle_image(sc)
+    assert image.shape == ref.shape
+    err = float(np.abs(image - ref).max())
+    assert err < 1e-5, err
+
+
+@pytest.mark.parametrize("seed", [3, 6])
+def test_gradients_match_fp64_analytic(ctx, seed):
+    sc = diff.random_scene(seed, 160, 112, fills=24, strokes=8, radius_mu=math.log(18.0))
+    w = np.random.default_rng(seed).normal(size=(112, 160, 4))
+    _, got = gpu_grads(ctx, sc, w)
+    analytic_grads(sc, w)
+    ref = [p.grad for p in (sc.fills, sc.strokes, sc.widths, sc.colors)]
+    assert_grads_close(got, ref)
+
+
+def test_gradients_with_opaque_overlaps_and_even_odd(ctx):
+    sc = diff.random_scene(9, 96, 96, fills=10, strokes=4, cubics=3, radius_mu=math.log(20.0))
+    with torch.no_grad():
+        sc.colors[::2, 3] = 1.0
+    sc.rules[:] = 1
+    sc.clear = (0.2, 0.1, 0.0, 0.4)
+    w = np.random.default_rng(9).normal(size=(96, 96, 4))
+    image, got = gpu_grads(ctx, sc, w)
+    err = float(np.abs(image.detach().cpu().double().numpy() - oracle_image(sc)).max())
+    assert err < 1e-5, err
+    analytic_grads(sc, w)
+    assert_grads_close(got, [p.grad for p in (sc.fills, sc.strokes, sc.widths, sc.colors)])
+
+
+def test_gradients_match_finite_differences_at_1024(ctx):
+    # 1024^2, 2048 closed paths of 4 cubics, loss against a target whose control
+    # points are perturbed by N(0, 4 px) and colours by N(0, 0.05); central
+    # differences in FP64 (h = 1e-3 px) on 64 random parameters.
+    sc = diff.random_scene(41, 1024, 1024, fills=2048, cubics=4)
+    tgt = diff.random_scene(41, 1024, 1024, fills=2048, cubics=4)
+    rng = np.random.default_rng(42)
+    with torch.no_grad():
+        tgt.fills += torch.as_tensor(rng.normal(0, 4, tuple(tgt.fills.shape)))
+        tgt.colors += torch.as_tensor(rng.normal(0, 0.05, tuple(tgt.colors.shape)))
+        tgt.colors.clamp_(0, 1)
+        target = diff.render(ctx, tgt, DEV).double()
+    image = diff.render(ctx, sc, DEV)
+    # Segment positions are absolute FP32 coordinates: near x = 1024 one ulp is
+    # 6.1e-5 px, so coverage (and the image) carries errors of that order here.
+    err = float(np.abs(image.detach().cpu().double().numpy() - oracle_image(sc)).max())
+    assert err < 5e-5, err
+    sc.fills.requires_grad_(True)
+    sc.colors.requires_grad_(True)
+    image = diff.render(ctx, sc, DEV)
+    loss = ((image.double() - target) ** 2).sum()
+    loss.backward()
+    w = 2.0 * (image.detach().double() - target).cpu().numpy()  # dL/dimage, fixed for the differences
+    st = sc.structure()
+    h = 1e-3
+    picks = [("fills", sc.fills, i) for i in rng.choice(sc.fills.numel(), 48, replace=False)] + \
+            [("colors", sc.colors, i) for i in rng.choice(sc.colors.numel(), 16, replace=False)]
+    scale = {n: float(p.grad.abs().max()) for n, p in (("fills", sc.fills), ("colors", sc.colors))}
+    for name, p, i in picks:
+        flat = p.data.view(-1)
+        o = float(flat[i])
+        flat[i] = o + h
+        lp = fd_loss(sc, st, w)
+        flat[i] = o - h
+        lm = fd_loss(sc, st, w)
+        flat[i] = o
+        fd = (lp - lm) / (2 * h)
+        g = float(p.grad.view(-1)[i])
+        assert abs(g - fd) <= 1e-3 * max(abs(fd), 0.02 * scale[name]), (name, i, g, fd)
+
+
+def test_batch_equals_each_scene_alone(ctx):
+    scenes = [diff.random_scene(20 + i, w, h, fills=f, strokes=s, radius_mu=math.log(12.0))
+              for i, (w, h, f, s) in enumerate([(64, 48, 12, 3), (200, 90, 30, 6), (33, 130, 8, 2)])]
+    ws = [np.random.default_rng(i).normal(size=(s.height, s.width, 4)) for i, s in enumerate(scenes)]
+    singles = []
+    for sc, w in zip(scenes, ws):
+        image, grads = gpu_grads(ctx, sc, w)
+        singles.append((image.detach().cpu().numpy(), grads))
+    for sc in scenes:
+        for p in (sc.fills, sc.strokes, sc.widths, sc.colors):
+            p.grad = None
+    images = diff.render(ctx, scenes, DEV)
+    loss = sum((im.double() * torch.as_tensor(w, device=DEV)).sum() for im, w in zip(images, ws))
+    loss.backward()
+    for sc, im, (ref_im, ref_g) in zip(scenes, images, singles):
+        assert np.array_equal(im.detach().cpu().numpy(), ref_im)
+        # Line gradients are summed in a fixed order; colour gradients with float
+        # atomics, so they agree to rounding.
+        assert_grads_close([p.grad for p in (sc.fills, sc.strokes, sc.widths, sc.colors)], ref_g, rtol=1e-4)
+
+
+def test_scenes_with_layers_are_rejected(ctx):
+    canvas = tk.Canvas(64, 64)
+    canvas.push_clip_sdf_rect_layer(4.0, 4.0, 60.0, 60.0, 3.0)
+    canvas.push_polygon(np.array([[10.0, 10.0], [50.0, 12.0], [30.0, 50.0]]), tk.Color(1, 0, 0, 1))
+    canvas.pop_layer()
+    frame = tk.Frame(ctx, canvas.encode(0))
+    colors = torch.ones((1, 4), device=DEV)
+    image = torch.zeros((64 * 64, 4), device=DEV)
+    with pytest.raises(tk.TileinkError):
+        tk._abi.check(tk._abi.lib().tk_frame_render_diff(frame._h, colors.data_ptr(), image.data_ptr()))
+    grad = torch.zeros_like(image)
+    with pytest.raises(tk.TileinkError):  # no differentiable render ran
+        tk._abi.check(tk._abi.lib().tk_frame_backward(frame._h, colors.data_ptr(), grad.data_ptr(),
+                                                      image.data_ptr(), colors.data_ptr()))

@@ -260,6 +260,20 @@ class Canvas:
         self._scene = None
         return out.value
 
+    def push_polygon(self, points, brush, rule=FillRule.NonZero):
+        """A closed polygon, move_to + line_to per point + close_path, from an (n, 2) array."""
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 2))
+        if len(pts) == 0:
+            return None
+        verbs = np.full(len(pts) + 1, _abi.PATH_LINE, np.uint8)
+        verbs[0], verbs[-1] = _abi.PATH_MOVE, _abi.PATH_CLOSE
+        b, keep = _brush_arg(brush)
+        out = C.c_uint32()
+        check(lib().tk_canvas_push_path(self._h, ptr(verbs), len(verbs), ptr(pts), len(pts), C.byref(b), None,
+                                        int(rule), 0.25, C.byref(out)))
+        self._scene = None
+        return out.value
+
     def push_stroke(self, path, width, brush, transform=None, rule=FillRule.NonZero, tolerance=0.25):
         verbs, points = self._path_args(path)
         b, keep = _brush_arg(brush)

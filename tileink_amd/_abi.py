@@ -190,6 +190,9 @@ SIGNATURES = {
     "tk_frame_render": (C.c_int, [_P]),
     "tk_frame_render_tiles": (C.c_int, [_P, _P, _U32]),
     "tk_frame_set_graph": (C.c_int, [_P, C.c_int]),
+    "tk_frame_render_diff": (C.c_int, [_P, _P, _P]),
+    "tk_frame_backward": (C.c_int, [_P, _P, _P, _P, _P]),
+    "tk_frame_diff_status": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "tk_batch_create": (C.c_int, [_P, _P, _U32, _P]),
     "tk_batch_render": (C.c_int, [_P]),
     "tk_batch_read_image": (C.c_int, [_P, _U32, _P, _SZ]),

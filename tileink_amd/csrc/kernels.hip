@@ -1698,3 +1698,296 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_brush_words(tk_args_brush
     const uint32_t *q = a.requests + (size_t)id * 4u;
     a.out[id] = sample_brush(a.paint, a.paint_words, a.paint_brush_base, q[0], asfloat(q[1]), asfloat(q[2]));
 }
+
+// ===========================================================================
+// The differentiable form.
+//
+// Forward: the fine walk over FILL records with an FP32 premultiplied
+// source-over composite, per-draw float colours and no RGBA8 quantization;
+// coverage is computed exactly as in tk_k_fine.  Each (record, pixel)
+// coverage is kept in `adj` for the backward pass.
+//
+// Backward, per pixel, with g = dL/dI and f_k = 1 - alpha_k * c_k.a:
+//   I = sum_k alpha_k c_k T_k,  T_k = prod_{j>k} f_j,
+//   dL/dalpha_k = T_k (g.c_k - c_k.a g.R_{k-1}),   R_{k-1} = composite below k,
+//   dL/dc_k     = T_k alpha_k g  (minus T_k alpha_k g.R_{k-1} on the alpha channel).
+// A first sweep forms the stream's product of the nonzero f_j (mantissa and
+// exponent kept apart, so it never underflows) and counts the zero ones; the
+// second sweep rebuilds R_{k-1} bottom-up and takes T_k as the ratio of the
+// total to the running product (zero while a zero factor lies above k).  The
+// coverage derivative (dL/dcoverage = dL/dalpha * dalpha/dcoverage) replaces
+// the coverage in `adj`, and the line kernel turns it into endpoint gradients
+// with the boundary form of the area derivative: a pixel's coverage is the
+// integral of the winding number over it, so moving a line changes it by the
+// integral, over the part of the line inside the pixel, of the motion's
+// component along the line's normal.
+// ===========================================================================
+
+namespace {
+
+// The FP32 alpha fill_alpha_at derives from a coverage (before the u8 step),
+// with its derivative: NonZero min(|v|, 1), EvenOdd |v - 2 rint(v / 2)|.
+__device__ __forceinline__ float diff_alpha(float v, uint32_t rule, float &slope) {
+    if (rule == 1u) {
+        const float w = v - 2.0f * rintf(0.5f * v);
+        slope = w < 0.0f ? -1.0f : 1.0f;
+        return fabsf(w);
+    }
+    const float m = fabsf(v);
+    slope = m < 1.0f ? (v < 0.0f ? -1.0f : 1.0f) : 0.0f;
+    return fminf(m, 1.0f);
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Product of the nonzero factors as mantissa * 2^exponent, and the zero count.
+struct Transmit {
+    float m;
+    int e, zeros;
+    __device__ __forceinline__ void mul(float f) {
+        if (f == 0.0f) {
+            ++zeros;
+        } else {
+            int x;
+            m = frexpf(m * f, &x);
+            e += x;
+        }
+    }
+};
+
+struct DiffTile {
+    SceneView v;
+    uint32_t tile, lane, gx, gy;
+    bool inside;
+};
+
+template <bool kScenes>
+__device__ __forceinline__ bool diff_tile(const tk_args_fine_diff &a, DiffTile &t) {
+    const uint32_t linear = blockIdx.x + blockIdx.y * gridDim.x;
+    const uint32_t columns = a.list.columns;
+    if (linear >= xcd_grid(columns, TK_XCD_CHUNK_FINE)) return false;
+    const uint32_t column = xcd_chunked_block(linear, TK_XCD_CHUNK_FINE);
+    if (column >= columns) return false;
+    const tk_fine_config &cfg = a.cfg;
+    t.v = scene_view<kScenes>(a.scenes, column, columns, cfg.batch_count, cfg.tiles_width, cfg.tiles_height, cfg.width,
+                              cfg.height, cfg.clear_color);
+    t.tile = a.list.tiles ? a.list.tiles[column] : t.v.local;
+    if (t.tile >= cfg.tile_count) return false;
+    t.lane = threadIdx.x;
+    t.gx = (t.tile % t.v.tw) * 16u + (t.lane & 15u);
+    t.gy = (t.tile / t.v.tw) * 16u + (t.lane >> 4);
+    t.inside = t.gx < t.v.width && t.gy < t.v.height;
+    return true;
+}
+
+// Calls body(pi, rec) for every FILL record of the tile's streams, bottom to
+// top; flags any other record kind.
+template <class Body>
+__device__ __forceinline__ void for_each_fill(const tk_args_fine_diff &a, const DiffTile &t, Body body) {
+    const tk_fine_config &cfg = a.cfg;
+    for (uint32_t b = t.v.q0; b < t.v.q1; ++b) {
+        const size_t idx = table_index(t.v, b);
+        const uint32_t kind_word = a.in.tile_kinds[idx];
+        if ((kind_word & TK_TILE_KIND_MASK) == TK_TILE_KIND_EMPTY) continue;
+        const tk_segment_range r = a.in.tile_ptcl_ranges[idx];
+        const uint32_t pend = min(r.end, cfg.ptcl_capacity);
+        for (uint32_t pi = r.start + (kind_word >> TK_TILE_PREFIX_SHIFT); pi < pend; ++pi) {
+            const tk_ptcl_record rec = a.in.ptcl[pi];
+            if (rec.tag == TK_PTCL_END) break;
+            if (rec.tag != TK_PTCL_FILL || rec.color >= cfg.draw_count) {
+                if (t.lane == 0u && a.status) atomicOr(a.status, 1u);
+                continue;
+            }
+            body(pi, rec);
+        }
+    }
+}
+
+template <bool kScenes>
+__device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &a) {
+    __shared__ CoverageAcc cov_acc[2];
+    DiffTile t;
+    if (!diff_tile<kScenes>(a, t)) return;
+    const uint32_t lane = t.lane;
+    cov_acc[0].run[lane] = 0;
+    cov_acc[0].part[lane] = 0;
+    cov_acc[1].run[lane] = 0;
+    cov_acc[1].part[lane] = 0;
+    if (lane < 16u) {
+        cov_acc[0].base[lane] = 0;
+        cov_acc[1].base[lane] = 0;
+    }
+    uint32_t sel = 0u;
+    __syncthreads();
+    const float4 *colors = reinterpret_cast<const float4 *>(a.colors);
+    Px px = rgba8_to_unorm(t.v.clear);
+    bool overflow = false;
+    for_each_fill(a, t, [&](uint32_t pi, const tk_ptcl_record &rec) {
+        const uint32_t s0 = rec.segment_start, s1 = min(rec.segment_end, a.cfg.segment_capacity);
+        const float4 c = colors[rec.color];
+        CoverageAcc &acc = cov_acc[sel];
+        sel ^= 1u;
+        accumulate_record(a.in.segments + s0, s1 > s0 ? s1 - s0 : 0u, acc, lane);
+        __syncthreads();
+        const float cov = take_coverage(acc, rec.backdrop, lane);
+        float slope;
+        const float alpha = diff_alpha(cov, rec.fill_rule, slope);
+        const float f = 1.0f - alpha * c.w;
+        px.r = px.r * f + alpha * c.x;
+        px.g = px.g * f + alpha * c.y;
+        px.b = px.b * f + alpha * c.z;
+        px.a = px.a * f + alpha * c.w;
+        if (pi < a.adj_records) a.adj[(size_t)pi * 256u + lane] = cov;
+        else overflow = true;
+    });
+    if (overflow && lane == 0u && a.status) atomicOr(a.status, 2u);
+    if (t.inside)
+        reinterpret_cast<float4 *>(a.image)[t.v.image + (size_t)t.gy * t.v.width + t.gx] =
+            make_float4(px.r, px.g, px.b, px.a);
+}
+
+template <bool kScenes>
+__device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff &a) {
+    DiffTile t;
+    if (!diff_tile<kScenes>(a, t)) return;
+    const uint32_t lane = t.lane;
+    const float4 *colors = reinterpret_cast<const float4 *>(a.colors);
+    float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (t.inside) g = reinterpret_cast<const float4 *>(a.image_grad)[t.v.image + (size_t)t.gy * t.v.width + t.gx];
+    auto coverage = [&](uint32_t pi) { return pi < a.adj_records ? a.adj[(size_t)pi * 256u + lane] : 0.0f; };
+
+    Transmit total{1.0f, 0, 0};
+    for_each_fill(a, t, [&](uint32_t pi, const tk_ptcl_record &rec) {
+        float slope;
+        const float alpha = diff_alpha(coverage(pi), rec.fill_rule, slope);
+        total.mul(1.0f - alpha * colors[rec.color].w);
+    });
+
+    const uint32_t tx = t.tile % t.v.tw, ty = t.tile / t.v.tw;
+    Transmit below{1.0f, 0, 0};
+    Px res = rgba8_to_unorm(t.v.clear);
+    for_each_fill(a, t, [&](uint32_t pi, const tk_ptcl_record &rec) {
+        float slope;
+        const float cov = coverage(pi);
+        const float alpha = diff_alpha(cov, rec.fill_rule, slope);
+        const float4 c = colors[rec.color];
+        const float f = 1.0f - alpha * c.w;
+        below.mul(f);
+        const float T = total.zeros > below.zeros ? 0.0f : ldexpf(total.m / below.m, total.e - below.e);
+        const float g_res = g.x * res.r + g.y * res.g + g.z * res.b + g.w * res.a;
+        const float g_c = g.x * c.x + g.y * c.y + g.z * c.z + g.w * c.w;
+        const float d_alpha = T * (g_c - c.w * g_res);
+        if (pi < a.adj_records) a.adj[(size_t)pi * 256u + lane] = d_alpha * slope;
+        const float w = T * alpha;
+        const float cr = wave_sum(w * g.x), cg = wave_sum(w * g.y), cb = wave_sum(w * g.z),
+                    ca = wave_sum(w * (g.w - g_res));
+        if ((lane & 63u) == 0u) {
+            float *out = a.color_grad + (size_t)rec.color * 4u;
+            atomicAdd(out + 0, cr);
+            atomicAdd(out + 1, cg);
+            atomicAdd(out + 2, cb);
+            atomicAdd(out + 3, ca);
+        }
+        res.r = res.r * f + alpha * c.x;
+        res.g = res.g * f + alpha * c.y;
+        res.b = res.b * f + alpha * c.z;
+        res.a = res.a * f + alpha * c.w;
+        if (lane == 0u) {
+            const uint32_t path = a.in.draws[rec.color].path_id;
+            if (path < a.path_count) {
+                const tk_path_record &p = a.paths[path];
+                if (tx >= p.tile_x0 && ty >= p.tile_y0 && tx < p.tile_x1 && ty < p.tile_y1) {
+                    const uint32_t bd = p.data_offset + (ty - p.tile_y0) * (p.tile_x1 - p.tile_x0) + tx - p.tile_x0;
+                    if (bd < a.rec_of_count) a.rec_of[bd] = pi;
+                }
+            }
+        }
+    });
+}
+
+}  // namespace
+
+extern "C" __global__ __launch_bounds__(256) void tk_k_fine_diff(tk_args_fine_diff a) {
+    fine_diff_forward_tile<false>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void tk_k_fine_diff_scenes(tk_args_fine_diff a) {
+    fine_diff_forward_tile<true>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void tk_k_fine_backward(tk_args_fine_diff a) {
+    fine_diff_backward_tile<false>(a);
+}
+extern "C" __global__ __launch_bounds__(256) void tk_k_fine_backward_scenes(tk_args_fine_diff a) {
+    fine_diff_backward_tile<true>(a);
+}
+
+// One thread per line: walks the pixels the line crosses inside its path's
+// tile box, row slab by row slab and then column by column, and integrates
+// dL/dcoverage along it.  For the part of the line with parameter s in
+// [s_a, s_b] inside a pixel, the pixel's coverage moves by
+//   -n . (dp0 * int (1 - s) ds + dp1 * int s ds),   n = (p0.y - p1.y, p1.x - p0.x),
+// n pointing to the side where the winding number is one higher.
+extern "C" __global__ __launch_bounds__(256) void tk_k_line_backward(tk_args_line_backward a) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.line_count) return;
+    const tk_line ln = a.lines[i];
+    const float x0 = ln.p0[0], y0 = ln.p0[1], x1 = ln.p1[0], y1 = ln.p1[1];
+    const float dx = x1 - x0, dy = y1 - y0;
+    float g0 = 0.0f, g1 = 0.0f;
+    if (ln.path_id < a.path_count) {
+        const tk_path_record p = a.paths[ln.path_id];
+        if (p.tile_x1 > p.tile_x0 && p.tile_y1 > p.tile_y0) {
+            const int cx0 = (int)p.tile_x0 * 16, cx1 = (int)p.tile_x1 * 16 - 1;
+            const int cy0 = (int)p.tile_y0 * 16, cy1 = (int)p.tile_y1 * 16 - 1;
+            const uint32_t bw = p.tile_x1 - p.tile_x0;
+            // Clamped in float before the conversions, so huge coordinates stay in range.
+            const float ylo = fminf(y0, y1), yhi = fmaxf(y0, y1);
+            const int r0 = max((int)floorf(fmaxf(ylo, (float)cy0 - 1.0f)), cy0);
+            const int r1 = min(dy == 0.0f ? (int)floorf(fminf(ylo, (float)cy1 + 1.0f))
+                                          : (int)ceilf(fminf(yhi, (float)cy1 + 2.0f)) - 1,
+                               cy1);
+            for (int r = r0; r <= r1; ++r) {
+                float sa = 0.0f, sb = 1.0f;
+                if (dy != 0.0f) {
+                    const float ta = ((float)r - y0) / dy, tb = ((float)(r + 1) - y0) / dy;
+                    sa = fmaxf(fminf(ta, tb), 0.0f);
+                    sb = fminf(fmaxf(ta, tb), 1.0f);
+                }
+                if (!(sb > sa)) continue;
+                const float xa = x0 + sa * dx, xb = x0 + sb * dx;
+                const float xlo = fminf(xa, xb), xhi = fmaxf(xa, xb);
+                const int c0 = max((int)floorf(fmaxf(xlo, (float)cx0 - 1.0f)), cx0);
+                const int c1 = min(dx == 0.0f ? (int)floorf(fminf(xlo, (float)cx1 + 1.0f))
+                                              : (int)ceilf(fminf(xhi, (float)cx1 + 2.0f)) - 1,
+                                   cx1);
+                const uint32_t row_base = p.data_offset + ((uint32_t)r / 16u - p.tile_y0) * bw;
+                for (int c = c0; c <= c1; ++c) {
+                    float ua = sa, ub = sb;
+                    if (dx != 0.0f) {
+                        const float ta = ((float)c - x0) / dx, tb = ((float)(c + 1) - x0) / dx;
+                        ua = fmaxf(fminf(ta, tb), sa);
+                        ub = fminf(fmaxf(ta, tb), sb);
+                    }
+                    if (!(ub > ua)) continue;
+                    const uint32_t bd = row_base + (uint32_t)c / 16u - p.tile_x0;
+                    if (bd >= a.rec_of_count) continue;
+                    const uint32_t k = a.rec_of[bd];
+                    if (k >= a.adj_records) continue;  // no record there (kNone), or beyond the buffer
+                    const float gv = a.adj[(size_t)k * 256u + ((uint32_t)r & 15u) * 16u + ((uint32_t)c & 15u)];
+                    const float i1 = 0.5f * (ub * ub - ua * ua);
+                    g0 += gv * ((ub - ua) - i1);
+                    g1 += gv * i1;
+                }
+            }
+        }
+    }
+    const float nx = y0 - y1, ny = dx;
+    float *out = a.line_grad + (size_t)i * 4u;
+    out[0] = -nx * g0;
+    out[1] = -ny * g0;
+    out[2] = -nx * g1;
+    out[3] = -ny * g1;
+}

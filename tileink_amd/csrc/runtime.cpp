@@ -75,7 +75,9 @@ struct tk_context {
                   f_coarse_scan = nullptr, f_coarse_emit = nullptr, f_layer_classes = nullptr, f_active_paths = nullptr,
                   f_draw_brush = nullptr,
                   f_fine = nullptr, f_coarse_count_scenes = nullptr, f_coarse_emit_scenes = nullptr,
-                  f_fine_scenes = nullptr, f_fill_coverage = nullptr, f_pixel_math = nullptr, f_brush_words = nullptr;
+                  f_fine_scenes = nullptr, f_fine_diff = nullptr, f_fine_diff_scenes = nullptr,
+                  f_fine_backward = nullptr, f_fine_backward_scenes = nullptr, f_line_backward = nullptr,
+                  f_fill_coverage = nullptr, f_pixel_math = nullptr, f_brush_words = nullptr;
     // Scratch for the stage entry points (frames own their own).
     uint32_t *tickets = nullptr;  // [0] scan prefix, [1] coarse scan
     unsigned long long *status = nullptr;
@@ -326,6 +328,12 @@ struct tk_frame {
     hipGraphExec_t graph_exec = nullptr;
     hipEvent_t ev[13] = {};
     bool have_times = false;
+    // the differentiable form (tk_frame_render_diff, tk_frame_backward)
+    bool diff_scene = false;    // path draws only, no layers
+    bool diff_ready = false;    // a tk_frame_render_diff ran since the last upload
+    float *diff_adj = nullptr;  // 256 floats per PTCL record, grown on demand
+    uint64_t diff_adj_records = 0;
+    uint32_t *diff_rec_of = nullptr, *diff_status = nullptr;
 };
 
 namespace tk {
@@ -386,7 +394,22 @@ uint32_t layer_pairs(const tk_scene *s, std::vector<uint32_t> &pair_of, std::vec
 // coarse kernels evaluate each layer themselves.
 constexpr uint64_t kMaxLayerClassBytes = 64ull << 20;
 
+// The differentiable form takes scenes of path draws (fills and strokes) with
+// no layers: every PTCL record is then a FILL carrying its draw index.
+bool diff_compatible(const tk_scene *s) {
+    if (s->layer_count != 0) return false;
+    for (uint32_t d = 0; d < s->draw_count; ++d) {
+        const tk_draw_record &r = s->draws[d];
+        if (r.tag != TK_DRAW_BRUSH || r.path_id >= s->path_count || r.sdf_offset != TK_INVALID_INDEX ||
+            r.sdf_shadow_offset != TK_INVALID_INDEX)
+            return false;
+    }
+    return true;
+}
+
 tk_status frame_upload(tk_frame *f, const tk_scene *s) {
+    f->diff_scene = diff_compatible(s);
+    f->diff_ready = false;
     TK_TRY(upload(f, f->lines, s->lines, s->line_count));
     TK_TRY(upload(f, f->paths, s->paths, s->path_count));
     TK_TRY(upload(f, f->draws, s->draws, s->draw_count));
@@ -494,7 +517,70 @@ tk_status record(tk_frame *f, int mark) {
 // with f->damage_sat uploaded: only paths whose tile box meets a listed tile are
 // scanned, and coarse and fine run on the listed tiles, whose tables take one
 // column each in list order.  Other tiles of the image keep their pixels.
-tk_status frame_render(tk_frame *f, uint32_t active_count = 0) {
+// Where a differentiable render sends its fine stage.
+struct DiffTarget {
+    const float *colors;
+    float *image;
+};
+
+tk_args_fine_diff diff_args(tk_frame *f, const float *colors) {
+    tk_args_fine_diff a{};
+    a.cfg = frame_fine_config(f);
+    a.in.draws = f->draws;
+    a.in.paint = f->paint;
+    a.in.segments = f->segments;
+    a.in.tile_ptcl_ranges = f->tile_ranges;
+    a.in.tile_kinds = f->tile_kinds;
+    a.in.batch_ptcl_bases = f->batch_bases;
+    a.in.ptcl = f->ptcl;
+    a.paths = f->paths;
+    a.path_count = f->meta.path_count;
+    a.rec_of_count = f->meta.backdrop_len;
+    a.colors = colors;
+    a.adj = f->diff_adj;
+    a.adj_records = f->diff_adj_records;
+    a.rec_of = f->diff_rec_of;
+    a.status = f->diff_status;
+    a.list = tk_tile_list{nullptr, a.cfg.tile_count};
+    a.scenes = tk_scene_slots{f->scene_slots, f->scene_of_column, f->scene_count, f->table_entries};
+    return a;
+}
+
+tk_status launch_tiles(tk_context *ctx, hipFunction_t fn, uint32_t columns, const tk_args_fine_diff &a) {
+    if (columns == 0) return TK_OK;
+    const uint32_t blocks = xcd_grid(columns, TK_XCD_CHUNK_FINE);
+    const uint32_t gx = std::min<uint32_t>(blocks, 65535u);
+    return launch(ctx, fn, gx, div_up(blocks, gx), 256, a);
+}
+
+// The fine stage of a differentiable render: sizes the coverage buffer from the
+// PTCL record count coarse wrote (one host round trip), then composites.
+tk_status run_fine_diff(tk_frame *f, const DiffTarget &t) {
+    tk_context *ctx = f->ctx;
+    uint32_t records = 0;
+    TK_HIP(hipMemcpyAsync(&records, f->status + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    const uint64_t need = std::min<uint64_t>(records, f->ptcl_capacity);
+    if (need > f->diff_adj_records || !f->diff_adj) {
+        if (f->diff_adj) TK_HIP(hipFree(f->diff_adj));
+        f->diff_adj = nullptr;
+        f->diff_adj_records = 0;
+        const uint64_t cap = std::max<uint64_t>(need + need / 4, 64);
+        hipError_t e = hipMalloc(&f->diff_adj, cap * 256 * sizeof(float));
+        if (e != hipSuccess) {
+            f->diff_adj = nullptr;
+            set_error("hipMalloc of the coverage buffer (%llu records) failed: %s", (unsigned long long)cap,
+                      hipGetErrorString(e));
+            return e == hipErrorOutOfMemory ? TK_ERR_OOM : TK_ERR_HIP;
+        }
+        f->diff_adj_records = cap;
+    }
+    tk_args_fine_diff a = diff_args(f, t.colors);
+    a.image = t.image;
+    return launch_tiles(ctx, f->scene_count ? ctx->f_fine_diff_scenes : ctx->f_fine_diff, a.cfg.tile_count, a);
+}
+
+tk_status frame_render(tk_frame *f, uint32_t active_count = 0, const DiffTarget *diff = nullptr) {
     tk_context *ctx = f->ctx;
     const tk_scene &s = f->meta;
     const tk_tile_list list{active_count ? f->active_tiles : nullptr, active_count};
@@ -586,7 +672,10 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0) {
     // Join: fine reads the segments and the PTCL.
     TK_HIP(hipStreamWaitEvent(ctx->stream, ctx->join, 0));
     TK_TRY(record(f, 12));
-    {
+    if (diff) {
+        TK_TRY(fill_u32(ctx, f->diff_status, 1, 0u));
+        TK_TRY(run_fine_diff(f, *diff));
+    } else {
         tk_fine_config fc = frame_fine_config(f);
         tk_fine_inputs in{};
         in.draws = f->draws;
@@ -699,6 +788,7 @@ tk_status frame_create(tk_context *ctx, const tk_scene *s, const BatchLayout *ba
     ALLOC(tile_kinds, QT);
     ALLOC(batch_bases, uint64_t(s->batch_count) + 1);
     ALLOC(status, 4);
+    ALLOC(diff_status, 1);
     ALLOC(ptcl, f->ptcl_capacity);
     ALLOC(draw_brush, 2 * std::max<uint64_t>(s->draw_count, 1));
     ALLOC(active_tiles, T);
@@ -812,6 +902,11 @@ tk_status tk_context_create(int device, tk_context **out) {
         {&ctx->f_coarse_count_scenes, "tk_k_coarse_count_scenes"},
         {&ctx->f_coarse_emit_scenes, "tk_k_coarse_emit_scenes"},
         {&ctx->f_fine_scenes, "tk_k_fine_scenes"},
+        {&ctx->f_fine_diff, "tk_k_fine_diff"},
+        {&ctx->f_fine_diff_scenes, "tk_k_fine_diff_scenes"},
+        {&ctx->f_fine_backward, "tk_k_fine_backward"},
+        {&ctx->f_fine_backward_scenes, "tk_k_fine_backward_scenes"},
+        {&ctx->f_line_backward, "tk_k_line_backward"},
         {&ctx->f_fill_coverage, "tk_k_fill_coverage"},
         {&ctx->f_pixel_math, "tk_k_pixel_math"},
         {&ctx->f_brush_words, "tk_k_brush_words"},
@@ -1176,6 +1271,7 @@ void tk_frame_destroy(tk_frame *f) {
     if (f->ctx && f->ctx->stream) (void)hipStreamSynchronize(f->ctx->stream);
     drop_graph(f);
     for (void *p : f->allocations) (void)hipFree(p);
+    if (f->diff_adj) (void)hipFree(f->diff_adj);
     for (auto &e : f->ev)
         if (e) (void)hipEventDestroy(e);
     delete f;
@@ -1185,6 +1281,45 @@ tk_status tk_frame_render(tk_frame *f) {
     if (!f) return fail(TK_ERR_INVALID, "null frame");
     if (f->use_graph && !f->timing && !f->capture_raw) return frame_render_graph(f);
     return frame_render(f);
+}
+
+tk_status tk_frame_render_diff(tk_frame *f, const float *colors, float *image) {
+    if (!f || !colors || !image) return fail(TK_ERR_INVALID, "null argument");
+    if (!f->diff_scene)
+        return fail(TK_ERR_INVALID, "the differentiable form takes path draws (fills, strokes) without layers");
+    const tk::DiffTarget t{colors, image};
+    TK_TRY(tk::frame_render(f, 0, &t));
+    f->diff_ready = true;
+    return TK_OK;
+}
+
+tk_status tk_frame_backward(tk_frame *f, const float *colors, const float *image_grad, float *line_grad,
+                            float *color_grad) {
+    if (!f || !colors || !image_grad || !line_grad || !color_grad) return fail(TK_ERR_INVALID, "null argument");
+    if (!f->diff_ready) return fail(TK_ERR_INVALID, "tk_frame_backward needs a tk_frame_render_diff of the scene first");
+    tk_context *ctx = f->ctx;
+    const tk_scene &s = f->meta;
+    if (!f->diff_rec_of && s.backdrop_len) TK_TRY(tk::dalloc(f, &f->diff_rec_of, s.backdrop_len));
+    TK_TRY(tk::fill_u32(ctx, f->diff_rec_of, s.backdrop_len, TK_INVALID_INDEX));
+    TK_TRY(tk::fill_u32(ctx, color_grad, uint64_t(s.draw_count) * 4, 0u));
+    tk_args_fine_diff a = tk::diff_args(f, colors);
+    a.image_grad = image_grad;
+    a.color_grad = color_grad;
+    TK_TRY(tk::launch_tiles(ctx, f->scene_count ? ctx->f_fine_backward_scenes : ctx->f_fine_backward,
+                            a.cfg.tile_count, a));
+    if (s.line_count) {
+        tk_args_line_backward b{f->lines, s.line_count, s.path_count, f->paths, f->diff_rec_of, s.backdrop_len,
+                                f->diff_adj, f->diff_adj_records, line_grad};
+        TK_TRY(tk::launch(ctx, ctx->f_line_backward, div_up(s.line_count, 256), 1, 256, b));
+    }
+    return TK_OK;
+}
+
+tk_status tk_frame_diff_status(tk_frame *f, uint32_t *flags) {
+    if (!f || !flags) return fail(TK_ERR_INVALID, "null argument");
+    TK_HIP(hipMemcpyAsync(flags, f->diff_status, 4, hipMemcpyDeviceToHost, f->ctx->stream));
+    TK_HIP(hipStreamSynchronize(f->ctx->stream));
+    return TK_OK;
 }
 
 tk_status tk_frame_set_graph(tk_frame *f, int enable) {

@@ -187,3 +187,34 @@ struct tk_args_brush_words {
     uint32_t paint_words, paint_brush_base;
     uint32_t *out;
 };
+
+// The differentiable form: an FP32 composite of FILL records with per-draw
+// premultiplied colours and no RGBA8 quantization, and its backward pass.
+struct tk_args_fine_diff {
+    tk_fine_config cfg;
+    tk_fine_inputs in;
+    const tk_path_record *paths;
+    uint32_t path_count;
+    uint32_t rec_of_count;      // backdrop tiles (the length of rec_of)
+    const float *colors;        // premultiplied RGBA per draw
+    float *image;               // forward: premultiplied RGBA per pixel
+    const float *image_grad;    // backward: dL/d(image) per pixel
+    float *color_grad;          // backward: dL/d(colour) per draw, accumulated
+    float *adj;                 // 256 floats per PTCL record: the forward's coverage, then dL/d(coverage)
+    uint64_t adj_records;
+    uint32_t *rec_of;           // backward: per backdrop tile, the PTCL record of its path there
+    uint32_t *status;           // bit 0: a record kind the differentiable form does not take; bit 1: adj too small
+    tk_tile_list list;
+    tk_scene_slots scenes;
+};
+
+struct tk_args_line_backward {
+    const tk_line *lines;
+    uint32_t line_count, path_count;
+    const tk_path_record *paths;
+    const uint32_t *rec_of;
+    uint32_t rec_of_count;
+    const float *adj;
+    uint64_t adj_records;
+    float *line_grad;           // dL/d(p0.x, p0.y, p1.x, p1.y) per line
+};

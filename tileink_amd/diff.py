@@ -1,0 +1,513 @@
+"""The differentiable form: gradients of the FP32 coverage and composite with
+respect to control points, stroke widths and paint colours.
+
+The GPU side is two C-ABI entry points (tk_frame_render_diff, tk_frame_backward):
+the forward composites the scene's FILL records in FP32 without the RGBA8 round
+trips of the plain renderer, and the backward returns dL/d(line endpoints) and
+dL/d(colour) for a given dL/d(image).  The geometry that turns parameters into
+lines is written here in torch, so autograd carries the line gradients back to
+the parameters:
+
+  * a fill is a closed contour of cubics; each cubic is cut at uniform parameter
+    steps t = k/n, with n fixed from the control polygon so that the chord error
+    stays under `tolerance` (|B''| <= 6 max|P0 - 2P1 + P2|, error <= |B''|/(8 n^2));
+  * a stroke is an open chain of cubics, flattened the same way; its outline is
+    one closed polygon: the left offsets, a round end cap, the right offsets
+    backwards and a round start cap.  At a vertex turning by at most 120 degrees
+    each side gets the miter point q + h m, m = (n0 + n1) / (1 + n0.n1); a
+    sharper turn gets both segments' offset points (a bevel).
+
+The structure of the geometry (subdivision counts, miter or bevel, cap points)
+is fixed from the parameters' values and held constant under differentiation,
+as it is for any flattening.  Scenes may hold fills and strokes only (no
+layers), and the plain renderer's u8 quantization is not part of this form.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+import tileink_amd as tk
+from tileink_amd import _abi
+from tileink_amd._abi import check, lib
+
+TOLERANCE = 0.25
+_MAX_SUBDIVISIONS = 256
+
+
+def _bernstein(t):
+    mt = 1.0 - t
+    return torch.stack([mt * mt * mt, 3.0 * t * mt * mt, 3.0 * t * t * mt, t * t * t], -1)
+
+
+def _subdivisions(cubics, tolerance):
+    """Uniform steps per cubic ((N, 4, 2) array) for a chord error under tolerance."""
+    p = np.asarray(cubics, np.float64)
+    dd = np.maximum(np.linalg.norm(p[:, 0] - 2 * p[:, 1] + p[:, 2], axis=-1),
+                    np.linalg.norm(p[:, 1] - 2 * p[:, 2] + p[:, 3], axis=-1))
+    return np.clip(np.ceil(np.sqrt(0.75 * dd / tolerance)), 1, _MAX_SUBDIVISIONS).astype(np.int64)
+
+
+def _flatten(cubics, steps, closed_ends):
+    """Points of cubics (M, 4, 2) at t = k/n, k < n, plus t = 1 where closed_ends
+    marks a chain's last cubic (open chains end on their last point)."""
+    ci = np.repeat(np.arange(len(steps)), steps + closed_ends.astype(np.int64))
+    k = np.concatenate([np.arange(n + e) for n, e in zip(steps, closed_ends.astype(np.int64))]) if len(steps) else \
+        np.zeros(0, np.int64)
+    t = torch.as_tensor(k / np.repeat(steps, steps + closed_ends.astype(np.int64)), dtype=cubics.dtype,
+                        device=cubics.device)
+    w = _bernstein(t)
+    sel = cubics[torch.as_tensor(ci, device=cubics.device)]
+    return (w[:, :, None] * sel).sum(1)
+
+
+class Structure:
+    """The discrete choices of one geometry evaluation (kept fixed for its gradient)."""
+
+    def __init__(self):
+        self.fill_steps = None
+        self.stroke_steps = None
+        self.stroke_keep = None
+        self.recipe = None
+
+
+class DiffScene:
+    """Parameters of a differentiable scene.
+
+    fills:   (F, C, 3, 2) — per closed contour, per cubic: first control point,
+             second control point, end point (a cubic starts at the previous end).
+    strokes: (S, 3K + 1, 2) — open chains of K cubics; widths (S,).
+    colors:  (F + S, 4) straight-alpha RGBA per draw, in painter order.
+    order:   painter order as (kind, index) pairs, kind 0 fill / 1 stroke
+             (default: all fills, then all strokes).
+    rules:   fill rule per fill (0 NonZero, 1 EvenOdd); strokes use NonZero.
+    """
+
+    def __init__(self, width, height, fills=None, strokes=None, widths=None, colors=None, order=None, rules=None,
+                 clear=(0.0, 0.0, 0.0, 0.0), tolerance=TOLERANCE):
+        self.width, self.height = int(width), int(height)
+        self.fills = fills
+        self.strokes = strokes
+        self.widths = widths
+        self.colors = colors
+        nf = 0 if fills is None else int(fills.shape[0])
+        ns = 0 if strokes is None else int(strokes.shape[0])
+        self.order = list(order) if order is not None else [(0, i) for i in range(nf)] + [(1, i) for i in range(ns)]
+        self.rules = np.zeros(nf, np.int64) if rules is None else np.asarray(rules, np.int64)
+        self.clear = tuple(float(c) for c in clear)
+        self.tolerance = float(tolerance)
+
+    # --- geometry -------------------------------------------------------------------
+
+    def structure(self):
+        st = Structure()
+        if self.fills is not None and self.fills.shape[0]:
+            st.fill_steps = _subdivisions(_fill_cubics(self.fills).detach().cpu().numpy(), self.tolerance)
+        if self.strokes is not None and self.strokes.shape[0]:
+            cub = _stroke_cubics(self.strokes)
+            st.stroke_steps = _subdivisions(cub.detach().cpu().numpy(), self.tolerance)
+            q = _flatten(cub.detach().double(), st.stroke_steps, _chain_ends(self.strokes)).cpu().numpy()
+            st.stroke_keep, st.recipe, st.stroke_kept = _stroke_recipe(
+                q, _stroke_counts(self.strokes, st.stroke_steps), self.widths.detach().cpu().numpy() * 0.5,
+                self.tolerance)
+        return st
+
+    def polygons(self, st=None):
+        """(vertices (V, 2), vertices per draw in painter order) as torch tensors of
+        the parameters' dtype; differentiable in the parameters."""
+        st = st or self.structure()
+        fill_polys, stroke_polys = [], []
+        if st.fill_steps is not None:
+            cub = _fill_cubics(self.fills)
+            v = _flatten(cub, st.fill_steps, np.zeros(len(st.fill_steps), bool))
+            counts = st.fill_steps.reshape(self.fills.shape[0], -1).sum(1)
+            fill_polys = list(torch.split(v, counts.tolist()))
+        if st.stroke_steps is not None:
+            cub = _stroke_cubics(self.strokes)
+            q = _flatten(cub, st.stroke_steps, _chain_ends(self.strokes))
+            stroke_polys = _stroke_outlines(q, st.stroke_keep, st.recipe, st.stroke_kept, 0.5 * self.widths)
+        polys = [fill_polys[i] if kind == 0 else stroke_polys[i] for kind, i in self.order]
+        counts = np.array([len(p) for p in polys], np.int64)
+        dtype = (self.fills if self.fills is not None else self.strokes).dtype
+        device = (self.fills if self.fills is not None else self.strokes).device
+        verts = torch.cat(polys) if polys else torch.zeros((0, 2), dtype=dtype, device=device)
+        return verts, counts
+
+    def draw_rules(self):
+        return np.array([int(self.rules[i]) if kind == 0 else 0 for kind, i in self.order], np.uint32)
+
+    def premultiplied(self, colors=None):
+        c = self.colors if colors is None else colors
+        return torch.cat([c[:, :3] * c[:, 3:4], c[:, 3:4]], 1)
+
+
+def _fill_cubics(fills):
+    ends = fills[:, :, 2]
+    starts = torch.roll(ends, 1, dims=1)
+    return torch.stack([starts, fills[:, :, 0], fills[:, :, 1], ends], 2).reshape(-1, 4, 2)
+
+
+def _stroke_cubics(strokes):
+    S, n = strokes.shape[0], (strokes.shape[1] - 1) // 3
+    idx = torch.arange(n, device=strokes.device)[:, None] * 3 + torch.arange(4, device=strokes.device)
+    return strokes[:, idx].reshape(S * n, 4, 2)
+
+
+def _chain_ends(strokes):
+    n = (strokes.shape[1] - 1) // 3
+    ends = np.zeros((strokes.shape[0], n), bool)
+    ends[:, -1] = True
+    return ends.reshape(-1)
+
+
+def _stroke_counts(strokes, steps):
+    n = (strokes.shape[1] - 1) // 3
+    return steps.reshape(strokes.shape[0], n).sum(1) + 1
+
+
+def _cap_steps(h, tol):
+    step = 2.0 * math.acos(1.0 - tol / h) if h > tol else math.pi
+    return max(1, int(math.ceil(math.pi / step)))
+
+
+def _stroke_recipe(q, counts, half_widths, tol):
+    """Per stroke outline point: (centreline vertex, segment, a, b, c) for
+    q[v] + h (a n[seg] + b u[seg] + c m[v]); vertex and segment indices are
+    into the kept centreline points and the stroke's segments, laid end to end."""
+    keep = np.zeros(len(q), bool)
+    recipe, kept_counts = [], []
+    start = 0
+    seg_base = 0
+    kept_base = 0
+    for s, cnt in enumerate(counts):
+        pts = q[start:start + cnt]
+        k = [0]
+        for i in range(1, cnt):
+            if np.hypot(*(pts[i] - pts[k[-1]])) > 1e-9:
+                k.append(i)
+        keep[start + np.array(k)] = True
+        m = len(k) - 1  # segments
+        if m < 1:
+            raise ValueError(f"stroke {s} has no segment of nonzero length")
+        kept_counts.append(len(k))
+        rows = []
+        if m >= 1:
+            p = pts[k]
+            d = p[1:] - p[:-1]
+            u = d / np.linalg.norm(d, axis=1, keepdims=True)
+            n = np.stack([-u[:, 1], u[:, 0]], 1)
+            V, G = kept_base, seg_base
+            miter = [False] * (m + 1)
+            for i in range(1, m):
+                miter[i] = 1.0 + float(n[i - 1] @ n[i]) >= 0.5
+            left = [(V + 0, G + 0, 1.0, 0.0, 0.0)]
+            for i in range(1, m):
+                if miter[i]:
+                    left.append((V + i, G + i, 0.0, 0.0, 1.0))
+                else:
+                    left += [(V + i, G + i - 1, 1.0, 0.0, 0.0), (V + i, G + i, 1.0, 0.0, 0.0)]
+            left.append((V + m, G + m - 1, 1.0, 0.0, 0.0))
+            kc = _cap_steps(half_widths[s], 0.25 * tol)
+            end_cap = [(V + m, G + m - 1, math.cos(math.pi * j / kc), math.sin(math.pi * j / kc), 0.0)
+                       for j in range(1, kc)]
+            right = [(V + m, G + m - 1, -1.0, 0.0, 0.0)]
+            for i in range(m - 1, 0, -1):
+                if miter[i]:
+                    right.append((V + i, G + i, 0.0, 0.0, -1.0))
+                else:
+                    right += [(V + i, G + i, -1.0, 0.0, 0.0), (V + i, G + i - 1, -1.0, 0.0, 0.0)]
+            right.append((V + 0, G + 0, -1.0, 0.0, 0.0))
+            start_cap = [(V + 0, G + 0, -math.cos(math.pi * j / kc), -math.sin(math.pi * j / kc), 0.0)
+                         for j in range(1, kc)]
+            rows = left + end_cap + right + start_cap
+        recipe.append(np.array(rows, np.float64).reshape(-1, 5))
+        start += cnt
+        seg_base += m
+        kept_base += len(k)
+    return keep, recipe, np.array(kept_counts, np.int64)
+
+
+def _stroke_outlines(q, keep, recipe, kept_counts, half_widths):
+    """Outline polygons of the strokes from the recipe; differentiable in q and h."""
+    counts = [len(r) for r in recipe]
+    kept = q[torch.as_tensor(np.nonzero(keep)[0], device=q.device)]
+    # segments: consecutive kept points of one stroke
+    seg_from, seg_to = [], []
+    offs = 0
+    for nk in kept_counts:
+        seg_from += list(range(offs, offs + nk - 1))
+        seg_to += list(range(offs + 1, offs + nk))
+        offs += int(nk)
+    dev = q.device
+    a_i = torch.as_tensor(seg_from, device=dev)
+    b_i = torch.as_tensor(seg_to, device=dev)
+    d = kept[b_i] - kept[a_i]
+    u = d / torch.linalg.norm(d, dim=1, keepdim=True)
+    n = torch.stack([-u[:, 1], u[:, 0]], 1)
+    # miter vectors at every kept vertex (zero where unused): m = (n_prev + n_next) / (1 + n_prev . n_next)
+    m = torch.zeros_like(kept)
+    prev_seg = np.full(len(kept), -1)
+    next_seg = np.full(len(kept), -1)
+    prev_seg[np.asarray(seg_to)] = np.arange(len(seg_to))
+    next_seg[np.asarray(seg_from)] = np.arange(len(seg_from))
+    inner = np.nonzero((prev_seg >= 0) & (next_seg >= 0))[0]
+    if len(inner):
+        ip = torch.as_tensor(prev_seg[inner], device=dev)
+        inx = torch.as_tensor(next_seg[inner], device=dev)
+        num = n[ip] + n[inx]
+        den = 1.0 + (n[ip] * n[inx]).sum(1, keepdim=True)
+        m = m.index_put((torch.as_tensor(inner, device=dev),), num / den)
+    rec = np.concatenate([r for r in recipe if len(r)])
+    stroke_of_point = np.repeat(np.arange(len(recipe)), counts)
+    vi = torch.as_tensor(rec[:, 0].astype(np.int64), device=dev)
+    si = torch.as_tensor(rec[:, 1].astype(np.int64), device=dev)
+    coef = torch.as_tensor(rec[:, 2:5], dtype=q.dtype, device=dev)
+    h = half_widths[torch.as_tensor(stroke_of_point, device=dev)][:, None]
+    pts = kept[vi] + h * (coef[:, 0:1] * n[si] + coef[:, 1:2] * u[si] + coef[:, 2:3] * m[vi])
+    return list(torch.split(pts, counts))
+
+
+# --- scene encoding and the line <-> vertex map ----------------------------------------
+
+
+def edge_map(verts32, counts):
+    """The lines the recorder emits for closed polygons: edge j of a polygon runs
+    from vertex j to vertex j + 1 (the last back to the first); edges shorter than
+    1e-12 are dropped, as push_line drops them.  Returns (line -> first vertex,
+    line -> second vertex, lines per draw)."""
+    a_idx, b_idx, per_draw = [], [], []
+    base = 0
+    for k in counts:
+        k = int(k)
+        if k == 0:
+            per_draw.append(0)
+            continue
+        ia = np.arange(base, base + k)
+        ib = np.roll(ia, -1)
+        p = verts32.astype(np.float64)
+        d = p[ib] - p[ia]
+        ok = (d * d).sum(1) > 1.0e-24
+        if k == 1:
+            ok[:] = False
+        a_idx.append(ia[ok])
+        b_idx.append(ib[ok])
+        per_draw.append(int(ok.sum()))
+        base += k
+    cat = lambda xs: np.concatenate(xs) if xs else np.zeros(0, np.int64)
+    return cat(a_idx), cat(b_idx), np.array(per_draw, np.int64)
+
+
+def encode(scene, verts, counts, colors=None):
+    """Records the polygons on a canvas and encodes it.  Returns (canvas, encoded
+    scene, line_a, line_b, lines per draw)."""
+    v32 = verts.detach().to("cpu", torch.float32).numpy()
+    canvas = tk.Canvas(scene.width, scene.height)
+    rules = scene.draw_rules()
+    col = (scene.colors if colors is None else colors).detach().cpu().double().numpy()
+    base = 0
+    for d, k in enumerate(counts):
+        c = np.clip(col[d], 0.0, 1.0)
+        canvas.push_polygon(v32[base:base + k], tk.Color(*c), tk.FillRule(int(rules[d])))
+        base += int(k)
+    clear = np.clip(np.asarray(scene.clear), 0, 1)
+    cw = int(round(clear[0] * 255)) | int(round(clear[1] * 255)) << 8 | int(round(clear[2] * 255)) << 16 | \
+        int(round(clear[3] * 255)) << 24
+    enc = canvas.encode(cw)
+    la, lb, per_draw = edge_map(v32, counts)
+    lines = _abi.view(enc.lines, _abi.LINE_DTYPE, enc.line_count)
+    got = np.concatenate([lines["p0"], lines["p1"]], 1) if len(lines) else np.zeros((0, 4), np.float32)
+    want = np.concatenate([v32[la], v32[lb]], 1) if len(la) else np.zeros((0, 4), np.float32)
+    if got.shape != want.shape or not np.array_equal(got, want):
+        raise tk.TileinkError(_abi.TK_ERR_INVALID, "recorded lines do not match the polygon edges")
+    return canvas, enc, la, lb, per_draw
+
+
+def clear_color(scene):
+    """The clear colour the frame starts from: each channel of scene.clear taken
+    as premultiplied and rounded to 8 bits, as encode packs it."""
+    return np.round(np.clip(np.asarray(scene.clear, np.float64), 0, 1) * 255.0) / 255.0
+
+
+# --- the autograd function ------------------------------------------------------------
+
+
+class Plan:
+    """One encoded scene (or batch of scenes) on the device, for forward and backward."""
+
+    def __init__(self, context, scenes, verts, counts, colors=None):
+        self.context = context
+        self.scenes = scenes
+        self.encoded, self.canvases, la, lb, self.sizes = [], [], [], [], []
+        v_off, l_off = 0, 0
+        self.line_ranges = []
+        colors_of = colors or [None] * len(scenes)
+        for s, v, k, col in zip(scenes, verts, counts, colors_of):
+            canvas, enc, a, b, per_draw = encode(s, v, k, col)
+            self.canvases.append(canvas)
+            self.encoded.append(enc)
+            la.append(a + v_off)
+            lb.append(b + v_off)
+            self.line_ranges.append(per_draw)
+            v_off += len(v)
+            self.sizes.append((s.width, s.height, enc.tiles_width * enc.tiles_height))
+        self.vertex_count = v_off
+        if len(scenes) == 1:
+            self.frame = tk.Frame(context, self.encoded[0])
+            self._batch = None
+            self.offsets = [0]
+            self.pixels = scenes[0].width * scenes[0].height
+        else:
+            self._batch = tk.Batch(context, self.encoded)
+            self.frame = self._batch.frame
+            cols = np.cumsum([0] + [t for _, _, t in self.sizes])
+            self.offsets = [int(c) * 256 for c in cols[:-1]]
+            self.pixels = int(cols[-1]) * 256
+        self.line_a = np.concatenate(la)
+        self.line_b = np.concatenate(lb)
+        self._la_t = self._lb_t = None
+        self.line_count = len(self.line_a)
+        self.draw_count = sum(len(k) for k in counts)
+
+    def _index(self, device):
+        if self._la_t is None or self._la_t.device != device:
+            self._la_t = torch.as_tensor(self.line_a, device=device)
+            self._lb_t = torch.as_tensor(self.line_b, device=device)
+        return self._la_t, self._lb_t
+
+    def images(self, flat):
+        out = []
+        for (w, h, _), off in zip(self.sizes, self.offsets):
+            out.append(flat[off:off + w * h].reshape(h, w, 4))
+        return out
+
+    def forward(self, colors):
+        """colors: (D, 4) premultiplied float32 on the GPU; returns the flat image."""
+        colors = colors.detach().to(torch.float32).contiguous()
+        image = torch.zeros((self.pixels, 4), dtype=torch.float32, device=colors.device)
+        torch.cuda.synchronize(colors.device)
+        check(lib().tk_frame_render_diff(self.frame._h, C.c_void_p(colors.data_ptr()), C.c_void_p(image.data_ptr())))
+        self.context.synchronize()
+        flags = C.c_uint32()
+        check(lib().tk_frame_diff_status(self.frame._h, C.byref(flags)))
+        if flags.value:
+            raise tk.TileinkError(_abi.TK_ERR_INVALID, f"differentiable render flagged {flags.value:#x}")
+        self._colors = colors
+        return image
+
+    def backward(self, grad_flat):
+        grad = grad_flat.detach().to(torch.float32).contiguous()
+        dev = grad.device
+        line_grad = torch.empty((max(self.line_count, 1), 4), dtype=torch.float32, device=dev)
+        color_grad = torch.empty((max(self.draw_count, 1), 4), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize(dev)
+        check(lib().tk_frame_backward(self.frame._h, C.c_void_p(self._colors.data_ptr()), C.c_void_p(grad.data_ptr()),
+                                      C.c_void_p(line_grad.data_ptr()), C.c_void_p(color_grad.data_ptr())))
+        self.context.synchronize()
+        return line_grad[:self.line_count], color_grad[:self.draw_count]
+
+    def vertex_grad(self, line_grad, like):
+        la, lb = self._index(like.device)
+        g = torch.zeros((self.vertex_count, 2), dtype=like.dtype, device=like.device)
+        lg = line_grad.to(like.dtype).to(like.device)
+        g.index_add_(0, la, lg[:, 0:2])
+        g.index_add_(0, lb, lg[:, 2:4])
+        return g
+
+
+class _Render(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, verts, colors, plan, device):
+        flat = plan.forward(colors.to(device))
+        ctx.plan = plan
+        ctx.vdtype, ctx.vdevice = verts.dtype, verts.device
+        ctx.cdtype, ctx.cdevice = colors.dtype, colors.device
+        return flat
+
+    @staticmethod
+    def backward(ctx, grad_flat):
+        plan = ctx.plan
+        line_grad, color_grad = plan.backward(grad_flat)
+        gv = plan.vertex_grad(line_grad, torch.empty(0, dtype=ctx.vdtype, device=ctx.vdevice))
+        return gv, color_grad.to(ctx.cdtype).to(ctx.cdevice), None, None
+
+
+def render(context, scenes, device="cuda"):
+    """FP32 images of the scenes (a list of DiffScene), differentiable in their
+    parameters.  One scene renders as one frame, several as one batch."""
+    single = isinstance(scenes, DiffScene)
+    scenes = [scenes] if single else list(scenes)
+    structs = [s.structure() for s in scenes]
+    polys = [s.polygons(st) for s, st in zip(scenes, structs)]
+    verts = [v for v, _ in polys]
+    counts = [k for _, k in polys]
+    plan = Plan(context, scenes, verts, counts)
+    colors = torch.cat([s.premultiplied() for s in scenes])
+    flat = _Render.apply(torch.cat(verts), colors, plan, torch.device(device))
+    images = plan.images(flat)
+    return images[0] if single else images
+
+
+def oracle_inputs(scene, verts, counts, recorded=False):
+    """The FP64 oracle's arguments for one scene: draws (line_start, line_count, rule)
+    and lines (L, 4), from the polygon vertices in their own precision, or, with
+    recorded=True, rounded to FP32 as the recorder stores them."""
+    v32 = verts.detach().to("cpu", torch.float32).numpy()
+    la, lb, per_draw = edge_map(v32, counts)
+    v = v32.astype(np.float64) if recorded else verts.detach().cpu().double().numpy()
+    lines = np.concatenate([v[la], v[lb]], 1) if len(la) else np.zeros((0, 4))
+    starts = np.concatenate([[0], np.cumsum(per_draw)[:-1]]) if len(per_draw) else np.zeros(0, np.int64)
+    rules = scene.draw_rules()
+    draws = [(int(s), int(n), int(r)) for s, n, r in zip(starts, per_draw, rules)]
+    return draws, lines, la, lb
+
+
+# --- synthetic scenes -----------------------------------------------------------------
+
+
+def blob_fills(rng, count, width, height, cubics=4, radius_mu=math.log(24.0), radius_sigma=0.5):
+    """Closed blobs of `cubics` cubics: ends on a jittered circle, control points a
+    third of the way along jittered tangents."""
+    cx = rng.uniform(0, width, count)
+    cy = rng.uniform(0, height, count)
+    r = np.clip(rng.lognormal(radius_mu, radius_sigma, count), 4.0, 128.0)
+    ang = np.linspace(0, 2 * math.pi, cubics + 1)[:-1][None, :] + rng.uniform(0, 2 * math.pi, (count, 1))
+    rad = r[:, None] * rng.uniform(0.6, 1.2, (count, cubics))
+    ex, ey = cx[:, None] + rad * np.cos(ang), cy[:, None] + rad * np.sin(ang)
+    ends = np.stack([ex, ey], -1)
+    starts = np.roll(ends, 1, axis=1)
+    span = ends - starts
+    k = 4.0 / 3.0 * math.tan(math.pi / (2 * cubics))
+    tang_s = np.stack([-np.sin(np.roll(ang, 1, 1)), np.cos(np.roll(ang, 1, 1))], -1) * np.roll(rad, 1, 1)[..., None]
+    tang_e = np.stack([-np.sin(ang), np.cos(ang)], -1) * rad[..., None]
+    c1 = starts + k * tang_s + rng.normal(0, 0.15, starts.shape) * np.linalg.norm(span, axis=-1, keepdims=True)
+    c2 = ends - k * tang_e + rng.normal(0, 0.15, ends.shape) * np.linalg.norm(span, axis=-1, keepdims=True)
+    return np.stack([c1, c2, ends], 2)
+
+
+def wave_strokes(rng, count, width, height, cubics=2):
+    """Open chains of cubics meandering from a random anchor."""
+    out = np.zeros((count, 3 * cubics + 1, 2))
+    for s in range(count):
+        p = np.array([rng.uniform(0, width), rng.uniform(0, height)])
+        pts = [p]
+        for _ in range(cubics):
+            step = rng.normal(0, 18, (3, 2))
+            for j in range(3):
+                pts.append(pts[-1] + step[j])
+        out[s] = np.array(pts)
+    return out
+
+
+def random_scene(seed, width, height, fills, strokes=0, cubics=4, dtype=torch.float64, device="cpu",
+                 radius_mu=math.log(24.0)):
+    rng = np.random.default_rng(seed)
+    f = blob_fills(rng, fills, width, height, cubics, radius_mu)
+    s = wave_strokes(rng, strokes, width, height) if strokes else None
+    widths = rng.lognormal(math.log(3.0), 0.4, strokes) if strokes else None
+    colors = np.concatenate([rng.uniform(0, 1, (fills + strokes, 3)), rng.uniform(0.3, 0.9, (fills + strokes, 1))], 1)
+    order = [(0, i) for i in range(fills)] + [(1, i) for i in range(strokes)]
+    rng.shuffle(order)
+    t = lambda a: None if a is None else torch.tensor(a, dtype=dtype, device=device)
+    return DiffScene(width, height, fills=t(f), strokes=t(s), widths=t(widths), colors=t(colors), order=order,
+                     rules=rng.integers(0, 2, fills))
