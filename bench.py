@@ -156,6 +156,51 @@ def scene_loop(ctx, count=64, repeats=5):
     return out
 
 
+def diff_batch(ctx, count=256, repeats=3):
+    """Forward + backward of the differentiable form over many scenes: 256 scenes of
+    256x256, each with 64-512 closed paths of 4 cubics (solid colours, no layers),
+    loss = squared error against a random target.  A step renders every scene in FP32
+    (tk_frame_render_diff) and back-propagates dL/d(image) to line endpoints and
+    colours (tk_frame_backward), from scenes already resident in HBM.  Timed once as
+    a per-scene loop (one frame per scene) and once as one batch."""
+    import torch
+    from tileink_amd import diff
+    rng = np.random.default_rng(5)
+    paths = rng.integers(64, 513, count)
+    sc = [diff.random_scene(1000 + i, 256, 256, fills=int(n), cubics=4, radius_mu=np.log(16.0))
+          for i, n in enumerate(paths)]
+    polys = [s.polygons() for s in sc]
+    colors = [s.premultiplied().to("cuda", torch.float32) for s in sc]
+    singles = [diff.Plan(ctx, [s], [v], [k]) for s, (v, k) in zip(sc, polys)]
+    batch = diff.Plan(ctx, sc, [v for v, _ in polys], [k for _, k in polys])
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    targets = [torch.rand((p.pixels, 4), device="cuda", generator=gen) for p in singles]
+    target_b = torch.zeros((batch.pixels, 4), device="cuda")
+    for (w, h, _), off, t in zip(batch.sizes, batch.offsets, targets):
+        target_b[off:off + w * h] = t
+    color_b = torch.cat(colors)
+
+    def step(plan, c, t):
+        image = plan.forward(c)
+        plan.backward(2.0 * (image - t))
+
+    out = {"scenes": count, "size": "256x256", "paths": int(paths.sum()), "lines": int(batch.line_count)}
+    for name, run in (("loop", lambda: [step(p, c, t) for p, c, t in zip(singles, colors, targets)]),
+                      ("batch", lambda: step(batch, color_b, target_b))):
+        run()  # warm-up: sizes the coverage buffers
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            run()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / repeats * 1e3
+        out[f"{name}_ms"] = round(ms, 3)
+        out[f"{name}_scenes_per_s"] = round(count / (ms * 1e-3), 1)
+    out["batch_speedup"] = round(out["loop_ms"] / out["batch_ms"], 3)
+    del singles, batch
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +211,7 @@ def main():
     ap.add_argument("--clips", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-scene-loop", action="store_true", help="skip the many-small-scenes loop")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable forward+backward batch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device three times with tk_k_copy (counter calibration)")
     args = ap.parse_args()
@@ -228,6 +274,9 @@ def main():
     fine_gbs = fine_bytes / (stage_avg["fine"] * 1e-3) / 1e9
     incremental = incremental_render(frame, W // 16, H // 16, args.steps)
     loop = scene_loop(ctx) if not args.no_scene_loop else None
+    fwd_bwd = diff_batch(ctx) if not args.no_diff else None
+    if fwd_bwd:
+        log(f"differentiable fwd+bwd: {fwd_bwd}")
     result = {
         "metric": "ms/frame & Gpix/s at 4096², 200k paths; batched fwd+bwd scenes/s vs plain",
         "value": round(gpix, 4),
@@ -251,6 +300,7 @@ def main():
             "stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
             "incremental": incremental,
             "scene_loop": loop,
+            "diff_fwd_bwd": fwd_bwd,
         },
         "roofline": {"bound": "hbm", "kernel": "tk_k_fine", "achieved": round(fine_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(fine_gbs / HBM_PEAK_GBS, 5), "traffic": measured_traffic("tk_k_fine"),

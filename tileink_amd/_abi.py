@@ -228,6 +228,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise TileinkError(TK_ERR_UNAVAILABLE,
                                f"{LIB_PATH} is missing; build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        # torch's ROCm wheel carries its own libamdhip64.so.7 under the same soname.
+        # Whichever copy loads first serves the whole process, and torch does not
+        # find the GPU through another release's runtime, so torch (when present)
+        # loads first and the library binds to its copy.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = C.CDLL(LIB_PATH)
         for name, (restype, argtypes) in SIGNATURES.items():
             fn = getattr(handle, name)
