@@ -212,6 +212,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-scene-loop", action="store_true", help="skip the many-small-scenes loop")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable forward+backward batch")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device three times with tk_k_copy (counter calibration)")
     args = ap.parse_args()
@@ -242,6 +243,11 @@ def main():
     if stats["overflow"]:
         raise SystemExit(f"capacity overflow flags {stats['overflow']}")
     log(f"warmup done: K={stats['ptcl_records']} PTCL records, S={stats['segments_emitted']} segments")
+    tuning = frame.tuning()
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = frame.autotune(repeats=3)
+        log(f"autotune: {tuning} in {time.perf_counter() - t0:.1f} s")
 
     # Per-stage HIP events on the pipeline's stream, over the timed steps.
     frame.enable_timing(True)
@@ -299,6 +305,7 @@ def main():
             "wall_ms_per_step": round(wall / args.steps * 1e3, 4),
             "stage_ms": {k: round(v, 4) for k, v in stage_avg.items()},
             "incremental": incremental,
+            "tuning": tuning,
             "scene_loop": loop,
             "diff_fwd_bwd": fwd_bwd,
         },

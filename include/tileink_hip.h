@@ -321,6 +321,22 @@ tk_status tk_frame_render(tk_frame *frame);
  * TK_ERR_INVALID.  Afterwards the per-(batch, tile) tables read back through
  * tk_frame_read_buffer hold one column per listed tile, in list order. */
 tk_status tk_frame_render_tiles(tk_frame *frame, const uint32_t *tiles, uint32_t tile_count);
+/* Launch parameters of a frame's kernels (no counterpart in the reference, whose
+ * dispatch shapes are fixed).  Each *_xcd_run is how many consecutive workgroups
+ * one XCD takes before the next XCD's run: a power of two up to 1024, or 0 for
+ * the plain launch order.  Scan counts 256-line blocks, coarse four-tile
+ * workgroups, fine (and the differentiable kernels) tiles.  The 16x16 tile and
+ * the workgroup sizes are fixed by the record formats and the kernels' lane maps.
+ * tk_frame_autotune renders the frame's own scene with candidate values (one
+ * warm-up and `repeats` timed renders each, stage times from HIP events), keeps
+ * the fastest value per stage and returns the result in *chosen.               */
+typedef struct tk_tuning {
+    uint32_t scan_xcd_run, coarse_xcd_run, fine_xcd_run, reserved;
+} tk_tuning;
+tk_status tk_frame_get_tuning(tk_frame *frame, tk_tuning *out);
+tk_status tk_frame_set_tuning(tk_frame *frame, const tk_tuning *tuning);
+tk_status tk_frame_autotune(tk_frame *frame, uint32_t repeats, tk_tuning *chosen);
+
 /* The differentiable form (no counterpart in the reference, which only renders).
  * Its forward pass is the fine stage (fine/main.hlsl:23-40) with the RGBA8 round
  * trips between batches left out: an FP32 premultiplied source-over composite of

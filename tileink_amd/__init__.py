@@ -430,6 +430,24 @@ class Frame:
         check(lib().tk_frame_update(self._h, C.byref(scene)))
         self._scene = scene
 
+    def tuning(self):
+        """The launch parameters as a dict (tk_frame_get_tuning)."""
+        t = _abi.Tuning()
+        check(lib().tk_frame_get_tuning(self._h, C.byref(t)))
+        return {"scan_xcd_run": t.scan_xcd_run, "coarse_xcd_run": t.coarse_xcd_run, "fine_xcd_run": t.fine_xcd_run}
+
+    def set_tuning(self, scan_xcd_run=None, coarse_xcd_run=None, fine_xcd_run=None):
+        cur = self.tuning()
+        t = _abi.Tuning(*(cur[k] if v is None else int(v) for k, v in (
+            ("scan_xcd_run", scan_xcd_run), ("coarse_xcd_run", coarse_xcd_run), ("fine_xcd_run", fine_xcd_run))), 0)
+        check(lib().tk_frame_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, repeats=3):
+        """Times candidate launch parameters on this scene and keeps the fastest (tk_frame_autotune)."""
+        t = _abi.Tuning()
+        check(lib().tk_frame_autotune(self._h, int(repeats), C.byref(t)))
+        return {"scan_xcd_run": t.scan_xcd_run, "coarse_xcd_run": t.coarse_xcd_run, "fine_xcd_run": t.fine_xcd_run}
+
     def set_graph(self, enable=True):
         """Replay full renders from a captured hipGraph (tk_frame_set_graph)."""
         check(lib().tk_frame_set_graph(self._h, int(bool(enable))))

@@ -119,6 +119,10 @@ class StageTimes(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("scan_clear", "scan_count", "scan_prefix", "scan_emit", "cumsum", "coarse_count", "coarse_prefix", "coarse_emit", "fine", "total")]
 
 
+class Tuning(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("scan_xcd_run", "coarse_xcd_run", "fine_xcd_run", "reserved")]
+
+
 class CoarseConfig(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("tile_count", "tiles_width", "tiles_height", "batch_count", "draw_count",
                                           "path_count", "paint_words", "paint_brush_base", "layer_count",
@@ -190,6 +194,9 @@ SIGNATURES = {
     "tk_frame_render": (C.c_int, [_P]),
     "tk_frame_render_tiles": (C.c_int, [_P, _P, _U32]),
     "tk_frame_set_graph": (C.c_int, [_P, C.c_int]),
+    "tk_frame_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "tk_frame_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "tk_frame_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(Tuning)]),
     "tk_frame_render_diff": (C.c_int, [_P, _P, _P]),
     "tk_frame_backward": (C.c_int, [_P, _P, _P, _P, _P]),
     "tk_frame_diff_status": (C.c_int, [_P, C.POINTER(C.c_uint32)]),

@@ -80,15 +80,16 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
 // cache lines that neighbouring blocks share (tile table words, a path's
 // consecutive lines and their counters) are then fetched into one XCD's L2, and
 // the runs stay interleaved across the XCDs so that spatially uneven work is
-// still spread evenly.  chunk == 0 keeps the launch order.
+// still spread evenly.  chunk is a power of two, chosen at launch; 0 keeps the
+// launch order.
 __device__ __forceinline__ uint32_t xcd_chunked_block(uint32_t linear, uint32_t chunk) {
     if (chunk == 0u) return linear;
-    const uint32_t x = linear & 7u, k = linear >> 3;
-    return ((k / chunk) * 8u + x) * chunk + (k % chunk);
+    const uint32_t x = linear & 7u, k = linear >> 3, shift = (uint32_t)__builtin_ctz(chunk);
+    return ((((k >> shift) << 3) + x) << shift) + (k & (chunk - 1u));
 }
 __device__ __forceinline__ uint32_t xcd_grid(uint64_t blocks, uint32_t chunk) {
     const uint64_t unit = chunk ? 8u * (uint64_t)chunk : 1u;
-    return (uint32_t)((blocks + unit - 1u) / unit * unit);
+    return (uint32_t)((blocks + unit - 1u) & ~(unit - 1u));
 }
 
 __device__ __forceinline__ unsigned long long pack_status(uint32_t value, uint32_t flag, uint32_t epoch) {
@@ -708,7 +709,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_active_paths(tk_args_acti
 
 // scan_count (scan/count.hlsl:13-40).
 extern "C" __global__ __launch_bounds__(256) void tk_k_scan_count(tk_args_scan_lines a) {
-    const uint32_t block = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_SCAN);
+    const uint32_t block = xcd_chunked_block(blockIdx.x, a.xcd_run);
     uint32_t id = block * 256u + threadIdx.x;
     if (id >= a.cfg.line_count) return;
     if (a.path_active) {  // incremental: only paths whose tile box meets the damage
@@ -873,7 +874,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_chunk_apply(tk_args_
 
 // scan_emit (scan/emit.hlsl:14-28).
 extern "C" __global__ __launch_bounds__(256) void tk_k_scan_emit(tk_args_scan_lines a) {
-    const uint32_t block = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_SCAN);
+    const uint32_t block = xcd_chunked_block(blockIdx.x, a.xcd_run);
     uint32_t id = block * 256u + threadIdx.x;
     if (id >= a.cfg.line_count) return;
     if (a.path_active) {  // incremental: only paths whose tile box meets the damage
@@ -1172,7 +1173,7 @@ __device__ __forceinline__ void coarse_count_tile(const tk_args_coarse &a) {
     const tk_coarse_config &cfg = a.cfg;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t column = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + wave;
+    const uint32_t column = xcd_chunked_block(blockIdx.x, a.xcd_run) * 4u + wave;
     if (column >= a.list.columns) return;
     const SceneView v = scene_view<kScenes>(a.scenes, column, a.list.columns, cfg.batch_count, cfg.tiles_width,
                                             cfg.tiles_height, 0u, 0u, 0u);
@@ -1316,7 +1317,7 @@ template <bool kScenes>
 __device__ __forceinline__ void coarse_emit_tile(const tk_args_coarse &a) {
     const tk_coarse_config &cfg = a.cfg;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t column = xcd_chunked_block(blockIdx.x, TK_XCD_CHUNK_COARSE) * 4u + (threadIdx.x >> 6);
+    const uint32_t column = xcd_chunked_block(blockIdx.x, a.xcd_run) * 4u + (threadIdx.x >> 6);
     if (column >= a.list.columns) return;
     const SceneView v = scene_view<kScenes>(a.scenes, column, a.list.columns, cfg.batch_count, cfg.tiles_width,
                                             cfg.tiles_height, 0u, 0u, 0u);
@@ -1472,8 +1473,8 @@ __device__ __forceinline__ void fine_tile(const tk_args_fine &a) {
     const tk_fine_config &cfg = a.cfg;
     const uint32_t linear = blockIdx.x + blockIdx.y * gridDim.x;
     const uint32_t columns = a.list.columns;  // table columns: listed tiles, or every tile
-    if (linear >= xcd_grid(columns, TK_XCD_CHUNK_FINE)) return;
-    const uint32_t column = xcd_chunked_block(linear, TK_XCD_CHUNK_FINE);
+    if (linear >= xcd_grid(columns, a.xcd_run)) return;
+    const uint32_t column = xcd_chunked_block(linear, a.xcd_run);
     if (column >= columns) return;
     const SceneView v = scene_view<kScenes>(a.scenes, column, columns, cfg.batch_count, cfg.tiles_width,
                                             cfg.tiles_height, cfg.width, cfg.height, cfg.clear_color);
@@ -1769,8 +1770,8 @@ template <bool kScenes>
 __device__ __forceinline__ bool diff_tile(const tk_args_fine_diff &a, DiffTile &t) {
     const uint32_t linear = blockIdx.x + blockIdx.y * gridDim.x;
     const uint32_t columns = a.list.columns;
-    if (linear >= xcd_grid(columns, TK_XCD_CHUNK_FINE)) return false;
-    const uint32_t column = xcd_chunked_block(linear, TK_XCD_CHUNK_FINE);
+    if (linear >= xcd_grid(columns, a.xcd_run)) return false;
+    const uint32_t column = xcd_chunked_block(linear, a.xcd_run);
     if (column >= columns) return false;
     const tk_fine_config &cfg = a.cfg;
     t.v = scene_view<kScenes>(a.scenes, column, columns, cfg.batch_count, cfg.tiles_width, cfg.tiles_height, cfg.width,

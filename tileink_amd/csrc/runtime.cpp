@@ -165,16 +165,16 @@ tk_status fill_u32(tk_context *ctx, void *dst, uint64_t count, uint32_t value) {
 
 tk_status run_scan_count(tk_context *ctx, const tk_scan_config &cfg, const tk_line *lines, const tk_path_record *paths,
                          uint32_t path_record_count, int32_t *backdrops, uint32_t *counts,
-                         const uint32_t *path_active = nullptr) {
-    tk_args_scan_lines a{cfg, lines, paths, path_record_count, backdrops, counts, nullptr, path_active};
-    return launch(ctx, ctx->f_scan_count, xcd_grid(div_up(cfg.line_count, 256), TK_XCD_CHUNK_SCAN), 1, 256, a);
+                         const uint32_t *path_active = nullptr, uint32_t xcd_run = TK_XCD_CHUNK_SCAN) {
+    tk_args_scan_lines a{cfg, lines, paths, path_record_count, backdrops, counts, nullptr, path_active, xcd_run};
+    return launch(ctx, ctx->f_scan_count, xcd_grid(div_up(cfg.line_count, 256), xcd_run), 1, 256, a);
 }
 
 tk_status run_scan_emit(tk_context *ctx, const tk_scan_config &cfg, const tk_line *lines, const tk_path_record *paths,
                         uint32_t path_record_count, uint32_t *cursors, tk_line_segment *segments,
-                        const uint32_t *path_active = nullptr) {
-    tk_args_scan_lines a{cfg, lines, paths, path_record_count, nullptr, cursors, segments, path_active};
-    return launch(ctx, ctx->f_scan_emit, xcd_grid(div_up(cfg.line_count, 256), TK_XCD_CHUNK_SCAN), 1, 256, a);
+                        const uint32_t *path_active = nullptr, uint32_t xcd_run = TK_XCD_CHUNK_SCAN) {
+    tk_args_scan_lines a{cfg, lines, paths, path_record_count, nullptr, cursors, segments, path_active, xcd_run};
+    return launch(ctx, ctx->f_scan_emit, xcd_grid(div_up(cfg.line_count, 256), xcd_run), 1, 256, a);
 }
 
 tk_status run_scan_prefix(tk_context *ctx, const tk_scan_config &cfg, const tk_scan_chunk *chunks,
@@ -222,23 +222,25 @@ tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coar
                      const tk_coarse_outputs &out, unsigned long long *status, uint32_t *ticket, uint32_t epoch,
                      hipEvent_t *marks, const LayerClassPlan &plan = LayerClassPlan{},
                      tk_tile_list list = tk_tile_list{nullptr, 0u},
-                     tk_scene_slots scenes = tk_scene_slots{nullptr, nullptr, 0u, 0ull}) {
+                     tk_scene_slots scenes = tk_scene_slots{nullptr, nullptr, 0u, 0ull},
+                     uint32_t xcd_run = TK_XCD_CHUNK_COARSE) {
     if (!list.tiles) list.columns = cfg.tile_count;
     if (cfg.batch_count == 0 || list.columns == 0) return TK_OK;
     const uint32_t blocks = scenes.slots ? coarse_scan_blocks_for(scenes.table_entries)
                                          : coarse_scan_blocks(cfg, list.columns);
-    tk_args_coarse a{cfg, in, out, status, ticket, epoch, blocks, tk_layer_classes{nullptr, nullptr, 0u}, list, scenes};
+    tk_args_coarse a{cfg, in, out, status, ticket, epoch, blocks, tk_layer_classes{nullptr, nullptr, 0u}, list, scenes,
+                     xcd_run};
     if (plan.bits && plan.pair_count) {
         tk_args_layer_classes b{cfg, in, plan.pair_layer, plan.pair_count, plan.bits, list};
         TK_TRY(launch(ctx, ctx->f_layer_classes, uint32_t(div_up(uint64_t(list.columns) * plan.pair_count, 256)), 1,
                       256, b));
         a.classes = tk_layer_classes{plan.bits, plan.pair_of, plan.pair_count};
     }
-    TK_TRY(launch(ctx, scenes.slots ? ctx->f_coarse_count_scenes : ctx->f_coarse_count, xcd_grid(div_up(list.columns, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
+    TK_TRY(launch(ctx, scenes.slots ? ctx->f_coarse_count_scenes : ctx->f_coarse_count, xcd_grid(div_up(list.columns, 4), xcd_run), 1, 256, a));
     if (marks) TK_HIP(hipEventRecord(marks[0], launch_stream(ctx)));
     TK_TRY(launch(ctx, ctx->f_coarse_scan, a.scan_blocks, 1, 256, a));
     if (marks) TK_HIP(hipEventRecord(marks[1], launch_stream(ctx)));
-    TK_TRY(launch(ctx, scenes.slots ? ctx->f_coarse_emit_scenes : ctx->f_coarse_emit, xcd_grid(div_up(list.columns, 4), TK_XCD_CHUNK_COARSE), 1, 256, a));
+    TK_TRY(launch(ctx, scenes.slots ? ctx->f_coarse_emit_scenes : ctx->f_coarse_emit, xcd_grid(div_up(list.columns, 4), xcd_run), 1, 256, a));
     return TK_OK;
 }
 
@@ -246,7 +248,8 @@ tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coar
 // and paint before fine reads it.
 tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inputs &in, uint32_t *rgba,
                    uint32_t *draw_brush, tk_tile_list list = tk_tile_list{nullptr, 0u},
-                   tk_scene_slots scenes = tk_scene_slots{nullptr, nullptr, 0u, 0ull}) {
+                   tk_scene_slots scenes = tk_scene_slots{nullptr, nullptr, 0u, 0ull},
+                   uint32_t xcd_run = TK_XCD_CHUNK_FINE) {
     if (!list.tiles) list.columns = cfg.tile_count;
     if (cfg.clip_spill_depth > TK_FINE_MAX_CLIP_SPILL) {
         set_error("clip nesting deeper than %u is not supported", 4u + TK_FINE_MAX_CLIP_SPILL);
@@ -256,9 +259,9 @@ tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inp
         tk_args_draw_brush b{in.draws, cfg.draw_count, in.paint, cfg.paint_words, cfg.paint_brush_base, draw_brush};
         TK_TRY(launch(ctx, ctx->f_draw_brush, div_up(cfg.draw_count, 256), 1, 256, b));
     }
-    tk_args_fine a{cfg, in, rgba, draw_brush, list, scenes};
+    tk_args_fine a{cfg, in, rgba, draw_brush, list, scenes, xcd_run};
     if (list.columns == 0) return TK_OK;
-    const uint32_t blocks = xcd_grid(list.columns, TK_XCD_CHUNK_FINE);
+    const uint32_t blocks = xcd_grid(list.columns, xcd_run);
     uint32_t gx = std::min<uint32_t>(blocks, 65535u);
     uint32_t gy = div_up(blocks, gx);
     return launch(ctx, scenes.slots ? ctx->f_fine_scenes : ctx->f_fine, gx, gy, 256, a, cfg.clip_spill_depth * 256u);
@@ -334,6 +337,8 @@ struct tk_frame {
     float *diff_adj = nullptr;  // 256 floats per PTCL record, grown on demand
     uint64_t diff_adj_records = 0;
     uint32_t *diff_rec_of = nullptr, *diff_status = nullptr;
+    // launch parameters (tk_frame_set_tuning, tk_frame_autotune)
+    tk_tuning tuning{TK_XCD_CHUNK_SCAN, TK_XCD_CHUNK_COARSE, TK_XCD_CHUNK_FINE, 0u};
 };
 
 namespace tk {
@@ -543,12 +548,13 @@ tk_args_fine_diff diff_args(tk_frame *f, const float *colors) {
     a.status = f->diff_status;
     a.list = tk_tile_list{nullptr, a.cfg.tile_count};
     a.scenes = tk_scene_slots{f->scene_slots, f->scene_of_column, f->scene_count, f->table_entries};
+    a.xcd_run = f->tuning.fine_xcd_run;
     return a;
 }
 
 tk_status launch_tiles(tk_context *ctx, hipFunction_t fn, uint32_t columns, const tk_args_fine_diff &a) {
     if (columns == 0) return TK_OK;
-    const uint32_t blocks = xcd_grid(columns, TK_XCD_CHUNK_FINE);
+    const uint32_t blocks = xcd_grid(columns, a.xcd_run);
     const uint32_t gx = std::min<uint32_t>(blocks, 65535u);
     return launch(ctx, fn, gx, div_up(blocks, gx), 256, a);
 }
@@ -605,7 +611,8 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0, const DiffTarget 
         path_active = f->path_active;
     }
     TK_TRY(record(f, 1));
-    TK_TRY(run_scan_count(ctx, scfg, f->lines, f->paths, s.path_count, f->backdrops, f->seg_counts, path_active));
+    TK_TRY(run_scan_count(ctx, scfg, f->lines, f->paths, s.path_count, f->backdrops, f->seg_counts, path_active,
+                          f->tuning.scan_xcd_run));
     TK_TRY(record(f, 2));
     // Fork A: the backdrop prefix needs only scan_count's deltas.
     TK_HIP(hipEventRecord(ctx->fork_a, ctx->stream));
@@ -659,7 +666,7 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0, const DiffTarget 
             plan.bits = f->layer_bits;
         }
         TK_TRY(run_coarse(ctx, cc, in, out, f->coarse_status, &f->tickets[1], f->epoch, f->timing ? &f->ev[7] : nullptr,
-                          plan, list, scenes));
+                          plan, list, scenes, f->tuning.coarse_xcd_run));
         if (cc.batch_count == 0 && f->timing) {
             TK_TRY(record(f, 7));
             TK_TRY(record(f, 8));
@@ -667,7 +674,8 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0, const DiffTarget 
         TK_TRY(record(f, 9));
         TK_HIP(hipEventRecord(ctx->join, ctx->side));
     }
-    TK_TRY(run_scan_emit(ctx, scfg, f->lines, f->paths, s.path_count, f->cursors, f->segments, path_active));
+    TK_TRY(run_scan_emit(ctx, scfg, f->lines, f->paths, s.path_count, f->cursors, f->segments, path_active,
+                         f->tuning.scan_xcd_run));
     TK_TRY(record(f, 4));
     // Join: fine reads the segments and the PTCL.
     TK_HIP(hipStreamWaitEvent(ctx->stream, ctx->join, 0));
@@ -687,7 +695,7 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0, const DiffTarget 
         in.ptcl = f->ptcl;
         in.status = f->status + 2;
         (void)T;
-        TK_TRY(run_fine(ctx, fc, in, f->image, f->draw_brush, list, scenes));
+        TK_TRY(run_fine(ctx, fc, in, f->image, f->draw_brush, list, scenes, f->tuning.fine_xcd_run));
     }
     TK_TRY(record(f, 10));
     f->have_times = f->timing;
@@ -1281,6 +1289,83 @@ tk_status tk_frame_render(tk_frame *f) {
     if (!f) return fail(TK_ERR_INVALID, "null frame");
     if (f->use_graph && !f->timing && !f->capture_raw) return frame_render_graph(f);
     return frame_render(f);
+}
+
+namespace {
+bool valid_run(uint32_t v) { return v == 0u || (v <= 1024u && (v & (v - 1u)) == 0u); }
+}  // namespace
+
+tk_status tk_frame_get_tuning(tk_frame *f, tk_tuning *out) {
+    if (!f || !out) return fail(TK_ERR_INVALID, "null argument");
+    *out = f->tuning;
+    return TK_OK;
+}
+
+tk_status tk_frame_set_tuning(tk_frame *f, const tk_tuning *t) {
+    if (!f || !t) return fail(TK_ERR_INVALID, "null argument");
+    if (!valid_run(t->scan_xcd_run) || !valid_run(t->coarse_xcd_run) || !valid_run(t->fine_xcd_run))
+        return fail(TK_ERR_INVALID, "XCD run lengths are 0 or powers of two up to 1024");
+    if (f->graph_exec) {  // the captured launches carry the old values
+        TK_HIP(hipStreamSynchronize(f->ctx->stream));
+        tk::drop_graph(f);
+    }
+    f->tuning = *t;
+    f->tuning.reserved = 0u;
+    return TK_OK;
+}
+
+// Coordinate search: fine first (the largest stage), then coarse, then scan, each
+// over its candidate run lengths with the others held at the best so far.
+tk_status tk_frame_autotune(tk_frame *f, uint32_t repeats, tk_tuning *chosen) {
+    if (!f) return fail(TK_ERR_INVALID, "null frame");
+    repeats = std::max<uint32_t>(repeats, 1u);
+    const bool was_timing = f->timing;
+    f->timing = true;
+    tk_tuning best = f->tuning;
+    auto cost = [&](const tk_tuning &t, int stage, float *out) -> tk_status {
+        TK_TRY(tk_frame_set_tuning(f, &t));
+        TK_TRY(tk::frame_render(f));  // warm-up
+        float sum = 0.0f;
+        for (uint32_t r = 0; r < repeats; ++r) {
+            TK_TRY(tk::frame_render(f));
+            tk_stage_times st;
+            TK_TRY(tk_frame_stage_times(f, &st));
+            sum += stage == 0 ? st.fine : stage == 1 ? st.coarse_count + st.coarse_emit : st.scan_count + st.scan_emit;
+        }
+        *out = sum / float(repeats);
+        return TK_OK;
+    };
+    static const uint32_t fine_runs[] = {0u, 32u, 64u, 128u, 256u, 512u};
+    static const uint32_t coarse_runs[] = {0u, 16u, 32u, 64u, 128u};
+    static const uint32_t scan_runs[] = {0u, 16u, 64u};
+    struct Axis {
+        uint32_t tk_tuning::*field;
+        const uint32_t *runs;
+        size_t count;
+    } axes[3] = {{&tk_tuning::fine_xcd_run, fine_runs, 6}, {&tk_tuning::coarse_xcd_run, coarse_runs, 5},
+                 {&tk_tuning::scan_xcd_run, scan_runs, 3}};
+    tk_status st = TK_OK;
+    for (int stage = 0; stage < 3 && st == TK_OK; ++stage) {
+        float best_ms = 0.0f;
+        uint32_t best_run = best.*(axes[stage].field);
+        for (size_t i = 0; i < axes[stage].count && st == TK_OK; ++i) {
+            tk_tuning t = best;
+            t.*(axes[stage].field) = axes[stage].runs[i];
+            float ms = 0.0f;
+            st = cost(t, stage, &ms);
+            if (st == TK_OK && (i == 0 || ms < best_ms)) {
+                best_ms = ms;
+                best_run = axes[stage].runs[i];
+            }
+        }
+        best.*(axes[stage].field) = best_run;
+    }
+    f->timing = was_timing;
+    if (st != TK_OK) return st;
+    TK_TRY(tk_frame_set_tuning(f, &best));
+    TK_TRY(tk::frame_render(f));  // leave the scene's image in the frame
+    if (chosen) *chosen = best;
+    return TK_OK;
 }
 
 tk_status tk_frame_render_diff(tk_frame *f, const float *colors, float *image) {

@@ -54,6 +54,7 @@ struct tk_args_scan_lines {
     uint32_t *counts_or_cursors; // count: segment counts, emit: cursors
     tk_line_segment *segments;   // emit only
     const uint32_t *path_active; // incremental: lines of paths whose flag is 0 are skipped; null: all
+    uint32_t xcd_run;            // XCD run length in blocks (a power of two; 0: launch order)
 };
 
 // A batch of independent scenes rendered as one frame: each scene keeps its own
@@ -136,6 +137,7 @@ struct tk_args_coarse {
     tk_layer_classes classes;
     tk_tile_list list;
     tk_scene_slots scenes;
+    uint32_t xcd_run;  // XCD run length in four-tile workgroups (a power of two; 0: launch order)
 };
 
 struct tk_args_layer_classes {
@@ -164,6 +166,7 @@ struct tk_args_fine {
     const uint32_t *draw_brush;  // tk_args_draw_brush::table, draw_count entries
     tk_tile_list list;
     tk_scene_slots scenes;
+    uint32_t xcd_run;  // XCD run length in tiles (a power of two; 0: launch order)
 };
 
 struct tk_args_fill_coverage {
@@ -206,6 +209,7 @@ struct tk_args_fine_diff {
     uint32_t *status;           // bit 0: a record kind the differentiable form does not take; bit 1: adj too small
     tk_tile_list list;
     tk_scene_slots scenes;
+    uint32_t xcd_run;           // as tk_args_fine
 };
 
 struct tk_args_line_backward {
