@@ -213,6 +213,9 @@ def main():
     ap.add_argument("--no-scene-loop", action="store_true", help="skip the many-small-scenes loop")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable forward+backward batch")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
+    ap.add_argument("--no-incremental", action="store_true", help="skip the damaged-tile re-render")
+    ap.add_argument("--diff-only", action="store_true",
+                    help="run only the differentiable forward+backward batch and print its figures as JSON")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device three times with tk_k_copy (counter calibration)")
     args = ap.parse_args()
@@ -220,6 +223,11 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the scene would shard by tile "
               f"stripes, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.diff_only:
+        ctx = tk.NativeContext(0)
+        print(json.dumps({"diff_fwd_bwd": diff_batch(ctx)}), flush=True)
+        return
 
     t0 = time.perf_counter()
     canvas = scenes.config2(args.size, args.size, args.paths, clips=args.clips, workload=2)
@@ -278,7 +286,7 @@ def main():
     walked_records, used_segments = fine_walked_work(frame, scene, stats)
     fine_bytes = fine_algorithmic_bytes(walked_records, used_segments, scene)
     fine_gbs = fine_bytes / (stage_avg["fine"] * 1e-3) / 1e9
-    incremental = incremental_render(frame, W // 16, H // 16, args.steps)
+    incremental = incremental_render(frame, W // 16, H // 16, args.steps) if not args.no_incremental else None
     loop = scene_loop(ctx) if not args.no_scene_loop else None
     fwd_bwd = diff_batch(ctx) if not args.no_diff else None
     if fwd_bwd:
