@@ -1712,10 +1712,10 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_brush_words(tk_args_brush
 //   I = sum_k alpha_k c_k T_k,  T_k = prod_{j>k} f_j,
 //   dL/dalpha_k = T_k (g.c_k - c_k.a g.R_{k-1}),   R_{k-1} = composite below k,
 //   dL/dc_k     = T_k alpha_k g  (minus T_k alpha_k g.R_{k-1} on the alpha channel).
-// A first sweep forms the stream's product of the nonzero f_j (mantissa and
-// exponent kept apart, so it never underflows) and counts the zero ones; the
-// second sweep rebuilds R_{k-1} bottom-up and takes T_k as the ratio of the
-// total to the running product (zero while a zero factor lies above k).  The
+// The forward keeps each pixel's product of the nonzero f_j over its whole
+// stream (mantissa and exponent apart, so it never underflows) and the count of
+// zero ones; the backward rebuilds R_{k-1} bottom-up and takes T_k as the ratio
+// of that total to its running product (zero while a zero factor lies above k).  The
 // coverage derivative (dL/dcoverage = dL/dalpha * dalpha/dcoverage) replaces
 // the coverage in `adj`, and the line kernel turns it into endpoint gradients
 // with the boundary form of the area derivative: a pixel's coverage is the
@@ -1739,10 +1739,32 @@ __device__ __forceinline__ float diff_alpha(float v, uint32_t rule, float &slope
     return fminf(m, 1.0f);
 }
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+template <int kCtrl, int kRowMask, int kBankMask>
+__device__ __forceinline__ float dpp_add(float v, float src) {
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(src), kCtrl, kRowMask, kBankMask, true));
+}
+
+template <int kCtrl>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xf, 0xf, false));
+}
+
+// Sums four per-lane values over the wave at the cost of about one: two quad
+// exchanges (quad_perm [1,0,3,2], then [2,3,0,1]) leave each lane with one of the
+// four channels summed over its quad -- channel ((l & 1) << 1) | ((l >> 1) & 1) --
+// row shifts by 4 and 8 sum the quads of a row into lanes 12-15, and two xor
+// shuffles add the rows.  Lanes 12-15 of every row end with the wave totals.
+__device__ __forceinline__ uint32_t sum4_channel(uint32_t lane) { return ((lane & 1u) << 1) | ((lane >> 1) & 1u); }
+__device__ __forceinline__ float wave_sum4(float v0, float v1, float v2, float v3, uint32_t lane) {
+    const bool odd = (lane & 1u) != 0u, hi = (lane & 2u) != 0u;
+    const float x = (odd ? v2 : v0) + dpp_mov<0xB1>(odd ? v0 : v2);
+    const float y = (odd ? v3 : v1) + dpp_mov<0xB1>(odd ? v1 : v3);
+    float z = (hi ? y : x) + dpp_mov<0x4E>(hi ? x : y);
+    z = dpp_add<0x114, 0xf, 0xe>(z, z);  // row_shr:4, banks 1-3
+    z = dpp_add<0x118, 0xf, 0xc>(z, z);  // row_shr:8, banks 2-3
+    z += __shfl_xor(z, 16, 64);
+    z += __shfl_xor(z, 32, 64);
+    return z;
 }
 
 // Product of the nonzero factors as mantissa * 2^exponent, and the zero count.
@@ -1762,7 +1784,7 @@ struct Transmit {
 
 struct DiffTile {
     SceneView v;
-    uint32_t tile, lane, gx, gy;
+    uint32_t column, tile, lane, gx, gy;
     bool inside;
 };
 
@@ -1773,6 +1795,7 @@ __device__ __forceinline__ bool diff_tile(const tk_args_fine_diff &a, DiffTile &
     if (linear >= xcd_grid(columns, a.xcd_run)) return false;
     const uint32_t column = xcd_chunked_block(linear, a.xcd_run);
     if (column >= columns) return false;
+    t.column = column;
     const tk_fine_config &cfg = a.cfg;
     t.v = scene_view<kScenes>(a.scenes, column, columns, cfg.batch_count, cfg.tiles_width, cfg.tiles_height, cfg.width,
                               cfg.height, cfg.clear_color);
@@ -1827,9 +1850,13 @@ __device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &
     const float4 *colors = reinterpret_cast<const float4 *>(a.colors);
     Px px = rgba8_to_unorm(t.v.clear);
     bool overflow = false;
+    Transmit total{1.0f, 0, 0};
+    const uint32_t tx = t.tile % t.v.tw, ty = t.tile / t.v.tw;
     for_each_fill(a, t, [&](uint32_t pi, const tk_ptcl_record &rec) {
         const uint32_t s0 = rec.segment_start, s1 = min(rec.segment_end, a.cfg.segment_capacity);
         const float4 c = colors[rec.color];
+        // Issued before the coverage work so that its latency overlaps it.
+        const uint4 box = reinterpret_cast<const uint4 *>(a.draw_tiles)[rec.color];
         CoverageAcc &acc = cov_acc[sel];
         sel ^= 1u;
         accumulate_record(a.in.segments + s0, s1 > s0 ? s1 - s0 : 0u, acc, lane);
@@ -1838,14 +1865,24 @@ __device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &
         float slope;
         const float alpha = diff_alpha(cov, rec.fill_rule, slope);
         const float f = 1.0f - alpha * c.w;
+        total.mul(f);
         px.r = px.r * f + alpha * c.x;
         px.g = px.g * f + alpha * c.y;
         px.b = px.b * f + alpha * c.z;
         px.a = px.a * f + alpha * c.w;
         if (pi < a.adj_records) a.adj[(size_t)pi * 256u + lane] = cov;
         else overflow = true;
+        if (lane == 0u && box.w != 0u) {  // where the line pass finds this path's record in this tile
+            const uint32_t x0 = box.y & 0xFFFFu, y0 = box.y >> 16, x1 = box.z & 0xFFFFu, y1 = box.z >> 16;
+            if (tx >= x0 && ty >= y0 && tx < x1 && ty < y1) {
+                const uint32_t bd = box.x + (ty - y0) * (x1 - x0) + tx - x0;
+                if (bd < a.rec_of_count) a.rec_of[bd] = pi;
+            }
+        }
     });
     if (overflow && lane == 0u && a.status) atomicOr(a.status, 2u);
+    reinterpret_cast<uint4 *>(a.trans)[(size_t)t.column * 256u + lane] =
+        make_uint4(__float_as_uint(total.m), (uint32_t)total.e, (uint32_t)total.zeros, 0u);
     if (t.inside)
         reinterpret_cast<float4 *>(a.image)[t.v.image + (size_t)t.gy * t.v.width + t.gx] =
             make_float4(px.r, px.g, px.b, px.a);
@@ -1860,22 +1897,13 @@ __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff 
     float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (t.inside) g = reinterpret_cast<const float4 *>(a.image_grad)[t.v.image + (size_t)t.gy * t.v.width + t.gx];
     auto coverage = [&](uint32_t pi) { return pi < a.adj_records ? a.adj[(size_t)pi * 256u + lane] : 0.0f; };
-
-    Transmit total{1.0f, 0, 0};
-    for_each_fill(a, t, [&](uint32_t pi, const tk_ptcl_record &rec) {
-        float slope;
-        const float alpha = diff_alpha(coverage(pi), rec.fill_rule, slope);
-        total.mul(1.0f - alpha * colors[rec.color].w);
-    });
-
-    const uint32_t tx = t.tile % t.v.tw, ty = t.tile / t.v.tw;
+    const uint4 tw = reinterpret_cast<const uint4 *>(a.trans)[(size_t)t.column * 256u + lane];
+    const Transmit total{__uint_as_float(tw.x), (int)tw.y, (int)tw.z};
     Transmit below{1.0f, 0, 0};
     Px res = rgba8_to_unorm(t.v.clear);
-    for_each_fill(a, t, [&](uint32_t pi, const tk_ptcl_record &rec) {
+    auto record = [&](uint32_t pi, const tk_ptcl_record &rec, float cov, const float4 &c) {
         float slope;
-        const float cov = coverage(pi);
         const float alpha = diff_alpha(cov, rec.fill_rule, slope);
-        const float4 c = colors[rec.color];
         const float f = 1.0f - alpha * c.w;
         below.mul(f);
         const float T = total.zeros > below.zeros ? 0.0f : ldexpf(total.m / below.m, total.e - below.e);
@@ -1884,33 +1912,63 @@ __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff 
         const float d_alpha = T * (g_c - c.w * g_res);
         if (pi < a.adj_records) a.adj[(size_t)pi * 256u + lane] = d_alpha * slope;
         const float w = T * alpha;
-        const float cr = wave_sum(w * g.x), cg = wave_sum(w * g.y), cb = wave_sum(w * g.z),
-                    ca = wave_sum(w * (g.w - g_res));
-        if ((lane & 63u) == 0u) {
-            float *out = a.color_grad + (size_t)rec.color * 4u;
-            atomicAdd(out + 0, cr);
-            atomicAdd(out + 1, cg);
-            atomicAdd(out + 2, cb);
-            atomicAdd(out + 3, ca);
-        }
+        const float sum = wave_sum4(w * g.x, w * g.y, w * g.z, w * (g.w - g_res), lane);
+        if ((lane & 63u) >= 60u) atomicAdd(a.color_grad + (size_t)rec.color * 4u + sum4_channel(lane), sum);
         res.r = res.r * f + alpha * c.x;
         res.g = res.g * f + alpha * c.y;
         res.b = res.b * f + alpha * c.z;
         res.a = res.a * f + alpha * c.w;
-        if (lane == 0u) {
-            const uint32_t path = a.in.draws[rec.color].path_id;
-            if (path < a.path_count) {
-                const tk_path_record &p = a.paths[path];
-                if (tx >= p.tile_x0 && ty >= p.tile_y0 && tx < p.tile_x1 && ty < p.tile_y1) {
-                    const uint32_t bd = p.data_offset + (ty - p.tile_y0) * (p.tile_x1 - p.tile_x0) + tx - p.tile_x0;
-                    if (bd < a.rec_of_count) a.rec_of[bd] = pi;
-                }
-            }
+    };
+    // The walk of for_each_fill, software-pipelined: the next record, its colour
+    // and its coverage are loaded while the current record is processed (the
+    // loop does little arithmetic per record, so it is bound by these loads).
+    const uint32_t draws = a.cfg.draw_count;
+    auto color_of = [&](const tk_ptcl_record &rec) {
+        return rec.tag == TK_PTCL_FILL && rec.color < draws ? colors[rec.color] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    };
+    for (uint32_t b = t.v.q0; b < t.v.q1; ++b) {
+        const size_t idx = table_index(t.v, b);
+        const uint32_t kind_word = a.in.tile_kinds[idx];
+        if ((kind_word & TK_TILE_KIND_MASK) == TK_TILE_KIND_EMPTY) continue;
+        const tk_segment_range r = a.in.tile_ptcl_ranges[idx];
+        const uint32_t pend = min(r.end, a.cfg.ptcl_capacity);
+        uint32_t pi = r.start + (kind_word >> TK_TILE_PREFIX_SHIFT);
+        if (pi >= pend) continue;
+        tk_ptcl_record rec = a.in.ptcl[pi];
+        float cov = coverage(pi);
+        float4 c = color_of(rec);
+        while (rec.tag != TK_PTCL_END) {
+            const uint32_t next = pi + 1u;
+            const bool more = next < pend;
+            tk_ptcl_record nrec{};
+            if (more) nrec = a.in.ptcl[next];
+            const float ncov = more ? coverage(next) : 0.0f;
+            const float4 nc = more ? color_of(nrec) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (rec.tag == TK_PTCL_FILL && rec.color < draws) record(pi, rec, cov, c);
+            else if (lane == 0u && a.status) atomicOr(a.status, 1u);
+            if (!more) break;
+            pi = next;
+            rec = nrec;
+            cov = ncov;
+            c = nc;
         }
-    });
+    }
 }
 
 }  // namespace
+
+extern "C" __global__ __launch_bounds__(256) void tk_k_draw_tiles(tk_args_draw_tiles a) {
+    const uint32_t d = blockIdx.x * 256u + threadIdx.x;
+    if (d >= a.draw_count) return;
+    const uint32_t path = a.draws[d].path_id;
+    uint4 e = make_uint4(0u, 0u, 0u, 0u);
+    if (path < a.path_count) {
+        const tk_path_record &p = a.paths[path];
+        e = make_uint4(p.data_offset, (p.tile_x0 & 0xFFFFu) | (p.tile_y0 << 16), (p.tile_x1 & 0xFFFFu) | (p.tile_y1 << 16),
+                       1u);
+    }
+    reinterpret_cast<uint4 *>(a.table)[d] = e;
+}
 
 extern "C" __global__ __launch_bounds__(256) void tk_k_fine_diff(tk_args_fine_diff a) {
     fine_diff_forward_tile<false>(a);

@@ -77,6 +77,7 @@ struct tk_context {
                   f_fine = nullptr, f_coarse_count_scenes = nullptr, f_coarse_emit_scenes = nullptr,
                   f_fine_scenes = nullptr, f_fine_diff = nullptr, f_fine_diff_scenes = nullptr,
                   f_fine_backward = nullptr, f_fine_backward_scenes = nullptr, f_line_backward = nullptr,
+                  f_draw_tiles = nullptr,
                   f_fill_coverage = nullptr, f_pixel_math = nullptr, f_brush_words = nullptr;
     // Scratch for the stage entry points (frames own their own).
     uint32_t *tickets = nullptr;  // [0] scan prefix, [1] coarse scan
@@ -337,6 +338,8 @@ struct tk_frame {
     float *diff_adj = nullptr;  // 256 floats per PTCL record, grown on demand
     uint64_t diff_adj_records = 0;
     uint32_t *diff_rec_of = nullptr, *diff_status = nullptr;
+    uint32_t *diff_trans = nullptr;  // 16 bytes per pixel of every column
+    uint32_t *diff_draw_tiles = nullptr;  // 16 bytes per draw
     // launch parameters (tk_frame_set_tuning, tk_frame_autotune)
     tk_tuning tuning{TK_XCD_CHUNK_SCAN, TK_XCD_CHUNK_COARSE, TK_XCD_CHUNK_FINE, 0u};
 };
@@ -544,6 +547,8 @@ tk_args_fine_diff diff_args(tk_frame *f, const float *colors) {
     a.colors = colors;
     a.adj = f->diff_adj;
     a.adj_records = f->diff_adj_records;
+    a.trans = f->diff_trans;
+    a.draw_tiles = f->diff_draw_tiles;
     a.rec_of = f->diff_rec_of;
     a.status = f->diff_status;
     a.list = tk_tile_list{nullptr, a.cfg.tile_count};
@@ -580,6 +585,15 @@ tk_status run_fine_diff(tk_frame *f, const DiffTarget &t) {
             return e == hipErrorOutOfMemory ? TK_ERR_OOM : TK_ERR_HIP;
         }
         f->diff_adj_records = cap;
+    }
+    const tk_fine_config fc = frame_fine_config(f);
+    if (!f->diff_trans) TK_TRY(dalloc(f, &f->diff_trans, uint64_t(fc.tile_count) * 256 * 4));
+    if (!f->diff_rec_of) TK_TRY(dalloc(f, &f->diff_rec_of, std::max<uint64_t>(f->meta.backdrop_len, 1)));
+    TK_TRY(fill_u32(ctx, f->diff_rec_of, f->meta.backdrop_len, TK_INVALID_INDEX));
+    if (!f->diff_draw_tiles) TK_TRY(dalloc(f, &f->diff_draw_tiles, std::max<uint64_t>(f->meta.draw_count, 1) * 4));
+    if (f->meta.draw_count) {
+        tk_args_draw_tiles dt{f->draws, f->meta.draw_count, f->meta.path_count, f->paths, f->diff_draw_tiles};
+        TK_TRY(launch(ctx, ctx->f_draw_tiles, div_up(f->meta.draw_count, 256), 1, 256, dt));
     }
     tk_args_fine_diff a = diff_args(f, t.colors);
     a.image = t.image;
@@ -915,6 +929,7 @@ tk_status tk_context_create(int device, tk_context **out) {
         {&ctx->f_fine_backward, "tk_k_fine_backward"},
         {&ctx->f_fine_backward_scenes, "tk_k_fine_backward_scenes"},
         {&ctx->f_line_backward, "tk_k_line_backward"},
+        {&ctx->f_draw_tiles, "tk_k_draw_tiles"},
         {&ctx->f_fill_coverage, "tk_k_fill_coverage"},
         {&ctx->f_pixel_math, "tk_k_pixel_math"},
         {&ctx->f_brush_words, "tk_k_brush_words"},
@@ -1384,8 +1399,6 @@ tk_status tk_frame_backward(tk_frame *f, const float *colors, const float *image
     if (!f->diff_ready) return fail(TK_ERR_INVALID, "tk_frame_backward needs a tk_frame_render_diff of the scene first");
     tk_context *ctx = f->ctx;
     const tk_scene &s = f->meta;
-    if (!f->diff_rec_of && s.backdrop_len) TK_TRY(tk::dalloc(f, &f->diff_rec_of, s.backdrop_len));
-    TK_TRY(tk::fill_u32(ctx, f->diff_rec_of, s.backdrop_len, TK_INVALID_INDEX));
     TK_TRY(tk::fill_u32(ctx, color_grad, uint64_t(s.draw_count) * 4, 0u));
     tk_args_fine_diff a = tk::diff_args(f, colors);
     a.image_grad = image_grad;

@@ -205,7 +205,10 @@ struct tk_args_fine_diff {
     float *color_grad;          // backward: dL/d(colour) per draw, accumulated
     float *adj;                 // 256 floats per PTCL record: the forward's coverage, then dL/d(coverage)
     uint64_t adj_records;
-    uint32_t *rec_of;           // backward: per backdrop tile, the PTCL record of its path there
+    uint32_t *trans;            // 4 words per pixel (256 per column): the forward's transmittance product
+                                // over the pixel's whole stream {mantissa bits, exponent, zero factors, 0}
+    uint32_t *rec_of;           // forward: per backdrop tile, the PTCL record of its path there
+    const uint32_t *draw_tiles; // forward: 4 words per draw, tk_args_draw_tiles::table
     uint32_t *status;           // bit 0: a record kind the differentiable form does not take; bit 1: adj too small
     tk_tile_list list;
     tk_scene_slots scenes;
@@ -221,4 +224,14 @@ struct tk_args_line_backward {
     const float *adj;
     uint64_t adj_records;
     float *line_grad;           // dL/d(p0.x, p0.y, p1.x, p1.y) per line
+};
+
+// Per draw, where its path's backdrop tiles start and its tile box:
+// {data_offset, tile_x0 | tile_y0 << 16, tile_x1 | tile_y1 << 16, 1} (0 in the
+// last word: no path).
+struct tk_args_draw_tiles {
+    const tk_draw_record *draws;
+    uint32_t draw_count, path_count;
+    const tk_path_record *paths;
+    uint32_t *table;
 };
