@@ -226,9 +226,13 @@ def diff_area(lines, start, count, width, height):
     return box, area
 
 
-def diff_render(width, height, clear, draws, colors, lines):
+def diff_render(width, height, clear, draws, colors, lines, masks=None):
     """FP64 forward of the differentiable form.  draws: (line_start, line_count, rule)
-    per draw in painter order; colors: premultiplied RGBA per draw; lines: (L, 4)."""
+    per draw in painter order; colors: premultiplied RGBA per draw; lines: (L, 4).
+    masks: optional per-draw clip masks (H, W) (None where a draw has no clip);
+    a masked draw's alpha is multiplied by its mask."""
+    if masks is not None and any(m is not None for m in masks):
+        return _masked_render(width, height, clear, draws, colors, lines, masks)
     starts, counts, rules, lines = _diff_args(draws, lines)
     clear = np.ascontiguousarray(clear, np.float64)
     colors = np.ascontiguousarray(colors, np.float64).reshape(-1, 4)
@@ -238,8 +242,62 @@ def diff_render(width, height, clear, draws, colors, lines):
     return image
 
 
-def diff_backward(width, height, clear, draws, colors, lines, image_grad):
+def _alpha(a, rule):
+    if rule == 1:
+        w = a - 2.0 * np.rint(0.5 * a)
+        return np.abs(w), np.where(w < 0, -1.0, 1.0)
+    m = np.abs(a)
+    return np.minimum(m, 1.0), np.where(m < 1.0, np.where(a < 0, -1.0, 1.0), 0.0)
+
+
+def _masked_render(width, height, clear, draws, colors, lines, masks, image_grad=None):
+    """The composite with clip masks, per draw over its box: the areas come from
+    or_diff_area_f64; the composite and its adjoint are restated here in numpy."""
+    lines = np.ascontiguousarray(lines, np.float64).reshape(-1, 4)
+    colors = np.asarray(colors, np.float64).reshape(-1, 4)
+    image = np.broadcast_to(np.asarray(clear, np.float64), (height, width, 4)).copy()
+    layers = []
+    for d, (start, count, rule) in enumerate(draws):
+        box, area = diff_area(lines, start, count, width, height)
+        x0, y0, w, h = (int(v) for v in box)
+        alpha, slope = _alpha(area, rule)
+        if masks[d] is not None:
+            m = np.asarray(masks[d], np.float64)[y0:y0 + h, x0:x0 + w]
+            alpha, slope = alpha * m, slope * m
+        layers.append((x0, y0, w, h, alpha, slope))
+    if image_grad is None:
+        for d, (x0, y0, w, h, alpha, _) in enumerate(layers):
+            px = image[y0:y0 + h, x0:x0 + w]
+            px *= (1.0 - alpha * colors[d, 3])[..., None]
+            px += alpha[..., None] * colors[d]
+        return image
+    g = np.asarray(image_grad, np.float64).reshape(height, width, 4)
+    T = np.ones((height, width))
+    trans = [None] * len(layers)
+    for d in range(len(layers) - 1, -1, -1):
+        x0, y0, w, h, alpha, _ = layers[d]
+        trans[d] = T[y0:y0 + h, x0:x0 + w].copy()
+        T[y0:y0 + h, x0:x0 + w] *= 1.0 - alpha * colors[d, 3]
+    color_grad = np.zeros((len(layers), 4))
+    cov_grad = []
+    for d, (x0, y0, w, h, alpha, slope) in enumerate(layers):
+        r = image[y0:y0 + h, x0:x0 + w]
+        gg = g[y0:y0 + h, x0:x0 + w]
+        t = trans[d]
+        g_r = (gg * r).sum(-1)
+        g_c = gg @ colors[d]
+        cov_grad.append((t * (g_c - colors[d, 3] * g_r) * slope, x0, y0, w, h))
+        color_grad[d, :3] = ((t * alpha)[..., None] * gg[..., :3]).sum((0, 1))
+        color_grad[d, 3] = (t * alpha * (gg[..., 3] - g_r)).sum()
+        r *= (1.0 - alpha * colors[d, 3])[..., None]
+        r += alpha[..., None] * colors[d]
+    return cov_grad, color_grad
+
+
+def diff_backward(width, height, clear, draws, colors, lines, image_grad, masks=None):
     """FP64 analytic gradient: (dL/d lines (L, 4), dL/d colours (D, 4)) for dL/d image."""
+    if masks is not None and any(m is not None for m in masks):
+        return _masked_backward(width, height, clear, draws, colors, lines, image_grad, masks)
     starts, counts, rules, lines = _diff_args(draws, lines)
     clear = np.ascontiguousarray(clear, np.float64)
     colors = np.ascontiguousarray(colors, np.float64).reshape(-1, 4)
@@ -250,3 +308,28 @@ def diff_backward(width, height, clear, draws, colors, lines, image_grad):
                                _p(starts), _p(counts), _p(rules), _p(colors), _p(lines), _p(image_grad),
                                _p(line_grad), _p(color_grad))
     return line_grad, color_grad
+
+
+def _masked_backward(width, height, clear, draws, colors, lines, image_grad, masks):
+    """With clip masks: the composite's adjoint in numpy (_masked_render), then the
+    line pass of each draw (or_diff_line_grad_f64) on its dL/dcoverage."""
+    lines = np.ascontiguousarray(lines, np.float64).reshape(-1, 4)
+    cov_grad, color_grad = _masked_render(width, height, clear, draws, colors, lines, masks, image_grad)
+    line_grad = np.zeros_like(lines)
+    for d, (start, count, rule) in enumerate(draws):
+        gmap = np.ascontiguousarray(cov_grad[d][0], np.float64)
+        if count and gmap.size:
+            lib().or_diff_line_grad_f64(_p(lines), C.c_uint32(start), C.c_uint32(count), C.c_uint32(width),
+                                        C.c_uint32(height), _p(gmap), _p(line_grad))
+    return line_grad, color_grad
+
+
+def sdf_draw_coverage(arrays, draw_index):
+    """FP32 SDF coverage of one clip draw at every pixel centre (H, W)."""
+    draws = np.ascontiguousarray(arrays.draws).view(np.uint32).reshape(-1)
+    paint = np.ascontiguousarray(arrays.paint, np.uint32)
+    out = np.zeros((arrays.height, arrays.width), np.float32)
+    lib().or_sdf_draw_coverage(_p(draws), C.c_uint32(len(arrays.draws)), C.c_uint32(draw_index), _p(paint),
+                               C.c_uint32(paint.size), C.c_uint32(arrays.paint_sdf_shadow_base),
+                               C.c_uint32(arrays.width), C.c_uint32(arrays.height), _p(out))
+    return out

@@ -1655,6 +1655,55 @@ void or_diff_render_f64(uint32_t width, uint32_t height, const double *clear, ui
     }
 }
 
+/* dL/d(line endpoints) of lines [start, start + count) from dL/dcoverage over
+ * their draw's box b (gmap, b.w * b.h doubles): the coverage of a pixel moves by
+ * -n.(dp0 int(1-s) + dp1 int s) over the part of a line inside it. */
+static void diff_line_pass(const double *lines, uint32_t start, uint32_t count, DiffBox b, const double *gmap,
+                           double *line_grad) {
+    for (uint32_t i = start; i < start + count; ++i) {
+        const double *l = lines + 4 * (size_t)i;
+        double *out = line_grad + 4 * (size_t)i;
+        double g0 = 0.0, g1 = 0.0;
+        const double x0 = l[0], y0 = l[1], dx = l[2] - l[0], dy = l[3] - l[1];
+        if (b.w && b.h) {
+            int r0 = (int)floor(fmin(y0, l[3])), r1 = dy == 0.0 ? r0 : (int)ceil(fmax(y0, l[3])) - 1;
+            if (r0 < b.y0) r0 = b.y0;
+            if (r1 > b.y0 + b.h - 1) r1 = b.y0 + b.h - 1;
+            for (int r = r0; r <= r1; ++r) {
+                double sa = 0.0, sb = 1.0;
+                if (dy != 0.0) {
+                    double ta = ((double)r - y0) / dy, tb = ((double)(r + 1) - y0) / dy;
+                    sa = fmax(fmin(ta, tb), 0.0);
+                    sb = fmin(fmax(ta, tb), 1.0);
+                }
+                if (!(sb > sa)) continue;
+                double xa = x0 + sa * dx, xb = x0 + sb * dx;
+                int c0 = (int)floor(fmin(xa, xb)), c1 = dx == 0.0 ? c0 : (int)ceil(fmax(xa, xb)) - 1;
+                if (c0 < b.x0) c0 = b.x0;
+                if (c1 > b.x0 + b.w - 1) c1 = b.x0 + b.w - 1;
+                for (int c = c0; c <= c1; ++c) {
+                    double ua = sa, ub = sb;
+                    if (dx != 0.0) {
+                        double qa = ((double)c - x0) / dx, qb = ((double)(c + 1) - x0) / dx;
+                        ua = fmax(fmin(qa, qb), sa);
+                        ub = fmin(fmax(qa, qb), sb);
+                    }
+                    if (!(ub > ua)) continue;
+                    double gv = gmap[(size_t)(r - b.y0) * b.w + (c - b.x0)];
+                    double i1 = 0.5 * (ub * ub - ua * ua);
+                    g0 += gv * ((ub - ua) - i1);
+                    g1 += gv * i1;
+                }
+            }
+        }
+        double nx = y0 - l[3], ny = dx;
+        out[0] = -nx * g0;
+        out[1] = -ny * g0;
+        out[2] = -nx * g1;
+        out[3] = -ny * g1;
+    }
+}
+
 /* dL/d(line endpoints) and dL/d(colour) for a given dL/d(image).  Per pixel,
  * with T_k the product of (1 - alpha_j c_j.a) over the draws above k and R the
  * composite below k: dL/dalpha_k = T_k (g.c_k - c_k.a g.R).  The line pass
@@ -1717,52 +1766,7 @@ void or_diff_backward_f64(uint32_t width, uint32_t height, const double *clear, 
                 for (int k = 0; k < 4; ++k) r[k] = r[k] * f + alpha * c[k];
             }
     }
-    /* lines: integrate dL/dcoverage over the pixels each line crosses */
-    for (uint32_t d = 0; d < draw_count; ++d) {
-        DiffBox b = boxes[d];
-        for (uint32_t i = line_start[d]; i < line_start[d] + line_count[d]; ++i) {
-            const double *l = lines + 4 * (size_t)i;
-            double *out = line_grad + 4 * (size_t)i;
-            double g0 = 0.0, g1 = 0.0;
-            const double x0 = l[0], y0 = l[1], dx = l[2] - l[0], dy = l[3] - l[1];
-            if (b.w && b.h) {
-                int r0 = (int)floor(fmin(y0, l[3])), r1 = dy == 0.0 ? r0 : (int)ceil(fmax(y0, l[3])) - 1;
-                if (r0 < b.y0) r0 = b.y0;
-                if (r1 > b.y0 + b.h - 1) r1 = b.y0 + b.h - 1;
-                for (int r = r0; r <= r1; ++r) {
-                    double sa = 0.0, sb = 1.0;
-                    if (dy != 0.0) {
-                        double ta = ((double)r - y0) / dy, tb = ((double)(r + 1) - y0) / dy;
-                        sa = fmax(fmin(ta, tb), 0.0);
-                        sb = fmin(fmax(ta, tb), 1.0);
-                    }
-                    if (!(sb > sa)) continue;
-                    double xa = x0 + sa * dx, xb = x0 + sb * dx;
-                    int c0 = (int)floor(fmin(xa, xb)), c1 = dx == 0.0 ? c0 : (int)ceil(fmax(xa, xb)) - 1;
-                    if (c0 < b.x0) c0 = b.x0;
-                    if (c1 > b.x0 + b.w - 1) c1 = b.x0 + b.w - 1;
-                    for (int c = c0; c <= c1; ++c) {
-                        double ua = sa, ub = sb;
-                        if (dx != 0.0) {
-                            double qa = ((double)c - x0) / dx, qb = ((double)(c + 1) - x0) / dx;
-                            ua = fmax(fmin(qa, qb), sa);
-                            ub = fmin(fmax(qa, qb), sb);
-                        }
-                        if (!(ub > ua)) continue;
-                        double gv = maps[d][(size_t)(r - b.y0) * b.w + (c - b.x0)];
-                        double i1 = 0.5 * (ub * ub - ua * ua);
-                        g0 += gv * ((ub - ua) - i1);
-                        g1 += gv * i1;
-                    }
-                }
-            }
-            double nx = y0 - l[3], ny = dx;
-            out[0] = -nx * g0;
-            out[1] = -ny * g0;
-            out[2] = -nx * g1;
-            out[3] = -ny * g1;
-        }
-    }
+    for (uint32_t d = 0; d < draw_count; ++d) diff_line_pass(lines, line_start[d], line_count[d], boxes[d], maps[d], line_grad);
     for (uint32_t d = 0; d < draw_count; ++d) {
         free(maps[d]);
         free(trans[d]);
@@ -1772,4 +1776,26 @@ void or_diff_backward_f64(uint32_t width, uint32_t height, const double *clear, 
     free(boxes);
     free(T);
     free(R);
+}
+
+/* The clip mask one SDF clip draw gives each pixel centre of a width x height
+ * canvas: its FP32 coverage as fine evaluates it (sdf_coverage_from_blob at the
+ * draw's inverse transform), before the u8 step. */
+void or_sdf_draw_coverage(const uint32_t *draws, uint32_t draw_count, uint32_t draw_index, const uint32_t *paint,
+                          uint32_t paint_words, uint32_t paint_sdf_shadow_base, uint32_t width, uint32_t height,
+                          float *out) {
+    or_fine_config cfg;
+    memset(&cfg, 0, sizeof(cfg));
+    cfg.paint_words = paint_words;
+    cfg.paint_sdf_shadow_base = paint_sdf_shadow_base;
+    const FDraw d = load_fdraw(draws, draw_count, draw_index);
+    for (uint32_t y = 0; y < height; ++y)
+        for (uint32_t x = 0; x < width; ++x)
+            out[(size_t)y * width + x] = draw_sdf_coverage(&cfg, paint, &d, (float)x + 0.5f, (float)y + 0.5f);
+}
+
+void or_diff_line_grad_f64(const double *lines, uint32_t start, uint32_t count, uint32_t width, uint32_t height,
+                           const double *coverage_grad, double *line_grad) {
+    DiffBox b = diff_box(lines, start, count, width, height);
+    diff_line_pass(lines, start, count, b, coverage_grad, line_grad);
 }

@@ -147,6 +147,18 @@ void or_diff_backward_f64(uint32_t width, uint32_t height, const double *clear, 
                           const double *colors, const double *lines, const double *image_grad, double *line_grad,
                           double *color_grad);
 
+/* The line pass alone: dL/d(line endpoints) of lines [start, start + count) of
+ * one draw from dL/dcoverage over the draw's box (or_diff_area_f64's box, w * h
+ * doubles), written to line_grad[4 * i ..] for each of those lines. */
+void or_diff_line_grad_f64(const double *lines, uint32_t start, uint32_t count, uint32_t width, uint32_t height,
+                           const double *coverage_grad, double *line_grad);
+
+/* The FP32 SDF coverage of clip draw `draw_index` (draw records: 31 words each)
+ * at every pixel centre of the canvas: the mask its clip layer applies. */
+void or_sdf_draw_coverage(const uint32_t *draws, uint32_t draw_count, uint32_t draw_index, const uint32_t *paint,
+                          uint32_t paint_words, uint32_t paint_sdf_shadow_base, uint32_t width, uint32_t height,
+                          float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,25 @@ def weights(h, w, seed=0):
     return np.random.default_rng(seed).normal(size=(h, w, 4))
 
 
+def clip_masks(sc, verts, counts):
+    """Per painter-order draw, the product of the FP32 coverages of the SDF clip
+    layers around it (None outside every clip), from the oracle's SDF evaluation
+    of the encoded clip draws; None for a scene without clips."""
+    if not sc.clips:
+        return None
+    e = diff.encode(sc, verts, counts)
+    arrays = tk.SceneArrays(e.scene)
+    cov = [oracle.sdf_draw_coverage(arrays, cid).astype(np.float64) for cid in e.clip_ids]
+    masks = []
+    for d in range(len(counts)):
+        m = None
+        for c, clip in enumerate(sc.clips):
+            if clip[5] <= d < clip[6]:
+                m = cov[c] if m is None else m * cov[c]
+        masks.append(m)
+    return masks
+
+
 def analytic_grads(sc, w):
     """dL/d(parameters) for L = sum(w * image) from the FP64 oracle's line and colour
     gradients, carried through the geometry by torch."""
@@ -28,7 +47,8 @@ def analytic_grads(sc, w):
     v, k = sc.polygons(st)
     pm = sc.premultiplied()
     draws, lines, la, lb = diff.oracle_inputs(sc, v, k)
-    lg, cg = oracle.diff_backward(sc.width, sc.height, diff.clear_color(sc), draws, pm.detach().numpy(), lines, w)
+    lg, cg = oracle.diff_backward(sc.width, sc.height, diff.clear_color(sc), draws, pm.detach().numpy(), lines, w,
+                                  masks=clip_masks(sc, v, k))
     gv = torch.zeros_like(v)
     lgt = torch.as_tensor(lg)
     gv.index_add_(0, torch.as_tensor(la), lgt[:, :2])
@@ -37,16 +57,19 @@ def analytic_grads(sc, w):
     return st
 
 
-def fd_loss(sc, st, w):
+def fd_loss(sc, st, w, masks=None):
     with torch.no_grad():
         v, k = sc.polygons(st)
         draws, lines, _, _ = diff.oracle_inputs(sc, v, k)
-        image = oracle.diff_render(sc.width, sc.height, diff.clear_color(sc), draws, sc.premultiplied().numpy(), lines)
+        image = oracle.diff_render(sc.width, sc.height, diff.clear_color(sc), draws, sc.premultiplied().numpy(), lines,
+                                   masks=masks)
     return float((image * w).sum())
 
 
 def check_fd(sc, w, per_param=12, h=1e-3, seed=1, rtol=2e-5):
     st = analytic_grads(sc, w)
+    with torch.no_grad():
+        masks = clip_masks(sc, *sc.polygons(st))
     rng = np.random.default_rng(seed)
     checked = 0
     for p in (sc.fills, sc.strokes, sc.widths, sc.colors):
@@ -58,9 +81,9 @@ def check_fd(sc, w, per_param=12, h=1e-3, seed=1, rtol=2e-5):
         for i in rng.choice(flat.numel(), size=min(per_param, flat.numel()), replace=False):
             o = float(flat[i])
             flat[i] = o + h
-            lp = fd_loss(sc, st, w)
+            lp = fd_loss(sc, st, w, masks)
             flat[i] = o - h
-            lm = fd_loss(sc, st, w)
+            lm = fd_loss(sc, st, w, masks)
             flat[i] = o
             fd = (lp - lm) / (2 * h)
             assert abs(fd - float(g[i])) <= rtol * max(abs(fd), 0.01 * scale) + 1e-9, (i, fd, float(g[i]))
@@ -164,10 +187,15 @@ def test_uniform_flattening_stays_within_tolerance():
 def test_recorded_lines_are_the_polygon_edges():
     sc = diff.random_scene(11, 120, 90, fills=6, strokes=4)
     v, k = sc.polygons()
-    canvas, enc, la, lb, per_draw = diff.encode(sc, v, k)  # raises when they differ
-    assert enc.line_count == len(la) == int(per_draw.sum())
-    assert enc.draw_count == len(k) and enc.layer_count == 0
-    assert per_draw.min() >= 3
+    e = diff.encode(sc, v, k)  # raises when they differ
+    assert e.scene.line_count == len(e.line_a) == int(e.per_draw.sum())
+    assert e.scene.draw_count == len(k) == e.draw_total and e.scene.layer_count == 0
+    assert e.per_draw.min() >= 3
+    assert list(e.draw_ids) == list(range(len(k)))
+    # with clip layers the clip draws take encoded draw indices of their own
+    sc.clips = [(5.0, 5.0, 100.0, 80.0, 4.0, 2, 7)]
+    e = diff.encode(sc, v, k)
+    assert e.draw_total == len(k) + 1 and e.clip_ids == [2] and list(e.draw_ids[:3]) == [0, 1, 3]
 
 
 def test_stroke_outline_width_is_the_stroke_width():
@@ -183,3 +211,34 @@ def test_stroke_outline_width_is_the_stroke_width():
     image = oracle.diff_render(64, 48, [0, 0, 0, 0], draws, sc.premultiplied().detach().numpy(), lines)
     # capsule area: 30 x 6 rectangle + a disc of radius 3 (its polygon is slightly smaller)
     assert 30 * 6 + 0.97 * math.pi * 9 < image[:, :, 3].sum() <= 30 * 6 + math.pi * 9
+
+
+def test_clipped_gradients_match_finite_differences():
+    # Nested rounded-rect clips: the draws inside take the clips' coverage as a mask.
+    sc = diff.random_scene(12, 96, 80, fills=6, strokes=3, cubics=3, radius_mu=math.log(14.0))
+    sc.clips = [(10.5, 8.25, 80.0, 66.0, 7.0, 1, 8), (25.0, 12.5, 62.0, 50.0, 4.5, 3, 6)]
+    w = weights(80, 96, 12)
+    assert check_fd(sc, w) >= 30
+    # the masks matter: the clipped image differs from the unclipped one
+    with torch.no_grad():
+        v, k = sc.polygons()
+        draws, lines, _, _ = diff.oracle_inputs(sc, v, k)
+        pm = sc.premultiplied().numpy()
+        clipped = oracle.diff_render(96, 80, [0, 0, 0, 0], draws, pm, lines, masks=clip_masks(sc, v, k))
+        plain = oracle.diff_render(96, 80, [0, 0, 0, 0], draws, pm, lines)
+    assert np.abs(clipped - plain).max() > 0.1
+
+
+def test_masked_oracle_equals_plain_oracle_under_unit_masks():
+    sc = diff.random_scene(13, 64, 48, fills=5, strokes=2, radius_mu=math.log(12.0))
+    with torch.no_grad():
+        v, k = sc.polygons()
+        draws, lines, _, _ = diff.oracle_inputs(sc, v, k)
+        pm = sc.premultiplied().numpy()
+    ones = [np.ones((48, 64))] * len(draws)
+    g = weights(48, 64, 13)
+    assert np.allclose(oracle.diff_render(64, 48, [0.1, 0, 0, 0.2], draws, pm, lines, masks=ones),
+                       oracle.diff_render(64, 48, [0.1, 0, 0, 0.2], draws, pm, lines), atol=1e-12)
+    la, ca = oracle.diff_backward(64, 48, [0.1, 0, 0, 0.2], draws, pm, lines, g, masks=ones)
+    lb, cb = oracle.diff_backward(64, 48, [0.1, 0, 0, 0.2], draws, pm, lines, g)
+    assert np.allclose(la, lb, atol=1e-10) and np.allclose(ca, cb, atol=1e-10)

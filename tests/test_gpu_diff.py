@@ -6,7 +6,9 @@ tk_frame_backward) through the torch.autograd harness:
   analytic gradient and against FP64 central finite differences (h = 1e-3 px),
   within 1e-3 relative;
 * a batch of scenes against each scene alone;
-* scenes outside the form (layers) are rejected."""
+* clipped scenes (SDF clip layers) against the same oracle with the clips'
+  coverage as masks;
+* scenes outside the form (path clip layers) are rejected."""
 import math
 
 import numpy as np
@@ -17,7 +19,7 @@ import oracle
 import tileink_amd as tk
 from tileink_amd import diff
 
-from test_diff import analytic_grads, fd_loss
+from test_diff import analytic_grads, clip_masks, fd_loss
 
 pytestmark = pytest.mark.gpu
 
@@ -37,10 +39,11 @@ def gpu_grads(ctx, sc, w):
 
 def oracle_image(sc):
     """The FP64 composite of the lines as recorded (FP32 vertices)."""
-    v, k = sc.polygons()
-    draws, lines, _, _ = diff.oracle_inputs(sc, v, k, recorded=True)
-    return oracle.diff_render(sc.width, sc.height, diff.clear_color(sc), draws,
-                              sc.premultiplied().detach().numpy(), lines)
+    with torch.no_grad():
+        v, k = sc.polygons()
+        draws, lines, _, _ = diff.oracle_inputs(sc, v, k, recorded=True)
+        return oracle.diff_render(sc.width, sc.height, diff.clear_color(sc), draws,
+                                  sc.premultiplied().detach().numpy(), lines, masks=clip_masks(sc, v, k))
 
 
 def assert_grads_close(got, ref, rtol=1e-3, floor=0.02):
@@ -80,6 +83,21 @@ def test_gradients_with_opaque_overlaps_and_even_odd(ctx):
     sc.rules[:] = 1
     sc.clear = (0.2, 0.1, 0.0, 0.4)
     w = np.random.default_rng(9).normal(size=(96, 96, 4))
+    image, got = gpu_grads(ctx, sc, w)
+    err = float(np.abs(image.detach().cpu().double().numpy() - oracle_image(sc)).max())
+    assert err < 1e-5, err
+    analytic_grads(sc, w)
+    assert_grads_close(got, [p.grad for p in (sc.fills, sc.strokes, sc.widths, sc.colors)])
+
+
+@pytest.mark.parametrize("seed", [14, 15])
+def test_clipped_scenes_match_fp64(ctx, seed):
+    # Nested rounded-rect clip layers around ranges of draws (SDF clips: the draws'
+    # alpha is multiplied by the clips' FP32 coverage).
+    sc = diff.random_scene(seed, 160, 112, fills=20, strokes=6, radius_mu=math.log(18.0))
+    sc.clips = [(12.5, 9.25, 140.0, 100.0, 9.0, 2, 20), (40.0, 20.5, 110.0, 80.0, 5.5, 6, 14),
+                (0.0, 0.0, 60.0, 112.0, 0.0, 22, 26)]
+    w = np.random.default_rng(seed).normal(size=(112, 160, 4))
     image, got = gpu_grads(ctx, sc, w)
     err = float(np.abs(image.detach().cpu().double().numpy() - oracle_image(sc)).max())
     assert err < 1e-5, err
@@ -131,6 +149,7 @@ def test_gradients_match_finite_differences_at_1024(ctx):
 def test_batch_equals_each_scene_alone(ctx):
     scenes = [diff.random_scene(20 + i, w, h, fills=f, strokes=s, radius_mu=math.log(12.0))
               for i, (w, h, f, s) in enumerate([(64, 48, 12, 3), (200, 90, 30, 6), (33, 130, 8, 2)])]
+    scenes[1].clips = [(20.5, 10.0, 180.0, 80.0, 6.0, 5, 30)]
     ws = [np.random.default_rng(i).normal(size=(s.height, s.width, 4)) for i, s in enumerate(scenes)]
     singles = []
     for sc, w in zip(scenes, ws):
@@ -149,13 +168,15 @@ def test_batch_equals_each_scene_alone(ctx):
         assert_grads_close([p.grad for p in (sc.fills, sc.strokes, sc.widths, sc.colors)], ref_g, rtol=1e-4)
 
 
-def test_scenes_with_layers_are_rejected(ctx):
+def test_scenes_with_path_clip_layers_are_rejected(ctx):
+    # Path clip layers (BEGIN_CLIP records with their own coverage) are not in the form.
     canvas = tk.Canvas(64, 64)
-    canvas.push_clip_sdf_rect_layer(4.0, 4.0, 60.0, 60.0, 3.0)
+    canvas.push_clip_layer(tk.BezPath().move_to(4, 4).line_to(60, 6).line_to(30, 60).close_path())
     canvas.push_polygon(np.array([[10.0, 10.0], [50.0, 12.0], [30.0, 50.0]]), tk.Color(1, 0, 0, 1))
     canvas.pop_layer()
-    frame = tk.Frame(ctx, canvas.encode(0))
-    colors = torch.ones((1, 4), device=DEV)
+    scene = canvas.encode(0)
+    frame = tk.Frame(ctx, scene)
+    colors = torch.ones((scene.draw_count, 4), device=DEV)
     image = torch.zeros((64 * 64, 4), device=DEV)
     with pytest.raises(tk.TileinkError):
         tk._abi.check(tk._abi.lib().tk_frame_render_diff(frame._h, colors.data_ptr(), image.data_ptr()))

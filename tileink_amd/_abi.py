@@ -18,6 +18,7 @@ INVALID = 0xFFFFFFFF
 
 PATH_MOVE, PATH_LINE, PATH_QUAD, PATH_CUBIC, PATH_CLOSE = range(5)
 FILL_NONZERO, FILL_EVEN_ODD = 0, 1
+DRAW_BRUSH, DRAW_CLIP = 0, 1
 BRUSH_SOLID, BRUSH_LINEAR, BRUSH_RADIAL, BRUSH_SWEEP, BRUSH_FOUR_CORNER = 1, 2, 3, 4, 5
 
 # Frame buffer identifiers (tk_frame_buffer).

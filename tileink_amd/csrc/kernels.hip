@@ -1808,9 +1808,46 @@ __device__ __forceinline__ bool diff_tile(const tk_args_fine_diff &a, DiffTile &
     return true;
 }
 
-// Calls body(pi, rec) for every FILL record of the tile's streams, bottom to
-// top; flags any other record kind.
-template <class Body>
+// The clip mask of the SDF clip layers open at a pixel: the product of their
+// FP32 coverages (the mask fine keeps as combine_alpha of u8 coverages), with up
+// to four levels; deeper nesting is flagged.
+struct ClipMask {
+    float m, s0, s1, s2, s3;
+    uint32_t depth;
+    __device__ __forceinline__ bool push(float coverage) {
+        if (depth == 0u) s0 = m;
+        else if (depth == 1u) s1 = m;
+        else if (depth == 2u) s2 = m;
+        else if (depth == 3u) s3 = m;
+        else return false;
+        ++depth;
+        m *= coverage;
+        return true;
+    }
+    __device__ __forceinline__ void pop() {
+        if (depth == 0u) return;
+        --depth;
+        m = depth == 0u ? s0 : depth == 1u ? s1 : depth == 2u ? s2 : s3;
+    }
+};
+
+// Takes a clip record (BEGIN_SDF_CLIP, END_CLIP) into the mask; false for a
+// record the differentiable form does not take.
+__device__ __forceinline__ bool clip_record(const tk_args_fine_diff &a, const DiffTile &t, const tk_ptcl_record &rec,
+                                            ClipMask &clip) {
+    if (rec.tag == TK_PTCL_END_CLIP) {
+        clip.pop();
+        return true;
+    }
+    if (rec.tag != TK_PTCL_BEGIN_SDF_CLIP) return false;
+    const FineDraw d = load_fine_draw(a.in.draws, a.cfg.draw_count, rec.color);
+    return clip.push(sdf_coverage_from_draw(a.cfg, a.in.paint, d, (float)t.gx + 0.5f, (float)t.gy + 0.5f));
+}
+
+// Calls body(pi, rec, mask) for every FILL record of the tile's streams, bottom
+// to top, with the pixel's clip mask; flags any record kind it does not take.
+// kClips: the scene has SDF clip layers (their records are taken into the mask).
+template <bool kClips, class Body>
 __device__ __forceinline__ void for_each_fill(const tk_args_fine_diff &a, const DiffTile &t, Body body) {
     const tk_fine_config &cfg = a.cfg;
     for (uint32_t b = t.v.q0; b < t.v.q1; ++b) {
@@ -1819,19 +1856,20 @@ __device__ __forceinline__ void for_each_fill(const tk_args_fine_diff &a, const 
         if ((kind_word & TK_TILE_KIND_MASK) == TK_TILE_KIND_EMPTY) continue;
         const tk_segment_range r = a.in.tile_ptcl_ranges[idx];
         const uint32_t pend = min(r.end, cfg.ptcl_capacity);
+        ClipMask clip{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0u};
         for (uint32_t pi = r.start + (kind_word >> TK_TILE_PREFIX_SHIFT); pi < pend; ++pi) {
             const tk_ptcl_record rec = a.in.ptcl[pi];
             if (rec.tag == TK_PTCL_END) break;
-            if (rec.tag != TK_PTCL_FILL || rec.color >= cfg.draw_count) {
+            if (rec.tag == TK_PTCL_FILL && rec.color < cfg.draw_count) {
+                body(pi, rec, clip.m);
+            } else if (!kClips || !clip_record(a, t, rec, clip)) {
                 if (t.lane == 0u && a.status) atomicOr(a.status, 1u);
-                continue;
             }
-            body(pi, rec);
         }
     }
 }
 
-template <bool kScenes>
+template <bool kScenes, bool kClips>
 __device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &a) {
     __shared__ CoverageAcc cov_acc[2];
     DiffTile t;
@@ -1852,7 +1890,7 @@ __device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &
     bool overflow = false;
     Transmit total{1.0f, 0, 0};
     const uint32_t tx = t.tile % t.v.tw, ty = t.tile / t.v.tw;
-    for_each_fill(a, t, [&](uint32_t pi, const tk_ptcl_record &rec) {
+    for_each_fill<kClips>(a, t, [&](uint32_t pi, const tk_ptcl_record &rec, float mask) {
         const uint32_t s0 = rec.segment_start, s1 = min(rec.segment_end, a.cfg.segment_capacity);
         const float4 c = colors[rec.color];
         // Issued before the coverage work so that its latency overlaps it.
@@ -1863,7 +1901,7 @@ __device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &
         __syncthreads();
         const float cov = take_coverage(acc, rec.backdrop, lane);
         float slope;
-        const float alpha = diff_alpha(cov, rec.fill_rule, slope);
+        const float alpha = diff_alpha(cov, rec.fill_rule, slope) * mask;
         const float f = 1.0f - alpha * c.w;
         total.mul(f);
         px.r = px.r * f + alpha * c.x;
@@ -1888,7 +1926,7 @@ __device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &
             make_float4(px.r, px.g, px.b, px.a);
 }
 
-template <bool kScenes>
+template <bool kScenes, bool kClips>
 __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff &a) {
     DiffTile t;
     if (!diff_tile<kScenes>(a, t)) return;
@@ -1901,9 +1939,10 @@ __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff 
     const Transmit total{__uint_as_float(tw.x), (int)tw.y, (int)tw.z};
     Transmit below{1.0f, 0, 0};
     Px res = rgba8_to_unorm(t.v.clear);
-    auto record = [&](uint32_t pi, const tk_ptcl_record &rec, float cov, const float4 &c) {
+    auto record = [&](uint32_t pi, const tk_ptcl_record &rec, float cov, const float4 &c, float mask) {
         float slope;
-        const float alpha = diff_alpha(cov, rec.fill_rule, slope);
+        const float alpha = diff_alpha(cov, rec.fill_rule, slope) * mask;
+        slope *= mask;
         const float f = 1.0f - alpha * c.w;
         below.mul(f);
         const float T = total.zeros > below.zeros ? 0.0f : ldexpf(total.m / below.m, total.e - below.e);
@@ -1937,6 +1976,7 @@ __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff 
         tk_ptcl_record rec = a.in.ptcl[pi];
         float cov = coverage(pi);
         float4 c = color_of(rec);
+        ClipMask clip{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0u};
         while (rec.tag != TK_PTCL_END) {
             const uint32_t next = pi + 1u;
             const bool more = next < pend;
@@ -1944,8 +1984,8 @@ __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff 
             if (more) nrec = a.in.ptcl[next];
             const float ncov = more ? coverage(next) : 0.0f;
             const float4 nc = more ? color_of(nrec) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (rec.tag == TK_PTCL_FILL && rec.color < draws) record(pi, rec, cov, c);
-            else if (lane == 0u && a.status) atomicOr(a.status, 1u);
+            if (rec.tag == TK_PTCL_FILL && rec.color < draws) record(pi, rec, cov, c, clip.m);
+            else if ((!kClips || !clip_record(a, t, rec, clip)) && lane == 0u && a.status) atomicOr(a.status, 1u);
             if (!more) break;
             pi = next;
             rec = nrec;
@@ -1970,18 +2010,21 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_draw_tiles(tk_args_draw_t
     reinterpret_cast<uint4 *>(a.table)[d] = e;
 }
 
-extern "C" __global__ __launch_bounds__(256) void tk_k_fine_diff(tk_args_fine_diff a) {
-    fine_diff_forward_tile<false>(a);
-}
-extern "C" __global__ __launch_bounds__(256) void tk_k_fine_diff_scenes(tk_args_fine_diff a) {
-    fine_diff_forward_tile<true>(a);
-}
-extern "C" __global__ __launch_bounds__(256) void tk_k_fine_backward(tk_args_fine_diff a) {
-    fine_diff_backward_tile<false>(a);
-}
-extern "C" __global__ __launch_bounds__(256) void tk_k_fine_backward_scenes(tk_args_fine_diff a) {
-    fine_diff_backward_tile<true>(a);
-}
+// One canvas or a batch of scenes (_scenes), without or with SDF clip layers
+// (_clips: the clip evaluation raises the register count, so scenes without
+// layers run the build that has none).
+#define TK_DIFF_KERNELS(suffix, scenes, clips)                                                                  \
+    extern "C" __global__ __launch_bounds__(256) void tk_k_fine_diff##suffix(tk_args_fine_diff a) {             \
+        fine_diff_forward_tile<scenes, clips>(a);                                                                \
+    }                                                                                                           \
+    extern "C" __global__ __launch_bounds__(256) void tk_k_fine_backward##suffix(tk_args_fine_diff a) {         \
+        fine_diff_backward_tile<scenes, clips>(a);                                                               \
+    }
+TK_DIFF_KERNELS(, false, false)
+TK_DIFF_KERNELS(_scenes, true, false)
+TK_DIFF_KERNELS(_clips, false, true)
+TK_DIFF_KERNELS(_scenes_clips, true, true)
+#undef TK_DIFF_KERNELS
 
 // One thread per line: walks the pixels the line crosses inside its path's
 // tile box, row slab by row slab and then column by column, and integrates

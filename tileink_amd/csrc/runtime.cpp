@@ -77,7 +77,8 @@ struct tk_context {
                   f_fine = nullptr, f_coarse_count_scenes = nullptr, f_coarse_emit_scenes = nullptr,
                   f_fine_scenes = nullptr, f_fine_diff = nullptr, f_fine_diff_scenes = nullptr,
                   f_fine_backward = nullptr, f_fine_backward_scenes = nullptr, f_line_backward = nullptr,
-                  f_draw_tiles = nullptr,
+                  f_draw_tiles = nullptr, f_fine_diff_clips = nullptr, f_fine_diff_scenes_clips = nullptr,
+                  f_fine_backward_clips = nullptr, f_fine_backward_scenes_clips = nullptr,
                   f_fill_coverage = nullptr, f_pixel_math = nullptr, f_brush_words = nullptr;
     // Scratch for the stage entry points (frames own their own).
     uint32_t *tickets = nullptr;  // [0] scan prefix, [1] coarse scan
@@ -402,12 +403,19 @@ uint32_t layer_pairs(const tk_scene *s, std::vector<uint32_t> &pair_of, std::vec
 // coarse kernels evaluate each layer themselves.
 constexpr uint64_t kMaxLayerClassBytes = 64ull << 20;
 
-// The differentiable form takes scenes of path draws (fills and strokes) with
-// no layers: every PTCL record is then a FILL carrying its draw index.
+// The differentiable form takes scenes of path draws (fills and strokes) inside
+// any nesting of SDF (rounded-rectangle) clip layers: every PTCL record is then a
+// FILL carrying its draw index, a BEGIN_SDF_CLIP or an END_CLIP.
 bool diff_compatible(const tk_scene *s) {
-    if (s->layer_count != 0) return false;
+    for (uint32_t l = 0; l < s->layer_count; ++l) {
+        const tk_layer_record &layer = s->layers[l];
+        if (layer.tag != 0u || layer.draw >= s->draw_count) return false;
+        const tk_draw_record &r = s->draws[layer.draw];
+        if (r.tag != TK_DRAW_CLIP || r.sdf_offset == TK_INVALID_INDEX) return false;
+    }
     for (uint32_t d = 0; d < s->draw_count; ++d) {
         const tk_draw_record &r = s->draws[d];
+        if (r.tag == TK_DRAW_CLIP && r.sdf_offset != TK_INVALID_INDEX) continue;
         if (r.tag != TK_DRAW_BRUSH || r.path_id >= s->path_count || r.sdf_offset != TK_INVALID_INDEX ||
             r.sdf_shadow_offset != TK_INVALID_INDEX)
             return false;
@@ -597,7 +605,10 @@ tk_status run_fine_diff(tk_frame *f, const DiffTarget &t) {
     }
     tk_args_fine_diff a = diff_args(f, t.colors);
     a.image = t.image;
-    return launch_tiles(ctx, f->scene_count ? ctx->f_fine_diff_scenes : ctx->f_fine_diff, a.cfg.tile_count, a);
+    const bool clips = f->meta.layer_count != 0;
+    hipFunction_t fn = f->scene_count ? (clips ? ctx->f_fine_diff_scenes_clips : ctx->f_fine_diff_scenes)
+                                      : (clips ? ctx->f_fine_diff_clips : ctx->f_fine_diff);
+    return launch_tiles(ctx, fn, a.cfg.tile_count, a);
 }
 
 tk_status frame_render(tk_frame *f, uint32_t active_count = 0, const DiffTarget *diff = nullptr) {
@@ -930,6 +941,10 @@ tk_status tk_context_create(int device, tk_context **out) {
         {&ctx->f_fine_backward_scenes, "tk_k_fine_backward_scenes"},
         {&ctx->f_line_backward, "tk_k_line_backward"},
         {&ctx->f_draw_tiles, "tk_k_draw_tiles"},
+        {&ctx->f_fine_diff_clips, "tk_k_fine_diff_clips"},
+        {&ctx->f_fine_diff_scenes_clips, "tk_k_fine_diff_scenes_clips"},
+        {&ctx->f_fine_backward_clips, "tk_k_fine_backward_clips"},
+        {&ctx->f_fine_backward_scenes_clips, "tk_k_fine_backward_scenes_clips"},
         {&ctx->f_fill_coverage, "tk_k_fill_coverage"},
         {&ctx->f_pixel_math, "tk_k_pixel_math"},
         {&ctx->f_brush_words, "tk_k_brush_words"},
@@ -1386,7 +1401,8 @@ tk_status tk_frame_autotune(tk_frame *f, uint32_t repeats, tk_tuning *chosen) {
 tk_status tk_frame_render_diff(tk_frame *f, const float *colors, float *image) {
     if (!f || !colors || !image) return fail(TK_ERR_INVALID, "null argument");
     if (!f->diff_scene)
-        return fail(TK_ERR_INVALID, "the differentiable form takes path draws (fills, strokes) without layers");
+        return fail(TK_ERR_INVALID,
+                    "the differentiable form takes path draws (fills, strokes) and SDF clip layers only");
     const tk::DiffTarget t{colors, image};
     TK_TRY(tk::frame_render(f, 0, &t));
     f->diff_ready = true;
@@ -1403,8 +1419,10 @@ tk_status tk_frame_backward(tk_frame *f, const float *colors, const float *image
     tk_args_fine_diff a = tk::diff_args(f, colors);
     a.image_grad = image_grad;
     a.color_grad = color_grad;
-    TK_TRY(tk::launch_tiles(ctx, f->scene_count ? ctx->f_fine_backward_scenes : ctx->f_fine_backward,
-                            a.cfg.tile_count, a));
+    const bool clips = s.layer_count != 0;
+    hipFunction_t fn = f->scene_count ? (clips ? ctx->f_fine_backward_scenes_clips : ctx->f_fine_backward_scenes)
+                                      : (clips ? ctx->f_fine_backward_clips : ctx->f_fine_backward);
+    TK_TRY(tk::launch_tiles(ctx, fn, a.cfg.tile_count, a));
     if (s.line_count) {
         tk_args_line_backward b{f->lines, s.line_count, s.path_count, f->paths, f->diff_rec_of, s.backdrop_len,
                                 f->diff_adj, f->diff_adj_records, line_grad};

@@ -82,10 +82,13 @@ class DiffScene:
     order:   painter order as (kind, index) pairs, kind 0 fill / 1 stroke
              (default: all fills, then all strokes).
     rules:   fill rule per fill (0 NonZero, 1 EvenOdd); strokes use NonZero.
+    clips:   SDF clip layers as (x0, y0, x1, y1, radius, first, end): a rounded
+             rectangle clipping the draws [first, end) of painter order; ranges
+             nest or are disjoint.  The clip geometry is not differentiated.
     """
 
     def __init__(self, width, height, fills=None, strokes=None, widths=None, colors=None, order=None, rules=None,
-                 clear=(0.0, 0.0, 0.0, 0.0), tolerance=TOLERANCE):
+                 clear=(0.0, 0.0, 0.0, 0.0), tolerance=TOLERANCE, clips=None):
         self.width, self.height = int(width), int(height)
         self.fills = fills
         self.strokes = strokes
@@ -97,6 +100,7 @@ class DiffScene:
         self.rules = np.zeros(nf, np.int64) if rules is None else np.asarray(rules, np.int64)
         self.clear = tuple(float(c) for c in clear)
         self.tolerance = float(tolerance)
+        self.clips = [tuple(c) for c in (clips or [])]
 
     # --- geometry -------------------------------------------------------------------
 
@@ -298,18 +302,41 @@ def edge_map(verts32, counts):
     return cat(a_idx), cat(b_idx), np.array(per_draw, np.int64)
 
 
+class Encoded:
+    """A scene recorded from polygons: the canvas and encoded scene, the line ->
+    vertex map (line_a, line_b), lines per painter-order draw, each painter-order
+    draw's index among the encoded draws (draw_ids), the encoded draw count and
+    the encoded index of each clip layer's draw (clip_ids)."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
 def encode(scene, verts, counts, colors=None):
-    """Records the polygons on a canvas and encodes it.  Returns (canvas, encoded
-    scene, line_a, line_b, lines per draw)."""
+    """Records the polygons (inside the scene's SDF clip layers) on a canvas and
+    encodes it; checks that the recorded lines are the polygon edges."""
     v32 = verts.detach().to("cpu", torch.float32).numpy()
     canvas = tk.Canvas(scene.width, scene.height)
     rules = scene.draw_rules()
     col = (scene.colors if colors is None else colors).detach().cpu().double().numpy()
-    base = 0
+    # outer clips first where several open at one draw, inner first where several close
+    opens = sorted(range(len(scene.clips)), key=lambda i: (scene.clips[i][5], -scene.clips[i][6]))
+    base, draw_ids, o = 0, [], 0
+    open_stack = []
     for d, k in enumerate(counts):
+        while o < len(opens) and scene.clips[opens[o]][5] == d:
+            x0, y0, x1, y1, radius = scene.clips[opens[o]][:5]
+            canvas.push_clip_sdf_rect_layer(x0, y0, x1, y1, radius)
+            open_stack.append(opens[o])
+            o += 1
         c = np.clip(col[d], 0.0, 1.0)
-        canvas.push_polygon(v32[base:base + k], tk.Color(*c), tk.FillRule(int(rules[d])))
+        draw_ids.append(canvas.push_polygon(v32[base:base + k], tk.Color(*c), tk.FillRule(int(rules[d]))))
         base += int(k)
+        while open_stack and scene.clips[open_stack[-1]][6] == d + 1:
+            canvas.pop_layer()
+            open_stack.pop()
+    if o != len(opens) or open_stack:
+        raise ValueError("clip ranges must nest, lie within the draws and not be empty")
     clear = np.clip(np.asarray(scene.clear), 0, 1)
     cw = int(round(clear[0] * 255)) | int(round(clear[1] * 255)) << 8 | int(round(clear[2] * 255)) << 16 | \
         int(round(clear[3] * 255)) << 24
@@ -320,7 +347,13 @@ def encode(scene, verts, counts, colors=None):
     want = np.concatenate([v32[la], v32[lb]], 1) if len(la) else np.zeros((0, 4), np.float32)
     if got.shape != want.shape or not np.array_equal(got, want):
         raise tk.TileinkError(_abi.TK_ERR_INVALID, "recorded lines do not match the polygon edges")
-    return canvas, enc, la, lb, per_draw
+    tags = _abi.view(enc.draws, _abi.DrawRecord, enc.draw_count)["tag"] if enc.draw_count else np.zeros(0)
+    clip_draws = [int(i) for i in np.nonzero(tags == _abi.DRAW_CLIP)[0]]
+    clip_ids = [0] * len(scene.clips)
+    for n, i in enumerate(opens):
+        clip_ids[i] = clip_draws[n]
+    return Encoded(canvas=canvas, scene=enc, line_a=la, line_b=lb, per_draw=per_draw,
+                   draw_ids=np.asarray(draw_ids, np.int64), draw_total=int(enc.draw_count), clip_ids=clip_ids)
 
 
 def clear_color(scene):
@@ -342,15 +375,21 @@ class Plan:
         v_off, l_off = 0, 0
         self.line_ranges = []
         colors_of = colors or [None] * len(scenes)
+        draw_index, d_off = [], 0
+        self.encodings = []
         for s, v, k, col in zip(scenes, verts, counts, colors_of):
-            canvas, enc, a, b, per_draw = encode(s, v, k, col)
-            self.canvases.append(canvas)
-            self.encoded.append(enc)
-            la.append(a + v_off)
-            lb.append(b + v_off)
-            self.line_ranges.append(per_draw)
+            e = encode(s, v, k, col)
+            self.encodings.append(e)
+            self.canvases.append(e.canvas)
+            self.encoded.append(e.scene)
+            la.append(e.line_a + v_off)
+            lb.append(e.line_b + v_off)
+            self.line_ranges.append(e.per_draw)
+            draw_index.append(e.draw_ids + d_off)
             v_off += len(v)
-            self.sizes.append((s.width, s.height, enc.tiles_width * enc.tiles_height))
+            d_off += e.draw_total
+            self.sizes.append((s.width, s.height, e.scene.tiles_width * e.scene.tiles_height))
+        self.draw_index = np.concatenate(draw_index)
         self.vertex_count = v_off
         if len(scenes) == 1:
             self.frame = tk.Frame(context, self.encoded[0])
@@ -367,13 +406,20 @@ class Plan:
         self.line_b = np.concatenate(lb)
         self._la_t = self._lb_t = None
         self.line_count = len(self.line_a)
-        self.draw_count = sum(len(k) for k in counts)
+        self.draw_count = d_off  # encoded draws, clip draws included
 
     def _index(self, device):
         if self._la_t is None or self._la_t.device != device:
             self._la_t = torch.as_tensor(self.line_a, device=device)
             self._lb_t = torch.as_tensor(self.line_b, device=device)
         return self._la_t, self._lb_t
+
+    def full_colors(self, premultiplied):
+        """One colour per encoded draw (clip draws get zeros) from the painter-order
+        path-draw colours; differentiable."""
+        idx = torch.as_tensor(self.draw_index, device=premultiplied.device)
+        full = torch.zeros((self.draw_count, 4), dtype=premultiplied.dtype, device=premultiplied.device)
+        return full.index_copy(0, idx, premultiplied)
 
     def images(self, flat):
         out = []
@@ -442,8 +488,8 @@ def render(context, scenes, device="cuda"):
     verts = [v for v, _ in polys]
     counts = [k for _, k in polys]
     plan = Plan(context, scenes, verts, counts)
-    colors = torch.cat([s.premultiplied() for s in scenes])
-    flat = _Render.apply(torch.cat(verts), colors, plan, torch.device(device))
+    flat = _Render.apply(torch.cat(verts), plan.full_colors(torch.cat([s.premultiplied() for s in scenes])), plan,
+                         torch.device(device))
     images = plan.images(flat)
     return images[0] if single else images
 
