@@ -158,8 +158,9 @@ def scene_loop(ctx, count=64, repeats=5):
 
 def diff_batch(ctx, count=256, repeats=3):
     """Forward + backward of the differentiable form over many scenes: 256 scenes of
-    256x256, each with 64-512 closed paths of 4 cubics (solid colours, no layers),
-    loss = squared error against a random target.  A step renders every scene in FP32
+    256x256, each with 64-512 closed paths of 4 cubics and solid colours under one
+    rounded-rectangle clip (inset 4-24 px, radius 4-16 px), loss = squared error
+    against a random target.  A step renders every scene in FP32
     (tk_frame_render_diff) and back-propagates dL/d(image) to line endpoints and
     colours (tk_frame_backward), from scenes already resident in HBM.  Timed once as
     a per-scene loop (one frame per scene) and once as one batch."""
@@ -169,22 +170,26 @@ def diff_batch(ctx, count=256, repeats=3):
     paths = rng.integers(64, 513, count)
     sc = [diff.random_scene(1000 + i, 256, 256, fills=int(n), cubics=4, radius_mu=np.log(16.0))
           for i, n in enumerate(paths)]
+    for s, n in zip(sc, paths):
+        inset = rng.uniform(4, 24, 4)
+        s.clips = [(inset[0], inset[1], 256 - inset[2], 256 - inset[3], float(rng.uniform(4, 16)), 0, int(n))]
     polys = [s.polygons() for s in sc]
-    colors = [s.premultiplied().to("cuda", torch.float32) for s in sc]
     singles = [diff.Plan(ctx, [s], [v], [k]) for s, (v, k) in zip(sc, polys)]
     batch = diff.Plan(ctx, sc, [v for v, _ in polys], [k for _, k in polys])
+    colors = [p.full_colors(s.premultiplied()).to("cuda", torch.float32) for p, s in zip(singles, sc)]
     gen = torch.Generator(device="cuda").manual_seed(5)
     targets = [torch.rand((p.pixels, 4), device="cuda", generator=gen) for p in singles]
     target_b = torch.zeros((batch.pixels, 4), device="cuda")
     for (w, h, _), off, t in zip(batch.sizes, batch.offsets, targets):
         target_b[off:off + w * h] = t
-    color_b = torch.cat(colors)
+    color_b = batch.full_colors(torch.cat([s.premultiplied() for s in sc])).to("cuda", torch.float32)
 
     def step(plan, c, t):
         image = plan.forward(c)
         plan.backward(2.0 * (image - t))
 
-    out = {"scenes": count, "size": "256x256", "paths": int(paths.sum()), "lines": int(batch.line_count)}
+    out = {"scenes": count, "size": "256x256", "paths": int(paths.sum()), "lines": int(batch.line_count),
+           "clips_per_scene": 1}
     for name, run in (("loop", lambda: [step(p, c, t) for p, c, t in zip(singles, colors, targets)]),
                       ("batch", lambda: step(batch, color_b, target_b))):
         run()  # warm-up: sizes the coverage buffers

@@ -1814,12 +1814,13 @@ __device__ __forceinline__ bool diff_tile(const tk_args_fine_diff &a, DiffTile &
 struct ClipMask {
     float m, s0, s1, s2, s3;
     uint32_t depth;
+    // Selects on every slot (depth is uniform), so the stack stays in registers.
     __device__ __forceinline__ bool push(float coverage) {
-        if (depth == 0u) s0 = m;
-        else if (depth == 1u) s1 = m;
-        else if (depth == 2u) s2 = m;
-        else if (depth == 3u) s3 = m;
-        else return false;
+        if (depth >= 4u) return false;
+        s0 = depth == 0u ? m : s0;
+        s1 = depth == 1u ? m : s1;
+        s2 = depth == 2u ? m : s2;
+        s3 = depth == 3u ? m : s3;
         ++depth;
         m *= coverage;
         return true;
@@ -1827,21 +1828,35 @@ struct ClipMask {
     __device__ __forceinline__ void pop() {
         if (depth == 0u) return;
         --depth;
-        m = depth == 0u ? s0 : depth == 1u ? s1 : depth == 2u ? s2 : s3;
+        m = depth == 0u ? s0 : m;
+        m = depth == 1u ? s1 : m;
+        m = depth == 2u ? s2 : m;
+        m = depth == 3u ? s3 : m;
     }
 };
 
 // Takes a clip record (BEGIN_SDF_CLIP, END_CLIP) into the mask; false for a
-// record the differentiable form does not take.
-__device__ __forceinline__ bool clip_record(const tk_args_fine_diff &a, const DiffTile &t, const tk_ptcl_record &rec,
-                                            ClipMask &clip) {
+// record the differentiable form does not take.  The forward evaluates the
+// clip's SDF coverage and keeps it in the record's slot of `adj`; the backward
+// (kKept) reads it from there.
+template <bool kKept>
+__device__ __forceinline__ bool clip_record(const tk_args_fine_diff &a, const DiffTile &t, uint32_t pi,
+                                            const tk_ptcl_record &rec, ClipMask &clip) {
     if (rec.tag == TK_PTCL_END_CLIP) {
         clip.pop();
         return true;
     }
     if (rec.tag != TK_PTCL_BEGIN_SDF_CLIP) return false;
-    const FineDraw d = load_fine_draw(a.in.draws, a.cfg.draw_count, rec.color);
-    return clip.push(sdf_coverage_from_draw(a.cfg, a.in.paint, d, (float)t.gx + 0.5f, (float)t.gy + 0.5f));
+    const size_t slot = (size_t)pi * 256u + t.lane;
+    float coverage = 0.0f;
+    if constexpr (kKept) {
+        if (pi < a.adj_records) coverage = a.adj[slot];
+    } else {
+        const FineDraw d = load_fine_draw(a.in.draws, a.cfg.draw_count, rec.color);
+        coverage = sdf_coverage_from_draw(a.cfg, a.in.paint, d, (float)t.gx + 0.5f, (float)t.gy + 0.5f);
+        if (pi < a.adj_records) a.adj[slot] = coverage;
+    }
+    return clip.push(coverage);
 }
 
 // Calls body(pi, rec, mask) for every FILL record of the tile's streams, bottom
@@ -1862,7 +1877,7 @@ __device__ __forceinline__ void for_each_fill(const tk_args_fine_diff &a, const 
             if (rec.tag == TK_PTCL_END) break;
             if (rec.tag == TK_PTCL_FILL && rec.color < cfg.draw_count) {
                 body(pi, rec, clip.m);
-            } else if (!kClips || !clip_record(a, t, rec, clip)) {
+            } else if (!kClips || !clip_record<false>(a, t, pi, rec, clip)) {
                 if (t.lane == 0u && a.status) atomicOr(a.status, 1u);
             }
         }
@@ -1985,7 +2000,8 @@ __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff 
             const float ncov = more ? coverage(next) : 0.0f;
             const float4 nc = more ? color_of(nrec) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (rec.tag == TK_PTCL_FILL && rec.color < draws) record(pi, rec, cov, c, clip.m);
-            else if ((!kClips || !clip_record(a, t, rec, clip)) && lane == 0u && a.status) atomicOr(a.status, 1u);
+            else if ((!kClips || !clip_record<true>(a, t, pi, rec, clip)) && lane == 0u && a.status)
+                atomicOr(a.status, 1u);
             if (!more) break;
             pi = next;
             rec = nrec;
