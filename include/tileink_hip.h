@@ -236,6 +236,25 @@ tk_status tk_canvas_push_stroke(tk_canvas *canvas, const uint8_t *verbs, size_t 
                                 const double *points, size_t point_count, double width,
                                 const tk_brush *brush, const double *transform, uint32_t fill_rule,
                                 double tolerance, uint32_t *draw_id);
+/* Canvas::push_stroke with a full kurbo::Stroke (src/canvas/primitives/paths.rs:24-48):
+ * join and cap styles in kurbo's order (Join {Bevel, Miter, Round}, Cap {Butt,
+ * Square, Round}); a miter join longer than miter_limit x width becomes a bevel.
+ * Dashes are not supported.  tk_canvas_push_stroke is this with round joins and
+ * caps (kurbo::Stroke::new). */
+#define TK_JOIN_BEVEL 0u
+#define TK_JOIN_MITER 1u
+#define TK_JOIN_ROUND 2u
+#define TK_CAP_BUTT 0u
+#define TK_CAP_SQUARE 1u
+#define TK_CAP_ROUND 2u
+typedef struct tk_stroke_style {
+    double width, miter_limit;
+    uint32_t join, start_cap, end_cap, reserved;
+} tk_stroke_style;
+tk_status tk_canvas_push_stroke_styled(tk_canvas *canvas, const uint8_t *verbs, size_t verb_count,
+                                       const double *points, size_t point_count, const tk_stroke_style *style,
+                                       const tk_brush *brush, const double *transform, uint32_t fill_rule,
+                                       double tolerance, uint32_t *draw_id);
 /* Canvas::push_rect (src/canvas/primitives/rectangles.rs:33-43); radii = tl,tr,bl,br. */
 tk_status tk_canvas_push_rect(tk_canvas *canvas, double x0, double y0, double x1, double y1,
                               const float radii[4], const tk_brush *brush, uint32_t *draw_id);

@@ -1,0 +1,55 @@
+"""Copies the reference's SVG rendering fixtures that tileink_amd.svg takes into
+this folder: each test SVG (the input) with its `.native.png` (the image the
+reference's own native renderer produced for it at 300 px width,
+examples/svg_fixture_render.rs:62-77).  Both are data: the tests read them as
+inputs and expected outputs.  Run from the repository root with the reference
+checked out at /root/reference:
+
+    python tests/golden/svg/make_fixtures.py
+"""
+import json
+import os
+import shutil
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(HERE)))
+sys.path.insert(0, ROOT)
+
+from tileink_amd import svg  # noqa: E402
+
+SOURCE = "/root/reference/src/svg/tests"
+FOLDERS = ["shapes/circle", "shapes/ellipse", "shapes/line", "shapes/path", "shapes/polygon", "shapes/polyline",
+           "shapes/rect", "painting/fill-rule", "painting/stroke-linejoin", "painting/stroke-linecap",
+           "painting/stroke-width", "painting/stroke-miterlimit"]
+
+
+def main():
+    taken, skipped = [], []
+    for folder in FOLDERS:
+        src = os.path.join(SOURCE, folder)
+        for name in sorted(os.listdir(src)):
+            if not name.endswith(".svg"):
+                continue
+            gold = name[:-4] + ".native.png"
+            if not os.path.exists(os.path.join(src, gold)):
+                continue
+            with open(os.path.join(src, name)) as f:
+                text = f.read()
+            try:
+                svg.load(text, 300)
+            except svg.SvgUnsupported as e:
+                skipped.append(f"{folder}/{name}: {e}")
+                continue
+            dst = os.path.join(HERE, folder)
+            os.makedirs(dst, exist_ok=True)
+            shutil.copyfile(os.path.join(src, name), os.path.join(dst, name))
+            shutil.copyfile(os.path.join(src, gold), os.path.join(dst, gold))
+            taken.append(f"{folder}/{name}")
+    with open(os.path.join(HERE, "manifest.json"), "w") as f:
+        json.dump({"fixtures": taken, "not_in_subset": skipped}, f, indent=1)
+    print(f"{len(taken)} fixtures, {len(skipped)} outside the subset")
+
+
+if __name__ == "__main__":
+    main()

@@ -64,7 +64,8 @@ MIRRORS = {"tk_scan_config": "ScanConfig", "tk_cumsum_config": "CumsumConfig",
            "tk_scene": "Scene", "tk_limits": "Limits", "tk_frame_stats": "FrameStats", "tk_stage_times": "StageTimes",
            "tk_coarse_config": "CoarseConfig", "tk_coarse_inputs": "CoarseInputs",
            "tk_coarse_outputs": "CoarseOutputs", "tk_fine_config": "FineConfig", "tk_fine_inputs": "FineInputs",
-           "tk_brush": "Brush", "tk_gradient_stop": "GradientStop", "tk_tuning": "Tuning"}
+           "tk_brush": "Brush", "tk_gradient_stop": "GradientStop", "tk_tuning": "Tuning",
+           "tk_stroke_style": "StrokeStyle"}
 
 
 @pytest.mark.parametrize("struct", sorted(k for k, v in MIRRORS.items() if v))

@@ -1,0 +1,39 @@
+"""The reference's SVG test fixtures through the gfx950 pipeline: every fixture
+tileink_amd.svg takes, rendered on the device, against the image the reference's
+own native renderer produced for it (tests/golden/svg), and bit for bit against
+the oracle on a sample."""
+import numpy as np
+import pytest
+
+import oracle
+import tileink_amd as tk
+from tileink_amd import svg
+
+import svg_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def render(ctx, canvas):
+    frame = tk.Frame(ctx, canvas.encode(0))
+    frame.render()
+    assert frame.stats()["overflow"] == 0
+    return frame.image()
+
+
+def test_every_fixture_matches_reference_golden(ctx):
+    worst = []
+    for name in svg_golden.fixtures():
+        image = render(ctx, svg.load(svg_golden.read_svg(name), 300))
+        over8, over32, mean = svg_golden.compare(image, name)
+        worst.append((over8, over32, mean, name))
+        assert over8 <= svg_golden.MAX_SHARE_OVER_8 and over32 <= svg_golden.MAX_SHARE_OVER_32 and \
+            mean <= svg_golden.MAX_MEAN, (name, over8, over32, mean)
+    assert len(worst) >= 140
+
+
+@pytest.mark.parametrize("name", svg_golden.fixtures()[::9])
+def test_device_equals_oracle_on_fixtures(ctx, name):
+    canvas = svg.load(svg_golden.read_svg(name), 300)
+    expected, _ = oracle.render_frame(canvas.arrays(0))
+    assert np.array_equal(render(ctx, canvas), expected)

@@ -21,7 +21,7 @@ from ._abi import (BRUSH_FOUR_CORNER, BRUSH_LINEAR, BRUSH_RADIAL, BRUSH_SOLID, B
 
 __all__ = [
     "Batch", "BezPath", "Canvas", "Color", "LinearGradient", "RadialGradient", "SweepGradient", "FourCornerGradient",
-    "NativeBackend", "NativeContext", "NativeRenderer", "Frame",
+    "NativeBackend", "NativeContext", "NativeRenderer", "Frame", "Join", "Cap", "FillRule",
     "SceneArrays", "TileinkError", "FILL_NONZERO", "FILL_EVEN_ODD",
 ]
 
@@ -29,6 +29,20 @@ __all__ = [
 class FillRule(enum.IntEnum):
     NonZero = FILL_NONZERO
     EvenOdd = FILL_EVEN_ODD
+
+
+class Join(enum.IntEnum):
+    """kurbo::Join."""
+    Bevel = 0
+    Miter = 1
+    Round = 2
+
+
+class Cap(enum.IntEnum):
+    """kurbo::Cap."""
+    Butt = 0
+    Square = 1
+    Round = 2
 
 
 class NativeBackend(enum.Enum):
@@ -274,13 +288,24 @@ class Canvas:
         self._scene = None
         return out.value
 
-    def push_stroke(self, path, width, brush, transform=None, rule=FillRule.NonZero, tolerance=0.25):
+    def push_stroke(self, path, width, brush, transform=None, rule=FillRule.NonZero, tolerance=0.25, join=None,
+                    start_cap=None, end_cap=None, miter_limit=4.0):
+        """Canvas::push_stroke.  join / start_cap / end_cap (Join, Cap) default to round,
+        as kurbo::Stroke::new; any other style goes through tk_canvas_push_stroke_styled."""
         verbs, points = self._path_args(path)
         b, keep = _brush_arg(brush)
         tp, tarr = _affine_arg(transform)
         out = C.c_uint32()
-        check(lib().tk_canvas_push_stroke(self._h, ptr(verbs), len(verbs), ptr(points), len(points), float(width),
-                                          C.byref(b), tp, int(rule), float(tolerance), C.byref(out)))
+        if join is None and start_cap is None and end_cap is None:
+            check(lib().tk_canvas_push_stroke(self._h, ptr(verbs), len(verbs), ptr(points), len(points), float(width),
+                                              C.byref(b), tp, int(rule), float(tolerance), C.byref(out)))
+        else:
+            st = _abi.StrokeStyle(float(width), float(miter_limit), int(Join.Round if join is None else join),
+                                  int(Cap.Round if start_cap is None else start_cap),
+                                  int(Cap.Round if end_cap is None else end_cap), 0)
+            check(lib().tk_canvas_push_stroke_styled(self._h, ptr(verbs), len(verbs), ptr(points), len(points),
+                                                     C.byref(st), C.byref(b), tp, int(rule), float(tolerance),
+                                                     C.byref(out)))
         self._scene = None
         return out.value
 

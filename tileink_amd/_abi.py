@@ -120,6 +120,11 @@ class StageTimes(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("scan_clear", "scan_count", "scan_prefix", "scan_emit", "cumsum", "coarse_count", "coarse_prefix", "coarse_emit", "fine", "total")]
 
 
+class StrokeStyle(C.Structure):
+    _fields_ = [("width", C.c_double), ("miter_limit", C.c_double), ("join", C.c_uint32), ("start_cap", C.c_uint32),
+                ("end_cap", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class Tuning(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("scan_xcd_run", "coarse_xcd_run", "fine_xcd_run", "reserved")]
 
@@ -195,6 +200,8 @@ SIGNATURES = {
     "tk_frame_render": (C.c_int, [_P]),
     "tk_frame_render_tiles": (C.c_int, [_P, _P, _U32]),
     "tk_frame_set_graph": (C.c_int, [_P, C.c_int]),
+    "tk_canvas_push_stroke_styled": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_size_t, C.POINTER(StrokeStyle), _P, _P,
+                                               C.c_uint32, C.c_double, C.POINTER(C.c_uint32)]),
     "tk_frame_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "tk_frame_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "tk_frame_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(Tuning)]),

@@ -429,7 +429,14 @@ bool segment_cross(Pt a0, Pt a1, Pt b0, Pt b1, Pt *x) {
     return true;
 }
 
-// Stroke outline of a polyline with round joins and caps.
+// Join and cap styles of a stroke (kurbo::Stroke: Join {Bevel, Miter, Round},
+// Cap {Butt, Square, Round}, miter_limit).
+struct StrokeStyle {
+    uint32_t join = TK_JOIN_ROUND, start_cap = TK_CAP_ROUND, end_cap = TK_CAP_ROUND;
+    double miter_limit = 4.0;
+};
+
+// Stroke outline of a polyline.
 //
 // The stroked region is the union of one rectangle per segment, a disc sector
 // at every outer joint and a half disc at each open end.  Tracing the sum of
@@ -440,9 +447,12 @@ bool segment_cross(Pt a0, Pt a1, Pt b0, Pt b1, Pt *x) {
 // Two reductions keep the line count down: an inner detour is replaced by the
 // crossing of the two inner edges when that crossing lies on both (this only
 // removes a loop lying inside both rectangles), and an outer arc whose miter
-// is within the join tolerance of it is replaced by the miter point.
+// is within the join tolerance of it is replaced by the miter point.  Miter and
+// bevel joins replace the outer disc sector by the miter triangle, while the
+// miter length stays within miter_limit times the stroke width (1 / cos of half
+// the turn, as SVG and kurbo measure it), or else by nothing (the bevel).
 void stroke_side(std::vector<Pt> &out, const std::vector<Pt> &p, const std::vector<Pt> &n, const std::vector<Pt> &d,
-                 bool closed, double side, double h, double join_tol) {
+                 bool closed, double side, double h, double join_tol, const StrokeStyle &style) {
     size_t m = n.size();
     auto vert = [&](size_t i) { return p[i % p.size()]; };
     auto off = [&](size_t i, size_t seg) {
@@ -470,6 +480,15 @@ void stroke_side(std::vector<Pt> &out, const std::vector<Pt> &p, const std::vect
         if (ang == 0.0) {
             pts.push_back(e);
             cur = e;
+        } else if (outer && style.join != TK_JOIN_ROUND) {
+            double sweep = std::fabs(ang);
+            double mx = side * (n[i].x + n[j].x), my = side * (n[i].y + n[j].y);
+            double den = 1.0 + (n[i].x * n[j].x + n[i].y * n[j].y);
+            pts.push_back(e);
+            if (style.join == TK_JOIN_MITER && den > 0.0 && 1.0 / std::cos(0.5 * sweep) <= style.miter_limit)
+                pts.push_back(Pt{c.x + h * mx / den, c.y + h * my / den});
+            pts.push_back(sj);
+            cur = sj;
         } else if (outer) {
             double sweep = std::fabs(ang);
             double mx = side * (n[i].x + n[j].x), my = side * (n[i].y + n[j].y);
@@ -503,19 +522,27 @@ void stroke_side(std::vector<Pt> &out, const std::vector<Pt> &p, const std::vect
     out.insert(out.end(), pts.begin(), pts.end());
 }
 
-void stroke_polyline(OutlineWriter &w, std::vector<Pt> pts, bool closed, double h, double tol) {
+void stroke_polyline(OutlineWriter &w, std::vector<Pt> pts, bool closed, double h, double tol,
+                     const StrokeStyle &style) {
     std::vector<Pt> p;
     for (const Pt &q : pts)
         if (p.empty() || std::hypot(q.x - p.back().x, q.y - p.back().y) > 1e-9) p.push_back(q);
     if (closed && p.size() > 1 && std::hypot(p.front().x - p.back().x, p.front().y - p.back().y) <= 1e-9)
         p.pop_back();
     if (p.size() < 2) {
-        if (p.size() == 1 && !closed) {  // a dot: round caps make a disc
+        if (p.size() == 1 && !closed && style.start_cap == TK_CAP_ROUND) {  // a dot: round caps make a disc
             Pt c = p[0];
             std::vector<Pt> ring{Pt{c.x + h, c.y}};
             arc_points(ring, c, h, 0.0, -2.0 * M_PI, tol);
             w.move(ring[0]);
             for (size_t i = 1; i < ring.size(); ++i) w.line(ring[i]);
+            w.close();
+        } else if (p.size() == 1 && !closed && style.start_cap == TK_CAP_SQUARE) {  // square caps: a square
+            Pt c = p[0];
+            w.move(Pt{c.x - h, c.y - h});
+            w.line(Pt{c.x + h, c.y - h});
+            w.line(Pt{c.x + h, c.y + h});
+            w.line(Pt{c.x - h, c.y + h});
             w.close();
         }
         return;
@@ -529,8 +556,8 @@ void stroke_polyline(OutlineWriter &w, std::vector<Pt> pts, bool closed, double 
         n[i] = Pt{-d[i].y, d[i].x};
     }
     std::vector<Pt> left, right;
-    stroke_side(left, p, n, d, closed, 1.0, h, tol);
-    stroke_side(right, p, n, d, closed, -1.0, h, tol);
+    stroke_side(left, p, n, d, closed, 1.0, h, tol, style);
+    stroke_side(right, p, n, d, closed, -1.0, h, tol, style);
     w.move(left[0]);
     for (size_t i = 1; i < left.size(); ++i) w.line(left[i]);
     if (closed) {
@@ -540,26 +567,39 @@ void stroke_polyline(OutlineWriter &w, std::vector<Pt> pts, bool closed, double 
         w.close();
         return;
     }
-    // End cap: from +n to -n through the direction of travel.
+    // End cap: from +n to -n through the direction of travel (butt: straight across).
     Pt pe = p.back();
     std::vector<Pt> cap;
-    arc_points(cap, pe, h, std::atan2(n[m - 1].y, n[m - 1].x), -M_PI, tol);
+    if (style.end_cap == TK_CAP_ROUND) {
+        arc_points(cap, pe, h, std::atan2(n[m - 1].y, n[m - 1].x), -M_PI, tol);
+    } else if (style.end_cap == TK_CAP_SQUARE) {
+        const Pt dd = d[m - 1], nn = n[m - 1];
+        cap.push_back(Pt{pe.x + h * (dd.x + nn.x), pe.y + h * (dd.y + nn.y)});
+        cap.push_back(Pt{pe.x + h * (dd.x - nn.x), pe.y + h * (dd.y - nn.y)});
+    }
     for (const Pt &q : cap) w.line(q);
     for (size_t i = right.size(); i-- > 0;) w.line(right[i]);
     // Start cap: from -n back to +n through the reverse direction.
     Pt ps = p.front();
     cap.clear();
-    arc_points(cap, ps, h, std::atan2(-n[0].y, -n[0].x), -M_PI, tol);
+    if (style.start_cap == TK_CAP_ROUND) {
+        arc_points(cap, ps, h, std::atan2(-n[0].y, -n[0].x), -M_PI, tol);
+    } else if (style.start_cap == TK_CAP_SQUARE) {
+        const Pt dd = d[0], nn = n[0];
+        cap.push_back(Pt{ps.x - h * (dd.x + nn.x), ps.y - h * (dd.y + nn.y)});
+        cap.push_back(Pt{ps.x - h * (dd.x - nn.x), ps.y - h * (dd.y - nn.y)});
+    }
     for (const Pt &q : cap) w.line(q);
     w.close();
 }
 
-BezPath stroke_outline(const BezPath &path, double width, double tol) {
+BezPath stroke_outline(const BezPath &path, double width, double tol, const StrokeStyle &style = StrokeStyle{}) {
     OutlineWriter w;
     double h = 0.5 * width;
     if (!(h > 0.0)) return w.path;
     // The centreline gets three quarters of the tolerance, joins and caps the rest.
-    for (const Polyline &c : flatten_contours(path, 0.75 * tol)) stroke_polyline(w, c.pts, c.closed, h, 0.25 * tol);
+    for (const Polyline &c : flatten_contours(path, 0.75 * tol))
+        stroke_polyline(w, c.pts, c.closed, h, 0.25 * tol, style);
     return w.path;
 }
 
@@ -1278,7 +1318,8 @@ uint32_t tk_canvas_line_count(const tk_canvas *canvas) { return canvas ? uint32_
 
 static tk_status push_path_common(tk_canvas *c, const uint8_t *verbs, size_t verb_count, const double *points,
                                   size_t point_count, double stroke_width, bool stroke, const tk_brush *brush,
-                                  const double *transform, uint32_t fill_rule, double tolerance, uint32_t *draw_id) {
+                                  const double *transform, uint32_t fill_rule, double tolerance, uint32_t *draw_id,
+                                  const StrokeStyle &style = StrokeStyle{}) {
     if (!c) return fail(TK_ERR_INVALID, "null canvas");
     if (fill_rule > 1u) return fail(TK_ERR_INVALID, "fill rule must be 0 (nonzero) or 1 (even-odd)");
     if (!(tolerance > 0.0)) return fail(TK_ERR_INVALID, "tolerance must be positive");
@@ -1287,7 +1328,7 @@ static tk_status push_path_common(tk_canvas *c, const uint8_t *verbs, size_t ver
         return fail(TK_ERR_INVALID, "malformed path: verbs and points disagree or non-finite");
     if (stroke) {
         if (!(stroke_width >= 0.0) || !std::isfinite(stroke_width)) return fail(TK_ERR_INVALID, "invalid stroke width");
-        path = stroke_outline(path, stroke_width, tolerance);
+        path = stroke_outline(path, stroke_width, tolerance, style);
     }
     double m[6];
     device_matrix(c, transform, m);
@@ -1311,6 +1352,24 @@ tk_status tk_canvas_push_stroke(tk_canvas *canvas, const uint8_t *verbs, size_t 
                                 uint32_t fill_rule, double tolerance, uint32_t *draw_id) {
     return push_path_common(canvas, verbs, verb_count, points, point_count, width, true, brush, transform, fill_rule,
                             tolerance, draw_id);
+}
+
+tk_status tk_canvas_push_stroke_styled(tk_canvas *canvas, const uint8_t *verbs, size_t verb_count, const double *points,
+                                       size_t point_count, const tk_stroke_style *style, const tk_brush *brush,
+                                       const double *transform, uint32_t fill_rule, double tolerance,
+                                       uint32_t *draw_id) {
+    if (!style) return fail(TK_ERR_INVALID, "null stroke style");
+    if (style->join > TK_JOIN_ROUND || style->start_cap > TK_CAP_ROUND || style->end_cap > TK_CAP_ROUND)
+        return fail(TK_ERR_INVALID, "unknown join or cap");
+    if (!(style->miter_limit >= 1.0) && style->join == TK_JOIN_MITER)
+        return fail(TK_ERR_INVALID, "miter limit must be at least 1");
+    StrokeStyle st;
+    st.join = style->join;
+    st.start_cap = style->start_cap;
+    st.end_cap = style->end_cap;
+    st.miter_limit = style->miter_limit;
+    return push_path_common(canvas, verbs, verb_count, points, point_count, style->width, true, brush, transform,
+                            fill_rule, tolerance, draw_id, st);
 }
 
 tk_status tk_canvas_push_rect(tk_canvas *canvas, double x0, double y0, double x1, double y1, const float radii[4],

@@ -1,0 +1,519 @@
+"""SVG import: a subset of SVG 1.1 recorded on a Canvas.
+
+Mirrors what the reference's SVG front end does for plain vector content
+(src/svg/geometry.rs, src/svg/paint.rs via usvg, loaded at a target width as
+examples/common/svg.rs:9-52 does): shapes and paths become BezPaths in user
+units, filled (fill, fill-opacity, fill-rule) and then stroked (stroke,
+stroke-opacity, stroke-width, stroke-linejoin, stroke-linecap,
+stroke-miterlimit), each under its accumulated transform times the viewBox
+mapping.  Elements this subset does not take (text, images, gradients,
+patterns, clipping, masks, filters, markers, dashes, group opacity, use)
+raise SvgUnsupported instead of being dropped silently.
+"""
+import math
+import re
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+import tileink_amd as tk
+
+SVG_NS = "{http://www.w3.org/2000/svg}"
+TOLERANCE = 0.1  # SvgOptions::default().tolerance (src/svg.rs:32-40)
+FONT_SIZE = 12.0  # usvg's default font-size, the base of em lengths
+
+COLORS = {
+    "black": (0, 0, 0), "white": (255, 255, 255), "red": (255, 0, 0), "green": (0, 128, 0), "blue": (0, 0, 255),
+    "yellow": (255, 255, 0), "cyan": (0, 255, 255), "aqua": (0, 255, 255), "magenta": (255, 0, 255),
+    "fuchsia": (255, 0, 255), "gray": (128, 128, 128), "grey": (128, 128, 128), "silver": (192, 192, 192),
+    "maroon": (128, 0, 0), "olive": (128, 128, 0), "lime": (0, 255, 0), "teal": (0, 128, 128), "navy": (0, 0, 128),
+    "purple": (128, 0, 128), "orange": (255, 165, 0), "pink": (255, 192, 203), "brown": (165, 42, 42),
+    "seagreen": (46, 139, 87), "lightblue": (173, 216, 230), "darkgreen": (0, 100, 0),
+}
+UNSUPPORTED = {"text", "image", "use", "linearGradient", "radialGradient", "pattern", "clipPath", "mask", "filter",
+               "marker", "symbol", "switch", "foreignObject", "style", "a", "tspan", "textPath"}
+INHERITED = ("fill", "fill-opacity", "fill-rule", "stroke", "stroke-opacity", "stroke-width", "stroke-linejoin",
+             "stroke-linecap", "stroke-miterlimit", "visibility", "font-size")
+
+
+class SvgUnsupported(ValueError):
+    pass
+
+
+# --- numbers, lengths, colours, transforms ------------------------------------------
+
+_NUM = re.compile(r"[+-]?(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?")
+
+
+def _length(text, ref=None, font_size=FONT_SIZE):
+    m = re.fullmatch(r"\s*([+-]?(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?)\s*([a-z%]*)\s*", text or "")
+    if not m:
+        raise SvgUnsupported(f"length {text!r}")
+    v, unit = float(m.group(1)), m.group(2)
+    scale = {"": 1.0, "px": 1.0, "in": 96.0, "cm": 96.0 / 2.54, "mm": 96.0 / 25.4, "pt": 4.0 / 3.0, "pc": 16.0,
+             "em": font_size, "ex": font_size / 2.0}
+    if unit == "%":
+        if ref is None:
+            raise SvgUnsupported("percentage without a reference length")
+        return v / 100.0 * ref
+    if unit not in scale:
+        raise SvgUnsupported(f"length unit {unit!r}")
+    return v * scale[unit]
+
+
+def _number(text, default):
+    """A plain number attribute; an absent or invalid value is the default."""
+    try:
+        return float(text) if text is not None else default
+    except ValueError:
+        return default
+
+
+def _color(text):
+    """(r, g, b) in 0-255, or None for 'none'."""
+    t = text.strip().lower()
+    if t == "none":
+        return None
+    if t in COLORS:
+        return COLORS[t]
+    if t.startswith("#"):
+        h = t[1:]
+        if len(h) == 3:
+            return tuple(int(c * 2, 16) for c in h)
+        if len(h) == 6:
+            return tuple(int(h[i:i + 2], 16) for i in (0, 2, 4))
+    m = re.fullmatch(r"rgb\(\s*([^,]+),\s*([^,]+),\s*([^)]+)\)", t)
+    if m:
+        out = []
+        for c in m.groups():
+            c = c.strip()
+            out.append(round(float(c[:-1]) * 2.55) if c.endswith("%") else int(float(c)))
+        return tuple(max(0, min(255, v)) for v in out)
+    raise SvgUnsupported(f"colour {text!r}")
+
+
+def _mul(a, b):
+    """a after b, both (a, b, c, d, e, f) with x' = a x + c y + e, y' = b x + d y + f."""
+    return (a[0] * b[0] + a[2] * b[1], a[1] * b[0] + a[3] * b[1], a[0] * b[2] + a[2] * b[3],
+            a[1] * b[2] + a[3] * b[3], a[0] * b[4] + a[2] * b[5] + a[4], a[1] * b[4] + a[3] * b[5] + a[5])
+
+
+IDENTITY = (1.0, 0.0, 0.0, 1.0, 0.0, 0.0)
+
+
+def _transform(text):
+    t = IDENTITY
+    for name, args in re.findall(r"([a-zA-Z]+)\s*\(([^)]*)\)", text or ""):
+        v = [float(x) for x in _NUM.findall(args)]
+        if name == "matrix" and len(v) == 6:
+            m = tuple(v)
+        elif name == "translate":
+            m = (1.0, 0.0, 0.0, 1.0, v[0], v[1] if len(v) > 1 else 0.0)
+        elif name == "scale":
+            m = (v[0], 0.0, 0.0, v[1] if len(v) > 1 else v[0], 0.0, 0.0)
+        elif name == "rotate":
+            a = math.radians(v[0])
+            r = (math.cos(a), math.sin(a), -math.sin(a), math.cos(a), 0.0, 0.0)
+            if len(v) == 3:
+                r = _mul((1.0, 0.0, 0.0, 1.0, v[1], v[2]), _mul(r, (1.0, 0.0, 0.0, 1.0, -v[1], -v[2])))
+            m = r
+        elif name == "skewX":
+            m = (1.0, 0.0, math.tan(math.radians(v[0])), 1.0, 0.0, 0.0)
+        elif name == "skewY":
+            m = (1.0, math.tan(math.radians(v[0])), 0.0, 1.0, 0.0, 0.0)
+        else:
+            raise SvgUnsupported(f"transform {name}")
+        t = _mul(t, m)
+    return t
+
+
+# --- path data ------------------------------------------------------------------------
+
+
+def _arc_to_cubics(p0, rx, ry, phi, large, sweep, p1):
+    """Endpoint arc (SVG 1.1 F.6.5) as cubic Beziers, at most 90 degrees each."""
+    if rx == 0 or ry == 0:
+        return [("L", p1)]
+    rx, ry = abs(rx), abs(ry)
+    cp, sp = math.cos(phi), math.sin(phi)
+    dx, dy = (p0[0] - p1[0]) / 2.0, (p0[1] - p1[1]) / 2.0
+    x1, y1 = cp * dx + sp * dy, -sp * dx + cp * dy
+    lam = x1 * x1 / (rx * rx) + y1 * y1 / (ry * ry)
+    if lam > 1:
+        rx, ry = rx * math.sqrt(lam), ry * math.sqrt(lam)
+    num = rx * rx * ry * ry - rx * rx * y1 * y1 - ry * ry * x1 * x1
+    den = rx * rx * y1 * y1 + ry * ry * x1 * x1
+    co = math.sqrt(max(num, 0.0) / den) if den else 0.0
+    if large == sweep:
+        co = -co
+    cx1, cy1 = co * rx * y1 / ry, -co * ry * x1 / rx
+    cx = cp * cx1 - sp * cy1 + (p0[0] + p1[0]) / 2.0
+    cy = sp * cx1 + cp * cy1 + (p0[1] + p1[1]) / 2.0
+
+    def angle(ux, uy, vx, vy):
+        a = math.atan2(ux * vy - uy * vx, ux * vx + uy * vy)
+        return a
+
+    t1 = angle(1.0, 0.0, (x1 - cx1) / rx, (y1 - cy1) / ry)
+    dt = angle((x1 - cx1) / rx, (y1 - cy1) / ry, (-x1 - cx1) / rx, (-y1 - cy1) / ry)
+    if not sweep and dt > 0:
+        dt -= 2 * math.pi
+    elif sweep and dt < 0:
+        dt += 2 * math.pi
+    n = max(1, int(math.ceil(abs(dt) / (math.pi / 2) - 1e-9)))
+    step = dt / n
+    k = 4.0 / 3.0 * math.tan(step / 4.0)
+    out = []
+
+    def point(t):
+        x, y = rx * math.cos(t), ry * math.sin(t)
+        return (cp * x - sp * y + cx, sp * x + cp * y + cy)
+
+    def deriv(t):
+        x, y = -rx * math.sin(t), ry * math.cos(t)
+        return (cp * x - sp * y, sp * x + cp * y)
+
+    t = t1
+    for i in range(n):
+        a, b = t, t + step
+        pa, pb = point(a), point(b)
+        da, db = deriv(a), deriv(b)
+        c1 = (pa[0] + k * da[0], pa[1] + k * da[1])
+        c2 = (pb[0] - k * db[0], pb[1] - k * db[1])
+        out.append(("C", c1, c2, p1 if i == n - 1 else pb))
+        t = b
+    return out
+
+
+_NUMBER_AT = re.compile(r"[+-]?(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?")
+_ARGS = {"M": 2, "L": 2, "H": 1, "V": 1, "C": 6, "S": 4, "Q": 4, "T": 2, "A": 7, "Z": 0}
+
+
+class _Scanner:
+    """SVG path data tokens: command letters, numbers, and the single-character
+    flags of arcs (which need no separator); stops at the first invalid character."""
+
+    def __init__(self, d):
+        self.d, self.i = d or "", 0
+
+    def skip(self):
+        while self.i < len(self.d) and self.d[self.i] in " \t\r\n,":
+            self.i += 1
+
+    def command(self):
+        self.skip()
+        if self.i < len(self.d) and self.d[self.i] in "MmLlHhVvCcSsQqTtAaZz":
+            self.i += 1
+            return self.d[self.i - 1]
+        return None
+
+    def at_number(self):
+        self.skip()
+        return bool(_NUMBER_AT.match(self.d, self.i))
+
+    def number(self):
+        self.skip()
+        m = _NUMBER_AT.match(self.d, self.i)
+        if not m:
+            raise ValueError("number expected")
+        self.i = m.end()
+        return float(m.group(0))
+
+    def flag(self):
+        self.skip()
+        if self.i < len(self.d) and self.d[self.i] in "01":
+            self.i += 1
+            return self.d[self.i - 1] == "1"
+        raise ValueError("flag expected")
+
+
+def parse_path(d):
+    """SVG path data as a BezPath (absolute coordinates).  Stops at the first error,
+    keeping the segments read before it, as SVG's error handling does; a command
+    after ClosePath starts its subpath at the closed subpath's start."""
+    sc = _Scanner(d)
+    path = tk.BezPath()
+    cur = start = (0.0, 0.0)
+    last_c2 = last_q = None
+    cmd, have_move, need_move = None, False, False
+    while True:
+        c = sc.command()
+        if c is None:
+            if cmd is None or cmd.upper() == "Z" or not sc.at_number():
+                break
+            c = cmd  # implicit repetition
+        elif not have_move and c not in "Mm":
+            break  # a path must start with MoveTo
+        cmd = c
+        rel, C = c.islower(), c.upper()
+        try:
+            if C == "A":
+                rx, ry, phi = sc.number(), sc.number(), sc.number()
+                large, sweep = sc.flag(), sc.flag()
+                args = [rx, ry, phi, large, sweep, sc.number(), sc.number()]
+            else:
+                args = [sc.number() for _ in range(_ARGS[C])]
+        except ValueError:
+            break
+        ox, oy = cur if rel else (0.0, 0.0)
+        if need_move and C != "M":
+            path.move_to(*start)
+            need_move = False
+        if C == "M":
+            cur = start = (ox + args[0], oy + args[1])
+            path.move_to(*cur)
+            have_move, need_move = True, False
+            cmd = "l" if rel else "L"  # further pairs are implicit LineTo
+            last_c2 = last_q = None
+        elif C == "L":
+            cur = (ox + args[0], oy + args[1])
+            path.line_to(*cur)
+            last_c2 = last_q = None
+        elif C == "H":
+            cur = ((cur[0] if rel else 0.0) + args[0], cur[1])
+            path.line_to(*cur)
+            last_c2 = last_q = None
+        elif C == "V":
+            cur = (cur[0], (cur[1] if rel else 0.0) + args[0])
+            path.line_to(*cur)
+            last_c2 = last_q = None
+        elif C in "CS":
+            if C == "C":
+                c1 = (ox + args[0], oy + args[1])
+                x2, y2, x, y = args[2:]
+            else:
+                c1 = (2 * cur[0] - last_c2[0], 2 * cur[1] - last_c2[1]) if last_c2 else cur
+                x2, y2, x, y = args
+            c2, end = (ox + x2, oy + y2), (ox + x, oy + y)
+            path.curve_to(*c1, *c2, *end)
+            cur, last_c2, last_q = end, c2, None
+        elif C in "QT":
+            if C == "Q":
+                q = (ox + args[0], oy + args[1])
+                x, y = args[2:]
+            else:
+                q = (2 * cur[0] - last_q[0], 2 * cur[1] - last_q[1]) if last_q else cur
+                x, y = args
+            end = (ox + x, oy + y)
+            path.quad_to(*q, *end)
+            cur, last_q, last_c2 = end, q, None
+        elif C == "A":
+            rx, ry, phi, large, sweep, x, y = args
+            end = (ox + x, oy + y)
+            if end != cur:
+                for seg in _arc_to_cubics(cur, rx, ry, math.radians(phi), large, sweep, end):
+                    if seg[0] == "L":
+                        path.line_to(*seg[1])
+                    else:
+                        path.curve_to(*seg[1], *seg[2], *seg[3])
+            cur, last_c2, last_q = end, None, None
+        elif C == "Z":
+            path.close_path()
+            cur = start
+            last_c2 = last_q = None
+            need_move = True
+    return path
+
+
+def _ellipse_path(cx, cy, rx, ry):
+    k = 4.0 / 3.0 * (math.sqrt(2.0) - 1.0)
+    p = tk.BezPath().move_to(cx + rx, cy)
+    p.curve_to(cx + rx, cy + k * ry, cx + k * rx, cy + ry, cx, cy + ry)
+    p.curve_to(cx - k * rx, cy + ry, cx - rx, cy + k * ry, cx - rx, cy)
+    p.curve_to(cx - rx, cy - k * ry, cx - k * rx, cy - ry, cx, cy - ry)
+    p.curve_to(cx + k * rx, cy - ry, cx + rx, cy - k * ry, cx + rx, cy)
+    return p.close_path()
+
+
+def _rect_path(x, y, w, h, rx, ry):
+    if rx <= 0 or ry <= 0:
+        return tk.BezPath().move_to(x, y).line_to(x + w, y).line_to(x + w, y + h).line_to(x, y + h).close_path()
+    k = 4.0 / 3.0 * (math.sqrt(2.0) - 1.0)
+    p = tk.BezPath().move_to(x + rx, y)
+    p.line_to(x + w - rx, y).curve_to(x + w - rx + k * rx, y, x + w, y + ry - k * ry, x + w, y + ry)
+    p.line_to(x + w, y + h - ry).curve_to(x + w, y + h - ry + k * ry, x + w - rx + k * rx, y + h, x + w - rx, y + h)
+    p.line_to(x + rx, y + h).curve_to(x + rx - k * rx, y + h, x, y + h - ry + k * ry, x, y + h - ry)
+    p.line_to(x, y + ry).curve_to(x, y + ry - k * ry, x + rx - k * rx, y, x + rx, y)
+    return p.close_path()
+
+
+# --- the document ---------------------------------------------------------------------
+
+
+def _attrs(el, parent):
+    a = {k: v for k, v in parent.items() if k in INHERITED}
+    for k, v in el.attrib.items():
+        a[k] = v
+    for decl in (el.get("style") or "").split(";"):
+        if ":" in decl:
+            k, v = decl.split(":", 1)
+            a[k.strip()] = v.strip()
+    for k in ("opacity", "stroke-dasharray", "clip-path", "mask", "filter", "marker-start", "marker-mid",
+              "marker-end"):
+        v = a.get(k)
+        if v is not None and v.strip() not in ("none", "1", ""):
+            raise SvgUnsupported(f"{k}={v}")
+    return a
+
+
+def _local(tag):
+    return tag[len(SVG_NS):] if tag.startswith(SVG_NS) else tag
+
+
+class Document:
+    """A parsed SVG: size (width, height) and the draw list (path, transform,
+    fill (rgba, rule) or None, stroke (rgba, style) or None) in painter order."""
+
+    def __init__(self, text):
+        root = ET.fromstring(text)
+        if _local(root.tag) != "svg":
+            raise SvgUnsupported("not an svg document")
+        vb = [float(v) for v in _NUM.findall(root.get("viewBox") or "")]
+        w_attr, h_attr = root.get("width"), root.get("height")
+        if len(vb) == 4 and vb[2] > 0 and vb[3] > 0:
+            vw, vh = vb[2], vb[3]
+        else:
+            vb = None
+        self.width = _length(w_attr) if w_attr and not w_attr.strip().endswith("%") else (vb[2] if vb else 100.0)
+        self.height = _length(h_attr) if h_attr and not h_attr.strip().endswith("%") else (vb[3] if vb else 100.0)
+        view = IDENTITY
+        if vb:
+            sx, sy = self.width / vb[2], self.height / vb[3]
+            s = min(sx, sy)  # preserveAspectRatio xMidYMid meet
+            par = (root.get("preserveAspectRatio") or "xMidYMid").split()
+            if par and par[0] == "none":
+                view = (sx, 0.0, 0.0, sy, -vb[0] * sx, -vb[1] * sy)
+            else:
+                tx = (self.width - vb[2] * s) / 2.0 - vb[0] * s
+                ty = (self.height - vb[3] * s) / 2.0 - vb[1] * s
+                view = (s, 0.0, 0.0, s, tx, ty)
+        self.view_w, self.view_h = (vb[2], vb[3]) if vb else (self.width, self.height)
+        self.draws = []
+        self._walk(root, {}, view)
+
+    def _walk(self, el, parent, transform):
+        tag = _local(el.tag)
+        if tag in ("title", "desc", "metadata", "defs"):
+            return
+        if tag in UNSUPPORTED:
+            raise SvgUnsupported(tag)
+        a = _attrs(el, parent)
+        if a.get("display") == "none":
+            return
+        t = _mul(transform, _transform(a.get("transform"))) if "transform" in el.attrib else transform
+        if tag in ("svg", "g"):
+            if tag == "svg" and self.draws is not None and getattr(self, "_entered", False):
+                raise SvgUnsupported("nested svg")
+            self._entered = True
+            for child in el:
+                self._walk(child, a, t)
+            return
+        path = self._shape(tag, a)
+        if path is None or a.get("visibility") in ("hidden", "collapse"):
+            return
+        fill = _color(a.get("fill", "black"))
+        fill_op = _number(a.get("fill-opacity"), 1.0)
+        stroke = _color(a.get("stroke", "none"))
+        stroke_op = _number(a.get("stroke-opacity"), 1.0)
+        if tag in ("line", "polyline"):
+            fill = fill if tag == "polyline" else None
+        rule = tk.FillRule.EvenOdd if a.get("fill-rule") == "evenodd" else tk.FillRule.NonZero
+        if fill is not None and fill_op > 0:
+            self.draws.append((path, t, (fill, min(fill_op, 1.0), rule), None))
+        if stroke is not None and stroke_op > 0:
+            width = _length(a.get("stroke-width", "1"), math.hypot(self.view_w, self.view_h) / math.sqrt(2))
+            if width > 0:
+                jv = a.get("stroke-linejoin", "miter")
+                if jv in ("miter-clip", "arcs"):
+                    raise SvgUnsupported(f"stroke-linejoin {jv}")
+                # an invalid value is ignored: the default (miter joins, butt caps)
+                join = {"round": tk.Join.Round, "bevel": tk.Join.Bevel}.get(jv, tk.Join.Miter)
+                cap = {"round": tk.Cap.Round, "square": tk.Cap.Square}.get(a.get("stroke-linecap", "butt"), tk.Cap.Butt)
+                limit = max(_number(a.get("stroke-miterlimit"), 4.0), 1.0)
+                self.draws.append((path, t, None, (stroke, min(stroke_op, 1.0), width, join, cap, limit)))
+
+    def _shape(self, tag, a):
+        fs = _length(a["font-size"]) if "font-size" in a else FONT_SIZE
+        L = lambda k, ref, default="0": _length(a.get(k, default), ref, fs)
+        W, H = self.view_w, self.view_h
+        if tag == "path":
+            return parse_path(a.get("d"))
+        if tag == "rect":
+            x, y, w, h = L("x", W), L("y", H), L("width", W), L("height", H)
+            if w <= 0 or h <= 0:
+                return None
+            rx, ry = a.get("rx"), a.get("ry")
+            rx = L("rx", W) if rx not in (None, "auto") else None
+            ry = L("ry", H) if ry not in (None, "auto") else None
+            rx = None if rx is not None and rx < 0 else rx  # a negative radius is taken as auto
+            ry = None if ry is not None and ry < 0 else ry
+            if rx is None and ry is None:
+                rx = ry = 0.0
+            elif rx is None:
+                rx = ry
+            elif ry is None:
+                ry = rx
+            return _rect_path(x, y, w, h, min(max(rx, 0.0), w / 2), min(max(ry, 0.0), h / 2))
+        if tag == "circle":
+            r = L("r", math.hypot(W, H) / math.sqrt(2))
+            return _ellipse_path(L("cx", W), L("cy", H), r, r) if r > 0 else None
+        if tag == "ellipse":
+            rx, ry = a.get("rx"), a.get("ry")
+            rx = L("rx", W) if rx not in (None, "auto") else None
+            ry = L("ry", H) if ry not in (None, "auto") else None
+            if rx is None and ry is None:
+                return None
+            rx = None if rx is not None and rx < 0 else rx  # a negative radius is taken as auto
+            ry = None if ry is not None and ry < 0 else ry
+            if rx is None and ry is None:
+                return None
+            rx, ry = (ry if rx is None else rx), (rx if ry is None else ry)
+            return _ellipse_path(L("cx", W), L("cy", H), rx, ry) if rx > 0 and ry > 0 else None
+        if tag == "line":
+            return tk.BezPath().move_to(L("x1", W), L("y1", H)).line_to(L("x2", W), L("y2", H))
+        if tag in ("polyline", "polygon"):
+            v = [float(x) for x in _NUM.findall(a.get("points", ""))]
+            if len(v) < 4:
+                return None
+            p = tk.BezPath().move_to(v[0], v[1])
+            for i in range(2, len(v) - 1, 2):
+                p.line_to(v[i], v[i + 1])
+            return p.close_path() if tag == "polygon" else p
+        raise SvgUnsupported(tag)
+
+
+def render_size(doc, target_width):
+    """The canvas size the reference's fixture loader uses for a target width
+    (examples/common/svg.rs:30-40: the SVG size scaled to the width, height rounded up)."""
+    w, h = doc.width, doc.height
+    iw, ih = max(1, int(math.ceil(w))), max(1, int(math.ceil(h)))
+    nh = int(math.ceil(target_width * ih / iw))
+    return target_width, max(nh, 1), target_width / w, nh / h
+
+
+def load(text, target_width=300):
+    """Records an SVG document on a new Canvas at the target width; returns the canvas."""
+    doc = Document(text)
+    width, height, sx, sy = render_size(doc, target_width)
+    canvas = tk.Canvas(width, height)
+    scale = (sx, 0.0, 0.0, sy, 0.0, 0.0)
+    for path, t, fill, stroke in doc.draws:
+        m = _mul(scale, t)
+        if fill is not None:
+            (r, g, b), op, rule = fill
+            canvas.push_path(path, tk.Color(r / 255, g / 255, b / 255, op), m, rule, TOLERANCE)
+        if stroke is not None:
+            (r, g, b), op, width_, join, cap, limit = stroke
+            canvas.push_stroke(path, width_, tk.Color(r / 255, g / 255, b / 255, op), m, tk.FillRule.NonZero,
+                               TOLERANCE, join=join, start_cap=cap, end_cap=cap, miter_limit=limit)
+    return canvas
+
+
+def straight_rgba8(premultiplied):
+    """Premultiplied RGBA8 words (H, W) as straight-alpha RGBA bytes (H, W, 4), with
+    the rounding of the reference's PNG writer (src/shared/image.rs:198-214)."""
+    px = np.ascontiguousarray(premultiplied, np.uint32).view(np.uint8).reshape(premultiplied.shape + (4,)).astype(
+        np.uint32)
+    a = px[..., 3:4]
+    rgb = np.where(a == 0, 0, np.minimum((px[..., :3] * 255 + a // 2) // np.maximum(a, 1), 255))
+    return np.concatenate([rgb, a], -1).astype(np.uint8)
