@@ -230,3 +230,52 @@ def test_invalid_arguments_are_rejected():
         c.push_path(tk.BezPath().move_to(0, 0).line_to(float("nan"), 1), tk.Color(1, 0, 0))
     with pytest.raises(tk.TileinkError):
         tk.Canvas(0, 10)
+
+
+def _covered_area(canvas):
+    import oracle
+    image, _ = oracle.render_frame(canvas.arrays(0))
+    return (image.view(np.uint8).reshape(image.shape + (4,))[..., 3].astype(np.float64) / 255.0).sum()
+
+
+def test_stroke_joins_and_caps_add_their_closed_form_areas():
+    # A V of two 72.1 px segments, width 10 (h = 5), turning by 112.6 degrees at the joint.
+    def area(join, cap):
+        c = tk.Canvas(120, 100)
+        c.push_stroke(tk.BezPath().move_to(20, 20).line_to(60, 80).line_to(100, 20), 10.0, tk.Color(0, 0, 0, 1),
+                      join=join, start_cap=cap, end_cap=cap)
+        return _covered_area(c)
+    h = 5.0
+    turn = math.pi - math.acos(((-40) * 40 + (-60) * (-60)) / (40 ** 2 + 60 ** 2))
+    bevel = area(tk.Join.Bevel, tk.Cap.Butt)
+    # square caps add an h x 2h rectangle at each end
+    assert abs(area(tk.Join.Bevel, tk.Cap.Square) - bevel - 2 * 2 * h * h) < 0.5
+    # the miter triangle beyond the bevel: h^2 (tan(t/2) - sin(t/2) cos(t/2))
+    miter_extra = h * h * (math.tan(turn / 2) - math.sin(turn / 2) * math.cos(turn / 2))
+    assert abs(area(tk.Join.Miter, tk.Cap.Butt) - bevel - miter_extra) < 0.5
+    # the round join adds the circular segment h^2 (t - sin t) / 2 (its polygon slightly less)
+    round_extra = h * h * (turn - math.sin(turn)) / 2
+    assert 0.9 * round_extra < area(tk.Join.Round, tk.Cap.Butt) - bevel < round_extra + 0.3
+    # a miter limit below 1 / cos(t/2) turns the miter into a bevel
+    c = tk.Canvas(120, 100)
+    c.push_stroke(tk.BezPath().move_to(20, 20).line_to(60, 80).line_to(100, 20), 10.0, tk.Color(0, 0, 0, 1),
+                  join=tk.Join.Miter, start_cap=tk.Cap.Butt, end_cap=tk.Cap.Butt, miter_limit=1.5)
+    assert abs(_covered_area(c) - bevel) < 0.05
+
+
+def test_default_stroke_is_round_joins_and_caps():
+    path = tk.BezPath().move_to(20, 20).line_to(60, 80).line_to(100, 20)
+    a, b = tk.Canvas(120, 100), tk.Canvas(120, 100)
+    a.push_stroke(path, 6.0, tk.Color(0, 0, 0, 1))
+    b.push_stroke(path, 6.0, tk.Color(0, 0, 0, 1), join=tk.Join.Round, start_cap=tk.Cap.Round, end_cap=tk.Cap.Round)
+    la, lb = a.arrays(0).lines, b.arrays(0).lines
+    assert np.array_equal(la, lb)
+
+
+def test_invalid_stroke_styles_are_rejected():
+    c = tk.Canvas(64, 64)
+    path = tk.BezPath().move_to(10, 10).line_to(50, 50)
+    with pytest.raises(tk.TileinkError):
+        c.push_stroke(path, 4.0, tk.Color(0, 0, 0, 1), join=7)
+    with pytest.raises(tk.TileinkError):
+        c.push_stroke(path, 4.0, tk.Color(0, 0, 0, 1), join=tk.Join.Miter, miter_limit=0.5)
