@@ -206,6 +206,57 @@ def diff_batch(ctx, count=256, repeats=3):
     return out
 
 
+def fit(ctx, count=256, steps=20, lr=0.05):
+    """Batched image fitting with the differentiable form: 256 scenes of 256x256 as
+    in diff_batch; each scene's target is the render of a copy whose control points
+    are perturbed by N(0, 4 px) and colours by N(0, 0.05); `steps` Adam steps on
+    control points and colours (FP32) against the squared error, every step
+    re-recording the moved geometry on the host and running forward + backward on
+    the device for all scenes at once.  Returns the loss curve and per-step times."""
+    import torch
+    from tileink_amd import diff
+    rng = np.random.default_rng(9)
+    paths = rng.integers(64, 513, count)
+    make = lambda i, n: diff.random_scene(3000 + i, 256, 256, fills=int(n), cubics=4, radius_mu=np.log(16.0),
+                                          dtype=torch.float32)
+    scenes_, targets = [], []
+    for i, n in enumerate(paths):
+        t, s = make(i, n), make(i, n)
+        inset = rng.uniform(4, 24, 4)
+        clip = [(inset[0], inset[1], 256 - inset[2], 256 - inset[3], float(rng.uniform(4, 16)), 0, int(n))]
+        t.clips, s.clips = clip, list(clip)
+        with torch.no_grad():
+            t.fills += torch.as_tensor(rng.normal(0, 4.0, tuple(t.fills.shape)), dtype=torch.float32)
+            t.colors += torch.as_tensor(rng.normal(0, 0.05, tuple(t.colors.shape)), dtype=torch.float32)
+            t.colors.clamp_(0, 1)
+        scenes_.append(s)
+        targets.append(t)
+    with torch.no_grad():
+        ref = [im.clone() for im in diff.render(ctx, targets, "cuda")]
+    params = [p for s in scenes_ for p in (s.fills, s.colors)]
+    for p in params:
+        p.requires_grad_(True)
+    opt = torch.optim.Adam(params, lr=lr)
+    losses, step_ms = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        opt.zero_grad()
+        images = diff.render(ctx, scenes_, "cuda")
+        loss = sum(((im - r) ** 2).sum() for im, r in zip(images, ref))
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            for s in scenes_:
+                s.colors.clamp_(0, 1)
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - t0) * 1e3)
+        losses.append(round(float(loss.detach()), 3))
+    return {"scenes": count, "size": "256x256", "paths": int(paths.sum()), "clips_per_scene": 1, "steps": steps,
+            "loss": losses, "loss_ratio": round(losses[-1] / losses[0], 4),
+            "step_ms_median": round(float(np.median(step_ms)), 1),
+            "note": "each step re-records all scenes on the host; the device part is diff_fwd_bwd.batch_ms"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +272,8 @@ def main():
     ap.add_argument("--no-incremental", action="store_true", help="skip the damaged-tile re-render")
     ap.add_argument("--diff-only", action="store_true",
                     help="run only the differentiable forward+backward batch and print its figures as JSON")
+    ap.add_argument("--fit", type=int, default=0, metavar="STEPS",
+                    help="run only batched Adam image fitting for STEPS steps and print its loss curve as JSON")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device three times with tk_k_copy (counter calibration)")
     args = ap.parse_args()
@@ -229,6 +282,10 @@ def main():
               f"stripes, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.fit:
+        ctx = tk.NativeContext(0)
+        print(json.dumps({"fit": fit(ctx, steps=args.fit)}), flush=True)
+        return
     if args.diff_only:
         ctx = tk.NativeContext(0)
         print(json.dumps({"diff_fwd_bwd": diff_batch(ctx)}), flush=True)

@@ -236,6 +236,14 @@ tk_status tk_canvas_push_stroke(tk_canvas *canvas, const uint8_t *verbs, size_t 
                                 const double *points, size_t point_count, double width,
                                 const tk_brush *brush, const double *transform, uint32_t fill_rule,
                                 double tolerance, uint32_t *draw_id);
+/* Many closed polygons with solid brushes in one call (no counterpart in the
+ * reference, which records path by path): polygon i is counts[i] points of
+ * `points` (x, y pairs, laid end to end) recorded as push_path of
+ * move_to + line_to... + close_path with straight-alpha colour rgba[4i..4i+3]
+ * and fill_rules[i]; draws get consecutive ids from *first_draw_id. */
+tk_status tk_canvas_push_polygons(tk_canvas *canvas, const double *points, const uint32_t *counts,
+                                  uint32_t polygon_count, const float *rgba, const uint32_t *fill_rules,
+                                  uint32_t *first_draw_id);
 /* Canvas::push_stroke with a full kurbo::Stroke (src/canvas/primitives/paths.rs:24-48):
  * join and cap styles in kurbo's order (Join {Bevel, Miter, Round}, Cap {Butt,
  * Square, Round}); a miter join longer than miter_limit x width becomes a bevel.

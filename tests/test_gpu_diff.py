@@ -184,3 +184,31 @@ def test_scenes_with_path_clip_layers_are_rejected(ctx):
     with pytest.raises(tk.TileinkError):  # no differentiable render ran
         tk._abi.check(tk._abi.lib().tk_frame_backward(frame._h, colors.data_ptr(), grad.data_ptr(),
                                                       image.data_ptr(), colors.data_ptr()))
+
+
+def test_adam_fitting_recovers_a_perturbed_scene(ctx):
+    # Image fitting: control points and colours perturbed from a target scene are
+    # optimised with Adam against the target's image; the loss falls.
+    target = diff.random_scene(31, 96, 80, fills=8, cubics=4, dtype=torch.float32, radius_mu=math.log(14.0))
+    target.clips = [(4.0, 4.0, 92.0, 76.0, 6.0, 0, 8)]
+    sc = diff.random_scene(31, 96, 80, fills=8, cubics=4, dtype=torch.float32, radius_mu=math.log(14.0))
+    sc.clips = list(target.clips)
+    rng = np.random.default_rng(32)
+    with torch.no_grad():
+        ref = diff.render(ctx, target, DEV).clone()
+        sc.fills += torch.as_tensor(rng.normal(0, 2.0, tuple(sc.fills.shape)), dtype=torch.float32)
+        sc.colors += torch.as_tensor(rng.normal(0, 0.05, tuple(sc.colors.shape)), dtype=torch.float32)
+        sc.colors.clamp_(0, 1)
+    sc.fills.requires_grad_(True)
+    sc.colors.requires_grad_(True)
+    opt = torch.optim.Adam([sc.fills, sc.colors], lr=0.05)
+    losses = []
+    for _ in range(40):
+        opt.zero_grad()
+        loss = ((diff.render(ctx, sc, DEV) - ref) ** 2).sum()
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            sc.colors.clamp_(0, 1)
+        losses.append(float(loss.detach()))
+    assert losses[-1] < 0.5 * losses[0], losses

@@ -288,6 +288,21 @@ class Canvas:
         self._scene = None
         return out.value
 
+    def push_polygons(self, points, counts, rgba, rules):
+        """Closed polygons with solid colours in one call (tk_canvas_push_polygons):
+        points (V, 2), counts (n,) points per polygon, rgba (n, 4) straight alpha,
+        rules (n,).  Returns the draw ids."""
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 2))
+        cnt = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32).reshape(-1))
+        col = np.ascontiguousarray(np.asarray(rgba, dtype=np.float32).reshape(-1, 4))
+        rul = np.ascontiguousarray(np.asarray(rules, dtype=np.uint32).reshape(-1))
+        if not (len(cnt) == len(col) == len(rul)) or int(cnt.sum()) != len(pts):
+            raise TileinkError(_abi.TK_ERR_INVALID, "points, counts, colours and rules disagree")
+        first = C.c_uint32()
+        check(lib().tk_canvas_push_polygons(self._h, ptr(pts), ptr(cnt), len(cnt), ptr(col), ptr(rul), C.byref(first)))
+        self._scene = None
+        return np.arange(first.value, first.value + len(cnt), dtype=np.int64)
+
     def push_stroke(self, path, width, brush, transform=None, rule=FillRule.NonZero, tolerance=0.25, join=None,
                     start_cap=None, end_cap=None, miter_limit=4.0):
         """Canvas::push_stroke.  join / start_cap / end_cap (Join, Cap) default to round,

@@ -200,6 +200,7 @@ SIGNATURES = {
     "tk_frame_render": (C.c_int, [_P]),
     "tk_frame_render_tiles": (C.c_int, [_P, _P, _U32]),
     "tk_frame_set_graph": (C.c_int, [_P, C.c_int]),
+    "tk_canvas_push_polygons": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P, C.POINTER(C.c_uint32)]),
     "tk_canvas_push_stroke_styled": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_size_t, C.POINTER(StrokeStyle), _P, _P,
                                                C.c_uint32, C.c_double, C.POINTER(C.c_uint32)]),
     "tk_frame_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),

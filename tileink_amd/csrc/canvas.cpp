@@ -1340,6 +1340,36 @@ static tk_status push_path_common(tk_canvas *c, const uint8_t *verbs, size_t ver
     return TK_OK;
 }
 
+tk_status tk_canvas_push_polygons(tk_canvas *canvas, const double *points, const uint32_t *counts, uint32_t polygon_count,
+                                  const float *rgba, const uint32_t *fill_rules, uint32_t *first_draw_id) {
+    if (!canvas) return fail(TK_ERR_INVALID, "null canvas");
+    if (polygon_count && (!points || !counts || !rgba || !fill_rules)) return fail(TK_ERR_INVALID, "null argument");
+    size_t base = 0;
+    for (uint32_t i = 0; i < polygon_count; ++i) {
+        if (fill_rules[i] > 1u) return fail(TK_ERR_INVALID, "fill rule must be 0 (nonzero) or 1 (even-odd)");
+        base += counts[i];
+    }
+    std::vector<uint8_t> verbs;
+    base = 0;
+    for (uint32_t i = 0; i < polygon_count; ++i) {
+        const uint32_t n = counts[i];
+        verbs.assign(n + 1u, uint8_t(TK_PATH_LINE));
+        if (n) verbs[0] = uint8_t(TK_PATH_MOVE);
+        verbs[n] = uint8_t(TK_PATH_CLOSE);
+        tk_brush brush{};
+        brush.kind = TK_BRUSH_SOLID;
+        for (int k = 0; k < 4; ++k) brush.rgba[k] = rgba[4 * size_t(i) + k];
+        uint32_t id = 0;
+        tk_status st = n ? push_path_common(canvas, verbs.data(), verbs.size(), points + 2 * base, n, 0.0, false, &brush,
+                                            nullptr, fill_rules[i], 0.25, &id)
+                         : fail(TK_ERR_INVALID, "empty polygon");
+        if (st != TK_OK) return st;
+        if (i == 0 && first_draw_id) *first_draw_id = id;
+        base += n;
+    }
+    return TK_OK;
+}
+
 tk_status tk_canvas_push_path(tk_canvas *canvas, const uint8_t *verbs, size_t verb_count, const double *points,
                               size_t point_count, const tk_brush *brush, const double *transform, uint32_t fill_rule,
                               double tolerance, uint32_t *draw_id) {

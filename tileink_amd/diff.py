@@ -280,26 +280,19 @@ def edge_map(verts32, counts):
     from vertex j to vertex j + 1 (the last back to the first); edges shorter than
     1e-12 are dropped, as push_line drops them.  Returns (line -> first vertex,
     line -> second vertex, lines per draw)."""
-    a_idx, b_idx, per_draw = [], [], []
-    base = 0
-    for k in counts:
-        k = int(k)
-        if k == 0:
-            per_draw.append(0)
-            continue
-        ia = np.arange(base, base + k)
-        ib = np.roll(ia, -1)
-        p = verts32.astype(np.float64)
-        d = p[ib] - p[ia]
-        ok = (d * d).sum(1) > 1.0e-24
-        if k == 1:
-            ok[:] = False
-        a_idx.append(ia[ok])
-        b_idx.append(ib[ok])
-        per_draw.append(int(ok.sum()))
-        base += k
-    cat = lambda xs: np.concatenate(xs) if xs else np.zeros(0, np.int64)
-    return cat(a_idx), cat(b_idx), np.array(per_draw, np.int64)
+    counts = np.asarray(counts, np.int64)
+    ia = np.arange(int(counts.sum()), dtype=np.int64)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]]) if len(counts) else np.zeros(0, np.int64)
+    ib = ia + 1
+    ends = starts + counts - 1
+    ib[ends[counts > 0]] = starts[counts > 0]
+    p = np.asarray(verts32, np.float32).astype(np.float64)
+    d = p[ib] - p[ia] if len(ia) else np.zeros((0, 2))
+    ok = (d * d).sum(1) > 1.0e-24
+    poly = np.repeat(np.arange(len(counts)), counts)
+    ok &= np.repeat(counts, counts) > 1
+    per_draw = np.bincount(poly[ok], minlength=len(counts)).astype(np.int64)
+    return ia[ok], ib[ok], per_draw
 
 
 class Encoded:
@@ -321,18 +314,22 @@ def encode(scene, verts, counts, colors=None):
     col = (scene.colors if colors is None else colors).detach().cpu().double().numpy()
     # outer clips first where several open at one draw, inner first where several close
     opens = sorted(range(len(scene.clips)), key=lambda i: (scene.clips[i][5], -scene.clips[i][6]))
-    base, draw_ids, o = 0, [], 0
-    open_stack = []
-    for d, k in enumerate(counts):
-        while o < len(opens) and scene.clips[opens[o]][5] == d:
+    counts = np.asarray(counts, np.int64)
+    offsets = np.concatenate([[0], np.cumsum(counts)])
+    colors32 = np.clip(col, 0.0, 1.0).astype(np.float32)
+    # runs of draws between clip boundaries go down in one call each
+    bounds = sorted({0, len(counts)} | {int(c[5]) for c in scene.clips} | {int(c[6]) for c in scene.clips})
+    draw_ids, o, open_stack = [], 0, []
+    for lo, hi in zip(bounds[:-1], bounds[1:]):
+        while o < len(opens) and scene.clips[opens[o]][5] == lo:
             x0, y0, x1, y1, radius = scene.clips[opens[o]][:5]
             canvas.push_clip_sdf_rect_layer(x0, y0, x1, y1, radius)
             open_stack.append(opens[o])
             o += 1
-        c = np.clip(col[d], 0.0, 1.0)
-        draw_ids.append(canvas.push_polygon(v32[base:base + k], tk.Color(*c), tk.FillRule(int(rules[d]))))
-        base += int(k)
-        while open_stack and scene.clips[open_stack[-1]][6] == d + 1:
+        if hi > lo:
+            draw_ids.extend(canvas.push_polygons(v32[offsets[lo]:offsets[hi]], counts[lo:hi], colors32[lo:hi],
+                                                 rules[lo:hi]))
+        while open_stack and scene.clips[open_stack[-1]][6] == hi:
             canvas.pop_layer()
             open_stack.pop()
     if o != len(opens) or open_stack:
