@@ -158,9 +158,9 @@ def scene_loop(ctx, count=64, repeats=5):
 
 def diff_batch(ctx, count=256, repeats=3):
     """Forward + backward of the differentiable form over many scenes: 256 scenes of
-    256x256, each with 64-512 closed paths of 4 cubics and solid colours under one
-    rounded-rectangle clip (inset 4-24 px, radius 4-16 px), loss = squared error
-    against a random target.  A step renders every scene in FP32
+    256x256, each with 64-512 closed paths of 4 cubics under one rounded-rectangle
+    clip (inset 4-24 px, radius 4-16 px), 10 % of the paints linear gradients of
+    2-4 stops and the rest solid, loss = squared error against a random target.  A step renders every scene in FP32
     (tk_frame_render_diff) and back-propagates dL/d(image) to line endpoints and
     colours (tk_frame_backward), from scenes already resident in HBM.  Timed once as
     a per-scene loop (one frame per scene) and once as one batch."""
@@ -170,26 +170,27 @@ def diff_batch(ctx, count=256, repeats=3):
     paths = rng.integers(64, 513, count)
     sc = [diff.random_scene(1000 + i, 256, 256, fills=int(n), cubics=4, radius_mu=np.log(16.0))
           for i, n in enumerate(paths)]
-    for s, n in zip(sc, paths):
+    for i, (s, n) in enumerate(zip(sc, paths)):
         inset = rng.uniform(4, 24, 4)
         s.clips = [(inset[0], inset[1], 256 - inset[2], 256 - inset[3], float(rng.uniform(4, 16)), 0, int(n))]
+        diff.random_gradients(s, 0.1, 5000 + i)
     polys = [s.polygons() for s in sc]
     singles = [diff.Plan(ctx, [s], [v], [k]) for s, (v, k) in zip(sc, polys)]
     batch = diff.Plan(ctx, sc, [v for v, _ in polys], [k for _, k in polys])
-    colors = [p.full_colors(s.premultiplied()).to("cuda", torch.float32) for p, s in zip(singles, sc)]
+    colors = [p.full_paint(s.paint_records()).to("cuda", torch.float32) for p, s in zip(singles, sc)]
     gen = torch.Generator(device="cuda").manual_seed(5)
     targets = [torch.rand((p.pixels, 4), device="cuda", generator=gen) for p in singles]
     target_b = torch.zeros((batch.pixels, 4), device="cuda")
     for (w, h, _), off, t in zip(batch.sizes, batch.offsets, targets):
         target_b[off:off + w * h] = t
-    color_b = batch.full_colors(torch.cat([s.premultiplied() for s in sc])).to("cuda", torch.float32)
+    color_b = batch.full_paint(torch.cat([s.paint_records() for s in sc])).to("cuda", torch.float32)
 
     def step(plan, c, t):
         image = plan.forward(c)
         plan.backward(2.0 * (image - t))
 
     out = {"scenes": count, "size": "256x256", "paths": int(paths.sum()), "lines": int(batch.line_count),
-           "clips_per_scene": 1}
+           "clips_per_scene": 1, "gradient_paints": int(sum(len(s.gradients) for s in sc))}
     for name, run in (("loop", lambda: [step(p, c, t) for p, c, t in zip(singles, colors, targets)]),
                       ("batch", lambda: step(batch, color_b, target_b))):
         run()  # warm-up: sizes the coverage buffers
@@ -209,8 +210,9 @@ def diff_batch(ctx, count=256, repeats=3):
 def fit(ctx, count=256, steps=20, lr=0.05):
     """Batched image fitting with the differentiable form: 256 scenes of 256x256 as
     in diff_batch; each scene's target is the render of a copy whose control points
-    are perturbed by N(0, 4 px) and colours by N(0, 0.05); `steps` Adam steps on
-    control points and colours (FP32) against the squared error, every step
+    are perturbed by N(0, 4 px) and colours (and gradient stops) by N(0, 0.05);
+    `steps` Adam steps on control points, colours and stops (FP32) against the
+    squared error, every step
     re-recording the moved geometry on the host and running forward + backward on
     the device for all scenes at once.  Returns the loss curve and per-step times."""
     import torch
@@ -225,6 +227,12 @@ def fit(ctx, count=256, steps=20, lr=0.05):
         inset = rng.uniform(4, 24, 4)
         clip = [(inset[0], inset[1], 256 - inset[2], 256 - inset[3], float(rng.uniform(4, 16)), 0, int(n))]
         t.clips, s.clips = clip, list(clip)
+        diff.random_gradients(t, 0.1, 7000 + i)
+        diff.random_gradients(s, 0.1, 7000 + i)
+        with torch.no_grad():
+            for d, (line, offsets, stops) in t.gradients.items():
+                stops += torch.as_tensor(rng.normal(0, 0.05, tuple(stops.shape)), dtype=torch.float32)
+                stops.clamp_(0, 1)
         with torch.no_grad():
             t.fills += torch.as_tensor(rng.normal(0, 4.0, tuple(t.fills.shape)), dtype=torch.float32)
             t.colors += torch.as_tensor(rng.normal(0, 0.05, tuple(t.colors.shape)), dtype=torch.float32)
@@ -233,7 +241,7 @@ def fit(ctx, count=256, steps=20, lr=0.05):
         targets.append(t)
     with torch.no_grad():
         ref = [im.clone() for im in diff.render(ctx, targets, "cuda")]
-    params = [p for s in scenes_ for p in (s.fills, s.colors)]
+    params = [p for s in scenes_ for p in (s.fills, s.colors, *[g[2] for g in s.gradients.values()])]
     for p in params:
         p.requires_grad_(True)
     opt = torch.optim.Adam(params, lr=lr)
@@ -248,10 +256,13 @@ def fit(ctx, count=256, steps=20, lr=0.05):
         with torch.no_grad():
             for s in scenes_:
                 s.colors.clamp_(0, 1)
+                for g in s.gradients.values():
+                    g[2].clamp_(0, 1)
         torch.cuda.synchronize()
         step_ms.append((time.perf_counter() - t0) * 1e3)
         losses.append(round(float(loss.detach()), 3))
-    return {"scenes": count, "size": "256x256", "paths": int(paths.sum()), "clips_per_scene": 1, "steps": steps,
+    return {"scenes": count, "size": "256x256", "paths": int(paths.sum()), "clips_per_scene": 1,
+            "gradient_paints": int(sum(len(s.gradients) for s in scenes_)), "steps": steps,
             "loss": losses, "loss_ratio": round(losses[-1] / losses[0], 4),
             "step_ms_median": round(float(np.median(step_ms)), 1),
             "note": "each step re-records all scenes on the host; the device part is diff_fwd_bwd.batch_ms"}

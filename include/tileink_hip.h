@@ -367,18 +367,24 @@ tk_status tk_frame_autotune(tk_frame *frame, uint32_t repeats, tk_tuning *chosen
 /* The differentiable form (no counterpart in the reference, which only renders).
  * Its forward pass is the fine stage (fine/main.hlsl:23-40) with the RGBA8 round
  * trips between batches left out: an FP32 premultiplied source-over composite of
- * the scene's FILL records.  Scenes: path draws only (fills and strokes), no
- * layers; each draw's colour is colors[4 * draw .. 4 * draw + 3] (premultiplied
- * RGBA, FP32), not its brush.  All float pointers are device pointers.
+ * the scene's FILL records.  Scenes: path draws (fills and strokes) inside SDF
+ * clip layers.  Each draw's paint is paint[32 * draw ..] (FP32), not its brush:
+ *   {kind: 0 solid, 1 linear gradient; x0, y0, x1, y1: the gradient line in
+ *    pixels; n: stop count 1-4; 4 stop offsets; 6 unused; 4 stops of
+ *    premultiplied RGBA},
+ * a solid paint being its first stop; a linear gradient clamps the projection
+ * of the pixel centre on its line to [0, 1] and interpolates the stops linearly.
+ * All float pointers are device pointers.
  * tk_frame_render_diff: the whole pipeline; writes `image` (4 floats per pixel,
  *   row-major; a batch's scenes at their image offsets, one after another).
  * tk_frame_backward: after tk_frame_render_diff; from image_grad = dL/d(image)
  *   writes line_grad = dL/d(p0.x, p0.y, p1.x, p1.y) for every line of the scene
- *   and color_grad = dL/d(colour) for every draw (both overwritten).
+ *   and paint_grad = dL/d(paint record) for every draw: 32 floats, its stops'
+ *   colours at 16-31 (both overwritten).
  * tk_frame_diff_status: bit 0 when a record the form does not take was met.   */
-tk_status tk_frame_render_diff(tk_frame *frame, const float *colors, float *image);
-tk_status tk_frame_backward(tk_frame *frame, const float *colors, const float *image_grad, float *line_grad,
-                            float *color_grad);
+tk_status tk_frame_render_diff(tk_frame *frame, const float *paint, float *image);
+tk_status tk_frame_backward(tk_frame *frame, const float *paint, const float *image_grad, float *line_grad,
+                            float *paint_grad);
 tk_status tk_frame_diff_status(tk_frame *frame, uint32_t *flags);
 
 /* enable != 0: tk_frame_render captures its launch sequence (both streams) into a

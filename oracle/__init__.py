@@ -230,8 +230,9 @@ def diff_render(width, height, clear, draws, colors, lines, masks=None):
     """FP64 forward of the differentiable form.  draws: (line_start, line_count, rule)
     per draw in painter order; colors: premultiplied RGBA per draw; lines: (L, 4).
     masks: optional per-draw clip masks (H, W) (None where a draw has no clip);
-    a masked draw's alpha is multiplied by its mask."""
-    if masks is not None and any(m is not None for m in masks):
+    a masked draw's alpha is multiplied by its mask.  A colour may also be an
+    (H, W, 4) image (a gradient paint evaluated at the pixel centres)."""
+    if (masks is not None and any(m is not None for m in masks)) or _per_pixel(colors):
         return _masked_render(width, height, clear, draws, colors, lines, masks)
     starts, counts, rules, lines = _diff_args(draws, lines)
     clear = np.ascontiguousarray(clear, np.float64)
@@ -240,6 +241,10 @@ def diff_render(width, height, clear, draws, colors, lines, masks=None):
     lib().or_diff_render_f64(C.c_uint32(width), C.c_uint32(height), _p(clear), C.c_uint32(len(draws)), _p(starts),
                              _p(counts), _p(rules), _p(colors), _p(lines), _p(image))
     return image
+
+
+def _per_pixel(colors):
+    return isinstance(colors, (list, tuple)) and any(np.ndim(c) == 3 for c in colors)
 
 
 def _alpha(a, rule):
@@ -254,7 +259,9 @@ def _masked_render(width, height, clear, draws, colors, lines, masks, image_grad
     """The composite with clip masks, per draw over its box: the areas come from
     or_diff_area_f64; the composite and its adjoint are restated here in numpy."""
     lines = np.ascontiguousarray(lines, np.float64).reshape(-1, 4)
-    colors = np.asarray(colors, np.float64).reshape(-1, 4)
+    if not _per_pixel(colors):
+        colors = list(np.asarray(colors, np.float64).reshape(-1, 4))
+    masks = masks if masks is not None else [None] * len(draws)
     image = np.broadcast_to(np.asarray(clear, np.float64), (height, width, 4)).copy()
     layers = []
     for d, (start, count, rule) in enumerate(draws):
@@ -264,40 +271,47 @@ def _masked_render(width, height, clear, draws, colors, lines, masks, image_grad
         if masks[d] is not None:
             m = np.asarray(masks[d], np.float64)[y0:y0 + h, x0:x0 + w]
             alpha, slope = alpha * m, slope * m
-        layers.append((x0, y0, w, h, alpha, slope))
+        c = np.asarray(colors[d], np.float64)
+        c = c[y0:y0 + h, x0:x0 + w] if c.ndim == 3 else np.broadcast_to(c, (h, w, 4))
+        layers.append((x0, y0, w, h, alpha, slope, c))
     if image_grad is None:
-        for d, (x0, y0, w, h, alpha, _) in enumerate(layers):
+        for x0, y0, w, h, alpha, _, c in layers:
             px = image[y0:y0 + h, x0:x0 + w]
-            px *= (1.0 - alpha * colors[d, 3])[..., None]
-            px += alpha[..., None] * colors[d]
+            px *= (1.0 - alpha * c[..., 3])[..., None]
+            px += alpha[..., None] * c
         return image
     g = np.asarray(image_grad, np.float64).reshape(height, width, 4)
     T = np.ones((height, width))
     trans = [None] * len(layers)
     for d in range(len(layers) - 1, -1, -1):
-        x0, y0, w, h, alpha, _ = layers[d]
+        x0, y0, w, h, alpha, _, c = layers[d]
         trans[d] = T[y0:y0 + h, x0:x0 + w].copy()
-        T[y0:y0 + h, x0:x0 + w] *= 1.0 - alpha * colors[d, 3]
-    color_grad = np.zeros((len(layers), 4))
+        T[y0:y0 + h, x0:x0 + w] *= 1.0 - alpha * c[..., 3]
+    color_grad, color_maps = np.zeros((len(layers), 4)), []
     cov_grad = []
-    for d, (x0, y0, w, h, alpha, slope) in enumerate(layers):
+    for d, (x0, y0, w, h, alpha, slope, c) in enumerate(layers):
         r = image[y0:y0 + h, x0:x0 + w]
         gg = g[y0:y0 + h, x0:x0 + w]
         t = trans[d]
         g_r = (gg * r).sum(-1)
-        g_c = gg @ colors[d]
-        cov_grad.append((t * (g_c - colors[d, 3] * g_r) * slope, x0, y0, w, h))
-        color_grad[d, :3] = ((t * alpha)[..., None] * gg[..., :3]).sum((0, 1))
-        color_grad[d, 3] = (t * alpha * (gg[..., 3] - g_r)).sum()
-        r *= (1.0 - alpha * colors[d, 3])[..., None]
-        r += alpha[..., None] * colors[d]
-    return cov_grad, color_grad
+        g_c = (gg * c).sum(-1)
+        cov_grad.append((t * (g_c - c[..., 3] * g_r) * slope, x0, y0, w, h))
+        cmap = np.concatenate([(t * alpha)[..., None] * gg[..., :3], (t * alpha * (gg[..., 3] - g_r))[..., None]], -1)
+        color_grad[d] = cmap.sum((0, 1))
+        full = np.zeros((height, width, 4))
+        full[y0:y0 + h, x0:x0 + w] = cmap
+        color_maps.append(full)
+        r *= (1.0 - alpha * c[..., 3])[..., None]
+        r += alpha[..., None] * c
+    return cov_grad, (color_grad, color_maps)
 
 
-def diff_backward(width, height, clear, draws, colors, lines, image_grad, masks=None):
-    """FP64 analytic gradient: (dL/d lines (L, 4), dL/d colours (D, 4)) for dL/d image."""
-    if masks is not None and any(m is not None for m in masks):
-        return _masked_backward(width, height, clear, draws, colors, lines, image_grad, masks)
+def diff_backward(width, height, clear, draws, colors, lines, image_grad, masks=None, color_maps=False):
+    """FP64 analytic gradient: (dL/d lines (L, 4), dL/d colours (D, 4)) for dL/d image;
+    with color_maps=True the second item is the per-pixel dL/d colour (H, W, 4) of
+    each draw instead (for per-pixel colours)."""
+    if (masks is not None and any(m is not None for m in masks)) or _per_pixel(colors) or color_maps:
+        return _masked_backward(width, height, clear, draws, colors, lines, image_grad, masks, color_maps)
     starts, counts, rules, lines = _diff_args(draws, lines)
     clear = np.ascontiguousarray(clear, np.float64)
     colors = np.ascontiguousarray(colors, np.float64).reshape(-1, 4)
@@ -310,18 +324,18 @@ def diff_backward(width, height, clear, draws, colors, lines, image_grad, masks=
     return line_grad, color_grad
 
 
-def _masked_backward(width, height, clear, draws, colors, lines, image_grad, masks):
+def _masked_backward(width, height, clear, draws, colors, lines, image_grad, masks, color_maps=False):
     """With clip masks: the composite's adjoint in numpy (_masked_render), then the
     line pass of each draw (or_diff_line_grad_f64) on its dL/dcoverage."""
     lines = np.ascontiguousarray(lines, np.float64).reshape(-1, 4)
-    cov_grad, color_grad = _masked_render(width, height, clear, draws, colors, lines, masks, image_grad)
+    cov_grad, (color_grad, maps) = _masked_render(width, height, clear, draws, colors, lines, masks, image_grad)
     line_grad = np.zeros_like(lines)
     for d, (start, count, rule) in enumerate(draws):
         gmap = np.ascontiguousarray(cov_grad[d][0], np.float64)
         if count and gmap.size:
             lib().or_diff_line_grad_f64(_p(lines), C.c_uint32(start), C.c_uint32(count), C.c_uint32(width),
                                         C.c_uint32(height), _p(gmap), _p(line_grad))
-    return line_grad, color_grad
+    return line_grad, (maps if color_maps else color_grad)
 
 
 def sdf_draw_coverage(arrays, draw_index):

@@ -19,7 +19,7 @@ import oracle
 import tileink_amd as tk
 from tileink_amd import diff
 
-from test_diff import analytic_grads, clip_masks, fd_loss
+from test_diff import analytic_grads, clip_masks, fd_loss, paint_colors
 
 pytestmark = pytest.mark.gpu
 
@@ -105,6 +105,32 @@ def test_clipped_scenes_match_fp64(ctx, seed):
     assert_grads_close(got, [p.grad for p in (sc.fills, sc.strokes, sc.widths, sc.colors)])
 
 
+@pytest.mark.parametrize("seed", [17, 18])
+def test_linear_gradient_paints_match_fp64(ctx, seed):
+    # Linear-gradient paints (2-4 stops) on some draws, inside a clip: the image and
+    # the gradients of geometry, colours and stop colours against the FP64 oracle.
+    from test_diff import add_gradients
+    sc = diff.random_scene(seed, 160, 112, fills=16, strokes=4, radius_mu=math.log(18.0))
+    add_gradients(sc, [0, 3, 5, 8, 13], seed)
+    sc.clips = [(10.0, 8.0, 150.0, 104.0, 8.0, 2, 18)]
+    w = np.random.default_rng(seed).normal(size=(112, 160, 4))
+    stops = [g[2] for g in sc.gradients.values()]
+    for p in stops:
+        p.requires_grad_(True)
+    image, got = gpu_grads(ctx, sc, w)
+    got_stops = [p.grad.clone() for p in stops]
+    with torch.no_grad():
+        v, k = sc.polygons()
+        draws, lines, _, _ = diff.oracle_inputs(sc, v, k, recorded=True)
+        ref_image = oracle.diff_render(160, 112, diff.clear_color(sc), draws, paint_colors(sc), lines,
+                                       masks=clip_masks(sc, v, k))
+    err = float(np.abs(image.detach().cpu().double().numpy() - ref_image).max())
+    assert err < 1e-5, err
+    analytic_grads(sc, w)
+    assert_grads_close(got + got_stops, [p.grad for p in (sc.fills, sc.strokes, sc.widths, sc.colors)] +
+                       [p.grad for p in stops])
+
+
 def test_gradients_match_finite_differences_at_1024(ctx):
     # 1024^2, 2048 closed paths of 4 cubics, loss against a target whose control
     # points are perturbed by N(0, 4 px) and colours by N(0, 0.05); central
@@ -176,7 +202,7 @@ def test_scenes_with_path_clip_layers_are_rejected(ctx):
     canvas.pop_layer()
     scene = canvas.encode(0)
     frame = tk.Frame(ctx, scene)
-    colors = torch.ones((scene.draw_count, 4), device=DEV)
+    colors = torch.ones((scene.draw_count, 32), device=DEV)
     image = torch.zeros((64 * 64, 4), device=DEV)
     with pytest.raises(tk.TileinkError):
         tk._abi.check(tk._abi.lib().tk_frame_render_diff(frame._h, colors.data_ptr(), image.data_ptr()))

@@ -1782,6 +1782,60 @@ struct Transmit {
     }
 };
 
+// A draw's paint in the differentiable form, 32 floats: {kind (0 solid, 1
+// linear gradient), x0, y0, x1, y1 (the gradient line, pixels), stop count n
+// (1-4), 4 stop offsets, 6 unused, 4 stops of premultiplied RGBA}; a solid
+// paint is its first stop.  The linear gradient is evaluated exactly (no 8-bit
+// ramp): t = the projection of the pixel centre on the line, clamped to [0, 1]
+// (pad), the colour piecewise linear between the stops in premultiplied space
+// and constant beyond the end stops.  w holds each stop's weight in the colour.
+constexpr uint32_t kDiffPaintFloats = 32u;
+struct DiffPaint {
+    float4 c;
+    float w0, w1, w2, w3;
+};
+__device__ __forceinline__ DiffPaint diff_paint(const float *paint, uint32_t draw, float x, float y) {
+    const float4 *r = reinterpret_cast<const float4 *>(paint + (size_t)draw * kDiffPaintFloats);
+    const float4 h0 = r[0];
+    DiffPaint p{r[4], 1.0f, 0.0f, 0.0f, 0.0f};
+    if (h0.x == 0.0f) return p;
+    const float4 h1 = r[1], h2 = r[2];
+    const float dx = h0.w - h0.y, dy = h1.x - h0.z, den = dx * dx + dy * dy;
+    float t = den > 0.0f ? ((x - h0.y) * dx + (y - h0.z) * dy) / den : 0.0f;
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+    const float n = h1.y, o0 = h1.z, o1 = h1.w, o2 = h2.x, o3 = h2.y;
+    float w0 = 0.0f, w1 = 0.0f, w2 = 0.0f, w3 = 0.0f;
+    if (n <= 1.0f || t <= o0) {
+        w0 = 1.0f;
+    } else if (t < o1) {
+        const float u = (t - o0) / (o1 - o0);
+        w0 = 1.0f - u;
+        w1 = u;
+    } else if (n <= 2.0f) {
+        w1 = 1.0f;
+    } else if (t < o2) {
+        const float u = (t - o1) / (o2 - o1);
+        w1 = 1.0f - u;
+        w2 = u;
+    } else if (n <= 3.0f) {
+        w2 = 1.0f;
+    } else if (t < o3) {
+        const float u = (t - o2) / (o3 - o2);
+        w2 = 1.0f - u;
+        w3 = u;
+    } else {
+        w3 = 1.0f;
+    }
+    const float4 s0 = r[4], s1 = r[5], s2 = r[6], s3 = r[7];
+    p.c = make_float4(w0 * s0.x + w1 * s1.x + w2 * s2.x + w3 * s3.x, w0 * s0.y + w1 * s1.y + w2 * s2.y + w3 * s3.y,
+                      w0 * s0.z + w1 * s1.z + w2 * s2.z + w3 * s3.z, w0 * s0.w + w1 * s1.w + w2 * s2.w + w3 * s3.w);
+    p.w0 = w0;
+    p.w1 = w1;
+    p.w2 = w2;
+    p.w3 = w3;
+    return p;
+}
+
 struct DiffTile {
     SceneView v;
     uint32_t column, tile, lane, gx, gy;
@@ -1900,14 +1954,14 @@ __device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &
     }
     uint32_t sel = 0u;
     __syncthreads();
-    const float4 *colors = reinterpret_cast<const float4 *>(a.colors);
+    const float sx = (float)t.gx + 0.5f, sy = (float)t.gy + 0.5f;
     Px px = rgba8_to_unorm(t.v.clear);
     bool overflow = false;
     Transmit total{1.0f, 0, 0};
     const uint32_t tx = t.tile % t.v.tw, ty = t.tile / t.v.tw;
     for_each_fill<kClips>(a, t, [&](uint32_t pi, const tk_ptcl_record &rec, float mask) {
         const uint32_t s0 = rec.segment_start, s1 = min(rec.segment_end, a.cfg.segment_capacity);
-        const float4 c = colors[rec.color];
+        const float4 c = diff_paint(a.colors, rec.color, sx, sy).c;
         // Issued before the coverage work so that its latency overlaps it.
         const uint4 box = reinterpret_cast<const uint4 *>(a.draw_tiles)[rec.color];
         CoverageAcc &acc = cov_acc[sel];
@@ -1946,7 +2000,7 @@ __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff 
     DiffTile t;
     if (!diff_tile<kScenes>(a, t)) return;
     const uint32_t lane = t.lane;
-    const float4 *colors = reinterpret_cast<const float4 *>(a.colors);
+    const float sx = (float)t.gx + 0.5f, sy = (float)t.gy + 0.5f;
     float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (t.inside) g = reinterpret_cast<const float4 *>(a.image_grad)[t.v.image + (size_t)t.gy * t.v.width + t.gx];
     auto coverage = [&](uint32_t pi) { return pi < a.adj_records ? a.adj[(size_t)pi * 256u + lane] : 0.0f; };
@@ -1954,7 +2008,9 @@ __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff 
     const Transmit total{__uint_as_float(tw.x), (int)tw.y, (int)tw.z};
     Transmit below{1.0f, 0, 0};
     Px res = rgba8_to_unorm(t.v.clear);
-    auto record = [&](uint32_t pi, const tk_ptcl_record &rec, float cov, const float4 &c, float mask) {
+    auto record = [&](uint32_t pi, const tk_ptcl_record &rec, float cov, float mask) {
+        const DiffPaint paint = diff_paint(a.colors, rec.color, sx, sy);
+        const float4 c = paint.c;
         float slope;
         const float alpha = diff_alpha(cov, rec.fill_rule, slope) * mask;
         slope *= mask;
@@ -1965,21 +2021,36 @@ __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff 
         const float g_c = g.x * c.x + g.y * c.y + g.z * c.z + g.w * c.w;
         const float d_alpha = T * (g_c - c.w * g_res);
         if (pi < a.adj_records) a.adj[(size_t)pi * 256u + lane] = d_alpha * slope;
+        // dL/d(stop j) = w_j T alpha g (the alpha channel less T alpha w_j g.R); one
+        // four-channel wave sum per stop the paint has (the stop count is uniform).
         const float w = T * alpha;
-        const float sum = wave_sum4(w * g.x, w * g.y, w * g.z, w * (g.w - g_res), lane);
-        if ((lane & 63u) >= 60u) atomicAdd(a.color_grad + (size_t)rec.color * 4u + sum4_channel(lane), sum);
+        const float gr = w * g.x, gg = w * g.y, gb = w * g.z, ga = w * (g.w - g_res);
+        float *out = a.color_grad + (size_t)rec.color * kDiffPaintFloats + 16u + sum4_channel(lane);
+        const float stops = a.colors[(size_t)rec.color * kDiffPaintFloats] == 0.0f
+                                ? 1.0f
+                                : a.colors[(size_t)rec.color * kDiffPaintFloats + 5u];
+        const float sum0 = wave_sum4(paint.w0 * gr, paint.w0 * gg, paint.w0 * gb, paint.w0 * ga, lane);
+        if ((lane & 63u) >= 60u) atomicAdd(out, sum0);
+        if (stops > 1.0f) {
+            const float sum1 = wave_sum4(paint.w1 * gr, paint.w1 * gg, paint.w1 * gb, paint.w1 * ga, lane);
+            if ((lane & 63u) >= 60u) atomicAdd(out + 4, sum1);
+        }
+        if (stops > 2.0f) {
+            const float sum2 = wave_sum4(paint.w2 * gr, paint.w2 * gg, paint.w2 * gb, paint.w2 * ga, lane);
+            if ((lane & 63u) >= 60u) atomicAdd(out + 8, sum2);
+        }
+        if (stops > 3.0f) {
+            const float sum3 = wave_sum4(paint.w3 * gr, paint.w3 * gg, paint.w3 * gb, paint.w3 * ga, lane);
+            if ((lane & 63u) >= 60u) atomicAdd(out + 12, sum3);
+        }
         res.r = res.r * f + alpha * c.x;
         res.g = res.g * f + alpha * c.y;
         res.b = res.b * f + alpha * c.z;
         res.a = res.a * f + alpha * c.w;
     };
-    // The walk of for_each_fill, software-pipelined: the next record, its colour
-    // and its coverage are loaded while the current record is processed (the
-    // loop does little arithmetic per record, so it is bound by these loads).
+    // The walk of for_each_fill, software-pipelined: the next record and its
+    // coverage are loaded while the current record is processed.
     const uint32_t draws = a.cfg.draw_count;
-    auto color_of = [&](const tk_ptcl_record &rec) {
-        return rec.tag == TK_PTCL_FILL && rec.color < draws ? colors[rec.color] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    };
     for (uint32_t b = t.v.q0; b < t.v.q1; ++b) {
         const size_t idx = table_index(t.v, b);
         const uint32_t kind_word = a.in.tile_kinds[idx];
@@ -1990,7 +2061,6 @@ __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff 
         if (pi >= pend) continue;
         tk_ptcl_record rec = a.in.ptcl[pi];
         float cov = coverage(pi);
-        float4 c = color_of(rec);
         ClipMask clip{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0u};
         while (rec.tag != TK_PTCL_END) {
             const uint32_t next = pi + 1u;
@@ -1998,15 +2068,13 @@ __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff 
             tk_ptcl_record nrec{};
             if (more) nrec = a.in.ptcl[next];
             const float ncov = more ? coverage(next) : 0.0f;
-            const float4 nc = more ? color_of(nrec) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (rec.tag == TK_PTCL_FILL && rec.color < draws) record(pi, rec, cov, c, clip.m);
+            if (rec.tag == TK_PTCL_FILL && rec.color < draws) record(pi, rec, cov, clip.m);
             else if ((!kClips || !clip_record<true>(a, t, pi, rec, clip)) && lane == 0u && a.status)
                 atomicOr(a.status, 1u);
             if (!more) break;
             pi = next;
             rec = nrec;
             cov = ncov;
-            c = nc;
         }
     }
 }

@@ -1415,7 +1415,7 @@ tk_status tk_frame_backward(tk_frame *f, const float *colors, const float *image
     if (!f->diff_ready) return fail(TK_ERR_INVALID, "tk_frame_backward needs a tk_frame_render_diff of the scene first");
     tk_context *ctx = f->ctx;
     const tk_scene &s = f->meta;
-    TK_TRY(tk::fill_u32(ctx, color_grad, uint64_t(s.draw_count) * 4, 0u));
+    TK_TRY(tk::fill_u32(ctx, color_grad, uint64_t(s.draw_count) * 32, 0u));
     tk_args_fine_diff a = tk::diff_args(f, colors);
     a.image_grad = image_grad;
     a.color_grad = color_grad;

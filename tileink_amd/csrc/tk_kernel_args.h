@@ -199,10 +199,10 @@ struct tk_args_fine_diff {
     const tk_path_record *paths;
     uint32_t path_count;
     uint32_t rec_of_count;      // backdrop tiles (the length of rec_of)
-    const float *colors;        // premultiplied RGBA per draw
+    const float *colors;        // 32-float paint record per draw (solid or linear gradient)
     float *image;               // forward: premultiplied RGBA per pixel
     const float *image_grad;    // backward: dL/d(image) per pixel
-    float *color_grad;          // backward: dL/d(colour) per draw, accumulated
+    float *color_grad;          // backward: dL/d(paint record) per draw (its stop colours), accumulated
     float *adj;                 // 256 floats per PTCL record: the forward's coverage, then dL/d(coverage)
     uint64_t adj_records;
     uint32_t *trans;            // 4 words per pixel (256 per column): the forward's transmittance product

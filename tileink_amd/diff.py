@@ -85,10 +85,14 @@ class DiffScene:
     clips:   SDF clip layers as (x0, y0, x1, y1, radius, first, end): a rounded
              rectangle clipping the draws [first, end) of painter order; ranges
              nest or are disjoint.  The clip geometry is not differentiated.
+    gradients: {painter-order draw: (line (x0, y0, x1, y1), offsets (n,),
+             stops (n, 4) straight RGBA tensor)} -- linear-gradient paints, 1-4
+             stops; the stop colours are parameters, the line and offsets are not.
+             Such a draw's entry in `colors` is unused.
     """
 
     def __init__(self, width, height, fills=None, strokes=None, widths=None, colors=None, order=None, rules=None,
-                 clear=(0.0, 0.0, 0.0, 0.0), tolerance=TOLERANCE, clips=None):
+                 clear=(0.0, 0.0, 0.0, 0.0), tolerance=TOLERANCE, clips=None, gradients=None):
         self.width, self.height = int(width), int(height)
         self.fills = fills
         self.strokes = strokes
@@ -101,6 +105,7 @@ class DiffScene:
         self.clear = tuple(float(c) for c in clear)
         self.tolerance = float(tolerance)
         self.clips = [tuple(c) for c in (clips or [])]
+        self.gradients = dict(gradients or {})
 
     # --- geometry -------------------------------------------------------------------
 
@@ -144,6 +149,30 @@ class DiffScene:
     def premultiplied(self, colors=None):
         c = self.colors if colors is None else colors
         return torch.cat([c[:, :3] * c[:, 3:4], c[:, 3:4]], 1)
+
+    def paint_records(self):
+        """(draws, 32) paint records in painter order (the layout tk_frame_render_diff
+        reads), differentiable in the colours and the gradient stops."""
+        pm = self.premultiplied()
+        n = pm.shape[0]
+        head = np.zeros((n, 16), np.float64)
+        head[:, 5] = 1.0
+        stops = torch.cat([pm, pm.new_zeros((n, 12))], 1)
+        if self.gradients:
+            rows, blocks = [], []
+            for d, (line, offsets, st) in sorted(self.gradients.items()):
+                k = len(offsets)
+                if not 1 <= k <= 4 or st.shape[0] != k:
+                    raise ValueError("a linear gradient has 1-4 stops, one colour each")
+                head[d, 0] = 1.0
+                head[d, 1:5] = line
+                head[d, 5] = k
+                head[d, 6:6 + k] = offsets
+                spm = torch.cat([st[:, :3] * st[:, 3:4], st[:, 3:4]], 1).reshape(-1)
+                blocks.append(torch.cat([spm, spm.new_zeros(16 - spm.numel())]))
+                rows.append(d)
+            stops = stops.index_put((torch.as_tensor(rows, device=pm.device),), torch.stack(blocks).to(pm.dtype))
+        return torch.cat([torch.as_tensor(head, dtype=pm.dtype, device=pm.device), stops], 1)
 
 
 def _fill_cubics(fills):
@@ -411,12 +440,12 @@ class Plan:
             self._lb_t = torch.as_tensor(self.line_b, device=device)
         return self._la_t, self._lb_t
 
-    def full_colors(self, premultiplied):
-        """One colour per encoded draw (clip draws get zeros) from the painter-order
-        path-draw colours; differentiable."""
-        idx = torch.as_tensor(self.draw_index, device=premultiplied.device)
-        full = torch.zeros((self.draw_count, 4), dtype=premultiplied.dtype, device=premultiplied.device)
-        return full.index_copy(0, idx, premultiplied)
+    def full_paint(self, records):
+        """One 32-float paint record per encoded draw (clip draws get zeros) from the
+        painter-order path-draw records (DiffScene.paint_records); differentiable."""
+        idx = torch.as_tensor(self.draw_index, device=records.device)
+        full = torch.zeros((self.draw_count, 32), dtype=records.dtype, device=records.device)
+        return full.index_copy(0, idx, records)
 
     def images(self, flat):
         out = []
@@ -442,7 +471,7 @@ class Plan:
         grad = grad_flat.detach().to(torch.float32).contiguous()
         dev = grad.device
         line_grad = torch.empty((max(self.line_count, 1), 4), dtype=torch.float32, device=dev)
-        color_grad = torch.empty((max(self.draw_count, 1), 4), dtype=torch.float32, device=dev)
+        color_grad = torch.empty((max(self.draw_count, 1), 32), dtype=torch.float32, device=dev)
         torch.cuda.synchronize(dev)
         check(lib().tk_frame_backward(self.frame._h, C.c_void_p(self._colors.data_ptr()), C.c_void_p(grad.data_ptr()),
                                       C.c_void_p(line_grad.data_ptr()), C.c_void_p(color_grad.data_ptr())))
@@ -485,7 +514,7 @@ def render(context, scenes, device="cuda"):
     verts = [v for v, _ in polys]
     counts = [k for _, k in polys]
     plan = Plan(context, scenes, verts, counts)
-    flat = _Render.apply(torch.cat(verts), plan.full_colors(torch.cat([s.premultiplied() for s in scenes])), plan,
+    flat = _Render.apply(torch.cat(verts), plan.full_paint(torch.cat([s.paint_records() for s in scenes])), plan,
                          torch.device(device))
     images = plan.images(flat)
     return images[0] if single else images
@@ -540,6 +569,24 @@ def wave_strokes(rng, count, width, height, cubics=2):
                 pts.append(pts[-1] + step[j])
         out[s] = np.array(pts)
     return out
+
+
+def random_gradients(scene, fraction, seed):
+    """Gives a `fraction` of the scene's draws linear-gradient paints of 2-4 stops:
+    a random line of 16-96 px, sorted offsets, opaque-ish stop colours."""
+    rng = np.random.default_rng(seed)
+    n = len(scene.order)
+    for d in np.nonzero(rng.uniform(0, 1, n) < fraction)[0]:
+        k = int(rng.integers(2, 5))
+        x0, y0 = rng.uniform(0, scene.width), rng.uniform(0, scene.height)
+        ang, length = rng.uniform(0, 2 * math.pi), rng.uniform(16, 96)
+        line = (x0, y0, x0 + length * math.cos(ang), y0 + length * math.sin(ang))
+        offsets = np.sort(rng.uniform(0, 1, k))
+        offsets[0], offsets[-1] = 0.0, 1.0
+        stops = np.concatenate([rng.uniform(0, 1, (k, 3)), rng.uniform(0.3, 1.0, (k, 1))], 1)
+        scene.gradients[int(d)] = (line, offsets, torch.tensor(stops, dtype=scene.colors.dtype,
+                                                              device=scene.colors.device))
+    return scene
 
 
 def random_scene(seed, width, height, fills, strokes=0, cubics=4, dtype=torch.float64, device="cpu",
