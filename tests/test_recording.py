@@ -279,3 +279,20 @@ def test_invalid_stroke_styles_are_rejected():
         c.push_stroke(path, 4.0, tk.Color(0, 0, 0, 1), join=7)
     with pytest.raises(tk.TileinkError):
         c.push_stroke(path, 4.0, tk.Color(0, 0, 0, 1), join=tk.Join.Miter, miter_limit=0.5)
+
+
+def test_bulk_polygons_record_what_single_polygons_record():
+    rng = np.random.default_rng(21)
+    polys = [rng.uniform(5, 120, (int(rng.integers(3, 12)), 2)) for _ in range(9)]
+    colors = rng.uniform(0, 1, (9, 4)).astype(np.float32)
+    rules = rng.integers(0, 2, 9)
+    a, b = tk.Canvas(128, 128), tk.Canvas(128, 128)
+    for p, c, r in zip(polys, colors, rules):
+        a.push_polygon(p, tk.Color(*c), tk.FillRule(int(r)))
+    ids = b.push_polygons(np.concatenate(polys), [len(p) for p in polys], colors, rules)
+    assert list(ids) == list(range(9))
+    sa, sb = a.arrays(0), b.arrays(0)
+    for name in ("lines", "paths", "draws", "paint", "tile_draw_indices"):
+        assert np.array_equal(np.asarray(getattr(sa, name)), np.asarray(getattr(sb, name))), name
+    with pytest.raises(tk.TileinkError):
+        b.push_polygons(np.zeros((4, 2)), [3], colors[:1], rules[:1])  # counts disagree with the points

@@ -24,6 +24,8 @@ layers), and the plain renderer's u8 quantization is not part of this form.
 """
 import ctypes as C
 import math
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -52,11 +54,10 @@ def _subdivisions(cubics, tolerance):
 def _flatten(cubics, steps, closed_ends):
     """Points of cubics (M, 4, 2) at t = k/n, k < n, plus t = 1 where closed_ends
     marks a chain's last cubic (open chains end on their last point)."""
-    ci = np.repeat(np.arange(len(steps)), steps + closed_ends.astype(np.int64))
-    k = np.concatenate([np.arange(n + e) for n, e in zip(steps, closed_ends.astype(np.int64))]) if len(steps) else \
-        np.zeros(0, np.int64)
-    t = torch.as_tensor(k / np.repeat(steps, steps + closed_ends.astype(np.int64)), dtype=cubics.dtype,
-                        device=cubics.device)
+    m = steps + closed_ends.astype(np.int64)  # points per cubic
+    ci = np.repeat(np.arange(len(steps)), m)
+    k = np.arange(int(m.sum())) - np.repeat(np.cumsum(m) - m, m)
+    t = torch.as_tensor(k / np.repeat(steps, m), dtype=cubics.dtype, device=cubics.device)
     w = _bernstein(t)
     sel = cubics[torch.as_tensor(ci, device=cubics.device)]
     return (w[:, :, None] * sel).sum(1)
@@ -126,21 +127,36 @@ class DiffScene:
         """(vertices (V, 2), vertices per draw in painter order) as torch tensors of
         the parameters' dtype; differentiable in the parameters."""
         st = st or self.structure()
-        fill_polys, stroke_polys = [], []
+        dtype = (self.fills if self.fills is not None else self.strokes).dtype
+        device = (self.fills if self.fills is not None else self.strokes).device
+        parts, part_counts = [], [np.zeros(0, np.int64), np.zeros(0, np.int64)]
         if st.fill_steps is not None:
             cub = _fill_cubics(self.fills)
-            v = _flatten(cub, st.fill_steps, np.zeros(len(st.fill_steps), bool))
-            counts = st.fill_steps.reshape(self.fills.shape[0], -1).sum(1)
-            fill_polys = list(torch.split(v, counts.tolist()))
+            parts.append(_flatten(cub, st.fill_steps, np.zeros(len(st.fill_steps), bool)))
+            part_counts[0] = st.fill_steps.reshape(self.fills.shape[0], -1).sum(1)
         if st.stroke_steps is not None:
             cub = _stroke_cubics(self.strokes)
             q = _flatten(cub, st.stroke_steps, _chain_ends(self.strokes))
-            stroke_polys = _stroke_outlines(q, st.stroke_keep, st.recipe, st.stroke_kept, 0.5 * self.widths)
-        polys = [fill_polys[i] if kind == 0 else stroke_polys[i] for kind, i in self.order]
-        counts = np.array([len(p) for p in polys], np.int64)
-        dtype = (self.fills if self.fills is not None else self.strokes).dtype
-        device = (self.fills if self.fills is not None else self.strokes).device
-        verts = torch.cat(polys) if polys else torch.zeros((0, 2), dtype=dtype, device=device)
+            outlines = _stroke_outlines(q, st.stroke_keep, st.recipe, st.stroke_kept, 0.5 * self.widths)
+            parts.append(torch.cat(outlines) if outlines else torch.zeros((0, 2), dtype=dtype, device=device))
+            part_counts[1] = np.array([len(r) for r in st.recipe], np.int64)
+        if not parts:
+            return torch.zeros((0, 2), dtype=dtype, device=device), np.zeros(0, np.int64)
+        # painter order: one gather over fills then strokes laid end to end
+        allv = torch.cat(parts) if len(parts) > 1 else parts[0]
+        fill_vertices = int(part_counts[0].sum())
+        first_of = [np.cumsum(part_counts[0]) - part_counts[0],
+                    fill_vertices + np.cumsum(part_counts[1]) - part_counts[1]]
+        kinds = np.array([kd for kd, _ in self.order], np.int64)
+        idx = np.array([i for _, i in self.order], np.int64)
+        counts = np.zeros(len(idx), np.int64)
+        first = np.zeros(len(idx), np.int64)
+        for kind in (0, 1):
+            sel = kinds == kind
+            counts[sel] = part_counts[kind][idx[sel]]
+            first[sel] = first_of[kind][idx[sel]]
+        gather = np.arange(int(counts.sum())) - np.repeat(np.cumsum(counts) - counts, counts) + np.repeat(first, counts)
+        verts = allv.index_select(0, torch.as_tensor(gather, device=allv.device))
         return verts, counts
 
     def draw_rules(self):
@@ -403,8 +419,15 @@ class Plan:
         colors_of = colors or [None] * len(scenes)
         draw_index, d_off = [], 0
         self.encodings = []
-        for s, v, k, col in zip(scenes, verts, counts, colors_of):
-            e = encode(s, v, k, col)
+        jobs = list(zip(scenes, verts, counts, colors_of))
+        if len(jobs) > 1:
+            # Scenes record on their own canvases; the native calls release the
+            # interpreter lock, so the recordings run side by side.
+            with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as pool:
+                encodings = list(pool.map(lambda job: encode(*job), jobs))
+        else:
+            encodings = [encode(*job) for job in jobs]
+        for (s, v, k, col), e in zip(jobs, encodings):
             self.encodings.append(e)
             self.canvases.append(e.canvas)
             self.encoded.append(e.scene)
