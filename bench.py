@@ -156,14 +156,15 @@ def scene_loop(ctx, count=64, repeats=5):
     return out
 
 
-def diff_batch(ctx, count=256, repeats=3):
+def diff_batch(ctx, count=256, repeats=3, gradients=0.1, clips=True):
     """Forward + backward of the differentiable form over many scenes: 256 scenes of
     256x256, each with 64-512 closed paths of 4 cubics under one rounded-rectangle
-    clip (inset 4-24 px, radius 4-16 px), 10 % of the paints linear gradients of
-    2-4 stops and the rest solid, loss = squared error against a random target.  A step renders every scene in FP32
+    clip (inset 4-24 px, radius 4-16 px), 10 % of the paints gradients of 2-4 stops
+    (half linear, half radial) and the rest solid, loss = squared error against a random target.  A step renders every scene in FP32
     (tk_frame_render_diff) and back-propagates dL/d(image) to line endpoints and
     colours (tk_frame_backward), from scenes already resident in HBM.  Timed once as
-    a per-scene loop (one frame per scene) and once as one batch."""
+    a per-scene loop (one frame per scene) and once as one batch.  `gradients`
+    (the share of gradient paints) and `clips` vary the workload for comparison."""
     import torch
     from tileink_amd import diff
     rng = np.random.default_rng(5)
@@ -172,8 +173,9 @@ def diff_batch(ctx, count=256, repeats=3):
           for i, n in enumerate(paths)]
     for i, (s, n) in enumerate(zip(sc, paths)):
         inset = rng.uniform(4, 24, 4)
-        s.clips = [(inset[0], inset[1], 256 - inset[2], 256 - inset[3], float(rng.uniform(4, 16)), 0, int(n))]
-        diff.random_gradients(s, 0.1, 5000 + i)
+        if clips:
+            s.clips = [(inset[0], inset[1], 256 - inset[2], 256 - inset[3], float(rng.uniform(4, 16)), 0, int(n))]
+        diff.random_gradients(s, gradients, 5000 + i, radial=0.5)
     polys = [s.polygons() for s in sc]
     singles = [diff.Plan(ctx, [s], [v], [k]) for s, (v, k) in zip(sc, polys)]
     batch = diff.Plan(ctx, sc, [v for v, _ in polys], [k for _, k in polys])
@@ -190,7 +192,8 @@ def diff_batch(ctx, count=256, repeats=3):
         plan.backward(2.0 * (image - t))
 
     out = {"scenes": count, "size": "256x256", "paths": int(paths.sum()), "lines": int(batch.line_count),
-           "clips_per_scene": 1, "gradient_paints": int(sum(len(s.gradients) for s in sc))}
+           "clips_per_scene": int(bool(clips)), "gradient_paints": int(sum(len(s.gradients) for s in sc)),
+           "radial_paints": int(sum(diff.gradient_parts(g)[3] == "radial" for s in sc for g in s.gradients.values()))}
     for name, run in (("loop", lambda: [step(p, c, t) for p, c, t in zip(singles, colors, targets)]),
                       ("batch", lambda: step(batch, color_b, target_b))):
         run()  # warm-up: sizes the coverage buffers
@@ -227,10 +230,11 @@ def fit(ctx, count=256, steps=20, lr=0.05):
         inset = rng.uniform(4, 24, 4)
         clip = [(inset[0], inset[1], 256 - inset[2], 256 - inset[3], float(rng.uniform(4, 16)), 0, int(n))]
         t.clips, s.clips = clip, list(clip)
-        diff.random_gradients(t, 0.1, 7000 + i)
-        diff.random_gradients(s, 0.1, 7000 + i)
+        diff.random_gradients(t, 0.1, 7000 + i, radial=0.5)
+        diff.random_gradients(s, 0.1, 7000 + i, radial=0.5)
         with torch.no_grad():
-            for d, (line, offsets, stops) in t.gradients.items():
+            for g in t.gradients.values():
+                stops = g[2]
                 stops += torch.as_tensor(rng.normal(0, 0.05, tuple(stops.shape)), dtype=torch.float32)
                 stops.clamp_(0, 1)
         with torch.no_grad():

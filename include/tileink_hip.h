@@ -369,11 +369,13 @@ tk_status tk_frame_autotune(tk_frame *frame, uint32_t repeats, tk_tuning *chosen
  * trips between batches left out: an FP32 premultiplied source-over composite of
  * the scene's FILL records.  Scenes: path draws (fills and strokes) inside SDF
  * clip layers.  Each draw's paint is paint[32 * draw ..] (FP32), not its brush:
- *   {kind: 0 solid, 1 linear gradient; x0, y0, x1, y1: the gradient line in
- *    pixels; n: stop count 1-4; 4 stop offsets; 6 unused; 4 stops of
- *    premultiplied RGBA},
- * a solid paint being its first stop; a linear gradient clamps the projection
- * of the pixel centre on its line to [0, 1] and interpolates the stops linearly.
+ *   {kind: 0 solid, 1 linear gradient, 2 radial gradient; x0, y0, x1, y1: the
+ *    gradient line in pixels; n: stop count 1-4; 4 stop offsets; 6 unused;
+ *    4 stops of premultiplied RGBA},
+ * a solid paint being its first stop; a linear gradient takes t = the
+ * projection of the pixel centre on its line, a radial one t = the centre's
+ * distance from (x0, y0) over the line's length; t is clamped to [0, 1] and the
+ * stops are interpolated linearly.
  * All float pointers are device pointers.
  * tk_frame_render_diff: the whole pipeline; writes `image` (4 floats per pixel,
  *   row-major; a batch's scenes at their image offsets, one after another).

@@ -36,15 +36,29 @@ def clip_masks(sc, verts, counts):
     return masks
 
 
-def gradient_weights(line, offsets, width, height):
-    """Each stop's weight (H, W, n) in a linear-gradient paint at the pixel centres:
-    the projection on the line clamped to [0, 1], linear between stops, constant
-    beyond the end stops."""
+def gradient_weights(line, offsets, width, height, kind="linear"):
+    """Each stop's weight (H, W, n) in a gradient paint at the pixel centres: t =
+    the projection on the line (linear) or the distance from its start over its
+    length (radial), clamped to [0, 1], or the angle about its start from its
+    direction over a full turn (sweep); linear between stops, constant beyond the
+    end stops.  A four-corner paint is bilinear over the box the line spans."""
     x0, y0, x1, y1 = line
     X, Y = np.meshgrid(np.arange(width) + 0.5, np.arange(height) + 0.5)
     dx, dy = x1 - x0, y1 - y0
     den = dx * dx + dy * dy
-    t = np.clip(((X - x0) * dx + (Y - y0) * dy) / den, 0.0, 1.0) if den > 0 else np.zeros_like(X)
+    if kind == "four_corner":
+        u = np.clip((X - x0) / dx, 0.0, 1.0) if dx else np.zeros_like(X)
+        v = np.clip((Y - y0) / dy, 0.0, 1.0) if dy else np.zeros_like(X)
+        return np.stack([(1 - u) * (1 - v), u * (1 - v), (1 - u) * v, u * v], -1)
+    if kind == "sweep":
+        start = math.atan2(dy, dx) if den > 0 else 0.0
+        t = np.mod((np.arctan2(Y - y0, X - x0) - start) / (2 * math.pi), 1.0)
+    elif den <= 0:
+        t = np.zeros_like(X)
+    elif kind == "radial":
+        t = np.clip(np.hypot(X - x0, Y - y0) / math.sqrt(den), 0.0, 1.0)
+    else:
+        t = np.clip(((X - x0) * dx + (Y - y0) * dy) / den, 0.0, 1.0)
     o = list(offsets)
     n = len(o)
     w = np.zeros(t.shape + (n,))
@@ -68,10 +82,11 @@ def paint_colors(sc):
     with torch.no_grad():
         pm = sc.premultiplied().detach().cpu().double().numpy()
         out = list(pm)
-        for d, (line, offsets, stops) in sc.gradients.items():
+        for d, entry in sc.gradients.items():
+            line, offsets, stops, kind = diff.gradient_parts(entry)
             st = stops.detach().cpu().double().numpy()
             spm = np.concatenate([st[:, :3] * st[:, 3:4], st[:, 3:4]], 1)
-            out[d] = gradient_weights(line, offsets, sc.width, sc.height) @ spm
+            out[d] = gradient_weights(line, offsets, sc.width, sc.height, kind) @ spm
     return out
 
 
@@ -95,9 +110,10 @@ def analytic_grads(sc, w):
     gv.index_add_(0, torch.as_tensor(lb), lgt[:, 2:])
     cg = np.stack([m.sum((0, 1)) for m in maps]) if maps else np.zeros((0, 4))
     tensors, grads = [v, pm], [gv, None]
-    for d, (line, offsets, stops) in sc.gradients.items():
+    for d, entry in sc.gradients.items():
+        line, offsets, stops, kind = diff.gradient_parts(entry)
         cg[d] = 0.0  # the draw's solid colour is unused
-        wts = gradient_weights(line, offsets, sc.width, sc.height)
+        wts = gradient_weights(line, offsets, sc.width, sc.height, kind)
         spm = torch.cat([stops[:, :3] * stops[:, 3:4], stops[:, 3:4]], 1)
         tensors.append(spm)
         grads.append(torch.as_tensor(np.einsum("hwn,hwc->nc", wts, maps[d]), dtype=spm.dtype))
@@ -293,17 +309,21 @@ def test_masked_oracle_equals_plain_oracle_under_unit_masks():
     assert np.allclose(la, lb, atol=1e-10) and np.allclose(ca, cb, atol=1e-10)
 
 
-def add_gradients(sc, draws, seed):
+def add_gradients(sc, draws, seed, kinds=None):
+    """Gradient paints of 2-4 stops on `draws`, of kinds[draw] (linear when not
+    given; a four-corner paint has 4)."""
     rng = np.random.default_rng(seed)
+    kinds = kinds or {}
     for d in draws:
-        n = int(rng.integers(2, 5))
+        kind = kinds.get(d, "linear")
+        n = 4 if kind == "four_corner" else int(rng.integers(2, 5))
         x0, y0 = rng.uniform(0, sc.width), rng.uniform(0, sc.height)
         line = (x0, y0, x0 + rng.uniform(-60, 60), y0 + rng.uniform(-60, 60))
         offsets = np.sort(rng.uniform(0, 1, n))
         offsets[0] = min(offsets[0], 0.2)
         stops = torch.tensor(np.concatenate([rng.uniform(0, 1, (n, 3)), rng.uniform(0.4, 1.0, (n, 1))], 1),
                              dtype=sc.colors.dtype)
-        sc.gradients[d] = (line, offsets, stops)
+        sc.gradients[d] = (line, offsets, stops, kind)
 
 
 def test_gradient_weights_reproduce_the_stops_and_interpolate():
@@ -313,6 +333,61 @@ def test_gradient_weights_reproduce_the_stops_and_interpolate():
     x = np.arange(64) + 0.5
     t = np.clip((x - 10) / 40, 0, 1)
     assert np.allclose(w[0, :, 1], np.where(t < 0.5, t / 0.5, (1 - t) / 0.5))  # the middle stop's tent
+
+
+def test_radial_gradient_weights_on_integer_distances():
+    # Centre (0.5, 0.5) at a pixel centre, radius 10: the pixel 3 right and 4 down
+    # lies at distance 5 (t = 0.5) and takes the middle stop alone; 6 right and 8
+    # down lies at 10 (t = 1) and takes the last; 2 along either axis (t = 0.2) is
+    # 2/5 of the way from the first stop to the second; past the radius the end pads.
+    w = gradient_weights((0.5, 0.5, 6.5, 8.5), [0.0, 0.5, 1.0], 16, 16, "radial")
+    assert np.allclose(w.sum(-1), 1.0)
+    assert np.allclose(w[4, 3], [0, 1, 0]) and np.allclose(w[8, 6], [0, 0, 1])
+    assert np.allclose(w[0, 0], [1, 0, 0])
+    assert np.allclose(w[0, 2], [1 - 2 / 5, 2 / 5, 0]) and np.allclose(w[2, 0], w[0, 2])
+    assert np.allclose(w[15, 15], [0, 0, 1])
+    # a radial paint is symmetric about its centre, unlike a linear one
+    wl = gradient_weights((0.5, 0.5, 6.5, 8.5), [0.0, 0.5, 1.0], 16, 16)
+    assert np.allclose(w[5, 2], w[2, 5]) and not np.allclose(wl[5, 2], wl[2, 5])
+
+
+def test_gradient_kinds_in_paint_records():
+    sc = diff.random_scene(3, 32, 32, fills=3)
+    add_gradients(sc, [0, 2], 3, kinds={2: "radial"})
+    rec = sc.paint_records().detach().numpy()
+    assert rec[0, 0] == 1.0 and rec[1, 0] == 0.0 and rec[2, 0] == 2.0
+    sc.gradients[1] = (*sc.gradients[0][:3], "conic")
+    with pytest.raises(ValueError):
+        sc.paint_records()
+    sc.gradients[1] = (*sc.gradients[0][:3], "four_corner")  # needs 4 stops
+    if len(sc.gradients[0][1]) != 4:
+        with pytest.raises(ValueError):
+            sc.paint_records()
+
+
+def test_sweep_and_four_corner_weights_closed_forms():
+    # Sweep about (8.5, 8.5) from +x: the pixel centre straight below (+y) lies a
+    # quarter turn on (t = 0.25), the one to the left half a turn, straight above
+    # three quarters; from a start pointing along +y everything turns back by 0.25.
+    w = gradient_weights((8.5, 8.5, 20.0, 8.5), [0.0, 1.0], 17, 17, "sweep")
+    assert np.allclose(w[12, 8], [0.75, 0.25]) and np.allclose(w[8, 2], [0.5, 0.5])
+    assert np.allclose(w[2, 8], [0.25, 0.75]) and np.allclose(w[8, 12], [1.0, 0.0])
+    w = gradient_weights((8.5, 8.5, 8.5, 30.0), [0.0, 1.0], 17, 17, "sweep")
+    assert np.allclose(w[12, 8], [1.0, 0.0]) and np.allclose(w[8, 2], [0.75, 0.25])
+    # Four corners over the box (0.5, 0.5)-(8.5, 4.5): each corner pixel takes its
+    # stop alone; the box centre an equal quarter of each; beyond, the edges pad.
+    w = gradient_weights((0.5, 0.5, 8.5, 4.5), None, 12, 8, "four_corner")
+    assert np.allclose(w[0, 0], [1, 0, 0, 0]) and np.allclose(w[0, 8], [0, 1, 0, 0])
+    assert np.allclose(w[4, 0], [0, 0, 1, 0]) and np.allclose(w[4, 8], [0, 0, 0, 1])
+    assert np.allclose(w[2, 4], 0.25) and np.allclose(w[7, 11], [0, 0, 0, 1])
+    assert np.allclose(w[1, 2], [0.75 * 0.75, 0.25 * 0.75, 0.75 * 0.25, 0.25 * 0.25])
+
+
+@pytest.mark.parametrize("kind", ["radial", "sweep", "four_corner"])
+def test_gradient_kind_stops_match_finite_differences(kind):
+    sc = diff.random_scene(19, 96, 80, fills=6, strokes=1, cubics=3, radius_mu=math.log(16.0))
+    add_gradients(sc, [0, 2, 4, 6], 19, kinds=dict.fromkeys([0, 2, 4, 6], kind))
+    assert check_fd(sc, weights(80, 96, 19), per_param=10, rtol=1e-4) >= 30
 
 
 def test_linear_gradient_stops_match_finite_differences():

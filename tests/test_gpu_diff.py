@@ -106,12 +106,14 @@ def test_clipped_scenes_match_fp64(ctx, seed):
 
 
 @pytest.mark.parametrize("seed", [17, 18])
-def test_linear_gradient_paints_match_fp64(ctx, seed):
-    # Linear-gradient paints (2-4 stops) on some draws, inside a clip: the image and
-    # the gradients of geometry, colours and stop colours against the FP64 oracle.
+def test_gradient_paints_match_fp64(ctx, seed):
+    # Linear, radial, sweep and four-corner gradient paints (2-4 stops) on some
+    # draws, inside a clip: the image and the gradients of geometry, colours and
+    # stop colours against the FP64 oracle.
     from test_diff import add_gradients
     sc = diff.random_scene(seed, 160, 112, fills=16, strokes=4, radius_mu=math.log(18.0))
-    add_gradients(sc, [0, 3, 5, 8, 13], seed)
+    add_gradients(sc, [0, 3, 5, 8, 11, 13, 16, 18], seed,
+                  kinds={3: "radial", 8: "radial", 5: "sweep", 16: "sweep", 11: "four_corner", 18: "four_corner"})
     sc.clips = [(10.0, 8.0, 150.0, 104.0, 8.0, 2, 18)]
     w = np.random.default_rng(seed).normal(size=(112, 160, 4))
     stops = [g[2] for g in sc.gradients.values()]

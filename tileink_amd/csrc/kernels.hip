@@ -1782,13 +1782,20 @@ struct Transmit {
     }
 };
 
-// A draw's paint in the differentiable form, 32 floats: {kind (0 solid, 1
-// linear gradient), x0, y0, x1, y1 (the gradient line, pixels), stop count n
-// (1-4), 4 stop offsets, 6 unused, 4 stops of premultiplied RGBA}; a solid
-// paint is its first stop.  The linear gradient is evaluated exactly (no 8-bit
-// ramp): t = the projection of the pixel centre on the line, clamped to [0, 1]
-// (pad), the colour piecewise linear between the stops in premultiplied space
-// and constant beyond the end stops.  w holds each stop's weight in the colour.
+// A draw's paint in the differentiable form, 32 floats: {kind, x0, y0, x1, y1
+// (the gradient line, pixels), stop count n (1-4), 4 stop offsets, 6 unused,
+// 4 stops of premultiplied RGBA}.  Kinds: 0 solid (the first stop), 1 linear
+// gradient, 2 radial gradient, 3 sweep gradient, 4 four-corner gradient.
+// Gradients are evaluated exactly (no 8-bit ramp), in premultiplied space.  The
+// first three take a position t and interpolate the stops linearly in t,
+// constant beyond the end stops: t is, for a linear gradient, the projection
+// of the pixel centre on the line, clamped to [0, 1]; for a radial one, the
+// centre's distance from (x0, y0) over the line's length, clamped to [0, 1];
+// for a sweep, the angle from the line's direction to the centre about
+// (x0, y0), turning from +x towards +y, over a full turn, in [0, 1).  A
+// four-corner gradient (n = 4) is bilinear over the box (x0, y0)-(x1, y1) with
+// stops at its corners (x0, y0), (x1, y0), (x0, y1), (x1, y1), the position
+// clamped to the box.  w holds each stop's weight in the colour.
 constexpr uint32_t kDiffPaintFloats = 32u;
 struct DiffPaint {
     float4 c;
@@ -1801,30 +1808,48 @@ __device__ __forceinline__ DiffPaint diff_paint(const float *paint, uint32_t dra
     if (h0.x == 0.0f) return p;
     const float4 h1 = r[1], h2 = r[2];
     const float dx = h0.w - h0.y, dy = h1.x - h0.z, den = dx * dx + dy * dy;
-    float t = den > 0.0f ? ((x - h0.y) * dx + (y - h0.z) * dy) / den : 0.0f;
-    t = fminf(fmaxf(t, 0.0f), 1.0f);
-    const float n = h1.y, o0 = h1.z, o1 = h1.w, o2 = h2.x, o3 = h2.y;
+    const float px = x - h0.y, py = y - h0.z;
     float w0 = 0.0f, w1 = 0.0f, w2 = 0.0f, w3 = 0.0f;
-    if (n <= 1.0f || t <= o0) {
-        w0 = 1.0f;
-    } else if (t < o1) {
-        const float u = (t - o0) / (o1 - o0);
-        w0 = 1.0f - u;
-        w1 = u;
-    } else if (n <= 2.0f) {
-        w1 = 1.0f;
-    } else if (t < o2) {
-        const float u = (t - o1) / (o2 - o1);
-        w1 = 1.0f - u;
-        w2 = u;
-    } else if (n <= 3.0f) {
-        w2 = 1.0f;
-    } else if (t < o3) {
-        const float u = (t - o2) / (o3 - o2);
-        w2 = 1.0f - u;
-        w3 = u;
+    if (h0.x == 4.0f) {
+        const float u = dx != 0.0f ? fminf(fmaxf(px / dx, 0.0f), 1.0f) : 0.0f;
+        const float v = dy != 0.0f ? fminf(fmaxf(py / dy, 0.0f), 1.0f) : 0.0f;
+        w0 = (1.0f - u) * (1.0f - v);
+        w1 = u * (1.0f - v);
+        w2 = (1.0f - u) * v;
+        w3 = u * v;
     } else {
-        w3 = 1.0f;
+        float t = 0.0f;
+        if (h0.x == 3.0f) {
+            const float start = den > 0.0f ? sweep_atan2(dy, dx) : 0.0f;
+            const float angle = px != 0.0f || py != 0.0f ? sweep_atan2(py, px) : 0.0f;
+            t = (angle - start) / kGradientTau;
+            t -= floorf(t);
+        } else if (den > 0.0f) {
+            t = h0.x == 2.0f ? sqrtf((px * px + py * py) / den) : (px * dx + py * dy) / den;
+        }
+        t = fminf(fmaxf(t, 0.0f), 1.0f);
+        const float n = h1.y, o0 = h1.z, o1 = h1.w, o2 = h2.x, o3 = h2.y;
+        if (n <= 1.0f || t <= o0) {
+            w0 = 1.0f;
+        } else if (t < o1) {
+            const float u = (t - o0) / (o1 - o0);
+            w0 = 1.0f - u;
+            w1 = u;
+        } else if (n <= 2.0f) {
+            w1 = 1.0f;
+        } else if (t < o2) {
+            const float u = (t - o1) / (o2 - o1);
+            w1 = 1.0f - u;
+            w2 = u;
+        } else if (n <= 3.0f) {
+            w2 = 1.0f;
+        } else if (t < o3) {
+            const float u = (t - o2) / (o3 - o2);
+            w2 = 1.0f - u;
+            w3 = u;
+        } else {
+            w3 = 1.0f;
+        }
     }
     const float4 s0 = r[4], s1 = r[5], s2 = r[6], s3 = r[7];
     p.c = make_float4(w0 * s0.x + w1 * s1.x + w2 * s2.x + w3 * s3.x, w0 * s0.y + w1 * s1.y + w2 * s2.y + w3 * s3.y,

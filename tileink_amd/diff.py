@@ -87,9 +87,14 @@ class DiffScene:
              rectangle clipping the draws [first, end) of painter order; ranges
              nest or are disjoint.  The clip geometry is not differentiated.
     gradients: {painter-order draw: (line (x0, y0, x1, y1), offsets (n,),
-             stops (n, 4) straight RGBA tensor)} -- linear-gradient paints, 1-4
-             stops; the stop colours are parameters, the line and offsets are not.
-             Such a draw's entry in `colors` is unused.
+             stops (n, 4) straight RGBA tensor[, kind])} -- gradient paints of
+             1-4 stops, kind "linear" (the default: t = the projection on the
+             line), "radial" (t = the distance from the line's start over its
+             length), "sweep" (t = the angle about the line's start from its
+             direction, over a full turn) or "four_corner" (4 stops at the
+             corners of the box the line spans, bilinear; offsets unused); the
+             stop colours are parameters, the line and offsets are not.  Such a
+             draw's entry in `colors` is unused.
     """
 
     def __init__(self, width, height, fills=None, strokes=None, widths=None, colors=None, order=None, rules=None,
@@ -176,11 +181,12 @@ class DiffScene:
         stops = torch.cat([pm, pm.new_zeros((n, 12))], 1)
         if self.gradients:
             rows, blocks = [], []
-            for d, (line, offsets, st) in sorted(self.gradients.items()):
+            for d, entry in sorted(self.gradients.items()):
+                line, offsets, st, kind = gradient_parts(entry)
                 k = len(offsets)
-                if not 1 <= k <= 4 or st.shape[0] != k:
-                    raise ValueError("a linear gradient has 1-4 stops, one colour each")
-                head[d, 0] = 1.0
+                if not 1 <= k <= 4 or st.shape[0] != k or (kind == "four_corner" and k != 4):
+                    raise ValueError("a gradient has 1-4 stops (a four-corner one 4), one colour each")
+                head[d, 0] = GRADIENT_KINDS[kind]
                 head[d, 1:5] = line
                 head[d, 5] = k
                 head[d, 6:6 + k] = offsets
@@ -189,6 +195,19 @@ class DiffScene:
                 rows.append(d)
             stops = stops.index_put((torch.as_tensor(rows, device=pm.device),), torch.stack(blocks).to(pm.dtype))
         return torch.cat([torch.as_tensor(head, dtype=pm.dtype, device=pm.device), stops], 1)
+
+
+GRADIENT_KINDS = {"linear": 1.0, "radial": 2.0, "sweep": 3.0, "four_corner": 4.0}
+
+
+def gradient_parts(entry):
+    """(line, offsets, stops, kind) of a gradient entry, kind "linear" when not given."""
+    if len(entry) == 3:
+        return (*entry, "linear")
+    line, offsets, stops, kind = entry
+    if kind not in GRADIENT_KINDS:
+        raise ValueError(f"unknown gradient kind {kind!r}")
+    return line, offsets, stops, kind
 
 
 def _fill_cubics(fills):
@@ -594,12 +613,14 @@ def wave_strokes(rng, count, width, height, cubics=2):
     return out
 
 
-def random_gradients(scene, fraction, seed):
-    """Gives a `fraction` of the scene's draws linear-gradient paints of 2-4 stops:
-    a random line of 16-96 px, sorted offsets, opaque-ish stop colours."""
+def random_gradients(scene, fraction, seed, radial=0.0):
+    """Gives a `fraction` of the scene's draws gradient paints of 2-4 stops (a
+    `radial` share of them radial, the rest linear): a random line of 16-96 px,
+    sorted offsets, opaque-ish stop colours."""
     rng = np.random.default_rng(seed)
     n = len(scene.order)
     for d in np.nonzero(rng.uniform(0, 1, n) < fraction)[0]:
+        kind = "radial" if rng.uniform(0, 1) < radial else "linear"
         k = int(rng.integers(2, 5))
         x0, y0 = rng.uniform(0, scene.width), rng.uniform(0, scene.height)
         ang, length = rng.uniform(0, 2 * math.pi), rng.uniform(16, 96)
@@ -608,7 +629,7 @@ def random_gradients(scene, fraction, seed):
         offsets[0], offsets[-1] = 0.0, 1.0
         stops = np.concatenate([rng.uniform(0, 1, (k, 3)), rng.uniform(0.3, 1.0, (k, 1))], 1)
         scene.gradients[int(d)] = (line, offsets, torch.tensor(stops, dtype=scene.colors.dtype,
-                                                              device=scene.colors.device))
+                                                              device=scene.colors.device), kind)
     return scene
 
 
