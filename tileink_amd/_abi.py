@@ -273,9 +273,14 @@ def ptr(array):
     return array.ctypes.data
 
 
+_DTYPES = {}
+
+
 def view(address, dtype, count):
     """Copies `count` elements of `dtype` from a host address into a new array."""
-    dtype = np.dtype(dtype)
+    if dtype not in _DTYPES:
+        _DTYPES[dtype] = np.dtype(dtype)  # (a ctypes structure's dtype is slow to build)
+    dtype = _DTYPES[dtype]
     if count == 0 or not address:
         return np.zeros(0, dtype)
     raw = (C.c_uint8 * (dtype.itemsize * count)).from_address(address)
