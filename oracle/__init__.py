@@ -243,6 +243,19 @@ def diff_render(width, height, clear, draws, colors, lines, masks=None):
     return image
 
 
+def path_clip_coverage(points, rule, width, height):
+    """FP64 mask (H, W) of a path clip layer: the exact area coverage of the closed
+    polygon `points` (n, 2) at every pixel, turned into alpha by its fill rule (0
+    NonZero, 1 EvenOdd), as the differentiable form's clip masks are."""
+    p = np.asarray(points, np.float64).reshape(-1, 2)
+    lines = np.concatenate([p, np.roll(p, -1, 0)], 1)
+    box, area = diff_area(lines, 0, len(lines), width, height)
+    full = np.zeros((height, width), np.float64)
+    x0, y0, w, h = (int(b) for b in box)
+    full[y0:y0 + h, x0:x0 + w] = area
+    return _alpha(full, int(rule))[0]
+
+
 def _per_pixel(colors):
     return isinstance(colors, (list, tuple)) and any(np.ndim(c) == 3 for c in colors)
 

@@ -18,19 +18,23 @@ def weights(h, w, seed=0):
 
 
 def clip_masks(sc, verts, counts):
-    """Per painter-order draw, the product of the FP32 coverages of the SDF clip
-    layers around it (None outside every clip), from the oracle's SDF evaluation
-    of the encoded clip draws; None for a scene without clips."""
+    """Per painter-order draw, the product of the coverages of the clip layers
+    around it (None outside every clip): an SDF layer's FP32 coverage from the
+    oracle's SDF evaluation of the encoded clip draw, a path layer's FP64 area
+    coverage of its FP32-rounded polygon under its fill rule; None for a scene
+    without clips."""
     if not sc.clips:
         return None
     e = diff.encode(sc, verts, counts)
     arrays = tk.SceneArrays(e.scene)
-    cov = [oracle.sdf_draw_coverage(arrays, cid).astype(np.float64) for cid in e.clip_ids]
+    cov = [oracle.sdf_draw_coverage(arrays, cid).astype(np.float64) if len(clip) == 7 else
+           oracle.path_clip_coverage(np.asarray(clip[0], np.float32), clip[1], sc.width, sc.height)
+           for cid, clip in zip(e.clip_ids, sc.clips)]
     masks = []
     for d in range(len(counts)):
         m = None
         for c, clip in enumerate(sc.clips):
-            if clip[5] <= d < clip[6]:
+            if clip[-2] <= d < clip[-1]:
                 m = cov[c] if m is None else m * cov[c]
         masks.append(m)
     return masks
@@ -261,6 +265,29 @@ def test_recorded_lines_are_the_polygon_edges():
     sc.clips = [(5.0, 5.0, 100.0, 80.0, 4.0, 2, 7)]
     e = diff.encode(sc, v, k)
     assert e.draw_total == len(k) + 1 and e.clip_ids == [2] and list(e.draw_ids[:3]) == [0, 1, 3]
+    # a path clip's lines are recorded at its place in the stream and map to no vertex
+    tri = np.array([[4.0, 4.0], [100.0, 10.0], [50.0, 76.0]])
+    sc.clips = [(tri, 0, 1, 4)]
+    e = diff.encode(sc, v, k)
+    start = int(e.per_draw[0])
+    assert e.scene.line_count == len(e.line_a) == int(e.per_draw.sum()) + 3
+    assert list(e.line_a[start:start + 3]) == [-1, -1, -1] and (e.line_a[start + 3:] >= 0).all()
+    assert e.draw_total == len(k) + 1 and e.clip_ids == [1]
+
+
+def test_path_clip_masks_closed_forms():
+    # An axis-aligned square clip from (4, 4) to (12.5, 10): whole pixels inside
+    # take 1, the column cut at x = 12.5 takes one half, outside 0.  Traced twice
+    # (winding 2) the inside keeps 1 under NonZero and becomes 0 under EvenOdd; the
+    # cut column's area doubles to 1, which both rules take as 1.
+    sq = np.array([[4.0, 4.0], [12.5, 4.0], [12.5, 10.0], [4.0, 10.0]])
+    m = oracle.path_clip_coverage(sq, 0, 16, 16)
+    assert np.allclose(m[4:10, 4:12], 1.0) and np.allclose(m[4:10, 12], 0.5)
+    assert m.sum() == pytest.approx(8.5 * 6.0) and m[:4].sum() == 0 and m[10:].sum() == 0
+    twice = np.concatenate([sq, sq])
+    assert np.allclose(oracle.path_clip_coverage(twice, 0, 16, 16), np.where(m > 0, 1.0, 0.0))
+    assert np.allclose(oracle.path_clip_coverage(twice, 1, 16, 16)[4:10, 4:12], 0.0)
+    assert np.allclose(oracle.path_clip_coverage(twice, 1, 16, 16)[4:10, 12], 1.0)  # winding 1 over half
 
 
 def test_stroke_outline_width_is_the_stroke_width():

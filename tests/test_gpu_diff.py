@@ -6,9 +6,9 @@ tk_frame_backward) through the torch.autograd harness:
   analytic gradient and against FP64 central finite differences (h = 1e-3 px),
   within 1e-3 relative;
 * a batch of scenes against each scene alone;
-* clipped scenes (SDF clip layers) against the same oracle with the clips'
-  coverage as masks;
-* scenes outside the form (path clip layers) are rejected."""
+* clipped scenes (SDF and path clip layers) against the same oracle with the
+  clips' coverage as masks;
+* scenes outside the form (SDF draws) are rejected."""
 import math
 
 import numpy as np
@@ -83,6 +83,26 @@ def test_gradients_with_opaque_overlaps_and_even_odd(ctx):
     sc.rules[:] = 1
     sc.clear = (0.2, 0.1, 0.0, 0.4)
     w = np.random.default_rng(9).normal(size=(96, 96, 4))
+    image, got = gpu_grads(ctx, sc, w)
+    err = float(np.abs(image.detach().cpu().double().numpy() - oracle_image(sc)).max())
+    assert err < 1e-5, err
+    analytic_grads(sc, w)
+    assert_grads_close(got, [p.grad for p in (sc.fills, sc.strokes, sc.widths, sc.colors)])
+
+
+@pytest.mark.parametrize("seed", [24, 25])
+def test_path_clipped_scenes_match_fp64(ctx, seed):
+    # Path clip layers (closed polygons under either fill rule: the draws' alpha is
+    # multiplied by the clip's FP32 exact-area alpha), nested with an SDF clip.
+    sc = diff.random_scene(seed, 160, 112, fills=20, strokes=6, radius_mu=math.log(18.0))
+    star = np.array([[80 + 50 * math.cos(a) * (1.0 if i % 2 == 0 else 0.45),
+                      56 + 50 * math.sin(a) * (1.0 if i % 2 == 0 else 0.45)]
+                     for i, a in enumerate(np.linspace(0, 2 * math.pi, 11)[:-1] + 0.3)])
+    # a self-overlapping pentagram: its centre has winding 2 (kept by NonZero, a hole under EvenOdd)
+    penta = np.array([[40 + 30 * math.cos(a), 60 + 30 * math.sin(a)]
+                      for a in np.linspace(0, 4 * math.pi, 6)[:-1] + 0.1])
+    sc.clips = [(star, 0, 2, 20), (12.5, 9.25, 140.0, 100.0, 9.0, 6, 14), (penta, 1, 21, 26)]
+    w = np.random.default_rng(seed).normal(size=(112, 160, 4))
     image, got = gpu_grads(ctx, sc, w)
     err = float(np.abs(image.detach().cpu().double().numpy() - oracle_image(sc)).max())
     assert err < 1e-5, err
@@ -178,6 +198,7 @@ def test_batch_equals_each_scene_alone(ctx):
     scenes = [diff.random_scene(20 + i, w, h, fills=f, strokes=s, radius_mu=math.log(12.0))
               for i, (w, h, f, s) in enumerate([(64, 48, 12, 3), (200, 90, 30, 6), (33, 130, 8, 2)])]
     scenes[1].clips = [(20.5, 10.0, 180.0, 80.0, 6.0, 5, 30)]
+    scenes[2].clips = [(np.array([[2.0, 5.0], [31.0, 20.0], [16.5, 120.0]]), 0, 1, 9)]  # a path clip
     ws = [np.random.default_rng(i).normal(size=(s.height, s.width, 4)) for i, s in enumerate(scenes)]
     singles = []
     for sc, w in zip(scenes, ws):
@@ -196,12 +217,11 @@ def test_batch_equals_each_scene_alone(ctx):
         assert_grads_close([p.grad for p in (sc.fills, sc.strokes, sc.widths, sc.colors)], ref_g, rtol=1e-4)
 
 
-def test_scenes_with_path_clip_layers_are_rejected(ctx):
-    # Path clip layers (BEGIN_CLIP records with their own coverage) are not in the form.
+def test_scenes_with_sdf_draws_are_rejected(ctx):
+    # SDF draws (a rounded rectangle painted directly, SDF records) are not in the form.
     canvas = tk.Canvas(64, 64)
-    canvas.push_clip_layer(tk.BezPath().move_to(4, 4).line_to(60, 6).line_to(30, 60).close_path())
     canvas.push_polygon(np.array([[10.0, 10.0], [50.0, 12.0], [30.0, 50.0]]), tk.Color(1, 0, 0, 1))
-    canvas.pop_layer()
+    canvas.push_rect(8.0, 8.0, 40.0, 30.0, 4.0, tk.Color(0, 1, 0, 1))
     scene = canvas.encode(0)
     frame = tk.Frame(ctx, scene)
     colors = torch.ones((scene.draw_count, 32), device=DEV)

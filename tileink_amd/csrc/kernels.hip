@@ -1914,10 +1914,11 @@ struct ClipMask {
     }
 };
 
-// Takes a clip record (BEGIN_SDF_CLIP, END_CLIP) into the mask; false for a
-// record the differentiable form does not take.  The forward evaluates the
-// clip's SDF coverage and keeps it in the record's slot of `adj`; the backward
-// (kKept) reads it from there.
+// Takes a clip record (BEGIN_SDF_CLIP, END_CLIP, and in the backward (kKept)
+// BEGIN_CLIP) into the mask; false for a record the differentiable form does
+// not take.  The forward evaluates an SDF clip's coverage here (a path clip's,
+// which needs the tile's coverage accumulator, in for_each_fill) and keeps it in
+// the record's slot of `adj`; the backward reads it from there.
 template <bool kKept>
 __device__ __forceinline__ bool clip_record(const tk_args_fine_diff &a, const DiffTile &t, uint32_t pi,
                                             const tk_ptcl_record &rec, ClipMask &clip) {
@@ -1925,7 +1926,7 @@ __device__ __forceinline__ bool clip_record(const tk_args_fine_diff &a, const Di
         clip.pop();
         return true;
     }
-    if (rec.tag != TK_PTCL_BEGIN_SDF_CLIP) return false;
+    if (rec.tag != TK_PTCL_BEGIN_SDF_CLIP && !(kKept && rec.tag == TK_PTCL_BEGIN_CLIP)) return false;
     const size_t slot = (size_t)pi * 256u + t.lane;
     float coverage = 0.0f;
     if constexpr (kKept) {
@@ -1940,9 +1941,12 @@ __device__ __forceinline__ bool clip_record(const tk_args_fine_diff &a, const Di
 
 // Calls body(pi, rec, mask) for every FILL record of the tile's streams, bottom
 // to top, with the pixel's clip mask; flags any record kind it does not take.
-// kClips: the scene has SDF clip layers (their records are taken into the mask).
-template <bool kClips, class Body>
-__device__ __forceinline__ void for_each_fill(const tk_args_fine_diff &a, const DiffTile &t, Body body) {
+// kClips: the scene has clip layers (their records are taken into the mask; a
+// path clip's coverage, its alpha under its fill rule, comes from
+// path_clip(pi, rec), called by the whole workgroup).
+template <bool kClips, class Body, class PathClip>
+__device__ __forceinline__ void for_each_fill(const tk_args_fine_diff &a, const DiffTile &t, Body body,
+                                              PathClip path_clip) {
     const tk_fine_config &cfg = a.cfg;
     for (uint32_t b = t.v.q0; b < t.v.q1; ++b) {
         const size_t idx = table_index(t.v, b);
@@ -1956,6 +1960,8 @@ __device__ __forceinline__ void for_each_fill(const tk_args_fine_diff &a, const 
             if (rec.tag == TK_PTCL_END) break;
             if (rec.tag == TK_PTCL_FILL && rec.color < cfg.draw_count) {
                 body(pi, rec, clip.m);
+            } else if (kClips && rec.tag == TK_PTCL_BEGIN_CLIP) {
+                if (!clip.push(path_clip(pi, rec)) && t.lane == 0u && a.status) atomicOr(a.status, 1u);
             } else if (!kClips || !clip_record<false>(a, t, pi, rec, clip)) {
                 if (t.lane == 0u && a.status) atomicOr(a.status, 1u);
             }
@@ -1984,6 +1990,20 @@ __device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &
     bool overflow = false;
     Transmit total{1.0f, 0, 0};
     const uint32_t tx = t.tile % t.v.tw, ty = t.tile / t.v.tw;
+    // A path clip's mask: its coverage turned into alpha by its fill rule, kept
+    // in the record's slot of `adj` for the backward.
+    auto path_clip = [&](uint32_t pi, const tk_ptcl_record &rec) {
+        const uint32_t s0 = rec.segment_start, s1 = min(rec.segment_end, a.cfg.segment_capacity);
+        CoverageAcc &acc = cov_acc[sel];
+        sel ^= 1u;
+        accumulate_record(a.in.segments + s0, s1 > s0 ? s1 - s0 : 0u, acc, lane);
+        __syncthreads();
+        float slope;
+        const float m = diff_alpha(take_coverage(acc, rec.backdrop, lane), rec.fill_rule, slope);
+        if (pi < a.adj_records) a.adj[(size_t)pi * 256u + lane] = m;
+        else overflow = true;
+        return m;
+    };
     for_each_fill<kClips>(a, t, [&](uint32_t pi, const tk_ptcl_record &rec, float mask) {
         const uint32_t s0 = rec.segment_start, s1 = min(rec.segment_end, a.cfg.segment_capacity);
         const float4 c = diff_paint(a.colors, rec.color, sx, sy).c;
@@ -2011,7 +2031,7 @@ __device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &
                 if (bd < a.rec_of_count) a.rec_of[bd] = pi;
             }
         }
-    });
+    }, path_clip);
     if (overflow && lane == 0u && a.status) atomicOr(a.status, 2u);
     reinterpret_cast<uint4 *>(a.trans)[(size_t)t.column * 256u + lane] =
         make_uint4(__float_as_uint(total.m), (uint32_t)total.e, (uint32_t)total.zeros, 0u);

@@ -404,18 +404,19 @@ uint32_t layer_pairs(const tk_scene *s, std::vector<uint32_t> &pair_of, std::vec
 constexpr uint64_t kMaxLayerClassBytes = 64ull << 20;
 
 // The differentiable form takes scenes of path draws (fills and strokes) inside
-// any nesting of SDF (rounded-rectangle) clip layers: every PTCL record is then a
-// FILL carrying its draw index, a BEGIN_SDF_CLIP or an END_CLIP.
+// any nesting of clip layers, SDF (rounded-rectangle) or path: every PTCL record
+// is then a FILL carrying its draw index, a BEGIN_SDF_CLIP, a BEGIN_CLIP or an
+// END_CLIP.
 bool diff_compatible(const tk_scene *s) {
     for (uint32_t l = 0; l < s->layer_count; ++l) {
         const tk_layer_record &layer = s->layers[l];
         if (layer.tag != 0u || layer.draw >= s->draw_count) return false;
         const tk_draw_record &r = s->draws[layer.draw];
-        if (r.tag != TK_DRAW_CLIP || r.sdf_offset == TK_INVALID_INDEX) return false;
+        if (r.tag != TK_DRAW_CLIP) return false;
     }
     for (uint32_t d = 0; d < s->draw_count; ++d) {
         const tk_draw_record &r = s->draws[d];
-        if (r.tag == TK_DRAW_CLIP && r.sdf_offset != TK_INVALID_INDEX) continue;
+        if (r.tag == TK_DRAW_CLIP && (r.sdf_offset != TK_INVALID_INDEX || r.path_id < s->path_count)) continue;
         if (r.tag != TK_DRAW_BRUSH || r.path_id >= s->path_count || r.sdf_offset != TK_INVALID_INDEX ||
             r.sdf_shadow_offset != TK_INVALID_INDEX)
             return false;

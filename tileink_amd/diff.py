@@ -83,9 +83,11 @@ class DiffScene:
     order:   painter order as (kind, index) pairs, kind 0 fill / 1 stroke
              (default: all fills, then all strokes).
     rules:   fill rule per fill (0 NonZero, 1 EvenOdd); strokes use NonZero.
-    clips:   SDF clip layers as (x0, y0, x1, y1, radius, first, end): a rounded
-             rectangle clipping the draws [first, end) of painter order; ranges
-             nest or are disjoint.  The clip geometry is not differentiated.
+    clips:   clip layers around the draws [first, end) of painter order (ranges
+             nest or are disjoint): SDF layers as (x0, y0, x1, y1, radius, first,
+             end), a rounded rectangle; path layers as (points (n, 2), rule,
+             first, end), a closed polygon under fill rule `rule`.  The clip
+             geometry is not differentiated.
     gradients: {painter-order draw: (line (x0, y0, x1, y1), offsets (n,),
              stops (n, 4) straight RGBA tensor[, kind])} -- gradient paints of
              1-4 stops, kind "linear" (the default: t = the projection on the
@@ -376,24 +378,43 @@ def encode(scene, verts, counts, colors=None):
     canvas = tk.Canvas(scene.width, scene.height)
     rules = scene.draw_rules()
     col = (scene.colors if colors is None else colors).detach().cpu().double().numpy()
+    clips = scene.clips
     # outer clips first where several open at one draw, inner first where several close
-    opens = sorted(range(len(scene.clips)), key=lambda i: (scene.clips[i][5], -scene.clips[i][6]))
+    opens = sorted(range(len(clips)), key=lambda i: (clips[i][-2], -clips[i][-1]))
     counts = np.asarray(counts, np.int64)
     offsets = np.concatenate([[0], np.cumsum(counts)])
     colors32 = np.clip(col, 0.0, 1.0).astype(np.float32)
+    la, lb, per_draw = edge_map(v32, counts)
+    line_of = np.concatenate([[0], np.cumsum(per_draw)])
+    # The expected lines in recording order, as (p0|p1 rows, line -> vertex maps);
+    # a path clip's lines map to no vertex (-1).
+    pieces = []
     # runs of draws between clip boundaries go down in one call each
-    bounds = sorted({0, len(counts)} | {int(c[5]) for c in scene.clips} | {int(c[6]) for c in scene.clips})
+    bounds = sorted({0, len(counts)} | {int(c[-2]) for c in clips} | {int(c[-1]) for c in clips})
     draw_ids, o, open_stack = [], 0, []
     for lo, hi in zip(bounds[:-1], bounds[1:]):
-        while o < len(opens) and scene.clips[opens[o]][5] == lo:
-            x0, y0, x1, y1, radius = scene.clips[opens[o]][:5]
-            canvas.push_clip_sdf_rect_layer(x0, y0, x1, y1, radius)
+        while o < len(opens) and clips[opens[o]][-2] == lo:
+            c = clips[opens[o]]
+            if len(c) == 7:
+                canvas.push_clip_sdf_rect_layer(*c[:5])
+            else:
+                pts = np.asarray(c[0], np.float64).reshape(-1, 2)
+                path = tk.BezPath().move_to(*pts[0])
+                for q in pts[1:]:
+                    path.line_to(*q)
+                canvas.push_clip_layer(path.close_path(), rule=tk.FillRule(int(c[1])))
+                p32 = pts.astype(np.float32)
+                ca, cb, _ = edge_map(p32, [len(p32)])
+                none = np.full(len(ca), -1, np.int64)
+                pieces.append((np.concatenate([p32[ca], p32[cb]], 1), none, none))
             open_stack.append(opens[o])
             o += 1
         if hi > lo:
             draw_ids.extend(canvas.push_polygons(v32[offsets[lo]:offsets[hi]], counts[lo:hi], colors32[lo:hi],
                                                  rules[lo:hi]))
-        while open_stack and scene.clips[open_stack[-1]][6] == hi:
+            a, b = la[line_of[lo]:line_of[hi]], lb[line_of[lo]:line_of[hi]]
+            pieces.append((np.concatenate([v32[a], v32[b]], 1), a, b))
+        while open_stack and clips[open_stack[-1]][-1] == hi:
             canvas.pop_layer()
             open_stack.pop()
     if o != len(opens) or open_stack:
@@ -402,12 +423,14 @@ def encode(scene, verts, counts, colors=None):
     cw = int(round(clear[0] * 255)) | int(round(clear[1] * 255)) << 8 | int(round(clear[2] * 255)) << 16 | \
         int(round(clear[3] * 255)) << 24
     enc = canvas.encode(cw)
-    la, lb, per_draw = edge_map(v32, counts)
     lines = _abi.view(enc.lines, _abi.LINE_DTYPE, enc.line_count)
     got = np.concatenate([lines["p0"], lines["p1"]], 1) if len(lines) else np.zeros((0, 4), np.float32)
-    want = np.concatenate([v32[la], v32[lb]], 1) if len(la) else np.zeros((0, 4), np.float32)
+    want = np.concatenate([p[0] for p in pieces]) if pieces else np.zeros((0, 4), np.float32)
     if got.shape != want.shape or not np.array_equal(got, want):
         raise tk.TileinkError(_abi.TK_ERR_INVALID, "recorded lines do not match the polygon edges")
+    if any(len(c) != 7 for c in clips):
+        la = np.concatenate([p[1] for p in pieces])
+        lb = np.concatenate([p[2] for p in pieces])
     tags = _abi.view(enc.draws, _abi.DrawRecord, enc.draw_count)["tag"] if enc.draw_count else np.zeros(0)
     clip_draws = [int(i) for i in np.nonzero(tags == _abi.DRAW_CLIP)[0]]
     clip_ids = [0] * len(scene.clips)
@@ -450,8 +473,8 @@ class Plan:
             self.encodings.append(e)
             self.canvases.append(e.canvas)
             self.encoded.append(e.scene)
-            la.append(e.line_a + v_off)
-            lb.append(e.line_b + v_off)
+            la.append(np.where(e.line_a < 0, -1, e.line_a + v_off))
+            lb.append(np.where(e.line_b < 0, -1, e.line_b + v_off))
             self.line_ranges.append(e.per_draw)
             draw_index.append(e.draw_ids + d_off)
             v_off += len(v)
@@ -470,8 +493,11 @@ class Plan:
             cols = np.cumsum([0] + [t for _, _, t in self.sizes])
             self.offsets = [int(c) * 256 for c in cols[:-1]]
             self.pixels = int(cols[-1]) * 256
+        # path clip lines (-1) take their gradient into a sink row past the vertices
         self.line_a = np.concatenate(la)
         self.line_b = np.concatenate(lb)
+        self.line_a[self.line_a < 0] = v_off
+        self.line_b[self.line_b < 0] = v_off
         self._la_t = self._lb_t = None
         self.line_count = len(self.line_a)
         self.draw_count = d_off  # encoded draws, clip draws included
@@ -522,11 +548,11 @@ class Plan:
 
     def vertex_grad(self, line_grad, like):
         la, lb = self._index(like.device)
-        g = torch.zeros((self.vertex_count, 2), dtype=like.dtype, device=like.device)
+        g = torch.zeros((self.vertex_count + 1, 2), dtype=like.dtype, device=like.device)
         lg = line_grad.to(like.dtype).to(like.device)
         g.index_add_(0, la, lg[:, 0:2])
         g.index_add_(0, lb, lg[:, 2:4])
-        return g
+        return g[:self.vertex_count]
 
 
 class _Render(torch.autograd.Function):
