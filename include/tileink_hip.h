@@ -367,15 +367,20 @@ tk_status tk_frame_autotune(tk_frame *frame, uint32_t repeats, tk_tuning *chosen
 /* The differentiable form (no counterpart in the reference, which only renders).
  * Its forward pass is the fine stage (fine/main.hlsl:23-40) with the RGBA8 round
  * trips between batches left out: an FP32 premultiplied source-over composite of
- * the scene's FILL records.  Scenes: path draws (fills and strokes) inside SDF
- * clip layers.  Each draw's paint is paint[32 * draw ..] (FP32), not its brush:
- *   {kind: 0 solid, 1 linear gradient, 2 radial gradient; x0, y0, x1, y1: the
- *    gradient line in pixels; n: stop count 1-4; 4 stop offsets; 6 unused;
- *    4 stops of premultiplied RGBA},
- * a solid paint being its first stop; a linear gradient takes t = the
- * projection of the pixel centre on its line, a radial one t = the centre's
- * distance from (x0, y0) over the line's length; t is clamped to [0, 1] and the
- * stops are interpolated linearly.
+ * the scene's FILL records.  Scenes: path draws (fills and strokes) inside any
+ * nesting of clip layers, SDF or path; a clip multiplies the alpha of the draws
+ * inside it by its FP32 coverage (a path clip's: its exact-area alpha under its
+ * fill rule).  Each draw's paint is paint[32 * draw ..] (FP32), not its brush:
+ *   {kind: 0 solid, 1 linear, 2 radial, 3 sweep, 4 four-corner gradient;
+ *    x0, y0, x1, y1: the gradient line in pixels; n: stop count 1-4; 4 stop
+ *    offsets; 6 unused; 4 stops of premultiplied RGBA},
+ * a solid paint being its first stop.  The first three kinds interpolate the
+ * stops linearly in t, clamped to [0, 1]: t = the projection of the pixel centre
+ * on the line (linear), its distance from (x0, y0) over the line's length
+ * (radial), or its angle about (x0, y0) from the line's direction, turning from
+ * +x to +y, over a full turn (sweep).  A four-corner gradient (n = 4) is
+ * bilinear over the box (x0, y0)-(x1, y1), stops at (x0, y0), (x1, y0),
+ * (x0, y1), (x1, y1).
  * All float pointers are device pointers.
  * tk_frame_render_diff: the whole pipeline; writes `image` (4 floats per pixel,
  *   row-major; a batch's scenes at their image offsets, one after another).

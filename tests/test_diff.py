@@ -273,6 +273,10 @@ def test_recorded_lines_are_the_polygon_edges():
     assert e.scene.line_count == len(e.line_a) == int(e.per_draw.sum()) + 3
     assert list(e.line_a[start:start + 3]) == [-1, -1, -1] and (e.line_a[start + 3:] >= 0).all()
     assert e.draw_total == len(k) + 1 and e.clip_ids == [1]
+    # a pixel-aligned rectangle becomes an SDF clip, with no lines of its own
+    sc.clips = [(np.array([[8.0, 8.0], [90.0, 8.0], [90.0, 70.0], [8.0, 70.0]]), 0, 1, 4)]
+    e = diff.encode(sc, v, k)
+    assert e.scene.line_count == len(e.line_a) == int(e.per_draw.sum()) and (e.line_a >= 0).all()
 
 
 def test_path_clip_masks_closed_forms():

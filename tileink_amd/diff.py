@@ -372,8 +372,9 @@ class Encoded:
 
 
 def encode(scene, verts, counts, colors=None):
-    """Records the polygons (inside the scene's SDF clip layers) on a canvas and
-    encodes it; checks that the recorded lines are the polygon edges."""
+    """Records the polygons (inside the scene's clip layers) on a canvas and
+    encodes it; checks that the recorded lines are the polygon edges and the
+    path clips' edges."""
     v32 = verts.detach().to("cpu", torch.float32).numpy()
     canvas = tk.Canvas(scene.width, scene.height)
     rules = scene.draw_rules()
@@ -406,14 +407,14 @@ def encode(scene, verts, counts, colors=None):
                 p32 = pts.astype(np.float32)
                 ca, cb, _ = edge_map(p32, [len(p32)])
                 none = np.full(len(ca), -1, np.int64)
-                pieces.append((np.concatenate([p32[ca], p32[cb]], 1), none, none))
+                pieces.append((np.concatenate([p32[ca], p32[cb]], 1), none, none, opens[o]))
             open_stack.append(opens[o])
             o += 1
         if hi > lo:
             draw_ids.extend(canvas.push_polygons(v32[offsets[lo]:offsets[hi]], counts[lo:hi], colors32[lo:hi],
                                                  rules[lo:hi]))
             a, b = la[line_of[lo]:line_of[hi]], lb[line_of[lo]:line_of[hi]]
-            pieces.append((np.concatenate([v32[a], v32[b]], 1), a, b))
+            pieces.append((np.concatenate([v32[a], v32[b]], 1), a, b, None))
         while open_stack and clips[open_stack[-1]][-1] == hi:
             canvas.pop_layer()
             open_stack.pop()
@@ -423,19 +424,21 @@ def encode(scene, verts, counts, colors=None):
     cw = int(round(clear[0] * 255)) | int(round(clear[1] * 255)) << 8 | int(round(clear[2] * 255)) << 16 | \
         int(round(clear[3] * 255)) << 24
     enc = canvas.encode(cw)
+    records = _abi.view(enc.draws, _abi.DrawRecord, enc.draw_count)
+    clip_draws = [int(i) for i in np.nonzero(records["tag"] == _abi.DRAW_CLIP)[0]]
+    clip_ids = [0] * len(clips)
+    for n, i in enumerate(opens):
+        clip_ids[i] = clip_draws[n]
+    # a path clip the recorder took as an SDF clip (a pixel-aligned rectangle) has no lines
+    pieces = [p for p in pieces if p[3] is None or records["sdf_offset"][clip_ids[p[3]]] == _abi.INVALID]
     lines = _abi.view(enc.lines, _abi.LINE_DTYPE, enc.line_count)
     got = np.concatenate([lines["p0"], lines["p1"]], 1) if len(lines) else np.zeros((0, 4), np.float32)
     want = np.concatenate([p[0] for p in pieces]) if pieces else np.zeros((0, 4), np.float32)
     if got.shape != want.shape or not np.array_equal(got, want):
         raise tk.TileinkError(_abi.TK_ERR_INVALID, "recorded lines do not match the polygon edges")
     if any(len(c) != 7 for c in clips):
-        la = np.concatenate([p[1] for p in pieces])
-        lb = np.concatenate([p[2] for p in pieces])
-    tags = _abi.view(enc.draws, _abi.DrawRecord, enc.draw_count)["tag"] if enc.draw_count else np.zeros(0)
-    clip_draws = [int(i) for i in np.nonzero(tags == _abi.DRAW_CLIP)[0]]
-    clip_ids = [0] * len(scene.clips)
-    for n, i in enumerate(opens):
-        clip_ids[i] = clip_draws[n]
+        la = np.concatenate([p[1] for p in pieces]) if pieces else la
+        lb = np.concatenate([p[2] for p in pieces]) if pieces else lb
     return Encoded(canvas=canvas, scene=enc, line_a=la, line_b=lb, per_draw=per_draw,
                    draw_ids=np.asarray(draw_ids, np.int64), draw_total=int(enc.draw_count), clip_ids=clip_ids)
 
