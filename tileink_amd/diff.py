@@ -182,8 +182,8 @@ class DiffScene:
         head[:, 5] = 1.0
         stops = torch.cat([pm, pm.new_zeros((n, 12))], 1)
         if self.gradients:
-            rows, blocks = [], []
-            for d, entry in sorted(self.gradients.items()):
+            rows, stop_list, which, slot = [], [], [], []
+            for g, (d, entry) in enumerate(sorted(self.gradients.items())):
                 line, offsets, st, kind = gradient_parts(entry)
                 k = len(offsets)
                 if not 1 <= k <= 4 or st.shape[0] != k or (kind == "four_corner" and k != 4):
@@ -192,10 +192,16 @@ class DiffScene:
                 head[d, 1:5] = line
                 head[d, 5] = k
                 head[d, 6:6 + k] = offsets
-                spm = torch.cat([st[:, :3] * st[:, 3:4], st[:, 3:4]], 1).reshape(-1)
-                blocks.append(torch.cat([spm, spm.new_zeros(16 - spm.numel())]))
                 rows.append(d)
-            stops = stops.index_put((torch.as_tensor(rows, device=pm.device),), torch.stack(blocks).to(pm.dtype))
+                stop_list.append(st)
+                which.extend([g] * k)
+                slot.extend(range(k))
+            # every gradient's stops premultiplied at once and placed in their blocks
+            st = torch.cat(stop_list).to(device=pm.device, dtype=pm.dtype)
+            spm = torch.cat([st[:, :3] * st[:, 3:4], st[:, 3:4]], 1)
+            blocks = pm.new_zeros((len(rows), 4, 4)).index_put(
+                (torch.as_tensor(which, device=pm.device), torch.as_tensor(slot, device=pm.device)), spm)
+            stops = stops.index_put((torch.as_tensor(rows, device=pm.device),), blocks.reshape(-1, 16))
         return torch.cat([torch.as_tensor(head, dtype=pm.dtype, device=pm.device), stops], 1)
 
 
