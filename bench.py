@@ -87,7 +87,7 @@ def cpu_baseline(args):
     import oracle
     side = 1024
     paths = args.paths // 16
-    canvas = scenes.config2(side, side, paths, clips=args.clips, workload=2)
+    canvas = scenes.clipped_paths_scene(side, side, paths, clips=args.clips, workload=2)
     arrays = canvas.arrays()
     t0 = time.perf_counter()
     oracle.render_frame(arrays)
@@ -123,8 +123,8 @@ def scene_loop(ctx, count=64, repeats=5):
     kernel-by-kernel launches and with each frame replayed from its hipGraph.  The
     same scenes are then rendered as one batch (tk_batch_*: one frame, one launch
     per stage for all of them), eager and replayed from a hipGraph."""
-    sizes = scenes.config3_sizes(count, path_mu=np.log(150.0), path_sigma=0.8, max_paths=2000)
-    canvases = [scenes.config3_scene(w, h, n, i) for i, (w, h, n) in enumerate(sizes)]
+    sizes = scenes.scene_set_sizes(count, path_mu=np.log(150.0), path_sigma=0.8, max_paths=2000)
+    canvases = [scenes.scene_set_member(w, h, n, i) for i, (w, h, n) in enumerate(sizes)]
     encoded = [c.encode(0) for c in canvases]
     frames = [tk.Frame(ctx, sc) for sc in encoded]
     out = {"scenes": count, "paths": int(sum(n for _, _, n in sizes)),
@@ -293,8 +293,8 @@ def main():
                     help="first copy MIB MiB device-to-device three times with tk_k_copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the scene would shard by tile "
-              f"stripes, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the frame would shard by tile "
+              f"stripes, one stripe of the canvas per GPU, and a batch of scenes by scene)", file=sys.stderr)
         sys.exit(2)
 
     if args.fit:
@@ -307,7 +307,7 @@ def main():
         return
 
     t0 = time.perf_counter()
-    canvas = scenes.config2(args.size, args.size, args.paths, clips=args.clips, workload=2)
+    canvas = scenes.clipped_paths_scene(args.size, args.size, args.paths, clips=args.clips, workload=2)
     scene = canvas.encode(0)
     log(f"scene: {scene.path_count} paths, {scene.line_count} lines, {scene.batch_count} batches, "
         f"{scene.tile_draw_index_count} tile refs, generated in {time.perf_counter() - t0:.1f} s")

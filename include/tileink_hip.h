@@ -407,8 +407,8 @@ tk_status tk_frame_set_graph(tk_frame *frame, int enable);
  * to end and go through one launch sequence of scan, coarse and fine, whose
  * coarse and fine kernels read each tile's canvas from a per-scene slot.  Each
  * scene's image equals its own tk_frame_render's.  The batched form of the
- * pipeline (SURVEY config 3); no counterpart in the reference, which renders one
- * canvas per submission. */
+ * pipeline, for many small scenes; no counterpart in the reference, which
+ * renders one canvas per submission. */
 typedef struct tk_batch tk_batch;
 tk_status tk_batch_create(tk_context *ctx, const tk_scene *scenes, uint32_t scene_count, tk_batch **out);
 tk_status tk_batch_render(tk_batch *batch);

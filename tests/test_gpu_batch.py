@@ -15,8 +15,8 @@ pytestmark = pytest.mark.gpu
 
 
 def batch_cases():
-    canvases = [scenes.config1(), scenes.config3_scene(100, 37, 40, 0), scenes.config3_scene(333, 250, 300, 1),
-                tk.Canvas(17, 9), scenes.config2_small(16), gradient_canvas(["linear", "radial", "sweep", "four_corner"])]
+    canvases = [scenes.mixed_scene_640x360(), scenes.scene_set_member(100, 37, 40, 0), scenes.scene_set_member(333, 250, 300, 1),
+                tk.Canvas(17, 9), scenes.clipped_paths_small(16), gradient_canvas(["linear", "radial", "sweep", "four_corner"])]
     clears = [0, 0x80402010, 0, 0xFF0000FF, 0, 0x20202020]
     return canvases, [c.encode(clear) for c, clear in zip(canvases, clears)]
 

@@ -17,14 +17,14 @@ def _report(gpu, ref, bad):
     return f"mismatched stages {bad}; image max channel diff {channel_diff(gpu['image'], ref['image'])}"
 
 
-def test_config1_frame_matches_oracle(ctx):
-    gpu, ref, bad = compare_frame(ctx, scenes.config1())
+def test_mixed_640x360_frame_matches_oracle(ctx):
+    gpu, ref, bad = compare_frame(ctx, scenes.mixed_scene_640x360())
     assert not bad, _report(gpu, ref, bad)
 
 
 @pytest.mark.parametrize("scale", [16, 8, 4])
-def test_config2_small_frames_match_oracle(ctx, scale):
-    gpu, ref, bad = compare_frame(ctx, scenes.config2_small(scale))
+def test_small_clipped_path_frames_match_oracle(ctx, scale):
+    gpu, ref, bad = compare_frame(ctx, scenes.clipped_paths_small(scale))
     assert not bad, _report(gpu, ref, bad)
 
 
@@ -62,7 +62,7 @@ def test_frame_update_with_more_layer_pairs_than_its_class_table(ctx):
     """A frame sizes its per-tile layer class table for the distinct (draw, tag)
     pairs of the scene it was created with; an update that brings more pairs
     falls back to evaluating each layer, with the same records and image."""
-    arrays = scenes.config2_small(16).arrays()
+    arrays = scenes.clipped_paths_small(16).arrays()
     assert len({tuple(r) for r in arrays.layers[:, :2].tolist()}) > 1
     collapsed = tk.SceneArrays.__new__(tk.SceneArrays)
     collapsed.__dict__.update(arrays.__dict__)
@@ -83,7 +83,7 @@ def test_full_size_frame_is_deterministic_and_matches_its_incremental_form(ctx):
     the oracle would take minutes: two renders give the same image although emit's
     segment order comes from atomics (the coverage sums are exact and order-free),
     and re-rendering every tile through the incremental path reproduces it."""
-    canvas = scenes.config2(4096, 4096, 200_000, clips=32, workload=2)
+    canvas = scenes.clipped_paths_scene(4096, 4096, 200_000, clips=32, workload=2)
     scene = canvas.encode(0)  # points into the canvas's arrays: the canvas must outlive it
     frame = tk.Frame(ctx, scene)
     frame.render()

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_graph_replays_match_eager_renders(ctx):
-    arrays = scenes.config2_small(16).arrays()
+    arrays = scenes.clipped_paths_small(16).arrays()
     scene = arrays.to_scene()
     eager = tk.Frame(ctx, scene)
     eager.render()

@@ -38,7 +38,7 @@ def solid_draws(arrays):
 
 
 def test_all_tiles_in_any_order_equal_a_full_render(ctx):
-    arrays = scenes.config2_small(16).arrays()
+    arrays = scenes.clipped_paths_small(16).arrays()
     scene = arrays.to_scene()
     full = tk.Frame(ctx, scene)
     full.render()
@@ -51,7 +51,7 @@ def test_all_tiles_in_any_order_equal_a_full_render(ctx):
 
 
 def test_repainting_any_tiles_of_an_unchanged_scene_keeps_the_image(ctx):
-    arrays = scenes.config2_small(16).arrays()
+    arrays = scenes.clipped_paths_small(16).arrays()
     frame = tk.Frame(ctx, arrays.to_scene())
     frame.render()
     before = frame.image()
@@ -65,7 +65,7 @@ def test_repainting_any_tiles_of_an_unchanged_scene_keeps_the_image(ctx):
 
 
 def test_brush_change_rerendered_on_its_tiles_matches_full_render(ctx):
-    arrays = scenes.config2_small(16).arrays()
+    arrays = scenes.clipped_paths_small(16).arrays()
     frame = tk.Frame(ctx, arrays.to_scene())
     frame.render()
     changed = [d for d in solid_draws(arrays) if len(draw_tiles(arrays, d)) <= 12][::7][:6]
@@ -85,7 +85,7 @@ def test_brush_change_rerendered_on_its_tiles_matches_full_render(ctx):
 
 
 def test_clip_change_rerendered_on_its_tiles_matches_full_render(ctx):
-    arrays = scenes.config2_small(16).arrays()
+    arrays = scenes.clipped_paths_small(16).arrays()
     frame = tk.Frame(ctx, arrays.to_scene())
     frame.render()
     # The smallest clip rectangle wider and taller than 40 px shrinks by 9 px on every side.
@@ -109,7 +109,7 @@ def test_clip_change_rerendered_on_its_tiles_matches_full_render(ctx):
 
 
 def test_render_tiles_rejects_out_of_range_and_ignores_an_empty_list(ctx):
-    arrays = scenes.config1().arrays()
+    arrays = scenes.mixed_scene_640x360().arrays()
     frame = tk.Frame(ctx, arrays.to_scene())
     frame.render()
     before = frame.image()

@@ -103,7 +103,7 @@ def test_pixel_math_words_match_oracle(ctx):
 
 
 def test_fill_coverage_fp32_matches_oracle_and_fp64_area(ctx):
-    a = scenes.config1().arrays()
+    a = scenes.mixed_scene_640x360().arrays()
     _, out = oracle.render_frame(a)
     req = _coverage_requests(a, out, limit=60000, seed=2)
     segs = out["segments"]
@@ -144,7 +144,7 @@ def test_cumsum_rows_match_numpy(ctx):
 
 
 def test_render_to_image_entry_matches_frame(ctx):
-    canvas = scenes.config2_small(32)
+    canvas = scenes.clipped_paths_small(32)
     renderer = tk.NativeRenderer(tk.NativeBackend.Hip, 128, 128, context=ctx)
     image = renderer.render_to_image(canvas).readback()
     ref, _ = oracle.render_frame(canvas.arrays())

@@ -19,7 +19,7 @@ def render_with(frame, tuning):
 
 
 def test_every_run_length_renders_the_same_frame(ctx):
-    canvas = scenes.config2_small(16)
+    canvas = scenes.clipped_paths_small(16)
     frame = tk.Frame(ctx, canvas.encode(0))
     ref = render_with(frame, (0, 64, 256))
     kinds = frame.buffer(_abi.BUF_TILE_KINDS, np.uint32, frame.batch_count * frame.tiles)
@@ -30,7 +30,7 @@ def test_every_run_length_renders_the_same_frame(ctx):
 
 
 def test_run_lengths_apply_to_incremental_renders_and_batches(ctx):
-    arrays = scenes.config2_small(16).arrays()
+    arrays = scenes.clipped_paths_small(16).arrays()
     frame = tk.Frame(ctx, arrays.to_scene())
     frame.render()
     full = frame.image()
@@ -39,7 +39,7 @@ def test_run_lengths_apply_to_incremental_renders_and_batches(ctx):
     frame.render_tiles(np.random.default_rng(1).permutation(T)[: T // 3])
     assert np.array_equal(frame.image(), full)
 
-    canvases = [scenes.config3_scene(w, h, n, i) for i, (w, h, n) in enumerate([(96, 64, 40), (300, 140, 120)])]
+    canvases = [scenes.scene_set_member(w, h, n, i) for i, (w, h, n) in enumerate([(96, 64, 40), (300, 140, 120)])]
     batch = tk.Batch(ctx, [c.encode(0) for c in canvases])
     batch.render()
     ref = [batch.image(i) for i in range(len(canvases))]
@@ -50,7 +50,7 @@ def test_run_lengths_apply_to_incremental_renders_and_batches(ctx):
 
 
 def test_autotune_picks_valid_values_and_keeps_the_image(ctx):
-    frame = tk.Frame(ctx, scenes.config2_small(16).encode(0))
+    frame = tk.Frame(ctx, scenes.clipped_paths_small(16).encode(0))
     frame.render()
     ref = frame.image()
     chosen = frame.autotune(repeats=1)
@@ -63,7 +63,7 @@ def test_autotune_picks_valid_values_and_keeps_the_image(ctx):
 
 
 def test_invalid_run_lengths_are_rejected(ctx):
-    frame = tk.Frame(ctx, scenes.config1().encode(0))
+    frame = tk.Frame(ctx, scenes.mixed_scene_640x360().encode(0))
     before = frame.tuning()
     for bad in ((3, 0, 0), (0, 2048, 0), (0, 0, 100)):
         with pytest.raises(tk.TileinkError) as err:

@@ -191,7 +191,7 @@ def test_oracle_scan_count_kat_matches_cpu_counts_and_keeps_guards():
 def test_oracle_backdrops_give_winding_of_segmentless_tiles():
     """After the row prefix, a tile with no segments has one winding number; it must
     equal the path's winding number at the tile centre (f64 ray casting)."""
-    canvas = scenes.config1()
+    canvas = scenes.mixed_scene_640x360()
     a = canvas.arrays()
     _, out = oracle.render_frame(a)
     lines = a.lines
@@ -388,7 +388,7 @@ def _coverage_requests(a, out, limit=20000, seed=0):
 
 
 def test_fp32_row_sweep_within_1e5_of_fp64_area():
-    a = scenes.config1().arrays()
+    a = scenes.mixed_scene_640x360().arrays()
     _, out = oracle.render_frame(a)
     req = _coverage_requests(a, out)
     segs = out["segments"]
@@ -406,13 +406,13 @@ def _shuffle_within_ranges(segs, req, rng):
     return out
 
 
-@pytest.mark.parametrize("scene", ["config1", "config2_small"])
+@pytest.mark.parametrize("scene", ["mixed_scene_640x360", "clipped_paths_small"])
 def test_exact_sums_are_order_free_and_within_1_lsb_of_fp32_order(scene):
     """The fine stage's coverage (three exact sums) does not depend on the slot order
     emit's atomics give a tile's segments, and differs from the reference's literal
     FP32 row sweep (shared/coverage.hlsli:34-51), in any order, by at most 1 LSB of
     alpha and a few FP32 ulps of coverage."""
-    canvas = scenes.config1() if scene == "config1" else scenes.config2_small(16)
+    canvas = scenes.mixed_scene_640x360() if scene == "mixed_scene_640x360" else scenes.clipped_paths_small(16)
     a = canvas.arrays()
     _, out = oracle.render_frame(a)
     req = _coverage_requests(a, out, limit=40000, seed=5)

@@ -193,7 +193,7 @@ def test_pixel_aligned_path_clip_becomes_sdf_clip():
 
 
 def test_tile_bins_are_painter_ordered_per_tile():
-    c = scenes.config1()
+    c = scenes.mixed_scene_640x360()
     a = c.arrays()
     order = np.flatnonzero(a.draw_batch_ids != tk.INVALID)
     rank = np.empty(a.draw_count, np.int64)
@@ -208,7 +208,7 @@ def test_tile_bins_are_painter_ordered_per_tile():
 
 
 def test_scan_and_cumsum_chunk_plans_partition_each_path():
-    a = scenes.config2_small(16).arrays()
+    a = scenes.clipped_paths_small(16).arrays()
     for p in range(a.path_count):
         rec = a.paths[p]
         s, e = a.scan_chunk_ranges[p]

@@ -47,7 +47,7 @@ def _gradient(rng, lo, hi):
                           extend=int(rng.integers(0, 3)))
 
 
-def config1(width=640, height=360):
+def mixed_scene_640x360(width=640, height=360):
     """640x360: 256 rounded rects, 256 filled closed cubic paths, 64 stroked paths, shuffled draw order."""
     rng = _rng(1)
     canvas = Canvas(width, height, 1.0)
@@ -71,7 +71,7 @@ def config1(width=640, height=360):
     return canvas
 
 
-def config2(width=4096, height=4096, paths=200_000, clips=32, workload=2):
+def clipped_paths_scene(width=4096, height=4096, paths=200_000, clips=32, workload=2):
     """4096^2, 200k paths (60% fill, 40% stroke), 4-16 cubics each, 10% linear gradients,
     `clips` nested rounded-rect clip layers pushed then popped between equal draw groups."""
     rng = _rng(workload)
@@ -108,14 +108,14 @@ def config2(width=4096, height=4096, paths=200_000, clips=32, workload=2):
     return canvas
 
 
-def config2_small(scale=8, paths=None):
-    """Config-2 generator on a (4096/scale)^2 canvas with proportionally fewer paths (parity cases)."""
+def clipped_paths_small(scale=8, paths=None):
+    """clipped-paths generator on a (4096/scale)^2 canvas with proportionally fewer paths (parity cases)."""
     side = 4096 // scale
     n = paths if paths is not None else max(64, 200_000 // (scale * scale))
-    return config2(side, side, n, clips=min(32, max(2, n // 64)), workload=20 + scale)
+    return clipped_paths_scene(side, side, n, clips=min(32, max(2, n // 64)), workload=20 + scale)
 
 
-def config3_sizes(count, seed=3, path_mu=math.log(1500.0), path_sigma=1.2, max_paths=20_000):
+def scene_set_sizes(count, seed=3, path_mu=math.log(1500.0), path_sigma=1.2, max_paths=20_000):
     """(width, height, paths) of `count` independent scenes: side log-uniform in
     [64, 2048] px, aspect ratio uniform in [0.5, 2], paths lognormal(path_mu,
     path_sigma) clipped to [10, max_paths]."""
@@ -131,7 +131,7 @@ def config3_sizes(count, seed=3, path_mu=math.log(1500.0), path_sigma=1.2, max_p
     return out
 
 
-def config3_scene(width, height, paths, index):
-    """One scene of the independent-scene workload: the config-2 generator at this
+def scene_set_member(width, height, paths, index):
+    """One scene of the independent-scene workload: the clipped-paths generator at this
     size, with clip nesting scaled to the path count."""
-    return config2(width, height, paths, clips=min(32, max(1, paths // 256)), workload=1000 + index)
+    return clipped_paths_scene(width, height, paths, clips=min(32, max(1, paths // 256)), workload=1000 + index)
