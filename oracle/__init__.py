@@ -91,21 +91,56 @@ ARRAY_FIELDS = ("lines", "paths", "draws", "paint", "draw_batch_ids", "layers", 
                 "cumsum_row_starts", "cumsum_row_ends")
 
 
-def render_frame(scene, ptcl_capacity=None):
-    """Runs the whole frame on the CPU.  `scene` is a tileink_amd.SceneArrays.
-
-    Returns (image[h, w] u32, dict of intermediate arrays)."""
+def _scene_struct(scene):
+    """An or_scene over the arrays of `scene` (a tileink_amd.SceneArrays); the
+    second value keeps the word copies alive."""
     s = Scene()
     keep = []
     for name, _ in Scene._fields_:
         if name in ARRAY_FIELDS:
-            arr = np.ascontiguousarray(getattr(scene, name)).view(np.uint8).reshape(-1)
             arr = np.ascontiguousarray(np.asarray(getattr(scene, name)))
             raw = np.ascontiguousarray(arr.view(np.uint32).reshape(-1)) if arr.size else np.zeros(1, np.uint32)
             keep.append(raw)
             setattr(s, name, raw.ctypes.data)
         else:
             setattr(s, name, int(scene.scalars[name]))
+    return s, keep
+
+
+def tile_draw_bins(scene, cull=True):
+    """Tile draw lists of an immediate canvas (TileDrawBins::reset_transient), with
+    early occlusion unless cull is False.  Returns (tile_draws[T, 2] u32 {start,
+    len}, indices u32)."""
+    s, keep = _scene_struct(scene)
+    T = s.tiles_width * s.tiles_height
+    tile_draws = np.zeros((T, 2), np.uint32)
+    capacity = 0
+    tiles_w, tiles_h = s.tiles_width, s.tiles_height
+    draws = np.asarray(scene.draws, np.uint32).reshape(-1, 31)
+    ids = np.asarray(scene.draw_batch_ids, np.uint32)
+    for d in range(len(draws)):  # every member's tile box bounds its references
+        if ids[d] >= s.batch_count:
+            continue
+        pb = draws[d, 10:14].view(np.int32).astype(np.int64)
+        x0 = min(max(pb[0], 0) // 16, tiles_w)
+        y0 = min(max(pb[1], 0) // 16, tiles_h)
+        x1 = min((max(pb[2], 0) + 15) // 16, tiles_w)
+        y1 = min((max(pb[3], 0) + 15) // 16, tiles_h)
+        capacity += max(x1 - x0, 0) * max(y1 - y0, 0)
+    indices = np.zeros(max(capacity, 1), np.uint32)
+    lib().or_tile_draw_bins.restype = C.c_int64
+    n = lib().or_tile_draw_bins(C.byref(s), C.c_int(1 if cull else 0), _p(tile_draws), _p(indices),
+                                C.c_uint64(indices.size))
+    if n < 0:
+        raise RuntimeError("tile draw references exceed their bound")
+    return tile_draws, indices[:n].copy()
+
+
+def render_frame(scene, ptcl_capacity=None):
+    """Runs the whole frame on the CPU.  `scene` is a tileink_amd.SceneArrays.
+
+    Returns (image[h, w] u32, dict of intermediate arrays)."""
+    s, keep = _scene_struct(scene)
     B, P = s.backdrop_len, s.path_count
     T = s.tiles_width * s.tiles_height
     Q = s.batch_count

@@ -1123,6 +1123,167 @@ void or_coarse_emit_bins(const or_coarse_config *cfg, const or_coarse_inputs *in
     }
 }
 
+/* ---------- tile draw bins with early occlusion -------------------------------------
+ * TileDrawBins::reset_transient (src/shared/gpu_plan/tile_bins/transient.rs:7-146) with
+ * Occlusion::update (tile_bins/occlusion.rs:48-160), opaque_tiles / rect_tiles /
+ * batch_clips (occlusion/geometry.rs:17-135) and for_each_visible
+ * (occlusion/transient.rs:9-66).  An immediate canvas records its draws in painter
+ * order, so a batch's painter order is increasing draw index. */
+
+typedef struct { uint32_t x0, y0, x1, y1; } TBox;
+static uint32_t tbox_count(TBox b) { return (b.x1 > b.x0 ? b.x1 - b.x0 : 0u) * (b.y1 > b.y0 ? b.y1 - b.y0 : 0u); }
+static TBox tbox_and(TBox a, TBox b) {
+    TBox r = {umax(a.x0, b.x0), umax(a.y0, b.y0), umin(a.x1, b.x1), umin(a.y1, b.y1)};
+    return r;
+}
+/* PixelBounds::tile_bbox (src/shared/bounds.rs:113-128). */
+static TBox draw_tbox(const uint32_t *dw, uint32_t tw, uint32_t th) {
+    TBox b = {tmin_of((int32_t)dw[10], tw), tmin_of((int32_t)dw[11], th), tmax_of((int32_t)dw[12], tw),
+              tmax_of((int32_t)dw[13], th)};
+    return b;
+}
+static double rect_radius(float r, double max_radius) {
+    double v = (double)r;
+    return v < 0.0 ? 0.0 : (v > max_radius ? max_radius : v);
+}
+static uint32_t whole_tiles(double t) { return t > 0.0 ? (uint32_t)t : 0u; }
+
+/* rect_tiles (occlusion/geometry.rs:79-135). */
+static TBox rect_tiles(const uint32_t *dw, const uint32_t *paint, uint32_t paint_sdf_words, uint32_t tw, uint32_t th) {
+    TBox none = {0, 0, 0, 0};
+    if (dw[0] != NONE || dw[1] != NONE || dw[4] != NONE) return none;
+    if (bits_f(dw[19]) != 1.0f || bits_f(dw[20]) != 0.0f || bits_f(dw[21]) != 0.0f || bits_f(dw[22]) != 1.0f)
+        return none;
+    if (dw[2] == NONE || dw[3] != 17u || (uint64_t)dw[2] + 17u > paint_sdf_words) return none;
+    const uint32_t *rec = paint + dw[2];
+    if (rec[0] != 1u) return none; /* GPU_SDF_RECT */
+    float v[10];
+    int whole = 1;
+    for (int i = 0; i < 8; ++i) v[i] = bits_f(rec[1 + i]);
+    v[8] = bits_f(dw[23]);
+    v[9] = bits_f(dw[24]);
+    for (int i = 0; i < 10; ++i) {
+        if (!isfinite(v[i]) || fabsf(v[i]) > 65536.0f) return none;
+        if (v[i] != truncf(v[i])) whole = 0;
+    }
+    double x0 = fmin((double)v[0], (double)v[2]), x1 = fmax((double)v[0], (double)v[2]);
+    double y0 = fmin((double)v[1], (double)v[3]), y1 = fmax((double)v[1], (double)v[3]);
+    double max_radius = fmax(fmin(x1 - x0, y1 - y0) * 0.5, 0.0);
+    double tl = rect_radius(v[4], max_radius), tr = rect_radius(v[5], max_radius);
+    double bl = rect_radius(v[6], max_radius), br = rect_radius(v[7], max_radius);
+    double guard = whole ? 0.0 : 1.0 / 32.0, ex = (double)v[8], ey = (double)v[9];
+    TBox bb = draw_tbox(dw, tw, th), r;
+    r.x0 = umax(whole_tiles(ceil((x0 + fmax(tl, bl) + ex + guard) / 16.0)), bb.x0);
+    r.y0 = umax(whole_tiles(ceil((y0 + fmax(tl, tr) + ey + guard) / 16.0)), bb.y0);
+    r.x1 = umin(whole_tiles(floor((x1 - fmax(tr, br) + ex - guard) / 16.0)), bb.x1);
+    r.y1 = umin(whole_tiles(floor((y1 - fmax(bl, br) + ey - guard) / 16.0)), bb.y1);
+    return r;
+}
+
+/* Visits the tiles of draw d's box outside `hidden` (tile index, draw). */
+typedef struct {
+    uint32_t *tile_draws, *indices, *counts;
+    uint32_t tw, pass; /* pass 0 counts, pass 1 fills each list from its end */
+} BinVisit;
+static void bin_region(BinVisit *v, TBox r, uint32_t d) {
+    for (uint32_t y = r.y0; y < r.y1; ++y)
+        for (uint32_t x = r.x0; x < r.x1; ++x) {
+            uint32_t t = y * v->tw + x;
+            if (v->pass == 0) v->counts[t]++;
+            else v->indices[v->tile_draws[2 * t] + --v->counts[t]] = d;
+        }
+}
+
+int64_t or_tile_draw_bins(const or_scene *s, int cull, uint32_t *tile_draws, uint32_t *indices, uint64_t capacity) {
+    const uint32_t tw = s->tiles_width, th = s->tiles_height, T = tw * th, D = s->draw_count, Q = s->batch_count;
+    uint32_t *first = malloc(sizeof(uint32_t) * (Q + 1u));
+    uint8_t *enabled = malloc(Q + 1u), *occluder = malloc(Q + 1u);
+    TBox *clip = malloc(sizeof(TBox) * (Q + 1u)), *opaque = calloc(D + 1u, sizeof(TBox));
+    uint32_t *order = malloc(sizeof(uint32_t) * (D + 1u)), *counts = calloc(T + 1u, sizeof(uint32_t));
+    uint32_t *vis_batch = malloc(sizeof(uint32_t) * (D + 1u));
+    const uint32_t sdf_words = s->paint_sdf_shadow_base;
+    const uint32_t *brush = s->paint + s->paint_brush_base;
+    const uint32_t brush_words = s->paint_words - s->paint_brush_base;
+    uint32_t n = 0;
+    int active = 0;
+    for (uint32_t b = 0; b < Q; ++b) {
+        first[b] = NONE;
+        enabled[b] = 1;
+        occluder[b] = 0;
+        TBox e = {0, 0, 0, 0};
+        clip[b] = e;
+    }
+    /* painter order: batches in order, each batch's draws by index */
+    for (uint32_t b = 0; b < Q; ++b)
+        for (uint32_t d = 0; d < D; ++d)
+            if (s->draw_batch_ids[d] == b) {
+                order[n++] = d;
+                const uint32_t tag = s->draws[31 * d + 8];
+                enabled[b] &= (uint8_t)(tag == 0u || tag == 5u); /* Brush, PathGlyph */
+                if (first[b] == NONE) first[b] = d;
+            }
+    /* candidates (draw index order): opaque_tiles of every non-first member */
+    for (uint32_t d = 0; d < D && cull; ++d) {
+        const uint32_t *dw = s->draws + 31 * d, b = s->draw_batch_ids[d];
+        if (b >= Q || !enabled[b] || first[b] == d) continue;
+        if (dw[8] != 0u || dw[7] < 5u || (uint64_t)dw[6] + 5u > brush_words) continue;
+        if (brush[dw[6]] != 1u || (brush[dw[6] + 4u] >> 24) != 255u) continue; /* solid, alpha 255 */
+        opaque[d] = rect_tiles(dw, s->paint, sdf_words, tw, th);
+        if (tbox_count(opaque[d])) occluder[b] = 1;
+    }
+    for (uint32_t b = 0; b < Q; ++b) {
+        if (!occluder[b]) continue;
+        TBox c = {0, 0, tw, th};
+        for (uint32_t l = s->batches[2 * b]; l < s->batches[2 * b + 1]; ++l) {
+            const uint32_t *layer = s->layers + 3 * l;
+            if (layer[0] == 0u && layer[1] < D) /* LayerStackEntry::Clip */
+                c = tbox_and(c, rect_tiles(s->draws + 31 * layer[1], s->paint, sdf_words, tw, th));
+        }
+        clip[b] = c;
+    }
+    for (uint32_t d = 0; d < D; ++d) {
+        const uint32_t b = s->draw_batch_ids[d];
+        vis_batch[d] = b < Q && enabled[b] ? b : NONE;
+        if (vis_batch[d] != NONE) opaque[d] = tbox_and(opaque[d], clip[b]);
+        else opaque[d].x1 = opaque[d].x0;
+        if (tbox_count(opaque[d])) active = 1;
+    }
+    BinVisit v = {tile_draws, indices, counts, tw, 0};
+    int64_t total = 0;
+    for (uint32_t pass = 0; pass < 2; ++pass) {
+        v.pass = pass;
+        if (pass == 1) {
+            for (uint32_t t = 0; t < T; ++t) {
+                tile_draws[2 * t] = counts[t] ? (uint32_t)total : NONE;
+                tile_draws[2 * t + 1] = counts[t];
+                total += counts[t];
+            }
+            if ((uint64_t)total > capacity) break;
+        }
+        uint32_t batch = NONE;
+        TBox cover = {0, 0, 0, 0};
+        for (uint32_t k = n; k-- > 0;) {
+            const uint32_t d = order[k], vb = active ? vis_batch[d] : NONE;
+            if (batch != vb || batch == NONE) cover.x1 = cover.x0 = cover.y0 = cover.y1 = 0;
+            batch = vb;
+            TBox bb = draw_tbox(s->draws + 31 * d, tw, th), hid = tbox_and(bb, cover);
+            if (tbox_count(hid) == 0) {
+                bin_region(&v, bb, d);
+            } else {
+                TBox top = {bb.x0, bb.y0, bb.x1, hid.y0}, bottom = {bb.x0, hid.y1, bb.x1, bb.y1};
+                TBox left = {bb.x0, hid.y0, hid.x0, hid.y1}, right = {hid.x1, hid.y0, bb.x1, hid.y1};
+                bin_region(&v, top, d);
+                bin_region(&v, bottom, d);
+                bin_region(&v, left, d);
+                bin_region(&v, right, d);
+            }
+            if (vb != NONE && tbox_count(opaque[d]) > tbox_count(cover)) cover = opaque[d];
+        }
+    }
+    free(first); free(enabled); free(occluder); free(clip); free(opaque); free(order); free(counts); free(vis_batch);
+    return (uint64_t)total > capacity ? -1 : total;
+}
+
 /* ---------- fine (fine/main.hlsl, interpreter.hlsli, specialized.hlsli, clip.hlsli) ---- */
 
 typedef struct { uint32_t brush_offset, sdf_offset, shadow_offset; int32_t pb[4]; Aff inv; } FDraw;

@@ -105,6 +105,14 @@ typedef struct or_frame_outputs {
 /* Returns 0 on success, nonzero when a buffer bound was exceeded. */
 int or_render_frame(const or_scene *scene, uint32_t *image, or_frame_outputs *out);
 
+/* Painter-ordered tile draw lists of an immediate canvas from its draws, batch
+ * ids, batches, layers and paint (TileDrawBins::reset_transient), with early
+ * occlusion when `cull` is nonzero.  tile_draws: 2 words {start, len} per tile
+ * (start NONE for an empty list); indices: up to `capacity` draw ids.  Returns
+ * the number of references, or -1 when they exceed `capacity`. */
+int64_t or_tile_draw_bins(const or_scene *scene, int cull, uint32_t *tile_draws, uint32_t *indices,
+                          uint64_t capacity);
+
 /* Validation entries (validation/geometry.hlsl:22-29, validation/pixel.hlsl:9-34). */
 /* or_fill_coverage_words: fill_alpha_at with each of its three sums exact (32.32
  * fixed point), independent of segment order; the fine stage uses this form.

@@ -42,9 +42,14 @@ def compare_frame(ctx, canvas, clear=0):
     """Returns (gpu, ref, mismatches) where mismatches lists the differing stages."""
     frame, scene = gpu_frame(ctx, canvas, clear)
     gpu = read_frame(frame, scene)
-    ref_image, ref = oracle.render_frame(canvas.arrays(clear))
+    arrays = canvas.arrays(clear)
+    ref_image, ref = oracle.render_frame(arrays)
     ref["image"] = ref_image
     bad = []
+    # The device reads the recorder's tile bins; the oracle derives its own.
+    tile_draws, indices = oracle.tile_draw_bins(arrays)
+    if not (np.array_equal(arrays.tile_draws, tile_draws) and np.array_equal(arrays.tile_draw_indices, indices)):
+        bad.append("tile_bins")
     for key in ("raw_backdrops", "backdrops", "segment_counts", "segment_bumps", "segment_cursors",
                 "tile_ptcl_counts", "tile_kinds", "image"):
         if not np.array_equal(gpu[key], ref[key]):

@@ -234,6 +234,37 @@ def test_scenes_with_sdf_draws_are_rejected(ctx):
                                                       image.data_ptr(), colors.data_ptr()))
 
 
+def test_backward_needs_its_own_forward(ctx):
+    # The tile pass replaces the forward's kept coverages with their gradients, and a
+    # plain render replaces the PTCL: either way another backward needs another
+    # differentiable render and is refused until then, never answered from stale state.
+    sc = diff.random_scene(11, 64, 48, fills=4, cubics=3, radius_mu=math.log(10.0))
+    with torch.no_grad():
+        v, k = sc.polygons()
+        plan = diff.Plan(ctx, [sc], [v], [k])
+        colors = plan.full_paint(sc.paint_records().to(DEV)).to(torch.float32).contiguous()
+    frame = plan.frame
+    image = torch.zeros((plan.pixels, 4), device=DEV)
+    grad = torch.ones_like(image)
+    lines = torch.zeros((max(plan.line_count, 1), 4), device=DEV)
+    cgrad = torch.zeros_like(colors)
+    lib = tk._abi.lib()
+    args = (frame._h, colors.data_ptr(), grad.data_ptr(), lines.data_ptr(), cgrad.data_ptr())
+    tk._abi.check(lib.tk_frame_render_diff(frame._h, colors.data_ptr(), image.data_ptr()))
+    tk._abi.check(lib.tk_frame_backward(*args))
+    first = (lines.clone(), cgrad.clone())
+    with pytest.raises(tk.TileinkError):
+        tk._abi.check(lib.tk_frame_backward(*args))
+    tk._abi.check(lib.tk_frame_render_diff(frame._h, colors.data_ptr(), image.data_ptr()))
+    frame.render()
+    with pytest.raises(tk.TileinkError):
+        tk._abi.check(lib.tk_frame_backward(*args))
+    tk._abi.check(lib.tk_frame_render_diff(frame._h, colors.data_ptr(), image.data_ptr()))
+    tk._abi.check(lib.tk_frame_backward(*args))
+    assert torch.equal(lines, first[0])
+    assert torch.allclose(cgrad, first[1], rtol=1e-5, atol=1e-6)
+
+
 def test_adam_fitting_recovers_a_perturbed_scene(ctx):
     # Image fitting: control points and colours perturbed from a target scene are
     # optimised with Adam against the target's image; the loss falls.

@@ -96,3 +96,44 @@ def test_full_size_frame_is_deterministic_and_matches_its_incremental_form(ctx):
     frame.render_tiles(np.arange(T, dtype=np.uint32)[::-1].copy())
     assert np.array_equal(frame.image(), first)
     assert len(np.unique(first)) > 10_000
+
+
+def test_update_follows_a_new_paint_split(ctx):
+    # Two encodings of one scene with equal word counts but different paint
+    # sections: A = sdf | brush | 3 unused words, B = sdf | 3 shadow words | brush.
+    # A frame created from A and updated to B must read B's brush section.
+    canvas = scenes.mixed_scene_640x360()
+    base = canvas.arrays(0)
+    s0, b0 = base.paint_sdf_shadow_base, base.paint_brush_base
+    assert s0 == b0  # the recorder writes no shadow words
+    pad = np.array([0xDEADBEEF, 0x01020304, 0xFFFFFFFF], np.uint32)
+    a = tk.SceneArrays.__new__(tk.SceneArrays)
+    a.__dict__.update(base.__dict__)
+    a.scalars = dict(base.scalars)
+    a.paint = np.concatenate([base.paint, pad])
+    a.scalars["paint_words"] = len(a.paint)
+    b = tk.SceneArrays.__new__(tk.SceneArrays)
+    b.__dict__.update(a.__dict__)
+    b.scalars = dict(a.scalars)
+    b.paint = np.concatenate([base.paint[:s0], pad, base.paint[s0:]])
+    b.scalars["paint_brush_base"] = s0 + len(pad)
+    expected, _ = oracle.render_frame(b)
+    assert np.array_equal(expected, oracle.render_frame(a)[0])
+    frame = tk.Frame(ctx, a.to_scene())
+    frame.render()
+    assert np.array_equal(frame.image(), expected)
+    frame.update(b.to_scene())
+    frame.render()
+    assert np.array_equal(frame.image(), expected)
+    assert frame.stats()["overflow"] == 0
+
+
+def test_update_rejects_paint_bases_out_of_order(ctx):
+    base = scenes.mixed_scene_640x360().arrays(0)
+    frame = tk.Frame(ctx, base.to_scene())
+    bad = tk.SceneArrays.__new__(tk.SceneArrays)
+    bad.__dict__.update(base.__dict__)
+    bad.scalars = dict(base.scalars)
+    bad.scalars["paint_sdf_shadow_base"] = base.paint_brush_base + 1
+    with pytest.raises(tk.TileinkError):
+        frame.update(bad.to_scene())

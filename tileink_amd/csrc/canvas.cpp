@@ -714,6 +714,12 @@ uint32_t f2bits(float f) {
     return u;
 }
 
+float bits2f(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
 uint32_t premul_f32_to_u32(const float c[4]) {
     // premul_f32_to_u32 (src/shared/pixel.rs:14-21): (c * 255 + 0.5) as u8, saturating.
     uint32_t out = 0;
@@ -1101,6 +1107,69 @@ struct PlannedBatch {
     std::vector<uint32_t> stack;  // clip draw ids, outermost first
 };
 
+// ---------------------------------------------------------------------------
+// Early occlusion of tile bins (src/shared/gpu_plan/tile_bins/occlusion.rs,
+// occlusion/geometry.rs, occlusion/transient.rs): before coarse, an immediate
+// canvas drops the tile references of draws that a later opaque solid SDF
+// rectangle of the same DrawBatch proves hidden.
+// ---------------------------------------------------------------------------
+
+TileBox box_intersection(const TileBox &a, const TileBox &b) {
+    return TileBox{std::max(a.x0, b.x0), std::max(a.y0, b.y0), std::min(a.x1, b.x1), std::min(a.y1, b.y1)};
+}
+
+// rect_tiles (occlusion/geometry.rs:79-135): the whole tiles inside the central
+// rectangle of an axis-aligned SDF rect (rounded corners and the AA edge
+// excluded), with a 1/32 px guard when any input is fractional.  Empty for a
+// path, glyph, shadowed or non-translating draw, or a non-rect SDF.
+TileBox rect_tiles(const tk_canvas *c, const tk_draw_record &d) {
+    const TileBox none{0, 0, 0, 0};
+    if (d.path_id != kNone || d.glyph_run_id != kNone || d.sdf_shadow_offset != kNone) return none;
+    if (!(d.transform.a == 1.0f && d.transform.b == 0.0f && d.transform.c == 0.0f && d.transform.d == 1.0f))
+        return none;
+    // encoded_sdf (src/shared/gpu_sdf.rs:290-304): exactly one 17-word record.
+    if (d.sdf_offset == kNone || d.sdf_len != TK_SDF_RECORD_WORDS ||
+        uint64_t(d.sdf_offset) + d.sdf_len > c->sdf_blob.size())
+        return none;
+    const uint32_t *w = c->sdf_blob.data() + d.sdf_offset;
+    if (w[0] != TK_SDF_KIND_RECT) return none;
+    float v[10];  // coords x0 y0 x1 y1, radii tl tr bl br, translation e f
+    for (int i = 0; i < 8; ++i) v[i] = bits2f(w[1 + i]);
+    v[8] = d.transform.e;
+    v[9] = d.transform.f;
+    bool integral = true;
+    for (float x : v) {
+        if (!std::isfinite(x) || std::fabs(x) > 65536.0f) return none;
+        if (x - std::trunc(x) != 0.0f) integral = false;
+    }
+    const double x0 = std::min<double>(v[0], v[2]), x1 = std::max<double>(v[0], v[2]);
+    const double y0 = std::min<double>(v[1], v[3]), y1 = std::max<double>(v[1], v[3]);
+    const double max_radius = std::max(std::min(x1 - x0, y1 - y0) * 0.5, 0.0);
+    auto radius = [&](float r) { return std::min(std::max(double(r), 0.0), max_radius); };
+    const double tl = radius(v[4]), tr = radius(v[5]), bl = radius(v[6]), br = radius(v[7]);
+    const double guard = integral ? 0.0 : 1.0 / 32.0;
+    const double tx = v[8], ty = v[9], tile = double(TK_TILE_SIZE);
+    const TileBox bb = tile_bbox(d.pixel_bounds, c->tiles_w, c->tiles_h);
+    auto whole = [](double t) { return uint32_t(std::max(t, 0.0)); };
+    TileBox r;
+    r.x0 = std::max(whole(std::ceil((x0 + std::max(tl, bl) + tx + guard) / tile)), bb.x0);
+    r.y0 = std::max(whole(std::ceil((y0 + std::max(tl, tr) + ty + guard) / tile)), bb.y0);
+    r.x1 = std::min(whole(std::floor((x1 - std::max(tr, br) + tx - guard) / tile)), bb.x1);
+    r.y1 = std::min(whole(std::floor((y1 - std::max(bl, br) + ty - guard) / tile)), bb.y1);
+    return r;
+}
+
+// opaque_tiles (occlusion/geometry.rs:60-77): rect_tiles of a Brush draw whose
+// brush is solid with packed alpha 255.
+TileBox opaque_tiles(const tk_canvas *c, const tk_draw_record &d) {
+    const TileBox none{0, 0, 0, 0};
+    if (d.tag != TK_DRAW_BRUSH || d.brush_len < 5u) return none;
+    if (uint64_t(d.brush_offset) + 5u > c->brush_blob.size()) return none;
+    const uint32_t *b = c->brush_blob.data() + d.brush_offset;
+    if (b[0] != TK_BRUSH_SOLID || (b[4] >> 24) != 255u) return none;
+    return rect_tiles(c, d);
+}
+
 void compile_into(const tk_canvas *c, uint32_t list, std::vector<uint32_t> &stack,
                   std::vector<PlannedBatch> &ops) {
     std::vector<uint32_t> pending;
@@ -1156,14 +1225,83 @@ tk_status encode(tk_canvas *c, uint32_t clear_color, tk_scene *out) {
         }
     }
 
-    // Tile bins: painter-ordered candidate lists per tile.
-    std::vector<uint32_t> counts(tiles, 0u);
+    // Tile bins: painter-ordered candidate lists per tile (tile_bins/transient.rs:7-146).
     std::vector<TileBox> boxes(c->draws.size());
+    for (uint32_t d : draw_order) boxes[d] = tile_bbox(c->draws[d].pixel_bounds, tw, th);
+
+    // Occlusion::update (tile_bins/occlusion.rs:48-160): each batch's first draw
+    // in painter order, whether every member is a Brush / PathGlyph draw, and,
+    // for every later member with an opaque proof, its proof cut to the whole
+    // tiles its batch's clip rectangles cover.
+    struct BatchVis {
+        uint32_t first = kNone;
+        bool enabled = true, has_occluder = false;
+        TileBox clip{0, 0, 0, 0};
+    };
+    std::vector<BatchVis> vis(batches.size());
     for (uint32_t d : draw_order) {
-        boxes[d] = tile_bbox(c->draws[d].pixel_bounds, tw, th);
-        const TileBox &bb = boxes[d];
-        for (uint32_t y = bb.y0; y < bb.y1; ++y)
-            for (uint32_t x = bb.x0; x < bb.x1; ++x) counts[size_t(y) * tw + x]++;
+        BatchVis &bv = vis[c->draw_batch_ids[d]];
+        bv.enabled = bv.enabled && (c->draws[d].tag == TK_DRAW_BRUSH || c->draws[d].tag == TK_DRAW_PATH_GLYPH);
+        if (bv.first == kNone) bv.first = d;
+    }
+    std::vector<TileBox> opaque(c->draws.size(), TileBox{0, 0, 0, 0});
+    for (uint32_t d = 0; d < uint32_t(c->draws.size()); ++d) {
+        const uint32_t id = c->draw_batch_ids[d];
+        if (id == kNone || !vis[id].enabled || vis[id].first == d) continue;
+        opaque[d] = opaque_tiles(c, c->draws[d]);
+        if (opaque[d].area() != 0) vis[id].has_occluder = true;
+    }
+    bool occluded = false;
+    for (size_t b = 0; b < batches.size(); ++b) {  // batch_clips (occlusion/geometry.rs:17-56)
+        if (!vis[b].has_occluder) continue;
+        TileBox clip{0, 0, tw, th};
+        for (uint32_t cd : batches[b].stack) clip = box_intersection(clip, rect_tiles(c, c->draws[cd]));
+        vis[b].clip = clip;
+    }
+    for (uint32_t d = 0; d < uint32_t(c->draws.size()); ++d) {
+        const uint32_t id = c->draw_batch_ids[d];
+        if (id == kNone || !vis[id].enabled) continue;
+        opaque[d] = box_intersection(opaque[d], vis[id].clip);
+        if (opaque[d].area() != 0) occluded = true;
+    }
+    // for_each_visible (occlusion/transient.rs:9-66): draws in reverse painter
+    // order; within one batch the largest proof seen so far (a strictly larger
+    // one replaces it, never a union) hides the tiles it covers.
+    auto for_each_visible = [&](auto visit) {
+        uint32_t batch = kNone;
+        TileBox cover{0, 0, 0, 0};
+        for (size_t k = draw_order.size(); k-- > 0;) {
+            const uint32_t d = draw_order[k];
+            const uint32_t id = c->draw_batch_ids[d];
+            const uint32_t vb = id != kNone && vis[id].enabled ? id : kNone;
+            if (batch != vb || batch == kNone) cover = TileBox{0, 0, 0, 0};
+            batch = vb;
+            const TileBox &bb = boxes[d];
+            const TileBox hidden = box_intersection(bb, cover);
+            auto region = [&](const TileBox &r) {
+                for (uint32_t y = r.y0; y < r.y1; ++y)
+                    for (uint32_t x = r.x0; x < r.x1; ++x) visit(size_t(y) * tw + x, d);
+            };
+            if (hidden.area() == 0) {
+                region(bb);
+            } else {
+                region(TileBox{bb.x0, bb.y0, bb.x1, hidden.y0});
+                region(TileBox{bb.x0, hidden.y1, bb.x1, bb.y1});
+                region(TileBox{bb.x0, hidden.y0, hidden.x0, hidden.y1});
+                region(TileBox{hidden.x1, hidden.y0, bb.x1, hidden.y1});
+            }
+            if (vb != kNone && opaque[d].area() > cover.area()) cover = opaque[d];
+        }
+    };
+    std::vector<uint32_t> counts(tiles, 0u);
+    if (occluded) {
+        for_each_visible([&](size_t t, uint32_t) { counts[t]++; });
+    } else {
+        for (uint32_t d : draw_order) {
+            const TileBox &bb = boxes[d];
+            for (uint32_t y = bb.y0; y < bb.y1; ++y)
+                for (uint32_t x = bb.x0; x < bb.x1; ++x) counts[size_t(y) * tw + x]++;
+        }
     }
     c->tile_draws.assign(tiles, tk_tile_draw_record{kNone, 0u});
     uint64_t total = 0;
@@ -1174,14 +1312,18 @@ tk_status encode(tk_canvas *c, uint32_t clear_color, tk_scene *out) {
     }
     if (total > 0xFFFFFFFFull) return fail(TK_ERR_OVERFLOW, "tile draw references exceed u32");
     c->tile_draw_indices.assign(total, 0u);
-    std::vector<uint32_t> cursor(tiles, 0u);
-    for (uint32_t d : draw_order) {
-        const TileBox &bb = boxes[d];
-        for (uint32_t y = bb.y0; y < bb.y1; ++y)
-            for (uint32_t x = bb.x0; x < bb.x1; ++x) {
-                size_t t = size_t(y) * tw + x;
-                c->tile_draw_indices[c->tile_draws[t].start + cursor[t]++] = d;
-            }
+    if (occluded) {  // reverse painter order: each list fills from its end
+        for_each_visible([&](size_t t, uint32_t d) { c->tile_draw_indices[c->tile_draws[t].start + --counts[t]] = d; });
+    } else {
+        std::vector<uint32_t> cursor(tiles, 0u);
+        for (uint32_t d : draw_order) {
+            const TileBox &bb = boxes[d];
+            for (uint32_t y = bb.y0; y < bb.y1; ++y)
+                for (uint32_t x = bb.x0; x < bb.x1; ++x) {
+                    size_t t = size_t(y) * tw + x;
+                    c->tile_draw_indices[c->tile_draws[t].start + cursor[t]++] = d;
+                }
+        }
     }
 
     // PTCL bound over all batches: per batch, particles plus (2 depth + 1) per

@@ -104,6 +104,15 @@ uint32_t xcd_grid(uint64_t blocks, uint32_t chunk) {
     return uint32_t((blocks + unit - 1) / unit * unit);
 }
 
+// A 2-D grid of at least `blocks` workgroups with x <= 65535 and as few spare
+// workgroups as possible: rows of equal width, so at most gy - 1 of them exit at
+// once (65 536 tiles: 32 768 x 2, not 65 535 x 2).  Kernels number the
+// workgroups linearly (x fastest) and drop those past `blocks`.
+void tile_grid(uint32_t blocks, uint32_t &gx, uint32_t &gy) {
+    gy = div_up(blocks, 65535u);
+    gx = gy ? div_up(blocks, gy) : 0u;
+}
+
 hipStream_t launch_stream(const tk_context *ctx) { return ctx->cur ? ctx->cur : ctx->stream; }
 
 // Sends the launches made while it lives to `stream`.
@@ -263,9 +272,8 @@ tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inp
     }
     tk_args_fine a{cfg, in, rgba, draw_brush, list, scenes, xcd_run};
     if (list.columns == 0) return TK_OK;
-    const uint32_t blocks = xcd_grid(list.columns, xcd_run);
-    uint32_t gx = std::min<uint32_t>(blocks, 65535u);
-    uint32_t gy = div_up(blocks, gx);
+    uint32_t gx, gy;
+    tile_grid(xcd_grid(list.columns, xcd_run), gx, gy);
     return launch(ctx, scenes.slots ? ctx->f_fine_scenes : ctx->f_fine, gx, gy, 256, a, cfg.clip_spill_depth * 256u);
 }
 
@@ -291,6 +299,7 @@ struct tk_frame {
     tk_batch_record *batches = nullptr;
     tk_tile_draw_record *tile_draws = nullptr;
     uint32_t *tile_draw_indices = nullptr;
+    uint32_t index_capacity = 0;  // tile_draw_indices entries allocated
     tk_scan_chunk *scan_chunks = nullptr;
     tk_scan_chunk_range *scan_chunk_ranges = nullptr;
     uint32_t *cs_offsets = nullptr, *cs_lens = nullptr, *cs_row_starts = nullptr, *cs_row_ends = nullptr;
@@ -379,6 +388,8 @@ tk_status validate_scene(const tk_scene *s) {
     if (uint64_t(s->tiles_width) * s->tiles_height > 0xFFFFFFFFull) return fail(TK_ERR_INVALID, "too many tiles");
     if (s->max_clip_depth > 4u + TK_FINE_MAX_CLIP_SPILL) return fail(TK_ERR_INVALID, "clip nesting too deep");
     if (s->ptcl_capacity >= 0xFFFFFFFFull) return fail(TK_ERR_OVERFLOW, "PTCL capacity exceeds u32 record indices");
+    if (s->paint_sdf_shadow_base > s->paint_brush_base || s->paint_brush_base > s->paint_words)
+        return fail(TK_ERR_INVALID, "paint section bases out of order");
     return TK_OK;
 }
 
@@ -455,10 +466,13 @@ tk_status frame_upload(tk_frame *f, const tk_scene *s) {
     return TK_OK;
 }
 
-bool same_sizes(const tk_scene &a, const tk_scene &b) {
+// Whether scene `a` fits the buffers of a frame created for `b`.  The tile draw
+// index list may shrink or grow up to the frame's allocation (`index_capacity`):
+// an opacity change can hide or reveal draws behind an opaque rectangle.
+bool same_sizes(const tk_scene &a, const tk_scene &b, uint32_t index_capacity) {
     return a.width == b.width && a.height == b.height && a.line_count == b.line_count && a.path_count == b.path_count &&
            a.draw_count == b.draw_count && a.paint_words == b.paint_words && a.layer_count == b.layer_count &&
-           a.batch_count == b.batch_count && a.tile_draw_index_count == b.tile_draw_index_count &&
+           a.batch_count == b.batch_count && a.tile_draw_index_count <= index_capacity &&
            a.scan_chunk_count == b.scan_chunk_count && a.cumsum_chunk_count == b.cumsum_chunk_count &&
            a.cumsum_row_count == b.cumsum_row_count && a.backdrop_len == b.backdrop_len &&
            a.segment_capacity == b.segment_capacity && a.ptcl_capacity <= b.ptcl_capacity &&
@@ -568,9 +582,9 @@ tk_args_fine_diff diff_args(tk_frame *f, const float *colors) {
 
 tk_status launch_tiles(tk_context *ctx, hipFunction_t fn, uint32_t columns, const tk_args_fine_diff &a) {
     if (columns == 0) return TK_OK;
-    const uint32_t blocks = xcd_grid(columns, a.xcd_run);
-    const uint32_t gx = std::min<uint32_t>(blocks, 65535u);
-    return launch(ctx, fn, gx, div_up(blocks, gx), 256, a);
+    uint32_t gx, gy;
+    tile_grid(xcd_grid(columns, a.xcd_run), gx, gy);
+    return launch(ctx, fn, gx, gy, 256, a);
 }
 
 // The fine stage of a differentiable render: sizes the coverage buffer from the
@@ -614,6 +628,9 @@ tk_status run_fine_diff(tk_frame *f, const DiffTarget &t) {
 
 tk_status frame_render(tk_frame *f, uint32_t active_count = 0, const DiffTarget *diff = nullptr) {
     tk_context *ctx = f->ctx;
+    // A plain or incremental render rewrites the PTCL and tables a backward pass
+    // walks; only a differentiable render leaves them matching its own state.
+    f->diff_ready = false;
     const tk_scene &s = f->meta;
     const tk_tile_list list{active_count ? f->active_tiles : nullptr, active_count};
     const tk_scene_slots scenes{f->scene_slots, f->scene_of_column, f->scene_count, f->table_entries};
@@ -800,6 +817,7 @@ tk_status frame_create(tk_context *ctx, const tk_scene *s, const BatchLayout *ba
     ALLOC(batches, s->batch_count);
     ALLOC(tile_draws, T);
     ALLOC(tile_draw_indices, s->tile_draw_index_count);
+    f->index_capacity = s->tile_draw_index_count;
     ALLOC(scan_chunks, s->scan_chunk_count);
     ALLOC(scan_chunk_ranges, s->path_count);
     ALLOC(cs_offsets, s->cumsum_chunk_count);
@@ -1293,14 +1311,17 @@ tk_status tk_frame_update(tk_frame *f, const tk_scene *s) {
     if (!f) return fail(TK_ERR_INVALID, "null frame");
     if (f->scene_count) return fail(TK_ERR_INVALID, "a batch of scenes cannot be updated in place");
     TK_TRY(validate_scene(s));
-    if (!same_sizes(*s, f->meta)) return fail(TK_ERR_INVALID, "scene sizes differ from the frame's");
-    uint32_t clear = s->clear_color;
-    if (f->graph_exec) {  // the captured arguments (layer pair count, clear colour) may change
+    if (!same_sizes(*s, f->meta, f->index_capacity)) return fail(TK_ERR_INVALID, "scene sizes differ from the frame's");
+    if (f->graph_exec) {  // the captured arguments (layer pair count, clear colour, paint bases) may change
         TK_HIP(hipStreamSynchronize(f->ctx->stream));
         drop_graph(f);
     }
     TK_TRY(frame_upload(f, s));
-    f->meta.clear_color = clear;
+    // Scalars the kernels take as arguments follow the new scene.
+    f->meta.clear_color = s->clear_color;
+    f->meta.paint_sdf_shadow_base = s->paint_sdf_shadow_base;
+    f->meta.paint_brush_base = s->paint_brush_base;
+    f->meta.tile_draw_index_count = s->tile_draw_index_count;
     return TK_OK;
 }
 
@@ -1318,6 +1339,7 @@ void tk_frame_destroy(tk_frame *f) {
 
 tk_status tk_frame_render(tk_frame *f) {
     if (!f) return fail(TK_ERR_INVALID, "null frame");
+    f->diff_ready = false;  // graph replays do not pass through frame_render
     if (f->use_graph && !f->timing && !f->capture_raw) return frame_render_graph(f);
     return frame_render(f);
 }
@@ -1429,6 +1451,9 @@ tk_status tk_frame_backward(tk_frame *f, const float *colors, const float *image
                                 f->diff_adj, f->diff_adj_records, line_grad};
         TK_TRY(tk::launch(ctx, ctx->f_line_backward, div_up(s.line_count, 256), 1, 256, b));
     }
+    // The tile pass replaced the kept coverages with their gradients: another
+    // backward needs another forward.
+    f->diff_ready = false;
     return TK_OK;
 }
 
