@@ -81,21 +81,52 @@ def measured_traffic(kernel):
     return rec.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(args):
-    """The oracle (single-threaded C restatement) on a bounded sample of the same
-    generator: a 1024x1024 canvas with 1/16 of the paths and the same clip schedule."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_threads():
+    """The host cores this process may use: its CPU share (OMP_NUM_THREADS, set to the
+    share on the GPU boxes) within its affinity mask."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(avail, share) if share > 0 else avail)
+
+
+def cpu_baseline(args, arrays):
+    """The oracle (the C restatement of every stage) on the host cores, timed on the
+    same frame the GPU renders, threaded over lines, chunks and tiles; and, beside
+    it, one thread on a bounded sample of the generator (1024x1024, 1/16 of the
+    paths, the same clip schedule), since one thread takes minutes on the full frame."""
     import oracle
-    side = 1024
-    paths = args.paths // 16
-    canvas = scenes.clipped_paths_scene(side, side, paths, clips=args.clips, workload=2)
-    arrays = canvas.arrays()
+    threads = cpu_threads()
+    oracle.set_threads(threads)
     t0 = time.perf_counter()
-    oracle.render_frame(arrays)
+    _, out = oracle.render_frame(arrays, outputs=False)
     dt = time.perf_counter() - t0
-    return {"value": side * side / dt / 1e9, "unit": "Gpix/s", "cores": 1, "kind": "port",
-            "sample": f"oracle frame of the same generator at {side}x{side} with {paths} paths and {args.clips} "
-                      f"nested clips ({dt:.1f} s, 1 thread, {platform.processor() or platform.machine()})",
-            "ms_per_frame": dt * 1e3}
+    side, paths = 1024, args.paths // 16
+    sample = scenes.clipped_paths_scene(side, side, paths, clips=args.clips, workload=2).arrays()
+    oracle.set_threads(1)
+    t1 = time.perf_counter()
+    _, out1 = oracle.render_frame(sample, outputs=False)
+    dt1 = time.perf_counter() - t1
+    W, H = arrays.width, arrays.height
+    return {"value": W * H / dt / 1e9, "unit": "Gpix/s", "cores": threads, "kind": "port",
+            "sample": f"oracle, whole frame of this workload ({W}x{H}, {arrays.path_count} paths), {threads} "
+                      f"threads on {cpu_model()} ({os.cpu_count()} logical CPUs, {dt:.1f} s)",
+            "ms_per_frame": round(dt * 1e3, 1),
+            "stage_ms": {k: round(v, 1) for k, v in out["stage_ms"].items()},
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "single_thread": {"sample": f"{side}x{side}, {paths} paths, {args.clips} nested clips",
+                              "ms_per_frame": round(dt1 * 1e3, 1), "gpix_per_s": side * side / dt1 / 1e9,
+                              "stage_ms": {k: round(v, 1) for k, v in out1["stage_ms"].items()}}}
 
 
 def incremental_render(frame, tiles_w, tiles_h, steps, side=32):
@@ -400,8 +431,9 @@ def main():
                      "used_segments": used_segments, "fine_ms": round(stage_avg["fine"], 4)},
     }
     if not args.no_cpu_baseline:
-        log("cpu baseline: oracle on a bounded sample ...")
-        result["cpu_baseline"] = cpu_baseline(args)
+        log("cpu baseline: the oracle on the same frame ...")
+        result["cpu_baseline"] = cpu_baseline(args, canvas.arrays())
+        log(f"cpu baseline: {result['cpu_baseline']['ms_per_frame']} ms/frame")
     print(json.dumps(result), flush=True)
 
 

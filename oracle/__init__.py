@@ -83,7 +83,19 @@ class FrameOutputs(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("raw_backdrops", "backdrops", "segment_counts", "segment_ranges",
                                           "segment_cursors", "segment_bumps", "segments", "tile_ptcl_counts",
                                           "tile_ptcl_starts", "tile_ptcl_ends", "tile_kinds", "batch_bases",
-                                          "ptcl")] + [("ptcl_capacity_words", C.c_uint64)]
+                                          "ptcl")] + [("ptcl_capacity_words", C.c_uint64), ("stage_ms", C.c_void_p)]
+
+STAGES = ("scan_count", "scan_prefix", "scan_emit", "cumsum", "coarse", "fine")
+
+
+def set_threads(n):
+    """Threads for the oracle's independent work items (results do not depend on it)."""
+    lib().or_set_threads(C.c_uint32(int(n)))
+
+
+def threads():
+    lib().or_get_threads.restype = C.c_uint32
+    return int(lib().or_get_threads())
 
 
 ARRAY_FIELDS = ("lines", "paths", "draws", "paint", "draw_batch_ids", "layers", "batches", "tile_draws",
@@ -136,11 +148,21 @@ def tile_draw_bins(scene, cull=True):
     return tile_draws, indices[:n].copy()
 
 
-def render_frame(scene, ptcl_capacity=None):
+def render_frame(scene, ptcl_capacity=None, outputs=True):
     """Runs the whole frame on the CPU.  `scene` is a tileink_amd.SceneArrays.
 
-    Returns (image[h, w] u32, dict of intermediate arrays)."""
+    Returns (image[h, w] u32, dict of intermediate arrays and the stage times in
+    ms under "stage_ms"); outputs=False returns only the image and stage times."""
     s, keep = _scene_struct(scene)
+    if not outputs:
+        fo = FrameOutputs()
+        ms = np.zeros(6, np.float64)
+        fo.stage_ms = ms.ctypes.data
+        image = np.zeros((s.height, s.width), np.uint32)
+        status = lib().or_render_frame(C.byref(s), _p(image), C.byref(fo))
+        if status != 0:
+            raise RuntimeError(f"oracle frame exceeded a bound (status {status})")
+        return image, {"stage_ms": dict(zip(STAGES, ms.tolist()))}
     B, P = s.backdrop_len, s.path_count
     T = s.tiles_width * s.tiles_height
     Q = s.batch_count
@@ -159,11 +181,14 @@ def render_frame(scene, ptcl_capacity=None):
     for name, arr in out.items():
         setattr(fo, name, arr.ctypes.data)
     fo.ptcl_capacity_words = out["ptcl"].size
+    ms = np.zeros(6, np.float64)
+    fo.stage_ms = ms.ctypes.data
     image = np.zeros((s.height, s.width), np.uint32)
     status = lib().or_render_frame(C.byref(s), _p(image), C.byref(fo))
     if status != 0:
         raise RuntimeError(f"oracle frame exceeded a bound (status {status})")
     out["ptcl"] = out["ptcl"][: int(out["batch_bases"][Q])]
+    out["stage_ms"] = dict(zip(STAGES, ms.tolist()))
     return image, out
 
 

@@ -11,11 +11,70 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #define NONE 0xFFFFFFFFu
 #define TILE 16u
+
+/* ---------- threads ----------------------------------------------------------
+ * Stages whose work items are independent (lines of distinct paths, scan and
+ * cumsum chunks, tiles) run over a pool of threads taking items in chunks from
+ * a shared counter.  Integer results do not depend on the thread count: scan
+ * counts add with atomics, and each path's lines stay with one thread, in
+ * order, so emit claims its segment slots in index order exactly as one thread
+ * does. */
+
+static uint32_t g_threads = 1;
+
+void or_set_threads(uint32_t n) { g_threads = n == 0u ? 1u : (n > 256u ? 256u : n); }
+uint32_t or_get_threads(void) { return g_threads; }
+
+typedef void (*RangeFn)(void *ctx, uint32_t begin, uint32_t end);
+typedef struct {
+    RangeFn fn;
+    void *ctx;
+    uint64_t n, grain, next;
+} ParJob;
+
+static void *par_worker(void *arg) {
+    ParJob *j = (ParJob *)arg;
+    for (;;) {
+        uint64_t b = __atomic_fetch_add(&j->next, j->grain, __ATOMIC_RELAXED);
+        if (b >= j->n) break;
+        uint64_t e = b + j->grain < j->n ? b + j->grain : j->n;
+        j->fn(j->ctx, (uint32_t)b, (uint32_t)e);
+    }
+    return NULL;
+}
+
+/* fn(ctx, begin, end) over [0, n) in chunks of `grain` items. */
+static void par_for(uint32_t n, uint32_t grain, RangeFn fn, void *ctx) {
+    if (n == 0u) return;
+    if (grain == 0u) grain = 1u;
+    if (g_threads <= 1u || n <= grain) {
+        fn(ctx, 0u, n);
+        return;
+    }
+    ParJob j = {fn, ctx, n, grain, 0u};
+    pthread_t tid[256];
+    uint32_t spawned = 0;
+    for (uint32_t t = 1; t < g_threads; ++t)
+        if (pthread_create(&tid[spawned], NULL, par_worker, &j) == 0) spawned++;
+    par_worker(&j);
+    for (uint32_t t = 0; t < spawned; ++t) pthread_join(tid[t], NULL);
+}
+
+static double now_ms(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec * 1e3 + (double)ts.tv_nsec * 1e-6;
+}
+
+static void add_i32(int32_t *p, int32_t v) { __atomic_fetch_add((uint32_t *)p, (uint32_t)v, __ATOMIC_RELAXED); }
+static void add_u32(uint32_t *p, uint32_t v) { __atomic_fetch_add(p, v, __ATOMIC_RELAXED); }
 
 /* ---------- scalar helpers --------------------------------------------------- */
 
@@ -177,17 +236,44 @@ static uint32_t scan_tile_offset(const Traversal *s, int32_t tx, int32_t ty) {
     return s->data_offset + ((uint32_t)ty - s->bbox[1]) * (s->bbox[2] - s->bbox[0]) + (uint32_t)tx - s->bbox[0];
 }
 
-/* scan_count (scan/count.hlsl:13-40). */
-void or_scan_count(const or_scan_config *cfg, const uint32_t *lines, const uint32_t *paths,
-                   uint32_t path_record_count, int32_t *backdrops, uint32_t *counts) {
-    for (uint32_t id = 0; id < cfg->line_count; ++id) {
+/* The scan's work items: lines [begin, end) of `lines`. */
+typedef struct {
+    const or_scan_config *cfg;
+    const uint32_t *lines, *paths;
+    uint32_t path_record_count;
+    int32_t *backdrops;
+    uint32_t *counts, *cursors, *segments;
+    const uint32_t *bounds; /* chunk c is lines [bounds[c], bounds[c + 1]) */
+} ScanJob;
+
+/* Line-range chunks that never split a run of lines of one path. */
+static uint32_t *path_aligned_bounds(const uint32_t *lines, uint32_t line_count, uint32_t *chunk_count) {
+    uint32_t k = line_count / 4096u + 1u;
+    uint32_t *b = (uint32_t *)malloc(sizeof(uint32_t) * (k + 1u));
+    uint32_t n = 0;
+    b[n++] = 0u;
+    for (uint32_t c = 1; c < k; ++c) {
+        uint32_t at = (uint32_t)((uint64_t)line_count * c / k);
+        if (at <= b[n - 1]) continue;
+        while (at < line_count && lines[(size_t)at * 6u] == lines[(size_t)(at - 1u) * 6u]) ++at;
+        if (at < line_count && at > b[n - 1]) b[n++] = at;
+    }
+    b[n] = line_count;
+    *chunk_count = n;
+    return b;
+}
+
+static void scan_count_lines(void *ctx, uint32_t c0, uint32_t c1) {
+    const ScanJob *j = (const ScanJob *)ctx;
+    int32_t *backdrops = j->backdrops;
+    for (uint32_t id = j->bounds[c0]; id < j->bounds[c1]; ++id) {
         Traversal s;
-        if (!scan_geometry(lines, paths, path_record_count, id, &s)) continue;
+        if (!scan_geometry(j->lines, j->paths, j->path_record_count, id, &s)) continue;
         int32_t delta = s.down ? -1 : 1;
         uint32_t stride = s.bbox[2] - s.bbox[0];
         for (int32_t y = s.ymin; y < s.ymax; y++) {
             uint32_t local = (uint32_t)(y - (int32_t)s.bbox[1]) * stride;
-            backdrops[s.data_offset + local] = (int32_t)((uint32_t)backdrops[s.data_offset + local] + (uint32_t)delta);
+            add_i32(&backdrops[s.data_offset + local], delta);
         }
         float last_z = floorf(s.a * ((float)s.imin - 1.0f) + s.b);
         for (uint32_t i = s.imin; i < s.imax; i++) {
@@ -200,20 +286,45 @@ void or_scan_count(const or_scan_config *cfg, const uint32_t *lines, const uint3
                 if (top && tx + 1 < (int32_t)s.bbox[2]) {
                     int32_t x = tx + 1 > (int32_t)s.bbox[0] ? tx + 1 : (int32_t)s.bbox[0];
                     uint32_t local = (uint32_t)(ty - (int32_t)s.bbox[1]) * stride + (uint32_t)(x - (int32_t)s.bbox[0]);
-                    backdrops[s.data_offset + local] =
-                        (int32_t)((uint32_t)backdrops[s.data_offset + local] + (uint32_t)delta);
+                    add_i32(&backdrops[s.data_offset + local], delta);
                 }
-                counts[scan_tile_offset(&s, tx, ty)] += 1u;
+                add_u32(&j->counts[scan_tile_offset(&s, tx, ty)], 1u);
             }
             last_z = z;
         }
     }
 }
 
+/* scan_count (scan/count.hlsl:13-40). */
+void or_scan_count(const or_scan_config *cfg, const uint32_t *lines, const uint32_t *paths,
+                   uint32_t path_record_count, int32_t *backdrops, uint32_t *counts) {
+    uint32_t chunks;
+    uint32_t *bounds = path_aligned_bounds(lines, cfg->line_count, &chunks);
+    ScanJob j = {cfg, lines, paths, path_record_count, backdrops, counts, NULL, NULL, bounds};
+    par_for(chunks, 1u, scan_count_lines, &j);
+    free(bounds);
+}
+
 /* scan_prefix_chunks (scan/prefix_chunks.hlsl:16-38). chunks: 4 words {path, backdrop_offset, segment_start, len}. */
+typedef struct {
+    const or_scan_config *cfg;
+    const uint32_t *chunks, *counts, *paths, *chunk_ranges;
+    uint32_t *ranges, *chunk_totals, *bumps, *chunk_offsets, *cursors;
+} PrefixJob;
+static void prefix_chunk_range(void *ctx, uint32_t k0, uint32_t k1);
+static void chunk_offset_range(void *ctx, uint32_t p0, uint32_t p1);
+static void apply_chunk_range(void *ctx, uint32_t k0, uint32_t k1);
+
 void or_scan_prefix_chunks(const or_scan_config *cfg, const uint32_t *chunks, const uint32_t *counts,
                            uint32_t *ranges, uint32_t *chunk_totals) {
-    for (uint32_t k = 0; k < cfg->scan_chunk_count; ++k) {
+    PrefixJob j = {cfg, chunks, counts, NULL, NULL, ranges, chunk_totals, NULL, NULL, NULL};
+    par_for(cfg->scan_chunk_count, 256u, prefix_chunk_range, &j);
+}
+static void prefix_chunk_range(void *ctx, uint32_t k0, uint32_t k1) {
+    const PrefixJob *j = (const PrefixJob *)ctx;
+    const uint32_t *chunks = j->chunks, *counts = j->counts;
+    uint32_t *ranges = j->ranges, *chunk_totals = j->chunk_totals;
+    for (uint32_t k = k0; k < k1; ++k) {
         const uint32_t *chunk = chunks + (size_t)k * 4u;
         uint32_t inclusive = 0u;
         for (uint32_t lane = 0; lane < 256u; ++lane) {
@@ -231,7 +342,14 @@ void or_scan_prefix_chunks(const or_scan_config *cfg, const uint32_t *chunks, co
 /* scan_chunk_offsets (scan/chunk_offsets.hlsl:14-26). */
 void or_scan_chunk_offsets(const or_scan_config *cfg, const uint32_t *paths, const uint32_t *chunk_ranges,
                            uint32_t *bumps, const uint32_t *chunk_totals, uint32_t *chunk_offsets) {
-    for (uint32_t p = 0; p < cfg->path_count; ++p) {
+    PrefixJob j = {cfg, NULL, NULL, paths, chunk_ranges, NULL, (uint32_t *)chunk_totals, bumps, chunk_offsets, NULL};
+    par_for(cfg->path_count, 1024u, chunk_offset_range, &j);
+}
+static void chunk_offset_range(void *ctx, uint32_t p0, uint32_t p1) {
+    const PrefixJob *j = (const PrefixJob *)ctx;
+    const uint32_t *paths = j->paths, *chunk_ranges = j->chunk_ranges, *chunk_totals = j->chunk_totals;
+    uint32_t *bumps = j->bumps, *chunk_offsets = j->chunk_offsets;
+    for (uint32_t p = p0; p < p1; ++p) {
         uint32_t start = paths[(size_t)p * 19u + 10u];
         uint32_t next = start;
         for (uint32_t c = chunk_ranges[2 * p]; c < chunk_ranges[2 * p + 1]; ++c) {
@@ -245,7 +363,14 @@ void or_scan_chunk_offsets(const or_scan_config *cfg, const uint32_t *paths, con
 /* scan_apply_chunk_offsets (scan/apply_chunk_offsets.hlsl:15-27). */
 void or_scan_apply_chunk_offsets(const or_scan_config *cfg, const uint32_t *chunks, uint32_t *ranges,
                                  uint32_t *cursors, const uint32_t *chunk_offsets) {
-    for (uint32_t k = 0; k < cfg->scan_chunk_count; ++k) {
+    PrefixJob j = {cfg, chunks, NULL, NULL, NULL, ranges, NULL, NULL, (uint32_t *)chunk_offsets, cursors};
+    par_for(cfg->scan_chunk_count, 256u, apply_chunk_range, &j);
+}
+static void apply_chunk_range(void *ctx, uint32_t k0, uint32_t k1) {
+    const PrefixJob *j = (const PrefixJob *)ctx;
+    const uint32_t *chunks = j->chunks, *chunk_offsets = j->chunk_offsets;
+    uint32_t *ranges = j->ranges, *cursors = j->cursors;
+    for (uint32_t k = k0; k < k1; ++k) {
         const uint32_t *chunk = chunks + (size_t)k * 4u;
         uint32_t base = chunk_offsets[k];
         for (uint32_t lane = 0; lane < 256u && lane < chunk[3]; ++lane) {
@@ -335,48 +460,78 @@ static void write_clipped_segment(uint32_t *out, const Traversal *s, uint32_t su
     out[4] = f_bits(edge);
 }
 
-/* scan_emit (scan/emit.hlsl:14-28), lines in index order. */
-void or_scan_emit(const or_scan_config *cfg, const uint32_t *lines, const uint32_t *paths,
-                  uint32_t path_record_count, uint32_t *cursors, uint32_t *segments) {
-    for (uint32_t id = 0; id < cfg->line_count; ++id) {
+static void scan_emit_lines(void *ctx, uint32_t c0, uint32_t c1) {
+    const ScanJob *j = (const ScanJob *)ctx;
+    for (uint32_t id = j->bounds[c0]; id < j->bounds[c1]; ++id) {
         Traversal s;
-        if (!scan_geometry(lines, paths, path_record_count, id, &s)) continue;
+        if (!scan_geometry(j->lines, j->paths, j->path_record_count, id, &s)) continue;
         for (uint32_t i = s.imin; i < s.imax; i++) {
             float z = floorf(s.a * (float)i + s.b);
             int32_t tx, ty;
             scan_tile(&s, i, z, &tx, &ty);
             if (scan_tile_inside(&s, tx, ty)) {
-                uint32_t dst = cursors[scan_tile_offset(&s, tx, ty)]++;
-                if (dst < cfg->segment_capacity) write_clipped_segment(segments + (size_t)dst * 5u, &s, i, z, tx, ty);
+                uint32_t dst = __atomic_fetch_add(&j->cursors[scan_tile_offset(&s, tx, ty)], 1u, __ATOMIC_RELAXED);
+                if (dst < j->cfg->segment_capacity)
+                    write_clipped_segment(j->segments + (size_t)dst * 5u, &s, i, z, tx, ty);
             }
         }
     }
 }
 
+/* scan_emit (scan/emit.hlsl:14-28), each path's lines in index order. */
+void or_scan_emit(const or_scan_config *cfg, const uint32_t *lines, const uint32_t *paths,
+                  uint32_t path_record_count, uint32_t *cursors, uint32_t *segments) {
+    uint32_t chunks;
+    uint32_t *bounds = path_aligned_bounds(lines, cfg->line_count, &chunks);
+    ScanJob j = {cfg, lines, paths, path_record_count, NULL, NULL, cursors, segments, bounds};
+    par_for(chunks, 1u, scan_emit_lines, &j);
+    free(bounds);
+}
+
 /* cumsum_prefix_chunks, cumsum_chunk_offsets, cumsum_apply_chunk_offsets (cumsum.hlsl:16-71). */
+typedef struct {
+    const uint32_t *offs, *lens;
+    uint32_t *totals, *carries;
+    const uint32_t *row_starts, *row_ends;
+    int32_t *backdrops;
+} CumsumJob;
+static void cumsum_prefix_range(void *ctx, uint32_t c0, uint32_t c1) {
+    const CumsumJob *j = (const CumsumJob *)ctx;
+    for (uint32_t c = c0; c < c1; ++c) {
+        uint32_t sum = 0u;
+        for (uint32_t lane = 0; lane < 256u; ++lane) {
+            uint32_t v = lane < j->lens[c] ? (uint32_t)j->backdrops[j->offs[c] + lane] : 0u;
+            sum += v;
+            if (lane < j->lens[c]) j->backdrops[j->offs[c] + lane] = (int32_t)sum;
+        }
+        j->totals[c] = sum;
+    }
+}
+static void cumsum_carry_range(void *ctx, uint32_t r0, uint32_t r1) {
+    const CumsumJob *j = (const CumsumJob *)ctx;
+    for (uint32_t r = r0; r < r1; ++r) {
+        uint32_t carry = 0u;
+        for (uint32_t c = j->row_starts[r]; c < j->row_ends[r]; ++c) {
+            j->carries[c] = carry;
+            carry += j->totals[c];
+        }
+    }
+}
+static void cumsum_apply_range(void *ctx, uint32_t c0, uint32_t c1) {
+    const CumsumJob *j = (const CumsumJob *)ctx;
+    for (uint32_t c = c0; c < c1; ++c)
+        for (uint32_t lane = 0; lane < j->lens[c] && lane < 256u; ++lane)
+            j->backdrops[j->offs[c] + lane] = (int32_t)((uint32_t)j->backdrops[j->offs[c] + lane] + j->carries[c]);
+}
+
 void or_cumsum(uint32_t row_count, uint32_t chunk_count, const uint32_t *offs, const uint32_t *lens,
                const uint32_t *row_starts, const uint32_t *row_ends, int32_t *backdrops) {
     uint32_t *totals = (uint32_t *)calloc(chunk_count ? chunk_count : 1, 4);
     uint32_t *carries = (uint32_t *)calloc(chunk_count ? chunk_count : 1, 4);
-    for (uint32_t c = 0; c < chunk_count; ++c) {
-        uint32_t sum = 0u;
-        for (uint32_t lane = 0; lane < 256u; ++lane) {
-            uint32_t v = lane < lens[c] ? (uint32_t)backdrops[offs[c] + lane] : 0u;
-            sum += v;
-            if (lane < lens[c]) backdrops[offs[c] + lane] = (int32_t)sum;
-        }
-        totals[c] = sum;
-    }
-    for (uint32_t r = 0; r < row_count; ++r) {
-        uint32_t carry = 0u;
-        for (uint32_t c = row_starts[r]; c < row_ends[r]; ++c) {
-            carries[c] = carry;
-            carry += totals[c];
-        }
-    }
-    for (uint32_t c = 0; c < chunk_count; ++c)
-        for (uint32_t lane = 0; lane < lens[c] && lane < 256u; ++lane)
-            backdrops[offs[c] + lane] = (int32_t)((uint32_t)backdrops[offs[c] + lane] + carries[c]);
+    CumsumJob j = {offs, lens, totals, carries, row_starts, row_ends, backdrops};
+    par_for(chunk_count, 1024u, cumsum_prefix_range, &j);
+    par_for(row_count, 1024u, cumsum_carry_range, &j);
+    par_for(chunk_count, 1024u, cumsum_apply_range, &j);
     free(totals);
     free(carries);
 }
@@ -964,8 +1119,24 @@ static Particle draw_particle(const or_coarse_config *cfg, const or_coarse_input
 }
 
 /* coarse_count_bins (coarse/count.hlsl:48-73) + store_tile_counts (coarse/draw_list.hlsli:19-24). */
+typedef struct {
+    const or_coarse_config *cfg;
+    const or_coarse_inputs *in;
+    uint32_t *work, *kinds;
+} CoarseJob;
+static void coarse_count_range(void *ctx, uint32_t t0, uint32_t t1);
+static void coarse_emit_range(void *ctx, uint32_t t0, uint32_t t1);
+
 void or_coarse_count_bins(const or_coarse_config *cfg, const or_coarse_inputs *in, uint32_t *work, uint32_t *kinds) {
-    for (uint32_t tile = 0; tile < cfg->tile_count; ++tile) {
+    CoarseJob j = {cfg, in, work, kinds};
+    par_for(cfg->tile_count, 64u, coarse_count_range, &j);
+}
+static void coarse_count_range(void *ctx, uint32_t t0, uint32_t t1) {
+    const CoarseJob *j = (const CoarseJob *)ctx;
+    const or_coarse_config *cfg = j->cfg;
+    const or_coarse_inputs *in = j->in;
+    uint32_t *work = j->work, *kinds = j->kinds;
+    for (uint32_t tile = t0; tile < t1; ++tile) {
         uint32_t tx = tile % cfg->tiles_width, ty = tile / cfg->tiles_width;
         uint32_t w = stack_wrapper_count(cfg, in, tx, ty);
         uint32_t count = 0;
@@ -1059,7 +1230,15 @@ static uint32_t occluded_prefix(const or_coarse_config *cfg, const uint32_t *wor
 
 /* coarse_emit_bins (coarse/emit.hlsl:82-142) with emit_stack_begins/ends (coarse/emit_stack.hlsli:18-44). */
 void or_coarse_emit_bins(const or_coarse_config *cfg, const or_coarse_inputs *in, uint32_t *work, uint32_t *kinds) {
-    for (uint32_t tile = 0; tile < cfg->tile_count; ++tile) {
+    CoarseJob j = {cfg, in, work, kinds};
+    par_for(cfg->tile_count, 64u, coarse_emit_range, &j);
+}
+static void coarse_emit_range(void *ctx, uint32_t t0, uint32_t t1) {
+    const CoarseJob *j = (const CoarseJob *)ctx;
+    const or_coarse_config *cfg = j->cfg;
+    const or_coarse_inputs *in = j->in;
+    uint32_t *work = j->work, *kinds = j->kinds;
+    for (uint32_t tile = t0; tile < t1; ++tile) {
         uint32_t tx = tile % cfg->tiles_width, ty = tile / cfg->tiles_width;
         uint32_t *rec = work + (size_t)tile * 6u;
         uint32_t px = rec[1], py = rec[2], raster_start = rec[1];
@@ -1423,10 +1602,25 @@ static Px fine_pixel(const or_fine_config *cfg, const uint32_t *draws, const uin
 }
 
 /* fine_tile_main (fine/main.hlsl:23-40) with load_target: reads and writes the RGBA8 target. */
+typedef struct {
+    const or_fine_config *cfg;
+    const uint32_t *draws, *paint, *segments, *work, *kinds;
+    uint32_t *target;
+} FineJob;
+static void fine_range(void *ctx, uint32_t t0, uint32_t t1);
+
 void or_fine(const or_fine_config *cfg, const uint32_t *draws, const uint32_t *paint, const uint32_t *segments,
              const uint32_t *work, const uint32_t *kinds, uint32_t *target) {
+    FineJob j = {cfg, draws, paint, segments, work, kinds, target};
+    par_for(cfg->tile_count, 16u, fine_range, &j);
+}
+static void fine_range(void *ctx, uint32_t t0, uint32_t t1) {
+    const FineJob *j = (const FineJob *)ctx;
+    const or_fine_config *cfg = j->cfg;
+    const uint32_t *draws = j->draws, *paint = j->paint, *segments = j->segments, *work = j->work, *kinds = j->kinds;
+    uint32_t *target = j->target;
     uint8_t *spill = (uint8_t *)malloc(cfg->clip_spill_depth + 1u);
-    for (uint32_t tile = 0; tile < cfg->tile_count; ++tile) {
+    for (uint32_t tile = t0; tile < t1; ++tile) {
         for (uint32_t lane = 0; lane < 256u; ++lane) {
             uint32_t gx = (tile % cfg->tiles_width) * TILE + lane % TILE;
             uint32_t gy = (tile / cfg->tiles_width) * TILE + lane / TILE;
@@ -1463,16 +1657,25 @@ int or_render_frame(const or_scene *s, uint32_t *image, or_frame_outputs *out) {
     uint32_t *offsets = (uint32_t *)calloc(s->scan_chunk_count + 1u, 4);
     uint32_t *segments = (uint32_t *)calloc(5u * (size_t)s->segment_capacity + 5u, 4);
 
+    double t0 = now_ms(), ms[6] = {0, 0, 0, 0, 0, 0}; /* count, prefix, emit, cumsum, coarse, fine */
     or_scan_count(&sc, s->lines, s->paths, s->path_count, backdrops, counts);
+    double t1 = now_ms();
     or_scan_prefix_chunks(&sc, s->scan_chunks, counts, ranges, totals);
     or_scan_chunk_offsets(&sc, s->paths, s->scan_chunk_ranges, bumps, totals, offsets);
     or_scan_apply_chunk_offsets(&sc, s->scan_chunks, ranges, cursors, offsets);
+    double t2 = now_ms();
     or_scan_emit(&sc, s->lines, s->paths, s->path_count, cursors, segments);
+    double t3 = now_ms();
     for (uint32_t p = 0; p < s->path_count; ++p)
         if (bumps[p] > s->segment_capacity) status = 1;
     if (out && out->raw_backdrops) memcpy(out->raw_backdrops, backdrops, (size_t)B * 4u);
+    double t4 = now_ms();
     or_cumsum(s->cumsum_row_count, s->cumsum_chunk_count, s->cumsum_chunk_offsets, s->cumsum_chunk_lens,
               s->cumsum_row_starts, s->cumsum_row_ends, backdrops);
+    ms[0] = t1 - t0;
+    ms[1] = t2 - t1;
+    ms[2] = t3 - t2;
+    ms[3] = now_ms() - t4;
 
     /* root clear (filter_clear_region), then coarse + fine per batch on the RGBA8 target */
     for (size_t i = 0; i < (size_t)s->width * s->height; ++i) image[i] = s->clear_color;
@@ -1507,6 +1710,7 @@ int or_render_frame(const or_scene *s, uint32_t *image, or_frame_outputs *out) {
             work = (uint32_t *)calloc(work_words, 4);
         }
         cc.ptcl_capacity = 0;
+        double tc = now_ms();
         or_coarse_count_bins(&cc, &in, work, kinds);
         or_coarse_prefix(T, work, chunk_records);
         uint32_t total = 0;
@@ -1521,6 +1725,7 @@ int or_render_frame(const or_scene *s, uint32_t *image, or_frame_outputs *out) {
         }
         cc.ptcl_capacity = total;
         or_coarse_emit_bins(&cc, &in, work, kinds);
+        ms[4] += now_ms() - tc;
         if (out) {
             for (uint32_t t = 0; t < T; ++t) {
                 size_t idx = (size_t)b * T + t;
@@ -1549,8 +1754,11 @@ int or_render_frame(const or_scene *s, uint32_t *image, or_frame_outputs *out) {
         fc.paint_brush_base = s->paint_brush_base;
         fc.draw_count = s->draw_count;
         fc.ptcl_base_words = T * 6u;
+        double tf = now_ms();
         or_fine(&fc, s->draws, s->paint, segments, work, kinds, image);
+        ms[5] += now_ms() - tf;
     }
+    if (out && out->stage_ms) memcpy(out->stage_ms, ms, sizeof(ms));
     if (out && out->batch_bases) out->batch_bases[s->batch_count] = (uint32_t)base;
     if (out) {
         if (out->backdrops) memcpy(out->backdrops, backdrops, (size_t)B * 4u);

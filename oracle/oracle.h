@@ -101,7 +101,14 @@ typedef struct or_frame_outputs {
     uint32_t *segment_counts, *segment_ranges, *segment_cursors, *segment_bumps, *segments;
     uint32_t *tile_ptcl_counts, *tile_ptcl_starts, *tile_ptcl_ends, *tile_kinds, *batch_bases;
     uint32_t *ptcl; uint64_t ptcl_capacity_words;
+    double *stage_ms; /* 6 wall-clock times: scan count, segment prefix, scan emit, backdrop cumsum,
+                         coarse (all batches), fine (all batches) */
 } or_frame_outputs;
+/* Threads for the stages whose work items are independent (default 1); the
+ * results do not depend on the count. */
+void or_set_threads(uint32_t n);
+uint32_t or_get_threads(void);
+
 /* Returns 0 on success, nonzero when a buffer bound was exceeded. */
 int or_render_frame(const or_scene *scene, uint32_t *image, or_frame_outputs *out);
 

@@ -12,6 +12,15 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device")
 
 
+@pytest.fixture(autouse=True, scope="session")
+def oracle_threads():
+    # The checker's results do not depend on its thread count; threads only make
+    # the large frames finish sooner.
+    import oracle
+    oracle.set_threads(min(8, len(os.sched_getaffinity(0))))
+    yield
+
+
 @pytest.fixture(scope="session")
 def ctx():
     import tileink_amd as tk

@@ -38,10 +38,13 @@ def read_frame(frame, scene):
     return out
 
 
-def compare_frame(ctx, canvas, clear=0):
-    """Returns (gpu, ref, mismatches) where mismatches lists the differing stages."""
+def compare_frame(ctx, canvas, clear=0, hashed_segments=False):
+    """Returns (gpu, ref, mismatches) where mismatches lists the differing stages.
+    hashed_segments: compare each tile's segment multiset through an order-free
+    hash sum instead of sorting (for frames with tens of millions of segments)."""
     frame, scene = gpu_frame(ctx, canvas, clear)
     gpu = read_frame(frame, scene)
+    del frame
     arrays = canvas.arrays(clear)
     ref_image, ref = oracle.render_frame(arrays)
     ref["image"] = ref_image
@@ -56,8 +59,12 @@ def compare_frame(ctx, canvas, clear=0):
             bad.append(key)
     if not np.array_equal(gpu["segment_ranges"], ref["segment_ranges"]):
         bad.append("segment_ranges")
-    if not np.array_equal(tile_sorted_segments(gpu["segments"], gpu["segment_ranges"]),
-                          tile_sorted_segments(ref["segments"], ref["segment_ranges"])):
+    elif hashed_segments:
+        if not np.array_equal(tile_segment_hashes(gpu["segments"], gpu["segment_ranges"]),
+                              tile_segment_hashes(ref["segments"], ref["segment_ranges"])):
+            bad.append("segments")
+    elif not np.array_equal(tile_sorted_segments(gpu["segments"], gpu["segment_ranges"]),
+                            tile_sorted_segments(ref["segments"], ref["segment_ranges"])):
         bad.append("segments")
     if not np.array_equal(gpu["batch_bases"], ref["batch_bases"]):
         bad.append("batch_bases")
@@ -73,6 +80,23 @@ def compare_frame(ctx, canvas, clear=0):
     if gpu["stats"]["overflow"]:
         bad.append("overflow")
     return gpu, ref, bad
+
+
+def tile_segment_hashes(segments, ranges):
+    """Per backdrop tile, the wrapping sum of a 64-bit hash of each of its segments'
+    five words: equal for equal per-tile segment multisets, in any slot order."""
+    ranges = np.asarray(ranges, np.int64).reshape(-1, 2)
+    words = np.asarray(segments, np.uint32).reshape(-1, 5)
+    h = np.zeros(len(words), np.uint64)
+    with np.errstate(over="ignore"):
+        for k in range(5):
+            h = (h ^ words[:, k].astype(np.uint64)) * np.uint64(0x100000001B3)
+            h ^= h >> np.uint64(29)
+        h *= np.uint64(0x9E3779B97F4A7C15)
+        csum = np.concatenate([np.zeros(1, np.uint64), np.cumsum(h, dtype=np.uint64)])
+        end = np.clip(ranges[:, 1], 0, len(words))
+        start = np.clip(ranges[:, 0], 0, end)
+        return csum[end] - csum[start]
 
 
 def tile_sorted_segments(segments, ranges):

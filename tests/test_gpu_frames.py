@@ -22,6 +22,19 @@ def test_mixed_640x360_frame_matches_oracle(ctx):
     assert not bad, _report(gpu, ref, bad)
 
 
+def test_full_size_4096px_200k_path_frame_matches_oracle(ctx):
+    # The benchmark frame itself (4096x4096, 200,000 paths, 32 nested clips), every
+    # stage against the oracle run threaded on the host cores.
+    import os
+    oracle.set_threads(min(16, len(os.sched_getaffinity(0))))
+    canvas = scenes.clipped_paths_scene(4096, 4096, 200_000, clips=32, workload=2)
+    print("full-size frame: scene generated", flush=True)
+    gpu, ref, bad = compare_frame(ctx, canvas, hashed_segments=True)
+    oracle.set_threads(min(8, len(os.sched_getaffinity(0))))
+    assert not bad, _report(gpu, ref, bad)
+    assert gpu["stats"]["ptcl_records"] == len(ref["ptcl"]) > 3_000_000
+
+
 @pytest.mark.parametrize("scale", [16, 8, 4])
 def test_small_clipped_path_frames_match_oracle(ctx, scale):
     gpu, ref, bad = compare_frame(ctx, scenes.clipped_paths_small(scale))

@@ -37,3 +37,9 @@ def test_device_equals_oracle_on_fixtures(ctx, name):
     canvas = svg.load(svg_golden.read_svg(name), 300)
     expected, _ = oracle.render_frame(canvas.arrays(0))
     assert np.array_equal(render(ctx, canvas), expected)
+
+
+def test_tile_edge_area_pixel_kat(ctx):
+    # native_tile_edge_area_preserves_half_channel_rounding (renderer_rounding_gpu.rs:76-88)
+    image = render(ctx, svg.load(svg_golden.read_svg("shapes/line/simple-case.svg"), 300))
+    assert list(image.view(np.uint8).reshape(300, 300, 4)[132, 103]) == [63, 71, 0, 205]

@@ -10,6 +10,7 @@ Closed forms and brute force pin what those leave open (tile crossings, exact
 pixel areas, winding numbers).
 """
 import ctypes as C
+import os
 from fractions import Fraction
 
 import numpy as np
@@ -491,3 +492,20 @@ def test_sdf_rounded_rect_matches_f64_distance_coverage():
             cov = min(max(0.5 - d, 0.0), 1.0)
             worst = max(worst, abs(alpha[y, x] - int(cov * 255 + 0.5)))
     assert worst <= 1
+
+
+def test_oracle_results_do_not_depend_on_threads():
+    # Threads split lines by path, chunks and tiles; every output is identical to one
+    # thread's, the emitted segment order included.
+    arrays = scenes.clipped_paths_small(8).arrays()
+    runs = []
+    for n in (1, 3, 8):
+        oracle.set_threads(n)
+        runs.append(oracle.render_frame(arrays))
+    oracle.set_threads(min(8, len(os.sched_getaffinity(0))))
+    (image, out) = runs[0]
+    for other_image, other in runs[1:]:
+        assert np.array_equal(image, other_image)
+        for key in out:
+            if key != "stage_ms":
+                assert np.array_equal(out[key], other[key]), key

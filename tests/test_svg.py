@@ -57,3 +57,11 @@ def test_oracle_render_matches_reference_golden(name):
     canvas = svg.load(svg_golden.read_svg(name), 300)
     image, _ = oracle.render_frame(canvas.arrays(0))
     svg_golden.assert_close(image, name)
+
+
+def test_tile_edge_area_pixel_kat():
+    # native_tile_edge_area_preserves_half_channel_rounding (renderer_rounding_gpu.rs:76-88):
+    # shapes/line/simple-case.svg at 300 px, pixel (103, 132) is exactly [63, 71, 0, 205].
+    canvas = svg.load(svg_golden.read_svg("shapes/line/simple-case.svg"), 300)
+    image, _ = oracle.render_frame(canvas.arrays(0))
+    assert list(image.view(np.uint8).reshape(300, 300, 4)[132, 103]) == [63, 71, 0, 205]
