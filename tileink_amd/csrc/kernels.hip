@@ -1106,6 +1106,25 @@ __device__ __forceinline__ void load_list_entry(const tk_args_coarse &a, const t
     }
 }
 
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    return v;
+}
+
+// The smallest batch id at or above `from` among a tile list's entries (kNone if
+// none): the next batch of a list whose entries are not in batch order.
+__device__ uint32_t next_listed_batch(const tk_args_coarse &a, const tk_tile_draw_record &list, uint32_t len,
+                                      uint32_t lane, const SceneView &v, uint32_t from) {
+    uint32_t m = kNone;
+    for (uint32_t base = 0; base < len; base += 64u) {
+        uint32_t d, b;
+        load_list_entry(a, list, base, lane, v, d, b);
+        if (b != kNone && b >= from) m = min(m, b);
+    }
+    return wave_min_u32(m);
+}
+
 // Wrapper validity and count (stack_wrapper_count, coarse/classify.hlsli:11-33)
 // of up to 64 batches at once: lane j describes batch `batch` with `particles`
 // particles; the (batch, layer) pairs of the batches that have particles are laid
@@ -1210,35 +1229,57 @@ __device__ __forceinline__ void coarse_count_tile(const tk_args_coarse &a) {
         }
         if (++pending == 64u) resolve();
     };
-    for (uint32_t base = 0; base < len; base += 64u) {
-        uint32_t d, b;
-        load_list_entry(a, list, base, lane, v, d, b);
-        const bool valid = b != kNone && draw_particle(c, b, d).valid;
-        uint64_t pending_runs = __ballot(b != kNone);
-        const uint64_t valid_mask = __ballot(valid);
-        while (pending_runs) {
-            const uint32_t first = (uint32_t)__ffsll((unsigned long long)pending_runs) - 1u;
-            const uint32_t bb = (uint32_t)__shfl((int)b, (int)first, 64);
-            const uint64_t run = __ballot(b == bb) & pending_runs;
-            pending_runs &= ~run;
-            if (bb != cur) {
-                if (cur != kNone && bb < cur) {  // list not in batch order
-                    if (lane == 0u) atomicOr(&a.out.status[3], 1u);
-                    continue;
+    // A list out of batch order (the recorder never produces one; the stage entry
+    // accepts it, as the reference's per-batch dispatches filter any list by
+    // batch) is counted again one batch at a time, in batch order, each over the
+    // whole list filtered to that batch (`only`); every count is then rewritten.
+    bool disordered = false;
+    uint32_t only = kNone;
+    for (;;) {
+        for (uint32_t base = 0; base < len; base += 64u) {
+            uint32_t d, b;
+            load_list_entry(a, list, base, lane, v, d, b);
+            if (only != kNone && b != only) b = kNone;
+            const bool valid = b != kNone && draw_particle(c, b, d).valid;
+            uint64_t pending_runs = __ballot(b != kNone);
+            const uint64_t valid_mask = __ballot(valid);
+            while (pending_runs) {
+                const uint32_t first = (uint32_t)__ffsll((unsigned long long)pending_runs) - 1u;
+                const uint32_t bb = (uint32_t)__shfl((int)b, (int)first, 64);
+                const uint64_t run = __ballot(b == bb) & pending_runs;
+                pending_runs &= ~run;
+                if (bb != cur) {
+                    if (cur != kNone && bb < cur) {  // a batch met again after a later one
+                        disordered = true;
+                        continue;
+                    }
+                    flush();
+                    cur = bb;
+                    particles = 0u;
                 }
-                flush();
-                cur = bb;
-                particles = 0u;
+                particles += (uint32_t)__popcll(run & valid_mask);
             }
-            particles += (uint32_t)__popcll(run & valid_mask);
         }
+        if (!disordered) break;
+        uint32_t from = only + 1u;
+        if (only == kNone) {  // restart: drop what the first walk held
+            if (lane == 0u) atomicOr(&a.out.status[3], 1u);
+            pending = 0u;
+            next = v.q0;
+            cur = kNone;
+            from = v.q0;
+        }
+        only = next_listed_batch(a, list, len, lane, v, from);
+        if (only == kNone) break;
     }
     flush();
     resolve();
     for (uint32_t bb = next + lane; bb < Q; bb += 64u) counts[table_index(v, bb)] = 0u;
 }
 
-extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count(tk_args_coarse a) { coarse_count_tile<false>(a); }
+extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void tk_k_coarse_count(tk_args_coarse a) {
+    coarse_count_tile<false>(a);
+}
 extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count_scenes(tk_args_coarse a) {
     coarse_count_tile<true>(a);
 }
@@ -1385,51 +1426,71 @@ __device__ __forceinline__ void coarse_emit_tile(const tk_args_coarse &a) {
         }
     };
 
-    for (uint32_t base = 0; base < len; base += 64u) {
-        uint32_t d, b;
-        load_list_entry(a, list, base, lane, v, d, b);
-        // Every lane's particle and class at once (a lane's batch is its own b), so
-        // the particle code runs once per 64 entries however many batches they span.
-        Particle p{false, TK_PTCL_FILL, 0u, 0u, 0u, 0u, 0u};
-        if (b != kNone) p = draw_particle(c, b, d);
-        uint32_t cls = 0u;
-        if (p.valid) cls = particle_class_flags(c, d, p.tag);
-        const bool p_opaque = p.valid && p.tag == TK_PTCL_COLOR && (p.color >> 24) == 255u;
-        const bool p_begins = p.valid && is_begin_tag(p.tag);
-        uint64_t pending = __ballot(b != kNone);
-        while (pending) {
-            const uint32_t first = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
-            const uint32_t bb = (uint32_t)__shfl((int)b, (int)first, 64);
-            const uint64_t run = __ballot(b == bb) & pending;
-            pending &= ~run;
-            if (bb != cur) {
-                if (cur != kNone && bb < cur) continue;
-                finish();
-                empty_until(bb);
-                begin(bb);
+    // A list out of batch order (see coarse_count_tile) is emitted again one batch
+    // at a time in batch order, from the whole list filtered to that batch
+    // (`only`); every stream and kind word of the tile is then rewritten.
+    bool disordered = false;
+    uint32_t only = kNone;
+    for (;;) {
+        for (uint32_t base = 0; base < len; base += 64u) {
+            uint32_t d, b;
+            load_list_entry(a, list, base, lane, v, d, b);
+            if (only != kNone && b != only) b = kNone;
+            // Every lane's particle and class at once (a lane's batch is its own b), so
+            // the particle code runs once per 64 entries however many batches they span.
+            Particle p{false, TK_PTCL_FILL, 0u, 0u, 0u, 0u, 0u};
+            if (b != kNone) p = draw_particle(c, b, d);
+            uint32_t cls = 0u;
+            if (p.valid) cls = particle_class_flags(c, d, p.tag);
+            const bool p_opaque = p.valid && p.tag == TK_PTCL_COLOR && (p.color >> 24) == 255u;
+            const bool p_begins = p.valid && is_begin_tag(p.tag);
+            uint64_t pending = __ballot(b != kNone);
+            while (pending) {
+                const uint32_t first = (uint32_t)__ffsll((unsigned long long)pending) - 1u;
+                const uint32_t bb = (uint32_t)__shfl((int)b, (int)first, 64);
+                const uint64_t run = __ballot(b == bb) & pending;
+                pending &= ~run;
+                if (bb != cur) {
+                    if (cur != kNone && bb < cur) {  // a batch met again after a later one
+                        disordered = true;
+                        continue;
+                    }
+                    finish();
+                    empty_until(bb);
+                    begin(bb);
+                }
+                if (!active) continue;
+                const bool mine = (run >> lane) & 1ull;
+                const bool emits = mine && p.valid;
+                const uint64_t valid = __ballot(emits);
+                const uint32_t pos = dst + (uint32_t)__popcll(valid & lanes_below(lane));
+                const uint32_t my_cls = emits ? cls : 0u;
+                const bool opaque = emits && p_opaque, begins = emits && p_begins;
+                if (emits && pos < end) e.store(pos, p.tag, p.winding, p.fill_rule, p.seg_start, p.seg_end, p.color);
+                if (__ballot(my_cls & kClassColor)) flags |= kClassColor;
+                if (__ballot(my_cls & kClassSdf)) flags |= kClassSdf;
+                if (__ballot(my_cls & kClassOther)) flags |= kClassOther;
+                if (__ballot(opaque && wrappers == 0u && pos > start)) flags |= 8u;
+                const uint64_t opaque_mask = __ballot(opaque), begin_mask = __ballot(begins);
+                if (opaque_mask) {
+                    const uint32_t h = 63u - (uint32_t)__clzll((long long)opaque_mask);
+                    last_opaque = dst + (uint32_t)__popcll(valid & lanes_below(h)) - start;
+                    begin_after = (begin_mask >> h) > 1ull;
+                } else if (begin_mask) {
+                    begin_after = true;
+                }
+                dst += (uint32_t)__popcll(valid);
             }
-            if (!active) continue;
-            const bool mine = (run >> lane) & 1ull;
-            const bool emits = mine && p.valid;
-            const uint64_t valid = __ballot(emits);
-            const uint32_t pos = dst + (uint32_t)__popcll(valid & lanes_below(lane));
-            const uint32_t my_cls = emits ? cls : 0u;
-            const bool opaque = emits && p_opaque, begins = emits && p_begins;
-            if (emits && pos < end) e.store(pos, p.tag, p.winding, p.fill_rule, p.seg_start, p.seg_end, p.color);
-            if (__ballot(my_cls & kClassColor)) flags |= kClassColor;
-            if (__ballot(my_cls & kClassSdf)) flags |= kClassSdf;
-            if (__ballot(my_cls & kClassOther)) flags |= kClassOther;
-            if (__ballot(opaque && wrappers == 0u && pos > start)) flags |= 8u;
-            const uint64_t opaque_mask = __ballot(opaque), begin_mask = __ballot(begins);
-            if (opaque_mask) {
-                const uint32_t h = 63u - (uint32_t)__clzll((long long)opaque_mask);
-                last_opaque = dst + (uint32_t)__popcll(valid & lanes_below(h)) - start;
-                begin_after = (begin_mask >> h) > 1ull;
-            } else if (begin_mask) {
-                begin_after = true;
-            }
-            dst += (uint32_t)__popcll(valid);
         }
+        if (!disordered) break;
+        uint32_t from = only + 1u;
+        if (only == kNone) {  // restart: the first walk's streams are all rewritten
+            cur = kNone;
+            next = v.q0;
+            from = v.q0;
+        }
+        only = next_listed_batch(a, list, len, lane, v, from);
+        if (only == kNone) break;
     }
     finish();
     empty_until(Q);
