@@ -544,6 +544,24 @@ typedef struct tk_coarse_outputs {
 tk_status tk_coarse(tk_context *ctx, const tk_coarse_config *config, const tk_coarse_inputs *in,
                     const tk_coarse_outputs *out);
 
+/* tk_coarse without device allocation.  It has no counterpart in the reference;
+ * it is for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Inputs,
+ * outputs and results are those of tk_coarse.  The coarse scan's lookback ticket
+ * and status words (tk_coarse's only scratch, which it grows with hipMalloc)
+ * live instead in one fixed array of TK_NO_ALLOC_SCRATCH_BYTES per device,
+ * allocated by the HIP runtime when the library is loaded and held until the
+ * process ends; each call clears the part it uses with a fill kernel (no
+ * hipMemsetAsync).  batch_count * tile_count may not exceed
+ * TK_NO_ALLOC_MAX_TABLE; larger calls fail with TK_ERR_INVALID and launch
+ * nothing.  Every call on one GPU shares that GPU's array: calls that overlap
+ * on one device (other streams, other contexts, other threads) race. */
+#define TK_NO_ALLOC_SCRATCH_BYTES (1u << 20)
+#define TK_NO_ALLOC_TICKET_BYTES 256u
+#define TK_NO_ALLOC_MAX_TABLE 536739840ull /* ((1 MiB - 256) / 8) scan workgroups x 4096 counts */
+tk_status tk_coarse_no_alloc(tk_context *ctx, const tk_coarse_config *config, const tk_coarse_inputs *in,
+                             const tk_coarse_outputs *out);
+
 /* Fine, all batches at once: one 256-lane workgroup per 16x16 tile walks the
  * batches in order, re-quantising to RGBA8 at each batch boundary exactly where
  * the reference stores and re-loads its RGBA8 target (fine/main.hlsl:23-40,

@@ -95,3 +95,15 @@ def test_version_and_error_string():
     with pytest.raises(tk.TileinkError):
         tk.Canvas(16, 16).pop_layer()
     assert b"layer" in _abi.lib().tk_last_error()
+
+
+def test_no_alloc_limits_match_the_header():
+    text = open(HEADER).read()
+    scratch = re.search(r"#define TK_NO_ALLOC_SCRATCH_BYTES \(1u << (\d+)\)", text)
+    table = re.search(r"#define TK_NO_ALLOC_MAX_TABLE (\d+)ull", text)
+    ticket = re.search(r"#define TK_NO_ALLOC_TICKET_BYTES (\d+)u", text)
+    assert scratch and table and ticket
+    assert _abi.TK_NO_ALLOC_SCRATCH_BYTES == 1 << int(scratch.group(1))
+    assert _abi.TK_NO_ALLOC_MAX_TABLE == int(table.group(1))
+    # one u64 lookback status word per coarse scan workgroup of 4096 counts
+    assert _abi.TK_NO_ALLOC_MAX_TABLE == (_abi.TK_NO_ALLOC_SCRATCH_BYTES - int(ticket.group(1))) // 8 * 4096

@@ -15,6 +15,9 @@ LIB_PATH = os.path.join(_HERE, "libtileink_hip.so")
 TK_OK, TK_ERR_INVALID, TK_ERR_HIP, TK_ERR_OOM, TK_ERR_UNAVAILABLE, TK_ERR_OVERFLOW = range(6)
 STATUS_NAMES = {0: "ok", 1: "invalid", 2: "hip", 3: "oom", 4: "unavailable", 5: "overflow"}
 INVALID = 0xFFFFFFFF
+# tk_coarse_no_alloc's fixed scratch (include/tileink_hip.h)
+TK_NO_ALLOC_SCRATCH_BYTES = 1 << 20
+TK_NO_ALLOC_MAX_TABLE = 536_739_840
 
 PATH_MOVE, PATH_LINE, PATH_QUAD, PATH_CUBIC, PATH_CLOSE = range(5)
 FILL_NONZERO, FILL_EVEN_ODD = 0, 1
@@ -228,6 +231,7 @@ SIGNATURES = {
     "tk_scan_emit": (C.c_int, [_P, C.POINTER(ScanConfig), _P, _P, _U32, _P, _P]),
     "tk_cumsum": (C.c_int, [_P, C.POINTER(CumsumConfig), _P, _P, _P, _P, _P, _P, _P]),
     "tk_coarse": (C.c_int, [_P, C.POINTER(CoarseConfig), C.POINTER(CoarseInputs), C.POINTER(CoarseOutputs)]),
+    "tk_coarse_no_alloc": (C.c_int, [_P, C.POINTER(CoarseConfig), C.POINTER(CoarseInputs), C.POINTER(CoarseOutputs)]),
     "tk_fine": (C.c_int, [_P, C.POINTER(FineConfig), C.POINTER(FineInputs), _P]),
     "tk_fill_coverage_words": (C.c_int, [_P, _U32, _P, _P, _P, _P]),
     "tk_pixel_math_words": (C.c_int, [_P, _U32, _P, _P]),

@@ -37,6 +37,12 @@ def build(verbose=True):
         _run(["g++", *HOST_FLAGS, "-I", INCLUDE, "-I", CSRC, "-I", os.path.join(ROCM, "include"),
               "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # The no-alloc scratch array: an ordinary HIP translation unit (host and device
+    # parts), so that the HIP runtime registers and allocates its device array.
+    scratch = os.path.join(BUILD, "scratch.o")
+    _run([os.path.join(ROCM, "bin", "hipcc"), "-c", "-fPIC", "--no-offload-compress", *DEVICE_FLAGS, "-I", INCLUDE,
+          "-I", CSRC, os.path.join(CSRC, "scratch.hip"), "-o", scratch])
+    objs.append(scratch)
     blob = os.path.join(BUILD, "code_object.o")
     _run(["gcc", "-c", f'-DTK_HSACO_PATH="{hsaco}"', "-x", "assembler-with-cpp",
           os.path.join(CSRC, "code_object.S"), "-o", blob])

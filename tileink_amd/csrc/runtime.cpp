@@ -86,6 +86,9 @@ struct tk_context {
     size_t status_words = 0;
     uint32_t *draw_brush = nullptr;  // fine's per-draw brush words (tk_fine)
     size_t draw_brush_words = 0;
+    // This device's copy of the fixed scratch array of tk_coarse_no_alloc (not
+    // owned: the HIP runtime holds it until the process ends).
+    uint8_t *no_alloc_scratch = nullptr;
     uint32_t epoch = 0;
     uint32_t next_epoch() {
         epoch = (epoch + 1u) & 0x3FFFFFFFu;
@@ -976,6 +979,12 @@ tk_status tk_context_create(int device, tk_context **out) {
     }
     if (hipMalloc(&ctx->tickets, 16) != hipSuccess) return cleanup(fail(TK_ERR_OOM, "ticket allocation failed"));
     if (hipMemset(ctx->tickets, 0, 16) != hipSuccess) return cleanup(fail(TK_ERR_HIP, "ticket clear failed"));
+    {
+        void *scratch = nullptr;
+        const tk_status st = scratch_address(device, &scratch);
+        if (st != TK_OK) return cleanup(st);
+        ctx->no_alloc_scratch = static_cast<uint8_t *>(scratch);
+    }
     *out = ctx;
     return TK_OK;
 }
@@ -1685,6 +1694,33 @@ tk_status tk_coarse(tk_context *ctx, const tk_coarse_config *cfg, const tk_coars
         return fail(TK_ERR_INVALID, "tile_count must equal tiles_width * tiles_height");
     TK_TRY(ensure_status(ctx, coarse_scan_blocks(*cfg)));
     return run_coarse(ctx, *cfg, *in, *out, ctx->status, &ctx->tickets[1], ctx->next_epoch(), nullptr);
+}
+
+// The lookback ticket sits in the first 256 bytes of the scratch array, the
+// status words (one u64 per scan workgroup) after it.  Both are cleared by a fill
+// kernel at every call, so the epoch can stay fixed and nothing a previous call
+// left behind is read.
+static_assert(TK_NO_ALLOC_MAX_TABLE ==
+                  uint64_t(TK_NO_ALLOC_SCRATCH_BYTES - TK_NO_ALLOC_TICKET_BYTES) / 8u * TK_COARSE_SCAN_ITEMS,
+              "the no-alloc table limit follows from the scratch size");
+tk_status tk_coarse_no_alloc(tk_context *ctx, const tk_coarse_config *cfg, const tk_coarse_inputs *in,
+                             const tk_coarse_outputs *out) {
+    if (!ctx || !cfg || !in || !out) return fail(TK_ERR_INVALID, "null argument");
+    if (uint64_t(cfg->tiles_width) * cfg->tiles_height != cfg->tile_count)
+        return fail(TK_ERR_INVALID, "tile_count must equal tiles_width * tiles_height");
+    if (!ctx->no_alloc_scratch) return fail(TK_ERR_HIP, "the context has no no-alloc scratch");
+    const uint64_t entries = uint64_t(cfg->batch_count) * cfg->tile_count;
+    const uint32_t blocks = coarse_scan_blocks(*cfg);
+    const uint64_t bytes = TK_NO_ALLOC_TICKET_BYTES + uint64_t(blocks) * sizeof(unsigned long long);
+    if (bytes > TK_NO_ALLOC_SCRATCH_BYTES) {
+        set_error("batch_count * tile_count = %llu exceeds the %llu entries tk_coarse_no_alloc supports",
+                  (unsigned long long)entries, (unsigned long long)TK_NO_ALLOC_MAX_TABLE);
+        return TK_ERR_INVALID;
+    }
+    uint8_t *scratch = ctx->no_alloc_scratch;
+    TK_TRY(fill_u32(ctx, scratch, bytes / 4, 0u));
+    return run_coarse(ctx, *cfg, *in, *out, reinterpret_cast<unsigned long long *>(scratch + TK_NO_ALLOC_TICKET_BYTES),
+                      reinterpret_cast<uint32_t *>(scratch), 1u, nullptr);
 }
 
 tk_status tk_fine(tk_context *ctx, const tk_fine_config *cfg, const tk_fine_inputs *in, uint32_t *rgba) {

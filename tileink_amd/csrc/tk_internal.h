@@ -18,6 +18,11 @@ inline tk_status fail(tk_status status, const char *what) {
     return status;
 }
 
+// Device address of `device`'s copy of the fixed no-alloc scratch array
+// (TK_NO_ALLOC_SCRATCH_BYTES bytes, 256-byte aligned); `device` must be current.
+// Taken once per device with hipGetSymbolAddress (scratch.hip).
+tk_status scratch_address(int device, void **out);
+
 // Record sizes are part of the ABI; a mismatch is a build error, not a runtime one.
 static_assert(sizeof(tk_line) == 24, "Line is 24 bytes (src/shared/line.rs:4-9)");
 static_assert(sizeof(tk_path_record) == 76, "PathRecord is 76 bytes (src/shared/path.rs:6-23)");
