@@ -146,19 +146,24 @@ def incremental_render(frame, tiles_w, tiles_h, steps, side=32):
             "stage_ms": {k: round(v, 4) for k, v in total.items()}}
 
 
-def scene_loop(ctx, count=64, repeats=5):
-    """Many independent scenes rendered one after another (the per-scene loop):
-    sizes from the independent-scene distribution (side log-uniform 64-2048 px,
-    aspect 0.5-2) with path counts lognormal(ln 150, 0.8) clipped to 10-2000 so the
-    scenes generate in seconds.  Each scene is its own frame; the loop is timed with
-    kernel-by-kernel launches and with each frame replayed from its hipGraph.  The
-    same scenes are then rendered as one batch (tk_batch_*: one frame, one launch
-    per stage for all of them), eager and replayed from a hipGraph."""
-    sizes = scenes.scene_set_sizes(count, path_mu=np.log(150.0), path_sigma=0.8, max_paths=2000)
-    canvases = [scenes.scene_set_member(w, h, n, i) for i, (w, h, n) in enumerate(sizes)]
-    encoded = [c.encode(0) for c in canvases]
+def scene_loop(ctx, count=512, repeats=3):
+    """The independent-scene workload: 512 scenes, canvas side log-uniform in
+    64-2048 px, aspect 0.5-2, path counts lognormal(ln 1500, 1.2) clipped to
+    10-20 000, each recorded by the clipped-paths generator (scenes.scene_set, in
+    worker processes).  Each scene is its own frame in the per-scene loop, timed
+    with kernel-by-kernel launches and with each frame replayed from its hipGraph;
+    the loop issues every frame's launches back to back on the context's stream
+    with no host synchronisation between scenes.  The same scenes are then
+    rendered as one batch (tk_batch_*: one frame, one launch per stage for all of
+    them), eager and replayed from a hipGraph."""
+    t0 = time.perf_counter()
+    sizes = scenes.scene_set_sizes(count)
+    arrays = scenes.scene_set(sizes)
+    encoded = [a.to_scene() for a in arrays]
+    log(f"scene set: {count} scenes, {sum(n for _, _, n in sizes)} paths generated in {time.perf_counter() - t0:.1f} s")
     frames = [tk.Frame(ctx, sc) for sc in encoded]
     out = {"scenes": count, "paths": int(sum(n for _, _, n in sizes)),
+           "lines": int(sum(a.line_count for a in arrays)),
            "pixels": int(sum(w * h for w, h, _ in sizes))}
 
     def timed(name, render_all):
@@ -179,10 +184,14 @@ def scene_loop(ctx, count=64, repeats=5):
     out["graph_speedup"] = round(out["eager_ms"] / out["graph_ms"], 3)
     del frames
     batch = tk.Batch(ctx, encoded)
+    t0 = time.perf_counter()
+    out["batch_tuning"] = batch.frame.autotune(repeats=2)
+    log(f"scene set: batch autotune {out['batch_tuning']} in {time.perf_counter() - t0:.1f} s")
     for mode in ("eager", "graph"):
         batch.frame.set_graph(mode == "graph")
         timed(f"batch_{mode}", batch.render)
-    out["batch_speedup"] = round(out["graph_ms"] / min(out["batch_eager_ms"], out["batch_graph_ms"]), 3)
+    loop_ms = min(out["eager_ms"], out["graph_ms"])
+    out["batch_speedup"] = round(loop_ms / min(out["batch_eager_ms"], out["batch_graph_ms"]), 3)
     del batch
     return out
 

@@ -349,16 +349,36 @@ tk_status tk_frame_render(tk_frame *frame);
  * tk_frame_read_buffer hold one column per listed tile, in list order. */
 tk_status tk_frame_render_tiles(tk_frame *frame, const uint32_t *tiles, uint32_t tile_count);
 /* Launch parameters of a frame's kernels (no counterpart in the reference, whose
- * dispatch shapes are fixed).  Each *_xcd_run is how many consecutive workgroups
- * one XCD takes before the next XCD's run: a power of two up to 1024, or 0 for
- * the plain launch order.  Scan counts 256-line blocks, coarse four-tile
- * workgroups, fine (and the differentiable kernels) tiles.  The 16x16 tile and
- * the workgroup sizes are fixed by the record formats and the kernels' lane maps.
+ * dispatch shapes are fixed).
+ *  - *_xcd_run: how many consecutive workgroups (scan: line blocks; coarse:
+ *    workgroups of coarse_block tiles; fine and the differentiable kernels:
+ *    tiles) one XCD takes before the next XCD's run: a power of two up to 1024,
+ *    or 0 for the plain launch order.
+ *  - persistent: bit 0 (TK_PERSISTENT_COARSE) runs coarse count and emit, bit 1
+ *    (TK_PERSISTENT_FINE) runs fine, as persistent launches: as many workgroups
+ *    as are resident at once, each wave (coarse) or workgroup (fine) taking
+ *    tiles from eight per-XCD work queues (each XCD's queue holds the runs the
+ *    plain launch would deal it; an XCD whose queue is used up takes from the
+ *    others) until all are used up.
+ *  - scan_block: lines per scan count / emit workgroup: 64, 128, 256, 512, 1024.
+ *  - coarse_block: tiles (one 64-lane wave each) per coarse workgroup: 1, 2, 4.
+ *  - coarse_scan_items: counts per workgroup of the coarse exclusive scan
+ *    (decoupled lookback): 1024, 2048, 4096, 8192.
+ * The tile is 16x16 pixels: the record formats the reference's GPU ABI fixes
+ * (tile-local segment coordinates in [0, 16], 256-tile scan chunks, the PTCL)
+ * and the bit-exact contract with its results depend on it, so it is not a
+ * tuning choice; nor is fine's workgroup, one lane per pixel of a tile.
  * tk_frame_autotune renders the frame's own scene with candidate values (one
- * warm-up and `repeats` timed renders each, stage times from HIP events), keeps
- * the fastest value per stage and returns the result in *chosen.               */
+ * warm-up and `repeats` timed renders each, stage times from HIP events), and
+ * searches one parameter at a time with the others held at the best so far;
+ * it returns the result in *chosen.  No choice changes a result.             */
+#define TK_PERSISTENT_COARSE 1u
+#define TK_PERSISTENT_FINE 2u
 typedef struct tk_tuning {
-    uint32_t scan_xcd_run, coarse_xcd_run, fine_xcd_run, reserved;
+    uint32_t scan_xcd_run, coarse_xcd_run, fine_xcd_run;
+    uint32_t persistent;
+    uint32_t scan_block, coarse_block, coarse_scan_items;
+    uint32_t reserved;
 } tk_tuning;
 tk_status tk_frame_get_tuning(tk_frame *frame, tk_tuning *out);
 tk_status tk_frame_set_tuning(tk_frame *frame, const tk_tuning *tuning);

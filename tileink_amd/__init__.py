@@ -474,19 +474,25 @@ class Frame:
         """The launch parameters as a dict (tk_frame_get_tuning)."""
         t = _abi.Tuning()
         check(lib().tk_frame_get_tuning(self._h, C.byref(t)))
-        return {"scan_xcd_run": t.scan_xcd_run, "coarse_xcd_run": t.coarse_xcd_run, "fine_xcd_run": t.fine_xcd_run}
+        return {k: getattr(t, k) for k in _abi.TUNING_FIELDS}
 
-    def set_tuning(self, scan_xcd_run=None, coarse_xcd_run=None, fine_xcd_run=None):
+    def set_tuning(self, scan_xcd_run=None, coarse_xcd_run=None, fine_xcd_run=None, **more):
+        """Sets launch parameters (tk_frame_set_tuning); those not given keep their
+        values.  Keywords: the tk_tuning fields (persistent, scan_block,
+        coarse_block, coarse_scan_items and the three XCD run lengths)."""
         cur = self.tuning()
-        t = _abi.Tuning(*(cur[k] if v is None else int(v) for k, v in (
-            ("scan_xcd_run", scan_xcd_run), ("coarse_xcd_run", coarse_xcd_run), ("fine_xcd_run", fine_xcd_run))), 0)
+        given = dict(scan_xcd_run=scan_xcd_run, coarse_xcd_run=coarse_xcd_run, fine_xcd_run=fine_xcd_run, **more)
+        unknown = set(given) - set(_abi.TUNING_FIELDS)
+        if unknown:
+            raise TileinkError(_abi.TK_ERR_INVALID, f"unknown tuning fields {sorted(unknown)}")
+        t = _abi.Tuning(*(cur[k] if given.get(k) is None else int(given[k]) for k in _abi.TUNING_FIELDS), 0)
         check(lib().tk_frame_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, repeats=3):
         """Times candidate launch parameters on this scene and keeps the fastest (tk_frame_autotune)."""
         t = _abi.Tuning()
         check(lib().tk_frame_autotune(self._h, int(repeats), C.byref(t)))
-        return {"scan_xcd_run": t.scan_xcd_run, "coarse_xcd_run": t.coarse_xcd_run, "fine_xcd_run": t.fine_xcd_run}
+        return {k: getattr(t, k) for k in _abi.TUNING_FIELDS}
 
     def set_graph(self, enable=True):
         """Replay full renders from a captured hipGraph (tk_frame_set_graph)."""

@@ -128,8 +128,13 @@ class StrokeStyle(C.Structure):
                 ("end_cap", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+TUNING_FIELDS = ("scan_xcd_run", "coarse_xcd_run", "fine_xcd_run", "persistent", "scan_block", "coarse_block",
+                 "coarse_scan_items")
+TK_PERSISTENT_COARSE, TK_PERSISTENT_FINE = 1, 2
+
+
 class Tuning(C.Structure):
-    _fields_ = [(n, C.c_uint32) for n in ("scan_xcd_run", "coarse_xcd_run", "fine_xcd_run", "reserved")]
+    _fields_ = [(n, C.c_uint32) for n in TUNING_FIELDS + ("reserved",)]
 
 
 class CoarseConfig(C.Structure):

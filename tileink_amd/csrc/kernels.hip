@@ -92,6 +92,27 @@ __device__ __forceinline__ uint32_t xcd_grid(uint64_t blocks, uint32_t chunk) {
     return (uint32_t)((blocks + unit - 1u) & ~(unit - 1u));
 }
 
+// Work queues of the persistent launches: eight counters, one per XCD.  The
+// k-th item of XCD x is xcd_chunked_block(8k + x, run), the runs that the
+// one-workgroup-per-item launch deals to that XCD, so each XCD walks its own
+// runs in order and keeps their cache lines in its L2.  A caller whose XCD's
+// items are used up takes items from the other XCDs' counters in turn (`qi`
+// counts the counters it has left behind; it starts at 0 on its home XCD).
+// Returns kNone once all eight are used up.  Every call advances a counter, so
+// every caller reaches kNone after finitely many calls.
+__device__ uint32_t queue_next(uint32_t *queue, uint32_t home, uint32_t &qi, uint32_t run, uint32_t items) {
+    while (qi < 8u) {
+        const uint32_t x = (home + qi) & 7u;
+        const uint64_t linear = (uint64_t)atomicAdd(&queue[x], 1u) * 8u + x;
+        if (linear <= 0xFFFFFFFFull) {
+            const uint32_t item = xcd_chunked_block((uint32_t)linear, run);
+            if (item < items) return item;
+        }
+        ++qi;
+    }
+    return ~0u;
+}
+
 __device__ __forceinline__ unsigned long long pack_status(uint32_t value, uint32_t flag, uint32_t epoch) {
     return (unsigned long long)value | ((unsigned long long)flag << 32) | ((unsigned long long)(epoch & 0x3FFFFFFFu) << 34);
 }
@@ -708,9 +729,9 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_active_paths(tk_args_acti
 }
 
 // scan_count (scan/count.hlsl:13-40).
-extern "C" __global__ __launch_bounds__(256) void tk_k_scan_count(tk_args_scan_lines a) {
+extern "C" __global__ __launch_bounds__(1024) void tk_k_scan_count(tk_args_scan_lines a) {
     const uint32_t block = xcd_chunked_block(blockIdx.x, a.xcd_run);
-    uint32_t id = block * 256u + threadIdx.x;
+    uint32_t id = block * blockDim.x + threadIdx.x;  // lines per workgroup: a launch parameter
     if (id >= a.cfg.line_count) return;
     if (a.path_active) {  // incremental: only paths whose tile box meets the damage
         const uint32_t path = a.lines[id].path_id;
@@ -873,9 +894,9 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_chunk_apply(tk_args_
 }
 
 // scan_emit (scan/emit.hlsl:14-28).
-extern "C" __global__ __launch_bounds__(256) void tk_k_scan_emit(tk_args_scan_lines a) {
+extern "C" __global__ __launch_bounds__(1024) void tk_k_scan_emit(tk_args_scan_lines a) {
     const uint32_t block = xcd_chunked_block(blockIdx.x, a.xcd_run);
-    uint32_t id = block * 256u + threadIdx.x;
+    uint32_t id = block * blockDim.x + threadIdx.x;  // lines per workgroup: a launch parameter
     if (id >= a.cfg.line_count) return;
     if (a.path_active) {  // incremental: only paths whose tile box meets the damage
         const uint32_t path = a.lines[id].path_id;
@@ -1187,12 +1208,11 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_layer_classes(tk_args_lay
 // (stack_wrappers_of_batches) once 64 are pending or the list ends.
 // kScenes: a batch of scenes (per-tile canvases from a.scenes); else one canvas.
 template <bool kScenes>
-__device__ __forceinline__ void coarse_count_tile(const tk_args_coarse &a) {
+__device__ __forceinline__ void coarse_count_tile(const tk_args_coarse &a, uint32_t column) {
     __shared__ uint32_t invalid_flags[4][64], counted[4][64];
     const tk_coarse_config &cfg = a.cfg;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t column = xcd_chunked_block(blockIdx.x, a.xcd_run) * 4u + wave;
     if (column >= a.list.columns) return;
     const SceneView v = scene_view<kScenes>(a.scenes, column, a.list.columns, cfg.batch_count, cfg.tiles_width,
                                             cfg.tiles_height, 0u, 0u, 0u);
@@ -1277,20 +1297,50 @@ __device__ __forceinline__ void coarse_count_tile(const tk_args_coarse &a) {
     for (uint32_t bb = next + lane; bb < Q; bb += 64u) counts[table_index(v, bb)] = 0u;
 }
 
+// A coarse launch: one tile per wave, up to four per workgroup in XCD runs, or
+// (kPersistent: a grid of resident workgroups) each wave taking tiles from the
+// per-XCD work queues (the eight counters at a.queue + first) until they are
+// used up.
+template <bool kPersistent, class TileFn>
+__device__ __forceinline__ void coarse_tiles(const tk_args_coarse &a, uint32_t first, TileFn tile) {
+    if (!kPersistent) {  // blockDim.x / 64 tiles per workgroup (1, 2 or 4: a launch choice)
+        tile(xcd_chunked_block(blockIdx.x, a.xcd_run) * (blockDim.x >> 6) + (threadIdx.x >> 6));
+        return;
+    }
+    uint32_t *counters = a.queue + first;
+    uint32_t qi = 0u;
+    for (;;) {
+        uint32_t column = 0u;
+        if ((threadIdx.x & 63u) == 0u) column = queue_next(counters, blockIdx.x & 7u, qi, a.xcd_run * (blockDim.x >> 6), a.list.columns);
+        column = (uint32_t)__shfl((int)column, 0, 64);
+        if (column == ~0u) return;
+        tile(column);
+    }
+}
+
 extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void tk_k_coarse_count(tk_args_coarse a) {
-    coarse_count_tile<false>(a);
+    coarse_tiles<false>(a, 0u, [&](uint32_t column) { coarse_count_tile<false>(a, column); });
 }
 extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count_scenes(tk_args_coarse a) {
-    coarse_count_tile<true>(a);
+    coarse_tiles<false>(a, 0u, [&](uint32_t column) { coarse_count_tile<true>(a, column); });
+}
+extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void tk_k_coarse_count_persistent(
+    tk_args_coarse a) {
+    coarse_tiles<true>(a, 0u, [&](uint32_t column) { coarse_count_tile<false>(a, column); });
+}
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_count_scenes_persistent(tk_args_coarse a) {
+    coarse_tiles<true>(a, 0u, [&](uint32_t column) { coarse_count_tile<true>(a, column); });
 }
 
 // coarse_prefix_chunks + coarse_chunk_offsets + coarse_apply_chunk_offsets for
 // every batch: one wrapping exclusive scan over the batch-major count table.
 // A batch's reference (per-batch) offsets are start - batch_ptcl_bases[batch].
-extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coarse a) {
+// kItems consecutive counts per lane, 256 * kItems per workgroup (a.scan_blocks
+// workgroups): the scan block size is a launch choice among 1024-8192 counts.
+template <uint32_t kItems>
+__device__ __forceinline__ void coarse_scan(const tk_args_coarse &a) {
     __shared__ uint32_t s_ticket, s_base;
     __shared__ uint32_t s_wave[4];
-    constexpr uint32_t kItems = TK_COARSE_SCAN_ITEMS / 256u;  // 16 consecutive counts per lane
     const uint32_t k = draw_ticket(a.ticket, a.scan_blocks, &s_ticket);
     const uint64_t n = a.scenes.slots ? a.scenes.table_entries : (uint64_t)a.cfg.batch_count * a.list.columns;
     const uint64_t first = ((uint64_t)k * 256u + threadIdx.x) * kItems;
@@ -1349,16 +1399,22 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coars
     }
 }
 
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan(tk_args_coarse a) {
+    coarse_scan<TK_COARSE_SCAN_ITEMS / 256u>(a);  // 4096 counts per workgroup
+}
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan_1k(tk_args_coarse a) { coarse_scan<4u>(a); }
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan_2k(tk_args_coarse a) { coarse_scan<8u>(a); }
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_scan_8k(tk_args_coarse a) { coarse_scan<32u>(a); }
+
 // coarse_emit_bins for every batch (coarse/emit.hlsl:82-142): writes
 // [stack begins][particles][stack ends][END] for each (batch, tile) with a
 // nonzero count and the tile kind word, including the opaque-prefix skip that
 // occluded_particle_prefix (coarse/occlusion.hlsli:12-34) derives: the last
 // opaque colour particle, unless a begin record follows it.
 template <bool kScenes>
-__device__ __forceinline__ void coarse_emit_tile(const tk_args_coarse &a) {
+__device__ __forceinline__ void coarse_emit_tile(const tk_args_coarse &a, uint32_t column) {
     const tk_coarse_config &cfg = a.cfg;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t column = xcd_chunked_block(blockIdx.x, a.xcd_run) * 4u + (threadIdx.x >> 6);
     if (column >= a.list.columns) return;
     const SceneView v = scene_view<kScenes>(a.scenes, column, a.list.columns, cfg.batch_count, cfg.tiles_width,
                                             cfg.tiles_height, 0u, 0u, 0u);
@@ -1496,9 +1552,17 @@ __device__ __forceinline__ void coarse_emit_tile(const tk_args_coarse &a) {
     empty_until(Q);
 }
 
-extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coarse a) { coarse_emit_tile<false>(a); }
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit(tk_args_coarse a) {
+    coarse_tiles<false>(a, 8u, [&](uint32_t column) { coarse_emit_tile<false>(a, column); });
+}
 extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit_scenes(tk_args_coarse a) {
-    coarse_emit_tile<true>(a);
+    coarse_tiles<false>(a, 8u, [&](uint32_t column) { coarse_emit_tile<true>(a, column); });
+}
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit_persistent(tk_args_coarse a) {
+    coarse_tiles<true>(a, 8u, [&](uint32_t column) { coarse_emit_tile<false>(a, column); });
+}
+extern "C" __global__ __launch_bounds__(256) void tk_k_coarse_emit_scenes_persistent(tk_args_coarse a) {
+    coarse_tiles<true>(a, 8u, [&](uint32_t column) { coarse_emit_tile<true>(a, column); });
 }
 
 // The colour sample_draw_brush returns for a draw whose brush does not depend on
@@ -1523,7 +1587,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_draw_brush(tk_args_draw_b
 // fine_tile_main for every batch: one 256-lane workgroup per tile, one lane per
 // pixel.  Between batches the pixel goes through the RGBA8 store/load round trip
 // the reference performs on its target.  kScenes: a batch of scenes.
-template <bool kScenes>
+template <bool kScenes, bool kPersistent>
 __device__ __forceinline__ void fine_tile(const tk_args_fine &a) {
     // Clip levels beyond the four register slots: clip_spill_depth * 256 bytes of
     // dynamic LDS, sized by the launch.
@@ -1531,25 +1595,12 @@ __device__ __forceinline__ void fine_tile(const tk_args_fine &a) {
     __shared__ CoverageAcc cov_acc[2];  // double-buffered: one barrier per fill
     __shared__ uint32_t batch_kind[256];
     __shared__ tk_segment_range batch_range[256];
+    __shared__ uint32_t s_item;
     const tk_fine_config &cfg = a.cfg;
-    const uint32_t linear = blockIdx.x + blockIdx.y * gridDim.x;
     const uint32_t columns = a.list.columns;  // table columns: listed tiles, or every tile
-    if (linear >= xcd_grid(columns, a.xcd_run)) return;
-    const uint32_t column = xcd_chunked_block(linear, a.xcd_run);
-    if (column >= columns) return;
-    const SceneView v = scene_view<kScenes>(a.scenes, column, columns, cfg.batch_count, cfg.tiles_width,
-                                            cfg.tiles_height, cfg.width, cfg.height, cfg.clear_color);
-    const uint32_t tile_ix = a.list.tiles ? a.list.tiles[column] : v.local;  // the tile of its canvas
-    if (tile_ix >= cfg.tile_count) return;
     const uint32_t lane = threadIdx.x;
-    const uint32_t lx = lane & 15u, ly = lane >> 4;
-    const uint32_t gx = (tile_ix % v.tw) * 16u + lx;
-    const uint32_t gy = (tile_ix / v.tw) * 16u + ly;
-    const float sx = (float)gx + 0.5f, sy = (float)gy + 0.5f;
-    const uint32_t *paint = a.in.paint;
-    const tk_ptcl_record *ptcl = a.in.ptcl;
-    const tk_line_segment *segs = a.in.segments;
 
+    // Cleared once per workgroup: each take_coverage clears the entries it read.
     cov_acc[0].run[lane] = 0;
     cov_acc[0].part[lane] = 0;
     cov_acc[1].run[lane] = 0;
@@ -1560,161 +1611,202 @@ __device__ __forceinline__ void fine_tile(const tk_args_fine &a) {
     }
     uint32_t acc_sel = 0u;
 
-    Px pixel = rgba8_to_unorm(v.clear);
-    for (uint32_t b0 = v.q0; b0 < v.q1; b0 += 256u) {
-      // This tile's (kind, range) word pairs of up to 256 batches, one load per lane.
-      __syncthreads();
-      if (b0 + lane < v.q1) {
-          const size_t idx = table_index(v, b0 + lane);
-          batch_kind[lane] = a.in.tile_kinds[idx];
-          batch_range[lane] = a.in.tile_ptcl_ranges[idx];
-      }
-      __syncthreads();
-      const uint32_t chunk = min(256u, v.q1 - b0);
-      for (uint32_t bj = 0; bj < chunk; ++bj) {
-        const uint32_t kind_word = batch_kind[bj];
-        const uint32_t kind = kind_word & TK_TILE_KIND_MASK;
-        if (kind == TK_TILE_KIND_EMPTY) continue;
-        const tk_segment_range r = batch_range[bj];
-        const uint32_t pstart = r.start + (kind_word >> TK_TILE_PREFIX_SHIFT);
-        const uint32_t pend = min(r.end, cfg.ptcl_capacity);
-        if (kind == TK_TILE_KIND_INTERPRETER) {
-            uint32_t clip_mask = 255u, clip_depth = 0u;
-            uint32_t cs0 = 255u, cs1 = 255u, cs2 = 255u, cs3 = 255u;
-            auto push_clip = [&](uint32_t mask) {
-                uint32_t dd = clip_depth;
-                if (dd == 0u) { cs0 = mask; clip_depth = 1u; }
-                else if (dd == 1u) { cs1 = mask; clip_depth = 2u; }
-                else if (dd == 2u) { cs2 = mask; clip_depth = 3u; }
-                else if (dd == 3u) { cs3 = mask; clip_depth = 4u; }
-                else {
-                    uint32_t spill = dd - 4u;
-                    if (spill < cfg.clip_spill_depth) {
-                        clip_spill[spill * 256u + lane] = (uint8_t)mask;
-                        clip_depth = dd + 1u;
-                    } else if (a.in.status) {
-                        atomicOr(a.in.status, 2u);  // deeper than the scene's declared clip depth
+    auto tile = [&](uint32_t column) {
+        const SceneView v = scene_view<kScenes>(a.scenes, column, columns, cfg.batch_count, cfg.tiles_width,
+                                                cfg.tiles_height, cfg.width, cfg.height, cfg.clear_color);
+        const uint32_t tile_ix = a.list.tiles ? a.list.tiles[column] : v.local;  // the tile of its canvas
+        if (tile_ix >= cfg.tile_count) return;
+        const uint32_t lx = lane & 15u, ly = lane >> 4;
+        const uint32_t gx = (tile_ix % v.tw) * 16u + lx;
+        const uint32_t gy = (tile_ix / v.tw) * 16u + ly;
+        const float sx = (float)gx + 0.5f, sy = (float)gy + 0.5f;
+        const uint32_t *paint = a.in.paint;
+        const tk_ptcl_record *ptcl = a.in.ptcl;
+        const tk_line_segment *segs = a.in.segments;
+
+        Px pixel = rgba8_to_unorm(v.clear);
+        for (uint32_t b0 = v.q0; b0 < v.q1; b0 += 256u) {
+          // This tile's (kind, range) word pairs of up to 256 batches, one load per lane.
+          __syncthreads();
+          if (b0 + lane < v.q1) {
+              const size_t idx = table_index(v, b0 + lane);
+              batch_kind[lane] = a.in.tile_kinds[idx];
+              batch_range[lane] = a.in.tile_ptcl_ranges[idx];
+          }
+          __syncthreads();
+          const uint32_t chunk = min(256u, v.q1 - b0);
+          for (uint32_t bj = 0; bj < chunk; ++bj) {
+            const uint32_t kind_word = batch_kind[bj];
+            const uint32_t kind = kind_word & TK_TILE_KIND_MASK;
+            if (kind == TK_TILE_KIND_EMPTY) continue;
+            const tk_segment_range r = batch_range[bj];
+            const uint32_t pstart = r.start + (kind_word >> TK_TILE_PREFIX_SHIFT);
+            const uint32_t pend = min(r.end, cfg.ptcl_capacity);
+            if (kind == TK_TILE_KIND_INTERPRETER) {
+                uint32_t clip_mask = 255u, clip_depth = 0u;
+                uint32_t cs0 = 255u, cs1 = 255u, cs2 = 255u, cs3 = 255u;
+                auto push_clip = [&](uint32_t mask) {
+                    uint32_t dd = clip_depth;
+                    if (dd == 0u) { cs0 = mask; clip_depth = 1u; }
+                    else if (dd == 1u) { cs1 = mask; clip_depth = 2u; }
+                    else if (dd == 2u) { cs2 = mask; clip_depth = 3u; }
+                    else if (dd == 3u) { cs3 = mask; clip_depth = 4u; }
+                    else {
+                        uint32_t spill = dd - 4u;
+                        if (spill < cfg.clip_spill_depth) {
+                            clip_spill[spill * 256u + lane] = (uint8_t)mask;
+                            clip_depth = dd + 1u;
+                        } else if (a.in.status) {
+                            atomicOr(a.in.status, 2u);  // deeper than the scene's declared clip depth
+                        }
                     }
+                };
+                for (uint32_t pi = pstart; pi < pend; ++pi) {
+                    const tk_ptcl_record rec = ptcl[pi];
+                    const uint32_t tag = rec.tag;
+                    if (tag == TK_PTCL_END) break;
+                    if (tag == TK_PTCL_COLOR) {
+                        if (clip_mask != 0u) {
+                            if (clip_mask == 255u && premul_opaque(rec.color)) pixel = rgba8_to_unorm(rec.color);
+                            else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(rec.color, clip_mask));
+                        }
+                    } else if (tag == TK_PTCL_SDF) {
+                        if (clip_mask != 0u) {
+                            FineDraw d = load_fine_draw(a.in.draws, cfg.draw_count, rec.color);
+                            float coverage = sdf_coverage_from_draw(cfg, paint, d, sx, sy);
+                            uint32_t alpha = combine_alpha(coverage_to_u8(coverage), clip_mask);
+                            if (alpha != 0u) {
+                                uint32_t color = sample_draw_brush(cfg, paint, d, sx, sy);
+                                if (alpha == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
+                                else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, alpha));
+                            }
+                        }
+                    } else if (tag == TK_PTCL_END_CLIP) {
+                        if (clip_depth > 0u) {
+                            clip_depth -= 1u;
+                            if (clip_depth == 0u) clip_mask = cs0;
+                            else if (clip_depth == 1u) clip_mask = cs1;
+                            else if (clip_depth == 2u) clip_mask = cs2;
+                            else if (clip_depth == 3u) clip_mask = cs3;
+                            else {
+                                uint32_t spill = clip_depth - 4u;
+                                if (spill < cfg.clip_spill_depth) clip_mask = clip_spill[spill * 256u + lane];
+                            }
+                        } else {
+                            clip_mask = 255u;
+                        }
+                    } else if (tag == TK_PTCL_BEGIN_SDF_CLIP) {
+                        uint32_t parent = clip_mask, alpha = 0u;
+                        if (parent != 0u) {
+                            FineDraw d = load_fine_draw(a.in.draws, cfg.draw_count, rec.color);
+                            alpha = coverage_to_u8(sdf_coverage_from_draw(cfg, paint, d, sx, sy));
+                        }
+                        push_clip(parent);
+                        clip_mask = combine_alpha(parent, alpha);
+                    } else if (tag == TK_PTCL_FILL || tag == TK_PTCL_BEGIN_CLIP) {
+                        const uint32_t s0 = rec.segment_start, s1 = min(rec.segment_end, cfg.segment_capacity);
+                        const uint32_t n = s1 > s0 ? s1 - s0 : 0u;
+                        // Issued before the coverage work so that its latency overlaps it.
+                        uint32_t brush_flag = 0u, brush_color = 0u;
+                        if (tag == TK_PTCL_FILL && rec.color < cfg.draw_count) {
+                            brush_flag = a.draw_brush[2u * rec.color];
+                            brush_color = a.draw_brush[2u * rec.color + 1u];
+                        }
+                        CoverageAcc &acc = cov_acc[acc_sel];
+                        acc_sel ^= 1u;
+                        accumulate_record(segs + s0, n, acc, lane);
+                        __syncthreads();
+                        const float v = take_coverage(acc, rec.backdrop, lane);
+                        uint32_t alpha = coverage_to_alpha(v, rec.fill_rule);
+                        if (tag == TK_PTCL_BEGIN_CLIP) {
+                            push_clip(clip_mask);
+                            clip_mask = combine_alpha(clip_mask, alpha);
+                        } else {
+                            uint32_t masked = combine_alpha(alpha, clip_mask);
+                            if (masked != 0u) {
+                                uint32_t color = brush_color;
+                                if (brush_flag == 0u)
+                                    color = sample_draw_brush(cfg, paint, load_fine_draw(a.in.draws, cfg.draw_count, rec.color),
+                                                              sx, sy);
+                                if (masked == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
+                                else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, masked));
+                            }
+                        }
+                    }
+                    // Image, glyph, opacity and blend records are not produced on this path.
                 }
-            };
-            for (uint32_t pi = pstart; pi < pend; ++pi) {
-                const tk_ptcl_record rec = ptcl[pi];
-                const uint32_t tag = rec.tag;
-                if (tag == TK_PTCL_END) break;
-                if (tag == TK_PTCL_COLOR) {
-                    if (clip_mask != 0u) {
-                        if (clip_mask == 255u && premul_opaque(rec.color)) pixel = rgba8_to_unorm(rec.color);
-                        else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(rec.color, clip_mask));
-                    }
-                } else if (tag == TK_PTCL_SDF) {
-                    if (clip_mask != 0u) {
+            } else if (kind == TK_TILE_KIND_COLOR) {
+                for (uint32_t pi = pstart; pi < pend; ++pi) {
+                    const tk_ptcl_record rec = ptcl[pi];
+                    if (rec.tag == TK_PTCL_END) break;
+                    if (premul_opaque(rec.color)) pixel = rgba8_to_unorm(rec.color);
+                    else pixel = src_over_premul_unorm(pixel, rgba8_to_unorm(rec.color));
+                }
+            } else {  // TILE_KIND_SDF / TILE_KIND_MIXED
+                for (uint32_t pi = pstart; pi < pend; ++pi) {
+                    const tk_ptcl_record rec = ptcl[pi];
+                    if (rec.tag == TK_PTCL_END) break;
+                    if (rec.tag == TK_PTCL_COLOR) {
+                        if (premul_opaque(rec.color)) pixel = rgba8_to_unorm(rec.color);
+                        else pixel = src_over_premul_unorm(pixel, rgba8_to_unorm(rec.color));
+                    } else if (rec.tag == kPtclImage) {
+                        // image brushes are not produced on this path
+                    } else {
                         FineDraw d = load_fine_draw(a.in.draws, cfg.draw_count, rec.color);
-                        float coverage = sdf_coverage_from_draw(cfg, paint, d, sx, sy);
-                        uint32_t alpha = combine_alpha(coverage_to_u8(coverage), clip_mask);
+                        float coverage = 0.0f;
+                        if ((int32_t)gx >= d.pb[0] && (int32_t)gy >= d.pb[1] && (int32_t)gx < d.pb[2] && (int32_t)gy < d.pb[3])
+                            coverage = sdf_coverage_from_draw(cfg, paint, d, sx, sy);
+                        uint32_t alpha = coverage_to_u8(coverage);
                         if (alpha != 0u) {
-                            uint32_t color = sample_draw_brush(cfg, paint, d, sx, sy);
+                            uint32_t color = blob_word(paint, cfg.paint_words, cfg.paint_brush_base, d.brush_offset + 4u);
                             if (alpha == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
                             else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, alpha));
                         }
                     }
-                } else if (tag == TK_PTCL_END_CLIP) {
-                    if (clip_depth > 0u) {
-                        clip_depth -= 1u;
-                        if (clip_depth == 0u) clip_mask = cs0;
-                        else if (clip_depth == 1u) clip_mask = cs1;
-                        else if (clip_depth == 2u) clip_mask = cs2;
-                        else if (clip_depth == 3u) clip_mask = cs3;
-                        else {
-                            uint32_t spill = clip_depth - 4u;
-                            if (spill < cfg.clip_spill_depth) clip_mask = clip_spill[spill * 256u + lane];
-                        }
-                    } else {
-                        clip_mask = 255u;
-                    }
-                } else if (tag == TK_PTCL_BEGIN_SDF_CLIP) {
-                    uint32_t parent = clip_mask, alpha = 0u;
-                    if (parent != 0u) {
-                        FineDraw d = load_fine_draw(a.in.draws, cfg.draw_count, rec.color);
-                        alpha = coverage_to_u8(sdf_coverage_from_draw(cfg, paint, d, sx, sy));
-                    }
-                    push_clip(parent);
-                    clip_mask = combine_alpha(parent, alpha);
-                } else if (tag == TK_PTCL_FILL || tag == TK_PTCL_BEGIN_CLIP) {
-                    const uint32_t s0 = rec.segment_start, s1 = min(rec.segment_end, cfg.segment_capacity);
-                    const uint32_t n = s1 > s0 ? s1 - s0 : 0u;
-                    // Issued before the coverage work so that its latency overlaps it.
-                    uint32_t brush_flag = 0u, brush_color = 0u;
-                    if (tag == TK_PTCL_FILL && rec.color < cfg.draw_count) {
-                        brush_flag = a.draw_brush[2u * rec.color];
-                        brush_color = a.draw_brush[2u * rec.color + 1u];
-                    }
-                    CoverageAcc &acc = cov_acc[acc_sel];
-                    acc_sel ^= 1u;
-                    accumulate_record(segs + s0, n, acc, lane);
-                    __syncthreads();
-                    const float v = take_coverage(acc, rec.backdrop, lane);
-                    uint32_t alpha = coverage_to_alpha(v, rec.fill_rule);
-                    if (tag == TK_PTCL_BEGIN_CLIP) {
-                        push_clip(clip_mask);
-                        clip_mask = combine_alpha(clip_mask, alpha);
-                    } else {
-                        uint32_t masked = combine_alpha(alpha, clip_mask);
-                        if (masked != 0u) {
-                            uint32_t color = brush_color;
-                            if (brush_flag == 0u)
-                                color = sample_draw_brush(cfg, paint, load_fine_draw(a.in.draws, cfg.draw_count, rec.color),
-                                                          sx, sy);
-                            if (masked == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
-                            else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, masked));
-                        }
-                    }
-                }
-                // Image, glyph, opacity and blend records are not produced on this path.
-            }
-        } else if (kind == TK_TILE_KIND_COLOR) {
-            for (uint32_t pi = pstart; pi < pend; ++pi) {
-                const tk_ptcl_record rec = ptcl[pi];
-                if (rec.tag == TK_PTCL_END) break;
-                if (premul_opaque(rec.color)) pixel = rgba8_to_unorm(rec.color);
-                else pixel = src_over_premul_unorm(pixel, rgba8_to_unorm(rec.color));
-            }
-        } else {  // TILE_KIND_SDF / TILE_KIND_MIXED
-            for (uint32_t pi = pstart; pi < pend; ++pi) {
-                const tk_ptcl_record rec = ptcl[pi];
-                if (rec.tag == TK_PTCL_END) break;
-                if (rec.tag == TK_PTCL_COLOR) {
-                    if (premul_opaque(rec.color)) pixel = rgba8_to_unorm(rec.color);
-                    else pixel = src_over_premul_unorm(pixel, rgba8_to_unorm(rec.color));
-                } else if (rec.tag == kPtclImage) {
-                    // image brushes are not produced on this path
-                } else {
-                    FineDraw d = load_fine_draw(a.in.draws, cfg.draw_count, rec.color);
-                    float coverage = 0.0f;
-                    if ((int32_t)gx >= d.pb[0] && (int32_t)gy >= d.pb[1] && (int32_t)gx < d.pb[2] && (int32_t)gy < d.pb[3])
-                        coverage = sdf_coverage_from_draw(cfg, paint, d, sx, sy);
-                    uint32_t alpha = coverage_to_u8(coverage);
-                    if (alpha != 0u) {
-                        uint32_t color = blob_word(paint, cfg.paint_words, cfg.paint_brush_base, d.brush_offset + 4u);
-                        if (alpha == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
-                        else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, alpha));
-                    }
                 }
             }
+            pixel = rgba8_to_unorm(unorm_to_rgba8(pixel));
+          }
         }
-        pixel = rgba8_to_unorm(unorm_to_rgba8(pixel));
-      }
+        if (gx < v.width && gy < v.height) a.rgba[v.image + (size_t)gy * v.width + gx] = unorm_to_rgba8(pixel);
+    };
+
+    if (!kPersistent) {  // one tile per workgroup
+        const uint32_t linear = blockIdx.x + blockIdx.y * gridDim.x;
+        if (linear >= xcd_grid(columns, a.xcd_run)) return;
+        const uint32_t column = xcd_chunked_block(linear, a.xcd_run);
+        if (column < columns) tile(column);
+        return;
     }
-    if (gx < v.width && gy < v.height) a.rgba[v.image + (size_t)gy * v.width + gx] = unorm_to_rgba8(pixel);
+    // A persistent launch: tiles from the per-XCD work queues until they are used up.
+    uint32_t qi = 0u;
+    for (;;) {
+        if (lane == 0u) s_item = queue_next(a.queue, blockIdx.x & 7u, qi, a.xcd_run, columns);
+        __syncthreads();
+        const uint32_t column = s_item;
+        __syncthreads();
+        if (column == ~0u) return;
+        tile(column);
+    }
 }
 
 // Seven waves per SIMD: at most 72 VGPRs (a few dwords spill, in the per-batch-run
 // setup outside the record loop).
 extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void tk_k_fine(tk_args_fine a) {
-    fine_tile<false>(a);
+    fine_tile<false, false>(a);
 }
 extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void tk_k_fine_scenes(
     tk_args_fine a) {
-    fine_tile<true>(a);
+    fine_tile<true, false>(a);
+}
+// Persistent forms: a grid of resident workgroups draining the per-XCD queues.
+extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void tk_k_fine_persistent(
+    tk_args_fine a) {
+    fine_tile<false, true>(a);
+}
+extern "C" __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void tk_k_fine_scenes_persistent(
+    tk_args_fine a) {
+    fine_tile<true, true>(a);
 }
 
 // fill_coverage_words (validation/geometry.hlsl:22-29) plus the raw FP32 area.

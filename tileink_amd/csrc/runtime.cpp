@@ -80,6 +80,12 @@ struct tk_context {
                   f_draw_tiles = nullptr, f_fine_diff_clips = nullptr, f_fine_diff_scenes_clips = nullptr,
                   f_fine_backward_clips = nullptr, f_fine_backward_scenes_clips = nullptr,
                   f_fill_coverage = nullptr, f_pixel_math = nullptr, f_brush_words = nullptr;
+    // Persistent forms (work queues) and the coarse scan's other block sizes.
+    hipFunction_t f_fine_persistent = nullptr, f_fine_scenes_persistent = nullptr,
+                  f_coarse_count_persistent = nullptr, f_coarse_count_scenes_persistent = nullptr,
+                  f_coarse_emit_persistent = nullptr, f_coarse_emit_scenes_persistent = nullptr,
+                  f_coarse_scan_1k = nullptr, f_coarse_scan_2k = nullptr, f_coarse_scan_8k = nullptr;
+    uint32_t compute_units = 0;
     // Scratch for the stage entry points (frames own their own).
     uint32_t *tickets = nullptr;  // [0] scan prefix, [1] coarse scan
     unsigned long long *status = nullptr;
@@ -101,6 +107,9 @@ namespace tk {
 namespace {
 
 uint32_t div_up(uint64_t n, uint32_t d) { return uint32_t((n + d - 1) / d); }
+// Work counters of a frame's persistent launches: eight per-XCD counters each for
+// coarse count, coarse emit and fine.
+constexpr uint64_t kQueueWords = 24;
 // Grid of a kernel that renumbers its workgroups per XCD: a multiple of 8 * chunk.
 uint32_t xcd_grid(uint64_t blocks, uint32_t chunk) {
     const uint64_t unit = chunk ? 8ull * chunk : 1ull;
@@ -179,16 +188,18 @@ tk_status fill_u32(tk_context *ctx, void *dst, uint64_t count, uint32_t value) {
 
 tk_status run_scan_count(tk_context *ctx, const tk_scan_config &cfg, const tk_line *lines, const tk_path_record *paths,
                          uint32_t path_record_count, int32_t *backdrops, uint32_t *counts,
-                         const uint32_t *path_active = nullptr, uint32_t xcd_run = TK_XCD_CHUNK_SCAN) {
+                         const uint32_t *path_active = nullptr, uint32_t xcd_run = TK_XCD_CHUNK_SCAN,
+                         uint32_t block = 256u) {
     tk_args_scan_lines a{cfg, lines, paths, path_record_count, backdrops, counts, nullptr, path_active, xcd_run};
-    return launch(ctx, ctx->f_scan_count, xcd_grid(div_up(cfg.line_count, 256), xcd_run), 1, 256, a);
+    return launch(ctx, ctx->f_scan_count, xcd_grid(div_up(cfg.line_count, block), xcd_run), 1, block, a);
 }
 
 tk_status run_scan_emit(tk_context *ctx, const tk_scan_config &cfg, const tk_line *lines, const tk_path_record *paths,
                         uint32_t path_record_count, uint32_t *cursors, tk_line_segment *segments,
-                        const uint32_t *path_active = nullptr, uint32_t xcd_run = TK_XCD_CHUNK_SCAN) {
+                        const uint32_t *path_active = nullptr, uint32_t xcd_run = TK_XCD_CHUNK_SCAN,
+                        uint32_t block = 256u) {
     tk_args_scan_lines a{cfg, lines, paths, path_record_count, nullptr, cursors, segments, path_active, xcd_run};
-    return launch(ctx, ctx->f_scan_emit, xcd_grid(div_up(cfg.line_count, 256), xcd_run), 1, 256, a);
+    return launch(ctx, ctx->f_scan_emit, xcd_grid(div_up(cfg.line_count, block), xcd_run), 1, block, a);
 }
 
 tk_status run_scan_prefix(tk_context *ctx, const tk_scan_config &cfg, const tk_scan_chunk *chunks,
@@ -214,14 +225,36 @@ tk_status run_cumsum(tk_context *ctx, const tk_cumsum_config &cfg, const uint32_
 
 // Scan blocks over the batch x column count table (columns: the listed tiles of an
 // incremental render, else every tile).
-uint32_t coarse_scan_blocks(const tk_coarse_config &cfg, uint32_t columns) {
-    uint64_t n = uint64_t(cfg.batch_count) * columns;
-    return uint32_t(std::max<uint64_t>((n + TK_COARSE_SCAN_ITEMS - 1) / TK_COARSE_SCAN_ITEMS, 1));
+// (items: counts per scan workgroup, 1024-8192.)
+uint32_t coarse_scan_blocks_for(uint64_t entries, uint32_t items = TK_COARSE_SCAN_ITEMS) {
+    return uint32_t(std::max<uint64_t>((entries + items - 1) / items, 1));
+}
+uint32_t coarse_scan_blocks(const tk_coarse_config &cfg, uint32_t columns, uint32_t items = TK_COARSE_SCAN_ITEMS) {
+    return coarse_scan_blocks_for(uint64_t(cfg.batch_count) * columns, items);
 }
 uint32_t coarse_scan_blocks(const tk_coarse_config &cfg) { return coarse_scan_blocks(cfg, cfg.tile_count); }
-uint32_t coarse_scan_blocks_for(uint64_t entries) {
-    return uint32_t(std::max<uint64_t>((entries + TK_COARSE_SCAN_ITEMS - 1) / TK_COARSE_SCAN_ITEMS, 1));
+
+// Workgroups of a persistent launch: as many as are resident at once (the
+// occupancy the kernel's resources allow, on every CU), a multiple of the eight
+// XCDs so that each XCD's queue gets the same number of workers.
+tk_status resident_grid(tk_context *ctx, hipFunction_t fn, uint32_t block, uint32_t dynamic_lds, uint32_t items,
+                        uint32_t *grid) {
+    int per_cu = 0;
+    TK_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, int(block), size_t(dynamic_lds)));
+    uint64_t n = uint64_t(std::max(per_cu, 1)) * ctx->compute_units;
+    n = std::max<uint64_t>(n / 8u * 8u, 8u);
+    *grid = uint32_t(std::min<uint64_t>(n, std::max<uint64_t>(uint64_t(items + 7u) / 8u * 8u, 8u)));
+    return TK_OK;
 }
+
+// Launch choices of the coarse and fine stages (tk_tuning); the stage entry
+// points use the defaults.
+struct TileLaunch {
+    uint32_t xcd_run;
+    uint32_t coarse_block = 4u;                      // tiles (waves) per coarse workgroup
+    uint32_t scan_items = TK_COARSE_SCAN_ITEMS;      // counts per coarse scan workgroup
+    uint32_t *queue = nullptr;                       // persistent launches' counters; null: grid launches
+};
 
 // Where the frame keeps its per-tile layer classes: pair_layer[pair] is a layer
 // record of each distinct (draw, tag) pair, pair_of[layer] its pair, and bits
@@ -237,24 +270,42 @@ tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coar
                      hipEvent_t *marks, const LayerClassPlan &plan = LayerClassPlan{},
                      tk_tile_list list = tk_tile_list{nullptr, 0u},
                      tk_scene_slots scenes = tk_scene_slots{nullptr, nullptr, 0u, 0ull},
-                     uint32_t xcd_run = TK_XCD_CHUNK_COARSE) {
+                     TileLaunch shape = TileLaunch{TK_XCD_CHUNK_COARSE}) {
     if (!list.tiles) list.columns = cfg.tile_count;
     if (cfg.batch_count == 0 || list.columns == 0) return TK_OK;
-    const uint32_t blocks = scenes.slots ? coarse_scan_blocks_for(scenes.table_entries)
-                                         : coarse_scan_blocks(cfg, list.columns);
+    const uint32_t blocks = scenes.slots ? coarse_scan_blocks_for(scenes.table_entries, shape.scan_items)
+                                         : coarse_scan_blocks(cfg, list.columns, shape.scan_items);
     tk_args_coarse a{cfg, in, out, status, ticket, epoch, blocks, tk_layer_classes{nullptr, nullptr, 0u}, list, scenes,
-                     xcd_run};
+                     shape.xcd_run, shape.queue};
     if (plan.bits && plan.pair_count) {
         tk_args_layer_classes b{cfg, in, plan.pair_layer, plan.pair_count, plan.bits, list};
         TK_TRY(launch(ctx, ctx->f_layer_classes, uint32_t(div_up(uint64_t(list.columns) * plan.pair_count, 256)), 1,
                       256, b));
         a.classes = tk_layer_classes{plan.bits, plan.pair_of, plan.pair_count};
     }
-    TK_TRY(launch(ctx, scenes.slots ? ctx->f_coarse_count_scenes : ctx->f_coarse_count, xcd_grid(div_up(list.columns, 4), xcd_run), 1, 256, a));
+    const uint32_t block = 64u * shape.coarse_block;
+    hipFunction_t count_fn, emit_fn;
+    uint32_t count_grid, emit_grid;
+    if (shape.queue) {
+        count_fn = scenes.slots ? ctx->f_coarse_count_scenes_persistent : ctx->f_coarse_count_persistent;
+        emit_fn = scenes.slots ? ctx->f_coarse_emit_scenes_persistent : ctx->f_coarse_emit_persistent;
+        const uint32_t groups = div_up(list.columns, shape.coarse_block);
+        TK_TRY(resident_grid(ctx, count_fn, block, 0u, groups, &count_grid));
+        TK_TRY(resident_grid(ctx, emit_fn, block, 0u, groups, &emit_grid));
+    } else {
+        count_fn = scenes.slots ? ctx->f_coarse_count_scenes : ctx->f_coarse_count;
+        emit_fn = scenes.slots ? ctx->f_coarse_emit_scenes : ctx->f_coarse_emit;
+        count_grid = emit_grid = xcd_grid(div_up(list.columns, shape.coarse_block), shape.xcd_run);
+    }
+    hipFunction_t scan_fn = shape.scan_items == 1024u   ? ctx->f_coarse_scan_1k
+                            : shape.scan_items == 2048u ? ctx->f_coarse_scan_2k
+                            : shape.scan_items == 8192u ? ctx->f_coarse_scan_8k
+                                                        : ctx->f_coarse_scan;
+    TK_TRY(launch(ctx, count_fn, count_grid, 1, block, a));
     if (marks) TK_HIP(hipEventRecord(marks[0], launch_stream(ctx)));
-    TK_TRY(launch(ctx, ctx->f_coarse_scan, a.scan_blocks, 1, 256, a));
+    TK_TRY(launch(ctx, scan_fn, a.scan_blocks, 1, 256, a));
     if (marks) TK_HIP(hipEventRecord(marks[1], launch_stream(ctx)));
-    TK_TRY(launch(ctx, scenes.slots ? ctx->f_coarse_emit_scenes : ctx->f_coarse_emit, xcd_grid(div_up(list.columns, 4), xcd_run), 1, 256, a));
+    TK_TRY(launch(ctx, emit_fn, emit_grid, 1, block, a));
     return TK_OK;
 }
 
@@ -263,7 +314,7 @@ tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coar
 tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inputs &in, uint32_t *rgba,
                    uint32_t *draw_brush, tk_tile_list list = tk_tile_list{nullptr, 0u},
                    tk_scene_slots scenes = tk_scene_slots{nullptr, nullptr, 0u, 0ull},
-                   uint32_t xcd_run = TK_XCD_CHUNK_FINE) {
+                   uint32_t xcd_run = TK_XCD_CHUNK_FINE, uint32_t *queue = nullptr) {
     if (!list.tiles) list.columns = cfg.tile_count;
     if (cfg.clip_spill_depth > TK_FINE_MAX_CLIP_SPILL) {
         set_error("clip nesting deeper than %u is not supported", 4u + TK_FINE_MAX_CLIP_SPILL);
@@ -273,11 +324,18 @@ tk_status run_fine(tk_context *ctx, const tk_fine_config &cfg, const tk_fine_inp
         tk_args_draw_brush b{in.draws, cfg.draw_count, in.paint, cfg.paint_words, cfg.paint_brush_base, draw_brush};
         TK_TRY(launch(ctx, ctx->f_draw_brush, div_up(cfg.draw_count, 256), 1, 256, b));
     }
-    tk_args_fine a{cfg, in, rgba, draw_brush, list, scenes, xcd_run};
+    tk_args_fine a{cfg, in, rgba, draw_brush, list, scenes, xcd_run, queue};
     if (list.columns == 0) return TK_OK;
+    const uint32_t lds = cfg.clip_spill_depth * 256u;
+    if (queue) {
+        hipFunction_t fn = scenes.slots ? ctx->f_fine_scenes_persistent : ctx->f_fine_persistent;
+        uint32_t grid;
+        TK_TRY(resident_grid(ctx, fn, 256u, lds, list.columns, &grid));
+        return launch(ctx, fn, grid, 1, 256, a, lds);
+    }
     uint32_t gx, gy;
     tile_grid(xcd_grid(list.columns, xcd_run), gx, gy);
-    return launch(ctx, scenes.slots ? ctx->f_fine_scenes : ctx->f_fine, gx, gy, 256, a, cfg.clip_spill_depth * 256u);
+    return launch(ctx, scenes.slots ? ctx->f_fine_scenes : ctx->f_fine, gx, gy, 256, a, lds);
 }
 
 }  // namespace
@@ -354,7 +412,9 @@ struct tk_frame {
     uint32_t *diff_trans = nullptr;  // 16 bytes per pixel of every column
     uint32_t *diff_draw_tiles = nullptr;  // 16 bytes per draw
     // launch parameters (tk_frame_set_tuning, tk_frame_autotune)
-    tk_tuning tuning{TK_XCD_CHUNK_SCAN, TK_XCD_CHUNK_COARSE, TK_XCD_CHUNK_FINE, 0u};
+    // The autotuner's choice on the 4096x4096, 200 000-path frame.
+    tk_tuning tuning{TK_XCD_CHUNK_SCAN, 128u, TK_XCD_CHUNK_FINE, 0u, 64u, 1u, 8192u, 0u};
+    uint32_t *queues = nullptr;  // kQueueWords work counters of the persistent launches
 };
 
 namespace tk {
@@ -538,9 +598,21 @@ uint64_t frame_table_entries(const tk_frame *f, uint32_t active_count = 0) {
     const uint64_t columns = active_count ? active_count : uint64_t(f->meta.tiles_width) * f->meta.tiles_height;
     return uint64_t(f->meta.batch_count) * columns;
 }
+// Status words of the coarse scan at its smallest block size (1024 counts), the
+// most any tuning needs.
 uint32_t frame_scan_blocks(const tk_frame *f, uint32_t active_count = 0) {
-    return coarse_scan_blocks_for(frame_table_entries(f, active_count));
+    return coarse_scan_blocks_for(frame_table_entries(f, active_count), 1024u);
 }
+
+// The coarse and fine launch choices of the frame's tuning.
+TileLaunch coarse_launch(const tk_frame *f) {
+    TileLaunch t{f->tuning.coarse_xcd_run};
+    t.coarse_block = f->tuning.coarse_block;
+    t.scan_items = f->tuning.coarse_scan_items;
+    t.queue = (f->tuning.persistent & TK_PERSISTENT_COARSE) ? f->queues : nullptr;
+    return t;
+}
+uint32_t *fine_queue(const tk_frame *f) { return (f->tuning.persistent & TK_PERSISTENT_FINE) ? f->queues + 16 : nullptr; }
 
 tk_status record(tk_frame *f, int mark) {
     if (f->timing) TK_HIP(hipEventRecord(f->ev[mark], launch_stream(f->ctx)));
@@ -651,6 +723,8 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0, const DiffTarget 
     // The coarse scan's lookback words start each frame at zero, so a replayed
     // graph (whose epoch argument is fixed) never sees the previous frame's flags.
     TK_TRY(fill_u32(ctx, f->coarse_status, 2ull * frame_scan_blocks(f, active_count), 0u));
+    // The persistent launches' work counters (coarse count, coarse emit, fine) start at zero.
+    if (f->tuning.persistent) TK_TRY(fill_u32(ctx, f->queues, kQueueWords, 0u));
     if (active_count && s.path_count) {
         tk_args_active_paths ap{f->paths, s.path_count, s.tiles_width, s.tiles_height, f->damage_sat, f->path_active};
         TK_TRY(launch(ctx, ctx->f_active_paths, div_up(s.path_count, 256), 1, 256, ap));
@@ -658,7 +732,7 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0, const DiffTarget 
     }
     TK_TRY(record(f, 1));
     TK_TRY(run_scan_count(ctx, scfg, f->lines, f->paths, s.path_count, f->backdrops, f->seg_counts, path_active,
-                          f->tuning.scan_xcd_run));
+                          f->tuning.scan_xcd_run, f->tuning.scan_block));
     TK_TRY(record(f, 2));
     // Fork A: the backdrop prefix needs only scan_count's deltas.
     TK_HIP(hipEventRecord(ctx->fork_a, ctx->stream));
@@ -712,7 +786,7 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0, const DiffTarget 
             plan.bits = f->layer_bits;
         }
         TK_TRY(run_coarse(ctx, cc, in, out, f->coarse_status, &f->tickets[1], f->epoch, f->timing ? &f->ev[7] : nullptr,
-                          plan, list, scenes, f->tuning.coarse_xcd_run));
+                          plan, list, scenes, coarse_launch(f)));
         if (cc.batch_count == 0 && f->timing) {
             TK_TRY(record(f, 7));
             TK_TRY(record(f, 8));
@@ -721,7 +795,7 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0, const DiffTarget 
         TK_HIP(hipEventRecord(ctx->join, ctx->side));
     }
     TK_TRY(run_scan_emit(ctx, scfg, f->lines, f->paths, s.path_count, f->cursors, f->segments, path_active,
-                         f->tuning.scan_xcd_run));
+                         f->tuning.scan_xcd_run, f->tuning.scan_block));
     TK_TRY(record(f, 4));
     // Join: fine reads the segments and the PTCL.
     TK_HIP(hipStreamWaitEvent(ctx->stream, ctx->join, 0));
@@ -741,7 +815,7 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0, const DiffTarget 
         in.ptcl = f->ptcl;
         in.status = f->status + 2;
         (void)T;
-        TK_TRY(run_fine(ctx, fc, in, f->image, f->draw_brush, list, scenes, f->tuning.fine_xcd_run));
+        TK_TRY(run_fine(ctx, fc, in, f->image, f->draw_brush, list, scenes, f->tuning.fine_xcd_run, fine_queue(f)));
     }
     TK_TRY(record(f, 10));
     f->have_times = f->timing;
@@ -869,6 +943,7 @@ tk_status frame_create(tk_context *ctx, const tk_scene *s, const BatchLayout *ba
     ALLOC(image, uint64_t(s->width) * s->height);
     ALLOC(tickets, 4);
     ALLOC(coarse_status, frame_scan_blocks(f));
+    ALLOC(queues, kQueueWords);
 #undef ALLOC
     // batch_ptcl_bases: a scene without batches never runs coarse, so its one entry stays 0.
     if (hipMemsetAsync(f->tickets, 0, 16, ctx->stream) != hipSuccess ||
@@ -923,6 +998,7 @@ tk_status tk_context_create(int device, tk_context **out) {
     tk_context *ctx = new (std::nothrow) tk_context();
     if (!ctx) return fail(TK_ERR_OOM, "context allocation failed");
     ctx->device = device;
+    ctx->compute_units = uint32_t(std::max(prop.multiProcessorCount, 1));
     auto cleanup = [&](tk_status st) {
         tk_context_destroy(ctx);
         return st;
@@ -970,6 +1046,15 @@ tk_status tk_context_create(int device, tk_context **out) {
         {&ctx->f_fill_coverage, "tk_k_fill_coverage"},
         {&ctx->f_pixel_math, "tk_k_pixel_math"},
         {&ctx->f_brush_words, "tk_k_brush_words"},
+        {&ctx->f_fine_persistent, "tk_k_fine_persistent"},
+        {&ctx->f_fine_scenes_persistent, "tk_k_fine_scenes_persistent"},
+        {&ctx->f_coarse_count_persistent, "tk_k_coarse_count_persistent"},
+        {&ctx->f_coarse_count_scenes_persistent, "tk_k_coarse_count_scenes_persistent"},
+        {&ctx->f_coarse_emit_persistent, "tk_k_coarse_emit_persistent"},
+        {&ctx->f_coarse_emit_scenes_persistent, "tk_k_coarse_emit_scenes_persistent"},
+        {&ctx->f_coarse_scan_1k, "tk_k_coarse_scan_1k"},
+        {&ctx->f_coarse_scan_2k, "tk_k_coarse_scan_2k"},
+        {&ctx->f_coarse_scan_8k, "tk_k_coarse_scan_8k"},
     };
     for (auto &entry : table) {
         if (hipModuleGetFunction(entry.fn, ctx->module, entry.name) != hipSuccess) {
@@ -1367,6 +1452,13 @@ tk_status tk_frame_set_tuning(tk_frame *f, const tk_tuning *t) {
     if (!f || !t) return fail(TK_ERR_INVALID, "null argument");
     if (!valid_run(t->scan_xcd_run) || !valid_run(t->coarse_xcd_run) || !valid_run(t->fine_xcd_run))
         return fail(TK_ERR_INVALID, "XCD run lengths are 0 or powers of two up to 1024");
+    if (t->persistent > (TK_PERSISTENT_COARSE | TK_PERSISTENT_FINE))
+        return fail(TK_ERR_INVALID, "persistent takes TK_PERSISTENT_COARSE and TK_PERSISTENT_FINE bits only");
+    auto pow2_in = [](uint32_t v, uint32_t lo, uint32_t hi) { return v >= lo && v <= hi && (v & (v - 1u)) == 0u; };
+    if (!pow2_in(t->scan_block, 64u, 1024u)) return fail(TK_ERR_INVALID, "scan_block is 64, 128, 256, 512 or 1024");
+    if (!pow2_in(t->coarse_block, 1u, 4u)) return fail(TK_ERR_INVALID, "coarse_block is 1, 2 or 4");
+    if (!pow2_in(t->coarse_scan_items, 1024u, 8192u))
+        return fail(TK_ERR_INVALID, "coarse_scan_items is 1024, 2048, 4096 or 8192");
     if (f->graph_exec) {  // the captured launches carry the old values
         TK_HIP(hipStreamSynchronize(f->ctx->stream));
         tk::drop_graph(f);
@@ -1376,15 +1468,17 @@ tk_status tk_frame_set_tuning(tk_frame *f, const tk_tuning *t) {
     return TK_OK;
 }
 
-// Coordinate search: fine first (the largest stage), then coarse, then scan, each
-// over its candidate run lengths with the others held at the best so far.
+// Coordinate search: fine first (the largest stage), then coarse, then scan; one
+// parameter at a time over its candidates, the others held at the best so far,
+// each candidate scored by the time of its own stage.
 tk_status tk_frame_autotune(tk_frame *f, uint32_t repeats, tk_tuning *chosen) {
     if (!f) return fail(TK_ERR_INVALID, "null frame");
     repeats = std::max<uint32_t>(repeats, 1u);
     const bool was_timing = f->timing;
     f->timing = true;
     tk_tuning best = f->tuning;
-    auto cost = [&](const tk_tuning &t, int stage, float *out) -> tk_status {
+    enum Stage { kFine, kCoarse, kScan };
+    auto cost = [&](const tk_tuning &t, Stage stage, float *out) -> tk_status {
         TK_TRY(tk_frame_set_tuning(f, &t));
         TK_TRY(tk::frame_render(f));  // warm-up
         float sum = 0.0f;
@@ -1392,35 +1486,51 @@ tk_status tk_frame_autotune(tk_frame *f, uint32_t repeats, tk_tuning *chosen) {
             TK_TRY(tk::frame_render(f));
             tk_stage_times st;
             TK_TRY(tk_frame_stage_times(f, &st));
-            sum += stage == 0 ? st.fine : stage == 1 ? st.coarse_count + st.coarse_emit : st.scan_count + st.scan_emit;
+            sum += stage == kFine     ? st.fine
+                   : stage == kCoarse ? st.coarse_count + st.coarse_prefix + st.coarse_emit
+                                      : st.scan_count + st.scan_emit;
         }
         *out = sum / float(repeats);
         return TK_OK;
     };
-    static const uint32_t fine_runs[] = {0u, 32u, 64u, 128u, 256u, 512u};
-    static const uint32_t coarse_runs[] = {0u, 16u, 32u, 64u, 128u};
-    static const uint32_t scan_runs[] = {0u, 16u, 64u};
+    // Candidate values of each parameter; the persistent axes toggle one bit.
     struct Axis {
-        uint32_t tk_tuning::*field;
-        const uint32_t *runs;
-        size_t count;
-    } axes[3] = {{&tk_tuning::fine_xcd_run, fine_runs, 6}, {&tk_tuning::coarse_xcd_run, coarse_runs, 5},
-                 {&tk_tuning::scan_xcd_run, scan_runs, 3}};
+        Stage stage;
+        void (*apply)(tk_tuning &, uint32_t);
+        std::vector<uint32_t> values;
+    };
+    const Axis axes[] = {
+        {kFine, [](tk_tuning &t, uint32_t v) { t.fine_xcd_run = v; }, {0u, 32u, 64u, 128u, 256u, 512u}},
+        {kFine,
+         [](tk_tuning &t, uint32_t v) { t.persistent = (t.persistent & ~TK_PERSISTENT_FINE) | (v ? TK_PERSISTENT_FINE : 0u); },
+         {0u, 1u}},
+        {kCoarse, [](tk_tuning &t, uint32_t v) { t.coarse_xcd_run = v; }, {0u, 16u, 32u, 64u, 128u}},
+        {kCoarse, [](tk_tuning &t, uint32_t v) { t.coarse_block = v; }, {1u, 2u, 4u}},
+        {kCoarse,
+         [](tk_tuning &t, uint32_t v) {
+             t.persistent = (t.persistent & ~TK_PERSISTENT_COARSE) | (v ? TK_PERSISTENT_COARSE : 0u);
+         },
+         {0u, 1u}},
+        {kCoarse, [](tk_tuning &t, uint32_t v) { t.coarse_scan_items = v; }, {1024u, 2048u, 4096u, 8192u}},
+        {kScan, [](tk_tuning &t, uint32_t v) { t.scan_xcd_run = v; }, {0u, 16u, 64u}},
+        {kScan, [](tk_tuning &t, uint32_t v) { t.scan_block = v; }, {64u, 128u, 256u, 512u, 1024u}},
+    };
     tk_status st = TK_OK;
-    for (int stage = 0; stage < 3 && st == TK_OK; ++stage) {
+    for (const Axis &axis : axes) {
+        if (st != TK_OK) break;
         float best_ms = 0.0f;
-        uint32_t best_run = best.*(axes[stage].field);
-        for (size_t i = 0; i < axes[stage].count && st == TK_OK; ++i) {
+        tk_tuning axis_best = best;
+        for (size_t i = 0; i < axis.values.size() && st == TK_OK; ++i) {
             tk_tuning t = best;
-            t.*(axes[stage].field) = axes[stage].runs[i];
+            axis.apply(t, axis.values[i]);
             float ms = 0.0f;
-            st = cost(t, stage, &ms);
+            st = cost(t, axis.stage, &ms);
             if (st == TK_OK && (i == 0 || ms < best_ms)) {
                 best_ms = ms;
-                best_run = axes[stage].runs[i];
+                axis_best = t;
             }
         }
-        best.*(axes[stage].field) = best_run;
+        best = axis_best;
     }
     f->timing = was_timing;
     if (st != TK_OK) return st;

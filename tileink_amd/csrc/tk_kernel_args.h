@@ -138,6 +138,8 @@ struct tk_args_coarse {
     tk_tile_list list;
     tk_scene_slots scenes;
     uint32_t xcd_run;  // XCD run length in four-tile workgroups (a power of two; 0: launch order)
+    uint32_t *queue;   // 16 per-XCD work counters (count, emit), zero at launch: persistent count and emit
+                       // launches, each wave taking tiles; null: one four-tile workgroup per launch item
 };
 
 struct tk_args_layer_classes {
@@ -167,6 +169,7 @@ struct tk_args_fine {
     tk_tile_list list;
     tk_scene_slots scenes;
     uint32_t xcd_run;  // XCD run length in tiles (a power of two; 0: launch order)
+    uint32_t *queue;   // eight per-XCD work counters, zero at launch: a persistent launch; null: one workgroup per tile
 };
 
 struct tk_args_fill_coverage {

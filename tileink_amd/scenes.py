@@ -135,3 +135,31 @@ def scene_set_member(width, height, paths, index):
     """One scene of the independent-scene workload: the clipped-paths generator at this
     size, with clip nesting scaled to the path count."""
     return clipped_paths_scene(width, height, paths, clips=min(32, max(1, paths // 256)), workload=1000 + index)
+
+
+def _encoded_member(job):
+    from . import SceneArrays
+    index, (w, h, n) = job
+    return index, SceneArrays(scene_set_member(w, h, n, index).encode(0))
+
+
+def scene_set(sizes, workers=None):
+    """Records and encodes every scene of `sizes` (scene_set_sizes) in worker
+    processes; returns their SceneArrays in order.  Each scene comes from its own
+    seed, so the result does not depend on the worker count.  The workers load
+    the recorder only (they start fresh, not forked from a process that may hold
+    the GPU)."""
+    import multiprocessing as mp
+    import os
+    if workers is None:
+        workers = min(12, len(os.sched_getaffinity(0)))
+    jobs = list(enumerate(sizes))
+    if workers <= 1:
+        return [_encoded_member(j)[1] for j in jobs]
+    out = [None] * len(jobs)
+    # Largest scenes first, so that the pool ends evenly.
+    jobs.sort(key=lambda j: -j[1][2])
+    with mp.get_context("spawn").Pool(workers) as pool:
+        for index, arrays in pool.imap_unordered(_encoded_member, jobs, chunksize=1):
+            out[index] = arrays
+    return out
