@@ -24,6 +24,7 @@
 #include <new>
 #include <vector>
 
+#include "scan_line.h"
 #include "tk_internal.h"
 
 namespace tk {
@@ -603,103 +604,13 @@ BezPath stroke_outline(const BezPath &path, double width, double tol, const Stro
     return w.path;
 }
 
-// ---------------------------------------------------------------------------
-// line_scanned_tile_count: CPU restatement with Rust f32 semantics
-// (src/shared/scan_line.rs:17-151): round() is half-away-from-zero and float
-// to int casts saturate.
-// ---------------------------------------------------------------------------
-
-inline uint32_t rs_u32(float v) {
-    if (!(v > 0.0f)) return 0u;
-    if (v >= 4294967295.0f) return 0xFFFFFFFFu;
-    return uint32_t(v);
-}
-inline int32_t rs_i32(float v) {
-    if (v != v) return 0;
-    if (v <= -2147483648.0f) return INT32_MIN;
-    if (v >= 2147483647.0f) return INT32_MAX;
-    return int32_t(v);
-}
-inline uint32_t rs_span(float a, float b) {
-    float hi = std::ceil(std::fmax(a, b));
-    float lo = std::floor(std::fmin(a, b));
-    return rs_u32(std::fmax(hi - lo, 1.0f));
-}
-
 }  // namespace
 
+// line_scanned_tile_count (src/shared/scan_line.rs:17-151); the arithmetic lives
+// in scan_line.h, shared with the device front end.
 uint32_t line_scanned_tile_count(const tk_line &line, uint32_t bx0, uint32_t by0, uint32_t bx1,
                                  uint32_t by1) {
-    const float scale = 1.0f / 16.0f;
-    const float top_touch_epsilon = 1.0e-12f;
-    float p0x = line.p0[0], p0y = line.p0[1], p1x = line.p1[0], p1y = line.p1[1];
-    bool is_down = p1y >= p0y;
-    float xy0x = is_down ? p0x : p1x, xy0y = is_down ? p0y : p1y;
-    float xy1x = is_down ? p1x : p0x, xy1y = is_down ? p1y : p0y;
-    float s0x = xy0x * scale, s0y = xy0y * scale, s1x = xy1x * scale, s1y = xy1y * scale;
-    uint32_t count_x = rs_span(s0x, s1x) - 1u;
-    uint32_t count = count_x + rs_span(s0y, s1y);
-    float dx = std::fabs(s1x - s0x);
-    float dy = s1y - s0y;
-    if (dx + dy == 0.0f) return 0;
-    if (dy == 0.0f && std::floor(s0y) == s0y) return 0;
-    float idxdy = 1.0f / (dx + dy);
-    float a = dx * idxdy;
-    bool is_positive_slope = s1x >= s0x;
-    float sign = is_positive_slope ? 1.0f : -1.0f;
-    float xt0 = std::floor(s0x * sign);
-    float c = s0x * sign - xt0;
-    float y0 = std::floor(s0y);
-    float ytop = s0y == s1y ? std::ceil(s0y) : y0 + 1.0f;
-    float b = std::fmin((dy * c + dx * (ytop - s0y)) * idxdy, 0.99999994f);
-    float robust_err = std::floor(a * (float(count) - 1.0f) + b) - float(count_x);
-    if (robust_err != 0.0f) a -= std::copysign(2e-7f, robust_err);
-    float x0 = xt0 * sign + (is_positive_slope ? 0.0f : -1.0f);
-    float xmin = std::fmin(s0x, s1x);
-    if (s0y >= float(by1) || s1y <= float(by0) + top_touch_epsilon || xmin >= float(bx1)) return 0;
-    uint32_t imin = 0;
-    if (s0y < float(by0)) {
-        float iminf = std::round((float(by0) - y0 + b - a) / (1.0f - a)) - 1.0f;
-        if (y0 + iminf - std::floor(a * iminf + b) < float(by0)) iminf += 1.0f;
-        imin = rs_u32(iminf);
-    }
-    uint32_t imax = count;
-    if (s1y > float(by1)) {
-        float imaxf = std::round((float(by1) - y0 + b - a) / (1.0f - a)) - 1.0f;
-        if (y0 + imaxf - std::floor(a * imaxf + b) < float(by1)) imaxf += 1.0f;
-        imax = rs_u32(imaxf);
-    }
-    if (std::fmax(s0x, s1x) < float(bx0)) {
-        imax = imin;
-    } else {
-        float fudge = is_positive_slope ? 0.0f : 1.0f;
-        if (xmin < float(bx0)) {
-            float f = std::round((sign * (float(bx0) - x0) - b + fudge) / a);
-            if ((x0 + sign * std::floor(a * f + b) < float(bx0)) == is_positive_slope) f += 1.0f;
-            if (is_positive_slope) {
-                if (rs_u32(f) > imin) imin = rs_u32(f);
-            } else if (rs_u32(f) < imax) {
-                imax = rs_u32(f);
-            }
-        }
-        if (std::fmax(s0x, s1x) > float(bx1)) {
-            float f = std::round((sign * (float(bx1) - x0) - b + fudge) / a);
-            if ((x0 + sign * std::floor(a * f + b) < float(bx1)) == is_positive_slope) f += 1.0f;
-            if (is_positive_slope)
-                imax = std::min(imax, rs_u32(f));
-            else
-                imin = std::max(imin, rs_u32(f));
-        }
-    }
-    imax = std::max(imin, imax);
-    uint32_t total = 0;
-    for (uint32_t i = imin; i < imax; ++i) {
-        float z = std::floor(a * float(i) + b);
-        int32_t y = rs_i32(y0 + float(i) - z);
-        int32_t x = rs_i32(x0 + sign * z);
-        total += (y >= int32_t(by0) && y < int32_t(by1) && x >= int32_t(bx0) && x < int32_t(bx1)) ? 1u : 0u;
-    }
-    return total;
+    return scanned_tile_count(line.p0[0], line.p0[1], line.p1[0], line.p1[1], bx0, by0, bx1, by1);
 }
 
 namespace {
@@ -872,15 +783,11 @@ struct TileBox {
     uint64_t area() const { return uint64_t(x1 > x0 ? x1 - x0 : 0) * uint64_t(y1 > y0 ? y1 - y0 : 0); }
 };
 
-// PixelBounds::tile_bbox (src/shared/bounds.rs:113-128).
+// PixelBounds::tile_bbox (src/shared/bounds.rs:113-128), shared with the device front end.
 TileBox tile_bbox(const tk_pixel_bounds &b, uint32_t tw, uint32_t th) {
-    auto clamp0 = [](int32_t v) { return uint32_t(std::max(v, 0)); };
-    TileBox t;
-    t.x0 = std::min(clamp0(b.x0) / TK_TILE_SIZE, tw);
-    t.y0 = std::min(clamp0(b.y0) / TK_TILE_SIZE, th);
-    t.x1 = std::min(uint32_t((uint64_t(clamp0(b.x1)) + TK_TILE_SIZE - 1) / TK_TILE_SIZE), tw);
-    t.y1 = std::min(uint32_t((uint64_t(clamp0(b.y1)) + TK_TILE_SIZE - 1) / TK_TILE_SIZE), th);
-    return t;
+    uint32_t t[4];
+    tile_box_of(b.x0, b.y0, b.x1, b.y1, tw, th, t);
+    return TileBox{t[0], t[1], t[2], t[3]};
 }
 
 int32_t sat_i32(double v) {
@@ -1121,17 +1028,18 @@ TileBox box_intersection(const TileBox &a, const TileBox &b) {
 // rect_tiles (occlusion/geometry.rs:79-135): the whole tiles inside the central
 // rectangle of an axis-aligned SDF rect (rounded corners and the AA edge
 // excluded), with a 1/32 px guard when any input is fractional.  Empty for a
-// path, glyph, shadowed or non-translating draw, or a non-rect SDF.
-TileBox rect_tiles(const tk_canvas *c, const tk_draw_record &d) {
+// path, glyph, shadowed or non-translating draw, or a non-rect SDF.  `s` is an
+// encoded scene (its paint blob begins with the SDF records).
+TileBox rect_tiles(const tk_scene &s, const tk_draw_record &d) {
     const TileBox none{0, 0, 0, 0};
     if (d.path_id != kNone || d.glyph_run_id != kNone || d.sdf_shadow_offset != kNone) return none;
     if (!(d.transform.a == 1.0f && d.transform.b == 0.0f && d.transform.c == 0.0f && d.transform.d == 1.0f))
         return none;
     // encoded_sdf (src/shared/gpu_sdf.rs:290-304): exactly one 17-word record.
     if (d.sdf_offset == kNone || d.sdf_len != TK_SDF_RECORD_WORDS ||
-        uint64_t(d.sdf_offset) + d.sdf_len > c->sdf_blob.size())
+        uint64_t(d.sdf_offset) + d.sdf_len > s.paint_sdf_shadow_base)
         return none;
-    const uint32_t *w = c->sdf_blob.data() + d.sdf_offset;
+    const uint32_t *w = s.paint + d.sdf_offset;
     if (w[0] != TK_SDF_KIND_RECT) return none;
     float v[10];  // coords x0 y0 x1 y1, radii tl tr bl br, translation e f
     for (int i = 0; i < 8; ++i) v[i] = bits2f(w[1 + i]);
@@ -1149,7 +1057,7 @@ TileBox rect_tiles(const tk_canvas *c, const tk_draw_record &d) {
     const double tl = radius(v[4]), tr = radius(v[5]), bl = radius(v[6]), br = radius(v[7]);
     const double guard = integral ? 0.0 : 1.0 / 32.0;
     const double tx = v[8], ty = v[9], tile = double(TK_TILE_SIZE);
-    const TileBox bb = tile_bbox(d.pixel_bounds, c->tiles_w, c->tiles_h);
+    const TileBox bb = tile_bbox(d.pixel_bounds, s.tiles_width, s.tiles_height);
     auto whole = [](double t) { return uint32_t(std::max(t, 0.0)); };
     TileBox r;
     r.x0 = std::max(whole(std::ceil((x0 + std::max(tl, bl) + tx + guard) / tile)), bb.x0);
@@ -1161,14 +1069,82 @@ TileBox rect_tiles(const tk_canvas *c, const tk_draw_record &d) {
 
 // opaque_tiles (occlusion/geometry.rs:60-77): rect_tiles of a Brush draw whose
 // brush is solid with packed alpha 255.
-TileBox opaque_tiles(const tk_canvas *c, const tk_draw_record &d) {
+TileBox opaque_tiles(const tk_scene &s, const tk_draw_record &d) {
     const TileBox none{0, 0, 0, 0};
     if (d.tag != TK_DRAW_BRUSH || d.brush_len < 5u) return none;
-    if (uint64_t(d.brush_offset) + 5u > c->brush_blob.size()) return none;
-    const uint32_t *b = c->brush_blob.data() + d.brush_offset;
+    if (uint64_t(d.brush_offset) + 5u > uint64_t(s.paint_words) - s.paint_brush_base) return none;
+    const uint32_t *b = s.paint + s.paint_brush_base + d.brush_offset;
     if (b[0] != TK_BRUSH_SOLID || (b[4] >> 24) != 255u) return none;
-    return rect_tiles(c, d);
+    return rect_tiles(s, d);
 }
+
+}  // namespace
+
+// Early occlusion (Occlusion::update, tile_bins/occlusion.rs:48-160, and
+// for_each_visible, occlusion/transient.rs:9-66) as one rectangle per draw:
+// the whole tiles a later opaque solid SDF rect of the same DrawBatch hides.
+// A batch takes part when every member is a Brush or PathGlyph draw; its first
+// draw never occludes; a proof is cut to the tiles its batch's clip rectangles
+// cover; walking the batch in reverse painter order, the largest proof seen so
+// far (a strictly larger one replaces it, never a union) is the cover of the
+// draws before it.  Painter order is ascending draw id over the batched draws
+// (the recorder lays batches out in recording order).  Returns whether any
+// draw has a nonempty proof.
+bool draw_covers(const tk_scene &s, std::vector<uint32_t> &cover) {
+    const uint32_t D = s.draw_count, Q = s.batch_count;
+    cover.assign(4 * size_t(D), 0u);
+    struct BatchVis {
+        uint32_t first = kNone;
+        bool enabled = true, has_occluder = false;
+        TileBox clip{0, 0, 0, 0};
+    };
+    std::vector<BatchVis> vis(Q);
+    auto batch_of = [&](uint32_t d) { return s.draw_batch_ids[d] < Q ? s.draw_batch_ids[d] : kNone; };
+    for (uint32_t d = 0; d < D; ++d) {
+        const uint32_t b = batch_of(d);
+        if (b == kNone) continue;
+        vis[b].enabled = vis[b].enabled && (s.draws[d].tag == TK_DRAW_BRUSH || s.draws[d].tag == TK_DRAW_PATH_GLYPH);
+        if (vis[b].first == kNone) vis[b].first = d;
+    }
+    std::vector<TileBox> opaque(D, TileBox{0, 0, 0, 0});
+    for (uint32_t d = 0; d < D; ++d) {
+        const uint32_t b = batch_of(d);
+        if (b == kNone || !vis[b].enabled || vis[b].first == d) continue;
+        opaque[d] = opaque_tiles(s, s.draws[d]);
+        if (opaque[d].area() != 0) vis[b].has_occluder = true;
+    }
+    for (uint32_t b = 0; b < Q; ++b) {  // batch_clips (occlusion/geometry.rs:17-56)
+        if (!vis[b].has_occluder) continue;
+        TileBox clip{0, 0, s.tiles_width, s.tiles_height};
+        for (uint32_t l = s.batches[b].layer_start; l < s.batches[b].layer_end && l < s.layer_count; ++l)
+            if (s.layers[l].draw < D) clip = box_intersection(clip, rect_tiles(s, s.draws[s.layers[l].draw]));
+        vis[b].clip = clip;
+    }
+    bool occluded = false;
+    for (uint32_t d = 0; d < D; ++d) {
+        const uint32_t b = batch_of(d);
+        if (b == kNone || !vis[b].enabled) continue;
+        opaque[d] = box_intersection(opaque[d], vis[b].clip);
+        if (opaque[d].area() != 0) occluded = true;
+    }
+    uint32_t batch = kNone;
+    TileBox cur{0, 0, 0, 0};
+    for (uint32_t d = D; d-- > 0;) {
+        const uint32_t id = batch_of(d);
+        if (id == kNone) continue;  // not drawn by itself (a clip's own draw)
+        const uint32_t vb = vis[id].enabled ? id : kNone;
+        if (batch != vb || batch == kNone) cur = TileBox{0, 0, 0, 0};
+        batch = vb;
+        cover[4 * size_t(d)] = cur.x0;
+        cover[4 * size_t(d) + 1] = cur.y0;
+        cover[4 * size_t(d) + 2] = cur.x1;
+        cover[4 * size_t(d) + 3] = cur.y1;
+        if (vb != kNone && opaque[d].area() > cur.area()) cur = opaque[d];
+    }
+    return occluded;
+}
+
+namespace {
 
 void compile_into(const tk_canvas *c, uint32_t list, std::vector<uint32_t> &stack,
                   std::vector<PlannedBatch> &ops) {
@@ -1225,84 +1201,52 @@ tk_status encode(tk_canvas *c, uint32_t clear_color, tk_scene *out) {
         }
     }
 
-    // Tile bins: painter-ordered candidate lists per tile (tile_bins/transient.rs:7-146).
+    // Paint blob = sdf || sdf_shadow || brush (src/render/upload/paint.rs:112-113).
+    c->paint.clear();
+    c->paint.insert(c->paint.end(), c->sdf_blob.begin(), c->sdf_blob.end());
+    c->paint.insert(c->paint.end(), c->shadow_blob.begin(), c->shadow_blob.end());
+    c->paint.insert(c->paint.end(), c->brush_blob.begin(), c->brush_blob.end());
+
+    // Tile bins: painter-ordered candidate lists per tile (tile_bins/transient.rs:7-146),
+    // minus the tiles early occlusion hides (draw_covers).
     std::vector<TileBox> boxes(c->draws.size());
     for (uint32_t d : draw_order) boxes[d] = tile_bbox(c->draws[d].pixel_bounds, tw, th);
-
-    // Occlusion::update (tile_bins/occlusion.rs:48-160): each batch's first draw
-    // in painter order, whether every member is a Brush / PathGlyph draw, and,
-    // for every later member with an opaque proof, its proof cut to the whole
-    // tiles its batch's clip rectangles cover.
-    struct BatchVis {
-        uint32_t first = kNone;
-        bool enabled = true, has_occluder = false;
-        TileBox clip{0, 0, 0, 0};
-    };
-    std::vector<BatchVis> vis(batches.size());
-    for (uint32_t d : draw_order) {
-        BatchVis &bv = vis[c->draw_batch_ids[d]];
-        bv.enabled = bv.enabled && (c->draws[d].tag == TK_DRAW_BRUSH || c->draws[d].tag == TK_DRAW_PATH_GLYPH);
-        if (bv.first == kNone) bv.first = d;
-    }
-    std::vector<TileBox> opaque(c->draws.size(), TileBox{0, 0, 0, 0});
-    for (uint32_t d = 0; d < uint32_t(c->draws.size()); ++d) {
-        const uint32_t id = c->draw_batch_ids[d];
-        if (id == kNone || !vis[id].enabled || vis[id].first == d) continue;
-        opaque[d] = opaque_tiles(c, c->draws[d]);
-        if (opaque[d].area() != 0) vis[id].has_occluder = true;
-    }
-    bool occluded = false;
-    for (size_t b = 0; b < batches.size(); ++b) {  // batch_clips (occlusion/geometry.rs:17-56)
-        if (!vis[b].has_occluder) continue;
-        TileBox clip{0, 0, tw, th};
-        for (uint32_t cd : batches[b].stack) clip = box_intersection(clip, rect_tiles(c, c->draws[cd]));
-        vis[b].clip = clip;
-    }
-    for (uint32_t d = 0; d < uint32_t(c->draws.size()); ++d) {
-        const uint32_t id = c->draw_batch_ids[d];
-        if (id == kNone || !vis[id].enabled) continue;
-        opaque[d] = box_intersection(opaque[d], vis[id].clip);
-        if (opaque[d].area() != 0) occluded = true;
-    }
-    // for_each_visible (occlusion/transient.rs:9-66): draws in reverse painter
-    // order; within one batch the largest proof seen so far (a strictly larger
-    // one replaces it, never a union) hides the tiles it covers.
-    auto for_each_visible = [&](auto visit) {
-        uint32_t batch = kNone;
-        TileBox cover{0, 0, 0, 0};
-        for (size_t k = draw_order.size(); k-- > 0;) {
-            const uint32_t d = draw_order[k];
-            const uint32_t id = c->draw_batch_ids[d];
-            const uint32_t vb = id != kNone && vis[id].enabled ? id : kNone;
-            if (batch != vb || batch == kNone) cover = TileBox{0, 0, 0, 0};
-            batch = vb;
-            const TileBox &bb = boxes[d];
-            const TileBox hidden = box_intersection(bb, cover);
-            auto region = [&](const TileBox &r) {
-                for (uint32_t y = r.y0; y < r.y1; ++y)
-                    for (uint32_t x = r.x0; x < r.x1; ++x) visit(size_t(y) * tw + x, d);
-            };
-            if (hidden.area() == 0) {
-                region(bb);
-            } else {
-                region(TileBox{bb.x0, bb.y0, bb.x1, hidden.y0});
-                region(TileBox{bb.x0, hidden.y1, bb.x1, bb.y1});
-                region(TileBox{bb.x0, hidden.y0, hidden.x0, hidden.y1});
-                region(TileBox{hidden.x1, hidden.y0, bb.x1, hidden.y1});
-            }
-            if (vb != kNone && opaque[d].area() > cover.area()) cover = opaque[d];
+    tk_scene view{};
+    view.tiles_width = tw;
+    view.tiles_height = th;
+    view.draws = c->draws.data();
+    view.draw_count = uint32_t(c->draws.size());
+    view.paint = c->paint.data();
+    view.paint_words = uint32_t(c->paint.size());
+    view.paint_sdf_shadow_base = uint32_t(c->sdf_blob.size());
+    view.paint_brush_base = uint32_t(c->sdf_blob.size() + c->shadow_blob.size());
+    view.draw_batch_ids = c->draw_batch_ids.data();
+    view.layers = c->layer_records.data();
+    view.layer_count = uint32_t(c->layer_records.size());
+    view.batches = c->batch_records.data();
+    view.batch_count = uint32_t(c->batch_records.size());
+    std::vector<uint32_t> cover;
+    draw_covers(view, cover);
+    // A draw's list entries: the tiles of its box outside its cover.
+    auto visible = [&](uint32_t d, auto visit) {
+        const TileBox &bb = boxes[d];
+        const TileBox hidden =
+            box_intersection(bb, TileBox{cover[4 * d], cover[4 * d + 1], cover[4 * d + 2], cover[4 * d + 3]});
+        auto region = [&](const TileBox &r) {
+            for (uint32_t y = r.y0; y < r.y1; ++y)
+                for (uint32_t x = r.x0; x < r.x1; ++x) visit(size_t(y) * tw + x);
+        };
+        if (hidden.area() == 0) {
+            region(bb);
+        } else {
+            region(TileBox{bb.x0, bb.y0, bb.x1, hidden.y0});
+            region(TileBox{bb.x0, hidden.y1, bb.x1, bb.y1});
+            region(TileBox{bb.x0, hidden.y0, hidden.x0, hidden.y1});
+            region(TileBox{hidden.x1, hidden.y0, bb.x1, hidden.y1});
         }
     };
     std::vector<uint32_t> counts(tiles, 0u);
-    if (occluded) {
-        for_each_visible([&](size_t t, uint32_t) { counts[t]++; });
-    } else {
-        for (uint32_t d : draw_order) {
-            const TileBox &bb = boxes[d];
-            for (uint32_t y = bb.y0; y < bb.y1; ++y)
-                for (uint32_t x = bb.x0; x < bb.x1; ++x) counts[size_t(y) * tw + x]++;
-        }
-    }
+    for (uint32_t d : draw_order) visible(d, [&](size_t t) { counts[t]++; });
     c->tile_draws.assign(tiles, tk_tile_draw_record{kNone, 0u});
     uint64_t total = 0;
     for (size_t t = 0; t < tiles; ++t) {
@@ -1312,19 +1256,9 @@ tk_status encode(tk_canvas *c, uint32_t clear_color, tk_scene *out) {
     }
     if (total > 0xFFFFFFFFull) return fail(TK_ERR_OVERFLOW, "tile draw references exceed u32");
     c->tile_draw_indices.assign(total, 0u);
-    if (occluded) {  // reverse painter order: each list fills from its end
-        for_each_visible([&](size_t t, uint32_t d) { c->tile_draw_indices[c->tile_draws[t].start + --counts[t]] = d; });
-    } else {
-        std::vector<uint32_t> cursor(tiles, 0u);
-        for (uint32_t d : draw_order) {
-            const TileBox &bb = boxes[d];
-            for (uint32_t y = bb.y0; y < bb.y1; ++y)
-                for (uint32_t x = bb.x0; x < bb.x1; ++x) {
-                    size_t t = size_t(y) * tw + x;
-                    c->tile_draw_indices[c->tile_draws[t].start + cursor[t]++] = d;
-                }
-        }
-    }
+    std::vector<uint32_t> cursor(tiles, 0u);
+    for (uint32_t d : draw_order)
+        visible(d, [&](size_t t) { c->tile_draw_indices[c->tile_draws[t].start + cursor[t]++] = d; });
 
     // PTCL bound over all batches: per batch, particles plus (2 depth + 1) per
     // tile that can hold a particle; tiles outside any clip's bbox hold none.
@@ -1383,12 +1317,6 @@ tk_status encode(tk_canvas *c, uint32_t clear_color, tk_scene *out) {
         }
     }
     if (c->segment_total > 0xFFFFFFFFull) return fail(TK_ERR_OVERFLOW, "segment capacity exceeds u32");
-
-    // Paint blob = sdf || sdf_shadow || brush (src/render/upload/paint.rs:112-113).
-    c->paint.clear();
-    c->paint.insert(c->paint.end(), c->sdf_blob.begin(), c->sdf_blob.end());
-    c->paint.insert(c->paint.end(), c->shadow_blob.begin(), c->shadow_blob.end());
-    c->paint.insert(c->paint.end(), c->brush_blob.begin(), c->brush_blob.end());
 
     std::memset(out, 0, sizeof(*out));
     out->width = c->width;

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "tileink_hip.h"
 
@@ -17,6 +18,11 @@ inline tk_status fail(tk_status status, const char *what) {
     set_error("%s", what);
     return status;
 }
+
+// Per draw of an encoded scene (4 words each: x0, y0, x1, y1 in its canvas's
+// tiles): the rectangle early occlusion hides from the draw's tile bins, empty
+// when nothing does (canvas.cpp).  Returns whether any draw is occluded.
+bool draw_covers(const tk_scene &s, std::vector<uint32_t> &cover);
 
 // Device address of `device`'s copy of the fixed no-alloc scratch array
 // (TK_NO_ALLOC_SCRATCH_BYTES bytes, 256-byte aligned); `device` must be current.
