@@ -380,6 +380,31 @@ typedef struct tk_tuning {
     uint32_t scan_block, coarse_block, coarse_scan_items;
     uint32_t reserved;
 } tk_tuning;
+/* GPU front end for polygon scenes.  The reference records every path on the
+ * CPU (Canvas::push_path_inner_with_tag, src/canvas/paths.rs:65-156, with the
+ * scan plans of src/shared/gpu_plan/path_plans.rs:156-228,463-504 and the tile
+ * bins of src/shared/gpu_plan/tile_bins/build.rs:4-83 and transient.rs:7-146);
+ * this entry moves the geometry-dependent part of that work onto the device for
+ * scenes whose paths are closed polygons (recorded with tk_canvas_push_polygons,
+ * or move/line/close paths of one contour each).
+ * It replaces the vertices of every path of the frame (or batch frame) and
+ * rebuilds on the device exactly what recording the same polygons and encoding
+ * would: the lines (degenerate edges dropped), the path records (bounds, tile
+ * boxes, backdrop slabs, segment capacities), the path draws' pixel bounds, the
+ * painter-ordered tile bins minus the tiles early occlusion hides, the scan and
+ * cumsum chunk plans and the PTCL bound.  Draws, paints, layers and batches stay
+ * as recorded.  The frame's buffers grow when the new geometry needs more.
+ *   points:      device, (x, y) FP32 pairs in physical pixels, path after path;
+ *   starts:      device, path_count + 1 offsets into the vertices;
+ *   point_count: the number of vertices;
+ *   line_vertex: optional device output, point_count entries: for each line
+ *                (in order), the index of the vertex it starts from.
+ * One host synchronisation sizes the buffers.  A non-finite vertex is
+ * TK_ERR_INVALID, after which the frame must receive valid vertices before it
+ * renders.  Any captured graph is dropped. */
+tk_status tk_frame_set_polygons(tk_frame *frame, const float *points, const uint32_t *starts, uint64_t point_count,
+                                uint32_t *line_vertex);
+
 tk_status tk_frame_get_tuning(tk_frame *frame, tk_tuning *out);
 tk_status tk_frame_set_tuning(tk_frame *frame, const tk_tuning *tuning);
 tk_status tk_frame_autotune(tk_frame *frame, uint32_t repeats, tk_tuning *chosen);
@@ -458,6 +483,18 @@ typedef enum tk_frame_buffer {
     TK_BUF_BATCH_PTCL_BASES = 9,   /* u32 x Q: first record of each batch           */
     TK_BUF_PTCL = 10,              /* tk_ptcl_record x K                            */
     TK_BUF_RAW_BACKDROPS = 11,     /* i32 x B: backdrop deltas before the row prefix */
+    /* The scene as the frame holds it (as uploaded, or as tk_frame_set_polygons rebuilt it): */
+    TK_BUF_LINES = 12,             /* tk_line x L                                   */
+    TK_BUF_PATHS = 13,             /* tk_path_record x P                            */
+    TK_BUF_DRAWS = 14,             /* tk_draw_record x D                            */
+    TK_BUF_TILE_DRAWS = 15,        /* tk_tile_draw_record x T                       */
+    TK_BUF_TILE_DRAW_INDICES = 16, /* u32 x R                                       */
+    TK_BUF_SCAN_CHUNKS = 17,       /* tk_scan_chunk x scan_chunk_count              */
+    TK_BUF_SCAN_CHUNK_RANGES = 18, /* tk_scan_chunk_range x P                       */
+    TK_BUF_CUMSUM_CHUNK_OFFSETS = 19, /* u32 x cumsum_chunk_count                   */
+    TK_BUF_CUMSUM_CHUNK_LENS = 20,    /* u32 x cumsum_chunk_count                   */
+    TK_BUF_CUMSUM_ROW_STARTS = 21,    /* u32 x cumsum_row_count                     */
+    TK_BUF_CUMSUM_ROW_ENDS = 22,      /* u32 x cumsum_row_count                     */
 } tk_frame_buffer;
 tk_status tk_frame_read_buffer(tk_frame *frame, tk_frame_buffer which, void *out, size_t bytes);
 
@@ -471,6 +508,8 @@ typedef struct tk_frame_stats {
     uint32_t segments_emitted;                      /* S after the last render */
     uint32_t overflow;                              /* nonzero when a bound was exceeded */
     uint64_t device_bytes;                          /* device memory held by the frame */
+    uint32_t scan_chunk_count, cumsum_chunk_count, cumsum_row_count, pad;
+    uint64_t ptcl_bound;                            /* the scene's PTCL bound (<= ptcl_capacity) */
 } tk_frame_stats;
 /* Synchronises; ptcl_records / segments_emitted / overflow come from the device. */
 tk_status tk_frame_stats_read(tk_frame *frame, tk_frame_stats *out);

@@ -465,6 +465,51 @@ class Frame:
         tiles = np.ascontiguousarray(np.asarray(tiles, dtype=np.uint32).reshape(-1))
         check(lib().tk_frame_render_tiles(self._h, ptr(tiles) if tiles.size else None, int(tiles.size)))
 
+    def set_polygons(self, points, starts, line_vertex=False):
+        """GPU front end (tk_frame_set_polygons): new vertices for every path of a
+        polygon scene; the device rebuilds lines, path records, bounds, tile bins
+        and plans.  points: (N, 2) float32 and starts: (path_count + 1,) uint32,
+        either torch tensors on the GPU (read in place) or numpy arrays (uploaded).
+        With line_vertex, returns each line's first-vertex index (a torch int32
+        tensor for torch input, else a numpy array), trimmed to the line count."""
+        keep = []
+
+        def device(arr, dtype, torch_dtype):
+            if hasattr(arr, "data_ptr"):
+                t = arr.contiguous()
+                if t.dtype != torch_dtype or not t.is_cuda:
+                    raise TileinkError(_abi.TK_ERR_INVALID, f"expected a {torch_dtype} tensor on the GPU")
+                keep.append(t)
+                return t.data_ptr(), t.numel()
+            a = np.ascontiguousarray(arr, dtype=dtype).reshape(-1)
+            buf = self.context.to_device(a if a.size else np.zeros(1, dtype))
+            keep.append(buf)
+            return buf.ptr, a.size
+
+        import importlib
+        torch = importlib.import_module("torch") if any(hasattr(x, "data_ptr") for x in (points, starts)) else None
+        pp, n = device(points, np.float32, getattr(torch, "float32", None))
+        sp, ns = device(starts, np.uint32, getattr(torch, "int32", None))
+        if n % 2 or ns != self._path_count() + 1:
+            raise TileinkError(_abi.TK_ERR_INVALID, "points are (x, y) pairs and starts has path_count + 1 entries")
+        out = None
+        if line_vertex:
+            if torch is not None:
+                out = torch.empty(max(n // 2, 1), dtype=torch.int32, device=keep[0].device)
+                optr = out.data_ptr()
+            else:
+                out = DeviceBuffer(self.context, 4 * max(n // 2, 1))
+                optr = out.ptr
+        check(lib().tk_frame_set_polygons(self._h, C.c_void_p(pp), C.c_void_p(sp), n // 2,
+                                          C.c_void_p(optr) if line_vertex else None))
+        if not line_vertex:
+            return None
+        lines = int(self.stats()["lines"])
+        return out[:lines] if torch is not None else out.read(np.uint32, lines)
+
+    def _path_count(self):
+        return int(self.stats()["paths"])
+
     def update(self, scene):
         """Replaces the scene's arrays; every size must match (tk_frame_update)."""
         check(lib().tk_frame_update(self._h, C.byref(scene)))

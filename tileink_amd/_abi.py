@@ -28,6 +28,8 @@ BRUSH_SOLID, BRUSH_LINEAR, BRUSH_RADIAL, BRUSH_SWEEP, BRUSH_FOUR_CORNER = 1, 2, 
 BUF_BACKDROPS, BUF_SEGMENT_COUNTS, BUF_SEGMENT_RANGES, BUF_SEGMENT_CURSORS, BUF_SEGMENT_BUMPS, BUF_SEGMENTS = range(6)
 TK_TILE_KIND_INTERPRETER, TK_TILE_KIND_EMPTY, TK_TILE_KIND_COLOR, TK_TILE_KIND_SDF, TK_TILE_KIND_MIXED = range(5)
 BUF_TILE_PTCL_COUNTS, BUF_TILE_PTCL_RANGES, BUF_TILE_KINDS, BUF_BATCH_PTCL_BASES, BUF_PTCL, BUF_RAW_BACKDROPS = range(6, 12)
+(BUF_LINES, BUF_PATHS, BUF_DRAWS, BUF_TILE_DRAWS, BUF_TILE_DRAW_INDICES, BUF_SCAN_CHUNKS, BUF_SCAN_CHUNK_RANGES,
+ BUF_CUMSUM_CHUNK_OFFSETS, BUF_CUMSUM_CHUNK_LENS, BUF_CUMSUM_ROW_STARTS, BUF_CUMSUM_ROW_ENDS) = range(12, 23)
 
 
 class TileinkError(RuntimeError):
@@ -116,7 +118,9 @@ class FrameStats(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("paths", "lines", "draws", "batches", "tiles", "backdrop_len",
                                           "segment_capacity", "tile_draw_refs")] + [
         ("ptcl_capacity", C.c_uint64), ("ptcl_records", C.c_uint64), ("segments_emitted", C.c_uint32),
-        ("overflow", C.c_uint32), ("device_bytes", C.c_uint64)]
+        ("overflow", C.c_uint32), ("device_bytes", C.c_uint64),
+        ("scan_chunk_count", C.c_uint32), ("cumsum_chunk_count", C.c_uint32), ("cumsum_row_count", C.c_uint32),
+        ("pad", C.c_uint32), ("ptcl_bound", C.c_uint64)]
 
 
 class StageTimes(C.Structure):
@@ -211,6 +215,7 @@ SIGNATURES = {
     "tk_canvas_push_polygons": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P, C.POINTER(C.c_uint32)]),
     "tk_canvas_push_stroke_styled": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_size_t, C.POINTER(StrokeStyle), _P, _P,
                                                C.c_uint32, C.c_double, C.POINTER(C.c_uint32)]),
+    "tk_frame_set_polygons": (C.c_int, [_P, _P, _P, C.c_uint64, _P]),
     "tk_frame_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "tk_frame_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "tk_frame_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(Tuning)]),

@@ -2375,3 +2375,6 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_line_backward(tk_args_lin
     out[2] = -nx * g1;
     out[3] = -ny * g1;
 }
+
+// The GPU front end of polygon scenes (tk_frame_set_polygons).
+#include "frontend.hip"

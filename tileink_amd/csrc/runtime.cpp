@@ -85,6 +85,11 @@ struct tk_context {
                   f_coarse_count_persistent = nullptr, f_coarse_count_scenes_persistent = nullptr,
                   f_coarse_emit_persistent = nullptr, f_coarse_emit_scenes_persistent = nullptr,
                   f_coarse_scan_1k = nullptr, f_coarse_scan_2k = nullptr, f_coarse_scan_8k = nullptr;
+    // The GPU front end of polygon scenes.
+    hipFunction_t f_fe_paths = nullptr, f_fe_draw_bounds = nullptr, f_fe_batch_clips = nullptr,
+                  f_fe_draw_count = nullptr, f_fe_batch_caps = nullptr, f_fe_tile_counts = nullptr,
+                  f_fe_scan_blocks = nullptr, f_fe_scan_top = nullptr, f_fe_scan_apply = nullptr,
+                  f_fe_path_fill = nullptr, f_fe_tile_records = nullptr, f_fe_rows = nullptr, f_fe_tiles = nullptr;
     uint32_t compute_units = 0;
     // Scratch for the stage entry points (frames own their own).
     uint32_t *tickets = nullptr;  // [0] scan prefix, [1] coarse scan
@@ -415,6 +420,28 @@ struct tk_frame {
     // The autotuner's choice on the 4096x4096, 200 000-path frame.
     tk_tuning tuning{TK_XCD_CHUNK_SCAN, 128u, TK_XCD_CHUNK_FINE, 0u, 64u, 1u, 8192u, 0u};
     uint32_t *queues = nullptr;  // kQueueWords work counters of the persistent launches
+    // Allocated entries of the buffers whose sizes follow the geometry (the GPU
+    // front end grows them): lines, backdrop slabs, segments, scan chunks,
+    // cumsum chunks and rows.
+    uint64_t cap_lines = 0, cap_backdrops = 0, cap_segments = 0, cap_chunks = 0, cap_cs_chunks = 0, cap_cs_rows = 0;
+    // The GPU front end (tk_frame_set_polygons): host-side scene structure kept
+    // from creation, and its device scratch (allocated on first use).
+    std::vector<uint32_t> fe_cover_host;  // 4 words per draw (draw_covers)
+    std::vector<tk_fe_scene> fe_scenes_host;
+    std::vector<uint32_t> fe_scene_of_path_host, fe_scene_of_draw_host, fe_scene_of_batch_host, fe_scene_of_row_host;
+    uint32_t fe_rows = 0;
+    bool fe_ready = false;
+    uint32_t *fe_cover = nullptr, *fe_scene_of_path = nullptr, *fe_scene_of_draw = nullptr,
+             *fe_scene_of_batch = nullptr, *fe_scene_of_row = nullptr;
+    tk_fe_scene *fe_scenes = nullptr;
+    uint32_t *fe_path_vals = nullptr, *fe_batch_clip = nullptr, *fe_draw_box = nullptr, *fe_tile_counts = nullptr,
+             *fe_row_counts = nullptr, *fe_status = nullptr;
+    int32_t *fe_path_pb = nullptr, *fe_diff = nullptr;
+    unsigned long long *fe_batch_area = nullptr, *fe_totals = nullptr, *fe_block_sums = nullptr;
+    uint64_t fe_block_sum_words = 0;
+    uint4 *fe_row_entries = nullptr;
+    uint64_t fe_row_capacity = 0;
+    unsigned long long *fe_host = nullptr;  // pinned: the totals and the status word
 };
 
 namespace tk {
@@ -861,6 +888,10 @@ struct BatchLayout {
     std::vector<tk_scene_slot> slots;
     std::vector<uint32_t> scene_of_column;
     uint64_t table_entries = 0;
+    // for the GPU front end: each scene's draw covers and rows, and what belongs to which scene
+    std::vector<uint32_t> covers;
+    std::vector<tk_fe_scene> fe_scenes;
+    std::vector<uint32_t> scene_of_path, scene_of_draw, scene_of_batch, scene_of_row;
 };
 
 tk_status frame_create(tk_context *ctx, const tk_scene *s, const BatchLayout *batch, tk_frame **out) {
@@ -941,6 +972,25 @@ tk_status frame_create(tk_context *ctx, const tk_scene *s, const BatchLayout *ba
         }
     }
     ALLOC(image, uint64_t(s->width) * s->height);
+    f->cap_lines = s->line_count;
+    f->cap_backdrops = s->backdrop_len;
+    f->cap_segments = s->segment_capacity;
+    f->cap_chunks = s->scan_chunk_count;
+    f->cap_cs_chunks = s->cumsum_chunk_count;
+    f->cap_cs_rows = s->cumsum_row_count;
+    if (batch) {
+        f->fe_cover_host = batch->covers;
+        f->fe_scenes_host = batch->fe_scenes;
+        f->fe_scene_of_path_host = batch->scene_of_path;
+        f->fe_scene_of_draw_host = batch->scene_of_draw;
+        f->fe_scene_of_batch_host = batch->scene_of_batch;
+        f->fe_scene_of_row_host = batch->scene_of_row;
+        f->fe_rows = uint32_t(batch->scene_of_row.size());
+    } else {
+        draw_covers(*s, f->fe_cover_host);
+        f->fe_scenes_host.push_back(tk_fe_scene{s->tiles_width, s->tiles_height, 0u, 0u, 0u, s->draw_count});
+        f->fe_rows = s->tiles_height;
+    }
     ALLOC(tickets, 4);
     ALLOC(coarse_status, frame_scan_blocks(f));
     ALLOC(queues, kQueueWords);
@@ -1055,6 +1105,19 @@ tk_status tk_context_create(int device, tk_context **out) {
         {&ctx->f_coarse_scan_1k, "tk_k_coarse_scan_1k"},
         {&ctx->f_coarse_scan_2k, "tk_k_coarse_scan_2k"},
         {&ctx->f_coarse_scan_8k, "tk_k_coarse_scan_8k"},
+        {&ctx->f_fe_paths, "tk_k_fe_paths"},
+        {&ctx->f_fe_draw_bounds, "tk_k_fe_draw_bounds"},
+        {&ctx->f_fe_batch_clips, "tk_k_fe_batch_clips"},
+        {&ctx->f_fe_draw_count, "tk_k_fe_draw_count"},
+        {&ctx->f_fe_batch_caps, "tk_k_fe_batch_caps"},
+        {&ctx->f_fe_tile_counts, "tk_k_fe_tile_counts"},
+        {&ctx->f_fe_scan_blocks, "tk_k_fe_scan_blocks"},
+        {&ctx->f_fe_scan_top, "tk_k_fe_scan_top"},
+        {&ctx->f_fe_scan_apply, "tk_k_fe_scan_apply"},
+        {&ctx->f_fe_path_fill, "tk_k_fe_path_fill"},
+        {&ctx->f_fe_tile_records, "tk_k_fe_tile_records"},
+        {&ctx->f_fe_rows, "tk_k_fe_rows"},
+        {&ctx->f_fe_tiles, "tk_k_fe_tiles"},
     };
     for (auto &entry : table) {
         if (hipModuleGetFunction(entry.fn, ctx->module, entry.name) != hipSuccess) {
@@ -1163,6 +1226,242 @@ tk_status tk_frame_create(tk_context *ctx, const tk_scene *s, tk_frame **out) {
     return tk::frame_create(ctx, s, nullptr, out);
 }
 
+}  // extern "C"
+
+namespace tk {
+namespace {
+
+// Frees a frame buffer (the streams are idle) and allocates `count` entries.
+template <class T>
+tk_status reallocate(tk_frame *f, T **ptr, uint64_t count) {
+    if (*ptr) {
+        auto it = std::find(f->allocations.begin(), f->allocations.end(), static_cast<void *>(*ptr));
+        if (it != f->allocations.end()) f->allocations.erase(it);
+        TK_HIP(hipFree(*ptr));
+        *ptr = nullptr;
+    }
+    return dalloc(f, ptr, count);
+}
+
+// Capacity for `need` entries where `have` are allocated: unchanged when they
+// fit, else a quarter more than needed.
+bool grow(uint64_t &have, uint64_t need) {
+    if (need <= have) return false;
+    have = need + need / 4 + 16;
+    return true;
+}
+
+tk_status fe_prepare(tk_frame *f) {
+    if (f->fe_ready) return TK_OK;
+    const tk_scene &s = f->meta;
+    const uint64_t P = s.path_count, D = s.draw_count, Q = s.batch_count, R = f->fe_rows;
+    const uint64_t C = uint64_t(s.tiles_width) * s.tiles_height;  // table columns (a batch: all scenes' tiles)
+    tk_status st = TK_OK;
+#define FE_ALLOC(ptr, n) \
+    if ((st = dalloc(f, &f->ptr, (n))) != TK_OK) return st
+    FE_ALLOC(fe_cover, std::max<uint64_t>(4 * D, 4));
+    FE_ALLOC(fe_scenes, f->fe_scenes_host.size());
+    if (!f->fe_scene_of_path_host.empty()) {
+        FE_ALLOC(fe_scene_of_path, f->fe_scene_of_path_host.size());
+        FE_ALLOC(fe_scene_of_draw, std::max<size_t>(f->fe_scene_of_draw_host.size(), 1));
+        FE_ALLOC(fe_scene_of_batch, std::max<size_t>(f->fe_scene_of_batch_host.size(), 1));
+        FE_ALLOC(fe_scene_of_row, std::max<size_t>(f->fe_scene_of_row_host.size(), 1));
+        TK_TRY(upload(f, f->fe_scene_of_path, f->fe_scene_of_path_host.data(), f->fe_scene_of_path_host.size()));
+        TK_TRY(upload(f, f->fe_scene_of_draw, f->fe_scene_of_draw_host.data(), f->fe_scene_of_draw_host.size()));
+        TK_TRY(upload(f, f->fe_scene_of_batch, f->fe_scene_of_batch_host.data(), f->fe_scene_of_batch_host.size()));
+        TK_TRY(upload(f, f->fe_scene_of_row, f->fe_scene_of_row_host.data(), f->fe_scene_of_row_host.size()));
+    }
+    TK_TRY(upload(f, f->fe_cover, f->fe_cover_host.data(), f->fe_cover_host.size()));
+    TK_TRY(upload(f, f->fe_scenes, f->fe_scenes_host.data(), f->fe_scenes_host.size()));
+    FE_ALLOC(fe_path_vals, std::max<uint64_t>(TK_FE_PATH_COLUMNS * P, 1));
+    FE_ALLOC(fe_path_pb, std::max<uint64_t>(4 * P, 4));
+    FE_ALLOC(fe_batch_clip, std::max<uint64_t>(4 * Q, 4));
+    FE_ALLOC(fe_batch_area, std::max<uint64_t>(Q, 1));
+    FE_ALLOC(fe_diff, C + R + 1);
+    FE_ALLOC(fe_draw_box, std::max<uint64_t>(4 * D, 4));
+    FE_ALLOC(fe_tile_counts, std::max<uint64_t>(C, 1));
+    FE_ALLOC(fe_row_counts, std::max<uint64_t>(R, 1));
+    FE_ALLOC(fe_totals, TK_FE_TOTALS);
+    FE_ALLOC(fe_status, 4);
+    const uint64_t blocks = std::max<uint64_t>({div_up(P, 1024), div_up(C, 1024), div_up(R, 1024), 1});
+    f->fe_block_sum_words = blocks * TK_FE_PATH_COLUMNS;
+    FE_ALLOC(fe_block_sums, f->fe_block_sum_words);
+#undef FE_ALLOC
+    if (hipHostMalloc(reinterpret_cast<void **>(&f->fe_host), (TK_FE_TOTALS + 1) * sizeof(unsigned long long)) !=
+        hipSuccess)
+        return fail(TK_ERR_OOM, "pinned host allocation failed");
+    f->fe_ready = true;
+    return TK_OK;
+}
+
+tk_status fe_scan(tk_context *ctx, tk_frame *f, uint32_t *values, uint32_t count, uint32_t columns,
+                  unsigned long long *totals) {
+    if (count == 0) {
+        return fill_u32(ctx, totals, 2ull * columns, 0u);
+    }
+    const uint32_t blocks = div_up(count, 1024);
+    tk_args_fe_scan a{values, count, columns, f->fe_block_sums, blocks, totals, f->fe_status + 1};
+    TK_TRY(launch(ctx, ctx->f_fe_scan_blocks, blocks, columns, 256, a));
+    TK_TRY(launch(ctx, ctx->f_fe_scan_top, columns, 1, 256, a));
+    return launch(ctx, ctx->f_fe_scan_apply, blocks, columns, 256, a);
+}
+
+tk_status frame_set_polygons(tk_frame *f, const float *points, const uint32_t *starts, uint64_t point_count,
+                             uint32_t *line_vertex) {
+    tk_context *ctx = f->ctx;
+    TK_TRY(fe_prepare(f));
+    tk_scene &s = f->meta;
+    const uint32_t P = s.path_count, D = s.draw_count, Q = s.batch_count, R = f->fe_rows;
+    const uint32_t C = s.tiles_width * s.tiles_height;  // table columns (a batch: all scenes' tiles)
+    TK_HIP(hipStreamSynchronize(ctx->side));
+    tk_args_frontend a{};
+    a.points = points;
+    a.starts = starts;
+    a.path_count = P;
+    a.draw_count = D;
+    a.batch_count = Q;
+    a.layer_count = s.layer_count;
+    a.scene_count = uint32_t(f->fe_scenes_host.size());
+    a.row_count = R;
+    a.column_count = C;
+    a.scenes = f->fe_scenes;
+    a.scene_of_path = f->fe_scene_of_path;
+    a.scene_of_draw = f->fe_scene_of_draw;
+    a.scene_of_batch = f->fe_scene_of_batch;
+    a.scene_of_row = f->fe_scene_of_row;
+    a.covers = f->fe_cover;
+    a.path_vals = f->fe_path_vals;
+    a.path_pb = f->fe_path_pb;
+    a.batch_clip = f->fe_batch_clip;
+    a.batch_area = f->fe_batch_area;
+    a.diff = f->fe_diff;
+    a.draw_box = f->fe_draw_box;
+    a.tile_counts = f->fe_tile_counts;
+    a.row_counts = f->fe_row_counts;
+    a.totals = f->fe_totals;
+    a.status = f->fe_status;
+    a.paths = f->paths;
+    a.draws = f->draws;
+    a.batch_ids = f->batch_ids;
+    a.layers = f->layers;
+    a.batches = f->batches;
+    // Phase 1: counts, bounds and prefix sums.
+    TK_TRY(fill_u32(ctx, f->fe_diff, uint64_t(C) + R + 1, 0u));
+    TK_TRY(fill_u32(ctx, f->fe_row_counts, std::max<uint32_t>(R, 1), 0u));
+    TK_TRY(fill_u32(ctx, f->fe_batch_area, 2ull * std::max<uint32_t>(Q, 1), 0u));
+    TK_TRY(fill_u32(ctx, f->fe_totals, 2ull * TK_FE_TOTALS, 0u));
+    TK_TRY(fill_u32(ctx, f->fe_status, 4, 0u));
+    if (P) TK_TRY(launch(ctx, ctx->f_fe_paths, div_up(P, 4), 1, 256, a));
+    if (D) TK_TRY(launch(ctx, ctx->f_fe_draw_bounds, div_up(D, 256), 1, 256, a));
+    if (Q) TK_TRY(launch(ctx, ctx->f_fe_batch_clips, div_up(Q, 256), 1, 256, a));
+    if (D) TK_TRY(launch(ctx, ctx->f_fe_draw_count, div_up(D, 256), 1, 256, a));
+    if (Q) TK_TRY(launch(ctx, ctx->f_fe_batch_caps, div_up(Q, 256), 1, 256, a));
+    if (R) TK_TRY(launch(ctx, ctx->f_fe_tile_counts, div_up(R, 256), 1, 256, a));
+    TK_TRY(fe_scan(ctx, f, f->fe_path_vals, P, TK_FE_PATH_COLUMNS, f->fe_totals));
+    TK_TRY(fe_scan(ctx, f, f->fe_tile_counts, C, 1, f->fe_totals + TK_FE_TILE_REFS));
+    TK_TRY(fe_scan(ctx, f, f->fe_row_counts, R, 1, f->fe_totals + TK_FE_ROW_ENTRIES));
+    TK_HIP(hipMemcpyAsync(f->fe_host, f->fe_totals, TK_FE_TOTALS * 8, hipMemcpyDeviceToHost, ctx->stream));
+    TK_HIP(hipMemcpyAsync(f->fe_host + TK_FE_TOTALS, f->fe_status, 8, hipMemcpyDeviceToHost, ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    const unsigned long long *t = f->fe_host;
+    const uint32_t flags = uint32_t(t[TK_FE_TOTALS]), overflow = uint32_t(t[TK_FE_TOTALS] >> 32);
+    if (flags & 1u) return fail(TK_ERR_INVALID, "non-finite vertex: the frame needs valid vertices before it renders");
+    if (overflow || t[TK_FE_TILE_REFS] > 0xFFFFFFFFull || t[TK_FE_PTCL] >= 0xFFFFFFFFull ||
+        t[TK_FE_ROW_ENTRIES] > 0xFFFFFFFFull)
+        return fail(TK_ERR_OVERFLOW, "geometry exceeds 32-bit record indices");
+    if (t[TK_FE_LINES] > point_count) return fail(TK_ERR_INVALID, "more lines than vertices");
+
+    // Grow what the new geometry needs (the streams are idle).
+    drop_graph(f);
+    const uint64_t lines = t[TK_FE_LINES], backdrops = t[TK_FE_BACKDROPS], segments = t[TK_FE_SEGMENTS];
+    const uint64_t chunks = t[TK_FE_SCAN_CHUNKS], cs_rows = t[TK_FE_CUMSUM_ROWS], cs_chunks = t[TK_FE_CUMSUM_CHUNKS];
+    if (grow(f->cap_lines, lines)) TK_TRY(reallocate(f, &f->lines, f->cap_lines));
+    if (grow(f->cap_backdrops, backdrops)) {
+        TK_TRY(reallocate(f, &f->backdrops, f->cap_backdrops));
+        TK_TRY(reallocate(f, &f->raw_backdrops, f->cap_backdrops));
+        TK_TRY(reallocate(f, &f->seg_counts, f->cap_backdrops));
+        TK_TRY(reallocate(f, &f->seg_ranges, f->cap_backdrops));
+        TK_TRY(reallocate(f, &f->cursors, f->cap_backdrops));
+        if (f->diff_rec_of) TK_TRY(reallocate(f, &f->diff_rec_of, f->cap_backdrops));
+    }
+    if (grow(f->cap_segments, segments)) TK_TRY(reallocate(f, &f->segments, f->cap_segments));
+    if (grow(f->cap_chunks, chunks)) {
+        TK_TRY(reallocate(f, &f->scan_chunks, f->cap_chunks));
+        TK_TRY(reallocate(f, &f->chunk_totals, f->cap_chunks));
+        TK_TRY(reallocate(f, &f->chunk_offsets, f->cap_chunks));
+    }
+    if (grow(f->cap_cs_chunks, cs_chunks)) {
+        TK_TRY(reallocate(f, &f->cs_offsets, f->cap_cs_chunks));
+        TK_TRY(reallocate(f, &f->cs_lens, f->cap_cs_chunks));
+        TK_TRY(reallocate(f, &f->cs_totals, f->cap_cs_chunks));
+        TK_TRY(reallocate(f, &f->cs_carries, f->cap_cs_chunks));
+        TK_TRY(fill_u32(ctx, f->cs_carries, f->cap_cs_chunks, 0u));
+    }
+    if (grow(f->cap_cs_rows, cs_rows)) {
+        TK_TRY(reallocate(f, &f->cs_row_starts, f->cap_cs_rows));
+        TK_TRY(reallocate(f, &f->cs_row_ends, f->cap_cs_rows));
+    }
+    uint64_t index_cap = f->index_capacity;
+    if (grow(index_cap, t[TK_FE_TILE_REFS])) {
+        TK_TRY(reallocate(f, &f->tile_draw_indices, index_cap));
+        f->index_capacity = uint32_t(std::min<uint64_t>(index_cap, 0xFFFFFFFFull));
+    }
+    uint64_t ptcl_cap = f->ptcl_capacity;
+    if (grow(ptcl_cap, std::max<uint64_t>(t[TK_FE_PTCL], 1))) {
+        ptcl_cap = std::min<uint64_t>(ptcl_cap, 0xFFFFFFFEull);
+        TK_TRY(reallocate(f, &f->ptcl, ptcl_cap));
+        f->ptcl_capacity = ptcl_cap;
+    }
+    if (grow(f->fe_row_capacity, t[TK_FE_ROW_ENTRIES])) TK_TRY(reallocate(f, &f->fe_row_entries, f->fe_row_capacity));
+
+    // Phase 2: the fills.
+    a.lines = f->lines;
+    a.line_vertex = line_vertex;
+    a.scan_chunks = f->scan_chunks;
+    a.chunk_ranges = f->scan_chunk_ranges;
+    a.cs_offsets = f->cs_offsets;
+    a.cs_lens = f->cs_lens;
+    a.cs_row_starts = f->cs_row_starts;
+    a.cs_row_ends = f->cs_row_ends;
+    a.tile_draws = f->tile_draws;
+    a.tile_draw_indices = f->tile_draw_indices;
+    a.row_entries = f->fe_row_entries;
+    a.line_capacity = f->cap_lines;
+    a.chunk_capacity = f->cap_chunks;
+    a.cs_chunk_capacity = f->cap_cs_chunks;
+    a.cs_row_capacity = f->cap_cs_rows;
+    a.index_capacity = f->index_capacity;
+    a.row_capacity = f->fe_row_capacity;
+    if (P) TK_TRY(launch(ctx, ctx->f_fe_path_fill, div_up(P, 4), 1, 256, a));
+    if (C) TK_TRY(launch(ctx, ctx->f_fe_tile_records, div_up(C, 256), 1, 256, a));
+    if (R) TK_TRY(launch(ctx, ctx->f_fe_rows, R, 1, 256, a));
+    if (R) TK_TRY(launch(ctx, ctx->f_fe_tiles, R, 1, 256, a));
+    s.line_count = uint32_t(lines);
+    s.backdrop_len = uint32_t(backdrops);
+    s.segment_capacity = uint32_t(segments);
+    s.scan_chunk_count = uint32_t(chunks);
+    s.cumsum_chunk_count = uint32_t(cs_chunks);
+    s.cumsum_row_count = uint32_t(cs_rows);
+    s.tile_draw_index_count = uint32_t(t[TK_FE_TILE_REFS]);
+    s.ptcl_capacity = t[TK_FE_PTCL];
+    f->diff_ready = false;
+    return TK_OK;
+}
+
+}  // namespace
+}  // namespace tk
+
+extern "C" {
+
+tk_status tk_frame_set_polygons(tk_frame *f, const float *points, const uint32_t *starts, uint64_t point_count,
+                                uint32_t *line_vertex) {
+    if (!f || (point_count && !points) || !starts) return fail(TK_ERR_INVALID, "null argument");
+    if (point_count > 0x7FFFFFFFull) return fail(TK_ERR_INVALID, "too many vertices");
+    return tk::frame_set_polygons(f, points, starts, point_count, line_vertex);
+}
+
+
+
 
 // --- batches of independent scenes -------------------------------------------
 
@@ -1233,6 +1532,18 @@ tk_status merge_scenes(const tk_scene *scenes, uint32_t count, MergedScenes &m) 
         slot.image_offset = columns * 256u;  // the scene's tiles cover its pixels
         m.layout.slots.push_back(slot);
         m.layout.scene_of_column.insert(m.layout.scene_of_column.end(), T, i);
+        {
+            std::vector<uint32_t> cover;
+            draw_covers(s, cover);
+            m.layout.covers.insert(m.layout.covers.end(), cover.begin(), cover.end());
+            m.layout.fe_scenes.push_back(tk_fe_scene{s.tiles_width, s.tiles_height, uint32_t(columns),
+                                                     uint32_t(m.layout.scene_of_row.size()), uint32_t(draw_off),
+                                                     uint32_t(draw_off + s.draw_count)});
+            m.layout.scene_of_path.insert(m.layout.scene_of_path.end(), s.path_count, i);
+            m.layout.scene_of_draw.insert(m.layout.scene_of_draw.end(), s.draw_count, i);
+            m.layout.scene_of_batch.insert(m.layout.scene_of_batch.end(), s.batch_count, i);
+            m.layout.scene_of_row.insert(m.layout.scene_of_row.end(), s.tiles_height, i);
+        }
 
         for (uint32_t k = 0; k < s.line_count; ++k) {
             tk_line l = s.lines[k];
@@ -1426,6 +1737,7 @@ void tk_frame_destroy(tk_frame *f) {
     drop_graph(f);
     for (void *p : f->allocations) (void)hipFree(p);
     if (f->diff_adj) (void)hipFree(f->diff_adj);
+    if (f->fe_host) (void)hipHostFree(f->fe_host);
     for (auto &e : f->ev)
         if (e) (void)hipEventDestroy(e);
     delete f;
@@ -1694,6 +2006,17 @@ tk_status tk_frame_read_buffer(tk_frame *f, tk_frame_buffer which, void *out, si
     case TK_BUF_TILE_KINDS: src = f->tile_kinds; n = QT * 4; break;
     case TK_BUF_BATCH_PTCL_BASES: src = f->batch_bases; n = (uint64_t(s.batch_count) + 1) * 4; break;
     case TK_BUF_PTCL: src = f->ptcl; n = f->ptcl_capacity * 24; break;
+    case TK_BUF_LINES: src = f->lines; n = uint64_t(s.line_count) * sizeof(tk_line); break;
+    case TK_BUF_PATHS: src = f->paths; n = uint64_t(s.path_count) * sizeof(tk_path_record); break;
+    case TK_BUF_DRAWS: src = f->draws; n = uint64_t(s.draw_count) * sizeof(tk_draw_record); break;
+    case TK_BUF_TILE_DRAWS: src = f->tile_draws; n = uint64_t(s.tiles_width) * s.tiles_height * 8; break;
+    case TK_BUF_TILE_DRAW_INDICES: src = f->tile_draw_indices; n = uint64_t(s.tile_draw_index_count) * 4; break;
+    case TK_BUF_SCAN_CHUNKS: src = f->scan_chunks; n = uint64_t(s.scan_chunk_count) * sizeof(tk_scan_chunk); break;
+    case TK_BUF_SCAN_CHUNK_RANGES: src = f->scan_chunk_ranges; n = uint64_t(s.path_count) * 8; break;
+    case TK_BUF_CUMSUM_CHUNK_OFFSETS: src = f->cs_offsets; n = uint64_t(s.cumsum_chunk_count) * 4; break;
+    case TK_BUF_CUMSUM_CHUNK_LENS: src = f->cs_lens; n = uint64_t(s.cumsum_chunk_count) * 4; break;
+    case TK_BUF_CUMSUM_ROW_STARTS: src = f->cs_row_starts; n = uint64_t(s.cumsum_row_count) * 4; break;
+    case TK_BUF_CUMSUM_ROW_ENDS: src = f->cs_row_ends; n = uint64_t(s.cumsum_row_count) * 4; break;
     default: return fail(TK_ERR_INVALID, "unknown buffer");
     }
     if (bytes > n) return fail(TK_ERR_INVALID, "read larger than the buffer");
@@ -1719,6 +2042,10 @@ tk_status tk_frame_stats_read(tk_frame *f, tk_frame_stats *out) {
     out->segment_capacity = s.segment_capacity;
     out->tile_draw_refs = s.tile_draw_index_count;
     out->ptcl_capacity = f->ptcl_capacity;
+    out->ptcl_bound = s.ptcl_capacity;
+    out->scan_chunk_count = s.scan_chunk_count;
+    out->cumsum_chunk_count = s.cumsum_chunk_count;
+    out->cumsum_row_count = s.cumsum_row_count;
     out->ptcl_records = status[1];
     out->overflow = status[0] | (status[3] << 1) | (status[1] > f->ptcl_capacity ? 4u : 0u) | (status[2] << 4);
     uint32_t emitted = 0;

@@ -238,3 +238,82 @@ struct tk_args_draw_tiles {
     const tk_path_record *paths;
     uint32_t *table;
 };
+
+// GPU front end of polygon scenes (tk_frame_set_polygons): one scene of the
+// frame as the front end sees it.
+struct tk_fe_scene {
+    uint32_t tiles_width, tiles_height;
+    uint32_t column_start;  // its first column of the frame's tile tables
+    uint32_t row_start;     // its first tile row among all scenes' rows
+    uint32_t draw_start, draw_end;
+};
+
+// Per-path values the front end counts, then scans in place (column-major,
+// path_count words per column).
+enum tk_fe_column : uint32_t {
+    TK_FE_LINES = 0,
+    TK_FE_BACKDROPS = 1,
+    TK_FE_SEGMENTS = 2,
+    TK_FE_SCAN_CHUNKS = 3,
+    TK_FE_CUMSUM_ROWS = 4,
+    TK_FE_CUMSUM_CHUNKS = 5,
+    TK_FE_PATH_COLUMNS = 6,
+};
+// Totals of a front-end pass (u64 words): the path columns, then these.
+enum tk_fe_total : uint32_t {
+    TK_FE_TILE_REFS = 6,
+    TK_FE_ROW_ENTRIES = 7,
+    TK_FE_PTCL = 8,
+    TK_FE_TOTALS = 9,
+};
+
+struct tk_args_frontend {
+    const float *points;              // x, y of every vertex, physical pixels
+    const uint32_t *starts;           // path_count + 1 vertex offsets
+    uint32_t path_count, draw_count, batch_count, layer_count;
+    uint32_t scene_count, row_count, column_count, pad;
+    const tk_fe_scene *scenes;
+    const uint32_t *scene_of_path;    // null: every path in scene 0
+    const uint32_t *scene_of_draw;
+    const uint32_t *scene_of_batch;
+    const uint32_t *scene_of_row;
+    const uint32_t *covers;           // 4 words per draw: the tiles early occlusion hides
+    uint32_t *path_vals;              // TK_FE_PATH_COLUMNS x path_count
+    int32_t *path_pb;                 // 4 words per path: pixel bounds
+    uint32_t *batch_clip;             // 4 words per batch: its clip stack's tile box
+    unsigned long long *batch_area;   // per batch: its draws' tile boxes within its clip box
+    int32_t *diff;                    // per scene row y, tiles_width + 1 entries at scene column_start +
+                                      // row_start + y * (tiles_width + 1): +1 / -1 at the ends of visible runs
+    uint32_t *draw_box;               // 4 words per draw: its tile box (zeros: no tiles or not batched)
+    uint32_t *tile_counts;            // per column: list length, then (scanned) list start
+    uint32_t *row_counts;             // per row: entries, then (scanned) first entry
+    uint4 *row_entries;               // per row, its draws: {draw, x0, x1, hidden x0 | hidden x1 << 16}
+    unsigned long long *totals;       // TK_FE_TOTALS
+    uint32_t *status;                 // [0] bit 0: a non-finite vertex
+    // frame buffers the front end rewrites
+    tk_line *lines;
+    uint32_t *line_vertex;            // optional: per line, the global index of its first vertex
+    tk_path_record *paths;
+    tk_draw_record *draws;
+    const uint32_t *batch_ids;
+    const tk_layer_record *layers;
+    const tk_batch_record *batches;
+    tk_scan_chunk *scan_chunks;
+    tk_scan_chunk_range *chunk_ranges;
+    uint32_t *cs_offsets, *cs_lens, *cs_row_starts, *cs_row_ends;
+    tk_tile_draw_record *tile_draws;
+    uint32_t *tile_draw_indices;
+    // capacities of the rewritten buffers (entries); writes past them are dropped
+    uint64_t line_capacity, chunk_capacity, cs_chunk_capacity, cs_row_capacity, index_capacity, row_capacity;
+};
+
+// Exclusive scan of `columns` columns of `count` u32 each (column-major), in
+// place, in blocks of 1024; block sums and column totals are u64.
+struct tk_args_fe_scan {
+    uint32_t *values;
+    uint32_t count, columns;
+    unsigned long long *block_sums;   // columns x blocks
+    uint32_t blocks;
+    unsigned long long *totals;       // columns entries
+    uint32_t *overflow;               // set when a prefix exceeds u32
+};
