@@ -291,3 +291,73 @@ def test_adam_fitting_recovers_a_perturbed_scene(ctx):
             sc.colors.clamp_(0, 1)
         losses.append(float(loss.detach()))
     assert losses[-1] < 0.5 * losses[0], losses
+
+
+def _c5_like(count, seed, size=256, lo=64, hi=513):
+    """C5's shape at `count` scenes: size x size, lo..hi-1 closed paths of 4 cubics,
+    10 % gradient paints (half radial), one rounded-rectangle clip each."""
+    rng = np.random.default_rng(seed)
+    paths = rng.integers(lo, hi, count)
+    scenes = []
+    for i, n in enumerate(paths):
+        sc = diff.random_scene(seed * 1000 + i, size, size, fills=int(n), cubics=4, radius_mu=math.log(16.0))
+        inset = rng.uniform(4, 24, 4)
+        sc.clips = [(inset[0], inset[1], size - inset[2], size - inset[3], float(rng.uniform(4, 16)), 0, int(n))]
+        diff.random_gradients(sc, 0.1, seed * 7000 + i, radial=0.5)
+        scenes.append(sc)
+    return scenes
+
+
+def test_renderer_moves_geometry_through_the_gpu_front_end(ctx):
+    # The second and later renders of a diff.Renderer send only the new vertices to
+    # tk_frame_set_polygons; images and gradients equal a fresh host recording of
+    # the same parameters (a path clip's lines map to no vertex).
+    scenes = _c5_like(5, 3, size=160, lo=20, hi=60)
+    scenes[2].clips = [(np.array([[2.0, 5.0], [150.0, 20.0], [80.5, 150.0]]), 1, 0, len(scenes[2].order))]
+    r = diff.Renderer(ctx, scenes, DEV)
+    r()
+    rng = np.random.default_rng(4)
+    for step in range(3):
+        with torch.no_grad():
+            for sc in scenes:
+                sc.fills += torch.as_tensor(rng.normal(0, 1.5, tuple(sc.fills.shape)))
+        ws = [torch.as_tensor(np.random.default_rng(step * 10 + i).normal(size=(s.height, s.width, 4)), device=DEV)
+              for i, s in enumerate(scenes)]
+        results = []
+        for make in (lambda: r(), lambda: diff.render(ctx, scenes, DEV)):
+            params = [p for s in scenes for p in (s.fills, s.colors, *[g[2] for g in s.gradients.values()])]
+            for p in params:
+                p.grad = None
+                p.requires_grad_(True)
+            images = make()
+            sum((im.double() * w).sum() for im, w in zip(images, ws)).backward()
+            results.append(([im.detach().cpu() for im in images], [p.grad.clone() for p in params]))
+            for p in params:
+                p.requires_grad_(False)
+        (ia, ga), (ib, gb) = results
+        assert all(torch.equal(a, b) for a, b in zip(ia, ib)), step
+        assert_grads_close(ga, gb, rtol=1e-5)
+
+
+def test_c5_batch_gradients_match_fp64_on_sampled_scenes(ctx):
+    # 256 scenes of 256x256 with 64-512 paths each, gradient paints and one clip:
+    # the batched forward + backward, and on three sampled scenes every parameter
+    # gradient against the FP64 analytic gradient.
+    scenes = _c5_like(256, 5)
+    ws = [np.random.default_rng(100 + i).normal(size=(256, 256, 4)) for i in range(len(scenes))]
+    params = [[p for p in (s.fills, s.colors, *[g[2] for g in s.gradients.values()])] for s in scenes]
+    for ps in params:
+        for p in ps:
+            p.grad = None
+            p.requires_grad_(True)
+    images = diff.Renderer(ctx, scenes, DEV)()
+    sum((im.double() * torch.as_tensor(w, device=DEV)).sum() for im, w in zip(images, ws)).backward()
+    got = {i: [p.grad.clone() for p in params[i]] for i in range(len(scenes))}
+    sampled = [0, 97, 255]
+    assert any(scenes[i].gradients for i in sampled)
+    for i in sampled:
+        sc = scenes[i]
+        for p in params[i]:
+            p.grad = None
+        analytic_grads(sc, ws[i])
+        assert_grads_close(got[i], [p.grad for p in params[i]])

@@ -24,15 +24,14 @@
 #include <new>
 #include <vector>
 
+#include "flatten.h"
 #include "scan_line.h"
 #include "tk_internal.h"
 
 namespace tk {
 namespace {
 
-struct Pt {
-    double x, y;
-};
+using Pt = FPt;
 
 struct BezPath {
     std::vector<uint8_t> verbs;
@@ -97,34 +96,6 @@ struct Box {
     }
 };
 
-double cubic_at(double a, double b, double c, double d, double t) {
-    double mt = 1.0 - t;
-    return mt * mt * mt * a + 3.0 * mt * mt * t * b + 3.0 * mt * t * t * c + t * t * t * d;
-}
-
-void cubic_axis_extrema(double a, double b, double c, double d, double *t_out, int *n_out) {
-    // derivative / 3 = qa t^2 + qb t + qc
-    double qa = d - 3.0 * c + 3.0 * b - a;
-    double qb = 2.0 * (c - 2.0 * b + a);
-    double qc = b - a;
-    int n = 0;
-    if (std::fabs(qa) < 1e-12) {
-        if (std::fabs(qb) > 1e-12) {
-            double t = -qc / qb;
-            if (t > 0.0 && t < 1.0) t_out[n++] = t;
-        }
-    } else {
-        double disc = qb * qb - 4.0 * qa * qc;
-        if (disc >= 0.0) {
-            double s = std::sqrt(disc);
-            double t1 = (-qb + s) / (2.0 * qa), t2 = (-qb - s) / (2.0 * qa);
-            if (t1 > 0.0 && t1 < 1.0) t_out[n++] = t1;
-            if (t2 > 0.0 && t2 < 1.0) t_out[n++] = t2;
-        }
-    }
-    *n_out = n;
-}
-
 void add_cubic_bounds(Box &box, Pt p0, Pt p1, Pt p2, Pt p3) {
     box.add(p0);
     box.add(p3);
@@ -133,12 +104,12 @@ void add_cubic_bounds(Box &box, Pt p0, Pt p1, Pt p2, Pt p3) {
     cubic_axis_extrema(p0.x, p1.x, p2.x, p3.x, ts, &n);
     for (int i = 0; i < n; ++i) {
         double t = ts[i];
-        box.add(Pt{cubic_at(p0.x, p1.x, p2.x, p3.x, t), cubic_at(p0.y, p1.y, p2.y, p3.y, t)});
+        box.add(Pt{cubic_coord(p0.x, p1.x, p2.x, p3.x, t), cubic_coord(p0.y, p1.y, p2.y, p3.y, t)});
     }
     cubic_axis_extrema(p0.y, p1.y, p2.y, p3.y, ts, &n);
     for (int i = 0; i < n; ++i) {
         double t = ts[i];
-        box.add(Pt{cubic_at(p0.x, p1.x, p2.x, p3.x, t), cubic_at(p0.y, p1.y, p2.y, p3.y, t)});
+        box.add(Pt{cubic_coord(p0.x, p1.x, p2.x, p3.x, t), cubic_coord(p0.y, p1.y, p2.y, p3.y, t)});
     }
 }
 
@@ -185,135 +156,15 @@ struct Polyline {
     bool closed = false;
 };
 
-// Quadratic flattening by the parabola-integral method (R. Levien, "Flattening
-// quadratic Beziers", 2019): the quadratic is mapped onto a segment of the
-// parabola y = x^2, where the number of chords needed for a given chord error
-// is proportional to an integral that has a closed-form approximation; chords
-// are then placed at equal steps of that integral.  Cubics are first split
-// into quadratics whose approximation error is a tenth of the tolerance.
-double parabola_integral(double x) {
-    const double d = 0.67;
-    return x / (1.0 - d + std::sqrt(std::sqrt(d * d * d * d + 0.25 * x * x)));
-}
-double parabola_inv_integral(double x) {
-    const double b = 0.39;
-    return x * (1.0 - b + std::sqrt(b * b + 0.25 * x * x));
-}
-
-struct QuadParams {
-    Pt p0, p1, p2;
-    double a0 = 0, a1 = 0, u0 = 0, uscale = 0, val = 0;
-};
-
-QuadParams quad_params(Pt p0, Pt p1, Pt p2, double sqrt_tol) {
-    QuadParams q;
-    q.p0 = p0;
-    q.p1 = p1;
-    q.p2 = p2;
-    double ddx = 2.0 * p1.x - p0.x - p2.x, ddy = 2.0 * p1.y - p0.y - p2.y;
-    double cross = (p2.x - p0.x) * ddy - (p2.y - p0.y) * ddx;
-    double x0 = ((p1.x - p0.x) * ddx + (p1.y - p0.y) * ddy) / cross;
-    double x1 = ((p2.x - p1.x) * ddx + (p2.y - p1.y) * ddy) / cross;
-    double scale = std::fabs(cross / (std::hypot(ddx, ddy) * (x1 - x0)));
-    q.a0 = parabola_integral(x0);
-    q.a1 = parabola_integral(x1);
-    if (std::isfinite(scale) && std::isfinite(q.a0) && std::isfinite(q.a1)) {
-        double da = std::fabs(q.a1 - q.a0);
-        double sqrt_scale = std::sqrt(scale);
-        if ((x0 < 0.0) == (x1 < 0.0)) {
-            q.val = da * sqrt_scale;
-        } else {
-            // The vertex of the parabola lies inside the segment: bound the
-            // integrand near the cusp by the tolerance.
-            double xmin = sqrt_tol / sqrt_scale;
-            q.val = sqrt_tol * da / parabola_integral(xmin);
-        }
-        double u0 = parabola_inv_integral(q.a0), u2 = parabola_inv_integral(q.a1);
-        q.u0 = u0;
-        q.uscale = 1.0 / (u2 - u0);
-        if (!std::isfinite(q.val) || !std::isfinite(q.uscale)) q.val = -1.0;
-    } else {
-        q.val = -1.0;  // (nearly) a straight line: one chord
-    }
-    return q;
-}
-
-Pt quad_eval(const QuadParams &q, double t) {
-    double mt = 1.0 - t;
-    return Pt{mt * mt * q.p0.x + 2.0 * mt * t * q.p1.x + t * t * q.p2.x,
-              mt * mt * q.p0.y + 2.0 * mt * t * q.p1.y + t * t * q.p2.y};
-}
-
-// Parameter on the quadratic at a fraction u in [0, 1] of its integral.
-double quad_param_at(const QuadParams &q, double u) {
-    double a = q.a0 + (q.a1 - q.a0) * u;
-    double t = (parabola_inv_integral(a) - q.u0) * q.uscale;
-    return std::fmin(std::fmax(t, 0.0), 1.0);
-}
-
-// Emits the chord end points (excluding the start) of a run of quadratics
-// approximating one curve, sharing one chord budget across them.
-void flatten_quads(const std::vector<QuadParams> &quads, double sqrt_tol, std::vector<Pt> &out) {
-    double sum = 0.0;
-    for (const QuadParams &q : quads) sum += std::max(q.val, 0.0);
-    int n = int(std::ceil(0.5 * sum / sqrt_tol));
-    n = std::max(n, 1);
-    n = std::min(n, 1 << 16);
-    double step = sum / n;
-    double acc = 0.0;  // integral consumed before the current quadratic
-    int emitted = 0;
-    for (size_t k = 0; k < quads.size(); ++k) {
-        const QuadParams &q = quads[k];
-        double val = std::max(q.val, 0.0);
-        if (q.val < 0.0) {
-            // Degenerate parabola mapping: the quadratic is a straight chord.
-            if (k + 1 < quads.size()) out.push_back(q.p2);
-            continue;
-        }
-        while (emitted + 1 < n) {
-            double target = step * (emitted + 1);
-            if (target > acc + val) break;
-            double u = val > 0.0 ? (target - acc) / val : 1.0;
-            out.push_back(quad_eval(q, quad_param_at(q, u)));
-            ++emitted;
-        }
-        acc += val;
-        if (k + 1 < quads.size()) out.push_back(q.p2);  // keep sub-curve joints exact
-    }
-}
-
+// Curve flattening (flatten.h, shared with the device front end): quadratics by
+// the parabola-integral method, cubics through quadratics at a tenth of the
+// tolerance.
 void flatten_quad_to(Pt p0, Pt p1, Pt p2, double tol, std::vector<Pt> &out) {
-    double sqrt_tol = std::sqrt(tol);
-    std::vector<QuadParams> quads{quad_params(p0, p1, p2, sqrt_tol)};
-    flatten_quads(quads, sqrt_tol, out);
-    out.push_back(p2);
+    flatten_quad(p0, p1, p2, tol, [&](Pt p) { out.push_back(p); });
 }
 
 void flatten_cubic_to(Pt p0, Pt p1, Pt p2, Pt p3, double tol, std::vector<Pt> &out) {
-    const double quad_tol = 0.1 * tol;
-    const double sqrt_tol = std::sqrt(0.9 * tol);
-    // Midpoint-quadratic error of a cubic is sqrt(3)/36 |p3 - 3p2 + 3p1 - p0| h^3 on a span h.
-    double d3x = p3.x - 3.0 * p2.x + 3.0 * p1.x - p0.x, d3y = p3.y - 3.0 * p2.y + 3.0 * p1.y - p0.y;
-    double err = std::sqrt(3.0) / 36.0 * std::hypot(d3x, d3y);
-    int nq = int(std::ceil(std::cbrt(err / quad_tol)));
-    nq = std::max(1, std::min(nq, 1 << 12));
-    std::vector<QuadParams> quads;
-    quads.reserve(size_t(nq));
-    auto eval = [&](double t) { return Pt{cubic_at(p0.x, p1.x, p2.x, p3.x, t), cubic_at(p0.y, p1.y, p2.y, p3.y, t)}; };
-    auto deriv = [&](double t, double a, double b, double c, double d) {
-        double mt = 1.0 - t;
-        return 3.0 * (mt * mt * (b - a) + 2.0 * mt * t * (c - b) + t * t * (d - c));
-    };
-    for (int i = 0; i < nq; ++i) {
-        double t0 = double(i) / nq, t1 = double(i + 1) / nq, h = t1 - t0;
-        Pt q0 = i == 0 ? p0 : eval(t0), q2 = i + 1 == nq ? p3 : eval(t1);
-        Pt c1{q0.x + deriv(t0, p0.x, p1.x, p2.x, p3.x) * h / 3.0, q0.y + deriv(t0, p0.y, p1.y, p2.y, p3.y) * h / 3.0};
-        Pt c2{q2.x - deriv(t1, p0.x, p1.x, p2.x, p3.x) * h / 3.0, q2.y - deriv(t1, p0.y, p1.y, p2.y, p3.y) * h / 3.0};
-        Pt qc{(3.0 * (c1.x + c2.x) - q0.x - q2.x) * 0.25, (3.0 * (c1.y + c2.y) - q0.y - q2.y) * 0.25};
-        quads.push_back(quad_params(q0, qc, q2, sqrt_tol));
-    }
-    flatten_quads(quads, sqrt_tol, out);
-    out.push_back(p3);
+    flatten_cubic(p0, p1, p2, p3, tol, [&](Pt p) { out.push_back(p); });
 }
 
 // Flattens every contour into point lists; closed marks an explicit ClosePath.

@@ -396,6 +396,8 @@ def encode(scene, verts, counts, colors=None):
     # The expected lines in recording order, as (p0|p1 rows, line -> vertex maps);
     # a path clip's lines map to no vertex (-1).
     pieces = []
+    # what each recorded path is, in path order: ("clip", clip index) or ("draws", lo, hi)
+    sources = []
     # runs of draws between clip boundaries go down in one call each
     bounds = sorted({0, len(counts)} | {int(c[-2]) for c in clips} | {int(c[-1]) for c in clips})
     draw_ids, o, open_stack = [], 0, []
@@ -414,6 +416,7 @@ def encode(scene, verts, counts, colors=None):
                 ca, cb, _ = edge_map(p32, [len(p32)])
                 none = np.full(len(ca), -1, np.int64)
                 pieces.append((np.concatenate([p32[ca], p32[cb]], 1), none, none, opens[o]))
+                sources.append(("clip", opens[o]))
             open_stack.append(opens[o])
             o += 1
         if hi > lo:
@@ -421,6 +424,7 @@ def encode(scene, verts, counts, colors=None):
                                                  rules[lo:hi]))
             a, b = la[line_of[lo]:line_of[hi]], lb[line_of[lo]:line_of[hi]]
             pieces.append((np.concatenate([v32[a], v32[b]], 1), a, b, None))
+            sources.append(("draws", lo, hi))
         while open_stack and clips[open_stack[-1]][-1] == hi:
             canvas.pop_layer()
             open_stack.pop()
@@ -437,6 +441,7 @@ def encode(scene, verts, counts, colors=None):
         clip_ids[i] = clip_draws[n]
     # a path clip the recorder took as an SDF clip (a pixel-aligned rectangle) has no lines
     pieces = [p for p in pieces if p[3] is None or records["sdf_offset"][clip_ids[p[3]]] == _abi.INVALID]
+    sources = [c for c in sources if c[0] != "clip" or records["sdf_offset"][clip_ids[c[1]]] == _abi.INVALID]
     lines = _abi.view(enc.lines, _abi.LINE_DTYPE, enc.line_count)
     got = np.concatenate([lines["p0"], lines["p1"]], 1) if len(lines) else np.zeros((0, 4), np.float32)
     want = np.concatenate([p[0] for p in pieces]) if pieces else np.zeros((0, 4), np.float32)
@@ -446,7 +451,8 @@ def encode(scene, verts, counts, colors=None):
         la = np.concatenate([p[1] for p in pieces]) if pieces else la
         lb = np.concatenate([p[2] for p in pieces]) if pieces else lb
     return Encoded(canvas=canvas, scene=enc, line_a=la, line_b=lb, per_draw=per_draw,
-                   draw_ids=np.asarray(draw_ids, np.int64), draw_total=int(enc.draw_count), clip_ids=clip_ids)
+                   draw_ids=np.asarray(draw_ids, np.int64), draw_total=int(enc.draw_count), clip_ids=clip_ids,
+                   sources=sources)
 
 
 def clear_color(scene):
@@ -517,6 +523,49 @@ class Plan:
             self._lb_t = torch.as_tensor(self.line_b, device=device)
         return self._la_t, self._lb_t
 
+    def update(self, verts, counts):
+        """New vertices for the same scenes (the same draws, clips and structure;
+        a polygon's vertex count may change): the GPU front end rebuilds the
+        frame's lines, path records, bounds, tile bins and plans from them
+        (tk_frame_set_polygons) instead of recording and encoding again, and the
+        line -> vertex maps come from the lines it kept."""
+        dev = torch.device("cuda")
+        pts, p2v, lens = [], [], []
+        v_off = 0
+        for sc, e, v, k in zip(self.scenes, self.encodings, verts, counts):
+            k = np.asarray(k, np.int64)
+            offs = np.concatenate([[0], np.cumsum(k)])
+            v32 = v.detach().to(dev, torch.float32)
+            for src in e.sources:
+                if src[0] == "clip":
+                    c = torch.as_tensor(np.asarray(sc.clips[src[1]][0], np.float32).reshape(-1, 2), device=dev)
+                    pts.append(c)
+                    p2v.append(torch.full((len(c),), -1, dtype=torch.int64, device=dev))
+                    lens.append(len(c))
+                else:
+                    lo, hi = src[1], src[2]
+                    pts.append(v32[offs[lo]:offs[hi]])
+                    p2v.append(torch.arange(int(offs[lo]), int(offs[hi]), device=dev) + v_off)
+                    lens.extend(int(n) for n in k[lo:hi])
+            v_off += len(v)
+        points = torch.cat(pts) if pts else torch.zeros((0, 2), dtype=torch.float32, device=dev)
+        p2v = torch.cat(p2v) if p2v else torch.zeros(0, dtype=torch.int64, device=dev)
+        lens_t = torch.as_tensor(np.asarray(lens, np.int64), device=dev)
+        starts = torch.zeros(len(lens) + 1, dtype=torch.int64, device=dev)
+        starts[1:] = torch.cumsum(lens_t, 0)
+        lv = self.frame.set_polygons(points, starts.to(torch.int32), line_vertex=True).long()
+        # the vertex after each one in its polygon
+        path = torch.repeat_interleave(torch.arange(len(lens), device=dev), lens_t)
+        nxt = torch.arange(len(points), device=dev) + 1
+        nxt = torch.where(nxt == starts[1:][path], starts[:-1][path], nxt)
+        la, lb = p2v[lv], p2v[nxt[lv]]
+        la[la < 0] = v_off  # path clip lines: the gradient sink past the vertices
+        lb[lb < 0] = v_off
+        self.vertex_count = v_off
+        self._la_t, self._lb_t = la, lb
+        self.line_a, self.line_b = la.cpu().numpy(), lb.cpu().numpy()
+        self.line_count = len(la)
+
     def full_paint(self, records):
         """One 32-float paint record per encoded draw (clip draws get zeros) from the
         painter-order path-draw records (DiffScene.paint_records); differentiable."""
@@ -579,6 +628,33 @@ class _Render(torch.autograd.Function):
         line_grad, color_grad = plan.backward(grad_flat)
         gv = plan.vertex_grad(line_grad, torch.empty(0, dtype=ctx.vdtype, device=ctx.vdevice))
         return gv, color_grad.to(ctx.cdtype).to(ctx.cdevice), None, None
+
+
+class Renderer:
+    """Renders the same scenes again and again as their parameters move (an
+    optimiser loop): the first call records and encodes them on the host, later
+    calls give the GPU front end only the new vertices (Plan.update).  Calling
+    returns the FP32 images, differentiable in the parameters."""
+
+    def __init__(self, context, scenes, device="cuda"):
+        self.context, self.device = context, torch.device(device)
+        self.single = isinstance(scenes, DiffScene)
+        self.scenes = [scenes] if self.single else list(scenes)
+        self.plan = None
+
+    def __call__(self):
+        structs = [s.structure() for s in self.scenes]
+        polys = [s.polygons(st) for s, st in zip(self.scenes, structs)]
+        verts = [v for v, _ in polys]
+        counts = [k for _, k in polys]
+        if self.plan is None:
+            self.plan = Plan(self.context, self.scenes, verts, counts)
+        else:
+            self.plan.update(verts, counts)
+        flat = _Render.apply(torch.cat(verts), self.plan.full_paint(torch.cat([s.paint_records() for s in self.scenes])),
+                             self.plan, self.device)
+        images = self.plan.images(flat)
+        return images[0] if self.single else images
 
 
 def render(context, scenes, device="cuda"):
