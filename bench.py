@@ -234,8 +234,17 @@ def diff_batch(ctx, count=256, repeats=3, gradients=0.1, clips=True):
     out = {"scenes": count, "size": "256x256", "paths": int(paths.sum()), "lines": int(batch.line_count),
            "clips_per_scene": int(bool(clips)), "gradient_paints": int(sum(len(s.gradients) for s in sc)),
            "radial_paints": int(sum(diff.gradient_parts(g)[3] == "radial" for s in sc for g in s.gradients.values()))}
-    for name, run in (("loop", lambda: [step(p, c, t) for p, c, t in zip(singles, colors, targets)]),
-                      ("batch", lambda: step(batch, color_b, target_b))):
+    def set_sync(plans, sync):
+        for p in plans:
+            p.sync = sync
+
+    # loop: each call waits for the device and reads the status flags;
+    # loop_async: the same plans with stream-ordered calls only (no host wait
+    # between scenes), the flags read once after the timed steps.
+    for name, run, sync in (("loop", lambda: [step(p, c, t) for p, c, t in zip(singles, colors, targets)], True),
+                            ("loop_async", lambda: [step(p, c, t) for p, c, t in zip(singles, colors, targets)], False),
+                            ("batch", lambda: step(batch, color_b, target_b), True)):
+        set_sync(singles, sync)
         run()  # warm-up: sizes the coverage buffers
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -243,9 +252,24 @@ def diff_batch(ctx, count=256, repeats=3, gradients=0.1, clips=True):
             run()
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) / repeats * 1e3
+        if not sync:
+            for p in singles:
+                p.check_status()
         out[f"{name}_ms"] = round(ms, 3)
         out[f"{name}_scenes_per_s"] = round(count / (ms * 1e-3), 1)
     out["batch_speedup"] = round(out["loop_ms"] / out["batch_ms"], 3)
+    out["batch_speedup_vs_async_loop"] = round(out["loop_async_ms"] / out["batch_ms"], 3)
+    # The batch again with the forward's coverages kept for the backward instead of
+    # recomputed there (tk_tuning.diff_keep_coverage).
+    batch.frame.set_tuning(diff_keep_coverage=1)
+    step(batch, color_b, target_b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        step(batch, color_b, target_b)
+    torch.cuda.synchronize()
+    out["batch_keep_coverage_ms"] = round((time.perf_counter() - t0) / repeats * 1e3, 3)
+    batch.frame.set_tuning(diff_keep_coverage=0)
     del singles, batch
     return out
 
@@ -256,11 +280,13 @@ def fit(ctx, count=256, steps=200, lr=0.05, loop_steps=3):
     are perturbed by N(0, 4 px) and colours (and gradient stops) by N(0, 0.05);
     `steps` Adam steps on control points, colours and stops (FP32, on the GPU)
     against the squared error.  The first step records and encodes the scenes on
-    the host; every later step gives the GPU front end only the moved vertices
-    (diff.Renderer -> tk_frame_set_polygons), then runs forward + backward for all
-    scenes at once.  Beside it, `loop_steps` steps of the per-scene loop (each
-    scene its own Renderer, one frame per scene).  Returns the loss curve and the
-    per-step times."""
+    the host; every later step evaluates the geometry and paint of all scenes at
+    once (diff.batch_polygons, diff.batch_paint_records), gives the GPU front end
+    only the moved vertices (diff.Renderer -> tk_frame_set_polygons), then runs
+    forward + backward for all scenes at once.  Beside it, `loop_steps` steps of
+    the per-scene loop (each scene its own Renderer, one frame per scene), its
+    native calls ordered on streams with no host wait per scene.  Returns the loss
+    curve and the per-step times."""
     import torch
     from tileink_amd import diff
     rng = np.random.default_rng(9)
@@ -312,11 +338,17 @@ def fit(ctx, count=256, steps=200, lr=0.05, loop_steps=3):
         return losses, step_ms
 
     start = [p.detach().clone() for p in params]
-    losses, step_ms = run([diff.Renderer(ctx, scenes_, "cuda")], steps)
+    batch = diff.Renderer(ctx, scenes_, "cuda", sync=False)
+    losses, step_ms = run([batch], steps)
+    batch.plan.check_status()
     with torch.no_grad():  # the loop starts from the same parameters
         for p, v in zip(params, start):
             p.copy_(v)
-    loop_losses, loop_ms = run([diff.Renderer(ctx, s, "cuda") for s in scenes_], loop_steps)
+    # the per-scene loop with stream-ordered calls (no host wait per scene)
+    loop = [diff.Renderer(ctx, s, "cuda", sync=False) for s in scenes_]
+    loop_losses, loop_ms = run(loop, loop_steps)
+    for r in loop:
+        r.plan.check_status()
     med = float(np.median(step_ms[1:])) if len(step_ms) > 1 else step_ms[0]
     loop_med = float(np.median(loop_ms[1:])) if len(loop_ms) > 1 else loop_ms[0]
     return {"scenes": count, "size": "256x256", "paths": int(paths.sum()), "clips_per_scene": 1,

@@ -364,6 +364,10 @@ tk_status tk_frame_render_tiles(tk_frame *frame, const uint32_t *tiles, uint32_t
  *  - coarse_block: tiles (one 64-lane wave each) per coarse workgroup: 1, 2, 4.
  *  - coarse_scan_items: counts per workgroup of the coarse exclusive scan
  *    (decoupled lookback): 1024, 2048, 4096, 8192.
+ *  - diff_keep_coverage: 1: the differentiable forward keeps every (record,
+ *    pixel) coverage for the backward (4 bytes per pixel per PTCL record
+ *    written, then read); 0 (the default): the backward recomputes the
+ *    coverages from the tile segments, exactly as the forward computed them.
  * The tile is 16x16 pixels: the record formats the reference's GPU ABI fixes
  * (tile-local segment coordinates in [0, 16], 256-tile scan chunks, the PTCL)
  * and the bit-exact contract with its results depend on it, so it is not a
@@ -378,7 +382,7 @@ typedef struct tk_tuning {
     uint32_t scan_xcd_run, coarse_xcd_run, fine_xcd_run;
     uint32_t persistent;
     uint32_t scan_block, coarse_block, coarse_scan_items;
-    uint32_t reserved;
+    uint32_t diff_keep_coverage;
 } tk_tuning;
 /* GPU front end for polygon scenes.  The reference records every path on the
  * CPU (Canvas::push_path_inner_with_tag, src/canvas/paths.rs:65-156, with the
@@ -404,6 +408,16 @@ typedef struct tk_tuning {
  * renders.  Any captured graph is dropped. */
 tk_status tk_frame_set_polygons(tk_frame *frame, const float *points, const uint32_t *starts, uint64_t point_count,
                                 uint32_t *line_vertex);
+/* The same for scenes whose paths are each one closed contour of cubics (a move,
+ * cubics, a close: tk_canvas_push_path with those verbs), flattened on the
+ * device: PathFlatten::flatten (src/shared/path_flatten.rs:20-56) through the
+ * recorder's own flattener (flatten.h, the same FP64 arithmetic on both sides),
+ * with the curves' exact bounds as the paths' bounds.  ctrl: device, FP64 (x, y)
+ * pairs in physical pixels; path p's are ctrl[starts[p] .. starts[p+1]): the
+ * start point, then the two control points and the end point of each cubic.
+ * tolerance: the flattening tolerance in physical pixels (the recorder's
+ * tolerance times the canvas scale).  Two host synchronisations. */
+tk_status tk_frame_set_cubic_paths(tk_frame *frame, const double *ctrl, const uint32_t *starts, double tolerance);
 
 tk_status tk_frame_get_tuning(tk_frame *frame, tk_tuning *out);
 tk_status tk_frame_set_tuning(tk_frame *frame, const tk_tuning *tuning);

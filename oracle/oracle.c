@@ -740,12 +740,12 @@ static float fill_value_sequential(const uint32_t *segs, int32_t backdrop, uint3
     return base + running + partial;
 }
 
-/* The same three sums, each exact: every FP32 term (in (-2, 2) by construction)
- * is truncated to a multiple of 2^-30 (trunc(t * 2^30) as a saturating int32
- * conversion, NaN -> 0) and summed as a 64-bit integer, then the sums are
- * converted back (floored to 2^-24, rounded once) and combined as
- * (base + running) + partial in FP32, as fill_alpha_at combines them.  No
- * segment order enters the result. */
+/* The same three sums, exact: every FP32 term (in (-2, 2) by construction) is
+ * truncated to a multiple of 2^-30 (trunc(t * 2^30) as a saturating int32
+ * conversion, NaN -> 0) and summed as a 64-bit integer; the three sums are
+ * added exactly and the total converted back once (floored to 2^-24, rounded
+ * once), where fill_alpha_at rounds each sum and then (base + running) +
+ * partial.  No segment order enters the result. */
 static int64_t coverage_fixed(float t) { return (int64_t)f2i(t * 1073741824.0f); }
 /* Back to FP32: the sum floored to a multiple of 2^-24, then one rounding
  * (floor(v / 2^30) and the top 24 fraction bits convert exactly, the 2^-24
@@ -770,7 +770,7 @@ static float fill_value_at(const uint32_t *segs, int32_t backdrop, uint32_t star
             if ((int)x >= ps && (int)x < pe) partial += coverage_fixed(segment_area_at(p.xmin, p.xmax, x) * p.dy);
         }
     }
-    return (fixed_to_float(base) + fixed_to_float(running)) + fixed_to_float(partial);
+    return fixed_to_float(base + running + partial);
 }
 
 /* ---------- SDF + brush (shared/sdf/ and shared/brush/ headers) -------------------- */

@@ -94,9 +94,10 @@ def paint_colors(sc):
     return out
 
 
-def analytic_grads(sc, w):
+def analytic_grads(sc, w, recorded=False):
     """dL/d(parameters) for L = sum(w * image) from the FP64 oracle's line and colour
-    gradients, carried through the geometry by torch."""
+    gradients, carried through the geometry by torch; recorded=True evaluates them at
+    the vertices rounded to FP32, where the device renders."""
     st = sc.structure()
     params = [p for p in (sc.fills, sc.strokes, sc.widths, sc.colors) if p is not None]
     params += [g[2] for g in sc.gradients.values()]
@@ -105,7 +106,7 @@ def analytic_grads(sc, w):
         p.requires_grad_(True)
     v, k = sc.polygons(st)
     pm = sc.premultiplied()
-    draws, lines, la, lb = diff.oracle_inputs(sc, v, k)
+    draws, lines, la, lb = diff.oracle_inputs(sc, v, k, recorded=recorded)
     lg, maps = oracle.diff_backward(sc.width, sc.height, diff.clear_color(sc), draws, paint_colors(sc), lines, w,
                                     masks=clip_masks(sc, v, k), color_maps=True)
     gv = torch.zeros_like(v)

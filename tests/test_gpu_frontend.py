@@ -193,3 +193,105 @@ def test_set_polygons_refuses_non_finite_vertices(ctx):
         frame.set_polygons(pts, starts)
     assert err.value.status == _abi.TK_ERR_INVALID
     check_move(ctx, sc, frame, sc.polygons)  # valid vertices restore the frame
+
+
+class CubicScene:
+    """Closed contours of 4-16 cubics (the clipped-paths generator's fills), with
+    opaque SDF rects, an SDF clip and a path clip made of cubics."""
+
+    def __init__(self, seed, width, height, count=80):
+        rng = np.random.default_rng(seed)
+        self.width, self.height = width, height
+        self.contours, self.ops = [], []
+        for i in range(count):
+            if i == count // 3:
+                self.ops.append(("clip_sdf", (5.5, 3.25, width - 9.0, height - 4.5), 11.0))
+            if i == count // 2:
+                self.ops.append(("clip_path", self._contour(rng, width / 2, height / 2, 0.4 * min(width, height), 6),
+                                 int(rng.integers(0, 2))))
+            c = self._contour(rng, rng.uniform(0, width), rng.uniform(0, height),
+                              float(rng.lognormal(np.log(48.0), 1.0)), int(rng.integers(4, 17)))
+            color = tuple(rng.uniform(0, 1, 3)) + (float(rng.uniform(0.3, 1)),)
+            self.ops.append(("path", c, color, int(rng.integers(0, 2))))
+            if i % 20 == 7:
+                x0, y0 = float(np.floor(rng.uniform(0, width * 0.7))), float(np.floor(rng.uniform(0, height * 0.7)))
+                self.ops.append(("rect", (x0, y0, x0 + 80.0, y0 + 64.0), 0.0, tuple(rng.uniform(0, 1, 3)) + (1.0,)))
+        self.ops.extend([("pop",), ("pop",)])
+
+    def _contour(self, rng, cx, cy, extent, n):
+        pts = rng.uniform(-extent / 2, extent / 2, size=(3 * n + 1, 2)) + (cx, cy)
+        pts[-1] = pts[0]  # the last cubic returns to the start
+        k = len(self.contours)
+        self.contours.append(pts)
+        return k
+
+    def record(self, contours):
+        canvas = tk.Canvas(self.width, self.height)
+
+        def path(pts):
+            p = tk.BezPath().move_to(*pts[0])
+            for i in range(1, len(pts), 3):
+                p.curve_to(*pts[i], *pts[i + 1], *pts[i + 2])
+            return p.close_path()
+
+        for op in self.ops:
+            if op[0] == "path":
+                canvas.push_path(path(contours[op[1]]), tk.Color(*op[2]), rule=tk.FillRule(op[3]), tolerance=0.25)
+            elif op[0] == "rect":
+                canvas.push_rect(*op[1], op[2], tk.Color(*op[3]))
+            elif op[0] == "clip_sdf":
+                canvas.push_clip_sdf_rect_layer(*op[1], op[2])
+            elif op[0] == "clip_path":
+                canvas.push_clip_layer(path(contours[op[1]]), rule=tk.FillRule(op[2]), tolerance=0.25)
+            else:
+                canvas.pop_layer()
+        return canvas
+
+    def ctrl(self, contours):
+        order = [op[1] for op in self.ops if op[0] in ("path", "clip_path")]
+        pts = np.concatenate([contours[k] for k in order])
+        starts = np.concatenate([[0], np.cumsum([len(contours[k]) for k in order])]).astype(np.uint32)
+        return pts, starts
+
+    def moved(self, seed, sigma=5.0, scale=1.0):
+        rng = np.random.default_rng(seed)
+        out = []
+        for c in self.contours:
+            m = c.mean(0)
+            q = m + (c - m) * scale + rng.normal(0, sigma, c.shape)
+            q[-1] = q[0]
+            out.append(q)
+        return out
+
+
+@pytest.mark.parametrize("seed", [7, 8])
+def test_set_cubic_paths_flattens_on_the_device_like_the_recorder(ctx, seed):
+    # GPU flattening (flatten.h on the device) + the front end, against the host
+    # recorder's flattening and encoding of the same control points: every scene
+    # array word for word, then the images.
+    sc = CubicScene(seed, 640, 480)
+    frame = tk.Frame(ctx, sc.record(sc.contours).encode(0))
+    for contours in (sc.contours, sc.moved(seed), sc.moved(seed + 1, sigma=2.0, scale=2.0)):
+        ctrl, starts = sc.ctrl(contours)
+        frame.set_cubic_paths(ctrl, starts, tolerance=0.25)
+        ref = tk.Frame(ctx, sc.record(contours).encode(0))
+        assert_same_scene(scene_arrays(frame), scene_arrays(ref))
+        frame.render()
+        ref.render()
+        assert frame.stats()["overflow"] == 0
+        assert np.array_equal(frame.image(), ref.image())
+
+
+def test_set_cubic_paths_refuses_malformed_contours(ctx):
+    sc = CubicScene(3, 200, 160, count=10)
+    frame = tk.Frame(ctx, sc.record(sc.contours).encode(0))
+    ctrl, starts = sc.ctrl(sc.contours)
+    bad = starts.copy()
+    bad[1] += 1  # path 0 would hold 3k + 2 points
+    with pytest.raises(tk.TileinkError) as err:
+        frame.set_cubic_paths(ctrl, bad)
+    assert err.value.status == _abi.TK_ERR_INVALID
+    ctrl[5, 1] = np.inf
+    with pytest.raises(tk.TileinkError) as err:
+        frame.set_cubic_paths(ctrl, starts)
+    assert err.value.status == _abi.TK_ERR_INVALID

@@ -135,7 +135,8 @@ def test_invalid_run_lengths_are_rejected(ctx):
             frame.set_tuning(*bad)
         assert err.value.status == _abi.TK_ERR_INVALID
     for bad in (dict(persistent=4), dict(scan_block=32), dict(scan_block=384), dict(coarse_block=3),
-                dict(coarse_block=8), dict(coarse_scan_items=512), dict(coarse_scan_items=3000)):
+                dict(coarse_block=8), dict(coarse_scan_items=512), dict(coarse_scan_items=3000),
+                dict(diff_keep_coverage=2)):
         with pytest.raises(tk.TileinkError) as err:
             frame.set_tuning(**bad)
         assert err.value.status == _abi.TK_ERR_INVALID, bad

@@ -390,6 +390,11 @@ class NativeContext:
     def synchronize(self):
         check(lib().tk_context_synchronize(self._h))
 
+    def stream_handle(self):
+        """The context's HIP stream (hipStream_t as an integer): every call on this
+        context is ordered on it."""
+        return int(lib().tk_context_stream(self._h) or 0)
+
     def to_device(self, array):
         """Copies a numpy array into a new device buffer; returns its DeviceBuffer."""
         array = np.ascontiguousarray(array)
@@ -507,6 +512,29 @@ class Frame:
         lines = int(self.stats()["lines"])
         return out[:lines] if torch is not None else out.read(np.uint32, lines)
 
+    def set_cubic_paths(self, ctrl, starts, tolerance=0.25):
+        """GPU flattening + front end (tk_frame_set_cubic_paths) for scenes whose
+        paths are each one closed contour of cubics: ctrl (N, 2) float64 control
+        points (start point, then three per cubic, path after path) and starts
+        (path_count + 1,), numpy arrays (uploaded) or torch tensors on the GPU."""
+        keep = []
+
+        def device(arr, dtype):
+            if hasattr(arr, "data_ptr"):
+                t = arr.contiguous()
+                keep.append(t)
+                return t.data_ptr(), t.numel()
+            a = np.ascontiguousarray(arr, dtype=dtype).reshape(-1)
+            buf = self.context.to_device(a if a.size else np.zeros(1, dtype))
+            keep.append(buf)
+            return buf.ptr, a.size
+
+        cp, n = device(ctrl, np.float64)
+        sp, ns = device(starts, np.uint32)
+        if n % 2 or ns != self._path_count() + 1:
+            raise TileinkError(_abi.TK_ERR_INVALID, "ctrl holds (x, y) pairs and starts has path_count + 1 entries")
+        check(lib().tk_frame_set_cubic_paths(self._h, C.c_void_p(cp), C.c_void_p(sp), float(tolerance)))
+
     def _path_count(self):
         return int(self.stats()["paths"])
 
@@ -530,7 +558,7 @@ class Frame:
         unknown = set(given) - set(_abi.TUNING_FIELDS)
         if unknown:
             raise TileinkError(_abi.TK_ERR_INVALID, f"unknown tuning fields {sorted(unknown)}")
-        t = _abi.Tuning(*(cur[k] if given.get(k) is None else int(given[k]) for k in _abi.TUNING_FIELDS), 0)
+        t = _abi.Tuning(*(cur[k] if given.get(k) is None else int(given[k]) for k in _abi.TUNING_FIELDS))
         check(lib().tk_frame_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, repeats=3):

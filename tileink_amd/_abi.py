@@ -133,12 +133,12 @@ class StrokeStyle(C.Structure):
 
 
 TUNING_FIELDS = ("scan_xcd_run", "coarse_xcd_run", "fine_xcd_run", "persistent", "scan_block", "coarse_block",
-                 "coarse_scan_items")
+                 "coarse_scan_items", "diff_keep_coverage")
 TK_PERSISTENT_COARSE, TK_PERSISTENT_FINE = 1, 2
 
 
 class Tuning(C.Structure):
-    _fields_ = [(n, C.c_uint32) for n in TUNING_FIELDS + ("reserved",)]
+    _fields_ = [(n, C.c_uint32) for n in TUNING_FIELDS]
 
 
 class CoarseConfig(C.Structure):
@@ -216,6 +216,7 @@ SIGNATURES = {
     "tk_canvas_push_stroke_styled": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_size_t, C.POINTER(StrokeStyle), _P, _P,
                                                C.c_uint32, C.c_double, C.POINTER(C.c_uint32)]),
     "tk_frame_set_polygons": (C.c_int, [_P, _P, _P, C.c_uint64, _P]),
+    "tk_frame_set_cubic_paths": (C.c_int, [_P, _P, _P, C.c_double]),
     "tk_frame_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "tk_frame_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "tk_frame_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(Tuning)]),

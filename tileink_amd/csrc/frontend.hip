@@ -20,6 +20,7 @@
 // the same FP64 bounds, the same FP32 tile counts (scan_line.h) and the same
 // integer rules.
 
+#include "flatten.h"
 #include "scan_line.h"
 
 namespace {
@@ -56,16 +57,30 @@ struct FeEdge {
     float ax, ay, bx, by;
     bool keep;
 };
-__device__ __forceinline__ FeEdge fe_edge(const float *pts, uint32_t v0, uint32_t n, uint32_t i) {
+// The vertex coordinates in FP64 (exact for FP32 input); the line stores them
+// rounded to FP32, as push_line does.
+__device__ __forceinline__ FeEdge fe_edge(const tk_args_frontend &a, uint32_t v0, uint32_t n, uint32_t i) {
     FeEdge e{0.0f, 0.0f, 0.0f, 0.0f, false};
     if (i >= n) return e;
     const uint32_t j = i + 1u == n ? 0u : i + 1u;
-    e.ax = pts[2u * (v0 + i)];
-    e.ay = pts[2u * (v0 + i) + 1u];
-    e.bx = pts[2u * (v0 + j)];
-    e.by = pts[2u * (v0 + j) + 1u];
-    const double dx = (double)e.ax - (double)e.bx, dy = (double)e.ay - (double)e.by;
+    double ax, ay, bx, by;
+    if (a.points64) {
+        ax = a.points64[2u * (v0 + i)];
+        ay = a.points64[2u * (v0 + i) + 1u];
+        bx = a.points64[2u * (v0 + j)];
+        by = a.points64[2u * (v0 + j) + 1u];
+    } else {
+        ax = a.points[2u * (v0 + i)];
+        ay = a.points[2u * (v0 + i) + 1u];
+        bx = a.points[2u * (v0 + j)];
+        by = a.points[2u * (v0 + j) + 1u];
+    }
+    const double dx = ax - bx, dy = ay - by;
     e.keep = dx * dx + dy * dy > 1.0e-12 * 1.0e-12;
+    e.ax = (float)ax;
+    e.ay = (float)ay;
+    e.bx = (float)bx;
+    e.by = (float)by;
     return e;
 }
 
@@ -83,12 +98,13 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fe_paths(tk_args_frontend
     double x0 = __builtin_inf(), y0 = __builtin_inf(), x1 = -__builtin_inf(), y1 = -__builtin_inf();
     bool finite = true;
     for (uint32_t i = lane; i < n; i += 64u) {
-        const float x = a.points[2u * (v0 + i)], y = a.points[2u * (v0 + i) + 1u];
+        const double x = a.points64 ? a.points64[2u * (v0 + i)] : (double)a.points[2u * (v0 + i)];
+        const double y = a.points64 ? a.points64[2u * (v0 + i) + 1u] : (double)a.points[2u * (v0 + i) + 1u];
         finite = finite && isfinite(x) && isfinite(y);
-        x0 = fmin(x0, (double)x);
-        y0 = fmin(y0, (double)y);
-        x1 = fmax(x1, (double)x);
-        y1 = fmax(y1, (double)y);
+        x0 = fmin(x0, x);
+        y0 = fmin(y0, y);
+        x1 = fmax(x1, x);
+        y1 = fmax(y1, y);
     }
     if (__ballot(!finite)) {
         if (lane == 0u) atomicOr(a.status, 1u);
@@ -98,9 +114,16 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fe_paths(tk_args_frontend
     y0 = fe_wave_min(y0);
     x1 = fe_wave_max(x1);
     y1 = fe_wave_max(y1);
-    // path_bounds of a polygon: every vertex once the path has an edge (a line verb).
+    // path_bounds of a polygon: every vertex once the path has an edge (a line
+    // verb); of flattened curves: the curves' exact bounds, given.
+    if (a.bounds64 && n) {
+        x0 = a.bounds64[4u * p];
+        y0 = a.bounds64[4u * p + 1u];
+        x1 = a.bounds64[4u * p + 2u];
+        y1 = a.bounds64[4u * p + 3u];
+    }
     int32_t pb[4] = {0, 0, 0, 0};
-    if (n >= 2u) {
+    if (a.bounds64 ? (n && x0 <= x1) : n >= 2u) {
         pb[0] = fe_sat_i32(floor(x0));
         pb[1] = fe_sat_i32(floor(y0));
         pb[2] = fe_sat_i32(ceil(x1));
@@ -111,7 +134,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fe_paths(tk_args_frontend
     uint32_t lines = 0u;
     unsigned long long capacity = 0ull;
     for (uint32_t base = 0; base < n; base += 64u) {
-        const FeEdge e = fe_edge(a.points, v0, n, base + lane);
+        const FeEdge e = fe_edge(a, v0, n, base + lane);
         lines += (uint32_t)__popcll(__ballot(e.keep));
         if (e.keep) capacity += tk::scanned_tile_count(e.ax, e.ay, e.bx, e.by, tb[0], tb[1], tb[2], tb[3]);
     }
@@ -346,7 +369,7 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fe_path_fill(tk_args_fron
     const uint32_t n = n_lines ? (v1 > v0 ? v1 - v0 : 0u) : 0u;
     uint32_t written = 0u;
     for (uint32_t base = 0; base < n; base += 64u) {
-        const FeEdge e = fe_edge(a.points, v0, n, base + lane);
+        const FeEdge e = fe_edge(a, v0, n, base + lane);
         const uint64_t keep = __ballot(e.keep);
         const uint64_t slot = (uint64_t)line_start + written + (uint32_t)__popcll(keep & lanes_below(lane));
         if (e.keep && slot < a.line_capacity) {
@@ -445,5 +468,91 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_fe_tiles(tk_args_frontend
                 ++dst;
             }
         }
+    }
+}
+
+// GPU flattening: one thread per path, whose closed contour of cubics goes
+// through the shared flattener (flatten.h) exactly as push_path records it: the
+// start point, then each cubic's points after its start; the path's bounds are
+// the cubics' exact bounds (add_cubic_bounds).  Pass 1 counts and bounds,
+// pass 2 (after a scan of the counts) writes the points.
+__device__ __forceinline__ bool fl_contour(const tk_args_flatten &a, uint32_t p, uint32_t &first, uint32_t &cubics) {
+    first = a.starts[p];
+    const uint32_t end = a.starts[p + 1u];
+    const uint32_t n = end > first ? end - first : 0u;
+    cubics = n >= 4u ? (n - 1u) / 3u : 0u;
+    return n >= 4u && (n - 1u) % 3u == 0u;
+}
+
+extern "C" __global__ __launch_bounds__(256) void tk_k_flatten_count(tk_args_flatten a) {
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= a.path_count) return;
+    uint32_t first, cubics;
+    uint32_t count = 0u;
+    double x0 = __builtin_inf(), y0 = __builtin_inf(), x1 = -__builtin_inf(), y1 = -__builtin_inf();
+    if (!fl_contour(a, p, first, cubics)) {
+        atomicOr(a.status, 2u);
+    } else {
+        const double *c = a.ctrl + 2u * first;
+        bool finite = true;
+        for (uint32_t i = 0; i < 2u * (3u * cubics + 1u); ++i) finite = finite && isfinite(c[i]);
+        if (!finite) {
+            atomicOr(a.status, 1u);
+        } else {
+            count = 1u;
+            auto add = [&](tk::FPt q) {
+                x0 = fmin(x0, q.x);
+                y0 = fmin(y0, q.y);
+                x1 = fmax(x1, q.x);
+                y1 = fmax(y1, q.y);
+            };
+            for (uint32_t k = 0; k < cubics; ++k) {
+                const tk::FPt p0{c[6u * k], c[6u * k + 1u]}, p1{c[6u * k + 2u], c[6u * k + 3u]},
+                    p2{c[6u * k + 4u], c[6u * k + 5u]}, p3{c[6u * k + 6u], c[6u * k + 7u]};
+                tk::flatten_cubic(p0, p1, p2, p3, a.tolerance, [&](tk::FPt) { ++count; });
+                add(p0);
+                add(p3);
+                double ts[2];
+                int nt;
+                tk::cubic_axis_extrema(p0.x, p1.x, p2.x, p3.x, ts, &nt);
+                for (int i = 0; i < nt; ++i)
+                    add(tk::FPt{tk::cubic_coord(p0.x, p1.x, p2.x, p3.x, ts[i]),
+                                tk::cubic_coord(p0.y, p1.y, p2.y, p3.y, ts[i])});
+                tk::cubic_axis_extrema(p0.y, p1.y, p2.y, p3.y, ts, &nt);
+                for (int i = 0; i < nt; ++i)
+                    add(tk::FPt{tk::cubic_coord(p0.x, p1.x, p2.x, p3.x, ts[i]),
+                                tk::cubic_coord(p0.y, p1.y, p2.y, p3.y, ts[i])});
+            }
+        }
+    }
+    a.counts[p] = count;
+    a.bounds[4u * p] = x0;
+    a.bounds[4u * p + 1u] = y0;
+    a.bounds[4u * p + 2u] = x1;
+    a.bounds[4u * p + 3u] = y1;
+}
+
+extern "C" __global__ __launch_bounds__(256) void tk_k_flatten_emit(tk_args_flatten a) {
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= a.path_count) return;
+    const uint32_t at = a.counts[p];
+    const uint32_t next = p + 1u < a.path_count ? a.counts[p + 1u] : 0xFFFFFFFFu;
+    if (next == at) return;  // a path without points (malformed or non-finite)
+    uint32_t first, cubics;
+    if (!fl_contour(a, p, first, cubics)) return;
+    const double *c = a.ctrl + 2u * first;
+    uint64_t w = at;
+    auto put = [&](tk::FPt q) {
+        if (w < a.point_capacity) {
+            a.points[2u * w] = q.x;
+            a.points[2u * w + 1u] = q.y;
+        }
+        ++w;
+    };
+    put(tk::FPt{c[0], c[1]});
+    for (uint32_t k = 0; k < cubics; ++k) {
+        const tk::FPt p0{c[6u * k], c[6u * k + 1u]}, p1{c[6u * k + 2u], c[6u * k + 3u]},
+            p2{c[6u * k + 4u], c[6u * k + 5u]}, p3{c[6u * k + 6u], c[6u * k + 7u]};
+        tk::flatten_cubic(p0, p1, p2, p3, a.tolerance, put);
     }
 }

@@ -579,7 +579,8 @@ struct CoverageAcc {
 __device__ __forceinline__ void accumulate_record(const tk_line_segment *segs, uint32_t n, CoverageAcc &acc,
                                                   uint32_t lane) {
     const uint32_t row = lane & 15u;
-    for (uint64_t i = lane >> 4; i < n; i += 16u) {
+    // A 32-bit index (n is 32-bit): no 64-bit counter compare and step per pass.
+    for (uint32_t i = lane >> 4; i < n; i += 16u) {
         const tk_line_segment seg = segs[i];
         const RowParts p = segment_row_parts(seg.p0x, seg.p0y, seg.p1x, seg.p1y, seg.y_edge, row);
         if (p.edge != 0.0f) atomicAdd((unsigned long long *)&acc.base[row], (unsigned long long)coverage_fixed(p.edge));
@@ -587,9 +588,10 @@ __device__ __forceinline__ void accumulate_record(const tk_line_segment *segs, u
             const int fs = clamp_cell16(ceilf(p.xmax));
             const int ps = clamp_cell16(floorf(p.xmin));
             if (fs < 16) atomicAdd((unsigned long long *)&acc.run[row * 16u + fs], (unsigned long long)coverage_fixed(p.dy));
-            for (int x = ps; x < fs; ++x)
+            float xf = (float)ps;  // exact: 0..16
+            for (int x = ps; x < fs; ++x, xf += 1.0f)
                 atomicAdd((unsigned long long *)&acc.part[row * 16u + x],
-                          (unsigned long long)coverage_fixed(segment_area_at(p.xmin, p.xmax, (uint32_t)x) * p.dy));
+                          (unsigned long long)coverage_fixed(segment_area_at_f(p.xmin, p.xmax, xf) * p.dy));
         }
     }
 }
@@ -625,7 +627,7 @@ __device__ __forceinline__ float take_coverage(CoverageAcc &acc, int32_t backdro
     acc.run[lane] = 0;
     acc.part[lane] = 0;
     if ((lane & 15u) == 0u) acc.base[row] = 0;
-    return (fixed_to_float(base) + fixed_to_float(running)) + fixed_to_float(partial);
+    return fixed_to_float(base + running + partial);
 }
 
 }  // namespace
@@ -2122,7 +2124,9 @@ __device__ __forceinline__ void for_each_fill(const tk_args_fine_diff &a, const 
     }
 }
 
-template <bool kScenes, bool kClips>
+// kKeep: keep each record's coverage (a path clip's mask) in its slot of `adj`
+// for the backward; otherwise the backward recomputes it from the segments.
+template <bool kScenes, bool kClips, bool kKeep>
 __device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &a) {
     __shared__ CoverageAcc cov_acc[2];
     DiffTile t;
@@ -2153,8 +2157,10 @@ __device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &
         __syncthreads();
         float slope;
         const float m = diff_alpha(take_coverage(acc, rec.backdrop, lane), rec.fill_rule, slope);
-        if (pi < a.adj_records) a.adj[(size_t)pi * 256u + lane] = m;
-        else overflow = true;
+        if (kKeep) {
+            if (pi < a.adj_records) a.adj[(size_t)pi * 256u + lane] = m;
+            else overflow = true;
+        }
         return m;
     };
     for_each_fill<kClips>(a, t, [&](uint32_t pi, const tk_ptcl_record &rec, float mask) {
@@ -2175,8 +2181,10 @@ __device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &
         px.g = px.g * f + alpha * c.y;
         px.b = px.b * f + alpha * c.z;
         px.a = px.a * f + alpha * c.w;
-        if (pi < a.adj_records) a.adj[(size_t)pi * 256u + lane] = cov;
-        else overflow = true;
+        if (kKeep) {
+            if (pi < a.adj_records) a.adj[(size_t)pi * 256u + lane] = cov;
+            else overflow = true;
+        }
         if (lane == 0u && box.w != 0u) {  // where the line pass finds this path's record in this tile
             const uint32_t x0 = box.y & 0xFFFFu, y0 = box.y >> 16, x1 = box.z & 0xFFFFu, y1 = box.z >> 16;
             if (tx >= x0 && ty >= y0 && tx < x1 && ty < y1) {
@@ -2193,8 +2201,12 @@ __device__ __forceinline__ void fine_diff_forward_tile(const tk_args_fine_diff &
             make_float4(px.r, px.g, px.b, px.a);
 }
 
-template <bool kScenes, bool kClips>
+// kKeep: the coverages come from the forward's slots of `adj`; otherwise they
+// are recomputed here from the tile's segments, exactly as the forward computed
+// them (the exact sums do not depend on the segment order).
+template <bool kScenes, bool kClips, bool kKeep>
 __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff &a) {
+    __shared__ CoverageAcc cov_acc[kKeep ? 1 : 2];
     DiffTile t;
     if (!diff_tile<kScenes>(a, t)) return;
     const uint32_t lane = t.lane;
@@ -2246,6 +2258,34 @@ __device__ __forceinline__ void fine_diff_backward_tile(const tk_args_fine_diff 
         res.b = res.b * f + alpha * c.z;
         res.a = res.a * f + alpha * c.w;
     };
+    if constexpr (!kKeep) {
+        cov_acc[0].run[lane] = 0;
+        cov_acc[0].part[lane] = 0;
+        cov_acc[1].run[lane] = 0;
+        cov_acc[1].part[lane] = 0;
+        if (lane < 16u) {
+            cov_acc[0].base[lane] = 0;
+            cov_acc[1].base[lane] = 0;
+        }
+        uint32_t sel = 0u;
+        __syncthreads();
+        auto cover = [&](const tk_ptcl_record &rec) {
+            const uint32_t s0 = rec.segment_start, s1 = min(rec.segment_end, a.cfg.segment_capacity);
+            CoverageAcc &acc = cov_acc[sel];
+            sel ^= 1u;
+            accumulate_record(a.in.segments + s0, s1 > s0 ? s1 - s0 : 0u, acc, lane);
+            __syncthreads();
+            return take_coverage(acc, rec.backdrop, lane);
+        };
+        auto path_clip = [&](uint32_t, const tk_ptcl_record &rec) {
+            float slope;
+            return diff_alpha(cover(rec), rec.fill_rule, slope);
+        };
+        for_each_fill<kClips>(a, t, [&](uint32_t pi, const tk_ptcl_record &rec, float mask) {
+            record(pi, rec, cover(rec), mask);
+        }, path_clip);
+        return;
+    }
     // The walk of for_each_fill, software-pipelined: the next record and its
     // coverage are loaded while the current record is processed.
     const uint32_t draws = a.cfg.draw_count;
@@ -2295,12 +2335,20 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_draw_tiles(tk_args_draw_t
 // One canvas or a batch of scenes (_scenes), without or with SDF clip layers
 // (_clips: the clip evaluation raises the register count, so scenes without
 // layers run the build that has none).
+// _keep: the forward keeps every coverage for the backward; otherwise the
+// backward recomputes it.
 #define TK_DIFF_KERNELS(suffix, scenes, clips)                                                                  \
     extern "C" __global__ __launch_bounds__(256) void tk_k_fine_diff##suffix(tk_args_fine_diff a) {             \
-        fine_diff_forward_tile<scenes, clips>(a);                                                                \
+        fine_diff_forward_tile<scenes, clips, false>(a);                                                         \
     }                                                                                                           \
     extern "C" __global__ __launch_bounds__(256) void tk_k_fine_backward##suffix(tk_args_fine_diff a) {         \
-        fine_diff_backward_tile<scenes, clips>(a);                                                               \
+        fine_diff_backward_tile<scenes, clips, false>(a);                                                        \
+    }                                                                                                           \
+    extern "C" __global__ __launch_bounds__(256) void tk_k_fine_diff##suffix##_keep(tk_args_fine_diff a) {      \
+        fine_diff_forward_tile<scenes, clips, true>(a);                                                          \
+    }                                                                                                           \
+    extern "C" __global__ __launch_bounds__(256) void tk_k_fine_backward##suffix##_keep(tk_args_fine_diff a) {  \
+        fine_diff_backward_tile<scenes, clips, true>(a);                                                         \
     }
 TK_DIFF_KERNELS(, false, false)
 TK_DIFF_KERNELS(_scenes, true, false)

@@ -281,7 +281,9 @@ struct Px {
     float r, g, b, a;
 };
 
-TKD uint32_t coverage_to_u8(float c) { return f2u(clampf(c, 0.0f, 1.0f) * 255.0f + 0.5f); }
+// The clamps keep the value in [0.5, 255.5] (a NaN clamps to 0), where a plain
+// truncating conversion equals f2u's, without f2u's range branches.
+TKD uint32_t coverage_to_u8(float c) { return (uint32_t)(clampf(c, 0.0f, 1.0f) * 255.0f + 0.5f); }
 TKD uint32_t coverage_to_alpha(float v, uint32_t rule) {
     float alpha = fminf(fabsf(v), 1.0f);
     if (rule == 1u) alpha = fabsf(v - 2.0f * rintf(0.5f * v));
@@ -289,7 +291,7 @@ TKD uint32_t coverage_to_alpha(float v, uint32_t rule) {
 }
 TKD float signum_f32(float v) { return v < 0.0f ? -1.0f : 1.0f; }
 TKD uint32_t rgba8_pack(uint32_t r, uint32_t g, uint32_t b, uint32_t a) { return r | (g << 8) | (b << 16) | (a << 24); }
-TKD uint32_t chan_u8(float v) { return f2u(clampf(v * 255.0f + 0.5f, 0.0f, 255.0f)); }
+TKD uint32_t chan_u8(float v) { return (uint32_t)clampf(v * 255.0f + 0.5f, 0.0f, 255.0f); }  // in [0, 255], as above
 TKD uint32_t unorm_to_rgba8(Px p) { return rgba8_pack(chan_u8(p.r), chan_u8(p.g), chan_u8(p.b), chan_u8(p.a)); }
 TKD Px rgba8_to_unorm(uint32_t p) {
     return Px{(float)(p & 255u) * kChannelScale, (float)((p >> 8) & 255u) * kChannelScale,
@@ -367,8 +369,9 @@ TKD RowParts segment_row_parts(float p0x, float p0y, float p1x, float p1y, float
     return RowParts{row_edge, dy, fminf(sx0, sx1), fmaxf(sx0, sx1)};
 }
 
-TKD float segment_area_at(float xmin_abs, float xmax_abs, uint32_t x) {
-    float xmin = xmin_abs - (float)x, xmax = xmax_abs - (float)x;
+// segment_area_at with the pixel's x already in FP32 (an integer 0..15, exact).
+TKD float segment_area_at_f(float xmin_abs, float xmax_abs, float xf) {
+    float xmin = xmin_abs - xf, xmax = xmax_abs - xf;
     float width = xmax - xmin;
     if (width == 0.0f) return clampf(1.0f - xmin, 0.0f, 1.0f);
     if (xmin >= 0.0f && xmax <= 1.0f) return fmaf(-0.5f, xmin + xmax, 1.0f);
@@ -376,12 +379,15 @@ TKD float segment_area_at(float xmin_abs, float xmax_abs, uint32_t x) {
     float full = clampf(-xmin, 0.0f, width);
     return fmaf(right - left, fmaf(-0.5f, left + right, 1.0f), full) / width;
 }
+TKD float segment_area_at(float xmin_abs, float xmax_abs, uint32_t x) {
+    return segment_area_at_f(xmin_abs, xmax_abs, (float)x);
+}
 
 // fill_alpha_at keeps three sums per pixel (base, running, partial) and combines
-// them as (base + running) + partial.  Each sum here is exact: every FP32 term is
-// truncated to a multiple of 2^-30 and the terms are added as 64-bit integers, so
-// a sum's value does not depend on the order in which the tile's segments were
-// emitted.  Only the conversion back to FP32 rounds.
+// them as (base + running) + partial in FP32.  Here every FP32 term is truncated
+// to a multiple of 2^-30 and the terms are added as 64-bit integers, the three
+// sums included, so the value does not depend on the order in which the tile's
+// segments were emitted.  Only the one conversion of the total to FP32 rounds.
 //
 // Every term lies in (-2, 2): the edge term is signum * a clamp to [0, 1], dy is
 // the difference of two values clamped to [0, 1], and the area factor is at most
@@ -418,7 +424,7 @@ TKD float exact_fill_value(const tk_line_segment *segs, int32_t backdrop, uint32
                 partial += coverage_fixed(segment_area_at(parts.xmin, parts.xmax, x) * parts.dy);
         }
     }
-    return (fixed_to_float(base) + fixed_to_float(running)) + fixed_to_float(partial);
+    return fixed_to_float(base + running + partial);
 }
 
 // ---------------------------------------------------------------------------

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <initializer_list>
 #include <cstring>
 #include <new>
@@ -79,6 +80,8 @@ struct tk_context {
                   f_fine_backward = nullptr, f_fine_backward_scenes = nullptr, f_line_backward = nullptr,
                   f_draw_tiles = nullptr, f_fine_diff_clips = nullptr, f_fine_diff_scenes_clips = nullptr,
                   f_fine_backward_clips = nullptr, f_fine_backward_scenes_clips = nullptr,
+                  // the same kernels keeping the forward's coverages for the backward
+                  f_fine_diff_keep[4] = {}, f_fine_backward_keep[4] = {},
                   f_fill_coverage = nullptr, f_pixel_math = nullptr, f_brush_words = nullptr;
     // Persistent forms (work queues) and the coarse scan's other block sizes.
     hipFunction_t f_fine_persistent = nullptr, f_fine_scenes_persistent = nullptr,
@@ -89,7 +92,8 @@ struct tk_context {
     hipFunction_t f_fe_paths = nullptr, f_fe_draw_bounds = nullptr, f_fe_batch_clips = nullptr,
                   f_fe_draw_count = nullptr, f_fe_batch_caps = nullptr, f_fe_tile_counts = nullptr,
                   f_fe_scan_blocks = nullptr, f_fe_scan_top = nullptr, f_fe_scan_apply = nullptr,
-                  f_fe_path_fill = nullptr, f_fe_tile_records = nullptr, f_fe_rows = nullptr, f_fe_tiles = nullptr;
+                  f_fe_path_fill = nullptr, f_fe_tile_records = nullptr, f_fe_rows = nullptr, f_fe_tiles = nullptr,
+                  f_flatten_count = nullptr, f_flatten_emit = nullptr;
     uint32_t compute_units = 0;
     // Scratch for the stage entry points (frames own their own).
     uint32_t *tickets = nullptr;  // [0] scan prefix, [1] coarse scan
@@ -419,6 +423,7 @@ struct tk_frame {
     // launch parameters (tk_frame_set_tuning, tk_frame_autotune)
     // The autotuner's choice on the 4096x4096, 200 000-path frame.
     tk_tuning tuning{TK_XCD_CHUNK_SCAN, 128u, TK_XCD_CHUNK_FINE, 0u, 64u, 1u, 8192u, 0u};
+    bool diff_kept = false;     // the last differentiable forward kept its coverages
     uint32_t *queues = nullptr;  // kQueueWords work counters of the persistent launches
     // Allocated entries of the buffers whose sizes follow the geometry (the GPU
     // front end grows them): lines, backdrop slabs, segments, scan chunks,
@@ -442,6 +447,12 @@ struct tk_frame {
     uint4 *fe_row_entries = nullptr;
     uint64_t fe_row_capacity = 0;
     unsigned long long *fe_host = nullptr;  // pinned: the totals and the status word
+    // GPU flattening of cubic contours (tk_frame_set_cubic_paths): per-path point
+    // counts (then starts), exact bounds, and the flattened FP64 points.
+    uint32_t *fl_counts = nullptr;
+    double *fl_bounds = nullptr, *fl_points = nullptr;
+    unsigned long long *fl_totals = nullptr;
+    uint64_t fl_point_capacity = 0;
 };
 
 namespace tk {
@@ -689,14 +700,14 @@ tk_status launch_tiles(tk_context *ctx, hipFunction_t fn, uint32_t columns, cons
     return launch(ctx, fn, gx, gy, 256, a);
 }
 
-// The fine stage of a differentiable render: sizes the coverage buffer from the
-// PTCL record count coarse wrote (one host round trip), then composites.
+// The fine stage of a differentiable render: the per-record buffer (one 1 KiB
+// slot per PTCL record: the coverage gradients, and the coverages when they are
+// kept) is sized to the frame's PTCL capacity, the host-side bound coarse writes
+// within (1.2-1.6x the records of typical scenes), so no host round trip waits
+// for the record count; then composites.
 tk_status run_fine_diff(tk_frame *f, const DiffTarget &t) {
     tk_context *ctx = f->ctx;
-    uint32_t records = 0;
-    TK_HIP(hipMemcpyAsync(&records, f->status + 1, 4, hipMemcpyDeviceToHost, ctx->stream));
-    TK_HIP(hipStreamSynchronize(ctx->stream));
-    const uint64_t need = std::min<uint64_t>(records, f->ptcl_capacity);
+    const uint64_t need = f->ptcl_capacity;
     if (need > f->diff_adj_records || !f->diff_adj) {
         if (f->diff_adj) TK_HIP(hipFree(f->diff_adj));
         f->diff_adj = nullptr;
@@ -725,6 +736,8 @@ tk_status run_fine_diff(tk_frame *f, const DiffTarget &t) {
     const bool clips = f->meta.layer_count != 0;
     hipFunction_t fn = f->scene_count ? (clips ? ctx->f_fine_diff_scenes_clips : ctx->f_fine_diff_scenes)
                                       : (clips ? ctx->f_fine_diff_clips : ctx->f_fine_diff);
+    f->diff_kept = f->tuning.diff_keep_coverage != 0u;
+    if (f->diff_kept) fn = ctx->f_fine_diff_keep[(f->scene_count ? 1 : 0) + (clips ? 2 : 0)];
     return launch_tiles(ctx, fn, a.cfg.tile_count, a);
 }
 
@@ -1093,6 +1106,14 @@ tk_status tk_context_create(int device, tk_context **out) {
         {&ctx->f_fine_diff_scenes_clips, "tk_k_fine_diff_scenes_clips"},
         {&ctx->f_fine_backward_clips, "tk_k_fine_backward_clips"},
         {&ctx->f_fine_backward_scenes_clips, "tk_k_fine_backward_scenes_clips"},
+        {&ctx->f_fine_diff_keep[0], "tk_k_fine_diff_keep"},
+        {&ctx->f_fine_diff_keep[1], "tk_k_fine_diff_scenes_keep"},
+        {&ctx->f_fine_diff_keep[2], "tk_k_fine_diff_clips_keep"},
+        {&ctx->f_fine_diff_keep[3], "tk_k_fine_diff_scenes_clips_keep"},
+        {&ctx->f_fine_backward_keep[0], "tk_k_fine_backward_keep"},
+        {&ctx->f_fine_backward_keep[1], "tk_k_fine_backward_scenes_keep"},
+        {&ctx->f_fine_backward_keep[2], "tk_k_fine_backward_clips_keep"},
+        {&ctx->f_fine_backward_keep[3], "tk_k_fine_backward_scenes_clips_keep"},
         {&ctx->f_fill_coverage, "tk_k_fill_coverage"},
         {&ctx->f_pixel_math, "tk_k_pixel_math"},
         {&ctx->f_brush_words, "tk_k_brush_words"},
@@ -1118,6 +1139,8 @@ tk_status tk_context_create(int device, tk_context **out) {
         {&ctx->f_fe_tile_records, "tk_k_fe_tile_records"},
         {&ctx->f_fe_rows, "tk_k_fe_rows"},
         {&ctx->f_fe_tiles, "tk_k_fe_tiles"},
+        {&ctx->f_flatten_count, "tk_k_flatten_count"},
+        {&ctx->f_flatten_emit, "tk_k_flatten_emit"},
     };
     for (auto &entry : table) {
         if (hipModuleGetFunction(entry.fn, ctx->module, entry.name) != hipSuccess) {
@@ -1306,8 +1329,21 @@ tk_status fe_scan(tk_context *ctx, tk_frame *f, uint32_t *values, uint32_t count
     return launch(ctx, ctx->f_fe_scan_apply, blocks, columns, 256, a);
 }
 
-tk_status frame_set_polygons(tk_frame *f, const float *points, const uint32_t *starts, uint64_t point_count,
-                             uint32_t *line_vertex) {
+// The vertices of every path: FP32 (`points`) or FP64 (`points64`, with the
+// paths' exact curve bounds in `bounds64`), path p's at [starts[p], starts[p+1]).
+struct FeInput {
+    const float *points = nullptr;
+    const double *points64 = nullptr, *bounds64 = nullptr;
+    const uint32_t *starts = nullptr;
+    uint64_t point_count = 0;
+    uint32_t *line_vertex = nullptr;
+};
+
+tk_status frame_set_geometry(tk_frame *f, const FeInput &in) {
+    const float *points = in.points;
+    const uint32_t *starts = in.starts;
+    const uint64_t point_count = in.point_count;
+    uint32_t *line_vertex = in.line_vertex;
     tk_context *ctx = f->ctx;
     TK_TRY(fe_prepare(f));
     tk_scene &s = f->meta;
@@ -1316,6 +1352,8 @@ tk_status frame_set_polygons(tk_frame *f, const float *points, const uint32_t *s
     TK_HIP(hipStreamSynchronize(ctx->side));
     tk_args_frontend a{};
     a.points = points;
+    a.points64 = in.points64;
+    a.bounds64 = in.bounds64;
     a.starts = starts;
     a.path_count = P;
     a.draw_count = D;
@@ -1448,6 +1486,52 @@ tk_status frame_set_polygons(tk_frame *f, const float *points, const uint32_t *s
     return TK_OK;
 }
 
+// GPU flattening of one closed cubic contour per path, then the front end on
+// the flattened FP64 points and the curves' exact bounds.
+tk_status frame_set_cubic_paths(tk_frame *f, const double *ctrl, const uint32_t *starts, double tolerance) {
+    tk_context *ctx = f->ctx;
+    TK_TRY(fe_prepare(f));
+    const uint32_t P = f->meta.path_count;
+    tk_status st = TK_OK;
+    if (!f->fl_counts) {
+        if ((st = dalloc(f, &f->fl_counts, uint64_t(P) + 1)) != TK_OK ||
+            (st = dalloc(f, &f->fl_bounds, 4 * std::max<uint64_t>(P, 1))) != TK_OK ||
+            (st = dalloc(f, &f->fl_totals, 2)) != TK_OK)
+            return st;
+    }
+    TK_HIP(hipStreamSynchronize(ctx->side));
+    tk_args_flatten a{ctrl, starts, P, tolerance, f->fl_counts, f->fl_bounds, f->fl_points, f->fl_point_capacity,
+                      f->fe_status};
+    TK_TRY(fill_u32(ctx, f->fe_status, 4, 0u));
+    TK_TRY(fill_u32(ctx, f->fl_totals, 4, 0u));
+    if (P) TK_TRY(launch(ctx, ctx->f_flatten_count, div_up(P, 256), 1, 256, a));
+    TK_TRY(fe_scan(ctx, f, f->fl_counts, P, 1, f->fl_totals));
+    // starts[P] = the total (the low word of the scan's u64 total)
+    TK_HIP(hipMemcpyAsync(f->fl_counts + P, f->fl_totals, 4, hipMemcpyDeviceToDevice, ctx->stream));
+    TK_HIP(hipMemcpyAsync(f->fe_host, f->fl_totals, 8, hipMemcpyDeviceToHost, ctx->stream));
+    TK_HIP(hipMemcpyAsync(f->fe_host + 1, f->fe_status, 8, hipMemcpyDeviceToHost, ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    const uint64_t total = f->fe_host[0];
+    const uint32_t flags = uint32_t(f->fe_host[1]), overflow = uint32_t(f->fe_host[1] >> 32);
+    if (flags & 1u) return fail(TK_ERR_INVALID, "non-finite control point: the frame needs valid geometry before it renders");
+    if (flags & 2u) return fail(TK_ERR_INVALID, "a path's points are not a start point and three per cubic");
+    if (overflow || total > 0x7FFFFFFFull) return fail(TK_ERR_OVERFLOW, "too many flattened points");
+    uint64_t cap = f->fl_point_capacity;
+    if (grow(cap, std::max<uint64_t>(total, 1))) {
+        TK_TRY(reallocate(f, &f->fl_points, 2 * cap));
+        f->fl_point_capacity = cap;
+    }
+    a.points = f->fl_points;
+    a.point_capacity = f->fl_point_capacity;
+    if (P) TK_TRY(launch(ctx, ctx->f_flatten_emit, div_up(P, 256), 1, 256, a));
+    FeInput in;
+    in.points64 = f->fl_points;
+    in.bounds64 = f->fl_bounds;
+    in.starts = f->fl_counts;
+    in.point_count = total;
+    return frame_set_geometry(f, in);
+}
+
 }  // namespace
 }  // namespace tk
 
@@ -1457,7 +1541,18 @@ tk_status tk_frame_set_polygons(tk_frame *f, const float *points, const uint32_t
                                 uint32_t *line_vertex) {
     if (!f || (point_count && !points) || !starts) return fail(TK_ERR_INVALID, "null argument");
     if (point_count > 0x7FFFFFFFull) return fail(TK_ERR_INVALID, "too many vertices");
-    return tk::frame_set_polygons(f, points, starts, point_count, line_vertex);
+    tk::FeInput in;
+    in.points = points;
+    in.starts = starts;
+    in.point_count = point_count;
+    in.line_vertex = line_vertex;
+    return tk::frame_set_geometry(f, in);
+}
+
+tk_status tk_frame_set_cubic_paths(tk_frame *f, const double *ctrl, const uint32_t *starts, double tolerance) {
+    if (!f || !ctrl || !starts) return fail(TK_ERR_INVALID, "null argument");
+    if (!(tolerance > 0.0) || !std::isfinite(tolerance)) return fail(TK_ERR_INVALID, "tolerance must be positive");
+    return tk::frame_set_cubic_paths(f, ctrl, starts, tolerance);
 }
 
 
@@ -1771,12 +1866,12 @@ tk_status tk_frame_set_tuning(tk_frame *f, const tk_tuning *t) {
     if (!pow2_in(t->coarse_block, 1u, 4u)) return fail(TK_ERR_INVALID, "coarse_block is 1, 2 or 4");
     if (!pow2_in(t->coarse_scan_items, 1024u, 8192u))
         return fail(TK_ERR_INVALID, "coarse_scan_items is 1024, 2048, 4096 or 8192");
+    if (t->diff_keep_coverage > 1u) return fail(TK_ERR_INVALID, "diff_keep_coverage is 0 or 1");
     if (f->graph_exec) {  // the captured launches carry the old values
         TK_HIP(hipStreamSynchronize(f->ctx->stream));
         tk::drop_graph(f);
     }
     f->tuning = *t;
-    f->tuning.reserved = 0u;
     return TK_OK;
 }
 
@@ -1876,6 +1971,7 @@ tk_status tk_frame_backward(tk_frame *f, const float *colors, const float *image
     const bool clips = s.layer_count != 0;
     hipFunction_t fn = f->scene_count ? (clips ? ctx->f_fine_backward_scenes_clips : ctx->f_fine_backward_scenes)
                                       : (clips ? ctx->f_fine_backward_clips : ctx->f_fine_backward);
+    if (f->diff_kept) fn = ctx->f_fine_backward_keep[(f->scene_count ? 1 : 0) + (clips ? 2 : 0)];
     TK_TRY(tk::launch_tiles(ctx, fn, a.cfg.tile_count, a));
     if (s.line_count) {
         tk_args_line_backward b{f->lines, s.line_count, s.path_count, f->paths, f->diff_rec_of, s.backdrop_len,

@@ -268,7 +268,9 @@ enum tk_fe_total : uint32_t {
 };
 
 struct tk_args_frontend {
-    const float *points;              // x, y of every vertex, physical pixels
+    const float *points;              // x, y of every vertex, physical pixels (FP32 input)
+    const double *points64;           // or FP64 input (flattened curves); used when set
+    const double *bounds64;           // optional: per path x0, y0, x1, y1 (exact curve bounds) instead of the vertices'
     const uint32_t *starts;           // path_count + 1 vertex offsets
     uint32_t path_count, draw_count, batch_count, layer_count;
     uint32_t scene_count, row_count, column_count, pad;
@@ -316,4 +318,19 @@ struct tk_args_fe_scan {
     uint32_t blocks;
     unsigned long long *totals;       // columns entries
     uint32_t *overflow;               // set when a prefix exceeds u32
+};
+
+// GPU flattening of closed cubic contours, one per path: path p's control
+// points are ctrl[starts[p] .. starts[p+1]) = the start point, then three per
+// cubic (the two control points and the end point).
+struct tk_args_flatten {
+    const double *ctrl;               // x, y pairs (FP64), physical pixels
+    const uint32_t *starts;           // path_count + 1 offsets
+    uint32_t path_count;
+    double tolerance;
+    uint32_t *counts;                 // per path: flattened points, then (scanned) their first index
+    double *bounds;                   // per path: exact curve bounds x0, y0, x1, y1
+    double *points;                   // the flattened points (x, y pairs)
+    uint64_t point_capacity;
+    uint32_t *status;                 // bit 0: a non-finite point; bit 1: a malformed contour
 };

@@ -122,13 +122,16 @@ class DiffScene:
         if self.fills is not None and self.fills.shape[0]:
             st.fill_steps = _subdivisions(_fill_cubics(self.fills).detach().cpu().numpy(), self.tolerance)
         if self.strokes is not None and self.strokes.shape[0]:
-            cub = _stroke_cubics(self.strokes)
-            st.stroke_steps = _subdivisions(cub.detach().cpu().numpy(), self.tolerance)
-            q = _flatten(cub.detach().double(), st.stroke_steps, _chain_ends(self.strokes)).cpu().numpy()
-            st.stroke_keep, st.recipe, st.stroke_kept = _stroke_recipe(
-                q, _stroke_counts(self.strokes, st.stroke_steps), self.widths.detach().cpu().numpy() * 0.5,
-                self.tolerance)
+            self._stroke_structure(st)
         return st
+
+    def _stroke_structure(self, st):
+        cub = _stroke_cubics(self.strokes)
+        st.stroke_steps = _subdivisions(cub.detach().cpu().numpy(), self.tolerance)
+        q = _flatten(cub.detach().double(), st.stroke_steps, _chain_ends(self.strokes)).cpu().numpy()
+        st.stroke_keep, st.recipe, st.stroke_kept = _stroke_recipe(
+            q, _stroke_counts(self.strokes, st.stroke_steps), self.widths.detach().cpu().numpy() * 0.5,
+            self.tolerance)
 
     def polygons(self, st=None):
         """(vertices (V, 2), vertices per draw in painter order) as torch tensors of
@@ -154,17 +157,29 @@ class DiffScene:
         fill_vertices = int(part_counts[0].sum())
         first_of = [np.cumsum(part_counts[0]) - part_counts[0],
                     fill_vertices + np.cumsum(part_counts[1]) - part_counts[1]]
-        kinds = np.array([kd for kd, _ in self.order], np.int64)
-        idx = np.array([i for _, i in self.order], np.int64)
+        gather, counts = self._painter_gather(part_counts, first_of)
+        verts = allv.index_select(0, torch.as_tensor(gather, device=allv.device))
+        return verts, counts
+
+    def _painter_gather(self, part_counts, first_of):
+        """(indices of the painter-order vertices among the fill and stroke vertices,
+        vertices per draw) from each fill's and stroke's vertex count and first
+        vertex; (kind, index) arrays of the order are kept while it is unchanged."""
+        key = (id(self.order), len(self.order))
+        if getattr(self, "_order_key", None) != key:
+            self._order_kinds = np.array([kd for kd, _ in self.order], np.int64)
+            self._order_index = np.array([i for _, i in self.order], np.int64)
+            self._order_key = key
+        kinds, idx = self._order_kinds, self._order_index
         counts = np.zeros(len(idx), np.int64)
         first = np.zeros(len(idx), np.int64)
         for kind in (0, 1):
             sel = kinds == kind
-            counts[sel] = part_counts[kind][idx[sel]]
-            first[sel] = first_of[kind][idx[sel]]
+            if sel.any():
+                counts[sel] = part_counts[kind][idx[sel]]
+                first[sel] = first_of[kind][idx[sel]]
         gather = np.arange(int(counts.sum())) - np.repeat(np.cumsum(counts) - counts, counts) + np.repeat(first, counts)
-        verts = allv.index_select(0, torch.as_tensor(gather, device=allv.device))
-        return verts, counts
+        return gather, counts
 
     def draw_rules(self):
         return np.array([int(self.rules[i]) if kind == 0 else 0 for kind, i in self.order], np.uint32)
@@ -176,14 +191,21 @@ class DiffScene:
     def paint_records(self):
         """(draws, 32) paint records in painter order (the layout tk_frame_render_diff
         reads), differentiable in the colours and the gradient stops."""
-        pm = self.premultiplied()
-        n = pm.shape[0]
-        head = np.zeros((n, 16), np.float64)
-        head[:, 5] = 1.0
-        stops = torch.cat([pm, pm.new_zeros((n, 12))], 1)
-        if self.gradients:
-            rows, stop_list, which, slot = [], [], [], []
-            for g, (d, entry) in enumerate(sorted(self.gradients.items())):
+        return batch_paint_records([self])
+
+    def _paint_layout(self):
+        """The fixed part of the paint records: (head (draws, 16): kind, line,
+        stop count and offsets per draw; each gradient's draw; per stop, its
+        gradient and slot; the stop tensors in gradient order).  Kept while the
+        gradient entries and the draw count are unchanged."""
+        n = int(self.colors.shape[0])
+        items = sorted(self.gradients.items())
+        key = (n, tuple((d, id(entry)) for d, entry in items))
+        if getattr(self, "_layout_key", None) != key:
+            head = np.zeros((n, 16), np.float64)
+            head[:, 5] = 1.0
+            rows, which, slot = [], [], []
+            for g, (d, entry) in enumerate(items):
                 line, offsets, st, kind = gradient_parts(entry)
                 k = len(offsets)
                 if not 1 <= k <= 4 or st.shape[0] != k or (kind == "four_corner" and k != 4):
@@ -193,16 +215,13 @@ class DiffScene:
                 head[d, 5] = k
                 head[d, 6:6 + k] = offsets
                 rows.append(d)
-                stop_list.append(st)
                 which.extend([g] * k)
                 slot.extend(range(k))
-            # every gradient's stops premultiplied at once and placed in their blocks
-            st = torch.cat(stop_list).to(device=pm.device, dtype=pm.dtype)
-            spm = torch.cat([st[:, :3] * st[:, 3:4], st[:, 3:4]], 1)
-            blocks = pm.new_zeros((len(rows), 4, 4)).index_put(
-                (torch.as_tensor(which, device=pm.device), torch.as_tensor(slot, device=pm.device)), spm)
-            stops = stops.index_put((torch.as_tensor(rows, device=pm.device),), blocks.reshape(-1, 16))
-        return torch.cat([torch.as_tensor(head, dtype=pm.dtype, device=pm.device), stops], 1)
+            self._layout = (head, np.asarray(rows, np.int64), np.asarray(which, np.int64),
+                            np.asarray(slot, np.int64))
+            self._layout_key = key
+            self._layout_entries = [entry for _, entry in items]  # held, so their ids stay theirs
+        return self._layout + ([gradient_parts(entry)[2] for _, entry in items],)
 
 
 GRADIENT_KINDS = {"linear": 1.0, "radial": 2.0, "sweep": 3.0, "four_corner": 4.0}
@@ -216,6 +235,116 @@ def gradient_parts(entry):
     if kind not in GRADIENT_KINDS:
         raise ValueError(f"unknown gradient kind {kind!r}")
     return line, offsets, stops, kind
+
+
+# --- the geometry and paint of many scenes at once ------------------------------------
+#
+# A batch of scenes (an optimiser step over hundreds of them) evaluates each step
+# of the geometry once for all scenes: the fills of every scene flatten in one
+# pass (one host copy of the subdivision counts, one set of device operations),
+# and every scene's paint records come from one premultiplication and one scatter
+# of all gradient stops.  Elementwise, the values are those of the per-scene
+# methods bit for bit.
+
+
+def _fill_groups(scenes):
+    """Indices of the scenes with fills, grouped by cubics per contour."""
+    groups = {}
+    for i, s in enumerate(scenes):
+        if s.fills is not None and s.fills.shape[0]:
+            groups.setdefault(int(s.fills.shape[1]), []).append(i)
+    return groups
+
+
+def batch_structures(scenes):
+    """DiffScene.structure of every scene, the fill subdivisions of all scenes from
+    one host copy per cubic count."""
+    structs = [Structure() for _ in scenes]
+    for c, idx in _fill_groups(scenes).items():
+        fills = torch.cat([scenes[i].fills.detach() for i in idx])
+        sizes = [int(scenes[i].fills.shape[0]) * c for i in idx]
+        tol = np.repeat(np.array([scenes[i].tolerance for i in idx], np.float64), sizes)
+        steps = _subdivisions(_fill_cubics(fills).cpu().numpy(), tol)
+        ends = np.cumsum(sizes)
+        for i, lo, hi in zip(idx, ends - sizes, ends):
+            structs[i].fill_steps = steps[lo:hi]
+    for s, st in zip(scenes, structs):
+        if s.strokes is not None and s.strokes.shape[0]:
+            s._stroke_structure(st)
+    return structs
+
+
+def batch_polygons(scenes, structs=None):
+    """DiffScene.polygons of every scene as one tensor: (vertices (V, 2) of all
+    scenes laid end to end in painter order, vertices per draw of each scene, the
+    first vertex of each scene and the total); differentiable in the parameters."""
+    structs = structs or batch_structures(scenes)
+    parts, base = [], 0
+    none = np.zeros(0, np.int64)
+    part_counts = [[none, none] for _ in scenes]
+    first_of = [[none, none] for _ in scenes]
+    for c, idx in _fill_groups(scenes).items():
+        cub = _fill_cubics(torch.cat([scenes[i].fills for i in idx]))
+        steps = np.concatenate([structs[i].fill_steps for i in idx])
+        parts.append(_flatten(cub, steps, np.zeros(len(steps), bool)))
+        for i in idx:
+            per = structs[i].fill_steps.reshape(scenes[i].fills.shape[0], -1).sum(1)
+            part_counts[i][0] = per
+            first_of[i][0] = base + np.cumsum(per) - per
+            base += int(per.sum())
+    for i, (s, st) in enumerate(zip(scenes, structs)):
+        if st.stroke_steps is None:
+            continue
+        q = _flatten(_stroke_cubics(s.strokes), st.stroke_steps, _chain_ends(s.strokes))
+        outlines = _stroke_outlines(q, st.stroke_keep, st.recipe, st.stroke_kept, 0.5 * s.widths)
+        per = np.array([len(r) for r in st.recipe], np.int64)
+        if outlines:
+            parts.append(torch.cat(outlines))
+        part_counts[i][1] = per
+        first_of[i][1] = base + np.cumsum(per) - per
+        base += int(per.sum())
+    gathers, counts, firsts = [], [], [0]
+    for i, s in enumerate(scenes):
+        g, k = s._painter_gather(part_counts[i], first_of[i])
+        gathers.append(g)
+        counts.append(k)
+        firsts.append(firsts[-1] + int(k.sum()))
+    ref = next((t for s in scenes for t in (s.fills, s.strokes) if t is not None), None)
+    if not parts:
+        return torch.zeros((0, 2), dtype=ref.dtype, device=ref.device), counts, np.asarray(firsts, np.int64)
+    allv = torch.cat(parts) if len(parts) > 1 else parts[0]
+    verts = allv.index_select(0, torch.as_tensor(np.concatenate(gathers), device=allv.device))
+    return verts, counts, np.asarray(firsts, np.int64)
+
+
+def batch_paint_records(scenes):
+    """DiffScene.paint_records of every scene laid end to end: one premultiplication
+    of all colours and one scatter of all gradient stops; differentiable."""
+    colors = torch.cat([s.colors for s in scenes]) if len(scenes) > 1 else scenes[0].colors
+    pm = torch.cat([colors[:, :3] * colors[:, 3:4], colors[:, 3:4]], 1)
+    n = pm.shape[0]
+    heads, rows, which, slot, stop_list = [], [], [], [], []
+    d0 = g0 = 0
+    for s in scenes:
+        head, r, w, sl, st = s._paint_layout()
+        heads.append(head)
+        rows.append(r + d0)
+        which.append(w + g0)
+        slot.append(sl)
+        stop_list.extend(st)
+        d0 += head.shape[0]
+        g0 += len(r)
+    head = np.concatenate(heads) if len(heads) > 1 else heads[0]
+    stops = torch.cat([pm, pm.new_zeros((n, 12))], 1)
+    if stop_list:
+        st = torch.cat(stop_list).to(device=pm.device, dtype=pm.dtype)
+        spm = torch.cat([st[:, :3] * st[:, 3:4], st[:, 3:4]], 1)
+        dev = pm.device
+        blocks = pm.new_zeros((g0, 4, 4)).index_put(
+            (torch.as_tensor(np.concatenate(which), device=dev), torch.as_tensor(np.concatenate(slot), device=dev)),
+            spm)
+        stops = stops.index_put((torch.as_tensor(np.concatenate(rows), device=dev),), blocks.reshape(-1, 16))
+    return torch.cat([torch.as_tensor(head, dtype=pm.dtype, device=pm.device), stops], 1)
 
 
 def _fill_cubics(fills):
@@ -465,11 +594,18 @@ def clear_color(scene):
 
 
 class Plan:
-    """One encoded scene (or batch of scenes) on the device, for forward and backward."""
+    """One encoded scene (or batch of scenes) on the device, for forward and backward.
 
-    def __init__(self, context, scenes, verts, counts, colors=None):
+    sync=True: each forward and backward waits for the device and forward reads the
+    render's status flags (raising on an overflow).  sync=False: the calls are only
+    ordered against torch's current stream (stream waits, no host wait), and the
+    flags are read by check_status(), for loops that issue many plans back to back."""
+
+    def __init__(self, context, scenes, verts, counts, colors=None, sync=True):
         self.context = context
         self.scenes = scenes
+        self.sync = sync
+        self._stream = None
         self.encoded, self.canvases, la, lb, self.sizes = [], [], [], [], []
         v_off, l_off = 0, 0
         self.line_ranges = []
@@ -523,34 +659,63 @@ class Plan:
             self._lb_t = torch.as_tensor(self.line_b, device=device)
         return self._la_t, self._lb_t
 
-    def update(self, verts, counts):
+    def _clip_points(self, dev):
+        """Every path clip's points (FP32, in the scenes' source order) on the device,
+        and the first point of each (scene, clip)."""
+        if getattr(self, "_clip_pts", None) is None or self._clip_pts.device != dev:
+            pts, first = [], {}
+            n = 0
+            for i, (sc, e) in enumerate(zip(self.scenes, self.encodings)):
+                for src in e.sources:
+                    if src[0] == "clip":
+                        c = np.asarray(sc.clips[src[1]][0], np.float32).reshape(-1, 2)
+                        first[(i, src[1])] = (n, len(c))
+                        pts.append(c)
+                        n += len(c)
+            arr = np.concatenate(pts) if pts else np.zeros((0, 2), np.float32)
+            self._clip_pts = torch.as_tensor(arr, device=dev)
+            self._clip_first = first
+        return self._clip_pts, self._clip_first
+
+    def update(self, verts, counts, firsts=None):
         """New vertices for the same scenes (the same draws, clips and structure;
         a polygon's vertex count may change): the GPU front end rebuilds the
         frame's lines, path records, bounds, tile bins and plans from them
         (tk_frame_set_polygons) instead of recording and encoding again, and the
-        line -> vertex maps come from the lines it kept."""
+        line -> vertex maps come from the lines it kept.  verts: a tensor per scene,
+        or one tensor of all scenes' vertices with each scene's first vertex and
+        the total in `firsts` (batch_polygons)."""
         dev = torch.device("cuda")
-        pts, p2v, lens = [], [], []
-        v_off = 0
-        for sc, e, v, k in zip(self.scenes, self.encodings, verts, counts):
+        if firsts is None:
+            firsts = np.concatenate([[0], np.cumsum([len(v) for v in verts])]).astype(np.int64)
+            verts = torch.cat(list(verts)) if len(verts) else torch.zeros((0, 2), device=dev)
+        v32 = verts.detach().to(dev, torch.float32)
+        v_total = int(firsts[-1])
+        clip_pts, clip_first = self._clip_points(dev)
+        # each point's row in [scene vertices | clip points], its vertex (-1: a clip
+        # point), and the vertex count of each polygon
+        rows, p2v, lens = [], [], []
+        for i, (e, k) in enumerate(zip(self.encodings, counts)):
             k = np.asarray(k, np.int64)
-            offs = np.concatenate([[0], np.cumsum(k)])
-            v32 = v.detach().to(dev, torch.float32)
+            offs = firsts[i] + np.concatenate([[0], np.cumsum(k)])
             for src in e.sources:
                 if src[0] == "clip":
-                    c = torch.as_tensor(np.asarray(sc.clips[src[1]][0], np.float32).reshape(-1, 2), device=dev)
-                    pts.append(c)
-                    p2v.append(torch.full((len(c),), -1, dtype=torch.int64, device=dev))
-                    lens.append(len(c))
+                    c0, n = clip_first[(i, src[1])]
+                    rows.append(v_total + c0 + np.arange(n))
+                    p2v.append(np.full(n, -1, np.int64))
+                    lens.append(np.array([n], np.int64))
                 else:
                     lo, hi = src[1], src[2]
-                    pts.append(v32[offs[lo]:offs[hi]])
-                    p2v.append(torch.arange(int(offs[lo]), int(offs[hi]), device=dev) + v_off)
-                    lens.extend(int(n) for n in k[lo:hi])
-            v_off += len(v)
-        points = torch.cat(pts) if pts else torch.zeros((0, 2), dtype=torch.float32, device=dev)
-        p2v = torch.cat(p2v) if p2v else torch.zeros(0, dtype=torch.int64, device=dev)
-        lens_t = torch.as_tensor(np.asarray(lens, np.int64), device=dev)
+                    r = np.arange(offs[lo], offs[hi])
+                    rows.append(r)
+                    p2v.append(r)
+                    lens.append(k[lo:hi])
+        cat = lambda parts: np.concatenate(parts) if parts else np.zeros(0, np.int64)
+        rows, p2v_np, lens = cat(rows), cat(p2v), cat(lens)
+        points = torch.cat([v32, clip_pts]).index_select(0, torch.as_tensor(rows, device=dev))
+        p2v = torch.as_tensor(p2v_np, device=dev)
+        v_off = v_total
+        lens_t = torch.as_tensor(lens, device=dev)
         starts = torch.zeros(len(lens) + 1, dtype=torch.int64, device=dev)
         starts[1:] = torch.cumsum(lens_t, 0)
         lv = self.frame.set_polygons(points, starts.to(torch.int32), line_vertex=True).long()
@@ -579,17 +744,41 @@ class Plan:
             out.append(flat[off:off + w * h].reshape(h, w, 4))
         return out
 
-    def forward(self, colors):
-        """colors: (D, 4) premultiplied float32 on the GPU; returns the flat image."""
-        colors = colors.detach().to(torch.float32).contiguous()
-        image = torch.zeros((self.pixels, 4), dtype=torch.float32, device=colors.device)
-        torch.cuda.synchronize(colors.device)
-        check(lib().tk_frame_render_diff(self.frame._h, C.c_void_p(colors.data_ptr()), C.c_void_p(image.data_ptr())))
+    def _before(self, device):
+        """Orders the context's stream after torch's current stream."""
+        if self.sync:
+            torch.cuda.synchronize(device)
+            return
+        if self._stream is None:
+            self._stream = torch.cuda.ExternalStream(self.context.stream_handle(), device=device)
+        self._stream.wait_stream(torch.cuda.current_stream(device))
+
+    def _after(self, device):
+        """Orders torch's current stream after the context's (the outputs, and any
+        tensor the call read, are then safe to use or free there)."""
+        if self.sync:
+            self.context.synchronize()
+        else:
+            torch.cuda.current_stream(device).wait_stream(self._stream)
+
+    def check_status(self):
+        """Waits for the device and raises if the last differentiable render flagged
+        an overflow."""
         self.context.synchronize()
         flags = C.c_uint32()
         check(lib().tk_frame_diff_status(self.frame._h, C.byref(flags)))
         if flags.value:
             raise tk.TileinkError(_abi.TK_ERR_INVALID, f"differentiable render flagged {flags.value:#x}")
+
+    def forward(self, colors):
+        """colors: (D, 4) premultiplied float32 on the GPU; returns the flat image."""
+        colors = colors.detach().to(torch.float32).contiguous()
+        image = torch.zeros((self.pixels, 4), dtype=torch.float32, device=colors.device)
+        self._before(colors.device)
+        check(lib().tk_frame_render_diff(self.frame._h, C.c_void_p(colors.data_ptr()), C.c_void_p(image.data_ptr())))
+        self._after(colors.device)
+        if self.sync:
+            self.check_status()
         self._colors = colors
         return image
 
@@ -598,10 +787,10 @@ class Plan:
         dev = grad.device
         line_grad = torch.empty((max(self.line_count, 1), 4), dtype=torch.float32, device=dev)
         color_grad = torch.empty((max(self.draw_count, 1), 32), dtype=torch.float32, device=dev)
-        torch.cuda.synchronize(dev)
+        self._before(dev)
         check(lib().tk_frame_backward(self.frame._h, C.c_void_p(self._colors.data_ptr()), C.c_void_p(grad.data_ptr()),
                                       C.c_void_p(line_grad.data_ptr()), C.c_void_p(color_grad.data_ptr())))
-        self.context.synchronize()
+        self._after(dev)
         return line_grad[:self.line_count], color_grad[:self.draw_count]
 
     def vertex_grad(self, line_grad, like):
@@ -636,23 +825,21 @@ class Renderer:
     calls give the GPU front end only the new vertices (Plan.update).  Calling
     returns the FP32 images, differentiable in the parameters."""
 
-    def __init__(self, context, scenes, device="cuda"):
+    def __init__(self, context, scenes, device="cuda", sync=True):
         self.context, self.device = context, torch.device(device)
         self.single = isinstance(scenes, DiffScene)
         self.scenes = [scenes] if self.single else list(scenes)
         self.plan = None
+        self.sync = sync  # Plan's: False orders the native calls on streams only
 
     def __call__(self):
-        structs = [s.structure() for s in self.scenes]
-        polys = [s.polygons(st) for s, st in zip(self.scenes, structs)]
-        verts = [v for v, _ in polys]
-        counts = [k for _, k in polys]
+        verts, counts, firsts = batch_polygons(self.scenes)
         if self.plan is None:
-            self.plan = Plan(self.context, self.scenes, verts, counts)
+            self.plan = Plan(self.context, self.scenes, [verts[a:b] for a, b in zip(firsts[:-1], firsts[1:])], counts,
+                             sync=self.sync)
         else:
-            self.plan.update(verts, counts)
-        flat = _Render.apply(torch.cat(verts), self.plan.full_paint(torch.cat([s.paint_records() for s in self.scenes])),
-                             self.plan, self.device)
+            self.plan.update(verts, counts, firsts)
+        flat = _Render.apply(verts, self.plan.full_paint(batch_paint_records(self.scenes)), self.plan, self.device)
         images = self.plan.images(flat)
         return images[0] if self.single else images
 
@@ -662,13 +849,9 @@ def render(context, scenes, device="cuda"):
     parameters.  One scene renders as one frame, several as one batch."""
     single = isinstance(scenes, DiffScene)
     scenes = [scenes] if single else list(scenes)
-    structs = [s.structure() for s in scenes]
-    polys = [s.polygons(st) for s, st in zip(scenes, structs)]
-    verts = [v for v, _ in polys]
-    counts = [k for _, k in polys]
-    plan = Plan(context, scenes, verts, counts)
-    flat = _Render.apply(torch.cat(verts), plan.full_paint(torch.cat([s.paint_records() for s in scenes])), plan,
-                         torch.device(device))
+    verts, counts, firsts = batch_polygons(scenes)
+    plan = Plan(context, scenes, [verts[a:b] for a, b in zip(firsts[:-1], firsts[1:])], counts)
+    flat = _Render.apply(verts, plan.full_paint(batch_paint_records(scenes)), plan, torch.device(device))
     images = plan.images(flat)
     return images[0] if single else images
 
@@ -745,10 +928,10 @@ def random_gradients(scene, fraction, seed, radial=0.0):
 
 
 def random_scene(seed, width, height, fills, strokes=0, cubics=4, dtype=torch.float64, device="cpu",
-                 radius_mu=math.log(24.0)):
+                 radius_mu=math.log(24.0), stroke_cubics=2):
     rng = np.random.default_rng(seed)
     f = blob_fills(rng, fills, width, height, cubics, radius_mu)
-    s = wave_strokes(rng, strokes, width, height) if strokes else None
+    s = wave_strokes(rng, strokes, width, height, stroke_cubics) if strokes else None
     widths = rng.lognormal(math.log(3.0), 0.4, strokes) if strokes else None
     colors = np.concatenate([rng.uniform(0, 1, (fills + strokes, 3)), rng.uniform(0.3, 0.9, (fills + strokes, 1))], 1)
     order = [(0, i) for i in range(fills)] + [(1, i) for i in range(strokes)]
