@@ -279,9 +279,10 @@ def fit(ctx, count=256, steps=200, lr=0.05, loop_steps=3):
     in diff_batch; each scene's target is the render of a copy whose control points
     are perturbed by N(0, 4 px) and colours (and gradient stops) by N(0, 0.05);
     `steps` Adam steps on control points, colours and stops (FP32, on the GPU)
-    against the squared error.  The first step records and encodes the scenes on
-    the host; every later step evaluates the geometry and paint of all scenes at
-    once (diff.batch_polygons, diff.batch_paint_records), gives the GPU front end
+    against the squared error, the batch's parameters held as three tensors that
+    every scene's tensors are slices of.  A renderer's first call records and encodes the
+    scenes on the host (timed apart); every step evaluates the geometry and paint
+    of all scenes at once (diff.batch_polygons, diff.batch_paint_records), gives the GPU front end
     only the moved vertices (diff.Renderer -> tk_frame_set_polygons), then runs
     forward + backward for all scenes at once.  Beside it, `loop_steps` steps of
     the per-scene loop (each scene its own Renderer, one frame per scene), its
@@ -313,40 +314,85 @@ def fit(ctx, count=256, steps=200, lr=0.05, loop_steps=3):
         targets.append(t)
     with torch.no_grad():
         ref = [im.clone() for im in diff.render(ctx, targets, "cuda")]
-    params = [p for s in scenes_ for p in (s.fills, s.colors, *[g[2] for g in s.gradients.values()])]
-    for p in params:
-        p.requires_grad_(True)
+    # The optimiser's parameters as three tensors (all control points, all colours,
+    # all stop colours); each scene's tensors are slices of them laid end to end,
+    # which the batched geometry and paint take whole (one leaf each for autograd
+    # and Adam instead of one per scene and per gradient).
+    fills = torch.cat([s.fills for s in scenes_]).requires_grad_(True)
+    colors = torch.cat([s.colors for s in scenes_]).requires_grad_(True)
+    stop_tensors = [g[2] for s in scenes_ for _, g in sorted(s.gradients.items())]
+    stops = torch.cat(stop_tensors).requires_grad_(True) if stop_tensors else None
+    f0 = c0 = g0 = 0
+    for s in scenes_:
+        n = s.fills.shape[0]
+        s.fills = fills[f0:f0 + n]
+        f0 += n
+        n = s.colors.shape[0]
+        s.colors = colors[c0:c0 + n]
+        c0 += n
+        for d, entry in sorted(s.gradients.items()):
+            line, offsets, st, kind = diff.gradient_parts(entry)
+            s.gradients[d] = (line, offsets, stops[g0:g0 + st.shape[0]], kind)
+            g0 += st.shape[0]
+    params = [p for p in (fills, colors, stops) if p is not None]
 
-    def run(renderers, n_steps):
+
+    def run(renderers, n_steps, params, clamped):
         opt = torch.optim.Adam(params, lr=lr)
         losses, step_ms = [], []
+        refs = []  # each renderer's target in its flat pixel layout (zero padding)
+        first = 0
+        t0 = time.perf_counter()
+        for r in renderers:
+            r()  # the first call records and encodes on the host
+            flat_ref = torch.zeros_like(r.flat.detach())
+            for (w, h, _), off in zip(r.plan.sizes, r.plan.offsets):
+                flat_ref[off:off + w * h] = ref[first].reshape(-1, 4)
+                first += 1
+            refs.append(flat_ref)
+        torch.cuda.synchronize()
+        record_ms = (time.perf_counter() - t0) * 1e3
         for _ in range(n_steps):
             t0 = time.perf_counter()
             opt.zero_grad()
-            images = [im for r in renderers for im in _as_list(r())]
-            loss = sum(((im - r_) ** 2).sum() for im, r_ in zip(images, ref))
+            for r in renderers:
+                r()
+            loss = sum(((r.flat - t) ** 2).sum() for r, t in zip(renderers, refs))
             loss.backward()
             opt.step()
             with torch.no_grad():
-                for s in scenes_:
-                    s.colors.clamp_(0, 1)
-                    for g in s.gradients.values():
-                        g[2].clamp_(0, 1)
+                if clamped:  # colours and stop colours
+                    torch._foreach_clamp_min_(clamped, 0.0)
+                    torch._foreach_clamp_max_(clamped, 1.0)
             torch.cuda.synchronize()
             step_ms.append((time.perf_counter() - t0) * 1e3)
             losses.append(round(float(loss.detach()), 3))
-        return losses, step_ms
+        return losses, step_ms, record_ms
 
     start = [p.detach().clone() for p in params]
     batch = diff.Renderer(ctx, scenes_, "cuda", sync=False)
-    losses, step_ms = run([batch], steps)
+    losses, step_ms, record_ms = run([batch], steps, params, params[1:])
     batch.plan.check_status()
-    with torch.no_grad():  # the loop starts from the same parameters
+    # The per-scene loop from the same starting values, each scene with its own
+    # parameter tensors (as a loop over independent scenes holds them), its native
+    # calls ordered on streams with no host wait per scene.
+    with torch.no_grad():
         for p, v in zip(params, start):
             p.copy_(v)
-    # the per-scene loop with stream-ordered calls (no host wait per scene)
+    loop_params, loop_clamped = [], []
+    for s in scenes_:
+        s.fills = s.fills.detach().clone().requires_grad_(True)
+        s.colors = s.colors.detach().clone().requires_grad_(True)
+        loop_params += [s.fills, s.colors]
+        loop_clamped.append(s.colors)
+        for d, entry in sorted(s.gradients.items()):
+            line, offsets, st, kind = diff.gradient_parts(entry)
+            st = st.detach().clone().requires_grad_(True)
+            s.gradients[d] = (line, offsets, st, kind)
+            loop_params.append(st)
+            loop_clamped.append(st)
     loop = [diff.Renderer(ctx, s, "cuda", sync=False) for s in scenes_]
-    loop_losses, loop_ms = run(loop, loop_steps)
+    loop_losses, loop_ms, loop_record_ms = run(loop, loop_steps, loop_params, loop_clamped)
     for r in loop:
         r.plan.check_status()
     med = float(np.median(step_ms[1:])) if len(step_ms) > 1 else step_ms[0]
@@ -354,17 +400,14 @@ def fit(ctx, count=256, steps=200, lr=0.05, loop_steps=3):
     return {"scenes": count, "size": "256x256", "paths": int(paths.sum()), "clips_per_scene": 1,
             "gradient_paints": int(sum(len(s.gradients) for s in scenes_)), "steps": steps,
             "loss": losses, "loss_ratio": round(losses[-1] / losses[0], 4),
+            "record_ms": round(record_ms, 1), "loop_record_ms": round(loop_record_ms, 1),
             "first_step_ms": round(step_ms[0], 1), "step_ms_median": round(med, 2),
             "scenes_per_s": round(count / (med * 1e-3), 1),
             "loop_steps": loop_steps, "loop_step_ms_median": round(loop_med, 2),
             "loop_scenes_per_s": round(count / (loop_med * 1e-3), 1), "loop_loss": loop_losses,
             "speedup": round(loop_med / med, 3),
-            "note": "first step records and encodes on the host; later steps move the vertices through the GPU "
-                    "front end (tk_frame_set_polygons)"}
-
-
-def _as_list(images):
-    return images if isinstance(images, list) else [images]
+            "note": "each renderer's first call records and encodes on the host (record_ms, untimed in the "
+                    "steps); every step moves the vertices through the GPU front end (tk_frame_set_polygons)"}
 
 
 def main():

@@ -1193,15 +1193,19 @@ __device__ uint32_t stack_wrappers_of_batches(const CoarseCtx &c, uint32_t pendi
 }
 
 // The class table: layer_class_bits of every distinct (draw, tag) pair of the layer
-// records on every table column's tile, one thread per (column, pair).
+// records on every table column's tile, bits[column * pair_count + pair].  A
+// workgroup takes one pair over 256 consecutive columns, so the layer, its draw
+// record and its SDF words are the same in every lane (scalar loads, loaded once
+// per wave) and only the tile position varies.
 extern "C" __global__ __launch_bounds__(256) void tk_k_layer_classes(tk_args_layer_classes a) {
-    const uint64_t id = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    if (id >= (uint64_t)a.list.columns * a.pair_count) return;
-    const uint32_t column = (uint32_t)(id / a.pair_count), pair = (uint32_t)(id % a.pair_count);
+    const uint32_t groups = (a.list.columns + 255u) / 256u;  // workgroups per pair
+    const uint32_t pair = blockIdx.x / groups;
+    const uint32_t column = (blockIdx.x - pair * groups) * 256u + threadIdx.x;
+    if (pair >= a.pair_count || column >= a.list.columns) return;
     const uint32_t tile = a.list.tiles ? a.list.tiles[column] : column;
     const CoarseCtx c{&a.cfg,   &a.in,   tile % a.cfg.tiles_width, tile / a.cfg.tiles_width, nullptr, nullptr,
                       a.cfg.tiles_width, a.cfg.tiles_height};
-    a.bits[id] = (uint8_t)layer_class_bits(c, a.in.layers[a.pair_layer[pair]]);
+    a.bits[(uint64_t)column * a.pair_count + pair] = (uint8_t)layer_class_bits(c, a.in.layers[a.pair_layer[pair]]);
 }
 
 // coarse_count_bins for every batch (coarse/count.hlsl:48-73): one wave per tile.

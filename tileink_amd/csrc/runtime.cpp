@@ -288,7 +288,9 @@ tk_status run_coarse(tk_context *ctx, const tk_coarse_config &cfg, const tk_coar
                      shape.xcd_run, shape.queue};
     if (plan.bits && plan.pair_count) {
         tk_args_layer_classes b{cfg, in, plan.pair_layer, plan.pair_count, plan.bits, list};
-        TK_TRY(launch(ctx, ctx->f_layer_classes, uint32_t(div_up(uint64_t(list.columns) * plan.pair_count, 256)), 1,
+        // pair-major: div_up(columns, 256) workgroups per pair (the table is at most
+        // 64 MiB, so the grid fits 32 bits)
+        TK_TRY(launch(ctx, ctx->f_layer_classes, uint32_t(div_up(list.columns, 256) * uint64_t(plan.pair_count)), 1,
                       256, b));
         a.classes = tk_layer_classes{plan.bits, plan.pair_of, plan.pair_count};
     }

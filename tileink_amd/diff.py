@@ -44,22 +44,45 @@ def _bernstein(t):
 
 
 def _subdivisions(cubics, tolerance):
-    """Uniform steps per cubic ((N, 4, 2) array) for a chord error under tolerance."""
+    """Uniform steps per cubic ((N, 4, 2) array) for a chord error under tolerance.
+    The second differences' norms are sqrt(x * x + y * y) in FP64, term by term as
+    _subdivisions_on_device evaluates them, so both give the same counts."""
     p = np.asarray(cubics, np.float64)
-    dd = np.maximum(np.linalg.norm(p[:, 0] - 2 * p[:, 1] + p[:, 2], axis=-1),
-                    np.linalg.norm(p[:, 1] - 2 * p[:, 2] + p[:, 3], axis=-1))
+    a = p[:, 0] - 2 * p[:, 1] + p[:, 2]
+    b = p[:, 1] - 2 * p[:, 2] + p[:, 3]
+    dd = np.maximum(np.sqrt(a[:, 0] * a[:, 0] + a[:, 1] * a[:, 1]), np.sqrt(b[:, 0] * b[:, 0] + b[:, 1] * b[:, 1]))
     return np.clip(np.ceil(np.sqrt(0.75 * dd / tolerance)), 1, _MAX_SUBDIVISIONS).astype(np.int64)
 
 
-def _flatten(cubics, steps, closed_ends):
-    """Points of cubics (M, 4, 2) at t = k/n, k < n, plus t = 1 where closed_ends
-    marks a chain's last cubic (open chains end on their last point)."""
+def _subdivisions_on_device(cubics, tolerance):
+    """_subdivisions of a (N, 4, 2) tensor, evaluated where it lives (FP64, the same
+    operations in the same order); only the counts come back to the host."""
+    p = cubics.detach().double()
+    a = p[:, 0] - 2 * p[:, 1] + p[:, 2]
+    b = p[:, 1] - 2 * p[:, 2] + p[:, 3]
+    dd = torch.maximum(torch.sqrt(a[:, 0] * a[:, 0] + a[:, 1] * a[:, 1]), torch.sqrt(b[:, 0] * b[:, 0] + b[:, 1] * b[:, 1]))
+    if np.ndim(tolerance):
+        tolerance = torch.as_tensor(tolerance, dtype=torch.float64, device=p.device)
+    steps = torch.clamp(torch.ceil(torch.sqrt(0.75 * dd / tolerance)), 1, _MAX_SUBDIVISIONS)
+    return steps.to(torch.int64).cpu().numpy()
+
+
+def _flatten_index(steps, closed_ends, dtype, device):
+    """(cubic of each point, its t) for _flatten, as device tensors."""
     m = steps + closed_ends.astype(np.int64)  # points per cubic
     ci = np.repeat(np.arange(len(steps)), m)
     k = np.arange(int(m.sum())) - np.repeat(np.cumsum(m) - m, m)
-    t = torch.as_tensor(k / np.repeat(steps, m), dtype=cubics.dtype, device=cubics.device)
+    t = torch.as_tensor(k / np.repeat(steps, m), dtype=dtype, device=device)
+    return torch.as_tensor(ci, device=device), t
+
+
+def _flatten(cubics, steps, closed_ends, index=None):
+    """Points of cubics (M, 4, 2) at t = k/n, k < n, plus t = 1 where closed_ends
+    marks a chain's last cubic (open chains end on their last point).  index: the
+    _flatten_index of these steps, when the caller keeps it."""
+    ci, t = index if index is not None else _flatten_index(steps, closed_ends, cubics.dtype, cubics.device)
     w = _bernstein(t)
-    sel = cubics[torch.as_tensor(ci, device=cubics.device)]
+    sel = cubics[ci]
     return (w[:, :, None] * sel).sum(1)
 
 
@@ -247,6 +270,26 @@ def gradient_parts(entry):
 # methods bit for bit.
 
 
+def _joined(tensors):
+    """torch.cat(tensors) along dim 0; when they are contiguous slices laid end to
+    end over all of one tensor (a caller keeping many scenes' parameters in one
+    tensor), that tensor itself, so autograd reaches it without a copy and a
+    split per scene."""
+    if len(tensors) == 1:
+        return tensors[0]
+    base = tensors[0]._base
+    if base is not None and base.is_contiguous() and all(t._base is base and t.is_contiguous() for t in tensors):
+        pos = base.storage_offset()
+        for t in tensors:
+            if t.storage_offset() != pos:
+                break
+            pos += t.numel()
+        else:
+            if pos == base.storage_offset() + base.numel():
+                return base.reshape(-1, *tensors[0].shape[1:])
+    return torch.cat(tensors)
+
+
 def _fill_groups(scenes):
     """Indices of the scenes with fills, grouped by cubics per contour."""
     groups = {}
@@ -261,10 +304,11 @@ def batch_structures(scenes):
     one host copy per cubic count."""
     structs = [Structure() for _ in scenes]
     for c, idx in _fill_groups(scenes).items():
-        fills = torch.cat([scenes[i].fills.detach() for i in idx])
+        fills = _joined([scenes[i].fills for i in idx]).detach()
         sizes = [int(scenes[i].fills.shape[0]) * c for i in idx]
-        tol = np.repeat(np.array([scenes[i].tolerance for i in idx], np.float64), sizes)
-        steps = _subdivisions(_fill_cubics(fills).cpu().numpy(), tol)
+        tols = np.array([scenes[i].tolerance for i in idx], np.float64)
+        tol = float(tols[0]) if (tols == tols[0]).all() else np.repeat(tols, sizes)
+        steps = _subdivisions_on_device(_fill_cubics(fills), tol)
         ends = np.cumsum(sizes)
         for i, lo, hi in zip(idx, ends - sizes, ends):
             structs[i].fill_steps = steps[lo:hi]
@@ -274,19 +318,29 @@ def batch_structures(scenes):
     return structs
 
 
-def batch_polygons(scenes, structs=None):
+def batch_polygons(scenes, structs=None, cache=None):
     """DiffScene.polygons of every scene as one tensor: (vertices (V, 2) of all
     scenes laid end to end in painter order, vertices per draw of each scene, the
-    first vertex of each scene and the total); differentiable in the parameters."""
+    first vertex of each scene and the total); differentiable in the parameters.
+    cache: a dict the caller keeps between calls on the same scenes (diff.Renderer
+    does); the index tensors are built again only when the subdivision counts
+    change."""
     structs = structs or batch_structures(scenes)
+    cache = {} if cache is None else cache
     parts, base = [], 0
     none = np.zeros(0, np.int64)
     part_counts = [[none, none] for _ in scenes]
     first_of = [[none, none] for _ in scenes]
+    signature = [np.array([[id(s.order), len(s.order)] for s in scenes], np.int64)]
     for c, idx in _fill_groups(scenes).items():
-        cub = _fill_cubics(torch.cat([scenes[i].fills for i in idx]))
+        cub = _fill_cubics(_joined([scenes[i].fills for i in idx]))
         steps = np.concatenate([structs[i].fill_steps for i in idx])
-        parts.append(_flatten(cub, steps, np.zeros(len(steps), bool)))
+        signature.append(steps)
+        hit = cache.get(("flatten", c))
+        if hit is None or not np.array_equal(hit[0], steps) or hit[1].device != cub.device or hit[2].dtype != cub.dtype:
+            hit = (steps, *_flatten_index(steps, np.zeros(len(steps), bool), cub.dtype, cub.device))
+            cache[("flatten", c)] = hit
+        parts.append(_flatten(cub, steps, None, index=hit[1:]))
         for i in idx:
             per = structs[i].fill_steps.reshape(scenes[i].fills.shape[0], -1).sum(1)
             part_counts[i][0] = per
@@ -303,24 +357,31 @@ def batch_polygons(scenes, structs=None):
         part_counts[i][1] = per
         first_of[i][1] = base + np.cumsum(per) - per
         base += int(per.sum())
-    gathers, counts, firsts = [], [], [0]
-    for i, s in enumerate(scenes):
-        g, k = s._painter_gather(part_counts[i], first_of[i])
-        gathers.append(g)
-        counts.append(k)
-        firsts.append(firsts[-1] + int(k.sum()))
+        signature.append(per)
     ref = next((t for s in scenes for t in (s.fills, s.strokes) if t is not None), None)
+    hit = cache.get("gather")
+    if (hit is None or len(hit[0]) != len(signature) or not all(np.array_equal(a, b) for a, b in zip(hit[0], signature))
+            or hit[1].device != ref.device):
+        gathers, counts, firsts = [], [], [0]
+        for i, s in enumerate(scenes):
+            g, k = s._painter_gather(part_counts[i], first_of[i])
+            gathers.append(g)
+            counts.append(k)
+            firsts.append(firsts[-1] + int(k.sum()))
+        gather = torch.as_tensor(np.concatenate(gathers) if gathers else none, device=ref.device)
+        hit = ([a.copy() for a in signature], gather, counts, np.asarray(firsts, np.int64))
+        cache["gather"] = hit
+    _, gather, counts, firsts = hit
     if not parts:
-        return torch.zeros((0, 2), dtype=ref.dtype, device=ref.device), counts, np.asarray(firsts, np.int64)
+        return torch.zeros((0, 2), dtype=ref.dtype, device=ref.device), counts, firsts
     allv = torch.cat(parts) if len(parts) > 1 else parts[0]
-    verts = allv.index_select(0, torch.as_tensor(np.concatenate(gathers), device=allv.device))
-    return verts, counts, np.asarray(firsts, np.int64)
+    return allv.index_select(0, gather), counts, firsts
 
 
 def batch_paint_records(scenes):
     """DiffScene.paint_records of every scene laid end to end: one premultiplication
     of all colours and one scatter of all gradient stops; differentiable."""
-    colors = torch.cat([s.colors for s in scenes]) if len(scenes) > 1 else scenes[0].colors
+    colors = _joined([s.colors for s in scenes])
     pm = torch.cat([colors[:, :3] * colors[:, 3:4], colors[:, 3:4]], 1)
     n = pm.shape[0]
     heads, rows, which, slot, stop_list = [], [], [], [], []
@@ -337,7 +398,7 @@ def batch_paint_records(scenes):
     head = np.concatenate(heads) if len(heads) > 1 else heads[0]
     stops = torch.cat([pm, pm.new_zeros((n, 12))], 1)
     if stop_list:
-        st = torch.cat(stop_list).to(device=pm.device, dtype=pm.dtype)
+        st = _joined(stop_list).to(device=pm.device, dtype=pm.dtype)
         spm = torch.cat([st[:, :3] * st[:, 3:4], st[:, 3:4]], 1)
         dev = pm.device
         blocks = pm.new_zeros((g0, 4, 4)).index_put(
@@ -655,8 +716,10 @@ class Plan:
 
     def _index(self, device):
         if self._la_t is None or self._la_t.device != device:
-            self._la_t = torch.as_tensor(self.line_a, device=device)
-            self._lb_t = torch.as_tensor(self.line_b, device=device)
+            src_a = self.line_a if self.line_a is not None else self._la_t
+            src_b = self.line_b if self.line_b is not None else self._lb_t
+            self._la_t = torch.as_tensor(src_a, device=device)
+            self._lb_t = torch.as_tensor(src_b, device=device)
         return self._la_t, self._lb_t
 
     def _clip_points(self, dev):
@@ -692,43 +755,51 @@ class Plan:
         v32 = verts.detach().to(dev, torch.float32)
         v_total = int(firsts[-1])
         clip_pts, clip_first = self._clip_points(dev)
-        # each point's row in [scene vertices | clip points], its vertex (-1: a clip
-        # point), and the vertex count of each polygon
-        rows, p2v, lens = [], [], []
-        for i, (e, k) in enumerate(zip(self.encodings, counts)):
-            k = np.asarray(k, np.int64)
-            offs = firsts[i] + np.concatenate([[0], np.cumsum(k)])
-            for src in e.sources:
-                if src[0] == "clip":
-                    c0, n = clip_first[(i, src[1])]
-                    rows.append(v_total + c0 + np.arange(n))
-                    p2v.append(np.full(n, -1, np.int64))
-                    lens.append(np.array([n], np.int64))
-                else:
-                    lo, hi = src[1], src[2]
-                    r = np.arange(offs[lo], offs[hi])
-                    rows.append(r)
-                    p2v.append(r)
-                    lens.append(k[lo:hi])
-        cat = lambda parts: np.concatenate(parts) if parts else np.zeros(0, np.int64)
-        rows, p2v_np, lens = cat(rows), cat(p2v), cat(lens)
-        points = torch.cat([v32, clip_pts]).index_select(0, torch.as_tensor(rows, device=dev))
-        p2v = torch.as_tensor(p2v_np, device=dev)
+        key = np.concatenate([np.asarray(firsts, np.int64)] + [np.asarray(k, np.int64) for k in counts])
+        hit = getattr(self, "_update_index", None)
+        if hit is None or not np.array_equal(hit[0], key) or hit[1].device != dev:
+            # each point's row in [scene vertices | clip points], its vertex (-1: a
+            # clip point), and the vertex count of each polygon
+            rows, p2v, lens = [], [], []
+            for i, (e, k) in enumerate(zip(self.encodings, counts)):
+                k = np.asarray(k, np.int64)
+                offs = firsts[i] + np.concatenate([[0], np.cumsum(k)])
+                for src in e.sources:
+                    if src[0] == "clip":
+                        c0, n = clip_first[(i, src[1])]
+                        rows.append(v_total + c0 + np.arange(n))
+                        p2v.append(np.full(n, -1, np.int64))
+                        lens.append(np.array([n], np.int64))
+                    else:
+                        lo, hi = src[1], src[2]
+                        r = np.arange(offs[lo], offs[hi])
+                        rows.append(r)
+                        p2v.append(r)
+                        lens.append(k[lo:hi])
+            cat = lambda parts: np.concatenate(parts) if parts else np.zeros(0, np.int64)
+            rows, p2v_np, lens = cat(rows), cat(p2v), cat(lens)
+            rows_t = torch.as_tensor(rows, device=dev)
+            p2v_t = torch.as_tensor(p2v_np, device=dev)
+            lens_t = torch.as_tensor(lens, device=dev)
+            starts = torch.zeros(len(lens) + 1, dtype=torch.int64, device=dev)
+            starts[1:] = torch.cumsum(lens_t, 0)
+            # the point after each one in its polygon
+            path = torch.repeat_interleave(torch.arange(len(lens), device=dev), lens_t, output_size=len(rows))
+            nxt = torch.arange(len(rows), device=dev) + 1
+            nxt = torch.where(nxt == starts[1:][path], starts[:-1][path], nxt)
+            hit = (key, rows_t, p2v_t, starts.to(torch.int32), nxt)
+            self._update_index = hit
+        _, rows_t, p2v, starts32, nxt = hit
+        points = torch.cat([v32, clip_pts]).index_select(0, rows_t)
         v_off = v_total
-        lens_t = torch.as_tensor(lens, device=dev)
-        starts = torch.zeros(len(lens) + 1, dtype=torch.int64, device=dev)
-        starts[1:] = torch.cumsum(lens_t, 0)
-        lv = self.frame.set_polygons(points, starts.to(torch.int32), line_vertex=True).long()
-        # the vertex after each one in its polygon
-        path = torch.repeat_interleave(torch.arange(len(lens), device=dev), lens_t)
-        nxt = torch.arange(len(points), device=dev) + 1
-        nxt = torch.where(nxt == starts[1:][path], starts[:-1][path], nxt)
+        lv = self.frame.set_polygons(points, starts32, line_vertex=True).long()
         la, lb = p2v[lv], p2v[nxt[lv]]
-        la[la < 0] = v_off  # path clip lines: the gradient sink past the vertices
-        lb[lb < 0] = v_off
+        # path clip lines: the gradient sink past the vertices
+        la = torch.where(la < 0, v_off, la)
+        lb = torch.where(lb < 0, v_off, lb)
         self.vertex_count = v_off
         self._la_t, self._lb_t = la, lb
-        self.line_a, self.line_b = la.cpu().numpy(), lb.cpu().numpy()
+        self.line_a = self.line_b = None  # host copies made on demand (_index)
         self.line_count = len(la)
 
     def full_paint(self, records):
@@ -831,15 +902,17 @@ class Renderer:
         self.scenes = [scenes] if self.single else list(scenes)
         self.plan = None
         self.sync = sync  # Plan's: False orders the native calls on streams only
+        self._cache = {}  # batch_polygons' index tensors
 
     def __call__(self):
-        verts, counts, firsts = batch_polygons(self.scenes)
+        verts, counts, firsts = batch_polygons(self.scenes, cache=self._cache)
         if self.plan is None:
             self.plan = Plan(self.context, self.scenes, [verts[a:b] for a, b in zip(firsts[:-1], firsts[1:])], counts,
                              sync=self.sync)
         else:
             self.plan.update(verts, counts, firsts)
         flat = _Render.apply(verts, self.plan.full_paint(batch_paint_records(self.scenes)), self.plan, self.device)
+        self.flat = flat  # all scenes' pixels in the plan's layout (Plan.images), differentiable
         images = self.plan.images(flat)
         return images[0] if self.single else images
 
