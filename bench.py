@@ -508,6 +508,13 @@ def main():
     fwd_bwd = diff_batch(ctx) if not args.no_diff else None
     if fwd_bwd:
         log(f"differentiable fwd+bwd: {fwd_bwd}")
+    fitting = None
+    if args.fit_steps and not args.no_diff:
+        fitting = fit(ctx, steps=args.fit_steps)
+        curve = fitting.pop("loss")
+        # the loss every 10 steps and at the last step
+        fitting["loss_curve"] = {str(i): curve[i] for i in sorted(set(range(0, len(curve), 10)) | {len(curve) - 1})}
+        log(f"fitting: {fitting}")
     result = {
         "metric": "ms/frame & Gpix/s at 4096², 200k paths; batched fwd+bwd scenes/s vs plain",
         "value": round(gpix, 4),
@@ -533,6 +540,7 @@ def main():
             "tuning": tuning,
             "scene_loop": loop,
             "diff_fwd_bwd": fwd_bwd,
+            "fit": fitting,
         },
         "roofline": {"bound": "hbm", "kernel": "tk_k_fine", "achieved": round(fine_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(fine_gbs / HBM_PEAK_GBS, 5), "traffic": measured_traffic("tk_k_fine"),

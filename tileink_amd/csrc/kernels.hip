@@ -690,17 +690,6 @@ __device__ __forceinline__ LaneRun lane_run(uint32_t index) {
     return r;
 }
 
-// Claims one slot of cursors[index] per lane; the lanes of a run receive
-// consecutive slots from one atomic.
-__device__ __forceinline__ uint32_t wave_claim_slot_runs(uint32_t *cursors, uint32_t index) {
-    const uint32_t lane = __lane_id();
-    const LaneRun r = lane_run(index);
-    uint32_t first = 0u;
-    if (r.head == lane) first = atomicAdd(&cursors[index], r.length);
-    first = (uint32_t)__shfl((int)first, (int)r.head, 64);
-    return first + (lane - r.head);
-}
-
 // Adds +1 or -1 per lane to words[index], one atomic per run of equal indices.
 __device__ __forceinline__ void wave_add_pm1_runs(uint32_t *words, uint32_t index, bool plus) {
     const uint32_t lane = __lane_id();
@@ -895,24 +884,110 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_scan_chunk_apply(tk_args_
     apply_scan_chunk(a, c, lane, base);
 }
 
-// scan_emit (scan/emit.hlsl:14-28).
+// scan_emit (scan/emit.hlsl:14-28): each segment takes a slot of its (path, tile)
+// range from that range's cursor.  The reference claims one slot per segment with
+// an atomic; here a workgroup first gathers its segments: each goes into a
+// staging area in LDS with its (path, tile) key, whose entry in an LDS key table
+// counts the workgroup's segments of that key and gives the segment its rank.
+// Then one cursor atomic per key claims the key's whole run of slots, and the
+// staged segments are written to run start + rank.  The slots a range receives,
+// and so its multiset of segments, are the reference's; a workgroup's segments
+// of one tile land contiguous (written together, so L2 merges their lines), and
+// the cursor atomics drop to one per (workgroup, key).  A segment that finds the
+// staging area or its key's probe window full claims its slot on its own.
+constexpr uint32_t kEmitNoKey = 0xFFFFFFFFu;
+constexpr uint32_t kEmitProbes = 16u;
 extern "C" __global__ __launch_bounds__(1024) void tk_k_scan_emit(tk_args_scan_lines a) {
+    extern __shared__ uint32_t emit_lds[];
+    const uint32_t H = a.hash_slots, S = a.stage_slots;
+    uint32_t *hkey = emit_lds, *hcount = emit_lds + H, *hbase = emit_lds + 2u * H, *stage_n = emit_lds + 3u * H;
+    // 8 words per staged segment, two 16-byte LDS accesses: p0x p0y p1x p1y | y_edge key-slot rank -
+    uint4 *stage = reinterpret_cast<uint4 *>(emit_lds + 3u * H + 4u);
+    for (uint32_t h = threadIdx.x; h < H; h += blockDim.x) {
+        hkey[h] = kEmitNoKey;
+        hcount[h] = 0u;
+    }
+    if (threadIdx.x == 0u) *stage_n = 0u;
+    __syncthreads();
+
     const uint32_t block = xcd_chunked_block(blockIdx.x, a.xcd_run);
-    uint32_t id = block * blockDim.x + threadIdx.x;  // lines per workgroup: a launch parameter
-    if (id >= a.cfg.line_count) return;
-    if (a.path_active) {  // incremental: only paths whose tile box meets the damage
+    const uint32_t id = block * blockDim.x + threadIdx.x;  // lines per workgroup: a launch parameter
+    bool live = id < a.cfg.line_count;
+    if (live && a.path_active) {  // incremental: only paths whose tile box meets the damage
         const uint32_t path = a.lines[id].path_id;
-        if (path < a.path_record_count && a.path_active[path] == 0u) return;
+        if (path < a.path_record_count && a.path_active[path] == 0u) live = false;
     }
     ScanTraversal s;
-    if (!scan_geometry(a.lines, a.paths, a.path_record_count, id, s)) return;
-    for (uint32_t i = s.imin; i < s.imax; i++) {
-        float z = floorf(s.a * (float)i + s.b);
-        int32_t tx, ty;
-        scan_tile(s, i, z, tx, ty);
-        if (scan_tile_inside(s, tx, ty)) {
-            uint32_t dst = wave_claim_slot_runs(a.counts_or_cursors, scan_tile_offset(s, tx, ty));
-            if (dst < a.cfg.segment_capacity) a.segments[dst] = clipped_segment(s, i, z, tx, ty);
+    if (live) live = scan_geometry(a.lines, a.paths, a.path_record_count, id, s);
+    uint32_t i = live ? s.imin : 0u;
+    const uint32_t iend = live ? s.imax : 0u;
+    while (__ballot(i < iend)) {  // the wave steps together; a lane past its tiles idles
+        bool have = false;
+        uint32_t key = 0u;
+        tk_line_segment seg{};
+        if (i < iend) {
+            const float z = floorf(s.a * (float)i + s.b);
+            int32_t tx, ty;
+            scan_tile(s, i, z, tx, ty);
+            if (scan_tile_inside(s, tx, ty)) {
+                have = true;
+                key = scan_tile_offset(s, tx, ty);
+                seg = clipped_segment(s, i, z, tx, ty);
+            }
+            ++i;
+        }
+        // this step's staging positions: one LDS atomic per wave
+        const unsigned long long m = __ballot(have);
+        uint32_t first = 0u;
+        if (m) {
+            const uint32_t leader = (uint32_t)__ffsll(m) - 1u;
+            if (__lane_id() == leader) first = atomicAdd(stage_n, (uint32_t)__popcll(m));
+            first = (uint32_t)__shfl((int)first, (int)leader, 64);
+        }
+        bool staged = false;
+        if (have) {
+            const uint32_t pos = first + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (pos < S) {
+                uint32_t h = (key * 2654435761u) & (H - 1u), slot = kEmitNoKey;
+                for (uint32_t probe = 0; probe < kEmitProbes; ++probe, h = (h + 1u) & (H - 1u)) {
+                    const uint32_t old = atomicCAS(&hkey[h], kEmitNoKey, key);
+                    if (old == kEmitNoKey || old == key) {
+                        slot = h;
+                        break;
+                    }
+                }
+                uint32_t rank = 0u;
+                if (slot != kEmitNoKey) {
+                    stage[2u * pos] = make_uint4(__float_as_uint(seg.p0x), __float_as_uint(seg.p0y),
+                                                 __float_as_uint(seg.p1x), __float_as_uint(seg.p1y));
+                    rank = atomicAdd(&hcount[slot], 1u);
+                    staged = true;
+                }
+                // slot kEmitNoKey: a hole, skipped below
+                stage[2u * pos + 1u] = make_uint4(__float_as_uint(seg.y_edge), slot, rank, 0u);
+            }
+        }
+        if (have && !staged) {
+            const uint32_t dst = atomicAdd(&a.counts_or_cursors[key], 1u);
+            if (dst < a.cfg.segment_capacity) a.segments[dst] = seg;
+        }
+    }
+    __syncthreads();
+    // one claim per key: the key's run of slots
+    for (uint32_t h = threadIdx.x; h < H; h += blockDim.x) {
+        const uint32_t n = hcount[h];
+        if (n) hbase[h] = atomicAdd(&a.counts_or_cursors[hkey[h]], n);
+    }
+    __syncthreads();
+    const uint32_t staged_total = min(*stage_n, S);
+    for (uint32_t e = threadIdx.x; e < staged_total; e += blockDim.x) {
+        const uint4 tail = stage[2u * e + 1u];
+        if (tail.y == kEmitNoKey) continue;
+        const uint32_t dst = hbase[tail.y] + tail.z;
+        if (dst < a.cfg.segment_capacity) {
+            const uint4 head = stage[2u * e];
+            a.segments[dst] = tk_line_segment{__uint_as_float(head.x), __uint_as_float(head.y), __uint_as_float(head.z),
+                                              __uint_as_float(head.w), __uint_as_float(tail.x)};
         }
     }
 }

@@ -199,7 +199,7 @@ tk_status run_scan_count(tk_context *ctx, const tk_scan_config &cfg, const tk_li
                          uint32_t path_record_count, int32_t *backdrops, uint32_t *counts,
                          const uint32_t *path_active = nullptr, uint32_t xcd_run = TK_XCD_CHUNK_SCAN,
                          uint32_t block = 256u) {
-    tk_args_scan_lines a{cfg, lines, paths, path_record_count, backdrops, counts, nullptr, path_active, xcd_run};
+    tk_args_scan_lines a{cfg, lines, paths, path_record_count, backdrops, counts, nullptr, path_active, xcd_run, 0u, 0u};
     return launch(ctx, ctx->f_scan_count, xcd_grid(div_up(cfg.line_count, block), xcd_run), 1, block, a);
 }
 
@@ -207,8 +207,15 @@ tk_status run_scan_emit(tk_context *ctx, const tk_scan_config &cfg, const tk_lin
                         uint32_t path_record_count, uint32_t *cursors, tk_line_segment *segments,
                         const uint32_t *path_active = nullptr, uint32_t xcd_run = TK_XCD_CHUNK_SCAN,
                         uint32_t block = 256u) {
-    tk_args_scan_lines a{cfg, lines, paths, path_record_count, nullptr, cursors, segments, path_active, xcd_run};
-    return launch(ctx, ctx->f_scan_emit, xcd_grid(div_up(cfg.line_count, block), xcd_run), 1, block, a);
+    // The workgroup's key table (12 B an entry) and staging area (32 B), in dynamic
+    // LDS of at most 60 KiB: twice as many entries as lines up to 512 lines,
+    // 1024 and 1536 at 1024.
+    const uint32_t hash_slots = block <= 512u ? 2u * block : 1024u;
+    const uint32_t stage_slots = block <= 512u ? 2u * block : 1536u;
+    tk_args_scan_lines a{cfg,     lines,   paths,   path_record_count, nullptr,   cursors,
+                         segments, path_active, xcd_run, hash_slots,    stage_slots};
+    const uint32_t lds = (3u * hash_slots + 4u + 8u * stage_slots) * 4u;
+    return launch(ctx, ctx->f_scan_emit, xcd_grid(div_up(cfg.line_count, block), xcd_run), 1, block, a, lds);
 }
 
 tk_status run_scan_prefix(tk_context *ctx, const tk_scan_config &cfg, const tk_scan_chunk *chunks,

@@ -55,6 +55,8 @@ struct tk_args_scan_lines {
     tk_line_segment *segments;   // emit only
     const uint32_t *path_active; // incremental: lines of paths whose flag is 0 are skipped; null: all
     uint32_t xcd_run;            // XCD run length in blocks (a power of two; 0: launch order)
+    uint32_t hash_slots;         // emit: workgroup key table entries (a power of two), in dynamic LDS
+    uint32_t stage_slots;        // emit: staged segments per workgroup, in dynamic LDS
 };
 
 // A batch of independent scenes rendered as one frame: each scene keeps its own
