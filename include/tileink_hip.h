@@ -91,6 +91,10 @@ uint32_t tk_version(void);
 #define TK_BRUSH_RADIAL 3u
 #define TK_BRUSH_SWEEP 4u
 #define TK_BRUSH_FOUR_CORNER 5u
+#define TK_BRUSH_PATTERN 6u            /* a pattern without an image: transparent */
+#define TK_BRUSH_IMAGE 7u              /* BRUSH_PATTERN_RESOURCE: an image of the scene's atlas */
+#define TK_PATTERN_NEAREST 0u
+#define TK_PATTERN_BILINEAR 1u
 #define TK_BRUSH_HEADER_WORDS 9u
 #define TK_BRUSH_PARAM_WORDS 12u
 #define TK_SDF_KIND_RECT 1u
@@ -190,6 +194,15 @@ typedef struct tk_scene {
     uint32_t segment_capacity;       /* S bound = sum of path segment_capacity */
     uint32_t max_clip_depth;         /* deepest fused clip stack of any batch */
     uint64_t ptcl_capacity;          /* PTCL records bound over all batches */
+
+    /* The image atlas image brushes sample (the Texture2DArray the reference
+     * samples, src/shaders/hlsl/shared/brush/pattern.hlsli:18-55): atlas_pages
+     * pages of atlas_width x atlas_height premultiplied RGBA8 texels, page-major,
+     * then row-major; an image brush's header holds its placement (x, y, page,
+     * width, height in words 2-6, as resource_brush_header patches it,
+     * src/shared/gpu_brush.rs:92-127).  NULL with zero pages when no draw has one. */
+    const uint32_t *atlas;
+    uint32_t atlas_width, atlas_height, atlas_pages, atlas_reserved;
 } tk_scene;
 
 /* ------------------------------------------------------------------------- */
@@ -222,6 +235,14 @@ typedef struct tk_brush {
     float bounds[4];                 /* four-corner: x0, y0, x1, y1 (user space) */
     float corners[4][4];             /* four-corner: top-left, top-right, bottom-right,
                                         bottom-left, straight-alpha sRGB */
+    /* image (TK_BRUSH_IMAGE, peniko's image brush as PatternBrush, src/shared/brush.rs:
+     * 520-583): an image of the canvas (tk_canvas_add_image), the transform taking
+     * user space to the image's unit square [a b c d e f] (x' = a x + c y + e,
+     * y' = b x + d y + f), TK_PATTERN_NEAREST or _BILINEAR, opacity 0-255;
+     * extend as above */
+    uint32_t image;
+    float pattern_transform[6];
+    uint32_t sampling, opacity;
 } tk_brush;
 
 tk_status tk_canvas_create(uint32_t width, uint32_t height, double scale, tk_canvas **out);
@@ -266,6 +287,16 @@ tk_status tk_canvas_push_stroke_styled(tk_canvas *canvas, const uint8_t *verbs, 
 /* Canvas::push_rect (src/canvas/primitives/rectangles.rs:33-43); radii = tl,tr,bl,br. */
 tk_status tk_canvas_push_rect(tk_canvas *canvas, double x0, double y0, double x1, double y1,
                               const float radii[4], const tk_brush *brush, uint32_t *draw_id);
+/* Canvas::register_scene_image (src/canvas/primitives/images.rs:44-54): an image of
+ * premultiplied RGBA8 pixels (Image, src/shared/image.rs:9-19; row-major, red in the
+ * low byte), copied; *image_id names it in brushes.  A zero-sized image is refused. */
+tk_status tk_canvas_add_image(tk_canvas *canvas, uint32_t width, uint32_t height, const uint32_t *premul_rgba8,
+                              uint32_t *image_id);
+/* Canvas::push_image (src/canvas/primitives/images.rs:9-26): the image scaled into the
+ * rect (PatternBrush::for_rect_resource, src/shared/brush.rs:557-583) as an SDF rect
+ * draw.  An empty or non-finite rect records nothing (*draw_id = TK_INVALID_INDEX). */
+tk_status tk_canvas_push_image(tk_canvas *canvas, double x0, double y0, double x1, double y1, uint32_t image,
+                               uint32_t extend, uint32_t sampling, uint32_t *draw_id);
 /* Canvas::push_clip_sdf_rect_layer (src/canvas/layers.rs:34-40). */
 tk_status tk_canvas_push_clip_sdf_rect_layer(tk_canvas *canvas, double x0, double y0, double x1,
                                              double y1, const float radii[4]);
@@ -659,6 +690,8 @@ typedef struct tk_fine_inputs {
     const tk_ptcl_record *ptcl;
     uint32_t *status;  /* optional; bit 1 set when a tile's clip stack went deeper than
                           4 + clip_spill_depth levels (the extra levels were dropped) */
+    const uint32_t *atlas;  /* tk_scene's image atlas (NULL: image brushes sample 0) */
+    uint32_t atlas_width, atlas_height, atlas_pages, atlas_reserved;
 } tk_fine_inputs;
 tk_status tk_fine(tk_context *ctx, const tk_fine_config *config, const tk_fine_inputs *in,
                   uint32_t *rgba_out);

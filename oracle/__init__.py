@@ -76,6 +76,7 @@ class Scene(C.Structure):
         ("cumsum_chunk_offsets", C.c_void_p), ("cumsum_chunk_lens", C.c_void_p), ("cumsum_chunk_count", C.c_uint32),
         ("cumsum_row_starts", C.c_void_p), ("cumsum_row_ends", C.c_void_p), ("cumsum_row_count", C.c_uint32),
         ("backdrop_len", C.c_uint32), ("segment_capacity", C.c_uint32), ("max_clip_depth", C.c_uint32),
+        ("atlas", C.c_void_p), ("atlas_width", C.c_uint32), ("atlas_height", C.c_uint32), ("atlas_pages", C.c_uint32),
     ]
 
 
@@ -100,7 +101,7 @@ def threads():
 
 ARRAY_FIELDS = ("lines", "paths", "draws", "paint", "draw_batch_ids", "layers", "batches", "tile_draws",
                 "tile_draw_indices", "scan_chunks", "scan_chunk_ranges", "cumsum_chunk_offsets", "cumsum_chunk_lens",
-                "cumsum_row_starts", "cumsum_row_ends")
+                "cumsum_row_starts", "cumsum_row_ends", "atlas")
 
 
 def _scene_struct(scene):

@@ -858,7 +858,106 @@ static uint32_t sample_ramp(const uint32_t *paint, uint32_t n, uint32_t bb, uint
     if (fraction <= 0.00000011920929f || li == ri) return left;
     return lerp_premul_u8(left, right, fraction);
 }
+/* Image brushes: sample_resource_pattern_atlas and sample_atlas_brush
+ * (shared/brush/pattern.hlsli:9-55) over the scene's atlas, with
+ * euclidean_remainder_i32 (shared/integer.hlsli:5-9) and
+ * pattern_transform_component (shared/pattern_transform.hlsli:7-11).  The padded
+ * bilinear case is the reference's hardware sampler (SampleLevel): position
+ * clamped to the image, neighbours clamped to it, the position resolved to 8
+ * fractional bits (weights k / 256, the subtexel precision Direct3D requires) and
+ * each 0-255 channel blended in integers and rounded once. */
+typedef struct { const uint32_t *texels; uint32_t width, height, pages; } Atlas;
+static uint32_t euclid_rem_i32(int32_t value, uint32_t period) {
+    uint32_t magnitude = value < 0 ? 0u - (uint32_t)value : (uint32_t)value;
+    uint32_t remainder = magnitude % period;
+    return value < 0 && remainder != 0u ? period - remainder : remainder;
+}
+static uint32_t extend_coord(int32_t value, uint32_t size, uint32_t extend) {
+    if (size <= 1u) return 0u;
+    if (extend == 1u || extend == 2u) {
+        uint32_t period = size * (extend == 2u ? 2u : 1u);
+        uint32_t coord = euclid_rem_i32(value, period);
+        return coord < size ? coord : period - coord - 1u;
+    }
+    int32_t hi = (int32_t)size - 1;
+    return (uint32_t)(value < 0 ? 0 : (value > hi ? hi : value));
+}
+static uint32_t atlas_texel(const Atlas *a, uint32_t page, uint32_t x, uint32_t y) {
+    if (!a || !a->texels || page >= a->pages || x >= a->width || y >= a->height) return 0u;
+    return a->texels[((size_t)page * a->height + y) * a->width + x];
+}
+static uint32_t atlas_pixel(const Atlas *a, uint32_t page, uint32_t ox, uint32_t oy, uint32_t sx, uint32_t sy,
+                            uint32_t extend, int32_t px, int32_t py) {
+    return atlas_texel(a, page, ox + extend_coord(px, sx, extend), oy + extend_coord(py, sy, extend));
+}
+static float pattern_component(float a, float b, float offset, float x, float y) {
+    float by = b * y;
+    float dot = fmaf(a, x, by) + fmaf(b, y, -by);
+    return dot + offset;
+}
+/* The texture unit's 8-bit subtexel precision: position q split into a texel
+ * index and a weight k / 256, the fraction rounded to the nearest 1/256. */
+static void subtexel_position(float q, int32_t *index, uint32_t *weight) {
+    float base = floorf(q);
+    *weight = (uint32_t)rintf((q - base) * 256.0f);
+    *index = f2i(base);
+    if (*weight == 256u) {
+        *index += 1;
+        *weight = 0u;
+    }
+}
+static uint32_t sample_image(const uint32_t *paint, uint32_t n, uint32_t bb, uint32_t db, const Atlas *atlas, float x,
+                             float y) {
+    uint32_t sx = blob_word(paint, n, bb, db + 5u), sy = blob_word(paint, n, bb, db + 6u);
+    if (sx == 0u || sy == 0u) return 0u;
+    uint32_t page = blob_word(paint, n, bb, db + 4u);
+    if (page & 0x80000000u) return 0u; /* a texture-table image: only the atlas is modelled */
+    uint32_t ox = blob_word(paint, n, bb, db + 2u), oy = blob_word(paint, n, bb, db + 3u);
+    uint32_t opacity = blob_word(paint, n, bb, db + 7u), extend = blob_word(paint, n, bb, db + 1u);
+    uint32_t sampling = blob_word(paint, n, bb, db + 8u);
+    float p[6];
+    for (uint32_t i = 0; i < 6u; ++i) p[i] = blob_float(paint, n, bb, db + 9u + i);
+    float px = pattern_component(p[0], p[2], p[4], x, y) * (float)sx;
+    float py = pattern_component(p[1], p[3], p[5], x, y) * (float)sy;
+    uint32_t color;
+    if (sampling == 1u && extend == 0u) {
+        int32_t ix, iy;
+        uint32_t wx, wy;
+        subtexel_position(clampf(px, 0.0f, (float)sx) - 0.5f, &ix, &wx);
+        subtexel_position(clampf(py, 0.0f, (float)sy) - 0.5f, &iy, &wy);
+        uint32_t t[4] = {atlas_pixel(atlas, page, ox, oy, sx, sy, 0u, ix, iy),
+                         atlas_pixel(atlas, page, ox, oy, sx, sy, 0u, ix + 1, iy),
+                         atlas_pixel(atlas, page, ox, oy, sx, sy, 0u, ix, iy + 1),
+                         atlas_pixel(atlas, page, ox, oy, sx, sy, 0u, ix + 1, iy + 1)};
+        uint32_t w[4] = {(256u - wx) * (256u - wy), wx * (256u - wy), (256u - wx) * wy, wx * wy};
+        color = 0u;
+        for (int k = 0; k < 4; ++k) {
+            uint32_t sum = 32768u;
+            for (int n = 0; n < 4; ++n) sum += w[n] * ((t[n] >> (8 * k)) & 255u);
+            color |= (sum >> 16) << (8 * k);
+        }
+    } else if (sampling == 1u) {
+        float qx = px - 0.5f, qy = py - 0.5f;
+        float bx = floorf(qx), by = floorf(qy), fx = qx - bx, fy = qy - by;
+        int32_t ix = f2i(bx), iy = f2i(by);
+        uint32_t tl = atlas_pixel(atlas, page, ox, oy, sx, sy, extend, ix, iy);
+        uint32_t tr = atlas_pixel(atlas, page, ox, oy, sx, sy, extend, ix + 1, iy);
+        uint32_t bl = atlas_pixel(atlas, page, ox, oy, sx, sy, extend, ix, iy + 1);
+        uint32_t br = atlas_pixel(atlas, page, ox, oy, sx, sy, extend, ix + 1, iy + 1);
+        color = lerp_premul_u8(lerp_premul_u8(tl, tr, fx), lerp_premul_u8(bl, br, fx), fy);
+    } else {
+        color = atlas_pixel(atlas, page, ox, oy, sx, sy, extend, f2i(floorf(px)), f2i(floorf(py)));
+    }
+    return scale_premul_u8(color, opacity);
+}
+
+static uint32_t sample_brush_atlas(const uint32_t *paint, uint32_t n, uint32_t bb, uint32_t db, float x, float y,
+                                   const Atlas *atlas);
 static uint32_t sample_brush(const uint32_t *paint, uint32_t n, uint32_t bb, uint32_t db, float x, float y) {
+    return sample_brush_atlas(paint, n, bb, db, x, y, NULL);
+}
+static uint32_t sample_brush_atlas(const uint32_t *paint, uint32_t n, uint32_t bb, uint32_t db, float x, float y,
+                                   const Atlas *atlas) {
     uint32_t kind = blob_word(paint, n, bb, db);
     uint32_t extend = blob_word(paint, n, bb, db + 1u);
     uint32_t payload = db + blob_word(paint, n, bb, db + 2u);
@@ -925,6 +1024,7 @@ static uint32_t sample_brush(const uint32_t *paint, uint32_t n, uint32_t bb, uin
         return lerp_premul_u8(lerp_premul_u8(tl, tr, u), lerp_premul_u8(bl, br, u), v);
     }
     if (kind == 6u) return 0u;
+    if (kind == 7u) return sample_image(paint, n, bb, db, atlas, x, y);
     return color;
 }
 
@@ -1486,7 +1586,8 @@ static uint32_t draw_brush(const or_fine_config *cfg, const uint32_t *paint, con
         return blob_word(paint, cfg->paint_words, cfg->paint_brush_base, d->brush_offset + 4u);
     float lx, ly;
     aff_point(&d->inv, x, y, &lx, &ly);
-    return sample_brush(paint, cfg->paint_words, cfg->paint_brush_base, d->brush_offset, lx, ly);
+    const Atlas atlas = {cfg->atlas, cfg->atlas_width, cfg->atlas_height, cfg->atlas_pages};
+    return sample_brush_atlas(paint, cfg->paint_words, cfg->paint_brush_base, d->brush_offset, lx, ly, &atlas);
 }
 static float draw_sdf_coverage(const or_fine_config *cfg, const uint32_t *paint, const FDraw *d, float x, float y) {
     float lx, ly;
@@ -1524,8 +1625,10 @@ static Px fine_pixel(const or_fine_config *cfg, const uint32_t *draws, const uin
             if (r[0] == 0u) break;
             if (r[0] == 2u) {
                 pixel = premul_opaque(r[5]) ? rgba8_to_unorm(r[5]) : src_over_premul_unorm(pixel, rgba8_to_unorm(r[5]));
-            } else if (r[0] == 13u) {
-                /* image brushes are not produced on this path */
+            } else if (r[0] == 13u) { /* PTCL_IMAGE (fine/specialized.hlsli:84-91) */
+                FDraw d = load_fdraw(draws, cfg->draw_count, r[5]);
+                uint32_t color = draw_brush(cfg, paint, &d, sx, sy);
+                pixel = premul_opaque(color) ? rgba8_to_unorm(color) : src_over_premul_unorm(pixel, rgba8_to_unorm(color));
             } else {
                 FDraw d = load_fdraw(draws, cfg->draw_count, r[5]);
                 float coverage = 0.0f;
@@ -1551,6 +1654,13 @@ static Px fine_pixel(const or_fine_config *cfg, const uint32_t *draws, const uin
             if (clip_mask != 0u) {
                 if (clip_mask == 255u && premul_opaque(r[5])) pixel = rgba8_to_unorm(r[5]);
                 else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(r[5], clip_mask));
+            }
+        } else if (tag == 13u) { /* PTCL_IMAGE (fine/interpreter.hlsli:65-74) */
+            if (clip_mask != 0u) {
+                FDraw d = load_fdraw(draws, cfg->draw_count, r[5]);
+                uint32_t color = draw_brush(cfg, paint, &d, sx, sy);
+                if (clip_mask == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
+                else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, clip_mask));
             }
         } else if (tag == 9u) {
             if (clip_mask != 0u) {
@@ -1754,6 +1864,10 @@ int or_render_frame(const or_scene *s, uint32_t *image, or_frame_outputs *out) {
         fc.paint_brush_base = s->paint_brush_base;
         fc.draw_count = s->draw_count;
         fc.ptcl_base_words = T * 6u;
+        fc.atlas = s->atlas;
+        fc.atlas_width = s->atlas_width;
+        fc.atlas_height = s->atlas_height;
+        fc.atlas_pages = s->atlas ? s->atlas_pages : 0u;
         double tf = now_ms();
         or_fine(&fc, s->draws, s->paint, segments, work, kinds, image);
         ms[5] += now_ms() - tf;

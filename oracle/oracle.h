@@ -71,6 +71,9 @@ typedef struct or_fine_config {
     uint32_t width, height, tile_count, tiles_width, tiles_height;
     uint32_t clip_spill_depth, paint_words, paint_sdf_shadow_base, paint_brush_base, draw_count;
     uint32_t ptcl_base_words; /* word offset of the PTCL array inside `work` */
+    /* the image atlas image brushes sample: pages x height x width premultiplied
+     * RGBA8 texels (NULL: image brushes sample 0) */
+    const uint32_t *atlas; uint32_t atlas_width, atlas_height, atlas_pages;
 } or_fine_config;
 void or_fine(const or_fine_config *cfg, const uint32_t *draws, const uint32_t *paint, const uint32_t *segments,
              const uint32_t *work, const uint32_t *kinds, uint32_t *target);
@@ -91,6 +94,7 @@ typedef struct or_scene {
     const uint32_t *cumsum_chunk_offsets, *cumsum_chunk_lens; uint32_t cumsum_chunk_count;
     const uint32_t *cumsum_row_starts, *cumsum_row_ends; uint32_t cumsum_row_count;
     uint32_t backdrop_len, segment_capacity, max_clip_depth;
+    const uint32_t *atlas; uint32_t atlas_width, atlas_height, atlas_pages;
 } or_scene;
 
 /* Optional outputs of or_render_frame; every pointer may be NULL.

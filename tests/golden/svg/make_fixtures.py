@@ -1,7 +1,9 @@
 """Copies the reference's SVG rendering fixtures that tileink_amd.svg takes into
 this folder: each test SVG (the input) with its `.native.png` (the image the
 reference's own native renderer produced for it at 300 px width,
-examples/svg_fixture_render.rs:62-77).  Both are data: the tests read them as
+examples/svg_fixture_render.rs:62-77), and the raster files image fixtures name
+by relative path, kept at that relative place (../../../resources from
+structure/image lands in tests/golden/resources).  Both are data: the tests read them as
 inputs and expected outputs.  Run from the repository root with the reference
 checked out at /root/reference:
 
@@ -9,6 +11,7 @@ checked out at /root/reference:
 """
 import json
 import os
+import re
 import shutil
 import sys
 
@@ -21,7 +24,8 @@ from tileink_amd import svg  # noqa: E402
 SOURCE = "/root/reference/src/svg/tests"
 FOLDERS = ["shapes/circle", "shapes/ellipse", "shapes/line", "shapes/path", "shapes/polygon", "shapes/polyline",
            "shapes/rect", "painting/fill-rule", "painting/stroke-linejoin", "painting/stroke-linecap",
-           "painting/stroke-width", "painting/stroke-miterlimit"]
+           "painting/stroke-width", "painting/stroke-miterlimit", "structure/image"]
+HREF = re.compile(r'href="([^"]*)"')
 
 
 def main():
@@ -37,12 +41,21 @@ def main():
             with open(os.path.join(src, name)) as f:
                 text = f.read()
             try:
-                svg.load(text, 300)
+                svg.load(text, 300, base_dir=src)
             except svg.SvgUnsupported as e:
                 skipped.append(f"{folder}/{name}: {e}")
                 continue
             dst = os.path.join(HERE, folder)
             os.makedirs(dst, exist_ok=True)
+            # image files a fixture names by relative path, at the same relative place
+            for href in HREF.findall(text):
+                if href.startswith("data:") or "://" in href:
+                    continue
+                source = os.path.normpath(os.path.join(src, href))
+                if os.path.isfile(source):
+                    target = os.path.normpath(os.path.join(dst, href))
+                    os.makedirs(os.path.dirname(target), exist_ok=True)
+                    shutil.copyfile(source, target)
             shutil.copyfile(os.path.join(src, name), os.path.join(dst, name))
             shutil.copyfile(os.path.join(src, gold), os.path.join(dst, gold))
             taken.append(f"{folder}/{name}")

@@ -28,6 +28,12 @@ def read_svg(name):
         return f.read()
 
 
+def load_fixture(name, target_width=300):
+    """A fixture recorded on a Canvas; its relative image hrefs resolve beside it."""
+    from tileink_amd import svg
+    return svg.load(read_svg(name), target_width, base_dir=os.path.dirname(os.path.join(GOLDEN, name)))
+
+
 def golden_premultiplied(name):
     """The golden PNG (straight RGBA8) as premultiplied RGBA8 bytes (H, W, 4)."""
     from PIL import Image

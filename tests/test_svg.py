@@ -54,7 +54,7 @@ def test_unsupported_content_is_refused():
 
 @pytest.mark.parametrize("name", svg_golden.fixtures())
 def test_oracle_render_matches_reference_golden(name):
-    canvas = svg.load(svg_golden.read_svg(name), 300)
+    canvas = svg_golden.load_fixture(name)
     image, _ = oracle.render_frame(canvas.arrays(0))
     svg_golden.assert_close(image, name)
 
@@ -62,6 +62,74 @@ def test_oracle_render_matches_reference_golden(name):
 def test_tile_edge_area_pixel_kat():
     # native_tile_edge_area_preserves_half_channel_rounding (renderer_rounding_gpu.rs:76-88):
     # shapes/line/simple-case.svg at 300 px, pixel (103, 132) is exactly [63, 71, 0, 205].
-    canvas = svg.load(svg_golden.read_svg("shapes/line/simple-case.svg"), 300)
+    canvas = svg_golden.load_fixture("shapes/line/simple-case.svg")
     image, _ = oracle.render_frame(canvas.arrays(0))
     assert list(image.view(np.uint8).reshape(300, 300, 4)[132, 103]) == [63, 71, 0, 205]
+
+
+def test_pattern_transform_pixel_kat():
+    # native_pattern_transform_preserves_product_residual (renderer_rounding_gpu.rs:90-99):
+    # structure/image/with-transform.svg at 300 px, pixel (128, 135) is exactly
+    # [28, 28, 28, 255] (straight alpha; opaque here).  The image is skewed and
+    # scaled, sampled bilinear with pad: the pattern transform's fused residual
+    # and the sampler's 1/256 weights both decide this pixel.
+    canvas = svg_golden.load_fixture("structure/image/with-transform.svg")
+    image, _ = oracle.render_frame(canvas.arrays(0))
+    assert list(image.view(np.uint8).reshape(300, 300, 4)[135, 128]) == [28, 28, 28, 255]
+
+
+def test_image_placement_follows_preserve_aspect_ratio():
+    # a 2 x 1 image in a 40 x 40 viewport: meet fits it to 40 x 20 at the
+    # aligned offset; slice covers 80 x 40 and clips to the viewport
+    png = _png_bytes(np.array([[[255, 0, 0, 255], [0, 0, 255, 255]]], np.uint8))
+    for par, expect_ts in [("xMinYMin meet", (20.0, 0.0, 0.0, 20.0, 10.0, 10.0)),
+                           ("xMidYMid meet", (20.0, 0.0, 0.0, 20.0, 10.0, 20.0)),
+                           ("xMaxYMax meet", (20.0, 0.0, 0.0, 20.0, 10.0, 30.0)),
+                           ("none", (20.0, 0.0, 0.0, 40.0, 10.0, 10.0)),
+                           ("xMidYMid slice", (40.0, 0.0, 0.0, 40.0, -10.0, 10.0)),
+                           ("xMaxYMin slice", (40.0, 0.0, 0.0, 40.0, -30.0, 10.0))]:
+        doc = svg.Document(_image_svg(png, 'x="10" y="10" width="40" height="40" '
+                                           f'preserveAspectRatio="{par}"'))
+        kinds = [d[0] for d in doc.draws]
+        image = [d for d in doc.draws if d[0] == "image"][0]
+        assert image[2] == expect_ts, (par, image[2])
+        assert kinds == (["clip", "image", "pop"] if "slice" in par else ["image"])
+
+
+def test_image_auto_sizes_and_rendering_mode():
+    png = _png_bytes(np.zeros((4, 8, 4), np.uint8) + 200)
+    doc = svg.Document(_image_svg(png, 'width="16"'))
+    assert doc.draws[0][2] == (2.0, 0.0, 0.0, 2.0, 0.0, 0.0)  # height follows the image's ratio
+    doc = svg.Document(_image_svg(png, 'height="2" image-rendering="pixelated"'))
+    assert doc.draws[0][2] == (0.5, 0.0, 0.0, 0.5, 0.0, 0.0) and doc.draws[0][3] == tk.PATTERN_NEAREST
+    doc = svg.Document(_image_svg(png, 'width="auto" height="auto"'))
+    assert doc.draws[0][2] == svg.IDENTITY and doc.draws[0][3] == tk.PATTERN_BILINEAR
+    assert svg.Document(_image_svg(png, 'width="0"')).draws == []  # an empty viewport draws nothing
+    doc = svg.Document('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 10 10">'
+                       '<image href="no-such-file.png" width="5" height="5"/></svg>')
+    assert doc.draws == []  # an image that cannot be read is skipped, as usvg skips it
+
+
+def test_png_decoding_matches_the_reference_unit_test():
+    # decode_png_image_preserves_rgba_pixels (src/svg/image.rs:104-130): a 2 x 1
+    # RGBA PNG decodes to premul_rgba8_pack of each pixel
+    png = _png_bytes(np.array([[[255, 0, 0, 128], [0, 128, 255, 64]]], np.uint8))
+    image = svg.decode_image(png)
+    assert (image.width, image.height) == (2, 1)
+    px = image.pixels.view(np.uint8).reshape(2, 4).tolist()
+    assert px == [[128, 0, 0, 128], [0, 32, 64, 64]]
+
+
+def _png_bytes(rgba):
+    import io as _io
+    from PIL import Image
+    buf = _io.BytesIO()
+    Image.fromarray(rgba, "RGBA").save(buf, format="PNG")
+    return buf.getvalue()
+
+
+def _image_svg(png, attrs):
+    import base64 as _b64
+    data = _b64.b64encode(png).decode()
+    return ('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 100 100">'
+            f'<image href="data:image/png;base64,{data}" {attrs}/></svg>')

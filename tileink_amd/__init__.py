@@ -16,11 +16,13 @@ import enum
 import numpy as np
 
 from . import _abi
-from ._abi import (BRUSH_FOUR_CORNER, BRUSH_LINEAR, BRUSH_RADIAL, BRUSH_SOLID, BRUSH_SWEEP, FILL_EVEN_ODD, FILL_NONZERO, INVALID, LINE_DTYPE, PTCL_DTYPE,
+from ._abi import (BRUSH_FOUR_CORNER, BRUSH_LINEAR, BRUSH_RADIAL, BRUSH_SOLID, BRUSH_SWEEP, PATTERN_BILINEAR,
+                   PATTERN_NEAREST, FILL_EVEN_ODD, FILL_NONZERO, INVALID, LINE_DTYPE, PTCL_DTYPE,
                    SEGMENT_DTYPE, TileinkError, check, lib, ptr)
 
 __all__ = [
     "Batch", "BezPath", "Canvas", "Color", "LinearGradient", "RadialGradient", "SweepGradient", "FourCornerGradient",
+    "Image", "ImageBrush", "PATTERN_NEAREST", "PATTERN_BILINEAR",
     "NativeBackend", "NativeContext", "NativeRenderer", "Frame", "Join", "Cap", "FillRule",
     "SceneArrays", "TileinkError", "FILL_NONZERO", "FILL_EVEN_ODD",
 ]
@@ -180,6 +182,52 @@ class FourCornerGradient:
         return b, None
 
 
+def _mul_div255(a, b):
+    """mul_div255 (src/shared/pixel.rs): a * b / 255 rounded, on integer arrays."""
+    product = a.astype(np.uint32) * b.astype(np.uint32) + 128
+    return (product + (product >> 8)) >> 8
+
+
+class Image:
+    """Premultiplied RGBA8 pixels (src/shared/image.rs:9-19): `pixels` is a
+    (height, width) uint32 array, red in the low byte."""
+
+    def __init__(self, pixels):
+        self.pixels = np.ascontiguousarray(np.asarray(pixels, dtype=np.uint32))
+        if self.pixels.ndim != 2 or self.pixels.size == 0:
+            raise TileinkError(_abi.TK_ERR_INVALID, "an image is a non-empty (height, width) array")
+        self.height, self.width = self.pixels.shape
+
+    @classmethod
+    def from_rgba8(cls, rgba):
+        """Image::from_rgba8: straight-alpha (height, width, 4) uint8, premultiplied
+        channel by channel with mul_div255 (premul_rgba8_pack, src/shared/image.rs:180-187)."""
+        px = np.asarray(rgba, dtype=np.uint8)
+        a = px[..., 3]
+        r, g, b = (_mul_div255(px[..., k], a) for k in range(3))
+        packed = r | (g << 8) | (b << 16) | (a.astype(np.uint32) << 24)
+        return cls(packed.astype(np.uint32))
+
+
+class ImageBrush:
+    """An image of a canvas (Canvas.add_image) as a brush (Brush::Pattern with an
+    image resource): `transform` (a, b, c, d, e, f) takes user space to the image's
+    unit square; extend 0 pad, 1 repeat, 2 reflect; sampling PATTERN_NEAREST or
+    PATTERN_BILINEAR; opacity 0-255."""
+
+    def __init__(self, image_id, transform, extend=0, sampling=1, opacity=255):
+        self.image_id, self.transform = int(image_id), tuple(float(v) for v in transform)
+        self.extend, self.sampling, self.opacity = int(extend), int(sampling), int(opacity)
+
+    def _brush(self):
+        b = _abi.Brush()
+        b.kind = _abi.BRUSH_IMAGE
+        b.image = self.image_id
+        b.pattern_transform[:] = self.transform
+        b.extend, b.sampling, b.opacity = self.extend, self.sampling, self.opacity
+        return b, None
+
+
 def _brush_arg(brush):
     if isinstance(brush, tuple):
         brush = Color(*brush)
@@ -203,7 +251,7 @@ class SceneArrays:
 
     FIELDS = ("lines", "paths", "draws", "paint", "draw_batch_ids", "layers", "batches", "tile_draws",
               "tile_draw_indices", "scan_chunks", "scan_chunk_ranges", "cumsum_chunk_offsets", "cumsum_chunk_lens",
-              "cumsum_row_starts", "cumsum_row_ends")
+              "cumsum_row_starts", "cumsum_row_ends", "atlas")
 
     def __init__(self, scene):
         s = scene
@@ -224,6 +272,8 @@ class SceneArrays:
         self.cumsum_chunk_lens = _abi.view(s.cumsum_chunk_lens, np.uint32, s.cumsum_chunk_count)
         self.cumsum_row_starts = _abi.view(s.cumsum_row_starts, np.uint32, s.cumsum_row_count)
         self.cumsum_row_ends = _abi.view(s.cumsum_row_ends, np.uint32, s.cumsum_row_count)
+        self.atlas = _abi.view(s.atlas, np.uint32, s.atlas_width * s.atlas_height * s.atlas_pages) \
+            if s.atlas else np.zeros(0, np.uint32)
 
     def __getattr__(self, name):
         scalars = self.__dict__.get("scalars", {})
@@ -332,6 +382,25 @@ class Canvas:
                                         C.byref(out)))
         self._scene = None
         return out.value
+
+    def add_image(self, image):
+        """Registers an Image (Canvas::register_scene_image); returns its id for
+        ImageBrush and push_image."""
+        if not isinstance(image, Image):
+            image = Image(image)
+        out = C.c_uint32()
+        check(lib().tk_canvas_add_image(self._h, image.width, image.height, ptr(image.pixels), C.byref(out)))
+        self._scene = None
+        return out.value
+
+    def push_image(self, x0, y0, x1, y1, image_id, extend=0, sampling=PATTERN_BILINEAR):
+        """Canvas::push_image: the image scaled into the rect as an SDF rect draw;
+        None when the rect is empty or not finite."""
+        out = C.c_uint32()
+        check(lib().tk_canvas_push_image(self._h, float(x0), float(y0), float(x1), float(y1), int(image_id),
+                                         int(extend), int(sampling), C.byref(out)))
+        self._scene = None
+        return None if out.value == _abi.INVALID else out.value
 
     def push_clip_sdf_rect_layer(self, x0, y0, x1, y1, radii=0.0):
         r = (C.c_float * 4)(*((radii,) * 4 if np.isscalar(radii) else radii))

@@ -22,7 +22,8 @@ TK_NO_ALLOC_MAX_TABLE = 536_739_840
 PATH_MOVE, PATH_LINE, PATH_QUAD, PATH_CUBIC, PATH_CLOSE = range(5)
 FILL_NONZERO, FILL_EVEN_ODD = 0, 1
 DRAW_BRUSH, DRAW_CLIP = 0, 1
-BRUSH_SOLID, BRUSH_LINEAR, BRUSH_RADIAL, BRUSH_SWEEP, BRUSH_FOUR_CORNER = 1, 2, 3, 4, 5
+BRUSH_SOLID, BRUSH_LINEAR, BRUSH_RADIAL, BRUSH_SWEEP, BRUSH_FOUR_CORNER, BRUSH_PATTERN, BRUSH_IMAGE = 1, 2, 3, 4, 5, 6, 7
+PATTERN_NEAREST, PATTERN_BILINEAR = 0, 1
 
 # Frame buffer identifiers (tk_frame_buffer).
 BUF_BACKDROPS, BUF_SEGMENT_COUNTS, BUF_SEGMENT_RANGES, BUF_SEGMENT_CURSORS, BUF_SEGMENT_BUMPS, BUF_SEGMENTS = range(6)
@@ -95,6 +96,8 @@ class Scene(C.Structure):
         ("cumsum_row_starts", C.c_void_p), ("cumsum_row_ends", C.c_void_p), ("cumsum_row_count", C.c_uint32),
         ("backdrop_len", C.c_uint32), ("segment_capacity", C.c_uint32), ("max_clip_depth", C.c_uint32),
         ("ptcl_capacity", C.c_uint64),
+        ("atlas", C.c_void_p), ("atlas_width", C.c_uint32), ("atlas_height", C.c_uint32), ("atlas_pages", C.c_uint32),
+        ("atlas_reserved", C.c_uint32),
     ]
 
 
@@ -106,7 +109,9 @@ class Brush(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("rgba", C.c_float * 4), ("start", C.c_float * 2), ("end", C.c_float * 2),
                 ("extend", C.c_uint32), ("stops", C.POINTER(GradientStop)), ("stop_count", C.c_uint32),
                 ("start_radius", C.c_float), ("end_radius", C.c_float), ("start_angle", C.c_float),
-                ("end_angle", C.c_float), ("bounds", C.c_float * 4), ("corners", (C.c_float * 4) * 4)]
+                ("end_angle", C.c_float), ("bounds", C.c_float * 4), ("corners", (C.c_float * 4) * 4),
+                ("image", C.c_uint32), ("pattern_transform", C.c_float * 6), ("sampling", C.c_uint32),
+                ("opacity", C.c_uint32)]
 
 
 class Limits(C.Structure):
@@ -166,7 +171,8 @@ class FineConfig(C.Structure):
 
 class FineInputs(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("draws", "paint", "segments", "tile_ptcl_ranges", "tile_kinds",
-                                          "batch_ptcl_bases", "ptcl", "status")]
+                                          "batch_ptcl_bases", "ptcl", "status", "atlas")] + \
+               [(n, C.c_uint32) for n in ("atlas_width", "atlas_height", "atlas_pages", "atlas_reserved")]
 
 
 # numpy dtypes for reading record arrays back
@@ -190,6 +196,9 @@ SIGNATURES = {
     "tk_canvas_push_path": (C.c_int, [_P, _P, _SZ, _P, _SZ, _P, _P, _U32, C.c_double, C.POINTER(_U32)]),
     "tk_canvas_push_stroke": (C.c_int, [_P, _P, _SZ, _P, _SZ, C.c_double, _P, _P, _U32, C.c_double, C.POINTER(_U32)]),
     "tk_canvas_push_rect": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double, _P, _P, C.POINTER(_U32)]),
+    "tk_canvas_add_image": (C.c_int, [_P, _U32, _U32, _P, C.POINTER(_U32)]),
+    "tk_canvas_push_image": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double, _U32, _U32, _U32,
+                                       C.POINTER(_U32)]),
     "tk_canvas_push_clip_sdf_rect_layer": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double, _P]),
     "tk_canvas_push_clip_layer": (C.c_int, [_P, _P, _SZ, _P, _SZ, _P, _U32, C.c_double]),
     "tk_canvas_pop_layer": (C.c_int, [_P]),

@@ -553,11 +553,14 @@ double rem_euclid_f64(double a, double b) {
     return r < 0.0 ? r + std::fabs(b) : r;
 }
 
+// The scope word of a canvas image's id (ImageResourceId::scene).
+constexpr uint32_t kImageScopeScene = 1u;
+
 // Encodes a brush into its 9 header + 12 parameter words and its payload
 // (push_encoded_brush, src/shared/brush.rs:249-345, after Surface::physical_brush,
 // src/canvas/surface.rs:395-420, has applied the canvas scale).
 bool encode_brush(const tk_brush *brush, double scale, std::vector<uint32_t> &blob, uint32_t *offset,
-                  uint32_t *len) {
+                  uint32_t *len, uint32_t image_count) {
     uint32_t data[TK_BRUSH_HEADER_WORDS] = {TK_BRUSH_SOLID, 0u, 0u, 0u, 0u, 0u, 0u, 255u, 0u};
     uint32_t params[TK_BRUSH_PARAM_WORDS] = {};
     std::vector<uint32_t> payload;
@@ -611,6 +614,22 @@ bool encode_brush(const tk_brush *brush, double scale, std::vector<uint32_t> &bl
     } else if (kind == TK_BRUSH_FOUR_CORNER) {
         for (int i = 0; i < 4; ++i) params[i] = f2bits(brush->bounds[i] * scale_f);
         for (int i = 0; i < 4; ++i) payload.push_back(premultiply_pack(brush->corners[i]));
+    } else if (kind == TK_BRUSH_IMAGE) {
+        // Brush::Pattern (src/shared/brush.rs:325-340): extend, the transform scaled
+        // by 1 / scale (Surface::physical_brush, src/canvas/surface.rs:428-431),
+        // opacity, sampling, and the three-word image id the atlas placement
+        // replaces at encoding (resource_brush_header, src/shared/gpu_brush.rs:92-127).
+        if (brush->extend > 2u || brush->sampling > 1u || brush->image >= image_count) return false;
+        for (float v : brush->pattern_transform)
+            if (!std::isfinite(v)) return false;
+        const float inv_scale = 1.0f / scale_f;
+        data[1] = brush->extend;
+        data[7] = std::min(brush->opacity, 255u);
+        data[8] = brush->sampling;
+        for (int i = 0; i < 4; ++i) params[i] = f2bits(brush->pattern_transform[i] * inv_scale);
+        params[4] = f2bits(brush->pattern_transform[4]);
+        params[5] = f2bits(brush->pattern_transform[5]);
+        payload = {kImageScopeScene, brush->image, 0u};
     } else {
         return false;
     }
@@ -685,6 +704,16 @@ struct tk_canvas {
     std::vector<tk_scan_chunk> scan_chunks;
     std::vector<tk_scan_chunk_range> scan_chunk_ranges;
     std::vector<uint32_t> cumsum_offsets, cumsum_lens, cumsum_row_starts, cumsum_row_ends;
+
+    // Images the canvas registered (premultiplied RGBA8), and the atlas the
+    // encoding packs them into.
+    struct Image {
+        uint32_t width, height;
+        std::vector<uint32_t> pixels;
+    };
+    std::vector<Image> images;
+    std::vector<uint32_t> atlas;
+    uint32_t atlas_width = 0, atlas_height = 0, atlas_pages = 0;
 };
 
 namespace tk {
@@ -740,7 +769,7 @@ uint32_t push_physical_path(tk_canvas *c, const BezPath &path, double device_tol
     c->paths.push_back(rec);
 
     uint32_t brush_offset = 0, brush_len = 0;
-    if (!encode_brush(brush, c->scale, c->brush_blob, &brush_offset, &brush_len)) {
+    if (!encode_brush(brush, c->scale, c->brush_blob, &brush_offset, &brush_len, uint32_t(c->images.size()))) {
         *status = TK_ERR_INVALID;
         return kNone;
     }
@@ -782,7 +811,7 @@ uint32_t push_sdf_rect(tk_canvas *c, double x0, double y0, double x1, double y1,
     for (float v : r) c->sdf_blob.push_back(f2bits(v));
     for (int i = 0; i < 8; ++i) c->sdf_blob.push_back(0u);  // stroke[4], shadow[4]
     uint32_t brush_offset = 0, brush_len = 0;
-    if (!encode_brush(brush, s, c->brush_blob, &brush_offset, &brush_len)) {
+    if (!encode_brush(brush, s, c->brush_blob, &brush_offset, &brush_len, uint32_t(c->images.size()))) {
         *status = TK_ERR_INVALID;
         return kNone;
     }
@@ -1018,6 +1047,69 @@ void compile_into(const tk_canvas *c, uint32_t list, std::vector<uint32_t> &stac
     flush();
 }
 
+// The atlas of the canvas's images and their placement in the encoded paint.
+// Images stack down pages of atlas_width (the widest image) by atlas_height (at
+// least the tallest image, else 4096 rows) texels, each at x = 0 and the next
+// free row; every image brush header then gets its image's placement in words
+// 2-6 (x, y, page, width, height), as resource_brush_header patches a header
+// from the renderer's resource table (src/shared/gpu_brush.rs:92-127).  Only
+// the encoded paint changes: the brush blob keeps the image ids.
+void pack_images(tk_canvas *c) {
+    c->atlas.clear();
+    c->atlas_width = c->atlas_height = c->atlas_pages = 0;
+    if (c->images.empty()) return;
+    struct Place {
+        uint32_t x, y, page;
+    };
+    std::vector<Place> place(c->images.size());
+    uint32_t width = 0, tallest = 0;
+    for (const auto &im : c->images) {
+        width = std::max(width, im.width);
+        tallest = std::max(tallest, im.height);
+    }
+    const uint32_t height = std::max<uint32_t>(tallest, 4096u);
+    uint32_t page = 0, row = 0, used = 0;
+    for (size_t i = 0; i < c->images.size(); ++i) {
+        if (row + c->images[i].height > height) {
+            ++page;
+            row = 0;
+        }
+        place[i] = Place{0u, row, page};
+        row += c->images[i].height;
+        used = std::max(used, row);
+    }
+    const uint32_t pages = page + 1;
+    // a single page is cut to the rows it uses
+    const uint32_t page_height = pages == 1 ? used : height;
+    c->atlas.assign(size_t(pages) * page_height * width, 0u);
+    for (size_t i = 0; i < c->images.size(); ++i) {
+        const auto &im = c->images[i];
+        for (uint32_t y = 0; y < im.height; ++y)
+            std::memcpy(&c->atlas[(size_t(place[i].page) * page_height + place[i].y + y) * width],
+                        &im.pixels[size_t(y) * im.width], size_t(im.width) * 4);
+    }
+    c->atlas_width = width;
+    c->atlas_height = page_height;
+    c->atlas_pages = pages;
+    const size_t brush_base = c->sdf_blob.size() + c->shadow_blob.size();
+    for (const tk_draw_record &d : c->draws) {
+        if (d.brush_offset == kNone) continue;
+        uint32_t *h = &c->paint[brush_base + d.brush_offset];
+        // a header two draws share is patched once (its x word is then 0, never
+        // the payload offset)
+        if (h[0] != TK_BRUSH_IMAGE || h[2] != TK_BRUSH_HEADER_WORDS + TK_BRUSH_PARAM_WORDS) continue;
+        const uint32_t id = h[h[2] + 1u];
+        if (id < c->images.size()) {
+            const uint32_t v[5] = {place[id].x, place[id].y, place[id].page, c->images[id].width, c->images[id].height};
+            std::memcpy(h + 2, v, sizeof(v));
+        } else {  // absent: a transparent pattern
+            h[0] = TK_BRUSH_PATTERN;
+            const uint32_t v[5] = {TK_BRUSH_HEADER_WORDS, 0u, 0u, 0u, 0u};
+            std::memcpy(h + 2, v, sizeof(v));
+        }
+    }
+}
+
 tk_status encode(tk_canvas *c, uint32_t clear_color, tk_scene *out) {
     const uint32_t tw = c->tiles_w, th = c->tiles_h;
     const uint64_t tiles = uint64_t(tw) * th;
@@ -1206,6 +1298,12 @@ tk_status encode(tk_canvas *c, uint32_t clear_color, tk_scene *out) {
     out->segment_capacity = uint32_t(c->segment_total);
     out->max_clip_depth = max_depth;
     out->ptcl_capacity = ptcl_capacity;
+    pack_images(c);
+    out->atlas = c->atlas_pages ? c->atlas.data() : nullptr;
+    out->atlas_width = c->atlas_width;
+    out->atlas_height = c->atlas_height;
+    out->atlas_pages = c->atlas_pages;
+    out->atlas_reserved = 0;
     return TK_OK;
 }
 
@@ -1331,6 +1429,45 @@ tk_status tk_canvas_push_rect(tk_canvas *canvas, double x0, double y0, double x1
     tk_status status;
     uint32_t d = push_sdf_rect(canvas, x0, y0, x1, y1, radii, brush, TK_DRAW_BRUSH, true, &status);
     if (status != TK_OK) return fail(status, "unsupported brush");
+    if (draw_id) *draw_id = d;
+    return TK_OK;
+}
+
+tk_status tk_canvas_add_image(tk_canvas *canvas, uint32_t width, uint32_t height, const uint32_t *premul_rgba8,
+                              uint32_t *image_id) {
+    if (!canvas || !premul_rgba8 || !image_id) return fail(TK_ERR_INVALID, "null argument");
+    if (width == 0 || height == 0) return fail(TK_ERR_INVALID, "an image has at least one pixel");
+    if (uint64_t(width) * height > (1ull << 28)) return fail(TK_ERR_OVERFLOW, "image larger than 2^28 pixels");
+    tk_canvas::Image im{width, height, std::vector<uint32_t>(premul_rgba8, premul_rgba8 + size_t(width) * height)};
+    *image_id = uint32_t(canvas->images.size());
+    canvas->images.push_back(std::move(im));
+    return TK_OK;
+}
+
+tk_status tk_canvas_push_image(tk_canvas *canvas, double x0, double y0, double x1, double y1, uint32_t image,
+                               uint32_t extend, uint32_t sampling, uint32_t *draw_id) {
+    if (!canvas) return fail(TK_ERR_INVALID, "null canvas");
+    if (draw_id) *draw_id = kNone;
+    if (image >= canvas->images.size()) return fail(TK_ERR_INVALID, "unknown image");
+    if (extend > 2u || sampling > 1u) return fail(TK_ERR_INVALID, "extend is 0-2 and sampling 0-1");
+    // image_rect_is_valid (src/canvas.rs:221-228): nothing is recorded otherwise.
+    if (!std::isfinite(x0) || !std::isfinite(y0) || !std::isfinite(x1) || !std::isfinite(y1) || !(x1 - x0 > 0.0) ||
+        !(y1 - y0 > 0.0))
+        return TK_OK;
+    // PatternBrush::for_rect_resource (src/shared/brush.rs:557-583), in f32.
+    tk_brush b{};
+    b.kind = TK_BRUSH_IMAGE;
+    b.image = image;
+    b.extend = extend;
+    b.sampling = sampling;
+    b.opacity = 255u;
+    const float sx = 1.0f / float(x1 - x0), sy = 1.0f / float(y1 - y0);
+    const float t[6] = {sx, 0.0f, 0.0f, sy, -float(x0) * sx, -float(y0) * sy};
+    std::memcpy(b.pattern_transform, t, sizeof(t));
+    const float zero[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    tk_status status;
+    uint32_t d = push_sdf_rect(canvas, x0, y0, x1, y1, zero, &b, TK_DRAW_BRUSH, true, &status);
+    if (status != TK_OK) return fail(status, "image brush encoding failed");
     if (draw_id) *draw_id = d;
     return TK_OK;
 }

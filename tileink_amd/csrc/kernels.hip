@@ -542,12 +542,15 @@ __device__ __forceinline__ FineDraw load_fine_draw(const tk_draw_record *draws, 
 }
 
 __device__ __forceinline__ uint32_t sample_draw_brush(const tk_fine_config &cfg, const uint32_t *paint,
-                                                      const FineDraw &d, float x, float y) {
+                                                      const FineDraw &d, float x, float y, const AtlasView &atlas) {
     if (blob_word(paint, cfg.paint_words, cfg.paint_brush_base, d.brush_offset) == TK_BRUSH_SOLID)
         return blob_word(paint, cfg.paint_words, cfg.paint_brush_base, d.brush_offset + 4u);
     float lx, ly;
     affine_point(d.inv, x, y, lx, ly);
-    return sample_brush(paint, cfg.paint_words, cfg.paint_brush_base, d.brush_offset, lx, ly);
+    return sample_brush(paint, cfg.paint_words, cfg.paint_brush_base, d.brush_offset, lx, ly, atlas);
+}
+__device__ __forceinline__ AtlasView atlas_of(const tk_fine_inputs &in) {
+    return AtlasView{in.atlas, in.atlas_width, in.atlas_height, in.atlas_pages};
 }
 
 __device__ __forceinline__ float sdf_coverage_from_draw(const tk_fine_config &cfg, const uint32_t *paint,
@@ -1657,7 +1660,8 @@ extern "C" __global__ __launch_bounds__(256) void tk_k_draw_brush(tk_args_draw_b
     const uint32_t kind = blob_word(a.paint, a.paint_words, a.paint_brush_base, offset);
     const uint32_t color = blob_word(a.paint, a.paint_words, a.paint_brush_base, offset + 4u);
     uint32_t flag = 1u, value = color;
-    if (kind == TK_BRUSH_LINEAR || kind == TK_BRUSH_RADIAL || kind == TK_BRUSH_SWEEP || kind == TK_BRUSH_FOUR_CORNER)
+    if (kind == TK_BRUSH_LINEAR || kind == TK_BRUSH_RADIAL || kind == TK_BRUSH_SWEEP || kind == TK_BRUSH_FOUR_CORNER ||
+        kind == TK_BRUSH_IMAGE)
         flag = 0u, value = 0u;
     else if (kind == 6u)  // BRUSH_PATTERN
         value = 0u;
@@ -1751,13 +1755,21 @@ __device__ __forceinline__ void fine_tile(const tk_args_fine &a) {
                             if (clip_mask == 255u && premul_opaque(rec.color)) pixel = rgba8_to_unorm(rec.color);
                             else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(rec.color, clip_mask));
                         }
+                    } else if (tag == kPtclImage) {
+                        // interpreter.hlsli:65-74: an image covering the whole tile
+                        if (clip_mask != 0u) {
+                            const uint32_t color = sample_draw_brush(
+                                cfg, paint, load_fine_draw(a.in.draws, cfg.draw_count, rec.color), sx, sy, atlas_of(a.in));
+                            if (clip_mask == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
+                            else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, clip_mask));
+                        }
                     } else if (tag == TK_PTCL_SDF) {
                         if (clip_mask != 0u) {
                             FineDraw d = load_fine_draw(a.in.draws, cfg.draw_count, rec.color);
                             float coverage = sdf_coverage_from_draw(cfg, paint, d, sx, sy);
                             uint32_t alpha = combine_alpha(coverage_to_u8(coverage), clip_mask);
                             if (alpha != 0u) {
-                                uint32_t color = sample_draw_brush(cfg, paint, d, sx, sy);
+                                uint32_t color = sample_draw_brush(cfg, paint, d, sx, sy, atlas_of(a.in));
                                 if (alpha == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
                                 else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, alpha));
                             }
@@ -1808,7 +1820,7 @@ __device__ __forceinline__ void fine_tile(const tk_args_fine &a) {
                                 uint32_t color = brush_color;
                                 if (brush_flag == 0u)
                                     color = sample_draw_brush(cfg, paint, load_fine_draw(a.in.draws, cfg.draw_count, rec.color),
-                                                              sx, sy);
+                                                              sx, sy, atlas_of(a.in));
                                 if (masked == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
                                 else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, masked));
                             }
@@ -1831,7 +1843,10 @@ __device__ __forceinline__ void fine_tile(const tk_args_fine &a) {
                         if (premul_opaque(rec.color)) pixel = rgba8_to_unorm(rec.color);
                         else pixel = src_over_premul_unorm(pixel, rgba8_to_unorm(rec.color));
                     } else if (rec.tag == kPtclImage) {
-                        // image brushes are not produced on this path
+                        const uint32_t color = sample_draw_brush(
+                            cfg, paint, load_fine_draw(a.in.draws, cfg.draw_count, rec.color), sx, sy, atlas_of(a.in));
+                        if (premul_opaque(color)) pixel = rgba8_to_unorm(color);
+                        else pixel = src_over_premul_unorm(pixel, rgba8_to_unorm(color));
                     } else {
                         FineDraw d = load_fine_draw(a.in.draws, cfg.draw_count, rec.color);
                         float coverage = 0.0f;

@@ -681,8 +681,130 @@ TKD uint32_t sample_four_corner(const uint32_t *paint, uint32_t count, uint32_t 
 
 // sample_brush (shared/brush/sample.hlsli:15-42) for the solid, linear, radial,
 // sweep and four-corner brushes.
+// ---------------------------------------------------------------------------
+// Image brushes (shared/brush/pattern.hlsli:9-55, shared/pattern_transform.hlsli,
+// shared/integer.hlsli): the scene's atlas of premultiplied RGBA8 texels.
+// ---------------------------------------------------------------------------
+
+struct AtlasView {
+    const uint32_t *texels;
+    uint32_t width, height, pages;
+};
+
+// euclidean_remainder_i32 (shared/integer.hlsli:5-9).
+TKD uint32_t euclidean_remainder_i32(int32_t value, uint32_t period) {
+    const uint32_t magnitude = value < 0 ? 0u - (uint32_t)value : (uint32_t)value;
+    const uint32_t remainder = magnitude % period;
+    return value < 0 && remainder != 0u ? period - remainder : remainder;
+}
+// extend_coord_i32 (pattern.hlsli:9-17): pad clamps, repeat wraps, reflect mirrors.
+TKD uint32_t extend_coord_i32(int32_t value, uint32_t size, uint32_t extend) {
+    if (size <= 1u) return 0u;
+    if (extend == 1u || extend == 2u) {
+        const uint32_t period = size * (extend == 2u ? 2u : 1u);
+        const uint32_t coord = euclidean_remainder_i32(value, period);
+        return coord < size ? coord : period - coord - 1u;
+    }
+    return (uint32_t)clampi(value, 0, (int)size - 1);
+}
+// atlas.Load of one texel; outside the atlas reads 0, as a Load out of range does.
+TKD uint32_t atlas_texel(const AtlasView &a, uint32_t page, uint32_t x, uint32_t y) {
+    if (!a.texels || page >= a.pages || x >= a.width || y >= a.height) return 0u;
+    return a.texels[((size_t)page * a.height + y) * a.width + x];
+}
+// atlas_pattern_pixel (pattern.hlsli:18-21).
+TKD uint32_t atlas_pattern_pixel(const AtlasView &a, uint32_t page, uint32_t ox, uint32_t oy, uint32_t sx,
+                                 uint32_t sy, uint32_t extend, int32_t px, int32_t py) {
+    return atlas_texel(a, page, ox + extend_coord_i32(px, sx, extend), oy + extend_coord_i32(py, sy, extend));
+}
+// pattern_transform_component (shared/pattern_transform.hlsli:7-11): the second
+// product's rounding error is recovered by a fused multiply-add.
+TKD float pattern_transform_component(float a, float b, float offset, float x, float y) {
+    const float by = b * y;
+    const float dot = fmaf(a, x, by) + fmaf(b, y, -by);
+    return dot + offset;
+}
+// The filtered read the reference makes through a bilinear sampler for a padded
+// bilinear pattern (atlas.SampleLevel, pattern.hlsli:25-28): position clamped to
+// the image, texel centres at half-integers, the four neighbours clamped to the
+// image (an edge replicated, as an atlas gutter does).  A texture unit resolves
+// the position to 8 fractional bits (the subtexel precision Direct3D requires),
+// so each weight is k / 256 with k the position's fraction rounded to the
+// nearest 1/256; the blend is then the integer sum of weighted texels, rounded
+// once to 8 bits.
+TKD void subtexel_position(float q, int32_t &index, uint32_t &weight) {
+    const float base = floorf(q);
+    weight = (uint32_t)rintf((q - base) * 256.0f);  // q - base is exact
+    index = f2i(base);
+    if (weight == 256u) {
+        index += 1;
+        weight = 0u;
+    }
+}
+TKD uint32_t filtered_pad_sample(const AtlasView &a, uint32_t page, uint32_t ox, uint32_t oy, uint32_t sx, uint32_t sy,
+                                 float px, float py) {
+    int32_t ix, iy;
+    uint32_t wx, wy;
+    subtexel_position(clampf(px, 0.0f, (float)sx) - 0.5f, ix, wx);
+    subtexel_position(clampf(py, 0.0f, (float)sy) - 0.5f, iy, wy);
+    const uint32_t t[4] = {atlas_pattern_pixel(a, page, ox, oy, sx, sy, 0u, ix, iy),
+                           atlas_pattern_pixel(a, page, ox, oy, sx, sy, 0u, ix + 1, iy),
+                           atlas_pattern_pixel(a, page, ox, oy, sx, sy, 0u, ix, iy + 1),
+                           atlas_pattern_pixel(a, page, ox, oy, sx, sy, 0u, ix + 1, iy + 1)};
+    const uint32_t w[4] = {(256u - wx) * (256u - wy), wx * (256u - wy), (256u - wx) * wy, wx * wy};
+    uint32_t out = 0u;
+    for (int k = 0; k < 4; ++k) {
+        uint32_t sum = 32768u;
+        for (int n = 0; n < 4; ++n) sum += w[n] * ((t[n] >> (8 * k)) & 255u);
+        out |= (sum >> 16) << (8 * k);
+    }
+    return out;
+}
+// sample_resource_pattern_atlas (pattern.hlsli:22-43).
+TKD uint32_t sample_pattern_atlas(const AtlasView &a, float px, float py, uint32_t page, uint32_t ox, uint32_t oy,
+                                  uint32_t sx, uint32_t sy, uint32_t opacity, uint32_t extend, uint32_t sampling) {
+    if (sx == 0u || sy == 0u) return 0u;
+    uint32_t color;
+    if (sampling == 1u && extend == 0u) {
+        color = filtered_pad_sample(a, page, ox, oy, sx, sy, px, py);
+    } else if (sampling == 1u) {
+        const float spx = px - 0.5f, spy = py - 0.5f;
+        const float bx = floorf(spx), by = floorf(spy);
+        const float fx = spx - bx, fy = spy - by;
+        const int32_t ix = f2i(bx), iy = f2i(by);
+        const uint32_t tl = atlas_pattern_pixel(a, page, ox, oy, sx, sy, extend, ix, iy);
+        const uint32_t tr = atlas_pattern_pixel(a, page, ox, oy, sx, sy, extend, ix + 1, iy);
+        const uint32_t bl = atlas_pattern_pixel(a, page, ox, oy, sx, sy, extend, ix, iy + 1);
+        const uint32_t br = atlas_pattern_pixel(a, page, ox, oy, sx, sy, extend, ix + 1, iy + 1);
+        color = lerp_premul_u8(lerp_premul_u8(tl, tr, fx), lerp_premul_u8(bl, br, fx), fy);
+    } else {
+        color = atlas_pattern_pixel(a, page, ox, oy, sx, sy, extend, f2i(floorf(px)), f2i(floorf(py)));
+    }
+    return scale_premul_u8(color, opacity);
+}
+// sample_atlas_brush (pattern.hlsli:44-54), for a header whose placement the
+// encoding patched: words 2-6 = x, y, page, width, height.
+TKD uint32_t sample_image_brush(const uint32_t *paint, uint32_t count, uint32_t brush_base, uint32_t data_base,
+                                const AtlasView &atlas, float x, float y) {
+    const uint32_t base = data_base + TK_BRUSH_HEADER_WORDS;
+    const uint32_t sx = blob_word(paint, count, brush_base, data_base + 5u);
+    const uint32_t sy = blob_word(paint, count, brush_base, data_base + 6u);
+    if (sx == 0u || sy == 0u) return 0u;
+    const uint32_t placement = blob_word(paint, count, brush_base, data_base + 4u);
+    if (placement & 0x80000000u) return 0u;  // a texture-table image: this path holds only the atlas
+    float p[6];
+    for (int i = 0; i < 6; ++i) p[i] = blob_float(paint, count, brush_base, base + (uint32_t)i);
+    const float tx = pattern_transform_component(p[0], p[2], p[4], x, y) * (float)sx;
+    const float ty = pattern_transform_component(p[1], p[3], p[5], x, y) * (float)sy;
+    return sample_pattern_atlas(atlas, tx, ty, placement, blob_word(paint, count, brush_base, data_base + 2u),
+                                blob_word(paint, count, brush_base, data_base + 3u), sx, sy,
+                                blob_word(paint, count, brush_base, data_base + 7u),
+                                blob_word(paint, count, brush_base, data_base + 1u),
+                                blob_word(paint, count, brush_base, data_base + 8u));
+}
+
 TKD uint32_t sample_brush(const uint32_t *paint, uint32_t count, uint32_t brush_base, uint32_t data_base, float x,
-                          float y) {
+                          float y, const AtlasView &atlas = AtlasView{nullptr, 0u, 0u, 0u}) {
     uint32_t kind = blob_word(paint, count, brush_base, data_base);
     uint32_t extend = blob_word(paint, count, brush_base, data_base + 1u);
     uint32_t payload = data_base + blob_word(paint, count, brush_base, data_base + 2u);
@@ -698,8 +820,9 @@ TKD uint32_t sample_brush(const uint32_t *paint, uint32_t count, uint32_t brush_
         return has_t ? sample_ramp(paint, count, brush_base, payload, len, t, extend) : 0u;
     }
     if (kind == TK_BRUSH_FOUR_CORNER) return sample_four_corner(paint, count, brush_base, x, y, base, payload);
-    if (kind == 6u) return 0u;  // BRUSH_PATTERN
-    // Mesh, program and image brushes are not produced on this path.
+    if (kind == TK_BRUSH_PATTERN) return 0u;
+    if (kind == TK_BRUSH_IMAGE) return sample_image_brush(paint, count, brush_base, data_base, atlas, x, y);
+    // Mesh and program brushes are not produced on this path.
     return color;
 }
 

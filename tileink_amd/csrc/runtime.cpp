@@ -374,6 +374,7 @@ struct tk_frame {
     tk_path_record *paths = nullptr;
     tk_draw_record *draws = nullptr;
     uint32_t *paint = nullptr, *batch_ids = nullptr;
+    uint32_t *atlas = nullptr;  // the scene's image atlas (atlas_pages pages), or null
     tk_layer_record *layers = nullptr;
     tk_batch_record *batches = nullptr;
     tk_tile_draw_record *tile_draws = nullptr;
@@ -500,6 +501,10 @@ tk_status validate_scene(const tk_scene *s) {
     if (s->ptcl_capacity >= 0xFFFFFFFFull) return fail(TK_ERR_OVERFLOW, "PTCL capacity exceeds u32 record indices");
     if (s->paint_sdf_shadow_base > s->paint_brush_base || s->paint_brush_base > s->paint_words)
         return fail(TK_ERR_INVALID, "paint section bases out of order");
+    if (s->atlas && (s->atlas_width == 0 || s->atlas_height == 0 || s->atlas_pages == 0))
+        return fail(TK_ERR_INVALID, "an atlas has at least one texel");
+    if (uint64_t(s->atlas_width) * s->atlas_height * s->atlas_pages > (1ull << 31))
+        return fail(TK_ERR_OVERFLOW, "atlas larger than 2^31 texels");
     return TK_OK;
 }
 
@@ -545,6 +550,11 @@ bool diff_compatible(const tk_scene *s) {
     return true;
 }
 
+// Texels of a scene's image atlas (0 without one).
+uint64_t atlas_words(const tk_scene &s) {
+    return s.atlas ? uint64_t(s.atlas_width) * s.atlas_height * s.atlas_pages : 0ull;
+}
+
 tk_status frame_upload(tk_frame *f, const tk_scene *s) {
     f->diff_scene = diff_compatible(s);
     f->diff_ready = false;
@@ -552,6 +562,7 @@ tk_status frame_upload(tk_frame *f, const tk_scene *s) {
     TK_TRY(upload(f, f->paths, s->paths, s->path_count));
     TK_TRY(upload(f, f->draws, s->draws, s->draw_count));
     TK_TRY(upload(f, f->paint, s->paint, s->paint_words));
+    if (f->atlas) TK_TRY(upload(f, f->atlas, s->atlas, atlas_words(*s)));
     TK_TRY(upload(f, f->batch_ids, s->draw_batch_ids, s->draw_count));
     TK_TRY(upload(f, f->layers, s->layers, s->layer_count));
     TK_TRY(upload(f, f->batches, s->batches, s->batch_count));
@@ -580,7 +591,8 @@ tk_status frame_upload(tk_frame *f, const tk_scene *s) {
 // index list may shrink or grow up to the frame's allocation (`index_capacity`):
 // an opacity change can hide or reveal draws behind an opaque rectangle.
 bool same_sizes(const tk_scene &a, const tk_scene &b, uint32_t index_capacity) {
-    return a.width == b.width && a.height == b.height && a.line_count == b.line_count && a.path_count == b.path_count &&
+    return a.atlas_width == b.atlas_width && a.atlas_height == b.atlas_height && a.atlas_pages == b.atlas_pages &&
+           a.width == b.width && a.height == b.height && a.line_count == b.line_count && a.path_count == b.path_count &&
            a.draw_count == b.draw_count && a.paint_words == b.paint_words && a.layer_count == b.layer_count &&
            a.batch_count == b.batch_count && a.tile_draw_index_count <= index_capacity &&
            a.scan_chunk_count == b.scan_chunk_count && a.cumsum_chunk_count == b.cumsum_chunk_count &&
@@ -863,6 +875,10 @@ tk_status frame_render(tk_frame *f, uint32_t active_count = 0, const DiffTarget 
         in.batch_ptcl_bases = f->batch_bases;
         in.ptcl = f->ptcl;
         in.status = f->status + 2;
+        in.atlas = f->atlas;
+        in.atlas_width = s.atlas_width;
+        in.atlas_height = s.atlas_height;
+        in.atlas_pages = f->atlas ? s.atlas_pages : 0u;
         (void)T;
         TK_TRY(run_fine(ctx, fc, in, f->image, f->draw_brush, list, scenes, f->tuning.fine_xcd_run, fine_queue(f)));
     }
@@ -942,6 +958,7 @@ tk_status frame_create(tk_context *ctx, const tk_scene *s, const BatchLayout *ba
     ALLOC(paths, s->path_count);
     ALLOC(draws, s->draw_count);
     ALLOC(paint, s->paint_words);
+    if (atlas_words(*s)) ALLOC(atlas, atlas_words(*s));
     ALLOC(batch_ids, s->draw_count);
     ALLOC(layers, s->layer_count);
     ALLOC(batches, s->batch_count);
@@ -1598,6 +1615,7 @@ struct MergedScenes {
     std::vector<tk_tile_draw_record> tile_draws;
     std::vector<tk_scan_chunk> chunks;
     std::vector<tk_scan_chunk_range> chunk_ranges;
+    std::vector<uint32_t> atlas;  // the scenes' atlas pages, each re-laid to the widest and tallest page
     tk_scene scene{};
     BatchLayout layout;
 };
@@ -1618,10 +1636,37 @@ tk_status merge_scenes(const tk_scene *scenes, uint32_t count, MergedScenes &m) 
              entries = 0, ptcl = 0;
     uint32_t depth = 0;
     std::vector<uint32_t> sdf, shadow, brush;
+    // The merged atlas: every scene's pages in turn, as pages of the widest width
+    // and the tallest height; an image brush's page word moves by the pages of
+    // the scenes before its own.
+    uint32_t atlas_w = 0, atlas_h = 0;
+    uint64_t atlas_pages = 0, page_off = 0;
+    for (uint32_t i = 0; i < count; ++i)
+        if (scenes[i].atlas && scenes[i].atlas_pages) {
+            atlas_w = std::max(atlas_w, scenes[i].atlas_width);
+            atlas_h = std::max(atlas_h, scenes[i].atlas_height);
+            atlas_pages += scenes[i].atlas_pages;
+        }
+    if (uint64_t(atlas_w) * atlas_h * atlas_pages > (1ull << 31)) return fail(TK_ERR_OVERFLOW, "batch atlas too large");
+    m.atlas.assign(size_t(atlas_w) * atlas_h * atlas_pages, 0u);
+    std::vector<std::pair<uint64_t, uint32_t>> image_brushes;  // (offset in the brush region, page shift)
     for (uint32_t i = 0; i < count; ++i) {
         const tk_scene &s = scenes[i];
         if (s.paint_sdf_shadow_base > s.paint_brush_base || s.paint_brush_base > s.paint_words)
             return fail(TK_ERR_INVALID, "scene paint regions out of order");
+        if (s.atlas && s.atlas_pages) {
+            for (uint64_t p = 0; p < s.atlas_pages; ++p)
+                for (uint32_t y = 0; y < s.atlas_height; ++y)
+                    std::memcpy(&m.atlas[((page_off + p) * atlas_h + y) * atlas_w],
+                                &s.atlas[(p * s.atlas_height + y) * s.atlas_width], size_t(s.atlas_width) * 4);
+            for (uint32_t k = 0; k < s.draw_count; ++k) {
+                const uint32_t b = s.draws[k].brush_offset;
+                if (b != kNoIndex && uint64_t(s.paint_brush_base) + b < s.paint_words &&
+                    s.paint[s.paint_brush_base + b] == TK_BRUSH_IMAGE)
+                    image_brushes.emplace_back(brush_off + b, uint32_t(page_off));
+            }
+            page_off += s.atlas_pages;
+        }
         const uint64_t T = uint64_t(s.tiles_width) * s.tiles_height;
         tk_scene_slot slot{};
         slot.width = s.width;
@@ -1734,6 +1779,13 @@ tk_status merge_scenes(const tk_scene *scenes, uint32_t count, MergedScenes &m) 
     m.paint = std::move(sdf);
     m.paint.insert(m.paint.end(), shadow.begin(), shadow.end());
     m.paint.insert(m.paint.end(), brush.begin(), brush.end());
+    // a header several draws share moves once
+    std::sort(image_brushes.begin(), image_brushes.end());
+    image_brushes.erase(std::unique(image_brushes.begin(), image_brushes.end()), image_brushes.end());
+    for (const auto &ib : image_brushes) {
+        const uint64_t word = sdf_total + shadow_total + ib.first + 4u;  // the page word
+        if (word < m.paint.size() && !(m.paint[word] & 0x80000000u)) m.paint[word] += ib.second;
+    }
     m.layout.table_entries = entries;
     tk_scene &o = m.scene;
     o.tiles_width = uint32_t(columns);
@@ -1764,6 +1816,10 @@ tk_status merge_scenes(const tk_scene *scenes, uint32_t count, MergedScenes &m) 
     o.segment_capacity = uint32_t(segment_off);
     o.max_clip_depth = depth;
     o.ptcl_capacity = ptcl;
+    o.atlas = atlas_pages ? m.atlas.data() : nullptr;
+    o.atlas_width = atlas_w;
+    o.atlas_height = atlas_h;
+    o.atlas_pages = uint32_t(atlas_pages);
     return TK_OK;
 }
 

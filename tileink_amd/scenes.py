@@ -9,7 +9,8 @@ import math
 
 import numpy as np
 
-from . import BezPath, Canvas, Color, FillRule, LinearGradient
+from . import (PATTERN_BILINEAR, PATTERN_NEAREST, BezPath, Canvas, Color, FillRule, Image, ImageBrush,
+               LinearGradient)
 
 SEED_BASE = 0x7117E1C0
 
@@ -68,6 +69,63 @@ def mixed_scene_640x360(width=640, height=360):
             path = _cubic_path(rng, int(rng.integers(4, 13)), (0.0, 0.0), tuple(size), True)
             width_px = float(rng.lognormal(math.log(2.0), 0.5))
             canvas.push_stroke(path, width_px, _color(rng), tolerance=0.25)
+    return canvas
+
+
+def synthetic_image(rng, height, width, opaque=False):
+    """A straight-alpha RGBA8 image of smooth ramps plus noise, premultiplied."""
+    y, x = np.mgrid[0:height, 0:width].astype(np.float64)
+    phase = rng.uniform(0.0, 6.3, 4)
+    base = [127.5 + 127.5 * np.sin(x / max(width, 1) * (3 + k) + y / max(height, 1) * (2 + k) + phase[k])
+            for k in range(4)]
+    rgba = np.stack(base, -1) + rng.normal(0.0, 12.0, size=(height, width, 4))
+    rgba = np.clip(np.rint(rgba), 0, 255).astype(np.uint8)
+    if opaque:
+        rgba[..., 3] = 255
+    return Image.from_rgba8(rgba)
+
+
+def image_scene(width=320, height=240, scale=1.0, seed=0):
+    """Images drawn every way the canvas offers: image rects with fractional
+    edges (bilinear pad and nearest), image brushes on rounded rects, filled and
+    stroked paths under rotations (repeat and reflect, bilinear and nearest,
+    opacity below 255), a clip layer around an image, over a gradient background."""
+    rng = _rng(20, seed)
+    canvas = Canvas(width, height, scale)
+    size = np.array([width, height], dtype=np.float64)
+    canvas.push_rect(0.0, 0.0, width, height, 0.0, _gradient(rng, (0.0, 0.0), tuple(size)))
+    photo = canvas.add_image(synthetic_image(rng, 47, 61, opaque=True))
+    icon = canvas.add_image(synthetic_image(rng, 9, 7))
+    strip = canvas.add_image(synthetic_image(rng, 3, 40))
+    canvas.push_image(0.11 * width + 0.25, 0.05 * height + 0.5, 0.62 * width + 0.75, 0.55 * height + 0.1, photo)
+    canvas.push_image(0.7 * width, 0.04 * height, 0.95 * width, 0.3 * height, icon, sampling=PATTERN_NEAREST)
+    canvas.push_image(0.05 * width, 0.8 * height, 0.9 * width, 0.86 * height, strip, extend=1)
+    for k in range(12):
+        c = rng.uniform(0.0, 1.0, 2) * size
+        angle = rng.uniform(0.0, 2 * math.pi)
+        cos, sin = math.cos(angle), math.sin(angle)
+        cell = rng.uniform(6.0, 30.0)
+        # user space -> a rotated and scaled cell of the unit square
+        xf = (cos / cell, -sin / cell, sin / cell, cos / cell, rng.uniform(-1, 1), rng.uniform(-1, 1))
+        brush = ImageBrush([photo, icon, strip][k % 3], xf, extend=int(rng.integers(0, 3)),
+                           sampling=[PATTERN_BILINEAR, PATTERN_NEAREST][k % 2],
+                           opacity=255 if k % 4 else int(rng.integers(60, 250)))
+        if k % 3 == 0:
+            w, h = rng.uniform(20.0, 0.5 * width, 2)
+            canvas.push_rect(c[0] - w / 2, c[1] - h / 2, c[0] + w / 2, c[1] + h / 2, float(rng.uniform(0.0, 16.0)),
+                             brush)
+        elif k % 3 == 1:
+            path = _cubic_path(rng, int(rng.integers(3, 8)), (0.0, 0.0), tuple(size), True)
+            canvas.push_path(path, brush, transform=(cos, sin, -sin, cos, 0.0, 0.0) if k % 2 else None,
+                             rule=FillRule.EvenOdd if k % 2 else FillRule.NonZero)
+        else:
+            path = _cubic_path(rng, int(rng.integers(3, 8)), (0.0, 0.0), tuple(size), False)
+            canvas.push_stroke(path, float(rng.uniform(3.0, 14.0)), brush)
+    clip = _cubic_path(rng, 6, (0.2 * width, 0.2 * height), (0.8 * width, 0.8 * height), True)
+    canvas.push_clip_layer(clip)
+    canvas.push_image(0.0, 0.0, width, height, photo, extend=2)
+    canvas.push_rect(0.3 * width, 0.3 * height, 0.5 * width, 0.6 * height, 6.0, _color(rng))
+    canvas.pop_layer()
     return canvas
 
 

@@ -6,11 +6,19 @@ examples/common/svg.rs:9-52 does): shapes and paths become BezPaths in user
 units, filled (fill, fill-opacity, fill-rule) and then stroked (stroke,
 stroke-opacity, stroke-width, stroke-linejoin, stroke-linecap,
 stroke-miterlimit), each under its accumulated transform times the viewBox
-mapping.  Elements this subset does not take (text, images, gradients,
-patterns, clipping, masks, filters, markers, dashes, group opacity, use)
-raise SvgUnsupported instead of being dropped silently.
+mapping.  Raster images (PNG, JPEG, GIF, WebP: embedded as base64 data URIs
+or files beside the document) become image brushes on a rect path, as
+src/svg/scene_image.rs:25-83 lowers usvg's image node, with the node's
+placement computed as usvg computes it, in f32 (x, y, width, height, auto
+sizes, preserveAspectRatio with a clip layer for slice, image-rendering).
+Elements this subset does not take (text, SVG images, gradients, patterns,
+clipping, masks, filters, markers, dashes, group opacity, use) raise
+SvgUnsupported instead of being dropped silently.
 """
+import base64
+import io
 import math
+import os
 import re
 import xml.etree.ElementTree as ET
 
@@ -30,10 +38,11 @@ COLORS = {
     "purple": (128, 0, 128), "orange": (255, 165, 0), "pink": (255, 192, 203), "brown": (165, 42, 42),
     "seagreen": (46, 139, 87), "lightblue": (173, 216, 230), "darkgreen": (0, 100, 0),
 }
-UNSUPPORTED = {"text", "image", "use", "linearGradient", "radialGradient", "pattern", "clipPath", "mask", "filter",
+UNSUPPORTED = {"text", "use", "linearGradient", "radialGradient", "pattern", "clipPath", "mask", "filter",
                "marker", "symbol", "switch", "foreignObject", "style", "a", "tspan", "textPath"}
 INHERITED = ("fill", "fill-opacity", "fill-rule", "stroke", "stroke-opacity", "stroke-width", "stroke-linejoin",
-             "stroke-linecap", "stroke-miterlimit", "visibility", "font-size")
+             "stroke-linecap", "stroke-miterlimit", "visibility", "font-size", "image-rendering")
+XLINK_HREF = "{http://www.w3.org/1999/xlink}href"
 
 
 class SvgUnsupported(ValueError):
@@ -125,6 +134,134 @@ def _transform(text):
             raise SvgUnsupported(f"transform {name}")
         t = _mul(t, m)
     return t
+
+
+def _f32(v):
+    return float(np.float32(v))
+
+
+def _to_f32(t):
+    """An f64 transform as the f32 transform usvg keeps (each coefficient rounded)."""
+    return tuple(_f32(v) for v in t)
+
+
+def _concat_f32(a, b):
+    """a after b in f32, as tiny-skia's Transform::pre_concat: identity shortcuts,
+    separate f32 products for scale-and-translate pairs, else each pair of
+    products summed in f64 and rounded once, then the translation added in f32."""
+    if a == IDENTITY:
+        return b
+    if b == IDENTITY:
+        return a
+    f = np.float32
+    A, B = [f(v) for v in a], [f(v) for v in b]
+    if a[1] == 0 and a[2] == 0 and b[1] == 0 and b[2] == 0:
+        return (float(A[0] * B[0]), 0.0, 0.0, float(A[3] * B[3]), float(A[0] * B[4] + A[4]),
+                float(A[3] * B[5] + A[5]))
+
+    def mam(p, q, r, t):
+        return f(float(p) * float(q) + float(r) * float(t))
+
+    return (float(mam(A[0], B[0], A[2], B[1])), float(mam(A[1], B[0], A[3], B[1])),
+            float(mam(A[0], B[2], A[2], B[3])), float(mam(A[1], B[2], A[3], B[3])),
+            float(mam(A[0], B[4], A[2], B[5]) + A[4]), float(mam(A[1], B[4], A[3], B[5]) + A[5]))
+
+
+def _inverse(t):
+    """The inverse of an f64 affine in kurbo's order of operations; None when it
+    is not invertible (inverse_affine, src/svg/geometry.rs:143-153)."""
+    det = t[0] * t[3] - t[1] * t[2]
+    if not math.isfinite(det) or abs(det) <= 2.220446049250313e-16:
+        return None
+    inv = 1.0 / det
+    out = (inv * t[3], -inv * t[1], -inv * t[2], inv * t[0], inv * (t[2] * t[5] - t[3] * t[4]),
+           inv * (t[1] * t[4] - t[0] * t[5]))
+    return out if all(math.isfinite(v) for v in out) else None
+
+
+# --- raster images ----------------------------------------------------------------------
+
+
+def _image_bytes(href, base_dir):
+    """The encoded bytes an image href names, or None where usvg skips the image
+    (a remote URL, a file that is not there)."""
+    href = href.strip()
+    if href.startswith("data:"):
+        head, sep, data = href[5:].partition(",")
+        if not sep or ";base64" not in head:
+            raise SvgUnsupported("image data URI without base64")
+        return base64.b64decode(re.sub(r"\s+", "", data))
+    if re.match(r"^[A-Za-z][A-Za-z0-9+.-]*://", href) or base_dir is None:
+        return None
+    path = os.path.join(base_dir, href)
+    if not os.path.isfile(path):
+        return None
+    with open(path, "rb") as f:
+        return f.read()
+
+
+def decode_image(data):
+    """A tk.Image of encoded raster bytes: PNG with 16-bit channels cut to their
+    high byte and every colour type expanded to RGBA8 (decode_png_image,
+    src/svg/image.rs:11-59), JPEG, GIF (first frame) and WebP as RGBA8
+    (decode_encoded_image, :61-80), each pixel premultiplied with mul_div255."""
+    from PIL import Image as PILImage
+    if data[:4] != b"\x89PNG" and data[:3] != b"\xff\xd8\xff" and data[:4] != b"GIF8" and not (
+            data[:4] == b"RIFF" and data[8:12] == b"WEBP"):
+        raise SvgUnsupported("image format (only PNG, JPEG, GIF and WebP rasters)")
+    im = PILImage.open(io.BytesIO(data))
+    im.seek(0)
+    if im.mode in ("I;16", "I;16B", "I;16L", "I"):
+        grey = (np.asarray(im, dtype=np.uint32) >> 8).astype(np.uint8)  # 16-bit grey: the high byte
+        rgba = np.stack([grey, grey, grey, np.full_like(grey, 255)], -1)
+    else:
+        rgba = np.asarray(im.convert("RGBA"), dtype=np.uint8)
+    return tk.Image.from_rgba8(rgba)
+
+
+def _aspect(text):
+    """(align, slice) of a preserveAspectRatio value; align 'none' or (x, y) in {0, 1, 2}."""
+    words = (text or "").split()
+    if words and words[0] == "defer":
+        words = words[1:]
+    if not words:
+        return (1, 1), False
+    if words[0] == "none":
+        return "none", False
+    m = re.fullmatch(r"x(Min|Mid|Max)Y(Min|Mid|Max)", words[0])
+    if not m:
+        return (1, 1), False
+    pos = {"Min": 0, "Mid": 1, "Max": 2}
+    return (pos[m.group(1)], pos[m.group(2)]), len(words) > 1 and words[1] == "slice"
+
+
+def _image_placement(actual, x, y, width, height, aspect):
+    """usvg's image node (converter/image.rs convert_inner): the image transform
+    (x', y') = (sx x + tx, sy y + ty) taking image pixels to user space, in f32,
+    and the viewport rect (x, y, w, h) a slice clips to."""
+    f = np.float32
+    iw, ih = f(actual[0]), f(actual[1])
+    x, y, w, h = f(x), f(y), f(width), f(height)
+    rw, rh = f(f(x + w) - x), f(f(y + h) - y)  # NonZeroRect::from_xywh keeps right and bottom
+    align, slice_ = aspect
+    if align == "none":
+        aw, ah = rw, rh
+    else:
+        # Size::scale_to / expand_to
+        scaled_w = f(f(rh * iw) / ih)
+        keep_height = scaled_w <= rw if slice_ else scaled_w >= rw
+        if not keep_height:
+            aw, ah = scaled_w, rh
+        else:
+            aw, ah = rw, f(f(rw * ih) / iw)
+    ax, ay = x, y
+    if align != "none":
+        dx, dy = f(rw - aw), f(rh - ah)
+        ax = x if align[0] == 0 else (f(x + f(dx / f(2))) if align[0] == 1 else f(x + dx))
+        ay = y if align[1] == 0 else (f(y + f(dy / f(2))) if align[1] == 1 else f(y + dy))
+    vw, vh = f(f(ax + aw) - ax), f(f(ay + ah) - ay)
+    ts = (float(f(vw / iw)), 0.0, 0.0, float(f(vh / ih)), float(ax), float(ay))
+    return ts, (float(x), float(y), float(rw), float(rh))
 
 
 # --- path data ------------------------------------------------------------------------
@@ -361,10 +498,13 @@ def _local(tag):
 
 
 class Document:
-    """A parsed SVG: size (width, height) and the draw list (path, transform,
-    fill (rgba, rule) or None, stroke (rgba, style) or None) in painter order."""
+    """A parsed SVG: size (width, height) and the draw list in painter order:
+    ("shape", path, transform, fill (rgba, rule) or None, stroke (rgba, style) or
+    None), ("image", tk.Image, transform, sampling), ("clip", path, transform) and
+    ("pop",).  Relative image hrefs resolve against base_dir."""
 
-    def __init__(self, text):
+    def __init__(self, text, base_dir=None):
+        self.base_dir = base_dir
         root = ET.fromstring(text)
         if _local(root.tag) != "svg":
             raise SvgUnsupported("not an svg document")
@@ -389,9 +529,9 @@ class Document:
                 view = (s, 0.0, 0.0, s, tx, ty)
         self.view_w, self.view_h = (vb[2], vb[3]) if vb else (self.width, self.height)
         self.draws = []
-        self._walk(root, {}, view)
+        self._walk(root, {}, view, _to_f32(view))
 
-    def _walk(self, el, parent, transform):
+    def _walk(self, el, parent, transform, transform32):
         tag = _local(el.tag)
         if tag in ("title", "desc", "metadata", "defs"):
             return
@@ -401,12 +541,17 @@ class Document:
         if a.get("display") == "none":
             return
         t = _mul(transform, _transform(a.get("transform"))) if "transform" in el.attrib else transform
+        t32 = _concat_f32(transform32, _to_f32(_transform(a.get("transform")))) if "transform" in el.attrib \
+            else transform32
         if tag in ("svg", "g"):
             if tag == "svg" and self.draws is not None and getattr(self, "_entered", False):
                 raise SvgUnsupported("nested svg")
             self._entered = True
             for child in el:
-                self._walk(child, a, t)
+                self._walk(child, a, t, t32)
+            return
+        if tag == "image":
+            self._image(el, a, t32)
             return
         path = self._shape(tag, a)
         if path is None or a.get("visibility") in ("hidden", "collapse"):
@@ -419,7 +564,7 @@ class Document:
             fill = fill if tag == "polyline" else None
         rule = tk.FillRule.EvenOdd if a.get("fill-rule") == "evenodd" else tk.FillRule.NonZero
         if fill is not None and fill_op > 0:
-            self.draws.append((path, t, (fill, min(fill_op, 1.0), rule), None))
+            self.draws.append(("shape", path, t, (fill, min(fill_op, 1.0), rule), None))
         if stroke is not None and stroke_op > 0:
             width = _length(a.get("stroke-width", "1"), math.hypot(self.view_w, self.view_h) / math.sqrt(2))
             if width > 0:
@@ -430,7 +575,51 @@ class Document:
                 join = {"round": tk.Join.Round, "bevel": tk.Join.Bevel}.get(jv, tk.Join.Miter)
                 cap = {"round": tk.Cap.Round, "square": tk.Cap.Square}.get(a.get("stroke-linecap", "butt"), tk.Cap.Butt)
                 limit = max(_number(a.get("stroke-miterlimit"), 4.0), 1.0)
-                self.draws.append((path, t, None, (stroke, min(stroke_op, 1.0), width, join, cap, limit)))
+                self.draws.append(("shape", path, t, None, (stroke, min(stroke_op, 1.0), width, join, cap, limit)))
+
+    def _image(self, el, a, t32):
+        """usvg's image conversion (converter/image.rs) and the lowering of
+        src/svg/scene_image.rs:25-83: the image's pixels on a rect path of its
+        pixel size, under the node's f32 transform and placement."""
+        if a.get("visibility") in ("hidden", "collapse"):
+            return
+        href = el.get(XLINK_HREF) or el.get("href")
+        if not href:
+            return
+        data = _image_bytes(href, self.base_dir)
+        if data is None:
+            return
+        image = decode_image(data)
+        fs = _length(a["font-size"]) if "font-size" in a else FONT_SIZE
+        W, H = self.view_w, self.view_h
+
+        def given(k):
+            v = a.get(k)
+            if v is None or v.strip() == "auto":
+                return None
+            return _length(v, W if k in ("x", "width") else H, fs)
+
+        x, y = given("x") or 0.0, given("y") or 0.0
+        w, h = given("width"), given("height")
+        f = np.float32
+        iw, ih = f(image.width), f(image.height)
+        if w is not None and h is None:
+            h = float(ih * f(f(w) / iw))
+        elif h is not None and w is None:
+            w = float(iw * f(f(h) / ih))
+        w = float(iw) if w is None else w
+        h = float(ih) if h is None else h
+        if not (w > 0 and h > 0):
+            return  # usvg skips an image of invalid size
+        aspect = _aspect(a.get("preserveAspectRatio"))
+        ts, (vx, vy, vw, vh) = _image_placement((image.width, image.height), x, y, w, h, aspect)
+        rendering = a.get("image-rendering", "optimizeQuality").strip()
+        nearest = rendering in ("optimizeSpeed", "crisp-edges", "pixelated")
+        if aspect[1]:  # a slice clips to the viewport rect, in the parent's space
+            self.draws.append(("clip", _rect_path(vx, vy, vw, vh, 0.0, 0.0), t32))
+        self.draws.append(("image", image, _concat_f32(t32, ts), tk.PATTERN_NEAREST if nearest else tk.PATTERN_BILINEAR))
+        if aspect[1]:
+            self.draws.append(("pop",))
 
     def _shape(self, tag, a):
         fs = _length(a["font-size"]) if "font-size" in a else FONT_SIZE
@@ -491,13 +680,34 @@ def render_size(doc, target_width):
     return target_width, max(nh, 1), target_width / w, nh / h
 
 
-def load(text, target_width=300):
-    """Records an SVG document on a new Canvas at the target width; returns the canvas."""
-    doc = Document(text)
+def load(text, target_width=300, base_dir=None):
+    """Records an SVG document on a new Canvas at the target width; returns the
+    canvas.  Relative image hrefs resolve against base_dir (none are read without it)."""
+    doc = Document(text, base_dir)
     width, height, sx, sy = render_size(doc, target_width)
     canvas = tk.Canvas(width, height)
     scale = (sx, 0.0, 0.0, sy, 0.0, 0.0)
-    for path, t, fill, stroke in doc.draws:
+    for entry in doc.draws:
+        if entry[0] == "image":
+            _, image, abs32, sampling = entry
+            m = _mul(scale, abs32)
+            world_to_local = _inverse(m)
+            if world_to_local is None:
+                raise SvgUnsupported("non-invertible image transform")
+            # PatternBrush::new_resource(scale(1 / w, 1 / h) * world_to_local, Pad, sampling, 255)
+            brush_xf = _mul((1.0 / image.width, 0.0, 0.0, 1.0 / image.height, 0.0, 0.0), world_to_local)
+            image_id = canvas.add_image(image)
+            brush = tk.ImageBrush(image_id, tuple(_f32(v) for v in brush_xf), extend=0, sampling=sampling)
+            canvas.push_path(_rect_path(0.0, 0.0, float(image.width), float(image.height), 0.0, 0.0), brush, m,
+                             tk.FillRule.NonZero, TOLERANCE)
+            continue
+        if entry[0] == "clip":
+            canvas.push_clip_layer(entry[1], _mul(scale, entry[2]), tk.FillRule.NonZero, TOLERANCE)
+            continue
+        if entry[0] == "pop":
+            canvas.pop_layer()
+            continue
+        _, path, t, fill, stroke = entry
         m = _mul(scale, t)
         if fill is not None:
             (r, g, b), op, rule = fill
