@@ -511,6 +511,25 @@ void tk_batch_destroy(tk_batch *batch);
 /* Synchronises and copies the RGBA8 image (premultiplied, row-major, tightly
  * packed little-endian u32, src/native/submission.rs:125-146). */
 tk_status tk_frame_read_image(tk_frame *frame, uint32_t *rgba_out, size_t pixel_count);
+
+/* Asynchronous submission with a completion receipt: NativeRenderer's submit and
+ * NativeImageSubmission::{is_complete, readback} (src/native/submission.rs:100-146)
+ * over the adapter's Receipt (src/native/runtime/adapter.rs:17-47, 159-250).
+ * tk_frame_submit enqueues a render of the frame's current scene and a copy of its
+ * image and status words into pinned host memory owned by the submission, then
+ * returns without waiting.  Later renders of the same frame do not disturb an
+ * earlier submission's image.  A batch's frame is refused (read its scenes with
+ * tk_batch_read_image). */
+typedef struct tk_submission tk_submission;
+tk_status tk_frame_submit(tk_frame *frame, tk_submission **out);
+/* *complete = 1 once the submission's work has finished, else 0; never waits.
+ * A device error is TK_ERR_HIP (the reference's Unconfirmed). */
+tk_status tk_submission_is_complete(tk_submission *submission, uint32_t *complete);
+/* Waits for the submission, then copies its image.  A frame that exceeded a
+ * capacity is TK_ERR_OVERFLOW (the reference's Rejected); the image is not copied. */
+tk_status tk_submission_readback(tk_submission *submission, uint32_t *rgba_out, size_t pixel_count);
+/* Waits for the submission if it is still running, then frees it. */
+void tk_submission_destroy(tk_submission *submission);
 /* Device address of the RGBA8 image. */
 void *tk_frame_image_device(tk_frame *frame);
 

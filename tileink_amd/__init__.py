@@ -656,6 +656,12 @@ class Frame:
         check(lib().tk_frame_read_image(self._h, ptr(out), out.size))
         return out
 
+    def submit(self):
+        """Enqueues a render and returns its Submission without waiting (tk_frame_submit)."""
+        h = C.c_void_p()
+        check(lib().tk_frame_submit(self._h, C.byref(h)))
+        return Submission(h, self.width, self.height, self)
+
     def stats(self):
         out = _abi.FrameStats()
         check(lib().tk_frame_stats_read(self._h, C.byref(out)))
@@ -706,14 +712,36 @@ class Batch:
         return out
 
 
-class ImageSubmission:
-    """Result of render_to_image (src/native/submission.rs:120-146)."""
+class Submission:
+    """A submitted frame and its completion receipt (NativeImageSubmission,
+    src/native/submission.rs:100-146): is_complete() never waits; readback()
+    waits, then returns the (H, W) premultiplied RGBA8 image, raising a
+    TileinkError with TK_ERR_OVERFLOW when the frame exceeded a capacity."""
 
-    def __init__(self, rgba):
-        self._rgba = rgba
+    def __init__(self, handle, width, height, owner=None):
+        self._h, self.width, self.height = handle, width, height
+        self._owner = owner  # keeps the frame (and its context) alive while the work runs
+
+    def is_complete(self):
+        done = C.c_uint32()
+        check(lib().tk_submission_is_complete(self._h, C.byref(done)))
+        return bool(done.value)
 
     def readback(self):
-        return self._rgba
+        out = np.empty((self.height, self.width), dtype=np.uint32)
+        check(lib().tk_submission_readback(self._h, ptr(out), out.size))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().tk_submission_destroy(h)
+            self._h = None
+
+
+class ImageSubmission(Submission):
+    """Result of render_to_image (src/native/submission.rs:100-146): the frame is
+    submitted when it is returned and read back on readback()."""
 
 
 class NativeRenderer:
@@ -730,9 +758,10 @@ class NativeRenderer:
         if (scene.width, scene.height) != (self.width, self.height):
             raise TileinkError(_abi.TK_ERR_INVALID,
                                f"canvas is {scene.width}x{scene.height} physical pixels, renderer {self.width}x{self.height}")
-        out = np.empty((scene.height, scene.width), dtype=np.uint32)
-        check(lib().tk_render_to_image(self.context.handle, C.byref(scene), ptr(out), out.size))
-        return ImageSubmission(out)
+        frame = Frame(self.context, scene)
+        h = C.c_void_p()
+        check(lib().tk_frame_submit(frame._h, C.byref(h)))
+        return ImageSubmission(h, scene.width, scene.height, frame)
 
 
 def line_scanned_tile_count(p0, p1, bbox):

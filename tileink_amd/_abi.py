@@ -238,6 +238,10 @@ SIGNATURES = {
     "tk_batch_frame": (_P, [_P]),
     "tk_batch_destroy": (None, [_P]),
     "tk_frame_read_image": (C.c_int, [_P, _P, _SZ]),
+    "tk_frame_submit": (C.c_int, [_P, _P]),
+    "tk_submission_is_complete": (C.c_int, [_P, _P]),
+    "tk_submission_readback": (C.c_int, [_P, _P, _SZ]),
+    "tk_submission_destroy": (None, [_P]),
     "tk_frame_image_device": (_P, [_P]),
     "tk_frame_read_buffer": (C.c_int, [_P, C.c_int, _P, _SZ]),
     "tk_frame_stats_read": (C.c_int, [_P, C.POINTER(FrameStats)]),

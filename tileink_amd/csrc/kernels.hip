@@ -1826,7 +1826,7 @@ __device__ __forceinline__ void fine_tile(const tk_args_fine &a) {
                             }
                         }
                     }
-                    // Image, glyph, opacity and blend records are not produced on this path.
+                    // Glyph, opacity and blend records are not produced on this path.
                 }
             } else if (kind == TK_TILE_KIND_COLOR) {
                 for (uint32_t pi = pstart; pi < pend; ++pi) {

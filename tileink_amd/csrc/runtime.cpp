@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <cmath>
 #include <initializer_list>
 #include <cstring>
@@ -2147,6 +2148,70 @@ tk_status tk_frame_read_image(tk_frame *f, uint32_t *rgba, size_t pixel_count) {
 }
 
 void *tk_frame_image_device(tk_frame *f) { return f ? f->image : nullptr; }
+
+// --- asynchronous submissions -------------------------------------------------
+
+struct tk_submission {
+    hipEvent_t done = nullptr;
+    uint32_t *pixels = nullptr;  // pinned: the image as of this submission
+    uint32_t *status = nullptr;  // pinned: the frame's four status words
+    uint64_t pixel_count = 0;
+    uint64_t ptcl_capacity = 0;
+};
+
+tk_status tk_frame_submit(tk_frame *f, tk_submission **out) {
+    if (!f || !out) return fail(TK_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (f->scene_count) return fail(TK_ERR_INVALID, "a batch's frame is read with tk_batch_read_image");
+    TK_TRY(tk_frame_render(f));
+    std::unique_ptr<tk_submission, void (*)(tk_submission *)> sub(new tk_submission, tk_submission_destroy);
+    sub->pixel_count = uint64_t(f->meta.width) * f->meta.height;
+    sub->ptcl_capacity = f->ptcl_capacity;
+    TK_HIP(hipHostMalloc((void **)&sub->pixels, std::max<uint64_t>(sub->pixel_count, 1) * 4, hipHostMallocDefault));
+    TK_HIP(hipHostMalloc((void **)&sub->status, 16, hipHostMallocDefault));
+    TK_HIP(hipEventCreateWithFlags(&sub->done, hipEventDisableTiming));
+    hipStream_t st = f->ctx->stream;
+    if (sub->pixel_count)
+        TK_HIP(hipMemcpyAsync(sub->pixels, f->image, sub->pixel_count * 4, hipMemcpyDeviceToHost, st));
+    TK_HIP(hipMemcpyAsync(sub->status, f->status, 16, hipMemcpyDeviceToHost, st));
+    TK_HIP(hipEventRecord(sub->done, st));
+    *out = sub.release();
+    return TK_OK;
+}
+
+tk_status tk_submission_is_complete(tk_submission *s, uint32_t *complete) {
+    if (!s || !complete) return fail(TK_ERR_INVALID, "null argument");
+    const hipError_t e = hipEventQuery(s->done);
+    if (e == hipErrorNotReady) {
+        *complete = 0u;
+        return TK_OK;
+    }
+    TK_HIP(e);
+    *complete = 1u;
+    return TK_OK;
+}
+
+tk_status tk_submission_readback(tk_submission *s, uint32_t *rgba, size_t pixel_count) {
+    if (!s || (!rgba && s->pixel_count)) return fail(TK_ERR_INVALID, "null argument");
+    if (pixel_count < s->pixel_count) return fail(TK_ERR_INVALID, "output buffer too small");
+    TK_HIP(hipEventSynchronize(s->done));
+    const uint32_t *w = s->status;
+    if (w[0] | w[2] | w[3] || w[1] > s->ptcl_capacity)
+        return fail(TK_ERR_OVERFLOW, "a pipeline capacity was exceeded");
+    if (s->pixel_count) std::memcpy(rgba, s->pixels, s->pixel_count * 4);
+    return TK_OK;
+}
+
+void tk_submission_destroy(tk_submission *s) {
+    if (!s) return;
+    if (s->done) {
+        (void)hipEventSynchronize(s->done);  // the copies into the pinned buffers end first
+        (void)hipEventDestroy(s->done);
+    }
+    if (s->pixels) (void)hipHostFree(s->pixels);
+    if (s->status) (void)hipHostFree(s->status);
+    delete s;
+}
 
 tk_status tk_frame_read_buffer(tk_frame *f, tk_frame_buffer which, void *out, size_t bytes) {
     if (!f || !out) return fail(TK_ERR_INVALID, "null argument");
