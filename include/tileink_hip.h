@@ -284,6 +284,14 @@ tk_status tk_canvas_push_stroke_styled(tk_canvas *canvas, const uint8_t *verbs, 
                                        const double *points, size_t point_count, const tk_stroke_style *style,
                                        const tk_brush *brush, const double *transform, uint32_t fill_rule,
                                        double tolerance, uint32_t *draw_id);
+/* The outlines of a laid-out text run as one path with text compositing
+ * (Canvas::push_text_layout_as_path, src/canvas/primitives/text.rs:104-146, after
+ * its shaping: a nonzero path draw tagged DRAW_PATH_GLYPH, composited by
+ * src_over_mask_linear_auto_u8, fine/text/auto.hlsli:28-33).  The caller passes
+ * the outline; an empty one records nothing (*draw_id = TK_INVALID_INDEX). */
+tk_status tk_canvas_push_text_outline(tk_canvas *canvas, const uint8_t *verbs, size_t verb_count,
+                                      const double *points, size_t point_count, const tk_brush *brush,
+                                      const double *transform, double tolerance, uint32_t *draw_id);
 /* Canvas::push_rect (src/canvas/primitives/rectangles.rs:33-43); radii = tl,tr,bl,br. */
 tk_status tk_canvas_push_rect(tk_canvas *canvas, double x0, double y0, double x1, double y1,
                               const float radii[4], const tk_brush *brush, uint32_t *draw_id);

@@ -255,6 +255,16 @@ def pixel_math_words(requests):
     return out
 
 
+def text_words(requests):
+    """Text compositing of path glyphs per request {destination, source, coverage,
+    foreground f32 bits}: (n, 4) words {composite, corrected coverage bits,
+    srgb_to_linear bits, linear_to_srgb bits}."""
+    requests = np.ascontiguousarray(requests, dtype=np.uint32).reshape(-1, 4)
+    out = np.zeros((requests.shape[0], 4), np.uint32)
+    lib().or_text_words(C.c_uint32(requests.shape[0]), _p(requests), _p(out))
+    return out
+
+
 def brush_words(paint, paint_brush_base, requests):
     """sample_brush at each request {data_base, x_bits, y_bits, 0} (validation/brush.hlsl:14-18)."""
     paint = np.ascontiguousarray(paint, dtype=np.uint32).reshape(-1)

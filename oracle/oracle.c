@@ -686,6 +686,77 @@ static uint32_t lerp_premul_u8(uint32_t a, uint32_t b, float t) {
     return pack4(v[0], v[1], v[2], v[3]);
 }
 static float rem_euclid_f32(float v, float m) { return fmaf(-floorf(v / m), m, v); }
+
+/* ---------- text compositing of path glyphs ---------------------------------------
+ * srgb_to_linear / linear_to_srgb (fine/text/gamma.hlsli:5-21), text_foreground and
+ * text_correcting_coverage (fine/text/auto.hlsli:6-26), src_over_subpixel_mask_f32
+ * with one coverage for the three channels (fine/text/basic.hlsli:101-115) and
+ * src_over_mask_linear_auto_u8 (auto.hlsli:28-33).  lerp(a, b, t) is
+ * fmaf(b - a, t, a); dot() sums its products in order. */
+static float srgb_to_linear(float value) {
+    float v = clampf(value, 0.0f, 1.0f);
+    float result = v / 12.92f;
+    if (v > 0.04045f) result = powf((v + 0.055f) / 1.055f, 2.4f);
+    return result;
+}
+static float linear_to_srgb(float value) {
+    float v = clampf(value, 0.0f, 1.0f);
+    float result = v * 12.92f;
+    if (v > 0.0031308f) result = 1.055f * powf(v, 1.0f / 2.4f) - 0.055f;
+    return result;
+}
+static float text_foreground_luma(uint32_t source) {
+    float alpha = (float)(source >> 24);
+    if (!(alpha > 0.0f)) return 0.0f;
+    float r = clampf((float)(source & 255u) / alpha, 0.0f, 1.0f);
+    float g = clampf((float)((source >> 8) & 255u) / alpha, 0.0f, 1.0f);
+    float b = clampf((float)((source >> 16) & 255u) / alpha, 0.0f, 1.0f);
+    return r * 0.2126f + g * 0.7152f + b * 0.0722f;
+}
+static float text_correcting_coverage(uint32_t coverage, float foreground) {
+    if (coverage == 0u || coverage == 255u) return (float)coverage * (1.0f / 255.0f);
+    float raw = (float)coverage * (1.0f / 255.0f);
+    float background = 1.0f - foreground;
+    float bg_linear = srgb_to_linear(background);
+    float enhanced = raw + raw * (1.0f - raw) * bg_linear;
+    float difference = foreground - background;
+    if (fabsf(difference) < (1.0f / 256.0f)) return enhanced;
+    float fg_linear = srgb_to_linear(foreground);
+    float output = linear_to_srgb(fmaf(fg_linear - bg_linear, enhanced, bg_linear));
+    return clampf((output - background) / difference, 0.0f, 1.0f);
+}
+static uint32_t src_over_mask_f32(uint32_t destination, uint32_t source, float coverage) {
+    if ((source >> 24) == 0u || coverage == 0.0f) return destination;
+    float cov = coverage;
+    if ((destination >> 24) != 255u) cov = (coverage + coverage + coverage) / 3.0f;
+    float alpha = (float)(source >> 24) * (1.0f / 255.0f);
+    float amount = alpha * cov;
+    uint32_t out = 0u;
+    for (int k = 0; k < 3; ++k) {
+        float sv = (float)((source >> (8 * k)) & 255u), dv = (float)((destination >> (8 * k)) & 255u);
+        out |= f2u(sv * cov + dv * (1.0f - amount) + 0.5f) << (8 * k);
+    }
+    float out_alpha = amount * 255.0f + (float)(destination >> 24) * (1.0f - amount);
+    return out | (f2u(out_alpha + 0.5f) << 24);
+}
+static uint32_t src_over_mask_linear_auto_u8(uint32_t destination, uint32_t source, uint32_t coverage) {
+    if ((source >> 24) == 0u || coverage == 0u) return destination;
+    return src_over_mask_f32(destination, source, text_correcting_coverage(coverage, text_foreground_luma(source)));
+}
+
+/* Per request {destination, source, coverage, foreground bits}: the composite, the
+ * corrected coverage, srgb_to_linear(foreground) and linear_to_srgb(foreground). */
+void or_text_words(uint32_t count, const uint32_t *requests, uint32_t *out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t *r = requests + (size_t)i * 4u;
+        uint32_t *o = out + (size_t)i * 4u;
+        float fg = bits_f(r[3]);
+        o[0] = src_over_mask_linear_auto_u8(r[0], r[1], r[2]);
+        o[1] = f_bits(text_correcting_coverage(r[2], fg));
+        o[2] = f_bits(srgb_to_linear(fg));
+        o[3] = f_bits(linear_to_srgb(fg));
+    }
+}
 static int premul_opaque(uint32_t c) { return (c >> 24) == 255u; }
 
 /* ---------- row-sweep coverage (shared/coverage.hlsli:7-51) --------------------- */
@@ -1680,7 +1751,7 @@ static Px fine_pixel(const or_fine_config *cfg, const uint32_t *draws, const uin
             } else {
                 clip_mask = 255u;
             }
-        } else if (tag == 12u || tag == 1u || tag == 3u) {
+        } else if (tag == 12u || tag == 1u || tag == 3u || tag == 11u) {
             uint32_t parent = clip_mask;
             uint32_t alpha;
             if (tag == 12u) {
@@ -1692,12 +1763,14 @@ static Px fine_pixel(const or_fine_config *cfg, const uint32_t *draws, const uin
             } else {
                 alpha = coverage_to_alpha(fill_value_at(segs, (int32_t)r[1], r[3], r[4], lx, ly), r[2]);
             }
-            if (tag == 1u) {
+            if (tag == 1u || tag == 11u) {
                 uint32_t masked = combine_alpha(alpha, clip_mask);
                 if (masked != 0u) {
                     FDraw d = load_fdraw(draws, cfg->draw_count, r[5]);
                     uint32_t color = draw_brush(cfg, paint, &d, sx, sy);
-                    if (masked == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
+                    if (tag == 11u) /* PTCL_PATH_GLYPH (fine/interpreter.hlsli:205-206) */
+                        pixel = rgba8_to_unorm(src_over_mask_linear_auto_u8(unorm_to_rgba8(pixel), color, masked));
+                    else if (masked == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
                     else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, masked));
                 }
             } else {

@@ -134,6 +134,11 @@ void or_fill_coverage_words(uint32_t count, const uint32_t *requests, const uint
 void or_fill_coverage_words_sequential(uint32_t count, const uint32_t *requests, const uint32_t *segments,
                                        uint32_t *alpha_out, float *coverage_out);
 void or_pixel_math_words(uint32_t count, const uint32_t *requests, uint32_t *out);
+/* Text compositing of path glyphs, per request {destination, source, coverage,
+ * foreground f32 bits}: four words out {src_over_mask_linear_auto_u8,
+ * text_correcting_coverage(coverage, foreground) bits, srgb_to_linear(foreground)
+ * bits, linear_to_srgb(foreground) bits}. */
+void or_text_words(uint32_t count, const uint32_t *requests, uint32_t *out);
 /* brush_words: sample_brush of the brush at word q[0] of the brush blob
  * (paint + paint_brush_base) at (bits q[1], bits q[2]); 4 request words each. */
 void or_brush_words(const uint32_t *paint, uint32_t paint_words, uint32_t paint_brush_base, uint32_t count,

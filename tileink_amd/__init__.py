@@ -324,6 +324,19 @@ class Canvas:
         self._scene = None
         return out.value
 
+    def push_text_outline(self, path, brush, transform=None, tolerance=0.25):
+        """The outlines of a laid-out text run as one nonzero path with text
+        compositing (Canvas::push_text_layout_as_path after shaping,
+        src/canvas/primitives/text.rs:104-146); None for an empty outline."""
+        verbs, points = self._path_args(path)
+        b, keep = _brush_arg(brush)
+        tp, tarr = _affine_arg(transform)
+        out = C.c_uint32()
+        check(lib().tk_canvas_push_text_outline(self._h, ptr(verbs), len(verbs), ptr(points), len(points), C.byref(b),
+                                                tp, float(tolerance), C.byref(out)))
+        self._scene = None
+        return None if out.value == _abi.INVALID else out.value
+
     def push_polygon(self, points, brush, rule=FillRule.NonZero):
         """A closed polygon, move_to + line_to per point + close_path, from an (n, 2) array."""
         pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 2))

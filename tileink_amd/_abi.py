@@ -194,6 +194,7 @@ SIGNATURES = {
     "tk_canvas_create": (C.c_int, [_U32, _U32, C.c_double, C.POINTER(_P)]),
     "tk_canvas_destroy": (None, [_P]),
     "tk_canvas_push_path": (C.c_int, [_P, _P, _SZ, _P, _SZ, _P, _P, _U32, C.c_double, C.POINTER(_U32)]),
+    "tk_canvas_push_text_outline": (C.c_int, [_P, _P, _SZ, _P, _SZ, _P, _P, C.c_double, C.POINTER(_U32)]),
     "tk_canvas_push_stroke": (C.c_int, [_P, _P, _SZ, _P, _SZ, C.c_double, _P, _P, _U32, C.c_double, C.POINTER(_U32)]),
     "tk_canvas_push_rect": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double, _P, _P, C.POINTER(_U32)]),
     "tk_canvas_add_image": (C.c_int, [_P, _U32, _U32, _P, C.POINTER(_U32)]),

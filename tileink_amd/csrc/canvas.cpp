@@ -1338,7 +1338,7 @@ uint32_t tk_canvas_line_count(const tk_canvas *canvas) { return canvas ? uint32_
 static tk_status push_path_common(tk_canvas *c, const uint8_t *verbs, size_t verb_count, const double *points,
                                   size_t point_count, double stroke_width, bool stroke, const tk_brush *brush,
                                   const double *transform, uint32_t fill_rule, double tolerance, uint32_t *draw_id,
-                                  const StrokeStyle &style = StrokeStyle{}) {
+                                  const StrokeStyle &style = StrokeStyle{}, uint32_t tag = TK_DRAW_BRUSH) {
     if (!c) return fail(TK_ERR_INVALID, "null canvas");
     if (fill_rule > 1u) return fail(TK_ERR_INVALID, "fill rule must be 0 (nonzero) or 1 (even-odd)");
     if (!(tolerance > 0.0)) return fail(TK_ERR_INVALID, "tolerance must be positive");
@@ -1353,7 +1353,7 @@ static tk_status push_path_common(tk_canvas *c, const uint8_t *verbs, size_t ver
     device_matrix(c, transform, m);
     BezPath physical = transform_path(path, m);
     tk_status status;
-    uint32_t d = push_physical_path(c, physical, tolerance * c->scale, fill_rule, TK_DRAW_BRUSH, brush, true, &status);
+    uint32_t d = push_physical_path(c, physical, tolerance * c->scale, fill_rule, tag, brush, true, &status);
     if (status != TK_OK) return fail(status, "unsupported brush");
     if (draw_id) *draw_id = d;
     return TK_OK;
@@ -1394,6 +1394,15 @@ tk_status tk_canvas_push_path(tk_canvas *canvas, const uint8_t *verbs, size_t ve
                               double tolerance, uint32_t *draw_id) {
     return push_path_common(canvas, verbs, verb_count, points, point_count, 0.0, false, brush, transform, fill_rule,
                             tolerance, draw_id);
+}
+
+tk_status tk_canvas_push_text_outline(tk_canvas *canvas, const uint8_t *verbs, size_t verb_count, const double *points,
+                                      size_t point_count, const tk_brush *brush, const double *transform,
+                                      double tolerance, uint32_t *draw_id) {
+    if (draw_id) *draw_id = kNone;
+    if (verb_count == 0) return TK_OK;  // an empty outline records nothing
+    return push_path_common(canvas, verbs, verb_count, points, point_count, 0.0, false, brush, transform,
+                            TK_FILL_NONZERO, tolerance, draw_id, StrokeStyle{}, TK_DRAW_PATH_GLYPH);
 }
 
 tk_status tk_canvas_push_stroke(tk_canvas *canvas, const uint8_t *verbs, size_t verb_count, const double *points,

@@ -1796,12 +1796,12 @@ __device__ __forceinline__ void fine_tile(const tk_args_fine &a) {
                         }
                         push_clip(parent);
                         clip_mask = combine_alpha(parent, alpha);
-                    } else if (tag == TK_PTCL_FILL || tag == TK_PTCL_BEGIN_CLIP) {
+                    } else if (tag == TK_PTCL_FILL || tag == TK_PTCL_BEGIN_CLIP || tag == kPtclPathGlyph) {
                         const uint32_t s0 = rec.segment_start, s1 = min(rec.segment_end, cfg.segment_capacity);
                         const uint32_t n = s1 > s0 ? s1 - s0 : 0u;
                         // Issued before the coverage work so that its latency overlaps it.
                         uint32_t brush_flag = 0u, brush_color = 0u;
-                        if (tag == TK_PTCL_FILL && rec.color < cfg.draw_count) {
+                        if (tag != TK_PTCL_BEGIN_CLIP && rec.color < cfg.draw_count) {
                             brush_flag = a.draw_brush[2u * rec.color];
                             brush_color = a.draw_brush[2u * rec.color + 1u];
                         }
@@ -1821,12 +1821,14 @@ __device__ __forceinline__ void fine_tile(const tk_args_fine &a) {
                                 if (brush_flag == 0u)
                                     color = sample_draw_brush(cfg, paint, load_fine_draw(a.in.draws, cfg.draw_count, rec.color),
                                                               sx, sy, atlas_of(a.in));
-                                if (masked == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
+                                if (tag == kPtclPathGlyph)  // text compositing (interpreter.hlsli:205-206)
+                                    pixel = rgba8_to_unorm(src_over_mask_linear_auto_u8(unorm_to_rgba8(pixel), color, masked));
+                                else if (masked == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
                                 else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, masked));
                             }
                         }
                     }
-                    // Glyph, opacity and blend records are not produced on this path.
+                    // Glyph-image, opacity and blend records are not produced on this path.
                 }
             } else if (kind == TK_TILE_KIND_COLOR) {
                 for (uint32_t pi = pstart; pi < pend; ++pi) {

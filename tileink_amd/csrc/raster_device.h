@@ -679,8 +679,69 @@ TKD uint32_t sample_four_corner(const uint32_t *paint, uint32_t count, uint32_t 
     return lerp_premul_u8(lerp_premul_u8(tl, tr, u), lerp_premul_u8(bl, br, u), v);
 }
 
-// sample_brush (shared/brush/sample.hlsli:15-42) for the solid, linear, radial,
-// sweep and four-corner brushes.
+// ---------------------------------------------------------------------------
+// Text compositing of path glyphs (PTCL_PATH_GLYPH, fine/interpreter.hlsli:205-206):
+// fine/text/gamma.hlsli:5-21, fine/text/auto.hlsli:6-33, fine/text/basic.hlsli:101-115.
+// HLSL lerp(a, b, t) is written fmaf(b - a, t, a), as lerp_premul_u8 does; every
+// other product and sum rounds separately; dot() sums its products in order.
+// ---------------------------------------------------------------------------
+
+TKD float srgb_to_linear(float value) {
+    const float v = clampf(value, 0.0f, 1.0f);
+    float result = v / 12.92f;
+    if (v > 0.04045f) result = powf((v + 0.055f) / 1.055f, 2.4f);
+    return result;
+}
+TKD float linear_to_srgb(float value) {
+    const float v = clampf(value, 0.0f, 1.0f);
+    float result = v * 12.92f;
+    if (v > 0.0031308f) result = 1.055f * powf(v, 1.0f / 2.4f) - 0.055f;
+    return result;
+}
+// text_foreground: the straight colour, as a luminance weight (dot with Rec. 709).
+TKD float text_foreground_luma(uint32_t source) {
+    const float alpha = (float)(source >> 24);
+    if (!(alpha > 0.0f)) return 0.0f;
+    const float r = clampf((float)(source & 255u) / alpha, 0.0f, 1.0f);
+    const float g = clampf((float)((source >> 8) & 255u) / alpha, 0.0f, 1.0f);
+    const float b = clampf((float)((source >> 16) & 255u) / alpha, 0.0f, 1.0f);
+    return r * 0.2126f + g * 0.7152f + b * 0.0722f;
+}
+TKD float text_correcting_coverage(uint32_t coverage, float foreground) {
+    if (coverage == 0u || coverage == 255u) return (float)coverage * (1.0f / 255.0f);
+    const float raw = (float)coverage * (1.0f / 255.0f);
+    const float background = 1.0f - foreground;
+    const float bg_linear = srgb_to_linear(background);
+    const float enhanced = raw + raw * (1.0f - raw) * bg_linear;
+    const float difference = foreground - background;
+    if (fabsf(difference) < (1.0f / 256.0f)) return enhanced;
+    const float fg_linear = srgb_to_linear(foreground);
+    const float output = linear_to_srgb(fmaf(fg_linear - bg_linear, enhanced, bg_linear));
+    return clampf((output - background) / difference, 0.0f, 1.0f);
+}
+// src_over_subpixel_mask_f32 with one coverage for the three channels (the
+// path-glyph case; the grey reduction of a non-opaque destination is then the
+// same value, (c + c + c) / 3 rounded).
+TKD uint32_t src_over_mask_f32(uint32_t destination, uint32_t source, float coverage) {
+    if ((source >> 24) == 0u || coverage == 0.0f) return destination;
+    float cov = coverage;
+    if ((destination >> 24) != 255u) cov = (coverage + coverage + coverage) / 3.0f;
+    const float alpha = (float)(source >> 24) * (1.0f / 255.0f);
+    const float amount = alpha * cov;
+    uint32_t out = 0u;
+    for (int k = 0; k < 3; ++k) {
+        const float s = (float)((source >> (8 * k)) & 255u), d = (float)((destination >> (8 * k)) & 255u);
+        out |= f2u(s * cov + d * (1.0f - amount) + 0.5f) << (8 * k);
+    }
+    const float out_alpha = amount * 255.0f + (float)(destination >> 24) * (1.0f - amount);
+    return out | (f2u(out_alpha + 0.5f) << 24);
+}
+TKD uint32_t src_over_mask_linear_auto_u8(uint32_t destination, uint32_t source, uint32_t coverage) {
+    if ((source >> 24) == 0u || coverage == 0u) return destination;
+    const float corrected = text_correcting_coverage(coverage, text_foreground_luma(source));
+    return src_over_mask_f32(destination, source, corrected);
+}
+
 // ---------------------------------------------------------------------------
 // Image brushes (shared/brush/pattern.hlsli:9-55, shared/pattern_transform.hlsli,
 // shared/integer.hlsli): the scene's atlas of premultiplied RGBA8 texels.
@@ -803,6 +864,8 @@ TKD uint32_t sample_image_brush(const uint32_t *paint, uint32_t count, uint32_t 
                                 blob_word(paint, count, brush_base, data_base + 8u));
 }
 
+// sample_brush (shared/brush/sample.hlsli:15-42) for the solid, linear, radial,
+// sweep, four-corner and image brushes.
 TKD uint32_t sample_brush(const uint32_t *paint, uint32_t count, uint32_t brush_base, uint32_t data_base, float x,
                           float y, const AtlasView &atlas = AtlasView{nullptr, 0u, 0u, 0u}) {
     uint32_t kind = blob_word(paint, count, brush_base, data_base);
