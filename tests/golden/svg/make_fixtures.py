@@ -24,7 +24,8 @@ from tileink_amd import svg  # noqa: E402
 SOURCE = "/root/reference/src/svg/tests"
 FOLDERS = ["shapes/circle", "shapes/ellipse", "shapes/line", "shapes/path", "shapes/polygon", "shapes/polyline",
            "shapes/rect", "painting/fill-rule", "painting/stroke-linejoin", "painting/stroke-linecap",
-           "painting/stroke-width", "painting/stroke-miterlimit", "structure/image"]
+           "painting/stroke-width", "painting/stroke-miterlimit", "structure/image", "masking/clipPath",
+           "masking/clip-rule", "masking/clip"]
 HREF = re.compile(r'href="([^"]*)"')
 
 

@@ -133,3 +133,44 @@ def _image_svg(png, attrs):
     data = _b64.b64encode(png).decode()
     return ('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 100 100">'
             f'<image href="data:image/png;base64,{data}" {attrs}/></svg>')
+
+
+CLIP_DOC = ('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 100 100">'
+            '<clipPath id="c" {clip_attrs}>{children}</clipPath>'
+            '<rect x="10" y="20" width="40" height="20" fill="green" clip-path="url(#c)" {rect_attrs}/></svg>')
+
+
+def clip_doc(children, clip_attrs="", rect_attrs=""):
+    return svg.Document(CLIP_DOC.format(children=children, clip_attrs=clip_attrs, rect_attrs=rect_attrs))
+
+
+def test_clip_path_of_one_shape_is_a_clip_layer():
+    # src/svg/clip.rs:172-186: one visible path is fused into a clip layer around the element
+    doc = clip_doc('<circle cx="30" cy="30" r="10" clip-rule="evenodd"/>')
+    kinds = [d[0] for d in doc.draws]
+    assert kinds == ["clip", "shape", "pop"]
+    assert doc.draws[0][3] == tk.FillRule.EvenOdd and doc.draws[0][2] == svg.IDENTITY
+    # objectBoundingBox: the unit square is the element's box (x 10..50, y 20..40)
+    doc = clip_doc('<rect x="0" y="0" width="1" height="1"/>', clip_attrs='clipPathUnits="objectBoundingBox"')
+    assert doc.draws[0][2] == (40.0, 0.0, 0.0, 20.0, 10.0, 20.0)
+    # the clip lives in the element's user space, after its own transform
+    doc = clip_doc('<rect x="0" y="0" width="5" height="5"/>', clip_attrs='transform="scale(2)"',
+                   rect_attrs='transform="translate(3 4)"')
+    assert doc.draws[0][2] == (2.0, 0.0, 0.0, 2.0, 3.0, 4.0)
+
+
+def test_clip_path_edge_cases():
+    # no visible child: everything is clipped, so nothing is drawn
+    assert clip_doc('<rect x="0" y="0" width="5" height="5" visibility="hidden"/>').draws == []
+    assert clip_doc('').draws == []
+    # several shapes need the reference's mask layers
+    with pytest.raises(svg.SvgUnsupported):
+        clip_doc('<rect x="0" y="0" width="5" height="5"/><circle cx="1" cy="1" r="1"/>')
+    with pytest.raises(svg.SvgUnsupported):
+        svg.Document('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 10 10">'
+                     '<rect width="5" height="5" clip-path="url(#missing)"/></svg>')
+    # clip-rule inherited from the clipPath's ancestors
+    doc = svg.Document('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 10 10"><g clip-rule="evenodd">'
+                       '<clipPath id="c"><rect width="4" height="4"/></clipPath></g>'
+                       '<rect width="5" height="5" clip-path="url(#c)"/></svg>')
+    assert doc.draws[0][3] == tk.FillRule.EvenOdd

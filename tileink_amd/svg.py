@@ -6,14 +6,16 @@ examples/common/svg.rs:9-52 does): shapes and paths become BezPaths in user
 units, filled (fill, fill-opacity, fill-rule) and then stroked (stroke,
 stroke-opacity, stroke-width, stroke-linejoin, stroke-linecap,
 stroke-miterlimit), each under its accumulated transform times the viewBox
-mapping.  Raster images (PNG, JPEG, GIF, WebP: embedded as base64 data URIs
+mapping.  A clip-path naming a clipPath of one shape becomes a clip layer of that
+shape (src/svg/clip.rs:60-90, 172-186), in user space or the element's bounding
+box.  Raster images (PNG, JPEG, GIF, WebP: embedded as base64 data URIs
 or files beside the document) become image brushes on a rect path, as
 src/svg/scene_image.rs:25-83 lowers usvg's image node, with the node's
 placement computed as usvg computes it, in f32 (x, y, width, height, auto
 sizes, preserveAspectRatio with a clip layer for slice, image-rendering).
 Elements this subset does not take (text, SVG images, gradients, patterns,
-clipping, masks, filters, markers, dashes, group opacity, use) raise
-SvgUnsupported instead of being dropped silently.
+clipPaths of several shapes or nested clips, masks, filters, markers, dashes,
+group opacity, use) raise SvgUnsupported instead of being dropped silently.
 """
 import base64
 import io
@@ -38,10 +40,11 @@ COLORS = {
     "purple": (128, 0, 128), "orange": (255, 165, 0), "pink": (255, 192, 203), "brown": (165, 42, 42),
     "seagreen": (46, 139, 87), "lightblue": (173, 216, 230), "darkgreen": (0, 100, 0),
 }
-UNSUPPORTED = {"text", "use", "linearGradient", "radialGradient", "pattern", "clipPath", "mask", "filter",
+UNSUPPORTED = {"text", "use", "linearGradient", "radialGradient", "pattern", "mask", "filter",
                "marker", "symbol", "switch", "foreignObject", "style", "a", "tspan", "textPath"}
 INHERITED = ("fill", "fill-opacity", "fill-rule", "stroke", "stroke-opacity", "stroke-width", "stroke-linejoin",
-             "stroke-linecap", "stroke-miterlimit", "visibility", "font-size", "image-rendering")
+             "stroke-linecap", "stroke-miterlimit", "visibility", "font-size", "image-rendering", "clip-rule")
+SHAPES = ("path", "rect", "circle", "ellipse", "line", "polyline", "polygon")
 XLINK_HREF = "{http://www.w3.org/1999/xlink}href"
 
 
@@ -485,12 +488,22 @@ def _attrs(el, parent):
         if ":" in decl:
             k, v = decl.split(":", 1)
             a[k.strip()] = v.strip()
-    for k in ("opacity", "stroke-dasharray", "clip-path", "mask", "filter", "marker-start", "marker-mid",
-              "marker-end"):
+    for k in ("opacity", "stroke-dasharray", "mask", "filter", "marker-start", "marker-mid", "marker-end"):
         v = a.get(k)
         if v is not None and v.strip() not in ("none", "1", ""):
             raise SvgUnsupported(f"{k}={v}")
     return a
+
+
+def _presentation(el, name):
+    """An element's own value of a presentation attribute (style wins), or None."""
+    value = el.get(name)
+    for decl in (el.get("style") or "").split(";"):
+        if ":" in decl:
+            k, v = decl.split(":", 1)
+            if k.strip() == name:
+                value = v.strip()
+    return value
 
 
 def _local(tag):
@@ -528,12 +541,95 @@ class Document:
                 ty = (self.height - vb[3] * s) / 2.0 - vb[1] * s
                 view = (s, 0.0, 0.0, s, tx, ty)
         self.view_w, self.view_h = (vb[2], vb[3]) if vb else (self.width, self.height)
+        self.clips = {el.get("id"): el for el in root.iter() if _local(el.tag) == "clipPath" and el.get("id")}
+        self.parent_of = {child: node for node in root.iter() for child in node}
         self.draws = []
         self._walk(root, {}, view, _to_f32(view))
 
+    def _clip(self, value, a, tag, t):
+        """The clip layer of a clip-path reference, as src/svg/clip.rs:172-186 lowers
+        usvg's clipPath: one visible shape is a clip layer of its outline under the
+        clip's transform ("clip", path, transform, rule); none clips everything
+        ("empty"); anything else needs the reference's mask layers."""
+        m = re.fullmatch(r"\s*url\(\s*#([^)\s]+)\s*\)\s*", value)
+        if not m or m.group(1) not in self.clips:
+            raise SvgUnsupported(f"clip-path={value}")
+        clip = self.clips[m.group(1)]
+        # clip-rule is inherited through the clipPath's own ancestors in the document
+        rule, node = "nonzero", clip
+        while node is not None:
+            v = _presentation(node, "clip-rule")
+            if v is not None:
+                rule = v
+                break
+            node = self.parent_of.get(node)
+        ca = _attrs(clip, {"clip-rule": rule})
+        if ca.get("clip-path", "none").strip() != "none":
+            raise SvgUnsupported("nested clip-path")
+        shapes = []
+        for child in clip:
+            ctag = _local(child.tag)
+            if ctag in ("title", "desc", "metadata"):
+                continue
+            if ctag not in SHAPES:
+                raise SvgUnsupported(f"clipPath child {ctag}")
+            cattrs = _attrs(child, {"clip-rule": ca.get("clip-rule", "nonzero")})
+            if cattrs.get("clip-path", "none").strip() != "none":
+                raise SvgUnsupported("clip-path inside clipPath")
+            if cattrs.get("display") == "none" or cattrs.get("visibility") in ("hidden", "collapse"):
+                continue
+            path = self._shape(ctag, cattrs)
+            if path is None:
+                continue
+            shapes.append((path, _transform(cattrs.get("transform")), cattrs.get("clip-rule", "nonzero")))
+        if len(shapes) > 1:
+            raise SvgUnsupported("clipPath of several shapes (a mask layer)")
+        if not shapes:
+            return ("empty",)
+        ct = _mul(t, _transform(ca.get("transform")))
+        if ca.get("clipPathUnits") == "objectBoundingBox":
+            box = self._bbox(tag, a)
+            if box is None:
+                raise SvgUnsupported("objectBoundingBox of this element")
+            x0, y0, x1, y1 = box
+            if not (x1 > x0 and y1 > y0):
+                return ("empty",)
+            ct = _mul(ct, (x1 - x0, 0.0, 0.0, y1 - y0, x0, y0))
+        path, child_t, rule = shapes[0]
+        return ("clip", path, _mul(ct, child_t), tk.FillRule.EvenOdd if rule == "evenodd" else tk.FillRule.NonZero)
+
+    def _bbox(self, tag, a):
+        """The object bounding box (geometry only) of a basic shape."""
+        if tag not in SHAPES:
+            return None
+        path = self._shape(tag, a)
+        if path is None:
+            return None
+        verbs, pts = path.arrays()
+        pts = np.asarray(pts, np.float64).reshape(-1, 2)
+        u = np.linspace(0.0, 1.0, 257)[:, None]
+        out, i, last = [], 0, None
+        for v in np.asarray(verbs):
+            if v in (tk._abi.PATH_MOVE, tk._abi.PATH_LINE):
+                last = pts[i]
+                out.append(pts[i:i + 1])
+                i += 1
+            elif v == tk._abi.PATH_QUAD:  # the tight extremes, from dense samples
+                p1, p2 = pts[i], pts[i + 1]
+                out.append((1 - u) ** 2 * last + 2 * (1 - u) * u * p1 + u * u * p2)
+                last, i = p2, i + 2
+            elif v == tk._abi.PATH_CUBIC:
+                p1, p2, p3 = pts[i], pts[i + 1], pts[i + 2]
+                out.append((1 - u) ** 3 * last + 3 * (1 - u) ** 2 * u * p1 + 3 * (1 - u) * u * u * p2 + u ** 3 * p3)
+                last, i = p3, i + 3
+        if not out:
+            return None
+        pts = np.concatenate(out)
+        return pts[:, 0].min(), pts[:, 1].min(), pts[:, 0].max(), pts[:, 1].max()
+
     def _walk(self, el, parent, transform, transform32):
         tag = _local(el.tag)
-        if tag in ("title", "desc", "metadata", "defs"):
+        if tag in ("title", "desc", "metadata", "defs", "clipPath"):
             return
         if tag in UNSUPPORTED:
             raise SvgUnsupported(tag)
@@ -543,6 +639,22 @@ class Document:
         t = _mul(transform, _transform(a.get("transform"))) if "transform" in el.attrib else transform
         t32 = _concat_f32(transform32, _to_f32(_transform(a.get("transform")))) if "transform" in el.attrib \
             else transform32
+        clip_ref = a.get("clip-path", "none").strip()
+        if clip_ref != "none":
+            layer = self._clip(clip_ref, a, tag, t)
+            if layer[0] == "empty":
+                return  # an empty clip hides the element
+            self.draws.append(layer)
+            try:
+                a = dict(a)
+                del a["clip-path"]
+                self._walk_clipped(el, tag, a, t, t32)
+            finally:
+                self.draws.append(("pop",))
+            return
+        self._walk_clipped(el, tag, a, t, t32)
+
+    def _walk_clipped(self, el, tag, a, t, t32):
         if tag in ("svg", "g"):
             if tag == "svg" and self.draws is not None and getattr(self, "_entered", False):
                 raise SvgUnsupported("nested svg")
@@ -702,7 +814,8 @@ def load(text, target_width=300, base_dir=None):
                              tk.FillRule.NonZero, TOLERANCE)
             continue
         if entry[0] == "clip":
-            canvas.push_clip_layer(entry[1], _mul(scale, entry[2]), tk.FillRule.NonZero, TOLERANCE)
+            rule = entry[3] if len(entry) > 3 else tk.FillRule.NonZero
+            canvas.push_clip_layer(entry[1], _mul(scale, entry[2]), rule, TOLERANCE)
             continue
         if entry[0] == "pop":
             canvas.pop_layer()
