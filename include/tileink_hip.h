@@ -243,7 +243,17 @@ typedef struct tk_brush {
     uint32_t image;
     float pattern_transform[6];
     uint32_t sampling, opacity;
+    /* linear and radial: gradient_flags bit 0 (TK_GRADIENT_TRANSFORM) takes
+     * gradient_transform [a b c d e f] from user space to the gradient's own space
+     * (peniko's brush transform, e.g. the inverse of an SVG gradientTransform; the
+     * identity without the bit); bit 1 (TK_GRADIENT_UNPREMULTIPLIED) interpolates
+     * the stops in straight alpha (InterpolationAlphaSpace::Unpremultiplied, as SVG
+     * gradients do; premultiplied without it). */
+    float gradient_transform[6];
+    uint32_t gradient_flags;
 } tk_brush;
+#define TK_GRADIENT_TRANSFORM 1u
+#define TK_GRADIENT_UNPREMULTIPLIED 2u
 
 tk_status tk_canvas_create(uint32_t width, uint32_t height, double scale, tk_canvas **out);
 void tk_canvas_destroy(tk_canvas *canvas);

@@ -25,7 +25,8 @@ SOURCE = "/root/reference/src/svg/tests"
 FOLDERS = ["shapes/circle", "shapes/ellipse", "shapes/line", "shapes/path", "shapes/polygon", "shapes/polyline",
            "shapes/rect", "painting/fill-rule", "painting/stroke-linejoin", "painting/stroke-linecap",
            "painting/stroke-width", "painting/stroke-miterlimit", "structure/image", "masking/clipPath",
-           "masking/clip-rule", "masking/clip"]
+           "masking/clip-rule", "masking/clip", "paint-servers/linearGradient", "paint-servers/radialGradient",
+           "paint-servers/stop", "paint-servers/stop-color", "paint-servers/stop-opacity"]
 HREF = re.compile(r'href="([^"]*)"')
 
 

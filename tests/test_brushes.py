@@ -224,3 +224,22 @@ def test_oracle_frame_samples_the_brush_at_pixel_centres(name):
     req[:, 1:3] = pts.astype(np.float32).view(np.uint32)
     expected = oracle.brush_words(paint, base, req).reshape(48, 64)
     assert np.array_equal(image, expected)
+
+
+def test_gradient_interpolation_space_and_transform():
+    stops = [(0.0, tk.Color(1, 0, 0, 1)), (1.0, tk.Color(0, 0, 1, 0))]
+    xf = (0.5, 0.25, -0.125, 2.0, 3.0, -4.0)
+    for unpremul in (False, True):
+        brush = tk.LinearGradient((0, 0), (10, 0), stops, 0, transform=xf, unpremultiplied=unpremul)
+        header, params, ramp = params_of(brush, scale=2.0)
+        # the brush transform scaled by 1 / scale in its linear part (scale_brush_transform)
+        got = params[4:10].view(np.float32)
+        assert np.array_equal(got, np.float32([0.25, 0.125, -0.0625, 1.0, 3.0, -4.0]))
+        n = len(ramp)
+        t = np.arange(n, dtype=np.float32) / np.float32(n - 1)
+        if unpremul:  # straight colour (1 - t, 0, t) with alpha 1 - t, premultiplied after
+            expect = np.stack([(1 - t) * (1 - t), 0 * t, t * (1 - t), 1 - t], 1)
+        else:  # premultiplied endpoints (1, 0, 0, 1) and (0, 0, 0, 0)
+            expect = np.stack([1 - t, 0 * t, 0 * t, 1 - t], 1)
+        words = ramp.view(np.uint8).reshape(-1, 4).astype(np.int64)
+        assert np.abs(words - np.floor(expect * 255 + 0.5)).max() <= 1

@@ -174,3 +174,28 @@ def test_clip_path_edge_cases():
                        '<clipPath id="c"><rect width="4" height="4"/></clipPath></g>'
                        '<rect width="5" height="5" clip-path="url(#c)"/></svg>')
     assert doc.draws[0][3] == tk.FillRule.EvenOdd
+
+
+def test_gradient_ramp_pixel_kat():
+    # native_gradient_ramp_preserves_fused_fraction (renderer_rounding_gpu.rs:57-68):
+    # paint-servers/linearGradient/attributes-via-xlink-href.svg at 300 px, pixel
+    # (121, 30) is exactly [237, 237, 237, 255]: a separately rounded t * 63 gives 238
+    canvas = svg_golden.load_fixture("paint-servers/linearGradient/attributes-via-xlink-href.svg")
+    image, _ = oracle.render_frame(canvas.arrays(0))
+    assert list(image.view(np.uint8).reshape(300, 300, 4)[30, 121]) == [237, 237, 237, 255]
+
+
+def test_gradient_stops_follow_usvg():
+    # offsets clamped and made non-decreasing; equal neighbours pulled apart by an epsilon
+    doc = svg.Document('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 10 10"><linearGradient id="g">'
+                       '<stop offset="0.5" stop-color="red"/><stop offset="0.3" stop-color="blue"/>'
+                       '<stop offset="150%" stop-color="#00ff00" stop-opacity="50%"/></linearGradient>'
+                       '<rect width="10" height="10" fill="url(#g)"/></svg>')
+    g = doc.draws[0][3][0]
+    offsets = [o for o, _ in g["stops"]]
+    eps = float(np.finfo(np.float32).eps)
+    assert offsets == [np.float32(0.5 - eps), 0.5, 1.0]
+    assert g["stops"][2][1] == (0, 255, 0, 0.5)
+    # objectBoundingBox units: the gradient's unit square is the element's box
+    assert g["transform"] == (10.0, 0.0, 0.0, 10.0, 0.0, 0.0)
+    assert g["geom"] == ("linear", 0.0, 0.0, 1.0, 0.0)

@@ -119,31 +119,47 @@ def _gradient_brush(kind, stops, extend):
     return b, arr
 
 
+def _gradient_options(b, transform, unpremultiplied):
+    if transform is not None:
+        b.gradient_transform[:] = tuple(float(v) for v in transform)
+        b.gradient_flags |= _abi.GRADIENT_TRANSFORM
+    if unpremultiplied:
+        b.gradient_flags |= _abi.GRADIENT_UNPREMULTIPLIED
+
+
 class LinearGradient:
     """Linear gradient brush (peniko Gradient::new_linear): stops are (offset, Color);
-    extend 0 pad, 1 repeat, 2 reflect."""
+    extend 0 pad, 1 repeat, 2 reflect.  transform (a, b, c, d, e, f) takes user space
+    to the gradient's space (the brush transform; identity when None);
+    unpremultiplied interpolates the stops in straight alpha, as SVG does."""
 
-    def __init__(self, start, end, stops, extend=0):
+    def __init__(self, start, end, stops, extend=0, transform=None, unpremultiplied=False):
         self.start, self.end, self.stops, self.extend = start, end, list(stops), int(extend)
+        self.transform, self.unpremultiplied = transform, bool(unpremultiplied)
 
     def _brush(self):
         b, arr = _gradient_brush(BRUSH_LINEAR, self.stops, self.extend)
         b.start[:] = self.start
         b.end[:] = self.end
+        _gradient_options(b, self.transform, self.unpremultiplied)
         return b, arr
 
 
 class RadialGradient:
     """Two-point conical gradient (peniko Gradient::new_two_point_radial): circles
-    (start_center, start_radius) to (end_center, end_radius)."""
+    (start_center, start_radius) to (end_center, end_radius); transform and
+    unpremultiplied as for LinearGradient."""
 
-    def __init__(self, start_center, start_radius, end_center, end_radius, stops, extend=0):
+    def __init__(self, start_center, start_radius, end_center, end_radius, stops, extend=0, transform=None,
+                 unpremultiplied=False):
         self.start_center, self.end_center = start_center, end_center
         self.start_radius, self.end_radius = float(start_radius), float(end_radius)
         self.stops, self.extend = list(stops), int(extend)
+        self.transform, self.unpremultiplied = transform, bool(unpremultiplied)
 
     def _brush(self):
         b, arr = _gradient_brush(BRUSH_RADIAL, self.stops, self.extend)
+        _gradient_options(b, self.transform, self.unpremultiplied)
         b.start[:] = self.start_center
         b.end[:] = self.end_center
         b.start_radius, b.end_radius = self.start_radius, self.end_radius

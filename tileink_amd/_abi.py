@@ -20,6 +20,7 @@ TK_NO_ALLOC_SCRATCH_BYTES = 1 << 20
 TK_NO_ALLOC_MAX_TABLE = 536_739_840
 
 PATH_MOVE, PATH_LINE, PATH_QUAD, PATH_CUBIC, PATH_CLOSE = range(5)
+GRADIENT_TRANSFORM, GRADIENT_UNPREMULTIPLIED = 1, 2
 FILL_NONZERO, FILL_EVEN_ODD = 0, 1
 DRAW_BRUSH, DRAW_CLIP = 0, 1
 BRUSH_SOLID, BRUSH_LINEAR, BRUSH_RADIAL, BRUSH_SWEEP, BRUSH_FOUR_CORNER, BRUSH_PATTERN, BRUSH_IMAGE = 1, 2, 3, 4, 5, 6, 7
@@ -111,7 +112,7 @@ class Brush(C.Structure):
                 ("start_radius", C.c_float), ("end_radius", C.c_float), ("start_angle", C.c_float),
                 ("end_angle", C.c_float), ("bounds", C.c_float * 4), ("corners", (C.c_float * 4) * 4),
                 ("image", C.c_uint32), ("pattern_transform", C.c_float * 6), ("sampling", C.c_uint32),
-                ("opacity", C.c_uint32)]
+                ("opacity", C.c_uint32), ("gradient_transform", C.c_float * 6), ("gradient_flags", C.c_uint32)]
 
 
 class Limits(C.Structure):

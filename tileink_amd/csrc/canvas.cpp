@@ -508,9 +508,11 @@ size_t quantize_ramp_size(float span, size_t stop_count) {
     return std::min<size_t>(p, 4096);
 }
 
-// Premultiplied-sRGB interpolation between sorted stops (peniko's default
-// interpolation space); peniko's exact colour arithmetic is not restated.
-std::vector<uint32_t> build_ramp(std::vector<tk_gradient_stop> stops, size_t ramp_size) {
+// build_ramp (src/shared/brush.rs:638-687): sRGB interpolation between sorted
+// stops, premultiplied (peniko's default) or in straight alpha and premultiplied
+// after (InterpolationAlphaSpace::Unpremultiplied); peniko's exact colour
+// arithmetic is not restated.
+std::vector<uint32_t> build_ramp(std::vector<tk_gradient_stop> stops, size_t ramp_size, bool unpremultiplied) {
     ramp_size = std::max<size_t>(ramp_size, 2);
     std::vector<uint32_t> ramp(ramp_size, 0u);
     if (stops.empty()) return ramp;
@@ -526,10 +528,21 @@ std::vector<uint32_t> build_ramp(std::vector<tk_gradient_stop> stops, size_t ram
         else { left = stops[upper - 1]; right = stops[upper]; }
         float span = right.offset - left.offset;
         float lt = std::fabs(span) <= 1.1920929e-7f ? 0.0f : std::fmin(std::fmax((t - left.offset) / span, 0.0f), 1.0f);
-        float lp[4] = {left.rgba[0] * left.rgba[3], left.rgba[1] * left.rgba[3], left.rgba[2] * left.rgba[3], left.rgba[3]};
-        float rp[4] = {right.rgba[0] * right.rgba[3], right.rgba[1] * right.rgba[3], right.rgba[2] * right.rgba[3], right.rgba[3]};
         float c[4];
-        for (int k = 0; k < 4; ++k) c[k] = lp[k] + (rp[k] - lp[k]) * lt;
+        if (unpremultiplied) {
+            float u[4];
+            for (int k = 0; k < 4; ++k) u[k] = left.rgba[k] + (right.rgba[k] - left.rgba[k]) * lt;
+            c[0] = u[0] * u[3];
+            c[1] = u[1] * u[3];
+            c[2] = u[2] * u[3];
+            c[3] = u[3];
+        } else {
+            float lp[4] = {left.rgba[0] * left.rgba[3], left.rgba[1] * left.rgba[3], left.rgba[2] * left.rgba[3],
+                           left.rgba[3]};
+            float rp[4] = {right.rgba[0] * right.rgba[3], right.rgba[1] * right.rgba[3],
+                           right.rgba[2] * right.rgba[3], right.rgba[3]};
+            for (int k = 0; k < 4; ++k) c[k] = lp[k] + (rp[k] - lp[k]) * lt;
+        }
         ramp[ix] = premul_f32_to_u32(c);
     }
     return ramp;
@@ -566,11 +579,19 @@ bool encode_brush(const tk_brush *brush, double scale, std::vector<uint32_t> &bl
     std::vector<uint32_t> payload;
     const uint32_t kind = brush ? brush->kind : TK_BRUSH_SOLID;
     const float scale_f = float(scale), inv = float(1.0 / scale);
-    // scale_brush_transform(identity, 1 / scale) (src/canvas.rs:203-212).
-    const float xf[6] = {inv, 0.0f, 0.0f, inv, 0.0f, 0.0f};
+    // scale_brush_transform(the brush's transform, 1 / scale) (src/canvas.rs:203-212).
+    float xf[6] = {inv, 0.0f, 0.0f, inv, 0.0f, 0.0f};
+    if (brush && (brush->kind == TK_BRUSH_LINEAR || brush->kind == TK_BRUSH_RADIAL) &&
+        (brush->gradient_flags & TK_GRADIENT_TRANSFORM)) {
+        for (int i = 0; i < 6; ++i)
+            if (!std::isfinite(brush->gradient_transform[i])) return false;
+        for (int i = 0; i < 4; ++i) xf[i] = brush->gradient_transform[i] * inv;
+        xf[4] = brush->gradient_transform[4];
+        xf[5] = brush->gradient_transform[5];
+    }
     auto ramp_of = [&](size_t size) {
         std::vector<tk_gradient_stop> stops(brush->stops, brush->stops + brush->stop_count);
-        return build_ramp(stops, size);
+        return build_ramp(stops, size, (brush->gradient_flags & TK_GRADIENT_UNPREMULTIPLIED) != 0u);
     };
     if (kind == TK_BRUSH_SOLID) {
         float transparent[4] = {0, 0, 0, 0};

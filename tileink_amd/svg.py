@@ -40,7 +40,7 @@ COLORS = {
     "purple": (128, 0, 128), "orange": (255, 165, 0), "pink": (255, 192, 203), "brown": (165, 42, 42),
     "seagreen": (46, 139, 87), "lightblue": (173, 216, 230), "darkgreen": (0, 100, 0),
 }
-UNSUPPORTED = {"text", "use", "linearGradient", "radialGradient", "pattern", "mask", "filter",
+UNSUPPORTED = {"text", "use", "pattern", "mask", "filter",
                "marker", "symbol", "switch", "foreignObject", "style", "a", "tspan", "textPath"}
 INHERITED = ("fill", "fill-opacity", "fill-rule", "stroke", "stroke-opacity", "stroke-width", "stroke-linejoin",
              "stroke-linecap", "stroke-miterlimit", "visibility", "font-size", "image-rendering", "clip-rule")
@@ -495,6 +495,65 @@ def _attrs(el, parent):
     return a
 
 
+def _opacity(text):
+    """An opacity value: a number or a percentage, clamped to [0, 1]; 1 when absent or invalid."""
+    if text is None:
+        return 1.0
+    t = text.strip()
+    try:
+        v = float(t[:-1]) / 100.0 if t.endswith("%") else float(t)
+    except ValueError:
+        return 1.0
+    return min(max(v, 0.0), 1.0)
+
+
+def _gradient_stops(el):
+    """usvg's stop conversion: offsets (numbers or percentages) clamped to [0, 1]
+    and non-decreasing, runs of three equal offsets cut to their ends, leading
+    zeros and equal neighbours pulled apart by an epsilon; colour and opacity
+    multiplied.  Returns [(offset, (r, g, b, alpha 0-1))], r, g, b in 0-255."""
+    if el is None:
+        return []
+    stops, prev = [], 0.0
+    for st in el:
+        if _local(st.tag) != "stop":
+            continue
+        raw = _presentation(st, "offset")
+        off = prev
+        if raw is not None:
+            mm = re.fullmatch(r"\s*([+-]?(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?)\s*(%?)\s*", raw)
+            if mm:
+                off = float(mm.group(1)) / (100.0 if mm.group(2) else 1.0)
+        prev = off
+        off = _f32(min(max(off, 0.0), 1.0))
+        cv = _presentation(st, "stop-color")
+        rgb = (0, 0, 0) if cv is None else _color(cv)
+        if rgb is None:
+            rgb = (0, 0, 0)
+        alpha = _opacity(_presentation(st, "stop-opacity"))
+        stops.append([off, rgb, alpha])
+    eps = float(np.finfo(np.float32).eps)
+
+    def close(x, y):
+        return abs(np.float32(x).view(np.int32).astype(np.int64) - np.float32(y).view(np.int32).astype(np.int64)) <= 4
+
+    i = 0
+    while len(stops) >= 3 and i < len(stops) - 2:
+        if close(stops[i][0], stops[i + 1][0]) and close(stops[i + 1][0], stops[i + 2][0]):
+            del stops[i + 1]
+        else:
+            i += 1
+    for i in range(len(stops) - 1):
+        if close(stops[i][0], 0.0) and close(stops[i + 1][0], 0.0):
+            stops[i + 1][0] = _f32(min(max(stops[i][0] + eps, 0.0), 1.0))
+    for i in range(1, len(stops)):
+        o1, o2 = stops[i - 1][0], stops[i][0]
+        if o1 > o2 or close(o1, o2):
+            stops[i - 1][0] = _f32(min(max(o1 - eps, 0.0), 1.0))
+            stops[i][0] = o1
+    return [(o, (c[0], c[1], c[2], a)) for o, c, a in stops]
+
+
 def _presentation(el, name):
     """An element's own value of a presentation attribute (style wins), or None."""
     value = el.get(name)
@@ -542,6 +601,8 @@ class Document:
                 view = (s, 0.0, 0.0, s, tx, ty)
         self.view_w, self.view_h = (vb[2], vb[3]) if vb else (self.width, self.height)
         self.clips = {el.get("id"): el for el in root.iter() if _local(el.tag) == "clipPath" and el.get("id")}
+        self.gradients = {el.get("id"): el for el in root.iter()
+                          if _local(el.tag) in ("linearGradient", "radialGradient") and el.get("id")}
         self.parent_of = {child: node for node in root.iter() for child in node}
         self.draws = []
         self._walk(root, {}, view, _to_f32(view))
@@ -598,6 +659,90 @@ class Document:
         path, child_t, rule = shapes[0]
         return ("clip", path, _mul(ct, child_t), tk.FillRule.EvenOdd if rule == "evenodd" else tk.FillRule.NonZero)
 
+    def _paint(self, value, tag, a):
+        """A fill or stroke value: (r, g, b) in 0-255, a gradient (a dict with the
+        user-space geometry, stops, spread and paint-server transform, as usvg's
+        paint-server conversion leaves them), or None (no paint)."""
+        value = value.strip()
+        if not value.startswith("url("):
+            return _color(value)
+        m = re.fullmatch(r"url\(\s*#([^)\s]+)\s*\)\s*(.*)", value)
+        if not m:
+            raise SvgUnsupported(f"paint {value}")
+        el = self.gradients.get(m.group(1))
+        if el is None:  # a missing server: the fallback colour, or no paint
+            return _color(m.group(2)) if m.group(2) else None
+        chain, seen = [el], {id(el)}
+        while True:  # xlink:href: attributes and stops from the referenced gradients
+            href = chain[-1].get(XLINK_HREF) or chain[-1].get("href")
+            if not href or not href.startswith("#") or href[1:] not in self.gradients:
+                break
+            nxt = self.gradients[href[1:]]
+            if id(nxt) in seen:
+                break
+            chain.append(nxt)
+            seen.add(id(nxt))
+
+        kind = _local(el.tag)
+
+        def attr(name, same_kind=False):
+            # geometry comes only from gradients of the same kind; units, spread,
+            # transform and stops from any in the chain
+            for g in chain:
+                if same_kind and _local(g.tag) != kind:
+                    continue
+                v = _presentation(g, name) if name in ("gradientTransform",) else g.get(name)
+                if v is not None:
+                    return v
+            return None
+
+        stops = _gradient_stops(next((g for g in chain if any(_local(c.tag) == "stop" for c in g)), None))
+        if len(stops) < 2:
+            return (stops[0][1][:3], stops[0][1][3]) if stops else None
+        units = attr("gradientUnits") or "objectBoundingBox"
+        bbox_units = units != "userSpaceOnUse"
+        W, H = self.view_w, self.view_h
+        diag = math.hypot(W, H) / math.sqrt(2)
+
+        def length(name, default, ref):
+            v = attr(name, same_kind=True)
+            text = default if v is None else v
+            mm = re.fullmatch(r"\s*([+-]?(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?)\s*(%?)\s*", text)
+            if not mm:
+                raise SvgUnsupported(f"gradient length {text!r}")
+            num = float(mm.group(1))
+            if mm.group(2):
+                return num / 100.0 if bbox_units else num / 100.0 * ref
+            return num if bbox_units else _length(text, ref)
+
+        spread = {"reflect": 2, "repeat": 1}.get((attr("spreadMethod") or "pad").strip(), 0)
+        ts = _transform(attr("gradientTransform"))
+        if bbox_units:
+            box = self._bbox(tag, a)
+            if box is None:
+                raise SvgUnsupported("objectBoundingBox of this element")
+            x0, y0, x1, y1 = box
+            if not (x1 > x0 and y1 > y0):
+                return None  # usvg drops a bounding-box paint on a zero-sized box
+            ts = _mul((x1 - x0, 0.0, 0.0, y1 - y0, x0, y0), ts)
+        if _local(el.tag) == "linearGradient" and _local(chain[0].tag) == "linearGradient":
+            geom = ("linear", length("x1", "0%", W), length("y1", "0%", H), length("x2", "100%", W),
+                    length("y2", "0%", H))
+        elif _local(el.tag) == "radialGradient":
+            cx, cy = length("cx", "50%", W), length("cy", "50%", H)
+            r = length("r", "50%", diag)
+            fx = length("fx", "50%", W) if attr("fx", True) is not None else cx
+            fy = length("fy", "50%", H) if attr("fy", True) is not None else cy
+            fr = length("fr", "0%", diag)
+            if r < 0 or fr < 0:
+                raise SvgUnsupported("negative gradient radius")  # undefined in SVG 2
+            if not (r > 0):
+                return (stops[-1][1][:3], stops[-1][1][3])  # r = 0: the last stop's colour
+            geom = ("radial", fx, fy, fr, cx, cy, r)
+        else:
+            raise SvgUnsupported("gradient type")
+        return {"geom": geom, "stops": stops, "spread": spread, "transform": ts}
+
     def _bbox(self, tag, a):
         """The object bounding box (geometry only) of a basic shape."""
         if tag not in SHAPES:
@@ -629,7 +774,7 @@ class Document:
 
     def _walk(self, el, parent, transform, transform32):
         tag = _local(el.tag)
-        if tag in ("title", "desc", "metadata", "defs", "clipPath"):
+        if tag in ("title", "desc", "metadata", "defs", "clipPath", "linearGradient", "radialGradient"):
             return
         if tag in UNSUPPORTED:
             raise SvgUnsupported(tag)
@@ -668,10 +813,10 @@ class Document:
         path = self._shape(tag, a)
         if path is None or a.get("visibility") in ("hidden", "collapse"):
             return
-        fill = _color(a.get("fill", "black"))
-        fill_op = _number(a.get("fill-opacity"), 1.0)
-        stroke = _color(a.get("stroke", "none"))
-        stroke_op = _number(a.get("stroke-opacity"), 1.0)
+        fill = self._paint(a.get("fill", "black"), tag, a)
+        fill_op = _opacity(a.get("fill-opacity"))
+        stroke = self._paint(a.get("stroke", "none"), tag, a)
+        stroke_op = _opacity(a.get("stroke-opacity"))
         if tag in ("line", "polyline"):
             fill = fill if tag == "polyline" else None
         rule = tk.FillRule.EvenOdd if a.get("fill-rule") == "evenodd" else tk.FillRule.NonZero
@@ -822,14 +967,45 @@ def load(text, target_width=300, base_dir=None):
             continue
         _, path, t, fill, stroke = entry
         m = _mul(scale, t)
-        if fill is not None:
+        if fill is not None and isinstance(fill[0], dict):
+            canvas.push_path(path, _gradient_brush(fill[0], fill[1], _mul(m, fill[0]["transform"])), m, fill[2],
+                             TOLERANCE)
+        elif fill is not None and isinstance(fill[0], tuple) and len(fill[0]) == 2:  # a one-stop gradient
+            (rgb, alpha), op, rule = fill
+            canvas.push_path(path, tk.Color(rgb[0] / 255, rgb[1] / 255, rgb[2] / 255, alpha * op), m, rule, TOLERANCE)
+        elif fill is not None:
             (r, g, b), op, rule = fill
             canvas.push_path(path, tk.Color(r / 255, g / 255, b / 255, op), m, rule, TOLERANCE)
         if stroke is not None:
-            (r, g, b), op, width_, join, cap, limit = stroke
-            canvas.push_stroke(path, width_, tk.Color(r / 255, g / 255, b / 255, op), m, tk.FillRule.NonZero,
+            paint, op, width_, join, cap, limit = stroke
+            if isinstance(paint, dict):
+                brush = _gradient_brush(paint, op, _mul(m, paint["transform"]))
+            elif len(paint) == 2:
+                brush = tk.Color(paint[0][0] / 255, paint[0][1] / 255, paint[0][2] / 255, paint[1] * op)
+            else:
+                brush = tk.Color(paint[0] / 255, paint[1] / 255, paint[2] / 255, op)
+            canvas.push_stroke(path, width_, brush, m, tk.FillRule.NonZero,
                                TOLERANCE, join=join, start_cap=cap, end_cap=cap, miter_limit=limit)
     return canvas
+
+
+def _gradient_brush(g, opacity, server_to_canvas):
+    """The reference's SVG gradient brush (src/svg/paint.rs:146-210): stops with
+    their opacity times the paint's, interpolated in straight alpha, the brush
+    transform the f32 inverse of (canvas transform x path transform x server
+    transform)."""
+    inv = _inverse(server_to_canvas)
+    if inv is None:
+        raise SvgUnsupported("non-invertible gradientTransform")
+    stops = [(o, tk.Color(c[0] / 255, c[1] / 255, c[2] / 255, float(np.float32(c[3]) * np.float32(opacity))))
+             for o, c in g["stops"]]
+    xf = tuple(_f32(v) for v in inv)
+    geom = g["geom"]
+    if geom[0] == "linear":
+        return tk.LinearGradient((geom[1], geom[2]), (geom[3], geom[4]), stops, g["spread"], transform=xf,
+                                 unpremultiplied=True)
+    _, fx, fy, fr, cx, cy, r = geom
+    return tk.RadialGradient((fx, fy), fr, (cx, cy), r, stops, g["spread"], transform=xf, unpremultiplied=True)
 
 
 def straight_rgba8(premultiplied):
