@@ -90,16 +90,21 @@ def _color(text):
         return COLORS[t]
     if t.startswith("#"):
         h = t[1:]
+        if not re.fullmatch(r"[0-9a-f]{3}|[0-9a-f]{6}", h):
+            raise SvgUnsupported(f"colour {text!r}")
         if len(h) == 3:
             return tuple(int(c * 2, 16) for c in h)
-        if len(h) == 6:
-            return tuple(int(h[i:i + 2], 16) for i in (0, 2, 4))
+        return tuple(int(h[i:i + 2], 16) for i in (0, 2, 4))
     m = re.fullmatch(r"rgb\(\s*([^,]+),\s*([^,]+),\s*([^)]+)\)", t)
     if m:
         out = []
         for c in m.groups():
             c = c.strip()
-            out.append(round(float(c[:-1]) * 2.55) if c.endswith("%") else int(float(c)))
+            try:
+                v = float(c[:-1]) / 100.0 * 255.0 if c.endswith("%") else float(c)
+            except ValueError:
+                raise SvgUnsupported(f"colour {text!r}") from None
+            out.append(int(math.floor(abs(v) + 0.5)) * (1 if v >= 0 else -1))
         return tuple(max(0, min(255, v)) for v in out)
     raise SvgUnsupported(f"colour {text!r}")
 
@@ -480,17 +485,38 @@ def _rect_path(x, y, w, h, rx, ry):
 # --- the document ---------------------------------------------------------------------
 
 
+# Properties a style attribute may set (presentation attributes, and transform):
+# usvg ignores geometry such as width or height there.
+PRESENTATION = set(INHERITED) | {"opacity", "stroke-dasharray", "stroke-dashoffset", "clip-path", "mask", "filter",
+                                 "marker-start", "marker-mid", "marker-end", "display", "stop-color", "stop-opacity",
+                                 "color", "mix-blend-mode", "isolation", "transform", "shape-rendering",
+                                 "transform-origin", "marker"}
+
+
+def _style(el):
+    """The declarations of an element's style attribute, CSS comments removed."""
+    text = re.sub(r"/\*.*?\*/", "", el.get("style") or "", flags=re.S)
+    out = {}
+    for decl in text.split(";"):
+        if ":" in decl:
+            k, v = decl.split(":", 1)
+            if k.strip() in PRESENTATION:
+                out[k.strip()] = v.strip()
+    return out
+
+
 def _attrs(el, parent):
     a = {k: v for k, v in parent.items() if k in INHERITED}
     for k, v in el.attrib.items():
         a[k] = v
-    for decl in (el.get("style") or "").split(";"):
-        if ":" in decl:
-            k, v = decl.split(":", 1)
-            a[k.strip()] = v.strip()
-    for k in ("opacity", "stroke-dasharray", "mask", "filter", "marker-start", "marker-mid", "marker-end"):
+    a.update(_style(el))
+    for k in ("opacity", "stroke-dasharray", "mask", "filter", "marker-start", "marker-mid", "marker-end", "marker"):
         v = a.get(k)
         if v is not None and v.strip() not in ("none", "1", ""):
+            raise SvgUnsupported(f"{k}={v}")
+    for k, ok in (("mix-blend-mode", ("normal",)), ("isolation", ("auto",)), ("transform-origin", ())):
+        v = a.get(k)
+        if v is not None and v.strip() not in ok:
             raise SvgUnsupported(f"{k}={v}")
     return a
 
@@ -556,13 +582,7 @@ def _gradient_stops(el):
 
 def _presentation(el, name):
     """An element's own value of a presentation attribute (style wins), or None."""
-    value = el.get(name)
-    for decl in (el.get("style") or "").split(";"):
-        if ":" in decl:
-            k, v = decl.split(":", 1)
-            if k.strip() == name:
-                value = v.strip()
-    return value
+    return _style(el).get(name, el.get(name))
 
 
 def _local(tag):
@@ -588,16 +608,21 @@ class Document:
             vb = None
         self.width = _length(w_attr) if w_attr and not w_attr.strip().endswith("%") else (vb[2] if vb else 100.0)
         self.height = _length(h_attr) if h_attr and not h_attr.strip().endswith("%") else (vb[3] if vb else 100.0)
+        if root.get("background-color") is not None or "background-color" in (root.get("style") or ""):
+            raise SvgUnsupported("background-color on the root")
+        if not vb and not (w_attr and h_attr):
+            raise SvgUnsupported("a root without a viewBox or a size")
         view = IDENTITY
         if vb:
             sx, sy = self.width / vb[2], self.height / vb[3]
-            s = min(sx, sy)  # preserveAspectRatio xMidYMid meet
-            par = (root.get("preserveAspectRatio") or "xMidYMid").split()
-            if par and par[0] == "none":
+            align, slice_ = _aspect(root.get("preserveAspectRatio"))
+            if align == "none":
                 view = (sx, 0.0, 0.0, sy, -vb[0] * sx, -vb[1] * sy)
             else:
-                tx = (self.width - vb[2] * s) / 2.0 - vb[0] * s
-                ty = (self.height - vb[3] * s) / 2.0 - vb[1] * s
+                s = max(sx, sy) if slice_ else min(sx, sy)
+                free_x, free_y = self.width - vb[2] * s, self.height - vb[3] * s
+                tx = free_x * align[0] / 2.0 - vb[0] * s
+                ty = free_y * align[1] / 2.0 - vb[1] * s
                 view = (s, 0.0, 0.0, s, tx, ty)
         self.view_w, self.view_h = (vb[2], vb[3]) if vb else (self.width, self.height)
         self.clips = {el.get("id"): el for el in root.iter() if _local(el.tag) == "clipPath" and el.get("id")}
@@ -664,6 +689,8 @@ class Document:
         user-space geometry, stops, spread and paint-server transform, as usvg's
         paint-server conversion leaves them), or None (no paint)."""
         value = value.strip()
+        if "icc-color" in value:
+            raise SvgUnsupported("icc-color")
         if not value.startswith("url("):
             return _color(value)
         m = re.fullmatch(r"url\(\s*#([^)\s]+)\s*\)\s*(.*)", value)
@@ -698,7 +725,9 @@ class Document:
 
         stops = _gradient_stops(next((g for g in chain if any(_local(c.tag) == "stop" for c in g)), None))
         if len(stops) < 2:
-            return (stops[0][1][:3], stops[0][1][3]) if stops else None
+            if stops:
+                return (stops[0][1][:3], stops[0][1][3])
+            return _color(m.group(2)) if m.group(2) else None  # no stops: the fallback, or no paint
         units = attr("gradientUnits") or "objectBoundingBox"
         bbox_units = units != "userSpaceOnUse"
         W, H = self.view_w, self.view_h
@@ -722,8 +751,8 @@ class Document:
             if box is None:
                 raise SvgUnsupported("objectBoundingBox of this element")
             x0, y0, x1, y1 = box
-            if not (x1 > x0 and y1 > y0):
-                return None  # usvg drops a bounding-box paint on a zero-sized box
+            if not (x1 > x0 and y1 > y0):  # usvg drops a bounding-box paint on a zero-sized box
+                return _color(m.group(2)) if m.group(2) else None
             ts = _mul((x1 - x0, 0.0, 0.0, y1 - y0, x0, y0), ts)
         if _local(el.tag) == "linearGradient" and _local(chain[0].tag) == "linearGradient":
             geom = ("linear", length("x1", "0%", W), length("y1", "0%", H), length("x2", "100%", W),
@@ -781,9 +810,14 @@ class Document:
         a = _attrs(el, parent)
         if a.get("display") == "none":
             return
-        t = _mul(transform, _transform(a.get("transform"))) if "transform" in el.attrib else transform
-        t32 = _concat_f32(transform32, _to_f32(_transform(a.get("transform")))) if "transform" in el.attrib \
-            else transform32
+        langs = el.get("systemLanguage")
+        if langs is not None:  # the system language is taken as en-US
+            tags = [x.strip().lower() for x in langs.split(",")]
+            if not any(x == "en" or x.startswith("en-") for x in tags):
+                return
+        has_t = "transform" in el.attrib or "transform" in _style(el)
+        t = _mul(transform, _transform(a.get("transform"))) if has_t else transform
+        t32 = _concat_f32(transform32, _to_f32(_transform(a.get("transform")))) if has_t else transform32
         clip_ref = a.get("clip-path", "none").strip()
         if clip_ref != "none":
             layer = self._clip(clip_ref, a, tag, t)
