@@ -29,7 +29,7 @@ FOLDERS = ["shapes/circle", "shapes/ellipse", "shapes/line", "shapes/path", "sha
            "paint-servers/stop", "paint-servers/stop-color", "paint-servers/stop-opacity", "painting/display",
            "painting/fill", "painting/fill-opacity", "painting/image-rendering", "painting/shape-rendering",
            "painting/stroke", "painting/stroke-opacity", "painting/visibility", "structure/defs", "structure/g",
-           "structure/style-attribute", "structure/svg", "structure/systemLanguage", "structure/transform"]
+           "structure/style-attribute", "structure/svg", "structure/systemLanguage", "structure/transform", "painting/color"]
 # Inside the subset but outside the golden bounds, for a reason the importer does not
 # model: the reference strokes curves with kurbo's offset curves, this recorder
 # flattens first (0.4-1.0 % of pixels off by more than 8/255 on these thin curves).
@@ -37,6 +37,9 @@ KNOWN_DIFFERENCES = {
     "painting/stroke/control-points-clamping-1.svg": "curve stroke offsetting",
     "painting/stroke/line-as-curve-1.svg": "curve stroke offsetting",
     "painting/stroke/line-as-curve-2.svg": "curve stroke offsetting",
+    # a translucent hsla stop over a transparent one: the golden's mid-ramp colour
+    # (straight 195 for 191.5 from the stops) is not reproduced; mean 0.16/255
+    "paint-servers/linearGradient/hsla-color.svg": "translucent hsla stop interpolation",
 }
 HREF = re.compile(r'href="([^"]*)"')
 

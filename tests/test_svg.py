@@ -199,3 +199,14 @@ def test_gradient_stops_follow_usvg():
     # objectBoundingBox units: the gradient's unit square is the element's box
     assert g["transform"] == (10.0, 0.0, 0.0, 10.0, 0.0, 0.0)
     assert g["geom"] == ("linear", 0.0, 0.0, 1.0, 0.0)
+
+
+def test_colour_syntax():
+    assert svg._color("hsl(120, 100%, 25%)") == (0, 128, 0)
+    assert svg._color("rgba(10, 20, 30, 0.25)") == (10, 20, 30, 0.25)
+    assert svg._color("RGB(10%, 50%, 100%)") == (26, 128, 255)  # percentages of 255, rounded half away
+    assert svg._color("currentColor", (1, 2, 3)) == (1, 2, 3)
+    assert svg._color("transparent") == (0, 0, 0, 0.0)
+    for bad in ("rgba(0, 127, 0, 50%)", "rgb(0, 50%, 0)"):  # forms the reference's parser rejects
+        with pytest.raises(svg.InvalidColor):
+            svg._color(bad)

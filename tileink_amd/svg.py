@@ -43,13 +43,17 @@ COLORS = {
 UNSUPPORTED = {"text", "use", "pattern", "mask", "filter",
                "marker", "symbol", "switch", "foreignObject", "style", "a", "tspan", "textPath"}
 INHERITED = ("fill", "fill-opacity", "fill-rule", "stroke", "stroke-opacity", "stroke-width", "stroke-linejoin",
-             "stroke-linecap", "stroke-miterlimit", "visibility", "font-size", "image-rendering", "clip-rule")
+             "stroke-linecap", "stroke-miterlimit", "visibility", "font-size", "image-rendering", "clip-rule", "color")
 SHAPES = ("path", "rect", "circle", "ellipse", "line", "polyline", "polygon")
 XLINK_HREF = "{http://www.w3.org/1999/xlink}href"
 
 
 class SvgUnsupported(ValueError):
     pass
+
+
+class InvalidColor(SvgUnsupported):
+    """A colour the reference's parser rejects (the property keeps its default)."""
 
 
 # --- numbers, lengths, colours, transforms ------------------------------------------
@@ -81,11 +85,29 @@ def _number(text, default):
         return default
 
 
-def _color(text):
-    """(r, g, b) in 0-255, or None for 'none'."""
+def _round_half_away(v):
+    return int(math.floor(abs(v) + 0.5)) * (1 if v >= 0 else -1)
+
+
+def _css_number(c, text, percent_scale):
+    c = c.strip()
+    try:
+        return float(c[:-1]) / 100.0 * percent_scale if c.endswith("%") else float(c)
+    except ValueError:
+        raise SvgUnsupported(f"colour {text!r}") from None
+
+
+def _color(text, current=(0, 0, 0)):
+    """(r, g, b) in 0-255 for an opaque colour, (r, g, b, alpha) with alpha in [0, 1)
+    for a translucent one (rgba, hsla, transparent), or None for 'none'.
+    currentColor is `current`."""
     t = text.strip().lower()
     if t == "none":
         return None
+    if t == "currentcolor":
+        return current
+    if t == "transparent":
+        return (0, 0, 0, 0.0)
     if t in COLORS:
         return COLORS[t]
     if t.startswith("#"):
@@ -95,18 +117,40 @@ def _color(text):
         if len(h) == 3:
             return tuple(int(c * 2, 16) for c in h)
         return tuple(int(h[i:i + 2], 16) for i in (0, 2, 4))
-    m = re.fullmatch(r"rgb\(\s*([^,]+),\s*([^,]+),\s*([^)]+)\)", t)
-    if m:
-        out = []
-        for c in m.groups():
-            c = c.strip()
-            try:
-                v = float(c[:-1]) / 100.0 * 255.0 if c.endswith("%") else float(c)
-            except ValueError:
-                raise SvgUnsupported(f"colour {text!r}") from None
-            out.append(int(math.floor(abs(v) + 0.5)) * (1 if v >= 0 else -1))
-        return tuple(max(0, min(255, v)) for v in out)
-    raise SvgUnsupported(f"colour {text!r}")
+    m = re.fullmatch(r"(rgba?|hsla?)\(([^)]*)\)", t)
+    if not m:
+        raise SvgUnsupported(f"colour {text!r}")
+    parts = [x for x in m.group(2).split(",")]
+    if len(parts) not in (3, 4):
+        raise SvgUnsupported(f"colour {text!r}")
+    alpha = 1.0
+    if len(parts) == 4:
+        if parts[3].strip().endswith("%"):
+            raise InvalidColor(f"colour {text!r}")  # a percentage alpha is not taken
+        alpha = min(max(_css_number(parts[3], text, 1.0), 0.0), 1.0)
+    if m.group(1).startswith("rgb"):
+        if len({c.strip().endswith("%") for c in parts[:3]}) > 1:
+            raise InvalidColor(f"colour {text!r}")  # numbers and percentages mixed
+        rgb = tuple(max(0, min(255, _round_half_away(_css_number(c, text, 255.0)))) for c in parts[:3])
+    else:  # CSS hsl(): hue in degrees, saturation and lightness in percent
+        hue = (_css_number(parts[0], text, 1.0) % 360.0) / 360.0
+        sat = min(max(_css_number(parts[1], text, 1.0), 0.0), 1.0)
+        light = min(max(_css_number(parts[2], text, 1.0), 0.0), 1.0)
+        t2 = light * (sat + 1.0) if light <= 0.5 else light + sat - light * sat
+        t1 = light * 2.0 - t2
+
+        def channel(h):
+            h = h % 1.0
+            if h * 6.0 < 1.0:
+                return t1 + (t2 - t1) * h * 6.0
+            if h * 2.0 < 1.0:
+                return t2
+            if h * 3.0 < 2.0:
+                return t1 + (t2 - t1) * (2.0 / 3.0 - h) * 6.0
+            return t1
+
+        rgb = tuple(max(0, min(255, _round_half_away(channel(hue + d) * 255.0))) for d in (1 / 3, 0.0, -1 / 3))
+    return rgb if alpha >= 1.0 else rgb + (alpha,)
 
 
 def _mul(a, b):
@@ -533,7 +577,7 @@ def _opacity(text):
     return min(max(v, 0.0), 1.0)
 
 
-def _gradient_stops(el):
+def _gradient_stops(el, inherited_color="black"):
     """usvg's stop conversion: offsets (numbers or percentages) clamped to [0, 1]
     and non-decreasing, runs of three equal offsets cut to their ends, leading
     zeros and equal neighbours pulled apart by an epsilon; colour and opacity
@@ -553,10 +597,16 @@ def _gradient_stops(el):
         prev = off
         off = _f32(min(max(off, 0.0), 1.0))
         cv = _presentation(st, "stop-color")
-        rgb = (0, 0, 0) if cv is None else _color(cv)
+        if cv is not None and cv.strip() == "inherit":  # from the gradient element
+            cv = _presentation(el, "stop-color")
+        current = _color(_presentation(st, "color") or inherited_color) or (0, 0, 0)
+        rgb = (0, 0, 0) if cv is None or cv.strip() == "inherit" else _color(cv, current[:3])
+        color_alpha = 1.0
         if rgb is None:
             rgb = (0, 0, 0)
-        alpha = _opacity(_presentation(st, "stop-opacity"))
+        elif len(rgb) == 4:
+            rgb, color_alpha = rgb[:3], rgb[3]
+        alpha = _opacity(_presentation(st, "stop-opacity")) * color_alpha
         stops.append([off, rgb, alpha])
     eps = float(np.finfo(np.float32).eps)
 
@@ -684,21 +734,30 @@ class Document:
         path, child_t, rule = shapes[0]
         return ("clip", path, _mul(ct, child_t), tk.FillRule.EvenOdd if rule == "evenodd" else tk.FillRule.NonZero)
 
-    def _paint(self, value, tag, a):
+    def _paint(self, value, tag, a, prop="fill"):
         """A fill or stroke value: (r, g, b) in 0-255, a gradient (a dict with the
         user-space geometry, stops, spread and paint-server transform, as usvg's
         paint-server conversion leaves them), or None (no paint)."""
         value = value.strip()
         if "icc-color" in value:
             raise SvgUnsupported("icc-color")
+        current = _color(a.get("color", "black")) or (0, 0, 0)
+        if len(current) == 4:
+            raise SvgUnsupported("a translucent currentColor")
+        if value == "inherit":
+            raise SvgUnsupported("paint inherit")
         if not value.startswith("url("):
-            return _color(value)
+            try:
+                c = _color(value, current)
+            except InvalidColor:  # the property keeps its initial value
+                c = (0, 0, 0) if prop == "fill" else None
+            return (c[:3], c[3]) if c is not None and len(c) == 4 else c
         m = re.fullmatch(r"url\(\s*#([^)\s]+)\s*\)\s*(.*)", value)
         if not m:
             raise SvgUnsupported(f"paint {value}")
         el = self.gradients.get(m.group(1))
         if el is None:  # a missing server: the fallback colour, or no paint
-            return _color(m.group(2)) if m.group(2) else None
+            return self._fallback(m.group(2), current)
         chain, seen = [el], {id(el)}
         while True:  # xlink:href: attributes and stops from the referenced gradients
             href = chain[-1].get(XLINK_HREF) or chain[-1].get("href")
@@ -723,11 +782,19 @@ class Document:
                     return v
             return None
 
-        stops = _gradient_stops(next((g for g in chain if any(_local(c.tag) == "stop" for c in g)), None))
+        holder = next((g for g in chain if any(_local(c.tag) == "stop" for c in g)), None)
+        inherited, node = "black", holder
+        while node is not None:  # `color` for currentColor stops, through the document's ancestors
+            v = _presentation(node, "color")
+            if v is not None:
+                inherited = v
+                break
+            node = self.parent_of.get(node)
+        stops = _gradient_stops(holder, inherited)
         if len(stops) < 2:
             if stops:
                 return (stops[0][1][:3], stops[0][1][3])
-            return _color(m.group(2)) if m.group(2) else None  # no stops: the fallback, or no paint
+            return self._fallback(m.group(2), current)  # no stops: the fallback, or no paint
         units = attr("gradientUnits") or "objectBoundingBox"
         bbox_units = units != "userSpaceOnUse"
         W, H = self.view_w, self.view_h
@@ -752,7 +819,7 @@ class Document:
                 raise SvgUnsupported("objectBoundingBox of this element")
             x0, y0, x1, y1 = box
             if not (x1 > x0 and y1 > y0):  # usvg drops a bounding-box paint on a zero-sized box
-                return _color(m.group(2)) if m.group(2) else None
+                return self._fallback(m.group(2), current)
             ts = _mul((x1 - x0, 0.0, 0.0, y1 - y0, x0, y0), ts)
         if _local(el.tag) == "linearGradient" and _local(chain[0].tag) == "linearGradient":
             geom = ("linear", length("x1", "0%", W), length("y1", "0%", H), length("x2", "100%", W),
@@ -771,6 +838,13 @@ class Document:
         else:
             raise SvgUnsupported("gradient type")
         return {"geom": geom, "stops": stops, "spread": spread, "transform": ts}
+
+    @staticmethod
+    def _fallback(text, current):
+        if not text:
+            return None
+        c = _color(text, current)
+        return (c[:3], c[3]) if c is not None and len(c) == 4 else c
 
     def _bbox(self, tag, a):
         """The object bounding box (geometry only) of a basic shape."""
@@ -849,7 +923,7 @@ class Document:
             return
         fill = self._paint(a.get("fill", "black"), tag, a)
         fill_op = _opacity(a.get("fill-opacity"))
-        stroke = self._paint(a.get("stroke", "none"), tag, a)
+        stroke = self._paint(a.get("stroke", "none"), tag, a, "stroke")
         stroke_op = _opacity(a.get("stroke-opacity"))
         if tag in ("line", "polyline"):
             fill = fill if tag == "polyline" else None
