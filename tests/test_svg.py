@@ -210,3 +210,15 @@ def test_colour_syntax():
     for bad in ("rgba(0, 127, 0, 50%)", "rgb(0, 50%, 0)"):  # forms the reference's parser rejects
         with pytest.raises(svg.InvalidColor):
             svg._color(bad)
+
+
+def test_use_clones_with_its_offset_and_properties():
+    doc = svg.Document('<svg xmlns="http://www.w3.org/2000/svg" xmlns:xlink="http://www.w3.org/1999/xlink" '
+                       'viewBox="0 0 100 100"><defs><rect id="r" width="10" height="10"/></defs>'
+                       '<use xlink:href="#r" x="5" y="7" fill="green" transform="scale(2)"/></svg>')
+    assert len(doc.draws) == 1
+    _, path, t, fill, _ = doc.draws[0]
+    assert t == (2.0, 0.0, 0.0, 2.0, 10.0, 14.0) and fill[0] == (0, 128, 0)
+    with pytest.raises(svg.SvgUnsupported):  # a use inside what it references
+        svg.Document('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 10 10">'
+                     '<g id="g"><use href="#g"/></g></svg>')

@@ -40,7 +40,7 @@ COLORS = {
     "purple": (128, 0, 128), "orange": (255, 165, 0), "pink": (255, 192, 203), "brown": (165, 42, 42),
     "seagreen": (46, 139, 87), "lightblue": (173, 216, 230), "darkgreen": (0, 100, 0),
 }
-UNSUPPORTED = {"text", "use", "pattern", "mask", "filter",
+UNSUPPORTED = {"text", "pattern", "mask", "filter",
                "marker", "symbol", "switch", "foreignObject", "style", "a", "tspan", "textPath"}
 INHERITED = ("fill", "fill-opacity", "fill-rule", "stroke", "stroke-opacity", "stroke-width", "stroke-linejoin",
              "stroke-linecap", "stroke-miterlimit", "visibility", "font-size", "image-rendering", "clip-rule", "color")
@@ -679,6 +679,11 @@ class Document:
         self.gradients = {el.get("id"): el for el in root.iter()
                           if _local(el.tag) in ("linearGradient", "radialGradient") and el.get("id")}
         self.parent_of = {child: node for node in root.iter() for child in node}
+        self.ids = {}
+        for el in root.iter():  # the first element of an id wins
+            if el.get("id") and el.get("id") not in self.ids:
+                self.ids[el.get("id")] = el
+        self.use_stack = []
         self.draws = []
         self._walk(root, {}, view, _to_f32(view))
 
@@ -840,6 +845,10 @@ class Document:
         return {"geom": geom, "stops": stops, "spread": spread, "transform": ts}
 
     @staticmethod
+    def _contains(outer, inner):
+        return any(node is inner for node in outer.iter())
+
+    @staticmethod
     def _fallback(text, current):
         if not text:
             return None
@@ -908,6 +917,26 @@ class Document:
         self._walk_clipped(el, tag, a, t, t32)
 
     def _walk_clipped(self, el, tag, a, t, t32):
+        if tag == "use":  # a clone of the referenced element under translate(x, y), inheriting from the use
+            href = (el.get(XLINK_HREF) or el.get("href") or "").strip()
+            target = self.ids.get(href[1:]) if href.startswith("#") else None
+            if target is None:
+                return
+            if _local(target.tag) in ("svg", "symbol"):
+                raise SvgUnsupported("use of an svg or symbol")
+            if target in self.use_stack or any(target is p or self._contains(target, p) for p in self.use_stack):
+                raise SvgUnsupported("recursive use")
+            fs = _length(a["font-size"]) if "font-size" in a else FONT_SIZE
+            x = _length(a.get("x", "0"), self.view_w, fs)
+            y = _length(a.get("y", "0"), self.view_h, fs)
+            shift = (1.0, 0.0, 0.0, 1.0, x, y)
+            self.use_stack.append(el)
+            try:
+                inherited = {k: v for k, v in a.items() if k in INHERITED}
+                self._walk(target, inherited, _mul(t, shift), _concat_f32(t32, _to_f32(shift)))
+            finally:
+                self.use_stack.pop()
+            return
         if tag in ("svg", "g"):
             if tag == "svg" and self.draws is not None and getattr(self, "_entered", False):
                 raise SvgUnsupported("nested svg")
