@@ -565,6 +565,12 @@ __device__ __forceinline__ float sdf_coverage_from_draw(const tk_fine_config &cf
 
 __device__ __forceinline__ bool premul_opaque(uint32_t c) { return (c >> 24) == 255u; }
 
+// The text composite out of line: only path-glyph records reach it, and inlined
+// into fine's record loop its temporaries cost the loop a spilled register.
+__device__ __attribute__((noinline)) uint32_t glyph_composite(uint32_t destination, uint32_t source, uint32_t coverage) {
+    return src_over_mask_linear_auto_u8(destination, source, coverage);
+}
+
 // Exact coverage sums of one FILL / BEGIN_CLIP record over a 16x16 tile, built
 // segment by segment instead of pixel by pixel, with 64-bit integer atomics in
 // LDS (exact, so the order of arrival does not matter):
@@ -1822,7 +1828,7 @@ __device__ __forceinline__ void fine_tile(const tk_args_fine &a) {
                                     color = sample_draw_brush(cfg, paint, load_fine_draw(a.in.draws, cfg.draw_count, rec.color),
                                                               sx, sy, atlas_of(a.in));
                                 if (tag == kPtclPathGlyph)  // text compositing (interpreter.hlsli:205-206)
-                                    pixel = rgba8_to_unorm(src_over_mask_linear_auto_u8(unorm_to_rgba8(pixel), color, masked));
+                                    pixel = rgba8_to_unorm(glyph_composite(unorm_to_rgba8(pixel), color, masked));
                                 else if (masked == 255u && premul_opaque(color)) pixel = rgba8_to_unorm(color);
                                 else pixel = src_over_premul_unorm(pixel, scale_premul_u8_to_unorm(color, masked));
                             }
