@@ -222,3 +222,14 @@ def test_use_clones_with_its_offset_and_properties():
     with pytest.raises(svg.SvgUnsupported):  # a use inside what it references
         svg.Document('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 10 10">'
                      '<g id="g"><use href="#g"/></g></svg>')
+
+
+def test_transform_origin_and_switch():
+    doc = svg.Document('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 200 100">'
+                       '<rect width="10" height="10" transform="scale(2)" transform-origin="right top"/></svg>')
+    # translate(200, 0) scale(2) translate(-200, 0)
+    assert doc.draws[0][2] == (2.0, 0.0, 0.0, 2.0, -200.0, 0.0)
+    doc = svg.Document('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 10 10"><switch>'
+                       '<rect width="1" height="1" systemLanguage="ru"/><rect width="2" height="2" fill="red"/>'
+                       '<rect width="3" height="3"/></switch></svg>')
+    assert len(doc.draws) == 1 and doc.draws[0][3][0] == (255, 0, 0)

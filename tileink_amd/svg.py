@@ -41,7 +41,7 @@ COLORS = {
     "seagreen": (46, 139, 87), "lightblue": (173, 216, 230), "darkgreen": (0, 100, 0),
 }
 UNSUPPORTED = {"text", "pattern", "mask", "filter",
-               "marker", "symbol", "switch", "foreignObject", "style", "a", "tspan", "textPath"}
+               "marker", "symbol", "foreignObject", "style", "a", "tspan", "textPath"}
 INHERITED = ("fill", "fill-opacity", "fill-rule", "stroke", "stroke-opacity", "stroke-width", "stroke-linejoin",
              "stroke-linecap", "stroke-miterlimit", "visibility", "font-size", "image-rendering", "clip-rule", "color")
 SHAPES = ("path", "rect", "circle", "ellipse", "line", "polyline", "polygon")
@@ -558,7 +558,7 @@ def _attrs(el, parent):
         v = a.get(k)
         if v is not None and v.strip() not in ("none", "1", ""):
             raise SvgUnsupported(f"{k}={v}")
-    for k, ok in (("mix-blend-mode", ("normal",)), ("isolation", ("auto",)), ("transform-origin", ())):
+    for k, ok in (("mix-blend-mode", ("normal",)), ("isolation", ("auto",))):
         v = a.get(k)
         if v is not None and v.strip() not in ok:
             raise SvgUnsupported(f"{k}={v}")
@@ -696,6 +696,8 @@ class Document:
         if not m or m.group(1) not in self.clips:
             raise SvgUnsupported(f"clip-path={value}")
         clip = self.clips[m.group(1)]
+        if _presentation(clip, "transform-origin") is not None:
+            raise SvgUnsupported("transform-origin on a clipPath")
         # clip-rule is inherited through the clipPath's own ancestors in the document
         rule, node = "nonzero", clip
         while node is not None:
@@ -775,6 +777,8 @@ class Document:
             seen.add(id(nxt))
 
         kind = _local(el.tag)
+        if any(_presentation(g, "transform-origin") is not None for g in chain):
+            raise SvgUnsupported("transform-origin on a gradient")
 
         def attr(name, same_kind=False):
             # geometry comes only from gradients of the same kind; units, spread,
@@ -844,6 +848,25 @@ class Document:
             raise SvgUnsupported("gradient type")
         return {"geom": geom, "stops": stops, "spread": spread, "transform": ts}
 
+    def _origin(self, text):
+        """transform-origin against the viewport (the view box of SVG content):
+        keywords, lengths and percentages; a single value leaves the other centred."""
+        words = text.replace(",", " ").split()
+        if not words or len(words) > 2:
+            raise SvgUnsupported(f"transform-origin {text!r}")
+        kw = {"left": ("x", 0.0), "right": ("x", 1.0), "top": ("y", 0.0), "bottom": ("y", 1.0), "center": (None, 0.5)}
+        if len(words) == 1:
+            words = [words[0], "center"] if kw.get(words[0], ("x",))[0] != "y" else ["center", words[0]]
+        if kw.get(words[0], (None,))[0] == "y" or kw.get(words[1], (None,))[0] == "x":
+            words = [words[1], words[0]]
+
+        def value(w, ref):
+            if w in kw:
+                return kw[w][1] * ref
+            return _length(w, ref)
+
+        return value(words[0], self.view_w), value(words[1], self.view_h)
+
     @staticmethod
     def _contains(outer, inner):
         return any(node is inner for node in outer.iter())
@@ -899,8 +922,13 @@ class Document:
             if not any(x == "en" or x.startswith("en-") for x in tags):
                 return
         has_t = "transform" in el.attrib or "transform" in _style(el)
-        t = _mul(transform, _transform(a.get("transform"))) if has_t else transform
-        t32 = _concat_f32(transform32, _to_f32(_transform(a.get("transform")))) if has_t else transform32
+        own = _transform(a.get("transform")) if has_t else IDENTITY
+        origin = _presentation(el, "transform-origin")
+        if has_t and origin is not None:  # translate(o) transform translate(-o)
+            ox, oy = self._origin(origin)
+            own = _mul((1.0, 0.0, 0.0, 1.0, ox, oy), _mul(own, (1.0, 0.0, 0.0, 1.0, -ox, -oy)))
+        t = _mul(transform, own) if has_t else transform
+        t32 = _concat_f32(transform32, _to_f32(own)) if has_t else transform32
         clip_ref = a.get("clip-path", "none").strip()
         if clip_ref != "none":
             layer = self._clip(clip_ref, a, tag, t)
@@ -936,6 +964,19 @@ class Document:
                 self._walk(target, inherited, _mul(t, shift), _concat_f32(t32, _to_f32(shift)))
             finally:
                 self.use_stack.pop()
+            return
+        if tag == "switch":  # the first child whose conditions pass (the system language is en-US)
+            for child in el:
+                if not isinstance(child.tag, str):
+                    continue
+                if child.get("requiredExtensions") is not None:
+                    continue
+                langs = child.get("systemLanguage")
+                if langs is not None and not any(x.strip().lower() == "en" or x.strip().lower().startswith("en-")
+                                                 for x in langs.split(",")):
+                    continue
+                self._walk(child, a, t, t32)
+                break
             return
         if tag in ("svg", "g"):
             if tag == "svg" and self.draws is not None and getattr(self, "_entered", False):
