@@ -49,7 +49,7 @@ def test_unsupported_content_is_refused():
         svg.load('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 10 10"><text>a</text></svg>', 100)
     with pytest.raises(svg.SvgUnsupported):
         svg.load('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 10 10">'
-                 '<path d="M0 0L5 5" stroke="red" stroke-dasharray="1 2"/></svg>', 100)
+                 '<path d="M0 0L5 5" stroke="red" marker-end="url(#m)"/></svg>', 100)
 
 
 @pytest.mark.parametrize("name", svg_golden.fixtures())
@@ -233,3 +233,15 @@ def test_transform_origin_and_switch():
                        '<rect width="1" height="1" systemLanguage="ru"/><rect width="2" height="2" fill="red"/>'
                        '<rect width="3" height="3"/></switch></svg>')
     assert len(doc.draws) == 1 and doc.draws[0][3][0] == (255, 0, 0)
+
+
+def test_dashes_cut_the_outline_along_its_length():
+    # a 30-unit line dashed 4 on, 2 off from offset 1: on [0, 3), [5, 9), ..., as open subpaths
+    line = tk.BezPath().move_to(0, 0).line_to(30, 0)
+    verbs, pts = svg._dash(line, [4.0, 2.0], 1.0).arrays()
+    pts = np.asarray(pts).reshape(-1, 2)
+    starts = pts[np.asarray(verbs) == tk._abi.PATH_MOVE][:, 0]
+    assert np.allclose(starts, [0, 5, 11, 17, 23, 29])
+    doc = svg.Document('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 10 10">'
+                       '<path d="M0 0L5 5" stroke="red" stroke-dasharray="1 -2"/></svg>')
+    assert doc.draws[1][1] is doc.draws[0][1]  # a negative entry leaves the stroke solid (undashed)

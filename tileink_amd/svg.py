@@ -514,6 +514,73 @@ def _ellipse_path(cx, cy, rx, ry):
     return p.close_path()
 
 
+def _polylines(path, pieces=64):
+    """The subpaths of a BezPath as point lists, curves cut into `pieces` chords,
+    each closed subpath ending back at its start (so its closing segment dashes too)."""
+    verbs, pts = path.arrays()
+    pts = np.asarray(pts, np.float64).reshape(-1, 2)
+    out, cur, i = [], None, 0
+    u = np.linspace(0.0, 1.0, pieces + 1)[1:, None]
+    for v in np.asarray(verbs):
+        if v == tk._abi.PATH_MOVE:
+            if cur is not None and len(cur) > 1:
+                out.append(cur)
+            cur = [pts[i]]
+            i += 1
+        elif v == tk._abi.PATH_LINE:
+            cur.append(pts[i])
+            i += 1
+        elif v == tk._abi.PATH_QUAD:
+            p0, p1, p2 = cur[-1], pts[i], pts[i + 1]
+            cur.extend((1 - u) ** 2 * p0 + 2 * (1 - u) * u * p1 + u * u * p2)
+            i += 2
+        elif v == tk._abi.PATH_CUBIC:
+            p0, p1, p2, p3 = cur[-1], pts[i], pts[i + 1], pts[i + 2]
+            cur.extend((1 - u) ** 3 * p0 + 3 * (1 - u) ** 2 * u * p1 + 3 * (1 - u) * u * u * p2 + u ** 3 * p3)
+            i += 3
+        elif v == tk._abi.PATH_CLOSE and cur is not None:
+            cur.append(cur[0])
+            out.append(cur)
+            cur = [cur[0]]
+    if cur is not None and len(cur) > 1:
+        out.append(cur)
+    return out
+
+
+def _dash(path, dashes, offset):
+    """The dashes of a path as open subpaths (kurbo's dash iterator on the
+    flattened outline): the pattern restarts at every subpath, shifted by offset."""
+    total = sum(dashes)
+    out = tk.BezPath()
+    for poly in _polylines(path):
+        pos = offset % total
+        k = 0
+        while pos >= dashes[k]:
+            pos -= dashes[k]
+            k = (k + 1) % len(dashes)
+        left, on, drawing = dashes[k] - pos, k % 2 == 0, False
+        if on:
+            out.move_to(*poly[0])
+            drawing = True
+        for a, b in zip(poly[:-1], poly[1:]):
+            seg = float(np.hypot(*(b - a)))
+            done = 0.0
+            while seg - done > left:
+                done += left
+                q = a + (b - a) * (done / seg)
+                if on:
+                    out.line_to(*q)
+                else:
+                    out.move_to(*q)
+                on = not on
+                k = (k + 1) % len(dashes)
+                left = dashes[k]
+            left -= seg - done
+            if on:
+                out.line_to(*b)
+    return out
+
+
 def _rect_path(x, y, w, h, rx, ry):
     if rx <= 0 or ry <= 0:
         return tk.BezPath().move_to(x, y).line_to(x + w, y).line_to(x + w, y + h).line_to(x, y + h).close_path()
@@ -554,7 +621,7 @@ def _attrs(el, parent):
     for k, v in el.attrib.items():
         a[k] = v
     a.update(_style(el))
-    for k in ("opacity", "stroke-dasharray", "mask", "filter", "marker-start", "marker-mid", "marker-end", "marker"):
+    for k in ("opacity", "mask", "filter", "marker-start", "marker-mid", "marker-end", "marker"):
         v = a.get(k)
         if v is not None and v.strip() not in ("none", "1", ""):
             raise SvgUnsupported(f"{k}={v}")
@@ -848,6 +915,24 @@ class Document:
             raise SvgUnsupported("gradient type")
         return {"geom": geom, "stops": stops, "spread": spread, "transform": ts}
 
+    def _dasharray(self, a, width):
+        """usvg's dash list: lengths (percentages of the viewport diagonal), an odd list
+        repeated; a negative entry or a zero sum leaves the stroke solid."""
+        text = a.get("stroke-dasharray")
+        if text is None or text.strip() in ("none", ""):
+            return None
+        diag = math.hypot(self.view_w, self.view_h) / math.sqrt(2)
+        fs = _length(a["font-size"]) if "font-size" in a else FONT_SIZE
+        try:
+            dashes = [_length(w, diag, fs) for w in text.replace(",", " ").split()]
+        except SvgUnsupported:
+            return None
+        if not dashes or any(d < 0 for d in dashes) or sum(dashes) <= 0:
+            return None
+        if any(d == 0 for d in dashes):
+            raise SvgUnsupported("zero-length dashes")  # their caps are not modelled
+        return dashes * 2 if len(dashes) % 2 else dashes
+
     def _origin(self, text):
         """transform-origin against the viewport (the view box of SVG content):
         keywords, lengths and percentages; a single value leaves the other centred."""
@@ -1010,6 +1095,12 @@ class Document:
                 join = {"round": tk.Join.Round, "bevel": tk.Join.Bevel}.get(jv, tk.Join.Miter)
                 cap = {"round": tk.Cap.Round, "square": tk.Cap.Square}.get(a.get("stroke-linecap", "butt"), tk.Cap.Butt)
                 limit = max(_number(a.get("stroke-miterlimit"), 4.0), 1.0)
+                dashes = self._dasharray(a, width)
+                if dashes is not None:
+                    fs = _length(a["font-size"]) if "font-size" in a else FONT_SIZE
+                    offset = _length(a.get("stroke-dashoffset", "0"), math.hypot(self.view_w, self.view_h) / math.sqrt(2),
+                                     fs)
+                    path = _dash(path, dashes, offset)
                 self.draws.append(("shape", path, t, None, (stroke, min(stroke_op, 1.0), width, join, cap, limit)))
 
     def _image(self, el, a, t32):
