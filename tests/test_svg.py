@@ -245,3 +245,13 @@ def test_dashes_cut_the_outline_along_its_length():
     doc = svg.Document('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 10 10">'
                        '<path d="M0 0L5 5" stroke="red" stroke-dasharray="1 -2"/></svg>')
     assert doc.draws[1][1] is doc.draws[0][1]  # a negative entry leaves the stroke solid (undashed)
+
+
+def test_nested_svg_is_a_clipped_viewport():
+    doc = svg.Document('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 200 200">'
+                       '<svg x="20" y="30" width="100" height="50" viewBox="0 0 10 10">'
+                       '<rect width="10" height="10"/></svg></svg>')
+    kinds = [d[0] for d in doc.draws]
+    assert kinds == ["clip", "shape", "pop"]
+    # viewBox 10 x 10 into 100 x 50, xMidYMid meet: scale 5, centred horizontally
+    assert doc.draws[1][2] == (5.0, 0.0, 0.0, 5.0, 45.0, 30.0)

@@ -915,6 +915,46 @@ class Document:
             raise SvgUnsupported("gradient type")
         return {"geom": geom, "stops": stops, "spread": spread, "transform": ts}
 
+    def _nested_svg(self, el, a, t, t32):
+        """A nested svg: a new viewport at (x, y, width, height) of the parent's, its
+        viewBox fitted by preserveAspectRatio, and (overflow not visible) a clip to
+        the viewport, as usvg converts it into a clipped, transformed group."""
+        fs = _length(a["font-size"]) if "font-size" in a else FONT_SIZE
+        W, H = self.view_w, self.view_h
+        x, y = _length(a.get("x", "0"), W, fs), _length(a.get("y", "0"), H, fs)
+        w, h = _length(a.get("width", "100%"), W, fs), _length(a.get("height", "100%"), H, fs)
+        if not (w > 0 and h > 0):
+            return
+        inner = (1.0, 0.0, 0.0, 1.0, x, y)
+        vb = [float(v) for v in _NUM.findall(el.get("viewBox") or "")]
+        view_w, view_h = w, h
+        if len(vb) == 4:
+            if not (vb[2] > 0 and vb[3] > 0):
+                return
+            sx, sy = w / vb[2], h / vb[3]
+            align, slice_ = _aspect(el.get("preserveAspectRatio"))
+            if align == "none":
+                fit = (sx, 0.0, 0.0, sy, -vb[0] * sx, -vb[1] * sy)
+            else:
+                sc = max(sx, sy) if slice_ else min(sx, sy)
+                fit = (sc, 0.0, 0.0, sc, (w - vb[2] * sc) * align[0] / 2.0 - vb[0] * sc,
+                       (h - vb[3] * sc) * align[1] / 2.0 - vb[1] * sc)
+            inner = _mul(inner, fit)
+            view_w, view_h = vb[2], vb[3]
+        clip = a.get("overflow", "hidden").strip() not in ("visible", "auto")
+        if clip:
+            self.draws.append(("clip", _rect_path(x, y, w, h, 0.0, 0.0), t, tk.FillRule.NonZero))
+        saved = (self.view_w, self.view_h)
+        self.view_w, self.view_h = view_w, view_h
+        try:
+            t2, t32b = _mul(t, inner), _concat_f32(t32, _to_f32(inner))
+            for child in el:
+                self._walk(child, a, t2, t32b)
+        finally:
+            self.view_w, self.view_h = saved
+            if clip:
+                self.draws.append(("pop",))
+
     def _dasharray(self, a, width):
         """usvg's dash list: lengths (percentages of the viewport diagonal), an odd list
         repeated; a negative entry or a zero sum leaves the stroke solid."""
@@ -1063,9 +1103,10 @@ class Document:
                 self._walk(child, a, t, t32)
                 break
             return
+        if tag == "svg" and getattr(self, "_entered", False):
+            self._nested_svg(el, a, t, t32)
+            return
         if tag in ("svg", "g"):
-            if tag == "svg" and self.draws is not None and getattr(self, "_entered", False):
-                raise SvgUnsupported("nested svg")
             self._entered = True
             for child in el:
                 self._walk(child, a, t, t32)
