@@ -255,3 +255,16 @@ def test_nested_svg_is_a_clipped_viewport():
     assert kinds == ["clip", "shape", "pop"]
     # viewBox 10 x 10 into 100 x 50, xMidYMid meet: scale 5, centred horizontally
     assert doc.draws[1][2] == (5.0, 0.0, 0.0, 5.0, 45.0, 30.0)
+
+
+def test_css_cascade():
+    doc = svg.Document('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 10 10"><style>'
+                       '#r { fill: red !important } rect { fill: blue } .c { fill: lime; height: 1px }'
+                       ' svg > rect[x] { stroke: black }</style>'
+                       '<rect id="r" x="1" width="4" height="4" fill="gray" style="fill: yellow"/>'
+                       '<rect class="c" width="4" height="4" fill="gray"/></svg>')
+    fills = [d[3][0] for d in doc.draws if d[3] is not None]
+    assert fills == [(255, 0, 0), (0, 255, 0)]  # !important beats style; a class beats a type
+    strokes = [d for d in doc.draws if d[4] is not None]
+    assert len(strokes) == 1  # only the rect with an x attribute
+    assert doc.draws[-1][1] is not None  # height stays the attribute's (not a presentation property)

@@ -41,7 +41,7 @@ COLORS = {
     "seagreen": (46, 139, 87), "lightblue": (173, 216, 230), "darkgreen": (0, 100, 0),
 }
 UNSUPPORTED = {"text", "pattern", "mask", "filter",
-               "marker", "symbol", "foreignObject", "style", "a", "tspan", "textPath"}
+               "marker", "symbol", "foreignObject", "a", "tspan", "textPath"}
 INHERITED = ("fill", "fill-opacity", "fill-rule", "stroke", "stroke-opacity", "stroke-width", "stroke-linejoin",
              "stroke-linecap", "stroke-miterlimit", "visibility", "font-size", "image-rendering", "clip-rule", "color")
 SHAPES = ("path", "rect", "circle", "ellipse", "line", "polyline", "polygon")
@@ -616,11 +616,16 @@ def _style(el):
     return out
 
 
-def _attrs(el, parent):
+def _attrs(el, parent, css=None):
+    """An element's properties: inherited ones, its attributes, then CSS rules
+    (normal), its style attribute, then CSS rules marked !important."""
     a = {k: v for k, v in parent.items() if k in INHERITED}
     for k, v in el.attrib.items():
         a[k] = v
+    normal, important = css if css is not None else ({}, {})
+    a.update(normal)
     a.update(_style(el))
+    a.update(important)
     for k in ("opacity", "mask", "filter", "marker-start", "marker-mid", "marker-end", "marker"):
         v = a.get(k)
         if v is not None and v.strip() not in ("none", "1", ""):
@@ -697,6 +702,103 @@ def _gradient_stops(el, inherited_color="black"):
     return [(o, (c[0], c[1], c[2], a)) for o, c, a in stops]
 
 
+_SIMPLE = re.compile(r"(\*|[A-Za-z][\w-]*)|\.([\w-]+)|#([\w-]+)|\[\s*([\w:-]+)\s*(?:=\s*['\"]?([^'\"\]]*)['\"]?\s*)?\]"
+                     r"|:(first-child)")
+
+
+def _parse_compound(text):
+    """A compound selector as (type, [(kind, name, value)]), or None when it uses
+    something this parser does not take (the rule is then dropped, as CSS does)."""
+    pos, typ, conds = 0, None, []
+    while pos < len(text):
+        m = _SIMPLE.match(text, pos)
+        if not m:
+            return None
+        if m.group(1):
+            if pos:
+                return None
+            typ = m.group(1)
+        elif m.group(2):
+            conds.append(("class", m.group(2), None))
+        elif m.group(3):
+            conds.append(("id", m.group(3), None))
+        elif m.group(4):
+            conds.append(("attr", m.group(4), m.group(5)))
+        else:
+            conds.append(("first-child", None, None))
+        pos = m.end()
+    return typ, conds
+
+
+def _compound_matches(el, compound, parent_of):
+    typ, conds = compound
+    if typ not in (None, "*") and _local(el.tag) != typ:
+        return False
+    for kind, name, value in conds:
+        if kind == "class" and name not in (el.get("class") or "").split():
+            return False
+        if kind == "id" and el.get("id") != name:
+            return False
+        if kind == "attr" and (el.get(name) is None or (value is not None and el.get(name) != value)):
+            return False
+        if kind == "first-child":
+            parent = parent_of.get(el)
+            if parent is None or next((c for c in parent if isinstance(c.tag, str)), None) is not el:
+                return False
+    return True
+
+
+def _css_rules(root):
+    """The rules of the document's CSS style elements: [(selector, specificity, {property: (value, important)})],
+    selector a list of (compound, combinator to its left)."""
+    rules = []
+    for st in root.iter():
+        if _local(st.tag) != "style" or (st.get("type") or "text/css").strip() not in ("text/css", ""):
+            continue
+        text = re.sub(r"/\*.*?\*/", "", "".join(st.itertext()), flags=re.S)
+        if "@" in text:
+            raise SvgUnsupported("CSS at-rules")
+        for block in text.split("}"):
+            if "{" not in block:
+                continue
+            sel_text, body = block.split("{", 1)
+            decls = {}
+            for decl in body.split(";"):
+                if ":" not in decl:
+                    continue
+                k, v = decl.split(":", 1)
+                k, v = k.strip(), v.strip()
+                imp = v.endswith("!important")
+                if imp:
+                    v = v[:-len("!important")].strip()
+                if k in PRESENTATION:
+                    decls[k] = (v, imp)
+            for sel in sel_text.split(","):
+                tokens = re.split(r"\s*(>)\s*|\s+", sel.strip())
+                parts, comb, ok = [], None, True
+                for tok in tokens:
+                    if tok is None or tok == "":
+                        continue
+                    if tok == ">":
+                        comb = ">"
+                        continue
+                    compound = _parse_compound(tok)
+                    if compound is None:
+                        ok = False
+                        break
+                    parts.append((compound, comb if parts else None))
+                    comb = " "
+                if not ok or not parts:
+                    continue
+                # combinator stored with the compound to its right
+                sel_parts = [(parts[0][0], None)] + [(c, cb) for c, cb in parts[1:]]
+                ids = sum(1 for c, _ in sel_parts for k, _, _ in c[1] if k == "id")
+                classes = sum(1 for c, _ in sel_parts for k, _, _ in c[1] if k != "id")
+                types = sum(1 for c, _ in sel_parts if c[0] not in (None, "*"))
+                rules.append((sel_parts, (ids, classes, types), decls))
+    return rules
+
+
 def _presentation(el, name):
     """An element's own value of a presentation attribute (style wins), or None."""
     return _style(el).get(name, el.get(name))
@@ -746,6 +848,7 @@ class Document:
         self.gradients = {el.get("id"): el for el in root.iter()
                           if _local(el.tag) in ("linearGradient", "radialGradient") and el.get("id")}
         self.parent_of = {child: node for node in root.iter() for child in node}
+        self.css = _css_rules(root)
         self.ids = {}
         for el in root.iter():  # the first element of an id wins
             if el.get("id") and el.get("id") not in self.ids:
@@ -773,7 +876,7 @@ class Document:
                 rule = v
                 break
             node = self.parent_of.get(node)
-        ca = _attrs(clip, {"clip-rule": rule})
+        ca = _attrs(clip, {"clip-rule": rule}, self._css(clip))
         if ca.get("clip-path", "none").strip() != "none":
             raise SvgUnsupported("nested clip-path")
         shapes = []
@@ -783,7 +886,7 @@ class Document:
                 continue
             if ctag not in SHAPES:
                 raise SvgUnsupported(f"clipPath child {ctag}")
-            cattrs = _attrs(child, {"clip-rule": ca.get("clip-rule", "nonzero")})
+            cattrs = _attrs(child, {"clip-rule": ca.get("clip-rule", "nonzero")}, self._css(child))
             if cattrs.get("clip-path", "none").strip() != "none":
                 raise SvgUnsupported("clip-path inside clipPath")
             if cattrs.get("display") == "none" or cattrs.get("visibility") in ("hidden", "collapse"):
@@ -915,6 +1018,40 @@ class Document:
             raise SvgUnsupported("gradient type")
         return {"geom": geom, "stops": stops, "spread": spread, "transform": ts}
 
+    def _css(self, el):
+        """(normal, important) declarations of the rules matching el, in cascade order."""
+        if not self.css:
+            return None
+        hits = []
+        for order, (selector, spec, decls) in enumerate(self.css):
+            if self._matches(el, selector):
+                hits.append((spec, order, decls))
+        normal, important = {}, {}
+        for _, _, decls in sorted(hits, key=lambda h: (h[0], h[1])):
+            for k, (v, imp) in decls.items():
+                (important if imp else normal)[k] = v
+        return normal, important
+
+    def _matches(self, el, selector):
+        parts = list(selector)  # [(compound, combinator to the left)], rightmost last
+        node = el
+        compound, comb = parts[-1]
+        if not _compound_matches(node, compound, self.parent_of):
+            return False
+        for compound_l, comb_l in reversed(parts[:-1]):
+            if comb == ">":
+                node = self.parent_of.get(node)
+                if node is None or not _compound_matches(node, compound_l, self.parent_of):
+                    return False
+            else:  # descendant
+                node = self.parent_of.get(node)
+                while node is not None and not _compound_matches(node, compound_l, self.parent_of):
+                    node = self.parent_of.get(node)
+                if node is None:
+                    return False
+            comb = comb_l
+        return True
+
     def _nested_svg(self, el, a, t, t32):
         """A nested svg: a new viewport at (x, y, width, height) of the parent's, its
         viewBox fitted by preserveAspectRatio, and (overflow not visible) a clip to
@@ -1034,11 +1171,11 @@ class Document:
 
     def _walk(self, el, parent, transform, transform32):
         tag = _local(el.tag)
-        if tag in ("title", "desc", "metadata", "defs", "clipPath", "linearGradient", "radialGradient"):
+        if tag in ("title", "desc", "metadata", "defs", "clipPath", "linearGradient", "radialGradient", "style"):
             return
         if tag in UNSUPPORTED:
             raise SvgUnsupported(tag)
-        a = _attrs(el, parent)
+        a = _attrs(el, parent, self._css(el))
         if a.get("display") == "none":
             return
         langs = el.get("systemLanguage")
@@ -1046,9 +1183,9 @@ class Document:
             tags = [x.strip().lower() for x in langs.split(",")]
             if not any(x == "en" or x.startswith("en-") for x in tags):
                 return
-        has_t = "transform" in el.attrib or "transform" in _style(el)
+        has_t = "transform" in a  # an attribute, a CSS rule or the style attribute
         own = _transform(a.get("transform")) if has_t else IDENTITY
-        origin = _presentation(el, "transform-origin")
+        origin = a.get("transform-origin")
         if has_t and origin is not None:  # translate(o) transform translate(-o)
             ox, oy = self._origin(origin)
             own = _mul((1.0, 0.0, 0.0, 1.0, ox, oy), _mul(own, (1.0, 0.0, 0.0, 1.0, -ox, -oy)))
