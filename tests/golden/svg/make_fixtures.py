@@ -31,7 +31,7 @@ FOLDERS = ["shapes/circle", "shapes/ellipse", "shapes/line", "shapes/path", "sha
            "painting/stroke", "painting/stroke-opacity", "painting/visibility", "structure/defs", "structure/g",
            "structure/style-attribute", "structure/svg", "structure/systemLanguage", "structure/transform", "painting/color", "structure/use",
            "structure/transform-origin", "structure/switch", "painting/stroke-dasharray",
-           "painting/stroke-dashoffset", "structure/style"]
+           "painting/stroke-dashoffset", "structure/style", "painting/opacity"]
 # Inside the subset but outside the golden bounds, for a reason the importer does not
 # model: the reference strokes curves with kurbo's offset curves, this recorder
 # flattens first (0.4-1.0 % of pixels off by more than 8/255 on these thin curves).

@@ -626,7 +626,7 @@ def _attrs(el, parent, css=None):
     a.update(normal)
     a.update(_style(el))
     a.update(important)
-    for k in ("opacity", "mask", "filter", "marker-start", "marker-mid", "marker-end", "marker"):
+    for k in ("mask", "filter", "marker-start", "marker-mid", "marker-end", "marker"):
         v = a.get(k)
         if v is not None and v.strip() not in ("none", "1", ""):
             raise SvgUnsupported(f"{k}={v}")
@@ -1207,6 +1207,9 @@ class Document:
         self._walk_clipped(el, tag, a, t, t32)
 
     def _walk_clipped(self, el, tag, a, t, t32):
+        opacity = _opacity(a.get("opacity"))
+        if opacity < 1.0 and tag not in SHAPES:
+            raise SvgUnsupported(f"group opacity={a.get('opacity')}")
         if tag == "use":  # a clone of the referenced element under translate(x, y), inheriting from the use
             href = (el.get(XLINK_HREF) or el.get("href") or "").strip()
             target = self.ids.get(href[1:]) if href.startswith("#") else None
@@ -1260,6 +1263,12 @@ class Document:
         stroke_op = _opacity(a.get("stroke-opacity"))
         if tag in ("line", "polyline"):
             fill = fill if tag == "polyline" else None
+        if opacity < 1.0:
+            # a shape's opacity fades its fill and stroke together as one layer; with one
+            # paint that is the paint's own opacity times it
+            if fill is not None and fill_op > 0 and stroke is not None and stroke_op > 0:
+                raise SvgUnsupported("opacity on a filled and stroked shape")
+            fill_op, stroke_op = fill_op * opacity, stroke_op * opacity
         rule = tk.FillRule.EvenOdd if a.get("fill-rule") == "evenodd" else tk.FillRule.NonZero
         if fill is not None and fill_op > 0:
             self.draws.append(("shape", path, t, (fill, min(fill_op, 1.0), rule), None))
