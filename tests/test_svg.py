@@ -268,3 +268,12 @@ def test_css_cascade():
     strokes = [d for d in doc.draws if d[4] is not None]
     assert len(strokes) == 1  # only the rect with an x attribute
     assert doc.draws[-1][1] is not None  # height stays the attribute's (not a presentation property)
+
+
+def test_paint_order():
+    assert not svg._stroke_first(None) and not svg._stroke_first("normal")
+    assert svg._stroke_first("stroke") and svg._stroke_first("markers stroke")
+    assert not svg._stroke_first("markers") and not svg._stroke_first("stroke stroke")  # repeated: the default
+    doc = svg.Document('<svg xmlns="http://www.w3.org/2000/svg" viewBox="0 0 10 10">'
+                       '<rect width="4" height="4" fill="red" stroke="blue" paint-order="stroke"/></svg>')
+    assert doc.draws[0][3] is None and doc.draws[1][4] is None  # the stroke first, then the fill

@@ -43,7 +43,8 @@ COLORS = {
 UNSUPPORTED = {"text", "pattern", "mask", "filter",
                "marker", "symbol", "foreignObject", "a", "tspan", "textPath"}
 INHERITED = ("fill", "fill-opacity", "fill-rule", "stroke", "stroke-opacity", "stroke-width", "stroke-linejoin",
-             "stroke-linecap", "stroke-miterlimit", "visibility", "font-size", "image-rendering", "clip-rule", "color")
+             "stroke-linecap", "stroke-miterlimit", "visibility", "font-size", "image-rendering", "clip-rule", "color",
+             "paint-order")
 SHAPES = ("path", "rect", "circle", "ellipse", "line", "polyline", "polygon")
 XLINK_HREF = "{http://www.w3.org/1999/xlink}href"
 
@@ -635,6 +636,19 @@ def _attrs(el, parent, css=None):
         if v is not None and v.strip() not in ok:
             raise SvgUnsupported(f"{k}={v}")
     return a
+
+
+def _stroke_first(text):
+    """paint-order: the keywords given, the rest in the default order (fill,
+    stroke, markers); a repeated or unknown keyword leaves the default."""
+    if text is None:
+        return False
+    words = text.split()
+    if words == ["normal"] or not words or len(set(words)) != len(words) or \
+            any(w not in ("fill", "stroke", "markers") for w in words):
+        return False
+    order = words + [w for w in ("fill", "stroke", "markers") if w not in words]
+    return order.index("stroke") < order.index("fill")
 
 
 def _opacity(text):
@@ -1270,6 +1284,7 @@ class Document:
                 raise SvgUnsupported("opacity on a filled and stroked shape")
             fill_op, stroke_op = fill_op * opacity, stroke_op * opacity
         rule = tk.FillRule.EvenOdd if a.get("fill-rule") == "evenodd" else tk.FillRule.NonZero
+        first = len(self.draws)
         if fill is not None and fill_op > 0:
             self.draws.append(("shape", path, t, (fill, min(fill_op, 1.0), rule), None))
         if stroke is not None and stroke_op > 0:
@@ -1289,6 +1304,8 @@ class Document:
                                      fs)
                     path = _dash(path, dashes, offset)
                 self.draws.append(("shape", path, t, None, (stroke, min(stroke_op, 1.0), width, join, cap, limit)))
+        if len(self.draws) - first == 2 and _stroke_first(a.get("paint-order")):
+            self.draws[first], self.draws[first + 1] = self.draws[first + 1], self.draws[first]
 
     def _image(self, el, a, t32):
         """usvg's image conversion (converter/image.rs) and the lowering of
